@@ -1,0 +1,148 @@
+"""Throughput of MagLua's spin-dynamics step on one MI355X.
+
+One step = one Heun step (PredictorCorrector.lua single corrector pass plus the
+thermal kick) of the 256x256x8 film (periodic in-plane, open z; J=1, K=0.05 along z,
+layered FFT dipole g=0.02 at truncation 1000, alpha=0.1, dt=0.002, T=0.5), all
+524,288 spins resident in HBM, through the fused kernels of libmsd.so replayed as
+a HIP graph.  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0   # MI355X published HBM3E bandwidth
+
+
+def kernel_bytes(spec, has_cls):
+    """Algorithmic HBM bytes per launch: every compulsory FP32 array read or written
+    once by the kernel that consumes or produces it; neighbour re-reads are free."""
+    n = spec.n
+    nxh = spec.nx // 2 + 1
+    spec_bytes = 3 * spec.nz * nxh * spec.ny * 8          # one complex spectrum, 3 components
+    tensor = 6 * spec.nz * nxh * spec.ny * 4               # one word per component, layer offsets 0..nz-1
+    cls = n if has_cls else 0
+    return {
+        # column pass: read M^ and the tensor, write H^
+        "k_cols": 2 * spec_bytes + tensor,
+        # predictor: read S (x,y,z,ms) and H^(S); write P, C and M^(P)
+        "k_step_pred": n * 16 + cls + spec_bytes + n * 24 + spec_bytes,
+        # corrector + thermal kick: read P, C, ms and H^(P); write S and M^(S)
+        "k_step_corr": n * 28 + cls + spec_bytes + n * 12 + spec_bytes,
+    }
+
+
+def step_model_bytes(spec):
+    """Whole-step compulsory bytes per spin-step (field evaluation E counted twice
+    plus predictor and corrector traffic): B = 2E + 96, E = 24 + 48 K/N + 24 K_T/N."""
+    KN = (spec.nx // 2 + 1) / spec.nx
+    KT = KN * (2 * spec.nz - 1) / spec.nz
+    E = 12 + 24 * KN + 24 * KN + 24 * KT + 12
+    return 2 * E + 96
+
+
+def cpu_baseline(spec, budget_s=15.0):
+    """FP64 oracle (this project's CPU restatement of the reference path), one thread,
+    on the same workload for as many whole steps as fit the budget after one warm-up step."""
+    from maglua_amd import workloads as W
+    from oracle import oracle as O
+
+    o = W.build(O, spec)
+    o["ss"].setSpins(*O.random_unit_spins(spec.seed, spec.n))
+    sim = O.Sim(o["ss"], o["ex"], o["ani"], o["dip"], o["temp"], o["llg"])
+    sim.heun(1)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        sim.heun(1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or steps >= 50:
+            break
+    return {"value": spec.n * steps / el, "unit": "spin-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} Heun steps of the full {spec.nx}x{spec.ny}x{spec.nz} film after 1 warm-up step, "
+                      f"FP64, single thread, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--profile-steps", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "replicas would shard with no communication, see the design notes", file=sys.stderr)
+        sys.exit(2)
+
+    import maglua_amd as M
+    from maglua_amd import workloads as W
+
+    spec = W.FILM_256_T05
+    t_setup = time.perf_counter()
+    g = W.build(M, spec)
+    g["ss"].randomize(spec.seed)
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"], graph=not args.no_graph)
+    heun.step(args.warmup)
+    M.synchronize()
+    print(f"# setup + {args.warmup} warm-up steps: {time.perf_counter() - t_setup:.2f} s", file=sys.stderr, flush=True)
+
+    M.synchronize()
+    t0 = time.perf_counter()
+    heun.step(args.steps)
+    M.synchronize()
+    el = time.perf_counter() - t0
+    value = spec.n * args.steps / el
+    ms_step = 1e3 * el / args.steps
+    print(f"# timed {args.steps} steps: {el:.3f} s", file=sys.stderr, flush=True)
+
+    # kernel durations: HIP events around every launch on the library stream
+    prof = heun.profile(args.profile_steps)
+    M.synchronize()
+    kb = kernel_bytes(spec, has_cls=True)
+    per_step = {k: v * (2 if k == "k_cols" else 1) for k, v in prof.items()}
+    dom = max(per_step, key=per_step.get)
+    achieved = kb[dom] / (prof[dom] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": dom, "kernel_avg_ms": round(prof[dom], 5), "algorithmic_bytes": kb[dom],
+            "step_model_bytes_per_spin": round(step_model_bytes(spec), 2),
+            "step_model_GBs": round(step_model_bytes(spec) * value / 1e9, 1),
+            "kernels_avg_ms": {k: round(v, 5) for k, v in prof.items()}}
+
+    cpu = None
+    if not args.no_cpu:
+        print("# CPU baseline (FP64 oracle, 1 thread)...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(spec)
+
+    out = {
+        "metric": "spin-steps/sec on 1 MI355X, plain FFT dipolar (learned form not built yet)",
+        "value": round(value, 1),
+        "unit": "spin-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded sphere-uniform spins, counter-based thermal noise)",
+        "config": {"workload": spec.name() + ", layered FFT dipole, Heun, plain form",
+                   "lattice": [spec.nx, spec.ny, spec.nz], "spins": spec.n, "graph": not args.no_graph},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,198 @@
+/*
+ * msd_abi.h -- C ABI of libmsd.so, the MI355X (gfx950) spin-dynamics hot path
+ * behind MagLua's operator API.
+ *
+ * Conventions
+ *   - Every entry point returns int: MSD_OK (0) or a negative MSD_E* code; the
+ *     message of the last failure on the calling thread is msd_last_error().
+ *     This replaces MagLua's "bool + errormsg, then luaL_error" convention
+ *     (src/modules/cpu/core/spinoperation.cpp:436-442).
+ *   - Objects are opaque handles with create/destroy (MagLua: refcounted
+ *     LuaBaseObject, common/luabaseobject/luabaseobject.h:191-215).
+ *   - Sites and coordinates are 0-based; a Lua binding subtracts 1 as MagLua's
+ *     own wrappers do (cpu/exchange/spinoperationexchange.cpp:384-387).
+ *   - Site index = x + nx*(y + ny*z) (cpu/core/spinsystem.cpp:1125-1142).
+ *   - Host arrays are FP64 and SoA; device storage is FP32 SoA.  Calls are
+ *     asynchronous on the library's stream except those that return host data.
+ *   - Host calls on one handle must be serialised by the caller (MagLua runs a
+ *     single Lua state on one thread).
+ */
+#ifndef MSD_ABI_H
+#define MSD_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSD_OK            0
+#define MSD_EINVAL       -1   /* bad argument (size, index, name, NULL) */
+#define MSD_EDEVICE      -2   /* HIP runtime failure */
+#define MSD_EUNSUPPORTED -3   /* valid MagLua input this build does not handle */
+#define MSD_EMISMATCH    -4   /* objects of different lattice sizes */
+#define MSD_ENOMEM       -5
+
+typedef struct msd_system     msd_system;
+typedef struct msd_exchange   msd_exchange;
+typedef struct msd_anisotropy msd_anisotropy;
+typedef struct msd_thermal    msd_thermal;
+typedef struct msd_dipole     msd_dipole;
+typedef struct msd_llg        msd_llg;
+typedef struct msd_stepper    msd_stepper;
+
+/* ------------------------------------------------------------------ library */
+const char* msd_last_error(void);
+int msd_version(void);
+/* Select the device and create the library stream (idempotent). */
+int msd_init(int device);
+/* Wait for all work queued on the library stream. */
+int msd_synchronize(void);
+/* Philox4x32-10 on the device, for known-answer checks: out[4*i..] for ctr[4*i..]. */
+int msd_philox_device(const uint32_t* ctr, int count, const uint32_t key[2], uint32_t* out);
+
+/* -------------------------------------------------------------- SpinSystem */
+/* SpinSystem.new(nx,ny,nz): cpu/core/spinsystem.cpp:29-41,502-566 (alpha=gamma=dt=1, time=0) */
+int msd_system_create(int nx, int ny, int nz, msd_system** out);
+int msd_system_destroy(msd_system* s);
+int msd_system_dims(const msd_system* s, int* nx, int* ny, int* nz);
+/* ss:copy() -- spins, parameters and field slots (spinsystem.cpp copy) */
+int msd_system_copy(const msd_system* s, msd_system** out);
+/* ss:copySpinsTo(dst) -- spins and moments only */
+int msd_system_copy_spins_to(const msd_system* src, msd_system* dst);
+/* Bulk SpinSystem::set (spinsystem.cpp:1051-1059); ms may be NULL (then ms = |s|). */
+int msd_system_set_spins(msd_system* s, const double* x, const double* y, const double* z, const double* ms);
+/* x,y,z,ms may individually be NULL. */
+int msd_system_get_spins(msd_system* s, double* x, double* y, double* z, double* ms);
+/* ss:setSpin({x,y,z},{sx,sy,sz}[,m]): m < 0 means m = |s|; the stored spin has length m. */
+int msd_system_set_spin(msd_system* s, int site, double sx, double sy, double sz, double m);
+/* ss:spin(site) -> sx, sy, sz, m */
+int msd_system_get_spin(msd_system* s, int site, double out[4]);
+/* Unit spins uniform on the sphere from a counter-based stream keyed by seed
+ * (the build's seeded initial state; MagLua scripts use math.random). */
+int msd_system_random_spins(msd_system* s, uint64_t seed);
+
+#define MSD_PARAM_ALPHA 0
+#define MSD_PARAM_GAMMA 1
+#define MSD_PARAM_DT    2
+#define MSD_PARAM_TIME  3
+/* setAlpha / setGamma / setTimeStep / setTime and their getters */
+int msd_system_set_param(msd_system* s, int which, double v);
+int msd_system_get_param(const msd_system* s, int which, double* v);
+/* per-site alpha/gamma arrays (SpinSystem site_alpha/site_gamma); NULL clears */
+int msd_system_set_site_alpha(msd_system* s, const double* a);
+int msd_system_set_site_gamma(msd_system* s, const double* g);
+
+/* ss:resetFields() -- SpinSystem::zeroFields, spinsystem.cpp:968-980 */
+int msd_system_reset_fields(msd_system* s);
+/* ss:sumFields() -- SpinSystem::sumFields, spinsystem.cpp:853-871 */
+int msd_system_sum_fields(msd_system* s);
+/* ss:fieldArrayX/Y/Z(slot) copied to the host; MSD_EINVAL if the slot is unknown */
+int msd_system_get_field(msd_system* s, const char* slot, double* hx, double* hy, double* hz);
+/* 1 if the named slot is registered and marked used since the last reset */
+int msd_system_slot_used(const msd_system* s, const char* slot, int* used);
+/* ss:netMoment(): sums (not means) -- cpu/core/spinsystem_luafuncs.lua:17-54.
+ * out = {Sx, Sy, Sz, |S|, Sx2, Sy2, Sz2, |S2|}, FP64 accumulation on the device. */
+int msd_system_net_moment(msd_system* s, double out[8]);
+/* sum_i h_slot(i) . s(i), FP64 accumulation (the scripts' fieldArray:dot(spinArray) sums) */
+int msd_system_field_dot(msd_system* s, const char* slot, double* out);
+/* ss:diff(other): {sum|dx|, sum|dy|, sum|dz|, |(those three)|} (SpinSystem::diff, spinsystem.cpp:301-307) */
+int msd_system_diff(msd_system* a, msd_system* b, double out[4]);
+
+/* ---------------------------------------------------------------- Exchange */
+/* Exchange.new: cpu/exchange/spinoperationexchange.cpp:28-39 (pbc 1,1,1, normalised) */
+int msd_exchange_create(int nx, int ny, int nz, msd_exchange** out);
+int msd_exchange_destroy(msd_exchange* e);
+int msd_exchange_set_pbc(msd_exchange* e, int px, int py, int pz);
+/* ex:normalizeMoments(bool) */
+int msd_exchange_set_normalize(msd_exchange* e, int yes);
+int msd_exchange_set_scale(msd_exchange* e, double s);
+/* ex:add({a},{b},J): l_addpath, spinoperationexchange.cpp:368-429.  Sets *added to 0
+ * when a coordinate lies outside a non-periodic dimension (the path is dropped). */
+int msd_exchange_add(msd_exchange* e, const int a[3], const int b[3], double J, int* added);
+/* Exchange::addPath(from, to, J), spinoperationexchange.cpp:252-307 (J == 0 ignored) */
+int msd_exchange_add_path(msd_exchange* e, int from_site, int to_site, double J);
+/* Bulk form of addPath for count paths (a loop of ex:addPath in a script). */
+int msd_exchange_add_paths(msd_exchange* e, const int* from_site, const int* to_site, const double* J, int count);
+int msd_exchange_num_paths(const msd_exchange* e, int* n);
+/* ex:apply(ss): h[to] += J s[from] / (m[from] m[to]) into slot "Exchange" */
+int msd_exchange_apply(msd_exchange* e, msd_system* s);
+
+/* -------------------------------------------------------------- Anisotropy */
+int msd_anisotropy_create(int nx, int ny, int nz, msd_anisotropy** out);
+int msd_anisotropy_destroy(msd_anisotropy* a);
+int msd_anisotropy_set_scale(msd_anisotropy* a, double s);
+/* ani:add({site}, {axis}, K): cpu/anisotropy/spinoperationanisotropy.cpp:367-413.
+ * The axis is normalised; a zero axis adds nothing (*added = 0). */
+int msd_anisotropy_add(msd_anisotropy* a, int site, const double axis[3], double K, int* added);
+/* Bulk form of add for count sites: axes[3*i..3*i+2], K[i] (a loop of ani:add). */
+int msd_anisotropy_add_many(msd_anisotropy* a, const int* site, const double* axes, const double* K, int count);
+/* ani:apply(ss): Anisotropy::apply, spinoperationanisotropy.cpp:247-284, slot "Anisotropy" */
+int msd_anisotropy_apply(msd_anisotropy* a, msd_system* s);
+
+/* ----------------------------------------------------------------- Thermal */
+int msd_thermal_create(int nx, int ny, int nz, msd_thermal** out);
+int msd_thermal_destroy(msd_thermal* t);
+/* temp:set(T) */
+int msd_thermal_set_temperature(msd_thermal* t, double T);
+int msd_thermal_set_scale(msd_thermal* t, double s);
+/* temp:scaleSite({site}, v) */
+int msd_thermal_scale_site(msd_thermal* t, int site, double v);
+/* Counter-based noise stream: (seed, draw, site) -> 3 normals.  apply() uses the
+ * draw counter and then increments it (MagLua: Random.* RNG objects). */
+int msd_thermal_set_seed(msd_thermal* t, uint64_t seed, uint64_t draw);
+int msd_thermal_get_draw(const msd_thermal* t, uint64_t* draw);
+/* temp:apply(ss): Thermal::apply, cpu/thermal/spinoperationthermal.cpp:135-191, slot "Thermal" */
+int msd_thermal_apply(msd_thermal* t, msd_system* s);
+
+/* ------------------------------------------------------- Dipole (LongRange) */
+/* Dipole.new: layered 2D FFT convolution, cpu/longrange/spinoperationlongrange.cpp:26-46.
+ * Needs nx even and nx, ny with prime factors in {2,3,5}. */
+int msd_dipole_create(int nx, int ny, int nz, msd_dipole** out);
+int msd_dipole_destroy(msd_dipole* d);
+/* dip:setStrength(g) */
+int msd_dipole_set_strength(msd_dipole* d, double g);
+int msd_dipole_set_scale(msd_dipole* d, double s);
+/* dip:setTruncation: gmax = {radial, x, y, z}; the x image limit reads gmax[2] and
+ * the y limit gmax[3] as in getWAB_range (cpu/dipole/dipolesupport.cpp:71-127). */
+int msd_dipole_set_truncation(msd_dipole* d, const int gmax[4]);
+/* dip:setUnitCell(A,B,C): ABC row-major {A;B;C} */
+int msd_dipole_set_unit_cell(msd_dipole* d, const double ABC[9]);
+/* dip:getMatrix/setMatrix("XX".."ZZ", {ox,oy,oz}, v): which 0..5 = XX XY XZ YY YZ ZZ,
+ * oz in [-(nz-1), nz-1] (LongRange::getAB/setAB, spinoperationlongrange.cpp:153-210). */
+int msd_dipole_get_matrix(msd_dipole* d, int which, int ox, int oy, int oz, double* v);
+int msd_dipole_set_matrix(msd_dipole* d, int which, int ox, int oy, int oz, double v);
+/* dip:apply(ss): LongRange::apply, spinoperationlongrange.cpp:354-430, slot "Dipole" */
+int msd_dipole_apply(msd_dipole* d, msd_system* s);
+
+/* --------------------------------------------------------------------- LLG */
+/* LLG.Cartesian: cpu/llg/llgcartesian.cpp:90-278 (thermalOnlyFirstTerm = true) */
+int msd_llg_create(msd_llg** out);
+int msd_llg_destroy(msd_llg* l);
+int msd_llg_set_thermal_only_first_term(msd_llg* l, int yes);
+int msd_llg_set_disable_renormalization(msd_llg* l, int yes);
+int msd_llg_set_scales(msd_llg* l, double precess_scale, double damp_scale);
+/* llg:apply(from [,scale] [,dmdt] [,to] [,advance]): cpu/llg/llg.cpp:100-171.
+ * dmdt and to may be NULL (= from). */
+int msd_llg_apply(msd_llg* l, msd_system* from, double scale, msd_system* dmdt, msd_system* to, int advance);
+
+/* ----------------------------------------------------- fused step driver */
+/* The step closure of common/scripts/PredictorCorrector.lua:54-109 with one
+ * corrector pass (Heun) and the thermal kick of :98-103, over the field sum
+ * Exchange + Anisotropy + Dipole, as fused device kernels.  Any operator may be
+ * NULL.  The operators' configuration is captured at create time. */
+int msd_stepper_create(msd_system* s, msd_exchange* ex, msd_anisotropy* ani, msd_dipole* dip,
+                       msd_thermal* th, msd_llg* llg, msd_stepper** out);
+int msd_stepper_destroy(msd_stepper* st);
+/* Replay steps through a captured HIP graph (default 1) */
+int msd_stepper_set_graph(msd_stepper* st, int yes);
+/* Advance nsteps Heun steps; asynchronous. */
+int msd_stepper_heun(msd_stepper* st, int nsteps);
+/* Kernel-level timing: runs nsteps more steps with HIP events around every
+ * launch on the library stream; names[i]/avg_ms[i] for up to cap kernels. */
+int msd_stepper_profile(msd_stepper* st, int nsteps, int cap, char names[][32], double* avg_ms, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,17 @@
+"""maglua_amd: MagLua's per-timestep spin-dynamics hot path on MI355X (gfx950).
+
+The compute lives in libmsd.so (hand-written HIP kernels behind the C ABI of
+include/msd_abi.h); this package is the host-side mirror of MagLua's Lua
+object API over that ABI.
+"""
+from ._lib import MagLuaError, declared_symbols, lib  # noqa: F401
+from .maglua import (  # noqa: F401
+    BUTCHER, LLG, Anisotropy, Dipole, Exchange, Heun, LongRange, Random, SpinSystem, Thermal,
+    make_pc_step_function, make_rk_step_function, philox_device, synchronize,
+)
+
+__all__ = [
+    "MagLuaError", "SpinSystem", "Exchange", "Anisotropy", "Thermal", "Dipole", "LongRange", "LLG",
+    "Random", "Heun", "make_pc_step_function", "make_rk_step_function", "BUTCHER", "synchronize",
+    "philox_device", "lib", "declared_symbols",
+]

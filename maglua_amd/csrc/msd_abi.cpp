@@ -1,0 +1,1326 @@
+// C ABI of libmsd.so: MagLua's hot-path objects (SpinSystem, Exchange,
+// Anisotropy, Thermal, Dipole, LLG.Cartesian) held on one MI355X and a fused
+// Heun step driver.  Host-side state is FP64; device state FP32 SoA.
+#include "msd_internal.h"
+#include "../../include/msd_abi.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+#include <strings.h>
+
+using namespace msd;
+
+// ===================================================================== errors
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) return fail(MSD_EDEVICE, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                          __FILE__, __LINE__);                                         \
+    } while (0)
+
+#define NONNULL(p) \
+    do { if (!(p)) return fail(MSD_EINVAL, "%s: NULL handle or pointer '%s'", __func__, #p); } while (0)
+
+// ===================================================================== device
+static struct {
+    bool ready = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t max_lds = 65536;
+} G;
+
+static int ensure_init()
+{
+    if (G.ready) return MSD_OK;
+    return msd_init(0);
+}
+
+template <class T>
+static int dalloc(T** p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * count);
+    if (e != hipSuccess) return fail(MSD_ENOMEM, "hipMalloc(%zu bytes): %s", sizeof(T) * count, hipGetErrorString(e));
+    return MSD_OK;
+}
+
+template <class T>
+static void dfree(T*& p)
+{
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+// upload FP64 host data as FP32 device data
+static int upload_f32(float* d, const double* h, size_t n)
+{
+    std::vector<float> t(n);
+    for (size_t i = 0; i < n; ++i) t[i] = (float)h[i];
+    HIPCHK(hipMemcpyAsync(d, t.data(), n * sizeof(float), hipMemcpyHostToDevice, G.stream));
+    HIPCHK(hipStreamSynchronize(G.stream));
+    return MSD_OK;
+}
+
+static int download_f32(double* h, const float* d, size_t n)
+{
+    std::vector<float> t(n);
+    HIPCHK(hipMemcpyAsync(t.data(), d, n * sizeof(float), hipMemcpyDeviceToHost, G.stream));
+    HIPCHK(hipStreamSynchronize(G.stream));
+    for (size_t i = 0; i < n; ++i) h[i] = t[i];
+    return MSD_OK;
+}
+
+extern "C" const char* msd_last_error(void) { return g_err.c_str(); }
+extern "C" int msd_version(void) { return 1; }
+
+extern "C" int msd_init(int device)
+{
+    if (G.ready) {
+        if (device != G.device) return fail(MSD_EINVAL, "msd_init: already initialised on device %d", G.device);
+        return MSD_OK;
+    }
+    int count = 0;
+    HIPCHK(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(MSD_EINVAL, "msd_init: device %d not present (%d devices)", device, count);
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    G.max_lds = prop.sharedMemPerBlock;
+    // a runtime without the attribute keeps its default; plans check G.max_lds
+    if (configure_lds(G.max_lds) != hipSuccess) G.max_lds = std::min<size_t>(G.max_lds, 65536);
+    HIPCHK(hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking));
+    G.device = device;
+    G.ready = true;
+    return MSD_OK;
+}
+
+extern "C" int msd_synchronize(void)
+{
+    if (!G.ready) return MSD_OK;
+    HIPCHK(hipStreamSynchronize(G.stream));
+    return MSD_OK;
+}
+
+extern "C" int msd_philox_device(const uint32_t* ctr, int count, const uint32_t key[2], uint32_t* out)
+{
+    NONNULL(ctr); NONNULL(key); NONNULL(out);
+    if (count <= 0) return fail(MSD_EINVAL, "msd_philox_device: count %d", count);
+    if (int rc = ensure_init()) return rc;
+    uint32_t *dc = nullptr, *dout = nullptr;
+    if (int rc = dalloc(&dc, (size_t)4 * count)) return rc;
+    if (int rc = dalloc(&dout, (size_t)4 * count)) { dfree(dc); return rc; }
+    hipError_t e = hipMemcpyAsync(dc, ctr, sizeof(uint32_t) * 4 * count, hipMemcpyHostToDevice, G.stream);
+    if (e == hipSuccess) e = launch_philox(dc, count, key[0], key[1], dout, G.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(uint32_t) * 4 * count, hipMemcpyDeviceToHost, G.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(G.stream);
+    dfree(dc); dfree(dout);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "msd_philox_device: %s", hipGetErrorString(e));
+    return MSD_OK;
+}
+
+// ================================================================ SpinSystem
+struct Slot {
+    std::string name;
+    float* h = nullptr;   // [3][n]
+    bool used = false;
+};
+
+struct msd_system {
+    Dims d{};
+    float* S = nullptr;           // [4][n] x y z ms
+    double alpha = 1, gamma = 1, dt = 1, time = 0;
+    float* site_alpha = nullptr;
+    float* site_gamma = nullptr;
+    std::vector<Slot> slots;
+    double* part = nullptr;       // reduction partials
+    double* out = nullptr;        // reduction result
+    uint64_t version = 1;         // bumped whenever spins change outside the step driver
+    ~msd_system()
+    {
+        dfree(S); dfree(site_alpha); dfree(site_gamma); dfree(part); dfree(out);
+        for (Slot& s : slots) dfree(s.h);
+    }
+    int find(const char* nm) const
+    {
+        for (size_t i = 0; i < slots.size(); ++i)
+            if (strcasecmp(slots[i].name.c_str(), nm) == 0) return (int)i;
+        return -1;
+    }
+    // SpinSystem::register_slot_name + ensureSlotExists: zeroed on creation
+    int slot(const char* nm, int* idx)
+    {
+        int k = find(nm);
+        if (k < 0) {
+            Slot s;
+            s.name = nm;
+            if (int rc = dalloc(&s.h, (size_t)3 * d.n)) return rc;
+            hipError_t e = launch_fill(s.h, 0.0f, (size_t)3 * d.n, G.stream);
+            if (e != hipSuccess) { dfree(s.h); return fail(MSD_EDEVICE, "slot fill: %s", hipGetErrorString(e)); }
+            slots.push_back(s);
+            k = (int)slots.size() - 1;
+        }
+        *idx = k;
+        return MSD_OK;
+    }
+    int mark(const char* nm, int* idx)
+    {
+        if (int rc = slot(nm, idx)) return rc;
+        slots[*idx].used = true;
+        return MSD_OK;
+    }
+    LLGDev llgdev(const msd_llg* l) const;
+};
+
+static bool same_dims(const Dims& a, int nx, int ny, int nz) { return a.nx == nx && a.ny == ny && a.nz == nz; }
+
+extern "C" int msd_system_create(int nx, int ny, int nz, msd_system** out)
+{
+    NONNULL(out);
+    *out = nullptr;
+    if (nx < 1 || ny < 1 || nz < 1) return fail(MSD_EINVAL, "SpinSystem.new: bad size %dx%dx%d", nx, ny, nz);
+    if ((double)nx * ny * nz > 2.0e9) return fail(MSD_EINVAL, "SpinSystem.new: %dx%dx%d exceeds 2^31 sites", nx, ny, nz);
+    if (int rc = ensure_init()) return rc;
+    std::unique_ptr<msd_system> s(new msd_system);
+    s->d = Dims{nx, ny, nz, nx * ny * nz};
+    if (int rc = dalloc(&s->S, (size_t)4 * s->d.n)) return rc;
+    if (int rc = dalloc(&s->part, (size_t)8 * 1024)) return rc;
+    if (int rc = dalloc(&s->out, 8)) return rc;
+    hipError_t e = launch_fill(s->S, 0.0f, (size_t)4 * s->d.n, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "SpinSystem.new: %s", hipGetErrorString(e));
+    int k;
+    if (int rc = s->slot("Total", &k)) return rc;
+    HIPCHK(hipStreamSynchronize(G.stream));
+    *out = s.release();
+    return MSD_OK;
+}
+
+extern "C" int msd_system_destroy(msd_system* s)
+{
+    if (!s) return MSD_OK;
+    (void)hipStreamSynchronize(G.stream);
+    delete s;
+    return MSD_OK;
+}
+
+extern "C" int msd_system_dims(const msd_system* s, int* nx, int* ny, int* nz)
+{
+    NONNULL(s);
+    if (nx) *nx = s->d.nx;
+    if (ny) *ny = s->d.ny;
+    if (nz) *nz = s->d.nz;
+    return MSD_OK;
+}
+
+static int copy_dev(float* dst, const float* src, size_t n)
+{
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyDeviceToDevice, G.stream));
+    return MSD_OK;
+}
+
+extern "C" int msd_system_copy(const msd_system* s, msd_system** out)
+{
+    NONNULL(s); NONNULL(out);
+    msd_system* c = nullptr;
+    if (int rc = msd_system_create(s->d.nx, s->d.ny, s->d.nz, &c)) return rc;
+    const size_t n = s->d.n;
+    int rc = copy_dev(c->S, s->S, 4 * n);
+    c->alpha = s->alpha; c->gamma = s->gamma; c->dt = s->dt; c->time = s->time;
+    if (!rc && s->site_alpha) { rc = dalloc(&c->site_alpha, n); if (!rc) rc = copy_dev(c->site_alpha, s->site_alpha, n); }
+    if (!rc && s->site_gamma) { rc = dalloc(&c->site_gamma, n); if (!rc) rc = copy_dev(c->site_gamma, s->site_gamma, n); }
+    for (size_t k = 0; !rc && k < s->slots.size(); ++k) {
+        int idx;
+        rc = c->slot(s->slots[k].name.c_str(), &idx);
+        if (!rc) { c->slots[idx].used = s->slots[k].used; rc = copy_dev(c->slots[idx].h, s->slots[k].h, 3 * n); }
+    }
+    if (rc) { delete c; return rc; }
+    *out = c;
+    return MSD_OK;
+}
+
+extern "C" int msd_system_copy_spins_to(const msd_system* src, msd_system* dst)
+{
+    NONNULL(src); NONNULL(dst);
+    if (!same_dims(src->d, dst->d.nx, dst->d.ny, dst->d.nz)) return fail(MSD_EMISMATCH, "copySpinsTo: size mismatch");
+    if (src == dst) return MSD_OK;
+    if (int rc = copy_dev(dst->S, src->S, (size_t)4 * src->d.n)) return rc;
+    dst->version++;
+    return MSD_OK;
+}
+
+extern "C" int msd_system_set_spins(msd_system* s, const double* x, const double* y, const double* z, const double* ms)
+{
+    NONNULL(s); NONNULL(x); NONNULL(y); NONNULL(z);
+    const size_t n = s->d.n;
+    std::vector<double> h(4 * n);
+    for (size_t i = 0; i < n; ++i) {
+        h[i] = x[i]; h[n + i] = y[i]; h[2 * n + i] = z[i];
+        h[3 * n + i] = ms ? ms[i] : std::sqrt(x[i] * x[i] + y[i] * y[i] + z[i] * z[i]);
+    }
+    if (int rc = upload_f32(s->S, h.data(), 4 * n)) return rc;
+    s->version++;
+    return MSD_OK;
+}
+
+extern "C" int msd_system_get_spins(msd_system* s, double* x, double* y, double* z, double* ms)
+{
+    NONNULL(s);
+    const size_t n = s->d.n;
+    std::vector<double> h(4 * n);
+    if (int rc = download_f32(h.data(), s->S, 4 * n)) return rc;
+    double* o[4] = {x, y, z, ms};
+    for (int c = 0; c < 4; ++c)
+        if (o[c]) std::memcpy(o[c], h.data() + c * n, n * sizeof(double));
+    return MSD_OK;
+}
+
+// l_setspin (cpu/core/spinsystem.cpp:1312-1361) then SpinSystem::set (:1051-1059)
+extern "C" int msd_system_set_spin(msd_system* s, int site, double sx, double sy, double sz, double m)
+{
+    NONNULL(s);
+    if (site < 0 || site >= s->d.n) return fail(MSD_EINVAL, "setSpin: site %d outside 0..%d", site, s->d.n - 1);
+    if (m >= 0) {
+        const double rr = sx * sx + sy * sy + sz * sz;
+        if (rr > 0) { const double f = m / std::sqrt(rr); sx *= f; sy *= f; sz *= f; }
+        else { sx = 0; sy = 0; sz = m; }
+    }
+    const float v[4] = {(float)sx, (float)sy, (float)sz, (float)std::sqrt(sx * sx + sy * sy + sz * sz)};
+    const size_t n = s->d.n;
+    for (int c = 0; c < 4; ++c)
+        HIPCHK(hipMemcpyAsync(s->S + c * n + site, &v[c], sizeof(float), hipMemcpyHostToDevice, G.stream));
+    HIPCHK(hipStreamSynchronize(G.stream));
+    s->version++;
+    return MSD_OK;
+}
+
+extern "C" int msd_system_get_spin(msd_system* s, int site, double out[4])
+{
+    NONNULL(s); NONNULL(out);
+    if (site < 0 || site >= s->d.n) return fail(MSD_EINVAL, "spin: site %d outside 0..%d", site, s->d.n - 1);
+    float v[4];
+    const size_t n = s->d.n;
+    for (int c = 0; c < 4; ++c)
+        HIPCHK(hipMemcpyAsync(&v[c], s->S + c * n + site, sizeof(float), hipMemcpyDeviceToHost, G.stream));
+    HIPCHK(hipStreamSynchronize(G.stream));
+    for (int c = 0; c < 4; ++c) out[c] = v[c];
+    return MSD_OK;
+}
+
+extern "C" int msd_system_random_spins(msd_system* s, uint64_t seed)
+{
+    NONNULL(s);
+    hipError_t e = launch_random_spins(s->d, seed, s->S, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "random_spins: %s", hipGetErrorString(e));
+    s->version++;
+    return MSD_OK;
+}
+
+extern "C" int msd_system_set_param(msd_system* s, int which, double v)
+{
+    NONNULL(s);
+    switch (which) {
+        case MSD_PARAM_ALPHA: s->alpha = v; dfree(s->site_alpha); break;   // setAlpha clears site values
+        case MSD_PARAM_GAMMA: s->gamma = v; dfree(s->site_gamma); break;
+        case MSD_PARAM_DT: s->dt = v; break;
+        case MSD_PARAM_TIME: s->time = v; break;
+        default: return fail(MSD_EINVAL, "set_param: unknown parameter %d", which);
+    }
+    return MSD_OK;
+}
+
+extern "C" int msd_system_get_param(const msd_system* s, int which, double* v)
+{
+    NONNULL(s); NONNULL(v);
+    switch (which) {
+        case MSD_PARAM_ALPHA: *v = s->alpha; break;
+        case MSD_PARAM_GAMMA: *v = s->gamma; break;
+        case MSD_PARAM_DT: *v = s->dt; break;
+        case MSD_PARAM_TIME: *v = s->time; break;
+        default: return fail(MSD_EINVAL, "get_param: unknown parameter %d", which);
+    }
+    return MSD_OK;
+}
+
+static int set_site_array(msd_system* s, float** dst, const double* a)
+{
+    if (!a) { (void)hipStreamSynchronize(G.stream); dfree(*dst); return MSD_OK; }
+    if (!*dst)
+        if (int rc = dalloc(dst, s->d.n)) return rc;
+    return upload_f32(*dst, a, s->d.n);
+}
+
+extern "C" int msd_system_set_site_alpha(msd_system* s, const double* a) { NONNULL(s); return set_site_array(s, &s->site_alpha, a); }
+extern "C" int msd_system_set_site_gamma(msd_system* s, const double* g) { NONNULL(s); return set_site_array(s, &s->site_gamma, g); }
+
+extern "C" int msd_system_reset_fields(msd_system* s)
+{
+    NONNULL(s);
+    for (Slot& sl : s->slots) {
+        sl.used = false;
+        hipError_t e = launch_fill(sl.h, 0.0f, (size_t)3 * s->d.n, G.stream);
+        if (e != hipSuccess) return fail(MSD_EDEVICE, "resetFields: %s", hipGetErrorString(e));
+    }
+    return MSD_OK;
+}
+
+extern "C" int msd_system_sum_fields(msd_system* s)
+{
+    NONNULL(s);
+    int T;
+    if (int rc = s->slot("Total", &T)) return rc;
+    std::vector<const float*> src;
+    for (size_t k = 0; k < s->slots.size(); ++k)
+        if ((int)k != T && s->slots[k].used) src.push_back(s->slots[k].h);
+    if (src.size() > 16) return fail(MSD_EUNSUPPORTED, "sumFields: more than 16 used slots");
+    hipError_t e = launch_sum_fields(s->slots[T].h, src.data(), (int)src.size(), (size_t)3 * s->d.n, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "sumFields: %s", hipGetErrorString(e));
+    return MSD_OK;
+}
+
+extern "C" int msd_system_get_field(msd_system* s, const char* slot, double* hx, double* hy, double* hz)
+{
+    NONNULL(s); NONNULL(slot);
+    const int k = s->find(slot);
+    if (k < 0) return fail(MSD_EINVAL, "fieldArray: unknown slot '%s'", slot);
+    const size_t n = s->d.n;
+    std::vector<double> h(3 * n);
+    if (int rc = download_f32(h.data(), s->slots[k].h, 3 * n)) return rc;
+    if (hx) std::memcpy(hx, h.data(), n * sizeof(double));
+    if (hy) std::memcpy(hy, h.data() + n, n * sizeof(double));
+    if (hz) std::memcpy(hz, h.data() + 2 * n, n * sizeof(double));
+    return MSD_OK;
+}
+
+extern "C" int msd_system_slot_used(const msd_system* s, const char* slot, int* used)
+{
+    NONNULL(s); NONNULL(slot); NONNULL(used);
+    const int k = s->find(slot);
+    *used = (k >= 0 && s->slots[k].used) ? 1 : 0;
+    return MSD_OK;
+}
+
+static int fetch_out(msd_system* s, double* dst, int count)
+{
+    HIPCHK(hipMemcpyAsync(dst, s->out, sizeof(double) * count, hipMemcpyDeviceToHost, G.stream));
+    HIPCHK(hipStreamSynchronize(G.stream));
+    return MSD_OK;
+}
+
+extern "C" int msd_system_net_moment(msd_system* s, double out[8])
+{
+    NONNULL(s); NONNULL(out);
+    hipError_t e = launch_reduce_moment(s->d, s->S, s->part, s->out, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "netMoment: %s", hipGetErrorString(e));
+    double v[6];
+    if (int rc = fetch_out(s, v, 6)) return rc;
+    out[0] = v[0]; out[1] = v[1]; out[2] = v[2];
+    out[3] = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    out[4] = v[3]; out[5] = v[4]; out[6] = v[5];
+    out[7] = std::sqrt(v[3] * v[3] + v[4] * v[4] + v[5] * v[5]);
+    return MSD_OK;
+}
+
+extern "C" int msd_system_field_dot(msd_system* s, const char* slot, double* out)
+{
+    NONNULL(s); NONNULL(slot); NONNULL(out);
+    const int k = s->find(slot);
+    if (k < 0) return fail(MSD_EINVAL, "field_dot: unknown slot '%s'", slot);
+    hipError_t e = launch_reduce_dot(s->d, s->slots[k].h, s->S, s->part, s->out, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "field_dot: %s", hipGetErrorString(e));
+    return fetch_out(s, out, 1);
+}
+
+extern "C" int msd_system_diff(msd_system* a, msd_system* b, double out[4])
+{
+    NONNULL(a); NONNULL(b); NONNULL(out);
+    if (!same_dims(a->d, b->d.nx, b->d.ny, b->d.nz)) return fail(MSD_EMISMATCH, "diff: size mismatch");
+    hipError_t e = launch_reduce_diff(a->d, a->S, b->S, a->part, a->out, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "diff: %s", hipGetErrorString(e));
+    if (int rc = fetch_out(a, out, 3)) return rc;
+    out[3] = std::sqrt(out[0] * out[0] + out[1] * out[1] + out[2] * out[2]);
+    return MSD_OK;
+}
+
+// ================================================================== Exchange
+struct msd_exchange {
+    Dims d{};
+    int pbc[3] = {1, 1, 1};
+    bool normalize = true;
+    double scale = 1;
+    struct Path { int from, to; double J; };
+    std::vector<Path> paths;
+    bool dirty = true;
+    // device forms
+    int *row = nullptr, *col = nullptr;
+    float* val = nullptr;
+    uint8_t* cls = nullptr;
+    int* cls_start = nullptr;
+    ExEntry* ent = nullptr;
+    bool lut_ok = false;
+    ~msd_exchange() { release(); }
+    void release()
+    {
+        (void)hipStreamSynchronize(G.stream);
+        dfree(row); dfree(col); dfree(val); dfree(cls); dfree(cls_start); dfree(ent);
+    }
+    int compile();
+    ExchangeDev csr_dev() const
+    {
+        ExchangeDev e{};
+        e.mode = paths.empty() ? 0 : 2;
+        e.normalize = normalize;
+        e.row = row; e.col = col; e.val = val;
+        return e;
+    }
+    ExchangeDev fused_dev() const
+    {
+        ExchangeDev e = csr_dev();
+        if (lut_ok && !paths.empty()) { e.mode = 1; e.cls = cls; e.cls_start = cls_start; e.ent = ent; }
+        return e;
+    }
+};
+
+static inline int min_image(int dlt, int n)
+{
+    int m = ((dlt % n) + n) % n;
+    if (2 * m >= n && n > 1) m -= n;
+    return m;
+}
+
+// CSR by target in path insertion order, plus the class table when the
+// couplings reduce to at most 256 distinct per-site stencils.
+int msd_exchange::compile()
+{
+    if (!dirty) return MSD_OK;
+    release();
+    const int n = d.n;
+    const size_t np = paths.size();
+    std::vector<int> cnt(n + 1, 0);
+    for (const Path& p : paths) cnt[p.to + 1]++;
+    for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int> hcol(np);
+    std::vector<float> hval(np);
+    std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
+    for (const Path& p : paths) {
+        const int k = fillp[p.to]++;
+        hcol[k] = p.from;
+        hval[k] = (float)(p.J * scale);
+    }
+    if (int rc = dalloc(&row, n + 1)) return rc;
+    if (int rc = dalloc(&col, np)) return rc;
+    if (int rc = dalloc(&val, np)) return rc;
+    HIPCHK(hipMemcpyAsync(row, cnt.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, G.stream));
+    if (np) {
+        HIPCHK(hipMemcpyAsync(col, hcol.data(), sizeof(int) * np, hipMemcpyHostToDevice, G.stream));
+        HIPCHK(hipMemcpyAsync(val, hval.data(), sizeof(float) * np, hipMemcpyHostToDevice, G.stream));
+    }
+    // class table
+    lut_ok = true;
+    std::map<std::string, int> ids;
+    std::vector<uint8_t> hcls(n);
+    std::vector<int> starts(1, 0);
+    std::vector<ExEntry> ents;
+    std::string key;
+    for (int t = 0; t < n && lut_ok; ++t) {
+        const int tx = t % d.nx, ty = (t / d.nx) % d.ny, tz = t / (d.nx * d.ny);
+        key.clear();
+        std::vector<ExEntry> mine;
+        for (int k = cnt[t]; k < cnt[t + 1]; ++k) {
+            const int f = hcol[k];
+            const int fx = f % d.nx, fy = (f / d.nx) % d.ny, fz = f / (d.nx * d.ny);
+            const int dx = min_image(fx - tx, d.nx), dy = min_image(fy - ty, d.ny), dz = min_image(fz - tz, d.nz);
+            if (std::abs(dx) > 127 || std::abs(dy) > 127 || std::abs(dz) > 127) { lut_ok = false; break; }
+            ExEntry e;
+            e.dx = (int8_t)dx; e.dy = (int8_t)dy; e.dz = (int8_t)dz; e.pad = 0; e.J = hval[k];
+            mine.push_back(e);
+            key.append(reinterpret_cast<const char*>(&e), sizeof e);
+        }
+        if (!lut_ok) break;
+        auto it = ids.find(key);
+        int id;
+        if (it == ids.end()) {
+            id = (int)ids.size();
+            if (id > 255) { lut_ok = false; break; }
+            ids.emplace(key, id);
+            ents.insert(ents.end(), mine.begin(), mine.end());
+            starts.push_back((int)ents.size());
+        } else {
+            id = it->second;
+        }
+        hcls[t] = (uint8_t)id;
+    }
+    if (lut_ok) {
+        if (ids.size() > 1) {
+            if (int rc = dalloc(&cls, n)) return rc;
+            HIPCHK(hipMemcpyAsync(cls, hcls.data(), n, hipMemcpyHostToDevice, G.stream));
+        }
+        if (int rc = dalloc(&cls_start, starts.size())) return rc;
+        if (int rc = dalloc(&ent, std::max<size_t>(ents.size(), 1))) return rc;
+        HIPCHK(hipMemcpyAsync(cls_start, starts.data(), sizeof(int) * starts.size(), hipMemcpyHostToDevice, G.stream));
+        if (!ents.empty())
+            HIPCHK(hipMemcpyAsync(ent, ents.data(), sizeof(ExEntry) * ents.size(), hipMemcpyHostToDevice, G.stream));
+    }
+    HIPCHK(hipStreamSynchronize(G.stream));
+    dirty = false;
+    return MSD_OK;
+}
+
+extern "C" int msd_exchange_create(int nx, int ny, int nz, msd_exchange** out)
+{
+    NONNULL(out);
+    if (nx < 1 || ny < 1 || nz < 1) return fail(MSD_EINVAL, "Exchange.new: bad size");
+    if (int rc = ensure_init()) return rc;
+    msd_exchange* e = new msd_exchange;
+    e->d = Dims{nx, ny, nz, nx * ny * nz};
+    *out = e;
+    return MSD_OK;
+}
+
+extern "C" int msd_exchange_destroy(msd_exchange* e) { delete e; return MSD_OK; }
+extern "C" int msd_exchange_set_pbc(msd_exchange* e, int px, int py, int pz)
+{
+    NONNULL(e);
+    e->pbc[0] = px != 0; e->pbc[1] = py != 0; e->pbc[2] = pz != 0;
+    return MSD_OK;
+}
+extern "C" int msd_exchange_set_normalize(msd_exchange* e, int yes) { NONNULL(e); e->normalize = yes != 0; e->dirty = true; return MSD_OK; }
+extern "C" int msd_exchange_set_scale(msd_exchange* e, double s) { NONNULL(e); e->scale = s; e->dirty = true; return MSD_OK; }
+
+extern "C" int msd_exchange_add_path(msd_exchange* e, int from, int to, double J)
+{
+    NONNULL(e);
+    if (from < 0 || from >= e->d.n || to < 0 || to >= e->d.n)
+        return fail(MSD_EINVAL, "addPath: site outside 0..%d", e->d.n - 1);
+    if (J != 0) { e->paths.push_back({from, to, J}); e->dirty = true; }
+    return MSD_OK;
+}
+
+extern "C" int msd_exchange_add(msd_exchange* e, const int a[3], const int b[3], double J, int* added)
+{
+    NONNULL(e); NONNULL(a); NONNULL(b);
+    int p[3] = {a[0], a[1], a[2]}, q[3] = {b[0], b[1], b[2]};
+    const int n[3] = {e->d.nx, e->d.ny, e->d.nz};
+    if (added) *added = 0;
+    for (int i = 0; i < 3; ++i) {
+        if (e->pbc[i]) {
+            p[i] = ((p[i] % n[i]) + n[i]) % n[i];
+            q[i] = ((q[i] % n[i]) + n[i]) % n[i];
+        }
+        if (p[i] < 0 || p[i] >= n[i] || q[i] < 0 || q[i] >= n[i]) return MSD_OK;
+    }
+    const int f = p[0] + n[0] * (p[1] + n[1] * p[2]);
+    const int t = q[0] + n[0] * (q[1] + n[1] * q[2]);
+    if (added) *added = 1;
+    return msd_exchange_add_path(e, f, t, J);
+}
+
+extern "C" int msd_exchange_add_paths(msd_exchange* e, const int* from, const int* to, const double* J, int count)
+{
+    NONNULL(e);
+    if (count < 0) return fail(MSD_EINVAL, "addPaths: count %d", count);
+    if (count && (!from || !to || !J)) return fail(MSD_EINVAL, "addPaths: NULL array");
+    for (int i = 0; i < count; ++i)
+        if (from[i] < 0 || from[i] >= e->d.n || to[i] < 0 || to[i] >= e->d.n)
+            return fail(MSD_EINVAL, "addPaths: path %d has a site outside 0..%d", i, e->d.n - 1);
+    e->paths.reserve(e->paths.size() + count);
+    for (int i = 0; i < count; ++i)
+        if (J[i] != 0) e->paths.push_back({from[i], to[i], J[i]});
+    e->dirty = true;
+    return MSD_OK;
+}
+
+extern "C" int msd_exchange_num_paths(const msd_exchange* e, int* n) { NONNULL(e); NONNULL(n); *n = (int)e->paths.size(); return MSD_OK; }
+
+extern "C" int msd_exchange_apply(msd_exchange* e, msd_system* s)
+{
+    NONNULL(e); NONNULL(s);
+    if (!same_dims(s->d, e->d.nx, e->d.ny, e->d.nz)) return fail(MSD_EMISMATCH, "Exchange:apply: size mismatch");
+    if (int rc = e->compile()) return rc;
+    int k;
+    if (int rc = s->mark("Exchange", &k)) return rc;
+    if (e->paths.empty()) return MSD_OK;
+    hipError_t err = launch_exchange_op(s->d, e->csr_dev(), s->S, s->slots[k].h, G.stream);
+    if (err != hipSuccess) return fail(MSD_EDEVICE, "Exchange:apply: %s", hipGetErrorString(err));
+    return MSD_OK;
+}
+
+// ================================================================ Anisotropy
+struct msd_anisotropy {
+    Dims d{};
+    double scale = 1;
+    struct Op { int site; double ax, ay, az, K; };
+    std::vector<Op> ops;
+    bool dirty = true;
+    float4* axk = nullptr;
+    float* mat = nullptr;
+    AnisoDev dev{};
+    ~msd_anisotropy() { (void)hipStreamSynchronize(G.stream); dfree(axk); dfree(mat); }
+    int compile();
+};
+
+int msd_anisotropy::compile()
+{
+    if (!dirty) return MSD_OK;
+    (void)hipStreamSynchronize(G.stream);
+    dfree(axk); dfree(mat);
+    dev = AnisoDev{};
+    const int n = d.n;
+    if (ops.empty()) { dirty = false; return MSD_OK; }
+    std::vector<int> per(n, 0);
+    for (const Op& o : ops) per[o.site]++;
+    bool single = true, full = true;
+    for (int i = 0; i < n; ++i) { single &= per[i] <= 1; full &= per[i] == 1; }
+    bool uniform = full;
+    for (size_t k = 1; uniform && k < ops.size(); ++k)
+        uniform = ops[k].ax == ops[0].ax && ops[k].ay == ops[0].ay && ops[k].az == ops[0].az && ops[k].K == ops[0].K;
+    if (uniform) {
+        const Op& o = ops[0];
+        const double K = o.K * scale;
+        const double m[6] = {K * o.ax * o.ax, K * o.ax * o.ay, K * o.ax * o.az, K * o.ay * o.ay, K * o.ay * o.az, K * o.az * o.az};
+        dev.mode = 1;
+        for (int k = 0; k < 6; ++k) dev.u[k] = (float)m[k];
+    } else if (single) {
+        std::vector<float4> h(n, make_float4(0.f, 0.f, 1.f, 0.f));
+        for (const Op& o : ops) h[o.site] = make_float4((float)o.ax, (float)o.ay, (float)o.az, (float)(o.K * scale));
+        if (int rc = dalloc(&axk, n)) return rc;
+        HIPCHK(hipMemcpyAsync(axk, h.data(), sizeof(float4) * n, hipMemcpyHostToDevice, G.stream));
+        HIPCHK(hipStreamSynchronize(G.stream));
+        dev.mode = 2;
+        dev.axk = axk;
+    } else {
+        std::vector<double> h((size_t)6 * n, 0.0);
+        for (const Op& o : ops) {
+            const double K = o.K * scale;
+            const double m[6] = {K * o.ax * o.ax, K * o.ax * o.ay, K * o.ax * o.az, K * o.ay * o.ay, K * o.ay * o.az, K * o.az * o.az};
+            for (int k = 0; k < 6; ++k) h[(size_t)k * n + o.site] += m[k];
+        }
+        if (int rc = dalloc(&mat, (size_t)6 * n)) return rc;
+        if (int rc = upload_f32(mat, h.data(), (size_t)6 * n)) return rc;
+        dev.mode = 3;
+        dev.mat = mat;
+    }
+    dirty = false;
+    return MSD_OK;
+}
+
+extern "C" int msd_anisotropy_create(int nx, int ny, int nz, msd_anisotropy** out)
+{
+    NONNULL(out);
+    if (nx < 1 || ny < 1 || nz < 1) return fail(MSD_EINVAL, "Anisotropy.new: bad size");
+    if (int rc = ensure_init()) return rc;
+    msd_anisotropy* a = new msd_anisotropy;
+    a->d = Dims{nx, ny, nz, nx * ny * nz};
+    *out = a;
+    return MSD_OK;
+}
+extern "C" int msd_anisotropy_destroy(msd_anisotropy* a) { delete a; return MSD_OK; }
+extern "C" int msd_anisotropy_set_scale(msd_anisotropy* a, double s) { NONNULL(a); a->scale = s; a->dirty = true; return MSD_OK; }
+
+extern "C" int msd_anisotropy_add(msd_anisotropy* a, int site, const double axis[3], double K, int* added)
+{
+    NONNULL(a); NONNULL(axis);
+    if (added) *added = 0;
+    if (site < 0 || site >= a->d.n) return fail(MSD_EINVAL, "Anisotropy:add: site %d outside 0..%d", site, a->d.n - 1);
+    const double len = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+    if (!(len > 0)) return MSD_OK;
+    a->ops.push_back({site, axis[0] / len, axis[1] / len, axis[2] / len, K});
+    a->dirty = true;
+    if (added) *added = 1;
+    return MSD_OK;
+}
+
+extern "C" int msd_anisotropy_add_many(msd_anisotropy* a, const int* site, const double* axes, const double* K, int count)
+{
+    NONNULL(a);
+    if (count < 0) return fail(MSD_EINVAL, "Anisotropy:add: count %d", count);
+    if (count && (!site || !axes || !K)) return fail(MSD_EINVAL, "Anisotropy:add: NULL array");
+    for (int i = 0; i < count; ++i) {
+        int added;
+        if (int rc = msd_anisotropy_add(a, site[i], axes + 3 * (size_t)i, K[i], &added)) return rc;
+    }
+    return MSD_OK;
+}
+
+extern "C" int msd_anisotropy_apply(msd_anisotropy* a, msd_system* s)
+{
+    NONNULL(a); NONNULL(s);
+    if (!same_dims(s->d, a->d.nx, a->d.ny, a->d.nz)) return fail(MSD_EMISMATCH, "Anisotropy:apply: size mismatch");
+    if (int rc = a->compile()) return rc;
+    int k;
+    if (int rc = s->mark("Anisotropy", &k)) return rc;
+    hipError_t e = launch_anisotropy_op(s->d, a->dev, s->S, s->slots[k].h, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "Anisotropy:apply: %s", hipGetErrorString(e));
+    return MSD_OK;
+}
+
+// =================================================================== Thermal
+struct msd_thermal {
+    Dims d{};
+    double T = 0, scale = 1;
+    std::vector<double> site;
+    bool site_dirty = false;
+    float* dsite = nullptr;
+    uint64_t seed = 0, draw = 0;
+    ~msd_thermal() { (void)hipStreamSynchronize(G.stream); dfree(dsite); }
+    int dev(ThermalDev& t)
+    {
+        if (site_dirty) {
+            bool ones = true;
+            for (double v : site) ones &= v == 1.0;
+            if (ones) { (void)hipStreamSynchronize(G.stream); dfree(dsite); }
+            else {
+                if (!dsite)
+                    if (int rc = dalloc(&dsite, d.n)) return rc;
+                if (int rc = upload_f32(dsite, site.data(), d.n)) return rc;
+            }
+            site_dirty = false;
+        }
+        t.on = 1;
+        t.T_scale = (float)(T * scale);
+        t.site = dsite;
+        t.seed = seed;
+        return MSD_OK;
+    }
+};
+
+extern "C" int msd_thermal_create(int nx, int ny, int nz, msd_thermal** out)
+{
+    NONNULL(out);
+    if (nx < 1 || ny < 1 || nz < 1) return fail(MSD_EINVAL, "Thermal.new: bad size");
+    if (int rc = ensure_init()) return rc;
+    msd_thermal* t = new msd_thermal;
+    t->d = Dims{nx, ny, nz, nx * ny * nz};
+    t->site.assign(t->d.n, 1.0);
+    *out = t;
+    return MSD_OK;
+}
+extern "C" int msd_thermal_destroy(msd_thermal* t) { delete t; return MSD_OK; }
+extern "C" int msd_thermal_set_temperature(msd_thermal* t, double T) { NONNULL(t); t->T = T; return MSD_OK; }
+extern "C" int msd_thermal_set_scale(msd_thermal* t, double s) { NONNULL(t); t->scale = s; return MSD_OK; }
+extern "C" int msd_thermal_scale_site(msd_thermal* t, int site, double v)
+{
+    NONNULL(t);
+    if (site < 0 || site >= t->d.n) return fail(MSD_EINVAL, "Thermal:scaleSite: site outside lattice");
+    t->site[site] = v;
+    t->site_dirty = true;
+    return MSD_OK;
+}
+extern "C" int msd_thermal_set_seed(msd_thermal* t, uint64_t seed, uint64_t draw) { NONNULL(t); t->seed = seed; t->draw = draw; return MSD_OK; }
+extern "C" int msd_thermal_get_draw(const msd_thermal* t, uint64_t* draw) { NONNULL(t); NONNULL(draw); *draw = t->draw; return MSD_OK; }
+
+struct msd_llg {
+    bool thermal_first = true;
+    bool no_renorm = false;
+    double ps = 1, ds = 1;
+};
+
+LLGDev msd_system::llgdev(const msd_llg* l) const
+{
+    LLGDev p{};
+    p.thermal_first = l ? l->thermal_first : 1;
+    p.renorm = l ? !l->no_renorm : 1;
+    p.ps = l ? (float)l->ps : 1.0f;
+    p.ds = l ? (float)l->ds : 1.0f;
+    p.alpha = (float)alpha;
+    p.gamma = (float)gamma;
+    p.site_alpha = site_alpha;
+    p.site_gamma = site_gamma;
+    return p;
+}
+
+extern "C" int msd_thermal_apply(msd_thermal* t, msd_system* s)
+{
+    NONNULL(t); NONNULL(s);
+    if (!same_dims(s->d, t->d.nx, t->d.ny, t->d.nz)) return fail(MSD_EMISMATCH, "Thermal:apply: size mismatch");
+    ThermalDev td;
+    if (int rc = t->dev(td)) return rc;
+    int k;
+    if (int rc = s->mark("Thermal", &k)) return rc;
+    hipError_t e = launch_thermal_op(s->d, td, s->llgdev(nullptr), (float)s->dt, t->draw, s->S, s->slots[k].h, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "Thermal:apply: %s", hipGetErrorString(e));
+    t->draw++;
+    return MSD_OK;
+}
+
+// ==================================================================== Dipole
+struct msd_dipole {
+    Dims d{};
+    int nxh = 0;
+    double g = 1, scale = 1;
+    int gmax[4] = {1000, 1000, 1000, 1000};
+    double ABC[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    bool loaded = false, custom = false, spectrum = false;
+    double words_mult = NAN;
+    double* T64 = nullptr;    // [6][2nz-1][ny][nx]
+    double2* U64 = nullptr;
+    double* W64 = nullptr;    // [6][nz][nxh][ny]
+    double2 *twx64 = nullptr, *twy64 = nullptr;
+    float* Tw = nullptr;
+    float2 *Mh = nullptr, *Hh = nullptr, *twx = nullptr, *twy = nullptr;
+    DipoleDev dev{};
+    ~msd_dipole()
+    {
+        (void)hipStreamSynchronize(G.stream);
+        dfree(T64); dfree(U64); dfree(W64); dfree(twx64); dfree(twy64);
+        dfree(Tw); dfree(Mh); dfree(Hh); dfree(twx); dfree(twy);
+    }
+    int prepare();
+};
+
+namespace msd {
+bool make_plan(int L, FFTPlan& p)
+{
+    p.L = L;
+    p.nst = 0;
+    int m = L;
+    const int order[5] = {8, 4, 2, 3, 5};
+    for (int r : order)
+        while (m % r == 0 && p.nst < 16) { p.rad[p.nst++] = r; m /= r; }
+    return m == 1 && L >= 1;
+}
+}  // namespace msd
+
+static int twiddles(int L, float2** d32, double2** d64)
+{
+    std::vector<float2> a(L);
+    std::vector<double2> b(L);
+    for (int k = 0; k < L; ++k) {
+        const double ang = -2.0 * M_PI * (double)k / (double)L;
+        b[k] = make_double2(std::cos(ang), std::sin(ang));
+        a[k] = make_float2((float)b[k].x, (float)b[k].y);
+    }
+    if (int rc = dalloc(d32, L)) return rc;
+    if (int rc = dalloc(d64, L)) return rc;
+    HIPCHK(hipMemcpyAsync(*d32, a.data(), sizeof(float2) * L, hipMemcpyHostToDevice, G.stream));
+    HIPCHK(hipMemcpyAsync(*d64, b.data(), sizeof(double2) * L, hipMemcpyHostToDevice, G.stream));
+    HIPCHK(hipStreamSynchronize(G.stream));
+    return MSD_OK;
+}
+
+extern "C" int msd_dipole_create(int nx, int ny, int nz, msd_dipole** out)
+{
+    NONNULL(out);
+    *out = nullptr;
+    if (nx < 1 || ny < 1 || nz < 1) return fail(MSD_EINVAL, "Dipole.new: bad size");
+    if (nx % 2) return fail(MSD_EUNSUPPORTED, "Dipole.new: nx=%d must be even (real-to-complex transform)", nx);
+    if (nz > 8) return fail(MSD_EUNSUPPORTED, "Dipole.new: nz=%d > 8 layers is not supported by the layered kernel", nz);
+    FFTPlan px, py;
+    if (!make_plan(nx, px) || !make_plan(ny, py))
+        return fail(MSD_EUNSUPPORTED, "Dipole.new: %dx%d needs prime factors in {2,3,5}", nx, ny);
+    if (int rc = ensure_init()) return rc;
+    std::unique_ptr<msd_dipole> p(new msd_dipole);
+    p->d = Dims{nx, ny, nz, nx * ny * nz};
+    p->nxh = nx / 2 + 1;
+    DipoleDev& dv = p->dev;
+    dv.nx = nx; dv.ny = ny; dv.nz = nz; dv.nxh = p->nxh;
+    dv.px = px; dv.py = py;
+    // rows per tile: largest of 8,4,2,1 dividing ny whose two LDS buffers fit 64 KiB
+    int ty = 1;
+    for (int c : {8, 4, 2}) {
+        const size_t need = 2 * (size_t)((3 * c + 1) / 2) * nx * sizeof(float2);
+        if (ny % c == 0 && need <= 65536) { ty = c; break; }
+    }
+    dv.ty = ty;
+    dv.lds_rows = 2 * (size_t)((3 * ty + 1) / 2) * nx * sizeof(float2);
+    dv.lds_cols = 2 * (size_t)3 * nz * ny * sizeof(float2);
+    if (dv.lds_rows > G.max_lds || dv.lds_cols > G.max_lds)
+        return fail(MSD_EUNSUPPORTED, "Dipole.new: %dx%dx%d needs %zu/%zu bytes of LDS per workgroup (device limit %zu)", nx, ny,
+                    nz, dv.lds_rows, dv.lds_cols, G.max_lds);
+    const size_t spec = (size_t)3 * nz * p->nxh * ny;
+    if (int rc = dalloc(&p->Mh, spec)) return rc;
+    if (int rc = dalloc(&p->Hh, spec)) return rc;
+    if (int rc = dalloc(&p->Tw, (size_t)6 * nz * p->nxh * ny)) return rc;
+    if (int rc = twiddles(nx, &p->twx, &p->twx64)) return rc;
+    if (int rc = twiddles(ny, &p->twy, &p->twy64)) return rc;
+    dv.Mh = p->Mh; dv.Hh = p->Hh; dv.Tw = p->Tw; dv.twx = p->twx; dv.twy = p->twy;
+    *out = p.release();
+    return MSD_OK;
+}
+
+extern "C" int msd_dipole_destroy(msd_dipole* d) { delete d; return MSD_OK; }
+extern "C" int msd_dipole_set_strength(msd_dipole* d, double g) { NONNULL(d); d->g = g; return MSD_OK; }
+extern "C" int msd_dipole_set_scale(msd_dipole* d, double s) { NONNULL(d); d->scale = s; return MSD_OK; }
+
+extern "C" int msd_dipole_set_truncation(msd_dipole* d, const int gmax[4])
+{
+    NONNULL(d); NONNULL(gmax);
+    if (gmax[0] < 1) return fail(MSD_EUNSUPPORTED, "setTruncation: radial limit %d (math.huge extrapolation is not supported)", gmax[0]);
+    for (int i = 0; i < 4; ++i) d->gmax[i] = gmax[i];
+    d->loaded = false; d->custom = false; d->spectrum = false;
+    return MSD_OK;
+}
+
+extern "C" int msd_dipole_set_unit_cell(msd_dipole* d, const double ABC[9])
+{
+    NONNULL(d); NONNULL(ABC);
+    for (int i = 0; i < 9; ++i) d->ABC[i] = ABC[i];
+    d->loaded = false; d->custom = false; d->spectrum = false;
+    return MSD_OK;
+}
+
+static int dipole_load(msd_dipole* d)
+{
+    if (d->loaded) return MSD_OK;
+    const size_t m = (size_t)6 * (2 * d->d.nz - 1) * d->d.nx * d->d.ny;
+    if (!d->T64)
+        if (int rc = dalloc(&d->T64, m)) return rc;
+    hipError_t e = launch_tensor_sum(d->d.nx, d->d.ny, d->d.nz, d->gmax, d->ABC, d->T64, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "dipole tensor: %s", hipGetErrorString(e));
+    d->loaded = true;
+    d->spectrum = false;
+    return MSD_OK;
+}
+
+// LongRange::getAB offset mapping (spinoperationlongrange.cpp:143-175)
+static int dipole_offset(msd_dipole* d, int which, int ox, int oy, int oz, size_t* idx)
+{
+    if (which < 0 || which > 5) return fail(MSD_EINVAL, "matrix index %d (0..5 = XX XY XZ YY YZ ZZ)", which);
+    const int nx = d->d.nx, ny = d->d.ny, nz = d->d.nz;
+    ox = ((ox % nx) + nx) % nx;
+    oy = ((oy % ny) + ny) % ny;
+    if (oz < -(nz - 1) || oz > nz - 1) return fail(MSD_EINVAL, "layer offset %d outside -(nz-1)..nz-1", oz);
+    *idx = (size_t)which * (2 * nz - 1) * nx * ny + ox + (size_t)nx * (oy + (size_t)ny * (oz + nz - 1));
+    return MSD_OK;
+}
+
+extern "C" int msd_dipole_get_matrix(msd_dipole* d, int which, int ox, int oy, int oz, double* v)
+{
+    NONNULL(d); NONNULL(v);
+    size_t idx;
+    if (int rc = dipole_offset(d, which, ox, oy, oz, &idx)) return rc;
+    if (int rc = dipole_load(d)) return rc;
+    HIPCHK(hipMemcpyAsync(v, d->T64 + idx, sizeof(double), hipMemcpyDeviceToHost, G.stream));
+    HIPCHK(hipStreamSynchronize(G.stream));
+    return MSD_OK;
+}
+
+extern "C" int msd_dipole_set_matrix(msd_dipole* d, int which, int ox, int oy, int oz, double v)
+{
+    NONNULL(d);
+    size_t idx;
+    if (int rc = dipole_offset(d, which, ox, oy, oz, &idx)) return rc;
+    if (int rc = dipole_load(d)) return rc;
+    HIPCHK(hipMemcpyAsync(d->T64 + idx, &v, sizeof(double), hipMemcpyHostToDevice, G.stream));
+    HIPCHK(hipStreamSynchronize(G.stream));
+    d->custom = true;
+    d->spectrum = false;
+    return MSD_OK;
+}
+
+// The layered kernel stores one word per component and uses the lattice-sum
+// parities: XX YY ZZ XY even in the layer offset, XZ YZ odd; in-plane, XX YY ZZ
+// even-even, XY odd-odd, XZ odd in x, YZ odd in y.  Custom tensors must keep them.
+static int check_symmetry(msd_dipole* d)
+{
+    const int nx = d->d.nx, ny = d->d.ny, nz = d->d.nz, nl = 2 * nz - 1;
+    const size_t plane = (size_t)nl * nx * ny;
+    std::vector<double> t((size_t)6 * plane);
+    HIPCHK(hipMemcpyAsync(t.data(), d->T64, sizeof(double) * t.size(), hipMemcpyDeviceToHost, G.stream));
+    HIPCHK(hipStreamSynchronize(G.stream));
+    const int zsg[6] = {1, 1, -1, 1, -1, 1}, xsg[6] = {1, -1, -1, 1, 1, 1}, ysg[6] = {1, -1, 1, 1, -1, 1};
+    double big = 0;
+    for (double v : t) big = std::max(big, std::fabs(v));
+    const double tol = 1e-12 * big;
+    for (int a = 0; a < 6; ++a)
+        for (int c = 0; c < nl; ++c)
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) {
+                    const double v = t[a * plane + x + (size_t)nx * (y + (size_t)ny * c)];
+                    const int cz = nl - 1 - c, xm = (nx - x) % nx, ym = (ny - y) % ny;
+                    const double vz = t[a * plane + x + (size_t)nx * (y + (size_t)ny * cz)];
+                    const double vx = t[a * plane + xm + (size_t)nx * (y + (size_t)ny * c)];
+                    const double vy = t[a * plane + x + (size_t)nx * (ym + (size_t)ny * c)];
+                    if (std::fabs(v - zsg[a] * vz) > tol || std::fabs(v - xsg[a] * vx) > tol || std::fabs(v - ysg[a] * vy) > tol)
+                        return fail(MSD_EUNSUPPORTED,
+                                    "Dipole: custom tensor component %d lacks the lattice-sum parity at (%d,%d,%d)", a, x, y,
+                                    c - nz + 1);
+                }
+    return MSD_OK;
+}
+
+int msd_dipole::prepare()
+{
+    if (int rc = dipole_load(this)) return rc;
+    if (!spectrum) {
+        if (custom)
+            if (int rc = check_symmetry(this)) return rc;
+        const size_t wn = (size_t)6 * d.nz * nxh * d.ny;
+        if (!U64)
+            if (int rc = dalloc(&U64, wn)) return rc;
+        if (!W64)
+            if (int rc = dalloc(&W64, wn)) return rc;
+        hipError_t e = launch_tensor_spectrum(d.nx, d.ny, d.nz, T64, U64, W64, twx64, twy64, G.stream);
+        if (e != hipSuccess) return fail(MSD_EDEVICE, "dipole spectrum: %s", hipGetErrorString(e));
+        spectrum = true;
+        words_mult = NAN;
+    }
+    const double mult = g * scale;
+    if (!(words_mult == mult)) {
+        hipError_t e = launch_tensor_words(W64, Tw, mult, (size_t)6 * d.nz * nxh * d.ny, G.stream);
+        if (e != hipSuccess) return fail(MSD_EDEVICE, "dipole words: %s", hipGetErrorString(e));
+        words_mult = mult;
+    }
+    return MSD_OK;
+}
+
+extern "C" int msd_dipole_apply(msd_dipole* d, msd_system* s)
+{
+    NONNULL(d); NONNULL(s);
+    if (!same_dims(s->d, d->d.nx, d->d.ny, d->d.nz)) return fail(MSD_EMISMATCH, "Dipole:apply: size mismatch");
+    if (int rc = d->prepare()) return rc;
+    int k;
+    if (int rc = s->mark("Dipole", &k)) return rc;
+    hipError_t e = launch_rows_fwd(d->dev, s->S, G.stream);
+    if (e == hipSuccess) e = launch_cols(d->dev, G.stream);
+    if (e == hipSuccess) e = launch_rows_inv(d->dev, s->slots[k].h, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "Dipole:apply: %s", hipGetErrorString(e));
+    return MSD_OK;
+}
+
+// ======================================================================= LLG
+extern "C" int msd_llg_create(msd_llg** out)
+{
+    NONNULL(out);
+    if (int rc = ensure_init()) return rc;
+    *out = new msd_llg;
+    return MSD_OK;
+}
+extern "C" int msd_llg_destroy(msd_llg* l) { delete l; return MSD_OK; }
+extern "C" int msd_llg_set_thermal_only_first_term(msd_llg* l, int yes) { NONNULL(l); l->thermal_first = yes != 0; return MSD_OK; }
+extern "C" int msd_llg_set_disable_renormalization(msd_llg* l, int yes) { NONNULL(l); l->no_renorm = yes != 0; return MSD_OK; }
+extern "C" int msd_llg_set_scales(msd_llg* l, double ps, double ds) { NONNULL(l); l->ps = ps; l->ds = ds; return MSD_OK; }
+
+// LLG::apply argument semantics (cpu/llg/llg.cpp:100-171) over LLGCartesian::apply
+extern "C" int msd_llg_apply(msd_llg* l, msd_system* from, double scale, msd_system* dmdt, msd_system* to, int advance)
+{
+    NONNULL(l); NONNULL(from);
+    if (!dmdt) dmdt = from;
+    if (!to) to = from;
+    if (!same_dims(from->d, dmdt->d.nx, dmdt->d.ny, dmdt->d.nz) || !same_dims(from->d, to->d.nx, to->d.ny, to->d.nz))
+        return fail(MSD_EMISMATCH, "LLG:apply: size mismatch");
+    int T;
+    if (int rc = dmdt->slot("Total", &T)) return rc;
+    const int TH = dmdt->find("Thermal");
+    const double dt = scale * dmdt->dt;
+    hipError_t e = launch_llg_op(from->d, dmdt->llgdev(l), (float)dt, from->S, dmdt->S, dmdt->slots[T].h,
+                                 TH >= 0 ? dmdt->slots[TH].h : nullptr, to->S, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "LLG:apply: %s", hipGetErrorString(e));
+    if (dt != 0 && advance) to->time = from->time + dt;
+    to->version++;
+    return MSD_OK;
+}
+
+// ============================================================ step driver
+struct msd_stepper {
+    msd_system* s = nullptr;
+    msd_exchange* ex = nullptr;
+    msd_anisotropy* an = nullptr;
+    msd_dipole* dip = nullptr;
+    msd_thermal* th = nullptr;
+    msd_llg llg;
+    float *P = nullptr, *C = nullptr;
+    unsigned long long* draw = nullptr;
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    StepArgs args{};
+    StepArgs graph_args{};
+    uint64_t spec_version = 0;
+    ~msd_stepper()
+    {
+        (void)hipStreamSynchronize(G.stream);
+        drop_graph();
+        dfree(P); dfree(C); dfree(draw);
+    }
+    void drop_graph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr; graph = nullptr;
+    }
+    int refresh();
+    int launch_step(hipStream_t st);
+};
+
+static bool args_equal(const StepArgs& a, const StepArgs& b) { return std::memcmp(&a, &b, sizeof(StepArgs)) == 0; }
+
+int msd_stepper::refresh()
+{
+    StepArgs a{};
+    a.d = s->d;
+    a.dt = (float)s->dt;
+    a.S = s->S; a.P = P; a.C = C;
+    if (ex) { if (int rc = ex->compile()) return rc; a.ex = ex->fused_dev(); }
+    if (an) { if (int rc = an->compile()) return rc; a.an = an->dev; }
+    if (th) { if (int rc = th->dev(a.th)) return rc; }
+    a.lp = s->llgdev(&llg);
+    a.has_dip = dip ? 1 : 0;
+    if (dip) { if (int rc = dip->prepare()) return rc; a.dp = dip->dev; a.ty = dip->dev.ty; }
+    else {
+        a.ty = 1;
+        for (int c : {8, 4, 2}) if (s->d.ny % c == 0) { a.ty = c; break; }
+    }
+    a.draw = draw;
+    args = a;
+    return MSD_OK;
+}
+
+int msd_stepper::launch_step(hipStream_t st)
+{
+    hipError_t e = hipSuccess;
+    if (args.has_dip) e = launch_cols(args.dp, st);
+    if (e == hipSuccess) e = launch_step_pred(args, st);
+    if (e == hipSuccess && args.has_dip) e = launch_cols(args.dp, st);
+    if (e == hipSuccess) e = launch_step_corr(args, st);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "step launch: %s", hipGetErrorString(e));
+    return MSD_OK;
+}
+
+extern "C" int msd_stepper_create(msd_system* s, msd_exchange* ex, msd_anisotropy* ani, msd_dipole* dip, msd_thermal* th,
+                                  msd_llg* llg, msd_stepper** out)
+{
+    NONNULL(s); NONNULL(out);
+    *out = nullptr;
+    const Dims& d = s->d;
+    if ((ex && !same_dims(d, ex->d.nx, ex->d.ny, ex->d.nz)) || (ani && !same_dims(d, ani->d.nx, ani->d.ny, ani->d.nz)) ||
+        (dip && !same_dims(d, dip->d.nx, dip->d.ny, dip->d.nz)) || (th && !same_dims(d, th->d.nx, th->d.ny, th->d.nz)))
+        return fail(MSD_EMISMATCH, "stepper: operator and SpinSystem sizes differ");
+    std::unique_ptr<msd_stepper> st(new msd_stepper);
+    st->s = s; st->ex = ex; st->an = ani; st->dip = dip; st->th = th;
+    if (llg) st->llg = *llg;
+    if (int rc = dalloc(&st->P, (size_t)3 * d.n)) return rc;
+    if (int rc = dalloc(&st->C, (size_t)3 * d.n)) return rc;
+    if (int rc = dalloc(&st->draw, 1)) return rc;
+    hipError_t e = launch_fill(st->P, 0.0f, (size_t)3 * d.n, G.stream);
+    if (e == hipSuccess) e = launch_fill(st->C, 0.0f, (size_t)3 * d.n, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "stepper: %s", hipGetErrorString(e));
+    if (int rc = st->refresh()) return rc;
+    *out = st.release();
+    return MSD_OK;
+}
+
+extern "C" int msd_stepper_destroy(msd_stepper* st) { delete st; return MSD_OK; }
+extern "C" int msd_stepper_set_graph(msd_stepper* st, int yes) { NONNULL(st); st->use_graph = yes != 0; return MSD_OK; }
+
+static int stepper_begin(msd_stepper* st)
+{
+    if (int rc = st->refresh()) return rc;
+    // spin spectrum of the current state for the first column pass
+    if (st->args.has_dip && st->spec_version != st->s->version) {
+        hipError_t e = launch_rows_fwd(st->args.dp, st->s->S, G.stream);
+        if (e != hipSuccess) return fail(MSD_EDEVICE, "stepper: %s", hipGetErrorString(e));
+    }
+    // device draw counter: the predictor increments it before the corrector reads it
+    if (st->th) {
+        const unsigned long long v = (unsigned long long)(st->th->draw - 1);
+        HIPCHK(hipMemcpyAsync(st->draw, &v, sizeof v, hipMemcpyHostToDevice, G.stream));
+        HIPCHK(hipStreamSynchronize(G.stream));
+    }
+    return MSD_OK;
+}
+
+static void stepper_end(msd_stepper* st, int nsteps)
+{
+    st->s->time += nsteps * st->s->dt;
+    if (st->th) st->th->draw += (uint64_t)nsteps;
+    st->s->version++;
+    st->spec_version = st->s->version;
+}
+
+extern "C" int msd_stepper_heun(msd_stepper* st, int nsteps)
+{
+    NONNULL(st);
+    if (nsteps < 0) return fail(MSD_EINVAL, "heun: nsteps %d", nsteps);
+    if (nsteps == 0) return MSD_OK;
+    if (int rc = stepper_begin(st)) return rc;
+    if (st->use_graph) {
+        if (!st->exec || !args_equal(st->args, st->graph_args)) {
+            st->drop_graph();
+            HIPCHK(hipStreamBeginCapture(G.stream, hipStreamCaptureModeThreadLocal));
+            int rc = st->launch_step(G.stream);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(G.stream, &g);
+            if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+            if (e != hipSuccess) return fail(MSD_EDEVICE, "graph capture: %s", hipGetErrorString(e));
+            st->graph = g;
+            HIPCHK(hipGraphInstantiate(&st->exec, st->graph, nullptr, nullptr, 0));
+            st->graph_args = st->args;
+        }
+        for (int i = 0; i < nsteps; ++i) HIPCHK(hipGraphLaunch(st->exec, G.stream));
+    } else {
+        for (int i = 0; i < nsteps; ++i)
+            if (int rc = st->launch_step(G.stream)) return rc;
+    }
+    stepper_end(st, nsteps);
+    return MSD_OK;
+}
+
+extern "C" int msd_stepper_profile(msd_stepper* st, int nsteps, int cap, char names[][32], double* avg_ms, int* count)
+{
+    NONNULL(st); NONNULL(names); NONNULL(avg_ms); NONNULL(count);
+    if (nsteps <= 0) return fail(MSD_EINVAL, "profile: nsteps %d", nsteps);
+    if (int rc = stepper_begin(st)) return rc;
+    const int per = st->args.has_dip ? 4 : 2;
+    std::vector<hipEvent_t> ev((size_t)per * nsteps + 1);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    // launch sequence of one step: 0 = column pass, 1 = predictor, 2 = corrector
+    const int seq4[4] = {0, 1, 0, 2}, seq2[2] = {1, 2};
+    const int* seq = per == 4 ? seq4 : seq2;
+    const char* kname[3] = {"k_cols", "k_step_pred", "k_step_corr"};
+    int rc = MSD_OK;
+    size_t q = 0;
+    HIPCHK(hipEventRecord(ev[q++], G.stream));
+    for (int i = 0; i < nsteps && !rc; ++i) {
+        hipError_t e = hipSuccess;
+        for (int k = 0; k < per && e == hipSuccess; ++k) {
+            if (seq[k] == 0) e = launch_cols(st->args.dp, G.stream);
+            else if (seq[k] == 1) e = launch_step_pred(st->args, G.stream);
+            else e = launch_step_corr(st->args, G.stream);
+            if (e == hipSuccess) e = hipEventRecord(ev[q++], G.stream);
+        }
+        if (e != hipSuccess) rc = fail(MSD_EDEVICE, "profile: %s", hipGetErrorString(e));
+    }
+    if (!rc) {
+        HIPCHK(hipStreamSynchronize(G.stream));
+        std::vector<std::string> nm;
+        std::vector<double> tot;
+        std::vector<int> cnt;
+        for (size_t j = 1; j < q; ++j) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, ev[j - 1], ev[j]));
+            const char* name = kname[seq[(j - 1) % per]];
+            size_t u = 0;
+            while (u < nm.size() && nm[u] != name) ++u;
+            if (u == nm.size()) { nm.push_back(name); tot.push_back(0); cnt.push_back(0); }
+            tot[u] += ms; cnt[u]++;
+        }
+        *count = std::min<int>((int)nm.size(), cap);
+        for (int u = 0; u < *count; ++u) {
+            std::snprintf(names[u], 32, "%s", nm[u].c_str());
+            avg_ms[u] = tot[u] / cnt[u];
+        }
+        stepper_end(st, nsteps);
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return rc;
+}

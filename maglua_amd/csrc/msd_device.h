@@ -1,0 +1,270 @@
+// Device-side building blocks for the gfx950 kernels: counter-based normals,
+// mixed-radix LDS FFT stages, LLG arithmetic.
+#pragma once
+
+#include "msd_internal.h"
+
+namespace msd {
+
+// ------------------------------------------------------------------ Philox
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): 10 rounds of two 32x32->64
+// multiplies with Weyl key increments.
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) { k.x += 0x9E3779B9u; k.y += 0xBB67AE85u; }
+        const uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
+        const uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
+        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+    }
+    return c;
+}
+
+// top 24 bits -> (0,1), exact in FP32
+__device__ __forceinline__ float u24f(uint32_t a) { return ((float)(a >> 8) + 0.5f) * (1.0f / 16777216.0f); }
+
+// Three standard normals for (seed, draw, site): Box-Muller on the four words.
+__device__ __forceinline__ float3 thermal_normals(uint64_t seed, uint64_t draw, int site)
+{
+    const uint4 r = philox4x32_10(make_uint4((uint32_t)site, 0u, (uint32_t)draw, (uint32_t)(draw >> 32)),
+                                  make_uint2((uint32_t)seed, (uint32_t)(seed >> 32)));
+    const float r0 = sqrtf(-2.0f * logf(u24f(r.x)));
+    const float r1 = sqrtf(-2.0f * logf(u24f(r.z)));
+    float s0, c0, s1, c1;
+    sincospif(2.0f * u24f(r.y), &s0, &c0);
+    sincospif(2.0f * u24f(r.w), &s1, &c1);
+    (void)s1;
+    return make_float3(r0 * c0, r0 * s0, r1 * c1);
+}
+
+// ---------------------------------------------------------------- complex
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) { return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y); }  // a*conj(b)
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by SIGN*i
+template <int SIGN> __device__ __forceinline__ float2 mul_si(float2 a)
+{
+    return SIGN > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// In-register DFT of size R, V[q] = sum_r v[r] exp(SIGN 2 pi i r q / R).
+template <int SIGN> __device__ __forceinline__ void dft2(float2* v)
+{
+    const float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b); v[1] = csub(a, b);
+}
+template <int SIGN> __device__ __forceinline__ void dft4(float2* v)
+{
+    const float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    const float2 t2 = cadd(v[1], v[3]), t3 = mul_si<SIGN>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2); v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3); v[3] = csub(t1, t3);
+}
+template <int SIGN> __device__ __forceinline__ void dft8(float2* v)
+{
+    const float h = 0.70710678118654752440f;
+    float2 a[4], b[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) { a[r] = cadd(v[r], v[r + 4]); b[r] = csub(v[r], v[r + 4]); }
+    // b[r] *= exp(SIGN 2 pi i r / 8)
+    b[1] = make_float2(h * (b[1].x - SIGN * b[1].y), h * (b[1].y + SIGN * b[1].x));
+    b[2] = mul_si<SIGN>(b[2]);
+    b[3] = make_float2(h * (-b[3].x - SIGN * b[3].y), h * (-b[3].y + SIGN * b[3].x));
+    dft4<SIGN>(a);
+    dft4<SIGN>(b);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) { v[2 * m] = a[m]; v[2 * m + 1] = b[m]; }
+}
+template <int SIGN> __device__ __forceinline__ void dft3(float2* v)
+{
+    const float c = -0.5f, s = 0.86602540378443864676f;
+    const float2 sum = cadd(v[1], v[2]);
+    const float2 dif = mul_si<SIGN>(csub(v[1], v[2]));
+    const float2 base = make_float2(v[0].x + c * sum.x, v[0].y + c * sum.y);
+    v[0] = cadd(v[0], sum);
+    v[1] = make_float2(base.x + s * dif.x, base.y + s * dif.y);
+    v[2] = make_float2(base.x - s * dif.x, base.y - s * dif.y);
+}
+template <int SIGN> __device__ __forceinline__ void dft5(float2* v)
+{
+    const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+    const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+    const float2 s14 = cadd(v[1], v[4]), d14 = csub(v[1], v[4]);
+    const float2 s23 = cadd(v[2], v[3]), d23 = csub(v[2], v[3]);
+    const float2 a1 = make_float2(v[0].x + c1 * s14.x + c2 * s23.x, v[0].y + c1 * s14.y + c2 * s23.y);
+    const float2 a2 = make_float2(v[0].x + c2 * s14.x + c1 * s23.x, v[0].y + c2 * s14.y + c1 * s23.y);
+    const float2 b1 = mul_si<SIGN>(make_float2(s1 * d14.x + s2 * d23.x, s1 * d14.y + s2 * d23.y));
+    const float2 b2 = mul_si<SIGN>(make_float2(s2 * d14.x - s1 * d23.x, s2 * d14.y - s1 * d23.y));
+    v[0] = make_float2(v[0].x + s14.x + s23.x, v[0].y + s14.y + s23.y);
+    v[1] = cadd(a1, b1); v[4] = csub(a1, b1);
+    v[2] = cadd(a2, b2); v[3] = csub(a2, b2);
+}
+
+// One Stockham stage of radix R over `count` lines of length L held
+// contiguously in LDS (src -> dst).  tw[k] = exp(-2 pi i k / L).
+template <int R, int SIGN>
+__device__ __forceinline__ void stockham_stage(const float2* __restrict__ src, float2* __restrict__ dst, int count, int L,
+                                               int Ns, const float2* __restrict__ tw, int tid, int nthr)
+{
+    const int nb = L / R;
+    const int total = count * nb;
+    const int tstep = L / (Ns * R);
+    for (int b = tid; b < total; b += nthr) {
+        const int seq = b / nb;
+        const int j = b - seq * nb;
+        const int k = j % Ns;
+        const float2* s = src + seq * L + j;
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = s[r * nb];
+        if (k) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const float2 w = tw[k * r * tstep];
+                v[r] = SIGN < 0 ? cmul(v[r], w) : cmulc(v[r], w);
+            }
+        }
+        if (R == 2) dft2<SIGN>(v);
+        else if (R == 3) dft3<SIGN>(v);
+        else if (R == 4) dft4<SIGN>(v);
+        else if (R == 5) dft5<SIGN>(v);
+        else dft8<SIGN>(v);
+        float2* d = dst + seq * L + (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int q = 0; q < R; ++q) d[q * Ns] = v[q];
+    }
+}
+
+// Full transform of `count` lines; returns the buffer holding the result.
+// Caller must have synchronised after filling `a`; returns after a barrier.
+template <int SIGN>
+__device__ float2* lds_fft(float2* a, float2* b, int count, const FFTPlan& p, const float2* __restrict__ tw)
+{
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    int Ns = 1;
+    for (int s = 0; s < p.nst; ++s) {
+        const int R = p.rad[s];
+        switch (R) {
+            case 2: stockham_stage<2, SIGN>(a, b, count, p.L, Ns, tw, tid, nthr); break;
+            case 3: stockham_stage<3, SIGN>(a, b, count, p.L, Ns, tw, tid, nthr); break;
+            case 4: stockham_stage<4, SIGN>(a, b, count, p.L, Ns, tw, tid, nthr); break;
+            case 5: stockham_stage<5, SIGN>(a, b, count, p.L, Ns, tw, tid, nthr); break;
+            default: stockham_stage<8, SIGN>(a, b, count, p.L, Ns, tw, tid, nthr); break;
+        }
+        Ns *= R;
+        __syncthreads();
+        float2* t = a; a = b; b = t;
+    }
+    return a;
+}
+
+// ---------------------------------------------------------------- LLG
+__device__ __forceinline__ float3 cross3(float3 a, float3 b)
+{
+    return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// dM = ps*(-gaa) M x H - ds*(alpha/ms) gaa M x (M x h)   (LLGCartesian with thermalOnlyFirstTerm)
+__device__ __forceinline__ float3 llg_rhs(float3 M, float3 H, float3 h, float alpha, float gamma, float ms, float ps, float ds)
+{
+    const float gaa = gamma / (1.0f + alpha * alpha);
+    const float3 Mh = cross3(M, h);
+    const float3 MMh = cross3(M, Mh);
+    const float3 MH = cross3(M, H);
+    const float damp = ds * (alpha / ms) * gaa;
+    const float prec = ps * -gaa;
+    return make_float3(prec * MH.x - damp * MMh.x, prec * MH.y - damp * MMh.y, prec * MH.z - damp * MMh.z);
+}
+
+__device__ __forceinline__ float3 renorm(float3 v, float ms)
+{
+    const float mm = v.x * v.x + v.y * v.y + v.z * v.z;
+    if (mm != 0.0f) {
+        const float inv = ms / sqrtf(mm);
+        v.x *= inv; v.y *= inv; v.z *= inv;
+    }
+    return v;
+}
+
+// ------------------------------------------------------------ fields
+// Anisotropy field (cpu/anisotropy/spinoperationanisotropy.cpp:247-284) as
+// h = 2 A s / ms^2 with A = sum over the site's ops of K n n^T.
+__device__ __forceinline__ float3 aniso_field(const AnisoDev& an, int i, int n, float3 s, float ms)
+{
+    if (!(ms > 0.0f) || an.mode == 0) return make_float3(0.f, 0.f, 0.f);
+    float3 h;
+    if (an.mode == 2) {
+        const float4 a = an.axk[i];
+        const float v = 2.0f * a.w * (s.x * a.x + s.y * a.y + s.z * a.z) / (ms * ms);
+        h = make_float3(a.x * v, a.y * v, a.z * v);
+    } else {
+        float m[6];
+        if (an.mode == 1) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) m[k] = an.u[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) m[k] = an.mat[(size_t)k * n + i];
+        }
+        const float f = 2.0f / (ms * ms);
+        h = make_float3(f * (m[0] * s.x + m[1] * s.y + m[2] * s.z),
+                        f * (m[1] * s.x + m[3] * s.y + m[4] * s.z),
+                        f * (m[2] * s.x + m[4] * s.y + m[5] * s.z));
+    }
+    return h;
+}
+
+// Thermal field amplitude (cpu/thermal/spinoperationthermal.cpp:165-178)
+__device__ __forceinline__ float thermal_sigma(const ThermalDev& th, const LLGDev& lp, int i, float ms, float dt)
+{
+    const float al = lp.site_alpha ? lp.site_alpha[i] : lp.alpha;
+    const float ga = lp.site_gamma ? lp.site_gamma[i] : lp.gamma;
+    if (ms == 0.0f || th.T_scale == 0.0f || !(ga > 0.0f)) return 0.0f;
+    const float sc = th.site ? th.site[i] : 1.0f;
+    return sqrtf((2.0f * al * th.T_scale * sc) / (ms * dt * ga));
+}
+
+// Exchange field at target i (cpu/exchange/spinoperationexchange.cpp:134-213):
+// sum_j J_j s_f / m_f over the site's couplings, then / m_i when normalising.
+__device__ __forceinline__ float3 exchange_field(const ExchangeDev& ex, const Dims& d, int i, int x, int y, int z,
+                                                 const float* __restrict__ s, const float* __restrict__ m)
+{
+    float3 h = make_float3(0.f, 0.f, 0.f);
+    const int n = d.n;
+    if (ex.mode == 1) {
+        const int c = ex.cls ? ex.cls[i] : 0;
+        const int e1 = ex.cls_start[c + 1];
+        for (int e = ex.cls_start[c]; e < e1; ++e) {
+            const ExEntry E = ex.ent[e];
+            int xn = x + E.dx, yn = y + E.dy, zn = z + E.dz;
+            xn += xn < 0 ? d.nx : 0; xn -= xn >= d.nx ? d.nx : 0;
+            yn += yn < 0 ? d.ny : 0; yn -= yn >= d.ny ? d.ny : 0;
+            zn += zn < 0 ? d.nz : 0; zn -= zn >= d.nz ? d.nz : 0;
+            const int f = xn + d.nx * (yn + d.ny * zn);
+            const float mf = m[f];
+            if (mf != 0.0f) {
+                const float w = E.J / mf;
+                h.x += s[f] * w; h.y += s[n + f] * w; h.z += s[2 * n + f] * w;
+            }
+        }
+    } else if (ex.mode == 2) {
+        for (int j = ex.row[i]; j < ex.row[i + 1]; ++j) {
+            const int f = ex.col[j];
+            const float mf = m[f];
+            if (mf != 0.0f) {
+                const float w = ex.val[j] / mf;
+                h.x += s[f] * w; h.y += s[n + f] * w; h.z += s[2 * n + f] * w;
+            }
+        }
+    }
+    if (ex.normalize && ex.mode) {
+        const float inv = 1.0f / m[i];
+        h.x *= inv; h.y *= inv; h.z *= inv;
+    }
+    return h;
+}
+
+}  // namespace msd

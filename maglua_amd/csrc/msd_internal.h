@@ -1,0 +1,141 @@
+// Internal declarations shared by the C-ABI host layer (msd_abi.cpp) and the
+// gfx950 kernels (*.hip).  Device storage is FP32 structure-of-arrays; site
+// index = x + nx*(y + ny*z).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+namespace msd {
+
+// ------------------------------------------------------------ FFT plan
+// Mixed-radix Stockham plan for one line length L: radices applied in order.
+struct FFTPlan {
+    int L;
+    int nst;
+    int rad[16];
+};
+bool make_plan(int L, FFTPlan& p);
+
+// ------------------------------------------------------------ exchange
+// Per-site class table: a class is the ordered list of (dx,dy,dz,J) couplings
+// a target site receives; neighbour = target + d with periodic wrap in every
+// dimension (the displacement is stored minimum-image).
+struct ExEntry {
+    int8_t dx, dy, dz, pad;
+    float J;
+};
+struct ExchangeDev {
+    int mode;               // 0 none, 1 class table, 2 CSR by target
+    int normalize;
+    const uint8_t* cls;     // [n] or nullptr (all class 0)
+    const int* cls_start;   // [ncls+1]
+    const ExEntry* ent;     // class entries
+    const int* row;         // CSR [n+1]
+    const int* col;         // CSR from-site
+    const float* val;       // CSR J*scale
+};
+
+// ---------------------------------------------------------- anisotropy
+// h = 2 A s / ms^2 with A = sum_ops K n n^T (scale folded in).
+struct AnisoDev {
+    int mode;               // 0 none, 1 uniform matrix, 2 per-site axis+K, 3 per-site matrix
+    float u[6];             // uniform: xx xy xz yy yz zz
+    const float4* axk;      // mode 2: (nx, ny, nz, K)
+    const float* mat;       // mode 3: [6][n]
+};
+
+// ------------------------------------------------------------- thermal
+struct ThermalDev {
+    int on;
+    float T_scale;          // temperature * global scale
+    const float* site;      // per-site scale or nullptr (1)
+    uint64_t seed;
+};
+
+// ----------------------------------------------------------------- LLG
+struct LLGDev {
+    int thermal_first;
+    int renorm;
+    float ps, ds;           // precession / damping scales
+    float alpha, gamma;     // uniform values
+    const float* site_alpha;
+    const float* site_gamma;
+};
+
+// ------------------------------------------------------ layered dipole
+struct DipoleDev {
+    int nx, ny, nz, nxh;
+    int ty;                 // rows per row-tile
+    FFTPlan px, py;
+    const float2* twx;      // exp(-2 pi i k / nx), k < nx
+    const float2* twy;      // exp(-2 pi i k / ny), k < ny
+    float2* Mh;             // spin spectrum [3][nz][nxh][ny]
+    float2* Hh;             // field spectrum [3][nz][nxh][ny]
+    const float* Tw;        // tensor words [6][nz][nxh][ny] (XX XY XZ YY YZ ZZ; XZ,YZ imaginary)
+    size_t lds_rows;        // dynamic LDS bytes of a row kernel
+    size_t lds_cols;        // dynamic LDS bytes of a column kernel
+};
+
+struct Dims {
+    int nx, ny, nz, n;
+};
+
+// ------------------------------------------------------------ launches
+// All launches go to stream st and return hipGetLastError().
+hipError_t launch_fill(float* p, float v, size_t n, hipStream_t st);
+hipError_t launch_sum_fields(float* total, const float* const* src, int nsrc, size_t n3, hipStream_t st);
+hipError_t launch_exchange_op(const Dims& d, const ExchangeDev& ex, const float* s, float* h, hipStream_t st);
+hipError_t launch_anisotropy_op(const Dims& d, const AnisoDev& an, const float* s, float* h, hipStream_t st);
+hipError_t launch_thermal_op(const Dims& d, const ThermalDev& th, const LLGDev& lp, float dt, uint64_t draw,
+                             const float* s, float* h, hipStream_t st);
+hipError_t launch_llg_op(const Dims& d, const LLGDev& lp, float dt, const float* from, const float* dmdt,
+                         const float* H, const float* Hth, float* to, hipStream_t st);
+hipError_t launch_random_spins(const Dims& d, uint64_t seed, float* s, hipStream_t st);
+hipError_t launch_philox(const uint32_t* ctr, int count, uint32_t k0, uint32_t k1, uint32_t* out, hipStream_t st);
+hipError_t launch_thermal_normals(const Dims& d, uint64_t seed, uint64_t draw, float* out, hipStream_t st);
+
+// FP64 reductions: moment -> out[8] sums; dot(h,s) -> out[0]; diff -> out[3]
+hipError_t launch_reduce_moment(const Dims& d, const float* s, double* part, double* out, hipStream_t st);
+hipError_t launch_reduce_dot(const Dims& d, const float* h, const float* s, double* part, double* out, hipStream_t st);
+hipError_t launch_reduce_diff(const Dims& d, const float* a, const float* b, double* part, double* out, hipStream_t st);
+int reduce_blocks(const Dims& d);
+
+// dipole: op-API pieces
+hipError_t launch_rows_fwd(const DipoleDev& dp, const float* s, hipStream_t st);
+hipError_t launch_cols(const DipoleDev& dp, hipStream_t st);
+hipError_t launch_rows_inv(const DipoleDev& dp, float* h, hipStream_t st);
+
+// dipole tensor precompute in FP64
+hipError_t launch_tensor_sum(int nx, int ny, int nz, const int gmax[4], const double ABC[9], double* T, hipStream_t st);
+// T: [6][2nz-1][ny][nx] real space; U scratch [6][nz][ny][nx/2+1] complex; W out [6][nz][nx/2+1][ny] words
+// twx/twy: FP64 tables (cos, -sin)(2 pi k / L)
+hipError_t launch_tensor_spectrum(int nx, int ny, int nz, const double* T, double2* U, double* W,
+                                  const double2* twx, const double2* twy, hipStream_t st);
+hipError_t launch_tensor_words(const double* W, float* Tw, double mult, size_t n, hipStream_t st);
+
+// fused Heun step
+struct StepArgs {
+    Dims d;
+    float dt;
+    float* S;               // [4][n]  x y z ms (updated in place by the corrector)
+    float* P;               // [3][n]  predictor spins
+    float* C;               // [3][n]  half-step spins
+    ExchangeDev ex;
+    AnisoDev an;
+    ThermalDev th;
+    LLGDev lp;
+    int has_dip;
+    DipoleDev dp;
+    unsigned long long* draw;   // device counter of the thermal draw
+    int ty;                 // rows per tile when there is no dipole
+};
+hipError_t launch_step_pred(const StepArgs& a, hipStream_t st);
+hipError_t launch_step_corr(const StepArgs& a, hipStream_t st);
+const char* kernel_name(int id);
+// allow row/column kernels the device's full LDS per workgroup
+hipError_t configure_lds(size_t max_bytes);
+
+}  // namespace msd

@@ -1,0 +1,678 @@
+"""MagLua's hot-path object API over libmsd.so.
+
+Names, argument meaning and error behaviour follow the Lua-visible API of the
+reference modules so that scripts port line by line:
+
+    ss  = SpinSystem.new(32, 32, 1)          -- SpinSystem.new(nx, ny, nz)
+    ex  = Exchange.new(ss)
+    ex:add({i,j}, {i+1,j}, J)                -- 1-based sites, periodic wrap
+    llg:apply(ss, 0.5, ssP, ssC)             -- LLG:apply(from, scale, dmdt, to)
+
+Sites are 1-based as in Lua (``{x,y,z}`` tuples, missing trailing coordinates
+are 1).  Failures raise MagLuaError carrying msd_last_error(), where MagLua
+raises luaL_error.  All state lives on the GPU; there is no CPU path.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import MagLuaError, call, lib
+
+_vp = C.c_void_p
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _site3(p, dims):
+    p = tuple(int(v) for v in p) if not np.isscalar(p) else (int(p),)
+    p = p + (1,) * (3 - len(p))
+    return p[0] - 1, p[1] - 1, p[2] - 1
+
+
+def _index(p, dims):
+    x, y, z = _site3(p, dims)
+    nx, ny, nz = dims
+    if not (0 <= x < nx and 0 <= y < ny and 0 <= z < nz):
+        raise MagLuaError(-1, f"site ({x + 1}, {y + 1}, {z + 1}) is not part of the {nx}x{ny}x{nz} lattice")
+    return x + nx * (y + ny * z)
+
+
+def _dims_of(a):
+    """Constructor arguments: (nx[,ny[,nz]]), ({nx,ny,nz}) or (ss) -- spinoperation.cpp:43-64."""
+    if len(a) == 1 and isinstance(a[0], SpinSystem):
+        return a[0].nx(), a[0].ny(), a[0].nz()
+    if len(a) >= 1 and isinstance(a[0], (tuple, list)):
+        a = tuple(a[0])
+    d = tuple(int(v) for v in a) + (1, 1, 1)
+    return d[0], d[1], d[2]
+
+
+class _Handle:
+    _destroy = None
+
+    def __init__(self, h):
+        self._h = h
+
+    @property
+    def h(self):
+        return self._h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and self._destroy:
+            try:
+                getattr(lib(), self._destroy)(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+# ---------------------------------------------------------------- SpinSystem
+class SpinSystem(_Handle):
+    """SpinSystem (cpu/core/spinsystem.h:33-140): FP32 SoA spins + named field slots on the GPU."""
+
+    _destroy = "msd_system_destroy"
+
+    def __init__(self, *a, _h=None):
+        if _h is None:
+            nx, ny, nz = _dims_of(a)
+            p = _vp()
+            call("msd_system_create", nx, ny, nz, C.byref(p))
+            _h = p.value
+        super().__init__(_h)
+        nx, ny, nz = C.c_int(), C.c_int(), C.c_int()
+        call("msd_system_dims", self._h, C.byref(nx), C.byref(ny), C.byref(nz))
+        self._dims = (nx.value, ny.value, nz.value)
+
+    @classmethod
+    def new(cls, *a):
+        return cls(*a)
+
+    def nx(self):
+        return self._dims[0]
+
+    def ny(self):
+        return self._dims[1]
+
+    def nz(self):
+        return self._dims[2]
+
+    def nxyz(self):
+        return self._dims[0] * self._dims[1] * self._dims[2]
+
+    def copy(self):
+        p = _vp()
+        call("msd_system_copy", self._h, C.byref(p))
+        return SpinSystem(_h=p.value)
+
+    def copySpinsTo(self, other):
+        call("msd_system_copy_spins_to", self._h, other.h)
+
+    # spins
+    def setSpin(self, site, s, m=None):
+        call("msd_system_set_spin", self._h, _index(site, self._dims), float(s[0]), float(s[1]), float(s[2]),
+             -1.0 if m is None else abs(float(m)))
+
+    def spin(self, *site):
+        if len(site) == 1 and not np.isscalar(site[0]):
+            site = site[0]
+        out = (C.c_double * 4)()
+        call("msd_system_get_spin", self._h, _index(site, self._dims), out)
+        return out[0], out[1], out[2], out[3]
+
+    def setSpins(self, x, y, z, m=None):
+        a = [np.ascontiguousarray(v, dtype=np.float64).reshape(-1) for v in (x, y, z)]
+        mm = None if m is None else np.ascontiguousarray(m, dtype=np.float64).reshape(-1)
+        for v in a + ([mm] if mm is not None else []):
+            if v.size != self.nxyz():
+                raise MagLuaError(-1, f"setSpins: expected {self.nxyz()} values, got {v.size}")
+        call("msd_system_set_spins", self._h, _d(a[0]), _d(a[1]), _d(a[2]), None if mm is None else _d(mm))
+
+    def spins(self):
+        """(x, y, z, ms) as FP64 arrays shaped (nz, ny, nx)."""
+        n = self.nxyz()
+        o = [np.zeros(n) for _ in range(4)]
+        call("msd_system_get_spins", self._h, *(_d(v) for v in o))
+        shp = (self._dims[2], self._dims[1], self._dims[0])
+        return tuple(v.reshape(shp) for v in o)
+
+    def spinArrayX(self):
+        return self.spins()[0]
+
+    def spinArrayY(self):
+        return self.spins()[1]
+
+    def spinArrayZ(self):
+        return self.spins()[2]
+
+    def spinArrayM(self):
+        return self.spins()[3]
+
+    def randomize(self, seed):
+        """Unit spins uniform on the sphere from the seeded counter-based stream."""
+        call("msd_system_random_spins", self._h, int(seed))
+
+    # parameters
+    def _set(self, k, v):
+        call("msd_system_set_param", self._h, k, float(v))
+
+    def _get(self, k):
+        v = C.c_double()
+        call("msd_system_get_param", self._h, k, C.byref(v))
+        return v.value
+
+    def setAlpha(self, a):
+        if np.isscalar(a):
+            self._set(0, a)
+        else:
+            self.setSiteAlpha(a)
+
+    def alpha(self):
+        return self._get(0)
+
+    def setGamma(self, g):
+        if np.isscalar(g):
+            self._set(1, g)
+        else:
+            self.setSiteGamma(g)
+
+    def gamma(self):
+        return self._get(1)
+
+    def setTimeStep(self, dt):
+        self._set(2, dt)
+
+    def timeStep(self):
+        return self._get(2)
+
+    def setTime(self, t):
+        self._set(3, t)
+
+    def time(self):
+        return self._get(3)
+
+    def setSiteAlpha(self, a):
+        if a is None:
+            call("msd_system_set_site_alpha", self._h, None)
+        else:
+            v = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+            call("msd_system_set_site_alpha", self._h, _d(v))
+
+    def setSiteGamma(self, g):
+        if g is None:
+            call("msd_system_set_site_gamma", self._h, None)
+        else:
+            v = np.ascontiguousarray(g, dtype=np.float64).reshape(-1)
+            call("msd_system_set_site_gamma", self._h, _d(v))
+
+    # fields
+    def resetFields(self):
+        call("msd_system_reset_fields", self._h)
+
+    zeroFields = resetFields
+
+    def sumFields(self):
+        call("msd_system_sum_fields", self._h)
+
+    def field(self, slot="Total"):
+        n = self.nxyz()
+        o = [np.zeros(n) for _ in range(3)]
+        call("msd_system_get_field", self._h, slot.encode(), *(_d(v) for v in o))
+        shp = (self._dims[2], self._dims[1], self._dims[0])
+        return tuple(v.reshape(shp) for v in o)
+
+    def fieldArrayX(self, slot="Total"):
+        return self.field(slot)[0]
+
+    def fieldArrayY(self, slot="Total"):
+        return self.field(slot)[1]
+
+    def fieldArrayZ(self, slot="Total"):
+        return self.field(slot)[2]
+
+    def slotUsed(self, slot):
+        u = C.c_int()
+        call("msd_system_slot_used", self._h, slot.encode(), C.byref(u))
+        return bool(u.value)
+
+    # observables
+    def netMoment(self):
+        """Sums (not means), as spinsystem_luafuncs.lua:17-54 returns them."""
+        out = (C.c_double * 8)()
+        call("msd_system_net_moment", self._h, out)
+        return tuple(out)
+
+    def fieldDot(self, slot):
+        v = C.c_double()
+        call("msd_system_field_dot", self._h, slot.encode(), C.byref(v))
+        return v.value
+
+    def diff(self, other):
+        out = (C.c_double * 4)()
+        call("msd_system_diff", self._h, other.h, out)
+        return tuple(out)
+
+
+# ------------------------------------------------------------------ operators
+class Exchange(_Handle):
+    """Exchange (cpu/exchange/spinoperationexchange.cpp): path list, slot 'Exchange'."""
+
+    _destroy = "msd_exchange_destroy"
+
+    def __init__(self, *a):
+        self._dims = _dims_of(a)
+        p = _vp()
+        call("msd_exchange_create", *self._dims, C.byref(p))
+        super().__init__(p.value)
+
+    new = classmethod(lambda cls, *a: cls(*a))
+
+    def setPeriodicXYZ(self, px, py, pz):
+        call("msd_exchange_set_pbc", self._h, int(bool(px)), int(bool(py)), int(bool(pz)))
+
+    def normalizeMoments(self, yes=True):
+        call("msd_exchange_set_normalize", self._h, int(bool(yes)))
+
+    def setScale(self, s):
+        call("msd_exchange_set_scale", self._h, float(s))
+
+    def add(self, a, b, J=1.0):
+        """ex:add({a}, {b}, J): couples from site a onto site b (1-based, periodic wrap)."""
+        pa = (C.c_int * 3)(*_site3(a, self._dims))
+        pb = (C.c_int * 3)(*_site3(b, self._dims))
+        added = C.c_int()
+        call("msd_exchange_add", self._h, pa, pb, float(J), C.byref(added))
+        return bool(added.value)
+
+    addPath = add
+
+    def addPaths(self, frm, to, J):
+        """Bulk addPath over 0-based site index arrays (a script's add loop)."""
+        f = np.ascontiguousarray(frm, dtype=np.int32).reshape(-1)
+        t = np.ascontiguousarray(to, dtype=np.int32).reshape(-1)
+        j = np.ascontiguousarray(np.broadcast_to(np.asarray(J, dtype=np.float64), f.shape))
+        if f.size != t.size:
+            raise MagLuaError(-1, "addPaths: from/to lengths differ")
+        call("msd_exchange_add_paths", self._h, f.ctypes.data_as(C.POINTER(C.c_int)),
+             t.ctypes.data_as(C.POINTER(C.c_int)), _d(j), int(f.size))
+
+    def numberOfPaths(self):
+        n = C.c_int()
+        call("msd_exchange_num_paths", self._h, C.byref(n))
+        return n.value
+
+    def apply(self, ss):
+        for s in (ss if isinstance(ss, (list, tuple)) else (ss,)):
+            call("msd_exchange_apply", self._h, s.h)
+
+
+class Anisotropy(_Handle):
+    """Anisotropy (cpu/anisotropy/spinoperationanisotropy.cpp): per-site axis and K, slot 'Anisotropy'."""
+
+    _destroy = "msd_anisotropy_destroy"
+
+    def __init__(self, *a):
+        self._dims = _dims_of(a)
+        p = _vp()
+        call("msd_anisotropy_create", *self._dims, C.byref(p))
+        super().__init__(p.value)
+
+    new = classmethod(lambda cls, *a: cls(*a))
+
+    def setScale(self, s):
+        call("msd_anisotropy_set_scale", self._h, float(s))
+
+    def add(self, site, axis, K):
+        ax = (C.c_double * 3)(*[float(v) for v in axis])
+        added = C.c_int()
+        call("msd_anisotropy_add", self._h, _index(site, self._dims), ax, float(K), C.byref(added))
+        return bool(added.value)
+
+    def addMany(self, sites, axes, K):
+        """Bulk add over 0-based site indices; axes (count,3), K (count,)."""
+        s = np.ascontiguousarray(sites, dtype=np.int32).reshape(-1)
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(axes, dtype=np.float64), (s.size, 3)))
+        k = np.ascontiguousarray(np.broadcast_to(np.asarray(K, dtype=np.float64), s.shape))
+        call("msd_anisotropy_add_many", self._h, s.ctypes.data_as(C.POINTER(C.c_int)), _d(a), _d(k), int(s.size))
+
+    def apply(self, ss):
+        for s in (ss if isinstance(ss, (list, tuple)) else (ss,)):
+            call("msd_anisotropy_apply", self._h, s.h)
+
+
+class Random:
+    """Seed holder standing in for Random.Isaac/MersenneTwister: the build's noise is
+    counter-based (Philox4x32-10), so an RNG object only carries the seed."""
+
+    def __init__(self, seed=0):
+        self.seed = int(seed)
+
+    @classmethod
+    def new(cls, seed=0):
+        return cls(seed)
+
+
+class Thermal(_Handle):
+    """Thermal (cpu/thermal/spinoperationthermal.cpp): h = sqrt(2 a T s/(ms dt g)) N(0,1), slot 'Thermal'."""
+
+    _destroy = "msd_thermal_destroy"
+
+    def __init__(self, *a, seed=0):
+        rng = [v for v in a if isinstance(v, Random)]
+        a = tuple(v for v in a if not isinstance(v, Random))
+        self._dims = _dims_of(a)
+        p = _vp()
+        call("msd_thermal_create", *self._dims, C.byref(p))
+        super().__init__(p.value)
+        self._T = 0.0
+        call("msd_thermal_set_seed", self._h, rng[0].seed if rng else int(seed), 0)
+
+    new = classmethod(lambda cls, *a, **k: cls(*a, **k))
+
+    def set(self, T):
+        self._T = float(T)
+        call("msd_thermal_set_temperature", self._h, self._T)
+
+    setTemperature = set
+
+    def temperature(self):
+        return self._T
+
+    def setScale(self, s):
+        call("msd_thermal_set_scale", self._h, float(s))
+
+    def scaleSite(self, site, v):
+        call("msd_thermal_scale_site", self._h, _index(site, self._dims), float(v))
+
+    def setSeed(self, seed, draw=0):
+        call("msd_thermal_set_seed", self._h, int(seed), int(draw))
+
+    def draw(self):
+        d = C.c_uint64()
+        call("msd_thermal_get_draw", self._h, C.byref(d))
+        return d.value
+
+    def apply(self, ss, rng=None):
+        for s in (ss if isinstance(ss, (list, tuple)) else (ss,)):
+            call("msd_thermal_apply", self._h, s.h)
+
+
+class Dipole(_Handle):
+    """Dipole : LongRange (cpu/dipole, cpu/longrange): layered FFT convolution, slot 'Dipole'."""
+
+    _destroy = "msd_dipole_destroy"
+    NAMES = ("XX", "XY", "XZ", "YY", "YZ", "ZZ")
+
+    def __init__(self, *a):
+        self._dims = _dims_of(a)
+        p = _vp()
+        call("msd_dipole_create", *self._dims, C.byref(p))
+        super().__init__(p.value)
+        self._g = 1.0
+        self._gmax = (1000, 1000, 1000, 1000)
+
+    new = classmethod(lambda cls, *a: cls(*a))
+
+    def setStrength(self, g):
+        self._g = float(g)
+        call("msd_dipole_set_strength", self._h, self._g)
+
+    def strength(self):
+        return self._g
+
+    def setScale(self, s):
+        call("msd_dipole_set_scale", self._h, float(s))
+
+    def setTruncation(self, *a):
+        """l_settrunc (spinoperationlongrange.cpp:505-573): a number sets all four
+        limits; a table {x,y,z} (or three numbers) sets x,y,z and radial = max."""
+        if len(a) == 1 and np.isscalar(a[0]):
+            if a[0] == float("inf"):
+                raise MagLuaError(-3, "setTruncation(math.huge): extrapolated lattice sums are not supported")
+            g = (int(a[0]),) * 4
+        else:
+            v = tuple(a[0]) if len(a) == 1 else tuple(a)
+            lim = [max(1, int(t)) for t in (tuple(v) + (1, 1, 1))[:3]]
+            g = (max(lim), lim[0], lim[1], lim[2])
+        self._gmax = g
+        call("msd_dipole_set_truncation", self._h, (C.c_int * 4)(*g))
+
+    def truncation(self):
+        return self._gmax
+
+    def setUnitCell(self, A, B, Cc):
+        m = np.ascontiguousarray(np.array([A, B, Cc], dtype=np.float64).reshape(-1))
+        call("msd_dipole_set_unit_cell", self._h, _d(m))
+
+    def _which(self, name):
+        try:
+            return self.NAMES.index(name.upper())
+        except ValueError:
+            raise MagLuaError(-1, "1st argument must be matrix name: XX, XY, XZ, YY, YZ or ZZ")
+
+    def getMatrix(self, name, offset):
+        o = tuple(int(v) for v in offset) + (0, 0, 0)
+        v = C.c_double()
+        call("msd_dipole_get_matrix", self._h, self._which(name), o[0], o[1], o[2], C.byref(v))
+        return v.value
+
+    def setMatrix(self, name, offset, value):
+        o = tuple(int(v) for v in offset) + (0, 0, 0)
+        call("msd_dipole_set_matrix", self._h, self._which(name), o[0], o[1], o[2], float(value))
+
+    def apply(self, ss):
+        for s in (ss if isinstance(ss, (list, tuple)) else (ss,)):
+            call("msd_dipole_apply", self._h, s.h)
+
+
+LongRange = Dipole
+
+
+class _LLGBase(_Handle):
+    """LLG.Cartesian (cpu/llg/llgcartesian.cpp:90-278) with LLG::apply argument rules (llg.cpp:100-171)."""
+
+    _destroy = "msd_llg_destroy"
+
+    def __init__(self):
+        p = _vp()
+        call("msd_llg_create", C.byref(p))
+        super().__init__(p.value)
+        self._tf = True
+        self._nr = False
+
+    def setThermalOnlyFirstTerm(self, yes=True):
+        self._tf = bool(yes)
+        call("msd_llg_set_thermal_only_first_term", self._h, int(self._tf))
+
+    def thermalOnlyFirstTerm(self):
+        return self._tf
+
+    def setDisableRenormalization(self, yes=True):
+        self._nr = bool(yes)
+        call("msd_llg_set_disable_renormalization", self._h, int(self._nr))
+
+    def setScales(self, precess=1.0, damp=1.0):
+        call("msd_llg_set_scales", self._h, float(precess), float(damp))
+
+    def apply(self, *args):
+        """apply(from [,scale] [,dmdt] [,to] [,advance]); lists apply element-wise."""
+        args = list(args)
+        advance = True
+        if args and isinstance(args[-1], bool):
+            advance = args.pop()
+        if not args:
+            raise MagLuaError(-1, "apply requires 1, 2 or 3 SpinSystems")
+        frm = args.pop(0)
+        scale = 1.0
+        if args and isinstance(args[0], (int, float)) and not isinstance(args[0], bool):
+            scale = float(args.pop(0))
+        dmdt = args.pop(0) if args else frm
+        to = args.pop(0) if args else frm
+        F = frm if isinstance(frm, (list, tuple)) else [frm]
+        D = dmdt if isinstance(dmdt, (list, tuple)) else [dmdt]
+        T = to if isinstance(to, (list, tuple)) else [to]
+        if not (len(F) == len(D) == len(T)):
+            raise MagLuaError(-4, "Size mismatch in number of SpinSystems given in each table")
+        for f, d, t in zip(F, D, T):
+            call("msd_llg_apply", self._h, f.h, scale, d.h, t.h, int(advance))
+
+
+class _LLGNamespace:
+    class Cartesian(_LLGBase):
+        @classmethod
+        def new(cls):
+            return cls()
+
+    @staticmethod
+    def new(kind="Cartesian"):
+        if kind != "Cartesian":
+            raise MagLuaError(-3, f"LLG.{kind} is not provided by this build (LLG.Cartesian is)")
+        return _LLGNamespace.Cartesian()
+
+
+LLG = _LLGNamespace
+
+
+# ------------------------------------------------------------------ drivers
+def make_pc_step_function(s, calcField, llg, tol, optional_temp=None):
+    """common/scripts/PredictorCorrector.lua:54-109, over the per-operator GPU kernels."""
+    state = {"P": s.copy(), "C": s.copy()}
+
+    def step(ss=None, skip_temperature=False):
+        ss = ss or s
+        P, Cc = state["P"], state["C"]
+        last = [-1.0]
+
+        def same(a, b):
+            d = a.diff(b)[3]
+            if d == last[0]:
+                raise MagLuaError(-1, "Failing to converge in Predictor Corrector, timestep too large?")
+            last[0] = d
+            return d < tol
+
+        calcField(ss)
+        llg.apply(ss, ss, P)
+        q = 0
+        while True:
+            llg.apply(ss, 0.5, ss, Cc)
+            calcField(P)
+            llg.apply(Cc, 0.5, P, Cc)
+            P, Cc = Cc, P
+            q += 1
+            if q > 100:
+                raise MagLuaError(-1, "Failing to converge in Predictor Corrector, timestep too large?")
+            if same(P, Cc):
+                break
+        state["P"], state["C"] = P, Cc
+        P.copySpinsTo(ss)
+        ss.setTime(P.time())
+        if optional_temp is not None and not skip_temperature:
+            ss.resetFields()
+            optional_temp.apply(ss)
+            ss.sumFields()
+            llg.apply(ss, False)
+        return True
+
+    return step
+
+
+BUTCHER = {
+    "RK1": [[1]],
+    "RK2": [[1 / 2], [0, 1]],
+    "K3": [[1 / 2], [-1, 2], [1 / 6, 2 / 3, 1 / 6]],
+    "RK4": [[1 / 2], [0, 1 / 2], [0, 0, 1], [1 / 6, 1 / 3, 1 / 3, 1 / 6]],
+    "RK3": [[1 / 3], [0, 2 / 3], [1 / 4, 0, 3 / 4]],
+    "RK4_38": [[1 / 3], [-1 / 3, 1], [1, -1, 1], [1 / 8, 3 / 8, 3 / 8, 1 / 8]],
+    "RK6": [[1 / 3], [0, 2 / 3], [1 / 12, 4 / 12, -1 / 12], [-1 / 16, 18 / 16, -3 / 16, -6 / 16],
+            [0, 9 / 8, -3 / 8, -6 / 8, 4 / 8], [9 / 44, -36 / 44, 63 / 44, 72 / 44, 0, -64 / 44],
+            [11 / 120, 0, 81 / 120, 81 / 120, -32 / 120, -32 / 120, 11 / 120]],
+    "BS3": [[1 / 2], [0, 3 / 4], [2 / 9, 1 / 3, 4 / 9], [7 / 24, 1 / 4, 1 / 3, 1 / 8]],
+}
+
+
+def make_rk_step_function(ss, kind, calcField, llg, optional_temp=None, dynamics=None):
+    """common/scripts/RungeKutta.lua:57-214: Butcher rows as chained renormalised Euler sub-steps."""
+    if isinstance(calcField, (list, tuple)):
+        ops = list(calcField)
+
+        def calcField(s):
+            s.resetFields()
+            for o in ops:
+                o.apply(s)
+            s.sumFields()
+    if kind not in BUTCHER:
+        raise MagLuaError(-1, 'RK type must be one of "' + '", "'.join(BUTCHER) + '"')
+    bt = BUTCHER[kind]
+    nstep = len(bt)
+    k = [ss] + [ss.copy() for _ in range(nstep - 1)]
+    dyn = dynamics or (lambda s: None)
+
+    def istep(src, f, dst):
+        llg.apply(src, f[0], src, dst)
+        for i in range(1, nstep):
+            if i < len(f) and f[i] != 0:
+                llg.apply(dst, f[i], k[i], dst)
+
+    def sfunc(ss_input=None, skip_temperature=False):
+        k[0] = ss_input or ss
+        dyn(k[0])
+        calcField(k[0])
+        for i in range(nstep - 1):
+            k[i + 1].setTimeStep(k[0].timeStep())
+            istep(k[0], bt[i], k[i + 1])
+            dyn(k[i + 1])
+            calcField(k[i + 1])
+        istep(k[0], bt[nstep - 1], k[0])
+        if optional_temp is not None and not skip_temperature:
+            k[0].resetFields()
+            optional_temp.apply(k[0])
+            k[0].sumFields()
+            llg.apply(k[0], False)
+        return True
+
+    return sfunc
+
+
+class Heun(_Handle):
+    """The fused device form of make_pc_step_function with a single corrector pass
+    (Heun) over Exchange + Anisotropy + Dipole, with the optional thermal kick."""
+
+    _destroy = "msd_stepper_destroy"
+
+    def __init__(self, ss, ex=None, ani=None, dip=None, temp=None, llg=None, graph=True):
+        self._keep = (ss, ex, ani, dip, temp, llg)
+        g = lambda o: o.h if o is not None else None
+        p = _vp()
+        call("msd_stepper_create", ss.h, g(ex), g(ani), g(dip), g(temp), g(llg), C.byref(p))
+        super().__init__(p.value)
+        call("msd_stepper_set_graph", self._h, int(bool(graph)))
+
+    def step(self, n=1):
+        call("msd_stepper_heun", self._h, int(n))
+        return True
+
+    __call__ = step
+
+    def profile(self, n):
+        """Kernel -> average ms per launch over n more steps (HIP events on the library stream)."""
+        cap = 8
+        names = (C.c_char * 32 * cap)()
+        avg = (C.c_double * cap)()
+        cnt = C.c_int()
+        call("msd_stepper_profile", self._h, int(n), cap, names, avg, C.byref(cnt))
+        return {bytes(names[i]).split(b"\0")[0].decode(): avg[i] for i in range(cnt.value)}
+
+
+def synchronize():
+    call("msd_synchronize")
+
+
+def philox_device(ctr, key):
+    ctr = np.ascontiguousarray(ctr, dtype=np.uint32).reshape(-1, 4)
+    out = np.zeros_like(ctr)
+    k = (C.c_uint32 * 2)(*key)
+    call("msd_philox_device", ctr.ctypes.data_as(C.POINTER(C.c_uint32)), ctr.shape[0], k,
+         out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return out

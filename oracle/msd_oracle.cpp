@@ -1,0 +1,1010 @@
+/*
+ * FP64 CPU oracle for MagLua's per-timestep spin-dynamics hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for libmsd.so and the CPU baseline
+ * timed by bench.py.  Nothing in maglua_amd/ links or calls this file.
+ *
+ * What each part restates (MagLua source paths) and what pins it:
+ *   ora_philox4x32_10      Philox4x32-10 of Salmon et al. (SC'11), the build's own
+ *                          counter-based generator replacing RNG::randNorm
+ *                          (src/modules/common/random/random.cpp:97-122).
+ *                          Pinned by the Random123 known-answer vectors (tests).
+ *   ora_fft1d / 2d         FFTW guru C2C as called by cpu/array/fourier.cpp:48-97,
+ *                          185-220: sign -1 forward, unnormalised, 2D = per-layer.
+ *                          FFTW itself is not in the tree; pinned against a naive
+ *                          O(n^2) DFT and numpy.fft (tests).
+ *   ora_dipole_tensor      getWAB/getWAB_range/gamma_ab_dip/dipoleLoad,
+ *                          cpu/dipole/dipolesupport.cpp:34-250,717-793.
+ *                          Pinned by brute-force image sums on tiny lattices, the
+ *                          trace identity and the square-lattice sum (tests).
+ *   ora_ex_*               Exchange::apply/addPath/l_addpath,
+ *                          cpu/exchange/spinoperationexchange.cpp:134-213,252-307,368-429.
+ *                          Pinned by the uniform-state closed form (tests).
+ *   ora_ani_*              Anisotropy::apply and l_add,
+ *                          cpu/anisotropy/spinoperationanisotropy.cpp:247-284,367-413.
+ *                          Pinned by the closed form 2K(s.n)n/ms^2 (tests).
+ *   ora_th_*               Thermal::apply, cpu/thermal/spinoperationthermal.cpp:135-191.
+ *                          Pinned by sigma^2 variance and the Langevin paramagnet (tests).
+ *   ora_lr_*               LongRange::updateData/apply,
+ *                          cpu/longrange/spinoperationlongrange.cpp:26-46,245-430;
+ *                          SpinSystem::fft cpu/core/spinsystem.cpp:936-966.
+ *                          Pinned by the real-space O(N^2) sum with explicit images (tests).
+ *   ora_llg_*              LLGCartesian::apply, cpu/llg/llgcartesian.cpp:90-278,
+ *                          flags cpu/llg/llg.cpp:22-43.  Pinned by the single-spin
+ *                          closed-form precession/damping and |m| conservation (tests).
+ *   ora_sim_heun           make_pc_step_function, one corrector pass plus the
+ *                          thermal kick, common/scripts/PredictorCorrector.lua:54-109.
+ *   ora_sim_rk             make_rk_step_function Butcher rows as chained
+ *                          renormalised Euler sub-steps, common/scripts/RungeKutta.lua:57-214.
+ *                          Both pinned by convergence order on a single spin against
+ *                          the closed form (tests).
+ *   ora_sys_net_moment     SpinSystem netMoment, cpu/core/spinsystem_luafuncs.lua:17-54
+ *                          (returns sums, not means).
+ */
+#include "msd_oracle.h"
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <complex>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef std::complex<double> cplx;
+static const double PI = 3.14159265358979323846264338327950288;
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10                                                             */
+/* ------------------------------------------------------------------------ */
+extern "C" void ora_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int round = 0; round < 10; ++round) {
+        if (round) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* uniform in (0,1) from the top 24 bits: exact in FP32 and FP64 alike */
+static inline double u24(uint32_t a) { return ((double)(a >> 8) + 0.5) * (1.0 / 16777216.0); }
+
+extern "C" void ora_thermal_normals(uint64_t seed, uint64_t step, int64_t site, double out[3])
+{
+    const uint32_t ctr[4] = {(uint32_t)site, (uint32_t)((uint64_t)site >> 32),
+                             (uint32_t)step, (uint32_t)(step >> 32)};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t r[4];
+    ora_philox4x32_10(ctr, key, r);
+    /* Box-Muller (1958): two pairs, the first complete, cos of the second */
+    const double r0 = std::sqrt(-2.0 * std::log(u24(r[0])));
+    const double t0 = 2.0 * PI * u24(r[1]);
+    const double r1 = std::sqrt(-2.0 * std::log(u24(r[2])));
+    const double t1 = 2.0 * PI * u24(r[3]);
+    out[0] = r0 * std::cos(t0);
+    out[1] = r0 * std::sin(t0);
+    out[2] = r1 * std::cos(t1);
+}
+
+extern "C" void ora_random_unit_spins(uint64_t seed, int64_t n, double* x, double* y, double* z)
+{
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (int64_t i = 0; i < n; ++i) {
+        const uint32_t ctr[4] = {(uint32_t)i, (uint32_t)((uint64_t)i >> 32), 0xFFFFFFFFu, 0x5350494Eu};
+        uint32_t r[4];
+        ora_philox4x32_10(ctr, key, r);
+        const double u1 = ((double)(r[0] >> 5) * 67108864.0 + (double)(r[1] >> 6) + 0.5) * (1.0 / 9007199254740992.0);
+        const double u2 = ((double)(r[2] >> 5) * 67108864.0 + (double)(r[3] >> 6) + 0.5) * (1.0 / 9007199254740992.0);
+        const double zc = 2.0 * u1 - 1.0;
+        const double rho = std::sqrt(std::fmax(0.0, 1.0 - zc * zc));
+        const double ph = 2.0 * PI * u2;
+        x[i] = rho * std::cos(ph);
+        y[i] = rho * std::sin(ph);
+        z[i] = zc;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* FFT: Stockham autosort, mixed radix, FP64                                 */
+/* ------------------------------------------------------------------------ */
+struct FFTPlan {
+    int n;
+    std::vector<int> radix;
+};
+
+static bool factor(int n, std::vector<int>& rad)
+{
+    rad.clear();
+    if (n < 1) return false;
+    int m = n;
+    while (m % 4 == 0) { rad.push_back(4); m /= 4; }
+    while (m % 2 == 0) { rad.push_back(2); m /= 2; }
+    for (int p = 3; p * p <= m; p += 2)
+        while (m % p == 0) { rad.push_back(p); m /= p; }
+    if (m > 1) rad.push_back(m);
+    return true;
+}
+
+/* exp(sign*2*pi*i*num/den) with the argument reduced exactly first */
+static inline cplx root(long long num, long long den, int sign)
+{
+    long long r = num % den;
+    if (r < 0) r += den;
+    const double a = sign * 2.0 * PI * (double)r / (double)den;
+    return cplx(std::cos(a), std::sin(a));
+}
+
+/* Per (n, sign): radices, the stage twiddles W_{Ns R}^{k r} and the radix roots,
+ * all from root() (exactly reduced arguments). */
+struct LinePlan {
+    std::vector<int> rad;
+    std::vector<std::vector<cplx>> tw;    /* per stage [k*R + r] */
+    std::vector<std::vector<cplx>> dft;   /* per stage [q*R + r] */
+};
+
+static const LinePlan& line_plan(int n, int sign)
+{
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, LinePlan> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find({n, sign});
+    if (it != cache.end()) return it->second;
+    LinePlan p;
+    factor(n, p.rad);
+    int Ns = 1;
+    for (int R : p.rad) {
+        std::vector<cplx> t((size_t)Ns * R), d((size_t)R * R);
+        for (int k = 0; k < Ns; ++k)
+            for (int r = 0; r < R; ++r) t[(size_t)k * R + r] = root((long long)k * r, (long long)Ns * R, sign);
+        for (int q = 0; q < R; ++q)
+            for (int r = 0; r < R; ++r) d[(size_t)q * R + r] = root((long long)r * q, R, sign);
+        p.tw.push_back(std::move(t));
+        p.dft.push_back(std::move(d));
+        Ns *= R;
+    }
+    return cache.emplace(std::make_pair(n, sign), std::move(p)).first->second;
+}
+
+static void fft_line(cplx* data, int n, int sign, cplx* work)
+{
+    if (n == 1) return;
+    const LinePlan& p = line_plan(n, sign);
+    cplx* in = data;
+    cplx* out = work;
+    int Ns = 1;
+    cplx v[64], V[64];
+    for (size_t st = 0; st < p.rad.size(); ++st) {
+        const int R = p.rad[st];
+        const cplx* tw = p.tw[st].data();
+        const cplx* D = p.dft[st].data();
+        const int nb = n / R;
+        for (int j = 0; j < nb; ++j) {
+            const int k = j % Ns;
+            if (R > 64) {
+                /* large prime factor: direct sum through the plan tables */
+                const int base = (j / Ns) * Ns * R + k;
+                for (int q = 0; q < R; ++q) {
+                    cplx s = 0;
+                    for (int r = 0; r < R; ++r) s += in[j + r * nb] * tw[(size_t)k * R + r] * D[(size_t)q * R + r];
+                    out[base + q * Ns] = s;
+                }
+                continue;
+            }
+            for (int r = 0; r < R; ++r) v[r] = in[j + r * nb] * tw[(size_t)k * R + r];
+            if (R == 2) {
+                V[0] = v[0] + v[1]; V[1] = v[0] - v[1];
+            } else if (R == 4) {
+                const cplx t0 = v[0] + v[2], t1 = v[0] - v[2], t2 = v[1] + v[3];
+                const cplx t3 = (v[1] - v[3]) * cplx(0, (double)sign);
+                V[0] = t0 + t2; V[2] = t0 - t2; V[1] = t1 + t3; V[3] = t1 - t3;
+            } else {
+                for (int q = 0; q < R; ++q) {
+                    cplx s = 0;
+                    for (int r = 0; r < R; ++r) s += v[r] * D[(size_t)q * R + r];
+                    V[q] = s;
+                }
+            }
+            const int base = (j / Ns) * Ns * R + k;
+            for (int q = 0; q < R; ++q) out[base + q * Ns] = V[q];
+        }
+        Ns *= R;
+        std::swap(in, out);
+    }
+    if (in != data) std::memcpy(data, in, sizeof(cplx) * n);
+}
+
+extern "C" int ora_fft1d(double* d, int n, int sign)
+{
+    if (n < 1) return -1;
+    std::vector<cplx> w(n);
+    fft_line(reinterpret_cast<cplx*>(d), n, sign, w.data());
+    return 0;
+}
+
+extern "C" void ora_dft1d_naive(const double* in, double* out, int n, int sign)
+{
+    const cplx* a = reinterpret_cast<const cplx*>(in);
+    cplx* b = reinterpret_cast<cplx*>(out);
+    for (int k = 0; k < n; ++k) {
+        cplx s = 0;
+        for (int j = 0; j < n; ++j) s += a[j] * root((long long)j * k, n, sign);
+        b[k] = s;
+    }
+}
+
+/* per-layer 2D C2C (Array::fft2DTo / ifft2DTo semantics), index x + nx*(y + ny*z) */
+static void fft2d_layers(cplx* d, int nx, int ny, int nz, int sign)
+{
+#pragma omp parallel
+    {
+        std::vector<cplx> line(std::max(nx, ny)), work(std::max(nx, ny));
+#pragma omp for collapse(2) schedule(static)
+        for (int z = 0; z < nz; ++z)
+            for (int y = 0; y < ny; ++y)
+                fft_line(d + (size_t)nx * (y + (size_t)ny * z), nx, sign, work.data());
+#pragma omp for collapse(2) schedule(static)
+        for (int z = 0; z < nz; ++z)
+            for (int x = 0; x < nx; ++x) {
+                cplx* L = d + (size_t)nx * ny * z + x;
+                for (int y = 0; y < ny; ++y) line[y] = L[(size_t)nx * y];
+                fft_line(line.data(), ny, sign, work.data());
+                for (int y = 0; y < ny; ++y) L[(size_t)nx * y] = line[y];
+            }
+    }
+}
+
+extern "C" int ora_fft2d_layers(double* d, int nx, int ny, int nz, int sign)
+{
+    if (nx < 1 || ny < 1 || nz < 1) return -1;
+    fft2d_layers(reinterpret_cast<cplx*>(d), nx, ny, nz, sign);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Dipole tensor                                                             */
+/* ------------------------------------------------------------------------ */
+/* Gamma_ab(r) = 3 r_a r_b / r^5 - delta_ab / r^3, zero at r < 1e-40 */
+static inline void gamma_dip(double rx, double ry, double rz, double g[6])
+{
+    const double r = std::sqrt(rx * rx + ry * ry + rz * rz);
+    if (r < 1e-40) { for (int i = 0; i < 6; ++i) g[i] = 0; return; }
+    const double ir = 1.0 / r, ir3 = ir * ir * ir, ir5 = ir3 * ir * ir;
+    g[0] = -1.0 * (ir3 - 3.0 * rx * rx * ir5);
+    g[1] = -1.0 * (-3.0 * rx * ry * ir5);
+    g[2] = -1.0 * (-3.0 * rx * rz * ir5);
+    g[3] = -1.0 * (ir3 - 3.0 * ry * ry * ir5);
+    g[4] = -1.0 * (-3.0 * ry * rz * ir5);
+    g[5] = -1.0 * (ir3 - 3.0 * rz * rz * ir5);
+}
+
+/* floor/ceil division for the image range */
+static inline long long fdiv(long long a, long long b) { long long q = a / b; if ((a % b) && ((a < 0) != (b < 0))) --q; return q; }
+static inline long long cdiv(long long a, long long b) { return -fdiv(-a, b); }
+
+/*
+ * One tensor element at in-plane offset (ix,iy) and layer offset iz: the sum of
+ * Gamma over periodic images X,Y in [-(gmax0-1), gmax0-1], visited in ascending
+ * X then Y order, keeping |xx| <= min(gmax[2],gmax[0]), |yy| <= min(gmax[3],gmax[0])
+ * and xx^2+yy^2+zz^2 <= gmax[0]^2.  The reference's z test reads gmax[4], one past
+ * the array; no z cut is applied here beyond the radial one.
+ */
+static void wab(const double* ABC, int nA, int nB, int ix, int iy, int iz, const int* gm, double acc[6])
+{
+    for (int i = 0; i < 6; ++i) acc[i] = 0;
+    const long long smax = gm[0];
+    const long long zz = iz;
+    if (std::llabs(zz) > gm[0]) return;
+    const long long limx = std::min(gm[2], gm[0]);
+    const long long limy = std::min(gm[3], gm[0]);
+    long long x0 = std::max(-(smax - 1), cdiv(-limx - ix, nA));
+    long long x1 = std::min(smax - 1, fdiv(limx - ix, nA));
+    long long y0 = std::max(-(smax - 1), cdiv(-limy - iy, nB));
+    long long y1 = std::min(smax - 1, fdiv(limy - iy, nB));
+    const double r2max = (double)gm[0] * (double)gm[0];
+    for (long long X = x0; X <= x1; ++X) {
+        const long long xx = X * nA + ix;
+        for (long long Y = y0; Y <= y1; ++Y) {
+            const long long yy = Y * nB + iy;
+            if ((double)xx * xx + (double)yy * yy + (double)zz * zz <= r2max) {
+                const double rx = (double)xx * ABC[0] + (double)yy * ABC[3] + (double)zz * ABC[6];
+                const double ry = (double)xx * ABC[1] + (double)yy * ABC[4] + (double)zz * ABC[7];
+                const double rz = (double)xx * ABC[2] + (double)yy * ABC[5] + (double)zz * ABC[8];
+                double g[6];
+                gamma_dip(rx, ry, rz, g);
+                for (int i = 0; i < 6; ++i) acc[i] += g[i];
+            }
+        }
+    }
+}
+
+extern "C" int ora_dipole_tensor(int nx, int ny, int nz, const int gmax[4], const double ABC[9],
+                                 double* XX, double* XY, double* XZ, double* YY, double* YZ, double* ZZ)
+{
+    if (nx < 1 || ny < 1 || nz < 1 || gmax[0] < 1) return -1;
+    const int nl = 2 * nz - 1;
+    double* out[6] = {XX, XY, XZ, YY, YZ, ZZ};
+#pragma omp parallel for collapse(3) schedule(dynamic, 16)
+    for (int c = 0; c < nl; ++c)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const int k = c - nz + 1;
+                double acc[6];
+                wab(ABC, nx, ny, i, j, k, gmax, acc);
+                const size_t idx = (size_t)i + (size_t)nx * (j + (size_t)ny * c);
+                for (int a = 0; a < 6; ++a) out[a][idx] = acc[a];
+            }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* SpinSystem                                                                */
+/* ------------------------------------------------------------------------ */
+struct ora_sys {
+    int nx, ny, nz, n;
+    std::vector<double> x, y, z, ms;
+    double alpha = 1, gamma = 1, dt = 1, time = 0;
+    std::vector<double> site_alpha, site_gamma;
+    std::vector<std::string> names;
+    std::vector<std::vector<double>> hx, hy, hz;
+    std::vector<char> used;
+
+    ora_sys(int a, int b, int c) : nx(a), ny(b), nz(c), n(a * b * c),
+        x(n, 0.0), y(n, 0.0), z(n, 0.0), ms(n, 0.0) { slot("Total"); }
+
+    int find(const char* nm) const {
+        for (size_t i = 0; i < names.size(); ++i)
+            if (strcasecmp(names[i].c_str(), nm) == 0) return (int)i;
+        return -1;
+    }
+    int slot(const char* nm) {
+        int s = find(nm);
+        if (s >= 0) return s;
+        names.push_back(nm);
+        hx.emplace_back(n, 0.0); hy.emplace_back(n, 0.0); hz.emplace_back(n, 0.0);
+        used.push_back(0);
+        return (int)names.size() - 1;
+    }
+    int mark(const char* nm) { int s = slot(nm); used[s] = 1; return s; }
+    double a(int i) const { return site_alpha.empty() ? alpha : site_alpha[i]; }
+    double g(int i) const { return site_gamma.empty() ? gamma : site_gamma[i]; }
+};
+
+extern "C" {
+
+ora_sys* ora_sys_new(int nx, int ny, int nz) { return (nx > 0 && ny > 0 && nz > 0) ? new ora_sys(nx, ny, nz) : nullptr; }
+void ora_sys_free(ora_sys* s) { delete s; }
+ora_sys* ora_sys_copy(const ora_sys* s) { return new ora_sys(*s); }
+
+void ora_sys_copy_spins_to(const ora_sys* s, ora_sys* d)
+{
+    d->x = s->x; d->y = s->y; d->z = s->z; d->ms = s->ms;
+}
+
+/* SpinSystem::set (cpu/core/spinsystem.cpp:1051-1059): ms = |s| unless given */
+void ora_sys_set_spins(ora_sys* s, const double* x, const double* y, const double* z)
+{
+    for (int i = 0; i < s->n; ++i) {
+        s->x[i] = x[i]; s->y[i] = y[i]; s->z[i] = z[i];
+        s->ms[i] = std::sqrt(x[i] * x[i] + y[i] * y[i] + z[i] * z[i]);
+    }
+}
+
+void ora_sys_set_spin(ora_sys* s, int i, double sx, double sy, double sz, double m)
+{
+    const double len = std::sqrt(sx * sx + sy * sy + sz * sz);
+    if (m < 0) m = len;
+    if (len > 0) { sx *= m / len; sy *= m / len; sz *= m / len; }
+    s->x[i] = sx; s->y[i] = sy; s->z[i] = sz; s->ms[i] = m;
+}
+
+void ora_sys_get_spins(const ora_sys* s, double* x, double* y, double* z, double* ms)
+{
+    for (int i = 0; i < s->n; ++i) {
+        if (x) x[i] = s->x[i];
+        if (y) y[i] = s->y[i];
+        if (z) z[i] = s->z[i];
+        if (ms) ms[i] = s->ms[i];
+    }
+}
+
+void ora_sys_set_params(ora_sys* s, double alpha, double gamma, double dt, double time)
+{
+    s->alpha = alpha; s->gamma = gamma; s->dt = dt; s->time = time;
+}
+
+void ora_sys_get_params(const ora_sys* s, double out[4])
+{
+    out[0] = s->alpha; out[1] = s->gamma; out[2] = s->dt; out[3] = s->time;
+}
+
+void ora_sys_set_site_alpha(ora_sys* s, const double* a)
+{
+    if (!a) s->site_alpha.clear(); else s->site_alpha.assign(a, a + s->n);
+}
+
+void ora_sys_set_site_gamma(ora_sys* s, const double* g)
+{
+    if (!g) s->site_gamma.clear(); else s->site_gamma.assign(g, g + s->n);
+}
+
+int ora_sys_slot(const ora_sys* s, const char* name) { return s->find(name); }
+
+int ora_sys_get_field(const ora_sys* s, const char* name, double* hx, double* hy, double* hz)
+{
+    const int k = s->find(name);
+    if (k < 0) return -1;
+    std::memcpy(hx, s->hx[k].data(), sizeof(double) * s->n);
+    std::memcpy(hy, s->hy[k].data(), sizeof(double) * s->n);
+    std::memcpy(hz, s->hz[k].data(), sizeof(double) * s->n);
+    return 0;
+}
+
+/* register + fill a slot and mark it used (SpinSystem::setFeildArray, spinsystem.cpp:585-600) */
+int ora_sys_set_field(ora_sys* s, const char* name, const double* hx, const double* hy, const double* hz)
+{
+    const int k = s->mark(name);
+    std::memcpy(s->hx[k].data(), hx, sizeof(double) * s->n);
+    std::memcpy(s->hy[k].data(), hy, sizeof(double) * s->n);
+    std::memcpy(s->hz[k].data(), hz, sizeof(double) * s->n);
+    return 0;
+}
+
+/* SpinSystem::zeroFields, cpu/core/spinsystem.cpp:968-980 */
+void ora_sys_reset_fields(ora_sys* s)
+{
+    for (size_t k = 0; k < s->names.size(); ++k) {
+        s->used[k] = 0;
+        std::fill(s->hx[k].begin(), s->hx[k].end(), 0.0);
+        std::fill(s->hy[k].begin(), s->hy[k].end(), 0.0);
+        std::fill(s->hz[k].begin(), s->hz[k].end(), 0.0);
+    }
+}
+
+/* SpinSystem::sumFields, cpu/core/spinsystem.cpp:853-871: Total = sum of used slots, in slot order */
+void ora_sys_sum_fields(ora_sys* s)
+{
+    const int T = s->slot("Total");
+    std::vector<double>* H[3] = {&s->hx[T], &s->hy[T], &s->hz[T]};
+    for (int c = 0; c < 3; ++c) {
+        std::vector<double>& d = *H[c];
+        std::fill(d.begin(), d.end(), 0.0);
+        for (size_t k = 1; k < s->names.size(); ++k) {
+            if ((int)k == T || !s->used[k]) continue;
+            const std::vector<double>& src = c == 0 ? s->hx[k] : (c == 1 ? s->hy[k] : s->hz[k]);
+            for (int i = 0; i < s->n; ++i) d[i] = 1.0 * d[i] + 1.0 * src[i];
+        }
+    }
+}
+
+void ora_sys_net_moment(const ora_sys* s, double out[8])
+{
+    double a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+    const std::vector<double>* S[3] = {&s->x, &s->y, &s->z};
+    for (int c = 0; c < 3; ++c)
+        for (int i = 0; i < s->n; ++i) { const double v = (*S[c])[i]; a[c] += v; b[c] += v * v; }
+    out[0] = a[0]; out[1] = a[1]; out[2] = a[2];
+    out[3] = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    out[4] = b[0]; out[5] = b[1]; out[6] = b[2];
+    out[7] = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+}
+
+double ora_sys_field_dot(const ora_sys* s, const char* name)
+{
+    const int k = s->find(name);
+    if (k < 0) return 0;
+    double e = 0;
+    for (int i = 0; i < s->n; ++i)
+        e += s->hx[k][i] * s->x[i] + s->hy[k][i] * s->y[i] + s->hz[k][i] * s->z[i];
+    return e;
+}
+
+} /* extern "C" */
+
+/* ------------------------------------------------------------------------ */
+/* Exchange                                                                  */
+/* ------------------------------------------------------------------------ */
+struct ora_ex {
+    int nx, ny, nz;
+    int pbc[3] = {1, 1, 1};
+    bool normalize = true;
+    double scale = 1;
+    struct path { int from, to; double J; };
+    std::vector<path> paths;
+};
+
+extern "C" {
+ora_ex* ora_ex_new(int nx, int ny, int nz) { ora_ex* e = new ora_ex; e->nx = nx; e->ny = ny; e->nz = nz; return e; }
+void ora_ex_free(ora_ex* e) { delete e; }
+void ora_ex_set_pbc(ora_ex* e, int px, int py, int pz) { e->pbc[0] = px; e->pbc[1] = py; e->pbc[2] = pz; }
+void ora_ex_set_normalize(ora_ex* e, int y) { e->normalize = y != 0; }
+void ora_ex_set_scale(ora_ex* e, double s) { e->scale = s; }
+
+void ora_ex_add_path(ora_ex* e, int f, int t, double J)
+{
+    if (J != 0) e->paths.push_back({f, t, J});
+}
+
+void ora_ex_add_paths(ora_ex* e, const int* f, const int* t, const double* J, int count)
+{
+    for (int i = 0; i < count; ++i) ora_ex_add_path(e, f[i], t[i], J[i]);
+}
+
+/* l_addpath, spinoperationexchange.cpp:368-429 (coordinates already 0-based) */
+int ora_ex_add(ora_ex* e, int ax, int ay, int az, int bx, int by, int bz, double J)
+{
+    int a[3] = {ax, ay, az}, b[3] = {bx, by, bz}, n[3] = {e->nx, e->ny, e->nz};
+    for (int i = 0; i < 3; ++i) {
+        if (e->pbc[i]) {
+            a[i] = ((a[i] % n[i]) + n[i]) % n[i];
+            b[i] = ((b[i] % n[i]) + n[i]) % n[i];
+        }
+        if (a[i] < 0 || a[i] >= n[i] || b[i] < 0 || b[i] >= n[i]) return 0;
+    }
+    ora_ex_add_path(e, a[0] + n[0] * (a[1] + n[1] * a[2]), b[0] + n[0] * (b[1] + n[1] * b[2]), J);
+    return 1;
+}
+
+/* Exchange::apply: three passes over the path list, accumulating */
+int ora_ex_apply(ora_ex* e, ora_sys* s)
+{
+    const int k = s->mark("Exchange");
+    std::vector<double>* H[3] = {&s->hx[k], &s->hy[k], &s->hz[k]};
+    const std::vector<double>* S[3] = {&s->x, &s->y, &s->z};
+    const std::vector<double>& m = s->ms;
+    for (int c = 0; c < 3; ++c) {
+        std::vector<double>& h = *H[c];
+        const std::vector<double>& sv = *S[c];
+        for (const ora_ex::path& p : e->paths) {
+            if (!m[p.from]) continue;
+            if (e->normalize) h[p.to] += sv[p.from] * p.J * e->scale / (m[p.from] * m[p.to]);
+            else              h[p.to] += sv[p.from] * p.J * e->scale / m[p.from];
+        }
+    }
+    return 0;
+}
+} /* extern "C" */
+
+/* ------------------------------------------------------------------------ */
+/* Anisotropy                                                                */
+/* ------------------------------------------------------------------------ */
+struct ora_ani {
+    int nx, ny, nz;
+    double scale = 1;
+    struct op { int site; double ax, ay, az, K; };
+    std::vector<op> ops;
+};
+
+extern "C" {
+ora_ani* ora_ani_new(int nx, int ny, int nz) { ora_ani* a = new ora_ani; a->nx = nx; a->ny = ny; a->nz = nz; return a; }
+void ora_ani_free(ora_ani* a) { delete a; }
+void ora_ani_set_scale(ora_ani* a, double s) { a->scale = s; }
+
+/* l_add: the axis is normalised; a zero axis adds nothing */
+int ora_ani_add(ora_ani* a, int site, double ax, double ay, double az, double K)
+{
+    const double len = std::sqrt(ax * ax + ay * ay + az * az);
+    if (!(len > 0)) return 0;
+    a->ops.push_back({site, ax / len, ay / len, az / len, K});
+    return 1;
+}
+
+void ora_ani_add_many(ora_ani* a, const int* site, const double* axes, const double* K, int count)
+{
+    for (int i = 0; i < count; ++i) ora_ani_add(a, site[i], axes[3 * i], axes[3 * i + 1], axes[3 * i + 2], K[i]);
+}
+
+int ora_ani_apply(ora_ani* a, ora_sys* s)
+{
+    const int k = s->mark("Anisotropy");
+    std::vector<double>& hx = s->hx[k];
+    std::vector<double>& hy = s->hy[k];
+    std::vector<double>& hz = s->hz[k];
+    std::fill(hx.begin(), hx.end(), 0.0);
+    std::fill(hy.begin(), hy.end(), 0.0);
+    std::fill(hz.begin(), hz.end(), 0.0);
+    for (const ora_ani::op& o : a->ops) {
+        const int i = o.site;
+        const double ms = s->ms[i];
+        if (ms > 0) {
+            const double sd = s->x[i] * o.ax + s->y[i] * o.ay + s->z[i] * o.az;
+            const double v = 2.0 * o.K * sd / (ms * ms);
+            hx[i] += o.ax * v * a->scale;
+            hy[i] += o.ay * v * a->scale;
+            hz[i] += o.az * v * a->scale;
+        }
+    }
+    return 0;
+}
+} /* extern "C" */
+
+/* ------------------------------------------------------------------------ */
+/* Thermal                                                                   */
+/* ------------------------------------------------------------------------ */
+struct ora_th {
+    int nx, ny, nz;
+    double T = 0, scale = 1;
+    std::vector<double> site_scale;
+};
+
+extern "C" {
+ora_th* ora_th_new(int nx, int ny, int nz)
+{
+    ora_th* t = new ora_th; t->nx = nx; t->ny = ny; t->nz = nz;
+    t->site_scale.assign((size_t)nx * ny * nz, 1.0);
+    return t;
+}
+void ora_th_free(ora_th* t) { delete t; }
+void ora_th_set_temperature(ora_th* t, double T) { t->T = T; }
+void ora_th_set_scale(ora_th* t, double s) { t->scale = s; }
+void ora_th_scale_site(ora_th* t, int site, double v) { t->site_scale[site] = v; }
+
+int ora_th_apply(ora_th* t, ora_sys* s, const double* normals, uint64_t seed, uint64_t step)
+{
+    const int k = s->mark("Thermal");
+    const int n = s->n;
+    for (int i = 0; i < n; ++i) {
+        double g3[3];
+        if (normals) { g3[0] = normals[i]; g3[1] = normals[n + i]; g3[2] = normals[2 * n + i]; }
+        else ora_thermal_normals(seed, step, i, g3);
+        double hx = 0, hy = 0, hz = 0;
+        if (s->ms[i] != 0 && t->T != 0) {
+            const double al = s->a(i), ga = s->g(i);
+            if (ga > 0) {
+                const double sd = std::sqrt((2.0 * al * t->scale * t->T * t->site_scale[i]) / (s->ms[i] * s->dt * ga));
+                hx = sd * g3[0]; hy = sd * g3[1]; hz = sd * g3[2];
+            }
+        }
+        s->hx[k][i] = hx; s->hy[k][i] = hy; s->hz[k][i] = hz;
+    }
+    return 0;
+}
+} /* extern "C" */
+
+/* ------------------------------------------------------------------------ */
+/* LongRange / Dipole (layered 2D FFT convolution)                           */
+/* ------------------------------------------------------------------------ */
+struct ora_lr {
+    int nx, ny, nz;
+    double g = 1, scale = 1;
+    int gmax[4] = {1000, 1000, 1000, 1000};
+    double ABC[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    bool loaded = false, fresh = false;
+    std::vector<double> T[6];   /* real space [nx*ny*(2nz-1)] */
+    std::vector<cplx> Q[6];     /* prescaled spectra */
+};
+
+extern "C" {
+ora_lr* ora_lr_new(int nx, int ny, int nz) { ora_lr* l = new ora_lr; l->nx = nx; l->ny = ny; l->nz = nz; return l; }
+void ora_lr_free(ora_lr* l) { delete l; }
+void ora_lr_set_strength(ora_lr* l, double g) { l->g = g; }
+void ora_lr_set_scale(ora_lr* l, double s) { l->scale = s; }
+void ora_lr_set_truncation(ora_lr* l, const int gm[4]) { for (int i = 0; i < 4; ++i) l->gmax[i] = gm[i]; l->loaded = false; }
+void ora_lr_set_unit_cell(ora_lr* l, const double A[9]) { for (int i = 0; i < 9; ++i) l->ABC[i] = A[i]; l->loaded = false; }
+}
+
+static void lr_load(ora_lr* l)
+{
+    if (l->loaded) return;
+    const size_t m = (size_t)l->nx * l->ny * (2 * l->nz - 1);
+    for (int a = 0; a < 6; ++a) l->T[a].assign(m, 0.0);
+    ora_dipole_tensor(l->nx, l->ny, l->nz, l->gmax, l->ABC, l->T[0].data(), l->T[1].data(), l->T[2].data(),
+                      l->T[3].data(), l->T[4].data(), l->T[5].data());
+    l->loaded = true;
+    l->fresh = true;
+}
+
+/* LongRange::updateData: tensor spectra, prescaled by 1/(nx*ny) */
+static void lr_update(ora_lr* l)
+{
+    lr_load(l);
+    if (!l->fresh) return;
+    l->fresh = false;
+    const size_t m = l->T[0].size();
+    const double inv = 1.0 / (double)(l->nx * l->ny);
+    for (int a = 0; a < 6; ++a) {
+        l->Q[a].assign(m, 0.0);
+        for (size_t i = 0; i < m; ++i) l->Q[a][i] = cplx(l->T[a][i], 0.0);
+        fft2d_layers(l->Q[a].data(), l->nx, l->ny, 2 * l->nz - 1, -1);
+        for (size_t i = 0; i < m; ++i) l->Q[a][i] *= cplx(inv, 0);
+    }
+}
+
+extern "C" {
+int ora_lr_set_tensor(ora_lr* l, int which, const double* t)
+{
+    if (which < 0 || which > 5) return -1;
+    lr_load(l);
+    std::memcpy(l->T[which].data(), t, sizeof(double) * l->T[which].size());
+    l->fresh = true;
+    return 0;
+}
+
+int ora_lr_get_tensor(ora_lr* l, int which, double* t)
+{
+    if (which < 0 || which > 5) return -1;
+    lr_load(l);
+    std::memcpy(t, l->T[which].data(), sizeof(double) * l->T[which].size());
+    return 0;
+}
+
+/* LongRange::apply: layer offset = source - target + nz - 1, H = g*scale*Re(IFFT(sum T^ M^)) */
+int ora_lr_apply(ora_lr* l, ora_sys* s)
+{
+    const int k = s->mark("Dipole");
+    lr_update(l);
+    const int nx = l->nx, ny = l->ny, nz = l->nz;
+    const size_t nxy = (size_t)nx * ny, n = nxy * nz;
+    std::vector<cplx> q[3];
+    const std::vector<double>* S[3] = {&s->x, &s->y, &s->z};
+    for (int c = 0; c < 3; ++c) {
+        q[c].resize(n);
+        for (size_t i = 0; i < n; ++i) q[c][i] = cplx((*S[c])[i], 0.0);
+        fft2d_layers(q[c].data(), nx, ny, nz, -1);
+    }
+    /* component table: row a of the symmetric tensor as indices into Q */
+    static const int row[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    std::vector<double>* H[3] = {&s->hx[k], &s->hy[k], &s->hz[k]};
+    std::vector<cplx> ws(n);
+    for (int a = 0; a < 3; ++a) {
+        std::fill(ws.begin(), ws.end(), cplx(0, 0));
+#pragma omp parallel for schedule(static)
+        for (int t = 0; t < nz; ++t)
+            for (int sl = 0; sl < nz; ++sl) {
+                const size_t off = (size_t)(sl - t + nz - 1);
+                for (int b = 0; b < 3; ++b) {
+                    const cplx* T = l->Q[row[a][b]].data() + off * nxy;
+                    const cplx* M = q[b].data() + (size_t)sl * nxy;
+                    cplx* D = ws.data() + (size_t)t * nxy;
+                    for (size_t i = 0; i < nxy; ++i) D[i] += T[i] * M[i];
+                }
+            }
+        fft2d_layers(ws.data(), nx, ny, nz, +1);
+        std::vector<double>& h = *H[a];
+        for (size_t i = 0; i < n; ++i) h[i] = ws[i].real() * (l->g * l->scale);
+    }
+    return 0;
+}
+} /* extern "C" */
+
+/* ------------------------------------------------------------------------ */
+/* LLG.Cartesian                                                             */
+/* ------------------------------------------------------------------------ */
+struct ora_llg {
+    bool thermal_first = true;
+    bool no_renorm = false;
+    double precess_scale = 1, damp_scale = 1;
+};
+
+extern "C" {
+ora_llg* ora_llg_new(void) { return new ora_llg; }
+void ora_llg_free(ora_llg* l) { delete l; }
+void ora_llg_set_flags(ora_llg* l, int tf, int nr) { l->thermal_first = tf != 0; l->no_renorm = nr != 0; }
+
+/*
+ * to = N(from + scale*dt*dM(dmdt)), dM = -g/(1+a^2) M x H - a/ms g/(1+a^2) M x (M x h),
+ * h = H - H_thermal when thermal_first; renormalised to |from| unless disabled.
+ * Spins M, fields H, alpha, gamma and dt come from dmdt; ms from "from".
+ */
+int ora_llg_apply(ora_llg* l, const ora_sys* from, double scale, const ora_sys* dmdt, ora_sys* to, int advance)
+{
+    const int S = dmdt->find("Total");
+    if (S < 0) {
+        if (advance) to->time = from->time + scale * dmdt->dt;
+        return 0;
+    }
+    const int TH = dmdt->find("Thermal");
+    const int n = from->n;
+    const double dt = scale * dmdt->dt;
+    /* from and to may alias: copy what is read per site first */
+    if (dt == 0) {
+        if (from != to) { to->x = from->x; to->y = from->y; to->z = from->z; }
+    } else {
+        for (int i = 0; i < n; ++i) {
+            const double msf = from->ms[i];
+            to->ms[i] = msf;
+            if (!(msf > 0)) continue;
+            const double M[3] = {dmdt->x[i], dmdt->y[i], dmdt->z[i]};
+            const double H[3] = {dmdt->hx[S][i], dmdt->hy[S][i], dmdt->hz[S][i]};
+            double h[3] = {H[0], H[1], H[2]};
+            double MH[3], MMH[3];
+            if (l->thermal_first && TH >= 0) {
+                h[0] = H[0] - dmdt->hx[TH][i];
+                h[1] = H[1] - dmdt->hy[TH][i];
+                h[2] = H[2] - dmdt->hz[TH][i];
+            }
+            double ps = 1, ds = 1;
+            if (l->thermal_first) {
+                MH[0] = M[1] * h[2] - M[2] * h[1];
+                MH[1] = M[2] * h[0] - M[0] * h[2];
+                MH[2] = M[0] * h[1] - M[1] * h[0];
+                MMH[0] = M[1] * MH[2] - M[2] * MH[1];
+                MMH[1] = M[2] * MH[0] - M[0] * MH[2];
+                MMH[2] = M[0] * MH[1] - M[1] * MH[0];
+                ps = l->precess_scale; ds = l->damp_scale;
+            } else {
+                MH[0] = M[1] * H[2] - M[2] * H[1];
+                MH[1] = M[2] * H[0] - M[0] * H[2];
+                MH[2] = M[0] * H[1] - M[1] * H[0];
+                MMH[0] = M[1] * MH[2] - M[2] * MH[1];
+                MMH[1] = M[2] * MH[0] - M[0] * MH[2];
+                MMH[2] = M[0] * MH[1] - M[1] * MH[0];
+            }
+            if (l->thermal_first) {
+                MH[0] = M[1] * H[2] - M[2] * H[1];
+                MH[1] = M[2] * H[0] - M[0] * H[2];
+                MH[2] = M[0] * H[1] - M[1] * H[0];
+            }
+            const double al = dmdt->a(i), ga = dmdt->g(i);
+            const double gaa = ga / (1.0 + al * al);
+            double dM[3];
+            for (int c = 0; c < 3; ++c) {
+                if (l->thermal_first) dM[c] = ps * -gaa * MH[c] - ds * (al / msf) * gaa * MMH[c];
+                else                  dM[c] = -gaa * MH[c] - (al / msf) * gaa * MMH[c];
+            }
+            const double fx = from->x[i], fy = from->y[i], fz = from->z[i];
+            to->x[i] = fx + dt * dM[0];
+            to->y[i] = fy + dt * dM[1];
+            to->z[i] = fz + dt * dM[2];
+        }
+        if (advance) to->time = from->time + dt;
+    }
+    if (!l->no_renorm) {
+        for (int i = 0; i < n; ++i) {
+            const double mm = to->x[i] * to->x[i] + to->y[i] * to->y[i] + to->z[i] * to->z[i];
+            if (mm != 0) {
+                const double inv = from->ms[i] / std::sqrt(mm);
+                to->x[i] *= inv; to->y[i] *= inv; to->z[i] *= inv;
+            }
+        }
+    }
+    return 0;
+}
+} /* extern "C" */
+
+/* ------------------------------------------------------------------------ */
+/* Integrators                                                               */
+/* ------------------------------------------------------------------------ */
+struct ora_sim {
+    ora_sys* ss;
+    ora_ex* ex; ora_ani* ani; ora_lr* lr; ora_th* th; ora_llg* llg;
+    uint64_t seed = 0, step = 0;
+    ora_sys* P = nullptr;
+    ora_sys* C = nullptr;
+    std::vector<ora_sys*> K;
+};
+
+extern "C" {
+ora_sim* ora_sim_new(ora_sys* ss, ora_ex* ex, ora_ani* ani, ora_lr* lr, ora_th* th, ora_llg* llg)
+{
+    ora_sim* s = new ora_sim;
+    s->ss = ss; s->ex = ex; s->ani = ani; s->lr = lr; s->th = th; s->llg = llg;
+    return s;
+}
+
+void ora_sim_free(ora_sim* s)
+{
+    delete s->P; delete s->C;
+    for (ora_sys* k : s->K) delete k;
+    delete s;
+}
+
+void ora_sim_set_seed(ora_sim* s, uint64_t seed) { s->seed = seed; s->step = 0; }
+
+void ora_sim_set_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
+/* calcField: resetFields; ex, ani, dip apply; sumFields */
+int ora_sim_calc_fields(ora_sim* s, ora_sys* t)
+{
+    ora_sys_reset_fields(t);
+    if (s->ex) ora_ex_apply(s->ex, t);
+    if (s->ani) ora_ani_apply(s->ani, t);
+    if (s->lr) ora_lr_apply(s->lr, t);
+    ora_sys_sum_fields(t);
+    return 0;
+}
+
+static void thermal_kick(ora_sim* s)
+{
+    if (!s->th) return;
+    ora_sys_reset_fields(s->ss);
+    ora_th_apply(s->th, s->ss, nullptr, s->seed, s->step);
+    ora_sys_sum_fields(s->ss);
+    ora_llg_apply(s->llg, s->ss, 1.0, s->ss, s->ss, 0);
+}
+
+/* PredictorCorrector.lua step with a single corrector pass (Heun) */
+int ora_sim_heun(ora_sim* s, int nsteps)
+{
+    ora_sys* ss = s->ss;
+    if (!s->P) { s->P = ora_sys_copy(ss); s->C = ora_sys_copy(ss); }
+    for (int it = 0; it < nsteps; ++it) {
+        ora_sys* P = s->P;
+        ora_sys* C = s->C;
+        P->dt = C->dt = ss->dt; P->alpha = C->alpha = ss->alpha; P->gamma = C->gamma = ss->gamma;
+        P->site_alpha = C->site_alpha = ss->site_alpha; P->site_gamma = C->site_gamma = ss->site_gamma;
+        ora_sim_calc_fields(s, ss);
+        ora_llg_apply(s->llg, ss, 1.0, ss, P, 1);
+        ora_llg_apply(s->llg, ss, 0.5, ss, C, 1);
+        ora_sim_calc_fields(s, P);
+        ora_llg_apply(s->llg, C, 0.5, P, C, 1);
+        ora_sys_copy_spins_to(C, ss);
+        ss->time = C->time;
+        thermal_kick(s);
+        s->step++;
+    }
+    return 0;
+}
+
+/* RungeKutta.lua Butcher tables (c column omitted), stored row-major lower triangle */
+static bool butcher(const char* name, std::vector<std::vector<double>>& t)
+{
+    std::string n(name);
+    if (n == "RK1") t = {{1}};
+    else if (n == "RK2") t = {{0.5}, {0, 1}};
+    else if (n == "K3") t = {{0.5}, {-1, 2}, {1.0 / 6, 2.0 / 3, 1.0 / 6}};
+    else if (n == "RK4") t = {{0.5}, {0, 0.5}, {0, 0, 1}, {1.0 / 6, 1.0 / 3, 1.0 / 3, 1.0 / 6}};
+    else if (n == "RK3") t = {{1.0 / 3}, {0, 2.0 / 3}, {0.25, 0, 0.75}};
+    else if (n == "RK4_38") t = {{1.0 / 3}, {-1.0 / 3, 1}, {1, -1, 1}, {1.0 / 8, 3.0 / 8, 3.0 / 8, 1.0 / 8}};
+    else if (n == "RK6") t = {{1.0 / 3}, {0, 2.0 / 3}, {1.0 / 12, 4.0 / 12, -1.0 / 12}, {-1.0 / 16, 18.0 / 16, -3.0 / 16, -6.0 / 16},
+                              {0, 9.0 / 8, -3.0 / 8, -6.0 / 8, 4.0 / 8}, {9.0 / 44, -36.0 / 44, 63.0 / 44, 72.0 / 44, 0, -64.0 / 44},
+                              {11.0 / 120, 0, 81.0 / 120, 81.0 / 120, -32.0 / 120, -32.0 / 120, 11.0 / 120}};
+    else if (n == "BS3") t = {{0.5}, {0, 0.75}, {2.0 / 9, 1.0 / 3, 4.0 / 9}, {7.0 / 24, 0.25, 1.0 / 3, 1.0 / 8}};
+    else return false;
+    return true;
+}
+
+int ora_sim_rk(ora_sim* s, const char* name, int nsteps)
+{
+    std::vector<std::vector<double>> bt;
+    if (!butcher(name, bt)) return -1;
+    const int nstep = (int)bt.size();
+    if ((int)s->K.size() != nstep) {
+        for (ora_sys* k : s->K) delete k;
+        s->K.assign(nstep, nullptr);
+        for (int i = 1; i < nstep; ++i) s->K[i] = ora_sys_copy(s->ss);
+    }
+    std::vector<ora_sys*> k = s->K;
+    k[0] = s->ss;
+    auto istep = [&](ora_sys* src, const std::vector<double>& f, ora_sys* dst) {
+        ora_llg_apply(s->llg, src, f[0], src, dst, 1);
+        for (int i = 1; i < nstep; ++i)
+            if (i < (int)f.size() && f[i] != 0) ora_llg_apply(s->llg, dst, f[i], k[i], dst, 1);
+    };
+    for (int it = 0; it < nsteps; ++it) {
+        for (int i = 1; i < nstep; ++i) {
+            k[i]->alpha = k[0]->alpha; k[i]->gamma = k[0]->gamma;
+            k[i]->site_alpha = k[0]->site_alpha; k[i]->site_gamma = k[0]->site_gamma;
+        }
+        ora_sim_calc_fields(s, k[0]);
+        for (int i = 0; i < nstep - 1; ++i) {
+            k[i + 1]->dt = k[0]->dt;
+            istep(k[0], bt[i], k[i + 1]);
+            ora_sim_calc_fields(s, k[i + 1]);
+        }
+        istep(k[0], bt[nstep - 1], k[0]);
+        thermal_kick(s);
+        s->step++;
+    }
+    return 0;
+}
+} /* extern "C" */

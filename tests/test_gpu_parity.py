@@ -1,0 +1,393 @@
+"""GPU parity: libmsd.so kernels (through the C ABI) against the FP64 oracle on
+the same seeded inputs.  Floating-point tolerances are stated per test; the
+integer Philox stream is bit-exact."""
+import numpy as np
+import pytest
+
+import maglua_amd as M
+from maglua_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def flat(t):
+    return np.stack([np.asarray(v).reshape(-1) for v in t])
+
+
+def twin(spec, seed=None, dipole=True, thermal=None):
+    """The same workload on the GPU and on the oracle, from the same FP64 state."""
+    g = W.build(M, spec, with_dipole=dipole, with_thermal=thermal)
+    o = W.build(O, spec, with_dipole=dipole, with_thermal=thermal)
+    x, y, z = O.random_unit_spins(spec.seed if seed is None else seed, spec.n)
+    g["ss"].setSpins(x, y, z)
+    o["ss"].setSpins(x, y, z)
+    return g, o
+
+
+# ------------------------------------------------------------------- RNG
+def test_philox_device_known_answers():
+    from tests.test_oracle import KAT
+    for ctr, key, want in KAT:
+        out = M.philox_device(np.array([ctr], dtype=np.uint32), key)
+        assert tuple(int(v) for v in out[0]) == want
+
+
+def test_philox_device_matches_oracle_stream():
+    rng = np.random.default_rng(0)
+    ctr = rng.integers(0, 2 ** 32, size=(257, 4), dtype=np.uint64).astype(np.uint32)
+    out = M.philox_device(ctr, (123, 456))
+    for i in range(0, 257, 16):
+        assert list(out[i]) == O.philox([int(v) for v in ctr[i]], (123, 456))
+
+
+def test_random_spins_match_oracle():
+    ss = M.SpinSystem(17, 9, 3)
+    ss.randomize(5)
+    x, y, z, m = ss.spins()
+    ox, oy, oz = O.random_unit_spins(5, 17 * 9 * 3)
+    assert np.abs(flat((x, y, z)) - np.stack([ox, oy, oz])).max() < 1e-7
+    assert np.allclose(m, 1.0, atol=1e-7)
+
+
+# ------------------------------------------------------------ operators
+@pytest.mark.parametrize("dims,pbc", [((8, 8, 3), (1, 1, 0)), ((5, 7, 2), (1, 1, 1)), ((16, 1, 1), (1, 0, 0))])
+def test_exchange_op(dims, pbc):
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    rng = np.random.default_rng(1)
+    s = rng.standard_normal((3, n)) * rng.uniform(0.5, 2.0, n)   # non-unit moments exercise normalisation
+    gs, os_ = M.SpinSystem(*dims), O.SpinSystem(*dims)
+    gs.setSpins(*s)
+    os_.setSpins(*s)
+    f, t = W.nn_paths(nx, ny, nz, pbc)
+    J = rng.uniform(-1, 2, f.size)
+    ge, oe = M.Exchange(*dims), O.Exchange(*dims)
+    ge.addPaths(f, t, J)
+    oe.addPaths(f, t, J)
+    ge.apply(gs)
+    oe.apply(os_)
+    hg, ho = flat(gs.field("Exchange")), flat(os_.field("Exchange"))
+    assert np.abs(hg - ho).max() <= 2e-6 * np.abs(ho).max()
+
+
+def test_anisotropy_op_modes():
+    dims = (6, 5, 2)
+    n = 60
+    rng = np.random.default_rng(2)
+    s = rng.standard_normal((3, n))
+    for mode in ("uniform", "per-site", "stacked"):
+        gs, os_ = M.SpinSystem(*dims), O.SpinSystem(*dims)
+        gs.setSpins(*s)
+        os_.setSpins(*s)
+        ga, oa = M.Anisotropy(*dims), O.Anisotropy(*dims)
+        if mode == "uniform":
+            ga.addMany(np.arange(n), (0.2, 0.1, 1.0), 0.3)
+            oa.addMany(np.arange(n), (0.2, 0.1, 1.0), 0.3)
+        else:
+            ax = rng.standard_normal((n, 3))
+            K = rng.uniform(-1, 1, n)
+            ga.addMany(np.arange(n), ax, K)
+            oa.addMany(np.arange(n), ax, K)
+            if mode == "stacked":
+                ga.addMany([3, 3], [(1, 0, 0), (0, 1, 1)], [0.5, -0.2])
+                oa.addMany([3, 3], [(1, 0, 0), (0, 1, 1)], [0.5, -0.2])
+        ga.apply(gs)
+        oa.apply(os_)
+        hg, ho = flat(gs.field("Anisotropy")), flat(os_.field("Anisotropy"))
+        assert np.abs(hg - ho).max() <= 2e-6 * max(1.0, np.abs(ho).max()), mode
+
+
+def test_thermal_op_same_stream_as_oracle():
+    dims = (16, 8, 2)
+    n = 256
+    rng = np.random.default_rng(3)
+    s = rng.standard_normal((3, n))
+    gs, os_ = M.SpinSystem(*dims), O.SpinSystem(*dims)
+    for ss in (gs, os_):
+        ss.setSpins(*s)
+        ss.setAlpha(0.2)
+        ss.setTimeStep(0.01)
+    gt = M.Thermal(*dims, seed=99)
+    gt.setTemperature(0.8)
+    gt.scaleSite((3, 1, 1), 0.5)
+    ot = O.Thermal(*dims)
+    ot.setTemperature(0.8)
+    ot.scaleSite(2, 0.5)
+    for draw in range(3):
+        gt.apply(gs)
+        ot.apply(os_, seed=99, step=draw)
+        hg, ho = flat(gs.field("Thermal")), flat(os_.field("Thermal"))
+        assert np.abs(hg - ho).max() <= 5e-6 * np.abs(ho).max(), draw
+    assert gt.draw() == 3
+
+
+@pytest.mark.parametrize("dims", [(8, 8, 1), (8, 8, 2), (12, 6, 3), (20, 10, 4), (32, 32, 1), (10, 15, 8), (64, 48, 5)])
+def test_dipole_op(dims):
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    rng = np.random.default_rng(nx * 100 + ny)
+    s = rng.standard_normal((3, n))
+    gs, os_ = M.SpinSystem(*dims), O.SpinSystem(*dims, threads=True)
+    gs.setSpins(*s)
+    os_.setSpins(*s)
+    gd, od = M.Dipole(*dims), O.Dipole(*dims, threads=True)
+    for d in (gd, od):
+        d.setStrength(0.3)
+        d.setTruncation(60)
+    gd.apply(gs)
+    od.apply(os_)
+    hg, ho = flat(gs.field("Dipole")), flat(os_.field("Dipole"))
+    # FP32 spectra and transforms against FP64: relative to the field scale
+    assert np.abs(hg - ho).max() <= 2e-5 * np.abs(ho).max()
+
+
+def test_dipole_op_full_size_film():
+    """256x256x8 at the default truncation (1000): one evaluation, FP32 vs FP64."""
+    spec = W.FILM_256_T0
+    g = M.SpinSystem(spec.nx, spec.ny, spec.nz)
+    o = O.SpinSystem(spec.nx, spec.ny, spec.nz, threads=True)
+    x, y, z = O.random_unit_spins(0, spec.n)
+    g.setSpins(x, y, z)
+    o.setSpins(x, y, z)
+    gd = M.Dipole(spec.nx, spec.ny, spec.nz)
+    od = O.Dipole(spec.nx, spec.ny, spec.nz, threads=True)
+    gd.setStrength(spec.g)
+    od.setStrength(spec.g)
+    gd.apply(g)
+    od.apply(o)
+    hg, ho = flat(g.field("Dipole")), flat(o.field("Dipole"))
+    assert np.abs(hg - ho).max() <= 2e-5 * np.abs(ho).max()
+
+
+def test_dipole_matrix_get_set():
+    gd, od = M.Dipole(8, 6, 2), O.Dipole(8, 6, 2)
+    for d in (gd, od):
+        d.setTruncation(30)
+    T = {k: od.tensor(k) for k in od.NAMES}
+    for name, off in (("XX", (1, 0, 0)), ("XZ", (3, 2, -1)), ("YZ", (-1, 2, 1)), ("ZZ", (0, 0, 0))):
+        want = T[name][off[2] + 1, off[1] % 6, off[0] % 8]
+        assert abs(gd.getMatrix(name, off) - want) <= 1e-12 * max(1.0, abs(want))
+    # a custom tensor keeping the lattice-sum parities is used as given
+    s = np.random.default_rng(0).standard_normal((3, 96))
+    gs, os_ = M.SpinSystem(8, 6, 2), O.SpinSystem(8, 6, 2)
+    gs.setSpins(*s)
+    os_.setSpins(*s)
+    for d in (gd,):
+        d.setMatrix("ZZ", (0, 0, 1), 0.5)
+        d.setMatrix("ZZ", (0, 0, -1), 0.5)
+    T["ZZ"][2, 0, 0] = 0.5
+    T["ZZ"][0, 0, 0] = 0.5
+    od.setTensor("ZZ", T["ZZ"])
+    gd.apply(gs)
+    od.apply(os_)
+    hg, ho = flat(gs.field("Dipole")), flat(os_.field("Dipole"))
+    assert np.abs(hg - ho).max() <= 2e-5 * np.abs(ho).max()
+    # a tensor without those parities is rejected, not silently mis-applied
+    gd.setMatrix("XY", (1, 0, 0), 3.0)
+    with pytest.raises(M.MagLuaError):
+        gd.apply(gs)
+
+
+def test_sum_and_reset_fields_semantics():
+    spec = W.FilmSpec(8, 8, 2, T=0.0)
+    g, o = twin(spec)
+    for d in (g, o):
+        ss = d["ss"]
+        ss.resetFields()
+        d["ex"].apply(ss)
+        d["ani"].apply(ss)
+        d["dip"].apply(ss)
+        ss.sumFields()
+    tg, to = flat(g["ss"].field("Total")), flat(o["ss"].field("Total"))
+    assert np.abs(tg - to).max() <= 2e-6 * np.abs(to).max()
+    g["ss"].resetFields()
+    assert not g["ss"].slotUsed("Exchange")
+    assert np.abs(flat(g["ss"].field("Exchange"))).max() == 0
+    g["ss"].sumFields()
+    assert np.abs(flat(g["ss"].field("Total"))).max() == 0
+
+
+def test_llg_op_apply_forms():
+    spec = W.FilmSpec(8, 8, 2, T=0.0, dt=0.01)
+    g, o = twin(spec)
+    for d, api in ((g, M), (o, O)):
+        ss = d["ss"]
+        ss.resetFields()
+        d["ex"].apply(ss)
+        d["ani"].apply(ss)
+        d["dip"].apply(ss)
+        ss.sumFields()
+    P_g, P_o = g["ss"].copy(), o["ss"].copy()
+    g["llg"].apply(g["ss"], 0.5, g["ss"], P_g)          # Lua form apply(from, scale, dmdt, to)
+    o["llg"].apply(o["ss"], 0.5, o["ss"], P_o)
+    assert np.abs(flat(P_g.spins()[:3]) - flat(P_o.spins()[:3])).max() < 1e-6
+    assert abs(P_g.time() - 0.005) < 1e-15
+    g["llg"].apply(g["ss"], False)                      # in place, time not advanced
+    o["llg"].apply(o["ss"], 1.0, o["ss"], o["ss"], False)
+    assert np.abs(flat(g["ss"].spins()[:3]) - flat(o["ss"].spins()[:3])).max() < 1e-6
+    assert g["ss"].time() == 0.0
+
+
+def test_reductions_fp64():
+    ss = M.SpinSystem(64, 32, 3)
+    ss.randomize(2)
+    x, y, z, m = ss.spins()
+    mom = ss.netMoment()
+    want = [x.sum(), y.sum(), z.sum()]
+    assert np.allclose(mom[:3], want, atol=1e-9)
+    assert np.allclose(mom[4:7], [(x * x).sum(), (y * y).sum(), (z * z).sum()], rtol=1e-12)
+    ss2 = ss.copy()
+    assert ss.diff(ss2)[3] == 0.0
+    ss2.setSpin((1, 1, 1), (0, 0, 1))
+    d = ss.diff(ss2)
+    assert abs(d[0] - abs(x[0, 0, 0])) < 1e-7 and abs(d[2] - abs(z[0, 0, 0] - 1)) < 1e-7
+    ex = M.Exchange(ss)
+    f, t = W.nn_paths(64, 32, 3, (1, 1, 1))
+    ex.addPaths(f, t, 1.0)
+    ss.resetFields()
+    ex.apply(ss)
+    hx, hy, hz = ss.field("Exchange")
+    assert abs(ss.fieldDot("Exchange") - (hx * x + hy * y + hz * z).sum()) < 1e-6 * abs((hx * x).sum() + 1)
+
+
+def test_errors_are_raised():
+    a = M.SpinSystem(4, 4, 1)
+    e = M.Exchange(5, 4, 1)
+    with pytest.raises(M.MagLuaError, match="size mismatch"):
+        e.apply(a)
+    with pytest.raises(M.MagLuaError):
+        M.Dipole(5, 4, 1)                                 # odd nx
+    with pytest.raises(M.MagLuaError):
+        M.Dipole(14, 4, 1)                                # factor 7
+    with pytest.raises(M.MagLuaError):
+        a.setSpin((5, 1), (1, 0, 0))
+
+
+# ------------------------------------------------------------ integrators
+def _max_dm(g, o):
+    return float(np.abs(flat(g["ss"].spins()[:3]) - flat(o["ss"].spins()[:3])).max())
+
+
+def test_fused_heun_film32_1000_steps():
+    """32x32x1, J=1, K=0.1, g=0.01, T=0, Heun dt=0.005, alpha=0.1, 1000 steps from seed 0:
+    FP32 GPU against the FP64 oracle, max|dm| <= 1e-5."""
+    spec = W.FILM_32
+    g, o = twin(spec)
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"])
+    sim = O.Sim(o["ss"], o["ex"], o["ani"], o["dip"], o["temp"], o["llg"])
+    for k in range(10):
+        heun.step(100)
+        sim.heun(100)
+        err = _max_dm(g, o)
+        print(f"film32 step {100 * (k + 1)}: max|dm| = {err:.3e}", flush=True)
+    assert abs(g["ss"].time() - o["ss"].time()) < 1e-9
+    assert err <= 1e-5
+
+
+def test_fused_heun_matches_golden_fixture():
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "film32_heun.npz")
+    gold = np.load(path)
+    spec = W.FILM_32
+    g = W.build(M, spec)
+    g["ss"].setSpins(gold["x0"], gold["y0"], gold["z0"])
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"])
+    done = 0
+    for k, step in enumerate(gold["steps"]):
+        heun.step(int(step) - done)
+        done = int(step)
+        s = flat(g["ss"].spins()[:3])
+        assert np.abs(s - gold["states"][k]).max() <= 1e-5, step
+
+
+def test_fused_heun_film256_50_steps():
+    spec = W.FILM_256_T0
+    g, o = twin(spec)
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"])
+    O.Sim.set_threads(8)
+    og = W.build(O, spec, threads=True)
+    og["ss"].setSpins(*O.random_unit_spins(spec.seed, spec.n))
+    sim = O.Sim(og["ss"], og["ex"], og["ani"], og["dip"], og["temp"], og["llg"])
+    heun.step(50)
+    sim.heun(50)
+    err = float(np.abs(flat(g["ss"].spins()[:3]) - flat(og["ss"].spins()[:3])).max())
+    print(f"film256 50 steps: max|dm| = {err:.3e}", flush=True)
+    assert err <= 1e-5
+
+
+@pytest.mark.parametrize("spec", [W.FilmSpec(32, 32, 1, T=0.3, dt=0.005, g=0.01, K=0.1),
+                                  W.FilmSpec(24, 16, 3, T=0.5, dt=0.002)])
+def test_fused_heun_thermal_same_noise(spec):
+    """T > 0 with the same counter-based noise on both sides: trajectories agree
+    over a short horizon (the FP32 Box-Muller differs from FP64 by ~1e-7)."""
+    g, o = twin(spec)
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"])
+    sim = O.Sim(o["ss"], o["ex"], o["ani"], o["dip"], o["temp"], o["llg"], seed=0)
+    heun.step(20)
+    sim.heun(20)
+    err = _max_dm(g, o)
+    print(f"thermal {spec.name()}: 20 steps max|dm| = {err:.3e}", flush=True)
+    assert err <= 1e-4
+    assert g["temp"].draw() == 20
+
+
+def test_graph_and_direct_launches_agree():
+    spec = W.FilmSpec(32, 32, 2, T=0.4, dt=0.005)
+    a = W.build(M, spec)
+    b = W.build(M, spec)
+    a["ss"].randomize(4)
+    b["ss"].randomize(4)
+    ha = M.Heun(a["ss"], a["ex"], a["ani"], a["dip"], a["temp"], a["llg"], graph=True)
+    hb = M.Heun(b["ss"], b["ex"], b["ani"], b["dip"], b["temp"], b["llg"], graph=False)
+    for _ in range(3):
+        ha.step(7)
+        hb.step(7)
+        M.synchronize()
+        assert np.array_equal(flat(a["ss"].spins()), flat(b["ss"].spins()))
+
+
+@pytest.mark.parametrize("kind", ["RK2", "RK4", "BS3"])
+def test_rk_driver_on_op_api(kind):
+    spec = W.FilmSpec(16, 16, 2, T=0.0, dt=0.01)
+    g, o = twin(spec)
+
+    def calc(d):
+        def f(ss):
+            ss.resetFields()
+            d["ex"].apply(ss)
+            d["ani"].apply(ss)
+            d["dip"].apply(ss)
+            ss.sumFields()
+        return f
+
+    step = M.make_rk_step_function(g["ss"], kind, calc(g), g["llg"])
+    sim = O.Sim(o["ss"], o["ex"], o["ani"], o["dip"], None, o["llg"])
+    for _ in range(20):
+        step()
+    sim.rk(kind, 20)
+    assert _max_dm(g, o) <= 1e-5
+    assert abs(g["ss"].time() - o["ss"].time()) < 1e-9
+
+
+def test_pc_driver_on_op_api_equals_fused_heun():
+    spec = W.FilmSpec(16, 16, 2, T=0.0, dt=0.01)
+    a = W.build(M, spec)
+    b = W.build(M, spec)
+    a["ss"].randomize(3)
+    b["ss"].randomize(3)
+
+    def calc(ss):
+        ss.resetFields()
+        a["ex"].apply(ss)
+        a["ani"].apply(ss)
+        a["dip"].apply(ss)
+        ss.sumFields()
+
+    step = M.make_pc_step_function(a["ss"], calc, a["llg"], 1e9)     # one corrector pass = Heun
+    heun = M.Heun(b["ss"], b["ex"], b["ani"], b["dip"], b["temp"], b["llg"])
+    for _ in range(10):
+        step()
+    heun.step(10)
+    assert float(np.abs(flat(a["ss"].spins()) - flat(b["ss"].spins())).max()) <= 1e-6
+    assert abs(a["ss"].time() - b["ss"].time()) < 1e-12

@@ -1,0 +1,361 @@
+"""Pins the FP64 oracle (oracle/msd_oracle.cpp) to known answers, closed forms,
+brute force and library routines, so that it can serve as the GPU checker.
+The reference holds no golden vectors for this path; these checks are chosen
+so that a dropped term, a wrong sign or a transposed index fails one of them."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+# ------------------------------------------------------------------- Philox
+# Random123 known-answer vectors for philox4x32-10 (kat_vectors)
+KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,want", KAT)
+def test_philox_known_answers(ctr, key, want):
+    assert tuple(O.philox(ctr, key)) == want
+
+
+def test_thermal_normals_moments():
+    g = np.array([O.thermal_normals(7, 3, i) for i in range(40000)])
+    assert abs(g.mean()) < 0.02
+    assert abs(g.var() - 1.0) < 0.02
+    # the three components are independent
+    c = np.corrcoef(g.T)
+    assert np.abs(c - np.eye(3)).max() < 0.03
+    # fourth moment of a standard normal is 3
+    assert abs((g ** 4).mean() - 3.0) < 0.15
+
+
+def test_random_unit_spins_uniform():
+    x, y, z = O.random_unit_spins(0, 50000)
+    assert np.allclose(x * x + y * y + z * z, 1.0, atol=1e-14)
+    # uniform on the sphere: <z> = 0, <z^2> = 1/3, phi uniform
+    assert abs(z.mean()) < 0.01 and abs((z * z).mean() - 1 / 3) < 0.01
+    assert abs(x.mean()) < 0.01 and abs(y.mean()) < 0.01
+    x2, _, _ = O.random_unit_spins(1, 10)
+    assert not np.allclose(x2, x[:10])
+
+
+# ---------------------------------------------------------------------- FFT
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 9, 12, 15, 16, 24, 25, 32, 96, 100, 128, 243, 256])
+def test_fft_vs_naive_dft_and_numpy(n):
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    f = O.fft1d(a, -1)
+    assert np.allclose(f, O.dft_naive(a, -1), atol=1e-11 * max(1, n))
+    assert np.allclose(f, np.fft.fft(a), atol=1e-11 * max(1, n))
+    # +1 is the unnormalised inverse (FFTW convention)
+    assert np.allclose(O.fft1d(a, +1), np.fft.ifft(a) * n, atol=1e-11 * max(1, n))
+
+
+def test_fft2d_layers_vs_numpy():
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((3, 12, 10)) + 1j * rng.standard_normal((3, 12, 10))
+    assert np.allclose(O.fft2d_layers(a, -1), np.fft.fft2(a), atol=1e-11)
+    assert np.allclose(O.fft2d_layers(a, +1), np.fft.ifft2(a) * 120, atol=1e-11)
+
+
+# ------------------------------------------------------------ dipole tensor
+def gamma_np(r):
+    rr = np.sqrt((r * r).sum(-1))
+    g = np.zeros(r.shape[:-1] + (6,))
+    ok = rr > 0
+    ir3 = np.where(ok, 1.0 / np.where(ok, rr, 1) ** 3, 0)
+    ir5 = np.where(ok, 1.0 / np.where(ok, rr, 1) ** 5, 0)
+    pairs = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    for k, (a, b) in enumerate(pairs):
+        g[..., k] = 3 * r[..., a] * r[..., b] * ir5 - (a == b) * ir3
+    return g
+
+
+def brute_tensor(nx, ny, nz, gm):
+    """Sum of Gamma over every lattice point (xx,yy) congruent to (ix,iy) within the
+    truncation box and radius -- the lattice sum stated without image indices."""
+    out = np.zeros((6, 2 * nz - 1, ny, nx))
+    R = gm[0]
+    lx, ly = min(gm[2], R), min(gm[3], R)
+    xs = np.arange(-lx, lx + 1)
+    ys = np.arange(-ly, ly + 1)
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    for c in range(2 * nz - 1):
+        zz = c - nz + 1
+        keep = X * X + Y * Y + zz * zz <= R * R
+        r = np.stack([X, Y, np.full_like(X, zz)], -1).astype(float)
+        g = gamma_np(r) * keep[..., None]
+        for iy in range(ny):
+            for ix in range(nx):
+                m = ((X - ix) % nx == 0) & ((Y - iy) % ny == 0)
+                out[:, c, iy, ix] = g[m].sum(0)
+    return out
+
+
+@pytest.mark.parametrize("dims,gm", [((4, 3, 2), (7, 7, 7, 7)), ((5, 6, 1), (9, 9, 9, 9)), ((2, 2, 3), (6, 6, 4, 5))])
+def test_dipole_tensor_brute_force(dims, gm):
+    nx, ny, nz = dims
+    T = O.dipole_tensor(nx, ny, nz, gm)
+    B = brute_tensor(nx, ny, nz, gm)
+    for k, name in enumerate(("XX", "XY", "XZ", "YY", "YZ", "ZZ")):
+        assert np.allclose(T[name], B[k], atol=1e-13), name
+
+
+def test_dipole_tensor_parities_and_trace():
+    nx, ny, nz = 8, 6, 3
+    T = O.dipole_tensor(nx, ny, nz, (40, 40, 40, 40))
+    tr = T["XX"] + T["YY"] + T["ZZ"]
+    assert np.abs(tr).max() < 1e-13
+    flipz = lambda a: a[::-1]
+    flipx = lambda a: np.roll(a[:, :, ::-1], 1, axis=2)
+    flipy = lambda a: np.roll(a[:, ::-1, :], 1, axis=1)
+    par = {"XX": (1, 1, 1), "XY": (1, -1, -1), "XZ": (-1, -1, 1), "YY": (1, 1, 1), "YZ": (-1, 1, -1), "ZZ": (1, 1, 1)}
+    for k, (pz, px, py) in par.items():
+        assert np.allclose(T[k], pz * flipz(T[k]), atol=1e-13), k
+        assert np.allclose(T[k], px * flipx(T[k]), atol=1e-13), k
+        assert np.allclose(T[k], py * flipy(T[k]), atol=1e-13), k
+
+
+def test_square_lattice_dipole_sum():
+    # sum' over the square lattice of 1/r^3 = 4 zeta(3/2) beta(3/2) = 9.0336217...;
+    # truncation at R removes ~ 2 pi / R of it
+    T = O.dipole_tensor(32, 32, 1, (1000, 1000, 1000, 1000))
+    s = -T["ZZ"].sum()
+    assert abs(s - (9.033621683 - 2 * np.pi / 1000)) < 2e-5
+    assert abs(T["XX"].sum() - s / 2) < 1e-9
+
+
+# ---------------------------------------------------------- layered dipole
+def realspace_dipole(spins, gm, g):
+    """h_t = g sum_s sum_images Gamma(r) m_s with in-plane r = r_t - r_s (+ images) and
+    layer displacement z_s - z_t (the reference's layer-offset convention)."""
+    x, y, z = spins
+    nz, ny, nx = x.shape
+    H = np.zeros((3, nz, ny, nx))
+    R = gm[0]
+    m = np.stack([x, y, z])
+    pairs = {(0, 0): 0, (0, 1): 1, (0, 2): 2, (1, 0): 1, (1, 1): 3, (1, 2): 4, (2, 0): 2, (2, 1): 4, (2, 2): 5}
+    xs = np.arange(-R, R + 1)
+    XX, YY = np.meshgrid(xs, xs, indexing="ij")
+    for zt in range(nz):
+        for zs in range(nz):
+            zz = zs - zt
+            keep = XX * XX + YY * YY + zz * zz <= R * R
+            r = np.stack([XX, YY, np.full_like(XX, zz)], -1).astype(float)
+            G = gamma_np(r) * keep[..., None]
+            # kernel over in-plane displacement classes (mod nx, ny)
+            Kd = np.zeros((6, ny, nx))
+            for iy in range(ny):
+                for ix in range(nx):
+                    sel = ((XX - ix) % nx == 0) & ((YY - iy) % ny == 0)
+                    Kd[:, iy, ix] = G[sel].sum(0)
+            for yt in range(ny):
+                for xt in range(nx):
+                    dy = (yt - np.arange(ny)) % ny
+                    dx = (xt - np.arange(nx)) % nx
+                    for a in range(3):
+                        for b in range(3):
+                            H[a, zt, yt, xt] += (Kd[pairs[(a, b)]][dy[:, None], dx[None, :]] * m[b, zs]).sum()
+    return g * H
+
+
+def test_layered_dipole_matches_realspace_sum():
+    nx, ny, nz = 6, 5, 3
+    gm = (6, 6, 6, 6)
+    rng = np.random.default_rng(3)
+    s = rng.standard_normal((3, nz, ny, nx))
+    ss = O.SpinSystem(nx, ny, nz)
+    ss.setSpins(*s)
+    dip = O.Dipole(nx, ny, nz)
+    dip.setTruncation(gm)
+    dip.setStrength(0.7)
+    dip.apply(ss)
+    h = np.stack(ss.field("Dipole")).reshape(3, nz, ny, nx)
+    ref = realspace_dipole(s, gm, 0.7)
+    assert np.allclose(h, ref, atol=1e-12)
+
+
+# ---------------------------------------------------------- local fields
+def nn_exchange(nx, ny, nz, J, pbc=(1, 1, 1)):
+    from maglua_amd.workloads import nn_paths
+    ex = O.Exchange(nx, ny, nz)
+    ex.setPeriodicXYZ(*pbc)
+    f, t = nn_paths(nx, ny, nz, pbc)
+    ex.addPaths(f, t, J)
+    return ex
+
+
+def test_exchange_uniform_state_closed_form():
+    nx, ny, nz = 5, 4, 3
+    ss = O.SpinSystem(nx, ny, nz)
+    d = np.array([0.3, -0.5, 0.8]) / np.linalg.norm([0.3, -0.5, 0.8])
+    n = nx * ny * nz
+    ss.setSpins(np.full(n, d[0]), np.full(n, d[1]), np.full(n, d[2]))
+    nn_exchange(nx, ny, nz, 0.7).apply(ss)
+    h = np.stack(ss.field("Exchange"))
+    assert np.allclose(h, 6 * 0.7 * d[:, None], atol=1e-14)
+    # open z: surface layers see 5 neighbours
+    ss2 = O.SpinSystem(nx, ny, nz)
+    ss2.setSpins(np.full(n, d[0]), np.full(n, d[1]), np.full(n, d[2]))
+    nn_exchange(nx, ny, nz, 0.7, (1, 1, 0)).apply(ss2)
+    h2 = np.stack(ss2.field("Exchange")).reshape(3, nz, ny, nx)
+    assert np.allclose(h2[:, 0], 5 * 0.7 * d[:, None, None])
+    assert np.allclose(h2[:, 1], 6 * 0.7 * d[:, None, None])
+
+
+def test_exchange_normalisation_and_add_wrap():
+    ss = O.SpinSystem(3, 1, 1)
+    ss.setSpins([2.0, 0.0, 0.0], [0.0, 3.0, 0.0], [0.0, 0.0, 0.5])
+    ex = O.Exchange(3, 1, 1)
+    assert ex.add((0, 0, 0), (-1, 0, 0), 2.0) == 1          # wraps to site 2
+    ex.setPeriodicXYZ(0, 1, 1)
+    assert ex.add((0, 0, 0), (-1, 0, 0), 2.0) == 0          # dropped when x is open
+    ex.apply(ss)
+    h = np.stack(ss.field("Exchange"))
+    # h[to=2] = J s[0] / (m[0] m[2]) = 2 * (2,0,0) / (2 * 0.5)
+    assert np.allclose(h[:, 2], [4.0, 0, 0]) and np.allclose(h[:, :2], 0)
+    ex2 = O.Exchange(3, 1, 1)
+    ex2.normalizeMoments(False)
+    ex2.add((1, 0, 0), (2, 0, 0), 2.0)
+    ss.resetFields()
+    ex2.apply(ss)
+    assert np.allclose(np.stack(ss.field("Exchange"))[:, 2], [0, 2.0, 0])
+
+
+def test_anisotropy_closed_form():
+    n = 6
+    rng = np.random.default_rng(4)
+    s = rng.standard_normal((3, n))
+    ss = O.SpinSystem(n)
+    ss.setSpins(*s)
+    ani = O.Anisotropy(n)
+    axes = rng.standard_normal((n, 3))
+    K = rng.uniform(-1, 1, n)
+    for i in range(n):
+        ani.add(i, axes[i], K[i])
+    ani.add(2, (0, 0, 0), 5.0)      # zero axis adds nothing
+    ani.add(3, (1, 0, 0), 0.25)     # a second op on one site accumulates
+    ani.apply(ss)
+    h = np.stack(ss.field("Anisotropy"))
+    u = axes / np.linalg.norm(axes, axis=1, keepdims=True)
+    ms2 = (s * s).sum(0)
+    want = 2 * K * (s * u.T).sum(0) / ms2 * u.T
+    want[:, 3] += 2 * 0.25 * s[0, 3] / ms2[3] * np.array([1, 0, 0])
+    assert np.allclose(h, want, atol=1e-14)
+
+
+def test_thermal_amplitude_and_variance():
+    n = 20000
+    ss = O.SpinSystem(n)
+    ss.setSpins(np.ones(n) * 2, np.zeros(n), np.zeros(n))
+    ss.setParams(alpha=0.3, gamma=1.5, dt=0.01)
+    th = O.Thermal(n)
+    th.setTemperature(0.7)
+    th.scaleSite(5, 0.0)
+    th.apply(ss, seed=11, step=2)
+    h = np.stack(ss.field("Thermal"))
+    sig = np.sqrt(2 * 0.3 * 0.7 / (2 * 0.01 * 1.5))
+    g = np.array([O.thermal_normals(11, 2, i) for i in range(50)]).T
+    assert np.allclose(h[:, :5], sig * g[:, :5], rtol=1e-14)
+    assert np.allclose(h[:, 5], 0)
+    assert abs(h[:, 6:].var() / sig ** 2 - 1) < 0.02
+
+
+# --------------------------------------------------------------- LLG
+B, ALPHA, GAMMA = 1.0, 0.1, 1.0
+
+
+def llg_exact(t, th0=1.0):
+    gp = GAMMA / (1 + ALPHA ** 2)
+    th = 2 * np.arctan(np.tan(th0 / 2) * np.exp(-ALPHA * gp * B * t))
+    ph = gp * B * t
+    return np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+
+
+def _spin(dt):
+    ss = O.SpinSystem(1)
+    ss.setSpins([np.sin(1.0)], [0.0], [np.cos(1.0)])
+    ss.setParams(alpha=ALPHA, gamma=GAMMA, dt=dt, time=0)
+    return ss
+
+
+def _applied(s):
+    s.resetFields()
+    s.setField("Applied", [0.0], [0.0], [B])
+    s.sumFields()
+
+
+def test_llg_euler_converges_to_closed_form():
+    errs = []
+    for dt in (0.02, 0.01):
+        ss, llg = _spin(dt), O.LLG()
+        for _ in range(int(round(2.0 / dt))):
+            _applied(ss)
+            llg.apply(ss)
+            x, y, z, m = ss.spins()
+            assert abs(x[0] ** 2 + y[0] ** 2 + z[0] ** 2 - 1) < 1e-14
+        x, y, z, m = ss.spins()
+        errs.append(np.abs(np.array([x[0], y[0], z[0]]) - llg_exact(2.0)).max())
+        assert abs(ss.time() - 2.0) < 1e-12
+    assert 1.8 < errs[0] / errs[1] < 2.2 and errs[1] < 5e-3
+
+
+def test_heun_second_order_against_closed_form():
+    errs = []
+    for dt in (0.02, 0.01):
+        ss, llg = _spin(dt), O.LLG()
+        P, Cc = ss.copy(), ss.copy()
+        for _ in range(int(round(2.0 / dt))):
+            _applied(ss)
+            llg.apply(ss, 1.0, ss, P)
+            llg.apply(ss, 0.5, ss, Cc)
+            _applied(P)
+            llg.apply(Cc, 0.5, P, Cc)
+            O.lib().ora_sys_copy_spins_to(Cc.h, ss.h)
+            ss.setTime(Cc.time())
+        x, y, z, m = ss.spins()
+        errs.append(np.abs(np.array([x[0], y[0], z[0]]) - llg_exact(2.0)).max())
+    assert 3.6 < errs[0] / errs[1] < 4.4 and errs[1] < 1e-5
+
+
+def aniso_exact(t, K, th0=1.0):
+    gp = GAMMA / (1 + ALPHA ** 2)
+    c = 2 * ALPHA * gp * K
+    A = np.tan(th0) ** 2
+    th = np.arctan(np.tan(th0) * np.exp(-c * t))
+    ph = (np.arcsinh(np.exp(c * t) / np.sqrt(A)) - np.arcsinh(1 / np.sqrt(A))) / ALPHA
+    return np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+
+
+def _aniso_run(kind, dt, K=0.5, T=2.0):
+    ss = _spin(dt)
+    ani = O.Anisotropy(1)
+    ani.add(0, (0, 0, 1), K)
+    sim = O.Sim(ss, None, ani, None, None, O.LLG())
+    n = int(round(T / dt))
+    sim.heun(n) if kind == "heun" else sim.rk(kind, n)
+    x, y, z, m = ss.spins()
+    assert abs(ss.time() - T) < 1e-9
+    return np.abs(np.array([x[0], y[0], z[0]]) - aniso_exact(T, K)).max()
+
+
+@pytest.mark.parametrize("kind,order", [("heun", 2), ("RK1", 1), ("RK2", 2), ("RK3", 2), ("K3", 2), ("RK4", 2),
+                                        ("RK4_38", 2), ("BS3", 2), ("RK6", 2)])
+def test_step_drivers_converge_to_closed_form(kind, order):
+    """Every Butcher row is a chain of renormalised Euler sub-steps
+    (RungeKutta.lua:185-192), which caps the order at 2 for all but RK1."""
+    e1, e2 = _aniso_run(kind, 0.04), _aniso_run(kind, 0.02)
+    rate = np.log2(e1 / e2)
+    assert abs(rate - order) < 0.25, (kind, e1, e2, rate)
+
+
+def test_net_moment_sums():
+    ss = O.SpinSystem(4)
+    ss.setSpins([1, 2, 3, 4], [0, 1, 0, 1], [1, 1, 1, 1])
+    m = ss.netMoment()
+    assert np.allclose(m[:3], [10, 2, 4]) and np.isclose(m[3], np.sqrt(100 + 4 + 16))
+    assert np.allclose(m[4:7], [30, 2, 4])
