@@ -77,6 +77,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--plain", action="store_true", help="plain FP32 spin state instead of the compensated pair")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -90,7 +91,8 @@ def main():
     t_setup = time.perf_counter()
     g = W.build(M, spec)
     g["ss"].randomize(spec.seed)
-    heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"], graph=not args.no_graph)
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"], graph=not args.no_graph,
+                  compensated=not args.plain)
     heun.step(args.warmup)
     M.synchronize()
     print(f"# setup + {args.warmup} warm-up steps: {time.perf_counter() - t_setup:.2f} s", file=sys.stderr, flush=True)
@@ -137,7 +139,8 @@ def main():
         "dtype": "fp32",
         "data": "synthetic (seeded sphere-uniform spins, counter-based thermal noise)",
         "config": {"workload": spec.name() + ", layered FFT dipole, Heun, plain form",
-                   "lattice": [spec.nx, spec.ny, spec.nz], "spins": spec.n, "graph": not args.no_graph},
+                   "lattice": [spec.nx, spec.ny, spec.nz], "spins": spec.n, "graph": not args.no_graph,
+                   "spin_state": "fp32" if args.plain else "fp32 hi+lo (compensated)"},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -186,6 +186,9 @@ int msd_stepper_create(msd_system* s, msd_exchange* ex, msd_anisotropy* ani, msd
 int msd_stepper_destroy(msd_stepper* st);
 /* Replay steps through a captured HIP graph (default 1) */
 int msd_stepper_set_graph(msd_stepper* st, int yes);
+/* Carry the spin state as FP32 hi + lo words with error-free updates (default 1),
+ * or as plain FP32 (0).  Field arithmetic is FP32 either way. */
+int msd_stepper_set_compensated(msd_stepper* st, int yes);
 /* Advance nsteps Heun steps; asynchronous. */
 int msd_stepper_heun(msd_stepper* st, int nsteps);
 /* Kernel-level timing: runs nsteps more steps with HIP events around every

@@ -113,6 +113,7 @@ _SIGS = {
     "msd_stepper_create": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _pp]),
     "msd_stepper_destroy": (C.c_int, [_vp]),
     "msd_stepper_set_graph": (C.c_int, [_vp, C.c_int]),
+    "msd_stepper_set_compensated": (C.c_int, [_vp, C.c_int]),
     "msd_stepper_heun": (C.c_int, [_vp, C.c_int]),
     "msd_stepper_profile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_void_p, _dp, _ip]),
 }

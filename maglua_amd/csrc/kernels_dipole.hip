@@ -13,12 +13,18 @@
 // 0..nz-1: XX XY YY ZZ spectra are real, XZ YZ imaginary, and XZ/YZ are odd in
 // the layer offset (source - target, spinoperationlongrange.cpp:377), so a
 // negative offset flips their sign.  1/(nx*ny), g and the global scale are folded in.
+//
+// Every kernel exists as a compile-time-shaped instance for power-of-two line
+// lengths 32..1024 (NX/NY > 0: unrolled stages, LDS twiddles) and as a runtime-
+// shaped instance (NX/NY == 0) for the other mixed-radix sizes.
 #include "msd_device.h"
 
 namespace msd {
 
-static constexpr int ROW_TPB = 256;
-static constexpr int COL_TPB = 256;
+static constexpr int ROW_TPB = 1024;
+static constexpr int COL_TPB = 512;
+// exchange class tables up to this size are staged in LDS by the fused kernels
+static constexpr int EX_LDS_CLS = 64, EX_LDS_ENT = 512;
 
 // workgroups on the same XCD (hardware assigns bid % 8) get contiguous tiles
 __device__ __forceinline__ int xcd_remap(int bid, int nb)
@@ -33,26 +39,27 @@ __device__ __forceinline__ float& rowval(float2* buf, int q, int x, int nx)
     return reinterpret_cast<float*>(buf + (size_t)(q >> 1) * nx + x)[q & 1];
 }
 
-__device__ __forceinline__ int row_lines(int ty) { return (3 * ty + 1) >> 1; }
+__host__ __device__ constexpr int row_lines(int ty) { return (3 * ty + 1) >> 1; }
 
 // Build the packed complex lines Z = A + iB from the half spectra of rows
 // (2l, 2l+1) read from `spec` (column layout), using A[nx-k] = conj(A[k]).
-__device__ void rows_pack_spectrum(const DipoleDev& dp, const float2* __restrict__ spec, int z, int y0, int ty,
-                                   float2* __restrict__ lds)
+template <int NT>
+__device__ __forceinline__ void rows_pack_spectrum(const float2* __restrict__ spec, int nx, int ny, int nz, int ty,
+                                                   int z, int y0, float2* __restrict__ lds)
 {
-    const int nx = dp.nx, lines = row_lines(ty), total = lines * nx, half = nx >> 1;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    const int nxh = nx / 2 + 1, lines = row_lines(ty), total = lines * nx, half = nx >> 1;
+    for (int idx = threadIdx.x; idx < total; idx += NT) {
         const int l = idx % lines;
         const int k = idx / lines;
         const bool mirror = k > half;
         const int kk = mirror ? nx - k : k;
         const int q0 = 2 * l, q1 = q0 + 1;
         const int c0 = q0 / ty, r0 = q0 - c0 * ty;
-        float2 A = spec[((size_t)(c0 * dp.nz + z) * dp.nxh + kk) * dp.ny + y0 + r0];
+        float2 A = spec[((size_t)(c0 * nz + z) * nxh + kk) * ny + y0 + r0];
         float2 B = make_float2(0.f, 0.f);
         if (q1 < 3 * ty) {
             const int c1 = q1 / ty, r1 = q1 - c1 * ty;
-            B = spec[((size_t)(c1 * dp.nz + z) * dp.nxh + kk) * dp.ny + y0 + r1];
+            B = spec[((size_t)(c1 * nz + z) * nxh + kk) * ny + y0 + r1];
         }
         if (mirror) { A.y = -A.y; B.y = -B.y; }
         lds[(size_t)l * nx + k] = make_float2(A.x - B.y, A.y + B.x);
@@ -61,266 +68,346 @@ __device__ void rows_pack_spectrum(const DipoleDev& dp, const float2* __restrict
 
 // Split the transformed packed lines into the half spectra of both rows and
 // store them in column layout.
-__device__ void rows_unpack_spectrum(const DipoleDev& dp, const float2* __restrict__ lds, int z, int y0, int ty,
-                                     float2* __restrict__ spec)
+template <int NT>
+__device__ __forceinline__ void rows_unpack_spectrum(const float2* __restrict__ lds, int nx, int ny, int nz, int ty, int z,
+                                                     int y0, float2* __restrict__ spec)
 {
-    const int nx = dp.nx, lines = row_lines(ty), total = lines * dp.nxh;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    const int nxh = nx / 2 + 1, lines = row_lines(ty), total = lines * nxh;
+    for (int idx = threadIdx.x; idx < total; idx += NT) {
         const int l = idx % lines;
         const int k = idx / lines;
         const float2 Zk = lds[(size_t)l * nx + k];
         const float2 Zm = lds[(size_t)l * nx + (k ? nx - k : 0)];
         const int q0 = 2 * l, q1 = q0 + 1;
         const int c0 = q0 / ty, r0 = q0 - c0 * ty;
-        spec[((size_t)(c0 * dp.nz + z) * dp.nxh + k) * dp.ny + y0 + r0] =
-            make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y));
+        spec[((size_t)(c0 * nz + z) * nxh + k) * ny + y0 + r0] = make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y));
         if (q1 < 3 * ty) {
             const int c1 = q1 / ty, r1 = q1 - c1 * ty;
-            spec[((size_t)(c1 * dp.nz + z) * dp.nxh + k) * dp.ny + y0 + r1] =
+            spec[((size_t)(c1 * nz + z) * nxh + k) * ny + y0 + r1] =
                 make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x));
         }
     }
 }
 
+// Copy a small exchange class table into LDS so the per-site lookups cost no
+// global round trip; larger tables stay in global memory.
+__device__ __forceinline__ ExchangeDev stage_exchange(const ExchangeDev& ex, int* s_start, ExEntry* s_ent)
+{
+    ExchangeDev e = ex;
+    if (ex.mode == 1 && ex.ncls + 1 <= EX_LDS_CLS && ex.nent <= EX_LDS_ENT) {
+        for (int i = threadIdx.x; i <= ex.ncls; i += blockDim.x) s_start[i] = ex.cls_start[i];
+        for (int i = threadIdx.x; i < ex.nent; i += blockDim.x) s_ent[i] = ex.ent[i];
+        e.cls_start = s_start;
+        e.ent = s_ent;
+    }
+    return e;
+}
+
+// stage the x twiddles in LDS (compile-time shapes only)
+template <int NX>
+__device__ __forceinline__ const float2* row_twiddles(const DipoleDev& dp, float2* lds_tw)
+{
+    if constexpr (NX > 0) {
+        for (int i = threadIdx.x; i < NX; i += ROW_TPB) lds_tw[i] = dp.twx[i];
+        return lds_tw;
+    } else {
+        return dp.twx;
+    }
+}
+
+template <int NX, int TY, int SIGN>
+__device__ __forceinline__ float2* row_fft(float2* A, float2* B, const DipoleDev& dp, const float2* tw)
+{
+    if constexpr (NX > 0) return fft_ct<NX, row_lines(TY), ROW_TPB, SIGN>(A, B, tw);
+    else return lds_fft<SIGN>(A, B, row_lines(dp.ty), dp.px, tw);
+}
+
 // ------------------------------------------------------------ op-API rows
 // SpinSystem::fft (cpu/core/spinsystem.cpp:936-966) along x: spins -> Mh
+template <int NX, int TY>
 __global__ void __launch_bounds__(ROW_TPB) k_rows_fwd(DipoleDev dp, const float* __restrict__ s)
 {
     extern __shared__ float2 lds[];
-    const int ty = dp.ty, tpl = dp.ny / ty;
+    const int nx = NX ? NX : dp.nx, ty = TY ? TY : dp.ty, lines = row_lines(ty), ny = dp.ny;
+    const int tpl = ny / ty;
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int z = tile / tpl, y0 = (tile - z * tpl) * ty;
-    const int nx = dp.nx, n = dp.nx * dp.ny * dp.nz, lines = row_lines(ty);
+    const int n = nx * ny * dp.nz;
     float2* A = lds;
     float2* B = lds + (size_t)lines * nx;
-    for (int li = threadIdx.x; li < lines * nx; li += blockDim.x) A[li] = make_float2(0.f, 0.f);
+    const float2* tw = row_twiddles<NX>(dp, B + (size_t)lines * nx);
+    for (int li = threadIdx.x; li < lines * nx; li += ROW_TPB) A[li] = make_float2(0.f, 0.f);
     __syncthreads();
-    for (int li = threadIdx.x; li < 3 * ty * nx; li += blockDim.x) {
+    for (int li = threadIdx.x; li < 3 * ty * nx; li += ROW_TPB) {
         const int q = li / nx, x = li - q * nx;
         const int c = q / ty, r = q - c * ty;
-        rowval(A, q, x, nx) = s[(size_t)c * n + x + (size_t)nx * (y0 + r + (size_t)dp.ny * z)];
+        rowval(A, q, x, nx) = s[(size_t)c * n + x + (size_t)nx * (y0 + r + (size_t)ny * z)];
     }
     __syncthreads();
-    const float2* R = lds_fft<-1>(A, B, lines, dp.px, dp.twx);
-    rows_unpack_spectrum(dp, R, z, y0, ty, dp.Mh);
+    const float2* R = row_fft<NX, TY, -1>(A, B, dp, tw);
+    rows_unpack_spectrum<ROW_TPB>(R, nx, ny, dp.nz, ty, z, y0, dp.Mh);
 }
 
 // inverse along x: Hh -> real field rows, written to slot h
+template <int NX, int TY>
 __global__ void __launch_bounds__(ROW_TPB) k_rows_inv(DipoleDev dp, float* __restrict__ h)
 {
     extern __shared__ float2 lds[];
-    const int ty = dp.ty, tpl = dp.ny / ty;
+    const int nx = NX ? NX : dp.nx, ty = TY ? TY : dp.ty, lines = row_lines(ty), ny = dp.ny;
+    const int tpl = ny / ty;
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int z = tile / tpl, y0 = (tile - z * tpl) * ty;
-    const int nx = dp.nx, n = dp.nx * dp.ny * dp.nz, lines = row_lines(ty);
+    const int n = nx * ny * dp.nz;
     float2* A = lds;
     float2* B = lds + (size_t)lines * nx;
-    rows_pack_spectrum(dp, dp.Hh, z, y0, ty, A);
+    const float2* tw = row_twiddles<NX>(dp, B + (size_t)lines * nx);
+    rows_pack_spectrum<ROW_TPB>(dp.Hh, nx, ny, dp.nz, ty, z, y0, A);
     __syncthreads();
-    float2* R = lds_fft<1>(A, B, lines, dp.px, dp.twx);
-    for (int li = threadIdx.x; li < 3 * ty * nx; li += blockDim.x) {
+    float2* R = row_fft<NX, TY, 1>(A, B, dp, tw);
+    for (int li = threadIdx.x; li < 3 * ty * nx; li += ROW_TPB) {
         const int q = li / nx, x = li - q * nx;
         const int c = q / ty, r = q - c * ty;
-        h[(size_t)c * n + x + (size_t)nx * (y0 + r + (size_t)dp.ny * z)] = rowval(R, q, x, nx);
+        h[(size_t)c * n + x + (size_t)nx * (y0 + r + (size_t)ny * z)] = rowval(R, q, x, nx);
     }
 }
 
 // ------------------------------------------------------------- column pass
-// For one kx: forward along y, per-k layer/component tensor product, inverse along y.
-template <int NZ>
+// For one kx and one output component a: forward along y of all 3nz input lines,
+// the per-k layer/component tensor product for component a, inverse along y of
+// its nz lines.  Three workgroups per kx (one per output component) give the
+// launch 3(nx/2+1) workgroups; the input column is read three times, from L2.
+template <int NY, int NZ>
 __global__ void __launch_bounds__(COL_TPB) k_cols(DipoleDev dp)
 {
     extern __shared__ float2 lds[];
-    const int kx = blockIdx.x;
-    const int ny = dp.ny, nxh = dp.nxh, lines = 3 * NZ;
+    const int kx = blockIdx.x / 3;
+    const int oa = blockIdx.x - 3 * kx;
+    const int ny = NY ? NY : dp.ny, nxh = dp.nxh;
+    constexpr int lines = 3 * NZ;
     float2* A = lds;
     float2* B = lds + (size_t)lines * ny;
-    for (int idx = threadIdx.x; idx < lines * ny; idx += blockDim.x) {
+    const float2* tw = dp.twy;
+    if constexpr (NY > 0) {
+        float2* twl = B + (size_t)lines * ny;
+        for (int i = threadIdx.x; i < NY; i += COL_TPB) twl[i] = dp.twy[i];
+        tw = twl;
+    }
+    for (int idx = threadIdx.x; idx < lines * ny; idx += COL_TPB) {
         const int l = idx / ny, y = idx - l * ny;
         A[idx] = dp.Mh[((size_t)l * nxh + kx) * ny + y];
     }
     __syncthreads();
-    float2* R = lds_fft<-1>(A, B, lines, dp.py, dp.twy);
-    for (int ky = threadIdx.x; ky < ny; ky += blockDim.x) {
+    float2* R;
+    if constexpr (NY > 0) R = fft_ct<NY, lines, COL_TPB, -1>(A, B, tw);
+    else R = lds_fft<-1>(A, B, lines, dp.py, tw);
+    // row oa of the symmetric tensor: components (oa,0) (oa,1) (oa,2) and which are odd in z
+    const int cidx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    const int c0 = cidx[oa][0], c1 = cidx[oa][1], c2 = cidx[oa][2];
+    const bool odd0 = (c0 == 2 || c0 == 4), odd1 = (c1 == 2 || c1 == 4), odd2 = (c2 == 2 || c2 == 4);
+    for (int ky = threadIdx.x; ky < ny; ky += COL_TPB) {
         float2 m[3][NZ];
-        float w[6][NZ];
+        float w[3][NZ];
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
             for (int s = 0; s < NZ; ++s) m[c][s] = R[(c * NZ + s) * ny + ky];
 #pragma unroll
-        for (int a = 0; a < 6; ++a)
-#pragma unroll
-            for (int o = 0; o < NZ; ++o) w[a][o] = dp.Tw[((size_t)(a * NZ + o) * nxh + kx) * ny + ky];
+        for (int o = 0; o < NZ; ++o) {
+            w[0][o] = dp.Tw[((size_t)(c0 * NZ + o) * nxh + kx) * ny + ky];
+            w[1][o] = dp.Tw[((size_t)(c1 * NZ + o) * nxh + kx) * ny + ky];
+            w[2][o] = dp.Tw[((size_t)(c2 * NZ + o) * nxh + kx) * ny + ky];
+        }
+        float2 h[NZ];
 #pragma unroll
         for (int t = 0; t < NZ; ++t) {
-            float2 hx = make_float2(0.f, 0.f), hy = hx, hz = hx;
+            float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
             for (int s = 0; s < NZ; ++s) {
                 const int o = s >= t ? s - t : t - s;
                 const float sg = s < t ? -1.0f : 1.0f;
-                const float xx = w[0][o], xy = w[1][o], xz = sg * w[2][o];
-                const float yy = w[3][o], yz = sg * w[4][o], zz = w[5][o];
-                const float2 mx = m[0][s], my = m[1][s], mz = m[2][s];
-                // i*w*m = w*(-m.y, m.x)
-                hx.x += xx * mx.x + xy * my.x - xz * mz.y;
-                hx.y += xx * mx.y + xy * my.y + xz * mz.x;
-                hy.x += xy * mx.x + yy * my.x - yz * mz.y;
-                hy.y += xy * mx.y + yy * my.y + yz * mz.x;
-                hz.x += -xz * mx.y - yz * my.y + zz * mz.x;
-                hz.y += xz * mx.x + yz * my.x + zz * mz.y;
+                // real components multiply, imaginary (XZ, YZ) ones give i*w*m = w*(-m.y, m.x)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    const bool odd = b == 0 ? odd0 : (b == 1 ? odd1 : odd2);
+                    const float2 mb = m[b][s];
+                    if (odd) {
+                        const float wv = sg * w[b][o];
+                        acc.x -= wv * mb.y;
+                        acc.y += wv * mb.x;
+                    } else {
+                        const float wv = w[b][o];
+                        acc.x += wv * mb.x;
+                        acc.y += wv * mb.y;
+                    }
+                }
             }
-            R[(0 * NZ + t) * ny + ky] = hx;
-            R[(1 * NZ + t) * ny + ky] = hy;
-            R[(2 * NZ + t) * ny + ky] = hz;
+            h[t] = acc;
         }
+#pragma unroll
+        for (int t = 0; t < NZ; ++t) R[(oa * NZ + t) * ny + ky] = h[t];
     }
     __syncthreads();
-    float2* O = lds_fft<1>(R, R == A ? B : A, lines, dp.py, dp.twy);
-    for (int idx = threadIdx.x; idx < lines * ny; idx += blockDim.x) {
+    float2* Rs = R + (size_t)oa * NZ * ny;
+    float2* Os = (R == A ? B : A) + (size_t)oa * NZ * ny;
+    float2* O;
+    if constexpr (NY > 0) O = fft_ct<NY, NZ, COL_TPB, 1>(Rs, Os, tw);
+    else O = lds_fft<1>(Rs, Os, NZ, dp.py, tw);
+    for (int idx = threadIdx.x; idx < NZ * ny; idx += COL_TPB) {
         const int l = idx / ny, y = idx - l * ny;
-        dp.Hh[((size_t)l * nxh + kx) * ny + y] = O[idx];
+        dp.Hh[((size_t)(oa * NZ + l) * nxh + kx) * ny + y] = O[idx];
     }
-}
-
-static int row_tiles(const DipoleDev& dp) { return dp.nz * (dp.ny / dp.ty); }
-
-hipError_t launch_rows_fwd(const DipoleDev& dp, const float* s, hipStream_t st)
-{
-    k_rows_fwd<<<row_tiles(dp), ROW_TPB, dp.lds_rows, st>>>(dp, s);
-    return hipGetLastError();
-}
-
-hipError_t launch_rows_inv(const DipoleDev& dp, float* h, hipStream_t st)
-{
-    k_rows_inv<<<row_tiles(dp), ROW_TPB, dp.lds_rows, st>>>(dp, h);
-    return hipGetLastError();
-}
-
-hipError_t launch_cols(const DipoleDev& dp, hipStream_t st)
-{
-    switch (dp.nz) {
-#define MSD_COLS(N) case N: k_cols<N><<<dp.nxh, COL_TPB, dp.lds_cols, st>>>(dp); break;
-        MSD_COLS(1) MSD_COLS(2) MSD_COLS(3) MSD_COLS(4) MSD_COLS(5) MSD_COLS(6) MSD_COLS(7) MSD_COLS(8)
-#undef MSD_COLS
-        default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
 }
 
 // ------------------------------------------------------------ fused Heun
 // Predictor half of the step closure in common/scripts/PredictorCorrector.lua:77-84:
 //   H(S) = Exchange + Anisotropy + Dipole,  dM = LLG(S, H(S)),
 //   P = N(S + dt dM),  C = N(S + dt/2 dM),
-// then the x-transform of P for the next column pass.
-template <bool DIP>
+// then the x-transform of P for the next column pass.  COMP carries S and C as
+// compensated FP32 pairs (hi + lo).
+template <int NX, int TY, bool DIP, bool COMP>
 __global__ void __launch_bounds__(ROW_TPB) k_step_pred(StepArgs a)
 {
     extern __shared__ float2 lds[];
     const DipoleDev& dp = a.dp;
-    const int nx = a.d.nx, ny = a.d.ny, n = a.d.n;
-    const int ty = a.ty, tpl = ny / ty;
+    const int nx = NX ? NX : a.d.nx, ty = TY ? TY : a.ty, lines = row_lines(ty);
+    const int ny = a.d.ny, n = a.d.n;
+    const int tpl = ny / ty;
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int z = tile / tpl, y0 = (tile - z * tpl) * ty;
-    const int lines = row_lines(ty);
     float2* A = lds;
     float2* B = lds + (size_t)lines * nx;
     float2* R = A;
-    if (DIP) {
-        rows_pack_spectrum(dp, dp.Hh, z, y0, ty, A);
+    const float2* tw = nullptr;
+    __shared__ int s_start[EX_LDS_CLS];
+    __shared__ ExEntry s_ent[EX_LDS_ENT];
+    const ExchangeDev exl = stage_exchange(a.ex, s_start, s_ent);
+    if constexpr (!DIP) __syncthreads();
+    if constexpr (DIP) {
+        tw = row_twiddles<NX>(dp, B + (size_t)lines * nx);
+        rows_pack_spectrum<ROW_TPB>(dp.Hh, nx, ny, a.d.nz, ty, z, y0, A);
         __syncthreads();
-        R = lds_fft<1>(A, B, lines, dp.px, dp.twx);
+        R = row_fft<NX, TY, 1>(A, B, dp, tw);
     }
     if (a.th.on && blockIdx.x == 0 && threadIdx.x == 0) *a.draw += 1ull;
     const float* __restrict__ S = a.S;
     const float* __restrict__ ms_a = a.S + 3 * (size_t)n;
     const float dt = a.dt, hdt = 0.5f * a.dt;
-    for (int li = threadIdx.x; li < ty * nx; li += blockDim.x) {
+    for (int li = threadIdx.x; li < ty * nx; li += ROW_TPB) {
         const int r = li / nx, x = li - r * nx, y = y0 + r;
         const int i = x + nx * (y + ny * z);
         const float ms = ms_a[i];
         const float3 s = make_float3(S[i], S[n + i], S[2 * n + i]);
-        float3 H = exchange_field(a.ex, a.d, i, x, y, z, S, ms_a);
+        float3 H = exchange_field(exl, a.d, i, x, y, z, S, ms_a);
         const float3 ha = aniso_field(a.an, i, n, s, ms);
         H.x += ha.x; H.y += ha.y; H.z += ha.z;
-        if (DIP) {
+        if constexpr (DIP) {
             H.x += rowval(R, r, x, nx);
             H.y += rowval(R, ty + r, x, nx);
             H.z += rowval(R, 2 * ty + r, x, nx);
         }
         float3 p = s, c = s;
+        float3 clo = make_float3(0.f, 0.f, 0.f);
+        if constexpr (COMP) clo = make_float3(a.Slo[i], a.Slo[n + i], a.Slo[2 * n + i]);
         if (ms > 0.0f) {
             const float al = a.lp.site_alpha ? a.lp.site_alpha[i] : a.lp.alpha;
             const float ga = a.lp.site_gamma ? a.lp.site_gamma[i] : a.lp.gamma;
             const float3 dM = a.lp.thermal_first ? llg_rhs(s, H, H, al, ga, ms, a.lp.ps, a.lp.ds)
                                                  : llg_rhs(s, H, H, al, ga, ms, 1.0f, 1.0f);
             p = make_float3(s.x + dt * dM.x, s.y + dt * dM.y, s.z + dt * dM.z);
-            c = make_float3(s.x + hdt * dM.x, s.y + hdt * dM.y, s.z + hdt * dM.z);
+            if constexpr (COMP) {
+                df_add(c.x, clo.x, hdt * dM.x);
+                df_add(c.y, clo.y, hdt * dM.y);
+                df_add(c.z, clo.z, hdt * dM.z);
+            } else {
+                c = make_float3(s.x + hdt * dM.x, s.y + hdt * dM.y, s.z + hdt * dM.z);
+            }
         }
-        if (a.lp.renorm) { p = renorm(p, ms); c = renorm(c, ms); }
+        if (a.lp.renorm) {
+            p = renorm(p, ms);
+            if constexpr (COMP) df_renorm(c, clo, ms);
+            else c = renorm(c, ms);
+        }
         a.P[i] = p.x; a.P[n + i] = p.y; a.P[2 * n + i] = p.z;
         a.C[i] = c.x; a.C[n + i] = c.y; a.C[2 * n + i] = c.z;
-        if (DIP) {
+        if constexpr (COMP) { a.Clo[i] = clo.x; a.Clo[n + i] = clo.y; a.Clo[2 * n + i] = clo.z; }
+        if constexpr (DIP) {
             rowval(R, r, x, nx) = p.x;
             rowval(R, ty + r, x, nx) = p.y;
             rowval(R, 2 * ty + r, x, nx) = p.z;
         }
     }
-    if (DIP) {
+    if constexpr (DIP) {
         __syncthreads();
-        const float2* F = lds_fft<-1>(R, R == A ? B : A, lines, dp.px, dp.twx);
-        rows_unpack_spectrum(dp, F, z, y0, ty, dp.Mh);
+        const float2* F = row_fft<NX, TY, -1>(R, R == A ? B : A, dp, tw);
+        rows_unpack_spectrum<ROW_TPB>(F, nx, ny, a.d.nz, ty, z, y0, dp.Mh);
     }
 }
 
 // Corrector half (PredictorCorrector.lua:81-85) and the thermal kick (:98-103):
 //   S' = N(C + dt/2 LLG(P, H(P))),  S'' = N(S' + dt LLG(S', H_th)) when Thermal is present,
 // then the x-transform of S'' for the next step's first column pass.
-template <bool DIP>
+template <int NX, int TY, bool DIP, bool COMP>
 __global__ void __launch_bounds__(ROW_TPB) k_step_corr(StepArgs a)
 {
     extern __shared__ float2 lds[];
     const DipoleDev& dp = a.dp;
-    const int nx = a.d.nx, ny = a.d.ny, n = a.d.n;
-    const int ty = a.ty, tpl = ny / ty;
+    const int nx = NX ? NX : a.d.nx, ty = TY ? TY : a.ty, lines = row_lines(ty);
+    const int ny = a.d.ny, n = a.d.n;
+    const int tpl = ny / ty;
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int z = tile / tpl, y0 = (tile - z * tpl) * ty;
-    const int lines = row_lines(ty);
     float2* A = lds;
     float2* B = lds + (size_t)lines * nx;
     float2* R = A;
-    if (DIP) {
-        rows_pack_spectrum(dp, dp.Hh, z, y0, ty, A);
+    const float2* tw = nullptr;
+    __shared__ int s_start[EX_LDS_CLS];
+    __shared__ ExEntry s_ent[EX_LDS_ENT];
+    const ExchangeDev exl = stage_exchange(a.ex, s_start, s_ent);
+    if constexpr (!DIP) __syncthreads();
+    if constexpr (DIP) {
+        tw = row_twiddles<NX>(dp, B + (size_t)lines * nx);
+        rows_pack_spectrum<ROW_TPB>(dp.Hh, nx, ny, a.d.nz, ty, z, y0, A);
         __syncthreads();
-        R = lds_fft<1>(A, B, lines, dp.px, dp.twx);
+        R = row_fft<NX, TY, 1>(A, B, dp, tw);
     }
     const uint64_t draw = a.th.on ? (uint64_t)*a.draw : 0ull;
     float* __restrict__ S = a.S;
     const float* __restrict__ ms_a = a.S + 3 * (size_t)n;
     const float* __restrict__ P = a.P;
     const float dt = a.dt, hdt = 0.5f * a.dt;
-    for (int li = threadIdx.x; li < ty * nx; li += blockDim.x) {
+    for (int li = threadIdx.x; li < ty * nx; li += ROW_TPB) {
         const int r = li / nx, x = li - r * nx, y = y0 + r;
         const int i = x + nx * (y + ny * z);
         const float ms = ms_a[i];
         const float3 p = make_float3(P[i], P[n + i], P[2 * n + i]);
-        float3 H = exchange_field(a.ex, a.d, i, x, y, z, P, ms_a);
+        float3 H = exchange_field(exl, a.d, i, x, y, z, P, ms_a);
         const float3 ha = aniso_field(a.an, i, n, p, ms);
         H.x += ha.x; H.y += ha.y; H.z += ha.z;
-        if (DIP) {
+        if constexpr (DIP) {
             H.x += rowval(R, r, x, nx);
             H.y += rowval(R, ty + r, x, nx);
             H.z += rowval(R, 2 * ty + r, x, nx);
         }
         float3 v = make_float3(a.C[i], a.C[n + i], a.C[2 * n + i]);
+        float3 vlo = make_float3(0.f, 0.f, 0.f);
+        if constexpr (COMP) vlo = make_float3(a.Clo[i], a.Clo[n + i], a.Clo[2 * n + i]);
         const float al = a.lp.site_alpha ? a.lp.site_alpha[i] : a.lp.alpha;
         const float ga = a.lp.site_gamma ? a.lp.site_gamma[i] : a.lp.gamma;
         if (ms > 0.0f) {
             const float3 dM = a.lp.thermal_first ? llg_rhs(p, H, H, al, ga, ms, a.lp.ps, a.lp.ds)
                                                  : llg_rhs(p, H, H, al, ga, ms, 1.0f, 1.0f);
-            v = make_float3(v.x + hdt * dM.x, v.y + hdt * dM.y, v.z + hdt * dM.z);
+            if constexpr (COMP) {
+                df_add(v.x, vlo.x, hdt * dM.x);
+                df_add(v.y, vlo.y, hdt * dM.y);
+                df_add(v.z, vlo.z, hdt * dM.z);
+            } else {
+                v = make_float3(v.x + hdt * dM.x, v.y + hdt * dM.y, v.z + hdt * dM.z);
+            }
         }
-        if (a.lp.renorm) v = renorm(v, ms);
+        if (a.lp.renorm) {
+            if constexpr (COMP) df_renorm(v, vlo, ms);
+            else v = renorm(v, ms);
+        }
         if (a.th.on) {
             const float sg = thermal_sigma(a.th, a.lp, i, ms, dt);
             if (ms > 0.0f) {
@@ -332,39 +419,131 @@ __global__ void __launch_bounds__(ROW_TPB) k_step_corr(StepArgs a)
                 const float3 zero = make_float3(0.f, 0.f, 0.f);
                 const float3 dM = a.lp.thermal_first ? llg_rhs(v, Ht, zero, al, ga, ms, a.lp.ps, a.lp.ds)
                                                      : llg_rhs(v, Ht, Ht, al, ga, ms, 1.0f, 1.0f);
-                v = make_float3(v.x + dt * dM.x, v.y + dt * dM.y, v.z + dt * dM.z);
+                if constexpr (COMP) {
+                    df_add(v.x, vlo.x, dt * dM.x);
+                    df_add(v.y, vlo.y, dt * dM.y);
+                    df_add(v.z, vlo.z, dt * dM.z);
+                } else {
+                    v = make_float3(v.x + dt * dM.x, v.y + dt * dM.y, v.z + dt * dM.z);
+                }
             }
-            if (a.lp.renorm) v = renorm(v, ms);
+            if (a.lp.renorm) {
+                if constexpr (COMP) df_renorm(v, vlo, ms);
+                else v = renorm(v, ms);
+            }
         }
         S[i] = v.x; S[n + i] = v.y; S[2 * n + i] = v.z;
-        if (DIP) {
+        if constexpr (COMP) { a.Slo[i] = vlo.x; a.Slo[n + i] = vlo.y; a.Slo[2 * n + i] = vlo.z; }
+        if constexpr (DIP) {
             rowval(R, r, x, nx) = v.x;
             rowval(R, ty + r, x, nx) = v.y;
             rowval(R, 2 * ty + r, x, nx) = v.z;
         }
     }
-    if (DIP) {
+    if constexpr (DIP) {
         __syncthreads();
-        const float2* F = lds_fft<-1>(R, R == A ? B : A, lines, dp.px, dp.twx);
-        rows_unpack_spectrum(dp, F, z, y0, ty, dp.Mh);
+        const float2* F = row_fft<NX, TY, -1>(R, R == A ? B : A, dp, tw);
+        rows_unpack_spectrum<ROW_TPB>(F, nx, ny, a.d.nz, ty, z, y0, dp.Mh);
     }
 }
 
-hipError_t launch_step_pred(const StepArgs& a, hipStream_t st)
+// ------------------------------------------------------------- dispatch
+// compile-time row shapes (nx, rows per tile) and column lengths
+#define MSD_ROW_SHAPES(X) X(32, 8) X(64, 8) X(128, 8) X(256, 8) X(512, 4) X(1024, 2)
+#define MSD_COL_LENGTHS(X) X(32) X(64) X(128) X(256) X(512) X(1024)
+
+static int row_tiles(int ny, int nz, int ty) { return nz * (ny / ty); }
+
+hipError_t launch_rows_fwd(const DipoleDev& dp, const float* s, hipStream_t st)
 {
-    const int tiles = a.d.nz * (a.d.ny / a.ty);
-    if (a.has_dip) k_step_pred<true><<<tiles, ROW_TPB, a.dp.lds_rows, st>>>(a);
-    else k_step_pred<false><<<tiles, ROW_TPB, 0, st>>>(a);
+    const int g = row_tiles(dp.ny, dp.nz, dp.ty);
+#define MSD_FWD(NXV, TYV)                                                           \
+    if (dp.nx == NXV && dp.ty == TYV) {                                             \
+        k_rows_fwd<NXV, TYV><<<g, ROW_TPB, dp.lds_rows, st>>>(dp, s);               \
+        return hipGetLastError();                                                   \
+    }
+    MSD_ROW_SHAPES(MSD_FWD)
+#undef MSD_FWD
+    k_rows_fwd<0, 0><<<g, ROW_TPB, dp.lds_rows, st>>>(dp, s);
     return hipGetLastError();
 }
 
-hipError_t launch_step_corr(const StepArgs& a, hipStream_t st)
+hipError_t launch_rows_inv(const DipoleDev& dp, float* h, hipStream_t st)
 {
-    const int tiles = a.d.nz * (a.d.ny / a.ty);
-    if (a.has_dip) k_step_corr<true><<<tiles, ROW_TPB, a.dp.lds_rows, st>>>(a);
-    else k_step_corr<false><<<tiles, ROW_TPB, 0, st>>>(a);
+    const int g = row_tiles(dp.ny, dp.nz, dp.ty);
+#define MSD_INV(NXV, TYV)                                                           \
+    if (dp.nx == NXV && dp.ty == TYV) {                                             \
+        k_rows_inv<NXV, TYV><<<g, ROW_TPB, dp.lds_rows, st>>>(dp, h);               \
+        return hipGetLastError();                                                   \
+    }
+    MSD_ROW_SHAPES(MSD_INV)
+#undef MSD_INV
+    k_rows_inv<0, 0><<<g, ROW_TPB, dp.lds_rows, st>>>(dp, h);
     return hipGetLastError();
 }
+
+template <int NZ>
+static void launch_cols_nz(const DipoleDev& dp, hipStream_t st)
+{
+#define MSD_COL(NYV)                                                                \
+    if (dp.ny == NYV) {                                                             \
+        k_cols<NYV, NZ><<<3 * dp.nxh, COL_TPB, dp.lds_cols, st>>>(dp);                  \
+        return;                                                                     \
+    }
+    MSD_COL_LENGTHS(MSD_COL)
+#undef MSD_COL
+    k_cols<0, NZ><<<3 * dp.nxh, COL_TPB, dp.lds_cols, st>>>(dp);
+}
+
+hipError_t launch_cols(const DipoleDev& dp, hipStream_t st)
+{
+    switch (dp.nz) {
+        case 1: launch_cols_nz<1>(dp, st); break;
+        case 2: launch_cols_nz<2>(dp, st); break;
+        case 3: launch_cols_nz<3>(dp, st); break;
+        case 4: launch_cols_nz<4>(dp, st); break;
+        case 5: launch_cols_nz<5>(dp, st); break;
+        case 6: launch_cols_nz<6>(dp, st); break;
+        case 7: launch_cols_nz<7>(dp, st); break;
+        case 8: launch_cols_nz<8>(dp, st); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <bool PRED, int NX, int TY, bool DIP, bool COMP>
+static void launch_step_t(const StepArgs& a, int g, size_t lds, hipStream_t st)
+{
+    if (PRED) k_step_pred<NX, TY, DIP, COMP><<<g, ROW_TPB, lds, st>>>(a);
+    else k_step_corr<NX, TY, DIP, COMP><<<g, ROW_TPB, lds, st>>>(a);
+}
+
+template <bool PRED>
+static hipError_t launch_step(const StepArgs& a, hipStream_t st)
+{
+    const int g = row_tiles(a.d.ny, a.d.nz, a.ty);
+    const bool comp = a.Slo != nullptr;
+    if (!a.has_dip) {
+        if (comp) launch_step_t<PRED, 0, 0, false, true>(a, g, 0, st);
+        else launch_step_t<PRED, 0, 0, false, false>(a, g, 0, st);
+        return hipGetLastError();
+    }
+    const size_t lds = a.dp.lds_rows;
+#define MSD_STEP(NXV, TYV)                                                          \
+    if (a.d.nx == NXV && a.ty == TYV) {                                             \
+        if (comp) launch_step_t<PRED, NXV, TYV, true, true>(a, g, lds, st);         \
+        else launch_step_t<PRED, NXV, TYV, true, false>(a, g, lds, st);             \
+        return hipGetLastError();                                                   \
+    }
+    MSD_ROW_SHAPES(MSD_STEP)
+#undef MSD_STEP
+    if (comp) launch_step_t<PRED, 0, 0, true, true>(a, g, lds, st);
+    else launch_step_t<PRED, 0, 0, true, false>(a, g, lds, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_pred(const StepArgs& a, hipStream_t st) { return launch_step<true>(a, st); }
+hipError_t launch_step_corr(const StepArgs& a, hipStream_t st) { return launch_step<false>(a, st); }
 
 const char* kernel_name(int id)
 {
@@ -372,159 +551,40 @@ const char* kernel_name(int id)
     return (id >= 0 && id < 4) ? names[id] : "?";
 }
 
+// Allow every LDS-staged kernel the device's full LDS per workgroup.
+static void set_lds(const void* f, int v, hipError_t& first)
+{
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+    if (e != hipSuccess && first == hipSuccess) first = e;
+}
+
+template <int NZ>
+static void cols_attr(int v, hipError_t& first)
+{
+#define MSD_CA(NYV) set_lds((const void*)k_cols<NYV, NZ>, v, first);
+    MSD_COL_LENGTHS(MSD_CA)
+#undef MSD_CA
+    set_lds((const void*)k_cols<0, NZ>, v, first);
+}
+
 hipError_t configure_lds(size_t max_bytes)
 {
     const int v = (int)max_bytes;
-    const void* fns[] = {(const void*)k_rows_fwd, (const void*)k_rows_inv,
-                         (const void*)k_cols<1>, (const void*)k_cols<2>, (const void*)k_cols<3>, (const void*)k_cols<4>,
-                         (const void*)k_cols<5>, (const void*)k_cols<6>, (const void*)k_cols<7>, (const void*)k_cols<8>,
-                         (const void*)k_step_pred<true>, (const void*)k_step_corr<true>};
     hipError_t first = hipSuccess;
-    for (const void* f : fns) {
-        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, v);
-        if (e != hipSuccess && first == hipSuccess) first = e;
-    }
+#define MSD_RA(NXV, TYV)                                                            \
+    set_lds((const void*)k_rows_fwd<NXV, TYV>, v, first);                           \
+    set_lds((const void*)k_rows_inv<NXV, TYV>, v, first);                           \
+    set_lds((const void*)k_step_pred<NXV, TYV, true, true>, v, first);              \
+    set_lds((const void*)k_step_pred<NXV, TYV, true, false>, v, first);             \
+    set_lds((const void*)k_step_corr<NXV, TYV, true, true>, v, first);              \
+    set_lds((const void*)k_step_corr<NXV, TYV, true, false>, v, first);
+    MSD_ROW_SHAPES(MSD_RA)
+    MSD_RA(0, 0)
+#undef MSD_RA
+    cols_attr<1>(v, first); cols_attr<2>(v, first); cols_attr<3>(v, first); cols_attr<4>(v, first);
+    cols_attr<5>(v, first); cols_attr<6>(v, first); cols_attr<7>(v, first); cols_attr<8>(v, first);
     (void)hipGetLastError();
     return first;
-}
-
-// --------------------------------------------------- tensor precompute (FP64)
-struct Mat9 { double a[9]; };
-
-__device__ __forceinline__ long long fdiv_ll(long long a, long long b)
-{
-    long long q = a / b;
-    if ((a % b) && ((a < 0) != (b < 0))) --q;
-    return q;
-}
-
-// getWAB / getWAB_range (cpu/dipole/dipolesupport.cpp:71-250) for smin = 0:
-// images X,Y in [-(gmax0-1), gmax0-1], ascending, |xx| <= min(gmax2,gmax0),
-// |yy| <= min(gmax3,gmax0), xx^2+yy^2+zz^2 <= gmax0^2; no z cut beyond the radial one.
-__global__ void k_tensor_sum(int nx, int ny, int nz, int4 gm, Mat9 M, double* __restrict__ T)
-{
-    const int nl = 2 * nz - 1;
-    const size_t total = (size_t)nx * ny * nl, plane = total;
-    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
-        const int i = (int)(idx % nx);
-        const int j = (int)((idx / nx) % ny);
-        const int c = (int)(idx / ((size_t)nx * ny));
-        const long long zz = c - nz + 1;
-        double acc[6] = {0, 0, 0, 0, 0, 0};
-        if (llabs(zz) <= gm.x) {
-            const long long smax = gm.x;
-            const long long limx = min(gm.z, gm.x), limy = min(gm.w, gm.x);
-            const long long x0 = max(-(smax - 1), -fdiv_ll(limx + i, nx));
-            const long long x1 = min(smax - 1, fdiv_ll(limx - i, nx));
-            const long long y0 = max(-(smax - 1), -fdiv_ll(limy + j, ny));
-            const long long y1 = min(smax - 1, fdiv_ll(limy - j, ny));
-            const double r2max = (double)gm.x * (double)gm.x;
-            for (long long X = x0; X <= x1; ++X) {
-                const long long xx = X * nx + i;
-                for (long long Y = y0; Y <= y1; ++Y) {
-                    const long long yy = Y * ny + j;
-                    if ((double)xx * xx + (double)yy * yy + (double)zz * zz <= r2max) {
-                        const double rx = (double)xx * M.a[0] + (double)yy * M.a[3] + (double)zz * M.a[6];
-                        const double ry = (double)xx * M.a[1] + (double)yy * M.a[4] + (double)zz * M.a[7];
-                        const double rz = (double)xx * M.a[2] + (double)yy * M.a[5] + (double)zz * M.a[8];
-                        const double r = sqrt(rx * rx + ry * ry + rz * rz);
-                        if (r < 1e-40) continue;
-                        const double ir = 1.0 / r, ir3 = ir * ir * ir, ir5 = ir3 * ir * ir;
-                        acc[0] += -1.0 * (ir3 - 3.0 * rx * rx * ir5);
-                        acc[1] += -1.0 * (-3.0 * rx * ry * ir5);
-                        acc[2] += -1.0 * (-3.0 * rx * rz * ir5);
-                        acc[3] += -1.0 * (ir3 - 3.0 * ry * ry * ir5);
-                        acc[4] += -1.0 * (-3.0 * ry * rz * ir5);
-                        acc[5] += -1.0 * (ir3 - 3.0 * rz * rz * ir5);
-                    }
-                }
-            }
-        }
-        for (int a = 0; a < 6; ++a) T[(size_t)a * plane + idx] = acc[a];
-    }
-}
-
-hipError_t launch_tensor_sum(int nx, int ny, int nz, const int gmax[4], const double ABC[9], double* T, hipStream_t st)
-{
-    Mat9 M;
-    for (int k = 0; k < 9; ++k) M.a[k] = ABC[k];
-    const size_t total = (size_t)nx * ny * (2 * nz - 1);
-    const int g = (int)std::min<size_t>((total + 127) / 128, 65536);
-    k_tensor_sum<<<g, 128, 0, st>>>(nx, ny, nz, make_int4(gmax[0], gmax[1], gmax[2], gmax[3]), M, T);
-    return hipGetLastError();
-}
-
-// x-DFT of layers z >= 0: U[a][zz][y][kx] = sum_x T[a][nz-1+zz][y][x] e^{-2 pi i kx x / nx}
-__global__ void k_tensor_dftx(int nx, int ny, int nz, const double* __restrict__ T, double2* __restrict__ U,
-                              const double2* __restrict__ tw)
-{
-    const int nxh = nx / 2 + 1, nl = 2 * nz - 1;
-    const size_t total = (size_t)6 * nz * ny * nxh;
-    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
-        const int kx = (int)(idx % nxh);
-        const int y = (int)((idx / nxh) % ny);
-        const int zz = (int)((idx / ((size_t)nxh * ny)) % nz);
-        const int a = (int)(idx / ((size_t)nxh * ny * nz));
-        const double* row = T + (((size_t)a * nl + (nz - 1 + zz)) * ny + y) * nx;
-        double re = 0, im = 0;
-        int m = 0;
-        for (int x = 0; x < nx; ++x) {
-            const double2 w = tw[m];
-            re += row[x] * w.x;
-            im += row[x] * w.y;
-            m += kx; if (m >= nx) m -= nx;
-        }
-        U[idx] = make_double2(re, im);
-    }
-}
-
-// y-DFT and word extraction: W[a][zz][kx][ky] = Re or Im of sum_y U e^{-2 pi i ky y / ny}, / (nx ny)
-__global__ void k_tensor_dfty(int nx, int ny, int nz, const double2* __restrict__ U, double* __restrict__ W,
-                              const double2* __restrict__ tw)
-{
-    const int nxh = nx / 2 + 1;
-    const size_t total = (size_t)6 * nz * nxh * ny;
-    const double inv = 1.0 / ((double)nx * (double)ny);
-    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
-        const int ky = (int)(idx % ny);
-        const int kx = (int)((idx / ny) % nxh);
-        const int zz = (int)((idx / ((size_t)ny * nxh)) % nz);
-        const int a = (int)(idx / ((size_t)ny * nxh * nz));
-        const double2* col = U + ((size_t)a * nz + zz) * ny * nxh + kx;
-        double re = 0, im = 0;
-        int m = 0;
-        for (int y = 0; y < ny; ++y) {
-            const double2 u = col[(size_t)y * nxh];
-            const double2 w = tw[m];
-            re += u.x * w.x - u.y * w.y;
-            im += u.x * w.y + u.y * w.x;
-            m += ky; if (m >= ny) m -= ny;
-        }
-        W[idx] = ((a == 2 || a == 4) ? im : re) * inv;
-    }
-}
-
-hipError_t launch_tensor_spectrum(int nx, int ny, int nz, const double* T, double2* U, double* W,
-                                  const double2* twx, const double2* twy, hipStream_t st)
-{
-    const size_t tot = (size_t)6 * nz * ny * (nx / 2 + 1);
-    const int g = (int)std::min<size_t>((tot + 255) / 256, 65536);
-    k_tensor_dftx<<<g, 256, 0, st>>>(nx, ny, nz, T, U, twx);
-    k_tensor_dfty<<<g, 256, 0, st>>>(nx, ny, nz, U, W, twy);
-    return hipGetLastError();
-}
-
-__global__ void k_tensor_words(const double* __restrict__ W, float* __restrict__ Tw, double mult, size_t n)
-{
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        Tw[i] = (float)(W[i] * mult);
-}
-
-hipError_t launch_tensor_words(const double* W, float* Tw, double mult, size_t n, hipStream_t st)
-{
-    const int g = (int)std::min<size_t>((n + 255) / 256, 65536);
-    k_tensor_words<<<g, 256, 0, st>>>(W, Tw, mult, n);
-    return hipGetLastError();
 }
 
 }  // namespace msd

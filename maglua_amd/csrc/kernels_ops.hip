@@ -171,7 +171,7 @@ hipError_t launch_llg_op(const Dims& d, const LLGDev& lp, float dt, const float*
 // ---------------------------------------------------- seeded initial spins
 // Unit spins uniform on the sphere: z = 2u1-1, phi = 2 pi u2 from 53-bit
 // uniforms of Philox4x32-10 keyed by seed, counter (site, 0, ~0, 'SPIN').
-__global__ void k_random_spins(Dims d, uint64_t seed, float* s)
+__global__ void k_random_spins(Dims d, uint64_t seed, float* s, float* lo)
 {
     const int n = d.n;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -185,13 +185,16 @@ __global__ void k_random_spins(Dims d, uint64_t seed, float* s)
         sincospi(2.0 * u2, &sp, &cp);
         const double x = rho * cp, y = rho * sp;
         s[i] = (float)x; s[n + i] = (float)y; s[2 * n + i] = (float)zc;
+        lo[i] = (float)(x - (double)(float)x);
+        lo[n + i] = (float)(y - (double)(float)y);
+        lo[2 * n + i] = (float)(zc - (double)(float)zc);
         s[3 * n + i] = (float)sqrt(x * x + y * y + zc * zc);
     }
 }
 
-hipError_t launch_random_spins(const Dims& d, uint64_t seed, float* s, hipStream_t st)
+hipError_t launch_random_spins(const Dims& d, uint64_t seed, float* s, float* lo, hipStream_t st)
 {
-    k_random_spins<<<grid_for(d.n), TPB, 0, st>>>(d, seed, s);
+    k_random_spins<<<grid_for(d.n), TPB, 0, st>>>(d, seed, s, lo);
     return hipGetLastError();
 }
 
@@ -252,14 +255,21 @@ __global__ void k_fold(const double* part, int nb, double* out)
     block_sum<K>(v, out);
 }
 
-__global__ void k_moment(Dims d, const float* __restrict__ s, double* part)
+// spin component c of site i with its optional low word (compensated state)
+__device__ __forceinline__ double spin_at(const float* s, const float* lo, int c, int n, int i)
+{
+    const double v = s[(size_t)c * n + i];
+    return lo ? v + (double)lo[(size_t)c * n + i] : v;
+}
+
+__global__ void k_moment(Dims d, const float* __restrict__ s, const float* __restrict__ lo, double* part)
 {
     double v[6] = {0, 0, 0, 0, 0, 0};
     const int n = d.n;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const double x = s[(size_t)c * n + i];
+            const double x = spin_at(s, lo, c, n, i);
             v[c] += x;
             v[3 + c] += x * x;
         }
@@ -267,46 +277,50 @@ __global__ void k_moment(Dims d, const float* __restrict__ s, double* part)
     block_sum<6>(v, part);
 }
 
-__global__ void k_dot(Dims d, const float* __restrict__ h, const float* __restrict__ s, double* part)
+__global__ void k_dot(Dims d, const float* __restrict__ h, const float* __restrict__ s, const float* __restrict__ lo,
+                      double* part)
 {
     double v[1] = {0};
     const int n = d.n;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) v[0] += (double)h[(size_t)c * n + i] * (double)s[(size_t)c * n + i];
+        for (int c = 0; c < 3; ++c) v[0] += (double)h[(size_t)c * n + i] * spin_at(s, lo, c, n, i);
     block_sum<1>(v, part);
 }
 
-__global__ void k_diff(Dims d, const float* __restrict__ a, const float* __restrict__ b, double* part)
+__global__ void k_diff(Dims d, const float* __restrict__ a, const float* __restrict__ alo, const float* __restrict__ b,
+                       const float* __restrict__ blo, double* part)
 {
     double v[3] = {0, 0, 0};
     const int n = d.n;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) v[c] += fabs((double)a[(size_t)c * n + i] - (double)b[(size_t)c * n + i]);
+        for (int c = 0; c < 3; ++c) v[c] += fabs(spin_at(a, alo, c, n, i) - spin_at(b, blo, c, n, i));
     block_sum<3>(v, part);
 }
 
-hipError_t launch_reduce_moment(const Dims& d, const float* s, double* part, double* out, hipStream_t st)
+hipError_t launch_reduce_moment(const Dims& d, const float* s, const float* lo, double* part, double* out, hipStream_t st)
 {
     const int nb = reduce_blocks(d);
-    k_moment<<<nb, RED_TPB, 0, st>>>(d, s, part);
+    k_moment<<<nb, RED_TPB, 0, st>>>(d, s, lo, part);
     k_fold<6><<<1, RED_TPB, 0, st>>>(part, nb, out);
     return hipGetLastError();
 }
 
-hipError_t launch_reduce_dot(const Dims& d, const float* h, const float* s, double* part, double* out, hipStream_t st)
+hipError_t launch_reduce_dot(const Dims& d, const float* h, const float* s, const float* lo, double* part, double* out,
+                             hipStream_t st)
 {
     const int nb = reduce_blocks(d);
-    k_dot<<<nb, RED_TPB, 0, st>>>(d, h, s, part);
+    k_dot<<<nb, RED_TPB, 0, st>>>(d, h, s, lo, part);
     k_fold<1><<<1, RED_TPB, 0, st>>>(part, nb, out);
     return hipGetLastError();
 }
 
-hipError_t launch_reduce_diff(const Dims& d, const float* a, const float* b, double* part, double* out, hipStream_t st)
+hipError_t launch_reduce_diff(const Dims& d, const float* a, const float* alo, const float* b, const float* blo,
+                              double* part, double* out, hipStream_t st)
 {
     const int nb = reduce_blocks(d);
-    k_diff<<<nb, RED_TPB, 0, st>>>(d, a, b, part);
+    k_diff<<<nb, RED_TPB, 0, st>>>(d, a, alo, b, blo, part);
     k_fold<3><<<1, RED_TPB, 0, st>>>(part, nb, out);
     return hipGetLastError();
 }

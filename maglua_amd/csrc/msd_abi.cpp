@@ -149,6 +149,8 @@ struct Slot {
 struct msd_system {
     Dims d{};
     float* S = nullptr;           // [4][n] x y z ms
+    float* Slo = nullptr;         // [3][n] low words of the compensated (hi + lo) spin state
+    bool lo_valid = false;        // Slo holds the residual of the current spins
     double alpha = 1, gamma = 1, dt = 1, time = 0;
     float* site_alpha = nullptr;
     float* site_gamma = nullptr;
@@ -158,7 +160,7 @@ struct msd_system {
     uint64_t version = 1;         // bumped whenever spins change outside the step driver
     ~msd_system()
     {
-        dfree(S); dfree(site_alpha); dfree(site_gamma); dfree(part); dfree(out);
+        dfree(S); dfree(Slo); dfree(site_alpha); dfree(site_gamma); dfree(part); dfree(out);
         for (Slot& s : slots) dfree(s.h);
     }
     int find(const char* nm) const
@@ -204,6 +206,7 @@ extern "C" int msd_system_create(int nx, int ny, int nz, msd_system** out)
     std::unique_ptr<msd_system> s(new msd_system);
     s->d = Dims{nx, ny, nz, nx * ny * nz};
     if (int rc = dalloc(&s->S, (size_t)4 * s->d.n)) return rc;
+    if (int rc = dalloc(&s->Slo, (size_t)3 * s->d.n)) return rc;
     if (int rc = dalloc(&s->part, (size_t)8 * 1024)) return rc;
     if (int rc = dalloc(&s->out, 8)) return rc;
     hipError_t e = launch_fill(s->S, 0.0f, (size_t)4 * s->d.n, G.stream);
@@ -245,6 +248,8 @@ extern "C" int msd_system_copy(const msd_system* s, msd_system** out)
     if (int rc = msd_system_create(s->d.nx, s->d.ny, s->d.nz, &c)) return rc;
     const size_t n = s->d.n;
     int rc = copy_dev(c->S, s->S, 4 * n);
+    if (!rc && s->lo_valid) rc = copy_dev(c->Slo, s->Slo, 3 * n);
+    c->lo_valid = s->lo_valid;
     c->alpha = s->alpha; c->gamma = s->gamma; c->dt = s->dt; c->time = s->time;
     if (!rc && s->site_alpha) { rc = dalloc(&c->site_alpha, n); if (!rc) rc = copy_dev(c->site_alpha, s->site_alpha, n); }
     if (!rc && s->site_gamma) { rc = dalloc(&c->site_gamma, n); if (!rc) rc = copy_dev(c->site_gamma, s->site_gamma, n); }
@@ -264,6 +269,9 @@ extern "C" int msd_system_copy_spins_to(const msd_system* src, msd_system* dst)
     if (!same_dims(src->d, dst->d.nx, dst->d.ny, dst->d.nz)) return fail(MSD_EMISMATCH, "copySpinsTo: size mismatch");
     if (src == dst) return MSD_OK;
     if (int rc = copy_dev(dst->S, src->S, (size_t)4 * src->d.n)) return rc;
+    if (src->lo_valid)
+        if (int rc = copy_dev(dst->Slo, src->Slo, (size_t)3 * src->d.n)) return rc;
+    dst->lo_valid = src->lo_valid;
     dst->version++;
     return MSD_OK;
 }
@@ -278,6 +286,11 @@ extern "C" int msd_system_set_spins(msd_system* s, const double* x, const double
         h[3 * n + i] = ms ? ms[i] : std::sqrt(x[i] * x[i] + y[i] * y[i] + z[i] * z[i]);
     }
     if (int rc = upload_f32(s->S, h.data(), 4 * n)) return rc;
+    // residuals below FP32 for the compensated state of the step driver
+    std::vector<double> lo(3 * n);
+    for (size_t i = 0; i < 3 * n; ++i) lo[i] = h[i] - (double)(float)h[i];
+    if (int rc = upload_f32(s->Slo, lo.data(), 3 * n)) return rc;
+    s->lo_valid = true;
     s->version++;
     return MSD_OK;
 }
@@ -288,6 +301,11 @@ extern "C" int msd_system_get_spins(msd_system* s, double* x, double* y, double*
     const size_t n = s->d.n;
     std::vector<double> h(4 * n);
     if (int rc = download_f32(h.data(), s->S, 4 * n)) return rc;
+    if (s->lo_valid) {
+        std::vector<double> lo(3 * n);
+        if (int rc = download_f32(lo.data(), s->Slo, 3 * n)) return rc;
+        for (size_t i = 0; i < 3 * n; ++i) h[i] += lo[i];
+    }
     double* o[4] = {x, y, z, ms};
     for (int c = 0; c < 4; ++c)
         if (o[c]) std::memcpy(o[c], h.data() + c * n, n * sizeof(double));
@@ -306,8 +324,12 @@ extern "C" int msd_system_set_spin(msd_system* s, int site, double sx, double sy
     }
     const float v[4] = {(float)sx, (float)sy, (float)sz, (float)std::sqrt(sx * sx + sy * sy + sz * sz)};
     const size_t n = s->d.n;
+    const float lo[3] = {(float)(sx - (double)v[0]), (float)(sy - (double)v[1]), (float)(sz - (double)v[2])};
     for (int c = 0; c < 4; ++c)
         HIPCHK(hipMemcpyAsync(s->S + c * n + site, &v[c], sizeof(float), hipMemcpyHostToDevice, G.stream));
+    if (s->lo_valid)
+        for (int c = 0; c < 3; ++c)
+            HIPCHK(hipMemcpyAsync(s->Slo + c * n + site, &lo[c], sizeof(float), hipMemcpyHostToDevice, G.stream));
     HIPCHK(hipStreamSynchronize(G.stream));
     s->version++;
     return MSD_OK;
@@ -329,8 +351,9 @@ extern "C" int msd_system_get_spin(msd_system* s, int site, double out[4])
 extern "C" int msd_system_random_spins(msd_system* s, uint64_t seed)
 {
     NONNULL(s);
-    hipError_t e = launch_random_spins(s->d, seed, s->S, G.stream);
+    hipError_t e = launch_random_spins(s->d, seed, s->S, s->Slo, G.stream);
     if (e != hipSuccess) return fail(MSD_EDEVICE, "random_spins: %s", hipGetErrorString(e));
+    s->lo_valid = true;
     s->version++;
     return MSD_OK;
 }
@@ -429,7 +452,7 @@ static int fetch_out(msd_system* s, double* dst, int count)
 extern "C" int msd_system_net_moment(msd_system* s, double out[8])
 {
     NONNULL(s); NONNULL(out);
-    hipError_t e = launch_reduce_moment(s->d, s->S, s->part, s->out, G.stream);
+    hipError_t e = launch_reduce_moment(s->d, s->S, s->lo_valid ? s->Slo : nullptr, s->part, s->out, G.stream);
     if (e != hipSuccess) return fail(MSD_EDEVICE, "netMoment: %s", hipGetErrorString(e));
     double v[6];
     if (int rc = fetch_out(s, v, 6)) return rc;
@@ -445,7 +468,7 @@ extern "C" int msd_system_field_dot(msd_system* s, const char* slot, double* out
     NONNULL(s); NONNULL(slot); NONNULL(out);
     const int k = s->find(slot);
     if (k < 0) return fail(MSD_EINVAL, "field_dot: unknown slot '%s'", slot);
-    hipError_t e = launch_reduce_dot(s->d, s->slots[k].h, s->S, s->part, s->out, G.stream);
+    hipError_t e = launch_reduce_dot(s->d, s->slots[k].h, s->S, s->lo_valid ? s->Slo : nullptr, s->part, s->out, G.stream);
     if (e != hipSuccess) return fail(MSD_EDEVICE, "field_dot: %s", hipGetErrorString(e));
     return fetch_out(s, out, 1);
 }
@@ -454,7 +477,8 @@ extern "C" int msd_system_diff(msd_system* a, msd_system* b, double out[4])
 {
     NONNULL(a); NONNULL(b); NONNULL(out);
     if (!same_dims(a->d, b->d.nx, b->d.ny, b->d.nz)) return fail(MSD_EMISMATCH, "diff: size mismatch");
-    hipError_t e = launch_reduce_diff(a->d, a->S, b->S, a->part, a->out, G.stream);
+    hipError_t e = launch_reduce_diff(a->d, a->S, a->lo_valid ? a->Slo : nullptr, b->S, b->lo_valid ? b->Slo : nullptr, a->part,
+                                      a->out, G.stream);
     if (e != hipSuccess) return fail(MSD_EDEVICE, "diff: %s", hipGetErrorString(e));
     if (int rc = fetch_out(a, out, 3)) return rc;
     out[3] = std::sqrt(out[0] * out[0] + out[1] * out[1] + out[2] * out[2]);
@@ -477,6 +501,7 @@ struct msd_exchange {
     int* cls_start = nullptr;
     ExEntry* ent = nullptr;
     bool lut_ok = false;
+    int ncls = 0, nent = 0;
     ~msd_exchange() { release(); }
     void release()
     {
@@ -495,7 +520,10 @@ struct msd_exchange {
     ExchangeDev fused_dev() const
     {
         ExchangeDev e = csr_dev();
-        if (lut_ok && !paths.empty()) { e.mode = 1; e.cls = cls; e.cls_start = cls_start; e.ent = ent; }
+        if (lut_ok && !paths.empty()) {
+            e.mode = 1; e.cls = cls; e.cls_start = cls_start; e.ent = ent;
+            e.ncls = ncls; e.nent = nent;
+        }
         return e;
     }
 };
@@ -570,6 +598,8 @@ int msd_exchange::compile()
         hcls[t] = (uint8_t)id;
     }
     if (lut_ok) {
+        ncls = (int)ids.size();
+        nent = (int)ents.size();
         if (ids.size() > 1) {
             if (int rc = dalloc(&cls, n)) return rc;
             HIPCHK(hipMemcpyAsync(cls, hcls.data(), n, hipMemcpyHostToDevice, G.stream));
@@ -937,12 +967,13 @@ extern "C" int msd_dipole_create(int nx, int ny, int nz, msd_dipole** out)
     // rows per tile: largest of 8,4,2,1 dividing ny whose two LDS buffers fit 64 KiB
     int ty = 1;
     for (int c : {8, 4, 2}) {
-        const size_t need = 2 * (size_t)((3 * c + 1) / 2) * nx * sizeof(float2);
+        const size_t need = (2 * (size_t)((3 * c + 1) / 2) + 1) * nx * sizeof(float2);
         if (ny % c == 0 && need <= 65536) { ty = c; break; }
     }
     dv.ty = ty;
-    dv.lds_rows = 2 * (size_t)((3 * ty + 1) / 2) * nx * sizeof(float2);
-    dv.lds_cols = 2 * (size_t)3 * nz * ny * sizeof(float2);
+    // two ping-pong line buffers plus the LDS twiddle table
+    dv.lds_rows = (2 * (size_t)((3 * ty + 1) / 2) + 1) * nx * sizeof(float2);
+    dv.lds_cols = (2 * (size_t)3 * nz + 1) * ny * sizeof(float2);
     if (dv.lds_rows > G.max_lds || dv.lds_cols > G.max_lds)
         return fail(MSD_EUNSUPPORTED, "Dipole.new: %dx%dx%d needs %zu/%zu bytes of LDS per workgroup (device limit %zu)", nx, ny,
                     nz, dv.lds_rows, dv.lds_cols, G.max_lds);
@@ -1126,6 +1157,7 @@ extern "C" int msd_llg_apply(msd_llg* l, msd_system* from, double scale, msd_sys
                                  TH >= 0 ? dmdt->slots[TH].h : nullptr, to->S, G.stream);
     if (e != hipSuccess) return fail(MSD_EDEVICE, "LLG:apply: %s", hipGetErrorString(e));
     if (dt != 0 && advance) to->time = from->time + dt;
+    to->lo_valid = false;
     to->version++;
     return MSD_OK;
 }
@@ -1138,9 +1170,10 @@ struct msd_stepper {
     msd_dipole* dip = nullptr;
     msd_thermal* th = nullptr;
     msd_llg llg;
-    float *P = nullptr, *C = nullptr;
+    float *P = nullptr, *C = nullptr, *Clo = nullptr;
     unsigned long long* draw = nullptr;
     bool use_graph = true;
+    bool compensated = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     StepArgs args{};
@@ -1150,7 +1183,7 @@ struct msd_stepper {
     {
         (void)hipStreamSynchronize(G.stream);
         drop_graph();
-        dfree(P); dfree(C); dfree(draw);
+        dfree(P); dfree(C); dfree(Clo); dfree(draw);
     }
     void drop_graph()
     {
@@ -1170,6 +1203,8 @@ int msd_stepper::refresh()
     a.d = s->d;
     a.dt = (float)s->dt;
     a.S = s->S; a.P = P; a.C = C;
+    a.Slo = compensated ? s->Slo : nullptr;
+    a.Clo = compensated ? Clo : nullptr;
     if (ex) { if (int rc = ex->compile()) return rc; a.ex = ex->fused_dev(); }
     if (an) { if (int rc = an->compile()) return rc; a.an = an->dev; }
     if (th) { if (int rc = th->dev(a.th)) return rc; }
@@ -1210,9 +1245,11 @@ extern "C" int msd_stepper_create(msd_system* s, msd_exchange* ex, msd_anisotrop
     if (llg) st->llg = *llg;
     if (int rc = dalloc(&st->P, (size_t)3 * d.n)) return rc;
     if (int rc = dalloc(&st->C, (size_t)3 * d.n)) return rc;
+    if (int rc = dalloc(&st->Clo, (size_t)3 * d.n)) return rc;
     if (int rc = dalloc(&st->draw, 1)) return rc;
     hipError_t e = launch_fill(st->P, 0.0f, (size_t)3 * d.n, G.stream);
     if (e == hipSuccess) e = launch_fill(st->C, 0.0f, (size_t)3 * d.n, G.stream);
+    if (e == hipSuccess) e = launch_fill(st->Clo, 0.0f, (size_t)3 * d.n, G.stream);
     if (e != hipSuccess) return fail(MSD_EDEVICE, "stepper: %s", hipGetErrorString(e));
     if (int rc = st->refresh()) return rc;
     *out = st.release();
@@ -1221,10 +1258,17 @@ extern "C" int msd_stepper_create(msd_system* s, msd_exchange* ex, msd_anisotrop
 
 extern "C" int msd_stepper_destroy(msd_stepper* st) { delete st; return MSD_OK; }
 extern "C" int msd_stepper_set_graph(msd_stepper* st, int yes) { NONNULL(st); st->use_graph = yes != 0; return MSD_OK; }
+extern "C" int msd_stepper_set_compensated(msd_stepper* st, int yes) { NONNULL(st); st->compensated = yes != 0; return MSD_OK; }
 
 static int stepper_begin(msd_stepper* st)
 {
     if (int rc = st->refresh()) return rc;
+    // a state set without its residual (operator API writes) starts with lo = 0
+    if (st->compensated && !st->s->lo_valid) {
+        hipError_t e = launch_fill(st->s->Slo, 0.0f, (size_t)3 * st->s->d.n, G.stream);
+        if (e != hipSuccess) return fail(MSD_EDEVICE, "stepper: %s", hipGetErrorString(e));
+        st->s->lo_valid = true;
+    }
     // spin spectrum of the current state for the first column pass
     if (st->args.has_dip && st->spec_version != st->s->version) {
         hipError_t e = launch_rows_fwd(st->args.dp, st->s->S, G.stream);
@@ -1242,6 +1286,7 @@ static int stepper_begin(msd_stepper* st)
 static void stepper_end(msd_stepper* st, int nsteps)
 {
     st->s->time += nsteps * st->s->dt;
+    if (!st->compensated) st->s->lo_valid = false;
     if (st->th) st->th->draw += (uint64_t)nsteps;
     st->s->version++;
     st->spec_version = st->s->version;

@@ -161,6 +161,75 @@ __device__ float2* lds_fft(float2* a, float2* b, int count, const FFTPlan& p, co
     return a;
 }
 
+// ------------------------------------------- compile-time Stockham transforms
+// Same stages as lds_fft with the line length, line count and thread count known
+// at compile time: index arithmetic folds to shifts and masks, loops unroll, and
+// twiddles come from a table already staged in LDS.
+__host__ __device__ constexpr int next_radix(int rem)
+{
+    return rem % 8 == 0 ? 8 : rem % 4 == 0 ? 4 : rem % 2 == 0 ? 2 : rem % 3 == 0 ? 3 : 5;
+}
+
+template <int L, int COUNT, int NT, int SIGN, int NS, int R>
+__device__ __forceinline__ void stage_ct(const float2* __restrict__ src, float2* __restrict__ dst,
+                                         const float2* __restrict__ tw)
+{
+    constexpr int nb = L / R;
+    constexpr int total = COUNT * nb;
+    constexpr int iters = (total + NT - 1) / NT;
+    constexpr int tstep = L / (NS * R);
+#pragma unroll
+    for (int it = 0; it < iters; ++it) {
+        const int b = (int)threadIdx.x + it * NT;
+        if (total % NT == 0 || b < total) {
+            const int seq = b / nb;
+            const int j = b - seq * nb;
+            const int k = j % NS;
+            const float2* s = src + seq * L + j;
+            float2 v[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r] = s[r * nb];
+            if (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    const float2 w = tw[k * r * tstep];
+                    v[r] = SIGN < 0 ? cmul(v[r], w) : cmulc(v[r], w);
+                }
+            }
+            if (R == 2) dft2<SIGN>(v);
+            else if (R == 3) dft3<SIGN>(v);
+            else if (R == 4) dft4<SIGN>(v);
+            else if (R == 5) dft5<SIGN>(v);
+            else dft8<SIGN>(v);
+            float2* d = dst + seq * L + (j / NS) * NS * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) d[q * NS] = v[q];
+        }
+    }
+}
+
+template <int L, int COUNT, int NT, int SIGN, int NS>
+struct FFTct {
+    static __device__ __forceinline__ float2* run(float2* a, float2* b, const float2* tw)
+    {
+        constexpr int R = next_radix(L / NS);
+        stage_ct<L, COUNT, NT, SIGN, NS, R>(a, b, tw);
+        __syncthreads();
+        return FFTct<L, COUNT, NT, SIGN, NS * R>::run(b, a, tw);
+    }
+};
+template <int L, int COUNT, int NT, int SIGN>
+struct FFTct<L, COUNT, NT, SIGN, L> {
+    static __device__ __forceinline__ float2* run(float2* a, float2*, const float2*) { return a; }
+};
+
+// COUNT lines of length L in LDS (a), result buffer returned; twl = LDS twiddles
+template <int L, int COUNT, int NT, int SIGN>
+__device__ __forceinline__ float2* fft_ct(float2* a, float2* b, const float2* twl)
+{
+    return FFTct<L, COUNT, NT, SIGN, 1>::run(a, b, twl);
+}
+
 // ---------------------------------------------------------------- LLG
 __device__ __forceinline__ float3 cross3(float3 a, float3 b)
 {
@@ -177,6 +246,41 @@ __device__ __forceinline__ float3 llg_rhs(float3 M, float3 H, float3 h, float al
     const float damp = ds * (alpha / ms) * gaa;
     const float prec = ps * -gaa;
     return make_float3(prec * MH.x - damp * MMh.x, prec * MH.y - damp * MMh.y, prec * MH.z - damp * MMh.z);
+}
+
+// -------------------------------------------- compensated (hi + lo) state
+// Error-free transformations (Knuth TwoSum, FMA-based product) keep the spin
+// state to ~1e-14 while every operation stays FP32.
+__device__ __forceinline__ void df_add(float& hi, float& lo, float b)
+{
+#pragma clang fp contract(off)
+    const float s = hi + b;
+    const float bb = s - hi;
+    const float e = ((hi - (s - bb)) + (b - bb)) + lo;
+    hi = s + e;
+    lo = e - (hi - s);
+}
+
+__device__ __forceinline__ void df_scale(float& hi, float& lo, float f)
+{
+#pragma clang fp contract(off)
+    const float p = hi * f;
+    const float e = fmaf(lo, f, fmaf(hi, f, -p));
+    hi = p + e;
+    lo = e - (hi - p);
+}
+
+// rescale (hi + lo) to length ms; the length is set to FP32 accuracy, the
+// direction is kept to the accuracy of the compensated pair
+__device__ __forceinline__ void df_renorm(float3& hi, float3& lo, float ms)
+{
+    const float mm = hi.x * hi.x + hi.y * hi.y + hi.z * hi.z + 2.0f * (hi.x * lo.x + hi.y * lo.y + hi.z * lo.z);
+    if (mm != 0.0f) {
+        const float inv = ms / sqrtf(mm);
+        df_scale(hi.x, lo.x, inv);
+        df_scale(hi.y, lo.y, inv);
+        df_scale(hi.z, lo.z, inv);
+    }
 }
 
 __device__ __forceinline__ float3 renorm(float3 v, float ms)
@@ -236,17 +340,42 @@ __device__ __forceinline__ float3 exchange_field(const ExchangeDev& ex, const Di
     const int n = d.n;
     if (ex.mode == 1) {
         const int c = ex.cls ? ex.cls[i] : 0;
-        const int e1 = ex.cls_start[c + 1];
-        for (int e = ex.cls_start[c]; e < e1; ++e) {
+        const int e0 = ex.cls_start[c];
+        const int cnt = ex.cls_start[c + 1] - e0;
+        // first 8 couplings unrolled so their loads are in flight together
+        float sx[8], sy[8], sz[8], mf[8], J[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            mf[u] = 0.0f;
+            if (u < cnt) {
+                const ExEntry E = ex.ent[e0 + u];
+                int xn = x + E.dx, yn = y + E.dy, zn = z + E.dz;
+                xn += xn < 0 ? d.nx : 0; xn -= xn >= d.nx ? d.nx : 0;
+                yn += yn < 0 ? d.ny : 0; yn -= yn >= d.ny ? d.ny : 0;
+                zn += zn < 0 ? d.nz : 0; zn -= zn >= d.nz ? d.nz : 0;
+                const int f = xn + d.nx * (yn + d.ny * zn);
+                J[u] = E.J;
+                mf[u] = m[f];
+                sx[u] = s[f]; sy[u] = s[n + f]; sz[u] = s[2 * n + f];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (mf[u] != 0.0f) {
+                const float w = J[u] / mf[u];
+                h.x += sx[u] * w; h.y += sy[u] * w; h.z += sz[u] * w;
+            }
+        }
+        for (int e = e0 + 8; e < e0 + cnt; ++e) {
             const ExEntry E = ex.ent[e];
             int xn = x + E.dx, yn = y + E.dy, zn = z + E.dz;
             xn += xn < 0 ? d.nx : 0; xn -= xn >= d.nx ? d.nx : 0;
             yn += yn < 0 ? d.ny : 0; yn -= yn >= d.ny ? d.ny : 0;
             zn += zn < 0 ? d.nz : 0; zn -= zn >= d.nz ? d.nz : 0;
             const int f = xn + d.nx * (yn + d.ny * zn);
-            const float mf = m[f];
-            if (mf != 0.0f) {
-                const float w = E.J / mf;
+            const float mfe = m[f];
+            if (mfe != 0.0f) {
+                const float w = E.J / mfe;
                 h.x += s[f] * w; h.y += s[n + f] * w; h.z += s[2 * n + f] * w;
             }
         }

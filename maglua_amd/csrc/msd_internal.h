@@ -33,6 +33,7 @@ struct ExchangeDev {
     const uint8_t* cls;     // [n] or nullptr (all class 0)
     const int* cls_start;   // [ncls+1]
     const ExEntry* ent;     // class entries
+    int ncls, nent;         // class table sizes (mode 1)
     const int* row;         // CSR [n+1]
     const int* col;         // CSR from-site
     const float* val;       // CSR J*scale
@@ -93,14 +94,17 @@ hipError_t launch_thermal_op(const Dims& d, const ThermalDev& th, const LLGDev& 
                              const float* s, float* h, hipStream_t st);
 hipError_t launch_llg_op(const Dims& d, const LLGDev& lp, float dt, const float* from, const float* dmdt,
                          const float* H, const float* Hth, float* to, hipStream_t st);
-hipError_t launch_random_spins(const Dims& d, uint64_t seed, float* s, hipStream_t st);
+hipError_t launch_random_spins(const Dims& d, uint64_t seed, float* s, float* lo, hipStream_t st);
 hipError_t launch_philox(const uint32_t* ctr, int count, uint32_t k0, uint32_t k1, uint32_t* out, hipStream_t st);
 hipError_t launch_thermal_normals(const Dims& d, uint64_t seed, uint64_t draw, float* out, hipStream_t st);
 
-// FP64 reductions: moment -> out[8] sums; dot(h,s) -> out[0]; diff -> out[3]
-hipError_t launch_reduce_moment(const Dims& d, const float* s, double* part, double* out, hipStream_t st);
-hipError_t launch_reduce_dot(const Dims& d, const float* h, const float* s, double* part, double* out, hipStream_t st);
-hipError_t launch_reduce_diff(const Dims& d, const float* a, const float* b, double* part, double* out, hipStream_t st);
+// FP64 reductions over spins (hi words plus optional low words lo):
+// moment -> out[6] sums; dot(h,s) -> out[0]; diff -> out[3]
+hipError_t launch_reduce_moment(const Dims& d, const float* s, const float* lo, double* part, double* out, hipStream_t st);
+hipError_t launch_reduce_dot(const Dims& d, const float* h, const float* s, const float* lo, double* part, double* out,
+                             hipStream_t st);
+hipError_t launch_reduce_diff(const Dims& d, const float* a, const float* alo, const float* b, const float* blo,
+                              double* part, double* out, hipStream_t st);
 int reduce_blocks(const Dims& d);
 
 // dipole: op-API pieces
@@ -123,6 +127,8 @@ struct StepArgs {
     float* S;               // [4][n]  x y z ms (updated in place by the corrector)
     float* P;               // [3][n]  predictor spins
     float* C;               // [3][n]  half-step spins
+    float* Slo;             // [3][n]  low words of S (compensated state) or nullptr
+    float* Clo;             // [3][n]  low words of C
     ExchangeDev ex;
     AnisoDev an;
     ThermalDev th;

@@ -641,13 +641,14 @@ class Heun(_Handle):
 
     _destroy = "msd_stepper_destroy"
 
-    def __init__(self, ss, ex=None, ani=None, dip=None, temp=None, llg=None, graph=True):
+    def __init__(self, ss, ex=None, ani=None, dip=None, temp=None, llg=None, graph=True, compensated=True):
         self._keep = (ss, ex, ani, dip, temp, llg)
         g = lambda o: o.h if o is not None else None
         p = _vp()
         call("msd_stepper_create", ss.h, g(ex), g(ani), g(dip), g(temp), g(llg), C.byref(p))
         super().__init__(p.value)
         call("msd_stepper_set_graph", self._h, int(bool(graph)))
+        call("msd_stepper_set_compensated", self._h, int(bool(compensated)))
 
     def step(self, n=1):
         call("msd_stepper_heun", self._h, int(n))
