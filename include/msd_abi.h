@@ -165,6 +165,26 @@ int msd_dipole_set_matrix(msd_dipole* d, int which, int ox, int oy, int oz, doub
 /* dip:apply(ss): LongRange::apply, spinoperationlongrange.cpp:354-430, slot "Dipole" */
 int msd_dipole_apply(msd_dipole* d, msd_system* s);
 
+/* ------------------------------------------------------------- LongRange3D */
+/* LongRange3D.new: 3D FFT convolution, cpu/longrange3d/spinoperationlongrange3d.cpp:331-432.
+ * open = 0: periodic in x, y, z; the tensor defaults to the dipole lattice sum over
+ * periodic images within radius R (default 1000, as Dipole's truncation).
+ * open = 1: open boundaries by zero padding to (2nx, 2ny, 2nz); the tensor is the
+ * bare dipole kernel.  Transform lengths need prime factors in {2,3,5}. */
+typedef struct msd_longrange3d msd_longrange3d;
+int msd_longrange3d_create(int nx, int ny, int nz, int open, msd_longrange3d** out);
+int msd_longrange3d_destroy(msd_longrange3d* l);
+int msd_longrange3d_set_strength(msd_longrange3d* l, double g);
+int msd_longrange3d_set_scale(msd_longrange3d* l, double s);
+int msd_longrange3d_set_radius(msd_longrange3d* l, int R);
+/* tensor element which = 0..5 (XX XY XZ YY YZ ZZ; YX = XY etc.) at FFT-grid offset
+ * (ox,oy,oz), taken modulo the grid (LongRange3D setMatrix/getMatrix).  A custom
+ * tensor must stay even, T(-r) = T(r). */
+int msd_longrange3d_get_matrix(msd_longrange3d* l, int which, int ox, int oy, int oz, double* v);
+int msd_longrange3d_set_matrix(msd_longrange3d* l, int which, int ox, int oy, int oz, double v);
+/* lr:apply(ss): slot "LongRange3D" */
+int msd_longrange3d_apply(msd_longrange3d* l, msd_system* s);
+
 /* --------------------------------------------------------------------- LLG */
 /* LLG.Cartesian: cpu/llg/llgcartesian.cpp:90-278 (thermalOnlyFirstTerm = true) */
 int msd_llg_create(msd_llg** out);
@@ -189,11 +209,28 @@ int msd_stepper_set_graph(msd_stepper* st, int yes);
 /* Carry the spin state as FP32 hi + lo words with error-free updates (default 1),
  * or as plain FP32 (0).  Field arithmetic is FP32 either way. */
 int msd_stepper_set_compensated(msd_stepper* st, int yes);
+/* Use a LongRange3D as the long-range field of this step driver (NULL removes it);
+ * not combinable with a Dipole. */
+int msd_stepper_set_longrange3d(msd_stepper* st, msd_longrange3d* l);
 /* Advance nsteps Heun steps; asynchronous. */
 int msd_stepper_heun(msd_stepper* st, int nsteps);
 /* Kernel-level timing: runs nsteps more steps with HIP events around every
  * launch on the library stream; names[i]/avg_ms[i] for up to cap kernels. */
 int msd_stepper_profile(msd_stepper* st, int nsteps, int cap, char names[][32], double* avg_ms, int* count);
+
+/* -------------------------------------------------------------- ensembles */
+/* Independent replicas -- SpinSystems of uneven sizes with their own operators,
+ * temperatures and seeds, each described by a step driver of the local or
+ * LongRange3D form -- advanced together: every pass of a Heun step is one launch
+ * over all replicas.  No reference counterpart: MagLua loops
+ * SpinOperation::apply over same-sized systems (cpu/core/spinoperation.cpp:288-295).
+ * Members must outlive the ensemble. */
+typedef struct msd_ensemble msd_ensemble;
+int msd_ensemble_create(msd_stepper* const* members, int count, msd_ensemble** out);
+int msd_ensemble_destroy(msd_ensemble* e);
+int msd_ensemble_set_graph(msd_ensemble* e, int yes);
+int msd_ensemble_set_compensated(msd_ensemble* e, int yes);
+int msd_ensemble_heun(msd_ensemble* e, int nsteps);
 
 #ifdef __cplusplus
 }

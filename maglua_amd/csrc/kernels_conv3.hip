@@ -1,0 +1,393 @@
+// 3D FFT convolution (LongRange3D::apply, cpu/longrange3d/spinoperationlongrange3d.cpp:369-432)
+// on gfx950, for one system or a batch of independent replicas of uneven sizes
+// advanced by the same launches.
+//
+// Every launch walks a table of work items (replica, p, q); a replica's sizes,
+// plans, arrays and physics come from its Rep3 descriptor, so replicas of
+// different lattice sizes share one launch.  Per field evaluation:
+//   rows  : x transforms (real <-> half complex, two real rows per complex line),
+//           fused with the Heun site updates of the step driver;
+//   ypass : y transforms of kx blocks (coalesced runs of w*8 bytes);
+//   zconv : forward z transform of the three components, the per-k 3x3 tensor
+//           product (real spectra: the tensor is even), inverse z transform.
+// With open boundaries the FFT grid is twice the lattice: input rows/planes
+// beyond the lattice are zero and output beyond it is never stored.
+#include "msd_device.h"
+
+#include <algorithm>
+
+namespace msd {
+
+static constexpr int R3_TPB = 512;
+static constexpr int Y3_TPB = 256;
+static constexpr int Z3_TPB = 512;
+
+__device__ __forceinline__ float& rowval3(float2* buf, int q, int x, int fx)
+{
+    return reinterpret_cast<float*>(buf + (size_t)(q >> 1) * fx + x)[q & 1];
+}
+
+__device__ __forceinline__ size_t widx(const Conv3Dev& cv, int nz, int c, int z, int y, int k)
+{
+    return (((size_t)c * nz + z) * cv.fy + y) * cv.fxh + k;
+}
+
+// ------------------------------------------------------------------- rows
+template <int MODE, bool COMP>
+__global__ void __launch_bounds__(R3_TPB) k3_rows(const Rep3* __restrict__ reps, const int4* __restrict__ items)
+{
+    extern __shared__ float2 lds[];
+    __shared__ int s_start[64];
+    __shared__ ExEntry s_ent[512];
+    const int4 it = items[blockIdx.x];
+    const Rep3& rp = reps[it.x];
+    const StepArgs& a = rp.a;
+    const Conv3Dev& cv = rp.cv;
+    const int nx = a.d.nx, ny = a.d.ny, nz = a.d.nz, n = a.d.n;
+    const int fx = cv.fx, fxh = cv.fxh, half = fx >> 1;
+    const int z = it.y, y0 = it.z;
+    const int ty = min(cv.ty, ny - y0);
+    const int lines = (3 * ty + 1) >> 1;
+    const bool lr = rp.has_lr != 0;
+    float2* A = lds;
+    float2* B = lds + (size_t)lines * fx;
+    float2* R = A;
+
+    ExchangeDev exl = a.ex;
+    if (MODE == R3_PRED || MODE == R3_CORR) {
+        if (exl.mode == 1 && exl.ncls + 1 <= 64 && exl.nent <= 512) {
+            for (int i = threadIdx.x; i <= exl.ncls; i += R3_TPB) s_start[i] = exl.cls_start[i];
+            for (int i = threadIdx.x; i < exl.nent; i += R3_TPB) s_ent[i] = exl.ent[i];
+            exl.cls_start = s_start;
+            exl.ent = s_ent;
+        }
+    }
+    // inverse x transform of the field spectrum rows
+    if (lr && MODE != R3_FWD) {
+        for (int idx = threadIdx.x; idx < lines * fx; idx += R3_TPB) {
+            const int l = idx % lines, k = idx / lines;
+            const bool mirror = k > half;
+            const int kk = mirror ? fx - k : k;
+            const int q0 = 2 * l, q1 = q0 + 1;
+            const int c0 = q0 / ty, r0 = q0 - c0 * ty;
+            float2 Av = cv.W[widx(cv, nz, c0, z, y0 + r0, kk)];
+            float2 Bv = make_float2(0.f, 0.f);
+            if (q1 < 3 * ty) {
+                const int c1 = q1 / ty, r1 = q1 - c1 * ty;
+                Bv = cv.W[widx(cv, nz, c1, z, y0 + r1, kk)];
+            }
+            if (mirror) { Av.y = -Av.y; Bv.y = -Bv.y; }
+            A[(size_t)l * fx + k] = make_float2(Av.x - Bv.y, Av.y + Bv.x);
+        }
+        __syncthreads();
+        R = lds_fft<1>(A, B, lines, cv.px, cv.twx);
+    } else {
+        __syncthreads();
+    }
+    if (MODE == R3_PRED && a.th.on && z == 0 && y0 == 0 && threadIdx.x == 0) *a.draw += 1ull;
+    const uint64_t draw = (MODE == R3_CORR && a.th.on) ? (uint64_t)*a.draw : 0ull;
+
+    // sites of the tile; the padded tail x in [nx, fx) of every row is zero
+    for (int li = threadIdx.x; li < ty * fx; li += R3_TPB) {
+        const int r = li / fx, x = li - r * fx, y = y0 + r;
+        float3 v = make_float3(0.f, 0.f, 0.f);
+        if (x < nx) {
+            const int i = x + nx * (y + ny * z);
+            if (MODE == R3_FWD) {
+                v = make_float3(cv.sin[i], cv.sin[n + i], cv.sin[2 * n + i]);
+            } else {
+                float3 Hd = make_float3(0.f, 0.f, 0.f);
+                if (lr) Hd = make_float3(rowval3(R, r, x, fx), rowval3(R, ty + r, x, fx), rowval3(R, 2 * ty + r, x, fx));
+                if (MODE == R3_INV) {
+                    cv.hout[i] = Hd.x; cv.hout[n + i] = Hd.y; cv.hout[2 * n + i] = Hd.z;
+                } else if (MODE == R3_PRED) {
+                    v = site_pred<COMP>(a, exl, i, x, y, z, Hd);
+                } else {
+                    v = site_corr<COMP>(a, exl, i, x, y, z, Hd, draw);
+                }
+            }
+        }
+        if (lr && MODE != R3_INV) {
+            rowval3(R, r, x, fx) = v.x;
+            rowval3(R, ty + r, x, fx) = v.y;
+            rowval3(R, 2 * ty + r, x, fx) = v.z;
+        }
+    }
+    if (!lr || MODE == R3_INV) return;
+    // the unpaired imaginary half of the last line (odd 3*ty) must be zero
+    if ((3 * ty) & 1)
+        for (int x = threadIdx.x; x < fx; x += R3_TPB)
+            reinterpret_cast<float*>(R + (size_t)(lines - 1) * fx + x)[1] = 0.0f;
+    __syncthreads();
+    const float2* F = lds_fft<-1>(R, R == A ? B : A, lines, cv.px, cv.twx);
+    for (int idx = threadIdx.x; idx < lines * fxh; idx += R3_TPB) {
+        const int l = idx % lines, k = idx / lines;
+        const float2 Zk = F[(size_t)l * fx + k];
+        const float2 Zm = F[(size_t)l * fx + (k ? fx - k : 0)];
+        const int q0 = 2 * l, q1 = q0 + 1;
+        const int c0 = q0 / ty, r0 = q0 - c0 * ty;
+        cv.W[widx(cv, nz, c0, z, y0 + r0, k)] = make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y));
+        if (q1 < 3 * ty) {
+            const int c1 = q1 / ty, r1 = q1 - c1 * ty;
+            cv.W[widx(cv, nz, c1, z, y0 + r1, k)] = make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x));
+        }
+    }
+}
+
+// ----------------------------------------------------------------- y pass
+// lines along y of kx block [kx0, kx0+w) in plane (c, z); forward reads rows
+// y < ny (zero beyond) and writes all fy, inverse reads fy and writes y < ny.
+template <int SIGN>
+__global__ void __launch_bounds__(Y3_TPB) k3_ypass(const Rep3* __restrict__ reps, const int4* __restrict__ items)
+{
+    extern __shared__ float2 lds[];
+    const int4 it = items[blockIdx.x];
+    const Rep3& rp = reps[it.x];
+    const Conv3Dev& cv = rp.cv;
+    const int ny = rp.a.d.ny;
+    const int fy = cv.fy, fxh = cv.fxh;
+    const int kx0 = it.z;
+    const int w = min(cv.wy, fxh - kx0);
+    float2* A = lds;
+    float2* B = lds + (size_t)w * fy;
+    float2* base = cv.W + (size_t)it.y * fy * fxh + kx0;
+    const int yin = SIGN < 0 ? ny : fy;
+    for (int idx = threadIdx.x; idx < w * fy; idx += Y3_TPB) {
+        const int y = idx / w, j = idx - y * w;
+        A[(size_t)j * fy + y] = y < yin ? base[(size_t)y * fxh + j] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    const float2* R = lds_fft<SIGN>(A, B, w, cv.py, cv.twy);
+    const int yout = SIGN < 0 ? fy : ny;
+    for (int idx = threadIdx.x; idx < w * yout; idx += Y3_TPB) {
+        const int y = idx / w, j = idx - y * w;
+        base[(size_t)y * fxh + j] = R[(size_t)j * fy + y];
+    }
+}
+
+// ----------------------------------------------------------------- z pass
+__global__ void __launch_bounds__(Z3_TPB) k3_zconv(const Rep3* __restrict__ reps, const int4* __restrict__ items)
+{
+    extern __shared__ float2 lds[];
+    const int4 it = items[blockIdx.x];
+    const Rep3& rp = reps[it.x];
+    const Conv3Dev& cv = rp.cv;
+    const int nz = rp.a.d.nz;
+    const int fy = cv.fy, fz = cv.fz, fxh = cv.fxh;
+    const int ky = it.y, kx0 = it.z;
+    const int w = min(cv.wz, fxh - kx0);
+    const int lines = 3 * w;
+    float2* A = lds;
+    float2* B = lds + (size_t)lines * fz;
+    for (int idx = threadIdx.x; idx < lines * fz; idx += Z3_TPB) {
+        const int zz = idx / lines, l = idx - zz * lines;
+        const int c = l / w, j = l - c * w;
+        A[(size_t)l * fz + zz] = zz < nz ? cv.W[widx(cv, nz, c, zz, ky, kx0 + j)] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    float2* R = lds_fft<-1>(A, B, lines, cv.pz, cv.twz);
+    const size_t plane = (size_t)fz * fy * fxh;
+    for (int idx = threadIdx.x; idx < w * fz; idx += Z3_TPB) {
+        const int kz = idx / w, j = idx - kz * w;
+        const size_t t = ((size_t)kz * fy + ky) * fxh + kx0 + j;
+        const float xx = cv.Tw[t], xy = cv.Tw[plane + t], xz = cv.Tw[2 * plane + t];
+        const float yy = cv.Tw[3 * plane + t], yz = cv.Tw[4 * plane + t], zz = cv.Tw[5 * plane + t];
+        const float2 mx = R[(size_t)(0 * w + j) * fz + kz];
+        const float2 my = R[(size_t)(1 * w + j) * fz + kz];
+        const float2 mz = R[(size_t)(2 * w + j) * fz + kz];
+        R[(size_t)(0 * w + j) * fz + kz] = make_float2(xx * mx.x + xy * my.x + xz * mz.x, xx * mx.y + xy * my.y + xz * mz.y);
+        R[(size_t)(1 * w + j) * fz + kz] = make_float2(xy * mx.x + yy * my.x + yz * mz.x, xy * mx.y + yy * my.y + yz * mz.y);
+        R[(size_t)(2 * w + j) * fz + kz] = make_float2(xz * mx.x + yz * my.x + zz * mz.x, xz * mx.y + yz * my.y + zz * mz.y);
+    }
+    __syncthreads();
+    const float2* O = lds_fft<1>(R, R == A ? B : A, lines, cv.pz, cv.twz);
+    for (int idx = threadIdx.x; idx < lines * nz; idx += Z3_TPB) {
+        const int zz = idx / lines, l = idx - zz * lines;
+        const int c = l / w, j = l - c * w;
+        cv.W[widx(cv, nz, c, zz, ky, kx0 + j)] = O[(size_t)l * fz + zz];
+    }
+}
+
+// --------------------------------------------------------------- launches
+hipError_t launch3_rows(int mode, bool comp, const Rep3* reps, const int4* items, int nitems, size_t lds, hipStream_t st)
+{
+    if (nitems <= 0) return hipSuccess;
+    switch (mode) {
+        case R3_FWD: k3_rows<R3_FWD, false><<<nitems, R3_TPB, lds, st>>>(reps, items); break;
+        case R3_INV: k3_rows<R3_INV, false><<<nitems, R3_TPB, lds, st>>>(reps, items); break;
+        case R3_PRED:
+            if (comp) k3_rows<R3_PRED, true><<<nitems, R3_TPB, lds, st>>>(reps, items);
+            else k3_rows<R3_PRED, false><<<nitems, R3_TPB, lds, st>>>(reps, items);
+            break;
+        default:
+            if (comp) k3_rows<R3_CORR, true><<<nitems, R3_TPB, lds, st>>>(reps, items);
+            else k3_rows<R3_CORR, false><<<nitems, R3_TPB, lds, st>>>(reps, items);
+            break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch3_ypass(int sign, const Rep3* reps, const int4* items, int nitems, size_t lds, hipStream_t st)
+{
+    if (nitems <= 0) return hipSuccess;
+    if (sign < 0) k3_ypass<-1><<<nitems, Y3_TPB, lds, st>>>(reps, items);
+    else k3_ypass<1><<<nitems, Y3_TPB, lds, st>>>(reps, items);
+    return hipGetLastError();
+}
+
+hipError_t launch3_zconv(const Rep3* reps, const int4* items, int nitems, size_t lds, hipStream_t st)
+{
+    if (nitems <= 0) return hipSuccess;
+    k3_zconv<<<nitems, Z3_TPB, lds, st>>>(reps, items);
+    return hipGetLastError();
+}
+
+hipError_t configure_lds3(size_t max_bytes)
+{
+    const int v = (int)max_bytes;
+    const void* fns[] = {(const void*)k3_rows<R3_FWD, false>, (const void*)k3_rows<R3_INV, false>,
+                         (const void*)k3_rows<R3_PRED, false>, (const void*)k3_rows<R3_PRED, true>,
+                         (const void*)k3_rows<R3_CORR, false>, (const void*)k3_rows<R3_CORR, true>,
+                         (const void*)k3_ypass<-1>, (const void*)k3_ypass<1>, (const void*)k3_zconv};
+    hipError_t first = hipSuccess;
+    for (const void* f : fns) {
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+        if (e != hipSuccess && first == hipSuccess) first = e;
+    }
+    (void)hipGetLastError();
+    return first;
+}
+
+// ------------------------------------------------------- tensor (FP64)
+__device__ __forceinline__ void gamma6(double rx, double ry, double rz, double* g)
+{
+    const double r = sqrt(rx * rx + ry * ry + rz * rz);
+    if (r < 1e-40) { for (int i = 0; i < 6; ++i) g[i] = 0; return; }
+    const double ir = 1.0 / r, ir3 = ir * ir * ir, ir5 = ir3 * ir * ir;
+    g[0] = -1.0 * (ir3 - 3.0 * rx * rx * ir5);
+    g[1] = -1.0 * (-3.0 * rx * ry * ir5);
+    g[2] = -1.0 * (-3.0 * rx * rz * ir5);
+    g[3] = -1.0 * (ir3 - 3.0 * ry * ry * ir5);
+    g[4] = -1.0 * (-3.0 * ry * rz * ir5);
+    g[5] = -1.0 * (ir3 - 3.0 * rz * rz * ir5);
+}
+
+__device__ __forceinline__ long long fdiv3(long long a, long long b)
+{
+    long long q = a / b;
+    if ((a % b) && ((a < 0) != (b < 0))) --q;
+    return q;
+}
+
+// R > 0: periodic lattice sum over images within radius R (grid = lattice);
+// R == 0: open boundaries, Gamma at the wrapped displacement on the doubled grid.
+__global__ void k3_tensor(int nx, int ny, int nz, int fx, int fy, int fz, int R, double* __restrict__ T)
+{
+    const size_t total = (size_t)fx * fy * fz;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(idx % fx);
+        const int j = (int)((idx / fx) % fy);
+        const int k = (int)(idx / ((size_t)fx * fy));
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        if (R > 0) {
+            const long long LR = R;
+            const long long x0 = -fdiv3(LR + i, nx), x1 = fdiv3(LR - i, nx);
+            const long long y0 = -fdiv3(LR + j, ny), y1 = fdiv3(LR - j, ny);
+            const long long z0 = -fdiv3(LR + k, nz), z1 = fdiv3(LR - k, nz);
+            const double r2max = (double)R * R;
+            for (long long X = x0; X <= x1; ++X)
+                for (long long Y = y0; Y <= y1; ++Y)
+                    for (long long Z = z0; Z <= z1; ++Z) {
+                        const double rx = (double)(X * nx + i), ry = (double)(Y * ny + j), rz = (double)(Z * nz + k);
+                        if (rx * rx + ry * ry + rz * rz > r2max) continue;
+                        double g[6];
+                        gamma6(rx, ry, rz, g);
+                        for (int a = 0; a < 6; ++a) acc[a] += g[a];
+                    }
+        } else if (i != nx && j != ny && k != nz) {
+            gamma6(i < nx ? i : i - fx, j < ny ? j : j - fy, k < nz ? k : k - fz, acc);
+        }
+        for (int a = 0; a < 6; ++a) T[(size_t)a * total + idx] = acc[a];
+    }
+}
+
+hipError_t launch3_tensor(int nx, int ny, int nz, int fx, int fy, int fz, int R, double* T, hipStream_t st)
+{
+    const size_t total = (size_t)fx * fy * fz;
+    const int g = (int)std::min<size_t>((total + 127) / 128, 65536);
+    k3_tensor<<<g, 128, 0, st>>>(nx, ny, nz, fx, fy, fz, R, T);
+    return hipGetLastError();
+}
+
+// x DFT (real -> half spectrum): U1[a][z][y][kx]
+__global__ void k3_dftx(int fx, int fy, int fz, const double* __restrict__ T, double2* __restrict__ U, const double2* __restrict__ tw)
+{
+    const int fxh = fx / 2 + 1;
+    const size_t total = (size_t)6 * fz * fy * fxh;
+    const size_t plane = (size_t)fx * fy * fz;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(idx % fxh);
+        const size_t rowi = idx / fxh;               // a*fz*fy + z*fy + y
+        const int a = (int)(rowi / ((size_t)fz * fy));
+        const size_t zy = rowi - (size_t)a * fz * fy;
+        const double* row = T + (size_t)a * plane + zy * fx;
+        double re = 0, im = 0;
+        int m = 0;
+        for (int x = 0; x < fx; ++x) {
+            const double2 w = tw[m];
+            re += row[x] * w.x;
+            im += row[x] * w.y;
+            m += kx; if (m >= fx) m -= fx;
+        }
+        U[idx] = make_double2(re, im);
+    }
+}
+
+// complex DFT along y (stride fxh) or z (stride fy*fxh) of [6][fz][fy][fxh]
+template <int AXIS>
+__global__ void k3_dftyz(int fy, int fz, int fxh, const double2* __restrict__ U, double2* __restrict__ V,
+                         const double2* __restrict__ tw)
+{
+    const size_t total = (size_t)6 * fz * fy * fxh;
+    const int L = AXIS == 1 ? fy : fz;
+    const size_t stride = AXIS == 1 ? (size_t)fxh : (size_t)fy * fxh;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(idx % fxh);
+        const int y = (int)((idx / fxh) % fy);
+        const int z = (int)((idx / ((size_t)fxh * fy)) % fz);
+        const size_t a = idx / ((size_t)fxh * fy * fz);
+        const int kk = AXIS == 1 ? y : z;
+        const size_t base = a * (size_t)fz * fy * fxh + (AXIS == 1 ? ((size_t)z * fy * fxh + kx) : ((size_t)y * fxh + kx));
+        double re = 0, im = 0;
+        int m = 0;
+        for (int t = 0; t < L; ++t) {
+            const double2 u = U[base + (size_t)t * stride];
+            const double2 w = tw[m];
+            re += u.x * w.x - u.y * w.y;
+            im += u.x * w.y + u.y * w.x;
+            m += kk; if (m >= L) m -= L;
+        }
+        V[idx] = make_double2(re, im);
+    }
+}
+
+__global__ void k3_words(const double2* __restrict__ V, float* __restrict__ Tw, double mult, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        Tw[i] = (float)(V[i].x * mult);
+}
+
+hipError_t launch3_spectrum(int fx, int fy, int fz, const double* T, double2* U1, double2* U2, float* Tw, double mult,
+                            const double2* twx, const double2* twy, const double2* twz, hipStream_t st)
+{
+    const int fxh = fx / 2 + 1;
+    const size_t tot = (size_t)6 * fz * fy * fxh;
+    const int g = (int)std::min<size_t>((tot + 255) / 256, 65536);
+    k3_dftx<<<g, 256, 0, st>>>(fx, fy, fz, T, U1, twx);
+    k3_dftyz<1><<<g, 256, 0, st>>>(fy, fz, fxh, U1, U2, twy);
+    k3_dftyz<2><<<g, 256, 0, st>>>(fy, fz, fxh, U2, U1, twz);
+    k3_words<<<g, 256, 0, st>>>(U1, Tw, mult, tot);
+    return hipGetLastError();
+}
+
+}  // namespace msd

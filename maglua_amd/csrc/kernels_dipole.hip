@@ -289,47 +289,13 @@ __global__ void __launch_bounds__(ROW_TPB) k_step_pred(StepArgs a)
         R = row_fft<NX, TY, 1>(A, B, dp, tw);
     }
     if (a.th.on && blockIdx.x == 0 && threadIdx.x == 0) *a.draw += 1ull;
-    const float* __restrict__ S = a.S;
-    const float* __restrict__ ms_a = a.S + 3 * (size_t)n;
-    const float dt = a.dt, hdt = 0.5f * a.dt;
+    (void)n;
     for (int li = threadIdx.x; li < ty * nx; li += ROW_TPB) {
         const int r = li / nx, x = li - r * nx, y = y0 + r;
         const int i = x + nx * (y + ny * z);
-        const float ms = ms_a[i];
-        const float3 s = make_float3(S[i], S[n + i], S[2 * n + i]);
-        float3 H = exchange_field(exl, a.d, i, x, y, z, S, ms_a);
-        const float3 ha = aniso_field(a.an, i, n, s, ms);
-        H.x += ha.x; H.y += ha.y; H.z += ha.z;
-        if constexpr (DIP) {
-            H.x += rowval(R, r, x, nx);
-            H.y += rowval(R, ty + r, x, nx);
-            H.z += rowval(R, 2 * ty + r, x, nx);
-        }
-        float3 p = s, c = s;
-        float3 clo = make_float3(0.f, 0.f, 0.f);
-        if constexpr (COMP) clo = make_float3(a.Slo[i], a.Slo[n + i], a.Slo[2 * n + i]);
-        if (ms > 0.0f) {
-            const float al = a.lp.site_alpha ? a.lp.site_alpha[i] : a.lp.alpha;
-            const float ga = a.lp.site_gamma ? a.lp.site_gamma[i] : a.lp.gamma;
-            const float3 dM = a.lp.thermal_first ? llg_rhs(s, H, H, al, ga, ms, a.lp.ps, a.lp.ds)
-                                                 : llg_rhs(s, H, H, al, ga, ms, 1.0f, 1.0f);
-            p = make_float3(s.x + dt * dM.x, s.y + dt * dM.y, s.z + dt * dM.z);
-            if constexpr (COMP) {
-                df_add(c.x, clo.x, hdt * dM.x);
-                df_add(c.y, clo.y, hdt * dM.y);
-                df_add(c.z, clo.z, hdt * dM.z);
-            } else {
-                c = make_float3(s.x + hdt * dM.x, s.y + hdt * dM.y, s.z + hdt * dM.z);
-            }
-        }
-        if (a.lp.renorm) {
-            p = renorm(p, ms);
-            if constexpr (COMP) df_renorm(c, clo, ms);
-            else c = renorm(c, ms);
-        }
-        a.P[i] = p.x; a.P[n + i] = p.y; a.P[2 * n + i] = p.z;
-        a.C[i] = c.x; a.C[n + i] = c.y; a.C[2 * n + i] = c.z;
-        if constexpr (COMP) { a.Clo[i] = clo.x; a.Clo[n + i] = clo.y; a.Clo[2 * n + i] = clo.z; }
+        float3 Hd = make_float3(0.f, 0.f, 0.f);
+        if constexpr (DIP) Hd = make_float3(rowval(R, r, x, nx), rowval(R, ty + r, x, nx), rowval(R, 2 * ty + r, x, nx));
+        const float3 p = site_pred<COMP>(a, exl, i, x, y, z, Hd);
         if constexpr (DIP) {
             rowval(R, r, x, nx) = p.x;
             rowval(R, ty + r, x, nx) = p.y;
@@ -371,69 +337,13 @@ __global__ void __launch_bounds__(ROW_TPB) k_step_corr(StepArgs a)
         R = row_fft<NX, TY, 1>(A, B, dp, tw);
     }
     const uint64_t draw = a.th.on ? (uint64_t)*a.draw : 0ull;
-    float* __restrict__ S = a.S;
-    const float* __restrict__ ms_a = a.S + 3 * (size_t)n;
-    const float* __restrict__ P = a.P;
-    const float dt = a.dt, hdt = 0.5f * a.dt;
+    (void)n;
     for (int li = threadIdx.x; li < ty * nx; li += ROW_TPB) {
         const int r = li / nx, x = li - r * nx, y = y0 + r;
         const int i = x + nx * (y + ny * z);
-        const float ms = ms_a[i];
-        const float3 p = make_float3(P[i], P[n + i], P[2 * n + i]);
-        float3 H = exchange_field(exl, a.d, i, x, y, z, P, ms_a);
-        const float3 ha = aniso_field(a.an, i, n, p, ms);
-        H.x += ha.x; H.y += ha.y; H.z += ha.z;
-        if constexpr (DIP) {
-            H.x += rowval(R, r, x, nx);
-            H.y += rowval(R, ty + r, x, nx);
-            H.z += rowval(R, 2 * ty + r, x, nx);
-        }
-        float3 v = make_float3(a.C[i], a.C[n + i], a.C[2 * n + i]);
-        float3 vlo = make_float3(0.f, 0.f, 0.f);
-        if constexpr (COMP) vlo = make_float3(a.Clo[i], a.Clo[n + i], a.Clo[2 * n + i]);
-        const float al = a.lp.site_alpha ? a.lp.site_alpha[i] : a.lp.alpha;
-        const float ga = a.lp.site_gamma ? a.lp.site_gamma[i] : a.lp.gamma;
-        if (ms > 0.0f) {
-            const float3 dM = a.lp.thermal_first ? llg_rhs(p, H, H, al, ga, ms, a.lp.ps, a.lp.ds)
-                                                 : llg_rhs(p, H, H, al, ga, ms, 1.0f, 1.0f);
-            if constexpr (COMP) {
-                df_add(v.x, vlo.x, hdt * dM.x);
-                df_add(v.y, vlo.y, hdt * dM.y);
-                df_add(v.z, vlo.z, hdt * dM.z);
-            } else {
-                v = make_float3(v.x + hdt * dM.x, v.y + hdt * dM.y, v.z + hdt * dM.z);
-            }
-        }
-        if (a.lp.renorm) {
-            if constexpr (COMP) df_renorm(v, vlo, ms);
-            else v = renorm(v, ms);
-        }
-        if (a.th.on) {
-            const float sg = thermal_sigma(a.th, a.lp, i, ms, dt);
-            if (ms > 0.0f) {
-                float3 Ht = make_float3(0.f, 0.f, 0.f);
-                if (sg != 0.0f) {
-                    const float3 g = thermal_normals(a.th.seed, draw, i);
-                    Ht = make_float3(sg * g.x, sg * g.y, sg * g.z);
-                }
-                const float3 zero = make_float3(0.f, 0.f, 0.f);
-                const float3 dM = a.lp.thermal_first ? llg_rhs(v, Ht, zero, al, ga, ms, a.lp.ps, a.lp.ds)
-                                                     : llg_rhs(v, Ht, Ht, al, ga, ms, 1.0f, 1.0f);
-                if constexpr (COMP) {
-                    df_add(v.x, vlo.x, dt * dM.x);
-                    df_add(v.y, vlo.y, dt * dM.y);
-                    df_add(v.z, vlo.z, dt * dM.z);
-                } else {
-                    v = make_float3(v.x + dt * dM.x, v.y + dt * dM.y, v.z + dt * dM.z);
-                }
-            }
-            if (a.lp.renorm) {
-                if constexpr (COMP) df_renorm(v, vlo, ms);
-                else v = renorm(v, ms);
-            }
-        }
-        S[i] = v.x; S[n + i] = v.y; S[2 * n + i] = v.z;
-        if constexpr (COMP) { a.Slo[i] = vlo.x; a.Slo[n + i] = vlo.y; a.Slo[2 * n + i] = vlo.z; }
+        float3 Hd = make_float3(0.f, 0.f, 0.f);
+        if constexpr (DIP) Hd = make_float3(rowval(R, r, x, nx), rowval(R, ty + r, x, nx), rowval(R, 2 * ty + r, x, nx));
+        const float3 v = site_corr<COMP>(a, exl, i, x, y, z, Hd, draw);
         if constexpr (DIP) {
             rowval(R, r, x, nx) = v.x;
             rowval(R, ty + r, x, nx) = v.y;

@@ -106,9 +106,11 @@ extern "C" int msd_init(int device)
     HIPCHK(hipSetDevice(device));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
-    G.max_lds = prop.sharedMemPerBlock;
-    // a runtime without the attribute keeps its default; plans check G.max_lds
-    if (configure_lds(G.max_lds) != hipSuccess) G.max_lds = std::min<size_t>(G.max_lds, 65536);
+    // dynamic LDS budget per workgroup: the device's LDS less 8 KiB kept for the
+    // kernels' static tables (exchange classes); plans are checked against it
+    G.max_lds = prop.sharedMemPerBlock > 16384 ? prop.sharedMemPerBlock - 8192 : prop.sharedMemPerBlock;
+    if (configure_lds(G.max_lds) != hipSuccess || configure_lds3(G.max_lds) != hipSuccess)
+        G.max_lds = std::min<size_t>(G.max_lds, 65536 - 8192);
     HIPCHK(hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking));
     G.device = device;
     G.ready = true;
@@ -1162,12 +1164,323 @@ extern "C" int msd_llg_apply(msd_llg* l, msd_system* from, double scale, msd_sys
     return MSD_OK;
 }
 
+// =============================================================== LongRange3D
+// LongRange3D (cpu/longrange3d/spinoperationlongrange3d.cpp:331-432) with the
+// dipole tensor built in FP64 on the device: the periodic lattice sum to radius R
+// (the 3D analogue of Dipole's truncated sum), or, with open boundaries, the
+// bare dipole kernel on the doubled FFT grid.
+struct msd_longrange3d {
+    Dims d{};
+    bool open = false;
+    int fx = 0, fy = 0, fz = 0, fxh = 0;
+    double g = 1, scale = 1;
+    int R = 1000;
+    bool loaded = false, custom = false, spectrum = false;
+    double words_mult = NAN;
+    double* T64 = nullptr;   // [6][fz][fy][fx]
+    float* Tw = nullptr;     // [6][fz][fy][fxh]
+    float2* W = nullptr;     // working spectrum [3][nz][fy][fxh]
+    float2 *twx = nullptr, *twy = nullptr, *twz = nullptr;
+    double2 *twx64 = nullptr, *twy64 = nullptr, *twz64 = nullptr;
+    FFTPlan px{}, py{}, pz{};
+    int ty = 1, wy = 1, wz = 1;
+    size_t lds_rows = 0, lds_y = 0, lds_z = 0;
+    // single-replica descriptor and work items for the operator API
+    Rep3* orep = nullptr;
+    int4 *oitems_r = nullptr, *oitems_y = nullptr, *oitems_z = nullptr;
+    int onr = 0, ony = 0, onz = 0;
+    ~msd_longrange3d()
+    {
+        (void)hipStreamSynchronize(G.stream);
+        dfree(T64); dfree(Tw); dfree(W); dfree(twx); dfree(twy); dfree(twz);
+        dfree(twx64); dfree(twy64); dfree(twz64); dfree(orep); dfree(oitems_r); dfree(oitems_y); dfree(oitems_z);
+    }
+    int prepare();
+    Conv3Dev conv() const
+    {
+        Conv3Dev c{};
+        c.fx = fx; c.fy = fy; c.fz = fz; c.fxh = fxh;
+        c.ty = ty; c.wy = wy; c.wz = wz;
+        c.px = px; c.py = py; c.pz = pz;
+        c.twx = twx; c.twy = twy; c.twz = twz;
+        c.W = W; c.Tw = Tw;
+        return c;
+    }
+};
+
+static int choose_width(int L, int lines_per, size_t budget)
+{
+    for (int w : {16, 8, 4, 2})
+        if (2 * (size_t)w * lines_per * L * sizeof(float2) <= budget) return w;
+    return 1;
+}
+
+extern "C" int msd_longrange3d_create(int nx, int ny, int nz, int open, msd_longrange3d** out)
+{
+    NONNULL(out);
+    *out = nullptr;
+    if (nx < 1 || ny < 1 || nz < 1) return fail(MSD_EINVAL, "LongRange3D.new: bad size");
+    if (int rc = ensure_init()) return rc;
+    std::unique_ptr<msd_longrange3d> l(new msd_longrange3d);
+    l->d = Dims{nx, ny, nz, nx * ny * nz};
+    l->open = open != 0;
+    l->fx = open ? 2 * nx : nx; l->fy = open ? 2 * ny : ny; l->fz = open ? 2 * nz : nz;
+    if (l->fx % 2) return fail(MSD_EUNSUPPORTED, "LongRange3D.new: the x transform length %d must be even", l->fx);
+    if (!make_plan(l->fx, l->px) || !make_plan(l->fy, l->py) || !make_plan(l->fz, l->pz))
+        return fail(MSD_EUNSUPPORTED, "LongRange3D.new: %dx%dx%d transform needs prime factors in {2,3,5}", l->fx, l->fy, l->fz);
+    l->fxh = l->fx / 2 + 1;
+    const size_t budget = 65536;
+    l->ty = 1;
+    for (int c : {8, 4, 2})
+        if ((2 * (size_t)((3 * c + 1) / 2)) * l->fx * sizeof(float2) <= budget) { l->ty = c; break; }
+    l->wy = choose_width(l->fy, 1, budget);
+    l->wz = choose_width(l->fz, 3, budget);
+    l->lds_rows = 2 * (size_t)((3 * l->ty + 1) / 2) * l->fx * sizeof(float2);
+    l->lds_y = 2 * (size_t)l->wy * l->fy * sizeof(float2);
+    l->lds_z = 2 * (size_t)3 * l->wz * l->fz * sizeof(float2);
+    if (l->lds_rows > G.max_lds || l->lds_y > G.max_lds || l->lds_z > G.max_lds)
+        return fail(MSD_EUNSUPPORTED, "LongRange3D.new: %dx%dx%d transform exceeds the LDS per workgroup", l->fx, l->fy, l->fz);
+    if (int rc = twiddles(l->fx, &l->twx, &l->twx64)) return rc;
+    if (int rc = twiddles(l->fy, &l->twy, &l->twy64)) return rc;
+    if (int rc = twiddles(l->fz, &l->twz, &l->twz64)) return rc;
+    if (int rc = dalloc(&l->W, (size_t)3 * nz * l->fy * l->fxh)) return rc;
+    if (int rc = dalloc(&l->Tw, (size_t)6 * l->fz * l->fy * l->fxh)) return rc;
+    *out = l.release();
+    return MSD_OK;
+}
+
+extern "C" int msd_longrange3d_destroy(msd_longrange3d* l) { delete l; return MSD_OK; }
+extern "C" int msd_longrange3d_set_strength(msd_longrange3d* l, double g) { NONNULL(l); l->g = g; return MSD_OK; }
+extern "C" int msd_longrange3d_set_scale(msd_longrange3d* l, double s) { NONNULL(l); l->scale = s; return MSD_OK; }
+extern "C" int msd_longrange3d_set_radius(msd_longrange3d* l, int R)
+{
+    NONNULL(l);
+    if (R < 1) return fail(MSD_EINVAL, "LongRange3D: lattice-sum radius %d", R);
+    l->R = R;
+    l->loaded = false; l->custom = false; l->spectrum = false;
+    return MSD_OK;
+}
+
+static int lr3_load(msd_longrange3d* l)
+{
+    if (l->loaded) return MSD_OK;
+    const size_t m = (size_t)6 * l->fx * l->fy * l->fz;
+    if (!l->T64)
+        if (int rc = dalloc(&l->T64, m)) return rc;
+    hipError_t e = launch3_tensor(l->d.nx, l->d.ny, l->d.nz, l->fx, l->fy, l->fz, l->open ? 0 : l->R, l->T64, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "LongRange3D tensor: %s", hipGetErrorString(e));
+    l->loaded = true;
+    l->spectrum = false;
+    return MSD_OK;
+}
+
+static int lr3_offset(msd_longrange3d* l, int which, int ox, int oy, int oz, size_t* idx)
+{
+    if (which < 0 || which > 5) return fail(MSD_EINVAL, "matrix index %d (0..5 = XX XY XZ YY YZ ZZ)", which);
+    ox = ((ox % l->fx) + l->fx) % l->fx;
+    oy = ((oy % l->fy) + l->fy) % l->fy;
+    oz = ((oz % l->fz) + l->fz) % l->fz;
+    *idx = (size_t)which * l->fx * l->fy * l->fz + ox + (size_t)l->fx * (oy + (size_t)l->fy * oz);
+    return MSD_OK;
+}
+
+extern "C" int msd_longrange3d_get_matrix(msd_longrange3d* l, int which, int ox, int oy, int oz, double* v)
+{
+    NONNULL(l); NONNULL(v);
+    size_t idx;
+    if (int rc = lr3_offset(l, which, ox, oy, oz, &idx)) return rc;
+    if (int rc = lr3_load(l)) return rc;
+    HIPCHK(hipMemcpyAsync(v, l->T64 + idx, sizeof(double), hipMemcpyDeviceToHost, G.stream));
+    HIPCHK(hipStreamSynchronize(G.stream));
+    return MSD_OK;
+}
+
+extern "C" int msd_longrange3d_set_matrix(msd_longrange3d* l, int which, int ox, int oy, int oz, double v)
+{
+    NONNULL(l);
+    size_t idx;
+    if (int rc = lr3_offset(l, which, ox, oy, oz, &idx)) return rc;
+    if (int rc = lr3_load(l)) return rc;
+    HIPCHK(hipMemcpyAsync(l->T64 + idx, &v, sizeof(double), hipMemcpyHostToDevice, G.stream));
+    HIPCHK(hipStreamSynchronize(G.stream));
+    l->custom = true;
+    l->spectrum = false;
+    return MSD_OK;
+}
+
+// the stored spectrum is the real part: the tensor must be even, T(-r) = T(r)
+static int lr3_check_even(msd_longrange3d* l)
+{
+    const size_t plane = (size_t)l->fx * l->fy * l->fz;
+    std::vector<double> t(6 * plane);
+    HIPCHK(hipMemcpyAsync(t.data(), l->T64, sizeof(double) * t.size(), hipMemcpyDeviceToHost, G.stream));
+    HIPCHK(hipStreamSynchronize(G.stream));
+    double big = 0;
+    for (double v : t) big = std::max(big, std::fabs(v));
+    for (int a = 0; a < 6; ++a)
+        for (int z = 0; z < l->fz; ++z)
+            for (int y = 0; y < l->fy; ++y)
+                for (int x = 0; x < l->fx; ++x) {
+                    const size_t i = a * plane + x + (size_t)l->fx * (y + (size_t)l->fy * z);
+                    const size_t j = a * plane + (l->fx - x) % l->fx + (size_t)l->fx * ((l->fy - y) % l->fy + (size_t)l->fy * ((l->fz - z) % l->fz));
+                    if (std::fabs(t[i] - t[j]) > 1e-12 * big)
+                        return fail(MSD_EUNSUPPORTED, "LongRange3D: custom tensor component %d is not even at (%d,%d,%d)", a, x, y, z);
+                }
+    return MSD_OK;
+}
+
+int msd_longrange3d::prepare()
+{
+    if (int rc = lr3_load(this)) return rc;
+    if (!spectrum) {
+        if (custom)
+            if (int rc = lr3_check_even(this)) return rc;
+        words_mult = NAN;
+        spectrum = true;
+    }
+    const double mult = g * scale / ((double)fx * fy * fz);
+    if (!(words_mult == mult)) {
+        const size_t wn = (size_t)6 * fz * fy * fxh;
+        double2 *U1 = nullptr, *U2 = nullptr;
+        if (int rc = dalloc(&U1, wn)) return rc;
+        if (int rc = dalloc(&U2, wn)) { dfree(U1); return rc; }
+        hipError_t e = launch3_spectrum(fx, fy, fz, T64, U1, U2, Tw, mult, twx64, twy64, twz64, G.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(G.stream);
+        dfree(U1); dfree(U2);
+        if (e != hipSuccess) return fail(MSD_EDEVICE, "LongRange3D spectrum: %s", hipGetErrorString(e));
+        words_mult = mult;
+    }
+    return MSD_OK;
+}
+
+// work items of one replica appended to the tables
+static void items3(int rep, const Dims& d, const Conv3Dev* cv, int ty, std::vector<int4>& rows, std::vector<int4>& ys,
+                   std::vector<int4>& zs)
+{
+    for (int z = 0; z < d.nz; ++z)
+        for (int y0 = 0; y0 < d.ny; y0 += ty) rows.push_back(make_int4(rep, z, y0, 0));
+    if (!cv) return;
+    for (int l = 0; l < 3 * d.nz; ++l)
+        for (int k = 0; k < cv->fxh; k += cv->wy) ys.push_back(make_int4(rep, l, k, 0));
+    for (int ky = 0; ky < cv->fy; ++ky)
+        for (int k = 0; k < cv->fxh; k += cv->wz) zs.push_back(make_int4(rep, ky, k, 0));
+}
+
+template <class T>
+static int upload_vec(T** dst, const std::vector<T>& v)
+{
+    dfree(*dst);
+    if (v.empty()) return MSD_OK;
+    if (int rc = dalloc(dst, v.size())) return rc;
+    HIPCHK(hipMemcpyAsync(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, G.stream));
+    return MSD_OK;
+}
+
+extern "C" int msd_longrange3d_apply(msd_longrange3d* l, msd_system* s)
+{
+    NONNULL(l); NONNULL(s);
+    if (!same_dims(s->d, l->d.nx, l->d.ny, l->d.nz)) return fail(MSD_EMISMATCH, "LongRange3D:apply: size mismatch");
+    if (int rc = l->prepare()) return rc;
+    int k;
+    if (int rc = s->mark("LongRange3D", &k)) return rc;
+    Rep3 r{};
+    r.a.d = s->d;
+    r.cv = l->conv();
+    r.cv.sin = s->S;
+    r.cv.hout = s->slots[k].h;
+    r.has_lr = 1;
+    if (!l->orep) {
+        std::vector<int4> rows, ys, zs;
+        items3(0, l->d, &r.cv, l->ty, rows, ys, zs);
+        if (int rc = upload_vec(&l->oitems_r, rows)) return rc;
+        if (int rc = upload_vec(&l->oitems_y, ys)) return rc;
+        if (int rc = upload_vec(&l->oitems_z, zs)) return rc;
+        l->onr = (int)rows.size(); l->ony = (int)ys.size(); l->onz = (int)zs.size();
+        if (int rc = dalloc(&l->orep, 1)) return rc;
+    }
+    HIPCHK(hipMemcpyAsync(l->orep, &r, sizeof r, hipMemcpyHostToDevice, G.stream));
+    hipError_t e = launch3_rows(R3_FWD, false, l->orep, l->oitems_r, l->onr, l->lds_rows, G.stream);
+    if (e == hipSuccess) e = launch3_ypass(-1, l->orep, l->oitems_y, l->ony, l->lds_y, G.stream);
+    if (e == hipSuccess) e = launch3_zconv(l->orep, l->oitems_z, l->onz, l->lds_z, G.stream);
+    if (e == hipSuccess) e = launch3_ypass(1, l->orep, l->oitems_y, l->ony, l->lds_y, G.stream);
+    if (e == hipSuccess) e = launch3_rows(R3_INV, false, l->orep, l->oitems_r, l->onr, l->lds_rows, G.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(G.stream);   // the descriptor lives on the host stack
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "LongRange3D:apply: %s", hipGetErrorString(e));
+    return MSD_OK;
+}
+
+// ------------------------------------------------------------ batch of replicas
+// Device descriptors and work tables of replicas stepped by the same launches.
+struct Batch3 {
+    Rep3* reps = nullptr;
+    int4 *rows = nullptr, *ys = nullptr, *zs = nullptr;
+    int nrep = 0, nrows = 0, nys = 0, nzs = 0;
+    size_t lds_rows = 0, lds_y = 0, lds_z = 0;
+    bool any_lr = false;
+    std::vector<Rep3> host;
+    ~Batch3() { release(); }
+    void release()
+    {
+        (void)hipStreamSynchronize(G.stream);
+        dfree(reps); dfree(rows); dfree(ys); dfree(zs);
+        host.clear();
+        nrep = nrows = nys = nzs = 0;
+    }
+    // rebuild the tables when the member set changed; refresh descriptors always
+    int update(const std::vector<Rep3>& r, const std::vector<int>& tys, const std::vector<size_t>& lds3)
+    {
+        const bool same_shape = host.size() == r.size() && reps != nullptr;
+        if (!same_shape) {
+            release();
+            std::vector<int4> R, Y, Z;
+            any_lr = false;
+            lds_rows = lds_y = lds_z = 0;
+            for (size_t i = 0; i < r.size(); ++i) {
+                items3((int)i, r[i].a.d, r[i].has_lr ? &r[i].cv : nullptr, tys[i], R, Y, Z);
+                any_lr |= r[i].has_lr != 0;
+                lds_rows = std::max(lds_rows, lds3[3 * i]);
+                lds_y = std::max(lds_y, lds3[3 * i + 1]);
+                lds_z = std::max(lds_z, lds3[3 * i + 2]);
+            }
+            if (int rc = upload_vec(&rows, R)) return rc;
+            if (int rc = upload_vec(&ys, Y)) return rc;
+            if (int rc = upload_vec(&zs, Z)) return rc;
+            nrows = (int)R.size(); nys = (int)Y.size(); nzs = (int)Z.size();
+            if (int rc = dalloc(&reps, r.size())) return rc;
+            nrep = (int)r.size();
+        }
+        if (!same_shape || std::memcmp(host.data(), r.data(), sizeof(Rep3) * r.size()) != 0) {
+            host = r;
+            HIPCHK(hipMemcpyAsync(reps, host.data(), sizeof(Rep3) * host.size(), hipMemcpyHostToDevice, G.stream));
+            HIPCHK(hipStreamSynchronize(G.stream));
+        }
+        return MSD_OK;
+    }
+    hipError_t evaluate(int row_mode, bool comp, hipStream_t st) const
+    {
+        hipError_t e = hipSuccess;
+        if (any_lr) {
+            e = launch3_ypass(-1, reps, ys, nys, lds_y, st);
+            if (e == hipSuccess) e = launch3_zconv(reps, zs, nzs, lds_z, st);
+            if (e == hipSuccess) e = launch3_ypass(1, reps, ys, nys, lds_y, st);
+        }
+        if (e == hipSuccess) e = launch3_rows(row_mode, comp, reps, rows, nrows, lds_rows, st);
+        return e;
+    }
+};
+
 // ============================================================ step driver
+// The step closure of PredictorCorrector.lua with one corrector pass, as fused
+// kernels.  Three forms: local fields only, layered dipole (Dipole), or the 3D
+// convolution (LongRange3D); a stepper of the 3D form is a batch of one.
+enum StepKind { SK_LOCAL = 0, SK_LAYERED = 1, SK_CONV3 = 2 };
+
 struct msd_stepper {
     msd_system* s = nullptr;
     msd_exchange* ex = nullptr;
     msd_anisotropy* an = nullptr;
     msd_dipole* dip = nullptr;
+    msd_longrange3d* lr3 = nullptr;
     msd_thermal* th = nullptr;
     msd_llg llg;
     float *P = nullptr, *C = nullptr, *Clo = nullptr;
@@ -1178,6 +1491,7 @@ struct msd_stepper {
     hipGraphExec_t exec = nullptr;
     StepArgs args{};
     StepArgs graph_args{};
+    Batch3 b3;
     uint64_t spec_version = 0;
     ~msd_stepper()
     {
@@ -1191,7 +1505,22 @@ struct msd_stepper {
         if (graph) (void)hipGraphDestroy(graph);
         exec = nullptr; graph = nullptr;
     }
+    int kind() const { return lr3 ? SK_CONV3 : (dip ? SK_LAYERED : SK_LOCAL); }
     int refresh();
+    Rep3 rep() const
+    {
+        Rep3 r{};
+        r.a = args;
+        if (lr3) {
+            r.cv = lr3->conv();
+            r.has_lr = 1;
+        } else {
+            r.cv.fx = s->d.nx;
+            r.cv.ty = args.ty;
+        }
+        r.cv.sin = s->S;
+        return r;
+    }
     int launch_step(hipStream_t st);
 };
 
@@ -1210,23 +1539,38 @@ int msd_stepper::refresh()
     if (th) { if (int rc = th->dev(a.th)) return rc; }
     a.lp = s->llgdev(&llg);
     a.has_dip = dip ? 1 : 0;
-    if (dip) { if (int rc = dip->prepare()) return rc; a.dp = dip->dev; a.ty = dip->dev.ty; }
-    else {
+    if (dip) {
+        if (int rc = dip->prepare()) return rc;
+        a.dp = dip->dev;
+        a.ty = dip->dev.ty;
+    } else if (lr3) {
+        if (int rc = lr3->prepare()) return rc;
+        a.ty = lr3->ty;
+    } else {
         a.ty = 1;
         for (int c : {8, 4, 2}) if (s->d.ny % c == 0) { a.ty = c; break; }
     }
     a.draw = draw;
     args = a;
+    if (lr3) {
+        const std::vector<size_t> lds = {lr3->lds_rows, lr3->lds_y, lr3->lds_z};
+        if (int rc = b3.update({rep()}, {args.ty}, lds)) return rc;
+    }
     return MSD_OK;
 }
 
 int msd_stepper::launch_step(hipStream_t st)
 {
     hipError_t e = hipSuccess;
-    if (args.has_dip) e = launch_cols(args.dp, st);
-    if (e == hipSuccess) e = launch_step_pred(args, st);
-    if (e == hipSuccess && args.has_dip) e = launch_cols(args.dp, st);
-    if (e == hipSuccess) e = launch_step_corr(args, st);
+    if (kind() == SK_CONV3) {
+        e = b3.evaluate(R3_PRED, compensated, st);
+        if (e == hipSuccess) e = b3.evaluate(R3_CORR, compensated, st);
+    } else {
+        if (args.has_dip) e = launch_cols(args.dp, st);
+        if (e == hipSuccess) e = launch_step_pred(args, st);
+        if (e == hipSuccess && args.has_dip) e = launch_cols(args.dp, st);
+        if (e == hipSuccess) e = launch_step_corr(args, st);
+    }
     if (e != hipSuccess) return fail(MSD_EDEVICE, "step launch: %s", hipGetErrorString(e));
     return MSD_OK;
 }
@@ -1258,27 +1602,56 @@ extern "C" int msd_stepper_create(msd_system* s, msd_exchange* ex, msd_anisotrop
 
 extern "C" int msd_stepper_destroy(msd_stepper* st) { delete st; return MSD_OK; }
 extern "C" int msd_stepper_set_graph(msd_stepper* st, int yes) { NONNULL(st); st->use_graph = yes != 0; return MSD_OK; }
-extern "C" int msd_stepper_set_compensated(msd_stepper* st, int yes) { NONNULL(st); st->compensated = yes != 0; return MSD_OK; }
+extern "C" int msd_stepper_set_compensated(msd_stepper* st, int yes)
+{
+    NONNULL(st);
+    st->compensated = yes != 0;
+    st->drop_graph();
+    return MSD_OK;
+}
 
-static int stepper_begin(msd_stepper* st)
+extern "C" int msd_stepper_set_longrange3d(msd_stepper* st, msd_longrange3d* l)
+{
+    NONNULL(st);
+    if (l && !same_dims(st->s->d, l->d.nx, l->d.ny, l->d.nz)) return fail(MSD_EMISMATCH, "stepper: LongRange3D size differs");
+    if (l && st->dip) return fail(MSD_EUNSUPPORTED, "stepper: a Dipole and a LongRange3D in one step driver");
+    st->lr3 = l;
+    st->drop_graph();
+    st->b3.release();
+    st->spec_version = 0;
+    return st->refresh();
+}
+
+// state checks shared by single steppers and ensembles before a run of steps
+static int member_begin(msd_stepper* st)
 {
     if (int rc = st->refresh()) return rc;
-    // a state set without its residual (operator API writes) starts with lo = 0
     if (st->compensated && !st->s->lo_valid) {
         hipError_t e = launch_fill(st->s->Slo, 0.0f, (size_t)3 * st->s->d.n, G.stream);
         if (e != hipSuccess) return fail(MSD_EDEVICE, "stepper: %s", hipGetErrorString(e));
         st->s->lo_valid = true;
-    }
-    // spin spectrum of the current state for the first column pass
-    if (st->args.has_dip && st->spec_version != st->s->version) {
-        hipError_t e = launch_rows_fwd(st->args.dp, st->s->S, G.stream);
-        if (e != hipSuccess) return fail(MSD_EDEVICE, "stepper: %s", hipGetErrorString(e));
     }
     // device draw counter: the predictor increments it before the corrector reads it
     if (st->th) {
         const unsigned long long v = (unsigned long long)(st->th->draw - 1);
         HIPCHK(hipMemcpyAsync(st->draw, &v, sizeof v, hipMemcpyHostToDevice, G.stream));
         HIPCHK(hipStreamSynchronize(G.stream));
+    }
+    return MSD_OK;
+}
+
+static bool spectrum_stale(const msd_stepper* st) { return st->spec_version != st->s->version; }
+
+static int stepper_begin(msd_stepper* st)
+{
+    if (int rc = member_begin(st)) return rc;
+    // spin spectrum of the current state for the first field evaluation
+    if (spectrum_stale(st)) {
+        hipError_t e = hipSuccess;
+        if (st->kind() == SK_LAYERED) e = launch_rows_fwd(st->args.dp, st->s->S, G.stream);
+        else if (st->kind() == SK_CONV3)
+            e = launch3_rows(R3_FWD, false, st->b3.reps, st->b3.rows, st->b3.nrows, st->b3.lds_rows, G.stream);
+        if (e != hipSuccess) return fail(MSD_EDEVICE, "stepper: %s", hipGetErrorString(e));
     }
     return MSD_OK;
 }
@@ -1320,27 +1693,49 @@ extern "C" int msd_stepper_heun(msd_stepper* st, int nsteps)
     return MSD_OK;
 }
 
+// one launch of a step's sequence, by code (for the event-timed profile)
+enum { L_COLS, L_PRED2, L_CORR2, L_Y_FWD, L_ZCONV, L_Y_INV, L_PRED3, L_CORR3 };
+static const char* launch_name(int code)
+{
+    static const char* names[] = {"k_cols", "k_step_pred", "k_step_corr", "k3_ypass_fwd",
+                                  "k3_zconv", "k3_ypass_inv", "k3_rows_pred", "k3_rows_corr"};
+    return names[code];
+}
+
+static hipError_t launch_code(const msd_stepper* st, int code, hipStream_t s)
+{
+    const Batch3& b = st->b3;
+    switch (code) {
+        case L_COLS: return launch_cols(st->args.dp, s);
+        case L_PRED2: return launch_step_pred(st->args, s);
+        case L_CORR2: return launch_step_corr(st->args, s);
+        case L_Y_FWD: return launch3_ypass(-1, b.reps, b.ys, b.nys, b.lds_y, s);
+        case L_ZCONV: return launch3_zconv(b.reps, b.zs, b.nzs, b.lds_z, s);
+        case L_Y_INV: return launch3_ypass(1, b.reps, b.ys, b.nys, b.lds_y, s);
+        case L_PRED3: return launch3_rows(R3_PRED, st->compensated, b.reps, b.rows, b.nrows, b.lds_rows, s);
+        default: return launch3_rows(R3_CORR, st->compensated, b.reps, b.rows, b.nrows, b.lds_rows, s);
+    }
+}
+
 extern "C" int msd_stepper_profile(msd_stepper* st, int nsteps, int cap, char names[][32], double* avg_ms, int* count)
 {
     NONNULL(st); NONNULL(names); NONNULL(avg_ms); NONNULL(count);
     if (nsteps <= 0) return fail(MSD_EINVAL, "profile: nsteps %d", nsteps);
     if (int rc = stepper_begin(st)) return rc;
-    const int per = st->args.has_dip ? 4 : 2;
+    std::vector<int> seq;
+    if (st->kind() == SK_CONV3) seq = {L_Y_FWD, L_ZCONV, L_Y_INV, L_PRED3, L_Y_FWD, L_ZCONV, L_Y_INV, L_CORR3};
+    else if (st->kind() == SK_LAYERED) seq = {L_COLS, L_PRED2, L_COLS, L_CORR2};
+    else seq = {L_PRED2, L_CORR2};
+    const int per = (int)seq.size();
     std::vector<hipEvent_t> ev((size_t)per * nsteps + 1);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-    // launch sequence of one step: 0 = column pass, 1 = predictor, 2 = corrector
-    const int seq4[4] = {0, 1, 0, 2}, seq2[2] = {1, 2};
-    const int* seq = per == 4 ? seq4 : seq2;
-    const char* kname[3] = {"k_cols", "k_step_pred", "k_step_corr"};
     int rc = MSD_OK;
     size_t q = 0;
     HIPCHK(hipEventRecord(ev[q++], G.stream));
     for (int i = 0; i < nsteps && !rc; ++i) {
         hipError_t e = hipSuccess;
         for (int k = 0; k < per && e == hipSuccess; ++k) {
-            if (seq[k] == 0) e = launch_cols(st->args.dp, G.stream);
-            else if (seq[k] == 1) e = launch_step_pred(st->args, G.stream);
-            else e = launch_step_corr(st->args, G.stream);
+            e = launch_code(st, seq[k], G.stream);
             if (e == hipSuccess) e = hipEventRecord(ev[q++], G.stream);
         }
         if (e != hipSuccess) rc = fail(MSD_EDEVICE, "profile: %s", hipGetErrorString(e));
@@ -1353,7 +1748,7 @@ extern "C" int msd_stepper_profile(msd_stepper* st, int nsteps, int cap, char na
         for (size_t j = 1; j < q; ++j) {
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, ev[j - 1], ev[j]));
-            const char* name = kname[seq[(j - 1) % per]];
+            const char* name = launch_name(seq[(j - 1) % per]);
             size_t u = 0;
             while (u < nm.size() && nm[u] != name) ++u;
             if (u == nm.size()) { nm.push_back(name); tot.push_back(0); cnt.push_back(0); }
@@ -1368,4 +1763,116 @@ extern "C" int msd_stepper_profile(msd_stepper* st, int nsteps, int cap, char na
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     return rc;
+}
+
+// ================================================================= ensemble
+// Independent replicas (uneven sizes, own temperatures and seeds) advanced by
+// one set of launches per step: every member is a step driver of the local or
+// the 3D form; its descriptor is one entry of the batch.
+struct msd_ensemble {
+    std::vector<msd_stepper*> m;
+    bool compensated = true;
+    bool use_graph = true;
+    Batch3 b;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~msd_ensemble()
+    {
+        (void)hipStreamSynchronize(G.stream);
+        drop_graph();
+    }
+    void drop_graph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr; graph = nullptr;
+    }
+    int refresh()
+    {
+        std::vector<Rep3> r;
+        std::vector<int> tys;
+        std::vector<size_t> lds;
+        for (msd_stepper* s : m) {
+            s->compensated = compensated;
+            if (int rc = s->refresh()) return rc;
+            r.push_back(s->rep());
+            tys.push_back(s->args.ty);
+            if (s->lr3) { lds.push_back(s->lr3->lds_rows); lds.push_back(s->lr3->lds_y); lds.push_back(s->lr3->lds_z); }
+            else { lds.push_back(0); lds.push_back(0); lds.push_back(0); }
+        }
+        return b.update(r, tys, lds);
+    }
+};
+
+extern "C" int msd_ensemble_create(msd_stepper* const* members, int count, msd_ensemble** out)
+{
+    NONNULL(members); NONNULL(out);
+    *out = nullptr;
+    if (count < 1) return fail(MSD_EINVAL, "ensemble: %d members", count);
+    std::unique_ptr<msd_ensemble> e(new msd_ensemble);
+    for (int i = 0; i < count; ++i) {
+        msd_stepper* s = members[i];
+        if (!s) return fail(MSD_EINVAL, "ensemble: member %d is NULL", i);
+        if (s->kind() == SK_LAYERED) return fail(MSD_EUNSUPPORTED, "ensemble: member %d uses the layered Dipole", i);
+        for (int j = 0; j < i; ++j)
+            if (members[j] == s || members[j]->s == s->s) return fail(MSD_EINVAL, "ensemble: member %d repeats a SpinSystem", i);
+        e->m.push_back(s);
+    }
+    if (int rc = e->refresh()) return rc;
+    *out = e.release();
+    return MSD_OK;
+}
+
+extern "C" int msd_ensemble_destroy(msd_ensemble* e) { delete e; return MSD_OK; }
+extern "C" int msd_ensemble_set_graph(msd_ensemble* e, int yes) { NONNULL(e); e->use_graph = yes != 0; return MSD_OK; }
+extern "C" int msd_ensemble_set_compensated(msd_ensemble* e, int yes)
+{
+    NONNULL(e);
+    e->compensated = yes != 0;
+    e->drop_graph();
+    return MSD_OK;
+}
+
+extern "C" int msd_ensemble_heun(msd_ensemble* e, int nsteps)
+{
+    NONNULL(e);
+    if (nsteps < 0) return fail(MSD_EINVAL, "ensemble heun: nsteps %d", nsteps);
+    if (nsteps == 0) return MSD_OK;
+    if (int rc = e->refresh()) return rc;
+    bool stale = false;
+    for (msd_stepper* s : e->m) {
+        if (int rc = member_begin(s)) return rc;
+        stale |= s->lr3 && spectrum_stale(s);
+    }
+    if (stale) {
+        hipError_t err = launch3_rows(R3_FWD, false, e->b.reps, e->b.rows, e->b.nrows, e->b.lds_rows, G.stream);
+        if (err != hipSuccess) return fail(MSD_EDEVICE, "ensemble: %s", hipGetErrorString(err));
+    }
+    auto one_step = [&](hipStream_t st) -> hipError_t {
+        hipError_t err = e->b.evaluate(R3_PRED, e->compensated, st);
+        if (err == hipSuccess) err = e->b.evaluate(R3_CORR, e->compensated, st);
+        return err;
+    };
+    if (e->use_graph) {
+        if (!e->exec) {
+            HIPCHK(hipStreamBeginCapture(G.stream, hipStreamCaptureModeThreadLocal));
+            hipError_t err = one_step(G.stream);
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(G.stream, &g);
+            if (err != hipSuccess || e2 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return fail(MSD_EDEVICE, "ensemble graph: %s", hipGetErrorString(err != hipSuccess ? err : e2));
+            }
+            e->graph = g;
+            HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+        }
+        for (int i = 0; i < nsteps; ++i) HIPCHK(hipGraphLaunch(e->exec, G.stream));
+    } else {
+        for (int i = 0; i < nsteps; ++i) {
+            hipError_t err = one_step(G.stream);
+            if (err != hipSuccess) return fail(MSD_EDEVICE, "ensemble step: %s", hipGetErrorString(err));
+        }
+    }
+    for (msd_stepper* s : e->m) stepper_end(s, nsteps);
+    return MSD_OK;
 }

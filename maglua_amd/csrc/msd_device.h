@@ -396,4 +396,113 @@ __device__ __forceinline__ float3 exchange_field(const ExchangeDev& ex, const Di
     return h;
 }
 
+// ------------------------------------------------------ Heun site updates
+// Predictor at site i (PredictorCorrector.lua:77-79 with LLGCartesian::apply):
+// H = exchange + anisotropy + Hd at S; P = N(S + dt dM), C = N(S + dt/2 dM).
+// Writes P, C (and C's low word when COMP); returns P for the next transform.
+template <bool COMP>
+__device__ __forceinline__ float3 site_pred(const StepArgs& a, const ExchangeDev& ex, int i, int x, int y, int z, float3 Hd)
+{
+    const int n = a.d.n;
+    const float* __restrict__ S = a.S;
+    const float* __restrict__ ms_a = a.S + 3 * (size_t)n;
+    const float ms = ms_a[i];
+    const float3 s = make_float3(S[i], S[n + i], S[2 * n + i]);
+    float3 H = exchange_field(ex, a.d, i, x, y, z, S, ms_a);
+    const float3 ha = aniso_field(a.an, i, n, s, ms);
+    H.x += ha.x + Hd.x; H.y += ha.y + Hd.y; H.z += ha.z + Hd.z;
+    const float dt = a.dt, hdt = 0.5f * a.dt;
+    float3 p = s, c = s;
+    float3 clo = make_float3(0.f, 0.f, 0.f);
+    if (COMP) clo = make_float3(a.Slo[i], a.Slo[n + i], a.Slo[2 * n + i]);
+    if (ms > 0.0f) {
+        const float al = a.lp.site_alpha ? a.lp.site_alpha[i] : a.lp.alpha;
+        const float ga = a.lp.site_gamma ? a.lp.site_gamma[i] : a.lp.gamma;
+        const float3 dM = a.lp.thermal_first ? llg_rhs(s, H, H, al, ga, ms, a.lp.ps, a.lp.ds)
+                                             : llg_rhs(s, H, H, al, ga, ms, 1.0f, 1.0f);
+        p = make_float3(s.x + dt * dM.x, s.y + dt * dM.y, s.z + dt * dM.z);
+        if (COMP) {
+            df_add(c.x, clo.x, hdt * dM.x);
+            df_add(c.y, clo.y, hdt * dM.y);
+            df_add(c.z, clo.z, hdt * dM.z);
+        } else {
+            c = make_float3(s.x + hdt * dM.x, s.y + hdt * dM.y, s.z + hdt * dM.z);
+        }
+    }
+    if (a.lp.renorm) {
+        p = renorm(p, ms);
+        if (COMP) df_renorm(c, clo, ms);
+        else c = renorm(c, ms);
+    }
+    a.P[i] = p.x; a.P[n + i] = p.y; a.P[2 * n + i] = p.z;
+    a.C[i] = c.x; a.C[n + i] = c.y; a.C[2 * n + i] = c.z;
+    if (COMP) { a.Clo[i] = clo.x; a.Clo[n + i] = clo.y; a.Clo[2 * n + i] = clo.z; }
+    return p;
+}
+
+// Corrector and thermal kick at site i (PredictorCorrector.lua:81-85,98-103):
+// S' = N(C + dt/2 dM(P, H(P))), then S'' = N(S' + dt LLG(S', H_th)) when Thermal
+// is on.  Writes S (and its low word when COMP); returns S''.
+template <bool COMP>
+__device__ __forceinline__ float3 site_corr(const StepArgs& a, const ExchangeDev& ex, int i, int x, int y, int z, float3 Hd,
+                                            uint64_t draw)
+{
+    const int n = a.d.n;
+    const float* __restrict__ ms_a = a.S + 3 * (size_t)n;
+    const float* __restrict__ P = a.P;
+    const float ms = ms_a[i];
+    const float3 p = make_float3(P[i], P[n + i], P[2 * n + i]);
+    float3 H = exchange_field(ex, a.d, i, x, y, z, P, ms_a);
+    const float3 ha = aniso_field(a.an, i, n, p, ms);
+    H.x += ha.x + Hd.x; H.y += ha.y + Hd.y; H.z += ha.z + Hd.z;
+    const float dt = a.dt, hdt = 0.5f * a.dt;
+    float3 v = make_float3(a.C[i], a.C[n + i], a.C[2 * n + i]);
+    float3 vlo = make_float3(0.f, 0.f, 0.f);
+    if (COMP) vlo = make_float3(a.Clo[i], a.Clo[n + i], a.Clo[2 * n + i]);
+    const float al = a.lp.site_alpha ? a.lp.site_alpha[i] : a.lp.alpha;
+    const float ga = a.lp.site_gamma ? a.lp.site_gamma[i] : a.lp.gamma;
+    if (ms > 0.0f) {
+        const float3 dM = a.lp.thermal_first ? llg_rhs(p, H, H, al, ga, ms, a.lp.ps, a.lp.ds)
+                                             : llg_rhs(p, H, H, al, ga, ms, 1.0f, 1.0f);
+        if (COMP) {
+            df_add(v.x, vlo.x, hdt * dM.x);
+            df_add(v.y, vlo.y, hdt * dM.y);
+            df_add(v.z, vlo.z, hdt * dM.z);
+        } else {
+            v = make_float3(v.x + hdt * dM.x, v.y + hdt * dM.y, v.z + hdt * dM.z);
+        }
+    }
+    if (a.lp.renorm) {
+        if (COMP) df_renorm(v, vlo, ms);
+        else v = renorm(v, ms);
+    }
+    if (a.th.on) {
+        const float sg = thermal_sigma(a.th, a.lp, i, ms, dt);
+        if (ms > 0.0f) {
+            float3 Ht = make_float3(0.f, 0.f, 0.f);
+            if (sg != 0.0f) {
+                const float3 g = thermal_normals(a.th.seed, draw, i);
+                Ht = make_float3(sg * g.x, sg * g.y, sg * g.z);
+            }
+            const float3 zero = make_float3(0.f, 0.f, 0.f);
+            const float3 dM = a.lp.thermal_first ? llg_rhs(v, Ht, zero, al, ga, ms, a.lp.ps, a.lp.ds)
+                                                 : llg_rhs(v, Ht, Ht, al, ga, ms, 1.0f, 1.0f);
+            if (COMP) {
+                df_add(v.x, vlo.x, dt * dM.x);
+                df_add(v.y, vlo.y, dt * dM.y);
+                df_add(v.z, vlo.z, dt * dM.z);
+            } else {
+                v = make_float3(v.x + dt * dM.x, v.y + dt * dM.y, v.z + dt * dM.z);
+            }
+        }
+        if (a.lp.renorm) {
+            if (COMP) df_renorm(v, vlo, ms);
+            else v = renorm(v, ms);
+        }
+    }
+    a.S[i] = v.x; a.S[n + i] = v.y; a.S[2 * n + i] = v.z;
+    if (COMP) { a.Slo[i] = vlo.x; a.Slo[n + i] = vlo.y; a.Slo[2 * n + i] = vlo.z; }
+    return v;
+}
+
 }  // namespace msd

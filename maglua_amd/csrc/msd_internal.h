@@ -140,6 +140,45 @@ struct StepArgs {
 };
 hipError_t launch_step_pred(const StepArgs& a, hipStream_t st);
 hipError_t launch_step_corr(const StepArgs& a, hipStream_t st);
+
+// ------------------------------------------------ 3D convolution (LongRange3D)
+// FFT grid (fx, fy, fz) equals the lattice when periodic and doubles it when
+// open (zero padding).  The working spectrum W is [3][nz][fy][fx/2+1]: planes
+// z >= nz of the transform are zero and never stored.
+struct Conv3Dev {
+    int fx, fy, fz, fxh;
+    int ty;                 // rows per row tile
+    int wy, wz;             // kx block widths of the y and z passes
+    FFTPlan px, py, pz;
+    const float2* twx;
+    const float2* twy;
+    const float2* twz;
+    float2* W;              // working spectrum
+    const float* Tw;        // tensor words [6][fz][fy][fxh] (real spectra), g/(fx fy fz) folded in
+    float* hout;            // op-API target slot [3][n] (row kernel mode INV)
+    const float* sin;       // op-API source spins (row kernel mode FWD)
+};
+// One replica: its step arguments and its convolution.
+struct Rep3 {
+    StepArgs a;
+    Conv3Dev cv;
+    int has_lr;
+};
+// row kernel modes: forward transform of cv.sin, inverse transform into cv.hout,
+// and the fused predictor / corrector of the Heun step (the predictor of the
+// item with z = y0 = 0 advances the replica's thermal draw counter)
+enum Row3Mode { R3_FWD = 0, R3_INV = 1, R3_PRED = 2, R3_CORR = 3 };
+// work items: int4 (replica, p, q, unused); rows: p = z, q = y0; y pass: p = c*nz+z,
+// q = kx0; z pass: p = ky, q = kx0
+hipError_t launch3_rows(int mode, bool comp, const Rep3* reps, const int4* items, int nitems, size_t lds, hipStream_t st);
+hipError_t launch3_ypass(int sign, const Rep3* reps, const int4* items, int nitems, size_t lds, hipStream_t st);
+hipError_t launch3_zconv(const Rep3* reps, const int4* items, int nitems, size_t lds, hipStream_t st);
+// tensor: periodic lattice sum (R > 0) or open kernel (R == 0) on the FFT grid, FP64
+hipError_t launch3_tensor(int nx, int ny, int nz, int fx, int fy, int fz, int R, double* T, hipStream_t st);
+// real spectrum words of T: U1/U2 complex FP64 scratch [6][fz][fy][fxh]; tw* FP64 tables
+hipError_t launch3_spectrum(int fx, int fy, int fz, const double* T, double2* U1, double2* U2, float* Tw, double mult,
+                            const double2* twx, const double2* twy, const double2* twz, hipStream_t st);
+hipError_t configure_lds3(size_t max_bytes);
 const char* kernel_name(int id);
 // allow row/column kernels the device's full LDS per workgroup
 hipError_t configure_lds(size_t max_bytes);

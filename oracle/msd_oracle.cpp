@@ -780,6 +780,163 @@ int ora_lr_apply(ora_lr* l, ora_sys* s)
 } /* extern "C" */
 
 /* ------------------------------------------------------------------------ */
+/* LongRange3D (3D FFT convolution) and 3D dipole tensors                    */
+/* ------------------------------------------------------------------------ */
+/* full 3D C2C transform of [nz][ny][nx] (Array::fft3DTo semantics: unnormalised) */
+static void fft3d(cplx* d, int nx, int ny, int nz, int sign)
+{
+    fft2d_layers(d, nx, ny, nz, sign);
+    const size_t nxy = (size_t)nx * ny;
+#pragma omp parallel
+    {
+        std::vector<cplx> line(nz), work(nz);
+#pragma omp for schedule(static)
+        for (long long p = 0; p < (long long)nxy; ++p) {
+            for (int z = 0; z < nz; ++z) line[z] = d[p + nxy * z];
+            fft_line(line.data(), nz, sign, work.data());
+            for (int z = 0; z < nz; ++z) d[p + nxy * z] = line[z];
+        }
+    }
+}
+
+extern "C" int ora_fft3d(double* d, int nx, int ny, int nz, int sign)
+{
+    if (nx < 1 || ny < 1 || nz < 1) return -1;
+    fft3d(reinterpret_cast<cplx*>(d), nx, ny, nz, sign);
+    return 0;
+}
+
+/* Fully periodic lattice sum: T(ix,iy,iz) = sum over images (X,Y,Z) of Gamma(r),
+ * r = (ix + X nx, iy + Y ny, iz + Z nz), |r| <= R, r != 0; images in ascending
+ * X, Y, Z.  The 3D analogue of getWAB with the radial cut only. */
+extern "C" int ora_dipole_tensor3d_periodic(int nx, int ny, int nz, int R,
+                                            double* XX, double* XY, double* XZ, double* YY, double* YZ, double* ZZ)
+{
+    if (nx < 1 || ny < 1 || nz < 1 || R < 1) return -1;
+    double* out[6] = {XX, XY, XZ, YY, YZ, ZZ};
+    const double r2max = (double)R * R;
+#pragma omp parallel for collapse(3) schedule(dynamic, 8)
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                double acc[6] = {0, 0, 0, 0, 0, 0};
+                const long long LR = R;
+                const long long x0 = cdiv(-LR - i, (long long)nx), x1 = fdiv(LR - i, (long long)nx);
+                const long long y0 = cdiv(-LR - j, (long long)ny), y1 = fdiv(LR - j, (long long)ny);
+                const long long z0 = cdiv(-LR - k, (long long)nz), z1 = fdiv(LR - k, (long long)nz);
+                for (long long X = x0; X <= x1; ++X)
+                    for (long long Y = y0; Y <= y1; ++Y)
+                        for (long long Z = z0; Z <= z1; ++Z) {
+                            const double rx = (double)(X * nx + i), ry = (double)(Y * ny + j), rz = (double)(Z * nz + k);
+                            if (rx * rx + ry * ry + rz * rz > r2max) continue;
+                            double g[6];
+                            gamma_dip(rx, ry, rz, g);
+                            for (int a = 0; a < 6; ++a) acc[a] += g[a];
+                        }
+                const size_t idx = (size_t)i + (size_t)nx * (j + (size_t)ny * k);
+                for (int a = 0; a < 6; ++a) out[a][idx] = acc[a];
+            }
+    return 0;
+}
+
+/* Open boundaries on the doubled grid (2nx, 2ny, 2nz): T(i) = Gamma(i') with
+ * i' = i for i < n and i - 2n for i > n, zero at i = n (no periodic images). */
+extern "C" int ora_dipole_tensor3d_open(int nx, int ny, int nz,
+                                        double* XX, double* XY, double* XZ, double* YY, double* YZ, double* ZZ)
+{
+    if (nx < 1 || ny < 1 || nz < 1) return -1;
+    double* out[6] = {XX, XY, XZ, YY, YZ, ZZ};
+    const int fx = 2 * nx, fy = 2 * ny, fz = 2 * nz;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < fz; ++k)
+        for (int j = 0; j < fy; ++j)
+            for (int i = 0; i < fx; ++i) {
+                const size_t idx = (size_t)i + (size_t)fx * (j + (size_t)fy * k);
+                double g[6] = {0, 0, 0, 0, 0, 0};
+                if (i != nx && j != ny && k != nz) {
+                    const int ii = i < nx ? i : i - fx, jj = j < ny ? j : j - fy, kk = k < nz ? k : k - fz;
+                    gamma_dip(ii, jj, kk, g);
+                }
+                for (int a = 0; a < 6; ++a) out[a][idx] = g[a];
+            }
+    return 0;
+}
+
+/* LongRange3D::compile/apply (cpu/longrange3d/spinoperationlongrange3d.cpp:331-432):
+ * nine real tensors XX XY XZ YX YY YZ ZX ZY ZZ on the FFT grid, spectra
+ * prescaled by 1/(fx fy fz); H_a = g scale Re IFFT(sum_b T^_ab M^_b).  With
+ * open = 1 the FFT grid is (2nx, 2ny, 2nz) and the spins are zero-padded. */
+struct ora_lr3 {
+    int nx, ny, nz, fx, fy, fz;
+    bool open;
+    double g = 1, scale = 1;
+    std::vector<double> T[9];
+    std::vector<cplx> Q[9];
+    bool fresh = true;
+};
+
+extern "C" {
+ora_lr3* ora_lr3_new(int nx, int ny, int nz, int open)
+{
+    ora_lr3* l = new ora_lr3;
+    l->nx = nx; l->ny = ny; l->nz = nz; l->open = open != 0;
+    l->fx = open ? 2 * nx : nx; l->fy = open ? 2 * ny : ny; l->fz = open ? 2 * nz : nz;
+    const size_t m = (size_t)l->fx * l->fy * l->fz;
+    for (int a = 0; a < 9; ++a) l->T[a].assign(m, 0.0);
+    return l;
+}
+void ora_lr3_free(ora_lr3* l) { delete l; }
+void ora_lr3_set_strength(ora_lr3* l, double g) { l->g = g; }
+int ora_lr3_set_tensor(ora_lr3* l, int which, const double* t)
+{
+    if (which < 0 || which > 8) return -1;
+    std::memcpy(l->T[which].data(), t, sizeof(double) * l->T[which].size());
+    l->fresh = true;
+    return 0;
+}
+
+int ora_lr3_apply(ora_lr3* l, ora_sys* s)
+{
+    const int k = s->mark("LongRange3D");
+    const size_t m = (size_t)l->fx * l->fy * l->fz;
+    const double inv = 1.0 / (double)m;
+    if (l->fresh) {
+        for (int a = 0; a < 9; ++a) {
+            l->Q[a].assign(m, 0.0);
+            for (size_t i = 0; i < m; ++i) l->Q[a][i] = cplx(l->T[a][i], 0.0);
+            fft3d(l->Q[a].data(), l->fx, l->fy, l->fz, -1);
+            for (size_t i = 0; i < m; ++i) l->Q[a][i] *= cplx(inv, 0);
+        }
+        l->fresh = false;
+    }
+    std::vector<cplx> q[3];
+    const std::vector<double>* S[3] = {&s->x, &s->y, &s->z};
+    for (int c = 0; c < 3; ++c) {
+        q[c].assign(m, 0.0);
+        for (int z = 0; z < l->nz; ++z)
+            for (int y = 0; y < l->ny; ++y)
+                for (int x = 0; x < l->nx; ++x)
+                    q[c][(size_t)x + (size_t)l->fx * (y + (size_t)l->fy * z)] =
+                        (*S[c])[(size_t)x + (size_t)l->nx * (y + (size_t)l->ny * z)];
+        fft3d(q[c].data(), l->fx, l->fy, l->fz, -1);
+    }
+    std::vector<double>* H[3] = {&s->hx[k], &s->hy[k], &s->hz[k]};
+    std::vector<cplx> ws(m);
+    for (int a = 0; a < 3; ++a) {
+        for (size_t i = 0; i < m; ++i)
+            ws[i] = l->Q[3 * a][i] * q[0][i] + l->Q[3 * a + 1][i] * q[1][i] + l->Q[3 * a + 2][i] * q[2][i];
+        fft3d(ws.data(), l->fx, l->fy, l->fz, +1);
+        for (int z = 0; z < l->nz; ++z)
+            for (int y = 0; y < l->ny; ++y)
+                for (int x = 0; x < l->nx; ++x)
+                    (*H[a])[(size_t)x + (size_t)l->nx * (y + (size_t)l->ny * z)] =
+                        ws[(size_t)x + (size_t)l->fx * (y + (size_t)l->fy * z)].real() * (l->g * l->scale);
+    }
+    return 0;
+}
+} /* extern "C" */
+
+/* ------------------------------------------------------------------------ */
 /* LLG.Cartesian                                                             */
 /* ------------------------------------------------------------------------ */
 struct ora_llg {
@@ -880,6 +1037,7 @@ int ora_llg_apply(ora_llg* l, const ora_sys* from, double scale, const ora_sys* 
 struct ora_sim {
     ora_sys* ss;
     ora_ex* ex; ora_ani* ani; ora_lr* lr; ora_th* th; ora_llg* llg;
+    ora_lr3* lr3 = nullptr;
     uint64_t seed = 0, step = 0;
     ora_sys* P = nullptr;
     ora_sys* C = nullptr;
@@ -902,6 +1060,7 @@ void ora_sim_free(ora_sim* s)
 }
 
 void ora_sim_set_seed(ora_sim* s, uint64_t seed) { s->seed = seed; s->step = 0; }
+void ora_sim_set_lr3(ora_sim* s, ora_lr3* l) { s->lr3 = l; }
 
 void ora_sim_set_threads(int n)
 {
@@ -919,6 +1078,7 @@ int ora_sim_calc_fields(ora_sim* s, ora_sys* t)
     if (s->ex) ora_ex_apply(s->ex, t);
     if (s->ani) ora_ani_apply(s->ani, t);
     if (s->lr) ora_lr_apply(s->lr, t);
+    if (s->lr3) ora_lr3_apply(s->lr3, t);
     ora_sys_sum_fields(t);
     return 0;
 }

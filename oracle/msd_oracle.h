@@ -97,6 +97,19 @@ int      ora_lr_set_tensor(ora_lr*, int which, const double* t);   /* [nx*ny*(2n
 int      ora_lr_get_tensor(ora_lr*, int which, double* t);
 int      ora_lr_apply(ora_lr*, ora_sys*);
 
+/* 3D FFT and LongRange3D (periodic, or open on the doubled grid) */
+int      ora_fft3d(double* data, int nx, int ny, int nz, int sign);
+int      ora_dipole_tensor3d_periodic(int nx, int ny, int nz, int R,
+                                      double* XX, double* XY, double* XZ, double* YY, double* YZ, double* ZZ);
+int      ora_dipole_tensor3d_open(int nx, int ny, int nz,
+                                  double* XX, double* XY, double* XZ, double* YY, double* YZ, double* ZZ);
+typedef struct ora_lr3 ora_lr3;
+ora_lr3* ora_lr3_new(int nx, int ny, int nz, int open);
+void     ora_lr3_free(ora_lr3*);
+void     ora_lr3_set_strength(ora_lr3*, double g);
+int      ora_lr3_set_tensor(ora_lr3*, int which, const double* t);   /* 0..8 = XX XY XZ YX YY YZ ZX ZY ZZ */
+int      ora_lr3_apply(ora_lr3*, ora_sys*);
+
 ora_llg* ora_llg_new(void);
 void     ora_llg_free(ora_llg*);
 void     ora_llg_set_flags(ora_llg*, int thermal_only_first_term, int disable_renorm);
@@ -107,6 +120,7 @@ typedef struct ora_sim ora_sim;
 ora_sim* ora_sim_new(ora_sys* ss, ora_ex* ex, ora_ani* ani, ora_lr* lr, ora_th* th, ora_llg* llg);
 void     ora_sim_free(ora_sim*);
 void     ora_sim_set_seed(ora_sim*, uint64_t seed);
+void     ora_sim_set_lr3(ora_sim*, ora_lr3* lr3);
 int      ora_sim_calc_fields(ora_sim*, ora_sys* target);
 int      ora_sim_heun(ora_sim*, int nsteps);                  /* one PC corrector pass per step */
 int      ora_sim_rk(ora_sim*, const char* name, int nsteps);  /* RK1, RK2, RK3, K3, RK4, RK4_38, RK6, BS3 */

@@ -89,6 +89,15 @@ def lib(threads=False):
         "ora_lr_set_tensor": (C.c_int, [vp, C.c_int, _dp]),
         "ora_lr_get_tensor": (C.c_int, [vp, C.c_int, _dp]),
         "ora_lr_apply": (C.c_int, [vp, vp]),
+        "ora_fft3d": (C.c_int, [_dp, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "ora_dipole_tensor3d_periodic": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "ora_dipole_tensor3d_open": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "ora_lr3_new": (vp, [C.c_int, C.c_int, C.c_int, C.c_int]),
+        "ora_lr3_free": (None, [vp]),
+        "ora_lr3_set_strength": (None, [vp, C.c_double]),
+        "ora_lr3_set_tensor": (C.c_int, [vp, C.c_int, _dp]),
+        "ora_lr3_apply": (C.c_int, [vp, vp]),
+        "ora_sim_set_lr3": (None, [vp, vp]),
         "ora_llg_new": (vp, []),
         "ora_llg_free": (None, [vp]),
         "ora_llg_set_flags": (None, [vp, C.c_int, C.c_int]),
@@ -378,6 +387,58 @@ class Dipole:
         self.L.ora_lr_apply(self.h, ss.h)
 
 
+def fft3d(a, sign=-1):
+    """a: complex (nz, ny, nx); full 3D transform, unnormalised."""
+    nz, ny, nx = a.shape
+    d = np.ascontiguousarray(a, dtype=np.complex128).copy()
+    lib().ora_fft3d(d.view(np.float64).reshape(-1), nx, ny, nz, sign)
+    return d
+
+
+def dipole_tensor3d(nx, ny, nz, R=None, open=False, threads=True):
+    """Six arrays (XX XY XZ YY YZ ZZ): periodic lattice sums to radius R shaped (nz, ny, nx),
+    or the open-boundary kernel on the doubled grid shaped (2nz, 2ny, 2nx)."""
+    L = lib(threads)
+    if open:
+        m = 8 * nx * ny * nz
+        outs = [np.zeros(m) for _ in range(6)]
+        assert L.ora_dipole_tensor3d_open(nx, ny, nz, *outs) == 0
+        shp = (2 * nz, 2 * ny, 2 * nx)
+    else:
+        m = nx * ny * nz
+        outs = [np.zeros(m) for _ in range(6)]
+        assert L.ora_dipole_tensor3d_periodic(nx, ny, nz, int(R), *outs) == 0
+        shp = (nz, ny, nx)
+    return {k: o.reshape(shp) for k, o in zip(("XX", "XY", "XZ", "YY", "YZ", "ZZ"), outs)}
+
+
+class LongRange3D:
+    """Nine-tensor 3D FFT convolution; open=True zero-pads to the doubled grid."""
+    NAMES = ("XX", "XY", "XZ", "YX", "YY", "YZ", "ZX", "ZY", "ZZ")
+
+    def __init__(self, nx, ny=1, nz=1, open=False, threads=False):
+        self.L = lib(threads)
+        self.h = self.L.ora_lr3_new(nx, ny, nz, int(open))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.ora_lr3_free(self.h)
+
+    def setStrength(self, g):
+        self.L.ora_lr3_set_strength(self.h, g)
+
+    def setTensor(self, name, t):
+        self.L.ora_lr3_set_tensor(self.h, self.NAMES.index(name), np.ascontiguousarray(t, dtype=np.float64).reshape(-1))
+
+    def setDipole(self, T6):
+        for a, b in (("XX", "XX"), ("XY", "XY"), ("XZ", "XZ"), ("YX", "XY"), ("YY", "YY"), ("YZ", "YZ"),
+                     ("ZX", "XZ"), ("ZY", "YZ"), ("ZZ", "ZZ")):
+            self.setTensor(a, T6[b])
+
+    def apply(self, ss):
+        self.L.ora_lr3_apply(self.h, ss.h)
+
+
 class LLG:
     def __init__(self, threads=False):
         self.L = lib(threads)
@@ -410,6 +471,10 @@ class Sim:
     def __del__(self):
         if getattr(self, "h", None):
             self.L.ora_sim_free(self.h)
+
+    def setLongRange3D(self, lr3):
+        self.keep3 = lr3
+        self.L.ora_sim_set_lr3(self.h, lr3.h if lr3 is not None else None)
 
     def calcFields(self, ss):
         self.L.ora_sim_calc_fields(self.h, ss.h)
