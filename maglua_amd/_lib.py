@@ -104,6 +104,20 @@ _SIGS = {
     "msd_dipole_get_matrix": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
     "msd_dipole_set_matrix": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]),
     "msd_dipole_apply": (C.c_int, [_vp, _vp]),
+    "msd_longrange3d_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _pp]),
+    "msd_longrange3d_destroy": (C.c_int, [_vp]),
+    "msd_longrange3d_set_strength": (C.c_int, [_vp, C.c_double]),
+    "msd_longrange3d_set_scale": (C.c_int, [_vp, C.c_double]),
+    "msd_longrange3d_set_radius": (C.c_int, [_vp, C.c_int]),
+    "msd_longrange3d_get_matrix": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
+    "msd_longrange3d_set_matrix": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]),
+    "msd_longrange3d_apply": (C.c_int, [_vp, _vp]),
+    "msd_stepper_set_longrange3d": (C.c_int, [_vp, _vp]),
+    "msd_ensemble_create": (C.c_int, [_pp, C.c_int, _pp]),
+    "msd_ensemble_destroy": (C.c_int, [_vp]),
+    "msd_ensemble_set_graph": (C.c_int, [_vp, C.c_int]),
+    "msd_ensemble_set_compensated": (C.c_int, [_vp, C.c_int]),
+    "msd_ensemble_heun": (C.c_int, [_vp, C.c_int]),
     "msd_llg_create": (C.c_int, [_pp]),
     "msd_llg_destroy": (C.c_int, [_vp]),
     "msd_llg_set_thermal_only_first_term": (C.c_int, [_vp, C.c_int]),

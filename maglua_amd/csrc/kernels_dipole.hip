@@ -118,7 +118,10 @@ __device__ __forceinline__ const float2* row_twiddles(const DipoleDev& dp, float
 template <int NX, int TY, int SIGN>
 __device__ __forceinline__ float2* row_fft(float2* A, float2* B, const DipoleDev& dp, const float2* tw)
 {
-    if constexpr (NX > 0) return fft_ct<NX, row_lines(TY), ROW_TPB, SIGN>(A, B, tw);
+    if constexpr (NX > 0) {
+        fft_ip<NX, row_lines(TY), ROW_TPB, SIGN>(A, tw);   // in place: B is not used
+        return A;
+    }
     else return lds_fft<SIGN>(A, B, row_lines(dp.ty), dp.px, tw);
 }
 
@@ -135,7 +138,7 @@ __global__ void __launch_bounds__(ROW_TPB) k_rows_fwd(DipoleDev dp, const float*
     const int n = nx * ny * dp.nz;
     float2* A = lds;
     float2* B = lds + (size_t)lines * nx;
-    const float2* tw = row_twiddles<NX>(dp, B + (size_t)lines * nx);
+    const float2* tw = row_twiddles<NX>(dp, A + (size_t)lines * nx);
     for (int li = threadIdx.x; li < lines * nx; li += ROW_TPB) A[li] = make_float2(0.f, 0.f);
     __syncthreads();
     for (int li = threadIdx.x; li < 3 * ty * nx; li += ROW_TPB) {
@@ -160,7 +163,7 @@ __global__ void __launch_bounds__(ROW_TPB) k_rows_inv(DipoleDev dp, float* __res
     const int n = nx * ny * dp.nz;
     float2* A = lds;
     float2* B = lds + (size_t)lines * nx;
-    const float2* tw = row_twiddles<NX>(dp, B + (size_t)lines * nx);
+    const float2* tw = row_twiddles<NX>(dp, A + (size_t)lines * nx);
     rows_pack_spectrum<ROW_TPB>(dp.Hh, nx, ny, dp.nz, ty, z, y0, A);
     __syncthreads();
     float2* R = row_fft<NX, TY, 1>(A, B, dp, tw);
@@ -188,7 +191,7 @@ __global__ void __launch_bounds__(COL_TPB) k_cols(DipoleDev dp)
     float2* B = lds + (size_t)lines * ny;
     const float2* tw = dp.twy;
     if constexpr (NY > 0) {
-        float2* twl = B + (size_t)lines * ny;
+        float2* twl = A + (size_t)lines * ny;   // in-place transforms: no second buffer
         for (int i = threadIdx.x; i < NY; i += COL_TPB) twl[i] = dp.twy[i];
         tw = twl;
     }
@@ -198,7 +201,10 @@ __global__ void __launch_bounds__(COL_TPB) k_cols(DipoleDev dp)
     }
     __syncthreads();
     float2* R;
-    if constexpr (NY > 0) R = fft_ct<NY, lines, COL_TPB, -1>(A, B, tw);
+    if constexpr (NY > 0) {
+        fft_ip<NY, lines, COL_TPB, -1>(A, tw);
+        R = A;
+    }
     else R = lds_fft<-1>(A, B, lines, dp.py, tw);
     // row oa of the symmetric tensor: components (oa,0) (oa,1) (oa,2) and which are odd in z
     const int cidx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
@@ -250,7 +256,10 @@ __global__ void __launch_bounds__(COL_TPB) k_cols(DipoleDev dp)
     float2* Rs = R + (size_t)oa * NZ * ny;
     float2* Os = (R == A ? B : A) + (size_t)oa * NZ * ny;
     float2* O;
-    if constexpr (NY > 0) O = fft_ct<NY, NZ, COL_TPB, 1>(Rs, Os, tw);
+    if constexpr (NY > 0) {
+        fft_ip<NY, NZ, COL_TPB, 1>(Rs, tw);
+        O = Rs;
+    }
     else O = lds_fft<1>(Rs, Os, NZ, dp.py, tw);
     for (int idx = threadIdx.x; idx < NZ * ny; idx += COL_TPB) {
         const int l = idx / ny, y = idx - l * ny;
@@ -283,7 +292,7 @@ __global__ void __launch_bounds__(ROW_TPB) k_step_pred(StepArgs a)
     const ExchangeDev exl = stage_exchange(a.ex, s_start, s_ent);
     if constexpr (!DIP) __syncthreads();
     if constexpr (DIP) {
-        tw = row_twiddles<NX>(dp, B + (size_t)lines * nx);
+        tw = row_twiddles<NX>(dp, A + (size_t)lines * nx);
         rows_pack_spectrum<ROW_TPB>(dp.Hh, nx, ny, a.d.nz, ty, z, y0, A);
         __syncthreads();
         R = row_fft<NX, TY, 1>(A, B, dp, tw);
@@ -331,7 +340,7 @@ __global__ void __launch_bounds__(ROW_TPB) k_step_corr(StepArgs a)
     const ExchangeDev exl = stage_exchange(a.ex, s_start, s_ent);
     if constexpr (!DIP) __syncthreads();
     if constexpr (DIP) {
-        tw = row_twiddles<NX>(dp, B + (size_t)lines * nx);
+        tw = row_twiddles<NX>(dp, A + (size_t)lines * nx);
         rows_pack_spectrum<ROW_TPB>(dp.Hh, nx, ny, a.d.nz, ty, z, y0, A);
         __syncthreads();
         R = row_fft<NX, TY, 1>(A, B, dp, tw);

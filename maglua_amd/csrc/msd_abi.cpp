@@ -973,9 +973,15 @@ extern "C" int msd_dipole_create(int nx, int ny, int nz, msd_dipole** out)
         if (ny % c == 0 && need <= 65536) { ty = c; break; }
     }
     dv.ty = ty;
-    // two ping-pong line buffers plus the LDS twiddle table
-    dv.lds_rows = (2 * (size_t)((3 * ty + 1) / 2) + 1) * nx * sizeof(float2);
-    dv.lds_cols = (2 * (size_t)3 * nz + 1) * ny * sizeof(float2);
+    // Compile-time shapes (the MSD_ROW_SHAPES / MSD_COL_LENGTHS instances of
+    // kernels_dipole.hip) transform in place: one line buffer plus the twiddle
+    // table.  Other sizes use two ping-pong buffers.
+    const bool pow2_len_x = nx >= 32 && nx <= 1024 && (nx & (nx - 1)) == 0;
+    const bool pow2_len_y = ny >= 32 && ny <= 1024 && (ny & (ny - 1)) == 0;
+    const bool row_ct = pow2_len_x && ty == (nx <= 256 ? 8 : (nx == 512 ? 4 : 2));
+    const size_t rbuf = (size_t)((3 * ty + 1) / 2) * nx, cbuf = (size_t)3 * nz * ny;
+    dv.lds_rows = (row_ct ? rbuf + nx : 2 * rbuf + nx) * sizeof(float2);
+    dv.lds_cols = (pow2_len_y ? cbuf + ny : 2 * cbuf + ny) * sizeof(float2);
     if (dv.lds_rows > G.max_lds || dv.lds_cols > G.max_lds)
         return fail(MSD_EUNSUPPORTED, "Dipole.new: %dx%dx%d needs %zu/%zu bytes of LDS per workgroup (device limit %zu)", nx, ny,
                     nz, dv.lds_rows, dv.lds_cols, G.max_lds);
@@ -1814,8 +1820,11 @@ extern "C" int msd_ensemble_create(msd_stepper* const* members, int count, msd_e
         msd_stepper* s = members[i];
         if (!s) return fail(MSD_EINVAL, "ensemble: member %d is NULL", i);
         if (s->kind() == SK_LAYERED) return fail(MSD_EUNSUPPORTED, "ensemble: member %d uses the layered Dipole", i);
-        for (int j = 0; j < i; ++j)
+        for (int j = 0; j < i; ++j) {
             if (members[j] == s || members[j]->s == s->s) return fail(MSD_EINVAL, "ensemble: member %d repeats a SpinSystem", i);
+            if (s->lr3 && members[j]->lr3 == s->lr3)
+                return fail(MSD_EINVAL, "ensemble: members %d and %d share a LongRange3D (its spectrum buffer)", j, i);
+        }
         e->m.push_back(s);
     }
     if (int rc = e->refresh()) return rc;

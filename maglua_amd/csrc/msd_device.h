@@ -230,6 +230,76 @@ __device__ __forceinline__ float2* fft_ct(float2* a, float2* b, const float2* tw
     return FFTct<L, COUNT, NT, SIGN, 1>::run(a, b, twl);
 }
 
+// In-place form: every thread holds its butterflies' inputs in registers across a
+// barrier, so one LDS buffer serves all stages (half the LDS of the ping-pong form).
+template <int L, int COUNT, int NT, int SIGN, int NS, int R>
+__device__ __forceinline__ void stage_ip(float2* __restrict__ buf, const float2* __restrict__ tw)
+{
+    constexpr int nb = L / R;
+    constexpr int total = COUNT * nb;
+    constexpr int iters = (total + NT - 1) / NT;
+    constexpr int tstep = L / (NS * R);
+    float2 v[iters][R];
+#pragma unroll
+    for (int it = 0; it < iters; ++it) {
+        const int b = (int)threadIdx.x + it * NT;
+        if (total % NT == 0 || b < total) {
+            const int seq = b / nb;
+            const int j = b - seq * nb;
+            const float2* s = buf + seq * L + j;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[it][r] = s[r * nb];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < iters; ++it) {
+        const int b = (int)threadIdx.x + it * NT;
+        if (total % NT == 0 || b < total) {
+            const int seq = b / nb;
+            const int j = b - seq * nb;
+            const int k = j % NS;
+            if (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    const float2 w = tw[k * r * tstep];
+                    v[it][r] = SIGN < 0 ? cmul(v[it][r], w) : cmulc(v[it][r], w);
+                }
+            }
+            if (R == 2) dft2<SIGN>(v[it]);
+            else if (R == 3) dft3<SIGN>(v[it]);
+            else if (R == 4) dft4<SIGN>(v[it]);
+            else if (R == 5) dft5<SIGN>(v[it]);
+            else dft8<SIGN>(v[it]);
+            float2* d = buf + seq * L + (j / NS) * NS * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) d[q * NS] = v[it][q];
+        }
+    }
+    __syncthreads();
+}
+
+template <int L, int COUNT, int NT, int SIGN, int NS>
+struct FFTip {
+    static __device__ __forceinline__ void run(float2* a, const float2* tw)
+    {
+        constexpr int R = next_radix(L / NS);
+        stage_ip<L, COUNT, NT, SIGN, NS, R>(a, tw);
+        FFTip<L, COUNT, NT, SIGN, NS * R>::run(a, tw);
+    }
+};
+template <int L, int COUNT, int NT, int SIGN>
+struct FFTip<L, COUNT, NT, SIGN, L> {
+    static __device__ __forceinline__ void run(float2*, const float2*) {}
+};
+
+// COUNT lines of length L transformed in place in LDS; caller synchronised before
+template <int L, int COUNT, int NT, int SIGN>
+__device__ __forceinline__ void fft_ip(float2* a, const float2* twl)
+{
+    FFTip<L, COUNT, NT, SIGN, 1>::run(a, twl);
+}
+
 // ---------------------------------------------------------------- LLG
 __device__ __forceinline__ float3 cross3(float3 a, float3 b)
 {

@@ -470,6 +470,57 @@ class Dipole(_Handle):
 LongRange = Dipole
 
 
+class LongRange3D(_Handle):
+    """LongRange3D (cpu/longrange3d/spinoperationlongrange3d.cpp): 3D FFT convolution, slot
+    'LongRange3D'.  open=False: periodic, dipole lattice sum to radius R (setRadius, default
+    1000); open=True: open boundaries, zero padding to the doubled grid."""
+
+    _destroy = "msd_longrange3d_destroy"
+    NAMES = {"XX": 0, "XY": 1, "XZ": 2, "YX": 1, "YY": 3, "YZ": 4, "ZX": 2, "ZY": 4, "ZZ": 5}
+
+    def __init__(self, *a, open=False):
+        self._dims = _dims_of(a)
+        p = _vp()
+        call("msd_longrange3d_create", *self._dims, int(bool(open)), C.byref(p))
+        super().__init__(p.value)
+        self._g = 1.0
+
+    new = classmethod(lambda cls, *a, **k: cls(*a, **k))
+
+    def setStrength(self, g):
+        self._g = float(g)
+        call("msd_longrange3d_set_strength", self._h, self._g)
+
+    def strength(self):
+        return self._g
+
+    def setScale(self, s):
+        call("msd_longrange3d_set_scale", self._h, float(s))
+
+    def setRadius(self, R):
+        call("msd_longrange3d_set_radius", self._h, int(R))
+
+    def _which(self, name):
+        try:
+            return self.NAMES[name.upper()]
+        except KeyError:
+            raise MagLuaError(-1, "1st argument must be a tensor name: XX, XY, ... ZZ")
+
+    def getMatrix(self, name, offset):
+        o = tuple(int(v) for v in offset) + (0, 0, 0)
+        v = C.c_double()
+        call("msd_longrange3d_get_matrix", self._h, self._which(name), o[0], o[1], o[2], C.byref(v))
+        return v.value
+
+    def setMatrix(self, name, offset, value):
+        o = tuple(int(v) for v in offset) + (0, 0, 0)
+        call("msd_longrange3d_set_matrix", self._h, self._which(name), o[0], o[1], o[2], float(value))
+
+    def apply(self, ss):
+        for s in (ss if isinstance(ss, (list, tuple)) else (ss,)):
+            call("msd_longrange3d_apply", self._h, s.h)
+
+
 class _LLGBase(_Handle):
     """LLG.Cartesian (cpu/llg/llgcartesian.cpp:90-278) with LLG::apply argument rules (llg.cpp:100-171)."""
 
@@ -642,13 +693,18 @@ class Heun(_Handle):
     _destroy = "msd_stepper_destroy"
 
     def __init__(self, ss, ex=None, ani=None, dip=None, temp=None, llg=None, graph=True, compensated=True):
-        self._keep = (ss, ex, ani, dip, temp, llg)
+        lr3 = dip if isinstance(dip, LongRange3D) else None
+        if lr3 is not None:
+            dip = None
+        self._keep = (ss, ex, ani, dip, temp, llg, lr3)
         g = lambda o: o.h if o is not None else None
         p = _vp()
         call("msd_stepper_create", ss.h, g(ex), g(ani), g(dip), g(temp), g(llg), C.byref(p))
         super().__init__(p.value)
         call("msd_stepper_set_graph", self._h, int(bool(graph)))
         call("msd_stepper_set_compensated", self._h, int(bool(compensated)))
+        if lr3 is not None:
+            call("msd_stepper_set_longrange3d", self._h, lr3.h)
 
     def step(self, n=1):
         call("msd_stepper_heun", self._h, int(n))
@@ -664,6 +720,28 @@ class Heun(_Handle):
         cnt = C.c_int()
         call("msd_stepper_profile", self._h, int(n), cap, names, avg, C.byref(cnt))
         return {bytes(names[i]).split(b"\0")[0].decode(): avg[i] for i in range(cnt.value)}
+
+
+class Ensemble(_Handle):
+    """Independent replicas (uneven sizes, own temperatures and seeds) advanced by one set
+    of launches per Heun pass.  members: Heun step drivers of the local or LongRange3D form."""
+
+    _destroy = "msd_ensemble_destroy"
+
+    def __init__(self, members, graph=True, compensated=True):
+        self._members = list(members)
+        arr = (C.c_void_p * len(self._members))(*[m.h for m in self._members])
+        p = _vp()
+        call("msd_ensemble_create", arr, len(self._members), C.byref(p))
+        super().__init__(p.value)
+        call("msd_ensemble_set_graph", self._h, int(bool(graph)))
+        call("msd_ensemble_set_compensated", self._h, int(bool(compensated)))
+
+    def step(self, n=1):
+        call("msd_ensemble_heun", self._h, int(n))
+        return True
+
+    __call__ = step
 
 
 def synchronize():

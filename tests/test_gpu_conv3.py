@@ -1,0 +1,157 @@
+"""GPU parity of the 3D FFT convolution (LongRange3D, periodic and open) and of
+batched replica ensembles, against the FP64 oracle on the same inputs."""
+import numpy as np
+import pytest
+
+import maglua_amd as M
+from maglua_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def flat(t):
+    return np.stack([np.asarray(v).reshape(-1) for v in t])
+
+
+def oracle_lr3(dims, open_, R, g):
+    nx, ny, nz = dims
+    lr = O.LongRange3D(nx, ny, nz, open=open_, threads=True)
+    lr.setDipole(O.dipole_tensor3d(nx, ny, nz, R=R, open=open_))
+    lr.setStrength(g)
+    return lr
+
+
+@pytest.mark.parametrize("dims,R", [((8, 8, 8), 12), ((12, 10, 6), 9), ((16, 8, 4), 10), ((6, 6, 20), 8)])
+def test_longrange3d_periodic_op(dims, R):
+    n = dims[0] * dims[1] * dims[2]
+    s = np.random.default_rng(sum(dims)).standard_normal((3, n))
+    gs, os_ = M.SpinSystem(*dims), O.SpinSystem(*dims, threads=True)
+    gs.setSpins(*s)
+    os_.setSpins(*s)
+    gl = M.LongRange3D(*dims)
+    gl.setRadius(R)
+    gl.setStrength(0.4)
+    gl.apply(gs)
+    oracle_lr3(dims, False, R, 0.4).apply(os_)
+    hg, ho = flat(gs.field("LongRange3D")), flat(os_.field("LongRange3D"))
+    assert np.abs(hg - ho).max() <= 2e-5 * np.abs(ho).max()
+
+
+@pytest.mark.parametrize("dims", [(8, 6, 4), (10, 12, 6), (16, 16, 16)])
+def test_longrange3d_open_op(dims):
+    n = dims[0] * dims[1] * dims[2]
+    s = np.random.default_rng(n).standard_normal((3, n))
+    gs, os_ = M.SpinSystem(*dims), O.SpinSystem(*dims, threads=True)
+    gs.setSpins(*s)
+    os_.setSpins(*s)
+    gl = M.LongRange3D(*dims, open=True)
+    gl.setStrength(0.25)
+    gl.apply(gs)
+    oracle_lr3(dims, True, None, 0.25).apply(os_)
+    hg, ho = flat(gs.field("LongRange3D")), flat(os_.field("LongRange3D"))
+    assert np.abs(hg - ho).max() <= 2e-5 * np.abs(ho).max()
+
+
+def test_longrange3d_matrix_and_custom_tensor():
+    dims = (6, 6, 6)
+    gl = M.LongRange3D(*dims)
+    gl.setRadius(9)
+    T = O.dipole_tensor3d(*dims, R=9)
+    for name, off, key in (("XX", (1, 0, 0), "XX"), ("YZ", (0, 2, 5), "YZ"), ("ZY", (0, 2, 5), "YZ")):
+        want = T[key][off[2], off[1], off[0]]
+        assert abs(gl.getMatrix(name, off) - want) <= 1e-12 * max(1.0, abs(want))
+    # an odd custom tensor cannot be represented by the real spectrum: rejected
+    gl.setMatrix("XX", (1, 0, 0), 5.0)
+    ss = M.SpinSystem(*dims)
+    ss.randomize(1)
+    with pytest.raises(M.MagLuaError):
+        gl.apply(ss)
+    gl.setMatrix("XX", (-1, 0, 0), 5.0)     # even again
+    gl.apply(ss)
+
+
+def cube(dims, T, seed, R=10, J=1.0, K=0.1, g=0.05, dt=0.005, alpha=0.1):
+    spec = W.FilmSpec(*dims, J=J, K=K, g=0.0, pbc=(1, 1, 1), alpha=alpha, dt=dt, T=T, seed=seed)
+    gpu = W.build(M, spec, with_thermal=T != 0)
+    ora = W.build(O, spec, with_thermal=T != 0)
+    if gpu["temp"] is not None:
+        gpu["temp"].setSeed(seed)
+    x, y, z = O.random_unit_spins(seed, spec.n)
+    gpu["ss"].setSpins(x, y, z)
+    ora["ss"].setSpins(x, y, z)
+    gl = None
+    if g:
+        gl = M.LongRange3D(*dims)
+        gl.setRadius(R)
+        gl.setStrength(g)
+    ol = oracle_lr3(dims, False, R, g) if g else None
+    return spec, gpu, ora, gl, ol
+
+
+@pytest.mark.parametrize("dims,T", [((16, 16, 16), 0.0), ((12, 10, 6), 0.0), ((8, 8, 8), 0.4)])
+def test_heun_with_longrange3d(dims, T):
+    spec, g, o, gl, ol = cube(dims, T, seed=3)
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], gl, g["temp"], g["llg"])
+    sim = O.Sim(o["ss"], o["ex"], o["ani"], None, o["temp"], o["llg"], seed=3)
+    sim.setLongRange3D(ol)
+    heun.step(40)
+    sim.heun(40)
+    err = float(np.abs(flat(g["ss"].spins()[:3]) - flat(o["ss"].spins()[:3])).max())
+    print(f"lr3 heun {dims} T={T}: max|dm| = {err:.3e}", flush=True)
+    assert err <= (1e-5 if T == 0 else 1e-4)
+
+
+def test_heun_open_longrange3d():
+    dims = (12, 12, 4)
+    spec = W.FilmSpec(*dims, J=1.0, K=0.05, g=0.0, pbc=(0, 0, 0), alpha=0.1, dt=0.005)
+    g = W.build(M, spec)
+    o = W.build(O, spec)
+    x, y, z = O.random_unit_spins(9, spec.n)
+    g["ss"].setSpins(x, y, z)
+    o["ss"].setSpins(x, y, z)
+    gl = M.LongRange3D(*dims, open=True)
+    gl.setStrength(0.05)
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], gl, None, g["llg"])
+    sim = O.Sim(o["ss"], o["ex"], o["ani"], None, None, o["llg"])
+    sim.setLongRange3D(oracle_lr3(dims, True, None, 0.05))
+    heun.step(40)
+    sim.heun(40)
+    assert float(np.abs(flat(g["ss"].spins()[:3]) - flat(o["ss"].spins()[:3])).max()) <= 1e-5
+
+
+def test_ensemble_matches_single_replicas():
+    """Uneven replicas stepped by shared launches give exactly the single-replica results."""
+    cfg = [((16, 16, 16), 0.3, 1), ((12, 10, 6), 0.8, 2), ((8, 8, 8), 1.5, 3), ((10, 6, 4), 0.0, 4)]
+    ens, solo = [], []
+    for dims, T, seed in cfg:
+        for bucket in (ens, solo):
+            spec, g, o, gl, ol = cube(dims, T, seed)
+            bucket.append((g, gl, M.Heun(g["ss"], g["ex"], g["ani"], gl if dims != (10, 6, 4) else None,
+                                         g["temp"], g["llg"])))
+    E = M.Ensemble([h for (_, _, h) in ens])
+    E.step(15)
+    for (_, _, h) in solo:
+        h.step(15)
+    for (ge, _, _), (gs, _, _) in zip(ens, solo):
+        a, b = flat(ge["ss"].spins()), flat(gs["ss"].spins())
+        assert np.array_equal(a, b)
+        assert ge["ss"].time() == gs["ss"].time()
+
+
+def test_ensemble_against_oracle():
+    cfg = [((8, 8, 8), 0.5, 11), ((12, 10, 6), 0.0, 12)]
+    members, refs = [], []
+    for dims, T, seed in cfg:
+        spec, g, o, gl, ol = cube(dims, T, seed)
+        h = M.Heun(g["ss"], g["ex"], g["ani"], gl, g["temp"], g["llg"])
+        sim = O.Sim(o["ss"], o["ex"], o["ani"], None, o["temp"], o["llg"], seed=seed)
+        sim.setLongRange3D(ol)
+        members.append((g, h))
+        refs.append((o, sim, ol))
+    E = M.Ensemble([h for (_, h) in members])
+    E.step(25)
+    for (g, _), (o, sim, _) in zip(members, refs):
+        sim.heun(25)
+        err = float(np.abs(flat(g["ss"].spins()[:3]) - flat(o["ss"].spins()[:3])).max())
+        assert err <= 1e-4

@@ -359,3 +359,66 @@ def test_net_moment_sums():
     m = ss.netMoment()
     assert np.allclose(m[:3], [10, 2, 4]) and np.isclose(m[3], np.sqrt(100 + 4 + 16))
     assert np.allclose(m[4:7], [30, 2, 4])
+
+
+# ------------------------------------------------------------- LongRange3D
+def test_fft3d_vs_numpy():
+    a = np.random.default_rng(5).standard_normal((6, 5, 4)) + 0j
+    assert np.allclose(O.fft3d(a, -1), np.fft.fftn(a), atol=1e-12)
+
+
+def _direct_field(s, pairs_fn):
+    """h_t = sum_s Gamma(r_t - r_s) m_s, Gamma from pairs_fn(displacement) -> 3x3."""
+    nz, ny, nx = s.shape[1:]
+    H = np.zeros_like(s)
+    pts = [(x, y, z) for z in range(nz) for y in range(ny) for x in range(nx)]
+    for (xt, yt, zt) in pts:
+        acc = np.zeros(3)
+        for (xs, ys, zs) in pts:
+            G = pairs_fn(xt - xs, yt - ys, zt - zs)
+            acc += G @ s[:, zs, ys, xs]
+        H[:, zt, yt, xt] = acc
+    return H
+
+
+def _gamma3(r):
+    g = gamma_np(np.asarray(r, dtype=float)[None])[0]
+    return np.array([[g[0], g[1], g[2]], [g[1], g[3], g[4]], [g[2], g[4], g[5]]])
+
+
+def test_longrange3d_open_matches_direct_sum():
+    nx, ny, nz = 4, 3, 3
+    s = np.random.default_rng(6).standard_normal((3, nz, ny, nx))
+    ss = O.SpinSystem(nx, ny, nz)
+    ss.setSpins(*s)
+    lr = O.LongRange3D(nx, ny, nz, open=True)
+    lr.setDipole(O.dipole_tensor3d(nx, ny, nz, open=True))
+    lr.setStrength(0.3)
+    lr.apply(ss)
+    h = np.stack(ss.field("LongRange3D")).reshape(3, nz, ny, nx)
+    ref = 0.3 * _direct_field(s, lambda dx, dy, dz: _gamma3((dx, dy, dz)))
+    assert np.allclose(h, ref, atol=1e-12)
+
+
+def test_longrange3d_periodic_matches_image_sum():
+    nx, ny, nz, R = 3, 4, 2, 5
+    s = np.random.default_rng(7).standard_normal((3, nz, ny, nx))
+    ss = O.SpinSystem(nx, ny, nz)
+    ss.setSpins(*s)
+    lr = O.LongRange3D(nx, ny, nz)
+    lr.setDipole(O.dipole_tensor3d(nx, ny, nz, R=R))
+    lr.apply(ss)
+    h = np.stack(ss.field("LongRange3D")).reshape(3, nz, ny, nx)
+
+    def images(dx, dy, dz):
+        G = np.zeros((3, 3))
+        for X in range(-R - 1, R + 2):
+            for Y in range(-R - 1, R + 2):
+                for Z in range(-R - 1, R + 2):
+                    r = (dx + X * nx, dy + Y * ny, dz + Z * nz)
+                    if 0 < r[0] ** 2 + r[1] ** 2 + r[2] ** 2 <= R * R:
+                        G += _gamma3(r)
+        return G
+
+    ref = _direct_field(s, images)
+    assert np.allclose(h, ref, atol=1e-12)
