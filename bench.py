@@ -20,31 +20,34 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0   # MI355X published HBM3E bandwidth
 
 
-def kernel_bytes(spec, has_cls):
+def kernel_bytes(spec, has_cls, comp):
     """Algorithmic HBM bytes per launch: every compulsory FP32 array read or written
-    once by the kernel that consumes or produces it; neighbour re-reads are free."""
+    once by the kernel that consumes or produces it; neighbour re-reads are free.
+    comp: the spin state and the corrector base carry FP32 low words (12 B per site)."""
     n = spec.n
     nxh = spec.nx // 2 + 1
     spec_bytes = 3 * spec.nz * nxh * spec.ny * 8          # one complex spectrum, 3 components
     tensor = 6 * spec.nz * nxh * spec.ny * 4               # one word per component, layer offsets 0..nz-1
-    cls = n if has_cls else 0
+    cls = n if has_cls else 0                              # one exchange-class byte per site
+    lo = n * 24 if comp else 0                             # one low-word array read, one written
     return {
         # column pass: read M^ and the tensor, write H^
         "k_cols": 2 * spec_bytes + tensor,
-        # predictor: read S (x,y,z,ms) and H^(S); write P, C and M^(P)
-        "k_step_pred": n * 16 + cls + spec_bytes + n * 24 + spec_bytes,
-        # corrector + thermal kick: read P, C, ms and H^(P); write S and M^(S)
-        "k_step_corr": n * 28 + cls + spec_bytes + n * 12 + spec_bytes,
+        # predictor: read S (x,y,z,ms) and H^(S); write P, C and M^(P) (+ read S_lo, write C_lo)
+        "k_step_pred": n * 16 + cls + spec_bytes + n * 24 + spec_bytes + lo,
+        # corrector + thermal kick: read P, C, ms and H^(P); write S and M^(S) (+ read C_lo, write S_lo)
+        "k_step_corr": n * 28 + cls + spec_bytes + n * 12 + spec_bytes + lo,
     }
 
 
-def step_model_bytes(spec):
+def step_model_bytes(spec, comp):
     """Whole-step compulsory bytes per spin-step (field evaluation E counted twice
-    plus predictor and corrector traffic): B = 2E + 96, E = 24 + 48 K/N + 24 K_T/N."""
+    plus predictor and corrector traffic): B = 2E + 96 (+ 48 with low words),
+    E = 24 + 48 K/N + 24 K_T/N."""
     KN = (spec.nx // 2 + 1) / spec.nx
     KT = KN * (2 * spec.nz - 1) / spec.nz
     E = 12 + 24 * KN + 24 * KN + 24 * KT + 12
-    return 2 * E + 96
+    return 2 * E + 96 + (48 if comp else 0)
 
 
 def cpu_baseline(spec, budget_s=15.0):
@@ -106,18 +109,19 @@ def main():
     ms_step = 1e3 * el / args.steps
     print(f"# timed {args.steps} steps: {el:.3f} s", file=sys.stderr, flush=True)
 
-    # kernel durations: HIP events around every launch on the library stream
+    # kernel durations: each launch of one step repeated back to back between HIP
+    # events on the library stream (the stream every kernel runs on)
     prof = heun.profile(args.profile_steps)
     M.synchronize()
-    kb = kernel_bytes(spec, has_cls=True)
+    kb = kernel_bytes(spec, has_cls=True, comp=not args.plain)
     per_step = {k: v * (2 if k == "k_cols" else 1) for k, v in prof.items()}
     dom = max(per_step, key=per_step.get)
     achieved = kb[dom] / (prof[dom] * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "kernel": dom, "kernel_avg_ms": round(prof[dom], 5), "algorithmic_bytes": kb[dom],
-            "step_model_bytes_per_spin": round(step_model_bytes(spec), 2),
-            "step_model_GBs": round(step_model_bytes(spec) * value / 1e9, 1),
+            "step_model_bytes_per_spin": round(step_model_bytes(spec, not args.plain), 2),
+            "step_model_GBs": round(step_model_bytes(spec, not args.plain) * value / 1e9, 1),
             "kernels_avg_ms": {k: round(v, 5) for k, v in prof.items()}}
 
     cpu = None

@@ -214,8 +214,11 @@ int msd_stepper_set_compensated(msd_stepper* st, int yes);
 int msd_stepper_set_longrange3d(msd_stepper* st, msd_longrange3d* l);
 /* Advance nsteps Heun steps; asynchronous. */
 int msd_stepper_heun(msd_stepper* st, int nsteps);
-/* Kernel-level timing: runs nsteps more steps with HIP events around every
- * launch on the library stream; names[i]/avg_ms[i] for up to cap kernels. */
+/* Kernel-level timing on the library stream, names[i]/avg_ms[i] for up to cap
+ * kernels.  Layered and local forms: each launch of one Heun step is repeated
+ * nsteps times back to back between two HIP events (the kernels are idempotent
+ * in their inputs), leaving the state one step ahead.  LongRange3D form: nsteps
+ * more steps with events around every launch. */
 int msd_stepper_profile(msd_stepper* st, int nsteps, int cap, char names[][32], double* avg_ms, int* count);
 
 /* -------------------------------------------------------------- ensembles */

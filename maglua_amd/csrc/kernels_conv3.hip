@@ -101,9 +101,9 @@ __global__ void __launch_bounds__(R3_TPB) k3_rows(const Rep3* __restrict__ reps,
                 if (MODE == R3_INV) {
                     cv.hout[i] = Hd.x; cv.hout[n + i] = Hd.y; cv.hout[2 * n + i] = Hd.z;
                 } else if (MODE == R3_PRED) {
-                    v = site_pred<COMP>(a, exl, i, x, y, z, Hd);
+                    v = site_pred<COMP>(a, exl, i, Hd);
                 } else {
-                    v = site_corr<COMP>(a, exl, i, x, y, z, Hd, draw);
+                    v = site_corr<COMP>(a, exl, i, Hd, draw);
                 }
             }
         }

@@ -21,6 +21,8 @@
 
 namespace msd {
 
+// threads per workgroup of the runtime-shaped row kernels (compile-time shapes
+// carry their own count)
 static constexpr int ROW_TPB = 1024;
 static constexpr int COL_TPB = 512;
 // exchange class tables up to this size are staged in LDS by the fused kernels
@@ -43,12 +45,13 @@ __host__ __device__ constexpr int row_lines(int ty) { return (3 * ty + 1) >> 1; 
 
 // Build the packed complex lines Z = A + iB from the half spectra of rows
 // (2l, 2l+1) read from `spec` (column layout), using A[nx-k] = conj(A[k]).
-template <int NT>
+// Compile-time shapes (NX > 0) issue the loads in batches.
+template <int NT, int NX = 0, int TY = 0>
 __device__ __forceinline__ void rows_pack_spectrum(const float2* __restrict__ spec, int nx, int ny, int nz, int ty,
                                                    int z, int y0, float2* __restrict__ lds)
 {
     const int nxh = nx / 2 + 1, lines = row_lines(ty), total = lines * nx, half = nx >> 1;
-    for (int idx = threadIdx.x; idx < total; idx += NT) {
+    auto ld = [&](int idx) {
         const int l = idx % lines;
         const int k = idx / lines;
         const bool mirror = k > half;
@@ -61,8 +64,18 @@ __device__ __forceinline__ void rows_pack_spectrum(const float2* __restrict__ sp
             const int c1 = q1 / ty, r1 = q1 - c1 * ty;
             B = spec[((size_t)(c1 * nz + z) * nxh + kk) * ny + y0 + r1];
         }
-        if (mirror) { A.y = -A.y; B.y = -B.y; }
-        lds[(size_t)l * nx + k] = make_float2(A.x - B.y, A.y + B.x);
+        return make_float4(A.x, A.y, B.x, B.y);
+    };
+    auto st = [&](int idx, float4 v) {
+        const int l = idx % lines;
+        const int k = idx / lines;
+        if (k > half) { v.y = -v.y; v.w = -v.w; }
+        lds[(size_t)l * nx + k] = make_float2(v.x - v.w, v.y + v.z);
+    };
+    if constexpr (NX > 0) {
+        batched_copy<row_lines(TY) * NX, NT, 8>(ld, st);
+    } else {
+        for (int idx = threadIdx.x; idx < total; idx += NT) st(idx, ld(idx));
     }
 }
 
@@ -108,18 +121,18 @@ template <int NX>
 __device__ __forceinline__ const float2* row_twiddles(const DipoleDev& dp, float2* lds_tw)
 {
     if constexpr (NX > 0) {
-        for (int i = threadIdx.x; i < NX; i += ROW_TPB) lds_tw[i] = dp.twx[i];
+        for (int i = threadIdx.x; i < NX; i += blockDim.x) lds_tw[i] = dp.twx[i];
         return lds_tw;
     } else {
         return dp.twx;
     }
 }
 
-template <int NX, int TY, int SIGN>
+template <int NX, int TY, int TPB, int SIGN>
 __device__ __forceinline__ float2* row_fft(float2* A, float2* B, const DipoleDev& dp, const float2* tw)
 {
     if constexpr (NX > 0) {
-        fft_ip<NX, row_lines(TY), ROW_TPB, SIGN>(A, tw);   // in place: B is not used
+        fft_ip<NX, row_lines(TY), TPB, SIGN>(A, tw);   // in place: B is not used
         return A;
     }
     else return lds_fft<SIGN>(A, B, row_lines(dp.ty), dp.px, tw);
@@ -127,8 +140,8 @@ __device__ __forceinline__ float2* row_fft(float2* A, float2* B, const DipoleDev
 
 // ------------------------------------------------------------ op-API rows
 // SpinSystem::fft (cpu/core/spinsystem.cpp:936-966) along x: spins -> Mh
-template <int NX, int TY>
-__global__ void __launch_bounds__(ROW_TPB) k_rows_fwd(DipoleDev dp, const float* __restrict__ s)
+template <int NX, int TY, int TPB>
+__global__ void __launch_bounds__(TPB) k_rows_fwd(DipoleDev dp, const float* __restrict__ s)
 {
     extern __shared__ float2 lds[];
     const int nx = NX ? NX : dp.nx, ty = TY ? TY : dp.ty, lines = row_lines(ty), ny = dp.ny;
@@ -139,21 +152,24 @@ __global__ void __launch_bounds__(ROW_TPB) k_rows_fwd(DipoleDev dp, const float*
     float2* A = lds;
     float2* B = lds + (size_t)lines * nx;
     const float2* tw = row_twiddles<NX>(dp, A + (size_t)lines * nx);
-    for (int li = threadIdx.x; li < lines * nx; li += ROW_TPB) A[li] = make_float2(0.f, 0.f);
+    for (int li = threadIdx.x; li < lines * nx; li += TPB) A[li] = make_float2(0.f, 0.f);
     __syncthreads();
-    for (int li = threadIdx.x; li < 3 * ty * nx; li += ROW_TPB) {
+    auto ld = [&](int li) {
         const int q = li / nx, x = li - q * nx;
         const int c = q / ty, r = q - c * ty;
-        rowval(A, q, x, nx) = s[(size_t)c * n + x + (size_t)nx * (y0 + r + (size_t)ny * z)];
-    }
+        return s[(size_t)c * n + x + (size_t)nx * (y0 + r + (size_t)ny * z)];
+    };
+    auto st = [&](int li, float v) { const int q = li / nx; rowval(A, q, li - q * nx, nx) = v; };
+    if constexpr (NX > 0) batched_copy<3 * TY * NX, TPB, 8>(ld, st);
+    else for (int li = threadIdx.x; li < 3 * ty * nx; li += TPB) st(li, ld(li));
     __syncthreads();
-    const float2* R = row_fft<NX, TY, -1>(A, B, dp, tw);
-    rows_unpack_spectrum<ROW_TPB>(R, nx, ny, dp.nz, ty, z, y0, dp.Mh);
+    const float2* R = row_fft<NX, TY, TPB, -1>(A, B, dp, tw);
+    rows_unpack_spectrum<TPB>(R, nx, ny, dp.nz, ty, z, y0, dp.Mh);
 }
 
 // inverse along x: Hh -> real field rows, written to slot h
-template <int NX, int TY>
-__global__ void __launch_bounds__(ROW_TPB) k_rows_inv(DipoleDev dp, float* __restrict__ h)
+template <int NX, int TY, int TPB>
+__global__ void __launch_bounds__(TPB) k_rows_inv(DipoleDev dp, float* __restrict__ h)
 {
     extern __shared__ float2 lds[];
     const int nx = NX ? NX : dp.nx, ty = TY ? TY : dp.ty, lines = row_lines(ty), ny = dp.ny;
@@ -164,10 +180,10 @@ __global__ void __launch_bounds__(ROW_TPB) k_rows_inv(DipoleDev dp, float* __res
     float2* A = lds;
     float2* B = lds + (size_t)lines * nx;
     const float2* tw = row_twiddles<NX>(dp, A + (size_t)lines * nx);
-    rows_pack_spectrum<ROW_TPB>(dp.Hh, nx, ny, dp.nz, ty, z, y0, A);
+    rows_pack_spectrum<TPB, NX, TY>(dp.Hh, nx, ny, dp.nz, ty, z, y0, A);
     __syncthreads();
-    float2* R = row_fft<NX, TY, 1>(A, B, dp, tw);
-    for (int li = threadIdx.x; li < 3 * ty * nx; li += ROW_TPB) {
+    float2* R = row_fft<NX, TY, TPB, 1>(A, B, dp, tw);
+    for (int li = threadIdx.x; li < 3 * ty * nx; li += TPB) {
         const int q = li / nx, x = li - q * nx;
         const int c = q / ty, r = q - c * ty;
         h[(size_t)c * n + x + (size_t)nx * (y0 + r + (size_t)ny * z)] = rowval(R, q, x, nx);
@@ -187,17 +203,71 @@ __global__ void __launch_bounds__(COL_TPB) k_cols(DipoleDev dp)
     const int oa = blockIdx.x - 3 * kx;
     const int ny = NY ? NY : dp.ny, nxh = dp.nxh;
     constexpr int lines = 3 * NZ;
+    // LDS: lines (one buffer in place, two ping-pong for runtime shapes) and the y twiddles
     float2* A = lds;
     float2* B = lds + (size_t)lines * ny;
     const float2* tw = dp.twy;
+    // row oa of the symmetric tensor: components (oa,0) (oa,1) (oa,2) and which are odd in z
+    const int cidx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    const int c0 = cidx[oa][0], c1 = cidx[oa][1], c2 = cidx[oa][2];
+    const bool odd0 = (c0 == 2 || c0 == 4), odd1 = (c1 == 2 || c1 == 4), odd2 = (c2 == 2 || c2 == 4);
+    // Compile-time lengths: the twiddles and this workgroup's tensor words
+    // (components c0 c1 c2, offsets 0..NZ-1, every ky) are staged in LDS after the
+    // lines, loaded in the same batch as the column so one memory latency covers all.
+    float* wl = nullptr;
     if constexpr (NY > 0) {
-        float2* twl = A + (size_t)lines * ny;   // in-place transforms: no second buffer
-        for (int i = threadIdx.x; i < NY; i += COL_TPB) twl[i] = dp.twy[i];
+        float2* twl = A + (size_t)lines * NY;   // in-place transforms: no second buffer
+        wl = reinterpret_cast<float*>(twl + NY);
+        constexpr int TWI = (NY + COL_TPB - 1) / COL_TPB;
+        constexpr int MI = (lines * NY + COL_TPB - 1) / COL_TPB;
+        constexpr int WI = (3 * NZ * NY + COL_TPB - 1) / COL_TPB;
+        auto ldm = [&](int idx) { const int l = idx / NY, y = idx - l * NY; return dp.Mh[((size_t)l * nxh + kx) * NY + y]; };
+        auto ldw = [&](int idx) {
+            const int l = idx / NY, y = idx - l * NY;     // l = b*NZ + o
+            const int b = l / NZ, o = l - b * NZ;
+            const int comp = b == 0 ? c0 : (b == 1 ? c1 : c2);
+            return dp.Tw[((size_t)(comp * NZ + o) * nxh + kx) * NY + y];
+        };
+        float2 twv[TWI];
+#pragma unroll
+        for (int k = 0; k < TWI; ++k)
+            if (NY % COL_TPB == 0 || (int)threadIdx.x + k * COL_TPB < NY) twv[k] = dp.twy[threadIdx.x + k * COL_TPB];
+        if constexpr (MI + WI <= 32) {
+            float2 mv[MI];
+            float wv[WI];
+#pragma unroll
+            for (int k = 0; k < MI; ++k) {
+                const int idx = (int)threadIdx.x + k * COL_TPB;
+                if ((lines * NY) % COL_TPB == 0 || idx < lines * NY) mv[k] = ldm(idx);
+            }
+#pragma unroll
+            for (int k = 0; k < WI; ++k) {
+                const int idx = (int)threadIdx.x + k * COL_TPB;
+                if ((3 * NZ * NY) % COL_TPB == 0 || idx < 3 * NZ * NY) wv[k] = ldw(idx);
+            }
+#pragma unroll
+            for (int k = 0; k < MI; ++k) {
+                const int idx = (int)threadIdx.x + k * COL_TPB;
+                if ((lines * NY) % COL_TPB == 0 || idx < lines * NY) A[idx] = mv[k];
+            }
+#pragma unroll
+            for (int k = 0; k < WI; ++k) {
+                const int idx = (int)threadIdx.x + k * COL_TPB;
+                if ((3 * NZ * NY) % COL_TPB == 0 || idx < 3 * NZ * NY) wl[idx] = wv[k];
+            }
+        } else {
+            batched_copy<lines * NY, COL_TPB, 16>(ldm, [&](int idx, float2 v) { A[idx] = v; });
+            batched_copy<3 * NZ * NY, COL_TPB, 16>(ldw, [&](int idx, float v) { wl[idx] = v; });
+        }
+#pragma unroll
+        for (int k = 0; k < TWI; ++k)
+            if (NY % COL_TPB == 0 || (int)threadIdx.x + k * COL_TPB < NY) twl[threadIdx.x + k * COL_TPB] = twv[k];
         tw = twl;
-    }
-    for (int idx = threadIdx.x; idx < lines * ny; idx += COL_TPB) {
-        const int l = idx / ny, y = idx - l * ny;
-        A[idx] = dp.Mh[((size_t)l * nxh + kx) * ny + y];
+    } else {
+        for (int idx = threadIdx.x; idx < lines * ny; idx += COL_TPB) {
+            const int l = idx / ny, y = idx - l * ny;
+            A[idx] = dp.Mh[((size_t)l * nxh + kx) * ny + y];
+        }
     }
     __syncthreads();
     float2* R;
@@ -206,51 +276,53 @@ __global__ void __launch_bounds__(COL_TPB) k_cols(DipoleDev dp)
         R = A;
     }
     else R = lds_fft<-1>(A, B, lines, dp.py, tw);
-    // row oa of the symmetric tensor: components (oa,0) (oa,1) (oa,2) and which are odd in z
-    const int cidx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
-    const int c0 = cidx[oa][0], c1 = cidx[oa][1], c2 = cidx[oa][2];
-    const bool odd0 = (c0 == 2 || c0 == 4), odd1 = (c1 == 2 || c1 == 4), odd2 = (c2 == 2 || c2 == 4);
     for (int ky = threadIdx.x; ky < ny; ky += COL_TPB) {
-        float2 m[3][NZ];
         float w[3][NZ];
+        if constexpr (NY > 0) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
+            for (int b = 0; b < 3; ++b)
 #pragma unroll
-            for (int s = 0; s < NZ; ++s) m[c][s] = R[(c * NZ + s) * ny + ky];
+                for (int o = 0; o < NZ; ++o) w[b][o] = wl[(b * NZ + o) * NY + ky];
+        } else {
 #pragma unroll
-        for (int o = 0; o < NZ; ++o) {
-            w[0][o] = dp.Tw[((size_t)(c0 * NZ + o) * nxh + kx) * ny + ky];
-            w[1][o] = dp.Tw[((size_t)(c1 * NZ + o) * nxh + kx) * ny + ky];
-            w[2][o] = dp.Tw[((size_t)(c2 * NZ + o) * nxh + kx) * ny + ky];
+            for (int o = 0; o < NZ; ++o) {
+                w[0][o] = dp.Tw[((size_t)(c0 * NZ + o) * nxh + kx) * ny + ky];
+                w[1][o] = dp.Tw[((size_t)(c1 * NZ + o) * nxh + kx) * ny + ky];
+                w[2][o] = dp.Tw[((size_t)(c2 * NZ + o) * nxh + kx) * ny + ky];
+            }
         }
-        float2 h[NZ];
+        // source layer outer, target layer inner: one source's three components
+        // live at a time next to the NZ accumulators
+        float2 acc[NZ];
 #pragma unroll
-        for (int t = 0; t < NZ; ++t) {
-            float2 acc = make_float2(0.f, 0.f);
+        for (int t = 0; t < NZ; ++t) acc[t] = make_float2(0.f, 0.f);
 #pragma unroll
-            for (int s = 0; s < NZ; ++s) {
+        for (int s = 0; s < NZ; ++s) {
+            float2 m[3];
+#pragma unroll
+            for (int b = 0; b < 3; ++b) m[b] = R[(b * NZ + s) * ny + ky];
+#pragma unroll
+            for (int t = 0; t < NZ; ++t) {
                 const int o = s >= t ? s - t : t - s;
                 const float sg = s < t ? -1.0f : 1.0f;
                 // real components multiply, imaginary (XZ, YZ) ones give i*w*m = w*(-m.y, m.x)
 #pragma unroll
                 for (int b = 0; b < 3; ++b) {
                     const bool odd = b == 0 ? odd0 : (b == 1 ? odd1 : odd2);
-                    const float2 mb = m[b][s];
                     if (odd) {
                         const float wv = sg * w[b][o];
-                        acc.x -= wv * mb.y;
-                        acc.y += wv * mb.x;
+                        acc[t].x -= wv * m[b].y;
+                        acc[t].y += wv * m[b].x;
                     } else {
                         const float wv = w[b][o];
-                        acc.x += wv * mb.x;
-                        acc.y += wv * mb.y;
+                        acc[t].x += wv * m[b].x;
+                        acc[t].y += wv * m[b].y;
                     }
                 }
             }
-            h[t] = acc;
         }
 #pragma unroll
-        for (int t = 0; t < NZ; ++t) R[(oa * NZ + t) * ny + ky] = h[t];
+        for (int t = 0; t < NZ; ++t) R[(oa * NZ + t) * ny + ky] = acc[t];
     }
     __syncthreads();
     float2* Rs = R + (size_t)oa * NZ * ny;
@@ -273,8 +345,8 @@ __global__ void __launch_bounds__(COL_TPB) k_cols(DipoleDev dp)
 //   P = N(S + dt dM),  C = N(S + dt/2 dM),
 // then the x-transform of P for the next column pass.  COMP carries S and C as
 // compensated FP32 pairs (hi + lo).
-template <int NX, int TY, bool DIP, bool COMP>
-__global__ void __launch_bounds__(ROW_TPB) k_step_pred(StepArgs a)
+template <int NX, int TY, int TPB, bool DIP, bool COMP>
+__global__ void __launch_bounds__(TPB) k_step_pred(StepArgs a)
 {
     extern __shared__ float2 lds[];
     const DipoleDev& dp = a.dp;
@@ -293,18 +365,18 @@ __global__ void __launch_bounds__(ROW_TPB) k_step_pred(StepArgs a)
     if constexpr (!DIP) __syncthreads();
     if constexpr (DIP) {
         tw = row_twiddles<NX>(dp, A + (size_t)lines * nx);
-        rows_pack_spectrum<ROW_TPB>(dp.Hh, nx, ny, a.d.nz, ty, z, y0, A);
+        rows_pack_spectrum<TPB, NX, TY>(dp.Hh, nx, ny, a.d.nz, ty, z, y0, A);
         __syncthreads();
-        R = row_fft<NX, TY, 1>(A, B, dp, tw);
+        R = row_fft<NX, TY, TPB, 1>(A, B, dp, tw);
     }
     if (a.th.on && blockIdx.x == 0 && threadIdx.x == 0) *a.draw += 1ull;
     (void)n;
-    for (int li = threadIdx.x; li < ty * nx; li += ROW_TPB) {
+    for (int li = threadIdx.x; li < ty * nx; li += TPB) {
         const int r = li / nx, x = li - r * nx, y = y0 + r;
         const int i = x + nx * (y + ny * z);
         float3 Hd = make_float3(0.f, 0.f, 0.f);
         if constexpr (DIP) Hd = make_float3(rowval(R, r, x, nx), rowval(R, ty + r, x, nx), rowval(R, 2 * ty + r, x, nx));
-        const float3 p = site_pred<COMP>(a, exl, i, x, y, z, Hd);
+        const float3 p = site_pred<COMP>(a, exl, i, Hd);
         if constexpr (DIP) {
             rowval(R, r, x, nx) = p.x;
             rowval(R, ty + r, x, nx) = p.y;
@@ -313,16 +385,16 @@ __global__ void __launch_bounds__(ROW_TPB) k_step_pred(StepArgs a)
     }
     if constexpr (DIP) {
         __syncthreads();
-        const float2* F = row_fft<NX, TY, -1>(R, R == A ? B : A, dp, tw);
-        rows_unpack_spectrum<ROW_TPB>(F, nx, ny, a.d.nz, ty, z, y0, dp.Mh);
+        const float2* F = row_fft<NX, TY, TPB, -1>(R, R == A ? B : A, dp, tw);
+        rows_unpack_spectrum<TPB>(F, nx, ny, a.d.nz, ty, z, y0, dp.Mh);
     }
 }
 
 // Corrector half (PredictorCorrector.lua:81-85) and the thermal kick (:98-103):
 //   S' = N(C + dt/2 LLG(P, H(P))),  S'' = N(S' + dt LLG(S', H_th)) when Thermal is present,
 // then the x-transform of S'' for the next step's first column pass.
-template <int NX, int TY, bool DIP, bool COMP>
-__global__ void __launch_bounds__(ROW_TPB) k_step_corr(StepArgs a)
+template <int NX, int TY, int TPB, bool DIP, bool COMP>
+__global__ void __launch_bounds__(TPB) k_step_corr(StepArgs a)
 {
     extern __shared__ float2 lds[];
     const DipoleDev& dp = a.dp;
@@ -341,18 +413,18 @@ __global__ void __launch_bounds__(ROW_TPB) k_step_corr(StepArgs a)
     if constexpr (!DIP) __syncthreads();
     if constexpr (DIP) {
         tw = row_twiddles<NX>(dp, A + (size_t)lines * nx);
-        rows_pack_spectrum<ROW_TPB>(dp.Hh, nx, ny, a.d.nz, ty, z, y0, A);
+        rows_pack_spectrum<TPB, NX, TY>(dp.Hh, nx, ny, a.d.nz, ty, z, y0, A);
         __syncthreads();
-        R = row_fft<NX, TY, 1>(A, B, dp, tw);
+        R = row_fft<NX, TY, TPB, 1>(A, B, dp, tw);
     }
     const uint64_t draw = a.th.on ? (uint64_t)*a.draw : 0ull;
     (void)n;
-    for (int li = threadIdx.x; li < ty * nx; li += ROW_TPB) {
+    for (int li = threadIdx.x; li < ty * nx; li += TPB) {
         const int r = li / nx, x = li - r * nx, y = y0 + r;
         const int i = x + nx * (y + ny * z);
         float3 Hd = make_float3(0.f, 0.f, 0.f);
         if constexpr (DIP) Hd = make_float3(rowval(R, r, x, nx), rowval(R, ty + r, x, nx), rowval(R, 2 * ty + r, x, nx));
-        const float3 v = site_corr<COMP>(a, exl, i, x, y, z, Hd, draw);
+        const float3 v = site_corr<COMP>(a, exl, i, Hd, draw);
         if constexpr (DIP) {
             rowval(R, r, x, nx) = v.x;
             rowval(R, ty + r, x, nx) = v.y;
@@ -361,43 +433,64 @@ __global__ void __launch_bounds__(ROW_TPB) k_step_corr(StepArgs a)
     }
     if constexpr (DIP) {
         __syncthreads();
-        const float2* F = row_fft<NX, TY, -1>(R, R == A ? B : A, dp, tw);
-        rows_unpack_spectrum<ROW_TPB>(F, nx, ny, a.d.nz, ty, z, y0, dp.Mh);
+        const float2* F = row_fft<NX, TY, TPB, -1>(R, R == A ? B : A, dp, tw);
+        rows_unpack_spectrum<TPB>(F, nx, ny, a.d.nz, ty, z, y0, dp.Mh);
     }
 }
 
 // ------------------------------------------------------------- dispatch
-// compile-time row shapes (nx, rows per tile) and column lengths
-#define MSD_ROW_SHAPES(X) X(32, 8) X(64, 8) X(128, 8) X(256, 8) X(512, 4) X(1024, 2)
+// Compile-time row shapes (nx, rows per tile, threads per workgroup); the first
+// listed for an nx is its default, the others are alternatives a tuning run can
+// select.  Column lengths likewise.
+#define MSD_ROW_SHAPES(X)                                                           \
+    X(32, 8, 256) X(64, 8, 256) X(128, 8, 512) X(256, 8, 1024) X(256, 4, 512)       \
+    X(256, 8, 512) X(256, 2, 256) X(512, 4, 1024) X(512, 2, 512) X(1024, 2, 1024)
 #define MSD_COL_LENGTHS(X) X(32) X(64) X(128) X(256) X(512) X(1024)
 
 static int row_tiles(int ny, int nz, int ty) { return nz * (ny / ty); }
 
+bool row_shape_compiled(int nx, int ty, int tpb)
+{
+#define MSD_HAS(NXV, TYV, TPBV) if (nx == NXV && ty == TYV && tpb == TPBV) return true;
+    MSD_ROW_SHAPES(MSD_HAS)
+#undef MSD_HAS
+    return false;
+}
+
+void row_shape_default(int nx, int ny, int* ty, int* tpb)
+{
+    *ty = 0;
+#define MSD_DEF(NXV, TYV, TPBV)                                                     \
+    if (*ty == 0 && nx == NXV && ny % TYV == 0) { *ty = TYV; *tpb = TPBV; }
+    MSD_ROW_SHAPES(MSD_DEF)
+#undef MSD_DEF
+}
+
 hipError_t launch_rows_fwd(const DipoleDev& dp, const float* s, hipStream_t st)
 {
     const int g = row_tiles(dp.ny, dp.nz, dp.ty);
-#define MSD_FWD(NXV, TYV)                                                           \
-    if (dp.nx == NXV && dp.ty == TYV) {                                             \
-        k_rows_fwd<NXV, TYV><<<g, ROW_TPB, dp.lds_rows, st>>>(dp, s);               \
+#define MSD_FWD(NXV, TYV, TPBV)                                                     \
+    if (dp.nx == NXV && dp.ty == TYV && dp.tpb == TPBV) {                           \
+        k_rows_fwd<NXV, TYV, TPBV><<<g, TPBV, dp.lds_rows, st>>>(dp, s);            \
         return hipGetLastError();                                                   \
     }
     MSD_ROW_SHAPES(MSD_FWD)
 #undef MSD_FWD
-    k_rows_fwd<0, 0><<<g, ROW_TPB, dp.lds_rows, st>>>(dp, s);
+    k_rows_fwd<0, 0, ROW_TPB><<<g, ROW_TPB, dp.lds_rows, st>>>(dp, s);
     return hipGetLastError();
 }
 
 hipError_t launch_rows_inv(const DipoleDev& dp, float* h, hipStream_t st)
 {
     const int g = row_tiles(dp.ny, dp.nz, dp.ty);
-#define MSD_INV(NXV, TYV)                                                           \
-    if (dp.nx == NXV && dp.ty == TYV) {                                             \
-        k_rows_inv<NXV, TYV><<<g, ROW_TPB, dp.lds_rows, st>>>(dp, h);               \
+#define MSD_INV(NXV, TYV, TPBV)                                                     \
+    if (dp.nx == NXV && dp.ty == TYV && dp.tpb == TPBV) {                           \
+        k_rows_inv<NXV, TYV, TPBV><<<g, TPBV, dp.lds_rows, st>>>(dp, h);            \
         return hipGetLastError();                                                   \
     }
     MSD_ROW_SHAPES(MSD_INV)
 #undef MSD_INV
-    k_rows_inv<0, 0><<<g, ROW_TPB, dp.lds_rows, st>>>(dp, h);
+    k_rows_inv<0, 0, ROW_TPB><<<g, ROW_TPB, dp.lds_rows, st>>>(dp, h);
     return hipGetLastError();
 }
 
@@ -430,11 +523,11 @@ hipError_t launch_cols(const DipoleDev& dp, hipStream_t st)
     return hipGetLastError();
 }
 
-template <bool PRED, int NX, int TY, bool DIP, bool COMP>
+template <bool PRED, int NX, int TY, int TPB, bool DIP, bool COMP>
 static void launch_step_t(const StepArgs& a, int g, size_t lds, hipStream_t st)
 {
-    if (PRED) k_step_pred<NX, TY, DIP, COMP><<<g, ROW_TPB, lds, st>>>(a);
-    else k_step_corr<NX, TY, DIP, COMP><<<g, ROW_TPB, lds, st>>>(a);
+    if (PRED) k_step_pred<NX, TY, TPB, DIP, COMP><<<g, TPB, lds, st>>>(a);
+    else k_step_corr<NX, TY, TPB, DIP, COMP><<<g, TPB, lds, st>>>(a);
 }
 
 template <bool PRED>
@@ -443,21 +536,21 @@ static hipError_t launch_step(const StepArgs& a, hipStream_t st)
     const int g = row_tiles(a.d.ny, a.d.nz, a.ty);
     const bool comp = a.Slo != nullptr;
     if (!a.has_dip) {
-        if (comp) launch_step_t<PRED, 0, 0, false, true>(a, g, 0, st);
-        else launch_step_t<PRED, 0, 0, false, false>(a, g, 0, st);
+        if (comp) launch_step_t<PRED, 0, 0, ROW_TPB, false, true>(a, g, 0, st);
+        else launch_step_t<PRED, 0, 0, ROW_TPB, false, false>(a, g, 0, st);
         return hipGetLastError();
     }
     const size_t lds = a.dp.lds_rows;
-#define MSD_STEP(NXV, TYV)                                                          \
-    if (a.d.nx == NXV && a.ty == TYV) {                                             \
-        if (comp) launch_step_t<PRED, NXV, TYV, true, true>(a, g, lds, st);         \
-        else launch_step_t<PRED, NXV, TYV, true, false>(a, g, lds, st);             \
+#define MSD_STEP(NXV, TYV, TPBV)                                                    \
+    if (a.d.nx == NXV && a.ty == TYV && a.dp.tpb == TPBV) {                         \
+        if (comp) launch_step_t<PRED, NXV, TYV, TPBV, true, true>(a, g, lds, st);   \
+        else launch_step_t<PRED, NXV, TYV, TPBV, true, false>(a, g, lds, st);       \
         return hipGetLastError();                                                   \
     }
     MSD_ROW_SHAPES(MSD_STEP)
 #undef MSD_STEP
-    if (comp) launch_step_t<PRED, 0, 0, true, true>(a, g, lds, st);
-    else launch_step_t<PRED, 0, 0, true, false>(a, g, lds, st);
+    if (comp) launch_step_t<PRED, 0, 0, ROW_TPB, true, true>(a, g, lds, st);
+    else launch_step_t<PRED, 0, 0, ROW_TPB, true, false>(a, g, lds, st);
     return hipGetLastError();
 }
 
@@ -490,15 +583,15 @@ hipError_t configure_lds(size_t max_bytes)
 {
     const int v = (int)max_bytes;
     hipError_t first = hipSuccess;
-#define MSD_RA(NXV, TYV)                                                            \
-    set_lds((const void*)k_rows_fwd<NXV, TYV>, v, first);                           \
-    set_lds((const void*)k_rows_inv<NXV, TYV>, v, first);                           \
-    set_lds((const void*)k_step_pred<NXV, TYV, true, true>, v, first);              \
-    set_lds((const void*)k_step_pred<NXV, TYV, true, false>, v, first);             \
-    set_lds((const void*)k_step_corr<NXV, TYV, true, true>, v, first);              \
-    set_lds((const void*)k_step_corr<NXV, TYV, true, false>, v, first);
+#define MSD_RA(NXV, TYV, TPBV)                                                      \
+    set_lds((const void*)k_rows_fwd<NXV, TYV, TPBV>, v, first);                     \
+    set_lds((const void*)k_rows_inv<NXV, TYV, TPBV>, v, first);                     \
+    set_lds((const void*)k_step_pred<NXV, TYV, TPBV, true, true>, v, first);        \
+    set_lds((const void*)k_step_pred<NXV, TYV, TPBV, true, false>, v, first);       \
+    set_lds((const void*)k_step_corr<NXV, TYV, TPBV, true, true>, v, first);        \
+    set_lds((const void*)k_step_corr<NXV, TYV, TPBV, true, false>, v, first);
     MSD_ROW_SHAPES(MSD_RA)
-    MSD_RA(0, 0)
+    MSD_RA(0, 0, ROW_TPB)
 #undef MSD_RA
     cols_attr<1>(v, first); cols_attr<2>(v, first); cols_attr<3>(v, first); cols_attr<4>(v, first);
     cols_attr<5>(v, first); cols_attr<6>(v, first); cols_attr<7>(v, first); cols_attr<8>(v, first);

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -203,7 +204,9 @@ extern "C" int msd_system_create(int nx, int ny, int nz, msd_system** out)
     NONNULL(out);
     *out = nullptr;
     if (nx < 1 || ny < 1 || nz < 1) return fail(MSD_EINVAL, "SpinSystem.new: bad size %dx%dx%d", nx, ny, nz);
-    if ((double)nx * ny * nz > 2.0e9) return fail(MSD_EINVAL, "SpinSystem.new: %dx%dx%d exceeds 2^31 sites", nx, ny, nz);
+    // the step kernels address per-site arrays (up to 16 bytes a site) with 32-bit byte offsets
+    if ((double)nx * ny * nz > (double)(1u << 28))
+        return fail(MSD_EUNSUPPORTED, "SpinSystem.new: %dx%dx%d exceeds 2^28 sites", nx, ny, nz);
     if (int rc = ensure_init()) return rc;
     std::unique_ptr<msd_system> s(new msd_system);
     s->d = Dims{nx, ny, nz, nx * ny * nz};
@@ -530,13 +533,6 @@ struct msd_exchange {
     }
 };
 
-static inline int min_image(int dlt, int n)
-{
-    int m = ((dlt % n) + n) % n;
-    if (2 * m >= n && n > 1) m -= n;
-    return m;
-}
-
 // CSR by target in path insertion order, plus the class table when the
 // couplings reduce to at most 256 distinct per-site stencils.
 int msd_exchange::compile()
@@ -572,20 +568,15 @@ int msd_exchange::compile()
     std::vector<ExEntry> ents;
     std::string key;
     for (int t = 0; t < n && lut_ok; ++t) {
-        const int tx = t % d.nx, ty = (t / d.nx) % d.ny, tz = t / (d.nx * d.ny);
         key.clear();
         std::vector<ExEntry> mine;
         for (int k = cnt[t]; k < cnt[t + 1]; ++k) {
-            const int f = hcol[k];
-            const int fx = f % d.nx, fy = (f / d.nx) % d.ny, fz = f / (d.nx * d.ny);
-            const int dx = min_image(fx - tx, d.nx), dy = min_image(fy - ty, d.ny), dz = min_image(fz - tz, d.nz);
-            if (std::abs(dx) > 127 || std::abs(dy) > 127 || std::abs(dz) > 127) { lut_ok = false; break; }
             ExEntry e;
-            e.dx = (int8_t)dx; e.dy = (int8_t)dy; e.dz = (int8_t)dz; e.pad = 0; e.J = hval[k];
+            e.dl = hcol[k] - t;
+            e.J = hval[k];
             mine.push_back(e);
             key.append(reinterpret_cast<const char*>(&e), sizeof e);
         }
-        if (!lut_ok) break;
         auto it = ids.find(key);
         int id;
         if (it == ids.end()) {
@@ -966,22 +957,35 @@ extern "C" int msd_dipole_create(int nx, int ny, int nz, msd_dipole** out)
     DipoleDev& dv = p->dev;
     dv.nx = nx; dv.ny = ny; dv.nz = nz; dv.nxh = p->nxh;
     dv.px = px; dv.py = py;
-    // rows per tile: largest of 8,4,2,1 dividing ny whose two LDS buffers fit 64 KiB
-    int ty = 1;
-    for (int c : {8, 4, 2}) {
-        const size_t need = (2 * (size_t)((3 * c + 1) / 2) + 1) * nx * sizeof(float2);
-        if (ny % c == 0 && need <= 65536) { ty = c; break; }
+    // Row tiles: the default compile-time shape for nx (kernels_dipole.hip's
+    // MSD_ROW_SHAPES) whose rows per tile divide ny; MSD_ROW_TILE="ty:tpb" picks
+    // another compiled shape for tuning runs.  Without one, runtime-shaped
+    // kernels take the largest of 8,4,2,1 rows dividing ny whose two LDS
+    // buffers fit 64 KiB.
+    int ty = 0, tpb = 0;
+    row_shape_default(nx, ny, &ty, &tpb);
+    if (const char* env = getenv("MSD_ROW_TILE")) {
+        int t = 0, b = 0;
+        if (sscanf(env, "%d:%d", &t, &b) == 2 && t > 0 && ny % t == 0 && row_shape_compiled(nx, t, b)) { ty = t; tpb = b; }
+    }
+    const bool row_ct = ty > 0;
+    if (!row_ct) {
+        ty = 1;
+        tpb = 1024;
+        for (int c : {8, 4, 2}) {
+            const size_t need = (2 * (size_t)((3 * c + 1) / 2) + 1) * nx * sizeof(float2);
+            if (ny % c == 0 && need <= 65536) { ty = c; break; }
+        }
     }
     dv.ty = ty;
-    // Compile-time shapes (the MSD_ROW_SHAPES / MSD_COL_LENGTHS instances of
-    // kernels_dipole.hip) transform in place: one line buffer plus the twiddle
-    // table.  Other sizes use two ping-pong buffers.
-    const bool pow2_len_x = nx >= 32 && nx <= 1024 && (nx & (nx - 1)) == 0;
+    dv.tpb = tpb;
+    // Compile-time shapes (MSD_ROW_SHAPES / MSD_COL_LENGTHS) transform in place:
+    // one line buffer plus the twiddle table.  Other sizes use two ping-pong buffers.
     const bool pow2_len_y = ny >= 32 && ny <= 1024 && (ny & (ny - 1)) == 0;
-    const bool row_ct = pow2_len_x && ty == (nx <= 256 ? 8 : (nx == 512 ? 4 : 2));
     const size_t rbuf = (size_t)((3 * ty + 1) / 2) * nx, cbuf = (size_t)3 * nz * ny;
     dv.lds_rows = (row_ct ? rbuf + nx : 2 * rbuf + nx) * sizeof(float2);
-    dv.lds_cols = (pow2_len_y ? cbuf + ny : 2 * cbuf + ny) * sizeof(float2);
+    // compile-time column lengths also stage the workgroup's tensor words (3 x nz x ny floats)
+    dv.lds_cols = pow2_len_y ? (cbuf + ny) * sizeof(float2) + (size_t)3 * nz * ny * sizeof(float) : 2 * cbuf * sizeof(float2);
     if (dv.lds_rows > G.max_lds || dv.lds_cols > G.max_lds)
         return fail(MSD_EUNSUPPORTED, "Dipole.new: %dx%dx%d needs %zu/%zu bytes of LDS per workgroup (device limit %zu)", nx, ny,
                     nz, dv.lds_rows, dv.lds_cols, G.max_lds);
@@ -1733,6 +1737,51 @@ extern "C" int msd_stepper_profile(msd_stepper* st, int nsteps, int cap, char na
     else if (st->kind() == SK_LAYERED) seq = {L_COLS, L_PRED2, L_COLS, L_CORR2};
     else seq = {L_PRED2, L_CORR2};
     const int per = (int)seq.size();
+    if (st->kind() != SK_CONV3) {
+        // Each launch of one step repeated nsteps times back to back between two
+        // events, so the averages carry no per-launch event gaps.  The layered and
+        // local kernels are idempotent in their inputs (columns: Mh -> Hh;
+        // predictor: S, Hh -> P, C, Mh; corrector: P, C, Hh -> S, Mh), so the
+        // repeats leave the state exactly one step ahead once the predictor's
+        // draw-counter increments are taken back to one.
+        std::vector<hipEvent_t> ev(2 * (size_t)per);
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        int rc = MSD_OK;
+        for (int k = 0; k < per && !rc; ++k) {
+            hipError_t e = hipEventRecord(ev[2 * k], G.stream);
+            for (int i = 0; i < nsteps && e == hipSuccess; ++i) e = launch_code(st, seq[k], G.stream);
+            if (e == hipSuccess) e = hipEventRecord(ev[2 * k + 1], G.stream);
+            if (e == hipSuccess && seq[k] == L_PRED2 && st->th) {
+                const unsigned long long v = (unsigned long long)st->th->draw;
+                e = hipMemcpyAsync(st->draw, &v, sizeof v, hipMemcpyHostToDevice, G.stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(G.stream);
+            }
+            if (e != hipSuccess) rc = fail(MSD_EDEVICE, "profile: %s", hipGetErrorString(e));
+        }
+        if (!rc) {
+            HIPCHK(hipStreamSynchronize(G.stream));
+            std::vector<std::string> nm;
+            std::vector<double> tot;
+            std::vector<int> cnt;
+            for (int k = 0; k < per; ++k) {
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+                const char* name = launch_name(seq[k]);
+                size_t u = 0;
+                while (u < nm.size() && nm[u] != name) ++u;
+                if (u == nm.size()) { nm.push_back(name); tot.push_back(0); cnt.push_back(0); }
+                tot[u] += ms; cnt[u] += nsteps;
+            }
+            *count = std::min<int>((int)nm.size(), cap);
+            for (int u = 0; u < *count; ++u) {
+                std::snprintf(names[u], 32, "%s", nm[u].c_str());
+                avg_ms[u] = tot[u] / cnt[u];
+            }
+            stepper_end(st, 1);
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        return rc;
+    }
     std::vector<hipEvent_t> ev((size_t)per * nsteps + 1);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     int rc = MSD_OK;

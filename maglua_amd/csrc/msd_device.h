@@ -232,7 +232,22 @@ __device__ __forceinline__ float2* fft_ct(float2* a, float2* b, const float2* tw
 
 // In-place form: every thread holds its butterflies' inputs in registers across a
 // barrier, so one LDS buffer serves all stages (half the LDS of the ping-pong form).
-template <int L, int COUNT, int NT, int SIGN, int NS, int R>
+//
+// Between stages the line is kept in a swizzled order, element a at
+// a ^ ((a >> 3) & 31) (a permutation of each power-of-two line of 32 or more).
+// Unswizzled, the first stage's writes (a thread's R outputs at R consecutive
+// places) and the second's (stride NS = 8) land lanes on a few LDS banks
+// (8-way and 4-way conflicts); the swizzle spreads both over all banks and
+// keeps the stride-1 reads conflict-free.  The first stage reads and the last
+// stage writes the natural order, so callers never see the swizzle.
+template <int L>
+__device__ __forceinline__ int lds_swz(int a)
+{
+    if constexpr (L >= 32 && (L & (L - 1)) == 0) return a ^ ((a >> 3) & 31);
+    else return a;
+}
+
+template <int L, int COUNT, int NT, int SIGN, int NS, int R, bool SWIN, bool SWOUT>
 __device__ __forceinline__ void stage_ip(float2* __restrict__ buf, const float2* __restrict__ tw)
 {
     constexpr int nb = L / R;
@@ -246,9 +261,9 @@ __device__ __forceinline__ void stage_ip(float2* __restrict__ buf, const float2*
         if (total % NT == 0 || b < total) {
             const int seq = b / nb;
             const int j = b - seq * nb;
-            const float2* s = buf + seq * L + j;
+            const float2* s = buf + seq * L;
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[it][r] = s[r * nb];
+            for (int r = 0; r < R; ++r) v[it][r] = s[SWIN ? lds_swz<L>(j + r * nb) : j + r * nb];
         }
     }
     __syncthreads();
@@ -271,9 +286,10 @@ __device__ __forceinline__ void stage_ip(float2* __restrict__ buf, const float2*
             else if (R == 4) dft4<SIGN>(v[it]);
             else if (R == 5) dft5<SIGN>(v[it]);
             else dft8<SIGN>(v[it]);
-            float2* d = buf + seq * L + (j / NS) * NS * R + k;
+            float2* d = buf + seq * L;
+            const int o = (j / NS) * NS * R + k;
 #pragma unroll
-            for (int q = 0; q < R; ++q) d[q * NS] = v[it][q];
+            for (int q = 0; q < R; ++q) d[SWOUT ? lds_swz<L>(o + q * NS) : o + q * NS] = v[it][q];
         }
     }
     __syncthreads();
@@ -284,7 +300,7 @@ struct FFTip {
     static __device__ __forceinline__ void run(float2* a, const float2* tw)
     {
         constexpr int R = next_radix(L / NS);
-        stage_ip<L, COUNT, NT, SIGN, NS, R>(a, tw);
+        stage_ip<L, COUNT, NT, SIGN, NS, R, NS != 1, NS * R != L>(a, tw);
         FFTip<L, COUNT, NT, SIGN, NS * R>::run(a, tw);
     }
 };
@@ -300,6 +316,52 @@ __device__ __forceinline__ void fft_ip(float2* a, const float2* twl)
     FFTip<L, COUNT, NT, SIGN, 1>::run(a, twl);
 }
 
+// ------------------------------------------------------- staged copies
+// This thread's share (idx = threadIdx.x + k*NT) of TOT items: ld(idx) for CH
+// items at a time, every load of a chunk issued before the first st(idx, v).
+// A loop that loads, waits and stores pays one memory latency per trip.
+template <int TOT, int NT, int CH, typename LD, typename ST>
+__device__ __forceinline__ void batched_copy(LD ld, ST st)
+{
+    constexpr int IT = (TOT + NT - 1) / NT;
+    using V = decltype(ld(0));
+#pragma unroll
+    for (int c0 = 0; c0 < IT; c0 += CH) {
+        V v[CH];
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+            const int it = c0 + k, idx = (int)threadIdx.x + it * NT;
+            if (it < IT && (TOT % NT == 0 || idx < TOT)) v[k] = ld(idx);
+        }
+#pragma unroll
+        for (int k = 0; k < CH; ++k) {
+            const int it = c0 + k, idx = (int)threadIdx.x + it * NT;
+            if (it < IT && (TOT % NT == 0 || idx < TOT)) st(idx, v[k]);
+        }
+    }
+}
+
+// ------------------------------------------------------ site addressing
+// Per-site arrays are addressed as a uniform base plus a 32-bit byte offset
+// (the global_load saddr form: one VALU op per address instead of a 64-bit
+// multiply-add).  Every array the step kernels index this way holds fewer than
+// 2^30 elements (the system size limit checked at creation).
+template <typename T>
+__device__ __forceinline__ T ldg(const T* __restrict__ base, uint32_t i)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (uint64_t)(i * (uint32_t)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void stg(T* __restrict__ base, uint32_t i, T v)
+{
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (uint64_t)(i * (uint32_t)sizeof(T))) = v;
+}
+
+// Hardware reciprocal and reciprocal square root (v_rcp_f32 / v_rsq_f32, 1 ulp);
+// the correctly rounded IEEE division sequence costs ~10 VALU ops per use.
+__device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float rsqf(float x) { return __builtin_amdgcn_rsqf(x); }
+
 // ---------------------------------------------------------------- LLG
 __device__ __forceinline__ float3 cross3(float3 a, float3 b)
 {
@@ -309,11 +371,11 @@ __device__ __forceinline__ float3 cross3(float3 a, float3 b)
 // dM = ps*(-gaa) M x H - ds*(alpha/ms) gaa M x (M x h)   (LLGCartesian with thermalOnlyFirstTerm)
 __device__ __forceinline__ float3 llg_rhs(float3 M, float3 H, float3 h, float alpha, float gamma, float ms, float ps, float ds)
 {
-    const float gaa = gamma / (1.0f + alpha * alpha);
+    const float gaa = gamma * rcpf(1.0f + alpha * alpha);
     const float3 Mh = cross3(M, h);
     const float3 MMh = cross3(M, Mh);
     const float3 MH = cross3(M, H);
-    const float damp = ds * (alpha / ms) * gaa;
+    const float damp = ds * alpha * rcpf(ms) * gaa;
     const float prec = ps * -gaa;
     return make_float3(prec * MH.x - damp * MMh.x, prec * MH.y - damp * MMh.y, prec * MH.z - damp * MMh.z);
 }
@@ -346,7 +408,7 @@ __device__ __forceinline__ void df_renorm(float3& hi, float3& lo, float ms)
 {
     const float mm = hi.x * hi.x + hi.y * hi.y + hi.z * hi.z + 2.0f * (hi.x * lo.x + hi.y * lo.y + hi.z * lo.z);
     if (mm != 0.0f) {
-        const float inv = ms / sqrtf(mm);
+        const float inv = ms * rsqf(mm);
         df_scale(hi.x, lo.x, inv);
         df_scale(hi.y, lo.y, inv);
         df_scale(hi.z, lo.z, inv);
@@ -357,7 +419,7 @@ __device__ __forceinline__ float3 renorm(float3 v, float ms)
 {
     const float mm = v.x * v.x + v.y * v.y + v.z * v.z;
     if (mm != 0.0f) {
-        const float inv = ms / sqrtf(mm);
+        const float inv = ms * rsqf(mm);
         v.x *= inv; v.y *= inv; v.z *= inv;
     }
     return v;
@@ -370,9 +432,10 @@ __device__ __forceinline__ float3 aniso_field(const AnisoDev& an, int i, int n, 
 {
     if (!(ms > 0.0f) || an.mode == 0) return make_float3(0.f, 0.f, 0.f);
     float3 h;
+    const float f = 2.0f * rcpf(ms * ms);
     if (an.mode == 2) {
-        const float4 a = an.axk[i];
-        const float v = 2.0f * a.w * (s.x * a.x + s.y * a.y + s.z * a.z) / (ms * ms);
+        const float4 a = ldg(an.axk, (uint32_t)i);
+        const float v = f * a.w * (s.x * a.x + s.y * a.y + s.z * a.z);
         h = make_float3(a.x * v, a.y * v, a.z * v);
     } else {
         float m[6];
@@ -381,9 +444,8 @@ __device__ __forceinline__ float3 aniso_field(const AnisoDev& an, int i, int n, 
             for (int k = 0; k < 6; ++k) m[k] = an.u[k];
         } else {
 #pragma unroll
-            for (int k = 0; k < 6; ++k) m[k] = an.mat[(size_t)k * n + i];
+            for (int k = 0; k < 6; ++k) m[k] = ldg(an.mat + (size_t)k * n, (uint32_t)i);
         }
-        const float f = 2.0f / (ms * ms);
         h = make_float3(f * (m[0] * s.x + m[1] * s.y + m[2] * s.z),
                         f * (m[1] * s.x + m[3] * s.y + m[4] * s.z),
                         f * (m[2] * s.x + m[4] * s.y + m[5] * s.z));
@@ -394,73 +456,69 @@ __device__ __forceinline__ float3 aniso_field(const AnisoDev& an, int i, int n, 
 // Thermal field amplitude (cpu/thermal/spinoperationthermal.cpp:165-178)
 __device__ __forceinline__ float thermal_sigma(const ThermalDev& th, const LLGDev& lp, int i, float ms, float dt)
 {
-    const float al = lp.site_alpha ? lp.site_alpha[i] : lp.alpha;
-    const float ga = lp.site_gamma ? lp.site_gamma[i] : lp.gamma;
+    const float al = lp.site_alpha ? ldg(lp.site_alpha, (uint32_t)i) : lp.alpha;
+    const float ga = lp.site_gamma ? ldg(lp.site_gamma, (uint32_t)i) : lp.gamma;
     if (ms == 0.0f || th.T_scale == 0.0f || !(ga > 0.0f)) return 0.0f;
-    const float sc = th.site ? th.site[i] : 1.0f;
-    return sqrtf((2.0f * al * th.T_scale * sc) / (ms * dt * ga));
+    const float sc = th.site ? ldg(th.site, (uint32_t)i) : 1.0f;
+    return sqrtf((2.0f * al * th.T_scale * sc) * rcpf(ms * dt * ga));
 }
 
 // Exchange field at target i (cpu/exchange/spinoperationexchange.cpp:134-213):
-// sum_j J_j s_f / m_f over the site's couplings, then / m_i when normalising.
-__device__ __forceinline__ float3 exchange_field(const ExchangeDev& ex, const Dims& d, int i, int x, int y, int z,
-                                                 const float* __restrict__ s, const float* __restrict__ m)
+// sum_j J_j s_f / m_f over the site's couplings, then / m_i when normalising
+// (mi = m_i).  Class-table entries carry the source as a linear offset f - i.
+__device__ __forceinline__ float3 exchange_field(const ExchangeDev& ex, uint32_t n, uint32_t i,
+                                                 const float* __restrict__ s, const float* __restrict__ m, float mi)
 {
+    const float* __restrict__ sx = s;
+    const float* __restrict__ sy = s + n;
+    const float* __restrict__ sz = s + 2 * (size_t)n;
     float3 h = make_float3(0.f, 0.f, 0.f);
-    const int n = d.n;
     if (ex.mode == 1) {
-        const int c = ex.cls ? ex.cls[i] : 0;
+        const int c = ex.cls ? (int)ldg(ex.cls, i) : 0;
         const int e0 = ex.cls_start[c];
         const int cnt = ex.cls_start[c + 1] - e0;
         // first 8 couplings unrolled so their loads are in flight together
-        float sx[8], sy[8], sz[8], mf[8], J[8];
+        float vx[8], vy[8], vz[8], mf[8], J[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             mf[u] = 0.0f;
             if (u < cnt) {
                 const ExEntry E = ex.ent[e0 + u];
-                int xn = x + E.dx, yn = y + E.dy, zn = z + E.dz;
-                xn += xn < 0 ? d.nx : 0; xn -= xn >= d.nx ? d.nx : 0;
-                yn += yn < 0 ? d.ny : 0; yn -= yn >= d.ny ? d.ny : 0;
-                zn += zn < 0 ? d.nz : 0; zn -= zn >= d.nz ? d.nz : 0;
-                const int f = xn + d.nx * (yn + d.ny * zn);
+                const uint32_t f = i + (uint32_t)E.dl;
                 J[u] = E.J;
-                mf[u] = m[f];
-                sx[u] = s[f]; sy[u] = s[n + f]; sz[u] = s[2 * n + f];
+                mf[u] = ldg(m, f);
+                vx[u] = ldg(sx, f); vy[u] = ldg(sy, f); vz[u] = ldg(sz, f);
             }
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             if (mf[u] != 0.0f) {
-                const float w = J[u] / mf[u];
-                h.x += sx[u] * w; h.y += sy[u] * w; h.z += sz[u] * w;
+                const float w = J[u] * rcpf(mf[u]);
+                h.x += vx[u] * w; h.y += vy[u] * w; h.z += vz[u] * w;
             }
         }
         for (int e = e0 + 8; e < e0 + cnt; ++e) {
             const ExEntry E = ex.ent[e];
-            int xn = x + E.dx, yn = y + E.dy, zn = z + E.dz;
-            xn += xn < 0 ? d.nx : 0; xn -= xn >= d.nx ? d.nx : 0;
-            yn += yn < 0 ? d.ny : 0; yn -= yn >= d.ny ? d.ny : 0;
-            zn += zn < 0 ? d.nz : 0; zn -= zn >= d.nz ? d.nz : 0;
-            const int f = xn + d.nx * (yn + d.ny * zn);
-            const float mfe = m[f];
+            const uint32_t f = i + (uint32_t)E.dl;
+            const float mfe = ldg(m, f);
             if (mfe != 0.0f) {
-                const float w = E.J / mfe;
-                h.x += s[f] * w; h.y += s[n + f] * w; h.z += s[2 * n + f] * w;
+                const float w = E.J * rcpf(mfe);
+                h.x += ldg(sx, f) * w; h.y += ldg(sy, f) * w; h.z += ldg(sz, f) * w;
             }
         }
     } else if (ex.mode == 2) {
-        for (int j = ex.row[i]; j < ex.row[i + 1]; ++j) {
-            const int f = ex.col[j];
-            const float mf = m[f];
-            if (mf != 0.0f) {
-                const float w = ex.val[j] / mf;
-                h.x += s[f] * w; h.y += s[n + f] * w; h.z += s[2 * n + f] * w;
+        const int j1 = ldg(ex.row, i + 1u);
+        for (int j = ldg(ex.row, i); j < j1; ++j) {
+            const uint32_t f = (uint32_t)ldg(ex.col, (uint32_t)j);
+            const float mfe = ldg(m, f);
+            if (mfe != 0.0f) {
+                const float w = ldg(ex.val, (uint32_t)j) * rcpf(mfe);
+                h.x += ldg(sx, f) * w; h.y += ldg(sy, f) * w; h.z += ldg(sz, f) * w;
             }
         }
     }
     if (ex.normalize && ex.mode) {
-        const float inv = 1.0f / m[i];
+        const float inv = rcpf(mi);
         h.x *= inv; h.y *= inv; h.z *= inv;
     }
     return h;
@@ -471,23 +529,23 @@ __device__ __forceinline__ float3 exchange_field(const ExchangeDev& ex, const Di
 // H = exchange + anisotropy + Hd at S; P = N(S + dt dM), C = N(S + dt/2 dM).
 // Writes P, C (and C's low word when COMP); returns P for the next transform.
 template <bool COMP>
-__device__ __forceinline__ float3 site_pred(const StepArgs& a, const ExchangeDev& ex, int i, int x, int y, int z, float3 Hd)
+__device__ __forceinline__ float3 site_pred(const StepArgs& a, const ExchangeDev& ex, int i, float3 Hd)
 {
-    const int n = a.d.n;
+    const uint32_t n = (uint32_t)a.d.n, u = (uint32_t)i;
     const float* __restrict__ S = a.S;
     const float* __restrict__ ms_a = a.S + 3 * (size_t)n;
-    const float ms = ms_a[i];
-    const float3 s = make_float3(S[i], S[n + i], S[2 * n + i]);
-    float3 H = exchange_field(ex, a.d, i, x, y, z, S, ms_a);
-    const float3 ha = aniso_field(a.an, i, n, s, ms);
+    const float ms = ldg(ms_a, u);
+    const float3 s = make_float3(ldg(S, u), ldg(S + n, u), ldg(S + 2 * (size_t)n, u));
+    float3 clo = make_float3(0.f, 0.f, 0.f);
+    if (COMP) clo = make_float3(ldg(a.Slo, u), ldg(a.Slo + n, u), ldg(a.Slo + 2 * (size_t)n, u));
+    float3 H = exchange_field(ex, n, u, S, ms_a, ms);
+    const float3 ha = aniso_field(a.an, i, (int)n, s, ms);
     H.x += ha.x + Hd.x; H.y += ha.y + Hd.y; H.z += ha.z + Hd.z;
     const float dt = a.dt, hdt = 0.5f * a.dt;
     float3 p = s, c = s;
-    float3 clo = make_float3(0.f, 0.f, 0.f);
-    if (COMP) clo = make_float3(a.Slo[i], a.Slo[n + i], a.Slo[2 * n + i]);
     if (ms > 0.0f) {
-        const float al = a.lp.site_alpha ? a.lp.site_alpha[i] : a.lp.alpha;
-        const float ga = a.lp.site_gamma ? a.lp.site_gamma[i] : a.lp.gamma;
+        const float al = a.lp.site_alpha ? ldg(a.lp.site_alpha, u) : a.lp.alpha;
+        const float ga = a.lp.site_gamma ? ldg(a.lp.site_gamma, u) : a.lp.gamma;
         const float3 dM = a.lp.thermal_first ? llg_rhs(s, H, H, al, ga, ms, a.lp.ps, a.lp.ds)
                                              : llg_rhs(s, H, H, al, ga, ms, 1.0f, 1.0f);
         p = make_float3(s.x + dt * dM.x, s.y + dt * dM.y, s.z + dt * dM.z);
@@ -504,9 +562,9 @@ __device__ __forceinline__ float3 site_pred(const StepArgs& a, const ExchangeDev
         if (COMP) df_renorm(c, clo, ms);
         else c = renorm(c, ms);
     }
-    a.P[i] = p.x; a.P[n + i] = p.y; a.P[2 * n + i] = p.z;
-    a.C[i] = c.x; a.C[n + i] = c.y; a.C[2 * n + i] = c.z;
-    if (COMP) { a.Clo[i] = clo.x; a.Clo[n + i] = clo.y; a.Clo[2 * n + i] = clo.z; }
+    stg(a.P, u, p.x); stg(a.P + n, u, p.y); stg(a.P + 2 * (size_t)n, u, p.z);
+    stg(a.C, u, c.x); stg(a.C + n, u, c.y); stg(a.C + 2 * (size_t)n, u, c.z);
+    if (COMP) { stg(a.Clo, u, clo.x); stg(a.Clo + n, u, clo.y); stg(a.Clo + 2 * (size_t)n, u, clo.z); }
     return p;
 }
 
@@ -514,23 +572,22 @@ __device__ __forceinline__ float3 site_pred(const StepArgs& a, const ExchangeDev
 // S' = N(C + dt/2 dM(P, H(P))), then S'' = N(S' + dt LLG(S', H_th)) when Thermal
 // is on.  Writes S (and its low word when COMP); returns S''.
 template <bool COMP>
-__device__ __forceinline__ float3 site_corr(const StepArgs& a, const ExchangeDev& ex, int i, int x, int y, int z, float3 Hd,
-                                            uint64_t draw)
+__device__ __forceinline__ float3 site_corr(const StepArgs& a, const ExchangeDev& ex, int i, float3 Hd, uint64_t draw)
 {
-    const int n = a.d.n;
+    const uint32_t n = (uint32_t)a.d.n, u = (uint32_t)i;
     const float* __restrict__ ms_a = a.S + 3 * (size_t)n;
     const float* __restrict__ P = a.P;
-    const float ms = ms_a[i];
-    const float3 p = make_float3(P[i], P[n + i], P[2 * n + i]);
-    float3 H = exchange_field(ex, a.d, i, x, y, z, P, ms_a);
-    const float3 ha = aniso_field(a.an, i, n, p, ms);
+    const float ms = ldg(ms_a, u);
+    const float3 p = make_float3(ldg(P, u), ldg(P + n, u), ldg(P + 2 * (size_t)n, u));
+    float3 v = make_float3(ldg(a.C, u), ldg(a.C + n, u), ldg(a.C + 2 * (size_t)n, u));
+    float3 vlo = make_float3(0.f, 0.f, 0.f);
+    if (COMP) vlo = make_float3(ldg(a.Clo, u), ldg(a.Clo + n, u), ldg(a.Clo + 2 * (size_t)n, u));
+    float3 H = exchange_field(ex, n, u, P, ms_a, ms);
+    const float3 ha = aniso_field(a.an, i, (int)n, p, ms);
     H.x += ha.x + Hd.x; H.y += ha.y + Hd.y; H.z += ha.z + Hd.z;
     const float dt = a.dt, hdt = 0.5f * a.dt;
-    float3 v = make_float3(a.C[i], a.C[n + i], a.C[2 * n + i]);
-    float3 vlo = make_float3(0.f, 0.f, 0.f);
-    if (COMP) vlo = make_float3(a.Clo[i], a.Clo[n + i], a.Clo[2 * n + i]);
-    const float al = a.lp.site_alpha ? a.lp.site_alpha[i] : a.lp.alpha;
-    const float ga = a.lp.site_gamma ? a.lp.site_gamma[i] : a.lp.gamma;
+    const float al = a.lp.site_alpha ? ldg(a.lp.site_alpha, u) : a.lp.alpha;
+    const float ga = a.lp.site_gamma ? ldg(a.lp.site_gamma, u) : a.lp.gamma;
     if (ms > 0.0f) {
         const float3 dM = a.lp.thermal_first ? llg_rhs(p, H, H, al, ga, ms, a.lp.ps, a.lp.ds)
                                              : llg_rhs(p, H, H, al, ga, ms, 1.0f, 1.0f);
@@ -570,8 +627,8 @@ __device__ __forceinline__ float3 site_corr(const StepArgs& a, const ExchangeDev
             else v = renorm(v, ms);
         }
     }
-    a.S[i] = v.x; a.S[n + i] = v.y; a.S[2 * n + i] = v.z;
-    if (COMP) { a.Slo[i] = vlo.x; a.Slo[n + i] = vlo.y; a.Slo[2 * n + i] = vlo.z; }
+    stg(a.S, u, v.x); stg(a.S + n, u, v.y); stg(a.S + 2 * (size_t)n, u, v.z);
+    if (COMP) { stg(a.Slo, u, vlo.x); stg(a.Slo + n, u, vlo.y); stg(a.Slo + 2 * (size_t)n, u, vlo.z); }
     return v;
 }
 

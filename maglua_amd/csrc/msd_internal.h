@@ -20,11 +20,11 @@ struct FFTPlan {
 bool make_plan(int L, FFTPlan& p);
 
 // ------------------------------------------------------------ exchange
-// Per-site class table: a class is the ordered list of (dx,dy,dz,J) couplings
-// a target site receives; neighbour = target + d with periodic wrap in every
-// dimension (the displacement is stored minimum-image).
+// Per-site class table: a class is the ordered list of couplings (dl, J) a
+// target site receives, with the source at linear index target + dl.  Sites on
+// a periodic face see wrapped offsets and so fall in classes of their own.
 struct ExEntry {
-    int8_t dx, dy, dz, pad;
+    int dl;
     float J;
 };
 struct ExchangeDev {
@@ -70,6 +70,7 @@ struct LLGDev {
 struct DipoleDev {
     int nx, ny, nz, nxh;
     int ty;                 // rows per row-tile
+    int tpb;                // threads per row-kernel workgroup
     FFTPlan px, py;
     const float2* twx;      // exp(-2 pi i k / nx), k < nx
     const float2* twy;      // exp(-2 pi i k / ny), k < ny
@@ -182,5 +183,9 @@ hipError_t configure_lds3(size_t max_bytes);
 const char* kernel_name(int id);
 // allow row/column kernels the device's full LDS per workgroup
 hipError_t configure_lds(size_t max_bytes);
+// Row-tile shapes with compile-time instances: whether (nx, ty, tpb) is one, and
+// the default (ty, tpb) for an nx x ny layer (ty = 0 when none fits).
+bool row_shape_compiled(int nx, int ty, int tpb);
+void row_shape_default(int nx, int ny, int* ty, int* tpb);
 
 }  // namespace msd

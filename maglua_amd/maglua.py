@@ -713,7 +713,9 @@ class Heun(_Handle):
     __call__ = step
 
     def profile(self, n):
-        """Kernel -> average ms per launch over n more steps (HIP events on the library stream)."""
+        """Kernel -> average ms per launch (HIP events on the library stream).  Layered and
+        local forms repeat each launch of one step n times back to back (one step of
+        progress); the LongRange3D form runs n more steps with events around every launch."""
         cap = 8
         names = (C.c_char * 32 * cap)()
         avg = (C.c_double * cap)()

@@ -391,3 +391,58 @@ def test_pc_driver_on_op_api_equals_fused_heun():
     heun.step(10)
     assert float(np.abs(flat(a["ss"].spins()) - flat(b["ss"].spins())).max()) <= 1e-6
     assert abs(a["ss"].time() - b["ss"].time()) < 1e-12
+
+
+# ------------------------------------------------------- row-tile shapes
+ROW_TILES_256 = ["8:1024", "4:512", "8:512", "2:256"]
+
+
+def _run_with_tile(spec, tile, steps, monkeypatch):
+    if tile is None:
+        monkeypatch.delenv("MSD_ROW_TILE", raising=False)
+    else:
+        monkeypatch.setenv("MSD_ROW_TILE", tile)
+    g = W.build(M, spec)
+    g["ss"].setSpins(*O.random_unit_spins(spec.seed, spec.n))
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"])
+    heun.step(steps)
+    monkeypatch.delenv("MSD_ROW_TILE", raising=False)
+    return flat(g["ss"].spins()), g
+
+
+def test_row_tile_shapes_bit_identical(monkeypatch):
+    """Every compiled row-tile shape for nx=256 pairs the same real rows per complex
+    line and runs the same butterflies per line, so all give bit-identical steps; the
+    default is also checked against the oracle."""
+    spec = W.FilmSpec(256, 64, 2, T=0.3, dt=0.004, g=0.02, K=0.05)
+    ref, _ = _run_with_tile(spec, None, 12, monkeypatch)
+    for tile in ROW_TILES_256:
+        got, _ = _run_with_tile(spec, tile, 12, monkeypatch)
+        print(f"row tile {tile}: max|diff| = {np.abs(got - ref).max():.3e}", flush=True)
+        assert np.array_equal(got, ref), tile
+    o = W.build(O, spec)
+    o["ss"].setSpins(*O.random_unit_spins(spec.seed, spec.n))
+    sim = O.Sim(o["ss"], o["ex"], o["ani"], o["dip"], o["temp"], o["llg"], seed=0)
+    sim.heun(12)
+    assert float(np.abs(ref[:3] - flat(o["ss"].spins()[:3])).max()) <= 1e-4
+
+
+def test_profile_leaves_state_one_step_ahead():
+    """profile(n) repeats each launch of one step n times; the state, time and
+    thermal draw end exactly one Heun step ahead."""
+    spec = W.FilmSpec(64, 32, 2, T=0.5, dt=0.005)
+    a = W.build(M, spec)
+    b = W.build(M, spec)
+    a["ss"].randomize(6)
+    b["ss"].randomize(6)
+    ha = M.Heun(a["ss"], a["ex"], a["ani"], a["dip"], a["temp"], a["llg"])
+    hb = M.Heun(b["ss"], b["ex"], b["ani"], b["dip"], b["temp"], b["llg"])
+    ha.step(2)
+    prof = ha.profile(5)
+    assert set(prof) == {"k_cols", "k_step_pred", "k_step_corr"}
+    assert all(v > 0 for v in prof.values())
+    ha.step(3)
+    hb.step(6)
+    assert np.array_equal(flat(a["ss"].spins()), flat(b["ss"].spins()))
+    assert a["ss"].time() == b["ss"].time()
+    assert a["temp"].draw() == b["temp"].draw() == 6
