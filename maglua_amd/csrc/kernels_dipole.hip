@@ -340,6 +340,64 @@ __global__ void __launch_bounds__(COL_TPB) k_cols(DipoleDev dp)
 }
 
 // ------------------------------------------------------------ fused Heun
+// Site loop of the fused kernels.  Compile-time shapes give each thread SPT
+// sites, software-pipelined: site 0's loads are issued before the inverse x
+// transform (their latency hides behind it), and site k+1's before site k is
+// finished.  Runtime shapes gather and finish each site in turn.
+template <int NX, int TY, int TPB, bool DIP, bool PRED, bool COMP>
+__device__ __forceinline__ void fused_sites(const StepArgs& a, const ExchangeDev& exl, float2* A, float2* B, float2*& R,
+                                            const float2* tw, int nx, int ny, int ty, int z, int y0, uint64_t draw)
+{
+    const DipoleDev& dp = a.dp;
+    auto site_of = [&](int li, int& r, int& x) {
+        r = li / nx;
+        x = li - r * nx;
+        return x + nx * (y0 + r + ny * z);
+    };
+    auto gather = [&](int i, SiteGather& g) {
+        if constexpr (PRED) pred_gather<COMP>(a, exl, i, g);
+        else corr_gather<COMP>(a, exl, i, g);
+    };
+    auto finish = [&](int li, const SiteGather& g) {
+        int r, x;
+        const int i = site_of(li, r, x);
+        float3 Hd = make_float3(0.f, 0.f, 0.f);
+        if constexpr (DIP) Hd = make_float3(rowval(R, r, x, nx), rowval(R, ty + r, x, nx), rowval(R, 2 * ty + r, x, nx));
+        float3 v;
+        if constexpr (PRED) v = pred_finish<COMP>(a, exl, i, Hd, g);
+        else v = corr_finish<COMP>(a, exl, i, Hd, draw, g);
+        if constexpr (DIP) {
+            rowval(R, r, x, nx) = v.x;
+            rowval(R, ty + r, x, nx) = v.y;
+            rowval(R, 2 * ty + r, x, nx) = v.z;
+        }
+    };
+    if constexpr (NX > 0) {
+        constexpr int SITES = TY * NX;
+        constexpr int SPT = (SITES + TPB - 1) / TPB;
+        SiteGather g[2];
+        int r, x;
+        const int li0 = (int)threadIdx.x;
+        if (SITES % TPB == 0 || li0 < SITES) gather(site_of(li0, r, x), g[0]);
+        if constexpr (DIP) R = row_fft<NX, TY, TPB, 1>(A, B, dp, tw);
+#pragma unroll
+        for (int k = 0; k < SPT; ++k) {
+            const int li = (int)threadIdx.x + k * TPB;
+            const int ln = li + TPB;
+            if (k + 1 < SPT && (SITES % TPB == 0 || ln < SITES)) gather(site_of(ln, r, x), g[(k + 1) & 1]);
+            if (SITES % TPB == 0 || li < SITES) finish(li, g[k & 1]);
+        }
+    } else {
+        if constexpr (DIP) R = row_fft<NX, TY, TPB, 1>(A, B, dp, tw);
+        for (int li = threadIdx.x; li < ty * nx; li += TPB) {
+            int r, x;
+            SiteGather g;
+            gather(site_of(li, r, x), g);
+            finish(li, g);
+        }
+    }
+}
+
 // Predictor half of the step closure in common/scripts/PredictorCorrector.lua:77-84:
 //   H(S) = Exchange + Anisotropy + Dipole,  dM = LLG(S, H(S)),
 //   P = N(S + dt dM),  C = N(S + dt/2 dM),
@@ -351,7 +409,7 @@ __global__ void __launch_bounds__(TPB) k_step_pred(StepArgs a)
     extern __shared__ float2 lds[];
     const DipoleDev& dp = a.dp;
     const int nx = NX ? NX : a.d.nx, ty = TY ? TY : a.ty, lines = row_lines(ty);
-    const int ny = a.d.ny, n = a.d.n;
+    const int ny = a.d.ny;
     const int tpl = ny / ty;
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int z = tile / tpl, y0 = (tile - z * tpl) * ty;
@@ -362,27 +420,13 @@ __global__ void __launch_bounds__(TPB) k_step_pred(StepArgs a)
     __shared__ int s_start[EX_LDS_CLS];
     __shared__ ExEntry s_ent[EX_LDS_ENT];
     const ExchangeDev exl = stage_exchange(a.ex, s_start, s_ent);
-    if constexpr (!DIP) __syncthreads();
     if constexpr (DIP) {
         tw = row_twiddles<NX>(dp, A + (size_t)lines * nx);
         rows_pack_spectrum<TPB, NX, TY>(dp.Hh, nx, ny, a.d.nz, ty, z, y0, A);
-        __syncthreads();
-        R = row_fft<NX, TY, TPB, 1>(A, B, dp, tw);
     }
+    __syncthreads();
     if (a.th.on && blockIdx.x == 0 && threadIdx.x == 0) *a.draw += 1ull;
-    (void)n;
-    for (int li = threadIdx.x; li < ty * nx; li += TPB) {
-        const int r = li / nx, x = li - r * nx, y = y0 + r;
-        const int i = x + nx * (y + ny * z);
-        float3 Hd = make_float3(0.f, 0.f, 0.f);
-        if constexpr (DIP) Hd = make_float3(rowval(R, r, x, nx), rowval(R, ty + r, x, nx), rowval(R, 2 * ty + r, x, nx));
-        const float3 p = site_pred<COMP>(a, exl, i, Hd);
-        if constexpr (DIP) {
-            rowval(R, r, x, nx) = p.x;
-            rowval(R, ty + r, x, nx) = p.y;
-            rowval(R, 2 * ty + r, x, nx) = p.z;
-        }
-    }
+    fused_sites<NX, TY, TPB, DIP, true, COMP>(a, exl, A, B, R, tw, nx, ny, ty, z, y0, 0ull);
     if constexpr (DIP) {
         __syncthreads();
         const float2* F = row_fft<NX, TY, TPB, -1>(R, R == A ? B : A, dp, tw);
@@ -399,7 +443,7 @@ __global__ void __launch_bounds__(TPB) k_step_corr(StepArgs a)
     extern __shared__ float2 lds[];
     const DipoleDev& dp = a.dp;
     const int nx = NX ? NX : a.d.nx, ty = TY ? TY : a.ty, lines = row_lines(ty);
-    const int ny = a.d.ny, n = a.d.n;
+    const int ny = a.d.ny;
     const int tpl = ny / ty;
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int z = tile / tpl, y0 = (tile - z * tpl) * ty;
@@ -410,27 +454,13 @@ __global__ void __launch_bounds__(TPB) k_step_corr(StepArgs a)
     __shared__ int s_start[EX_LDS_CLS];
     __shared__ ExEntry s_ent[EX_LDS_ENT];
     const ExchangeDev exl = stage_exchange(a.ex, s_start, s_ent);
-    if constexpr (!DIP) __syncthreads();
     if constexpr (DIP) {
         tw = row_twiddles<NX>(dp, A + (size_t)lines * nx);
         rows_pack_spectrum<TPB, NX, TY>(dp.Hh, nx, ny, a.d.nz, ty, z, y0, A);
-        __syncthreads();
-        R = row_fft<NX, TY, TPB, 1>(A, B, dp, tw);
     }
+    __syncthreads();
     const uint64_t draw = a.th.on ? (uint64_t)*a.draw : 0ull;
-    (void)n;
-    for (int li = threadIdx.x; li < ty * nx; li += TPB) {
-        const int r = li / nx, x = li - r * nx, y = y0 + r;
-        const int i = x + nx * (y + ny * z);
-        float3 Hd = make_float3(0.f, 0.f, 0.f);
-        if constexpr (DIP) Hd = make_float3(rowval(R, r, x, nx), rowval(R, ty + r, x, nx), rowval(R, 2 * ty + r, x, nx));
-        const float3 v = site_corr<COMP>(a, exl, i, Hd, draw);
-        if constexpr (DIP) {
-            rowval(R, r, x, nx) = v.x;
-            rowval(R, ty + r, x, nx) = v.y;
-            rowval(R, 2 * ty + r, x, nx) = v.z;
-        }
-    }
+    fused_sites<NX, TY, TPB, DIP, false, COMP>(a, exl, A, B, R, tw, nx, ny, ty, z, y0, draw);
     if constexpr (DIP) {
         __syncthreads();
         const float2* F = row_fft<NX, TY, TPB, -1>(R, R == A ? B : A, dp, tw);

@@ -8,15 +8,16 @@ namespace msd {
 
 // ------------------------------------------------------------------ Philox
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): 10 rounds of two 32x32->64
-// multiplies with Weyl key increments.
+// multiplies with Weyl key increments.  Each product is one v_mad_u64_u32 (both
+// halves at once) rather than a separate low and high multiply.
 __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k)
 {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         if (r) { k.x += 0x9E3779B9u; k.y += 0xBB67AE85u; }
-        const uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
-        const uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
-        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        c = make_uint4((uint32_t)(p1 >> 32) ^ c.y ^ k.x, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k.y, (uint32_t)p0);
     }
     return c;
 }
@@ -24,18 +25,19 @@ __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k)
 // top 24 bits -> (0,1), exact in FP32
 __device__ __forceinline__ float u24f(uint32_t a) { return ((float)(a >> 8) + 0.5f) * (1.0f / 16777216.0f); }
 
-// Three standard normals for (seed, draw, site): Box-Muller on the four words.
+// Three standard normals for (seed, draw, site): Box-Muller on the four words,
+// r = sqrt(-2 ln u0), (r cos 2 pi u1, r sin 2 pi u1), r' cos 2 pi u3 with r' from u2.
+// The transcendentals are the single hardware instructions: v_log_f32 (log2),
+// v_sqrt_f32, and v_sin_f32 / v_cos_f32, whose argument is in revolutions.
 __device__ __forceinline__ float3 thermal_normals(uint64_t seed, uint64_t draw, int site)
 {
     const uint4 r = philox4x32_10(make_uint4((uint32_t)site, 0u, (uint32_t)draw, (uint32_t)(draw >> 32)),
                                   make_uint2((uint32_t)seed, (uint32_t)(seed >> 32)));
-    const float r0 = sqrtf(-2.0f * logf(u24f(r.x)));
-    const float r1 = sqrtf(-2.0f * logf(u24f(r.z)));
-    float s0, c0, s1, c1;
-    sincospif(2.0f * u24f(r.y), &s0, &c0);
-    sincospif(2.0f * u24f(r.w), &s1, &c1);
-    (void)s1;
-    return make_float3(r0 * c0, r0 * s0, r1 * c1);
+    const float m2ln2 = -1.38629436111989061883f;   // -2 ln 2: -2 ln u = -2 ln2 log2 u
+    const float r0 = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u24f(r.x)));
+    const float r1 = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u24f(r.z)));
+    const float a0 = u24f(r.y), a1 = u24f(r.w);
+    return make_float3(r0 * __builtin_amdgcn_cosf(a0), r0 * __builtin_amdgcn_sinf(a0), r1 * __builtin_amdgcn_cosf(a1));
 }
 
 // ---------------------------------------------------------------- complex
@@ -464,40 +466,52 @@ __device__ __forceinline__ float thermal_sigma(const ThermalDev& th, const LLGDe
 }
 
 // Exchange field at target i (cpu/exchange/spinoperationexchange.cpp:134-213):
-// sum_j J_j s_f / m_f over the site's couplings, then / m_i when normalising
-// (mi = m_i).  Class-table entries carry the source as a linear offset f - i.
-__device__ __forceinline__ float3 exchange_field(const ExchangeDev& ex, uint32_t n, uint32_t i,
-                                                 const float* __restrict__ s, const float* __restrict__ m, float mi)
+// sum_j J_j s_f / m_f over the site's couplings, then / m_i when normalising.
+// Class-table entries carry the source as a linear offset f - i.  The field is
+// built in two parts so a caller can issue one site's loads (exchange_gather)
+// before it finishes another (exchange_finish): the first 8 couplings of a
+// class are loaded by the gather, further couplings and CSR rows by the finish.
+struct ExGather {
+    float vx[8], vy[8], vz[8], mf[8];
+    int e0, cnt;
+};
+
+__device__ __forceinline__ void exchange_gather(const ExchangeDev& ex, uint32_t n, uint32_t i, const float* __restrict__ s,
+                                                const float* __restrict__ m, ExGather& g)
+{
+    g.e0 = 0;
+    g.cnt = 0;
+    if (ex.mode != 1) return;
+    const int c = ex.cls ? (int)ldg(ex.cls, i) : 0;
+    g.e0 = ex.cls_start[c];
+    g.cnt = ex.cls_start[c + 1] - g.e0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        g.mf[u] = 0.0f;
+        if (u < g.cnt) {
+            const uint32_t f = i + (uint32_t)ex.ent[g.e0 + u].dl;
+            g.mf[u] = ldg(m, f);
+            g.vx[u] = ldg(s, f); g.vy[u] = ldg(s + n, f); g.vz[u] = ldg(s + 2 * (size_t)n, f);
+        }
+    }
+}
+
+__device__ __forceinline__ float3 exchange_finish(const ExchangeDev& ex, uint32_t n, uint32_t i, const float* __restrict__ s,
+                                                  const float* __restrict__ m, float mi, const ExGather& g)
 {
     const float* __restrict__ sx = s;
     const float* __restrict__ sy = s + n;
     const float* __restrict__ sz = s + 2 * (size_t)n;
     float3 h = make_float3(0.f, 0.f, 0.f);
     if (ex.mode == 1) {
-        const int c = ex.cls ? (int)ldg(ex.cls, i) : 0;
-        const int e0 = ex.cls_start[c];
-        const int cnt = ex.cls_start[c + 1] - e0;
-        // first 8 couplings unrolled so their loads are in flight together
-        float vx[8], vy[8], vz[8], mf[8], J[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            mf[u] = 0.0f;
-            if (u < cnt) {
-                const ExEntry E = ex.ent[e0 + u];
-                const uint32_t f = i + (uint32_t)E.dl;
-                J[u] = E.J;
-                mf[u] = ldg(m, f);
-                vx[u] = ldg(sx, f); vy[u] = ldg(sy, f); vz[u] = ldg(sz, f);
+            if (g.mf[u] != 0.0f) {
+                const float w = ex.ent[g.e0 + u].J * rcpf(g.mf[u]);
+                h.x += g.vx[u] * w; h.y += g.vy[u] * w; h.z += g.vz[u] * w;
             }
         }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            if (mf[u] != 0.0f) {
-                const float w = J[u] * rcpf(mf[u]);
-                h.x += vx[u] * w; h.y += vy[u] * w; h.z += vz[u] * w;
-            }
-        }
-        for (int e = e0 + 8; e < e0 + cnt; ++e) {
+        for (int e = g.e0 + 8; e < g.e0 + g.cnt; ++e) {
             const ExEntry E = ex.ent[e];
             const uint32_t f = i + (uint32_t)E.dl;
             const float mfe = ldg(m, f);
@@ -525,20 +539,38 @@ __device__ __forceinline__ float3 exchange_field(const ExchangeDev& ex, uint32_t
 }
 
 // ------------------------------------------------------ Heun site updates
+// Each half of the step at a site is split the same way: *_gather issues the
+// site's loads into a SiteGather, *_finish computes and stores.
+struct SiteGather {
+    float3 v;       // predictor: S; corrector: P
+    float3 c;       // corrector: C
+    float3 lo;      // predictor: S_lo; corrector: C_lo
+    float ms;
+    ExGather ex;
+};
+
 // Predictor at site i (PredictorCorrector.lua:77-79 with LLGCartesian::apply):
 // H = exchange + anisotropy + Hd at S; P = N(S + dt dM), C = N(S + dt/2 dM).
-// Writes P, C (and C's low word when COMP); returns P for the next transform.
 template <bool COMP>
-__device__ __forceinline__ float3 site_pred(const StepArgs& a, const ExchangeDev& ex, int i, float3 Hd)
+__device__ __forceinline__ void pred_gather(const StepArgs& a, const ExchangeDev& ex, int i, SiteGather& g)
 {
     const uint32_t n = (uint32_t)a.d.n, u = (uint32_t)i;
     const float* __restrict__ S = a.S;
-    const float* __restrict__ ms_a = a.S + 3 * (size_t)n;
-    const float ms = ldg(ms_a, u);
-    const float3 s = make_float3(ldg(S, u), ldg(S + n, u), ldg(S + 2 * (size_t)n, u));
-    float3 clo = make_float3(0.f, 0.f, 0.f);
-    if (COMP) clo = make_float3(ldg(a.Slo, u), ldg(a.Slo + n, u), ldg(a.Slo + 2 * (size_t)n, u));
-    float3 H = exchange_field(ex, n, u, S, ms_a, ms);
+    g.ms = ldg(S + 3 * (size_t)n, u);
+    g.v = make_float3(ldg(S, u), ldg(S + n, u), ldg(S + 2 * (size_t)n, u));
+    if (COMP) g.lo = make_float3(ldg(a.Slo, u), ldg(a.Slo + n, u), ldg(a.Slo + 2 * (size_t)n, u));
+    exchange_gather(ex, n, u, S, S + 3 * (size_t)n, g.ex);
+}
+
+// Writes P, C (and C's low word when COMP); returns P for the next transform.
+template <bool COMP>
+__device__ __forceinline__ float3 pred_finish(const StepArgs& a, const ExchangeDev& ex, int i, float3 Hd, const SiteGather& g)
+{
+    const uint32_t n = (uint32_t)a.d.n, u = (uint32_t)i;
+    const float ms = g.ms;
+    const float3 s = g.v;
+    float3 clo = COMP ? g.lo : make_float3(0.f, 0.f, 0.f);
+    float3 H = exchange_finish(ex, n, u, a.S, a.S + 3 * (size_t)n, ms, g.ex);
     const float3 ha = aniso_field(a.an, i, (int)n, s, ms);
     H.x += ha.x + Hd.x; H.y += ha.y + Hd.y; H.z += ha.z + Hd.z;
     const float dt = a.dt, hdt = 0.5f * a.dt;
@@ -568,21 +600,40 @@ __device__ __forceinline__ float3 site_pred(const StepArgs& a, const ExchangeDev
     return p;
 }
 
+template <bool COMP>
+__device__ __forceinline__ float3 site_pred(const StepArgs& a, const ExchangeDev& ex, int i, float3 Hd)
+{
+    SiteGather g;
+    pred_gather<COMP>(a, ex, i, g);
+    return pred_finish<COMP>(a, ex, i, Hd, g);
+}
+
 // Corrector and thermal kick at site i (PredictorCorrector.lua:81-85,98-103):
 // S' = N(C + dt/2 dM(P, H(P))), then S'' = N(S' + dt LLG(S', H_th)) when Thermal
-// is on.  Writes S (and its low word when COMP); returns S''.
+// is on.
 template <bool COMP>
-__device__ __forceinline__ float3 site_corr(const StepArgs& a, const ExchangeDev& ex, int i, float3 Hd, uint64_t draw)
+__device__ __forceinline__ void corr_gather(const StepArgs& a, const ExchangeDev& ex, int i, SiteGather& g)
 {
     const uint32_t n = (uint32_t)a.d.n, u = (uint32_t)i;
-    const float* __restrict__ ms_a = a.S + 3 * (size_t)n;
     const float* __restrict__ P = a.P;
-    const float ms = ldg(ms_a, u);
-    const float3 p = make_float3(ldg(P, u), ldg(P + n, u), ldg(P + 2 * (size_t)n, u));
-    float3 v = make_float3(ldg(a.C, u), ldg(a.C + n, u), ldg(a.C + 2 * (size_t)n, u));
-    float3 vlo = make_float3(0.f, 0.f, 0.f);
-    if (COMP) vlo = make_float3(ldg(a.Clo, u), ldg(a.Clo + n, u), ldg(a.Clo + 2 * (size_t)n, u));
-    float3 H = exchange_field(ex, n, u, P, ms_a, ms);
+    g.ms = ldg(a.S + 3 * (size_t)n, u);
+    g.v = make_float3(ldg(P, u), ldg(P + n, u), ldg(P + 2 * (size_t)n, u));
+    g.c = make_float3(ldg(a.C, u), ldg(a.C + n, u), ldg(a.C + 2 * (size_t)n, u));
+    if (COMP) g.lo = make_float3(ldg(a.Clo, u), ldg(a.Clo + n, u), ldg(a.Clo + 2 * (size_t)n, u));
+    exchange_gather(ex, n, u, P, a.S + 3 * (size_t)n, g.ex);
+}
+
+// Writes S (and its low word when COMP); returns S''.
+template <bool COMP>
+__device__ __forceinline__ float3 corr_finish(const StepArgs& a, const ExchangeDev& ex, int i, float3 Hd, uint64_t draw,
+                                              const SiteGather& g)
+{
+    const uint32_t n = (uint32_t)a.d.n, u = (uint32_t)i;
+    const float ms = g.ms;
+    const float3 p = g.v;
+    float3 v = g.c;
+    float3 vlo = COMP ? g.lo : make_float3(0.f, 0.f, 0.f);
+    float3 H = exchange_finish(ex, n, u, a.P, a.S + 3 * (size_t)n, ms, g.ex);
     const float3 ha = aniso_field(a.an, i, (int)n, p, ms);
     H.x += ha.x + Hd.x; H.y += ha.y + Hd.y; H.z += ha.z + Hd.z;
     const float dt = a.dt, hdt = 0.5f * a.dt;
@@ -608,8 +659,8 @@ __device__ __forceinline__ float3 site_corr(const StepArgs& a, const ExchangeDev
         if (ms > 0.0f) {
             float3 Ht = make_float3(0.f, 0.f, 0.f);
             if (sg != 0.0f) {
-                const float3 g = thermal_normals(a.th.seed, draw, i);
-                Ht = make_float3(sg * g.x, sg * g.y, sg * g.z);
+                const float3 gn = thermal_normals(a.th.seed, draw, i);
+                Ht = make_float3(sg * gn.x, sg * gn.y, sg * gn.z);
             }
             const float3 zero = make_float3(0.f, 0.f, 0.f);
             const float3 dM = a.lp.thermal_first ? llg_rhs(v, Ht, zero, al, ga, ms, a.lp.ps, a.lp.ds)
@@ -630,6 +681,14 @@ __device__ __forceinline__ float3 site_corr(const StepArgs& a, const ExchangeDev
     stg(a.S, u, v.x); stg(a.S + n, u, v.y); stg(a.S + 2 * (size_t)n, u, v.z);
     if (COMP) { stg(a.Slo, u, vlo.x); stg(a.Slo + n, u, vlo.y); stg(a.Slo + 2 * (size_t)n, u, vlo.z); }
     return v;
+}
+
+template <bool COMP>
+__device__ __forceinline__ float3 site_corr(const StepArgs& a, const ExchangeDev& ex, int i, float3 Hd, uint64_t draw)
+{
+    SiteGather g;
+    corr_gather<COMP>(a, ex, i, g);
+    return corr_finish<COMP>(a, ex, i, Hd, draw, g);
 }
 
 }  // namespace msd
