@@ -81,6 +81,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--plain", action="store_true", help="plain FP32 spin state instead of the compensated pair")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="after timing, copy the spin system once on the device (two copy kernels of 8 MiB and "
+                         "6 MiB) so a counter run can scale FETCH_SIZE / WRITE_SIZE by a copy of known size")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -113,6 +116,9 @@ def main():
     # events on the library stream (the stream every kernel runs on)
     prof = heun.profile(args.profile_steps)
     M.synchronize()
+    if args.calibrate:
+        g["ss"].copy()
+        M.synchronize()
     kb = kernel_bytes(spec, has_cls=True, comp=not args.plain)
     per_step = {k: v * (2 if k == "k_cols" else 1) for k, v in prof.items()}
     dom = max(per_step, key=per_step.get)

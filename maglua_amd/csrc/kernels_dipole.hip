@@ -28,11 +28,13 @@ static constexpr int COL_TPB = 512;
 // exchange class tables up to this size are staged in LDS by the fused kernels
 static constexpr int EX_LDS_CLS = 64, EX_LDS_ENT = 512;
 
-// workgroups on the same XCD (hardware assigns bid % 8) get contiguous tiles
+// Workgroups on the same XCD (the dispatcher deals bid % 8 to XCD bid % 8) get
+// contiguous tiles: XCD r takes the r-th run of nb/8 (+1 for the first nb % 8)
+// consecutive tiles.  A bijection of [0, nb) for any nb.
 __device__ __forceinline__ int xcd_remap(int bid, int nb)
 {
-    if (nb & 7) return bid;
-    return (bid & 7) * (nb >> 3) + (bid >> 3);
+    const int r = bid & 7, q = bid >> 3, base = nb >> 3, rem = nb & 7;
+    return r * base + (r < rem ? r : rem) + q;
 }
 
 // two real rows per complex line: real row q -> line q/2, component q&1
@@ -145,9 +147,9 @@ __global__ void __launch_bounds__(TPB) k_rows_fwd(DipoleDev dp, const float* __r
 {
     extern __shared__ float2 lds[];
     const int nx = NX ? NX : dp.nx, ty = TY ? TY : dp.ty, lines = row_lines(ty), ny = dp.ny;
-    const int tpl = ny / ty;
+    // layer fastest: an XCD's run of tiles is a band of rows through every layer
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
-    const int z = tile / tpl, y0 = (tile - z * tpl) * ty;
+    const int yb = tile / dp.nz, z = tile - yb * dp.nz, y0 = yb * ty;
     const int n = nx * ny * dp.nz;
     float2* A = lds;
     float2* B = lds + (size_t)lines * nx;
@@ -173,9 +175,9 @@ __global__ void __launch_bounds__(TPB) k_rows_inv(DipoleDev dp, float* __restric
 {
     extern __shared__ float2 lds[];
     const int nx = NX ? NX : dp.nx, ty = TY ? TY : dp.ty, lines = row_lines(ty), ny = dp.ny;
-    const int tpl = ny / ty;
+    // layer fastest: an XCD's run of tiles is a band of rows through every layer
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
-    const int z = tile / tpl, y0 = (tile - z * tpl) * ty;
+    const int yb = tile / dp.nz, z = tile - yb * dp.nz, y0 = yb * ty;
     const int n = nx * ny * dp.nz;
     float2* A = lds;
     float2* B = lds + (size_t)lines * nx;
@@ -199,8 +201,11 @@ template <int NY, int NZ>
 __global__ void __launch_bounds__(COL_TPB) k_cols(DipoleDev dp)
 {
     extern __shared__ float2 lds[];
-    const int kx = blockIdx.x / 3;
-    const int oa = blockIdx.x - 3 * kx;
+    // the three workgroups of one kx run on one XCD, so two of them find the
+    // column in that XCD's L2
+    const int wg = xcd_remap(blockIdx.x, gridDim.x);
+    const int kx = wg / 3;
+    const int oa = wg - 3 * kx;
     const int ny = NY ? NY : dp.ny, nxh = dp.nxh;
     constexpr int lines = 3 * NZ;
     // LDS: lines (one buffer in place, two ping-pong for runtime shapes) and the y twiddles
@@ -410,9 +415,10 @@ __global__ void __launch_bounds__(TPB) k_step_pred(StepArgs a)
     const DipoleDev& dp = a.dp;
     const int nx = NX ? NX : a.d.nx, ty = TY ? TY : a.ty, lines = row_lines(ty);
     const int ny = a.d.ny;
-    const int tpl = ny / ty;
+    // layer fastest: an XCD's run of tiles is a band of rows through every
+    // layer, so the z neighbours of its sites are mostly in its own L2
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
-    const int z = tile / tpl, y0 = (tile - z * tpl) * ty;
+    const int yb = tile / a.d.nz, z = tile - yb * a.d.nz, y0 = yb * ty;
     float2* A = lds;
     float2* B = lds + (size_t)lines * nx;
     float2* R = A;
@@ -444,9 +450,10 @@ __global__ void __launch_bounds__(TPB) k_step_corr(StepArgs a)
     const DipoleDev& dp = a.dp;
     const int nx = NX ? NX : a.d.nx, ty = TY ? TY : a.ty, lines = row_lines(ty);
     const int ny = a.d.ny;
-    const int tpl = ny / ty;
+    // layer fastest: an XCD's run of tiles is a band of rows through every
+    // layer, so the z neighbours of its sites are mostly in its own L2
     const int tile = xcd_remap(blockIdx.x, gridDim.x);
-    const int z = tile / tpl, y0 = (tile - z * tpl) * ty;
+    const int yb = tile / a.d.nz, z = tile - yb * a.d.nz, y0 = yb * ty;
     float2* A = lds;
     float2* B = lds + (size_t)lines * nx;
     float2* R = A;
