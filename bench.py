@@ -19,6 +19,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X published HBM3E bandwidth
 
+# HBM bytes per dispatch of each fused kernel on this workload (compensated state),
+# from two rocprofv3 counter runs: FETCH_SIZE x 1.9925 + WRITE_SIZE x 1.000, the
+# factors being what the counters read for a device copy of known size
+# (profiles/heun_film256x256x8_kernels.md has the commands and per-kernel figures).
+MEASURED_TRAFFIC = {"k_cols": 19.30e6, "k_step_pred": 47.44e6, "k_step_corr": 47.48e6}
+
 
 def kernel_bytes(spec, has_cls, comp):
     """Algorithmic HBM bytes per launch: every compulsory FP32 array read or written
@@ -124,7 +130,8 @@ def main():
     dom = max(per_step, key=per_step.get)
     achieved = kb[dom] / (prof[dom] * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None if args.plain else MEASURED_TRAFFIC.get(dom),
             "kernel": dom, "kernel_avg_ms": round(prof[dom], 5), "algorithmic_bytes": kb[dom],
             "step_model_bytes_per_spin": round(step_model_bytes(spec, not args.plain), 2),
             "step_model_GBs": round(step_model_bytes(spec, not args.plain) * value / 1e9, 1),
