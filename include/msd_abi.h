@@ -228,6 +228,49 @@ int msd_stepper_profile(msd_stepper* st, int nsteps, int cap, char names[][32], 
  * over all replicas.  No reference counterpart: MagLua loops
  * SpinOperation::apply over same-sized systems (cpu/core/spinoperation.cpp:288-295).
  * Members must outlive the ensemble. */
+/* ---------------------------------------------------------------- surrogate
+ * No counterpart in the reference: a learned predictor of the layered dipole
+ * field's change between exact refreshes (the field itself is LongRange::apply,
+ * cpu/longrange/spinoperationlongrange.cpp:354-430).  Three 5x5x3 convolution
+ * layers, 3 -> 16 -> 16 -> 3 channels, no bias, identity activation, periodic in
+ * x and y and zero-padded in z, BF16 MFMA with FP32 accumulation:
+ *   dH = s_out * L3(L2(L1(s_in * (m - m_ref)))).
+ * Weights of layer l are [75 taps][cout][cin] doubles, tap = (dz+1)*25 + (dy+2)*5
+ * + (dx+2) for offsets dz in -1..1, dy and dx in -2..2.  nx must be a multiple
+ * of 32.  Training minimises mean((dH - target)^2) / sigma2 with Adam. */
+typedef struct msd_surrogate msd_surrogate;
+int msd_surrogate_create(int nx, int ny, int nz, msd_surrogate** out);
+int msd_surrogate_destroy(msd_surrogate* s);
+int msd_surrogate_set_weights(msd_surrogate* s, int layer, const double* w);
+int msd_surrogate_get_weights(const msd_surrogate* s, int layer, double* w);
+int msd_surrogate_set_scales(msd_surrogate* s, double s_in, double s_out, double sigma2);
+int msd_surrogate_get_scales(const msd_surrogate* s, double* s_in, double* s_out, double* sigma2);
+/* input dm given on the host ([3][n] SoA), or as m - m_ref of two systems */
+int msd_surrogate_set_input(msd_surrogate* s, const double* dm);
+int msd_surrogate_set_input_systems(msd_surrogate* s, const msd_system* m, const msd_system* m_ref);
+/* training target: given on the host ([3][n]), or the exact Dipole field of m - m_ref */
+int msd_surrogate_set_target(msd_surrogate* s, const double* t);
+int msd_surrogate_set_target_dipole(msd_surrogate* s, msd_dipole* d, const msd_system* m, const msd_system* m_ref);
+/* dH of the current input; out ([3][n]) may be NULL */
+int msd_surrogate_forward(msd_surrogate* s, double* out);
+/* loss of the current input/target and the weight gradients (msd_surrogate_get_grads) */
+int msd_surrogate_loss_grads(msd_surrogate* s, double* loss);
+int msd_surrogate_get_grads(const msd_surrogate* s, int layer, double* g);
+int msd_surrogate_adam(msd_surrogate* s, double lr);
+int msd_surrogate_reset_optimizer(msd_surrogate* s);
+int msd_surrogate_train_step(msd_surrogate* s, double lr, double* loss);
+/* max|H_ref + dH(m - m_ref) - H(m)| / max|H(m)| with the exact Dipole fields,
+ * and (frozen_err, may be NULL) the same for H_ref alone */
+int msd_surrogate_field_error(msd_surrogate* s, msd_dipole* d, const msd_system* m, const msd_system* m_ref,
+                              double* rel_err, double* frozen_err);
+/* Learned dipole field in the step driver (a layered Dipole driver): windows of
+ * K steps, each started by an exact field evaluation that becomes the reference
+ * and checks the surrogate's prediction for that state; a window whose check
+ * exceeds tol runs exact steps.  sur = NULL returns to exact steps. */
+int msd_stepper_set_surrogate(msd_stepper* st, msd_surrogate* sur, int K, double tol);
+int msd_stepper_surrogate_stats(const msd_stepper* st, long* windows, long* exact_windows, long* fallbacks,
+                                double* last_residual, double* max_residual);
+
 typedef struct msd_ensemble msd_ensemble;
 int msd_ensemble_create(msd_stepper* const* members, int count, msd_ensemble** out);
 int msd_ensemble_destroy(msd_ensemble* e);

@@ -43,6 +43,7 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _u32p = C.POINTER(C.c_uint32)
 _u64p = C.POINTER(C.c_uint64)
+_lp = C.POINTER(C.c_long)
 
 _SIGS = {
     "msd_last_error": (C.c_char_p, []),
@@ -130,6 +131,25 @@ _SIGS = {
     "msd_stepper_set_compensated": (C.c_int, [_vp, C.c_int]),
     "msd_stepper_heun": (C.c_int, [_vp, C.c_int]),
     "msd_stepper_profile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_void_p, _dp, _ip]),
+    "msd_stepper_set_surrogate": (C.c_int, [_vp, _vp, C.c_int, C.c_double]),
+    "msd_stepper_surrogate_stats": (C.c_int, [_vp, _lp, _lp, _lp, _dp, _dp]),
+    "msd_surrogate_create": (C.c_int, [C.c_int, C.c_int, C.c_int, _pp]),
+    "msd_surrogate_destroy": (C.c_int, [_vp]),
+    "msd_surrogate_set_weights": (C.c_int, [_vp, C.c_int, _dp]),
+    "msd_surrogate_get_weights": (C.c_int, [_vp, C.c_int, _dp]),
+    "msd_surrogate_set_scales": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double]),
+    "msd_surrogate_get_scales": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "msd_surrogate_set_input": (C.c_int, [_vp, _dp]),
+    "msd_surrogate_set_input_systems": (C.c_int, [_vp, _vp, _vp]),
+    "msd_surrogate_set_target": (C.c_int, [_vp, _dp]),
+    "msd_surrogate_set_target_dipole": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "msd_surrogate_forward": (C.c_int, [_vp, _dp]),
+    "msd_surrogate_loss_grads": (C.c_int, [_vp, _dp]),
+    "msd_surrogate_get_grads": (C.c_int, [_vp, C.c_int, _dp]),
+    "msd_surrogate_adam": (C.c_int, [_vp, C.c_double]),
+    "msd_surrogate_reset_optimizer": (C.c_int, [_vp]),
+    "msd_surrogate_train_step": (C.c_int, [_vp, C.c_double, _dp]),
+    "msd_surrogate_field_error": (C.c_int, [_vp, _vp, _vp, _vp, _dp, _dp]),
 }
 
 

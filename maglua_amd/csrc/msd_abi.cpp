@@ -893,6 +893,9 @@ struct msd_dipole {
     int gmax[4] = {1000, 1000, 1000, 1000};
     double ABC[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     bool loaded = false, custom = false, spectrum = false;
+    // bumped by every op-API transform through this object's spectrum buffers, so a
+    // step driver sharing them knows its cached spin spectrum was overwritten
+    uint64_t epoch = 0;
     double words_mult = NAN;
     double* T64 = nullptr;    // [6][2nz-1][ny][nx]
     double2* U64 = nullptr;
@@ -1133,10 +1136,24 @@ extern "C" int msd_dipole_apply(msd_dipole* d, msd_system* s)
     if (int rc = d->prepare()) return rc;
     int k;
     if (int rc = s->mark("Dipole", &k)) return rc;
+    d->epoch++;
     hipError_t e = launch_rows_fwd(d->dev, s->S, G.stream);
     if (e == hipSuccess) e = launch_cols(d->dev, G.stream);
     if (e == hipSuccess) e = launch_rows_inv(d->dev, s->slots[k].h, G.stream);
     if (e != hipSuccess) return fail(MSD_EDEVICE, "Dipole:apply: %s", hipGetErrorString(e));
+    return MSD_OK;
+}
+
+// dipole field of arbitrary device spins s3 ([3][n]) into h ([3][n]): the layered
+// transform of the Dipole operator without a SpinSystem
+static int dipole_field_of(msd_dipole* dip, const float* s3, float* h)
+{
+    if (int rc = dip->prepare()) return rc;
+    dip->epoch++;
+    hipError_t e = launch_rows_fwd(dip->dev, s3, G.stream);
+    if (e == hipSuccess) e = launch_cols(dip->dev, G.stream);
+    if (e == hipSuccess) e = launch_rows_inv(dip->dev, h, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "dipole field: %s", hipGetErrorString(e));
     return MSD_OK;
 }
 
@@ -1485,6 +1502,319 @@ struct Batch3 {
 // convolution (LongRange3D); a stepper of the 3D form is a batch of one.
 enum StepKind { SK_LOCAL = 0, SK_LAYERED = 1, SK_CONV3 = 2 };
 
+// ============================================================ surrogate
+// Dipolar-field surrogate: three 5x5x3 convolution layers (3 -> 16 -> 16 -> 3
+// channels, BF16 on MFMA, FP32 accumulate) mapping s_in (m - m_ref) to the
+// change of the layered dipole field, dH = s_out Y3.  FP32 master weights
+// [layer][tap][out][in] (16 x 16 per tap, zero beyond a layer's channels) and
+// their Adam moments live on the device; the packed MFMA fragments of every
+// layer and of its adjoint are rebuilt after each update.
+struct msd_surrogate {
+    Dims d{};
+    CnnDims cd{};
+    int grid = 0;
+    static constexpr int NT = 75, WSZ = 75 * 256, PACK = 38 * 64 * 8;
+    float *W = nullptr, *Mo = nullptr, *Vo = nullptr, *gW = nullptr;     // [3][WSZ]
+    uint16_t *Af = nullptr, *At = nullptr;                               // [3][PACK]
+    uint16_t *X0 = nullptr, *X1 = nullptr, *X2 = nullptr;                // activations [n][16]
+    uint16_t *G1 = nullptr, *G2 = nullptr, *G3 = nullptr;                // back-propagated [n][16]
+    float *Y3 = nullptr, *T = nullptr, *D = nullptr;                     // [3][n]
+    float* partial = nullptr;                                            // [grid][WSZ]
+    double* lpart = nullptr;
+    int nlpart = 0;
+    double s_in = 1.0, s_out = 1.0, sigma2 = 1.0;
+    int adam_t = 0;
+    bool packed = false;
+    ~msd_surrogate()
+    {
+        (void)hipStreamSynchronize(G.stream);
+        dfree(W); dfree(Mo); dfree(Vo); dfree(gW); dfree(Af); dfree(At);
+        dfree(X0); dfree(X1); dfree(X2); dfree(G1); dfree(G2); dfree(G3);
+        dfree(Y3); dfree(T); dfree(D); dfree(partial); dfree(lpart);
+    }
+    static int cin(int l) { return l == 0 ? 3 : 16; }
+    static int cout(int l) { return l == 2 ? 3 : 16; }
+    int pack()
+    {
+        if (packed) return MSD_OK;
+        for (int l = 0; l < 3; ++l) {
+            hipError_t e = launch_cnn_pack_weights(W + (size_t)l * WSZ, 0, Af + (size_t)l * PACK, G.stream);
+            if (e == hipSuccess) e = launch_cnn_pack_weights(W + (size_t)l * WSZ, 1, At + (size_t)l * PACK, G.stream);
+            if (e != hipSuccess) return fail(MSD_EDEVICE, "surrogate: %s", hipGetErrorString(e));
+        }
+        packed = true;
+        return MSD_OK;
+    }
+    // X0 -> Y3 = s_out * L3(L2(L1(X0))) + href (FP32 [3][n] into out)
+    int forward(float* out, const float* href, hipStream_t st)
+    {
+        if (int rc = pack()) return rc;
+        hipError_t e = launch_cnn_layer(cd, grid, X0, Af, 0, X1, nullptr, nullptr, 1.0f, st);
+        if (e == hipSuccess) e = launch_cnn_layer(cd, grid, X1, Af + PACK, 0, X2, nullptr, nullptr, 1.0f, st);
+        if (e == hipSuccess) e = launch_cnn_layer(cd, grid, X2, Af + 2 * PACK, 1, nullptr, out, href, (float)s_out, st);
+        if (e != hipSuccess) return fail(MSD_EDEVICE, "surrogate forward: %s", hipGetErrorString(e));
+        return MSD_OK;
+    }
+};
+
+static int surrogate_dims_ok(int nx, int ny, int nz)
+{
+    if (nx < 32 || nx % 32) return fail(MSD_EUNSUPPORTED, "Surrogate.new: nx=%d must be a multiple of 32", nx);
+    if (ny < 1 || nz < 1) return fail(MSD_EINVAL, "Surrogate.new: bad size");
+    if ((double)nx * ny * nz > (double)(1u << 27)) return fail(MSD_EUNSUPPORTED, "Surrogate.new: more than 2^27 sites");
+    return MSD_OK;
+}
+
+extern "C" int msd_surrogate_create(int nx, int ny, int nz, msd_surrogate** out)
+{
+    NONNULL(out);
+    *out = nullptr;
+    if (int rc = surrogate_dims_ok(nx, ny, nz)) return rc;
+    if (int rc = ensure_init()) return rc;
+    std::unique_ptr<msd_surrogate> s(new msd_surrogate);
+    s->d = Dims{nx, ny, nz, nx * ny * nz};
+    CnnDims& c = s->cd;
+    c.nx = nx; c.ny = ny; c.nz = nz; c.n = s->d.n;
+    c.tiles_x = nx / 32; c.tiles_y = (ny + 3) / 4; c.ntiles = c.tiles_x * c.tiles_y * nz;
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, G.device));
+    s->grid = std::min(c.ntiles, 2 * prop.multiProcessorCount);
+    const size_t n = s->d.n, W3 = (size_t)3 * msd_surrogate::WSZ;
+    for (float** p : {&s->W, &s->Mo, &s->Vo, &s->gW})
+        if (int rc = dalloc(p, W3)) return rc;
+    if (int rc = dalloc(&s->Af, (size_t)3 * msd_surrogate::PACK)) return rc;
+    if (int rc = dalloc(&s->At, (size_t)3 * msd_surrogate::PACK)) return rc;
+    for (uint16_t** p : {&s->X0, &s->X1, &s->X2, &s->G1, &s->G2, &s->G3})
+        if (int rc = dalloc(p, 16 * n)) return rc;
+    for (float** p : {&s->Y3, &s->T, &s->D})
+        if (int rc = dalloc(p, 3 * n)) return rc;
+    if (int rc = dalloc(&s->partial, (size_t)s->grid * msd_surrogate::WSZ)) return rc;
+    s->nlpart = (int)((n + 255) / 256);
+    if (int rc = dalloc(&s->lpart, (size_t)s->nlpart)) return rc;
+    hipError_t e = hipSuccess;
+    for (float* p : {s->W, s->Mo, s->Vo, s->gW})
+        if (e == hipSuccess) e = launch_fill(p, 0.0f, W3, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "Surrogate.new: %s", hipGetErrorString(e));
+    if (configure_lds_cnn((size_t)cnn_layer_lds_bytes()) != hipSuccess)
+        return fail(MSD_EUNSUPPORTED, "Surrogate.new: %d bytes of LDS per workgroup not available", cnn_layer_lds_bytes());
+    HIPCHK(hipStreamSynchronize(G.stream));
+    *out = s.release();
+    return MSD_OK;
+}
+
+extern "C" int msd_surrogate_destroy(msd_surrogate* s) { delete s; return MSD_OK; }
+
+extern "C" int msd_surrogate_set_weights(msd_surrogate* s, int layer, const double* w)
+{
+    NONNULL(s); NONNULL(w);
+    if (layer < 0 || layer > 2) return fail(MSD_EINVAL, "Surrogate.setWeights: layer %d", layer);
+    const int co = msd_surrogate::cout(layer), ci = msd_surrogate::cin(layer);
+    std::vector<float> h(msd_surrogate::WSZ, 0.0f);
+    for (int t = 0; t < msd_surrogate::NT; ++t)
+        for (int o = 0; o < co; ++o)
+            for (int i = 0; i < ci; ++i) h[(t * 16 + o) * 16 + i] = (float)w[(t * co + o) * ci + i];
+    HIPCHK(hipMemcpyAsync(s->W + (size_t)layer * msd_surrogate::WSZ, h.data(), h.size() * sizeof(float),
+                          hipMemcpyHostToDevice, G.stream));
+    HIPCHK(hipStreamSynchronize(G.stream));
+    s->packed = false;
+    return MSD_OK;
+}
+
+static int surrogate_get(const float* dsrc, int layer, double* w)
+{
+    const int co = msd_surrogate::cout(layer), ci = msd_surrogate::cin(layer);
+    std::vector<float> h(msd_surrogate::WSZ);
+    HIPCHK(hipMemcpyAsync(h.data(), dsrc + (size_t)layer * msd_surrogate::WSZ, h.size() * sizeof(float),
+                          hipMemcpyDeviceToHost, G.stream));
+    HIPCHK(hipStreamSynchronize(G.stream));
+    for (int t = 0; t < msd_surrogate::NT; ++t)
+        for (int o = 0; o < co; ++o)
+            for (int i = 0; i < ci; ++i) w[(t * co + o) * ci + i] = h[(t * 16 + o) * 16 + i];
+    return MSD_OK;
+}
+
+extern "C" int msd_surrogate_get_weights(const msd_surrogate* s, int layer, double* w)
+{
+    NONNULL(s); NONNULL(w);
+    if (layer < 0 || layer > 2) return fail(MSD_EINVAL, "Surrogate.weights: layer %d", layer);
+    return surrogate_get(s->W, layer, w);
+}
+
+extern "C" int msd_surrogate_get_grads(const msd_surrogate* s, int layer, double* g)
+{
+    NONNULL(s); NONNULL(g);
+    if (layer < 0 || layer > 2) return fail(MSD_EINVAL, "Surrogate.grads: layer %d", layer);
+    return surrogate_get(s->gW, layer, g);
+}
+
+extern "C" int msd_surrogate_set_scales(msd_surrogate* s, double s_in, double s_out, double sigma2)
+{
+    NONNULL(s);
+    if (!(s_in > 0) || !(s_out > 0) || !(sigma2 > 0)) return fail(MSD_EINVAL, "Surrogate.setScales: scales must be > 0");
+    s->s_in = s_in; s->s_out = s_out; s->sigma2 = sigma2;
+    return MSD_OK;
+}
+
+extern "C" int msd_surrogate_get_scales(const msd_surrogate* s, double* s_in, double* s_out, double* sigma2)
+{
+    NONNULL(s); NONNULL(s_in); NONNULL(s_out); NONNULL(sigma2);
+    *s_in = s->s_in; *s_out = s->s_out; *sigma2 = s->sigma2;
+    return MSD_OK;
+}
+
+extern "C" int msd_surrogate_set_input(msd_surrogate* s, const double* dm)
+{
+    NONNULL(s); NONNULL(dm);
+    if (int rc = upload_f32(s->D, dm, (size_t)3 * s->d.n)) return rc;
+    hipError_t e = launch_cnn_input(s->d, s->D, nullptr, (float)s->s_in, s->X0, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "Surrogate.setInput: %s", hipGetErrorString(e));
+    return MSD_OK;
+}
+
+extern "C" int msd_surrogate_set_input_systems(msd_surrogate* s, const msd_system* m, const msd_system* m_ref)
+{
+    NONNULL(s); NONNULL(m); NONNULL(m_ref);
+    if (!same_dims(s->d, m->d.nx, m->d.ny, m->d.nz) || !same_dims(s->d, m_ref->d.nx, m_ref->d.ny, m_ref->d.nz))
+        return fail(MSD_EMISMATCH, "Surrogate.setInput: SpinSystem size differs");
+    hipError_t e = launch_cnn_input(s->d, m->S, m_ref->S, (float)s->s_in, s->X0, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "Surrogate.setInput: %s", hipGetErrorString(e));
+    return MSD_OK;
+}
+
+extern "C" int msd_surrogate_set_target(msd_surrogate* s, const double* t)
+{
+    NONNULL(s); NONNULL(t);
+    return upload_f32(s->T, t, (size_t)3 * s->d.n);
+}
+
+extern "C" int msd_surrogate_set_target_dipole(msd_surrogate* s, msd_dipole* dip, const msd_system* m,
+                                               const msd_system* m_ref)
+{
+    NONNULL(s); NONNULL(dip); NONNULL(m); NONNULL(m_ref);
+    if (!same_dims(s->d, m->d.nx, m->d.ny, m->d.nz) || !same_dims(s->d, m_ref->d.nx, m_ref->d.ny, m_ref->d.nz))
+        return fail(MSD_EMISMATCH, "Surrogate.setTarget: SpinSystem size differs");
+    hipError_t e = launch_cnn_sub((size_t)3 * s->d.n, m->S, m_ref->S, s->D, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "Surrogate.setTarget: %s", hipGetErrorString(e));
+    return dipole_field_of(dip, s->D, s->T);
+}
+
+extern "C" int msd_surrogate_forward(msd_surrogate* s, double* out)
+{
+    NONNULL(s);
+    if (int rc = s->forward(s->Y3, nullptr, G.stream)) return rc;
+    if (out) return download_f32(out, s->Y3, (size_t)3 * s->d.n);
+    return MSD_OK;
+}
+
+// forward, loss, back-propagation and the three weight gradients
+static int surrogate_loss_grads(msd_surrogate* s, double* loss)
+{
+    const size_t n = s->d.n;
+    if (int rc = s->forward(s->Y3, nullptr, G.stream)) return rc;
+    const float gscale = (float)(2.0 / (3.0 * (double)n * s->sigma2));
+    const int P = msd_surrogate::PACK, WS = msd_surrogate::WSZ;
+    // the forward output already carries s_out: dL/dY3 = s_out dL/d(dH)
+    hipError_t e = launch_cnn_loss(s->d, s->Y3, s->T, 1.0f, gscale * (float)s->s_out, s->G3, s->lpart, G.stream);
+    if (e == hipSuccess) e = launch_cnn_layer(s->cd, s->grid, s->G3, s->At + 2 * P, 0, s->G2, nullptr, nullptr, 1.0f, G.stream);
+    if (e == hipSuccess) e = launch_cnn_layer(s->cd, s->grid, s->G2, s->At + P, 0, s->G1, nullptr, nullptr, 1.0f, G.stream);
+    if (e == hipSuccess) e = launch_cnn_wgrad(s->cd, s->grid, s->X2, s->G3, s->partial, s->gW + 2 * WS, G.stream);
+    if (e == hipSuccess) e = launch_cnn_wgrad(s->cd, s->grid, s->X1, s->G2, s->partial, s->gW + WS, G.stream);
+    if (e == hipSuccess) e = launch_cnn_wgrad(s->cd, s->grid, s->X0, s->G1, s->partial, s->gW, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "surrogate backward: %s", hipGetErrorString(e));
+    if (loss) {
+        std::vector<double> h(s->nlpart);
+        HIPCHK(hipMemcpyAsync(h.data(), s->lpart, sizeof(double) * h.size(), hipMemcpyDeviceToHost, G.stream));
+        HIPCHK(hipStreamSynchronize(G.stream));
+        double acc = 0.0;
+        for (double v : h) acc += v;
+        *loss = acc / (3.0 * (double)n * s->sigma2);
+    }
+    return MSD_OK;
+}
+
+extern "C" int msd_surrogate_loss_grads(msd_surrogate* s, double* loss)
+{
+    NONNULL(s);
+    return surrogate_loss_grads(s, loss);
+}
+
+extern "C" int msd_surrogate_adam(msd_surrogate* s, double lr)
+{
+    NONNULL(s);
+    if (!(lr > 0)) return fail(MSD_EINVAL, "Surrogate.adam: lr must be > 0");
+    const double b1 = 0.9, b2 = 0.999;
+    s->adam_t++;
+    const float c1 = (float)(1.0 / (1.0 - std::pow(b1, s->adam_t)));
+    const float c2 = (float)(1.0 / (1.0 - std::pow(b2, s->adam_t)));
+    hipError_t e = launch_cnn_adam(s->W, s->gW, s->Mo, s->Vo, 3 * msd_surrogate::WSZ, (float)lr, (float)b1, (float)b2,
+                                   c1, c2, 1e-8f, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "Surrogate.adam: %s", hipGetErrorString(e));
+    s->packed = false;
+    return MSD_OK;
+}
+
+extern "C" int msd_surrogate_reset_optimizer(msd_surrogate* s)
+{
+    NONNULL(s);
+    hipError_t e = launch_fill(s->Mo, 0.0f, (size_t)3 * msd_surrogate::WSZ, G.stream);
+    if (e == hipSuccess) e = launch_fill(s->Vo, 0.0f, (size_t)3 * msd_surrogate::WSZ, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "Surrogate.resetOptimizer: %s", hipGetErrorString(e));
+    s->adam_t = 0;
+    return MSD_OK;
+}
+
+extern "C" int msd_surrogate_train_step(msd_surrogate* s, double lr, double* loss)
+{
+    NONNULL(s);
+    if (int rc = surrogate_loss_grads(s, loss)) return rc;
+    return msd_surrogate_adam(s, lr);
+}
+
+// max |a - b| and max |b| over [3][n] device fields
+static int field_maxdiff(const float* a, const float* b, size_t n3, double* maxdiff, double* maxb)
+{
+    const int blocks = 512;
+    float2* part = nullptr;
+    if (int rc = dalloc(&part, blocks)) return rc;
+    hipError_t e = launch_cnn_maxdiff(n3, a, b, part, blocks, G.stream);
+    std::vector<float2> h(blocks);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part, sizeof(float2) * blocks, hipMemcpyDeviceToHost, G.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(G.stream);
+    dfree(part);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "field check: %s", hipGetErrorString(e));
+    double md = 0.0, mb = 0.0;
+    for (const float2& v : h) { md = std::max(md, (double)v.x); mb = std::max(mb, (double)v.y); }
+    *maxdiff = md;
+    *maxb = mb;
+    return MSD_OK;
+}
+
+// Surrogate prediction of the dipole field at m (m_ref, H_ref its reference
+// state and exact field) against a field given on the host: returns
+// max|pred - exact| / max|exact| (the residual the step driver checks).
+extern "C" int msd_surrogate_field_error(msd_surrogate* s, msd_dipole* dip, const msd_system* m, const msd_system* m_ref,
+                                         double* rel_err, double* frozen_err)
+{
+    NONNULL(s); NONNULL(dip); NONNULL(m); NONNULL(m_ref); NONNULL(rel_err);
+    if (!same_dims(s->d, m->d.nx, m->d.ny, m->d.nz) || !same_dims(s->d, m_ref->d.nx, m_ref->d.ny, m_ref->d.nz))
+        return fail(MSD_EMISMATCH, "Surrogate.fieldError: SpinSystem size differs");
+    const size_t n3 = (size_t)3 * s->d.n;
+    // exact H(m) into T, exact H(m_ref) into D, prediction D + CNN(m - m_ref) into Y3
+    if (int rc = dipole_field_of(dip, m->S, s->T)) return rc;
+    if (int rc = dipole_field_of(dip, m_ref->S, s->D)) return rc;
+    hipError_t e = launch_cnn_input(s->d, m->S, m_ref->S, (float)s->s_in, s->X0, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "Surrogate.fieldError: %s", hipGetErrorString(e));
+    if (int rc = s->forward(s->Y3, s->D, G.stream)) return rc;
+    double md, mb;
+    if (int rc = field_maxdiff(s->Y3, s->T, n3, &md, &mb)) return rc;
+    *rel_err = mb > 0 ? md / mb : md;
+    if (frozen_err) {
+        if (int rc = field_maxdiff(s->D, s->T, n3, &md, &mb)) return rc;
+        *frozen_err = mb > 0 ? md / mb : md;
+    }
+    return MSD_OK;
+}
+
 struct msd_stepper {
     msd_system* s = nullptr;
     msd_exchange* ex = nullptr;
@@ -1503,17 +1833,34 @@ struct msd_stepper {
     StepArgs graph_args{};
     Batch3 b3;
     uint64_t spec_version = 0;
+    uint64_t dip_epoch = 0;
+    // learned dipole field (msd_stepper_set_surrogate)
+    msd_surrogate* sur = nullptr;
+    int sur_K = 10;
+    double sur_tol = 1e-3;
+    float *Href = nullptr, *Mref = nullptr, *Hext = nullptr, *Hx = nullptr;   // [3][n]
+    bool have_ref = false, sur_fallback = false;
+    long sur_windows = 0, sur_exact_windows = 0, sur_fallbacks = 0;
+    double sur_last_res = 0.0, sur_max_res = 0.0;
+    hipGraph_t lgraph = nullptr;
+    hipGraphExec_t lexec = nullptr;
+    StepArgs lgraph_args{};
+    double lgraph_sin = 0.0, lgraph_sout = 0.0;   // surrogate scales baked into the learned graph
     ~msd_stepper()
     {
         (void)hipStreamSynchronize(G.stream);
         drop_graph();
         dfree(P); dfree(C); dfree(Clo); dfree(draw);
+        dfree(Href); dfree(Mref); dfree(Hext); dfree(Hx);
     }
     void drop_graph()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        if (lexec) (void)hipGraphExecDestroy(lexec);
+        if (lgraph) (void)hipGraphDestroy(lgraph);
         exec = nullptr; graph = nullptr;
+        lexec = nullptr; lgraph = nullptr;
     }
     int kind() const { return lr3 ? SK_CONV3 : (dip ? SK_LAYERED : SK_LOCAL); }
     int refresh();
@@ -1532,6 +1879,7 @@ struct msd_stepper {
         return r;
     }
     int launch_step(hipStream_t st);
+    int launch_learned_step(hipStream_t st);
 };
 
 static bool args_equal(const StepArgs& a, const StepArgs& b) { return std::memcmp(&a, &b, sizeof(StepArgs)) == 0; }
@@ -1582,6 +1930,27 @@ int msd_stepper::launch_step(hipStream_t st)
         if (e == hipSuccess) e = launch_step_corr(args, st);
     }
     if (e != hipSuccess) return fail(MSD_EDEVICE, "step launch: %s", hipGetErrorString(e));
+    return MSD_OK;
+}
+
+// One Heun step with the surrogate's dipole field: H_dip(X) = H_ref + dH(X - m_ref)
+// for the predictor (X = S) and the corrector (X = P); the fused local kernels
+// add it to exchange + anisotropy (+ thermal).
+int msd_stepper::launch_learned_step(hipStream_t st)
+{
+    StepArgs a = args;
+    a.has_dip = 0;
+    a.Hext = Hext;
+    const float sin = (float)sur->s_in;
+    hipError_t e = launch_cnn_input(s->d, s->S, Mref, sin, sur->X0, st);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "learned step: %s", hipGetErrorString(e));
+    if (int rc = sur->forward(Hext, Href, st)) return rc;
+    e = launch_step_pred(a, st);
+    if (e == hipSuccess) e = launch_cnn_input(s->d, P, Mref, sin, sur->X0, st);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "learned step: %s", hipGetErrorString(e));
+    if (int rc = sur->forward(Hext, Href, st)) return rc;
+    e = launch_step_corr(a, st);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "learned step: %s", hipGetErrorString(e));
     return MSD_OK;
 }
 
@@ -1650,7 +2019,10 @@ static int member_begin(msd_stepper* st)
     return MSD_OK;
 }
 
-static bool spectrum_stale(const msd_stepper* st) { return st->spec_version != st->s->version; }
+static bool spectrum_stale(const msd_stepper* st)
+{
+    return st->spec_version != st->s->version || (st->dip && st->dip->epoch != st->dip_epoch);
+}
 
 static int stepper_begin(msd_stepper* st)
 {
@@ -1673,6 +2045,87 @@ static void stepper_end(msd_stepper* st, int nsteps)
     if (st->th) st->th->draw += (uint64_t)nsteps;
     st->s->version++;
     st->spec_version = st->s->version;
+    if (st->dip) st->dip_epoch = st->dip->epoch;
+}
+
+// capture one step (exact or learned) into a graph
+static int capture_step(msd_stepper* st, bool learned, hipGraph_t* graph, hipGraphExec_t* exec)
+{
+    HIPCHK(hipStreamBeginCapture(G.stream, hipStreamCaptureModeThreadLocal));
+    int rc = learned ? st->launch_learned_step(G.stream) : st->launch_step(G.stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(G.stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "graph capture: %s", hipGetErrorString(e));
+    *graph = g;
+    HIPCHK(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    return MSD_OK;
+}
+
+// nsteps exact (learned = false) or learned steps, graph-replayed when enabled
+static int run_steps(msd_stepper* st, int nsteps, bool learned)
+{
+    if (st->use_graph) {
+        hipGraphExec_t* exec = learned ? &st->lexec : &st->exec;
+        hipGraph_t* graph = learned ? &st->lgraph : &st->graph;
+        StepArgs* gargs = learned ? &st->lgraph_args : &st->graph_args;
+        const bool scales_moved = learned && (st->lgraph_sin != st->sur->s_in || st->lgraph_sout != st->sur->s_out);
+        if (!*exec || !args_equal(st->args, *gargs) || scales_moved) {
+            if (*exec) (void)hipGraphExecDestroy(*exec);
+            if (*graph) (void)hipGraphDestroy(*graph);
+            *exec = nullptr; *graph = nullptr;
+            if (int rc = capture_step(st, learned, graph, exec)) return rc;
+            *gargs = st->args;
+            if (learned) { st->lgraph_sin = st->sur->s_in; st->lgraph_sout = st->sur->s_out; }
+        }
+        for (int i = 0; i < nsteps; ++i) HIPCHK(hipGraphLaunch(*exec, G.stream));
+    } else {
+        for (int i = 0; i < nsteps; ++i)
+            if (int rc = learned ? st->launch_learned_step(G.stream) : st->launch_step(G.stream)) return rc;
+    }
+    return MSD_OK;
+}
+
+// Windows of K steps.  At each window start the exact layered field of the
+// current state is computed; the surrogate's prediction for that state (from the
+// previous window's reference) is checked against it, and the exact field and
+// state become the new reference.  A window runs learned steps unless the check
+// exceeded the tolerance, in which case it runs exact steps (and the surrogate is
+// checked again at the next window start).
+static int stepper_heun_learned(msd_stepper* st, int nsteps)
+{
+    if (int rc = member_begin(st)) return rc;
+    if (int rc = st->sur->pack()) return rc;
+    const size_t n3 = (size_t)3 * st->s->d.n;
+    int done = 0;
+    while (done < nsteps) {
+        if (int rc = dipole_field_of(st->dip, st->s->S, st->Hx)) return rc;
+        if (st->have_ref) {
+            hipError_t e = launch_cnn_input(st->s->d, st->s->S, st->Mref, (float)st->sur->s_in, st->sur->X0, G.stream);
+            if (e != hipSuccess) return fail(MSD_EDEVICE, "surrogate check: %s", hipGetErrorString(e));
+            if (int rc = st->sur->forward(st->Hext, st->Href, G.stream)) return rc;
+            double md, mb;
+            if (int rc = field_maxdiff(st->Hext, st->Hx, n3, &md, &mb)) return rc;
+            const double res = mb > 0 ? md / mb : md;
+            st->sur_last_res = res;
+            st->sur_max_res = std::max(st->sur_max_res, res);
+            st->sur_fallback = !(res <= st->sur_tol);
+            if (st->sur_fallback) st->sur_fallbacks++;
+        }
+        HIPCHK(hipMemcpyAsync(st->Href, st->Hx, n3 * sizeof(float), hipMemcpyDeviceToDevice, G.stream));
+        HIPCHK(hipMemcpyAsync(st->Mref, st->s->S, n3 * sizeof(float), hipMemcpyDeviceToDevice, G.stream));
+        st->have_ref = true;
+        const int L = std::min(st->sur_K, nsteps - done);
+        // the exact field pass above left the spin spectrum of S in the shared
+        // buffers, which is what an exact step's first column pass reads
+        if (int rc = run_steps(st, L, !st->sur_fallback)) return rc;
+        st->sur_windows++;
+        if (st->sur_fallback) st->sur_exact_windows++;
+        done += L;
+    }
+    stepper_end(st, nsteps);
+    st->spec_version = 0;    // learned steps leave no spin spectrum behind
+    return MSD_OK;
 }
 
 extern "C" int msd_stepper_heun(msd_stepper* st, int nsteps)
@@ -1680,26 +2133,52 @@ extern "C" int msd_stepper_heun(msd_stepper* st, int nsteps)
     NONNULL(st);
     if (nsteps < 0) return fail(MSD_EINVAL, "heun: nsteps %d", nsteps);
     if (nsteps == 0) return MSD_OK;
+    if (st->sur) return stepper_heun_learned(st, nsteps);
     if (int rc = stepper_begin(st)) return rc;
-    if (st->use_graph) {
-        if (!st->exec || !args_equal(st->args, st->graph_args)) {
-            st->drop_graph();
-            HIPCHK(hipStreamBeginCapture(G.stream, hipStreamCaptureModeThreadLocal));
-            int rc = st->launch_step(G.stream);
-            hipGraph_t g = nullptr;
-            hipError_t e = hipStreamEndCapture(G.stream, &g);
-            if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
-            if (e != hipSuccess) return fail(MSD_EDEVICE, "graph capture: %s", hipGetErrorString(e));
-            st->graph = g;
-            HIPCHK(hipGraphInstantiate(&st->exec, st->graph, nullptr, nullptr, 0));
-            st->graph_args = st->args;
-        }
-        for (int i = 0; i < nsteps; ++i) HIPCHK(hipGraphLaunch(st->exec, G.stream));
-    } else {
-        for (int i = 0; i < nsteps; ++i)
-            if (int rc = st->launch_step(G.stream)) return rc;
-    }
+    if (int rc = run_steps(st, nsteps, false)) return rc;
     stepper_end(st, nsteps);
+    return MSD_OK;
+}
+
+extern "C" int msd_stepper_set_surrogate(msd_stepper* st, msd_surrogate* sur, int K, double tol)
+{
+    NONNULL(st);
+    if (!sur) {
+        st->sur = nullptr;
+        st->have_ref = false;
+        st->spec_version = 0;
+        return MSD_OK;
+    }
+    if (!st->dip || st->lr3) return fail(MSD_EUNSUPPORTED, "stepper: the surrogate replaces a layered Dipole field");
+    if (!same_dims(st->s->d, sur->d.nx, sur->d.ny, sur->d.nz)) return fail(MSD_EMISMATCH, "stepper: Surrogate size differs");
+    if (K < 1) return fail(MSD_EINVAL, "stepper: refresh interval %d", K);
+    if (!(tol > 0)) return fail(MSD_EINVAL, "stepper: residual tolerance must be > 0");
+    const size_t n3 = (size_t)3 * st->s->d.n;
+    for (float** p : {&st->Href, &st->Mref, &st->Hext, &st->Hx})
+        if (!*p)
+            if (int rc = dalloc(p, n3)) return rc;
+    st->sur = sur;
+    st->sur_K = K;
+    st->sur_tol = tol;
+    st->have_ref = false;
+    st->sur_fallback = false;
+    st->sur_windows = st->sur_exact_windows = st->sur_fallbacks = 0;
+    st->sur_last_res = st->sur_max_res = 0.0;
+    if (st->lexec) (void)hipGraphExecDestroy(st->lexec);
+    if (st->lgraph) (void)hipGraphDestroy(st->lgraph);
+    st->lexec = nullptr; st->lgraph = nullptr;
+    return MSD_OK;
+}
+
+extern "C" int msd_stepper_surrogate_stats(const msd_stepper* st, long* windows, long* exact_windows, long* fallbacks,
+                                           double* last_residual, double* max_residual)
+{
+    NONNULL(st); NONNULL(windows); NONNULL(exact_windows); NONNULL(fallbacks); NONNULL(last_residual); NONNULL(max_residual);
+    *windows = st->sur_windows;
+    *exact_windows = st->sur_exact_windows;
+    *fallbacks = st->sur_fallbacks;
+    *last_residual = st->sur_last_res;
+    *max_residual = st->sur_max_res;
     return MSD_OK;
 }
 

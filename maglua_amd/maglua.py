@@ -712,6 +712,20 @@ class Heun(_Handle):
 
     __call__ = step
 
+    def setSurrogate(self, sur, K=10, tol=1e-3):
+        """Learned dipole field: windows of K steps, each started by an exact field evaluation
+        that becomes the reference and checks the surrogate (relative max-norm residual <= tol,
+        else the window runs exact steps).  sur=None returns to exact steps."""
+        self._sur = sur
+        call("msd_stepper_set_surrogate", self._h, sur.h if sur is not None else None, int(K), float(tol))
+
+    def surrogateStats(self):
+        w, e, f = C.c_long(), C.c_long(), C.c_long()
+        last, mx = C.c_double(), C.c_double()
+        call("msd_stepper_surrogate_stats", self._h, C.byref(w), C.byref(e), C.byref(f), C.byref(last), C.byref(mx))
+        return {"windows": w.value, "exact_windows": e.value, "fallbacks": f.value,
+                "last_residual": last.value, "max_residual": mx.value}
+
     def profile(self, n):
         """Kernel -> average ms per launch (HIP events on the library stream).  Layered and
         local forms repeat each launch of one step n times back to back (one step of
@@ -722,6 +736,96 @@ class Heun(_Handle):
         cnt = C.c_int()
         call("msd_stepper_profile", self._h, int(n), cap, names, avg, C.byref(cnt))
         return {bytes(names[i]).split(b"\0")[0].decode(): avg[i] for i in range(cnt.value)}
+
+
+class Surrogate(_Handle):
+    """Learned predictor of the layered dipole field's change between exact refreshes:
+    dH = s_out * L3(L2(L1(s_in * (m - m_ref)))), three 5x5x3 convolution layers with
+    3 -> 16 -> 16 -> 3 channels (BF16 MFMA, FP32 accumulate), periodic in x and y,
+    zero-padded in z.  Weights of layer l are arrays [3,5,5,cout,cin] indexed
+    [dz+1, dy+2, dx+2, out, in].  Field arrays are [3][n] like SpinSystem.spins()."""
+
+    _destroy = "msd_surrogate_destroy"
+    CHANNELS = (3, 16, 16, 3)
+
+    def __init__(self, nx, ny=None, nz=None):
+        if ny is None:
+            nx, ny, nz = _dims_of((nx,))
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.n = self.nx * self.ny * self.nz
+        p = _vp()
+        call("msd_surrogate_create", self.nx, self.ny, self.nz, C.byref(p))
+        super().__init__(p.value)
+
+    def _shape(self, l):
+        return (3, 5, 5, self.CHANNELS[l + 1], self.CHANNELS[l])
+
+    def setWeights(self, layer, W):
+        a = np.ascontiguousarray(np.asarray(W, dtype=np.float64).reshape(self._shape(layer)))
+        call("msd_surrogate_set_weights", self._h, int(layer), _d(a))
+
+    def weights(self, layer):
+        a = np.zeros(self._shape(layer))
+        call("msd_surrogate_get_weights", self._h, int(layer), _d(a))
+        return a
+
+    def setScales(self, s_in, s_out, sigma2=1.0):
+        call("msd_surrogate_set_scales", self._h, float(s_in), float(s_out), float(sigma2))
+
+    def scales(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        call("msd_surrogate_get_scales", self._h, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
+    def _field(self, v):
+        a = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(3, self.n))
+        return a
+
+    def setInput(self, dm):
+        a = self._field(dm)
+        call("msd_surrogate_set_input", self._h, _d(a))
+
+    def setInputSystems(self, m, m_ref):
+        call("msd_surrogate_set_input_systems", self._h, m.h, m_ref.h)
+
+    def setTarget(self, t):
+        a = self._field(t)
+        call("msd_surrogate_set_target", self._h, _d(a))
+
+    def setTargetDipole(self, dip, m, m_ref):
+        call("msd_surrogate_set_target_dipole", self._h, dip.h, m.h, m_ref.h)
+
+    def forward(self):
+        out = np.zeros((3, self.n))
+        call("msd_surrogate_forward", self._h, _d(out))
+        return out
+
+    def lossGrads(self):
+        loss = C.c_double()
+        call("msd_surrogate_loss_grads", self._h, C.byref(loss))
+        grads = []
+        for l in range(3):
+            g = np.zeros(self._shape(l))
+            call("msd_surrogate_get_grads", self._h, l, _d(g))
+            grads.append(g)
+        return loss.value, grads
+
+    def adam(self, lr):
+        call("msd_surrogate_adam", self._h, float(lr))
+
+    def resetOptimizer(self):
+        call("msd_surrogate_reset_optimizer", self._h)
+
+    def trainStep(self, lr):
+        loss = C.c_double()
+        call("msd_surrogate_train_step", self._h, float(lr), C.byref(loss))
+        return loss.value
+
+    def fieldError(self, dip, m, m_ref):
+        """(max|H_ref + dH - H| / max|H|, the same for H_ref alone) with exact Dipole fields."""
+        a, b = C.c_double(), C.c_double()
+        call("msd_surrogate_field_error", self._h, dip.h, m.h, m_ref.h, C.byref(a), C.byref(b))
+        return a.value, b.value
 
 
 class Ensemble(_Handle):

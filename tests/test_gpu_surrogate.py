@@ -1,0 +1,182 @@
+"""GPU parity of the dipolar-field surrogate (BF16 MFMA kernels through the C ABI)
+against the FP64 network oracle (oracle/cnn_oracle.py), which rounds to BF16 at
+the same places; then training, the gain over a frozen field, and the learned
+step driver with its exact fallback."""
+import numpy as np
+import pytest
+
+import maglua_amd as M
+from maglua_amd import surrogate as S
+from maglua_amd import workloads as W
+from oracle import cnn_oracle as N
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def soa(a):
+    """[nz,ny,nx,3] -> [3][n]"""
+    return np.moveaxis(a, -1, 0).reshape(3, -1)
+
+
+def aos(v, dims):
+    nx, ny, nz = dims
+    return np.moveaxis(np.asarray(v).reshape(3, nz, ny, nx), 0, -1)
+
+
+def setup(dims, seed, s_in=60.0, s_out=0.04, sigma2=2e-4):
+    nx, ny, nz = dims
+    rng = np.random.default_rng(seed)
+    Ws = N.init_weights(seed)
+    dm = rng.standard_normal((nz, ny, nx, 3)) * 0.01
+    T = rng.standard_normal((nz, ny, nx, 3)) * 0.02
+    sur = M.Surrogate(nx, ny, nz)
+    for l in range(3):
+        sur.setWeights(l, Ws[l])
+    sur.setScales(s_in, s_out, sigma2)
+    sur.setInput(soa(dm))
+    sur.setTarget(soa(T))
+    return sur, Ws, dm, T, (s_in, s_out, sigma2)
+
+
+@pytest.mark.parametrize("dims", [(32, 4, 1), (32, 8, 4), (64, 6, 3)])
+def test_forward_matches_oracle(dims):
+    sur, Ws, dm, T, (s_in, s_out, _) = setup(dims, 1)
+    got = aos(sur.forward(), dims)
+    want, _ = N.forward(Ws, dm, s_in, s_out, bf=True)
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print(f"surrogate forward {dims}: rel err {err:.2e}", flush=True)
+    assert err <= 2e-3
+
+
+def test_weights_round_trip():
+    sur, Ws, *_ = setup((32, 4, 2), 2)
+    for l in range(3):
+        assert np.allclose(sur.weights(l), np.asarray(Ws[l], dtype=np.float32), rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("dims", [(32, 8, 4), (64, 4, 2)])
+def test_gradients_match_oracle(dims):
+    sur, Ws, dm, T, sc = setup(dims, 3)
+    loss, grads = sur.lossGrads()
+    L, want = N.loss_and_grads(Ws, dm, T, *sc, bf=True)
+    print(f"surrogate loss {dims}: gpu {loss:.6e} oracle {L:.6e}", flush=True)
+    assert abs(loss - L) <= 1e-3 * L
+    for l in range(3):
+        rel = np.linalg.norm(grads[l] - want[l]) / np.linalg.norm(want[l])
+        print(f"  layer {l}: grad rel err {rel:.2e}", flush=True)
+        assert rel <= 1e-2, l
+
+
+def test_adam_step_matches_oracle():
+    sur, Ws, dm, T, sc = setup((32, 4, 2), 4)
+    _, g = sur.lossGrads()
+    w0 = [sur.weights(l) for l in range(3)]
+    sur.adam(1e-3)
+    for l in range(3):
+        w = w0[l].copy()
+        N.adam(w, g[l], np.zeros_like(w), np.zeros_like(w), 1, lr=1e-3)
+        assert np.abs(sur.weights(l) - w).max() <= 1e-6
+
+
+def test_training_fits_a_teacher_network():
+    """A target made by a known network of the same form is learned: the loss falls 20x."""
+    dims = (32, 8, 2)
+    nx, ny, nz = dims
+    rng = np.random.default_rng(5)
+    teacher = N.init_weights(50)
+    dm = rng.standard_normal((nz, ny, nx, 3)) * 0.01
+    T, _ = N.forward(teacher, dm, 100.0, 0.05)
+    sur = M.Surrogate(*dims)
+    for l, Wl in enumerate(N.init_weights(7)):
+        sur.setWeights(l, Wl)
+    sur.setScales(100.0, 0.05, float(np.mean(T * T)))
+    sur.setInput(soa(dm))
+    sur.setTarget(soa(T))
+    first = sur.trainStep(2e-3)
+    for _ in range(300):
+        last = sur.trainStep(2e-3)
+    print(f"teacher fit: loss {first:.3e} -> {last:.3e}", flush=True)
+    assert last <= first / 20
+
+
+def test_exact_target_is_the_dipole_field_of_the_difference():
+    spec = W.FilmSpec(32, 32, 2, g=0.1, K=0.02)
+    g = W.build(M, spec)
+    x, y, z = O.random_unit_spins(3, spec.n)
+    g["ss"].setSpins(x, y, z)
+    ref = g["ss"].copy()
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"])
+    heun.step(5)
+    sur = M.Surrogate(32, 32, 2)
+    sur.setTargetDipole(g["dip"], g["ss"], ref)
+    # the target read back through a zero network: loss = mean(T^2) / sigma2
+    sur.setScales(1.0, 1.0, 1.0)
+    sur.setInput(np.zeros((3, spec.n)))
+    loss, _ = sur.lossGrads()
+    o = W.build(O, spec)
+    a, b = g["ss"].spins(), ref.spins()
+    o["ss"].setSpins(*(np.asarray(a[c]) - np.asarray(b[c]) for c in range(3)))
+    o["dip"].apply(o["ss"])
+    T = np.stack([np.asarray(v).reshape(-1) for v in o["ss"].field("Dipole")])
+    assert abs(loss - np.mean(T * T)) <= 1e-3 * np.mean(T * T)
+
+
+def film(seed, nx=64, ny=64, nz=4, T=0.2):
+    spec = W.FilmSpec(nx, ny, nz, J=1.0, K=0.02, g=0.1, T=T, alpha=0.1, dt=0.005, seed=seed)
+    g = W.build(M, spec)
+    g["ss"].setSpins(*O.random_unit_spins(seed, spec.n))
+    if g["temp"] is not None:
+        g["temp"].setSeed(seed)
+    return spec, g
+
+
+def test_trained_surrogate_beats_the_frozen_field():
+    """After training on one trajectory, on another the network's field K=10 steps after a
+    refresh is closer to the exact field than the refreshed field held fixed."""
+    spec, g = film(11)
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"])
+    heun.step(20)
+    sur = M.Surrogate(spec.nx, spec.ny, spec.nz)
+    log = S.train_on_trajectory(sur, g["ss"], heun, g["dip"], steps=400, K=10, lr=2e-3, seed=1)
+    print(f"training loss {log[0]:.3e} -> {np.mean(log[-20:]):.3e}", flush=True)
+    spec2, h = film(12)
+    heun2 = M.Heun(h["ss"], h["ex"], h["ani"], h["dip"], h["temp"], h["llg"])
+    heun2.step(20)
+    ref = h["ss"].copy()
+    heun2.step(10)
+    rel, frozen = sur.fieldError(h["dip"], h["ss"], ref)
+    print(f"field error 10 steps after refresh: surrogate {rel:.3e}, frozen {frozen:.3e}", flush=True)
+    assert rel < 0.5 * frozen
+
+
+def test_learned_driver_tracks_the_exact_trajectory_and_falls_back():
+    spec, g = film(21)
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"])
+    heun.step(20)
+    sur = M.Surrogate(spec.nx, spec.ny, spec.nz)
+    S.train_on_trajectory(sur, g["ss"], heun, g["dip"], steps=300, K=10, lr=2e-3, seed=2)
+    # two identical states, one driven exactly, one with the surrogate
+    a = g["ss"].copy()
+    b = g["ss"].copy()
+    ea = M.Heun(a, g["ex"], g["ani"], g["dip"], g["temp"], g["llg"])
+    tb = W.build(M, spec)["temp"]
+    tb.setSeed(spec.seed)
+    ta = g["temp"]
+    for t in (ta, tb):
+        t.setSeed(21, draw=ta.draw())
+    eb = M.Heun(b, g["ex"], g["ani"], g["dip"], tb, g["llg"])
+    eb.setSurrogate(sur, K=10, tol=1e-2)
+    ea.step(40)
+    eb.step(40)
+    st = eb.surrogateStats()
+    dmax = float(np.abs(np.stack(a.spins()[:3]) - np.stack(b.spins()[:3])).max())
+    print(f"learned vs exact after 40 steps: max|dm| {dmax:.3e}; {st}", flush=True)
+    assert st["windows"] == 4
+    assert dmax <= 5e-3
+    assert abs(a.time() - b.time()) < 1e-12
+    # an impossible tolerance: every checked window falls back to exact steps
+    eb.setSurrogate(sur, K=10, tol=1e-12)
+    eb.step(30)
+    st = eb.surrogateStats()
+    assert st["windows"] == 3 and st["fallbacks"] == 2 and st["exact_windows"] == 2
