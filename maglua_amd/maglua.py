@@ -795,7 +795,11 @@ class Surrogate(_Handle):
     def setTargetDipole(self, dip, m, m_ref):
         call("msd_surrogate_set_target_dipole", self._h, dip.h, m.h, m_ref.h)
 
-    def forward(self):
+    def forward(self, download=True):
+        """dH of the current input as a [3][n] array (download=False: leave it on the device)."""
+        if not download:
+            call("msd_surrogate_forward", self._h, None)
+            return None
         out = np.zeros((3, self.n))
         call("msd_surrogate_forward", self._h, _d(out))
         return out
