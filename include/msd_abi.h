@@ -228,6 +228,12 @@ int msd_stepper_profile(msd_stepper* st, int nsteps, int cap, char names[][32], 
  * over all replicas.  No reference counterpart: MagLua loops
  * SpinOperation::apply over same-sized systems (cpu/core/spinoperation.cpp:288-295).
  * Members must outlive the ensemble. */
+/* Autotuner (no counterpart in the reference): times `steps` steps of the layered
+ * driver for every compiled row-tile shape (rows per tile, threads per workgroup)
+ * and keeps the fastest; the spin state, time and thermal draw are restored.
+ * Returns the chosen shape and its ms per step (-1 when only runtime shapes exist). */
+int msd_stepper_autotune(msd_stepper* st, int steps, int* best_ty, int* best_tpb, double* best_ms);
+
 /* ---------------------------------------------------------------- surrogate
  * No counterpart in the reference: a learned predictor of the layered dipole
  * field's change between exact refreshes (the field itself is LongRange::apply,

@@ -494,6 +494,16 @@ bool row_shape_compiled(int nx, int ty, int tpb)
     return false;
 }
 
+int row_shapes_for(int nx, int ny, int* ty, int* tpb, int cap)
+{
+    int k = 0;
+#define MSD_LIST(NXV, TYV, TPBV)                                                    \
+    if (nx == NXV && ny % TYV == 0 && k < cap) { ty[k] = TYV; tpb[k] = TPBV; ++k; }
+    MSD_ROW_SHAPES(MSD_LIST)
+#undef MSD_LIST
+    return k;
+}
+
 void row_shape_default(int nx, int ny, int* ty, int* tpb)
 {
     *ty = 0;

@@ -2140,6 +2140,78 @@ extern "C" int msd_stepper_heun(msd_stepper* st, int nsteps)
     return MSD_OK;
 }
 
+// Autotuner of the layered step: times `steps` graph-replayed steps for every
+// compiled row-tile shape (rows per tile, threads per workgroup) of the lattice
+// and keeps the fastest.  The spin state, its low words, time and thermal draw
+// are restored after every trial, so tuning leaves the trajectory untouched.
+extern "C" int msd_stepper_autotune(msd_stepper* st, int steps, int* best_ty, int* best_tpb, double* best_ms)
+{
+    NONNULL(st); NONNULL(best_ty); NONNULL(best_tpb); NONNULL(best_ms);
+    if (st->kind() != SK_LAYERED) return fail(MSD_EUNSUPPORTED, "autotune: needs a layered Dipole step driver");
+    if (steps < 1) return fail(MSD_EINVAL, "autotune: steps %d", steps);
+    msd_dipole* dip = st->dip;
+    int tys[16], tpbs[16];
+    const int nc = row_shapes_for(dip->d.nx, dip->d.ny, tys, tpbs, 16);
+    *best_ty = dip->dev.ty; *best_tpb = dip->dev.tpb; *best_ms = -1.0;
+    if (nc == 0) return MSD_OK;           // runtime-shaped kernels only: nothing to choose
+    const size_t n = st->s->d.n;
+    float *S0 = nullptr, *L0 = nullptr;
+    if (int rc = dalloc(&S0, 4 * n)) return rc;
+    if (int rc = dalloc(&L0, 3 * n)) { dfree(S0); return rc; }
+    const double time0 = st->s->time;
+    const uint64_t draw0 = st->th ? st->th->draw : 0;
+    const bool lo0 = st->s->lo_valid;
+    int rc = copy_dev(S0, st->s->S, 4 * n);
+    if (!rc) rc = copy_dev(L0, st->s->Slo, 3 * n);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = fail(MSD_EDEVICE, "autotune: events");
+    auto restore = [&]() -> int {
+        if (int r = copy_dev(st->s->S, S0, 4 * n)) return r;
+        if (int r = copy_dev(st->s->Slo, L0, 3 * n)) return r;
+        st->s->time = time0;
+        st->s->lo_valid = lo0;
+        if (st->th) st->th->draw = draw0;
+        st->s->version++;
+        return MSD_OK;
+    };
+    auto use = [&](int ty, int tpb) {
+        DipoleDev& dv = dip->dev;
+        dv.ty = ty;
+        dv.tpb = tpb;
+        dv.lds_rows = (size_t)((3 * ty + 1) / 2 + 1) * dip->d.nx * sizeof(float2);
+        st->drop_graph();
+    };
+    const int ty_orig = dip->dev.ty, tpb_orig = dip->dev.tpb;
+    const size_t lds_orig = dip->dev.lds_rows;
+    for (int c = 0; c < nc && !rc; ++c) {
+        const size_t need = (size_t)((3 * tys[c] + 1) / 2 + 1) * dip->d.nx * sizeof(float2);
+        if (need > G.max_lds) continue;
+        use(tys[c], tpbs[c]);
+        if ((rc = stepper_begin(st))) break;
+        if ((rc = run_steps(st, 2, false))) break;          // capture + warm
+        if (hipEventRecord(e0, G.stream) != hipSuccess) { rc = fail(MSD_EDEVICE, "autotune: event"); break; }
+        if ((rc = run_steps(st, steps, false))) break;
+        if (hipEventRecord(e1, G.stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            rc = fail(MSD_EDEVICE, "autotune: event");
+            break;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        const double per = ms / steps;
+        if (*best_ms < 0 || per < *best_ms) { *best_ms = per; *best_ty = tys[c]; *best_tpb = tpbs[c]; }
+        if ((rc = restore())) break;
+    }
+    if (!rc && *best_ms > 0) use(*best_ty, *best_tpb);
+    else { dip->dev.ty = ty_orig; dip->dev.tpb = tpb_orig; dip->dev.lds_rows = lds_orig; st->drop_graph(); }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(G.stream);
+    dfree(S0); dfree(L0);
+    if (!rc) rc = st->refresh();
+    st->spec_version = 0;
+    return rc;
+}
+
 extern "C" int msd_stepper_set_surrogate(msd_stepper* st, msd_surrogate* sur, int K, double tol)
 {
     NONNULL(st);

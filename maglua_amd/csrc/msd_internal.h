@@ -209,6 +209,8 @@ hipError_t configure_lds_cnn(size_t bytes);
 // Row-tile shapes with compile-time instances: whether (nx, ty, tpb) is one, and
 // the default (ty, tpb) for an nx x ny layer (ty = 0 when none fits).
 bool row_shape_compiled(int nx, int ty, int tpb);
+// every compiled (ty, tpb) for an nx x ny layer, default first; returns the count
+int row_shapes_for(int nx, int ny, int* ty, int* tpb, int cap);
 void row_shape_default(int nx, int ny, int* ty, int* tpb);
 
 }  // namespace msd

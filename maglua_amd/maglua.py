@@ -719,6 +719,13 @@ class Heun(_Handle):
         self._sur = sur
         call("msd_stepper_set_surrogate", self._h, sur.h if sur is not None else None, int(K), float(tol))
 
+    def autotune(self, steps=20):
+        """Time every compiled row-tile shape for `steps` steps, keep the fastest; the state
+        is restored.  Returns {"rows_per_tile", "threads", "ms_per_step"}."""
+        ty, tpb, ms = C.c_int(), C.c_int(), C.c_double()
+        call("msd_stepper_autotune", self._h, int(steps), C.byref(ty), C.byref(tpb), C.byref(ms))
+        return {"rows_per_tile": ty.value, "threads": tpb.value, "ms_per_step": ms.value}
+
     def surrogateStats(self):
         w, e, f = C.c_long(), C.c_long(), C.c_long()
         last, mx = C.c_double(), C.c_double()

@@ -446,3 +446,23 @@ def test_profile_leaves_state_one_step_ahead():
     assert np.array_equal(flat(a["ss"].spins()), flat(b["ss"].spins()))
     assert a["ss"].time() == b["ss"].time()
     assert a["temp"].draw() == b["temp"].draw() == 6
+
+
+def test_autotune_picks_a_compiled_shape_and_leaves_the_trajectory():
+    spec = W.FilmSpec(256, 64, 2, T=0.3, dt=0.004, g=0.02, K=0.05)
+    runs = []
+    for tune in (False, True):
+        g = W.build(M, spec)
+        g["ss"].setSpins(*O.random_unit_spins(spec.seed, spec.n))
+        heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"])
+        heun.step(3)
+        if tune:
+            best = heun.autotune(steps=4)
+            print(f"autotune: {best}", flush=True)
+            assert (best["rows_per_tile"], best["threads"]) in {(8, 1024), (4, 512), (8, 512), (2, 256)}
+            assert best["ms_per_step"] > 0
+            assert g["temp"].draw() == 3
+        heun.step(5)
+        runs.append((flat(g["ss"].spins()), g["ss"].time()))
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert runs[0][1] == runs[1][1]
