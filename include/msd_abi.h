@@ -188,6 +188,9 @@ int msd_longrange3d_apply(msd_longrange3d* l, msd_system* s);
 /* --------------------------------------------------------------------- LLG */
 /* LLG.Cartesian: cpu/llg/llgcartesian.cpp:90-278 (thermalOnlyFirstTerm = true) */
 int msd_llg_create(msd_llg** out);
+/* LLG.Quaternion: cpu/llg/llgquat.cpp:127-373 (rotation of S about the damped
+ * field).  Shares the flags and msd_llg_apply; the fused step driver rejects it. */
+int msd_llg_create_quaternion(msd_llg** out);
 int msd_llg_destroy(msd_llg* l);
 int msd_llg_set_thermal_only_first_term(msd_llg* l, int yes);
 int msd_llg_set_disable_renormalization(msd_llg* l, int yes);

@@ -120,6 +120,7 @@ _SIGS = {
     "msd_ensemble_set_compensated": (C.c_int, [_vp, C.c_int]),
     "msd_ensemble_heun": (C.c_int, [_vp, C.c_int]),
     "msd_llg_create": (C.c_int, [_pp]),
+    "msd_llg_create_quaternion": (C.c_int, [_pp]),
     "msd_llg_destroy": (C.c_int, [_vp]),
     "msd_llg_set_thermal_only_first_term": (C.c_int, [_vp, C.c_int]),
     "msd_llg_set_disable_renormalization": (C.c_int, [_vp, C.c_int]),

@@ -129,8 +129,40 @@ hipError_t launch_thermal_normals(const Dims& d, uint64_t seed, uint64_t draw, f
 }
 
 // --------------------------------------------------------------------- LLG
-// LLGCartesian::apply (cpu/llg/llgcartesian.cpp:90-278).  from/dmdt/to may alias;
-// every site is read before it is written by the same thread.
+// LLGQuaternion::apply at one site (cpu/llg/llgquat.cpp:127-373): S rotated about
+// A = ps H + ds a (M x h)/ms (thermalOnlyFirstTerm; h = H - H_thermal) or
+// A = H + a (S x H)/ms by g |A| dt / (1 + a^2), as the quaternion product
+// q (0,S) q* with q = (cos(th/2), sin(th/2) A/|A|).
+__device__ __forceinline__ float3 quat_step(const LLGDev& lp, float3 S, float3 M, float3 H, float3 h, float al, float ga,
+                                            float ms, float dt)
+{
+    float3 A;
+    if (lp.thermal_first) {
+        const float3 Mh = cross3(M, h);
+        const float f = lp.ds * al / ms;
+        A = make_float3(lp.ps * H.x + f * Mh.x, lp.ps * H.y + f * Mh.y, lp.ps * H.z + f * Mh.z);
+    } else {
+        const float3 SH = cross3(S, H);
+        const float f = al / ms;
+        A = make_float3(H.x + f * SH.x, H.y + f * SH.y, H.z + f * SH.z);
+    }
+    const float len = sqrtf(A.x * A.x + A.y * A.y + A.z * A.z);
+    if (!(len > 0.0f)) return S;
+    float sn, qw;
+    sincosf(0.5f * ga * len / (1.0f + al * al) * dt, &sn, &qw);
+    const float k = sn / len;
+    const float3 qv = make_float3(k * A.x, k * A.y, k * A.z);
+    // t = q (0,S); result = vector part of t q*
+    const float tw = -(qv.x * S.x + qv.y * S.y + qv.z * S.z);
+    const float3 qs = cross3(qv, S);
+    const float3 tv = make_float3(qw * S.x + qs.x, qw * S.y + qs.y, qw * S.z + qs.z);
+    const float3 c = cross3(tv, qv);
+    return make_float3(-tw * qv.x + qw * tv.x - c.x, -tw * qv.y + qw * tv.y - c.y, -tw * qv.z + qw * tv.z - c.z);
+}
+
+// LLGCartesian::apply (cpu/llg/llgcartesian.cpp:90-278) or, with lp.quaternion,
+// LLGQuaternion::apply.  from/dmdt/to may alias; every site is read before it is
+// written by the same thread.
 __global__ void k_llg_op(Dims d, LLGDev lp, float dt, const float* from, const float* dmdt, const float* H,
                          const float* Hth, float* to)
 {
@@ -138,7 +170,21 @@ __global__ void k_llg_op(Dims d, LLGDev lp, float dt, const float* from, const f
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float ms = from[3 * n + i];
         float3 v;
-        if (dt == 0.0f) {
+        if (lp.quaternion) {
+            if (ms > 0.0f) {
+                const float3 S = make_float3(from[i], from[n + i], from[2 * n + i]);
+                const float3 M = make_float3(dmdt[i], dmdt[n + i], dmdt[2 * n + i]);
+                const float3 Hv = make_float3(H[i], H[n + i], H[2 * n + i]);
+                float3 hv = Hv;
+                if (Hth) hv = make_float3(Hv.x - Hth[i], Hv.y - Hth[n + i], Hv.z - Hth[2 * n + i]);
+                const float al = lp.site_alpha ? lp.site_alpha[i] : lp.alpha;
+                const float ga = lp.site_gamma ? lp.site_gamma[i] : lp.gamma;
+                v = quat_step(lp, S, M, Hv, hv, al, ga, ms, dt);
+            } else {
+                v = make_float3(to[i], to[n + i], to[2 * n + i]);
+            }
+            to[3 * n + i] = ms;
+        } else if (dt == 0.0f) {
             v = make_float3(from[i], from[n + i], from[2 * n + i]);
         } else {
             if (ms > 0.0f) {

@@ -854,6 +854,7 @@ extern "C" int msd_thermal_get_draw(const msd_thermal* t, uint64_t* draw) { NONN
 struct msd_llg {
     bool thermal_first = true;
     bool no_renorm = false;
+    bool quaternion = false;    // LLG.Quaternion
     double ps = 1, ds = 1;
 };
 
@@ -862,6 +863,7 @@ LLGDev msd_system::llgdev(const msd_llg* l) const
     LLGDev p{};
     p.thermal_first = l ? l->thermal_first : 1;
     p.renorm = l ? !l->no_renorm : 1;
+    p.quaternion = l ? l->quaternion : 0;
     p.ps = l ? (float)l->ps : 1.0f;
     p.ds = l ? (float)l->ds : 1.0f;
     p.alpha = (float)alpha;
@@ -1165,6 +1167,15 @@ extern "C" int msd_llg_create(msd_llg** out)
     *out = new msd_llg;
     return MSD_OK;
 }
+extern "C" int msd_llg_create_quaternion(msd_llg** out)
+{
+    NONNULL(out);
+    if (int rc = ensure_init()) return rc;
+    msd_llg* l = new msd_llg;
+    l->quaternion = true;
+    *out = l;
+    return MSD_OK;
+}
 extern "C" int msd_llg_destroy(msd_llg* l) { delete l; return MSD_OK; }
 extern "C" int msd_llg_set_thermal_only_first_term(msd_llg* l, int yes) { NONNULL(l); l->thermal_first = yes != 0; return MSD_OK; }
 extern "C" int msd_llg_set_disable_renormalization(msd_llg* l, int yes) { NONNULL(l); l->no_renorm = yes != 0; return MSD_OK; }
@@ -1178,14 +1189,20 @@ extern "C" int msd_llg_apply(msd_llg* l, msd_system* from, double scale, msd_sys
     if (!to) to = from;
     if (!same_dims(from->d, dmdt->d.nx, dmdt->d.ny, dmdt->d.nz) || !same_dims(from->d, to->d.nx, to->d.ny, to->d.nz))
         return fail(MSD_EMISMATCH, "LLG:apply: size mismatch");
+    const double dt = scale * dmdt->dt;
+    if (l->quaternion && dmdt->find("Total") < 0) {
+        // nothing to rotate about: only the time moves (llgquat.cpp:131-137)
+        if (advance) to->time = from->time + dt;
+        return MSD_OK;
+    }
     int T;
     if (int rc = dmdt->slot("Total", &T)) return rc;
     const int TH = dmdt->find("Thermal");
-    const double dt = scale * dmdt->dt;
     hipError_t e = launch_llg_op(from->d, dmdt->llgdev(l), (float)dt, from->S, dmdt->S, dmdt->slots[T].h,
                                  TH >= 0 ? dmdt->slots[TH].h : nullptr, to->S, G.stream);
     if (e != hipSuccess) return fail(MSD_EDEVICE, "LLG:apply: %s", hipGetErrorString(e));
-    if (dt != 0 && advance) to->time = from->time + dt;
+    // LLG.Cartesian leaves the time alone for dt = 0 (llgcartesian.cpp:132-137); LLG.Quaternion sets it
+    if ((dt != 0 || l->quaternion) && advance) to->time = from->time + dt;
     to->lo_valid = false;
     to->version++;
     return MSD_OK;
@@ -1963,6 +1980,9 @@ extern "C" int msd_stepper_create(msd_system* s, msd_exchange* ex, msd_anisotrop
     if ((ex && !same_dims(d, ex->d.nx, ex->d.ny, ex->d.nz)) || (ani && !same_dims(d, ani->d.nx, ani->d.ny, ani->d.nz)) ||
         (dip && !same_dims(d, dip->d.nx, dip->d.ny, dip->d.nz)) || (th && !same_dims(d, th->d.nx, th->d.ny, th->d.nz)))
         return fail(MSD_EMISMATCH, "stepper: operator and SpinSystem sizes differ");
+    if (llg && llg->quaternion)
+        return fail(MSD_EUNSUPPORTED, "stepper: the fused driver integrates LLG.Cartesian; drive LLG.Quaternion "
+                                      "through the operator API (make_pc_step_function)");
     std::unique_ptr<msd_stepper> st(new msd_stepper);
     st->s = s; st->ex = ex; st->an = ani; st->dip = dip; st->th = th;
     if (llg) st->llg = *llg;

@@ -60,6 +60,7 @@ struct ThermalDev {
 struct LLGDev {
     int thermal_first;
     int renorm;
+    int quaternion;         // LLG.Quaternion (rotation) instead of LLG.Cartesian (Euler)
     float ps, ds;           // precession / damping scales
     float alpha, gamma;     // uniform values
     const float* site_alpha;

@@ -525,10 +525,11 @@ class _LLGBase(_Handle):
     """LLG.Cartesian (cpu/llg/llgcartesian.cpp:90-278) with LLG::apply argument rules (llg.cpp:100-171)."""
 
     _destroy = "msd_llg_destroy"
+    _create = "msd_llg_create"
 
     def __init__(self):
         p = _vp()
-        call("msd_llg_create", C.byref(p))
+        call(self._create, C.byref(p))
         super().__init__(p.value)
         self._tf = True
         self._nr = False
@@ -576,11 +577,22 @@ class _LLGNamespace:
         def new(cls):
             return cls()
 
+    class Quaternion(_LLGBase):
+        """LLG.Quaternion (cpu/llg/llgquat.cpp:127-373): each apply rotates the spin about
+        the damped field by g|H'|dt/(1+a^2); same argument rules and flags as Cartesian."""
+        _create = "msd_llg_create_quaternion"
+
+        @classmethod
+        def new(cls):
+            return cls()
+
     @staticmethod
     def new(kind="Cartesian"):
-        if kind != "Cartesian":
-            raise MagLuaError(-3, f"LLG.{kind} is not provided by this build (LLG.Cartesian is)")
-        return _LLGNamespace.Cartesian()
+        if kind == "Cartesian":
+            return _LLGNamespace.Cartesian()
+        if kind == "Quaternion":
+            return _LLGNamespace.Quaternion()
+        raise MagLuaError(-3, f"LLG.{kind} is not provided by this build (LLG.Cartesian and LLG.Quaternion are)")
 
 
 LLG = _LLGNamespace

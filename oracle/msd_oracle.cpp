@@ -942,13 +942,68 @@ int ora_lr3_apply(ora_lr3* l, ora_sys* s)
 struct ora_llg {
     bool thermal_first = true;
     bool no_renorm = false;
+    bool quaternion = false;     /* LLG.Quaternion instead of LLG.Cartesian */
     double precess_scale = 1, damp_scale = 1;
 };
+
+/*
+ * LLG.Quaternion (cpu/llg/llgquat.cpp:127-373): S from "from" is rotated about
+ * H' = ps H + ds a (M x h)/ms (thermal_first; M from dmdt, h = H - H_thermal) or
+ * H' = H + a (S x H)/ms (otherwise) by the angle g |H'| dt / (1 + a^2), written
+ * as the quaternion product q (0,S) q* with q = (cos(th/2), sin(th/2) H'/|H'|).
+ * to.ms = from.ms everywhere; sites with ms = 0 keep to's spin; the time is set
+ * whenever advance is asked (dt = 0 included); renormalised to |from| unless disabled.
+ */
+static void llg_quaternion(const ora_llg* l, const ora_sys* from, double scale, const ora_sys* dmdt, ora_sys* to,
+                           int advance, int S, int TH)
+{
+    const int n = from->n;
+    const double dt = scale * dmdt->dt;
+    for (int i = 0; i < n; ++i) {
+        const double msf = from->ms[i];
+        to->ms[i] = msf;
+        if (!(msf > 0)) continue;
+        const double al = dmdt->a(i), ga = dmdt->g(i);
+        const double Sv[3] = {from->x[i], from->y[i], from->z[i]};
+        double H[3] = {dmdt->hx[S][i], dmdt->hy[S][i], dmdt->hz[S][i]};
+        double A[3];
+        if (l->thermal_first) {
+            const double M[3] = {dmdt->x[i], dmdt->y[i], dmdt->z[i]};
+            double h[3] = {H[0], H[1], H[2]};
+            if (TH >= 0) { h[0] -= dmdt->hx[TH][i]; h[1] -= dmdt->hy[TH][i]; h[2] -= dmdt->hz[TH][i]; }
+            const double Mh[3] = {M[1] * h[2] - M[2] * h[1], M[2] * h[0] - M[0] * h[2], M[0] * h[1] - M[1] * h[0]};
+            for (int c = 0; c < 3; ++c) A[c] = l->precess_scale * H[c] + l->damp_scale * al * Mh[c] / msf;
+        } else {
+            const double SH[3] = {Sv[1] * H[2] - Sv[2] * H[1], Sv[2] * H[0] - Sv[0] * H[2], Sv[0] * H[1] - Sv[1] * H[0]};
+            for (int c = 0; c < 3; ++c) A[c] = H[c] + al * SH[c] / msf;
+        }
+        const double len = std::sqrt(A[0] * A[0] + A[1] * A[1] + A[2] * A[2]);
+        if (len > 0) {
+            const double half = 0.5 * ga * len / (1.0 + al * al) * dt;
+            const double qw = std::cos(half), sn = std::sin(half);
+            const double qv[3] = {sn * A[0] / len, sn * A[1] / len, sn * A[2] / len};
+            /* t = q (0, S); r = t q*, vector part */
+            const double tw = -(qv[0] * Sv[0] + qv[1] * Sv[1] + qv[2] * Sv[2]);
+            const double tv[3] = {qw * Sv[0] + qv[1] * Sv[2] - qv[2] * Sv[1],
+                                  qw * Sv[1] + qv[2] * Sv[0] - qv[0] * Sv[2],
+                                  qw * Sv[2] + qv[0] * Sv[1] - qv[1] * Sv[0]};
+            const double cx = tv[1] * qv[2] - tv[2] * qv[1], cy = tv[2] * qv[0] - tv[0] * qv[2],
+                         cz = tv[0] * qv[1] - tv[1] * qv[0];
+            to->x[i] = -tw * qv[0] + qw * tv[0] - cx;
+            to->y[i] = -tw * qv[1] + qw * tv[1] - cy;
+            to->z[i] = -tw * qv[2] + qw * tv[2] - cz;
+        } else {
+            to->x[i] = Sv[0]; to->y[i] = Sv[1]; to->z[i] = Sv[2];
+        }
+    }
+    if (advance) to->time = from->time + dt;
+}
 
 extern "C" {
 ora_llg* ora_llg_new(void) { return new ora_llg; }
 void ora_llg_free(ora_llg* l) { delete l; }
 void ora_llg_set_flags(ora_llg* l, int tf, int nr) { l->thermal_first = tf != 0; l->no_renorm = nr != 0; }
+void ora_llg_set_quaternion(ora_llg* l, int yes) { l->quaternion = yes != 0; }
 
 /*
  * to = N(from + scale*dt*dM(dmdt)), dM = -g/(1+a^2) M x H - a/ms g/(1+a^2) M x (M x h),
@@ -966,7 +1021,9 @@ int ora_llg_apply(ora_llg* l, const ora_sys* from, double scale, const ora_sys* 
     const int n = from->n;
     const double dt = scale * dmdt->dt;
     /* from and to may alias: copy what is read per site first */
-    if (dt == 0) {
+    if (l->quaternion) {
+        llg_quaternion(l, from, scale, dmdt, to, advance, S, TH);
+    } else if (dt == 0) {
         if (from != to) { to->x = from->x; to->y = from->y; to->z = from->z; }
     } else {
         for (int i = 0; i < n; ++i) {

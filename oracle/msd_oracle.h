@@ -113,6 +113,7 @@ int      ora_lr3_apply(ora_lr3*, ora_sys*);
 ora_llg* ora_llg_new(void);
 void     ora_llg_free(ora_llg*);
 void     ora_llg_set_flags(ora_llg*, int thermal_only_first_term, int disable_renorm);
+void     ora_llg_set_quaternion(ora_llg*, int yes);   /* LLG.Quaternion (llgquat.cpp) */
 int      ora_llg_apply(ora_llg*, const ora_sys* from, double scale, const ora_sys* dmdt, ora_sys* to, int advance);
 
 /* ---- integrators (PredictorCorrector.lua / RungeKutta.lua) over a fixed operator set ---- */

@@ -101,6 +101,7 @@ def lib(threads=False):
         "ora_llg_new": (vp, []),
         "ora_llg_free": (None, [vp]),
         "ora_llg_set_flags": (None, [vp, C.c_int, C.c_int]),
+        "ora_llg_set_quaternion": (None, [vp, C.c_int]),
         "ora_llg_apply": (C.c_int, [vp, vp, C.c_double, vp, vp, C.c_int]),
         "ora_sim_new": (vp, [vp, vp, vp, vp, vp, vp]),
         "ora_sim_free": (None, [vp]),
@@ -440,9 +441,11 @@ class LongRange3D:
 
 
 class LLG:
-    def __init__(self, threads=False):
+    def __init__(self, threads=False, quaternion=False):
         self.L = lib(threads)
         self.h = self.L.ora_llg_new()
+        if quaternion:
+            self.L.ora_llg_set_quaternion(self.h, 1)
 
     def __del__(self):
         if getattr(self, "h", None):

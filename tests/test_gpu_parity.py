@@ -466,3 +466,79 @@ def test_autotune_picks_a_compiled_shape_and_leaves_the_trajectory():
         runs.append((flat(g["ss"].spins()), g["ss"].time()))
     assert np.array_equal(runs[0][0], runs[1][0])
     assert runs[0][1] == runs[1][1]
+
+
+# ------------------------------------------------------- LLG.Quaternion
+@pytest.mark.parametrize("thermal_first", [True, False])
+def test_quaternion_llg_op_matches_oracle(thermal_first):
+    spec = W.FilmSpec(8, 8, 2, T=0.7, dt=0.03, alpha=0.3)
+    g, o = twin(spec)
+    ms = 0.5 + np.random.default_rng(4).random(spec.n)      # moment sizes other than 1 (|s| = ms)
+    x, y, z = O.random_unit_spins(spec.seed, spec.n)
+    for d in (g, o):
+        d["ss"].setSpins(x * ms, y * ms, z * ms)
+    lg, lo = M.LLG.Quaternion(), O.LLG(quaternion=True)
+    lg.setThermalOnlyFirstTerm(thermal_first)
+    lo.setFlags(thermal_only_first_term=thermal_first)
+    for d, seed in ((g, None), (o, 5)):
+        ss = d["ss"]
+        ss.resetFields()
+        d["ex"].apply(ss)
+        d["ani"].apply(ss)
+        d["dip"].apply(ss)
+        if seed is None:
+            d["temp"].setSeed(5)
+            d["temp"].apply(ss)
+        else:
+            d["temp"].apply(ss, seed=5, step=0)
+        ss.sumFields()
+    P_g, P_o = g["ss"].copy(), o["ss"].copy()
+    lg.apply(g["ss"], 0.5, g["ss"], P_g)
+    lo.apply(o["ss"], 0.5, o["ss"], P_o)
+    err = np.abs(flat(P_g.spins()) - flat(P_o.spins())).max()
+    print(f"quaternion op (thermal_first={thermal_first}): max err {err:.2e}", flush=True)
+    assert err < 2e-6
+    assert abs(P_g.time() - 0.015) < 1e-15
+
+
+def _oracle_pc_step(d, llg, calc):
+    ss = d["ss"]
+    P, Cc = d.setdefault("_P", ss.copy()), d.setdefault("_C", ss.copy())
+    calc(ss)
+    llg.apply(ss, 1.0, ss, P)
+    llg.apply(ss, 0.5, ss, Cc)
+    calc(P)
+    llg.apply(Cc, 0.5, P, Cc)
+    O.lib().ora_sys_copy_spins_to(Cc.h, ss.h)
+    ss.setTime(Cc.time())
+
+
+def test_quaternion_llg_pc_driver_matches_oracle():
+    spec = W.FilmSpec(16, 16, 2, T=0.0, dt=0.01, alpha=0.2, g=0.02)
+    g, o = twin(spec)
+
+    def calc_of(d):
+        def calc(ss):
+            ss.resetFields()
+            d["ex"].apply(ss)
+            d["ani"].apply(ss)
+            d["dip"].apply(ss)
+            ss.sumFields()
+        return calc
+
+    lg, lo = M.LLG.new("Quaternion"), O.LLG(quaternion=True)
+    step = M.make_pc_step_function(g["ss"], calc_of(g), lg, 1e9)
+    for _ in range(30):
+        step()
+        _oracle_pc_step(o, lo, calc_of(o))
+    err = _max_dm(g, o)
+    print(f"quaternion PC driver, 30 steps: max|dm| {err:.2e}", flush=True)
+    assert err <= 1e-5
+    assert abs(g["ss"].time() - o["ss"].time()) < 1e-12
+
+
+def test_fused_driver_rejects_quaternion():
+    spec = W.FilmSpec(16, 16, 1, g=0.0)
+    g = W.build(M, spec)
+    with pytest.raises(M.MagLuaError):
+        M.Heun(g["ss"], g["ex"], g["ani"], None, None, M.LLG.Quaternion())

@@ -322,6 +322,63 @@ def test_heun_second_order_against_closed_form():
     assert 3.6 < errs[0] / errs[1] < 4.4 and errs[1] < 1e-5
 
 
+def rodrigues(v, axis, angle):
+    """Rotation of v by `angle` about `axis` (right-handed), Rodrigues' formula."""
+    k = np.asarray(axis, dtype=float) / np.linalg.norm(axis)
+    v = np.asarray(v, dtype=float)
+    return v * np.cos(angle) + np.cross(k, v) * np.sin(angle) + k * np.dot(k, v) * (1 - np.cos(angle))
+
+
+def test_quaternion_llg_is_exact_rotation_without_damping():
+    """alpha = 0: every step is the exact precession, whatever dt."""
+    for dt in (0.3, 0.05):
+        ss, llg = _spin(dt), O.LLG(quaternion=True)
+        ss.setParams(alpha=0.0)
+        for _ in range(int(round(3.0 / dt))):
+            _applied(ss)
+            llg.apply(ss)
+        x, y, z, m = ss.spins()
+        want = np.array([np.sin(1.0) * np.cos(GAMMA * B * 3.0), np.sin(1.0) * np.sin(GAMMA * B * 3.0), np.cos(1.0)])
+        assert np.abs(np.array([x[0], y[0], z[0]]) - want).max() < 1e-12
+        assert abs(ss.time() - 3.0) < 1e-12
+
+
+def test_quaternion_llg_first_order_to_closed_form():
+    errs = []
+    for dt in (0.02, 0.01):
+        ss, llg = _spin(dt), O.LLG(quaternion=True)
+        for _ in range(int(round(2.0 / dt))):
+            _applied(ss)
+            llg.apply(ss)
+        x, y, z, m = ss.spins()
+        errs.append(np.abs(np.array([x[0], y[0], z[0]]) - llg_exact(2.0)).max())
+    assert 1.8 < errs[0] / errs[1] < 2.2 and errs[1] < 5e-3
+
+
+def test_quaternion_llg_step_matches_rodrigues_with_thermal_split():
+    """thermalOnlyFirstTerm: precession about the full field, damping from the field
+    without the thermal slot, one rotation about H' = H + a (M x h)/ms by g|H'|dt/(1+a^2)."""
+    rng = np.random.default_rng(8)
+    S0 = rng.standard_normal(3)
+    S0 *= 1.7 / np.linalg.norm(S0)
+    Happ, Hth = rng.standard_normal(3), rng.standard_normal(3) * 0.3
+    al, ga, dt = 0.2, 1.3, 0.07
+    ss = O.SpinSystem(1)
+    ss.setSpins([S0[0]], [S0[1]], [S0[2]])
+    ss.setParams(alpha=al, gamma=ga, dt=dt, time=0)
+    ss.resetFields()
+    ss.setField("Applied", *([v] for v in Happ))
+    ss.setField("Thermal", *([v] for v in Hth))
+    ss.sumFields()
+    O.LLG(quaternion=True).apply(ss)
+    H = Happ + Hth
+    A = H + al * np.cross(S0, Happ) / 1.7
+    want = rodrigues(S0, A, ga * np.linalg.norm(A) / (1 + al * al) * dt)
+    x, y, z, m = ss.spins()
+    assert np.abs(np.array([x[0], y[0], z[0]]) - want).max() < 1e-12
+    assert abs(m[0] - 1.7) < 1e-14
+
+
 def aniso_exact(t, K, th0=1.0):
     gp = GAMMA / (1 + ALPHA ** 2)
     c = 2 * ALPHA * gp * K
