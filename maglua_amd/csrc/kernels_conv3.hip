@@ -371,10 +371,117 @@ __global__ void k3_dftyz(int fy, int fz, int fxh, const double2* __restrict__ U,
     }
 }
 
+// ---------------------------------------------------------------- FP64 FFT
+// Mixed-radix Stockham transforms of FP64 lines in LDS for the tensor spectrum
+// (forward sign, the table tw[k] = exp(-2 pi i k / L) of the axis).  A workgroup
+// takes `lpb` lines at a time.  Line maps over the [6][fz][fy][*] arrays:
+//   kind 0: real rows T[a][z][y][0..fx) -> half spectra U[a][z][y][0..fxh);
+//   kind 1: columns along y of [6][fz][fy][fxh]; kind 2: columns along z.
+struct Lines64 {
+    int kind, fx, fy, fz, fxh;
+};
+
+__device__ __forceinline__ double2 zmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ size_t line_elem(const Lines64& m, int line, int t)
+{
+    if (m.kind == 1) {
+        const int a = line / (m.fz * m.fxh), r = line - a * m.fz * m.fxh, z = r / m.fxh, kx = r - z * m.fxh;
+        return (((size_t)a * m.fz + z) * m.fy + t) * m.fxh + kx;
+    }
+    const int a = line / (m.fy * m.fxh), r = line - a * m.fy * m.fxh, y = r / m.fxh, kx = r - y * m.fxh;
+    return (((size_t)a * m.fz + t) * m.fy + y) * m.fxh + kx;
+}
+
+__global__ void k3_fft64(Lines64 m, FFTPlan p, const double2* __restrict__ tw, const double* __restrict__ Treal,
+                         const double2* __restrict__ in, double2* __restrict__ out, int nlines, int lpb)
+{
+    extern __shared__ double2 z64[];
+    const int L = p.L;
+    for (int lb = blockIdx.x * lpb; lb < nlines; lb += gridDim.x * lpb) {
+        const int nl = min(lpb, nlines - lb);
+        double2* A = z64;
+        double2* Bf = z64 + (size_t)lpb * L;
+        // load: rows with t fastest (contiguous), columns with the line fastest (adjacent kx)
+        for (int e = threadIdx.x; e < nl * L; e += blockDim.x) {
+            int ln, t;
+            if (m.kind == 0) { ln = e / L; t = e - ln * L; }
+            else { t = e / nl; ln = e - t * nl; }
+            const int line = lb + ln;
+            double2 v;
+            if (m.kind == 0) v = make_double2(Treal[(size_t)line * m.fx + t], 0.0);
+            else v = in[line_elem(m, line, t)];
+            A[ln * L + t] = v;
+        }
+        __syncthreads();
+        int Ns = 1;
+        double2* src = A;
+        double2* dst = Bf;
+        for (int s = 0; s < p.nst; ++s) {
+            const int R = p.rad[s], nb = L / R, tstep = L / (Ns * R), rstep = L / R;
+            for (int b = threadIdx.x; b < nl * nb; b += blockDim.x) {
+                const int ln = b / nb, j = b - ln * nb, k = j % Ns;
+                double2 v[8], V[8];
+                for (int r = 0; r < R; ++r) {
+                    v[r] = src[ln * L + j + r * nb];
+                    if (Ns > 1 && r) v[r] = zmul(v[r], tw[k * r * tstep]);
+                }
+                for (int q = 0; q < R; ++q) {
+                    double2 acc = v[0];
+                    for (int r = 1; r < R; ++r) {
+                        const double2 w = tw[((r * q) % R) * rstep];
+                        acc.x += v[r].x * w.x - v[r].y * w.y;
+                        acc.y += v[r].x * w.y + v[r].y * w.x;
+                    }
+                    V[q] = acc;
+                }
+                double2* d = dst + ln * L + (j / Ns) * Ns * R + k;
+                for (int q = 0; q < R; ++q) d[q * Ns] = V[q];
+            }
+            __syncthreads();
+            double2* t2 = src; src = dst; dst = t2;
+            Ns *= R;
+        }
+        for (int e = threadIdx.x; e < nl * L; e += blockDim.x) {
+            int ln, t;
+            if (m.kind == 0) { ln = e / L; t = e - ln * L; }
+            else { t = e / nl; ln = e - t * nl; }
+            const int line = lb + ln;
+            if (m.kind == 0) { if (t < m.fxh) out[(size_t)line * m.fxh + t] = src[ln * L + t]; }
+            else out[line_elem(m, line, t)] = src[ln * L + t];
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void k3_words(const double2* __restrict__ V, float* __restrict__ Tw, double mult, size_t n)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         Tw[i] = (float)(V[i].x * mult);
+}
+
+// One axis of the FP64 tensor transform as LDS Stockham passes; lines per
+// workgroup sized to 64 KiB of LDS (two buffers).
+static hipError_t fft64_axis(int kind, int fx, int fy, int fz, const FFTPlan& p, const double2* tw, const double* T,
+                             const double2* in, double2* out, hipStream_t st)
+{
+    const int fxh = fx / 2 + 1;
+    const Lines64 m{kind, fx, fy, fz, fxh};
+    const int L = p.L;
+    const int nlines = kind == 0 ? 6 * fz * fy : (kind == 1 ? 6 * fz * fxh : 6 * fy * fxh);
+    const int lpb = std::max(1, std::min(64, (int)(65536 / (2 * (size_t)L * sizeof(double2)))));
+    const size_t lds = (size_t)2 * lpb * L * sizeof(double2);
+    const int g = std::min((nlines + lpb - 1) / lpb, 4096);
+    k3_fft64<<<g, 256, lds, st>>>(m, p, tw, T, in, out, nlines, lpb);
+    return hipGetLastError();
+}
+
+hipError_t configure_fft64(size_t bytes)
+{
+    return hipFuncSetAttribute((const void*)k3_fft64, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 hipError_t launch3_spectrum(int fx, int fy, int fz, const double* T, double2* U1, double2* U2, float* Tw, double mult,
@@ -383,9 +490,26 @@ hipError_t launch3_spectrum(int fx, int fy, int fz, const double* T, double2* U1
     const int fxh = fx / 2 + 1;
     const size_t tot = (size_t)6 * fz * fy * fxh;
     const int g = (int)std::min<size_t>((tot + 255) / 256, 65536);
-    k3_dftx<<<g, 256, 0, st>>>(fx, fy, fz, T, U1, twx);
-    k3_dftyz<1><<<g, 256, 0, st>>>(fy, fz, fxh, U1, U2, twy);
-    k3_dftyz<2><<<g, 256, 0, st>>>(fy, fz, fxh, U2, U1, twz);
+    FFTPlan px, py, pz;
+    // transforms need each length to factor into 2, 3 and 5 (radix <= 5 here); otherwise direct DFTs
+    auto plan64 = [](int L, FFTPlan& q) {
+        q.L = L;
+        q.nst = 0;
+        int mm = L;
+        for (int r : {4, 2, 3, 5})
+            while (mm % r == 0 && q.nst < 16) { q.rad[q.nst++] = r; mm /= r; }
+        return mm == 1;
+    };
+    if (plan64(fx, px) && plan64(fy, py) && plan64(fz, pz) && fx <= 4096 && fy <= 4096 && fz <= 4096) {
+        hipError_t e = fft64_axis(0, fx, fy, fz, px, twx, T, nullptr, U1, st);
+        if (e == hipSuccess) e = fft64_axis(1, fx, fy, fz, py, twy, nullptr, U1, U2, st);
+        if (e == hipSuccess) e = fft64_axis(2, fx, fy, fz, pz, twz, nullptr, U2, U1, st);
+        if (e != hipSuccess) return e;
+    } else {
+        k3_dftx<<<g, 256, 0, st>>>(fx, fy, fz, T, U1, twx);
+        k3_dftyz<1><<<g, 256, 0, st>>>(fy, fz, fxh, U1, U2, twy);
+        k3_dftyz<2><<<g, 256, 0, st>>>(fy, fz, fxh, U2, U1, twz);
+    }
     k3_words<<<g, 256, 0, st>>>(U1, Tw, mult, tot);
     return hipGetLastError();
 }

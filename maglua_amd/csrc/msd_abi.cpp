@@ -112,6 +112,7 @@ extern "C" int msd_init(int device)
     G.max_lds = prop.sharedMemPerBlock > 16384 ? prop.sharedMemPerBlock - 8192 : prop.sharedMemPerBlock;
     if (configure_lds(G.max_lds) != hipSuccess || configure_lds3(G.max_lds) != hipSuccess)
         G.max_lds = std::min<size_t>(G.max_lds, 65536 - 8192);
+    (void)configure_fft64(65536);
     HIPCHK(hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking));
     G.device = device;
     G.ready = true;
@@ -1273,6 +1274,7 @@ extern "C" int msd_longrange3d_create(int nx, int ny, int nz, int open, msd_long
     if (!make_plan(l->fx, l->px) || !make_plan(l->fy, l->py) || !make_plan(l->fz, l->pz))
         return fail(MSD_EUNSUPPORTED, "LongRange3D.new: %dx%dx%d transform needs prime factors in {2,3,5}", l->fx, l->fy, l->fz);
     l->fxh = l->fx / 2 + 1;
+    // 64 KiB tiles: measured faster at 256^3 open than the full LDS (occupancy)
     const size_t budget = 65536;
     l->ty = 1;
     for (int c : {8, 4, 2})

@@ -182,6 +182,7 @@ hipError_t launch3_tensor(int nx, int ny, int nz, int fx, int fy, int fz, int R,
 hipError_t launch3_spectrum(int fx, int fy, int fz, const double* T, double2* U1, double2* U2, float* Tw, double mult,
                             const double2* twx, const double2* twy, const double2* twz, hipStream_t st);
 hipError_t configure_lds3(size_t max_bytes);
+hipError_t configure_fft64(size_t bytes);
 const char* kernel_name(int id);
 // allow row/column kernels the device's full LDS per workgroup
 hipError_t configure_lds(size_t max_bytes);
