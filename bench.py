@@ -159,6 +159,83 @@ def learned_vs_plain(M, W, train_steps=300, K=10, tol=1e-2, steps=500):
     }
 
 
+def ensemble_leg(M, W, nrep=64, steps=50, R=16):
+    """64 periodic replicas with sizes drawn (seed 0) from {32^3, 64^3, 128x128x16, 96x96x24},
+    temperatures 0.1..2.0, each with its own seed, LongRange3D dipole (lattice sum to R)
+    g=0.01, J=1, K=0.1: one set of launches per Heun step for the whole batch."""
+    import numpy as np
+    rng = np.random.default_rng(0)
+    sizes = [(32, 32, 32), (64, 64, 64), (128, 128, 16), (96, 96, 24)]
+    pick = rng.integers(0, 4, nrep)
+    t0 = time.perf_counter()
+    members, keep, nsites, temps = [], [], 0, []
+    for i in range(nrep):
+        dims = sizes[pick[i]]
+        T = 0.1 + (2.0 - 0.1) * i / max(1, nrep - 1)
+        spec = W.FilmSpec(*dims, J=1.0, K=0.1, g=0.0, pbc=(1, 1, 1), alpha=0.1, dt=0.002, T=T, seed=i)
+        g = W.build(M, spec)
+        g["ss"].randomize(i)
+        g["temp"].setSeed(i)
+        lr = M.LongRange3D(*dims)
+        lr.setRadius(R)
+        lr.setStrength(0.01)
+        members.append(M.Heun(g["ss"], g["ex"], g["ani"], lr, g["temp"], g["llg"]))
+        keep.append((g, lr))
+        nsites += spec.n
+        temps.append(T)
+    E = M.Ensemble(members)
+    E.step(2)
+    M.synchronize()
+    setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    E.step(steps)
+    M.synchronize()
+    el = time.perf_counter() - t0
+    return {"workload": f"{nrep} periodic replicas, sizes from 32^3/64^3/128x128x16/96x96x24 (seed 0), "
+                        f"T 0.1..2.0, LongRange3D lattice sum R={R}, g=0.01, J=1, K=0.1",
+            "value": round(nsites * steps / el, 1), "unit": "spin-steps/s", "sites": nsites, "steps": steps,
+            "ms_per_step": round(1e3 * el / steps, 4), "setup_s": round(setup, 2)}
+
+
+def cube_leg(M, W, n=256, steps=20):
+    """256^3 open cube (zero-padded 512^3 transforms), J=1, K=0.05 along seeded random axes,
+    g=0.05, T=0.3, dt=0.002, and a 4-replica batch of 128^3 open cubes in one launch."""
+    import numpy as np
+    out = {}
+    for label, dims, count in (("single", (n, n, n), 1), ("batch4", (n // 2, n // 2, n // 2), 4)):
+        t0 = time.perf_counter()
+        members, keep, nsites = [], [], 0
+        for r in range(count):
+            spec = W.FilmSpec(*dims, J=1.0, K=0.05, g=0.0, pbc=(0, 0, 0), alpha=0.1, dt=0.002, T=0.3, seed=r)
+            g = W.build(M, spec, with_thermal=True)
+            rng = np.random.default_rng(100 + r)
+            ax = rng.standard_normal((spec.n, 3))
+            ax /= np.linalg.norm(ax, axis=1, keepdims=True)
+            ani = M.Anisotropy(*dims)
+            ani.addMany(np.arange(spec.n, dtype=np.int32), ax, 0.05)
+            g["ss"].randomize(r)
+            g["temp"].setSeed(r)
+            lr = M.LongRange3D(*dims, open=True)
+            lr.setStrength(0.05)
+            members.append(M.Heun(g["ss"], g["ex"], ani, lr, g["temp"], g["llg"]))
+            keep.append((g, ani, lr))
+            nsites += spec.n
+        drv = members[0] if count == 1 else M.Ensemble(members)
+        drv.step(2)
+        M.synchronize()
+        setup = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        drv.step(steps)
+        M.synchronize()
+        el = time.perf_counter() - t0
+        out[label] = {"lattice": list(dims), "replicas": count, "value": round(nsites * steps / el, 1),
+                      "ms_per_step": round(1e3 * el / steps, 3), "setup_s": round(setup, 2)}
+        del drv, members, keep
+    out["workload"] = "open cubes, LongRange3D zero-padded to twice the size, random easy axes, g=0.05, T=0.3"
+    out["unit"] = "spin-steps/s"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +244,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain comparison on the 512x512x4 film")
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched-replica and 256^3 open-cube legs")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--plain", action="store_true", help="plain FP32 spin state instead of the compensated pair")
     ap.add_argument("--calibrate", action="store_true",
@@ -224,6 +302,12 @@ def main():
         print("# learned vs plain (512x512x4, training the surrogate first)...", file=sys.stderr, flush=True)
         learned = learned_vs_plain(M, W)
 
+    ensemble = cube = None
+    if not args.no_extra:
+        print("# batched replicas and open cubes...", file=sys.stderr, flush=True)
+        ensemble = ensemble_leg(M, W)
+        cube = cube_leg(M, W)
+
     cpu = None
     if not args.no_cpu:
         print("# CPU baseline (FP64 oracle, 1 thread)...", file=sys.stderr, flush=True)
@@ -248,6 +332,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "learned": learned,
+        "ensemble": ensemble,
+        "cube": cube,
     }
     print(json.dumps(out), flush=True)
 
