@@ -83,7 +83,7 @@ def cpu_baseline(spec, budget_s=15.0):
 
 BF16_DENSE_PEAK = 2.5e15   # MI355X published dense BF16 MFMA FLOP/s
 CNN_USEFUL_FLOP = 2 * 75 * (3 * 16 + 16 * 16 + 16 * 3)      # per site per evaluation (52,800)
-CNN_EXEC_FLOP = 3 * 38 * 2 * 16 * 32                          # per site: 3 layers x 38 MFMA steps, 16 channels padded
+CNN_EXEC_FLOP = 64 * 2 * 16 * 16 * 32 // 16                  # per site: 64 16x16x32 MFMAs per 16 sites (10 + 45 + 9)
 
 
 def learned_vs_plain(M, W, train_steps=300, K=10, tol=1e-2, steps=500):

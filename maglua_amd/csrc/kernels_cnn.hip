@@ -5,17 +5,30 @@
 // field (LongRange::apply, cpu/longrange/spinoperationlongrange.cpp:354-430)
 // between exact refreshes.
 //
-// Activations are channels-last BF16 with 16 channels per site (32 bytes; the
-// 3-channel input and output are zero-padded).  A layer is
+// Activations are channels-last BF16: 16 channels (32 bytes) per site for the
+// hidden layers, 4 (8 bytes, the 4th zero) for the 3-channel network input and
+// output gradient.  A layer is
 //   Y[s][o] = sum_t sum_i W[t][o][i] X[s + off(t)][i],
 //   off(t) = (dx, dy, dz) = (t%5 - 2, (t/5)%5 - 2, t/25 - 1),
-// periodic in x and y, zero outside the lattice in z.  One MFMA covers two taps
-// (K = 2 taps x 16 input channels), so 38 MFMA steps cover the 75 taps (the
-// 76th tap's weights are zero).
+// periodic in x and y, zero outside the lattice in z.
+//
+// Row walk.  A wave owns 16 XF output sites along x of one layer z and walks the
+// input rows y' of its tile in order.  Input row y' feeds output rows y' - dy,
+// dy = -2..2, so every B fragment read from LDS feeds five MFMAs (one per dy, into
+// five rotating accumulators) and the A fragments (the weights) live in registers
+// for the whole kernel.  Per 16 sites and input row:
+//   CV_F16  (16 -> 16): K = 2 dx x 16 channels, 9 k-steps (dz, dx pair) x 5 dy;
+//   CV_IN3  ( 3 -> 16): K = 8 (dx, dz) taps x 4 channels, 2 k-steps x 5 dy;
+//   CV_OUT3 (16 ->  3): M = 15 (dy, channel) rows, 9 k-steps, one accumulator
+//                       whose rows are added into an LDS buffer of output rows.
+// That is 64 MFMAs per 16 sites for the network (114 with 16-channel padding of
+// every layer and one MFMA per B fragment).
 //
 // MFMA operand layout (16x16x32, checked on the device): lane l holds
 // A[l%16][8(l/16) .. +7], B[8(l/16) .. +7][l%16], and D[4(l/16) .. +3][l%16].
 #include "msd_device.h"
+
+#include <type_traits>
 
 namespace msd {
 
@@ -23,11 +36,18 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 static constexpr int CNN_TPB = 256;                  // 4 waves
-static constexpr int CTX = 32, CTY = 4;              // output tile: 32 x, 4 rows (one per wave), one layer
+static constexpr int CTX = 32, CTY = 4;              // weight-gradient tile: 32 x, 4 rows, one layer
 static constexpr int HX = CTX + 4, HY = CTY + 4, HZ = 3;
-static constexpr int HALO = HX * HY * HZ;            // input sites staged per tile
-static constexpr int NSTEP = 38;                     // MFMA steps per layer
+static constexpr int HALO = HX * HY * HZ;            // input sites staged per weight-gradient tile
 static constexpr int NTAP = 75;
+static constexpr int RXF = 2;                        // 16-site fragments per wave in the row walk
+static constexpr int RWX = 64 * RXF;                 // sites along x per row-walk workgroup
+static constexpr int RXW = RWX + 4;                  // staged sites per input row and layer
+
+template <int V> struct CnnShape {
+    static constexpr int NF = V == CV_F16 ? 45 : V == CV_IN3 ? 10 : 9;   // A fragments
+    static constexpr bool NARROW = V == CV_IN3;                          // 8-byte input sites
+};
 
 __device__ __forceinline__ uint16_t f2bf(float f)
 {
@@ -62,82 +82,282 @@ __device__ __forceinline__ int wrap(int v, int n)
     return v - (v >= n ? n : 0);
 }
 
-// ------------------------------------------------------------ layer
-// One persistent workgroup per CU pair: the layer's packed weights are staged
-// in LDS once, then each output tile's input halo is staged and every wave runs
-// the 38 MFMA steps for its row's two 16-site tiles.
-//   mode 0: BF16 channels-last output (hidden layers);
-//   mode 1: FP32 SoA output[c][n] = href[c][n] (if given) + scale * Y[c], c < 3.
-__global__ void __launch_bounds__(CNN_TPB) k_cnn_layer(CnnDims d, const uint4* __restrict__ in,
-                                                    const uint4* __restrict__ apack, int mode,
-                                                    uint4* __restrict__ out_bf, float* __restrict__ out_f,
-                                                    const float* __restrict__ href, float scale)
+// ------------------------------------------------------------ layer (row walk)
+// Persistent: workgroups take items (RWX sites along x, wk.ty output rows, one
+// layer) in turn and walk the input rows y0-2 .. y0+ny_out+1 of the item.  Each
+// input row goes straight from global memory into one of five LDS slots
+// (global_load_lds_dwordx4, no registers held for rows in flight), WALK_RPF rows
+// ahead of the row being multiplied.  The slots are separate arrays selected at
+// compile time, so the compiler sees that the LDS reads of the current row do not
+// alias the loads in flight; the barrier closing a row waits (vmcnt) only for the
+// loads of the next row: every wave issues at least WALK_GUAR(V) loads per row,
+// and the loads of later rows are younger.
+//
+// Every row issues all five dy products: an accumulator slot is cleared when its
+// output row completes, so the products of the first rows that belong to rows
+// before y0 never reach a stored row, and the last rows' products for rows past
+// the tile are dropped.  The dz layers outside the lattice are skipped at compile
+// time (four specialisations of the walk) where the taps of a k-step share one
+// layer (CV_F16, CV_OUT3); the 3-channel input (CV_IN3, taps of two layers per
+// k-step) stages zeros for them instead, read from a zeroed device buffer.
+//   CV_F16, CV_IN3: BF16 [n][16] output;
+//   CV_OUT3: FP32 SoA output[c][n] = scale * Y[c].  The product of input row j
+//   has rows (ky, c), ky = dy + 2: lane group g (rows 4g..4g+3) holds ky = g for
+//   c = 0..2 in its registers 0..2 and, for g < 3, (ky = 4, c = g) in register 3.
+//   The products of the last five rows stay in registers; output row o = j - 4
+//   takes ky = g from row o + g in each lane group (a select over the ring), and
+//   the four lane groups are summed with two cross-lane exchanges.  (The reference
+//   field a prediction adds to this is added by its consumer: a global load whose
+//   result is used in the same row would hold the walk.)
+static constexpr int WALK_RPF = 3;                   // rows in flight (five slots >= WALK_RPF + 2)
+typedef __attribute__((address_space(1))) void gvoid_t;
+typedef __attribute__((address_space(3))) void lvoid_t;
+
+template <int V> struct WalkShape {
+    static constexpr bool NARROW = V == CV_IN3;
+    // uint4 per staged input row: 3 layers x RXW sites, 8 or 32 bytes a site
+    static constexpr int ROWE = NARROW ? 3 * RXW / 2 : 3 * RXW * 2;
+    static constexpr int SLOT = ROWE;
+    // loads every wave issues per staged row (elements 256 k + tid, k < GUAR, exist in all waves)
+    static constexpr int GUAR = NARROW ? 1 : 3;
+    static_assert(NARROW ? (ROWE > 3 * 64 && ROWE <= 256) : (ROWE >= 768 && ROWE <= 1024), "staging split");
+};
+
+// s_waitcnt vmcnt(N) leaving the LDS and export counters alone (gfx9 encoding)
+template <int N> __device__ __forceinline__ void wait_vm()
 {
-    extern __shared__ uint4 cnn_lds[];
-    uint4* sA = cnn_lds;                             // [NSTEP][64]
-    uint4* sX = cnn_lds + NSTEP * 64;                // [HALO][2]
-    for (int i = threadIdx.x; i < NSTEP * 64; i += CNN_TPB) sA[i] = apack[i];
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63, col = l & 15, g = l >> 4;
+    static_assert(N >= 0 && N < 64, "vmcnt");
+    __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+template <int N> using IC = std::integral_constant<int, N>;
+
+template <int V>
+__global__ void __launch_bounds__(CNN_TPB) k_cnn_walk(CnnDims d, CnnWalk wk, const uint4* __restrict__ in,
+                                                   const uint4* __restrict__ zeros, const uint4* __restrict__ apack,
+                                                   uint2* __restrict__ out_bf, float* __restrict__ out_f, float scale)
+{
+    using Sh = CnnShape<V>;
+    using WS = WalkShape<V>;
+    constexpr int XF = RXF, XW = RXW, WX = RWX, RPF = WALK_RPF;
+    constexpr int ROWE = WS::ROWE, SLOT = WS::SLOT, GUAR = WS::GUAR;
+    constexpr int NK = (ROWE + CNN_TPB - 1) / CNN_TPB;  // staging loads per thread and row
+    constexpr bool NARROW = WS::NARROW;
+    __shared__ uint4 st0[SLOT], st1[SLOT], st2[SLOT], st3[SLOT], st4[SLOT];
+    auto slot = [&](auto sl) -> uint4* {
+        constexpr int S = decltype(sl)::value;
+        return S == 0 ? st0 : S == 1 ? st1 : S == 2 ? st2 : S == 3 ? st3 : st4;
+    };
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, col = l & 15, g = l >> 4;
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
     const uint32_t n = (uint32_t)d.n;
-    for (int tile = blockIdx.x; tile < d.ntiles; tile += gridDim.x) {
-        int x0, y0, z;
-        tile_of(d, tile, x0, y0, z);
-        __syncthreads();
-        // halo: HALO sites x 2 uint4, loaded in batches of 7 per thread
-        constexpr int HI = (HALO * 2 + CNN_TPB - 1) / CNN_TPB;
-        uint4 v[HI];
+    bf16x8 wa[Sh::NF];
 #pragma unroll
-        for (int k = 0; k < HI; ++k) {
-            const int i = (int)threadIdx.x + k * CNN_TPB;
-            v[k] = make_uint4(0u, 0u, 0u, 0u);
-            if (i < HALO * 2) {
-                const int site = i >> 1, half = i & 1;
-                const int hx = site % HX, r = site / HX, hy = r % HY, hz = r / HY;
-                const int zz = z + hz - 1;
-                if (zz >= 0 && zz < d.nz) {
-                    const int xx = wrap(x0 + hx - 2, d.nx), yy = wrap(y0 + hy - 2, d.ny);
-                    v[k] = ldg(in, (uint32_t)(xx + d.nx * (yy + d.ny * zz)) * 2u + (uint32_t)half);
-                }
+    for (int i = 0; i < Sh::NF; ++i) wa[i] = as_bf8(apack[i * 64 + l]);
+    // per-lane B offsets inside a staged row: uint2 (CV_IN3) or uint4 elements of fragment 0
+    const int cb = 16 * XF * w + col;
+    int boff[4];
+    if (NARROW) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {                    // k = 2 ks + q: tap slot 8 ks + 2 g + q
+            const int c = min(8 * (k >> 1) + 2 * g + (k & 1), 14);
+            boff[k] = (c / 5) * XW + c % 5 + cb;
+        }
+    } else {
+        boff[0] = (cb + (g >> 1)) * 2 + (g & 1);        // dx pairs (-2,-1) and (0,1): + 4 pair
+        boff[1] = (cb + 4) * 2 + (g & 1);               // pair (2, -): dx = 2 for both halves
+        boff[2] = boff[3] = 0;
+    }
+
+    for (int item = blockIdx.x; item < wk.rtiles; item += gridDim.x) {
+        const int xt = item % wk.rtiles_x;
+        const int rr = item / wk.rtiles_x;
+        const int z = rr / wk.rtiles_y;
+        const int y0 = (rr - z * wk.rtiles_y) * wk.ty;
+        const int x0 = xt * WX;
+        const int nout = min(wk.ty, ny - y0);
+        const int nrows = nout + 4;
+        const bool wv = x0 + 16 * XF * w < nx;           // this wave's sites exist (nx is a multiple of 16 XF)
+        // staged element e = tid + 256 k of a row, in uint4 units of the row of `in`;
+        // -1: a layer outside the lattice (CV_IN3 reads zeros; the others read the
+        // nearest layer, whose taps they skip)
+        int gsrc[NK];
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+            const int e = tid + k * CNN_TPB;
+            if (NARROW) {                                // two 8-byte sites per element
+                const int dzi = e / (XW / 2), q = e - dzi * (XW / 2);
+                const int zz = z + dzi - 1;
+                gsrc[k] = (zz >= 0 && zz < nz) ? ((zz * ny) * nx + wrap(x0 - 2 + 2 * q, nx)) / 2 : -1;
+            } else {
+                const int site = e >> 1, dzi = site / XW, xs = site - dzi * XW;
+                const int zz = min(max(z + dzi - 1, 0), nz - 1);
+                gsrc[k] = ((zz * ny) * nx + wrap(x0 + xs - 2, nx)) * 2 + (e & 1);
             }
         }
+        const int rstride = NARROW ? nx / 2 : nx * 2;    // uint4 per layer row of `in`
+        auto load_row = [&](auto sl, int j) __attribute__((always_inline)) {
+            uint4* dst = slot(sl) + 64 * w;
+            const int yo = wrap(y0 - 2 + j, ny) * rstride;
 #pragma unroll
-        for (int k = 0; k < HI; ++k) {
-            const int i = (int)threadIdx.x + k * CNN_TPB;
-            if (i < HALO * 2) sX[i] = v[k];
-        }
-        __syncthreads();
-        f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-        for (int p = 0; p < NSTEP; ++p) {
-            const bf16x8 a = as_bf8(sA[p * 64 + l]);
-            int t = 2 * p + (g >> 1);
-            t = t < NTAP ? t : NTAP - 1;                 // the zero-weight 76th tap reads tap 75's data
-            int dx, dy, dz;
-            tap_off(t, dx, dy, dz);
-            const int base = ((1 + dz) * HY + (w + 2 + dy)) * HX + 2 + dx + col;
-            const bf16x8 b0 = as_bf8(sX[base * 2 + (g & 1)]);
-            const bf16x8 b1 = as_bf8(sX[(base + 16) * 2 + (g & 1)]);
-            acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc1, 0, 0, 0);
-        }
-        const int y = y0 + w;
-        if (y < d.ny) {
+            for (int k = 0; k < NK; ++k) {
+                const int e = tid + k * CNN_TPB;
+                if (e < ROWE) {                          // lanes past the row idle; every wave keeps >= GUAR loads
+                    const uint4* src = gsrc[k] >= 0 ? in + gsrc[k] + yo : zeros;
+                    __builtin_amdgcn_global_load_lds((gvoid_t*)src, (lvoid_t*)(dst + k * CNN_TPB), 16, 0, 0);
+                }
+            }
+        };
+        // CV_F16, CV_IN3: output row o accumulates in acc[o % 5]; CV_OUT3: acc[j % 5]
+        // holds the product of input row j
+        f32x4 acc[5][XF];
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const f32x4 acc = j ? acc1 : acc0;
-                const uint32_t site = (uint32_t)(x0 + 16 * j + col + d.nx * (y + d.ny * z));
-                if (mode == 0) {
-                    const uint32_t lo = (uint32_t)f2bf(acc[0]) | ((uint32_t)f2bf(acc[1]) << 16);
-                    const uint32_t hi = (uint32_t)f2bf(acc[2]) | ((uint32_t)f2bf(acc[3]) << 16);
-                    // channels 4g..4g+3 of the site: 8 bytes at 32 site + 8 g
-                    stg(reinterpret_cast<uint2*>(out_bf), site * 4u + (uint32_t)g, make_uint2(lo, hi));
-                } else if (g == 0) {
+        for (int s = 0; s < 5; ++s)
 #pragma unroll
-                    for (int r = 0; r < 3; ++r) {
-                        const float h0 = href ? ldg(href + (size_t)r * n, site) : 0.0f;
-                        stg(out_f + (size_t)r * n, site, h0 + scale * acc[r]);
+            for (int f = 0; f < XF; ++f) acc[s][f] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+        auto walk = [&](auto dzm) __attribute__((always_inline)) {
+            constexpr int DZM = decltype(dzm)::value;    // bit dzi: layer z + dzi - 1 exists
+            // the previous item's rows are consumed (every wave passed its last barrier)
+            load_row(IC<0>{}, 0);
+            if (1 < nrows) load_row(IC<1>{}, 1);
+            if (2 < nrows) load_row(IC<2>{}, 2);
+            wait_vm<0>();
+            __builtin_amdgcn_s_barrier();
+            auto row = [&](auto ph, int j) __attribute__((always_inline)) {
+                constexpr int PH = decltype(ph)::value;
+                if (j >= nrows) return;
+                // Phase 0 follows the loop back-edge, where the compiler no longer
+                // knows which slots the loads in flight target and waits for all of
+                // them before the first LDS read: its row load goes after that read.
+                constexpr int DZ0 = (DZM & 1) ? 0 : 1;
+                const bool issue = j + RPF < nrows;
+                if (PH != 0 && issue) load_row(IC<(PH + RPF) % 5>{}, j + RPF);
+                const uint4* S = slot(IC<PH>{});
+                bool pending = PH == 0 && issue;
+                if (wv) {
+                    if (V == CV_IN3) {
+                        const uint2* S2 = reinterpret_cast<const uint2*>(S);
+#pragma unroll
+                        for (int ks = 0; ks < 2; ++ks) {
+                            bf16x8 b[XF];
+#pragma unroll
+                            for (int f = 0; f < XF; ++f) {
+                                const uint2 lo = S2[boff[2 * ks] + 16 * f];
+                                const uint2 hi = S2[boff[2 * ks + 1] + 16 * f];
+                                b[f] = as_bf8(make_uint4(lo.x, lo.y, hi.x, hi.y));
+                            }
+                            if (ks == 0 && pending) { load_row(IC<RPF % 5>{}, j + RPF); pending = false; }
+#pragma unroll
+                            for (int dyi = 0; dyi < 5; ++dyi)
+#pragma unroll
+                                for (int f = 0; f < XF; ++f) {
+                                    f32x4& a = acc[(PH - dyi + 10) % 5][f];
+                                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[dyi * 2 + ks], b[f], a, 0, 0, 0);
+                                }
+                        }
+                    } else {
+                        f32x4 (&c3)[XF] = acc[PH];
+                        if (V == CV_OUT3) {
+#pragma unroll
+                            for (int f = 0; f < XF; ++f) c3[f] = f32x4{0.f, 0.f, 0.f, 0.f};
+                        }
+#pragma unroll
+                        for (int dzi = 0; dzi < 3; ++dzi) {
+                            if (!((DZM >> dzi) & 1)) continue;
+#pragma unroll
+                            for (int pair = 0; pair < 3; ++pair) {
+                                bf16x8 b[XF];
+#pragma unroll
+                                for (int f = 0; f < XF; ++f)
+                                    b[f] = as_bf8(S[dzi * XW * 2 + 32 * f + (pair < 2 ? boff[0] + 4 * pair : boff[1])]);
+                                if (dzi == DZ0 && pair == 0 && pending) { load_row(IC<RPF % 5>{}, j + RPF); pending = false; }
+                                if (V == CV_F16) {
+#pragma unroll
+                                    for (int dyi = 0; dyi < 5; ++dyi)
+#pragma unroll
+                                        for (int f = 0; f < XF; ++f) {
+                                            f32x4& a = acc[(PH - dyi + 10) % 5][f];
+                                            a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                                wa[(dyi * 3 + dzi) * 3 + pair], b[f], a, 0, 0, 0);
+                                        }
+                                } else {
+#pragma unroll
+                                    for (int f = 0; f < XF; ++f)
+                                        c3[f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[dzi * 3 + pair], b[f], c3[f],
+                                                                                        0, 0, 0);
+                                }
+                            }
+                        }
+                        if (V == CV_OUT3) {
+                            const int o = j - 4;         // complete after this row
+                            if (o >= 0 && o < nout) {
+                                const uint32_t site0 = (uint32_t)(x0 + cb + nx * (y0 + o + ny * z));
+#pragma unroll
+                                for (int f = 0; f < XF; ++f) {
+                                    // ky = g from input row o + g (ring slot (PH + 1 + g) % 5)
+                                    const f32x4& r0 = acc[(PH + 1) % 5][f];
+                                    const f32x4& r1 = acc[(PH + 2) % 5][f];
+                                    const f32x4& r2 = acc[(PH + 3) % 5][f];
+                                    const f32x4& r3 = acc[(PH + 4) % 5][f];
+                                    float q[3];
+#pragma unroll
+                                    for (int c = 0; c < 3; ++c) {
+                                        q[c] = g == 0 ? r0[c] : g == 1 ? r1[c] : g == 2 ? r2[c] : r3[c];
+                                        if (c == g) q[c] += acc[PH][f][3];     // (ky = 4, c = g), this row
+                                    }
+#pragma unroll
+                                    for (int c = 0; c < 3; ++c) {
+                                        q[c] += __shfl_xor(q[c], 16);
+                                        q[c] += __shfl_xor(q[c], 32);
+                                    }
+                                    if (g < 3)
+                                        stg(out_f + (size_t)g * n, site0 + 16u * f,
+                                            scale * (g == 0 ? q[0] : g == 1 ? q[1] : q[2]));
+                                }
+                            }
+                        }
+                    }
+                    if (V != CV_OUT3) {
+                        const int o = j - 4;             // complete after this row
+                        if (o >= 0 && o < nout) {
+                            const uint32_t site0 = (uint32_t)(x0 + cb + nx * (y0 + o + ny * z));
+#pragma unroll
+                            for (int f = 0; f < XF; ++f) {
+                                const f32x4& a = acc[(PH + 1) % 5][f];
+                                const uint32_t lo = (uint32_t)f2bf(a[0]) | ((uint32_t)f2bf(a[1]) << 16);
+                                const uint32_t hi = (uint32_t)f2bf(a[2]) | ((uint32_t)f2bf(a[3]) << 16);
+                                // channels 4g..4g+3 of the site: 8 bytes at 32 site + 8 g
+                                stg(out_bf, (site0 + 16u * f) * 4u + (uint32_t)g, make_uint2(lo, hi));
+                            }
+                        }
+#pragma unroll
+                        for (int f = 0; f < XF; ++f) acc[(PH + 1) % 5][f] = f32x4{0.f, 0.f, 0.f, 0.f};
                     }
                 }
+                if (pending) load_row(IC<RPF % 5>{}, j + RPF);
+                // row j + 1 landed: the rows issued after it each add >= GUAR younger loads
+                if (j + 3 < nrows) wait_vm<2 * GUAR>();
+                else if (j + 2 < nrows) wait_vm<GUAR>();
+                else wait_vm<0>();
+                __builtin_amdgcn_s_barrier();
+            };
+            for (int j0 = 0; j0 < nrows; j0 += 5) {
+                row(IC<0>{}, j0);
+                row(IC<1>{}, j0 + 1);
+                row(IC<2>{}, j0 + 2);
+                row(IC<3>{}, j0 + 3);
+                row(IC<4>{}, j0 + 4);
             }
+        };
+        if (V == CV_IN3) {
+            walk(IC<7>{});
+        } else {
+            const bool lo = z > 0, hi = z + 1 < nz;
+            if (lo && hi) walk(IC<7>{});
+            else if (hi) walk(IC<6>{});
+            else if (lo) walk(IC<3>{});
+            else walk(IC<2>{});
         }
     }
 }
@@ -165,8 +385,11 @@ __device__ __forceinline__ bf16x8 lds_run8(const uint16_t* base, int e)
 
 static constexpr int WG_TAPS = (NTAP + 3) / 4;       // taps per wave (19)
 
-__global__ void __launch_bounds__(CNN_TPB) k_cnn_wgrad(CnnDims d, const uint16_t* __restrict__ X,
-                                                    const uint16_t* __restrict__ Gr, float* __restrict__ partial)
+// X and G hold xs and gs channels per site (4 or 16; channels past 15 of the
+// tile are read as zero).
+__global__ void __launch_bounds__(CNN_TPB) k_cnn_wgrad(CnnDims d, const uint16_t* __restrict__ X, int xs,
+                                                    const uint16_t* __restrict__ Gr, int gs,
+                                                    float* __restrict__ partial)
 {
     __shared__ uint16_t sG[16 * CTY * CTX];          // [o][row][x]
     __shared__ uint16_t sXc[16 * HZ * HY * HX];      // [i][hz][hy][hx]
@@ -183,7 +406,7 @@ __global__ void __launch_bounds__(CNN_TPB) k_cnn_wgrad(CnnDims d, const uint16_t
             const int o = i & 15, s = i >> 4, x = s % CTX, row = s / CTX;
             const int y = y0 + row;
             uint16_t v = 0;
-            if (y < d.ny) v = ldg(Gr, (uint32_t)(x0 + x + d.nx * (y + d.ny * z)) * 16u + (uint32_t)o);
+            if (y < d.ny && o < gs) v = ldg(Gr, (uint32_t)(x0 + x + d.nx * (y + d.ny * z)) * (uint32_t)gs + (uint32_t)o);
             sG[(o * CTY + row) * CTX + x] = v;
         }
         // X halo transposed into [i][hz][hy][hx]
@@ -192,9 +415,9 @@ __global__ void __launch_bounds__(CNN_TPB) k_cnn_wgrad(CnnDims d, const uint16_t
             const int hx = s % HX, r = s / HX, hy = r % HY, hz = r / HY;
             const int zz = z + hz - 1;
             uint16_t v = 0;
-            if (zz >= 0 && zz < d.nz) {
+            if (zz >= 0 && zz < d.nz && c < xs) {
                 const int xx = wrap(x0 + hx - 2, d.nx), yy = wrap(y0 + hy - 2, d.ny);
-                v = ldg(X, (uint32_t)(xx + d.nx * (yy + d.ny * zz)) * 16u + (uint32_t)c);
+                v = ldg(X, (uint32_t)(xx + d.nx * (yy + d.ny * zz)) * (uint32_t)xs + (uint32_t)c);
             }
             sXc[((c * HZ + hz) * HY + hy) * HX + hx] = v;
         }
@@ -241,27 +464,39 @@ __global__ void k_cnn_wreduce(const float* __restrict__ partial, int nparts, flo
 
 // ------------------------------------------------------------ weight packing
 // master FP32 W[t][o][i] (16 x 16 per tap, zero beyond the layer's channels) ->
-// MFMA A fragments [38][64][8] BF16 for the layer (transposed = 0) or for its
-// adjoint (transposed = 1: A'[t'] = W[74 - t']^T).
-__global__ void k_cnn_pack_weights(const float* __restrict__ W, int transposed, uint16_t* __restrict__ apack)
+// the row walk's A fragments [NF][64 lanes][8] BF16 for the layer (transposed = 0)
+// or for its adjoint (transposed = 1: W'[t] = W[74 - t]^T, the taps mirrored).
+// Fragment order: CV_F16 (dy, dz, dx pair); CV_IN3 (dy, k-step); CV_OUT3 (dz, dx pair),
+// whose rows are (ky = dy + 2, channel) pairs laid out for the row walk's epilogue.
+__global__ void k_cnn_pack(const float* __restrict__ W, int transposed, int variant, uint16_t* __restrict__ apack)
 {
+    const int nf = variant == CV_F16 ? 45 : variant == CV_IN3 ? 10 : 9;
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= NSTEP * 64 * 8) return;
-    const int j = e & 7, lane = (e >> 3) & 63, p = e >> 9;
-    const int g = lane >> 4, row = lane & 15, kk = 8 * (g & 1) + j;
-    const int t = 2 * p + (g >> 1);
-    float v = 0.0f;
-    if (t < NTAP) {
-        if (!transposed) v = W[(t * 16 + row) * 16 + kk];
-        else v = W[((NTAP - 1 - t) * 16 + kk) * 16 + row];
+    if (e >= nf * 64 * 8) return;
+    const int j = e & 7, lane = (e >> 3) & 63, fi = e >> 9;
+    const int r16 = lane & 15, g = lane >> 4;
+    int t = -1, o = 0, i = 0;
+    if (variant == CV_F16) {
+        const int dyi = fi / 9, dzi = (fi / 3) % 3, dxi = 2 * (fi % 3) + (g >> 1);
+        if (dxi <= 4) { t = dzi * 25 + dyi * 5 + dxi; o = r16; i = 8 * (g & 1) + j; }
+    } else if (variant == CV_IN3) {
+        const int dyi = fi / 2, c = 8 * (fi % 2) + 2 * g + (j >> 2), ch = j & 3;
+        if (c < 15 && ch < 3) { t = (c / 5) * 25 + dyi * 5 + c % 5; o = r16; i = ch; }
+    } else {
+        // row m = 4 gm + rm: (ky, c) = (gm, rm) for rm < 3, (4, gm) for rm = 3 and gm < 3
+        const int dzi = fi / 3, dxi = 2 * (fi % 3) + (g >> 1), gm = r16 >> 2, rm = r16 & 3;
+        const int ky = rm < 3 ? gm : 4, c = rm < 3 ? rm : gm;
+        if ((rm < 3 || gm < 3) && dxi <= 4) { t = dzi * 25 + ky * 5 + dxi; o = c; i = 8 * (g & 1) + j; }
     }
+    float v = 0.0f;
+    if (t >= 0) v = transposed ? W[((NTAP - 1 - t) * 16 + i) * 16 + o] : W[(t * 16 + o) * 16 + i];
     apack[e] = f2bf(v);
 }
 
 // ------------------------------------------------------------ input, loss
-// X0[s] = bf16(s_in (m - m_ref)) in channels 0..2, zero in 3..15
+// X0[s] = bf16(s_in (m - m_ref)) in channels 0..2, zero in 3 (4 channels per site)
 __global__ void k_cnn_input(Dims d, const float* __restrict__ m, const float* __restrict__ mref, float s_in,
-                            uint4* __restrict__ X0)
+                            uint2* __restrict__ X0)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = (uint32_t)d.n;
@@ -275,13 +510,13 @@ __global__ void k_cnn_input(Dims d, const float* __restrict__ m, const float* __
     }
     const uint32_t w0 = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
     const uint32_t w1 = (uint32_t)f2bf(v[2]);
-    stg(X0, 2u * i, make_uint4(w0, w1, 0u, 0u));
-    stg(X0, 2u * i + 1u, make_uint4(0u, 0u, 0u, 0u));
+    stg(X0, i, make_uint2(w0, w1));
 }
 
-// R = scale_out Y3 - T; loss partials sum R^2 (FP64); G3 = bf16(2 scale_out R / (3 n sigma2))
+// R = scale_out Y3 - T; loss partials sum R^2 (FP64); G3 = bf16(2 scale_out R / (3 n sigma2)),
+// 4 channels per site
 __global__ void k_cnn_loss(Dims d, const float* __restrict__ Y3, const float* __restrict__ T, float s_out,
-                           float gscale, uint4* __restrict__ G3, double* __restrict__ part)
+                           float gscale, uint2* __restrict__ G3, double* __restrict__ part)
 {
     __shared__ double red[256];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -297,8 +532,7 @@ __global__ void k_cnn_loss(Dims d, const float* __restrict__ Y3, const float* __
         }
         const uint32_t w0 = (uint32_t)f2bf(gv[0]) | ((uint32_t)f2bf(gv[1]) << 16);
         const uint32_t w1 = (uint32_t)f2bf(gv[2]);
-        stg(G3, 2u * i, make_uint4(w0, w1, 0u, 0u));
-        stg(G3, 2u * i + 1u, make_uint4(0u, 0u, 0u, 0u));
+        stg(G3, i, make_uint2(w0, w1));
     }
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -333,6 +567,13 @@ __global__ void k_cnn_sub(uint32_t n3, const float* __restrict__ a, const float*
     if (i < n3) D[i] = a[i] - b[i];
 }
 
+// S = a + b over the three components ([3][n])
+__global__ void k_cnn_add(uint32_t n3, const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ S)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) S[i] = a[i] + b[i];
+}
+
 // per-block (max |a - b|, max |b|) over n3 values
 __global__ void k_cnn_maxdiff(uint32_t n3, const float* __restrict__ a, const float* __restrict__ b, float2* __restrict__ part)
 {
@@ -362,45 +603,62 @@ hipError_t launch_cnn_sub(size_t n3, const float* a, const float* b, float* D, h
     return hipGetLastError();
 }
 
+hipError_t launch_cnn_add(size_t n3, const float* a, const float* b, float* S, hipStream_t st)
+{
+    k_cnn_add<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>((uint32_t)n3, a, b, S);
+    return hipGetLastError();
+}
+
 hipError_t launch_cnn_maxdiff(size_t n3, const float* a, const float* b, float2* part, int blocks, hipStream_t st)
 {
     k_cnn_maxdiff<<<blocks, 256, 0, st>>>((uint32_t)n3, a, b, part);
     return hipGetLastError();
 }
 
-hipError_t launch_cnn_layer(const CnnDims& d, int grid, const uint16_t* in, const uint16_t* apack, int mode,
-                            uint16_t* out_bf, float* out_f, const float* href, float scale, hipStream_t st)
+static const void* walk_kernel(int variant)
 {
-    const size_t lds = (size_t)(NSTEP * 64 + HALO * 2) * sizeof(uint4);
-    k_cnn_layer<<<grid, CNN_TPB, lds, st>>>(d, reinterpret_cast<const uint4*>(in), reinterpret_cast<const uint4*>(apack),
-                                           mode, reinterpret_cast<uint4*>(out_bf), out_f, href, scale);
+    return variant == CV_F16 ? (const void*)k_cnn_walk<CV_F16>
+         : variant == CV_IN3 ? (const void*)k_cnn_walk<CV_IN3> : (const void*)k_cnn_walk<CV_OUT3>;
+}
+
+hipError_t launch_cnn_rows(int variant, const CnnDims& d, const CnnWalk& wk, int grid, const uint16_t* in,
+                           const uint16_t* zeros, const uint16_t* apack, uint16_t* out_bf, float* out_f,
+                           float scale, hipStream_t st)
+{
+    const uint4* I = reinterpret_cast<const uint4*>(in);
+    const uint4* Z = reinterpret_cast<const uint4*>(zeros);
+    const uint4* A = reinterpret_cast<const uint4*>(apack);
+    uint2* O = reinterpret_cast<uint2*>(out_bf);
+    if (variant == CV_F16) k_cnn_walk<CV_F16><<<grid, CNN_TPB, 0, st>>>(d, wk, I, Z, A, O, out_f, scale);
+    else if (variant == CV_IN3) k_cnn_walk<CV_IN3><<<grid, CNN_TPB, 0, st>>>(d, wk, I, Z, A, O, out_f, scale);
+    else k_cnn_walk<CV_OUT3><<<grid, CNN_TPB, 0, st>>>(d, wk, I, Z, A, O, out_f, scale);
     return hipGetLastError();
 }
 
-hipError_t launch_cnn_wgrad(const CnnDims& d, int grid, const uint16_t* X, const uint16_t* Gr, float* partial,
-                            float* grad, hipStream_t st)
+hipError_t launch_cnn_wgrad(const CnnDims& d, int grid, const uint16_t* X, int xs, const uint16_t* Gr, int gs,
+                            float* partial, float* grad, hipStream_t st)
 {
-    k_cnn_wgrad<<<grid, CNN_TPB, 0, st>>>(d, X, Gr, partial);
+    k_cnn_wgrad<<<grid, CNN_TPB, 0, st>>>(d, X, xs, Gr, gs, partial);
     k_cnn_wreduce<<<(NTAP * 256 + 255) / 256, 256, 0, st>>>(partial, grid, grad);
     return hipGetLastError();
 }
 
-hipError_t launch_cnn_pack_weights(const float* W, int transposed, uint16_t* apack, hipStream_t st)
+hipError_t launch_cnn_pack(const float* W, int transposed, int variant, uint16_t* apack, hipStream_t st)
 {
-    k_cnn_pack_weights<<<(NSTEP * 64 * 8 + 255) / 256, 256, 0, st>>>(W, transposed, apack);
+    k_cnn_pack<<<(45 * 64 * 8 + 255) / 256, 256, 0, st>>>(W, transposed, variant, apack);
     return hipGetLastError();
 }
 
 hipError_t launch_cnn_input(const Dims& d, const float* m, const float* mref, float s_in, uint16_t* X0, hipStream_t st)
 {
-    k_cnn_input<<<(d.n + 255) / 256, 256, 0, st>>>(d, m, mref, s_in, reinterpret_cast<uint4*>(X0));
+    k_cnn_input<<<(d.n + 255) / 256, 256, 0, st>>>(d, m, mref, s_in, reinterpret_cast<uint2*>(X0));
     return hipGetLastError();
 }
 
 hipError_t launch_cnn_loss(const Dims& d, const float* Y3, const float* T, float s_out, float gscale, uint16_t* G3,
                            double* part, hipStream_t st)
 {
-    k_cnn_loss<<<(d.n + 255) / 256, 256, 0, st>>>(d, Y3, T, s_out, gscale, reinterpret_cast<uint4*>(G3), part);
+    k_cnn_loss<<<(d.n + 255) / 256, 256, 0, st>>>(d, Y3, T, s_out, gscale, reinterpret_cast<uint2*>(G3), part);
     return hipGetLastError();
 }
 
@@ -411,11 +669,14 @@ hipError_t launch_cnn_adam(float* W, const float* g, float* m, float* v, int cou
     return hipGetLastError();
 }
 
-int cnn_layer_lds_bytes() { return (NSTEP * 64 + HALO * 2) * (int)sizeof(uint4); }
+int cnn_rows_ty_max() { return 32; }
 
-hipError_t configure_lds_cnn(size_t bytes)
+// resident row-walk workgroups per CU of each layer shape
+int cnn_rows_occupancy(int variant)
 {
-    return hipFuncSetAttribute((const void*)k_cnn_layer, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, walk_kernel(variant), CNN_TPB, 0) != hipSuccess) return 1;
+    return nb > 0 ? nb : 1;
 }
 
 }  // namespace msd

@@ -1531,12 +1531,15 @@ enum StepKind { SK_LOCAL = 0, SK_LAYERED = 1, SK_CONV3 = 2 };
 struct msd_surrogate {
     Dims d{};
     CnnDims cd{};
-    int grid = 0;
-    static constexpr int NT = 75, WSZ = 75 * 256, PACK = 38 * 64 * 8;
+    int grid = 0;                                                        // weight-gradient workgroups
+    CnnWalk walk[3]{};                                                   // row-walk items per shape
+    int rgrid[3] = {0, 0, 0};                                            // row-walk workgroups per shape
+    static constexpr int NT = 75, WSZ = 75 * 256, PACK = 45 * 64 * 8;
     float *W = nullptr, *Mo = nullptr, *Vo = nullptr, *gW = nullptr;     // [3][WSZ]
     uint16_t *Af = nullptr, *At = nullptr;                               // [3][PACK]
-    uint16_t *X0 = nullptr, *X1 = nullptr, *X2 = nullptr;                // activations [n][16]
-    uint16_t *G1 = nullptr, *G2 = nullptr, *G3 = nullptr;                // back-propagated [n][16]
+    uint16_t *X0 = nullptr, *X1 = nullptr, *X2 = nullptr;                // activations [n][4], [n][16], [n][16]
+    uint16_t *G1 = nullptr, *G2 = nullptr, *G3 = nullptr;                // back-propagated [n][16], [n][16], [n][4]
+    uint16_t* zeros = nullptr;                                           // 16 zero bytes
     float *Y3 = nullptr, *T = nullptr, *D = nullptr;                     // [3][n]
     float* partial = nullptr;                                            // [grid][WSZ]
     double* lpart = nullptr;
@@ -1549,28 +1552,38 @@ struct msd_surrogate {
         (void)hipStreamSynchronize(G.stream);
         dfree(W); dfree(Mo); dfree(Vo); dfree(gW); dfree(Af); dfree(At);
         dfree(X0); dfree(X1); dfree(X2); dfree(G1); dfree(G2); dfree(G3);
-        dfree(Y3); dfree(T); dfree(D); dfree(partial); dfree(lpart);
+        dfree(Y3); dfree(T); dfree(D); dfree(partial); dfree(lpart); dfree(zeros);
     }
     static int cin(int l) { return l == 0 ? 3 : 16; }
     static int cout(int l) { return l == 2 ? 3 : 16; }
+    // row-walk shape of each layer and of the adjoints that back-propagation runs
+    static constexpr int FWD_SHAPE[3] = {CV_IN3, CV_F16, CV_OUT3};
+    static constexpr int ADJ_SHAPE[3] = {-1, CV_F16, CV_IN3};
     int pack()
     {
         if (packed) return MSD_OK;
         for (int l = 0; l < 3; ++l) {
-            hipError_t e = launch_cnn_pack_weights(W + (size_t)l * WSZ, 0, Af + (size_t)l * PACK, G.stream);
-            if (e == hipSuccess) e = launch_cnn_pack_weights(W + (size_t)l * WSZ, 1, At + (size_t)l * PACK, G.stream);
+            hipError_t e = launch_cnn_pack(W + (size_t)l * WSZ, 0, FWD_SHAPE[l], Af + (size_t)l * PACK, G.stream);
+            if (e == hipSuccess && ADJ_SHAPE[l] >= 0)
+                e = launch_cnn_pack(W + (size_t)l * WSZ, 1, ADJ_SHAPE[l], At + (size_t)l * PACK, G.stream);
             if (e != hipSuccess) return fail(MSD_EDEVICE, "surrogate: %s", hipGetErrorString(e));
         }
         packed = true;
         return MSD_OK;
     }
-    // X0 -> Y3 = s_out * L3(L2(L1(X0))) + href (FP32 [3][n] into out)
+    hipError_t rows(int shape, const uint16_t* in, const uint16_t* A, uint16_t* out_bf, float* out_f, float scale,
+                    hipStream_t st) const
+    {
+        return launch_cnn_rows(shape, cd, walk[shape], rgrid[shape], in, zeros, A, out_bf, out_f, scale, st);
+    }
+    // X0 -> dH = s_out * L3(L2(L1(X0))) (FP32 [3][n] into out), plus href if given
     int forward(float* out, const float* href, hipStream_t st)
     {
         if (int rc = pack()) return rc;
-        hipError_t e = launch_cnn_layer(cd, grid, X0, Af, 0, X1, nullptr, nullptr, 1.0f, st);
-        if (e == hipSuccess) e = launch_cnn_layer(cd, grid, X1, Af + PACK, 0, X2, nullptr, nullptr, 1.0f, st);
-        if (e == hipSuccess) e = launch_cnn_layer(cd, grid, X2, Af + 2 * PACK, 1, nullptr, out, href, (float)s_out, st);
+        hipError_t e = rows(CV_IN3, X0, Af, X1, nullptr, 1.0f, st);
+        if (e == hipSuccess) e = rows(CV_F16, X1, Af + PACK, X2, nullptr, 1.0f, st);
+        if (e == hipSuccess) e = rows(CV_OUT3, X2, Af + 2 * PACK, nullptr, out, (float)s_out, st);
+        if (e == hipSuccess && href) e = launch_cnn_add((size_t)3 * d.n, out, href, out, st);
         if (e != hipSuccess) return fail(MSD_EDEVICE, "surrogate forward: %s", hipGetErrorString(e));
         return MSD_OK;
     }
@@ -1597,14 +1610,35 @@ extern "C" int msd_surrogate_create(int nx, int ny, int nz, msd_surrogate** out)
     c.tiles_x = nx / 32; c.tiles_y = (ny + 3) / 4; c.ntiles = c.tiles_x * c.tiles_y * nz;
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, G.device));
-    s->grid = std::min(c.ntiles, 2 * prop.multiProcessorCount);
+    const int cus = prop.multiProcessorCount;
+    s->grid = std::min(c.ntiles, 2 * cus);
+    // row-walk items: 128 x ty sites of one layer.  A workgroup walks ty + 4 input
+    // rows per item; per layer shape pick ty (at most 32) for the fewest input rows
+    // per resident workgroup.
+    for (int v = 0; v < 3; ++v) {
+        CnnWalk& wk = s->walk[v];
+        wk.rtiles_x = (nx + 127) / 128;
+        double best = 0.0;
+        const long slots = (long)cnn_rows_occupancy(v) * cus;
+        for (int ty = std::min(ny, cnn_rows_ty_max()); ty >= 1; --ty) {
+            const long items = (long)wk.rtiles_x * nz * ((ny + ty - 1) / ty);
+            const double cost = (double)((items + slots - 1) / slots) * (ty + 4);
+            if (best == 0.0 || cost < best) { best = cost; wk.ty = ty; }
+        }
+        wk.rtiles_y = (ny + wk.ty - 1) / wk.ty;
+        wk.rtiles = wk.rtiles_x * wk.rtiles_y * nz;
+        s->rgrid[v] = (int)std::min<long>(wk.rtiles, slots);
+    }
     const size_t n = s->d.n, W3 = (size_t)3 * msd_surrogate::WSZ;
     for (float** p : {&s->W, &s->Mo, &s->Vo, &s->gW})
         if (int rc = dalloc(p, W3)) return rc;
     if (int rc = dalloc(&s->Af, (size_t)3 * msd_surrogate::PACK)) return rc;
     if (int rc = dalloc(&s->At, (size_t)3 * msd_surrogate::PACK)) return rc;
-    for (uint16_t** p : {&s->X0, &s->X1, &s->X2, &s->G1, &s->G2, &s->G3})
+    for (uint16_t** p : {&s->X1, &s->X2, &s->G1, &s->G2})
         if (int rc = dalloc(p, 16 * n)) return rc;
+    for (uint16_t** p : {&s->X0, &s->G3})
+        if (int rc = dalloc(p, 4 * n)) return rc;
+    if (int rc = dalloc(&s->zeros, 8)) return rc;
     for (float** p : {&s->Y3, &s->T, &s->D})
         if (int rc = dalloc(p, 3 * n)) return rc;
     if (int rc = dalloc(&s->partial, (size_t)s->grid * msd_surrogate::WSZ)) return rc;
@@ -1613,9 +1647,8 @@ extern "C" int msd_surrogate_create(int nx, int ny, int nz, msd_surrogate** out)
     hipError_t e = hipSuccess;
     for (float* p : {s->W, s->Mo, s->Vo, s->gW})
         if (e == hipSuccess) e = launch_fill(p, 0.0f, W3, G.stream);
+    if (e == hipSuccess) e = launch_fill(reinterpret_cast<float*>(s->zeros), 0.0f, 4, G.stream);
     if (e != hipSuccess) return fail(MSD_EDEVICE, "Surrogate.new: %s", hipGetErrorString(e));
-    if (configure_lds_cnn((size_t)cnn_layer_lds_bytes()) != hipSuccess)
-        return fail(MSD_EUNSUPPORTED, "Surrogate.new: %d bytes of LDS per workgroup not available", cnn_layer_lds_bytes());
     HIPCHK(hipStreamSynchronize(G.stream));
     *out = s.release();
     return MSD_OK;
@@ -1734,11 +1767,11 @@ static int surrogate_loss_grads(msd_surrogate* s, double* loss)
     const int P = msd_surrogate::PACK, WS = msd_surrogate::WSZ;
     // the forward output already carries s_out: dL/dY3 = s_out dL/d(dH)
     hipError_t e = launch_cnn_loss(s->d, s->Y3, s->T, 1.0f, gscale * (float)s->s_out, s->G3, s->lpart, G.stream);
-    if (e == hipSuccess) e = launch_cnn_layer(s->cd, s->grid, s->G3, s->At + 2 * P, 0, s->G2, nullptr, nullptr, 1.0f, G.stream);
-    if (e == hipSuccess) e = launch_cnn_layer(s->cd, s->grid, s->G2, s->At + P, 0, s->G1, nullptr, nullptr, 1.0f, G.stream);
-    if (e == hipSuccess) e = launch_cnn_wgrad(s->cd, s->grid, s->X2, s->G3, s->partial, s->gW + 2 * WS, G.stream);
-    if (e == hipSuccess) e = launch_cnn_wgrad(s->cd, s->grid, s->X1, s->G2, s->partial, s->gW + WS, G.stream);
-    if (e == hipSuccess) e = launch_cnn_wgrad(s->cd, s->grid, s->X0, s->G1, s->partial, s->gW, G.stream);
+    if (e == hipSuccess) e = s->rows(CV_IN3, s->G3, s->At + 2 * P, s->G2, nullptr, 1.0f, G.stream);
+    if (e == hipSuccess) e = s->rows(CV_F16, s->G2, s->At + P, s->G1, nullptr, 1.0f, G.stream);
+    if (e == hipSuccess) e = launch_cnn_wgrad(s->cd, s->grid, s->X2, 16, s->G3, 4, s->partial, s->gW + 2 * WS, G.stream);
+    if (e == hipSuccess) e = launch_cnn_wgrad(s->cd, s->grid, s->X1, 16, s->G2, 16, s->partial, s->gW + WS, G.stream);
+    if (e == hipSuccess) e = launch_cnn_wgrad(s->cd, s->grid, s->X0, 4, s->G1, 16, s->partial, s->gW, G.stream);
     if (e != hipSuccess) return fail(MSD_EDEVICE, "surrogate backward: %s", hipGetErrorString(e));
     if (loss) {
         std::vector<double> h(s->nlpart);
@@ -1954,20 +1987,21 @@ int msd_stepper::launch_step(hipStream_t st)
 
 // One Heun step with the surrogate's dipole field: H_dip(X) = H_ref + dH(X - m_ref)
 // for the predictor (X = S) and the corrector (X = P); the fused local kernels
-// add it to exchange + anisotropy (+ thermal).
+// add both parts to exchange + anisotropy (+ thermal).
 int msd_stepper::launch_learned_step(hipStream_t st)
 {
     StepArgs a = args;
     a.has_dip = 0;
     a.Hext = Hext;
+    a.Href = Href;
     const float sin = (float)sur->s_in;
     hipError_t e = launch_cnn_input(s->d, s->S, Mref, sin, sur->X0, st);
     if (e != hipSuccess) return fail(MSD_EDEVICE, "learned step: %s", hipGetErrorString(e));
-    if (int rc = sur->forward(Hext, Href, st)) return rc;
+    if (int rc = sur->forward(Hext, nullptr, st)) return rc;
     e = launch_step_pred(a, st);
     if (e == hipSuccess) e = launch_cnn_input(s->d, P, Mref, sin, sur->X0, st);
     if (e != hipSuccess) return fail(MSD_EDEVICE, "learned step: %s", hipGetErrorString(e));
-    if (int rc = sur->forward(Hext, Href, st)) return rc;
+    if (int rc = sur->forward(Hext, nullptr, st)) return rc;
     e = launch_step_corr(a, st);
     if (e != hipSuccess) return fail(MSD_EDEVICE, "learned step: %s", hipGetErrorString(e));
     return MSD_OK;

@@ -140,6 +140,7 @@ struct StepArgs {
     unsigned long long* draw;   // device counter of the thermal draw
     int ty;                 // rows per tile when there is no dipole
     const float* Hext;      // [3][n] field added at every site (the surrogate's dipole field) or nullptr
+    const float* Href;      // [3][n] second such field (the surrogate's reference field) or nullptr
 };
 hipError_t launch_step_pred(const StepArgs& a, hipStream_t st);
 hipError_t launch_step_corr(const StepArgs& a, hipStream_t st);
@@ -188,26 +189,37 @@ const char* kernel_name(int id);
 hipError_t configure_lds(size_t max_bytes);
 
 // ------------------------------------------------ dipolar-field surrogate network
-// Output tiles of 32 x 4 sites of one layer; nx must be a multiple of 32.
+// Weight-gradient tiles: 32 x 4 sites of one layer (nx must be a multiple of 32).
 struct CnnDims {
     int nx, ny, nz, n;
     int tiles_x, tiles_y, ntiles;
 };
-// BF16 activations are channels-last, 16 channels (32 bytes) per site.
-hipError_t launch_cnn_layer(const CnnDims& d, int grid, const uint16_t* in, const uint16_t* apack, int mode,
-                            uint16_t* out_bf, float* out_f, const float* href, float scale, hipStream_t st);
-hipError_t launch_cnn_wgrad(const CnnDims& d, int grid, const uint16_t* X, const uint16_t* G, float* partial,
-                            float* grad, hipStream_t st);
-hipError_t launch_cnn_pack_weights(const float* W, int transposed, uint16_t* apack, hipStream_t st);
+// Row-walk layer items: 128 sites along x, ty output rows, one layer.
+struct CnnWalk {
+    int ty, rtiles_x, rtiles_y, rtiles;
+};
+// layer shapes of the row walk: 16 -> 16, 3 -> 16 and 16 -> 3 channels
+enum { CV_F16 = 0, CV_IN3 = 1, CV_OUT3 = 2 };
+// BF16 activations are channels-last: 16 channels (32 bytes) per site, or 4
+// (8 bytes, channel 3 zero) for the network input and the output gradient.
+// zeros: 16 zero bytes of device memory (the 3-channel input's layers outside the lattice);
+// CV_OUT3 writes FP32 [3][n] scale * Y into out_f, the others BF16 [n][16] into out_bf
+hipError_t launch_cnn_rows(int variant, const CnnDims& d, const CnnWalk& wk, int grid, const uint16_t* in,
+                           const uint16_t* zeros, const uint16_t* apack, uint16_t* out_bf, float* out_f,
+                           float scale, hipStream_t st);
+hipError_t launch_cnn_wgrad(const CnnDims& d, int grid, const uint16_t* X, int xs, const uint16_t* G, int gs,
+                            float* partial, float* grad, hipStream_t st);
+hipError_t launch_cnn_pack(const float* W, int transposed, int variant, uint16_t* apack, hipStream_t st);
 hipError_t launch_cnn_input(const Dims& d, const float* m, const float* mref, float s_in, uint16_t* X0, hipStream_t st);
 hipError_t launch_cnn_loss(const Dims& d, const float* Y3, const float* T, float s_out, float gscale, uint16_t* G3,
                            double* part, hipStream_t st);
 hipError_t launch_cnn_adam(float* W, const float* g, float* m, float* v, int count, float lr, float b1, float b2,
                            float c1, float c2, float eps, hipStream_t st);
 hipError_t launch_cnn_sub(size_t n3, const float* a, const float* b, float* D, hipStream_t st);
+hipError_t launch_cnn_add(size_t n3, const float* a, const float* b, float* S, hipStream_t st);
 hipError_t launch_cnn_maxdiff(size_t n3, const float* a, const float* b, float2* part, int blocks, hipStream_t st);
-int cnn_layer_lds_bytes();
-hipError_t configure_lds_cnn(size_t bytes);
+int cnn_rows_ty_max();
+int cnn_rows_occupancy(int variant);
 // Row-tile shapes with compile-time instances: whether (nx, ty, tpb) is one, and
 // the default (ty, tpb) for an nx x ny layer (ty = 0 when none fits).
 bool row_shape_compiled(int nx, int ty, int tpb);
