@@ -86,9 +86,13 @@ CNN_USEFUL_FLOP = 2 * 75 * (3 * 16 + 16 * 16 + 16 * 3)      # per site per evalu
 CNN_EXEC_FLOP = 64 * 2 * 16 * 16 * 32 // 16                  # per site: 64 16x16x32 MFMAs per 16 sites (10 + 45 + 9)
 
 
-def learned_vs_plain(M, W, train_steps=300, K=10, tol=1e-2, steps=500):
+def learned_vs_plain(M, W, train_steps=300, train_K=10, Ks=(10, 8, 6, 5, 4, 3, 2), gate=1e-4, gate_steps=100,
+                     tol=1e-2, steps=500):
     """The 512x512x4 film (g=0.1, T=0.2): exact layered-FFT steps against the surrogate
-    with an exact refresh every K steps, from one trained network."""
+    with an exact refresh every K steps, from one trained network.  K is the largest
+    of Ks whose learned trajectory stays within max|dm| <= gate of the exact one over
+    gate_steps steps (identical states and noise); the learned rate is timed at it."""
+    import numpy as np
     from maglua_amd import surrogate as S
 
     spec = W.FILM_512
@@ -104,11 +108,11 @@ def learned_vs_plain(M, W, train_steps=300, K=10, tol=1e-2, steps=500):
     plain = spec.n * steps / (time.perf_counter() - t0)
     sur = M.Surrogate(spec.nx, spec.ny, spec.nz)
     t0 = time.perf_counter()
-    losses = S.train_on_trajectory(sur, g["ss"], heun, g["dip"], steps=train_steps, K=K, lr=2e-3, seed=1)
+    losses = S.train_on_trajectory(sur, g["ss"], heun, g["dip"], steps=train_steps, K=train_K, lr=2e-3, seed=1)
     M.synchronize()
     train_s = time.perf_counter() - t0
     ref = g["ss"].copy()
-    heun.step(K)
+    heun.step(train_K)
     err, frozen = sur.fieldError(g["dip"], g["ss"], ref)
     # one network evaluation, host-timed over back-to-back calls
     sur.setInputSystems(g["ss"], ref)
@@ -119,7 +123,31 @@ def learned_vs_plain(M, W, train_steps=300, K=10, tol=1e-2, steps=500):
         sur.forward(download=False)
     M.synchronize()
     cnn_s = (time.perf_counter() - t0) / 50
-    # learned throughput
+
+    def trajectory_dmax(K):
+        a2, b2 = g["ss"].copy(), g["ss"].copy()
+        ta2, tb2 = W.build(M, spec)["temp"], W.build(M, spec)["temp"]
+        ta2.setSeed(9)
+        tb2.setSeed(9)
+        ea2 = M.Heun(a2, g["ex"], g["ani"], g["dip"], ta2, g["llg"])
+        eb2 = M.Heun(b2, g["ex"], g["ani"], g["dip"], tb2, g["llg"])
+        eb2.setSurrogate(sur, K=K, tol=tol)
+        ea2.step(gate_steps)
+        eb2.step(gate_steps)
+        return float(np.abs(np.stack(a2.spins()[:3]) - np.stack(b2.spins()[:3])).max())
+
+    scan = []
+    K = None
+    for k in Ks:
+        d = trajectory_dmax(k)
+        scan.append({"K": k, "traj_max_dm": d})
+        if d <= gate:
+            K = k
+            break
+    gate_ok = K is not None
+    if K is None:
+        K = Ks[-1]
+    # learned throughput at the chosen K
     b = g["ss"].copy()
     tb = W.build(M, spec)["temp"]
     tb.setSeed(7)
@@ -132,26 +160,16 @@ def learned_vs_plain(M, W, train_steps=300, K=10, tol=1e-2, steps=500):
     M.synchronize()
     learned = spec.n * steps / (time.perf_counter() - t0)
     stats = eb.surrogateStats()
-    # trajectory gate: identical states and noise, 100 steps each way
-    a2, b2 = g["ss"].copy(), g["ss"].copy()
-    ta2, tb2 = W.build(M, spec)["temp"], W.build(M, spec)["temp"]
-    ta2.setSeed(9)
-    tb2.setSeed(9)
-    ea2 = M.Heun(a2, g["ex"], g["ani"], g["dip"], ta2, g["llg"])
-    eb2 = M.Heun(b2, g["ex"], g["ani"], g["dip"], tb2, g["llg"])
-    eb2.setSurrogate(sur, K=K, tol=tol)
-    ea2.step(100)
-    eb2.step(100)
-    import numpy as np
-    dmax = float(np.abs(np.stack(a2.spins()[:3]) - np.stack(b2.spins()[:3])).max())
     return {
         "workload": spec.name() + ", plain layered FFT vs surrogate refreshed every K steps",
         "plain_value": round(plain, 1), "learned_value": round(learned, 1), "ratio": round(learned / plain, 4),
-        "unit": "spin-steps/s", "K": K, "residual_tol": tol,
-        "field_error_after_K": round(err, 6), "frozen_field_error_after_K": round(frozen, 6),
-        "traj_max_dm_100_steps": dmax, "windows": stats["windows"], "fallback_windows": stats["exact_windows"],
+        "unit": "spin-steps/s", "K": K, "gate": f"max|dm| <= {gate:g} over {gate_steps} steps against the exact "
+        "trajectory (same state and noise)", "gate_met": gate_ok, "K_scan": scan, "residual_tol": tol,
+        "field_error_after_train_K": round(err, 6), "frozen_field_error_after_train_K": round(frozen, 6),
+        "windows": stats["windows"], "fallback_windows": stats["exact_windows"],
         "max_residual": round(stats["max_residual"], 6),
-        "train_pairs": train_steps, "train_s": round(train_s, 3), "final_loss": round(float(np.mean(losses[-20:])), 6),
+        "train_pairs": train_steps, "train_K": train_K, "train_s": round(train_s, 3),
+        "final_loss": round(float(np.mean(losses[-20:])), 6),
         "cnn_eval_us": round(1e6 * cnn_s, 2),
         "cnn_mfma_frac_useful": round(CNN_USEFUL_FLOP * spec.n / cnn_s / BF16_DENSE_PEAK, 4),
         "cnn_mfma_frac_executed": round(CNN_EXEC_FLOP * spec.n / cnn_s / BF16_DENSE_PEAK, 4),

@@ -59,6 +59,17 @@ __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint
 
 __device__ __forceinline__ bf16x8 as_bf8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+// channels 4g..4g+3 of an accumulator as 4 BF16 (v_cvt_pk_bf16_f32: round to nearest
+// even, the rounding of f2bf for finite values)
+__device__ __forceinline__ uint2 pack_bf16x4(const f32x4& a)
+{
+    const bf16x2 lo = __builtin_convertvector((f32x2){a[0], a[1]}, bf16x2);
+    const bf16x2 hi = __builtin_convertvector((f32x2){a[2], a[3]}, bf16x2);
+    return make_uint2(__builtin_bit_cast(uint32_t, lo), __builtin_bit_cast(uint32_t, hi));
+}
+
 __device__ __forceinline__ void tap_off(int t, int& dx, int& dy, int& dz)
 {
     dz = t / 25 - 1;
@@ -87,7 +98,8 @@ __device__ __forceinline__ int wrap(int v, int n)
 // layer) in turn and walk the input rows y0-2 .. y0+ny_out+1 of the item.  Each
 // input row goes straight from global memory into one of five LDS slots
 // (global_load_lds_dwordx4, no registers held for rows in flight), WALK_RPF rows
-// ahead of the row being multiplied.  The slots are separate arrays selected at
+// ahead of the row being multiplied (row j + 4 reuses the slot of row j - 1, free
+// once every wave passed the barrier closing row j - 1).  The slots are separate arrays selected at
 // compile time, so the compiler sees that the LDS reads of the current row do not
 // alias the loads in flight; the barrier closing a row waits (vmcnt) only for the
 // loads of the next row: every wave issues at least WALK_GUAR(V) loads per row,
@@ -109,7 +121,7 @@ __device__ __forceinline__ int wrap(int v, int n)
 //   the four lane groups are summed with two cross-lane exchanges.  (The reference
 //   field a prediction adds to this is added by its consumer: a global load whose
 //   result is used in the same row would hold the walk.)
-static constexpr int WALK_RPF = 3;                   // rows in flight (five slots >= WALK_RPF + 2)
+static constexpr int WALK_RPF = 4;                   // rows in flight (five slots >= WALK_RPF + 1)
 typedef __attribute__((address_space(1))) void gvoid_t;
 typedef __attribute__((address_space(3))) void lvoid_t;
 
@@ -123,11 +135,49 @@ template <int V> struct WalkShape {
     static_assert(NARROW ? (ROWE > 3 * 64 && ROWE <= 256) : (ROWE >= 768 && ROWE <= 1024), "staging split");
 };
 
+// x summed over the four 16-lane rows of the wave (lanes l, l^16, l^32, l^48), in
+// every lane: v_permlane16_swap / v_permlane32_swap with both operands x exchange
+// rows 2i <-> 2i+1 (then halves) between the two results, so their sum pairs them
+// (VALU; a ds_bpermute would make each step an LDS round trip)
+__device__ __forceinline__ float lane_rows_sum(float x)
+{
+    const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+    const float y = __uint_as_float(p[0]) + __uint_as_float(p[1]);
+    const auto q = __builtin_amdgcn_permlane32_swap(__float_as_uint(y), __float_as_uint(y), false, false);
+    return __uint_as_float(q[0]) + __uint_as_float(q[1]);
+}
+
 // s_waitcnt vmcnt(N) leaving the LDS and export counters alone (gfx9 encoding)
 template <int N> __device__ __forceinline__ void wait_vm()
 {
     static_assert(N >= 0 && N < 64, "vmcnt");
     __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+// At the end of row j (RPF rows in flight), wait for the loads of row j + 1 and
+// no others.  vmcnt counts loads and stores and completes them in issue order, so
+// the wait may leave outstanding exactly the operations issued after row j + 1's
+// loads:
+//   - the loads of rows j + 2 .. min(j + RPF, nrows - 1), >= GUAR each;
+//   - the output stores of rows j + 1 - RPF .. j (each row issues its next-row
+//     loads before its stores), NST store instructions in a row whose output row
+//     exists (4 <= row < nout + 4) when the wave stores at all.
+// Rows in the middle of the walk take the full count; the first and last rows
+// wait with the loads alone, a lower bound.  GUAR = 0 (a wave that issues no load
+// in some row) waits for everything.
+template <int RPF, int GUAR, int NST>
+__device__ __forceinline__ void wait_next_row(int j, int nrows, int nout, bool stores)
+{
+    if (GUAR == 0) { wait_vm<0>(); return; }
+    const int later = min(j + RPF, nrows - 1) - (j + 1);
+    if (stores && later == RPF - 1 && j - RPF + 1 >= 4 && j < nout + 4) {
+        wait_vm<(RPF - 1) * GUAR + RPF * NST>();
+        return;
+    }
+    if (RPF >= 4 && later >= 3) wait_vm<3 * GUAR>();
+    else if (later >= 2) wait_vm<2 * GUAR>();
+    else if (later >= 1) wait_vm<GUAR>();
+    else wait_vm<0>();
 }
 
 template <int N> using IC = std::integral_constant<int, N>;
@@ -222,7 +272,9 @@ __global__ void __launch_bounds__(CNN_TPB) k_cnn_walk(CnnDims d, CnnWalk wk, con
             load_row(IC<0>{}, 0);
             if (1 < nrows) load_row(IC<1>{}, 1);
             if (2 < nrows) load_row(IC<2>{}, 2);
-            wait_vm<0>();
+            if (3 < nrows) load_row(IC<3>{}, 3);
+            if (3 < nrows) wait_vm<3 * GUAR>();          // row 0 (rows 1..3 are younger)
+            else wait_vm<0>();
             __builtin_amdgcn_s_barrier();
             auto row = [&](auto ph, int j) __attribute__((always_inline)) {
                 constexpr int PH = decltype(ph)::value;
@@ -307,10 +359,7 @@ __global__ void __launch_bounds__(CNN_TPB) k_cnn_walk(CnnDims d, CnnWalk wk, con
                                         if (c == g) q[c] += acc[PH][f][3];     // (ky = 4, c = g), this row
                                     }
 #pragma unroll
-                                    for (int c = 0; c < 3; ++c) {
-                                        q[c] += __shfl_xor(q[c], 16);
-                                        q[c] += __shfl_xor(q[c], 32);
-                                    }
+                                    for (int c = 0; c < 3; ++c) q[c] = lane_rows_sum(q[c]);
                                     if (g < 3)
                                         stg(out_f + (size_t)g * n, site0 + 16u * f,
                                             scale * (g == 0 ? q[0] : g == 1 ? q[1] : q[2]));
@@ -323,23 +372,15 @@ __global__ void __launch_bounds__(CNN_TPB) k_cnn_walk(CnnDims d, CnnWalk wk, con
                         if (o >= 0 && o < nout) {
                             const uint32_t site0 = (uint32_t)(x0 + cb + nx * (y0 + o + ny * z));
 #pragma unroll
-                            for (int f = 0; f < XF; ++f) {
-                                const f32x4& a = acc[(PH + 1) % 5][f];
-                                const uint32_t lo = (uint32_t)f2bf(a[0]) | ((uint32_t)f2bf(a[1]) << 16);
-                                const uint32_t hi = (uint32_t)f2bf(a[2]) | ((uint32_t)f2bf(a[3]) << 16);
-                                // channels 4g..4g+3 of the site: 8 bytes at 32 site + 8 g
-                                stg(out_bf, (site0 + 16u * f) * 4u + (uint32_t)g, make_uint2(lo, hi));
-                            }
+                            for (int f = 0; f < XF; ++f)   // channels 4g..4g+3: 8 bytes at 32 site + 8 g
+                                stg(out_bf, (site0 + 16u * f) * 4u + (uint32_t)g, pack_bf16x4(acc[(PH + 1) % 5][f]));
                         }
 #pragma unroll
                         for (int f = 0; f < XF; ++f) acc[(PH + 1) % 5][f] = f32x4{0.f, 0.f, 0.f, 0.f};
                     }
                 }
                 if (pending) load_row(IC<RPF % 5>{}, j + RPF);
-                // row j + 1 landed: the rows issued after it each add >= GUAR younger loads
-                if (j + 3 < nrows) wait_vm<2 * GUAR>();
-                else if (j + 2 < nrows) wait_vm<GUAR>();
-                else wait_vm<0>();
+                wait_next_row<RPF, GUAR, XF>(j, nrows, nout, wv);
                 __builtin_amdgcn_s_barrier();
             };
             for (int j0 = 0; j0 < nrows; j0 += 5) {
@@ -358,6 +399,223 @@ __global__ void __launch_bounds__(CNN_TPB) k_cnn_walk(CnnDims d, CnnWalk wk, con
             else if (hi) walk(IC<6>{});
             else if (lo) walk(IC<3>{});
             else walk(IC<2>{});
+        }
+    }
+}
+
+// ------------------------------------------------------------ film walk (every layer)
+// The row walk of a film of NZ <= 8 layers with every layer in one item: a
+// workgroup owns FWX = 64 x-sites (16 per wave) and ty rows of all NZ layers.  An
+// input row of every layer is staged once (the per-layer walk stages each row for
+// the three output layers that read it), and each B fragment feeds every output
+// layer it reaches (z = zi - dz) times the five dy: up to 15 MFMAs per LDS read.
+// Staging, slots, waits and phases are those of k_cnn_walk.  The 3-channel input
+// stages a zero layer below and above the film (written once, never loaded over)
+// because its k-steps mix layers; the other shapes skip taps outside the film at
+// compile time.
+static constexpr int FWX = 64, FXW = FWX + 4;
+
+template <int V, int NZ> struct FilmShape {
+    static constexpr bool NARROW = V == CV_IN3;
+    static constexpr int NL = NARROW ? NZ + 2 : NZ;        // staged layers
+    static constexpr int SITE_B = NARROW ? 8 : 32;         // bytes per staged site
+    static constexpr int EB = NARROW ? 4 : 16;             // bytes per load element (2 per site)
+    static constexpr int SLOT16 = NL * FXW * SITE_B / 16;  // uint4 per slot
+    static constexpr int LOADE = NZ * FXW * 2;             // load elements per row
+    static constexpr int NK = (LOADE + CNN_TPB - 1) / CNN_TPB;
+    static constexpr int GUAR = LOADE / CNN_TPB;           // rounds in which every wave loads
+    static constexpr int ZOFF = NARROW ? FXW * SITE_B / EB : 0;   // first loaded element of a slot
+};
+
+template <int V, int NZ>
+__global__ void __launch_bounds__(CNN_TPB) k_cnn_film(CnnDims d, CnnWalk wk, const void* __restrict__ in_,
+                                                   const uint4* __restrict__ apack, uint2* __restrict__ out_bf,
+                                                   float* __restrict__ out_f, float scale)
+{
+    using Sh = CnnShape<V>;
+    using FS = FilmShape<V, NZ>;
+    constexpr int RPF = WALK_RPF, GUAR = FS::GUAR, SLOT16 = FS::SLOT16, EB = FS::EB;
+    constexpr bool NARROW = FS::NARROW;
+    __shared__ uint4 st0[SLOT16], st1[SLOT16], st2[SLOT16], st3[SLOT16], st4[SLOT16];
+    auto slot = [&](auto sl) -> uint4* {
+        constexpr int S = decltype(sl)::value;
+        return S == 0 ? st0 : S == 1 ? st1 : S == 2 ? st2 : S == 3 ? st3 : st4;
+    };
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, col = l & 15, g = l >> 4;
+    const int nx = d.nx, ny = d.ny;
+    const uint32_t n = (uint32_t)d.n;
+    const char* in = reinterpret_cast<const char*>(in_);
+    if (NARROW) {                                        // the zero layers of every slot
+        for (int i = tid; i < FXW / 2; i += CNN_TPB) {
+            const uint4 zv = make_uint4(0u, 0u, 0u, 0u);
+            st0[i] = zv; st1[i] = zv; st2[i] = zv; st3[i] = zv; st4[i] = zv;
+            const int t = (NZ + 1) * FXW / 2 + i;
+            st0[t] = zv; st1[t] = zv; st2[t] = zv; st3[t] = zv; st4[t] = zv;
+        }
+    }
+    bf16x8 wa[Sh::NF];
+#pragma unroll
+    for (int i = 0; i < Sh::NF; ++i) wa[i] = as_bf8(apack[i * 64 + l]);
+    const int cb = 16 * w + col;
+    int boff[4];
+    if (NARROW) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {                    // k = 2 ks + q: tap slot 8 ks + 2 g + q
+            const int c = min(8 * (k >> 1) + 2 * g + (k & 1), 14);
+            boff[k] = (c / 5) * FXW + c % 5 + cb;       // uint2 units, + z FXW for output layer z
+        }
+    } else {
+        boff[0] = (cb + (g >> 1)) * 2 + (g & 1);        // uint4 units, + 2 FXW zi
+        boff[1] = (cb + 4) * 2 + (g & 1);
+        boff[2] = boff[3] = 0;
+    }
+    __syncthreads();
+
+    for (int item = blockIdx.x; item < wk.rtiles; item += gridDim.x) {
+        const int xt = item % wk.rtiles_x;
+        const int y0 = (item / wk.rtiles_x) * wk.ty;
+        const int x0 = xt * FWX;
+        const int nout = min(wk.ty, ny - y0);
+        const int nrows = nout + 4;
+        const bool wv = x0 + 16 * w < nx;
+        // load element e = tid + 256 k of a row: (layer, site, half) -> element of `in`
+        int gsrc[FS::NK];
+#pragma unroll
+        for (int k = 0; k < FS::NK; ++k) {
+            const int e = min(tid + k * CNN_TPB, FS::LOADE - 1), site = e >> 1;
+            const int zl = site / FXW, xs = site - zl * FXW;
+            gsrc[k] = ((zl * ny) * nx + wrap(x0 + xs - 2, nx)) * 2 + (e & 1);
+        }
+        auto load_row = [&](auto sl, int j) __attribute__((always_inline)) {
+            char* dst = reinterpret_cast<char*>(slot(sl)) + (FS::ZOFF + 64 * w) * EB;
+            const char* src = in + (size_t)wrap(y0 - 2 + j, ny) * nx * 2 * EB;
+#pragma unroll
+            for (int k = 0; k < FS::NK; ++k)
+                if (tid + k * CNN_TPB < FS::LOADE) {
+                    gvoid_t* gp = (gvoid_t*)(src + (size_t)gsrc[k] * EB);
+                    lvoid_t* lp = (lvoid_t*)(dst + k * CNN_TPB * EB);
+                    // literal sizes: the host pass checks the builtin without gfx950's 16-byte form
+                    if constexpr (FS::EB == 16) __builtin_amdgcn_global_load_lds(gp, lp, 16, 0, 0);
+                    else __builtin_amdgcn_global_load_lds(gp, lp, 4, 0, 0);
+                }
+        };
+        // CV_F16, CV_IN3: output row o of layer z accumulates in acc[z][o % 5];
+        // CV_OUT3: acc[z][j % 5] holds the product of input row j for output layer z
+        f32x4 acc[NZ][5];
+#pragma unroll
+        for (int z = 0; z < NZ; ++z)
+#pragma unroll
+            for (int s2 = 0; s2 < 5; ++s2) acc[z][s2] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+        load_row(IC<0>{}, 0);
+        if (1 < nrows) load_row(IC<1>{}, 1);
+        if (2 < nrows) load_row(IC<2>{}, 2);
+        if (3 < nrows) load_row(IC<3>{}, 3);
+        if (GUAR > 0 && 3 < nrows) wait_vm<3 * GUAR>();  // row 0 (rows 1..3 are younger)
+        else wait_vm<0>();
+        __builtin_amdgcn_s_barrier();
+        auto row = [&](auto ph, int j) __attribute__((always_inline)) {
+            constexpr int PH = decltype(ph)::value;
+            if (j >= nrows) return;
+            const bool issue = j + RPF < nrows;
+            if (PH != 0 && issue) load_row(IC<(PH + RPF) % 5>{}, j + RPF);
+            bool pending = PH == 0 && issue;            // phase 0 loads after its first LDS read
+            const uint4* S = slot(IC<PH>{});
+            if (wv) {
+                if (V == CV_IN3) {
+                    const uint2* S2 = reinterpret_cast<const uint2*>(S);
+#pragma unroll
+                    for (int z = 0; z < NZ; ++z)
+#pragma unroll
+                        for (int ks = 0; ks < 2; ++ks) {
+                            const uint2 lo = S2[boff[2 * ks] + z * FXW];
+                            const uint2 hi = S2[boff[2 * ks + 1] + z * FXW];
+                            const bf16x8 b = as_bf8(make_uint4(lo.x, lo.y, hi.x, hi.y));
+                            if (z == 0 && ks == 0 && pending) { load_row(IC<RPF % 5>{}, j + RPF); pending = false; }
+#pragma unroll
+                            for (int dyi = 0; dyi < 5; ++dyi) {
+                                f32x4& a = acc[z][(PH - dyi + 10) % 5];
+                                a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[dyi * 2 + ks], b, a, 0, 0, 0);
+                            }
+                        }
+                } else {
+                    if (V == CV_OUT3) {
+#pragma unroll
+                        for (int z = 0; z < NZ; ++z) acc[z][PH] = f32x4{0.f, 0.f, 0.f, 0.f};
+                    }
+                    // B of k-step ks = 3 zi + pair, read one k-step ahead of its MFMAs
+                    auto bread = [&](int ks) {
+                        const int zi = ks / 3, pair = ks - 3 * zi;
+                        return as_bf8(S[zi * FXW * 2 + (pair < 2 ? boff[0] + 4 * pair : boff[1])]);
+                    };
+                    bf16x8 bnext = bread(0);
+#pragma unroll
+                    for (int zi = 0; zi < NZ; ++zi)
+#pragma unroll
+                        for (int pair = 0; pair < 3; ++pair) {
+                            const bf16x8 b = bnext;
+                            if (3 * zi + pair + 1 < 3 * NZ) bnext = bread(3 * zi + pair + 1);
+                            if (zi == 0 && pair == 0 && pending) { load_row(IC<RPF % 5>{}, j + RPF); pending = false; }
+#pragma unroll
+                            for (int dzi = 0; dzi < 3; ++dzi) {
+                                const int z = zi - dzi + 1;      // output layer reached through dz = dzi - 1
+                                if (z < 0 || z >= NZ) continue;
+                                if (V == CV_F16) {
+#pragma unroll
+                                    for (int dyi = 0; dyi < 5; ++dyi) {
+                                        f32x4& a = acc[z][(PH - dyi + 10) % 5];
+                                        a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[(dyi * 3 + dzi) * 3 + pair], b,
+                                                                                    a, 0, 0, 0);
+                                    }
+                                } else {
+                                    f32x4& a = acc[z][PH];
+                                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[dzi * 3 + pair], b, a, 0, 0, 0);
+                                }
+                            }
+                        }
+                }
+                const int o = j - 4;                     // complete after this row
+                if (V != CV_OUT3) {
+                    if (o >= 0 && o < nout) {
+#pragma unroll
+                        for (int z = 0; z < NZ; ++z) {
+                            const uint32_t site = (uint32_t)(x0 + cb + nx * (y0 + o + ny * z));
+                            stg(out_bf, site * 4u + (uint32_t)g, pack_bf16x4(acc[z][(PH + 1) % 5]));
+                        }
+                    }
+#pragma unroll
+                    for (int z = 0; z < NZ; ++z) acc[z][(PH + 1) % 5] = f32x4{0.f, 0.f, 0.f, 0.f};
+                } else if (o >= 0 && o < nout) {
+#pragma unroll
+                    for (int z = 0; z < NZ; ++z) {
+                        // ky = g from input row o + g (ring slot (PH + 1 + g) % 5)
+                        const f32x4& r0 = acc[z][(PH + 1) % 5];
+                        const f32x4& r1 = acc[z][(PH + 2) % 5];
+                        const f32x4& r2 = acc[z][(PH + 3) % 5];
+                        const f32x4& r3 = acc[z][(PH + 4) % 5];
+                        float q[3];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            q[c] = g == 0 ? r0[c] : g == 1 ? r1[c] : g == 2 ? r2[c] : r3[c];
+                            if (c == g) q[c] += acc[z][PH][3];   // (ky = 4, c = g), this row
+                        }
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) q[c] = lane_rows_sum(q[c]);
+                        const uint32_t site = (uint32_t)(x0 + cb + nx * (y0 + o + ny * z));
+                        if (g < 3) stg(out_f + (size_t)g * n, site, scale * (g == 0 ? q[0] : g == 1 ? q[1] : q[2]));
+                    }
+                }
+            }
+            if (pending) load_row(IC<RPF % 5>{}, j + RPF);
+            wait_next_row<RPF, GUAR, NZ>(j, nrows, nout, wv);
+            __builtin_amdgcn_s_barrier();
+        };
+        for (int j0 = 0; j0 < nrows; j0 += 5) {
+            row(IC<0>{}, j0);
+            row(IC<1>{}, j0 + 1);
+            row(IC<2>{}, j0 + 2);
+            row(IC<3>{}, j0 + 3);
+            row(IC<4>{}, j0 + 4);
         }
     }
 }
@@ -615,10 +873,34 @@ hipError_t launch_cnn_maxdiff(size_t n3, const float* a, const float* b, float2*
     return hipGetLastError();
 }
 
-static const void* walk_kernel(int variant)
+template <int NZ> static const void* film_kernel(int variant)
 {
+    return variant == CV_F16 ? (const void*)k_cnn_film<CV_F16, NZ>
+         : variant == CV_IN3 ? (const void*)k_cnn_film<CV_IN3, NZ> : (const void*)k_cnn_film<CV_OUT3, NZ>;
+}
+
+bool cnn_film_supported(int nz) { return nz == 1 || nz == 2 || nz == 4 || nz == 8; }
+
+static const void* walk_kernel(int variant, int film_nz)
+{
+    switch (film_nz) {
+    case 1: return film_kernel<1>(variant);
+    case 2: return film_kernel<2>(variant);
+    case 4: return film_kernel<4>(variant);
+    case 8: return film_kernel<8>(variant);
+    default: break;
+    }
     return variant == CV_F16 ? (const void*)k_cnn_walk<CV_F16>
          : variant == CV_IN3 ? (const void*)k_cnn_walk<CV_IN3> : (const void*)k_cnn_walk<CV_OUT3>;
+}
+
+template <int NZ>
+static void launch_film(int variant, const CnnDims& d, const CnnWalk& wk, int grid, const void* in, const uint4* A,
+                        uint2* O, float* out_f, float scale, hipStream_t st)
+{
+    if (variant == CV_F16) k_cnn_film<CV_F16, NZ><<<grid, CNN_TPB, 0, st>>>(d, wk, in, A, O, out_f, scale);
+    else if (variant == CV_IN3) k_cnn_film<CV_IN3, NZ><<<grid, CNN_TPB, 0, st>>>(d, wk, in, A, O, out_f, scale);
+    else k_cnn_film<CV_OUT3, NZ><<<grid, CNN_TPB, 0, st>>>(d, wk, in, A, O, out_f, scale);
 }
 
 hipError_t launch_cnn_rows(int variant, const CnnDims& d, const CnnWalk& wk, int grid, const uint16_t* in,
@@ -629,6 +911,16 @@ hipError_t launch_cnn_rows(int variant, const CnnDims& d, const CnnWalk& wk, int
     const uint4* Z = reinterpret_cast<const uint4*>(zeros);
     const uint4* A = reinterpret_cast<const uint4*>(apack);
     uint2* O = reinterpret_cast<uint2*>(out_bf);
+    if (wk.film) {
+        switch (d.nz) {
+        case 1: launch_film<1>(variant, d, wk, grid, in, A, O, out_f, scale, st); break;
+        case 2: launch_film<2>(variant, d, wk, grid, in, A, O, out_f, scale, st); break;
+        case 4: launch_film<4>(variant, d, wk, grid, in, A, O, out_f, scale, st); break;
+        case 8: launch_film<8>(variant, d, wk, grid, in, A, O, out_f, scale, st); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     if (variant == CV_F16) k_cnn_walk<CV_F16><<<grid, CNN_TPB, 0, st>>>(d, wk, I, Z, A, O, out_f, scale);
     else if (variant == CV_IN3) k_cnn_walk<CV_IN3><<<grid, CNN_TPB, 0, st>>>(d, wk, I, Z, A, O, out_f, scale);
     else k_cnn_walk<CV_OUT3><<<grid, CNN_TPB, 0, st>>>(d, wk, I, Z, A, O, out_f, scale);
@@ -671,11 +963,13 @@ hipError_t launch_cnn_adam(float* W, const float* g, float* m, float* v, int cou
 
 int cnn_rows_ty_max() { return 32; }
 
-// resident row-walk workgroups per CU of each layer shape
-int cnn_rows_occupancy(int variant)
+// resident row-walk workgroups per CU of each layer shape (film_nz: the film walk of
+// that many layers, 0: the per-layer walk)
+int cnn_rows_occupancy(int variant, int film_nz)
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, walk_kernel(variant), CNN_TPB, 0) != hipSuccess) return 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, walk_kernel(variant, film_nz), CNN_TPB, 0) != hipSuccess)
+        return 1;
     return nb > 0 ? nb : 1;
 }
 

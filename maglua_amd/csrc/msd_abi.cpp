@@ -1612,21 +1612,25 @@ extern "C" int msd_surrogate_create(int nx, int ny, int nz, msd_surrogate** out)
     HIPCHK(hipGetDeviceProperties(&prop, G.device));
     const int cus = prop.multiProcessorCount;
     s->grid = std::min(c.ntiles, 2 * cus);
-    // row-walk items: 128 x ty sites of one layer.  A workgroup walks ty + 4 input
-    // rows per item; per layer shape pick ty (at most 32) for the fewest input rows
-    // per resident workgroup.
+    // row-walk items: 128 x ty sites of one layer, or 64 x ty sites of every layer
+    // of a film of 1, 2, 4 or 8 layers.  A workgroup walks ty + 4 input rows per
+    // item; per layer shape pick ty (at most 32) for the fewest input rows per
+    // resident workgroup.
+    const bool film = cnn_film_supported(nz);
     for (int v = 0; v < 3; ++v) {
         CnnWalk& wk = s->walk[v];
-        wk.rtiles_x = (nx + 127) / 128;
+        wk.film = film ? 1 : 0;
+        wk.rtiles_x = film ? (nx + 63) / 64 : (nx + 127) / 128;
+        const int zitems = film ? 1 : nz;
         double best = 0.0;
-        const long slots = (long)cnn_rows_occupancy(v) * cus;
+        const long slots = (long)cnn_rows_occupancy(v, film ? nz : 0) * cus;
         for (int ty = std::min(ny, cnn_rows_ty_max()); ty >= 1; --ty) {
-            const long items = (long)wk.rtiles_x * nz * ((ny + ty - 1) / ty);
+            const long items = (long)wk.rtiles_x * zitems * ((ny + ty - 1) / ty);
             const double cost = (double)((items + slots - 1) / slots) * (ty + 4);
             if (best == 0.0 || cost < best) { best = cost; wk.ty = ty; }
         }
         wk.rtiles_y = (ny + wk.ty - 1) / wk.ty;
-        wk.rtiles = wk.rtiles_x * wk.rtiles_y * nz;
+        wk.rtiles = wk.rtiles_x * wk.rtiles_y * zitems;
         s->rgrid[v] = (int)std::min<long>(wk.rtiles, slots);
     }
     const size_t n = s->d.n, W3 = (size_t)3 * msd_surrogate::WSZ;

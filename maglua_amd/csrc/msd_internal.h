@@ -194,9 +194,11 @@ struct CnnDims {
     int nx, ny, nz, n;
     int tiles_x, tiles_y, ntiles;
 };
-// Row-walk layer items: 128 sites along x, ty output rows, one layer.
+// Row-walk layer items: 128 sites along x, ty output rows, one layer; or (film) 64
+// sites along x, ty output rows, every layer of a film of 1, 2, 4 or 8 layers.
 struct CnnWalk {
     int ty, rtiles_x, rtiles_y, rtiles;
+    int film;
 };
 // layer shapes of the row walk: 16 -> 16, 3 -> 16 and 16 -> 3 channels
 enum { CV_F16 = 0, CV_IN3 = 1, CV_OUT3 = 2 };
@@ -219,7 +221,8 @@ hipError_t launch_cnn_sub(size_t n3, const float* a, const float* b, float* D, h
 hipError_t launch_cnn_add(size_t n3, const float* a, const float* b, float* S, hipStream_t st);
 hipError_t launch_cnn_maxdiff(size_t n3, const float* a, const float* b, float2* part, int blocks, hipStream_t st);
 int cnn_rows_ty_max();
-int cnn_rows_occupancy(int variant);
+bool cnn_film_supported(int nz);
+int cnn_rows_occupancy(int variant, int film_nz);
 // Row-tile shapes with compile-time instances: whether (nx, ty, tpb) is one, and
 // the default (ty, tpb) for an nx x ny layer (ty = 0 when none fits).
 bool row_shape_compiled(int nx, int ty, int tpb);

@@ -39,7 +39,9 @@ def setup(dims, seed, s_in=60.0, s_out=0.04, sigma2=2e-4):
     return sur, Ws, dm, T, (s_in, s_out, sigma2)
 
 
-@pytest.mark.parametrize("dims", [(32, 4, 1), (32, 8, 4), (64, 6, 3)])
+# film walk (every layer in one item) for 1, 2, 4 and 8 layers; the per-layer walk
+# for 3 and 5
+@pytest.mark.parametrize("dims", [(32, 4, 1), (32, 8, 4), (64, 6, 3), (64, 5, 8), (32, 7, 5), (96, 3, 2)])
 def test_forward_matches_oracle(dims):
     sur, Ws, dm, T, (s_in, s_out, _) = setup(dims, 1)
     got = aos(sur.forward(), dims)
@@ -55,7 +57,7 @@ def test_weights_round_trip():
         assert np.allclose(sur.weights(l), np.asarray(Ws[l], dtype=np.float32), rtol=0, atol=0)
 
 
-@pytest.mark.parametrize("dims", [(32, 8, 4), (64, 4, 2)])
+@pytest.mark.parametrize("dims", [(32, 8, 4), (64, 4, 2), (32, 5, 8), (32, 6, 3)])
 def test_gradients_match_oracle(dims):
     sur, Ws, dm, T, sc = setup(dims, 3)
     loss, grads = sur.lossGrads()
