@@ -36,7 +36,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 static constexpr int CNN_TPB = 256;                  // 4 waves
-static constexpr int CTX = 32, CTY = 4;              // weight-gradient tile: 32 x, 4 rows, one layer
+static constexpr int CTX = 32, CTY = 8;              // weight-gradient tile: 32 x, 8 rows, one layer
 static constexpr int HX = CTX + 4, HY = CTY + 4, HZ = 3;
 static constexpr int HALO = HX * HY * HZ;            // input sites staged per weight-gradient tile
 static constexpr int NTAP = 75;
@@ -643,8 +643,8 @@ __device__ __forceinline__ bf16x8 lds_run8(const uint16_t* base, int e)
 
 static constexpr int WG_TAPS = (NTAP + 3) / 4;       // taps per wave (19)
 
-// X and G hold xs and gs channels per site (4 or 16; channels past 15 of the
-// tile are read as zero).
+// X and G hold xs and gs channels per site (4 or 16); the tile's channels past
+// them are zero.
 __global__ void __launch_bounds__(CNN_TPB) k_cnn_wgrad(CnnDims d, const uint16_t* __restrict__ X, int xs,
                                                     const uint16_t* __restrict__ Gr, int gs,
                                                     float* __restrict__ partial)
@@ -655,29 +655,38 @@ __global__ void __launch_bounds__(CNN_TPB) k_cnn_wgrad(CnnDims d, const uint16_t
     f32x4 acc[WG_TAPS];
 #pragma unroll
     for (int k = 0; k < WG_TAPS; ++k) acc[k] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // channels past gs / xs (4-channel tensors) stay zero for every tile
+    for (int i = threadIdx.x + gs * CTY * CTX; i < 16 * CTY * CTX; i += CNN_TPB) sG[i] = 0;
+    for (int i = threadIdx.x + xs * HZ * HY * HX; i < 16 * HZ * HY * HX; i += CNN_TPB) sXc[i] = 0;
     for (int tile = blockIdx.x; tile < d.ntiles; tile += gridDim.x) {
         int x0, y0, z;
         tile_of(d, tile, x0, y0, z);
         __syncthreads();
-        // G tile: 128 sites x 16 channels, transposed into [o][row][x]
-        for (int i = threadIdx.x; i < CTY * CTX * 16; i += CNN_TPB) {
-            const int o = i & 15, s = i >> 4, x = s % CTX, row = s / CTX;
+        // G tile: CTY rows x 32 sites, transposed into [o][row][x]; 8-byte loads of 4
+        // channels (gs / 4 per site)
+        const int gq = gs >> 2, xq = xs >> 2;
+        for (int i = threadIdx.x; i < CTY * CTX * gq; i += CNN_TPB) {
+            const int q = i % gq, s = i / gq, x = s % CTX, row = s / CTX;
             const int y = y0 + row;
-            uint16_t v = 0;
-            if (y < d.ny && o < gs) v = ldg(Gr, (uint32_t)(x0 + x + d.nx * (y + d.ny * z)) * (uint32_t)gs + (uint32_t)o);
-            sG[(o * CTY + row) * CTX + x] = v;
+            uint2 v = make_uint2(0u, 0u);
+            if (y < d.ny) v = ldg(reinterpret_cast<const uint2*>(Gr), (uint32_t)(x0 + x + d.nx * (y + d.ny * z)) * gq + q);
+            const int b0 = ((4 * q) * CTY + row) * CTX + x, st = CTY * CTX;
+            sG[b0] = (uint16_t)v.x; sG[b0 + st] = (uint16_t)(v.x >> 16);
+            sG[b0 + 2 * st] = (uint16_t)v.y; sG[b0 + 3 * st] = (uint16_t)(v.y >> 16);
         }
         // X halo transposed into [i][hz][hy][hx]
-        for (int i = threadIdx.x; i < HALO * 16; i += CNN_TPB) {
-            const int c = i & 15, s = i >> 4;
+        for (int i = threadIdx.x; i < HALO * xq; i += CNN_TPB) {
+            const int q = i % xq, s = i / xq;
             const int hx = s % HX, r = s / HX, hy = r % HY, hz = r / HY;
             const int zz = z + hz - 1;
-            uint16_t v = 0;
-            if (zz >= 0 && zz < d.nz && c < xs) {
+            uint2 v = make_uint2(0u, 0u);
+            if (zz >= 0 && zz < d.nz) {
                 const int xx = wrap(x0 + hx - 2, d.nx), yy = wrap(y0 + hy - 2, d.ny);
-                v = ldg(X, (uint32_t)(xx + d.nx * (yy + d.ny * zz)) * (uint32_t)xs + (uint32_t)c);
+                v = ldg(reinterpret_cast<const uint2*>(X), (uint32_t)(xx + d.nx * (yy + d.ny * zz)) * xq + q);
             }
-            sXc[((c * HZ + hz) * HY + hy) * HX + hx] = v;
+            const int b0 = (((4 * q) * HZ + hz) * HY + hy) * HX + hx, st = HZ * HY * HX;
+            sXc[b0] = (uint16_t)v.x; sXc[b0 + st] = (uint16_t)(v.x >> 16);
+            sXc[b0 + 2 * st] = (uint16_t)v.y; sXc[b0 + 3 * st] = (uint16_t)(v.y >> 16);
         }
         __syncthreads();
         bf16x8 a[CTY];
@@ -710,14 +719,26 @@ __global__ void __launch_bounds__(CNN_TPB) k_cnn_wgrad(CnnDims d, const uint16_t
     }
 }
 
-// sum the per-workgroup partials: grad[t][o][i] (16 x 16 per tap)
-__global__ void k_cnn_wreduce(const float* __restrict__ partial, int nparts, float* __restrict__ grad)
+// sum the per-workgroup partials: grad[t][o][i] (16 x 16 per tap).  A block of
+// 16 x 16 threads owns 16 consecutive entries; its 16 thread rows each sum a
+// strided slice of the partials (coalesced 64-byte rows) and LDS combines them.
+__global__ void __launch_bounds__(256) k_cnn_wreduce(const float* __restrict__ partial, int nparts,
+                                                     float* __restrict__ grad)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= NTAP * 256) return;
-    float s = 0.0f;
-    for (int b = 0; b < nparts; ++b) s += partial[(size_t)b * NTAP * 256 + e];
-    grad[e] = s;
+    __shared__ float red[16][17];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int e = blockIdx.x * 16 + tx;
+    float acc = 0.0f;
+    if (e < NTAP * 256)
+        for (int b = ty; b < nparts; b += 16) acc += partial[(size_t)b * NTAP * 256 + e];
+    red[ty][tx] = acc;
+    __syncthreads();
+    if (ty == 0 && e < NTAP * 256) {
+        float s2 = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) s2 += red[k][tx];
+        grad[e] = s2;
+    }
 }
 
 // ------------------------------------------------------------ weight packing
@@ -931,7 +952,7 @@ hipError_t launch_cnn_wgrad(const CnnDims& d, int grid, const uint16_t* X, int x
                             float* partial, float* grad, hipStream_t st)
 {
     k_cnn_wgrad<<<grid, CNN_TPB, 0, st>>>(d, X, xs, Gr, gs, partial);
-    k_cnn_wreduce<<<(NTAP * 256 + 255) / 256, 256, 0, st>>>(partial, grid, grad);
+    k_cnn_wreduce<<<(NTAP * 256 + 15) / 16, 256, 0, st>>>(partial, grid, grad);
     return hipGetLastError();
 }
 

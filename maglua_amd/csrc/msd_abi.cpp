@@ -1607,7 +1607,7 @@ extern "C" int msd_surrogate_create(int nx, int ny, int nz, msd_surrogate** out)
     s->d = Dims{nx, ny, nz, nx * ny * nz};
     CnnDims& c = s->cd;
     c.nx = nx; c.ny = ny; c.nz = nz; c.n = s->d.n;
-    c.tiles_x = nx / 32; c.tiles_y = (ny + 3) / 4; c.ntiles = c.tiles_x * c.tiles_y * nz;
+    c.tiles_x = nx / 32; c.tiles_y = (ny + 7) / 8; c.ntiles = c.tiles_x * c.tiles_y * nz;
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, G.device));
     const int cus = prop.multiProcessorCount;

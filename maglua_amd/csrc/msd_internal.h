@@ -189,7 +189,7 @@ const char* kernel_name(int id);
 hipError_t configure_lds(size_t max_bytes);
 
 // ------------------------------------------------ dipolar-field surrogate network
-// Weight-gradient tiles: 32 x 4 sites of one layer (nx must be a multiple of 32).
+// Weight-gradient tiles: 32 x 8 sites of one layer (nx must be a multiple of 32).
 struct CnnDims {
     int nx, ny, nz, n;
     int tiles_x, tiles_y, ntiles;
