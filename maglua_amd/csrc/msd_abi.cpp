@@ -1898,10 +1898,11 @@ struct msd_stepper {
     bool have_ref = false, sur_fallback = false;
     long sur_windows = 0, sur_exact_windows = 0, sur_fallbacks = 0;
     double sur_last_res = 0.0, sur_max_res = 0.0;
-    hipGraph_t lgraph = nullptr;
-    hipGraphExec_t lexec = nullptr;
-    StepArgs lgraph_args{};
-    double lgraph_sin = 0.0, lgraph_sout = 0.0;   // surrogate scales baked into the learned graph
+    // learned-step graphs: [0] a window's first step, [1] the others
+    hipGraph_t lgraph[2] = {nullptr, nullptr};
+    hipGraphExec_t lexec[2] = {nullptr, nullptr};
+    StepArgs lgraph_args[2]{};
+    double lgraph_sin[2] = {0.0, 0.0}, lgraph_sout[2] = {0.0, 0.0};   // surrogate scales baked into them
     ~msd_stepper()
     {
         (void)hipStreamSynchronize(G.stream);
@@ -1913,10 +1914,16 @@ struct msd_stepper {
     {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        if (lexec) (void)hipGraphExecDestroy(lexec);
-        if (lgraph) (void)hipGraphDestroy(lgraph);
         exec = nullptr; graph = nullptr;
-        lexec = nullptr; lgraph = nullptr;
+        drop_learned_graphs();
+    }
+    void drop_learned_graphs()
+    {
+        for (int k = 0; k < 2; ++k) {
+            if (lexec[k]) (void)hipGraphExecDestroy(lexec[k]);
+            if (lgraph[k]) (void)hipGraphDestroy(lgraph[k]);
+            lexec[k] = nullptr; lgraph[k] = nullptr;
+        }
     }
     int kind() const { return lr3 ? SK_CONV3 : (dip ? SK_LAYERED : SK_LOCAL); }
     int refresh();
@@ -1935,7 +1942,7 @@ struct msd_stepper {
         return r;
     }
     int launch_step(hipStream_t st);
-    int launch_learned_step(hipStream_t st);
+    int launch_learned_step(hipStream_t st, bool window_start);
 };
 
 static bool args_equal(const StepArgs& a, const StepArgs& b) { return std::memcmp(&a, &b, sizeof(StepArgs)) == 0; }
@@ -1992,17 +1999,26 @@ int msd_stepper::launch_step(hipStream_t st)
 // One Heun step with the surrogate's dipole field: H_dip(X) = H_ref + dH(X - m_ref)
 // for the predictor (X = S) and the corrector (X = P); the fused local kernels
 // add both parts to exchange + anisotropy (+ thermal).
-int msd_stepper::launch_learned_step(hipStream_t st)
+int msd_stepper::launch_learned_step(hipStream_t st, bool window_start)
 {
     StepArgs a = args;
     a.has_dip = 0;
     a.Hext = Hext;
     a.Href = Href;
     const float sin = (float)sur->s_in;
-    hipError_t e = launch_cnn_input(s->d, s->S, Mref, sin, sur->X0, st);
-    if (e != hipSuccess) return fail(MSD_EDEVICE, "learned step: %s", hipGetErrorString(e));
-    if (int rc = sur->forward(Hext, nullptr, st)) return rc;
-    e = launch_step_pred(a, st);
+    hipError_t e = hipSuccess;
+    if (window_start) {
+        // S is m_ref at a window's first step: the network's input is zero and so is
+        // its output (linear layers without bias), so the predictor takes H_ref alone
+        a.Hext = nullptr;
+        e = launch_step_pred(a, st);
+        a.Hext = Hext;
+    } else {
+        e = launch_cnn_input(s->d, s->S, Mref, sin, sur->X0, st);
+        if (e != hipSuccess) return fail(MSD_EDEVICE, "learned step: %s", hipGetErrorString(e));
+        if (int rc = sur->forward(Hext, nullptr, st)) return rc;
+        e = launch_step_pred(a, st);
+    }
     if (e == hipSuccess) e = launch_cnn_input(s->d, P, Mref, sin, sur->X0, st);
     if (e != hipSuccess) return fail(MSD_EDEVICE, "learned step: %s", hipGetErrorString(e));
     if (int rc = sur->forward(Hext, nullptr, st)) return rc;
@@ -2108,11 +2124,12 @@ static void stepper_end(msd_stepper* st, int nsteps)
     if (st->dip) st->dip_epoch = st->dip->epoch;
 }
 
-// capture one step (exact or learned) into a graph
-static int capture_step(msd_stepper* st, bool learned, hipGraph_t* graph, hipGraphExec_t* exec)
+// capture one step into a graph: kind 0 exact, 1 a learned window's first step, 2 a
+// later learned step
+static int capture_step(msd_stepper* st, int kind, hipGraph_t* graph, hipGraphExec_t* exec)
 {
     HIPCHK(hipStreamBeginCapture(G.stream, hipStreamCaptureModeThreadLocal));
-    int rc = learned ? st->launch_learned_step(G.stream) : st->launch_step(G.stream);
+    int rc = kind ? st->launch_learned_step(G.stream, kind == 1) : st->launch_step(G.stream);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(G.stream, &g);
     if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -2122,28 +2139,39 @@ static int capture_step(msd_stepper* st, bool learned, hipGraph_t* graph, hipGra
     return MSD_OK;
 }
 
-// nsteps exact (learned = false) or learned steps, graph-replayed when enabled
-static int run_steps(msd_stepper* st, int nsteps, bool learned)
+// nsteps of one kind (0 exact, 1 a learned window's first step, 2 later learned
+// steps), graph-replayed when enabled
+static int run_kind(msd_stepper* st, int nsteps, int kind)
 {
+    if (nsteps <= 0) return MSD_OK;
     if (st->use_graph) {
-        hipGraphExec_t* exec = learned ? &st->lexec : &st->exec;
-        hipGraph_t* graph = learned ? &st->lgraph : &st->graph;
-        StepArgs* gargs = learned ? &st->lgraph_args : &st->graph_args;
-        const bool scales_moved = learned && (st->lgraph_sin != st->sur->s_in || st->lgraph_sout != st->sur->s_out);
+        hipGraphExec_t* exec = kind ? &st->lexec[kind - 1] : &st->exec;
+        hipGraph_t* graph = kind ? &st->lgraph[kind - 1] : &st->graph;
+        StepArgs* gargs = kind ? &st->lgraph_args[kind - 1] : &st->graph_args;
+        const bool scales_moved =
+            kind && (st->lgraph_sin[kind - 1] != st->sur->s_in || st->lgraph_sout[kind - 1] != st->sur->s_out);
         if (!*exec || !args_equal(st->args, *gargs) || scales_moved) {
             if (*exec) (void)hipGraphExecDestroy(*exec);
             if (*graph) (void)hipGraphDestroy(*graph);
             *exec = nullptr; *graph = nullptr;
-            if (int rc = capture_step(st, learned, graph, exec)) return rc;
+            if (int rc = capture_step(st, kind, graph, exec)) return rc;
             *gargs = st->args;
-            if (learned) { st->lgraph_sin = st->sur->s_in; st->lgraph_sout = st->sur->s_out; }
+            if (kind) { st->lgraph_sin[kind - 1] = st->sur->s_in; st->lgraph_sout[kind - 1] = st->sur->s_out; }
         }
         for (int i = 0; i < nsteps; ++i) HIPCHK(hipGraphLaunch(*exec, G.stream));
     } else {
         for (int i = 0; i < nsteps; ++i)
-            if (int rc = learned ? st->launch_learned_step(G.stream) : st->launch_step(G.stream)) return rc;
+            if (int rc = kind ? st->launch_learned_step(G.stream, kind == 1) : st->launch_step(G.stream)) return rc;
     }
     return MSD_OK;
+}
+
+// nsteps exact (learned = false) steps, or a learned window of nsteps steps
+static int run_steps(msd_stepper* st, int nsteps, bool learned)
+{
+    if (!learned) return run_kind(st, nsteps, 0);
+    if (int rc = run_kind(st, 1, 1)) return rc;
+    return run_kind(st, nsteps - 1, 2);
 }
 
 // Windows of K steps.  At each window start the exact layered field of the
@@ -2296,9 +2324,7 @@ extern "C" int msd_stepper_set_surrogate(msd_stepper* st, msd_surrogate* sur, in
     st->sur_fallback = false;
     st->sur_windows = st->sur_exact_windows = st->sur_fallbacks = 0;
     st->sur_last_res = st->sur_max_res = 0.0;
-    if (st->lexec) (void)hipGraphExecDestroy(st->lexec);
-    if (st->lgraph) (void)hipGraphDestroy(st->lgraph);
-    st->lexec = nullptr; st->lgraph = nullptr;
+    st->drop_learned_graphs();
     return MSD_OK;
 }
 
