@@ -208,6 +208,80 @@ __global__ void __launch_bounds__(Z3_TPB) k3_zconv(const Rep3* __restrict__ reps
     }
 }
 
+// Compile-time z length FZ (a power of two, 32..512), W kx columns per workgroup:
+// the three components' z lines (3 W x FZ) transformed in place in one LDS buffer
+// (fft_ip), so W is 4x the runtime pass's at the same LDS and each z plane is read
+// and written as runs of W consecutive kx (64-128 bytes instead of 16).
+template <int FZ, int W>
+__global__ void __launch_bounds__(Z3_TPB) k3_zconv_t(const Rep3* __restrict__ reps, const int4* __restrict__ items)
+{
+    extern __shared__ float2 lds[];
+    constexpr int lines = 3 * W;
+    const int4 it = items[blockIdx.x];
+    const Rep3& rp = reps[it.x];
+    const Conv3Dev& cv = rp.cv;
+    const int nz = rp.a.d.nz;
+    const int fy = cv.fy, fxh = cv.fxh;
+    const int ky = it.y, kx0 = it.z;
+    const int w = min(W, fxh - kx0);
+    float2* A = lds;                                     // [c W + j][FZ]
+    // element idx = zz lines + l: consecutive threads walk the kx of one plane
+    batched_copy<lines * FZ, Z3_TPB, 8>(
+        [&](int idx) {
+            const int zz = idx / lines, l = idx - zz * lines;
+            const int c = l / W, j = l - c * W;
+            return (zz < nz && j < w) ? cv.W[widx(cv, nz, c, zz, ky, kx0 + j)] : make_float2(0.f, 0.f);
+        },
+        [&](int idx, float2 v) {
+            const int zz = idx / lines, l = idx - zz * lines;
+            A[l * FZ + zz] = v;
+        });
+    __syncthreads();
+    fft_ip<FZ, lines, Z3_TPB, -1>(A, cv.twz);
+    const size_t plane = (size_t)FZ * fy * fxh;
+    for (int idx = threadIdx.x; idx < W * FZ; idx += Z3_TPB) {
+        const int kz = idx / W, j = idx - kz * W;
+        if (j >= w) continue;
+        const size_t t = ((size_t)kz * fy + ky) * fxh + kx0 + j;
+        const float xx = cv.Tw[t], xy = cv.Tw[plane + t], xz = cv.Tw[2 * plane + t];
+        const float yy = cv.Tw[3 * plane + t], yz = cv.Tw[4 * plane + t], zz = cv.Tw[5 * plane + t];
+        const float2 mx = A[(0 * W + j) * FZ + kz];
+        const float2 my = A[(1 * W + j) * FZ + kz];
+        const float2 mz = A[(2 * W + j) * FZ + kz];
+        A[(0 * W + j) * FZ + kz] = make_float2(xx * mx.x + xy * my.x + xz * mz.x, xx * mx.y + xy * my.y + xz * mz.y);
+        A[(1 * W + j) * FZ + kz] = make_float2(xy * mx.x + yy * my.x + yz * mz.x, xy * mx.y + yy * my.y + yz * mz.y);
+        A[(2 * W + j) * FZ + kz] = make_float2(xz * mx.x + yz * my.x + zz * mz.x, xz * mx.y + yz * my.y + zz * mz.y);
+    }
+    __syncthreads();
+    fft_ip<FZ, lines, Z3_TPB, 1>(A, cv.twz);
+    for (int idx = threadIdx.x; idx < lines * nz; idx += Z3_TPB) {
+        const int zz = idx / lines, l = idx - zz * lines;
+        const int c = l / W, j = l - c * W;
+        if (j < w) cv.W[widx(cv, nz, c, zz, ky, kx0 + j)] = A[l * FZ + zz];
+    }
+}
+
+int zconv_template_width(int fz)
+{
+    switch (fz) {
+    case 32: case 64: case 128: case 256: return 16;
+    case 512: return 8;
+    default: return 0;
+    }
+}
+
+static const void* zconv_template(int fz)
+{
+    switch (fz) {
+    case 32: return (const void*)k3_zconv_t<32, 16>;
+    case 64: return (const void*)k3_zconv_t<64, 16>;
+    case 128: return (const void*)k3_zconv_t<128, 16>;
+    case 256: return (const void*)k3_zconv_t<256, 16>;
+    case 512: return (const void*)k3_zconv_t<512, 8>;
+    default: return nullptr;
+    }
+}
+
 // --------------------------------------------------------------- launches
 hipError_t launch3_rows(int mode, bool comp, const Rep3* reps, const int4* items, int nitems, size_t lds, hipStream_t st)
 {
@@ -235,10 +309,18 @@ hipError_t launch3_ypass(int sign, const Rep3* reps, const int4* items, int nite
     return hipGetLastError();
 }
 
-hipError_t launch3_zconv(const Rep3* reps, const int4* items, int nitems, size_t lds, hipStream_t st)
+hipError_t launch3_zconv(const Rep3* reps, const int4* items, int nitems, size_t lds, int zt, hipStream_t st)
 {
     if (nitems <= 0) return hipSuccess;
-    k3_zconv<<<nitems, Z3_TPB, lds, st>>>(reps, items);
+    switch (zt) {
+    case 0: k3_zconv<<<nitems, Z3_TPB, lds, st>>>(reps, items); break;
+    case 32: k3_zconv_t<32, 16><<<nitems, Z3_TPB, lds, st>>>(reps, items); break;
+    case 64: k3_zconv_t<64, 16><<<nitems, Z3_TPB, lds, st>>>(reps, items); break;
+    case 128: k3_zconv_t<128, 16><<<nitems, Z3_TPB, lds, st>>>(reps, items); break;
+    case 256: k3_zconv_t<256, 16><<<nitems, Z3_TPB, lds, st>>>(reps, items); break;
+    case 512: k3_zconv_t<512, 8><<<nitems, Z3_TPB, lds, st>>>(reps, items); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -252,6 +334,10 @@ hipError_t configure_lds3(size_t max_bytes)
     hipError_t first = hipSuccess;
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+        if (e != hipSuccess && first == hipSuccess) first = e;
+    }
+    for (int fz : {32, 64, 128, 256, 512}) {
+        hipError_t e = hipFuncSetAttribute(zconv_template(fz), hipFuncAttributeMaxDynamicSharedMemorySize, v);
         if (e != hipSuccess && first == hipSuccess) first = e;
     }
     (void)hipGetLastError();

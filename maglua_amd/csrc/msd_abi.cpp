@@ -1228,7 +1228,7 @@ struct msd_longrange3d {
     float2 *twx = nullptr, *twy = nullptr, *twz = nullptr;
     double2 *twx64 = nullptr, *twy64 = nullptr, *twz64 = nullptr;
     FFTPlan px{}, py{}, pz{};
-    int ty = 1, wy = 1, wz = 1;
+    int ty = 1, wy = 1, wz = 1, zt = 0;
     size_t lds_rows = 0, lds_y = 0, lds_z = 0;
     // single-replica descriptor and work items for the operator API
     Rep3* orep = nullptr;
@@ -1245,7 +1245,7 @@ struct msd_longrange3d {
     {
         Conv3Dev c{};
         c.fx = fx; c.fy = fy; c.fz = fz; c.fxh = fxh;
-        c.ty = ty; c.wy = wy; c.wz = wz;
+        c.ty = ty; c.wy = wy; c.wz = wz; c.zt = zt;
         c.px = px; c.py = py; c.pz = pz;
         c.twx = twx; c.twy = twy; c.twz = twz;
         c.W = W; c.Tw = Tw;
@@ -1284,6 +1284,11 @@ extern "C" int msd_longrange3d_create(int nx, int ny, int nz, int open, msd_long
     l->lds_rows = 2 * (size_t)((3 * l->ty + 1) / 2) * l->fx * sizeof(float2);
     l->lds_y = 2 * (size_t)l->wy * l->fy * sizeof(float2);
     l->lds_z = 2 * (size_t)3 * l->wz * l->fz * sizeof(float2);
+    // the compile-time z pass (in-place transforms, wider kx blocks) where it has an instance
+    if (const int wt = zconv_template_width(l->fz)) {
+        const size_t lz = (size_t)3 * wt * l->fz * sizeof(float2);
+        if (lz <= G.max_lds) { l->zt = l->fz; l->wz = wt; l->lds_z = lz; }
+    }
     if (l->lds_rows > G.max_lds || l->lds_y > G.max_lds || l->lds_z > G.max_lds)
         return fail(MSD_EUNSUPPORTED, "LongRange3D.new: %dx%dx%d transform exceeds the LDS per workgroup", l->fx, l->fy, l->fz);
     if (int rc = twiddles(l->fx, &l->twx, &l->twx64)) return rc;
@@ -1402,7 +1407,7 @@ int msd_longrange3d::prepare()
 // work items of one replica appended to the tables
 static void items3(int rep, const Dims& d, const Conv3Dev* cv, int ty, std::vector<int4>& rows, std::vector<int4>& ys,
                    std::vector<int4>& zs)
-{
+{   // zs: this replica's z-pass items (its own z-pass form, cv->zt)
     for (int z = 0; z < d.nz; ++z)
         for (int y0 = 0; y0 < d.ny; y0 += ty) rows.push_back(make_int4(rep, z, y0, 0));
     if (!cv) return;
@@ -1447,7 +1452,7 @@ extern "C" int msd_longrange3d_apply(msd_longrange3d* l, msd_system* s)
     HIPCHK(hipMemcpyAsync(l->orep, &r, sizeof r, hipMemcpyHostToDevice, G.stream));
     hipError_t e = launch3_rows(R3_FWD, false, l->orep, l->oitems_r, l->onr, l->lds_rows, G.stream);
     if (e == hipSuccess) e = launch3_ypass(-1, l->orep, l->oitems_y, l->ony, l->lds_y, G.stream);
-    if (e == hipSuccess) e = launch3_zconv(l->orep, l->oitems_z, l->onz, l->lds_z, G.stream);
+    if (e == hipSuccess) e = launch3_zconv(l->orep, l->oitems_z, l->onz, l->lds_z, l->zt, G.stream);
     if (e == hipSuccess) e = launch3_ypass(1, l->orep, l->oitems_y, l->ony, l->lds_y, G.stream);
     if (e == hipSuccess) e = launch3_rows(R3_INV, false, l->orep, l->oitems_r, l->onr, l->lds_rows, G.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(G.stream);   // the descriptor lives on the host stack
@@ -1458,19 +1463,31 @@ extern "C" int msd_longrange3d_apply(msd_longrange3d* l, msd_system* s)
 // ------------------------------------------------------------ batch of replicas
 // Device descriptors and work tables of replicas stepped by the same launches.
 struct Batch3 {
+    // the z pass runs once per z-pass form among the members (the compile-time
+    // pass for each z length that has one, the runtime pass for the rest), so every
+    // member is transformed by the same kernel as when it runs alone
+    struct ZGroup {
+        int zt = 0;
+        int4* items = nullptr;
+        int n = 0;
+        size_t lds = 0;
+    };
     Rep3* reps = nullptr;
-    int4 *rows = nullptr, *ys = nullptr, *zs = nullptr;
-    int nrep = 0, nrows = 0, nys = 0, nzs = 0;
-    size_t lds_rows = 0, lds_y = 0, lds_z = 0;
+    int4 *rows = nullptr, *ys = nullptr;
+    int nrep = 0, nrows = 0, nys = 0;
+    size_t lds_rows = 0, lds_y = 0;
+    std::vector<ZGroup> zg;
     bool any_lr = false;
     std::vector<Rep3> host;
     ~Batch3() { release(); }
     void release()
     {
         (void)hipStreamSynchronize(G.stream);
-        dfree(reps); dfree(rows); dfree(ys); dfree(zs);
+        dfree(reps); dfree(rows); dfree(ys);
+        for (ZGroup& z : zg) dfree(z.items);
+        zg.clear();
         host.clear();
-        nrep = nrows = nys = nzs = 0;
+        nrep = nrows = nys = 0;
     }
     // rebuild the tables when the member set changed; refresh descriptors always
     int update(const std::vector<Rep3>& r, const std::vector<int>& tys, const std::vector<size_t>& lds3)
@@ -1478,20 +1495,36 @@ struct Batch3 {
         const bool same_shape = host.size() == r.size() && reps != nullptr;
         if (!same_shape) {
             release();
-            std::vector<int4> R, Y, Z;
+            std::vector<int4> R, Y;
+            std::vector<std::vector<int4>> Z;
+            std::vector<ZGroup> groups;
             any_lr = false;
-            lds_rows = lds_y = lds_z = 0;
+            lds_rows = lds_y = 0;
             for (size_t i = 0; i < r.size(); ++i) {
-                items3((int)i, r[i].a.d, r[i].has_lr ? &r[i].cv : nullptr, tys[i], R, Y, Z);
+                std::vector<int4> zi;
+                items3((int)i, r[i].a.d, r[i].has_lr ? &r[i].cv : nullptr, tys[i], R, Y, zi);
                 any_lr |= r[i].has_lr != 0;
                 lds_rows = std::max(lds_rows, lds3[3 * i]);
                 lds_y = std::max(lds_y, lds3[3 * i + 1]);
-                lds_z = std::max(lds_z, lds3[3 * i + 2]);
+                if (zi.empty()) continue;
+                size_t gi = 0;
+                while (gi < groups.size() && groups[gi].zt != r[i].cv.zt) ++gi;
+                if (gi == groups.size()) {
+                    groups.push_back(ZGroup{});
+                    groups.back().zt = r[i].cv.zt;
+                    Z.emplace_back();
+                }
+                groups[gi].lds = std::max(groups[gi].lds, lds3[3 * i + 2]);
+                Z[gi].insert(Z[gi].end(), zi.begin(), zi.end());
             }
             if (int rc = upload_vec(&rows, R)) return rc;
             if (int rc = upload_vec(&ys, Y)) return rc;
-            if (int rc = upload_vec(&zs, Z)) return rc;
-            nrows = (int)R.size(); nys = (int)Y.size(); nzs = (int)Z.size();
+            for (size_t gi = 0; gi < groups.size(); ++gi) {
+                if (int rc = upload_vec(&groups[gi].items, Z[gi])) return rc;
+                groups[gi].n = (int)Z[gi].size();
+            }
+            zg = groups;
+            nrows = (int)R.size(); nys = (int)Y.size();
             if (int rc = dalloc(&reps, r.size())) return rc;
             nrep = (int)r.size();
         }
@@ -1502,12 +1535,19 @@ struct Batch3 {
         }
         return MSD_OK;
     }
+    hipError_t zpass(hipStream_t st) const
+    {
+        hipError_t e = hipSuccess;
+        for (const ZGroup& z : zg)
+            if (e == hipSuccess) e = launch3_zconv(reps, z.items, z.n, z.lds, z.zt, st);
+        return e;
+    }
     hipError_t evaluate(int row_mode, bool comp, hipStream_t st) const
     {
         hipError_t e = hipSuccess;
         if (any_lr) {
             e = launch3_ypass(-1, reps, ys, nys, lds_y, st);
-            if (e == hipSuccess) e = launch3_zconv(reps, zs, nzs, lds_z, st);
+            if (e == hipSuccess) e = zpass(st);
             if (e == hipSuccess) e = launch3_ypass(1, reps, ys, nys, lds_y, st);
         }
         if (e == hipSuccess) e = launch3_rows(row_mode, comp, reps, rows, nrows, lds_rows, st);
@@ -2357,7 +2397,7 @@ static hipError_t launch_code(const msd_stepper* st, int code, hipStream_t s)
         case L_PRED2: return launch_step_pred(st->args, s);
         case L_CORR2: return launch_step_corr(st->args, s);
         case L_Y_FWD: return launch3_ypass(-1, b.reps, b.ys, b.nys, b.lds_y, s);
-        case L_ZCONV: return launch3_zconv(b.reps, b.zs, b.nzs, b.lds_z, s);
+        case L_ZCONV: return b.zpass(s);
         case L_Y_INV: return launch3_ypass(1, b.reps, b.ys, b.nys, b.lds_y, s);
         case L_PRED3: return launch3_rows(R3_PRED, st->compensated, b.reps, b.rows, b.nrows, b.lds_rows, s);
         default: return launch3_rows(R3_CORR, st->compensated, b.reps, b.rows, b.nrows, b.lds_rows, s);

@@ -153,6 +153,7 @@ struct Conv3Dev {
     int fx, fy, fz, fxh;
     int ty;                 // rows per row tile
     int wy, wz;             // kx block widths of the y and z passes
+    int zt;                 // z length of the compile-time z pass (0: the runtime-length pass)
     FFTPlan px, py, pz;
     const float2* twx;
     const float2* twy;
@@ -176,7 +177,10 @@ enum Row3Mode { R3_FWD = 0, R3_INV = 1, R3_PRED = 2, R3_CORR = 3 };
 // q = kx0; z pass: p = ky, q = kx0
 hipError_t launch3_rows(int mode, bool comp, const Rep3* reps, const int4* items, int nitems, size_t lds, hipStream_t st);
 hipError_t launch3_ypass(int sign, const Rep3* reps, const int4* items, int nitems, size_t lds, hipStream_t st);
-hipError_t launch3_zconv(const Rep3* reps, const int4* items, int nitems, size_t lds, hipStream_t st);
+// zt: every replica's compile-time z length (Conv3Dev::zt), or 0 for the runtime-length pass
+hipError_t launch3_zconv(const Rep3* reps, const int4* items, int nitems, size_t lds, int zt, hipStream_t st);
+// kx width of the compile-time z pass for a z length (0 when it has no instance)
+int zconv_template_width(int fz);
 // tensor: periodic lattice sum (R > 0) or open kernel (R == 0) on the FFT grid, FP64
 hipError_t launch3_tensor(int nx, int ny, int nz, int fx, int fy, int fz, int R, double* T, hipStream_t st);
 // real spectrum words of T: U1/U2 complex FP64 scratch [6][fz][fy][fxh]; tw* FP64 tables

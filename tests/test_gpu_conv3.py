@@ -22,7 +22,9 @@ def oracle_lr3(dims, open_, R, g):
     return lr
 
 
-@pytest.mark.parametrize("dims,R", [((8, 8, 8), 12), ((12, 10, 6), 9), ((16, 8, 4), 10), ((6, 6, 20), 8)])
+# z lengths 32 and 64 take the compile-time z pass (partial kx blocks included)
+@pytest.mark.parametrize("dims,R", [((8, 8, 8), 12), ((12, 10, 6), 9), ((16, 8, 4), 10), ((6, 6, 20), 8),
+                                    ((8, 6, 32), 9), ((20, 4, 64), 8)])
 def test_longrange3d_periodic_op(dims, R):
     n = dims[0] * dims[1] * dims[2]
     s = np.random.default_rng(sum(dims)).standard_normal((3, n))
@@ -38,7 +40,9 @@ def test_longrange3d_periodic_op(dims, R):
     assert np.abs(hg - ho).max() <= 2e-5 * np.abs(ho).max()
 
 
-@pytest.mark.parametrize("dims", [(8, 6, 4), (10, 12, 6), (16, 16, 16)])
+# open z lengths 32, 64, 128, 256 and 512 (padded from 16..256) take the compile-time z pass
+@pytest.mark.parametrize("dims", [(8, 6, 4), (10, 12, 6), (16, 16, 16), (6, 4, 32), (4, 6, 64), (4, 4, 128),
+                                  (2, 4, 256)])
 def test_longrange3d_open_op(dims):
     n = dims[0] * dims[1] * dims[2]
     s = np.random.default_rng(n).standard_normal((3, n))
@@ -121,8 +125,10 @@ def test_heun_open_longrange3d():
 
 
 def test_ensemble_matches_single_replicas():
-    """Uneven replicas stepped by shared launches give exactly the single-replica results."""
-    cfg = [((16, 16, 16), 0.3, 1), ((12, 10, 6), 0.8, 2), ((8, 8, 8), 1.5, 3), ((10, 6, 4), 0.0, 4)]
+    """Uneven replicas stepped by shared launches give exactly the single-replica results
+    (the z lengths 32 and 64 take the compile-time z pass, the others the runtime one)."""
+    cfg = [((16, 16, 16), 0.3, 1), ((12, 10, 6), 0.8, 2), ((8, 8, 8), 1.5, 3), ((10, 6, 4), 0.0, 4),
+           ((8, 8, 32), 0.5, 5), ((6, 4, 64), 0.2, 6)]
     ens, solo = [], []
     for dims, T, seed in cfg:
         for bucket in (ens, solo):
