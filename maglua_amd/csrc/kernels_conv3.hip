@@ -165,6 +165,73 @@ __global__ void __launch_bounds__(Y3_TPB) k3_ypass(const Rep3* __restrict__ reps
     }
 }
 
+// Compile-time y length FY (a power of two, 32..512), W = 16 kx columns per
+// workgroup, transformed in place (fft_ip): twice the runtime pass's width at the
+// same LDS, so every y row is read and written as 128-byte runs.
+template <int SIGN, int FY, int W>
+__global__ void __launch_bounds__(Y3_TPB) k3_ypass_t(const Rep3* __restrict__ reps, const int4* __restrict__ items)
+{
+    extern __shared__ float2 lds[];
+    const int4 it = items[blockIdx.x];
+    const Rep3& rp = reps[it.x];
+    const Conv3Dev& cv = rp.cv;
+    const int ny = rp.a.d.ny;
+    const int fxh = cv.fxh;
+    const int kx0 = it.z;
+    const int w = min(W, fxh - kx0);
+    float2* A = lds;                                     // [j][FY]
+    float2* base = cv.W + (size_t)it.y * FY * fxh + kx0;
+    const int yin = SIGN < 0 ? ny : FY;
+    batched_copy<W * FY, Y3_TPB, 8>(
+        [&](int idx) {
+            const int y = idx / W, j = idx - y * W;
+            return (y < yin && j < w) ? base[(size_t)y * fxh + j] : make_float2(0.f, 0.f);
+        },
+        [&](int idx, float2 v) {
+            const int y = idx / W, j = idx - y * W;
+            A[j * FY + y] = v;
+        });
+    __syncthreads();
+    fft_ip<FY, W, Y3_TPB, SIGN>(A, cv.twy);
+    const int yout = SIGN < 0 ? FY : ny;
+    for (int idx = threadIdx.x; idx < W * yout; idx += Y3_TPB) {
+        const int y = idx / W, j = idx - y * W;
+        if (j < w) base[(size_t)y * fxh + j] = A[j * FY + y];
+    }
+}
+
+int ypass_template_width(int fy)
+{
+    return (fy == 32 || fy == 64 || fy == 128 || fy == 256 || fy == 512) ? 16 : 0;
+}
+
+template <int SIGN>
+static const void* ypass_template(int fy)
+{
+    switch (fy) {
+    case 32: return (const void*)k3_ypass_t<SIGN, 32, 16>;
+    case 64: return (const void*)k3_ypass_t<SIGN, 64, 16>;
+    case 128: return (const void*)k3_ypass_t<SIGN, 128, 16>;
+    case 256: return (const void*)k3_ypass_t<SIGN, 256, 16>;
+    case 512: return (const void*)k3_ypass_t<SIGN, 512, 16>;
+    default: return nullptr;
+    }
+}
+
+template <int SIGN>
+static hipError_t launch_ypass_t(int yt, const Rep3* reps, const int4* items, int nitems, size_t lds, hipStream_t st)
+{
+    switch (yt) {
+    case 32: k3_ypass_t<SIGN, 32, 16><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
+    case 64: k3_ypass_t<SIGN, 64, 16><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
+    case 128: k3_ypass_t<SIGN, 128, 16><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
+    case 256: k3_ypass_t<SIGN, 256, 16><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
+    case 512: k3_ypass_t<SIGN, 512, 16><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 // ----------------------------------------------------------------- z pass
 __global__ void __launch_bounds__(Z3_TPB) k3_zconv(const Rep3* __restrict__ reps, const int4* __restrict__ items)
 {
@@ -301,9 +368,11 @@ hipError_t launch3_rows(int mode, bool comp, const Rep3* reps, const int4* items
     return hipGetLastError();
 }
 
-hipError_t launch3_ypass(int sign, const Rep3* reps, const int4* items, int nitems, size_t lds, hipStream_t st)
+hipError_t launch3_ypass(int sign, const Rep3* reps, const int4* items, int nitems, size_t lds, int yt, hipStream_t st)
 {
     if (nitems <= 0) return hipSuccess;
+    if (yt) return sign < 0 ? launch_ypass_t<-1>(yt, reps, items, nitems, lds, st)
+                            : launch_ypass_t<1>(yt, reps, items, nitems, lds, st);
     if (sign < 0) k3_ypass<-1><<<nitems, Y3_TPB, lds, st>>>(reps, items);
     else k3_ypass<1><<<nitems, Y3_TPB, lds, st>>>(reps, items);
     return hipGetLastError();
@@ -336,10 +405,11 @@ hipError_t configure_lds3(size_t max_bytes)
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, v);
         if (e != hipSuccess && first == hipSuccess) first = e;
     }
-    for (int fz : {32, 64, 128, 256, 512}) {
-        hipError_t e = hipFuncSetAttribute(zconv_template(fz), hipFuncAttributeMaxDynamicSharedMemorySize, v);
-        if (e != hipSuccess && first == hipSuccess) first = e;
-    }
+    for (int f : {32, 64, 128, 256, 512})
+        for (const void* k : {zconv_template(f), ypass_template<-1>(f), ypass_template<1>(f)}) {
+            hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+            if (e != hipSuccess && first == hipSuccess) first = e;
+        }
     (void)hipGetLastError();
     return first;
 }

@@ -1228,7 +1228,7 @@ struct msd_longrange3d {
     float2 *twx = nullptr, *twy = nullptr, *twz = nullptr;
     double2 *twx64 = nullptr, *twy64 = nullptr, *twz64 = nullptr;
     FFTPlan px{}, py{}, pz{};
-    int ty = 1, wy = 1, wz = 1, zt = 0;
+    int ty = 1, wy = 1, wz = 1, zt = 0, yt = 0;
     size_t lds_rows = 0, lds_y = 0, lds_z = 0;
     // single-replica descriptor and work items for the operator API
     Rep3* orep = nullptr;
@@ -1245,7 +1245,7 @@ struct msd_longrange3d {
     {
         Conv3Dev c{};
         c.fx = fx; c.fy = fy; c.fz = fz; c.fxh = fxh;
-        c.ty = ty; c.wy = wy; c.wz = wz; c.zt = zt;
+        c.ty = ty; c.wy = wy; c.wz = wz; c.zt = zt; c.yt = yt;
         c.px = px; c.py = py; c.pz = pz;
         c.twx = twx; c.twy = twy; c.twz = twz;
         c.W = W; c.Tw = Tw;
@@ -1284,7 +1284,12 @@ extern "C" int msd_longrange3d_create(int nx, int ny, int nz, int open, msd_long
     l->lds_rows = 2 * (size_t)((3 * l->ty + 1) / 2) * l->fx * sizeof(float2);
     l->lds_y = 2 * (size_t)l->wy * l->fy * sizeof(float2);
     l->lds_z = 2 * (size_t)3 * l->wz * l->fz * sizeof(float2);
-    // the compile-time z pass (in-place transforms, wider kx blocks) where it has an instance
+    // the compile-time y and z passes (in-place transforms, wider kx blocks) where
+    // they have instances
+    if (const int wt = ypass_template_width(l->fy)) {
+        const size_t ly = (size_t)wt * l->fy * sizeof(float2);
+        if (ly <= G.max_lds) { l->yt = l->fy; l->wy = wt; l->lds_y = ly; }
+    }
     if (const int wt = zconv_template_width(l->fz)) {
         const size_t lz = (size_t)3 * wt * l->fz * sizeof(float2);
         if (lz <= G.max_lds) { l->zt = l->fz; l->wz = wt; l->lds_z = lz; }
@@ -1451,9 +1456,9 @@ extern "C" int msd_longrange3d_apply(msd_longrange3d* l, msd_system* s)
     }
     HIPCHK(hipMemcpyAsync(l->orep, &r, sizeof r, hipMemcpyHostToDevice, G.stream));
     hipError_t e = launch3_rows(R3_FWD, false, l->orep, l->oitems_r, l->onr, l->lds_rows, G.stream);
-    if (e == hipSuccess) e = launch3_ypass(-1, l->orep, l->oitems_y, l->ony, l->lds_y, G.stream);
+    if (e == hipSuccess) e = launch3_ypass(-1, l->orep, l->oitems_y, l->ony, l->lds_y, l->yt, G.stream);
     if (e == hipSuccess) e = launch3_zconv(l->orep, l->oitems_z, l->onz, l->lds_z, l->zt, G.stream);
-    if (e == hipSuccess) e = launch3_ypass(1, l->orep, l->oitems_y, l->ony, l->lds_y, G.stream);
+    if (e == hipSuccess) e = launch3_ypass(1, l->orep, l->oitems_y, l->ony, l->lds_y, l->yt, G.stream);
     if (e == hipSuccess) e = launch3_rows(R3_INV, false, l->orep, l->oitems_r, l->onr, l->lds_rows, G.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(G.stream);   // the descriptor lives on the host stack
     if (e != hipSuccess) return fail(MSD_EDEVICE, "LongRange3D:apply: %s", hipGetErrorString(e));
@@ -1463,9 +1468,9 @@ extern "C" int msd_longrange3d_apply(msd_longrange3d* l, msd_system* s)
 // ------------------------------------------------------------ batch of replicas
 // Device descriptors and work tables of replicas stepped by the same launches.
 struct Batch3 {
-    // the z pass runs once per z-pass form among the members (the compile-time
-    // pass for each z length that has one, the runtime pass for the rest), so every
-    // member is transformed by the same kernel as when it runs alone
+    // the y and z passes run once per pass form among the members (the
+    // compile-time pass for each length that has one, the runtime pass for the
+    // rest), so every member is transformed by the same kernels as when it runs alone
     struct ZGroup {
         int zt = 0;
         int4* items = nullptr;
@@ -1473,21 +1478,22 @@ struct Batch3 {
         size_t lds = 0;
     };
     Rep3* reps = nullptr;
-    int4 *rows = nullptr, *ys = nullptr;
-    int nrep = 0, nrows = 0, nys = 0;
-    size_t lds_rows = 0, lds_y = 0;
-    std::vector<ZGroup> zg;
+    int4* rows = nullptr;
+    int nrep = 0, nrows = 0;
+    size_t lds_rows = 0;
+    std::vector<ZGroup> yg, zg;          // y-pass and z-pass groups (key: Conv3Dev::yt / zt)
     bool any_lr = false;
     std::vector<Rep3> host;
     ~Batch3() { release(); }
     void release()
     {
         (void)hipStreamSynchronize(G.stream);
-        dfree(reps); dfree(rows); dfree(ys);
+        dfree(reps); dfree(rows);
+        for (ZGroup& z : yg) dfree(z.items);
         for (ZGroup& z : zg) dfree(z.items);
-        zg.clear();
+        yg.clear(); zg.clear();
         host.clear();
-        nrep = nrows = nys = 0;
+        nrep = nrows = 0;
     }
     // rebuild the tables when the member set changed; refresh descriptors always
     int update(const std::vector<Rep3>& r, const std::vector<int>& tys, const std::vector<size_t>& lds3)
@@ -1495,36 +1501,45 @@ struct Batch3 {
         const bool same_shape = host.size() == r.size() && reps != nullptr;
         if (!same_shape) {
             release();
-            std::vector<int4> R, Y;
-            std::vector<std::vector<int4>> Z;
-            std::vector<ZGroup> groups;
+            std::vector<int4> R;
+            std::vector<std::vector<int4>> Y, Z;
+            std::vector<ZGroup> ygroups, zgroups;
             any_lr = false;
-            lds_rows = lds_y = 0;
-            for (size_t i = 0; i < r.size(); ++i) {
-                std::vector<int4> zi;
-                items3((int)i, r[i].a.d, r[i].has_lr ? &r[i].cv : nullptr, tys[i], R, Y, zi);
-                any_lr |= r[i].has_lr != 0;
-                lds_rows = std::max(lds_rows, lds3[3 * i]);
-                lds_y = std::max(lds_y, lds3[3 * i + 1]);
-                if (zi.empty()) continue;
+            lds_rows = 0;
+            // append a member's pass items to the group of its pass form
+            auto add = [](std::vector<ZGroup>& groups, std::vector<std::vector<int4>>& lists, int key, size_t lds,
+                          const std::vector<int4>& items) {
+                if (items.empty()) return;
                 size_t gi = 0;
-                while (gi < groups.size() && groups[gi].zt != r[i].cv.zt) ++gi;
+                while (gi < groups.size() && groups[gi].zt != key) ++gi;
                 if (gi == groups.size()) {
                     groups.push_back(ZGroup{});
-                    groups.back().zt = r[i].cv.zt;
-                    Z.emplace_back();
+                    groups.back().zt = key;
+                    lists.emplace_back();
                 }
-                groups[gi].lds = std::max(groups[gi].lds, lds3[3 * i + 2]);
-                Z[gi].insert(Z[gi].end(), zi.begin(), zi.end());
+                groups[gi].lds = std::max(groups[gi].lds, lds);
+                lists[gi].insert(lists[gi].end(), items.begin(), items.end());
+            };
+            for (size_t i = 0; i < r.size(); ++i) {
+                std::vector<int4> yi, zi;
+                items3((int)i, r[i].a.d, r[i].has_lr ? &r[i].cv : nullptr, tys[i], R, yi, zi);
+                any_lr |= r[i].has_lr != 0;
+                lds_rows = std::max(lds_rows, lds3[3 * i]);
+                add(ygroups, Y, r[i].cv.yt, lds3[3 * i + 1], yi);
+                add(zgroups, Z, r[i].cv.zt, lds3[3 * i + 2], zi);
             }
             if (int rc = upload_vec(&rows, R)) return rc;
-            if (int rc = upload_vec(&ys, Y)) return rc;
-            for (size_t gi = 0; gi < groups.size(); ++gi) {
-                if (int rc = upload_vec(&groups[gi].items, Z[gi])) return rc;
-                groups[gi].n = (int)Z[gi].size();
+            for (size_t gi = 0; gi < ygroups.size(); ++gi) {
+                if (int rc = upload_vec(&ygroups[gi].items, Y[gi])) return rc;
+                ygroups[gi].n = (int)Y[gi].size();
             }
-            zg = groups;
-            nrows = (int)R.size(); nys = (int)Y.size();
+            for (size_t gi = 0; gi < zgroups.size(); ++gi) {
+                if (int rc = upload_vec(&zgroups[gi].items, Z[gi])) return rc;
+                zgroups[gi].n = (int)Z[gi].size();
+            }
+            yg = ygroups;
+            zg = zgroups;
+            nrows = (int)R.size();
             if (int rc = dalloc(&reps, r.size())) return rc;
             nrep = (int)r.size();
         }
@@ -1542,13 +1557,20 @@ struct Batch3 {
             if (e == hipSuccess) e = launch3_zconv(reps, z.items, z.n, z.lds, z.zt, st);
         return e;
     }
+    hipError_t ypass(int sign, hipStream_t st) const
+    {
+        hipError_t e = hipSuccess;
+        for (const ZGroup& y : yg)
+            if (e == hipSuccess) e = launch3_ypass(sign, reps, y.items, y.n, y.lds, y.zt, st);
+        return e;
+    }
     hipError_t evaluate(int row_mode, bool comp, hipStream_t st) const
     {
         hipError_t e = hipSuccess;
         if (any_lr) {
-            e = launch3_ypass(-1, reps, ys, nys, lds_y, st);
+            e = ypass(-1, st);
             if (e == hipSuccess) e = zpass(st);
-            if (e == hipSuccess) e = launch3_ypass(1, reps, ys, nys, lds_y, st);
+            if (e == hipSuccess) e = ypass(1, st);
         }
         if (e == hipSuccess) e = launch3_rows(row_mode, comp, reps, rows, nrows, lds_rows, st);
         return e;
@@ -2396,9 +2418,9 @@ static hipError_t launch_code(const msd_stepper* st, int code, hipStream_t s)
         case L_COLS: return launch_cols(st->args.dp, s);
         case L_PRED2: return launch_step_pred(st->args, s);
         case L_CORR2: return launch_step_corr(st->args, s);
-        case L_Y_FWD: return launch3_ypass(-1, b.reps, b.ys, b.nys, b.lds_y, s);
+        case L_Y_FWD: return b.ypass(-1, s);
         case L_ZCONV: return b.zpass(s);
-        case L_Y_INV: return launch3_ypass(1, b.reps, b.ys, b.nys, b.lds_y, s);
+        case L_Y_INV: return b.ypass(1, s);
         case L_PRED3: return launch3_rows(R3_PRED, st->compensated, b.reps, b.rows, b.nrows, b.lds_rows, s);
         default: return launch3_rows(R3_CORR, st->compensated, b.reps, b.rows, b.nrows, b.lds_rows, s);
     }
