@@ -33,6 +33,103 @@ __device__ __forceinline__ size_t widx(const Conv3Dev& cv, int nz, int c, int z,
 }
 
 // ------------------------------------------------------------------- rows
+// A row tile is ty rows y0.. of plane z.  Its 3*ty real rows (component-major,
+// q = c*ty + r) are packed two to a complex line of length fx: line l holds rows
+// 2l (real part) and 2l+1 (imaginary part).  Tiles read and write the spectrum
+// W[c][z][y][kx] row by row, kx fastest across the threads (contiguous runs).
+
+// Copy a small exchange class table into LDS; larger tables stay in global memory.
+template <int MODE>
+__device__ __forceinline__ ExchangeDev rows_exchange(const ExchangeDev& ex, int* s_start, ExEntry* s_ent)
+{
+    ExchangeDev e = ex;
+    if (MODE == R3_PRED || MODE == R3_CORR) {
+        if (ex.mode == 1 && ex.ncls + 1 <= 64 && ex.nent <= 512) {
+            for (int i = threadIdx.x; i <= ex.ncls; i += blockDim.x) s_start[i] = ex.cls_start[i];
+            for (int i = threadIdx.x; i < ex.nent; i += blockDim.x) s_ent[i] = ex.ent[i];
+            e.cls_start = s_start;
+            e.ent = s_ent;
+        }
+    }
+    return e;
+}
+
+// The two spectrum values of packed line l at kx = kk (0 beyond the tile's rows).
+__device__ __forceinline__ float4 rows_load_pair(const Conv3Dev& cv, int nz, int z, int y0, int ty, int l, int kk)
+{
+    const int q0 = 2 * l, q1 = q0 + 1;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (q0 < 3 * ty) {
+        const int c0 = q0 / ty, r0 = q0 - c0 * ty;
+        const float2 a = cv.W[widx(cv, nz, c0, z, y0 + r0, kk)];
+        v.x = a.x; v.y = a.y;
+        if (q1 < 3 * ty) {
+            const int c1 = q1 / ty, r1 = q1 - c1 * ty;
+            const float2 b = cv.W[widx(cv, nz, c1, z, y0 + r1, kk)];
+            v.z = b.x; v.w = b.y;
+        }
+    }
+    return v;
+}
+
+// Packed line value at k from the pair read at kk = min(k, fx - k): the half
+// spectra of two real rows A, B combine as A + iB, with k > fx/2 the conjugates.
+__device__ __forceinline__ float2 rows_pack(float4 v, bool mirror)
+{
+    if (mirror) { v.y = -v.y; v.w = -v.w; }
+    return make_float2(v.x - v.w, v.y + v.z);
+}
+
+// Split transformed line l at k (Zk) and fx - k (Zm) into the half spectra of
+// its two rows and store them.
+__device__ __forceinline__ void rows_store_pair(const Conv3Dev& cv, int nz, int z, int y0, int ty, int l, int k, float2 Zk,
+                                                float2 Zm)
+{
+    const int q0 = 2 * l, q1 = q0 + 1;
+    const int c0 = q0 / ty, r0 = q0 - c0 * ty;
+    cv.W[widx(cv, nz, c0, z, y0 + r0, k)] = make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y));
+    if (q1 < 3 * ty) {
+        const int c1 = q1 / ty, r1 = q1 - c1 * ty;
+        cv.W[widx(cv, nz, c1, z, y0 + r1, k)] = make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x));
+    }
+}
+
+// Site (x, y0 + r, z) of the tile: read its dipolar field from the transformed
+// rows R (with a convolution), then per MODE store the field (INV), or take the
+// spins (FWD) or the Heun predictor / corrector update, and put them in R for the
+// forward transform.  The padded tail x in [nx, fx) of every row is zero.
+template <int MODE, bool COMP>
+__device__ __forceinline__ void rows_site(const Rep3& rp, const ExchangeDev& exl, float2* R, int fx, int ty, int r, int x,
+                                          int y0, int z, uint64_t draw)
+{
+    const StepArgs& a = rp.a;
+    const Conv3Dev& cv = rp.cv;
+    const bool lr = rp.has_lr != 0;
+    const int nx = a.d.nx, n = a.d.n;
+    float3 v = make_float3(0.f, 0.f, 0.f);
+    if (x < nx) {
+        const int i = x + nx * (y0 + r + a.d.ny * z);
+        if (MODE == R3_FWD) {
+            v = make_float3(cv.sin[i], cv.sin[n + i], cv.sin[2 * n + i]);
+        } else {
+            float3 Hd = make_float3(0.f, 0.f, 0.f);
+            if (lr) Hd = make_float3(rowval3(R, r, x, fx), rowval3(R, ty + r, x, fx), rowval3(R, 2 * ty + r, x, fx));
+            if (MODE == R3_INV) {
+                cv.hout[i] = Hd.x; cv.hout[n + i] = Hd.y; cv.hout[2 * n + i] = Hd.z;
+            } else if (MODE == R3_PRED) {
+                v = site_pred<COMP>(a, exl, i, Hd);
+            } else {
+                v = site_corr<COMP>(a, exl, i, Hd, draw);
+            }
+        }
+    }
+    if (lr && MODE != R3_INV) {
+        rowval3(R, r, x, fx) = v.x;
+        rowval3(R, ty + r, x, fx) = v.y;
+        rowval3(R, 2 * ty + r, x, fx) = v.z;
+    }
+}
+
 template <int MODE, bool COMP>
 __global__ void __launch_bounds__(R3_TPB) k3_rows(const Rep3* __restrict__ reps, const int4* __restrict__ items)
 {
@@ -43,7 +140,7 @@ __global__ void __launch_bounds__(R3_TPB) k3_rows(const Rep3* __restrict__ reps,
     const Rep3& rp = reps[it.x];
     const StepArgs& a = rp.a;
     const Conv3Dev& cv = rp.cv;
-    const int nx = a.d.nx, ny = a.d.ny, nz = a.d.nz, n = a.d.n;
+    const int ny = a.d.ny, nz = a.d.nz;
     const int fx = cv.fx, fxh = cv.fxh, half = fx >> 1;
     const int z = it.y, y0 = it.z;
     const int ty = min(cv.ty, ny - y0);
@@ -53,31 +150,13 @@ __global__ void __launch_bounds__(R3_TPB) k3_rows(const Rep3* __restrict__ reps,
     float2* B = lds + (size_t)lines * fx;
     float2* R = A;
 
-    ExchangeDev exl = a.ex;
-    if (MODE == R3_PRED || MODE == R3_CORR) {
-        if (exl.mode == 1 && exl.ncls + 1 <= 64 && exl.nent <= 512) {
-            for (int i = threadIdx.x; i <= exl.ncls; i += R3_TPB) s_start[i] = exl.cls_start[i];
-            for (int i = threadIdx.x; i < exl.nent; i += R3_TPB) s_ent[i] = exl.ent[i];
-            exl.cls_start = s_start;
-            exl.ent = s_ent;
-        }
-    }
+    const ExchangeDev exl = rows_exchange<MODE>(a.ex, s_start, s_ent);
     // inverse x transform of the field spectrum rows
     if (lr && MODE != R3_FWD) {
         for (int idx = threadIdx.x; idx < lines * fx; idx += R3_TPB) {
-            const int l = idx % lines, k = idx / lines;
+            const int l = idx / fx, k = idx - l * fx;
             const bool mirror = k > half;
-            const int kk = mirror ? fx - k : k;
-            const int q0 = 2 * l, q1 = q0 + 1;
-            const int c0 = q0 / ty, r0 = q0 - c0 * ty;
-            float2 Av = cv.W[widx(cv, nz, c0, z, y0 + r0, kk)];
-            float2 Bv = make_float2(0.f, 0.f);
-            if (q1 < 3 * ty) {
-                const int c1 = q1 / ty, r1 = q1 - c1 * ty;
-                Bv = cv.W[widx(cv, nz, c1, z, y0 + r1, kk)];
-            }
-            if (mirror) { Av.y = -Av.y; Bv.y = -Bv.y; }
-            A[(size_t)l * fx + k] = make_float2(Av.x - Bv.y, Av.y + Bv.x);
+            A[idx] = rows_pack(rows_load_pair(cv, nz, z, y0, ty, l, mirror ? fx - k : k), mirror);
         }
         __syncthreads();
         R = lds_fft<1>(A, B, lines, cv.px, cv.twx);
@@ -87,31 +166,9 @@ __global__ void __launch_bounds__(R3_TPB) k3_rows(const Rep3* __restrict__ reps,
     if (MODE == R3_PRED && a.th.on && z == 0 && y0 == 0 && threadIdx.x == 0) *a.draw += 1ull;
     const uint64_t draw = (MODE == R3_CORR && a.th.on) ? (uint64_t)*a.draw : 0ull;
 
-    // sites of the tile; the padded tail x in [nx, fx) of every row is zero
     for (int li = threadIdx.x; li < ty * fx; li += R3_TPB) {
-        const int r = li / fx, x = li - r * fx, y = y0 + r;
-        float3 v = make_float3(0.f, 0.f, 0.f);
-        if (x < nx) {
-            const int i = x + nx * (y + ny * z);
-            if (MODE == R3_FWD) {
-                v = make_float3(cv.sin[i], cv.sin[n + i], cv.sin[2 * n + i]);
-            } else {
-                float3 Hd = make_float3(0.f, 0.f, 0.f);
-                if (lr) Hd = make_float3(rowval3(R, r, x, fx), rowval3(R, ty + r, x, fx), rowval3(R, 2 * ty + r, x, fx));
-                if (MODE == R3_INV) {
-                    cv.hout[i] = Hd.x; cv.hout[n + i] = Hd.y; cv.hout[2 * n + i] = Hd.z;
-                } else if (MODE == R3_PRED) {
-                    v = site_pred<COMP>(a, exl, i, Hd);
-                } else {
-                    v = site_corr<COMP>(a, exl, i, Hd, draw);
-                }
-            }
-        }
-        if (lr && MODE != R3_INV) {
-            rowval3(R, r, x, fx) = v.x;
-            rowval3(R, ty + r, x, fx) = v.y;
-            rowval3(R, 2 * ty + r, x, fx) = v.z;
-        }
+        const int r = li / fx, x = li - r * fx;
+        rows_site<MODE, COMP>(rp, exl, R, fx, ty, r, x, y0, z, draw);
     }
     if (!lr || MODE == R3_INV) return;
     // the unpaired imaginary half of the last line (odd 3*ty) must be zero
@@ -121,16 +178,75 @@ __global__ void __launch_bounds__(R3_TPB) k3_rows(const Rep3* __restrict__ reps,
     __syncthreads();
     const float2* F = lds_fft<-1>(R, R == A ? B : A, lines, cv.px, cv.twx);
     for (int idx = threadIdx.x; idx < lines * fxh; idx += R3_TPB) {
-        const int l = idx % lines, k = idx / lines;
-        const float2 Zk = F[(size_t)l * fx + k];
-        const float2 Zm = F[(size_t)l * fx + (k ? fx - k : 0)];
-        const int q0 = 2 * l, q1 = q0 + 1;
-        const int c0 = q0 / ty, r0 = q0 - c0 * ty;
-        cv.W[widx(cv, nz, c0, z, y0 + r0, k)] = make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y));
-        if (q1 < 3 * ty) {
-            const int c1 = q1 / ty, r1 = q1 - c1 * ty;
-            cv.W[widx(cv, nz, c1, z, y0 + r1, k)] = make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x));
-        }
+        const int l = idx / fxh, k = idx - l * fxh;
+        rows_store_pair(cv, nz, z, y0, ty, l, k, F[(size_t)l * fx + k], F[(size_t)l * fx + (k ? fx - k : 0)]);
+    }
+}
+
+// Compile-time x length FX (a power of two, 32..512) and tile height TY: both x
+// transforms run in place (fft_ip) with the twiddles staged in LDS, the spectrum
+// rows are loaded in batches, and the site loop has a fixed trip count.  A tile
+// at the lattice's y edge may hold fewer rows; its unused lines are zero.
+template <int FX>
+__host__ __device__ constexpr int rows_template_ty()
+{
+    return FX >= 512 ? 4 : FX == 256 ? 8 : FX == 128 ? 16 : 32;
+}
+
+template <int MODE, bool COMP, int FX>
+__global__ void __launch_bounds__(R3_TPB) k3_rows_t(const Rep3* __restrict__ reps, const int4* __restrict__ items)
+{
+    constexpr int TY = rows_template_ty<FX>(), LINES = (3 * TY + 1) / 2, FXH = FX / 2 + 1, HALF = FX / 2;
+    extern __shared__ float2 lds[];                     // [LINES][FX] lines, then [FX] twiddles
+    __shared__ int s_start[64];
+    __shared__ ExEntry s_ent[512];
+    const int4 it = items[blockIdx.x];
+    const Rep3& rp = reps[it.x];
+    const StepArgs& a = rp.a;
+    const Conv3Dev& cv = rp.cv;
+    const int nz = a.d.nz;
+    const int z = it.y, y0 = it.z;
+    const int ty = min(TY, a.d.ny - y0);
+    const int lines = (3 * ty + 1) >> 1;
+    const bool lr = rp.has_lr != 0;
+    float2* A = lds;
+    float2* s_tw = lds + LINES * FX;
+
+    const ExchangeDev exl = rows_exchange<MODE>(a.ex, s_start, s_ent);
+    if (lr)
+        for (int i = threadIdx.x; i < FX; i += R3_TPB) s_tw[i] = cv.twx[i];
+    if (lr && MODE != R3_FWD) {
+        batched_copy<LINES * FX, R3_TPB, 8>(
+            [&](int idx) {
+                const int l = idx / FX, k = idx & (FX - 1);
+                return l < lines ? rows_load_pair(cv, nz, z, y0, ty, l, k > HALF ? FX - k : k)
+                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+            },
+            [&](int idx, float4 v) { A[idx] = rows_pack(v, (idx & (FX - 1)) > HALF); });
+        __syncthreads();
+        fft_ip<FX, LINES, R3_TPB, 1>(A, s_tw);
+    } else {
+        __syncthreads();
+    }
+    if (MODE == R3_PRED && a.th.on && z == 0 && y0 == 0 && threadIdx.x == 0) *a.draw += 1ull;
+    const uint64_t draw = (MODE == R3_CORR && a.th.on) ? (uint64_t)*a.draw : 0ull;
+
+    constexpr int SITES = TY * FX;
+#pragma unroll 2
+    for (int li = threadIdx.x; li < SITES; li += R3_TPB) {
+        const int r = li / FX, x = li & (FX - 1);
+        if (r < ty) rows_site<MODE, COMP>(rp, exl, A, FX, ty, r, x, y0, z, draw);
+    }
+    if (!lr || MODE == R3_INV) return;
+    // the unpaired imaginary half of the last line (odd 3*ty) must be zero
+    if ((3 * ty) & 1)
+        for (int x = threadIdx.x; x < FX; x += R3_TPB)
+            reinterpret_cast<float*>(A + (size_t)(lines - 1) * FX + x)[1] = 0.0f;
+    __syncthreads();
+    fft_ip<FX, LINES, R3_TPB, -1>(A, s_tw);
+    for (int idx = threadIdx.x; idx < lines * FXH; idx += R3_TPB) {
+        const int l = idx / FXH, k = idx - l * FXH;
+        rows_store_pair(cv, nz, z, y0, ty, l, k, A[l * FX + k], A[l * FX + ((FX - k) & (FX - 1))]);
     }
 }
 
@@ -350,22 +466,50 @@ static const void* zconv_template(int fz)
 }
 
 // --------------------------------------------------------------- launches
-hipError_t launch3_rows(int mode, bool comp, const Rep3* reps, const int4* items, int nitems, size_t lds, hipStream_t st)
+int rows_template_height(int fx)
+{
+    switch (fx) {
+    case 32: return rows_template_ty<32>();
+    case 64: return rows_template_ty<64>();
+    case 128: return rows_template_ty<128>();
+    case 256: return rows_template_ty<256>();
+    case 512: return rows_template_ty<512>();
+    default: return 0;
+    }
+}
+
+template <int MODE, bool COMP>
+static const void* rows_kernel(int rt)
+{
+    switch (rt) {
+    case 0: return (const void*)k3_rows<MODE, COMP>;
+    case 32: return (const void*)k3_rows_t<MODE, COMP, 32>;
+    case 64: return (const void*)k3_rows_t<MODE, COMP, 64>;
+    case 128: return (const void*)k3_rows_t<MODE, COMP, 128>;
+    case 256: return (const void*)k3_rows_t<MODE, COMP, 256>;
+    case 512: return (const void*)k3_rows_t<MODE, COMP, 512>;
+    default: return nullptr;
+    }
+}
+
+static const void* rows_kernel(int mode, bool comp, int rt)
+{
+    switch (mode) {
+    case R3_FWD: return rows_kernel<R3_FWD, false>(rt);
+    case R3_INV: return rows_kernel<R3_INV, false>(rt);
+    case R3_PRED: return comp ? rows_kernel<R3_PRED, true>(rt) : rows_kernel<R3_PRED, false>(rt);
+    default: return comp ? rows_kernel<R3_CORR, true>(rt) : rows_kernel<R3_CORR, false>(rt);
+    }
+}
+
+hipError_t launch3_rows(int mode, bool comp, const Rep3* reps, const int4* items, int nitems, size_t lds, int rt,
+                        hipStream_t st)
 {
     if (nitems <= 0) return hipSuccess;
-    switch (mode) {
-        case R3_FWD: k3_rows<R3_FWD, false><<<nitems, R3_TPB, lds, st>>>(reps, items); break;
-        case R3_INV: k3_rows<R3_INV, false><<<nitems, R3_TPB, lds, st>>>(reps, items); break;
-        case R3_PRED:
-            if (comp) k3_rows<R3_PRED, true><<<nitems, R3_TPB, lds, st>>>(reps, items);
-            else k3_rows<R3_PRED, false><<<nitems, R3_TPB, lds, st>>>(reps, items);
-            break;
-        default:
-            if (comp) k3_rows<R3_CORR, true><<<nitems, R3_TPB, lds, st>>>(reps, items);
-            else k3_rows<R3_CORR, false><<<nitems, R3_TPB, lds, st>>>(reps, items);
-            break;
-    }
-    return hipGetLastError();
+    const void* f = rows_kernel(mode, comp, rt);
+    if (!f) return hipErrorInvalidValue;
+    void* args[] = {(void*)&reps, (void*)&items};
+    return hipLaunchKernel(f, dim3(nitems), dim3(R3_TPB), args, lds, st);
 }
 
 hipError_t launch3_ypass(int sign, const Rep3* reps, const int4* items, int nitems, size_t lds, int yt, hipStream_t st)
@@ -396,15 +540,18 @@ hipError_t launch3_zconv(const Rep3* reps, const int4* items, int nitems, size_t
 hipError_t configure_lds3(size_t max_bytes)
 {
     const int v = (int)max_bytes;
-    const void* fns[] = {(const void*)k3_rows<R3_FWD, false>, (const void*)k3_rows<R3_INV, false>,
-                         (const void*)k3_rows<R3_PRED, false>, (const void*)k3_rows<R3_PRED, true>,
-                         (const void*)k3_rows<R3_CORR, false>, (const void*)k3_rows<R3_CORR, true>,
-                         (const void*)k3_ypass<-1>, (const void*)k3_ypass<1>, (const void*)k3_zconv};
+    const void* fns[] = {(const void*)k3_ypass<-1>, (const void*)k3_ypass<1>, (const void*)k3_zconv};
     hipError_t first = hipSuccess;
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, v);
         if (e != hipSuccess && first == hipSuccess) first = e;
     }
+    for (int rt : {0, 32, 64, 128, 256, 512})
+        for (int mode : {R3_FWD, R3_INV, R3_PRED, R3_CORR})
+            for (bool comp : {false, true}) {
+                hipError_t e = hipFuncSetAttribute(rows_kernel(mode, comp, rt), hipFuncAttributeMaxDynamicSharedMemorySize, v);
+                if (e != hipSuccess && first == hipSuccess) first = e;
+            }
     for (int f : {32, 64, 128, 256, 512})
         for (const void* k : {zconv_template(f), ypass_template<-1>(f), ypass_template<1>(f)}) {
             hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, v);

@@ -1228,7 +1228,7 @@ struct msd_longrange3d {
     float2 *twx = nullptr, *twy = nullptr, *twz = nullptr;
     double2 *twx64 = nullptr, *twy64 = nullptr, *twz64 = nullptr;
     FFTPlan px{}, py{}, pz{};
-    int ty = 1, wy = 1, wz = 1, zt = 0, yt = 0;
+    int ty = 1, wy = 1, wz = 1, zt = 0, yt = 0, rt = 0;
     size_t lds_rows = 0, lds_y = 0, lds_z = 0;
     // single-replica descriptor and work items for the operator API
     Rep3* orep = nullptr;
@@ -1245,7 +1245,7 @@ struct msd_longrange3d {
     {
         Conv3Dev c{};
         c.fx = fx; c.fy = fy; c.fz = fz; c.fxh = fxh;
-        c.ty = ty; c.wy = wy; c.wz = wz; c.zt = zt; c.yt = yt;
+        c.ty = ty; c.wy = wy; c.wz = wz; c.zt = zt; c.yt = yt; c.rt = rt;
         c.px = px; c.py = py; c.pz = pz;
         c.twx = twx; c.twy = twy; c.twz = twz;
         c.W = W; c.Tw = Tw;
@@ -1286,6 +1286,12 @@ extern "C" int msd_longrange3d_create(int nx, int ny, int nz, int open, msd_long
     l->lds_z = 2 * (size_t)3 * l->wz * l->fz * sizeof(float2);
     // the compile-time y and z passes (in-place transforms, wider kx blocks) where
     // they have instances
+    if (const int th = rows_template_height(l->fx)) {
+        // the compile-time row pass transforms in place: one buffer of its lines
+        // and the x twiddles
+        const size_t lr = (size_t)((3 * th + 1) / 2 + 1) * l->fx * sizeof(float2);
+        if (th <= ny && lr <= G.max_lds) { l->rt = l->fx; l->ty = th; l->lds_rows = lr; }
+    }
     if (const int wt = ypass_template_width(l->fy)) {
         const size_t ly = (size_t)wt * l->fy * sizeof(float2);
         if (ly <= G.max_lds) { l->yt = l->fy; l->wy = wt; l->lds_y = ly; }
@@ -1455,11 +1461,11 @@ extern "C" int msd_longrange3d_apply(msd_longrange3d* l, msd_system* s)
         if (int rc = dalloc(&l->orep, 1)) return rc;
     }
     HIPCHK(hipMemcpyAsync(l->orep, &r, sizeof r, hipMemcpyHostToDevice, G.stream));
-    hipError_t e = launch3_rows(R3_FWD, false, l->orep, l->oitems_r, l->onr, l->lds_rows, G.stream);
+    hipError_t e = launch3_rows(R3_FWD, false, l->orep, l->oitems_r, l->onr, l->lds_rows, l->rt, G.stream);
     if (e == hipSuccess) e = launch3_ypass(-1, l->orep, l->oitems_y, l->ony, l->lds_y, l->yt, G.stream);
     if (e == hipSuccess) e = launch3_zconv(l->orep, l->oitems_z, l->onz, l->lds_z, l->zt, G.stream);
     if (e == hipSuccess) e = launch3_ypass(1, l->orep, l->oitems_y, l->ony, l->lds_y, l->yt, G.stream);
-    if (e == hipSuccess) e = launch3_rows(R3_INV, false, l->orep, l->oitems_r, l->onr, l->lds_rows, G.stream);
+    if (e == hipSuccess) e = launch3_rows(R3_INV, false, l->orep, l->oitems_r, l->onr, l->lds_rows, l->rt, G.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(G.stream);   // the descriptor lives on the host stack
     if (e != hipSuccess) return fail(MSD_EDEVICE, "LongRange3D:apply: %s", hipGetErrorString(e));
     return MSD_OK;
@@ -1468,7 +1474,7 @@ extern "C" int msd_longrange3d_apply(msd_longrange3d* l, msd_system* s)
 // ------------------------------------------------------------ batch of replicas
 // Device descriptors and work tables of replicas stepped by the same launches.
 struct Batch3 {
-    // the y and z passes run once per pass form among the members (the
+    // the row, y and z passes run once per pass form among the members (the
     // compile-time pass for each length that has one, the runtime pass for the
     // rest), so every member is transformed by the same kernels as when it runs alone
     struct ZGroup {
@@ -1478,22 +1484,21 @@ struct Batch3 {
         size_t lds = 0;
     };
     Rep3* reps = nullptr;
-    int4* rows = nullptr;
-    int nrep = 0, nrows = 0;
-    size_t lds_rows = 0;
-    std::vector<ZGroup> yg, zg;          // y-pass and z-pass groups (key: Conv3Dev::yt / zt)
+    int nrep = 0;
+    std::vector<ZGroup> rg, yg, zg;      // row, y-pass and z-pass groups (key: Conv3Dev::rt / yt / zt)
     bool any_lr = false;
     std::vector<Rep3> host;
     ~Batch3() { release(); }
     void release()
     {
         (void)hipStreamSynchronize(G.stream);
-        dfree(reps); dfree(rows);
+        dfree(reps);
+        for (ZGroup& z : rg) dfree(z.items);
         for (ZGroup& z : yg) dfree(z.items);
         for (ZGroup& z : zg) dfree(z.items);
-        yg.clear(); zg.clear();
+        rg.clear(); yg.clear(); zg.clear();
         host.clear();
-        nrep = nrows = 0;
+        nrep = 0;
     }
     // rebuild the tables when the member set changed; refresh descriptors always
     int update(const std::vector<Rep3>& r, const std::vector<int>& tys, const std::vector<size_t>& lds3)
@@ -1501,11 +1506,9 @@ struct Batch3 {
         const bool same_shape = host.size() == r.size() && reps != nullptr;
         if (!same_shape) {
             release();
-            std::vector<int4> R;
-            std::vector<std::vector<int4>> Y, Z;
-            std::vector<ZGroup> ygroups, zgroups;
+            std::vector<std::vector<int4>> R, Y, Z;
+            std::vector<ZGroup> rgroups, ygroups, zgroups;
             any_lr = false;
-            lds_rows = 0;
             // append a member's pass items to the group of its pass form
             auto add = [](std::vector<ZGroup>& groups, std::vector<std::vector<int4>>& lists, int key, size_t lds,
                           const std::vector<int4>& items) {
@@ -1521,14 +1524,17 @@ struct Batch3 {
                 lists[gi].insert(lists[gi].end(), items.begin(), items.end());
             };
             for (size_t i = 0; i < r.size(); ++i) {
-                std::vector<int4> yi, zi;
-                items3((int)i, r[i].a.d, r[i].has_lr ? &r[i].cv : nullptr, tys[i], R, yi, zi);
+                std::vector<int4> ri, yi, zi;
+                items3((int)i, r[i].a.d, r[i].has_lr ? &r[i].cv : nullptr, tys[i], ri, yi, zi);
                 any_lr |= r[i].has_lr != 0;
-                lds_rows = std::max(lds_rows, lds3[3 * i]);
+                add(rgroups, R, r[i].has_lr ? r[i].cv.rt : 0, lds3[3 * i], ri);
                 add(ygroups, Y, r[i].cv.yt, lds3[3 * i + 1], yi);
                 add(zgroups, Z, r[i].cv.zt, lds3[3 * i + 2], zi);
             }
-            if (int rc = upload_vec(&rows, R)) return rc;
+            for (size_t gi = 0; gi < rgroups.size(); ++gi) {
+                if (int rc = upload_vec(&rgroups[gi].items, R[gi])) return rc;
+                rgroups[gi].n = (int)R[gi].size();
+            }
             for (size_t gi = 0; gi < ygroups.size(); ++gi) {
                 if (int rc = upload_vec(&ygroups[gi].items, Y[gi])) return rc;
                 ygroups[gi].n = (int)Y[gi].size();
@@ -1537,9 +1543,9 @@ struct Batch3 {
                 if (int rc = upload_vec(&zgroups[gi].items, Z[gi])) return rc;
                 zgroups[gi].n = (int)Z[gi].size();
             }
+            rg = rgroups;
             yg = ygroups;
             zg = zgroups;
-            nrows = (int)R.size();
             if (int rc = dalloc(&reps, r.size())) return rc;
             nrep = (int)r.size();
         }
@@ -1564,6 +1570,13 @@ struct Batch3 {
             if (e == hipSuccess) e = launch3_ypass(sign, reps, y.items, y.n, y.lds, y.zt, st);
         return e;
     }
+    hipError_t rows(int row_mode, bool comp, hipStream_t st) const
+    {
+        hipError_t e = hipSuccess;
+        for (const ZGroup& g : rg)
+            if (e == hipSuccess) e = launch3_rows(row_mode, comp, reps, g.items, g.n, g.lds, g.zt, st);
+        return e;
+    }
     hipError_t evaluate(int row_mode, bool comp, hipStream_t st) const
     {
         hipError_t e = hipSuccess;
@@ -1572,7 +1585,7 @@ struct Batch3 {
             if (e == hipSuccess) e = zpass(st);
             if (e == hipSuccess) e = ypass(1, st);
         }
-        if (e == hipSuccess) e = launch3_rows(row_mode, comp, reps, rows, nrows, lds_rows, st);
+        if (e == hipSuccess) e = rows(row_mode, comp, st);
         return e;
     }
 };
@@ -2170,7 +2183,7 @@ static int stepper_begin(msd_stepper* st)
         hipError_t e = hipSuccess;
         if (st->kind() == SK_LAYERED) e = launch_rows_fwd(st->args.dp, st->s->S, G.stream);
         else if (st->kind() == SK_CONV3)
-            e = launch3_rows(R3_FWD, false, st->b3.reps, st->b3.rows, st->b3.nrows, st->b3.lds_rows, G.stream);
+            e = st->b3.rows(R3_FWD, false, G.stream);
         if (e != hipSuccess) return fail(MSD_EDEVICE, "stepper: %s", hipGetErrorString(e));
     }
     return MSD_OK;
@@ -2421,8 +2434,8 @@ static hipError_t launch_code(const msd_stepper* st, int code, hipStream_t s)
         case L_Y_FWD: return b.ypass(-1, s);
         case L_ZCONV: return b.zpass(s);
         case L_Y_INV: return b.ypass(1, s);
-        case L_PRED3: return launch3_rows(R3_PRED, st->compensated, b.reps, b.rows, b.nrows, b.lds_rows, s);
-        default: return launch3_rows(R3_CORR, st->compensated, b.reps, b.rows, b.nrows, b.lds_rows, s);
+        case L_PRED3: return b.rows(R3_PRED, st->compensated, s);
+        default: return b.rows(R3_CORR, st->compensated, s);
     }
 }
 
@@ -2602,7 +2615,7 @@ extern "C" int msd_ensemble_heun(msd_ensemble* e, int nsteps)
         stale |= s->lr3 && spectrum_stale(s);
     }
     if (stale) {
-        hipError_t err = launch3_rows(R3_FWD, false, e->b.reps, e->b.rows, e->b.nrows, e->b.lds_rows, G.stream);
+        hipError_t err = e->b.rows(R3_FWD, false, G.stream);
         if (err != hipSuccess) return fail(MSD_EDEVICE, "ensemble: %s", hipGetErrorString(err));
     }
     auto one_step = [&](hipStream_t st) -> hipError_t {

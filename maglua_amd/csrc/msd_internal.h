@@ -155,6 +155,7 @@ struct Conv3Dev {
     int wy, wz;             // kx block widths of the y and z passes
     int zt;                 // z length of the compile-time z pass (0: the runtime-length pass)
     int yt;                 // y length of the compile-time y pass (0: the runtime-length pass)
+    int rt;                 // x length of the compile-time row pass (0: the runtime-length pass)
     FFTPlan px, py, pz;
     const float2* twx;
     const float2* twy;
@@ -176,7 +177,9 @@ struct Rep3 {
 enum Row3Mode { R3_FWD = 0, R3_INV = 1, R3_PRED = 2, R3_CORR = 3 };
 // work items: int4 (replica, p, q, unused); rows: p = z, q = y0; y pass: p = c*nz+z,
 // q = kx0; z pass: p = ky, q = kx0
-hipError_t launch3_rows(int mode, bool comp, const Rep3* reps, const int4* items, int nitems, size_t lds, hipStream_t st);
+hipError_t launch3_rows(int mode, bool comp, const Rep3* reps, const int4* items, int nitems, size_t lds, int rt,
+                        hipStream_t st);
+int rows_template_height(int fx);   // tile height of the compile-time row pass of length fx, 0 if none
 // yt: every replica's compile-time y length (Conv3Dev::yt), or 0 for the runtime-length pass
 hipError_t launch3_ypass(int sign, const Rep3* reps, const int4* items, int nitems, size_t lds, int yt, hipStream_t st);
 // kx width of the compile-time y pass for a y length (0 when it has no instance)
