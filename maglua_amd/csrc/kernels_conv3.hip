@@ -32,6 +32,32 @@ __device__ __forceinline__ size_t widx(const Conv3Dev& cv, int nz, int c, int z,
     return (((size_t)c * nz + z) * cv.fy + y) * cv.fxh + k;
 }
 
+// The six tensor words (xx xy xz yy yz zz) of spectrum point (kx, ky, kz), stored
+// [6][fy][fxh][fzs] with kz fastest.  With fzs < fz only kz <= fz/2 is stored and
+// kz beyond reads fz - kz: the tensor is even in z except xz and yz, which are odd.
+struct TensorWords {
+    float xx, xy, xz, yy, yz, zz;
+};
+__device__ __forceinline__ TensorWords tensor_words(const Conv3Dev& cv, int ky, int kx, int kz)
+{
+    const bool mir = kz >= cv.fzs;
+    const size_t plane = (size_t)cv.fy * cv.fxh * cv.fzs;
+    const size_t t = ((size_t)ky * cv.fxh + kx) * cv.fzs + (mir ? cv.fz - kz : kz);
+    TensorWords w{cv.Tw[t], cv.Tw[plane + t], cv.Tw[2 * plane + t], cv.Tw[3 * plane + t], cv.Tw[4 * plane + t],
+                  cv.Tw[5 * plane + t]};
+    if (mir) { w.xz = -w.xz; w.yz = -w.yz; }
+    return w;
+}
+
+// Y = T X at one spectrum point for the three components held at px, py, pz
+__device__ __forceinline__ void tensor_apply(const TensorWords& w, float2& px, float2& py, float2& pz)
+{
+    const float2 mx = px, my = py, mz = pz;
+    px = make_float2(w.xx * mx.x + w.xy * my.x + w.xz * mz.x, w.xx * mx.y + w.xy * my.y + w.xz * mz.y);
+    py = make_float2(w.xy * mx.x + w.yy * my.x + w.yz * mz.x, w.xy * mx.y + w.yy * my.y + w.yz * mz.y);
+    pz = make_float2(w.xz * mx.x + w.yz * my.x + w.zz * mz.x, w.xz * mx.y + w.yz * my.y + w.zz * mz.y);
+}
+
 // ------------------------------------------------------------------- rows
 // A row tile is ty rows y0.. of plane z.  Its 3*ty real rows (component-major,
 // q = c*ty + r) are packed two to a complex line of length fx: line l holds rows
@@ -295,7 +321,8 @@ __global__ void __launch_bounds__(Y3_TPB) k3_ypass_t(const Rep3* __restrict__ re
     const int fxh = cv.fxh;
     const int kx0 = it.z;
     const int w = min(W, fxh - kx0);
-    float2* A = lds;                                     // [j][FY]
+    constexpr int LS = FY + CONV3_LPAD;
+    float2* A = lds;                                     // [j][LS]
     float2* base = cv.W + (size_t)it.y * FY * fxh + kx0;
     const int yin = SIGN < 0 ? ny : FY;
     batched_copy<W * FY, Y3_TPB, 8>(
@@ -305,14 +332,14 @@ __global__ void __launch_bounds__(Y3_TPB) k3_ypass_t(const Rep3* __restrict__ re
         },
         [&](int idx, float2 v) {
             const int y = idx / W, j = idx - y * W;
-            A[j * FY + y] = v;
+            A[j * LS + y] = v;
         });
     __syncthreads();
-    fft_ip<FY, W, Y3_TPB, SIGN>(A, cv.twy);
+    fft_ip<FY, W, Y3_TPB, SIGN, LS>(A, cv.twy);
     const int yout = SIGN < 0 ? FY : ny;
     for (int idx = threadIdx.x; idx < W * yout; idx += Y3_TPB) {
         const int y = idx / W, j = idx - y * W;
-        if (j < w) base[(size_t)y * fxh + j] = A[j * FY + y];
+        if (j < w) base[(size_t)y * fxh + j] = A[j * LS + y];
     }
 }
 
@@ -356,7 +383,7 @@ __global__ void __launch_bounds__(Z3_TPB) k3_zconv(const Rep3* __restrict__ reps
     const Rep3& rp = reps[it.x];
     const Conv3Dev& cv = rp.cv;
     const int nz = rp.a.d.nz;
-    const int fy = cv.fy, fz = cv.fz, fxh = cv.fxh;
+    const int fz = cv.fz, fxh = cv.fxh;
     const int ky = it.y, kx0 = it.z;
     const int w = min(cv.wz, fxh - kx0);
     const int lines = 3 * w;
@@ -369,18 +396,11 @@ __global__ void __launch_bounds__(Z3_TPB) k3_zconv(const Rep3* __restrict__ reps
     }
     __syncthreads();
     float2* R = lds_fft<-1>(A, B, lines, cv.pz, cv.twz);
-    const size_t plane = (size_t)fz * fy * fxh;
+    // kz fastest: the tensor words and the LDS lines are both read in runs
     for (int idx = threadIdx.x; idx < w * fz; idx += Z3_TPB) {
-        const int kz = idx / w, j = idx - kz * w;
-        const size_t t = ((size_t)kz * fy + ky) * fxh + kx0 + j;
-        const float xx = cv.Tw[t], xy = cv.Tw[plane + t], xz = cv.Tw[2 * plane + t];
-        const float yy = cv.Tw[3 * plane + t], yz = cv.Tw[4 * plane + t], zz = cv.Tw[5 * plane + t];
-        const float2 mx = R[(size_t)(0 * w + j) * fz + kz];
-        const float2 my = R[(size_t)(1 * w + j) * fz + kz];
-        const float2 mz = R[(size_t)(2 * w + j) * fz + kz];
-        R[(size_t)(0 * w + j) * fz + kz] = make_float2(xx * mx.x + xy * my.x + xz * mz.x, xx * mx.y + xy * my.y + xz * mz.y);
-        R[(size_t)(1 * w + j) * fz + kz] = make_float2(xy * mx.x + yy * my.x + yz * mz.x, xy * mx.y + yy * my.y + yz * mz.y);
-        R[(size_t)(2 * w + j) * fz + kz] = make_float2(xz * mx.x + yz * my.x + zz * mz.x, xz * mx.y + yz * my.y + zz * mz.y);
+        const int j = idx / fz, kz = idx - j * fz;
+        tensor_apply(tensor_words(cv, ky, kx0 + j, kz), R[(size_t)(0 * w + j) * fz + kz], R[(size_t)(1 * w + j) * fz + kz],
+                     R[(size_t)(2 * w + j) * fz + kz]);
     }
     __syncthreads();
     const float2* O = lds_fft<1>(R, R == A ? B : A, lines, cv.pz, cv.twz);
@@ -399,15 +419,14 @@ template <int FZ, int W>
 __global__ void __launch_bounds__(Z3_TPB) k3_zconv_t(const Rep3* __restrict__ reps, const int4* __restrict__ items)
 {
     extern __shared__ float2 lds[];
-    constexpr int lines = 3 * W;
+    constexpr int lines = 3 * W, LS = FZ + CONV3_LPAD;
     const int4 it = items[blockIdx.x];
     const Rep3& rp = reps[it.x];
     const Conv3Dev& cv = rp.cv;
     const int nz = rp.a.d.nz;
-    const int fy = cv.fy, fxh = cv.fxh;
     const int ky = it.y, kx0 = it.z;
-    const int w = min(W, fxh - kx0);
-    float2* A = lds;                                     // [c W + j][FZ]
+    const int w = min(W, cv.fxh - kx0);
+    float2* A = lds;                                     // [c W + j][LS]
     // element idx = zz lines + l: consecutive threads walk the kx of one plane
     batched_copy<lines * FZ, Z3_TPB, 8>(
         [&](int idx) {
@@ -417,30 +436,23 @@ __global__ void __launch_bounds__(Z3_TPB) k3_zconv_t(const Rep3* __restrict__ re
         },
         [&](int idx, float2 v) {
             const int zz = idx / lines, l = idx - zz * lines;
-            A[l * FZ + zz] = v;
+            A[l * LS + zz] = v;
         });
     __syncthreads();
-    fft_ip<FZ, lines, Z3_TPB, -1>(A, cv.twz);
-    const size_t plane = (size_t)FZ * fy * fxh;
+    fft_ip<FZ, lines, Z3_TPB, -1, LS>(A, cv.twz);
+    // kz fastest: the tensor words and the LDS lines are both read in runs
     for (int idx = threadIdx.x; idx < W * FZ; idx += Z3_TPB) {
-        const int kz = idx / W, j = idx - kz * W;
-        if (j >= w) continue;
-        const size_t t = ((size_t)kz * fy + ky) * fxh + kx0 + j;
-        const float xx = cv.Tw[t], xy = cv.Tw[plane + t], xz = cv.Tw[2 * plane + t];
-        const float yy = cv.Tw[3 * plane + t], yz = cv.Tw[4 * plane + t], zz = cv.Tw[5 * plane + t];
-        const float2 mx = A[(0 * W + j) * FZ + kz];
-        const float2 my = A[(1 * W + j) * FZ + kz];
-        const float2 mz = A[(2 * W + j) * FZ + kz];
-        A[(0 * W + j) * FZ + kz] = make_float2(xx * mx.x + xy * my.x + xz * mz.x, xx * mx.y + xy * my.y + xz * mz.y);
-        A[(1 * W + j) * FZ + kz] = make_float2(xy * mx.x + yy * my.x + yz * mz.x, xy * mx.y + yy * my.y + yz * mz.y);
-        A[(2 * W + j) * FZ + kz] = make_float2(xz * mx.x + yz * my.x + zz * mz.x, xz * mx.y + yz * my.y + zz * mz.y);
+        const int j = idx / FZ, kz = idx & (FZ - 1);
+        if (j < w)
+            tensor_apply(tensor_words(cv, ky, kx0 + j, kz), A[(0 * W + j) * LS + kz], A[(1 * W + j) * LS + kz],
+                         A[(2 * W + j) * LS + kz]);
     }
     __syncthreads();
-    fft_ip<FZ, lines, Z3_TPB, 1>(A, cv.twz);
+    fft_ip<FZ, lines, Z3_TPB, 1, LS>(A, cv.twz);
     for (int idx = threadIdx.x; idx < lines * nz; idx += Z3_TPB) {
         const int zz = idx / lines, l = idx - zz * lines;
         const int c = l / W, j = l - c * W;
-        if (j < w) cv.W[widx(cv, nz, c, zz, ky, kx0 + j)] = A[l * FZ + zz];
+        if (j < w) cv.W[widx(cv, nz, c, zz, ky, kx0 + j)] = A[l * LS + zz];
     }
 }
 
@@ -760,10 +772,21 @@ __global__ void k3_fft64(Lines64 m, FFTPlan p, const double2* __restrict__ tw, c
     }
 }
 
-__global__ void k3_words(const double2* __restrict__ V, float* __restrict__ Tw, double mult, size_t n)
+// FP32 tensor words [6][fy][fxh][fzs] (kz fastest, kz < fzs) from the FP64
+// spectrum V [6][fz][fy][fxh]
+__global__ void k3_words(const double2* __restrict__ V, float* __restrict__ Tw, double mult, int fy, int fz, int fxh,
+                         int fzs)
 {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        Tw[i] = (float)(V[i].x * mult);
+    const size_t n = (size_t)6 * fy * fxh * fzs;
+    for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < n; o += (size_t)gridDim.x * blockDim.x) {
+        const int kz = (int)(o % fzs);
+        const size_t r = o / fzs;
+        const int kx = (int)(r % fxh);
+        const size_t r2 = r / fxh;
+        const int ky = (int)(r2 % fy);
+        const int a = (int)(r2 / fy);
+        Tw[o] = (float)(V[(((size_t)a * fz + kz) * fy + ky) * fxh + kx].x * mult);
+    }
 }
 
 // One axis of the FP64 tensor transform as LDS Stockham passes; lines per
@@ -787,8 +810,8 @@ hipError_t configure_fft64(size_t bytes)
     return hipFuncSetAttribute((const void*)k3_fft64, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-hipError_t launch3_spectrum(int fx, int fy, int fz, const double* T, double2* U1, double2* U2, float* Tw, double mult,
-                            const double2* twx, const double2* twy, const double2* twz, hipStream_t st)
+hipError_t launch3_spectrum(int fx, int fy, int fz, int fzs, const double* T, double2* U1, double2* U2, float* Tw,
+                            double mult, const double2* twx, const double2* twy, const double2* twz, hipStream_t st)
 {
     const int fxh = fx / 2 + 1;
     const size_t tot = (size_t)6 * fz * fy * fxh;
@@ -813,7 +836,7 @@ hipError_t launch3_spectrum(int fx, int fy, int fz, const double* T, double2* U1
         k3_dftyz<1><<<g, 256, 0, st>>>(fy, fz, fxh, U1, U2, twy);
         k3_dftyz<2><<<g, 256, 0, st>>>(fy, fz, fxh, U2, U1, twz);
     }
-    k3_words<<<g, 256, 0, st>>>(U1, Tw, mult, tot);
+    k3_words<<<g, 256, 0, st>>>(U1, Tw, mult, fy, fz, fxh, fzs);
     return hipGetLastError();
 }
 

@@ -1223,12 +1223,12 @@ struct msd_longrange3d {
     bool loaded = false, custom = false, spectrum = false;
     double words_mult = NAN;
     double* T64 = nullptr;   // [6][fz][fy][fx]
-    float* Tw = nullptr;     // [6][fz][fy][fxh]
+    float* Tw = nullptr;     // [6][fy][fxh][fzs], fzs <= fz
     float2* W = nullptr;     // working spectrum [3][nz][fy][fxh]
     float2 *twx = nullptr, *twy = nullptr, *twz = nullptr;
     double2 *twx64 = nullptr, *twy64 = nullptr, *twz64 = nullptr;
     FFTPlan px{}, py{}, pz{};
-    int ty = 1, wy = 1, wz = 1, zt = 0, yt = 0, rt = 0;
+    int ty = 1, wy = 1, wz = 1, zt = 0, yt = 0, rt = 0, fzs = 0;
     size_t lds_rows = 0, lds_y = 0, lds_z = 0;
     // single-replica descriptor and work items for the operator API
     Rep3* orep = nullptr;
@@ -1245,7 +1245,7 @@ struct msd_longrange3d {
     {
         Conv3Dev c{};
         c.fx = fx; c.fy = fy; c.fz = fz; c.fxh = fxh;
-        c.ty = ty; c.wy = wy; c.wz = wz; c.zt = zt; c.yt = yt; c.rt = rt;
+        c.ty = ty; c.wy = wy; c.wz = wz; c.zt = zt; c.yt = yt; c.rt = rt; c.fzs = fzs;
         c.px = px; c.py = py; c.pz = pz;
         c.twx = twx; c.twy = twy; c.twz = twz;
         c.W = W; c.Tw = Tw;
@@ -1293,11 +1293,11 @@ extern "C" int msd_longrange3d_create(int nx, int ny, int nz, int open, msd_long
         if (th <= ny && lr <= G.max_lds) { l->rt = l->fx; l->ty = th; l->lds_rows = lr; }
     }
     if (const int wt = ypass_template_width(l->fy)) {
-        const size_t ly = (size_t)wt * l->fy * sizeof(float2);
+        const size_t ly = (size_t)wt * (l->fy + CONV3_LPAD) * sizeof(float2);
         if (ly <= G.max_lds) { l->yt = l->fy; l->wy = wt; l->lds_y = ly; }
     }
     if (const int wt = zconv_template_width(l->fz)) {
-        const size_t lz = (size_t)3 * wt * l->fz * sizeof(float2);
+        const size_t lz = (size_t)3 * wt * (l->fz + CONV3_LPAD) * sizeof(float2);
         if (lz <= G.max_lds) { l->zt = l->fz; l->wz = wt; l->lds_z = lz; }
     }
     if (l->lds_rows > G.max_lds || l->lds_y > G.max_lds || l->lds_z > G.max_lds)
@@ -1391,12 +1391,40 @@ static int lr3_check_even(msd_longrange3d* l)
     return MSD_OK;
 }
 
+// Whether every component has its parity in z alone, T(x, y, -z) = +T (xx xy yy zz)
+// or -T (xz yz): then the spectrum has the same parity in kz and only kz <= fz/2
+// is stored.  The built-in tensors have it (reflection symmetry of the lattice sums).
+static int lr3_z_parity(msd_longrange3d* l, bool* has)
+{
+    const size_t plane = (size_t)l->fx * l->fy * l->fz;
+    std::vector<double> t(6 * plane);
+    HIPCHK(hipMemcpyAsync(t.data(), l->T64, sizeof(double) * t.size(), hipMemcpyDeviceToHost, G.stream));
+    HIPCHK(hipStreamSynchronize(G.stream));
+    double big = 0;
+    for (double v : t) big = std::max(big, std::fabs(v));
+    *has = true;
+    for (int a = 0; a < 6 && *has; ++a) {
+        const double sg = (a == 2 || a == 4) ? -1.0 : 1.0;
+        for (int z = 0; z < l->fz && *has; ++z)
+            for (size_t xy = 0; xy < (size_t)l->fx * l->fy; ++xy) {
+                const double u = t[a * plane + (size_t)z * l->fx * l->fy + xy];
+                const double v = t[a * plane + (size_t)((l->fz - z) % l->fz) * l->fx * l->fy + xy];
+                if (std::fabs(u - sg * v) > 1e-12 * big) { *has = false; break; }
+            }
+    }
+    return MSD_OK;
+}
+
 int msd_longrange3d::prepare()
 {
     if (int rc = lr3_load(this)) return rc;
     if (!spectrum) {
-        if (custom)
+        bool zpar = true;
+        if (custom) {
             if (int rc = lr3_check_even(this)) return rc;
+            if (int rc = lr3_z_parity(this, &zpar)) return rc;
+        }
+        fzs = zpar ? fz / 2 + 1 : fz;
         words_mult = NAN;
         spectrum = true;
     }
@@ -1406,7 +1434,7 @@ int msd_longrange3d::prepare()
         double2 *U1 = nullptr, *U2 = nullptr;
         if (int rc = dalloc(&U1, wn)) return rc;
         if (int rc = dalloc(&U2, wn)) { dfree(U1); return rc; }
-        hipError_t e = launch3_spectrum(fx, fy, fz, T64, U1, U2, Tw, mult, twx64, twy64, twz64, G.stream);
+        hipError_t e = launch3_spectrum(fx, fy, fz, fzs, T64, U1, U2, Tw, mult, twx64, twy64, twz64, G.stream);
         if (e == hipSuccess) e = hipStreamSynchronize(G.stream);
         dfree(U1); dfree(U2);
         if (e != hipSuccess) return fail(MSD_EDEVICE, "LongRange3D spectrum: %s", hipGetErrorString(e));

@@ -249,7 +249,7 @@ __device__ __forceinline__ int lds_swz(int a)
     else return a;
 }
 
-template <int L, int COUNT, int NT, int SIGN, int NS, int R, bool SWIN, bool SWOUT>
+template <int L, int COUNT, int NT, int SIGN, int NS, int R, bool SWIN, bool SWOUT, int LS>
 __device__ __forceinline__ void stage_ip(float2* __restrict__ buf, const float2* __restrict__ tw)
 {
     constexpr int nb = L / R;
@@ -263,7 +263,7 @@ __device__ __forceinline__ void stage_ip(float2* __restrict__ buf, const float2*
         if (total % NT == 0 || b < total) {
             const int seq = b / nb;
             const int j = b - seq * nb;
-            const float2* s = buf + seq * L;
+            const float2* s = buf + seq * LS;
 #pragma unroll
             for (int r = 0; r < R; ++r) v[it][r] = s[SWIN ? lds_swz<L>(j + r * nb) : j + r * nb];
         }
@@ -288,7 +288,7 @@ __device__ __forceinline__ void stage_ip(float2* __restrict__ buf, const float2*
             else if (R == 4) dft4<SIGN>(v[it]);
             else if (R == 5) dft5<SIGN>(v[it]);
             else dft8<SIGN>(v[it]);
-            float2* d = buf + seq * L;
+            float2* d = buf + seq * LS;
             const int o = (j / NS) * NS * R + k;
 #pragma unroll
             for (int q = 0; q < R; ++q) d[SWOUT ? lds_swz<L>(o + q * NS) : o + q * NS] = v[it][q];
@@ -297,25 +297,27 @@ __device__ __forceinline__ void stage_ip(float2* __restrict__ buf, const float2*
     __syncthreads();
 }
 
-template <int L, int COUNT, int NT, int SIGN, int NS>
+template <int L, int COUNT, int NT, int SIGN, int NS, int LS>
 struct FFTip {
     static __device__ __forceinline__ void run(float2* a, const float2* tw)
     {
         constexpr int R = next_radix(L / NS);
-        stage_ip<L, COUNT, NT, SIGN, NS, R, NS != 1, NS * R != L>(a, tw);
-        FFTip<L, COUNT, NT, SIGN, NS * R>::run(a, tw);
+        stage_ip<L, COUNT, NT, SIGN, NS, R, NS != 1, NS * R != L, LS>(a, tw);
+        FFTip<L, COUNT, NT, SIGN, NS * R, LS>::run(a, tw);
     }
 };
-template <int L, int COUNT, int NT, int SIGN>
-struct FFTip<L, COUNT, NT, SIGN, L> {
+template <int L, int COUNT, int NT, int SIGN, int LS>
+struct FFTip<L, COUNT, NT, SIGN, L, LS> {
     static __device__ __forceinline__ void run(float2*, const float2*) {}
 };
 
-// COUNT lines of length L transformed in place in LDS; caller synchronised before
-template <int L, int COUNT, int NT, int SIGN>
+// COUNT lines of length L transformed in place in LDS, line s at a + s*LS (a
+// stride LS > L staggers the lines' banks for transposing copies); caller
+// synchronised before
+template <int L, int COUNT, int NT, int SIGN, int LS = L>
 __device__ __forceinline__ void fft_ip(float2* a, const float2* twl)
 {
-    FFTip<L, COUNT, NT, SIGN, 1>::run(a, twl);
+    FFTip<L, COUNT, NT, SIGN, 1, LS>::run(a, twl);
 }
 
 // ------------------------------------------------------- staged copies

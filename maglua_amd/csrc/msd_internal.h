@@ -149,6 +149,9 @@ hipError_t launch_step_corr(const StepArgs& a, hipStream_t st);
 // FFT grid (fx, fy, fz) equals the lattice when periodic and doubles it when
 // open (zero padding).  The working spectrum W is [3][nz][fy][fx/2+1]: planes
 // z >= nz of the transform are zero and never stored.
+// LDS lines that a pass fills by transposing copies are padded by this many
+// float2, which puts neighbouring lines on different banks.
+constexpr int CONV3_LPAD = 2;
 struct Conv3Dev {
     int fx, fy, fz, fxh;
     int ty;                 // rows per row tile
@@ -156,12 +159,13 @@ struct Conv3Dev {
     int zt;                 // z length of the compile-time z pass (0: the runtime-length pass)
     int yt;                 // y length of the compile-time y pass (0: the runtime-length pass)
     int rt;                 // x length of the compile-time row pass (0: the runtime-length pass)
+    int fzs;                // kz planes of the tensor words (fz/2 + 1 when the tensor has z parity, else fz)
     FFTPlan px, py, pz;
     const float2* twx;
     const float2* twy;
     const float2* twz;
     float2* W;              // working spectrum
-    const float* Tw;        // tensor words [6][fz][fy][fxh] (real spectra), g/(fx fy fz) folded in
+    const float* Tw;        // tensor words [6][fy][fxh][fzs] (real spectra), g/(fx fy fz) folded in
     float* hout;            // op-API target slot [3][n] (row kernel mode INV)
     const float* sin;       // op-API source spins (row kernel mode FWD)
 };
@@ -191,8 +195,8 @@ int zconv_template_width(int fz);
 // tensor: periodic lattice sum (R > 0) or open kernel (R == 0) on the FFT grid, FP64
 hipError_t launch3_tensor(int nx, int ny, int nz, int fx, int fy, int fz, int R, double* T, hipStream_t st);
 // real spectrum words of T: U1/U2 complex FP64 scratch [6][fz][fy][fxh]; tw* FP64 tables
-hipError_t launch3_spectrum(int fx, int fy, int fz, const double* T, double2* U1, double2* U2, float* Tw, double mult,
-                            const double2* twx, const double2* twy, const double2* twz, hipStream_t st);
+hipError_t launch3_spectrum(int fx, int fy, int fz, int fzs, const double* T, double2* U1, double2* U2, float* Tw,
+                            double mult, const double2* twx, const double2* twy, const double2* twz, hipStream_t st);
 hipError_t configure_lds3(size_t max_bytes);
 hipError_t configure_fft64(size_t bytes);
 const char* kernel_name(int id);

@@ -75,6 +75,32 @@ def test_longrange3d_matrix_and_custom_tensor():
     gl.apply(ss)
 
 
+@pytest.mark.parametrize("dims,R", [((6, 4, 8), 7), ((4, 4, 32), 9)])
+def test_longrange3d_custom_tensor_without_z_parity(dims, R):
+    """A custom tensor even under r -> -r whose XZ word is not odd in z alone (the
+    lattice sums' is): the spectrum keeps every kz plane instead of half of them.
+    z = 32 takes the compile-time z pass."""
+    n = dims[0] * dims[1] * dims[2]
+    T = O.dipole_tensor3d(*dims, R=R)
+    gl = M.LongRange3D(*dims)
+    gl.setRadius(R)
+    gl.setStrength(0.3)
+    for off in ((1, 0, 2), (-1, 0, -2)):    # one value at r and -r
+        gl.setMatrix("XZ", off, 0.7)
+        T["XZ"][off[2] % dims[2], off[1] % dims[1], off[0] % dims[0]] = 0.7
+    ol = O.LongRange3D(*dims, threads=True)
+    ol.setDipole(T)
+    ol.setStrength(0.3)
+    s = np.random.default_rng(n + 1).standard_normal((3, n))
+    gs, os_ = M.SpinSystem(*dims), O.SpinSystem(*dims, threads=True)
+    gs.setSpins(*s)
+    os_.setSpins(*s)
+    gl.apply(gs)
+    ol.apply(os_)
+    hg, ho = flat(gs.field("LongRange3D")), flat(os_.field("LongRange3D"))
+    assert np.abs(hg - ho).max() <= 2e-5 * np.abs(ho).max()
+
+
 def cube(dims, T, seed, R=10, J=1.0, K=0.1, g=0.05, dt=0.005, alpha=0.1):
     spec = W.FilmSpec(*dims, J=J, K=K, g=0.0, pbc=(1, 1, 1), alpha=alpha, dt=dt, T=T, seed=seed)
     gpu = W.build(M, spec, with_thermal=T != 0)
