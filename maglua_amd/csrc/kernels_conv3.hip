@@ -456,6 +456,108 @@ __global__ void __launch_bounds__(Z3_TPB) k3_zconv_t(const Rep3* __restrict__ re
     }
 }
 
+// Open boundaries, fz = 2 NZ: input planes z >= NZ are zero and only output planes
+// z < NZ are kept.  With w = exp(-2 pi i / fz) the length-fz transform splits by
+// the parity of kz into two length-NZ transforms,
+//   X(2m) = DFT_NZ(x)(m),   X(2m+1) = DFT_NZ(w^z x)(m),
+//   y(z) = IDFT_NZ(T X_even)(z) + w^-z IDFT_NZ(T X_odd)(z),   z < NZ,
+// so the LDS lines are NZ long, half those of the full pass, and a CU holds twice
+// the workgroups.  The halves run one after the other; a thread keeps its share of
+// the input, the even half's result and the odd half's tensor words in registers.
+template <int NZ, int W, int NT>
+__global__ void __launch_bounds__(NT) k3_zconv_o(const Rep3* __restrict__ reps, const int4* __restrict__ items)
+{
+    extern __shared__ float2 lds[];
+    constexpr int lines = 3 * W, LS = NZ + CONV3_LPAD;
+    constexpr int TOT = lines * NZ, PER = (TOT + NT - 1) / NT;
+    constexpr int MUL = W * NZ, MPER = (MUL + NT - 1) / NT;
+    const int4 it = items[blockIdx.x];
+    const Rep3& rp = reps[it.x];
+    const Conv3Dev& cv = rp.cv;
+    const int ky = it.y, kx0 = it.z;
+    const int w = min(W, cv.fxh - kx0);
+    float2* A = lds;                                     // [c W + j][LS]
+    float2* twh = lds + lines * LS;                      // exp(-2 pi i k / NZ), k < NZ
+    float2* wz = twh + NZ;                               // w^z, z < NZ
+    for (int i = threadIdx.x; i < NZ; i += NT) {
+        twh[i] = cv.twz[2 * i];
+        wz[i] = cv.twz[i];
+    }
+    // element idx = z lines + l: consecutive threads walk the kx of one plane
+    float2 x[PER], y[PER];
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int idx = (int)threadIdx.x + p * NT;
+        const int zz = idx / lines, l = idx - zz * lines;
+        const int c = l / W, j = l - c * W;
+        x[p] = ((TOT % NT == 0 || idx < TOT) && j < w) ? cv.W[widx(cv, NZ, c, zz, ky, kx0 + j)] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int idx = (int)threadIdx.x + p * NT;
+        const int zz = idx / lines, l = idx - zz * lines;
+        if (TOT % NT == 0 || idx < TOT) A[l * LS + zz] = x[p];
+    }
+    __syncthreads();
+    fft_ip<NZ, lines, NT, -1, LS>(A, twh);
+    // even kz; the odd kz words are read in the same pass and kept
+    TensorWords odd[MPER];
+#pragma unroll
+    for (int p = 0; p < MPER; ++p) {
+        const int idx = (int)threadIdx.x + p * NT;
+        const int j = idx / NZ, m = idx & (NZ - 1);
+        if ((MUL % NT == 0 || idx < MUL) && j < w) {
+            odd[p] = tensor_words(cv, ky, kx0 + j, 2 * m + 1);
+            tensor_apply(tensor_words(cv, ky, kx0 + j, 2 * m), A[(0 * W + j) * LS + m], A[(1 * W + j) * LS + m],
+                         A[(2 * W + j) * LS + m]);
+        }
+    }
+    __syncthreads();
+    fft_ip<NZ, lines, NT, 1, LS>(A, twh);
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int idx = (int)threadIdx.x + p * NT;
+        const int zz = idx / lines, l = idx - zz * lines;
+        if (TOT % NT == 0 || idx < TOT) y[p] = A[l * LS + zz];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int idx = (int)threadIdx.x + p * NT;
+        const int zz = idx / lines, l = idx - zz * lines;
+        if (TOT % NT == 0 || idx < TOT) A[l * LS + zz] = cmul(x[p], wz[zz]);
+    }
+    __syncthreads();
+    fft_ip<NZ, lines, NT, -1, LS>(A, twh);
+#pragma unroll
+    for (int p = 0; p < MPER; ++p) {
+        const int idx = (int)threadIdx.x + p * NT;
+        const int j = idx / NZ, m = idx & (NZ - 1);
+        if ((MUL % NT == 0 || idx < MUL) && j < w)
+            tensor_apply(odd[p], A[(0 * W + j) * LS + m], A[(1 * W + j) * LS + m], A[(2 * W + j) * LS + m]);
+    }
+    __syncthreads();
+    fft_ip<NZ, lines, NT, 1, LS>(A, twh);
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int idx = (int)threadIdx.x + p * NT;
+        const int zz = idx / lines, l = idx - zz * lines;
+        const int c = l / W, j = l - c * W;
+        if ((TOT % NT == 0 || idx < TOT) && j < w) {
+            const float2 o = cmulc(A[l * LS + zz], wz[zz]);
+            cv.W[widx(cv, NZ, c, zz, ky, kx0 + j)] = make_float2(y[p].x + o.x, y[p].y + o.y);
+        }
+    }
+}
+
+// kx columns and threads per workgroup of the open split pass of half length nz:
+// 1024 threads at the longer lengths keep each thread's register share of the
+// lines small enough for 16 waves per CU
+static constexpr int zopen_width(int nz) { return nz >= 256 ? 8 : 16; }
+static constexpr int zopen_threads(int nz) { return nz >= 128 ? 1024 : Z3_TPB; }
+template <int NZ>
+static const void* zopen_kernel() { return (const void*)k3_zconv_o<NZ, zopen_width(NZ), zopen_threads(NZ)>; }
+
 int zconv_template_width(int fz)
 {
     switch (fz) {
@@ -465,14 +567,32 @@ int zconv_template_width(int fz)
     }
 }
 
-static const void* zconv_template(int fz)
+int zconv_open_width(int fz)
 {
     switch (fz) {
+    case 32: return zopen_width(16);
+    case 64: return zopen_width(32);
+    case 128: return zopen_width(64);
+    case 256: return zopen_width(128);
+    case 512: return zopen_width(256);
+    default: return 0;
+    }
+}
+
+// zt: fz for the full compile-time pass, fz + 1 for the open split pass
+static const void* zconv_template(int zt)
+{
+    switch (zt) {
     case 32: return (const void*)k3_zconv_t<32, 16>;
     case 64: return (const void*)k3_zconv_t<64, 16>;
     case 128: return (const void*)k3_zconv_t<128, 16>;
     case 256: return (const void*)k3_zconv_t<256, 16>;
     case 512: return (const void*)k3_zconv_t<512, 8>;
+    case 33: return zopen_kernel<16>();
+    case 65: return zopen_kernel<32>();
+    case 129: return zopen_kernel<64>();
+    case 257: return zopen_kernel<128>();
+    case 513: return zopen_kernel<256>();
     default: return nullptr;
     }
 }
@@ -537,16 +657,11 @@ hipError_t launch3_ypass(int sign, const Rep3* reps, const int4* items, int nite
 hipError_t launch3_zconv(const Rep3* reps, const int4* items, int nitems, size_t lds, int zt, hipStream_t st)
 {
     if (nitems <= 0) return hipSuccess;
-    switch (zt) {
-    case 0: k3_zconv<<<nitems, Z3_TPB, lds, st>>>(reps, items); break;
-    case 32: k3_zconv_t<32, 16><<<nitems, Z3_TPB, lds, st>>>(reps, items); break;
-    case 64: k3_zconv_t<64, 16><<<nitems, Z3_TPB, lds, st>>>(reps, items); break;
-    case 128: k3_zconv_t<128, 16><<<nitems, Z3_TPB, lds, st>>>(reps, items); break;
-    case 256: k3_zconv_t<256, 16><<<nitems, Z3_TPB, lds, st>>>(reps, items); break;
-    case 512: k3_zconv_t<512, 8><<<nitems, Z3_TPB, lds, st>>>(reps, items); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    const void* f = zt ? zconv_template(zt) : (const void*)k3_zconv;
+    if (!f) return hipErrorInvalidValue;
+    const int nt = (zt & 1) ? zopen_threads((zt - 1) / 2) : Z3_TPB;   // open split: zt = 2 nz + 1
+    void* args[] = {(void*)&reps, (void*)&items};
+    return hipLaunchKernel(f, dim3(nitems), dim3(nt), args, lds, st);
 }
 
 hipError_t configure_lds3(size_t max_bytes)
@@ -565,7 +680,7 @@ hipError_t configure_lds3(size_t max_bytes)
                 if (e != hipSuccess && first == hipSuccess) first = e;
             }
     for (int f : {32, 64, 128, 256, 512})
-        for (const void* k : {zconv_template(f), ypass_template<-1>(f), ypass_template<1>(f)}) {
+        for (const void* k : {zconv_template(f), zconv_template(f + 1), ypass_template<-1>(f), ypass_template<1>(f)}) {
             hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, v);
             if (e != hipSuccess && first == hipSuccess) first = e;
         }

@@ -1300,6 +1300,12 @@ extern "C" int msd_longrange3d_create(int nx, int ny, int nz, int open, msd_long
         const size_t lz = (size_t)3 * wt * (l->fz + CONV3_LPAD) * sizeof(float2);
         if (lz <= G.max_lds) { l->zt = l->fz; l->wz = wt; l->lds_z = lz; }
     }
+    // open boundaries: the split pass of two half-length transforms (lines of nz
+    // plus the half-length and shift twiddles)
+    if (const int wt = l->open ? zconv_open_width(l->fz) : 0) {
+        const size_t lz = ((size_t)3 * wt * (nz + CONV3_LPAD) + 2 * (size_t)nz) * sizeof(float2);
+        if (lz <= G.max_lds) { l->zt = l->fz + 1; l->wz = wt; l->lds_z = lz; }
+    }
     if (l->lds_rows > G.max_lds || l->lds_y > G.max_lds || l->lds_z > G.max_lds)
         return fail(MSD_EUNSUPPORTED, "LongRange3D.new: %dx%dx%d transform exceeds the LDS per workgroup", l->fx, l->fy, l->fz);
     if (int rc = twiddles(l->fx, &l->twx, &l->twx64)) return rc;

@@ -156,7 +156,7 @@ struct Conv3Dev {
     int fx, fy, fz, fxh;
     int ty;                 // rows per row tile
     int wy, wz;             // kx block widths of the y and z passes
-    int zt;                 // z length of the compile-time z pass (0: the runtime-length pass)
+    int zt;                 // z length of the compile-time z pass (0: the runtime-length pass; fz + 1: open split pass)
     int yt;                 // y length of the compile-time y pass (0: the runtime-length pass)
     int rt;                 // x length of the compile-time row pass (0: the runtime-length pass)
     int fzs;                // kz planes of the tensor words (fz/2 + 1 when the tensor has z parity, else fz)
@@ -188,6 +188,7 @@ int rows_template_height(int fx);   // tile height of the compile-time row pass 
 hipError_t launch3_ypass(int sign, const Rep3* reps, const int4* items, int nitems, size_t lds, int yt, hipStream_t st);
 // kx width of the compile-time y pass for a y length (0 when it has no instance)
 int ypass_template_width(int fy);
+int zconv_open_width(int fz);       // kx per workgroup of the open split z pass of length fz, 0 if none
 // zt: every replica's compile-time z length (Conv3Dev::zt), or 0 for the runtime-length pass
 hipError_t launch3_zconv(const Rep3* reps, const int4* items, int nitems, size_t lds, int zt, hipStream_t st);
 // kx width of the compile-time z pass for a z length (0 when it has no instance)
