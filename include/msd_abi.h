@@ -238,11 +238,14 @@ int msd_stepper_profile(msd_stepper* st, int nsteps, int cap, char names[][32], 
 int msd_stepper_autotune(msd_stepper* st, int steps, int* best_ty, int* best_tpb, double* best_ms);
 
 /* ---------------------------------------------------------------- surrogate
- * No counterpart in the reference: a learned predictor of the layered dipole
- * field's change between exact refreshes (the field itself is LongRange::apply,
- * cpu/longrange/spinoperationlongrange.cpp:354-430).  Three 5x5x3 convolution
- * layers, 3 -> 16 -> 16 -> 3 channels, no bias, identity activation, periodic in
- * x and y and zero-padded in z, BF16 MFMA with FP32 accumulation:
+ * No counterpart in the reference: a learned predictor of the dipole field's
+ * change between exact refreshes (the field itself is the layered
+ * LongRange::apply, cpu/longrange/spinoperationlongrange.cpp:354-430, or the 3D
+ * LongRange3D::apply, cpu/longrange3d/spinoperationlongrange3d.cpp:369-432).
+ * Three 5x5x3 convolution layers, 3 -> 16 -> 16 -> 3 channels, no bias, identity
+ * activation, periodic in x and y (zero-padded there too after
+ * msd_surrogate_set_open(s, 1)) and zero-padded in z, BF16 MFMA with FP32
+ * accumulation:
  *   dH = s_out * L3(L2(L1(s_in * (m - m_ref)))).
  * Weights of layer l are [75 taps][cout][cin] doubles, tap = (dz+1)*25 + (dy+2)*5
  * + (dx+2) for offsets dz in -1..1, dy and dx in -2..2.  nx must be a multiple
@@ -250,6 +253,9 @@ int msd_stepper_autotune(msd_stepper* st, int steps, int* best_ty, int* best_tpb
 typedef struct msd_surrogate msd_surrogate;
 int msd_surrogate_create(int nx, int ny, int nz, msd_surrogate** out);
 int msd_surrogate_destroy(msd_surrogate* s);
+/* open = 1: zero outside the lattice in x and y (a lattice with open boundaries);
+ * 0 (the default): periodic in x and y */
+int msd_surrogate_set_open(msd_surrogate* s, int open);
 int msd_surrogate_set_weights(msd_surrogate* s, int layer, const double* w);
 int msd_surrogate_get_weights(const msd_surrogate* s, int layer, double* w);
 int msd_surrogate_set_scales(msd_surrogate* s, double s_in, double s_out, double sigma2);
@@ -257,9 +263,12 @@ int msd_surrogate_get_scales(const msd_surrogate* s, double* s_in, double* s_out
 /* input dm given on the host ([3][n] SoA), or as m - m_ref of two systems */
 int msd_surrogate_set_input(msd_surrogate* s, const double* dm);
 int msd_surrogate_set_input_systems(msd_surrogate* s, const msd_system* m, const msd_system* m_ref);
-/* training target: given on the host ([3][n]), or the exact Dipole field of m - m_ref */
+/* training target: given on the host ([3][n]), or the exact Dipole / LongRange3D
+ * field of m - m_ref */
 int msd_surrogate_set_target(msd_surrogate* s, const double* t);
 int msd_surrogate_set_target_dipole(msd_surrogate* s, msd_dipole* d, const msd_system* m, const msd_system* m_ref);
+int msd_surrogate_set_target_longrange3d(msd_surrogate* s, msd_longrange3d* l, const msd_system* m,
+                                         const msd_system* m_ref);
 /* dH of the current input; out ([3][n]) may be NULL */
 int msd_surrogate_forward(msd_surrogate* s, double* out);
 /* loss of the current input/target and the weight gradients (msd_surrogate_get_grads) */
@@ -268,11 +277,13 @@ int msd_surrogate_get_grads(const msd_surrogate* s, int layer, double* g);
 int msd_surrogate_adam(msd_surrogate* s, double lr);
 int msd_surrogate_reset_optimizer(msd_surrogate* s);
 int msd_surrogate_train_step(msd_surrogate* s, double lr, double* loss);
-/* max|H_ref + dH(m - m_ref) - H(m)| / max|H(m)| with the exact Dipole fields,
- * and (frozen_err, may be NULL) the same for H_ref alone */
+/* max|H_ref + dH(m - m_ref) - H(m)| / max|H(m)| with the exact Dipole (or
+ * LongRange3D) fields, and (frozen_err, may be NULL) the same for H_ref alone */
 int msd_surrogate_field_error(msd_surrogate* s, msd_dipole* d, const msd_system* m, const msd_system* m_ref,
                               double* rel_err, double* frozen_err);
-/* Learned dipole field in the step driver (a layered Dipole driver): windows of
+int msd_surrogate_field_error_longrange3d(msd_surrogate* s, msd_longrange3d* l, const msd_system* m,
+                                          const msd_system* m_ref, double* rel_err, double* frozen_err);
+/* Learned dipole field in the step driver (a Dipole or LongRange3D driver): windows of
  * K steps, each started by an exact field evaluation that becomes the reference
  * and checks the surrogate's prediction for that state; a window whose check
  * exceeds tol runs exact steps.  sur = NULL returns to exact steps. */

@@ -137,6 +137,7 @@ _SIGS = {
     "msd_stepper_surrogate_stats": (C.c_int, [_vp, _lp, _lp, _lp, _dp, _dp]),
     "msd_surrogate_create": (C.c_int, [C.c_int, C.c_int, C.c_int, _pp]),
     "msd_surrogate_destroy": (C.c_int, [_vp]),
+    "msd_surrogate_set_open": (C.c_int, [_vp, C.c_int]),
     "msd_surrogate_set_weights": (C.c_int, [_vp, C.c_int, _dp]),
     "msd_surrogate_get_weights": (C.c_int, [_vp, C.c_int, _dp]),
     "msd_surrogate_set_scales": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double]),
@@ -145,6 +146,7 @@ _SIGS = {
     "msd_surrogate_set_input_systems": (C.c_int, [_vp, _vp, _vp]),
     "msd_surrogate_set_target": (C.c_int, [_vp, _dp]),
     "msd_surrogate_set_target_dipole": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "msd_surrogate_set_target_longrange3d": (C.c_int, [_vp, _vp, _vp, _vp]),
     "msd_surrogate_forward": (C.c_int, [_vp, _dp]),
     "msd_surrogate_loss_grads": (C.c_int, [_vp, _dp]),
     "msd_surrogate_get_grads": (C.c_int, [_vp, C.c_int, _dp]),
@@ -152,6 +154,7 @@ _SIGS = {
     "msd_surrogate_reset_optimizer": (C.c_int, [_vp]),
     "msd_surrogate_train_step": (C.c_int, [_vp, C.c_double, _dp]),
     "msd_surrogate_field_error": (C.c_int, [_vp, _vp, _vp, _vp, _dp, _dp]),
+    "msd_surrogate_field_error_longrange3d": (C.c_int, [_vp, _vp, _vp, _vp, _dp, _dp]),
 }
 
 

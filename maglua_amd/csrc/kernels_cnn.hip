@@ -10,7 +10,8 @@
 // output gradient.  A layer is
 //   Y[s][o] = sum_t sum_i W[t][o][i] X[s + off(t)][i],
 //   off(t) = (dx, dy, dz) = (t%5 - 2, (t/5)%5 - 2, t/25 - 1),
-// periodic in x and y, zero outside the lattice in z.
+// periodic in x and y (or, with CnnDims::open, zero outside the lattice in x and y
+// too), zero outside the lattice in z.
 //
 // Row walk.  A wave owns 16 XF output sites along x of one layer z and walks the
 // input rows y' of its tile in order.  Input row y' feeds output rows y' - dy,
@@ -237,23 +238,27 @@ __global__ void __launch_bounds__(CNN_TPB) k_cnn_walk(CnnDims d, CnnWalk wk, con
             const int e = tid + k * CNN_TPB;
             if (NARROW) {                                // two 8-byte sites per element
                 const int dzi = e / (XW / 2), q = e - dzi * (XW / 2);
-                const int zz = z + dzi - 1;
-                gsrc[k] = (zz >= 0 && zz < nz) ? ((zz * ny) * nx + wrap(x0 - 2 + 2 * q, nx)) / 2 : -1;
+                const int zz = z + dzi - 1, xx = x0 - 2 + 2 * q;   // nx even: a pair is inside or outside
+                const bool in_x = !d.open || (unsigned)xx < (unsigned)nx;
+                gsrc[k] = (zz >= 0 && zz < nz && in_x) ? ((zz * ny) * nx + wrap(xx, nx)) / 2 : -1;
             } else {
                 const int site = e >> 1, dzi = site / XW, xs = site - dzi * XW;
-                const int zz = min(max(z + dzi - 1, 0), nz - 1);
-                gsrc[k] = ((zz * ny) * nx + wrap(x0 + xs - 2, nx)) * 2 + (e & 1);
+                const int zz = min(max(z + dzi - 1, 0), nz - 1), xx = x0 + xs - 2;
+                const bool in_x = !d.open || (unsigned)xx < (unsigned)nx;
+                gsrc[k] = in_x ? ((zz * ny) * nx + wrap(xx, nx)) * 2 + (e & 1) : -1;
             }
         }
         const int rstride = NARROW ? nx / 2 : nx * 2;    // uint4 per layer row of `in`
         auto load_row = [&](auto sl, int j) __attribute__((always_inline)) {
             uint4* dst = slot(sl) + 64 * w;
-            const int yo = wrap(y0 - 2 + j, ny) * rstride;
+            const int yy = y0 - 2 + j;
+            const bool in_y = !d.open || (unsigned)yy < (unsigned)ny;
+            const int yo = wrap(yy, ny) * rstride;
 #pragma unroll
             for (int k = 0; k < NK; ++k) {
                 const int e = tid + k * CNN_TPB;
                 if (e < ROWE) {                          // lanes past the row idle; every wave keeps >= GUAR loads
-                    const uint4* src = gsrc[k] >= 0 ? in + gsrc[k] + yo : zeros;
+                    const uint4* src = (gsrc[k] >= 0 && in_y) ? in + gsrc[k] + yo : zeros;
                     __builtin_amdgcn_global_load_lds((gvoid_t*)src, (lvoid_t*)(dst + k * CNN_TPB), 16, 0, 0);
                 }
             }
@@ -680,8 +685,10 @@ __global__ void __launch_bounds__(CNN_TPB) k_cnn_wgrad(CnnDims d, const uint16_t
             const int hx = s % HX, r = s / HX, hy = r % HY, hz = r / HY;
             const int zz = z + hz - 1;
             uint2 v = make_uint2(0u, 0u);
-            if (zz >= 0 && zz < d.nz) {
-                const int xx = wrap(x0 + hx - 2, d.nx), yy = wrap(y0 + hy - 2, d.ny);
+            const int xr = x0 + hx - 2, yr = y0 + hy - 2;
+            const bool in_xy = !d.open || ((unsigned)xr < (unsigned)d.nx && (unsigned)yr < (unsigned)d.ny);
+            if (zz >= 0 && zz < d.nz && in_xy) {
+                const int xx = wrap(xr, d.nx), yy = wrap(yr, d.ny);
                 v = ldg(reinterpret_cast<const uint2*>(X), (uint32_t)(xx + d.nx * (yy + d.ny * zz)) * xq + q);
             }
             const int b0 = (((4 * q) * HZ + hz) * HY + hy) * HX + hx, st = HZ * HY * HX;

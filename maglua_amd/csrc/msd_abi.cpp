@@ -896,8 +896,9 @@ struct msd_dipole {
     int gmax[4] = {1000, 1000, 1000, 1000};
     double ABC[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     bool loaded = false, custom = false, spectrum = false;
-    // bumped by every op-API transform through this object's spectrum buffers, so a
-    // step driver sharing them knows its cached spin spectrum was overwritten
+    // bumped by every op-API transform through this object's spectrum buffers and by
+    // every step driver's steps (which record the value), so a step driver sharing
+    // them knows whether its cached spin spectrum is still there
     uint64_t epoch = 0;
     double words_mult = NAN;
     double* T64 = nullptr;    // [6][2nz-1][ny][nx]
@@ -1234,6 +1235,7 @@ struct msd_longrange3d {
     Rep3* orep = nullptr;
     int4 *oitems_r = nullptr, *oitems_y = nullptr, *oitems_z = nullptr;
     int onr = 0, ony = 0, onz = 0;
+    uint64_t epoch = 0;      // bumped whenever W changes hands: field evaluations, a driver's steps
     ~msd_longrange3d()
     {
         (void)hipStreamSynchronize(G.stream);
@@ -1472,18 +1474,18 @@ static int upload_vec(T** dst, const std::vector<T>& v)
     return MSD_OK;
 }
 
-extern "C" int msd_longrange3d_apply(msd_longrange3d* l, msd_system* s)
+// Exact LongRange3D field of the spins S ([3][n]) into H ([3][n], overwritten): the
+// five passes with the single-replica descriptor.  The working spectrum W is left
+// holding the field's spectrum.
+static int lr3_field_of(msd_longrange3d* l, const float* S, float* H)
 {
-    NONNULL(l); NONNULL(s);
-    if (!same_dims(s->d, l->d.nx, l->d.ny, l->d.nz)) return fail(MSD_EMISMATCH, "LongRange3D:apply: size mismatch");
     if (int rc = l->prepare()) return rc;
-    int k;
-    if (int rc = s->mark("LongRange3D", &k)) return rc;
+    l->epoch++;
     Rep3 r{};
-    r.a.d = s->d;
+    r.a.d = l->d;
     r.cv = l->conv();
-    r.cv.sin = s->S;
-    r.cv.hout = s->slots[k].h;
+    r.cv.sin = S;
+    r.cv.hout = H;
     r.has_lr = 1;
     if (!l->orep) {
         std::vector<int4> rows, ys, zs;
@@ -1501,8 +1503,18 @@ extern "C" int msd_longrange3d_apply(msd_longrange3d* l, msd_system* s)
     if (e == hipSuccess) e = launch3_ypass(1, l->orep, l->oitems_y, l->ony, l->lds_y, l->yt, G.stream);
     if (e == hipSuccess) e = launch3_rows(R3_INV, false, l->orep, l->oitems_r, l->onr, l->lds_rows, l->rt, G.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(G.stream);   // the descriptor lives on the host stack
-    if (e != hipSuccess) return fail(MSD_EDEVICE, "LongRange3D:apply: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "LongRange3D field: %s", hipGetErrorString(e));
     return MSD_OK;
+}
+
+extern "C" int msd_longrange3d_apply(msd_longrange3d* l, msd_system* s)
+{
+    NONNULL(l); NONNULL(s);
+    if (!same_dims(s->d, l->d.nx, l->d.ny, l->d.nz)) return fail(MSD_EMISMATCH, "LongRange3D:apply: size mismatch");
+    if (int rc = l->prepare()) return rc;
+    int k;
+    if (int rc = s->mark("LongRange3D", &k)) return rc;
+    return lr3_field_of(l, s->S, s->slots[k].h);
 }
 
 // ------------------------------------------------------------ batch of replicas
@@ -1656,6 +1668,7 @@ struct msd_surrogate {
     double s_in = 1.0, s_out = 1.0, sigma2 = 1.0;
     int adam_t = 0;
     bool packed = false;
+    int cus = 0;
     ~msd_surrogate()
     {
         (void)hipStreamSynchronize(G.stream);
@@ -1665,6 +1678,31 @@ struct msd_surrogate {
     }
     static int cin(int l) { return l == 0 ? 3 : 16; }
     static int cout(int l) { return l == 2 ? 3 : 16; }
+    // Row-walk items: 128 x ty sites of one layer, or 64 x ty sites of every layer
+    // of a film of 1, 2, 4 or 8 layers (periodic x and y only).  A workgroup walks
+    // ty + 4 input rows per item; per layer shape pick ty (at most 32) for the
+    // fewest input rows per resident workgroup.
+    void plan_walks()
+    {
+        const int nx = d.nx, ny = d.ny, nz = d.nz;
+        const bool film = cnn_film_supported(nz) && !cd.open;
+        for (int v = 0; v < 3; ++v) {
+            CnnWalk& wk = walk[v];
+            wk.film = film ? 1 : 0;
+            wk.rtiles_x = film ? (nx + 63) / 64 : (nx + 127) / 128;
+            const int zitems = film ? 1 : nz;
+            double best = 0.0;
+            const long slots = (long)cnn_rows_occupancy(v, film ? nz : 0) * cus;
+            for (int ty = std::min(ny, cnn_rows_ty_max()); ty >= 1; --ty) {
+                const long items = (long)wk.rtiles_x * zitems * ((ny + ty - 1) / ty);
+                const double cost = (double)((items + slots - 1) / slots) * (ty + 4);
+                if (best == 0.0 || cost < best) { best = cost; wk.ty = ty; }
+            }
+            wk.rtiles_y = (ny + wk.ty - 1) / wk.ty;
+            wk.rtiles = wk.rtiles_x * wk.rtiles_y * zitems;
+            rgrid[v] = (int)std::min<long>(wk.rtiles, slots);
+        }
+    }
     // row-walk shape of each layer and of the adjoints that back-propagation runs
     static constexpr int FWD_SHAPE[3] = {CV_IN3, CV_F16, CV_OUT3};
     static constexpr int ADJ_SHAPE[3] = {-1, CV_F16, CV_IN3};
@@ -1706,6 +1744,14 @@ static int surrogate_dims_ok(int nx, int ny, int nz)
     return MSD_OK;
 }
 
+extern "C" int msd_surrogate_set_open(msd_surrogate* s, int open)
+{
+    NONNULL(s);
+    s->cd.open = open ? 1 : 0;
+    s->plan_walks();
+    return MSD_OK;
+}
+
 extern "C" int msd_surrogate_create(int nx, int ny, int nz, msd_surrogate** out)
 {
     NONNULL(out);
@@ -1719,29 +1765,9 @@ extern "C" int msd_surrogate_create(int nx, int ny, int nz, msd_surrogate** out)
     c.tiles_x = nx / 32; c.tiles_y = (ny + 7) / 8; c.ntiles = c.tiles_x * c.tiles_y * nz;
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, G.device));
-    const int cus = prop.multiProcessorCount;
-    s->grid = std::min(c.ntiles, 2 * cus);
-    // row-walk items: 128 x ty sites of one layer, or 64 x ty sites of every layer
-    // of a film of 1, 2, 4 or 8 layers.  A workgroup walks ty + 4 input rows per
-    // item; per layer shape pick ty (at most 32) for the fewest input rows per
-    // resident workgroup.
-    const bool film = cnn_film_supported(nz);
-    for (int v = 0; v < 3; ++v) {
-        CnnWalk& wk = s->walk[v];
-        wk.film = film ? 1 : 0;
-        wk.rtiles_x = film ? (nx + 63) / 64 : (nx + 127) / 128;
-        const int zitems = film ? 1 : nz;
-        double best = 0.0;
-        const long slots = (long)cnn_rows_occupancy(v, film ? nz : 0) * cus;
-        for (int ty = std::min(ny, cnn_rows_ty_max()); ty >= 1; --ty) {
-            const long items = (long)wk.rtiles_x * zitems * ((ny + ty - 1) / ty);
-            const double cost = (double)((items + slots - 1) / slots) * (ty + 4);
-            if (best == 0.0 || cost < best) { best = cost; wk.ty = ty; }
-        }
-        wk.rtiles_y = (ny + wk.ty - 1) / wk.ty;
-        wk.rtiles = wk.rtiles_x * wk.rtiles_y * zitems;
-        s->rgrid[v] = (int)std::min<long>(wk.rtiles, slots);
-    }
+    s->cus = prop.multiProcessorCount;
+    s->grid = std::min(c.ntiles, 2 * s->cus);
+    s->plan_walks();
     const size_t n = s->d.n, W3 = (size_t)3 * msd_surrogate::WSZ;
     for (float** p : {&s->W, &s->Mo, &s->Vo, &s->gW})
         if (int rc = dalloc(p, W3)) return rc;
@@ -1852,15 +1878,40 @@ extern "C" int msd_surrogate_set_target(msd_surrogate* s, const double* t)
     return upload_f32(s->T, t, (size_t)3 * s->d.n);
 }
 
+// The exact field operator a surrogate stands in for: a layered Dipole or a
+// LongRange3D (exactly one is set).
+struct ExactField {
+    msd_dipole* dip;
+    msd_longrange3d* lr3;
+    int of(const float* S, float* H) const { return dip ? dipole_field_of(dip, S, H) : lr3_field_of(lr3, S, H); }
+    bool fits(const Dims& d) const
+    {
+        return dip ? same_dims(d, dip->d.nx, dip->d.ny, dip->d.nz) : same_dims(d, lr3->d.nx, lr3->d.ny, lr3->d.nz);
+    }
+};
+
+static int surrogate_set_target(msd_surrogate* s, ExactField f, const msd_system* m, const msd_system* m_ref)
+{
+    if (!same_dims(s->d, m->d.nx, m->d.ny, m->d.nz) || !same_dims(s->d, m_ref->d.nx, m_ref->d.ny, m_ref->d.nz) ||
+        !f.fits(s->d))
+        return fail(MSD_EMISMATCH, "Surrogate.setTarget: SpinSystem or field operator size differs");
+    hipError_t e = launch_cnn_sub((size_t)3 * s->d.n, m->S, m_ref->S, s->D, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "Surrogate.setTarget: %s", hipGetErrorString(e));
+    return f.of(s->D, s->T);   // the field is linear in the spins
+}
+
 extern "C" int msd_surrogate_set_target_dipole(msd_surrogate* s, msd_dipole* dip, const msd_system* m,
                                                const msd_system* m_ref)
 {
     NONNULL(s); NONNULL(dip); NONNULL(m); NONNULL(m_ref);
-    if (!same_dims(s->d, m->d.nx, m->d.ny, m->d.nz) || !same_dims(s->d, m_ref->d.nx, m_ref->d.ny, m_ref->d.nz))
-        return fail(MSD_EMISMATCH, "Surrogate.setTarget: SpinSystem size differs");
-    hipError_t e = launch_cnn_sub((size_t)3 * s->d.n, m->S, m_ref->S, s->D, G.stream);
-    if (e != hipSuccess) return fail(MSD_EDEVICE, "Surrogate.setTarget: %s", hipGetErrorString(e));
-    return dipole_field_of(dip, s->D, s->T);
+    return surrogate_set_target(s, ExactField{dip, nullptr}, m, m_ref);
+}
+
+extern "C" int msd_surrogate_set_target_longrange3d(msd_surrogate* s, msd_longrange3d* lr, const msd_system* m,
+                                                    const msd_system* m_ref)
+{
+    NONNULL(s); NONNULL(lr); NONNULL(m); NONNULL(m_ref);
+    return surrogate_set_target(s, ExactField{nullptr, lr}, m, m_ref);
 }
 
 extern "C" int msd_surrogate_forward(msd_surrogate* s, double* out)
@@ -1957,16 +2008,16 @@ static int field_maxdiff(const float* a, const float* b, size_t n3, double* maxd
 // Surrogate prediction of the dipole field at m (m_ref, H_ref its reference
 // state and exact field) against a field given on the host: returns
 // max|pred - exact| / max|exact| (the residual the step driver checks).
-extern "C" int msd_surrogate_field_error(msd_surrogate* s, msd_dipole* dip, const msd_system* m, const msd_system* m_ref,
-                                         double* rel_err, double* frozen_err)
+static int surrogate_field_error(msd_surrogate* s, ExactField f, const msd_system* m, const msd_system* m_ref,
+                                 double* rel_err, double* frozen_err)
 {
-    NONNULL(s); NONNULL(dip); NONNULL(m); NONNULL(m_ref); NONNULL(rel_err);
-    if (!same_dims(s->d, m->d.nx, m->d.ny, m->d.nz) || !same_dims(s->d, m_ref->d.nx, m_ref->d.ny, m_ref->d.nz))
-        return fail(MSD_EMISMATCH, "Surrogate.fieldError: SpinSystem size differs");
+    if (!same_dims(s->d, m->d.nx, m->d.ny, m->d.nz) || !same_dims(s->d, m_ref->d.nx, m_ref->d.ny, m_ref->d.nz) ||
+        !f.fits(s->d))
+        return fail(MSD_EMISMATCH, "Surrogate.fieldError: SpinSystem or field operator size differs");
     const size_t n3 = (size_t)3 * s->d.n;
     // exact H(m) into T, exact H(m_ref) into D, prediction D + CNN(m - m_ref) into Y3
-    if (int rc = dipole_field_of(dip, m->S, s->T)) return rc;
-    if (int rc = dipole_field_of(dip, m_ref->S, s->D)) return rc;
+    if (int rc = f.of(m->S, s->T)) return rc;
+    if (int rc = f.of(m_ref->S, s->D)) return rc;
     hipError_t e = launch_cnn_input(s->d, m->S, m_ref->S, (float)s->s_in, s->X0, G.stream);
     if (e != hipSuccess) return fail(MSD_EDEVICE, "Surrogate.fieldError: %s", hipGetErrorString(e));
     if (int rc = s->forward(s->Y3, s->D, G.stream)) return rc;
@@ -1978,6 +2029,20 @@ extern "C" int msd_surrogate_field_error(msd_surrogate* s, msd_dipole* dip, cons
         *frozen_err = mb > 0 ? md / mb : md;
     }
     return MSD_OK;
+}
+
+extern "C" int msd_surrogate_field_error(msd_surrogate* s, msd_dipole* dip, const msd_system* m, const msd_system* m_ref,
+                                         double* rel_err, double* frozen_err)
+{
+    NONNULL(s); NONNULL(dip); NONNULL(m); NONNULL(m_ref); NONNULL(rel_err);
+    return surrogate_field_error(s, ExactField{dip, nullptr}, m, m_ref, rel_err, frozen_err);
+}
+
+extern "C" int msd_surrogate_field_error_longrange3d(msd_surrogate* s, msd_longrange3d* lr, const msd_system* m,
+                                                     const msd_system* m_ref, double* rel_err, double* frozen_err)
+{
+    NONNULL(s); NONNULL(lr); NONNULL(m); NONNULL(m_ref); NONNULL(rel_err);
+    return surrogate_field_error(s, ExactField{nullptr, lr}, m, m_ref, rel_err, frozen_err);
 }
 
 struct msd_stepper {
@@ -1998,7 +2063,7 @@ struct msd_stepper {
     StepArgs graph_args{};
     Batch3 b3;
     uint64_t spec_version = 0;
-    uint64_t dip_epoch = 0;
+    uint64_t dip_epoch = 0, lr3_epoch = 0;
     // learned dipole field (msd_stepper_set_surrogate)
     msd_surrogate* sur = nullptr;
     int sur_K = 10;
@@ -2206,7 +2271,8 @@ static int member_begin(msd_stepper* st)
 
 static bool spectrum_stale(const msd_stepper* st)
 {
-    return st->spec_version != st->s->version || (st->dip && st->dip->epoch != st->dip_epoch);
+    return st->spec_version != st->s->version || (st->dip && st->dip->epoch != st->dip_epoch) ||
+           (st->lr3 && st->lr3->epoch != st->lr3_epoch);
 }
 
 static int stepper_begin(msd_stepper* st)
@@ -2230,7 +2296,10 @@ static void stepper_end(msd_stepper* st, int nsteps)
     if (st->th) st->th->draw += (uint64_t)nsteps;
     st->s->version++;
     st->spec_version = st->s->version;
-    if (st->dip) st->dip_epoch = st->dip->epoch;
+    // the steps left this system's spin spectrum in the operator's shared buffers:
+    // claim them, so that another driver sharing the operator transforms again
+    if (st->dip) st->dip_epoch = ++st->dip->epoch;
+    if (st->lr3) st->lr3_epoch = ++st->lr3->epoch;
 }
 
 // capture one step into a graph: kind 0 exact, 1 a learned window's first step, 2 a
@@ -2295,8 +2364,9 @@ static int stepper_heun_learned(msd_stepper* st, int nsteps)
     if (int rc = st->sur->pack()) return rc;
     const size_t n3 = (size_t)3 * st->s->d.n;
     int done = 0;
+    const ExactField exact{st->dip, st->lr3};
     while (done < nsteps) {
-        if (int rc = dipole_field_of(st->dip, st->s->S, st->Hx)) return rc;
+        if (int rc = exact.of(st->s->S, st->Hx)) return rc;
         if (st->have_ref) {
             hipError_t e = launch_cnn_input(st->s->d, st->s->S, st->Mref, (float)st->sur->s_in, st->sur->X0, G.stream);
             if (e != hipSuccess) return fail(MSD_EDEVICE, "surrogate check: %s", hipGetErrorString(e));
@@ -2313,8 +2383,13 @@ static int stepper_heun_learned(msd_stepper* st, int nsteps)
         HIPCHK(hipMemcpyAsync(st->Mref, st->s->S, n3 * sizeof(float), hipMemcpyDeviceToDevice, G.stream));
         st->have_ref = true;
         const int L = std::min(st->sur_K, nsteps - done);
-        // the exact field pass above left the spin spectrum of S in the shared
-        // buffers, which is what an exact step's first column pass reads
+        // The layered field pass above left the spin spectrum of S in the shared
+        // buffers, which is what an exact step's first column pass reads; the 3D
+        // pass left the field's spectrum in W, so exact 3D steps transform S again.
+        if (st->sur_fallback && st->lr3) {
+            hipError_t e = st->b3.rows(R3_FWD, false, G.stream);
+            if (e != hipSuccess) return fail(MSD_EDEVICE, "surrogate fallback: %s", hipGetErrorString(e));
+        }
         if (int rc = run_steps(st, L, !st->sur_fallback)) return rc;
         st->sur_windows++;
         if (st->sur_fallback) st->sur_exact_windows++;
@@ -2418,7 +2493,7 @@ extern "C" int msd_stepper_set_surrogate(msd_stepper* st, msd_surrogate* sur, in
         st->spec_version = 0;
         return MSD_OK;
     }
-    if (!st->dip || st->lr3) return fail(MSD_EUNSUPPORTED, "stepper: the surrogate replaces a layered Dipole field");
+    if (!st->dip && !st->lr3) return fail(MSD_EUNSUPPORTED, "stepper: the surrogate replaces a Dipole or LongRange3D field");
     if (!same_dims(st->s->d, sur->d.nx, sur->d.ny, sur->d.nz)) return fail(MSD_EMISMATCH, "stepper: Surrogate size differs");
     if (K < 1) return fail(MSD_EINVAL, "stepper: refresh interval %d", K);
     if (!(tol > 0)) return fail(MSD_EINVAL, "stepper: residual tolerance must be > 0");

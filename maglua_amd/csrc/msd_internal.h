@@ -209,6 +209,7 @@ hipError_t configure_lds(size_t max_bytes);
 struct CnnDims {
     int nx, ny, nz, n;
     int tiles_x, tiles_y, ntiles;
+    int open;               // 0: periodic in x and y; 1: zero outside the lattice in x and y too
 };
 // Row-walk layer items: 128 sites along x, ty output rows, one layer; or (film) 64
 // sites along x, ty output rows, every layer of a film of 1, 2, 4 or 8 layers.

@@ -758,23 +758,28 @@ class Heun(_Handle):
 
 
 class Surrogate(_Handle):
-    """Learned predictor of the layered dipole field's change between exact refreshes:
+    """Learned predictor of the dipole field's change between exact refreshes (the
+    layered Dipole or the 3D LongRange3D field):
     dH = s_out * L3(L2(L1(s_in * (m - m_ref)))), three 5x5x3 convolution layers with
-    3 -> 16 -> 16 -> 3 channels (BF16 MFMA, FP32 accumulate), periodic in x and y,
+    3 -> 16 -> 16 -> 3 channels (BF16 MFMA, FP32 accumulate), periodic in x and y
+    (periodic=False: zero-padded there, for lattices with open boundaries),
     zero-padded in z.  Weights of layer l are arrays [3,5,5,cout,cin] indexed
     [dz+1, dy+2, dx+2, out, in].  Field arrays are [3][n] like SpinSystem.spins()."""
 
     _destroy = "msd_surrogate_destroy"
     CHANNELS = (3, 16, 16, 3)
 
-    def __init__(self, nx, ny=None, nz=None):
+    def __init__(self, nx, ny=None, nz=None, periodic=True):
         if ny is None:
             nx, ny, nz = _dims_of((nx,))
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.n = self.nx * self.ny * self.nz
+        self.periodic = bool(periodic)
         p = _vp()
         call("msd_surrogate_create", self.nx, self.ny, self.nz, C.byref(p))
         super().__init__(p.value)
+        if not self.periodic:
+            call("msd_surrogate_set_open", self._h, 1)
 
     def _shape(self, l):
         return (3, 5, 5, self.CHANNELS[l + 1], self.CHANNELS[l])
@@ -812,7 +817,9 @@ class Surrogate(_Handle):
         call("msd_surrogate_set_target", self._h, _d(a))
 
     def setTargetDipole(self, dip, m, m_ref):
-        call("msd_surrogate_set_target_dipole", self._h, dip.h, m.h, m_ref.h)
+        """Target = the exact field of m - m_ref from dip, a Dipole or a LongRange3D."""
+        fn = "msd_surrogate_set_target_longrange3d" if isinstance(dip, LongRange3D) else "msd_surrogate_set_target_dipole"
+        call(fn, self._h, dip.h, m.h, m_ref.h)
 
     def forward(self, download=True):
         """dH of the current input as a [3][n] array (download=False: leave it on the device)."""
@@ -845,9 +852,11 @@ class Surrogate(_Handle):
         return loss.value
 
     def fieldError(self, dip, m, m_ref):
-        """(max|H_ref + dH - H| / max|H|, the same for H_ref alone) with exact Dipole fields."""
+        """(max|H_ref + dH - H| / max|H|, the same for H_ref alone) with the exact fields of
+        dip, a Dipole or a LongRange3D."""
         a, b = C.c_double(), C.c_double()
-        call("msd_surrogate_field_error", self._h, dip.h, m.h, m_ref.h, C.byref(a), C.byref(b))
+        fn = "msd_surrogate_field_error_longrange3d" if isinstance(dip, LongRange3D) else "msd_surrogate_field_error"
+        call(fn, self._h, dip.h, m.h, m_ref.h, C.byref(a), C.byref(b))
         return a.value, b.value
 
 

@@ -2,8 +2,8 @@
 
 Pairs come from a trajectory of the exact method: within windows of K steps that
 start at a reference state m_ref, every step gives the input m - m_ref and the
-target H_dip(m) - H_dip(m_ref), which is the exact layered Dipole field of
-m - m_ref (the field is linear in m).  Every pair is one Adam step of the
+target H_dip(m) - H_dip(m_ref), which is the exact field of m - m_ref (the field
+is linear in m) from the layered Dipole or the 3D LongRange3D operator.  Every pair is one Adam step of the
 device-side forward/backward kernels; nothing leaves the GPU but the loss.
 """
 import numpy as np
@@ -39,7 +39,7 @@ def fit_scales(sur, ss, ref, dip):
 
 
 def train_on_trajectory(sur, ss, heun, dip, steps, K=10, lr=2e-3, seed=0, weights=None, log_every=0):
-    """Advance `heun` (an exact Dipole step driver on `ss`) `steps` steps, training `sur`
+    """Advance `heun` (an exact Dipole or LongRange3D step driver on `ss`) `steps` steps, training `sur`
     on every step's (m - m_ref, H(m) - H(m_ref)) pair, m_ref the state at the start of
     the current K-step window.  Returns the list of losses."""
     for l, Wl in enumerate(weights if weights is not None else init_weights(seed)):

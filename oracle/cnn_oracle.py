@@ -8,7 +8,8 @@ field between exact refreshes.  Conventions shared with the GPU kernels:
 
 * lattice arrays are [nz][ny][nx][channels] (site x + nx (y + ny z), channels last);
 * a layer is Y[z,y,x,o] = sum_{dz,dy,dx,i} W[dz+1,dy+2,dx+2,o,i] X[z+dz,(y+dy)%ny,(x+dx)%nx,i],
-  dz in -1..1, dy and dx in -2..2: periodic in x and y, zero outside 0 <= z < nz;
+  dz in -1..1, dy and dx in -2..2: periodic in x and y, zero outside 0 <= z < nz
+  (periodic=False: zero outside the lattice in x and y as well);
 * the network is X0 = s_in dm (3 channels), Y1 = L1(X0), Y2 = L2(Y1), Y3 = L3(Y2)
   (16, 16, 3 channels, no bias, identity activation: the target map is linear),
   dH = s_out Y3;
@@ -38,10 +39,18 @@ def bf16(a):
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def shift(X, dz, dy, dx):
-    """S[z,y,x] = X[z+dz, (y+dy)%ny, (x+dx)%nx], zero where z+dz is outside the lattice."""
-    nz = X.shape[0]
-    S = np.roll(X, shift=(-dy, -dx), axis=(1, 2))
+def shift(X, dz, dy, dx, periodic=True):
+    """S[z,y,x] = X[z+dz, (y+dy)%ny, (x+dx)%nx], zero where z+dz is outside the lattice
+    (periodic=False: zero where y+dy or x+dx is outside too)."""
+    nz, ny, nx = X.shape[:3]
+    if periodic:
+        S = np.roll(X, shift=(-dy, -dx), axis=(1, 2))
+    else:
+        S = np.zeros_like(X)
+        y0, y1 = max(0, -dy), min(ny, ny - dy)
+        x0, x1 = max(0, -dx), min(nx, nx - dx)
+        if y1 > y0 and x1 > x0:
+            S[:, y0:y1, x0:x1] = X[:, y0 + dy:y1 + dy, x0 + dx:x1 + dx]
     out = np.zeros_like(S)
     z0, z1 = max(0, -dz), min(nz, nz - dz)
     if z1 > z0:
@@ -56,57 +65,57 @@ def taps():
                 yield dz, dy, dx
 
 
-def layer(X, W):
+def layer(X, W, periodic=True):
     """Y = L_W(X); W is [KZ,KY,KX,cout,cin]."""
     Y = np.zeros(X.shape[:3] + (W.shape[3],))
     for dz, dy, dx in taps():
-        Y += shift(X, dz, dy, dx) @ W[dz + 1, dy + 2, dx + 2].T
+        Y += shift(X, dz, dy, dx, periodic) @ W[dz + 1, dy + 2, dx + 2].T
     return Y
 
 
-def layer_t(G, W):
+def layer_t(G, W, periodic=True):
     """Adjoint of layer(., W): X-gradient from the output gradient G."""
     X = np.zeros(G.shape[:3] + (W.shape[4],))
     for dz, dy, dx in taps():
-        X += shift(G @ W[dz + 1, dy + 2, dx + 2], -dz, -dy, -dx)
+        X += shift(G @ W[dz + 1, dy + 2, dx + 2], -dz, -dy, -dx, periodic)
     return X
 
 
-def layer_w(X, G):
+def layer_w(X, G, periodic=True):
     """dW[t,o,i] = sum_sites G[s,o] X[s+t,i] (the weight gradient of layer)."""
     dW = np.zeros((KZ, KY, KX, G.shape[3], X.shape[3]))
     g = G.reshape(-1, G.shape[3])
     for dz, dy, dx in taps():
-        dW[dz + 1, dy + 2, dx + 2] = g.T @ shift(X, dz, dy, dx).reshape(-1, X.shape[3])
+        dW[dz + 1, dy + 2, dx + 2] = g.T @ shift(X, dz, dy, dx, periodic).reshape(-1, X.shape[3])
     return dW
 
 
-def forward(Ws, dm, s_in, s_out, bf=False):
+def forward(Ws, dm, s_in, s_out, bf=False, periodic=True):
     """dm [nz,ny,nx,3] -> (dH, cache)."""
     r = bf16 if bf else (lambda a: np.asarray(a, dtype=np.float64))
     Wq = [r(W) for W in Ws]
     X0 = r(s_in * np.asarray(dm, dtype=np.float64))
-    Y1 = layer(X0, Wq[0])
+    Y1 = layer(X0, Wq[0], periodic)
     X1 = r(Y1)
-    Y2 = layer(X1, Wq[1])
+    Y2 = layer(X1, Wq[1], periodic)
     X2 = r(Y2)
-    Y3 = layer(X2, Wq[2])
+    Y3 = layer(X2, Wq[2], periodic)
     return s_out * Y3, dict(Wq=Wq, X=(X0, X1, X2), Y3=Y3)
 
 
-def loss_and_grads(Ws, dm, target, s_in, s_out, sigma2, bf=False):
+def loss_and_grads(Ws, dm, target, s_in, s_out, sigma2, bf=False, periodic=True):
     """Loss and its gradients with respect to the three weight tensors."""
     r = bf16 if bf else (lambda a: np.asarray(a, dtype=np.float64))
-    dH, c = forward(Ws, dm, s_in, s_out, bf)
+    dH, c = forward(Ws, dm, s_in, s_out, bf, periodic)
     R = dH - target
     count = R.size
     L = float(np.sum(R * R) / (count * sigma2))
     G3 = r(2.0 * s_out * R / (count * sigma2))
     X0, X1, X2 = c["X"]
     Wq = c["Wq"]
-    G2 = r(layer_t(G3, Wq[2]))
-    G1 = r(layer_t(G2, Wq[1]))
-    grads = [layer_w(X0, G1), layer_w(X1, G2), layer_w(X2, G3)]
+    G2 = r(layer_t(G3, Wq[2], periodic))
+    G1 = r(layer_t(G2, Wq[1], periodic))
+    grads = [layer_w(X0, G1, periodic), layer_w(X1, G2, periodic), layer_w(X2, G3, periodic)]
     return L, grads
 
 

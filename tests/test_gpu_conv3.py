@@ -119,6 +119,26 @@ def cube(dims, T, seed, R=10, J=1.0, K=0.1, g=0.05, dt=0.005, alpha=0.1):
     return spec, gpu, ora, gl, ol
 
 
+def test_drivers_sharing_a_longrange3d_alternate():
+    """Two step drivers on one LongRange3D (one working spectrum W) stepped in turn, with
+    an operator-API evaluation between: each matches a driver with its own LongRange3D."""
+    dims = (8, 8, 8)
+    runs = [cube(dims, 0.0, seed=5 + k % 2)[:4] for k in range(4)]
+    shared = runs[0][3]
+    h = [M.Heun(g["ss"], g["ex"], g["ani"], shared if k < 2 else gl, g["temp"], g["llg"])
+         for k, (spec, g, o, gl) in enumerate(runs)]
+    probe = M.SpinSystem(*dims)
+    probe.randomize(9)
+    for _ in range(3):
+        for k in range(4):
+            h[k].step(4)
+        shared.apply(probe)
+    M.synchronize()
+    for k in range(2):
+        a, b = runs[k][1]["ss"], runs[k + 2][1]["ss"]
+        assert np.array_equal(flat(a.spins()), flat(b.spins())), k
+
+
 @pytest.mark.parametrize("dims,T", [((16, 16, 16), 0.0), ((12, 10, 6), 0.0), ((8, 8, 8), 0.4)])
 def test_heun_with_longrange3d(dims, T):
     spec, g, o, gl, ol = cube(dims, T, seed=3)

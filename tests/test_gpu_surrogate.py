@@ -24,13 +24,13 @@ def aos(v, dims):
     return np.moveaxis(np.asarray(v).reshape(3, nz, ny, nx), 0, -1)
 
 
-def setup(dims, seed, s_in=60.0, s_out=0.04, sigma2=2e-4):
+def setup(dims, seed, s_in=60.0, s_out=0.04, sigma2=2e-4, periodic=True):
     nx, ny, nz = dims
     rng = np.random.default_rng(seed)
     Ws = N.init_weights(seed)
     dm = rng.standard_normal((nz, ny, nx, 3)) * 0.01
     T = rng.standard_normal((nz, ny, nx, 3)) * 0.02
-    sur = M.Surrogate(nx, ny, nz)
+    sur = M.Surrogate(nx, ny, nz, periodic=periodic)
     for l in range(3):
         sur.setWeights(l, Ws[l])
     sur.setScales(s_in, s_out, sigma2)
@@ -49,6 +49,33 @@ def test_forward_matches_oracle(dims):
     err = np.abs(got - want).max() / np.abs(want).max()
     print(f"surrogate forward {dims}: rel err {err:.2e}", flush=True)
     assert err <= 2e-3
+
+
+# zero padding in x and y (open lattices) runs the per-layer walk at every depth
+@pytest.mark.parametrize("dims", [(32, 4, 1), (32, 8, 4), (64, 6, 3), (96, 5, 2), (32, 9, 12)])
+def test_open_forward_matches_oracle(dims):
+    sur, Ws, dm, T, (s_in, s_out, _) = setup(dims, 11, periodic=False)
+    got = aos(sur.forward(), dims)
+    want, _ = N.forward(Ws, dm, s_in, s_out, bf=True, periodic=False)
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print(f"open surrogate forward {dims}: rel err {err:.2e}", flush=True)
+    assert err <= 2e-3
+    # the periodic network differs near the x and y faces: the test sees the padding
+    wrap, _ = N.forward(Ws, dm, s_in, s_out, bf=True)
+    assert np.abs(wrap - want).max() > 10 * np.abs(got - want).max()
+
+
+@pytest.mark.parametrize("dims", [(32, 8, 4), (32, 6, 3)])
+def test_open_gradients_match_oracle(dims):
+    sur, Ws, dm, T, sc = setup(dims, 13, periodic=False)
+    loss, grads = sur.lossGrads()
+    L, want = N.loss_and_grads(Ws, dm, T, *sc, bf=True, periodic=False)
+    print(f"open surrogate loss {dims}: gpu {loss:.6e} oracle {L:.6e}", flush=True)
+    assert abs(loss - L) <= 1e-3 * L
+    for l in range(3):
+        rel = np.linalg.norm(grads[l] - want[l]) / np.linalg.norm(want[l])
+        print(f"  layer {l}: grad rel err {rel:.2e}", flush=True)
+        assert rel <= 1e-2, l
 
 
 def test_weights_round_trip():
@@ -182,3 +209,60 @@ def test_learned_driver_tracks_the_exact_trajectory_and_falls_back():
     eb.step(30)
     st = eb.surrogateStats()
     assert st["windows"] == 3 and st["fallbacks"] == 2 and st["exact_windows"] == 2
+
+
+def open_cube(seed, n=32, T=0.3):
+    spec = W.FilmSpec(n, n, n, J=1.0, K=0.05, g=0.0, pbc=(0, 0, 0), alpha=0.1, dt=0.002, T=T, seed=seed)
+    g = W.build(M, spec)
+    g["ss"].setSpins(*O.random_unit_spins(seed, spec.n))
+    g["temp"].setSeed(seed)
+    lr = M.LongRange3D(n, n, n, open=True)
+    lr.setStrength(0.05)
+    return spec, g, lr
+
+
+def test_learned_driver_on_an_open_longrange3d_cube():
+    """The surrogate over the open 3D convolution, zero-padded in x and y: trained on a
+    trajectory it beats the frozen field, the learned driver tracks the exact trajectory
+    from the same state and noise, and an impossible tolerance falls back to exact steps
+    that continue the exact trajectory."""
+    spec, g, lr = open_cube(31)
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], lr, g["temp"], g["llg"])
+    heun.step(20)
+    sur = M.Surrogate(spec.nx, spec.ny, spec.nz, periodic=False)
+    log = S.train_on_trajectory(sur, g["ss"], heun, lr, steps=200, K=8, lr=2e-3, seed=3)
+    print(f"open cube training loss {log[0]:.3e} -> {np.mean(log[-20:]):.3e}", flush=True)
+    ref = g["ss"].copy()
+    heun.step(8)
+    rel, frozen = sur.fieldError(lr, g["ss"], ref)
+    print(f"open cube field error 8 steps after refresh: surrogate {rel:.3e}, frozen {frozen:.3e}", flush=True)
+    assert rel < 0.5 * frozen
+    a, b = g["ss"].copy(), g["ss"].copy()
+    ta, tb = W.build(M, spec)["temp"], W.build(M, spec)["temp"]
+    ta.setSeed(5)
+    tb.setSeed(5)
+    ea = M.Heun(a, g["ex"], g["ani"], lr, ta, g["llg"])
+    eb = M.Heun(b, g["ex"], g["ani"], lr, tb, g["llg"])
+    eb.setSurrogate(sur, K=8, tol=1e-1)
+    ea.step(32)
+    eb.step(32)
+    st = eb.surrogateStats()
+    dmax = float(np.abs(np.stack(a.spins()[:3]) - np.stack(b.spins()[:3])).max())
+    print(f"open cube learned vs exact after 32 steps: max|dm| {dmax:.3e}; {st}", flush=True)
+    assert st["windows"] == 4 and st["exact_windows"] == 0
+    assert dmax <= 5e-3
+    # an impossible tolerance: the first window (no reference yet) is learned, every
+    # checked one runs exact steps, which continue as an exact driver would
+    eb.setSurrogate(sur, K=8, tol=1e-12)
+    eb.step(8)
+    c = b.copy()
+    tc = W.build(M, spec)["temp"]
+    tc.setSeed(5, draw=tb.draw())
+    ec = M.Heun(c, g["ex"], g["ani"], lr, tc, g["llg"])
+    eb.step(16)
+    ec.step(16)
+    st = eb.surrogateStats()
+    assert st["windows"] == 3 and st["exact_windows"] == 2
+    dfb = float(np.abs(np.stack(b.spins()[:3]) - np.stack(c.spins()[:3])).max())
+    print(f"open cube fallback vs exact: max|dm| {dfb:.3e}", flush=True)
+    assert dfb <= 1e-6

@@ -16,6 +16,13 @@ def test_library_exports_every_declared_symbol():
     assert not missing
 
 
+def test_every_declared_symbol_has_a_ctypes_signature():
+    """A call without argtypes would pass 64-bit handles as C ints."""
+    from maglua_amd import _lib
+    unsigned = [s for s in M.declared_symbols() if s not in _lib._SIGS]
+    assert not unsigned
+
+
 def test_library_is_gfx950_code_object():
     data = open(M._lib.LIB_PATH, "rb").read()
     assert b"gfx950" in data
