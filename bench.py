@@ -219,9 +219,83 @@ def ensemble_leg(M, W, nrep=64, steps=50, R=16):
             "ms_per_step": round(1e3 * el / steps, 4), "setup_s": round(setup, 2)}
 
 
-def cube_leg(M, W, n=256, steps=20):
+def cube_learned(M, W, spec, g, ani, lr, heun, plain, train_steps=100, train_K=8, Ks=(8, 4, 2, 1), gate=1e-4,
+                 field_gate=1e-3, gate_steps=40, tol=1e-2, steps=40):
+    """The 256^3 open cube with the surrogate (zero-padded in x and y) standing in for
+    the 3D convolution between exact refreshes every K steps: trained on the cube's own
+    trajectory, K is the largest of Ks passing both gates over gate_steps steps (the
+    trajectory within max|dm| <= gate of the exact one from the same state and noise,
+    the field at every window start within field_gate of the exact one)."""
+    import numpy as np
+    from maglua_amd import surrogate as S
+    sur = M.Surrogate(spec.nx, spec.ny, spec.nz, periodic=False)
+    t0 = time.perf_counter()
+    losses = S.train_on_trajectory(sur, g["ss"], heun, lr, steps=train_steps, K=train_K, lr=2e-3, seed=1)
+    M.synchronize()
+    train_s = time.perf_counter() - t0
+    ref = g["ss"].copy()
+    heun.step(train_K)
+    err, frozen = sur.fieldError(lr, g["ss"], ref)
+    del ref
+
+    def driver(ss, seed, learned_K=None):
+        t = M.Thermal(spec.nx, spec.ny, spec.nz)
+        t.setTemperature(spec.T)
+        t.setSeed(seed)
+        h = M.Heun(ss, g["ex"], ani, lr, t, g["llg"])
+        if learned_K:
+            h.setSurrogate(sur, K=learned_K, tol=tol)
+        return h, t
+
+    scan, K = [], None
+    for k in Ks:
+        a, b = g["ss"].copy(), g["ss"].copy()
+        ea, _ta = driver(a, 9)
+        eb, _tb = driver(b, 9, k)
+        ea.step(gate_steps)
+        eb.step(gate_steps)
+        d = float(np.abs(np.stack(a.spins()[:3]) - np.stack(b.spins()[:3])).max())
+        fe = float(eb.surrogateStats()["max_residual"])
+        scan.append({"K": k, "traj_max_dm": d, "field_err_max": fe})
+        del a, b, ea, eb
+        if d <= gate and fe <= field_gate:
+            K = k
+            break
+    gate_ok = K is not None
+    if K is None:
+        K = Ks[0]
+    b = g["ss"].copy()
+    eb, _tb = driver(b, 7, K)
+    eb.step(2 * K)
+    M.synchronize()
+    t0 = time.perf_counter()
+    eb.step(steps)
+    M.synchronize()
+    learned = spec.n * steps / (time.perf_counter() - t0)
+    st = eb.surrogateStats()
+    # the learned form's own rate: every window learned (no residual fallback), K = train_K
+    eb.setSurrogate(sur, K=train_K, tol=1e30)
+    eb.step(2 * train_K)
+    M.synchronize()
+    t0 = time.perf_counter()
+    eb.step(steps)
+    M.synchronize()
+    raw = spec.n * steps / (time.perf_counter() - t0)
+    return {"learned_value": round(learned, 1), "plain_value": round(plain, 1), "ratio": round(learned / plain, 4),
+            "learned_steps_value_no_fallback": round(raw, 1), "learned_steps_ratio_no_fallback": round(raw / plain, 4),
+            "K": K, "gate_met": gate_ok, "K_scan": scan, "residual_tol": tol,
+            "gate": f"max|dm| <= {gate:g} over {gate_steps} steps against the exact trajectory (same state and "
+                    f"noise) and dipole-field error <= {field_gate:g} at every window start",
+            "timed_K": K, "timed_windows": st["windows"], "timed_fallback_windows": st["exact_windows"],
+            "field_error_after_train_K": round(err, 6), "frozen_field_error_after_train_K": round(frozen, 6),
+            "train_pairs": train_steps, "train_K": train_K, "train_s": round(train_s, 2),
+            "final_loss": round(float(np.mean(losses[-10:])), 6)}
+
+
+def cube_leg(M, W, n=256, steps=20, learned=True):
     """256^3 open cube (zero-padded 512^3 transforms), J=1, K=0.05 along seeded random axes,
-    g=0.05, T=0.3, dt=0.002, and a 4-replica batch of 128^3 open cubes in one launch."""
+    g=0.05, T=0.3, dt=0.002, and a 4-replica batch of 128^3 open cubes in one launch.  On
+    the single cube, the surrogate form too (cube_learned)."""
     import numpy as np
     out = {}
     for label, dims, count in (("single", (n, n, n), 1), ("batch4", (n // 2, n // 2, n // 2), 4)):
@@ -252,6 +326,9 @@ def cube_leg(M, W, n=256, steps=20):
         el = time.perf_counter() - t0
         out[label] = {"lattice": list(dims), "replicas": count, "value": round(nsites * steps / el, 1),
                       "ms_per_step": round(1e3 * el / steps, 3), "setup_s": round(setup, 2)}
+        if count == 1 and learned:
+            g, ani, lr = keep[0]
+            out[label]["learned"] = cube_learned(M, W, spec, g, ani, lr, drv, nsites * steps / el)
         del drv, members, keep
     out["workload"] = "open cubes, LongRange3D zero-padded to twice the size, random easy axes, g=0.05, T=0.3"
     out["unit"] = "spin-steps/s"
