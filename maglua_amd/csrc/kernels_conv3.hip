@@ -29,7 +29,7 @@ __device__ __forceinline__ float& rowval3(float2* buf, int q, int x, int fx)
 
 __device__ __forceinline__ size_t widx(const Conv3Dev& cv, int nz, int c, int z, int y, int k)
 {
-    return (((size_t)c * nz + z) * cv.fy + y) * cv.fxh + k;
+    return (((size_t)c * nz + z) * cv.fy + y) * cv.fxp + k;
 }
 
 // The six tensor words (xx xy xz yy yz zz) of spectrum point (kx, ky, kz), stored
@@ -287,23 +287,23 @@ __global__ void __launch_bounds__(Y3_TPB) k3_ypass(const Rep3* __restrict__ reps
     const Rep3& rp = reps[it.x];
     const Conv3Dev& cv = rp.cv;
     const int ny = rp.a.d.ny;
-    const int fy = cv.fy, fxh = cv.fxh;
+    const int fy = cv.fy, fxh = cv.fxh, fxp = cv.fxp;
     const int kx0 = it.z;
     const int w = min(cv.wy, fxh - kx0);
     float2* A = lds;
     float2* B = lds + (size_t)w * fy;
-    float2* base = cv.W + (size_t)it.y * fy * fxh + kx0;
+    float2* base = cv.W + (size_t)it.y * fy * fxp + kx0;
     const int yin = SIGN < 0 ? ny : fy;
     for (int idx = threadIdx.x; idx < w * fy; idx += Y3_TPB) {
         const int y = idx / w, j = idx - y * w;
-        A[(size_t)j * fy + y] = y < yin ? base[(size_t)y * fxh + j] : make_float2(0.f, 0.f);
+        A[(size_t)j * fy + y] = y < yin ? base[(size_t)y * fxp + j] : make_float2(0.f, 0.f);
     }
     __syncthreads();
     const float2* R = lds_fft<SIGN>(A, B, w, cv.py, cv.twy);
     const int yout = SIGN < 0 ? fy : ny;
     for (int idx = threadIdx.x; idx < w * yout; idx += Y3_TPB) {
         const int y = idx / w, j = idx - y * w;
-        base[(size_t)y * fxh + j] = R[(size_t)j * fy + y];
+        base[(size_t)y * fxp + j] = R[(size_t)j * fy + y];
     }
 }
 
@@ -318,17 +318,17 @@ __global__ void __launch_bounds__(Y3_TPB) k3_ypass_t(const Rep3* __restrict__ re
     const Rep3& rp = reps[it.x];
     const Conv3Dev& cv = rp.cv;
     const int ny = rp.a.d.ny;
-    const int fxh = cv.fxh;
+    const int fxp = cv.fxp;
     const int kx0 = it.z;
-    const int w = min(W, fxh - kx0);
+    const int w = min(W, cv.fxh - kx0);
     constexpr int LS = FY + CONV3_LPAD;
     float2* A = lds;                                     // [j][LS]
-    float2* base = cv.W + (size_t)it.y * FY * fxh + kx0;
+    float2* base = cv.W + (size_t)it.y * FY * fxp + kx0;
     const int yin = SIGN < 0 ? ny : FY;
     batched_copy<W * FY, Y3_TPB, 8>(
         [&](int idx) {
             const int y = idx / W, j = idx - y * W;
-            return (y < yin && j < w) ? base[(size_t)y * fxh + j] : make_float2(0.f, 0.f);
+            return (y < yin && j < w) ? base[(size_t)y * fxp + j] : make_float2(0.f, 0.f);
         },
         [&](int idx, float2 v) {
             const int y = idx / W, j = idx - y * W;
@@ -339,7 +339,7 @@ __global__ void __launch_bounds__(Y3_TPB) k3_ypass_t(const Rep3* __restrict__ re
     const int yout = SIGN < 0 ? FY : ny;
     for (int idx = threadIdx.x; idx < W * yout; idx += Y3_TPB) {
         const int y = idx / W, j = idx - y * W;
-        if (j < w) base[(size_t)y * fxh + j] = A[j * LS + y];
+        if (j < w) base[(size_t)y * fxp + j] = A[j * LS + y];
     }
 }
 

@@ -1225,11 +1225,11 @@ struct msd_longrange3d {
     double words_mult = NAN;
     double* T64 = nullptr;   // [6][fz][fy][fx]
     float* Tw = nullptr;     // [6][fy][fxh][fzs], fzs <= fz
-    float2* W = nullptr;     // working spectrum [3][nz][fy][fxh]
+    float2* W = nullptr;     // working spectrum [3][nz][fy][fxp]
     float2 *twx = nullptr, *twy = nullptr, *twz = nullptr;
     double2 *twx64 = nullptr, *twy64 = nullptr, *twz64 = nullptr;
     FFTPlan px{}, py{}, pz{};
-    int ty = 1, wy = 1, wz = 1, zt = 0, yt = 0, rt = 0, fzs = 0;
+    int ty = 1, wy = 1, wz = 1, zt = 0, yt = 0, rt = 0, fzs = 0, fxp = 0;
     size_t lds_rows = 0, lds_y = 0, lds_z = 0;
     // single-replica descriptor and work items for the operator API
     Rep3* orep = nullptr;
@@ -1246,7 +1246,7 @@ struct msd_longrange3d {
     Conv3Dev conv() const
     {
         Conv3Dev c{};
-        c.fx = fx; c.fy = fy; c.fz = fz; c.fxh = fxh;
+        c.fx = fx; c.fy = fy; c.fz = fz; c.fxh = fxh; c.fxp = fxp;
         c.ty = ty; c.wy = wy; c.wz = wz; c.zt = zt; c.yt = yt; c.rt = rt; c.fzs = fzs;
         c.px = px; c.py = py; c.pz = pz;
         c.twx = twx; c.twy = twy; c.twz = twz;
@@ -1276,6 +1276,7 @@ extern "C" int msd_longrange3d_create(int nx, int ny, int nz, int open, msd_long
     if (!make_plan(l->fx, l->px) || !make_plan(l->fy, l->py) || !make_plan(l->fz, l->pz))
         return fail(MSD_EUNSUPPORTED, "LongRange3D.new: %dx%dx%d transform needs prime factors in {2,3,5}", l->fx, l->fy, l->fz);
     l->fxh = l->fx / 2 + 1;
+    l->fxp = (l->fxh + 15) / 16 * 16;   // 128-byte W rows: the y and z passes' kx runs stay within cache lines
     // 64 KiB tiles: measured faster at 256^3 open than the full LDS (occupancy)
     const size_t budget = 65536;
     l->ty = 1;
@@ -1313,7 +1314,7 @@ extern "C" int msd_longrange3d_create(int nx, int ny, int nz, int open, msd_long
     if (int rc = twiddles(l->fx, &l->twx, &l->twx64)) return rc;
     if (int rc = twiddles(l->fy, &l->twy, &l->twy64)) return rc;
     if (int rc = twiddles(l->fz, &l->twz, &l->twz64)) return rc;
-    if (int rc = dalloc(&l->W, (size_t)3 * nz * l->fy * l->fxh)) return rc;
+    if (int rc = dalloc(&l->W, (size_t)3 * nz * l->fy * l->fxp)) return rc;
     if (int rc = dalloc(&l->Tw, (size_t)6 * l->fz * l->fy * l->fxh)) return rc;
     *out = l.release();
     return MSD_OK;
@@ -1464,6 +1465,24 @@ static void items3(int rep, const Dims& d, const Conv3Dev* cv, int ty, std::vect
         for (int k = 0; k < cv->fxh; k += cv->wz) zs.push_back(make_int4(rep, ky, k, 0));
 }
 
+// Workgroups go to the 8 XCDs round-robin by index.  Reorder a y or z pass's items
+// so that each XCD takes one contiguous stretch of the natural order (balanced to
+// within one item): neighbouring kx blocks, which share the cache lines at their
+// edges, then meet in one L2.  (Row tiles keep the natural order: measured faster.)
+static std::vector<int4> xcd_order(const std::vector<int4>& v)
+{
+    constexpr size_t parts = 8;
+    const size_t n = v.size();
+    std::vector<int4> out;
+    out.reserve(n);
+    for (size_t j = 0; out.size() < n; ++j)
+        for (size_t p = 0; p < parts; ++p) {
+            const size_t b = p * n / parts, e = (p + 1) * n / parts;
+            if (b + j < e) out.push_back(v[b + j]);
+        }
+    return out;
+}
+
 template <class T>
 static int upload_vec(T** dst, const std::vector<T>& v)
 {
@@ -1491,8 +1510,8 @@ static int lr3_field_of(msd_longrange3d* l, const float* S, float* H)
         std::vector<int4> rows, ys, zs;
         items3(0, l->d, &r.cv, l->ty, rows, ys, zs);
         if (int rc = upload_vec(&l->oitems_r, rows)) return rc;
-        if (int rc = upload_vec(&l->oitems_y, ys)) return rc;
-        if (int rc = upload_vec(&l->oitems_z, zs)) return rc;
+        if (int rc = upload_vec(&l->oitems_y, xcd_order(ys))) return rc;
+        if (int rc = upload_vec(&l->oitems_z, xcd_order(zs))) return rc;
         l->onr = (int)rows.size(); l->ony = (int)ys.size(); l->onz = (int)zs.size();
         if (int rc = dalloc(&l->orep, 1)) return rc;
     }
@@ -1582,11 +1601,11 @@ struct Batch3 {
                 rgroups[gi].n = (int)R[gi].size();
             }
             for (size_t gi = 0; gi < ygroups.size(); ++gi) {
-                if (int rc = upload_vec(&ygroups[gi].items, Y[gi])) return rc;
+                if (int rc = upload_vec(&ygroups[gi].items, xcd_order(Y[gi]))) return rc;
                 ygroups[gi].n = (int)Y[gi].size();
             }
             for (size_t gi = 0; gi < zgroups.size(); ++gi) {
-                if (int rc = upload_vec(&zgroups[gi].items, Z[gi])) return rc;
+                if (int rc = upload_vec(&zgroups[gi].items, xcd_order(Z[gi]))) return rc;
                 zgroups[gi].n = (int)Z[gi].size();
             }
             rg = rgroups;

@@ -154,6 +154,7 @@ hipError_t launch_step_corr(const StepArgs& a, hipStream_t st);
 constexpr int CONV3_LPAD = 2;
 struct Conv3Dev {
     int fx, fy, fz, fxh;
+    int fxp;                // row pitch of W in float2: fxh rounded up to 16 (128-byte rows)
     int ty;                 // rows per row tile
     int wy, wz;             // kx block widths of the y and z passes
     int zt;                 // z length of the compile-time z pass (0: the runtime-length pass; fz + 1: open split pass)
@@ -164,7 +165,7 @@ struct Conv3Dev {
     const float2* twx;
     const float2* twy;
     const float2* twz;
-    float2* W;              // working spectrum
+    float2* W;              // working spectrum [3][nz][fy][fxp], kx < fxh used
     const float* Tw;        // tensor words [6][fy][fxh][fzs] (real spectra), g/(fx fy fz) folded in
     float* hout;            // op-API target slot [3][n] (row kernel mode INV)
     const float* sin;       // op-API source spins (row kernel mode FWD)
