@@ -307,9 +307,10 @@ __global__ void __launch_bounds__(Y3_TPB) k3_ypass(const Rep3* __restrict__ reps
     }
 }
 
-// Compile-time y length FY (a power of two, 32..512), W = 16 kx columns per
-// workgroup, transformed in place (fft_ip): twice the runtime pass's width at the
-// same LDS, so every y row is read and written as 128-byte runs.
+// Compile-time y length FY (a power of two, 32..512), W = 8 kx columns per
+// workgroup, transformed in place (fft_ip) in half the runtime pass's LDS.  Every y
+// row is read and written as 64-byte runs; with W's 128-byte rows and XCD-local
+// items the two blocks of a line meet in one L2 (measured faster than W = 16).
 template <int SIGN, int FY, int W>
 __global__ void __launch_bounds__(Y3_TPB) k3_ypass_t(const Rep3* __restrict__ reps, const int4* __restrict__ items)
 {
@@ -345,18 +346,18 @@ __global__ void __launch_bounds__(Y3_TPB) k3_ypass_t(const Rep3* __restrict__ re
 
 int ypass_template_width(int fy)
 {
-    return (fy == 32 || fy == 64 || fy == 128 || fy == 256 || fy == 512) ? 16 : 0;
+    return (fy == 32 || fy == 64 || fy == 128 || fy == 256 || fy == 512) ? 8 : 0;
 }
 
 template <int SIGN>
 static const void* ypass_template(int fy)
 {
     switch (fy) {
-    case 32: return (const void*)k3_ypass_t<SIGN, 32, 16>;
-    case 64: return (const void*)k3_ypass_t<SIGN, 64, 16>;
-    case 128: return (const void*)k3_ypass_t<SIGN, 128, 16>;
-    case 256: return (const void*)k3_ypass_t<SIGN, 256, 16>;
-    case 512: return (const void*)k3_ypass_t<SIGN, 512, 16>;
+    case 32: return (const void*)k3_ypass_t<SIGN, 32, 8>;
+    case 64: return (const void*)k3_ypass_t<SIGN, 64, 8>;
+    case 128: return (const void*)k3_ypass_t<SIGN, 128, 8>;
+    case 256: return (const void*)k3_ypass_t<SIGN, 256, 8>;
+    case 512: return (const void*)k3_ypass_t<SIGN, 512, 8>;
     default: return nullptr;
     }
 }
@@ -365,11 +366,11 @@ template <int SIGN>
 static hipError_t launch_ypass_t(int yt, const Rep3* reps, const int4* items, int nitems, size_t lds, hipStream_t st)
 {
     switch (yt) {
-    case 32: k3_ypass_t<SIGN, 32, 16><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
-    case 64: k3_ypass_t<SIGN, 64, 16><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
-    case 128: k3_ypass_t<SIGN, 128, 16><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
-    case 256: k3_ypass_t<SIGN, 256, 16><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
-    case 512: k3_ypass_t<SIGN, 512, 16><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
+    case 32: k3_ypass_t<SIGN, 32, 8><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
+    case 64: k3_ypass_t<SIGN, 64, 8><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
+    case 128: k3_ypass_t<SIGN, 128, 8><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
+    case 256: k3_ypass_t<SIGN, 256, 8><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
+    case 512: k3_ypass_t<SIGN, 512, 8><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -550,11 +551,11 @@ __global__ void __launch_bounds__(NT) k3_zconv_o(const Rep3* __restrict__ reps, 
     }
 }
 
-// kx columns and threads per workgroup of the open split pass of half length nz:
-// 1024 threads at the longer lengths keep each thread's register share of the
-// lines small enough for 16 waves per CU
-static constexpr int zopen_width(int nz) { return nz >= 256 ? 8 : 16; }
-static constexpr int zopen_threads(int nz) { return nz >= 128 ? 1024 : Z3_TPB; }
+// kx columns and threads per workgroup of the open split pass of half length nz
+// (measured at nz = 128 and 256: narrow blocks, two to four workgroups per CU whose
+// memory phases overlap each other's transforms; a thread holds 6 of each line set)
+static constexpr int zopen_width(int nz) { return nz >= 128 ? 4 : 16; }
+static constexpr int zopen_threads(int nz) { return nz == 128 ? 256 : Z3_TPB; }
 template <int NZ>
 static const void* zopen_kernel() { return (const void*)k3_zconv_o<NZ, zopen_width(NZ), zopen_threads(NZ)>; }
 
