@@ -209,14 +209,14 @@ __global__ void __launch_bounds__(R3_TPB) k3_rows(const Rep3* __restrict__ reps,
     }
 }
 
-// Compile-time x length FX (a power of two, 32..512) and tile height TY: both x
+// Compile-time x length FX (32..512 powers of two, and 96) and tile height TY: both x
 // transforms run in place (fft_ip) with the twiddles staged in LDS, the spectrum
 // rows are loaded in batches, and the site loop has a fixed trip count.  A tile
 // at the lattice's y edge may hold fewer rows; its unused lines are zero.
 template <int FX>
 __host__ __device__ constexpr int rows_template_ty()
 {
-    return FX >= 512 ? 4 : FX == 256 ? 8 : FX == 128 ? 16 : 32;
+    return FX >= 512 ? 4 : FX >= 256 ? 8 : FX >= 96 ? 16 : 32;
 }
 
 template <int MODE, bool COMP, int FX>
@@ -244,11 +244,11 @@ __global__ void __launch_bounds__(R3_TPB) k3_rows_t(const Rep3* __restrict__ rep
     if (lr && MODE != R3_FWD) {
         batched_copy<LINES * FX, R3_TPB, 8>(
             [&](int idx) {
-                const int l = idx / FX, k = idx & (FX - 1);
+                const int l = idx / FX, k = idx % FX;
                 return l < lines ? rows_load_pair(cv, nz, z, y0, ty, l, k > HALF ? FX - k : k)
                                  : make_float4(0.f, 0.f, 0.f, 0.f);
             },
-            [&](int idx, float4 v) { A[idx] = rows_pack(v, (idx & (FX - 1)) > HALF); });
+            [&](int idx, float4 v) { A[idx] = rows_pack(v, idx % FX > HALF); });
         __syncthreads();
         fft_ip<FX, LINES, R3_TPB, 1>(A, s_tw);
     } else {
@@ -260,7 +260,7 @@ __global__ void __launch_bounds__(R3_TPB) k3_rows_t(const Rep3* __restrict__ rep
     constexpr int SITES = TY * FX;
 #pragma unroll 2
     for (int li = threadIdx.x; li < SITES; li += R3_TPB) {
-        const int r = li / FX, x = li & (FX - 1);
+        const int r = li / FX, x = li % FX;
         if (r < ty) rows_site<MODE, COMP>(rp, exl, A, FX, ty, r, x, y0, z, draw);
     }
     if (!lr || MODE == R3_INV) return;
@@ -272,7 +272,7 @@ __global__ void __launch_bounds__(R3_TPB) k3_rows_t(const Rep3* __restrict__ rep
     fft_ip<FX, LINES, R3_TPB, -1>(A, s_tw);
     for (int idx = threadIdx.x; idx < lines * FXH; idx += R3_TPB) {
         const int l = idx / FXH, k = idx - l * FXH;
-        rows_store_pair(cv, nz, z, y0, ty, l, k, A[l * FX + k], A[l * FX + ((FX - k) & (FX - 1))]);
+        rows_store_pair(cv, nz, z, y0, ty, l, k, A[l * FX + k], A[l * FX + (k ? FX - k : 0)]);
     }
 }
 
@@ -346,7 +346,7 @@ __global__ void __launch_bounds__(Y3_TPB) k3_ypass_t(const Rep3* __restrict__ re
 
 int ypass_template_width(int fy)
 {
-    return (fy == 32 || fy == 64 || fy == 128 || fy == 256 || fy == 512) ? 8 : 0;
+    return (fy == 32 || fy == 64 || fy == 96 || fy == 128 || fy == 256 || fy == 512) ? 8 : 0;
 }
 
 template <int SIGN>
@@ -355,6 +355,7 @@ static const void* ypass_template(int fy)
     switch (fy) {
     case 32: return (const void*)k3_ypass_t<SIGN, 32, 8>;
     case 64: return (const void*)k3_ypass_t<SIGN, 64, 8>;
+    case 96: return (const void*)k3_ypass_t<SIGN, 96, 8>;
     case 128: return (const void*)k3_ypass_t<SIGN, 128, 8>;
     case 256: return (const void*)k3_ypass_t<SIGN, 256, 8>;
     case 512: return (const void*)k3_ypass_t<SIGN, 512, 8>;
@@ -368,6 +369,7 @@ static hipError_t launch_ypass_t(int yt, const Rep3* reps, const int4* items, in
     switch (yt) {
     case 32: k3_ypass_t<SIGN, 32, 8><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
     case 64: k3_ypass_t<SIGN, 64, 8><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
+    case 96: k3_ypass_t<SIGN, 96, 8><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
     case 128: k3_ypass_t<SIGN, 128, 8><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
     case 256: k3_ypass_t<SIGN, 256, 8><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
     case 512: k3_ypass_t<SIGN, 512, 8><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
@@ -412,7 +414,7 @@ __global__ void __launch_bounds__(Z3_TPB) k3_zconv(const Rep3* __restrict__ reps
     }
 }
 
-// Compile-time z length FZ (a power of two, 32..512), W kx columns per workgroup:
+// Compile-time z length FZ (16..512 powers of two, and 24), W kx columns per workgroup:
 // the three components' z lines (3 W x FZ) transformed in place in one LDS buffer
 // (fft_ip), so W is 4x the runtime pass's at the same LDS and each z plane is read
 // and written as runs of W consecutive kx (64-128 bytes instead of 16).
@@ -443,7 +445,7 @@ __global__ void __launch_bounds__(Z3_TPB) k3_zconv_t(const Rep3* __restrict__ re
     fft_ip<FZ, lines, Z3_TPB, -1, LS>(A, cv.twz);
     // kz fastest: the tensor words and the LDS lines are both read in runs
     for (int idx = threadIdx.x; idx < W * FZ; idx += Z3_TPB) {
-        const int j = idx / FZ, kz = idx & (FZ - 1);
+        const int j = idx / FZ, kz = idx % FZ;
         if (j < w)
             tensor_apply(tensor_words(cv, ky, kx0 + j, kz), A[(0 * W + j) * LS + kz], A[(1 * W + j) * LS + kz],
                          A[(2 * W + j) * LS + kz]);
@@ -562,7 +564,7 @@ static const void* zopen_kernel() { return (const void*)k3_zconv_o<NZ, zopen_wid
 int zconv_template_width(int fz)
 {
     switch (fz) {
-    case 32: case 64: case 128: case 256: return 16;
+    case 16: case 24: case 32: case 64: case 128: case 256: return 16;
     case 512: return 8;
     default: return 0;
     }
@@ -584,6 +586,8 @@ int zconv_open_width(int fz)
 static const void* zconv_template(int zt)
 {
     switch (zt) {
+    case 16: return (const void*)k3_zconv_t<16, 16>;
+    case 24: return (const void*)k3_zconv_t<24, 16>;
     case 32: return (const void*)k3_zconv_t<32, 16>;
     case 64: return (const void*)k3_zconv_t<64, 16>;
     case 128: return (const void*)k3_zconv_t<128, 16>;
@@ -604,6 +608,7 @@ int rows_template_height(int fx)
     switch (fx) {
     case 32: return rows_template_ty<32>();
     case 64: return rows_template_ty<64>();
+    case 96: return rows_template_ty<96>();
     case 128: return rows_template_ty<128>();
     case 256: return rows_template_ty<256>();
     case 512: return rows_template_ty<512>();
@@ -618,6 +623,7 @@ static const void* rows_kernel(int rt)
     case 0: return (const void*)k3_rows<MODE, COMP>;
     case 32: return (const void*)k3_rows_t<MODE, COMP, 32>;
     case 64: return (const void*)k3_rows_t<MODE, COMP, 64>;
+    case 96: return (const void*)k3_rows_t<MODE, COMP, 96>;
     case 128: return (const void*)k3_rows_t<MODE, COMP, 128>;
     case 256: return (const void*)k3_rows_t<MODE, COMP, 256>;
     case 512: return (const void*)k3_rows_t<MODE, COMP, 512>;
@@ -674,14 +680,15 @@ hipError_t configure_lds3(size_t max_bytes)
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, v);
         if (e != hipSuccess && first == hipSuccess) first = e;
     }
-    for (int rt : {0, 32, 64, 128, 256, 512})
+    for (int rt : {0, 32, 64, 96, 128, 256, 512})
         for (int mode : {R3_FWD, R3_INV, R3_PRED, R3_CORR})
             for (bool comp : {false, true}) {
                 hipError_t e = hipFuncSetAttribute(rows_kernel(mode, comp, rt), hipFuncAttributeMaxDynamicSharedMemorySize, v);
                 if (e != hipSuccess && first == hipSuccess) first = e;
             }
-    for (int f : {32, 64, 128, 256, 512})
+    for (int f : {16, 24, 32, 64, 96, 128, 256, 512})
         for (const void* k : {zconv_template(f), zconv_template(f + 1), ypass_template<-1>(f), ypass_template<1>(f)}) {
+            if (!k) continue;    // no instance of that pass at this length
             hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, v);
             if (e != hipSuccess && first == hipSuccess) first = e;
         }

@@ -22,9 +22,10 @@ def oracle_lr3(dims, open_, R, g):
     return lr
 
 
-# z lengths 32 and 64 take the compile-time z pass (partial kx blocks included)
+# z lengths 16, 24, 32 and 64 take the compile-time z pass (partial kx blocks
+# included), x length 96 the compile-time row pass, y length 96 the compile-time y pass
 @pytest.mark.parametrize("dims,R", [((8, 8, 8), 12), ((12, 10, 6), 9), ((16, 8, 4), 10), ((6, 6, 20), 8),
-                                    ((8, 6, 32), 9), ((20, 4, 64), 8)])
+                                    ((8, 6, 32), 9), ((20, 4, 64), 8), ((96, 16, 16), 9), ((8, 96, 24), 8)])
 def test_longrange3d_periodic_op(dims, R):
     n = dims[0] * dims[1] * dims[2]
     s = np.random.default_rng(sum(dims)).standard_normal((3, n))
@@ -172,7 +173,7 @@ def test_heun_open_longrange3d():
 
 def test_ensemble_matches_single_replicas():
     """Uneven replicas stepped by shared launches give exactly the single-replica results
-    (the z lengths 32 and 64 take the compile-time z pass, the others the runtime one)."""
+    (the z lengths 16, 32 and 64 take the compile-time z pass, the others the runtime one)."""
     cfg = [((16, 16, 16), 0.3, 1), ((12, 10, 6), 0.8, 2), ((8, 8, 8), 1.5, 3), ((10, 6, 4), 0.0, 4),
            ((8, 8, 32), 0.5, 5), ((6, 4, 64), 0.2, 6)]
     ens, solo = [], []
