@@ -37,6 +37,7 @@ typedef struct msd_system     msd_system;
 typedef struct msd_exchange   msd_exchange;
 typedef struct msd_anisotropy msd_anisotropy;
 typedef struct msd_thermal    msd_thermal;
+typedef struct msd_appliedfield msd_appliedfield;
 typedef struct msd_dipole     msd_dipole;
 typedef struct msd_llg        msd_llg;
 typedef struct msd_stepper    msd_stepper;
@@ -130,6 +131,24 @@ int msd_anisotropy_add_many(msd_anisotropy* a, const int* site, const double* ax
 /* ani:apply(ss): Anisotropy::apply, spinoperationanisotropy.cpp:247-284, slot "Anisotropy" */
 int msd_anisotropy_apply(msd_anisotropy* a, msd_system* s);
 
+/* ------------------------------------------------------------ AppliedField
+ * AppliedField.new(n) (cpu/appliedfield/spinoperationappliedfield.cpp:18-25; x y z
+ * setX setY setZ :127-190, set/add/get in appliedfield_luafuncs.lua:7-40) and,
+ * heterogeneous = 1, AppliedField.Heterogeneous
+ * (spinoperationappliedfield_heterogeneous.cpp; per-site set/get,
+ * appliedfield_heterogeneous_luafuncs.lua:7-31).  Sites are 0-based here. */
+int msd_appliedfield_create(int nx, int ny, int nz, int heterogeneous, msd_appliedfield** out);
+int msd_appliedfield_destroy(msd_appliedfield* a);
+int msd_appliedfield_set(msd_appliedfield* a, double x, double y, double z);
+int msd_appliedfield_get(const msd_appliedfield* a, double* xyz);
+int msd_appliedfield_set_scale(msd_appliedfield* a, double scale);
+int msd_appliedfield_set_site(msd_appliedfield* a, int site, double x, double y, double z);
+int msd_appliedfield_get_site(const msd_appliedfield* a, int site, double* xyz);
+/* af:apply(ss): uniform, the Lua apply (appliedfield_luafuncs.lua:46-57) sets slot
+ * "Zeeman" to B*scale; heterogeneous (spinoperationappliedfield_heterogeneous.cpp:88-99)
+ * adds scale*h to slot "Zeeman_Heterogeneous" */
+int msd_appliedfield_apply(msd_appliedfield* a, msd_system* s);
+
 /* ----------------------------------------------------------------- Thermal */
 int msd_thermal_create(int nx, int ny, int nz, msd_thermal** out);
 int msd_thermal_destroy(msd_thermal* t);
@@ -215,6 +234,9 @@ int msd_stepper_set_compensated(msd_stepper* st, int yes);
 /* Use a LongRange3D as the long-range field of this step driver (NULL removes it);
  * not combinable with a Dipole. */
 int msd_stepper_set_longrange3d(msd_stepper* st, msd_longrange3d* l);
+/* Add an applied field to the fused step's field sum (a = NULL removes all);
+ * changes made to it later are picked up by the next msd_stepper_heun */
+int msd_stepper_add_applied_field(msd_stepper* st, msd_appliedfield* a);
 /* Advance nsteps Heun steps; asynchronous. */
 int msd_stepper_heun(msd_stepper* st, int nsteps);
 /* Kernel-level timing on the library stream, names[i]/avg_ms[i] for up to cap

@@ -88,6 +88,15 @@ _SIGS = {
     "msd_anisotropy_add": (C.c_int, [_vp, C.c_int, _dp, C.c_double, _ip]),
     "msd_anisotropy_add_many": (C.c_int, [_vp, _ip, _dp, _dp, C.c_int]),
     "msd_anisotropy_apply": (C.c_int, [_vp, _vp]),
+    "msd_appliedfield_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _pp]),
+    "msd_appliedfield_destroy": (C.c_int, [_vp]),
+    "msd_appliedfield_set": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double]),
+    "msd_appliedfield_get": (C.c_int, [_vp, _dp]),
+    "msd_appliedfield_set_scale": (C.c_int, [_vp, C.c_double]),
+    "msd_appliedfield_set_site": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_double]),
+    "msd_appliedfield_get_site": (C.c_int, [_vp, C.c_int, _dp]),
+    "msd_appliedfield_apply": (C.c_int, [_vp, _vp]),
+    "msd_stepper_add_applied_field": (C.c_int, [_vp, _vp]),
     "msd_thermal_create": (C.c_int, [C.c_int, C.c_int, C.c_int, _pp]),
     "msd_thermal_destroy": (C.c_int, [_vp]),
     "msd_thermal_set_temperature": (C.c_int, [_vp, C.c_double]),

@@ -26,6 +26,18 @@ hipError_t launch_fill(float* p, float v, size_t n, hipStream_t st)
     return hipGetLastError();
 }
 
+__global__ void k_axpy(float* y, float a, const float* __restrict__ x, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        y[i] += a * x[i];
+}
+
+hipError_t launch_axpy(float* y, float a, const float* x, size_t n, hipStream_t st)
+{
+    k_axpy<<<grid_for(n), TPB, 0, st>>>(y, a, x, n);
+    return hipGetLastError();
+}
+
 // SpinSystem::sumFields: total = sum of the used slots, in slot order
 struct SrcList { const float* p[16]; };
 __global__ void k_sum_fields(float* total, SrcList src, int nsrc, size_t n3)

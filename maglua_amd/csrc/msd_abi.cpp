@@ -797,6 +797,106 @@ extern "C" int msd_anisotropy_apply(msd_anisotropy* a, msd_system* s)
     return MSD_OK;
 }
 
+// ============================================================= AppliedField
+// AppliedField (cpu/appliedfield/spinoperationappliedfield.cpp): its Lua apply
+// (appliedfield_luafuncs.lua:46-57) overwrites slot "Zeeman" with B*scale at every
+// site.  AppliedField.Heterogeneous::apply
+// (spinoperationappliedfield_heterogeneous.cpp:88-99) adds scale*h(site) to slot
+// "Zeeman_Heterogeneous".  The step driver folds either into its site kernels.
+struct msd_appliedfield {
+    Dims d{};
+    bool het = false;
+    double B[3] = {0, 0, 0};
+    double scale = 1;
+    std::vector<float> h;    // [3][n], heterogeneous form
+    float* dh = nullptr;     // device copy of h
+    bool dirty = false;
+    ~msd_appliedfield()
+    {
+        (void)hipStreamSynchronize(G.stream);
+        dfree(dh);
+    }
+    int upload()
+    {
+        if (!het || !dirty) return MSD_OK;
+        if (!dh)
+            if (int rc = dalloc(&dh, h.size())) return rc;
+        HIPCHK(hipMemcpyAsync(dh, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice, G.stream));
+        HIPCHK(hipStreamSynchronize(G.stream));
+        dirty = false;
+        return MSD_OK;
+    }
+};
+
+extern "C" int msd_appliedfield_create(int nx, int ny, int nz, int heterogeneous, msd_appliedfield** out)
+{
+    NONNULL(out);
+    *out = nullptr;
+    if (nx < 1 || ny < 1 || nz < 1) return fail(MSD_EINVAL, "AppliedField.new: bad size");
+    if (int rc = ensure_init()) return rc;
+    std::unique_ptr<msd_appliedfield> a(new msd_appliedfield);
+    a->d = Dims{nx, ny, nz, nx * ny * nz};
+    a->het = heterogeneous != 0;
+    if (a->het) {
+        a->h.assign((size_t)3 * a->d.n, 0.0f);
+        a->dirty = true;
+    }
+    *out = a.release();
+    return MSD_OK;
+}
+extern "C" int msd_appliedfield_destroy(msd_appliedfield* a) { delete a; return MSD_OK; }
+extern "C" int msd_appliedfield_set(msd_appliedfield* a, double x, double y, double z)
+{
+    NONNULL(a);
+    if (a->het) return fail(MSD_EINVAL, "AppliedField.Heterogeneous: the field is set per site");
+    a->B[0] = x; a->B[1] = y; a->B[2] = z;
+    return MSD_OK;
+}
+extern "C" int msd_appliedfield_get(const msd_appliedfield* a, double* xyz)
+{
+    NONNULL(a); NONNULL(xyz);
+    for (int c = 0; c < 3; ++c) xyz[c] = a->B[c];
+    return MSD_OK;
+}
+extern "C" int msd_appliedfield_set_scale(msd_appliedfield* a, double s) { NONNULL(a); a->scale = s; return MSD_OK; }
+extern "C" int msd_appliedfield_set_site(msd_appliedfield* a, int site, double x, double y, double z)
+{
+    NONNULL(a);
+    if (!a->het) return fail(MSD_EINVAL, "AppliedField: the field is uniform (AppliedField.Heterogeneous sets sites)");
+    if (site < 0 || site >= a->d.n) return fail(MSD_EINVAL, "AppliedField.Heterogeneous: site %d outside the lattice", site);
+    const size_t n = a->d.n;
+    a->h[site] = (float)x; a->h[n + site] = (float)y; a->h[2 * n + site] = (float)z;
+    a->dirty = true;
+    return MSD_OK;
+}
+extern "C" int msd_appliedfield_get_site(const msd_appliedfield* a, int site, double* xyz)
+{
+    NONNULL(a); NONNULL(xyz);
+    if (!a->het) return fail(MSD_EINVAL, "AppliedField: the field is uniform");
+    if (site < 0 || site >= a->d.n) return fail(MSD_EINVAL, "AppliedField.Heterogeneous: site %d outside the lattice", site);
+    for (int c = 0; c < 3; ++c) xyz[c] = a->h[(size_t)c * a->d.n + site];
+    return MSD_OK;
+}
+extern "C" int msd_appliedfield_apply(msd_appliedfield* a, msd_system* s)
+{
+    NONNULL(a); NONNULL(s);
+    if (!same_dims(s->d, a->d.nx, a->d.ny, a->d.nz)) return fail(MSD_EMISMATCH, "AppliedField:apply: size mismatch");
+    const size_t n = s->d.n;
+    int k;
+    hipError_t e = hipSuccess;
+    if (a->het) {
+        if (int rc = a->upload()) return rc;
+        if (int rc = s->mark("Zeeman_Heterogeneous", &k)) return rc;
+        e = launch_axpy(s->slots[k].h, (float)a->scale, a->dh, 3 * n, G.stream);
+    } else {
+        if (int rc = s->mark("Zeeman", &k)) return rc;
+        for (int c = 0; c < 3 && e == hipSuccess; ++c)
+            e = launch_fill(s->slots[k].h + c * n, (float)(a->B[c] * a->scale), n, G.stream);
+    }
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "AppliedField:apply: %s", hipGetErrorString(e));
+    return MSD_OK;
+}
+
 // =================================================================== Thermal
 struct msd_thermal {
     Dims d{};
@@ -2088,6 +2188,8 @@ struct msd_stepper {
     int sur_K = 10;
     double sur_tol = 1e-3;
     float *Href = nullptr, *Mref = nullptr, *Hext = nullptr, *Hx = nullptr;   // [3][n]
+    std::vector<msd_appliedfield*> af;                                        // applied fields
+    float* Happ = nullptr;                                                    // [3][n] their per-site sum
     bool have_ref = false, sur_fallback = false;
     long sur_windows = 0, sur_exact_windows = 0, sur_fallbacks = 0;
     double sur_last_res = 0.0, sur_max_res = 0.0;
@@ -2101,7 +2203,7 @@ struct msd_stepper {
         (void)hipStreamSynchronize(G.stream);
         drop_graph();
         dfree(P); dfree(C); dfree(Clo); dfree(draw);
-        dfree(Href); dfree(Mref); dfree(Hext); dfree(Hx);
+        dfree(Href); dfree(Mref); dfree(Hext); dfree(Hx); dfree(Happ);
     }
     void drop_graph()
     {
@@ -2163,6 +2265,30 @@ int msd_stepper::refresh()
     } else {
         a.ty = 1;
         for (int c : {8, 4, 2}) if (s->d.ny % c == 0) { a.ty = c; break; }
+    }
+    if (!af.empty()) {
+        // uniform fields fold into one vector; per-site ones into one [3][n] sum
+        double B[3] = {0, 0, 0};
+        bool het = false;
+        for (msd_appliedfield* f : af) {
+            if (f->het) het = true;
+            else for (int c = 0; c < 3; ++c) B[c] += f->B[c] * f->scale;
+        }
+        a.has_app = 1;
+        a.Bapp = make_float3((float)B[0], (float)B[1], (float)B[2]);
+        if (het) {
+            const size_t n3 = (size_t)3 * s->d.n;
+            if (!Happ)
+                if (int rc = dalloc(&Happ, n3)) return rc;
+            hipError_t e = launch_fill(Happ, 0.0f, n3, G.stream);
+            for (msd_appliedfield* f : af) {
+                if (!f->het) continue;
+                if (int rc = f->upload()) return rc;
+                if (e == hipSuccess) e = launch_axpy(Happ, (float)f->scale, f->dh, n3, G.stream);
+            }
+            if (e != hipSuccess) return fail(MSD_EDEVICE, "stepper: applied fields: %s", hipGetErrorString(e));
+            a.Happ = Happ;
+        }
     }
     a.draw = draw;
     args = a;
@@ -2501,6 +2627,20 @@ extern "C" int msd_stepper_autotune(msd_stepper* st, int steps, int* best_ty, in
     if (!rc) rc = st->refresh();
     st->spec_version = 0;
     return rc;
+}
+
+extern "C" int msd_stepper_add_applied_field(msd_stepper* st, msd_appliedfield* a)
+{
+    NONNULL(st);
+    if (!a) {
+        st->af.clear();
+        return MSD_OK;
+    }
+    if (!same_dims(st->s->d, a->d.nx, a->d.ny, a->d.nz)) return fail(MSD_EMISMATCH, "stepper: AppliedField size differs");
+    for (msd_appliedfield* f : st->af)
+        if (f == a) return MSD_OK;
+    st->af.push_back(a);
+    return MSD_OK;
 }
 
 extern "C" int msd_stepper_set_surrogate(msd_stepper* st, msd_surrogate* sur, int K, double tol)

@@ -577,6 +577,10 @@ __device__ __forceinline__ float3 pred_finish(const StepArgs& a, const ExchangeD
     H.x += ha.x + Hd.x; H.y += ha.y + Hd.y; H.z += ha.z + Hd.z;
     if (a.Hext) { H.x += ldg(a.Hext, u); H.y += ldg(a.Hext + n, u); H.z += ldg(a.Hext + 2 * (size_t)n, u); }
     if (a.Href) { H.x += ldg(a.Href, u); H.y += ldg(a.Href + n, u); H.z += ldg(a.Href + 2 * (size_t)n, u); }
+    if (a.has_app) {
+        H.x += a.Bapp.x; H.y += a.Bapp.y; H.z += a.Bapp.z;
+        if (a.Happ) { H.x += ldg(a.Happ, u); H.y += ldg(a.Happ + n, u); H.z += ldg(a.Happ + 2 * (size_t)n, u); }
+    }
     const float dt = a.dt, hdt = 0.5f * a.dt;
     float3 p = s, c = s;
     if (ms > 0.0f) {
@@ -642,6 +646,10 @@ __device__ __forceinline__ float3 corr_finish(const StepArgs& a, const ExchangeD
     H.x += ha.x + Hd.x; H.y += ha.y + Hd.y; H.z += ha.z + Hd.z;
     if (a.Hext) { H.x += ldg(a.Hext, u); H.y += ldg(a.Hext + n, u); H.z += ldg(a.Hext + 2 * (size_t)n, u); }
     if (a.Href) { H.x += ldg(a.Href, u); H.y += ldg(a.Href + n, u); H.z += ldg(a.Href + 2 * (size_t)n, u); }
+    if (a.has_app) {
+        H.x += a.Bapp.x; H.y += a.Bapp.y; H.z += a.Bapp.z;
+        if (a.Happ) { H.x += ldg(a.Happ, u); H.y += ldg(a.Happ + n, u); H.z += ldg(a.Happ + 2 * (size_t)n, u); }
+    }
     const float dt = a.dt, hdt = 0.5f * a.dt;
     const float al = a.lp.site_alpha ? ldg(a.lp.site_alpha, u) : a.lp.alpha;
     const float ga = a.lp.site_gamma ? ldg(a.lp.site_gamma, u) : a.lp.gamma;

@@ -89,6 +89,7 @@ struct Dims {
 // ------------------------------------------------------------ launches
 // All launches go to stream st and return hipGetLastError().
 hipError_t launch_fill(float* p, float v, size_t n, hipStream_t st);
+hipError_t launch_axpy(float* y, float a, const float* x, size_t n, hipStream_t st);   // y += a x
 hipError_t launch_sum_fields(float* total, const float* const* src, int nsrc, size_t n3, hipStream_t st);
 hipError_t launch_exchange_op(const Dims& d, const ExchangeDev& ex, const float* s, float* h, hipStream_t st);
 hipError_t launch_anisotropy_op(const Dims& d, const AnisoDev& an, const float* s, float* h, hipStream_t st);
@@ -141,6 +142,9 @@ struct StepArgs {
     int ty;                 // rows per tile when there is no dipole
     const float* Hext;      // [3][n] field added at every site (the surrogate's dipole field) or nullptr
     const float* Href;      // [3][n] second such field (the surrogate's reference field) or nullptr
+    int has_app;            // applied fields (AppliedField): uniform Bapp plus per-site Happ
+    float3 Bapp;
+    const float* Happ;      // [3][n] or nullptr
 };
 hipError_t launch_step_pred(const StepArgs& a, hipStream_t st);
 hipError_t launch_step_corr(const StepArgs& a, hipStream_t st);

@@ -342,6 +342,129 @@ class Anisotropy(_Handle):
             call("msd_anisotropy_apply", self._h, s.h)
 
 
+class _AppliedFieldHeterogeneous(_Handle):
+    """AppliedField.Heterogeneous (cpu/appliedfield/spinoperationappliedfield_heterogeneous.cpp):
+    a field per site, added (times scale) to slot 'Zeeman_Heterogeneous' by apply."""
+
+    _destroy = "msd_appliedfield_destroy"
+
+    def __init__(self, *a):
+        self._dims = _dims_of(a)
+        p = _vp()
+        call("msd_appliedfield_create", *self._dims, 1, C.byref(p))
+        super().__init__(p.value)
+        self._scale = 1.0
+
+    new = classmethod(lambda cls, *a: cls(*a))
+
+    def set(self, p, h):
+        """set({x,y,z} 1-based site, {hx,hy,hz}) (appliedfield_heterogeneous_luafuncs.lua:7-18)."""
+        if not isinstance(p, (tuple, list)):
+            raise MagLuaError(-1, "first argument needs to be a table")
+        if not isinstance(h, (tuple, list)):
+            raise MagLuaError(-1, "second argument needs to be a table")
+        call("msd_appliedfield_set_site", self._h, _index(p, self._dims), float(h[0]), float(h[1]), float(h[2]))
+
+    def get(self, p):
+        if not isinstance(p, (tuple, list)):
+            raise MagLuaError(-1, "first argument needs to be a table")
+        v = (C.c_double * 3)()
+        call("msd_appliedfield_get_site", self._h, _index(p, self._dims), v)
+        return [v[0], v[1], v[2]]
+
+    def setScale(self, s):
+        self._scale = float(s)
+        call("msd_appliedfield_set_scale", self._h, self._scale)
+
+    def scale(self):
+        return self._scale
+
+    def slotName(self):
+        return "Zeeman_Heterogeneous"
+
+    def apply(self, ss):
+        for s in (ss if isinstance(ss, (list, tuple)) else (ss,)):
+            call("msd_appliedfield_apply", self._h, s.h)
+
+
+class AppliedField(_Handle):
+    """AppliedField (cpu/appliedfield/spinoperationappliedfield.cpp and its Lua methods,
+    appliedfield_luafuncs.lua): a uniform field; apply sets slot 'Zeeman' to B*scale."""
+
+    _destroy = "msd_appliedfield_destroy"
+    Heterogeneous = _AppliedFieldHeterogeneous
+
+    def __init__(self, *a):
+        self._dims = _dims_of(a)
+        p = _vp()
+        call("msd_appliedfield_create", *self._dims, 0, C.byref(p))
+        super().__init__(p.value)
+        self._scale = 1.0
+
+    new = classmethod(lambda cls, *a: cls(*a))
+
+    def _B(self):
+        v = (C.c_double * 3)()
+        call("msd_appliedfield_get", self._h, v)
+        return [v[0], v[1], v[2]]
+
+    def _setB(self, b):
+        call("msd_appliedfield_set", self._h, float(b[0]), float(b[1]), float(b[2]))
+
+    def x(self):
+        return self._B()[0]
+
+    def y(self):
+        return self._B()[1]
+
+    def z(self):
+        return self._B()[2]
+
+    def setX(self, v):
+        b = self._B(); b[0] = v; self._setB(b)
+
+    def setY(self, v):
+        b = self._B(); b[1] = v; self._setB(b)
+
+    def setZ(self, v):
+        b = self._B(); b[2] = v; self._setB(b)
+
+    def set(self, x=None, y=None, z=None):
+        """set({x,y,z}) or set(x,y,z); missing components are 0 (appliedfield_luafuncs.lua:7-20)."""
+        if isinstance(x, (tuple, list)):
+            t = list(x) + [None] * 3
+            x, y, z = (t[0] or 0), (t[1] or 0), (t[2] or 0)
+        self._setB([x or 0, y or 0, z or 0])
+
+    def add(self, x, y=None, z=None):
+        """add({x,y,z}) or add(x,y,z) (appliedfield_luafuncs.lua:23-32)."""
+        if isinstance(x, (tuple, list)):
+            t = list(x) + [None] * 3
+            x, y, z = (t[0] or 0), (t[1] or 0), (t[2] or 0)
+        b = self._B()
+        self._setB([b[0] + x, b[1] + y, b[2] + z])
+
+    def get(self):
+        return tuple(self._B())
+
+    def toTable(self):
+        return self._B()
+
+    def setScale(self, s):
+        self._scale = float(s)
+        call("msd_appliedfield_set_scale", self._h, self._scale)
+
+    def scale(self):
+        return self._scale
+
+    def slotName(self):
+        return "Zeeman"
+
+    def apply(self, ss):
+        for s in (ss if isinstance(ss, (list, tuple)) else (ss,)):
+            call("msd_appliedfield_apply", self._h, s.h)
+
+
 class Random:
     """Seed holder standing in for Random.Isaac/MersenneTwister: the build's noise is
     counter-based (Philox4x32-10), so an RNG object only carries the seed."""
@@ -723,6 +846,16 @@ class Heun(_Handle):
         return True
 
     __call__ = step
+
+    def addAppliedField(self, af):
+        """Add an AppliedField or AppliedField.Heterogeneous to the step's field sum (None
+        removes them all); later changes to it apply from the next step() call."""
+        if af is None:
+            self._af = []
+            call("msd_stepper_add_applied_field", self._h, None)
+            return
+        self._af = getattr(self, "_af", []) + [af]
+        call("msd_stepper_add_applied_field", self._h, af.h)
 
     def setSurrogate(self, sur, K=10, tol=1e-3):
         """Learned dipole field: windows of K steps, each started by an exact field evaluation

@@ -23,6 +23,11 @@
  *   ora_ani_*              Anisotropy::apply and l_add,
  *                          cpu/anisotropy/spinoperationanisotropy.cpp:247-284,367-413.
  *                          Pinned by the closed form 2K(s.n)n/ms^2 (tests).
+ *   ora_af_*               AppliedField (Lua apply, cpu/appliedfield/appliedfield_luafuncs.lua:46-57)
+ *                          and AppliedField.Heterogeneous::apply
+ *                          (spinoperationappliedfield_heterogeneous.cpp:88-99).
+ *                          Pinned by the field values and by the closed-form
+ *                          precession of a spin in a uniform field (tests).
  *   ora_th_*               Thermal::apply, cpu/thermal/spinoperationthermal.cpp:135-191.
  *                          Pinned by sigma^2 variance and the Langevin paramagnet (tests).
  *   ora_lr_*               LongRange::updateData/apply,
@@ -631,6 +636,59 @@ int ora_ani_apply(ora_ani* a, ora_sys* s)
 } /* extern "C" */
 
 /* ------------------------------------------------------------------------ */
+/* AppliedField                                                              */
+/* ------------------------------------------------------------------------ */
+/* Uniform: the Lua apply (cpu/appliedfield/appliedfield_luafuncs.lua:46-57)
+ * overwrites slot "Zeeman" with B*scale at every site.  Heterogeneous
+ * (spinoperationappliedfield_heterogeneous.cpp:88-99): slot
+ * "Zeeman_Heterogeneous" += scale*h(site), the per-site arrays zero until set. */
+struct ora_af {
+    int n = 0;
+    bool het = false;
+    double B[3] = {0, 0, 0};
+    double scale = 1;
+    std::vector<double> h[3];
+};
+
+extern "C" {
+ora_af* ora_af_new(int nx, int ny, int nz, int heterogeneous)
+{
+    ora_af* a = new ora_af;
+    a->n = nx * ny * nz;
+    a->het = heterogeneous != 0;
+    if (a->het)
+        for (auto& v : a->h) v.assign(a->n, 0.0);
+    return a;
+}
+void ora_af_free(ora_af* a) { delete a; }
+void ora_af_set(ora_af* a, double x, double y, double z) { a->B[0] = x; a->B[1] = y; a->B[2] = z; }
+void ora_af_set_scale(ora_af* a, double s) { a->scale = s; }
+int ora_af_set_site(ora_af* a, int site, double x, double y, double z)
+{
+    if (!a->het || site < 0 || site >= a->n) return -1;
+    a->h[0][site] = x; a->h[1][site] = y; a->h[2][site] = z;
+    return 0;
+}
+int ora_af_apply(ora_af* a, ora_sys* s)
+{
+    if (a->het) {
+        const int k = s->mark("Zeeman_Heterogeneous");
+        for (int i = 0; i < a->n; ++i) {
+            s->hx[k][i] += a->scale * a->h[0][i];
+            s->hy[k][i] += a->scale * a->h[1][i];
+            s->hz[k][i] += a->scale * a->h[2][i];
+        }
+    } else {
+        const int k = s->mark("Zeeman");
+        std::fill(s->hx[k].begin(), s->hx[k].end(), a->B[0] * a->scale);
+        std::fill(s->hy[k].begin(), s->hy[k].end(), a->B[1] * a->scale);
+        std::fill(s->hz[k].begin(), s->hz[k].end(), a->B[2] * a->scale);
+    }
+    return 0;
+}
+} /* extern "C" */
+
+/* ------------------------------------------------------------------------ */
 /* Thermal                                                                   */
 /* ------------------------------------------------------------------------ */
 struct ora_th {
@@ -1095,6 +1153,7 @@ struct ora_sim {
     ora_sys* ss;
     ora_ex* ex; ora_ani* ani; ora_lr* lr; ora_th* th; ora_llg* llg;
     ora_lr3* lr3 = nullptr;
+    std::vector<ora_af*> af;
     uint64_t seed = 0, step = 0;
     ora_sys* P = nullptr;
     ora_sys* C = nullptr;
@@ -1118,6 +1177,7 @@ void ora_sim_free(ora_sim* s)
 
 void ora_sim_set_seed(ora_sim* s, uint64_t seed) { s->seed = seed; s->step = 0; }
 void ora_sim_set_lr3(ora_sim* s, ora_lr3* l) { s->lr3 = l; }
+void ora_sim_add_af(ora_sim* s, ora_af* a) { s->af.push_back(a); }
 
 void ora_sim_set_threads(int n)
 {
@@ -1128,7 +1188,7 @@ void ora_sim_set_threads(int n)
 #endif
 }
 
-/* calcField: resetFields; ex, ani, dip apply; sumFields */
+/* calcField: resetFields; ex, ani, dip, applied fields apply; sumFields */
 int ora_sim_calc_fields(ora_sim* s, ora_sys* t)
 {
     ora_sys_reset_fields(t);
@@ -1136,6 +1196,7 @@ int ora_sim_calc_fields(ora_sim* s, ora_sys* t)
     if (s->ani) ora_ani_apply(s->ani, t);
     if (s->lr) ora_lr_apply(s->lr, t);
     if (s->lr3) ora_lr3_apply(s->lr3, t);
+    for (ora_af* a : s->af) ora_af_apply(a, t);
     ora_sys_sum_fields(t);
     return 0;
 }

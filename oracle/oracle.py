@@ -74,6 +74,13 @@ def lib(threads=False):
         "ora_ani_set_scale": (None, [vp, C.c_double]),
         "ora_ani_add": (C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]),
         "ora_ani_apply": (C.c_int, [vp, vp]),
+        "ora_af_new": (vp, [C.c_int, C.c_int, C.c_int, C.c_int]),
+        "ora_af_free": (None, [vp]),
+        "ora_af_set": (None, [vp, C.c_double, C.c_double, C.c_double]),
+        "ora_af_set_scale": (None, [vp, C.c_double]),
+        "ora_af_set_site": (C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_double]),
+        "ora_af_apply": (C.c_int, [vp, vp]),
+        "ora_sim_add_af": (None, [vp, vp]),
         "ora_th_new": (vp, [C.c_int, C.c_int, C.c_int]),
         "ora_th_free": (None, [vp]),
         "ora_th_set_temperature": (None, [vp, C.c_double]),
@@ -324,6 +331,31 @@ class Anisotropy:
         self.L.ora_ani_apply(self.h, ss.h)
 
 
+class AppliedField:
+    """Uniform applied field (slot "Zeeman", overwritten by apply) or, with
+    heterogeneous=True, per-site fields (slot "Zeeman_Heterogeneous", accumulated)."""
+
+    def __init__(self, nx, ny=1, nz=1, heterogeneous=False, threads=False):
+        self.L = lib(threads)
+        self.h = self.L.ora_af_new(nx, ny, nz, int(heterogeneous))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.ora_af_free(self.h)
+
+    def set(self, x, y=0.0, z=0.0):
+        self.L.ora_af_set(self.h, float(x), float(y), float(z))
+
+    def setScale(self, s):
+        self.L.ora_af_set_scale(self.h, float(s))
+
+    def setSite(self, site0, x, y, z):
+        assert self.L.ora_af_set_site(self.h, int(site0), float(x), float(y), float(z)) == 0
+
+    def apply(self, ss):
+        self.L.ora_af_apply(self.h, ss.h)
+
+
 class Thermal:
     def __init__(self, nx, ny=1, nz=1, threads=False):
         self.L = lib(threads)
@@ -474,6 +506,10 @@ class Sim:
     def __del__(self):
         if getattr(self, "h", None):
             self.L.ora_sim_free(self.h)
+
+    def addAppliedField(self, af):
+        self.keep_af = getattr(self, "keep_af", []) + [af]
+        self.L.ora_sim_add_af(self.h, af.h)
 
     def setLongRange3D(self, lr3):
         self.keep3 = lr3
