@@ -38,6 +38,7 @@ typedef struct msd_exchange   msd_exchange;
 typedef struct msd_anisotropy msd_anisotropy;
 typedef struct msd_thermal    msd_thermal;
 typedef struct msd_appliedfield msd_appliedfield;
+typedef struct msd_uniaxial   msd_uniaxial;
 typedef struct msd_dipole     msd_dipole;
 typedef struct msd_llg        msd_llg;
 typedef struct msd_stepper    msd_stepper;
@@ -130,6 +131,22 @@ int msd_anisotropy_add(msd_anisotropy* a, int site, const double axis[3], double
 int msd_anisotropy_add_many(msd_anisotropy* a, const int* site, const double* axes, const double* K, int count);
 /* ani:apply(ss): Anisotropy::apply, spinoperationanisotropy.cpp:247-284, slot "Anisotropy" */
 int msd_anisotropy_apply(msd_anisotropy* a, msd_system* s);
+
+/* ----------------------------------------------------- Anisotropy.Uniaxial
+ * Anisotropy.Uniaxial.new(n) (cpu/anisotropy/spinoperationanisotropy_uniaxial.cpp):
+ * add(site, axis, K1, K2) appends an op with the axis normalised, a zero axis
+ * adding nothing (l_add :387-431; its Lua binding passes K1 as K2 when a fourth
+ * number is given, which the host mirror reproduces); numberOfAxes / axisAt
+ * (0-based here); apply (:248-298) writes slot "UniaxialAnisotropy" with
+ * h = n (2 K1 d + 4 K2 d^3) scale, d = sx nx + sy ny + sz nx as the reference's
+ * line 287 computes it, s = spin / ms.  Sites are 0-based here. */
+int msd_uniaxial_create(int nx, int ny, int nz, msd_uniaxial** out);
+int msd_uniaxial_destroy(msd_uniaxial* u);
+int msd_uniaxial_set_scale(msd_uniaxial* u, double scale);
+int msd_uniaxial_add(msd_uniaxial* u, int site, const double* axis, double K1, double K2, int* added);
+int msd_uniaxial_num_axes(const msd_uniaxial* u, int* n);
+int msd_uniaxial_axis_at(const msd_uniaxial* u, int i, int* site, double* axis, double* K1, double* K2);
+int msd_uniaxial_apply(msd_uniaxial* u, msd_system* s);
 
 /* ------------------------------------------------------------ AppliedField
  * AppliedField.new(n) (cpu/appliedfield/spinoperationappliedfield.cpp:18-25; x y z

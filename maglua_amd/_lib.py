@@ -88,6 +88,13 @@ _SIGS = {
     "msd_anisotropy_add": (C.c_int, [_vp, C.c_int, _dp, C.c_double, _ip]),
     "msd_anisotropy_add_many": (C.c_int, [_vp, _ip, _dp, _dp, C.c_int]),
     "msd_anisotropy_apply": (C.c_int, [_vp, _vp]),
+    "msd_uniaxial_create": (C.c_int, [C.c_int, C.c_int, C.c_int, _pp]),
+    "msd_uniaxial_destroy": (C.c_int, [_vp]),
+    "msd_uniaxial_set_scale": (C.c_int, [_vp, C.c_double]),
+    "msd_uniaxial_add": (C.c_int, [_vp, C.c_int, _dp, C.c_double, C.c_double, _ip]),
+    "msd_uniaxial_num_axes": (C.c_int, [_vp, _ip]),
+    "msd_uniaxial_axis_at": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp, _dp]),
+    "msd_uniaxial_apply": (C.c_int, [_vp, _vp]),
     "msd_appliedfield_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _pp]),
     "msd_appliedfield_destroy": (C.c_int, [_vp]),
     "msd_appliedfield_set": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double]),

@@ -38,6 +38,39 @@ hipError_t launch_axpy(float* y, float a, const float* x, size_t n, hipStream_t 
     return hipGetLastError();
 }
 
+// Anisotropy.Uniaxial (cpu/anisotropy/spinoperationanisotropy_uniaxial.cpp:248-298):
+// one thread per site walks that site's ops in the order they were added,
+//   d = sx nx + sy ny + sz nx (the reference's line 287 pairs sz with nx),
+//   h += n (2 K1 d + 4 K2 d^3) scale,  s = spin / ms;
+// every site's slot value is written (zero without ops).  ops: [nops][5] = nx ny nz K1 K2.
+__global__ void k_uniaxial_op(Dims d, const int* __restrict__ start, const float* __restrict__ ops, float scale,
+                              const float* __restrict__ s, float* __restrict__ h)
+{
+    const size_t n = d.n;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        float hx = 0.f, hy = 0.f, hz = 0.f;
+        const float ms = s[3 * n + i];
+        if (ms > 0.f) {
+            const float inv = 1.0f / ms;
+            const float sx = s[i] * inv, sy = s[n + i] * inv, sz = s[2 * n + i] * inv;
+            for (int j = start[i]; j < start[i + 1]; ++j) {
+                const float* o = ops + 5 * (size_t)j;
+                const float dd = sx * o[0] + sy * o[1] + sz * o[0];
+                const float mag = (2.0f * o[3] * dd + 4.0f * o[4] * dd * dd * dd) * scale;
+                hx += o[0] * mag; hy += o[1] * mag; hz += o[2] * mag;
+            }
+        }
+        h[i] = hx; h[n + i] = hy; h[2 * n + i] = hz;
+    }
+}
+
+hipError_t launch_uniaxial_op(const Dims& d, const int* start, const float* ops, float scale, const float* s, float* h,
+                              hipStream_t st)
+{
+    k_uniaxial_op<<<grid_for(d.n), TPB, 0, st>>>(d, start, ops, scale, s, h);
+    return hipGetLastError();
+}
+
 // SpinSystem::sumFields: total = sum of the used slots, in slot order
 struct SrcList { const float* p[16]; };
 __global__ void k_sum_fields(float* total, SrcList src, int nsrc, size_t n3)

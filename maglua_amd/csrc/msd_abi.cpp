@@ -797,6 +797,102 @@ extern "C" int msd_anisotropy_apply(msd_anisotropy* a, msd_system* s)
     return MSD_OK;
 }
 
+// ======================================================= Anisotropy.Uniaxial
+// AnisotropyUniaxial (cpu/anisotropy/spinoperationanisotropy_uniaxial.cpp): ops of
+// (site, unit axis, K1, K2) appended by add (:158-179), applied per site in order of
+// addition; the device form is a site-sorted CSR of the ops (stable, so each site
+// keeps its order), rebuilt when ops change.
+struct msd_uniaxial {
+    Dims d{};
+    double scale = 1;
+    struct op { int site; double n[3]; double K1, K2; };
+    std::vector<op> ops;
+    int* start = nullptr;     // [n+1]
+    float* dops = nullptr;    // [nops][5]
+    bool dirty = true;
+    ~msd_uniaxial()
+    {
+        (void)hipStreamSynchronize(G.stream);
+        dfree(start); dfree(dops);
+    }
+    int compile()
+    {
+        if (!dirty) return MSD_OK;
+        const int n = d.n;
+        std::vector<int> cnt(n + 1, 0);
+        for (const op& o : ops) cnt[o.site + 1]++;
+        for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+        std::vector<float> packed(5 * std::max<size_t>(ops.size(), 1), 0.0f);
+        std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+        for (const op& o : ops) {
+            float* p = &packed[5 * (size_t)fill[o.site]++];
+            p[0] = (float)o.n[0]; p[1] = (float)o.n[1]; p[2] = (float)o.n[2]; p[3] = (float)o.K1; p[4] = (float)o.K2;
+        }
+        dfree(start); dfree(dops);
+        if (int rc = dalloc(&start, (size_t)n + 1)) return rc;
+        if (int rc = dalloc(&dops, packed.size())) return rc;
+        HIPCHK(hipMemcpyAsync(start, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice, G.stream));
+        HIPCHK(hipMemcpyAsync(dops, packed.data(), sizeof(float) * packed.size(), hipMemcpyHostToDevice, G.stream));
+        HIPCHK(hipStreamSynchronize(G.stream));
+        dirty = false;
+        return MSD_OK;
+    }
+};
+
+extern "C" int msd_uniaxial_create(int nx, int ny, int nz, msd_uniaxial** out)
+{
+    NONNULL(out);
+    *out = nullptr;
+    if (nx < 1 || ny < 1 || nz < 1) return fail(MSD_EINVAL, "Anisotropy.Uniaxial.new: bad size");
+    if (int rc = ensure_init()) return rc;
+    msd_uniaxial* u = new msd_uniaxial;
+    u->d = Dims{nx, ny, nz, nx * ny * nz};
+    *out = u;
+    return MSD_OK;
+}
+extern "C" int msd_uniaxial_destroy(msd_uniaxial* u) { delete u; return MSD_OK; }
+extern "C" int msd_uniaxial_set_scale(msd_uniaxial* u, double s) { NONNULL(u); u->scale = s; return MSD_OK; }
+extern "C" int msd_uniaxial_add(msd_uniaxial* u, int site, const double* axis, double K1, double K2, int* added)
+{
+    NONNULL(u); NONNULL(axis);
+    if (site < 0 || site >= u->d.n) return fail(MSD_EINVAL, "Anisotropy.Uniaxial:add: site %d is not part of the operator", site);
+    const double len = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+    if (added) *added = 0;
+    if (!(len > 0)) return MSD_OK;    // l_add: a zero axis adds nothing
+    u->ops.push_back(msd_uniaxial::op{site, {axis[0] / len, axis[1] / len, axis[2] / len}, K1, K2});
+    u->dirty = true;
+    if (added) *added = 1;
+    return MSD_OK;
+}
+extern "C" int msd_uniaxial_num_axes(const msd_uniaxial* u, int* n)
+{
+    NONNULL(u); NONNULL(n);
+    *n = (int)u->ops.size();
+    return MSD_OK;
+}
+/* i-th op (0-based, order of addition): site, axis[3], K1, K2 */
+extern "C" int msd_uniaxial_axis_at(const msd_uniaxial* u, int i, int* site, double* axis, double* K1, double* K2)
+{
+    NONNULL(u); NONNULL(site); NONNULL(axis); NONNULL(K1); NONNULL(K2);
+    if (i < 0 || i >= (int)u->ops.size()) return fail(MSD_EINVAL, "Anisotropy.Uniaxial:axisAt: index %d out of range", i + 1);
+    const msd_uniaxial::op& o = u->ops[i];
+    *site = o.site;
+    for (int c = 0; c < 3; ++c) axis[c] = o.n[c];
+    *K1 = o.K1; *K2 = o.K2;
+    return MSD_OK;
+}
+extern "C" int msd_uniaxial_apply(msd_uniaxial* u, msd_system* s)
+{
+    NONNULL(u); NONNULL(s);
+    if (!same_dims(s->d, u->d.nx, u->d.ny, u->d.nz)) return fail(MSD_EMISMATCH, "Anisotropy.Uniaxial:apply: size mismatch");
+    if (int rc = u->compile()) return rc;
+    int k;
+    if (int rc = s->mark("UniaxialAnisotropy", &k)) return rc;
+    hipError_t e = launch_uniaxial_op(s->d, u->start, u->dops, (float)u->scale, s->S, s->slots[k].h, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "Anisotropy.Uniaxial:apply: %s", hipGetErrorString(e));
+    return MSD_OK;
+}
+
 // ============================================================= AppliedField
 // AppliedField (cpu/appliedfield/spinoperationappliedfield.cpp): its Lua apply
 // (appliedfield_luafuncs.lua:46-57) overwrites slot "Zeeman" with B*scale at every

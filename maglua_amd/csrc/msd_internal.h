@@ -93,6 +93,8 @@ hipError_t launch_axpy(float* y, float a, const float* x, size_t n, hipStream_t 
 hipError_t launch_sum_fields(float* total, const float* const* src, int nsrc, size_t n3, hipStream_t st);
 hipError_t launch_exchange_op(const Dims& d, const ExchangeDev& ex, const float* s, float* h, hipStream_t st);
 hipError_t launch_anisotropy_op(const Dims& d, const AnisoDev& an, const float* s, float* h, hipStream_t st);
+hipError_t launch_uniaxial_op(const Dims& d, const int* start, const float* ops, float scale, const float* s, float* h,
+                              hipStream_t st);
 hipError_t launch_thermal_op(const Dims& d, const ThermalDev& th, const LLGDev& lp, float dt, uint64_t draw,
                              const float* s, float* h, hipStream_t st);
 hipError_t launch_llg_op(const Dims& d, const LLGDev& lp, float dt, const float* from, const float* dmdt,

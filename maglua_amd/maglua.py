@@ -342,6 +342,71 @@ class Anisotropy(_Handle):
             call("msd_anisotropy_apply", self._h, s.h)
 
 
+class _AnisotropyUniaxial(_Handle):
+    """Anisotropy.Uniaxial (cpu/anisotropy/spinoperationanisotropy_uniaxial.cpp): ops of
+    (site, unit axis, K1, K2), slot 'UniaxialAnisotropy',
+    h = n (2 K1 d + 4 K2 d^3) scale with d = sx nx + sy ny + sz nx as the reference
+    computes it (its line 287 pairs sz with nx)."""
+
+    _destroy = "msd_uniaxial_destroy"
+
+    def __init__(self, *a):
+        self._dims = _dims_of(a)
+        p = _vp()
+        call("msd_uniaxial_create", *self._dims, C.byref(p))
+        super().__init__(p.value)
+
+    new = classmethod(lambda cls, *a: cls(*a))
+
+    def setScale(self, s):
+        call("msd_uniaxial_set_scale", self._h, float(s))
+
+    def add(self, site, axis, K1=None, K2=None):
+        """add({x,y,z}, {ax,ay,az}, K1[, K2]) (l_add, :387-431).  As in the reference's
+        binding, a fourth number sets K2 to K1's value (it reads K1's stack slot twice)."""
+        if K1 is None:
+            raise MagLuaError(-1, "anisotropy needs strength")
+        k2 = float(K1) if K2 is not None else 0.0
+        ax = (C.c_double * 3)(*[float(v) for v in axis])
+        added = C.c_int()
+        call("msd_uniaxial_add", self._h, _index(site, self._dims), ax, float(K1), k2, C.byref(added))
+        return bool(added.value)
+
+    def numberOfAxes(self):
+        n = C.c_int()
+        call("msd_uniaxial_num_axes", self._h, C.byref(n))
+        return n.value
+
+    def _op(self, i):
+        site, K1, K2 = C.c_int(), C.c_double(), C.c_double()
+        ax = (C.c_double * 3)()
+        call("msd_uniaxial_axis_at", self._h, int(i), C.byref(site), ax, C.byref(K1), C.byref(K2))
+        return site.value, [ax[0], ax[1], ax[2]], K1.value, K2.value
+
+    def axis(self, i):
+        """i-th op, 1-based: (axis, K1, K2); l_axisat pushes the site too but returns
+        only the top three values."""
+        _, ax, k1, k2 = self._op(int(i) - 1)
+        return ax, k1, k2
+
+    def get(self, site):
+        """The first op at the site (getAnisotropy, :126-139): (axis, K1, K2), or
+        ({0,0,1}, 0, 0) when the site has none."""
+        idx = _index(site, self._dims)
+        for i in range(self.numberOfAxes()):
+            s, ax, k1, k2 = self._op(i)
+            if s == idx:
+                return ax, k1, k2
+        return [0.0, 0.0, 1.0], 0.0, 0.0
+
+    def apply(self, ss):
+        for s in (ss if isinstance(ss, (list, tuple)) else (ss,)):
+            call("msd_uniaxial_apply", self._h, s.h)
+
+
+Anisotropy.Uniaxial = _AnisotropyUniaxial
+
+
 class _AppliedFieldHeterogeneous(_Handle):
     """AppliedField.Heterogeneous (cpu/appliedfield/spinoperationappliedfield_heterogeneous.cpp):
     a field per site, added (times scale) to slot 'Zeeman_Heterogeneous' by apply."""

@@ -23,6 +23,9 @@
  *   ora_ani_*              Anisotropy::apply and l_add,
  *                          cpu/anisotropy/spinoperationanisotropy.cpp:247-284,367-413.
  *                          Pinned by the closed form 2K(s.n)n/ms^2 (tests).
+ *   ora_uni_*              Anisotropy.Uniaxial, cpu/anisotropy/spinoperationanisotropy_uniaxial.cpp:
+ *                          158-179,248-298 (its d = sx nx + sy ny + sz nx kept).  Pinned by
+ *                          hand-computed fields for chosen axes and spins (tests).
  *   ora_af_*               AppliedField (Lua apply, cpu/appliedfield/appliedfield_luafuncs.lua:46-57)
  *                          and AppliedField.Heterogeneous::apply
  *                          (spinoperationappliedfield_heterogeneous.cpp:88-99).
@@ -636,6 +639,61 @@ int ora_ani_apply(ora_ani* a, ora_sys* s)
 } /* extern "C" */
 
 /* ------------------------------------------------------------------------ */
+/* Anisotropy.Uniaxial                                                       */
+/* ------------------------------------------------------------------------ */
+/* AnisotropyUniaxial::apply, cpu/anisotropy/spinoperationanisotropy_uniaxial.cpp:248-298:
+ * slot "UniaxialAnisotropy" zeroed, then per op (site order of addition)
+ *   d = sx nx + sy ny + sz nx   (the reference's line 287 pairs sz with nx, kept),
+ *   h += n (2 K1 d + 4 K2 d^3) scale,  s = spin / ms, sites with ms = 0 skipped.
+ * addAnisotropy (:158-179) appends; the axis arrives normalised (l_add :408-418). */
+struct ora_uni {
+    struct op { int site; double n[3]; double K1, K2; };
+    int nsites = 0;
+    double scale = 1;
+    std::vector<op> ops;
+};
+
+extern "C" {
+ora_uni* ora_uni_new(int nx, int ny, int nz)
+{
+    ora_uni* u = new ora_uni;
+    u->nsites = nx * ny * nz;
+    return u;
+}
+void ora_uni_free(ora_uni* u) { delete u; }
+void ora_uni_set_scale(ora_uni* u, double s) { u->scale = s; }
+/* returns 1 if added, 0 for a zero axis (not added, as l_add), -1 for a bad site */
+int ora_uni_add(ora_uni* u, int site, double ax, double ay, double az, double K1, double K2)
+{
+    if (site < 0 || site >= u->nsites) return -1;
+    const double len = std::sqrt(ax * ax + ay * ay + az * az);
+    if (!(len > 0)) return 0;
+    ora_uni::op o{site, {ax / len, ay / len, az / len}, K1, K2};
+    u->ops.push_back(o);
+    return 1;
+}
+int ora_uni_apply(ora_uni* u, ora_sys* s)
+{
+    const int k = s->mark("UniaxialAnisotropy");
+    std::fill(s->hx[k].begin(), s->hx[k].end(), 0.0);
+    std::fill(s->hy[k].begin(), s->hy[k].end(), 0.0);
+    std::fill(s->hz[k].begin(), s->hz[k].end(), 0.0);
+    for (const ora_uni::op& o : u->ops) {
+        const int i = o.site;
+        const double ms = s->ms[i];
+        if (!(ms > 0)) continue;
+        const double sx = s->x[i] / ms, sy = s->y[i] / ms, sz = s->z[i] / ms;
+        const double d = sx * o.n[0] + sy * o.n[1] + sz * o.n[0];
+        const double mag = 2.0 * o.K1 * d + 4.0 * o.K2 * d * d * d;
+        s->hx[k][i] += o.n[0] * mag * u->scale;
+        s->hy[k][i] += o.n[1] * mag * u->scale;
+        s->hz[k][i] += o.n[2] * mag * u->scale;
+    }
+    return 0;
+}
+} /* extern "C" */
+
+/* ------------------------------------------------------------------------ */
 /* AppliedField                                                              */
 /* ------------------------------------------------------------------------ */
 /* Uniform: the Lua apply (cpu/appliedfield/appliedfield_luafuncs.lua:46-57)
@@ -1154,6 +1212,7 @@ struct ora_sim {
     ora_ex* ex; ora_ani* ani; ora_lr* lr; ora_th* th; ora_llg* llg;
     ora_lr3* lr3 = nullptr;
     std::vector<ora_af*> af;
+    std::vector<ora_uni*> uni;
     uint64_t seed = 0, step = 0;
     ora_sys* P = nullptr;
     ora_sys* C = nullptr;
@@ -1178,6 +1237,7 @@ void ora_sim_free(ora_sim* s)
 void ora_sim_set_seed(ora_sim* s, uint64_t seed) { s->seed = seed; s->step = 0; }
 void ora_sim_set_lr3(ora_sim* s, ora_lr3* l) { s->lr3 = l; }
 void ora_sim_add_af(ora_sim* s, ora_af* a) { s->af.push_back(a); }
+void ora_sim_add_uni(ora_sim* s, ora_uni* u) { s->uni.push_back(u); }
 
 void ora_sim_set_threads(int n)
 {
@@ -1196,6 +1256,7 @@ int ora_sim_calc_fields(ora_sim* s, ora_sys* t)
     if (s->ani) ora_ani_apply(s->ani, t);
     if (s->lr) ora_lr_apply(s->lr, t);
     if (s->lr3) ora_lr3_apply(s->lr3, t);
+    for (ora_uni* u : s->uni) ora_uni_apply(u, t);
     for (ora_af* a : s->af) ora_af_apply(a, t);
     ora_sys_sum_fields(t);
     return 0;

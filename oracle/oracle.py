@@ -74,6 +74,12 @@ def lib(threads=False):
         "ora_ani_set_scale": (None, [vp, C.c_double]),
         "ora_ani_add": (C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]),
         "ora_ani_apply": (C.c_int, [vp, vp]),
+        "ora_uni_new": (vp, [C.c_int, C.c_int, C.c_int]),
+        "ora_uni_free": (None, [vp]),
+        "ora_uni_set_scale": (None, [vp, C.c_double]),
+        "ora_uni_add": (C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]),
+        "ora_uni_apply": (C.c_int, [vp, vp]),
+        "ora_sim_add_uni": (None, [vp, vp]),
         "ora_af_new": (vp, [C.c_int, C.c_int, C.c_int, C.c_int]),
         "ora_af_free": (None, [vp]),
         "ora_af_set": (None, [vp, C.c_double, C.c_double, C.c_double]),
@@ -331,6 +337,29 @@ class Anisotropy:
         self.L.ora_ani_apply(self.h, ss.h)
 
 
+class AnisotropyUniaxial:
+    """Anisotropy.Uniaxial: per-op site, axis (normalised), K1, K2; slot "UniaxialAnisotropy"."""
+
+    def __init__(self, nx, ny=1, nz=1, threads=False):
+        self.L = lib(threads)
+        self.h = self.L.ora_uni_new(nx, ny, nz)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.ora_uni_free(self.h)
+
+    def setScale(self, s):
+        self.L.ora_uni_set_scale(self.h, float(s))
+
+    def add(self, site0, axis, K1, K2=0.0):
+        rc = self.L.ora_uni_add(self.h, int(site0), *[float(v) for v in axis], float(K1), float(K2))
+        assert rc >= 0
+        return rc == 1
+
+    def apply(self, ss):
+        self.L.ora_uni_apply(self.h, ss.h)
+
+
 class AppliedField:
     """Uniform applied field (slot "Zeeman", overwritten by apply) or, with
     heterogeneous=True, per-site fields (slot "Zeeman_Heterogeneous", accumulated)."""
@@ -506,6 +535,10 @@ class Sim:
     def __del__(self):
         if getattr(self, "h", None):
             self.L.ora_sim_free(self.h)
+
+    def addUniaxial(self, u):
+        self.keep_uni = getattr(self, "keep_uni", []) + [u]
+        self.L.ora_sim_add_uni(self.h, u.h)
 
     def addAppliedField(self, af):
         self.keep_af = getattr(self, "keep_af", []) + [af]

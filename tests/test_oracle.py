@@ -322,6 +322,30 @@ def test_heun_second_order_against_closed_form():
     assert 3.6 < errs[0] / errs[1] < 4.4 and errs[1] < 1e-5
 
 
+def test_uniaxial_anisotropy_hand_values():
+    """Anisotropy.Uniaxial at chosen spins and axes, by hand: h = n (2 K1 d + 4 K2 d^3)
+    scale with d = sx nx + sy ny + sz nx (the reference's line 287 pairs sz with nx),
+    s = spin / ms; ops on one site add; a zero axis is not added."""
+    ss = O.SpinSystem(3)
+    ss.setSpins([0.6, 0.0, 1.2], [0.0, 0.6, 0.0], [0.8, 0.8, 1.6])   # ms = 1, 1, 2
+    u = O.AnisotropyUniaxial(3)
+    u.setScale(0.5)
+    assert u.add(0, (2.0, 0.0, 0.0), 2.0, 0.5)           # n = x: d = 0.6 + 0.8 = 1.4
+    assert u.add(1, (0.0, 0.0, 3.0), 2.0, 0.5)           # n = z: d = 0 (the pairing)
+    assert u.add(2, (1.0, 1.0, 0.0), 1.0, 0.0)           # n = (1,1,0)/r2: d = (0.6 + 0.8)/r2
+    assert u.add(2, (1.0, 0.0, 0.0), 0.0, 1.0)           # second op on site 2: d = 0.6 + 0.8
+    assert not u.add(1, (0.0, 0.0, 0.0), 5.0, 5.0)
+    u.apply(ss)
+    h = np.stack(ss.field("UniaxialAnisotropy"))
+    m0 = 2 * 2.0 * 1.4 + 4 * 0.5 * 1.4 ** 3
+    r = 1 / np.sqrt(2)
+    d2 = (0.6 + 0.8) * r
+    want = np.zeros((3, 3))
+    want[:, 0] = [m0 * 0.5, 0, 0]
+    want[:, 2] = [r * 2 * d2 * 0.5 + 4 * 1.4 ** 3 * 0.5, r * 2 * d2 * 0.5, 0]
+    assert np.allclose(h, want, rtol=1e-14, atol=1e-15)
+
+
 def test_applied_field_slots():
     """AppliedField's Lua apply overwrites slot "Zeeman" with B*scale; the heterogeneous
     form adds scale*h(site) to slot "Zeeman_Heterogeneous" (zero where unset)."""
