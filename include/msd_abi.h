@@ -254,6 +254,9 @@ int msd_stepper_set_longrange3d(msd_stepper* st, msd_longrange3d* l);
 /* Add an applied field to the fused step's field sum (a = NULL removes all);
  * changes made to it later are picked up by the next msd_stepper_heun */
 int msd_stepper_add_applied_field(msd_stepper* st, msd_appliedfield* a);
+/* Anisotropy.Uniaxial in the fused step's field sum (u = NULL removes it); ops added
+ * later are picked up by the next msd_stepper_heun */
+int msd_stepper_set_uniaxial(msd_stepper* st, msd_uniaxial* u);
 /* Advance nsteps Heun steps; asynchronous. */
 int msd_stepper_heun(msd_stepper* st, int nsteps);
 /* Kernel-level timing on the library stream, names[i]/avg_ms[i] for up to cap

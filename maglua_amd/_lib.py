@@ -104,6 +104,7 @@ _SIGS = {
     "msd_appliedfield_get_site": (C.c_int, [_vp, C.c_int, _dp]),
     "msd_appliedfield_apply": (C.c_int, [_vp, _vp]),
     "msd_stepper_add_applied_field": (C.c_int, [_vp, _vp]),
+    "msd_stepper_set_uniaxial": (C.c_int, [_vp, _vp]),
     "msd_thermal_create": (C.c_int, [C.c_int, C.c_int, C.c_int, _pp]),
     "msd_thermal_destroy": (C.c_int, [_vp]),
     "msd_thermal_set_temperature": (C.c_int, [_vp, C.c_double]),

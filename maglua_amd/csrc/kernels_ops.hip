@@ -47,20 +47,10 @@ __global__ void k_uniaxial_op(Dims d, const int* __restrict__ start, const float
                               const float* __restrict__ s, float* __restrict__ h)
 {
     const size_t n = d.n;
+    const UniDev u{start, ops, scale};
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        float hx = 0.f, hy = 0.f, hz = 0.f;
-        const float ms = s[3 * n + i];
-        if (ms > 0.f) {
-            const float inv = 1.0f / ms;
-            const float sx = s[i] * inv, sy = s[n + i] * inv, sz = s[2 * n + i] * inv;
-            for (int j = start[i]; j < start[i + 1]; ++j) {
-                const float* o = ops + 5 * (size_t)j;
-                const float dd = sx * o[0] + sy * o[1] + sz * o[0];
-                const float mag = (2.0f * o[3] * dd + 4.0f * o[4] * dd * dd * dd) * scale;
-                hx += o[0] * mag; hy += o[1] * mag; hz += o[2] * mag;
-            }
-        }
-        h[i] = hx; h[n + i] = hy; h[2 * n + i] = hz;
+        const float3 v = uniaxial_field(u, (int)i, make_float3(s[i], s[n + i], s[2 * n + i]), s[3 * n + i]);
+        h[i] = v.x; h[n + i] = v.y; h[2 * n + i] = v.z;
     }
 }
 

@@ -2285,6 +2285,7 @@ struct msd_stepper {
     double sur_tol = 1e-3;
     float *Href = nullptr, *Mref = nullptr, *Hext = nullptr, *Hx = nullptr;   // [3][n]
     std::vector<msd_appliedfield*> af;                                        // applied fields
+    msd_uniaxial* uni = nullptr;                                              // Anisotropy.Uniaxial
     float* Happ = nullptr;                                                    // [3][n] their per-site sum
     bool have_ref = false, sur_fallback = false;
     long sur_windows = 0, sur_exact_windows = 0, sur_fallbacks = 0;
@@ -2361,6 +2362,10 @@ int msd_stepper::refresh()
     } else {
         a.ty = 1;
         for (int c : {8, 4, 2}) if (s->d.ny % c == 0) { a.ty = c; break; }
+    }
+    if (uni) {
+        if (int rc = uni->compile()) return rc;
+        a.uni = UniDev{uni->start, uni->dops, (float)uni->scale};
     }
     if (!af.empty()) {
         // uniform fields fold into one vector; per-site ones into one [3][n] sum
@@ -2723,6 +2728,14 @@ extern "C" int msd_stepper_autotune(msd_stepper* st, int steps, int* best_ty, in
     if (!rc) rc = st->refresh();
     st->spec_version = 0;
     return rc;
+}
+
+extern "C" int msd_stepper_set_uniaxial(msd_stepper* st, msd_uniaxial* u)
+{
+    NONNULL(st);
+    if (u && !same_dims(st->s->d, u->d.nx, u->d.ny, u->d.nz)) return fail(MSD_EMISMATCH, "stepper: Anisotropy.Uniaxial size differs");
+    st->uni = u;
+    return MSD_OK;
 }
 
 extern "C" int msd_stepper_add_applied_field(msd_stepper* st, msd_appliedfield* a)

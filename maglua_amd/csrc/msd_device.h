@@ -432,6 +432,24 @@ __device__ __forceinline__ float3 renorm(float3 v, float ms)
 // ------------------------------------------------------------ fields
 // Anisotropy field (cpu/anisotropy/spinoperationanisotropy.cpp:247-284) as
 // h = 2 A s / ms^2 with A = sum over the site's ops of K n n^T.
+// Anisotropy.Uniaxial at site i (cpu/anisotropy/spinoperationanisotropy_uniaxial.cpp:248-298):
+// the site's ops in order of addition, h += n (2 K1 d + 4 K2 d^3) scale with
+// d = sx nx + sy ny + sz nx (the reference's line 287 pairs sz with nx), s = spin / ms
+__device__ __forceinline__ float3 uniaxial_field(const UniDev& u, int i, float3 s, float ms)
+{
+    float3 h = make_float3(0.f, 0.f, 0.f);
+    if (!u.ops || !(ms > 0.0f)) return h;
+    const float inv = 1.0f / ms;
+    const float sx = s.x * inv, sy = s.y * inv, sz = s.z * inv;
+    for (int j = u.start[i]; j < u.start[i + 1]; ++j) {
+        const float* o = u.ops + 5 * (size_t)j;
+        const float dd = sx * o[0] + sy * o[1] + sz * o[0];
+        const float mag = (2.0f * o[3] * dd + 4.0f * o[4] * dd * dd * dd) * u.scale;
+        h.x += o[0] * mag; h.y += o[1] * mag; h.z += o[2] * mag;
+    }
+    return h;
+}
+
 __device__ __forceinline__ float3 aniso_field(const AnisoDev& an, int i, int n, float3 s, float ms)
 {
     if (!(ms > 0.0f) || an.mode == 0) return make_float3(0.f, 0.f, 0.f);
@@ -575,6 +593,7 @@ __device__ __forceinline__ float3 pred_finish(const StepArgs& a, const ExchangeD
     float3 H = exchange_finish(ex, n, u, a.S, a.S + 3 * (size_t)n, ms, g.ex);
     const float3 ha = aniso_field(a.an, i, (int)n, s, ms);
     H.x += ha.x + Hd.x; H.y += ha.y + Hd.y; H.z += ha.z + Hd.z;
+    if (a.uni.ops) { const float3 hu = uniaxial_field(a.uni, i, s, ms); H.x += hu.x; H.y += hu.y; H.z += hu.z; }
     if (a.Hext) { H.x += ldg(a.Hext, u); H.y += ldg(a.Hext + n, u); H.z += ldg(a.Hext + 2 * (size_t)n, u); }
     if (a.Href) { H.x += ldg(a.Href, u); H.y += ldg(a.Href + n, u); H.z += ldg(a.Href + 2 * (size_t)n, u); }
     if (a.has_app) {
@@ -644,6 +663,7 @@ __device__ __forceinline__ float3 corr_finish(const StepArgs& a, const ExchangeD
     float3 H = exchange_finish(ex, n, u, a.P, a.S + 3 * (size_t)n, ms, g.ex);
     const float3 ha = aniso_field(a.an, i, (int)n, p, ms);
     H.x += ha.x + Hd.x; H.y += ha.y + Hd.y; H.z += ha.z + Hd.z;
+    if (a.uni.ops) { const float3 hu = uniaxial_field(a.uni, i, p, ms); H.x += hu.x; H.y += hu.y; H.z += hu.z; }
     if (a.Hext) { H.x += ldg(a.Hext, u); H.y += ldg(a.Hext + n, u); H.z += ldg(a.Hext + 2 * (size_t)n, u); }
     if (a.Href) { H.x += ldg(a.Href, u); H.y += ldg(a.Href + n, u); H.z += ldg(a.Href + 2 * (size_t)n, u); }
     if (a.has_app) {

@@ -126,6 +126,12 @@ hipError_t launch_tensor_spectrum(int nx, int ny, int nz, const double* T, doubl
 hipError_t launch_tensor_words(const double* W, float* Tw, double mult, size_t n, hipStream_t st);
 
 // fused Heun step
+// Anisotropy.Uniaxial ops as a site-sorted CSR: start [n+1], ops [nops][5] = nx ny nz K1 K2
+struct UniDev {
+    const int* start;
+    const float* ops;       // nullptr: no uniaxial anisotropy
+    float scale;
+};
 struct StepArgs {
     Dims d;
     float dt;
@@ -144,6 +150,7 @@ struct StepArgs {
     int ty;                 // rows per tile when there is no dipole
     const float* Hext;      // [3][n] field added at every site (the surrogate's dipole field) or nullptr
     const float* Href;      // [3][n] second such field (the surrogate's reference field) or nullptr
+    UniDev uni;             // Anisotropy.Uniaxial
     int has_app;            // applied fields (AppliedField): uniform Bapp plus per-site Happ
     float3 Bapp;
     const float* Happ;      // [3][n] or nullptr

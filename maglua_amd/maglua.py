@@ -912,6 +912,11 @@ class Heun(_Handle):
 
     __call__ = step
 
+    def setUniaxial(self, u):
+        """Anisotropy.Uniaxial in the step's field sum (None removes it)."""
+        self._uni = u
+        call("msd_stepper_set_uniaxial", self._h, u.h if u is not None else None)
+
     def addAppliedField(self, af):
         """Add an AppliedField or AppliedField.Heterogeneous to the step's field sum (None
         removes them all); later changes to it apply from the next step() call."""

@@ -104,3 +104,28 @@ def test_uniaxial_in_the_pc_step_closure_matches_oracle():
     d = float(np.abs(flat(gpu["ss"].spins()[:3]) - flat(ora["ss"].spins()[:3])).max())
     print(f"uniaxial PC closure vs oracle over 40 steps: max|dm| {d:.2e}", flush=True)
     assert d <= 1e-5
+
+
+def test_uniaxial_in_the_fused_step_matches_oracle():
+    spec = W.FilmSpec(16, 16, 2, J=1.0, K=0.0, g=0.0, T=0.0, dt=0.005)
+    dims = (spec.nx, spec.ny, spec.nz)
+    gpu, ora = W.build(M, spec), W.build(O, spec)
+    x, y, z = O.random_unit_spins(8, spec.n)
+    gpu["ss"].setSpins(x, y, z)
+    ora["ss"].setSpins(x, y, z)
+    gu, ou = build_pair(dims, 9, 2 * spec.n, scale=1.0)
+    heun = M.Heun(gpu["ss"], gpu["ex"], None, None, None, gpu["llg"])
+    heun.setUniaxial(gu)
+    sim = O.Sim(ora["ss"], ora["ex"], None, None, None, ora["llg"])
+    sim.addUniaxial(ou)
+    heun.step(60)
+    sim.heun(60)
+    d1 = float(np.abs(flat(gpu["ss"].spins()[:3]) - flat(ora["ss"].spins()[:3])).max())
+    # ops added after setUniaxial apply from the next step() call
+    for i in range(0, spec.n, 5):
+        assert gu.add(site3(i, dims), [0.3, -0.4, 0.5], 0.8) == ou.add(i, [0.3, -0.4, 0.5], 0.8, 0.0)
+    heun.step(40)
+    sim.heun(40)
+    d2 = float(np.abs(flat(gpu["ss"].spins()[:3]) - flat(ora["ss"].spins()[:3])).max())
+    print(f"uniaxial fused step vs oracle: max|dm| {d1:.2e} after 60 steps, {d2:.2e} after 100", flush=True)
+    assert d1 <= 1e-5 and d2 <= 1e-5
