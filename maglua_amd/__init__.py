@@ -8,12 +8,14 @@ from ._lib import MagLuaError, declared_symbols, lib  # noqa: F401
 from .maglua import (  # noqa: F401
     BUTCHER, LLG, AppliedField, Anisotropy, Dipole, Ensemble, Exchange, Heun, LongRange, LongRange3D, Random,
     SpinSystem,
-    Surrogate, Thermal, make_pc_step_function, make_rk_step_function, philox_device, synchronize,
+    Surrogate, Thermal, make_adapt_step_function, make_pc_step_function, make_rk_step_function, philox_device,
+    synchronize,
 )
 
 __all__ = [
     "MagLuaError", "SpinSystem", "Exchange", "Anisotropy", "AppliedField", "Thermal", "Dipole", "LongRange",
     "LongRange3D",
-    "LLG", "Random", "Heun", "Ensemble", "Surrogate", "make_pc_step_function", "make_rk_step_function", "BUTCHER",
+    "LLG", "Random", "Heun", "Ensemble", "Surrogate", "make_pc_step_function", "make_rk_step_function",
+    "make_adapt_step_function", "BUTCHER",
     "synchronize", "philox_device", "lib", "declared_symbols",
 ]

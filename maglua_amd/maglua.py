@@ -829,6 +829,63 @@ def make_pc_step_function(s, calcField, llg, tol, optional_temp=None):
     return step
 
 
+def make_adapt_step_function(*a):
+    """common/scripts/AdaptiveTimeStep.lua: wrap a step function in step doubling.
+
+    Arguments are sorted by kind as the Lua does (userdata, functions, numbers): the
+    SpinSystem, then the LLG operator and an optional thermal operator; the step
+    function, then an optional dynamics function; the tolerance.  Each call takes one
+    full step and two half steps from the same state; when their max-norm difference
+    (SpinSystem.diff) is below the tolerance the two half steps are kept (plus the
+    thermal kick) and the next step is longer by 1 + clip(ee^2, 0.1, 0.5) - 0.1,
+    ee = (tol - diff)/tol; otherwise the state is kept and the step halves.  Returns
+    whether the step was accepted."""
+    objs = [v for v in a if v is not None and not callable(v) and not isinstance(v, (int, float))]
+    funcs = [v for v in a if callable(v) and not isinstance(v, _Handle)]
+    nums = [v for v in a if isinstance(v, (int, float)) and not isinstance(v, bool)]
+    ss, step, tol = objs[0], funcs[0], float(nums[0])
+    dynamics = funcs[1] if len(funcs) > 1 else (lambda s: None)
+    llg = objs[1] if len(objs) > 1 else None
+    temp = objs[2] if len(objs) > 2 else None
+    ss2, ss3 = ss.copy(), ss.copy()
+
+    def same(A, Bs):
+        d = A.diff(Bs)[3]
+        return d < tol, (tol - d) / tol
+
+    def adapt_step(s=None, skip_temperature=False):
+        s = s or ss
+        ss2.setTime(s.time())
+        ss3.setTime(s.time())
+        ss2.setTimeStep(s.timeStep())
+        ss3.setTimeStep(s.timeStep() / 2)
+        s.copySpinsTo(ss2)
+        s.copySpinsTo(ss3)
+        dynamics(ss2)
+        step(ss2, True)                  # one full step, no thermal kick
+        dynamics(ss3)
+        step(ss3, True)                  # two half steps
+        dynamics(ss3)
+        step(ss3, True)
+        ok, ee = same(ss2, ss3)
+        if ok:
+            ss3.copySpinsTo(s)
+            if temp is not None and not skip_temperature:
+                dynamics(s)
+                s.resetFields()
+                temp.apply(s)
+                s.sumFields()
+                llg.apply(s, False)      # False: the time is not advanced
+            s.setTime(ss3.time())
+            ee = min(max(ee * ee, 0.1), 0.5)
+            s.setTimeStep(s.timeStep() * (1 + ee - 0.1))
+        else:
+            s.setTimeStep(s.timeStep() * 0.5)
+        return ok
+
+    return adapt_step
+
+
 BUTCHER = {
     "RK1": [[1]],
     "RK2": [[1 / 2], [0, 1]],

@@ -109,3 +109,40 @@ def test_spin_precession_in_an_applied_field():
         errs.append(float(np.abs(s - exact(2.0)).max()))
     print(f"precession errors {errs}", flush=True)
     assert 3.4 < errs[0] / errs[1] < 4.6 and errs[1] < 2e-5
+
+
+def test_adaptive_rk_steps_follow_the_closed_form():
+    """make_adapt_step_function around the RK2 closure on the operator API (it passes
+    the current time step to its stage systems, RungeKutta.lua:198): a spin in a
+    uniform AppliedField reaches the closed-form state at t = 2 to the tolerance's
+    accuracy, with rejected and accepted steps along the way."""
+    B, alpha, gamma, th0 = 1.0, 0.1, 1.0, 1.0
+    gp = gamma / (1 + alpha ** 2)
+    th = 2 * np.arctan(np.tan(th0 / 2) * np.exp(-alpha * gp * B * 2.0))
+    want = np.array([np.sin(th) * np.cos(gp * B * 2.0), np.sin(th) * np.sin(gp * B * 2.0), np.cos(th)])
+    ss = M.SpinSystem(1, 1, 1)
+    ss.setSpins([np.sin(th0)], [0.0], [np.cos(th0)])
+    ss.setAlpha(alpha)
+    ss.setGamma(gamma)
+    ss.setTimeStep(0.5)
+    af = M.AppliedField(ss)
+    af.set(0.0, 0.0, B)
+    llg = M.LLG.Cartesian()
+
+    def calc(s):
+        s.resetFields()
+        af.apply(s)
+        s.sumFields()
+
+    step = M.make_rk_step_function(ss, "RK2", calc, llg)
+    adapt = M.make_adapt_step_function(ss, step, 1e-5, llg)
+    results = []
+    while ss.time() < 2.0 - 1e-9 and len(results) < 5000:
+        if ss.time() + ss.timeStep() > 2.0:
+            ss.setTimeStep(2.0 - ss.time())
+        results.append(adapt())
+    err = float(np.abs(flat(ss.spins()[:3])[:, 0] - want).max())
+    print(f"adaptive: {results.count(True)} accepted, {results.count(False)} rejected, err {err:.2e}", flush=True)
+    assert abs(ss.time() - 2.0) < 1e-9
+    assert results.count(False) > 0 and results.count(True) > 5
+    assert err < 1e-4
