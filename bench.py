@@ -59,26 +59,48 @@ def step_model_bytes(spec, comp):
     return 2 * E + 96 + (48 if comp else 0)
 
 
-def cpu_baseline(spec, budget_s=15.0):
-    """FP64 oracle (this project's CPU restatement of the reference path), one thread,
-    on the same workload for as many whole steps as fit the budget after one warm-up step."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(spec, budget_s=15.0, omp_budget_s=8.0):
+    """FP64 oracle (this project's CPU restatement of the reference path) on the same
+    workload for as many whole steps as fit the budget after one warm-up step: one
+    thread (the reference's loops and FFTW plans are serial), then the same build with
+    OpenMP over the host threads this process may use."""
     from maglua_amd import workloads as W
     from oracle import oracle as O
 
-    o = W.build(O, spec)
-    o["ss"].setSpins(*O.random_unit_spins(spec.seed, spec.n))
-    sim = O.Sim(o["ss"], o["ex"], o["ani"], o["dip"], o["temp"], o["llg"])
-    sim.heun(1)
-    steps, t0 = 0, time.perf_counter()
-    while True:
+    def timed(threads, budget):
+        o = W.build(O, spec, threads=threads)
+        o["ss"].setSpins(*O.random_unit_spins(spec.seed, spec.n))
+        sim = O.Sim(o["ss"], o["ex"], o["ani"], o["dip"], o["temp"], o["llg"])
         sim.heun(1)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or steps >= 50:
-            break
-    return {"value": spec.n * steps / el, "unit": "spin-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} Heun steps of the full {spec.nx}x{spec.ny}x{spec.nz} film after 1 warm-up step, "
-                      f"FP64, single thread, {el:.1f} s"}
+        steps, t0 = 0, time.perf_counter()
+        while True:
+            sim.heun(1)
+            steps += 1
+            el = time.perf_counter() - t0
+            if el > budget or steps >= 50:
+                break
+        return spec.n * steps / el, steps, el
+
+    v1, s1, e1 = timed(False, budget_s)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    O.Sim.set_threads(threads)
+    vt, st, et = timed(True, omp_budget_s)
+    return {"value": v1, "unit": "spin-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{s1} Heun steps of the full {spec.nx}x{spec.ny}x{spec.nz} film after 1 warm-up step, "
+                      f"FP64, single thread, {e1:.1f} s",
+            "cpu_model": _cpu_model(),
+            "openmp": {"value": vt, "unit": "spin-steps/s", "cores": threads,
+                       "sample": f"{st} Heun steps, same build with OpenMP, {et:.1f} s"}}
 
 
 BF16_DENSE_PEAK = 2.5e15   # MI355X published dense BF16 MFMA FLOP/s
