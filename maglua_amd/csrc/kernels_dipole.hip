@@ -24,7 +24,10 @@ namespace msd {
 // threads per workgroup of the runtime-shaped row kernels (compile-time shapes
 // carry their own count)
 static constexpr int ROW_TPB = 1024;
-static constexpr int COL_TPB = 512;
+// Threads per column workgroup: 256 for columns of up to 256 (one thread per ky, so
+// every thread takes part in the per-k product; measured +0.9% whole-step at
+// 256x256x8), 512 for longer columns (measured +1.1% at 512x512x4) and runtime lengths.
+__host__ __device__ constexpr int col_tpb(int ny) { return ny > 0 && ny <= 256 ? 256 : 512; }
 // exchange class tables up to this size are staged in LDS by the fused kernels
 static constexpr int EX_LDS_CLS = 64, EX_LDS_ENT = 512;
 
@@ -198,8 +201,9 @@ __global__ void __launch_bounds__(TPB) k_rows_inv(DipoleDev dp, float* __restric
 // its nz lines.  Three workgroups per kx (one per output component) give the
 // launch 3(nx/2+1) workgroups; the input column is read three times, from L2.
 template <int NY, int NZ>
-__global__ void __launch_bounds__(COL_TPB) k_cols(DipoleDev dp)
+__global__ void __launch_bounds__(col_tpb(NY)) k_cols(DipoleDev dp)
 {
+    constexpr int COL_TPB = col_tpb(NY);
     extern __shared__ float2 lds[];
     // the three workgroups of one kx run on one XCD, so two of them find the
     // column in that XCD's L2
@@ -546,12 +550,12 @@ static void launch_cols_nz(const DipoleDev& dp, hipStream_t st)
 {
 #define MSD_COL(NYV)                                                                \
     if (dp.ny == NYV) {                                                             \
-        k_cols<NYV, NZ><<<3 * dp.nxh, COL_TPB, dp.lds_cols, st>>>(dp);                  \
+        k_cols<NYV, NZ><<<3 * dp.nxh, col_tpb(NYV), dp.lds_cols, st>>>(dp);             \
         return;                                                                     \
     }
     MSD_COL_LENGTHS(MSD_COL)
 #undef MSD_COL
-    k_cols<0, NZ><<<3 * dp.nxh, COL_TPB, dp.lds_cols, st>>>(dp);
+    k_cols<0, NZ><<<3 * dp.nxh, col_tpb(0), dp.lds_cols, st>>>(dp);
 }
 
 hipError_t launch_cols(const DipoleDev& dp, hipStream_t st)
