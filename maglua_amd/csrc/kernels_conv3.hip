@@ -218,11 +218,19 @@ __host__ __device__ constexpr int rows_template_ty()
 {
     return FX >= 512 ? 4 : FX >= 256 ? 8 : FX >= 96 ? 16 : 32;
 }
+// threads per workgroup (measured: 256 at x length 512, +1.5% per 256^3 open step;
+// 256 at the ensemble's lengths 32..128 and 96 cost 10%, so those keep 512)
+template <int FX>
+__host__ __device__ constexpr int rows_template_threads()
+{
+    return FX >= 512 ? 256 : R3_TPB;
+}
 
 template <int MODE, bool COMP, int FX>
-__global__ void __launch_bounds__(R3_TPB) k3_rows_t(const Rep3* __restrict__ reps, const int4* __restrict__ items)
+__global__ void __launch_bounds__(rows_template_threads<FX>()) k3_rows_t(const Rep3* __restrict__ reps, const int4* __restrict__ items)
 {
     constexpr int TY = rows_template_ty<FX>(), LINES = (3 * TY + 1) / 2, FXH = FX / 2 + 1, HALF = FX / 2;
+    constexpr int NT = rows_template_threads<FX>();
     extern __shared__ float2 lds[];                     // [LINES][FX] lines, then [FX] twiddles
     __shared__ int s_start[64];
     __shared__ ExEntry s_ent[512];
@@ -240,9 +248,9 @@ __global__ void __launch_bounds__(R3_TPB) k3_rows_t(const Rep3* __restrict__ rep
 
     const ExchangeDev exl = rows_exchange<MODE>(a.ex, s_start, s_ent);
     if (lr)
-        for (int i = threadIdx.x; i < FX; i += R3_TPB) s_tw[i] = cv.twx[i];
+        for (int i = threadIdx.x; i < FX; i += NT) s_tw[i] = cv.twx[i];
     if (lr && MODE != R3_FWD) {
-        batched_copy<LINES * FX, R3_TPB, 8>(
+        batched_copy<LINES * FX, NT, 8>(
             [&](int idx) {
                 const int l = idx / FX, k = idx % FX;
                 return l < lines ? rows_load_pair(cv, nz, z, y0, ty, l, k > HALF ? FX - k : k)
@@ -250,7 +258,7 @@ __global__ void __launch_bounds__(R3_TPB) k3_rows_t(const Rep3* __restrict__ rep
             },
             [&](int idx, float4 v) { A[idx] = rows_pack(v, idx % FX > HALF); });
         __syncthreads();
-        fft_ip<FX, LINES, R3_TPB, 1>(A, s_tw);
+        fft_ip<FX, LINES, NT, 1>(A, s_tw);
     } else {
         __syncthreads();
     }
@@ -259,18 +267,18 @@ __global__ void __launch_bounds__(R3_TPB) k3_rows_t(const Rep3* __restrict__ rep
 
     constexpr int SITES = TY * FX;
 #pragma unroll 2
-    for (int li = threadIdx.x; li < SITES; li += R3_TPB) {
+    for (int li = threadIdx.x; li < SITES; li += NT) {
         const int r = li / FX, x = li % FX;
         if (r < ty) rows_site<MODE, COMP>(rp, exl, A, FX, ty, r, x, y0, z, draw);
     }
     if (!lr || MODE == R3_INV) return;
     // the unpaired imaginary half of the last line (odd 3*ty) must be zero
     if ((3 * ty) & 1)
-        for (int x = threadIdx.x; x < FX; x += R3_TPB)
+        for (int x = threadIdx.x; x < FX; x += NT)
             reinterpret_cast<float*>(A + (size_t)(lines - 1) * FX + x)[1] = 0.0f;
     __syncthreads();
-    fft_ip<FX, LINES, R3_TPB, -1>(A, s_tw);
-    for (int idx = threadIdx.x; idx < lines * FXH; idx += R3_TPB) {
+    fft_ip<FX, LINES, NT, -1>(A, s_tw);
+    for (int idx = threadIdx.x; idx < lines * FXH; idx += NT) {
         const int l = idx / FXH, k = idx - l * FXH;
         rows_store_pair(cv, nz, z, y0, ty, l, k, A[l * FX + k], A[l * FX + (k ? FX - k : 0)]);
     }
@@ -647,8 +655,9 @@ hipError_t launch3_rows(int mode, bool comp, const Rep3* reps, const int4* items
     if (nitems <= 0) return hipSuccess;
     const void* f = rows_kernel(mode, comp, rt);
     if (!f) return hipErrorInvalidValue;
+    const int nt = rt >= 512 ? rows_template_threads<512>() : R3_TPB;
     void* args[] = {(void*)&reps, (void*)&items};
-    return hipLaunchKernel(f, dim3(nitems), dim3(R3_TPB), args, lds, st);
+    return hipLaunchKernel(f, dim3(nitems), dim3(nt), args, lds, st);
 }
 
 hipError_t launch3_ypass(int sign, const Rep3* reps, const int4* items, int nitems, size_t lds, int yt, hipStream_t st)
