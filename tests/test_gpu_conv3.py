@@ -102,6 +102,30 @@ def test_longrange3d_custom_tensor_without_z_parity(dims, R):
     assert np.abs(hg - ho).max() <= 2e-5 * np.abs(ho).max()
 
 
+def test_open_custom_tensor():
+    """An open LongRange3D with a custom XY word that is not odd in y alone (the
+    lattice sum's is) against the oracle with the same doubled-grid tensor."""
+    dims = (8, 6, 32)
+    n = dims[0] * dims[1] * dims[2]
+    T = O.dipole_tensor3d(*dims, open=True)
+    gl = M.LongRange3D(*dims, open=True)
+    gl.setStrength(0.3)
+    for off in ((1, 2, 0), (-1, -2, 0)):    # one value at r and -r
+        gl.setMatrix("XY", off, 0.7)
+        T["XY"][off[2] % (2 * dims[2]), off[1] % (2 * dims[1]), off[0] % (2 * dims[0])] = 0.7
+    ol = O.LongRange3D(*dims, open=True, threads=True)
+    ol.setDipole(T)
+    ol.setStrength(0.3)
+    s = np.random.default_rng(n + 2).standard_normal((3, n))
+    gs, os_ = M.SpinSystem(*dims), O.SpinSystem(*dims, threads=True)
+    gs.setSpins(*s)
+    os_.setSpins(*s)
+    gl.apply(gs)
+    ol.apply(os_)
+    hg, ho = flat(gs.field("LongRange3D")), flat(os_.field("LongRange3D"))
+    assert np.abs(hg - ho).max() <= 2e-5 * np.abs(ho).max()
+
+
 def cube(dims, T, seed, R=10, J=1.0, K=0.1, g=0.05, dt=0.005, alpha=0.1):
     spec = W.FilmSpec(*dims, J=J, K=K, g=0.0, pbc=(1, 1, 1), alpha=alpha, dt=dt, T=T, seed=seed)
     gpu = W.build(M, spec, with_thermal=T != 0)
