@@ -315,7 +315,7 @@ __global__ void __launch_bounds__(Y3_TPB) k3_ypass(const Rep3* __restrict__ reps
     }
 }
 
-// Compile-time y length FY (a power of two, 32..512), W = 8 kx columns per
+// Compile-time y length FY (32..512 powers of two, and 96), W = 8 kx columns per
 // workgroup, transformed in place (fft_ip) in half the runtime pass's LDS.  Every y
 // row is read and written as 64-byte runs; with W's 128-byte rows and XCD-local
 // items the two blocks of a line meet in one L2 (measured faster than W = 16).

@@ -160,8 +160,9 @@ hipError_t launch_step_corr(const StepArgs& a, hipStream_t st);
 
 // ------------------------------------------------ 3D convolution (LongRange3D)
 // FFT grid (fx, fy, fz) equals the lattice when periodic and doubles it when
-// open (zero padding).  The working spectrum W is [3][nz][fy][fx/2+1]: planes
-// z >= nz of the transform are zero and never stored.
+// open (zero padding).  The working spectrum W is [3][nz][fy][fxp], kx < fx/2+1
+// used and rows padded to 128 bytes: planes z >= nz of the transform are zero and
+// never stored.
 // LDS lines that a pass fills by transposing copies are padded by this many
 // float2, which puts neighbouring lines on different banks.
 constexpr int CONV3_LPAD = 2;
