@@ -47,6 +47,7 @@ static struct {
     bool ready = false;
     int device = 0;
     hipStream_t stream = nullptr;
+    float2* part = nullptr;           // partial maxima of the field check [512]
     size_t max_lds = 65536;
 } G;
 
@@ -2205,13 +2206,12 @@ extern "C" int msd_surrogate_train_step(msd_surrogate* s, double lr, double* los
 static int field_maxdiff(const float* a, const float* b, size_t n3, double* maxdiff, double* maxb)
 {
     const int blocks = 512;
-    float2* part = nullptr;
-    if (int rc = dalloc(&part, blocks)) return rc;
-    hipError_t e = launch_cnn_maxdiff(n3, a, b, part, blocks, G.stream);
+    if (!G.part)    // kept for the process: the check runs at every learned window
+        if (int rc = dalloc(&G.part, blocks)) return rc;
+    hipError_t e = launch_cnn_maxdiff(n3, a, b, G.part, blocks, G.stream);
     std::vector<float2> h(blocks);
-    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part, sizeof(float2) * blocks, hipMemcpyDeviceToHost, G.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), G.part, sizeof(float2) * blocks, hipMemcpyDeviceToHost, G.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(G.stream);
-    dfree(part);
     if (e != hipSuccess) return fail(MSD_EDEVICE, "field check: %s", hipGetErrorString(e));
     double md = 0.0, mb = 0.0;
     for (const float2& v : h) { md = std::max(md, (double)v.x); mb = std::max(mb, (double)v.y); }
