@@ -235,10 +235,26 @@ def ensemble_leg(M, W, nrep=64, steps=50, R=16):
     E.step(steps)
     M.synchronize()
     el = time.perf_counter() - t0
+    # per-replica observables after the timed steps (outside the timing): |M| per
+    # site from netMoment and the total field's dot with the spins per site, both
+    # FP64 device reductions; the fields come from the operator API
+    m_abs, hdot = [], []
+    for (g, lr) in keep:
+        ss = g["ss"]
+        n = ss.nxyz()
+        m_abs.append(round(ss.netMoment()[3] / n, 6))
+        ss.resetFields()
+        g["ex"].apply(ss)
+        g["ani"].apply(ss)
+        lr.apply(ss)
+        ss.sumFields()
+        hdot.append(round(ss.fieldDot("Total") / n, 6))
     return {"workload": f"{nrep} periodic replicas, sizes from 32^3/64^3/128x128x16/96x96x24 (seed 0), "
                         f"T 0.1..2.0, LongRange3D lattice sum R={R}, g=0.01, J=1, K=0.1",
             "value": round(nsites * steps / el, 1), "unit": "spin-steps/s", "sites": nsites, "steps": steps,
-            "ms_per_step": round(1e3 * el / steps, 4), "setup_s": round(setup, 2)}
+            "ms_per_step": round(1e3 * el / steps, 4), "setup_s": round(setup, 2),
+            "temperatures": [round(t, 4) for t in temps], "m_abs_per_site": m_abs,
+            "total_field_dot_spin_per_site": hdot}
 
 
 def cube_learned(M, W, spec, g, ani, lr, heun, plain, train_steps=100, train_K=8, Ks=(8, 4, 2, 1), gate=1e-4,
