@@ -85,6 +85,26 @@ int msd_system_get_param(const msd_system* s, int which, double* v);
 int msd_system_set_site_alpha(msd_system* s, const double* a);
 int msd_system_set_site_gamma(msd_system* s, const double* g);
 
+/* Checkpoint byte stream.  MagLua's object encoding: int32 and IEEE double in
+ * little-endian order, objects opened by a magic char (111 new, 222 repeat) and
+ * their index in the stream's object list (common/luabaseobject/luabaseobject.cpp:140-220).
+ *
+ * hash32 (common/luabaseobject/factory.cpp:123-131): MurmurHash3_x86_32 of a type
+ * name with seed 1000, MagLua's type id ("SpinSystem", "Array.Double"). */
+int msd_hash32(const char* name, int32_t* out);
+/* SpinSystem::encode (cpu/core/spinsystem.cpp:669-760): the bytes the system appends to
+ * a checkpoint buffer whose object list already holds first_index objects (the system
+ * takes index first_index, its arrays the next ones; *n_objects counts them all).
+ * buf NULL: only *len (and *n_objects) are returned; else cap must be >= *len, and
+ * 242 + 48*n bytes always suffice.  Spins are the FP64 values of the compensated device
+ * state; ms and the site alpha/gamma arrays are their FP32 device values. */
+int msd_system_encode(msd_system* s, int first_index, void* buf, int64_t cap, int64_t* len, int* n_objects);
+/* SpinSystem::decode (cpu/core/spinsystem.cpp:764-846), read from the object's magic
+ * char (what msd_system_encode writes; the Lua type tag and type id before it belong to
+ * the container).  Creates *out; *used = bytes read.  Per-site extra Lua data and arrays
+ * that repeat an earlier object of the stream are MSD_EUNSUPPORTED. */
+int msd_system_decode(const void* buf, int64_t len, int first_index, msd_system** out, int64_t* used, int* n_objects);
+
 /* ss:resetFields() -- SpinSystem::zeroFields, spinsystem.cpp:968-980 */
 int msd_system_reset_fields(msd_system* s);
 /* ss:sumFields() -- SpinSystem::sumFields, spinsystem.cpp:853-871 */

@@ -5,6 +5,9 @@ include/msd_abi.h); this package is the host-side mirror of MagLua's Lua
 object API over that ABI.
 """
 from ._lib import MagLuaError, declared_symbols, lib  # noqa: F401
+from .checkpoint import (  # noqa: F401
+    Checkpointer, checkpointFromString, checkpointLoad, checkpointSave, checkpointToString,
+)
 from .maglua import (  # noqa: F401
     BUTCHER, LLG, AppliedField, Anisotropy, Dipole, Ensemble, Exchange, Heun, LongRange, LongRange3D, Random,
     SpinSystem,
@@ -18,4 +21,5 @@ __all__ = [
     "LLG", "Random", "Heun", "Ensemble", "Surrogate", "make_pc_step_function", "make_rk_step_function",
     "make_adapt_step_function", "BUTCHER",
     "synchronize", "philox_device", "lib", "declared_symbols",
+    "Checkpointer", "checkpointSave", "checkpointLoad", "checkpointToString", "checkpointFromString",
 ]

@@ -402,6 +402,218 @@ static int set_site_array(msd_system* s, float** dst, const double* a)
 extern "C" int msd_system_set_site_alpha(msd_system* s, const double* a) { NONNULL(s); return set_site_array(s, &s->site_alpha, a); }
 extern "C" int msd_system_set_site_gamma(msd_system* s, const double* g) { NONNULL(s); return set_site_array(s, &s->site_gamma, g); }
 
+// ------------------------------------------------------------------ checkpoint byte stream
+// MagLua's object encoding (common/luabaseobject/luabaseobject.cpp:140-170): int32 and
+// IEEE double in host (little-endian) order, chars as one byte.  An object opens with a
+// magic char (111: new to the stream, 222: written earlier) and its index in the stream's
+// object list (luabaseobject.h:63-72).
+enum : char { WIRE_NEW = 111, WIRE_OLD = (char)222 };
+enum : int { WIRE_LUA_TUSERDATA = 7 };
+
+// MurmurHash3_x86_32 (Austin Appleby's published, public-domain hash), seed 1000: MagLua's
+// type ids (common/luabaseobject/factory.cpp:123-131).  MagLua's copy rotates the tail
+// word by 16 where the published hash rotates by 15 (MurmurHash3.cpp:143), so names whose
+// length is not a multiple of 4 ("SpinSystem") hash differently; the ids follow MagLua.
+static uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+static uint32_t murmur3_x86_32(const uint8_t* p, size_t len, uint32_t seed)
+{
+    const uint32_t c1 = 0xcc9e2d51u, c2 = 0x1b873593u;
+    auto mix = [&](uint32_t k, int r) { k *= c1; k = rotl32(k, r); return k * c2; };
+    uint32_t h = seed;
+    size_t i = 0;
+    for (; i + 4 <= len; i += 4) {
+        uint32_t k;
+        std::memcpy(&k, p + i, 4);
+        h ^= mix(k, 15);
+        h = rotl32(h, 13) * 5 + 0xe6546b64u;
+    }
+    uint32_t k = 0;
+    for (size_t j = len - i; j > 0; --j) k = (k << 8) | p[i + j - 1];
+    if (len - i) h ^= mix(k, 16);
+    h ^= (uint32_t)len;
+    h ^= h >> 16; h *= 0x85ebca6bu;
+    h ^= h >> 13; h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+
+static int32_t type_id(const char* name) { return (int32_t)murmur3_x86_32((const uint8_t*)name, strlen(name), 1000u); }
+
+extern "C" int msd_hash32(const char* name, int32_t* out)
+{
+    NONNULL(name); NONNULL(out);
+    *out = type_id(name);
+    return MSD_OK;
+}
+
+struct WireOut {
+    std::vector<uint8_t> b;
+    void put(const void* p, size_t n) { const uint8_t* q = (const uint8_t*)p; b.insert(b.end(), q, q + n); }
+    void ch(char c) { put(&c, 1); }
+    void i32(int32_t v) { put(&v, 4); }
+    void f64(double v) { put(&v, 8); }
+};
+
+struct WireIn {
+    const uint8_t* p;
+    size_t n, at = 0;
+    bool get(void* d, size_t k)
+    {
+        if (k > n - at) return false;
+        std::memcpy(d, p + at, k);
+        at += k;
+        return true;
+    }
+    bool ch(char* c) { return get(c, 1); }
+    bool i32(int32_t* v) { return get(v, 4); }
+    bool f64(double* v) { return get(v, 8); }
+};
+
+// encodeT<dArray> (luabaseobject.h:101-106) then Array<double>::encode
+// (cpu/array/array_core_cpu.h:200-224): one value when all sites hold the same value,
+// else the raw doubles.
+static void wire_darray(WireOut& w, int index, const Dims& d, const double* v)
+{
+    w.i32(WIRE_LUA_TUSERDATA);
+    w.i32(type_id("Array.Double"));
+    w.ch(WIRE_NEW);
+    w.i32(index);
+    w.ch(0);
+    w.i32(d.nx); w.i32(d.ny); w.i32(d.nz);
+    bool same = true;
+    for (int i = 1; i < d.n && same; ++i) same = v[i] == v[0];
+    w.i32(same ? 1 : 0);
+    if (same) w.f64(v[0]);
+    else w.put(v, (size_t)d.n * sizeof(double));
+}
+
+// SpinSystem::encode (cpu/core/spinsystem.cpp:669-760).  Spins are the FP64 values of the
+// compensated device state (what msd_system_get_spins returns); site alpha/gamma and ms are
+// their FP32 device values widened.  No site carries extra Lua data, so the trailing
+// count is 0 (the "partial data" form with no entries).
+extern "C" int msd_system_encode(msd_system* s, int first_index, void* buf, int64_t cap, int64_t* len, int* n_objects)
+{
+    NONNULL(s); NONNULL(len);
+    if (first_index < 0) return fail(MSD_EINVAL, "SpinSystem.encode: object index %d < 0", first_index);
+    const size_t n = s->d.n;
+    std::vector<double> sp(4 * n);
+    if (int rc = msd_system_get_spins(s, sp.data(), sp.data() + n, sp.data() + 2 * n, sp.data() + 3 * n)) return rc;
+    WireOut w;
+    w.b.reserve(242 + 48 * n);
+    int idx = first_index;
+    w.ch(WIRE_NEW);
+    w.i32(idx++);
+    w.ch(0);
+    w.i32(s->d.nx); w.i32(s->d.ny); w.i32(s->d.nz);
+    w.f64(s->alpha); w.f64(s->dt); w.f64(s->gamma); w.f64(s->time);
+    for (int c = 0; c < 4; ++c) wire_darray(w, idx++, s->d, sp.data() + c * n);
+    std::vector<double> site(n);
+    for (float* a : {s->site_alpha, s->site_gamma}) {
+        w.i32(a ? 1 : 0);
+        if (!a) continue;
+        if (int rc = download_f32(site.data(), a, n)) return rc;
+        wire_darray(w, idx++, s->d, site.data());
+    }
+    w.i32(0);
+    *len = (int64_t)w.b.size();
+    if (n_objects) *n_objects = idx - first_index;
+    if (!buf) return MSD_OK;
+    if (cap < *len) return fail(MSD_EINVAL, "SpinSystem.encode: buffer of %lld bytes, %lld needed", (long long)cap, (long long)*len);
+    std::memcpy(buf, w.b.data(), w.b.size());
+    return MSD_OK;
+}
+
+// decodeT<dArray> (luabaseobject.cpp:303-350) then Array<double>::decode (array_core_cpu.h:226-252)
+static int unwire_darray(WireIn& r, int index, const Dims& d, double* v)
+{
+    int32_t tag, type, pos, nx, ny, nz, flag;
+    char magic, version;
+    if (!r.i32(&tag) || !r.i32(&type) || !r.ch(&magic) || !r.i32(&pos))
+        return fail(MSD_EINVAL, "SpinSystem.decode: stream ends inside an array header");
+    if (tag != WIRE_LUA_TUSERDATA || type != type_id("Array.Double"))
+        return fail(MSD_EINVAL, "SpinSystem.decode: expected an Array.Double (tag %d, type %d)", tag, type);
+    if (magic == WIRE_OLD)
+        return fail(MSD_EUNSUPPORTED, "SpinSystem.decode: array %d refers to an earlier object of the stream", pos);
+    if (magic != WIRE_NEW) return fail(MSD_EINVAL, "SpinSystem.decode: malformed stream (magic %d)", (int)magic);
+    if (pos != index) return fail(MSD_EINVAL, "SpinSystem.decode: object index %d, expected %d", pos, index);
+    if (!r.ch(&version) || !r.i32(&nx) || !r.i32(&ny) || !r.i32(&nz) || !r.i32(&flag))
+        return fail(MSD_EINVAL, "SpinSystem.decode: stream ends inside an array");
+    if (version != 0) return fail(MSD_EINVAL, "SpinSystem.decode: array version %d", (int)version);
+    if (!same_dims(d, nx, ny, nz))
+        return fail(MSD_EINVAL, "SpinSystem.decode: array of %dx%dx%d in a %dx%dx%d system", nx, ny, nz, d.nx, d.ny, d.nz);
+    if (flag == 1) {
+        double c;
+        if (!r.f64(&c)) return fail(MSD_EINVAL, "SpinSystem.decode: stream ends inside an array");
+        std::fill(v, v + d.n, c);
+        return MSD_OK;
+    }
+    if (flag != 0) return fail(MSD_EINVAL, "SpinSystem.decode: array flag %d", flag);
+    if (!r.get(v, (size_t)d.n * sizeof(double))) return fail(MSD_EINVAL, "SpinSystem.decode: stream ends inside an array");
+    return MSD_OK;
+}
+
+// SpinSystem::decode (cpu/core/spinsystem.cpp:764-846), entered at the object's magic
+// char so that it reads what msd_system_encode writes.
+extern "C" int msd_system_decode(const void* buf, int64_t len, int first_index, msd_system** out, int64_t* used, int* n_objects)
+{
+    NONNULL(buf); NONNULL(out);
+    *out = nullptr;
+    if (len < 0) return fail(MSD_EINVAL, "SpinSystem.decode: length %lld", (long long)len);
+    WireIn r{(const uint8_t*)buf, (size_t)len};
+    char magic, version;
+    int32_t pos, nx, ny, nz;
+    double alpha, dt, gamma, time;
+    if (!r.ch(&magic) || !r.i32(&pos)) return fail(MSD_EINVAL, "SpinSystem.decode: stream too short");
+    if (magic == WIRE_OLD)
+        return fail(MSD_EINVAL, "SpinSystem.decode: object %d was decoded earlier in the stream", pos);
+    if (magic != WIRE_NEW) return fail(MSD_EINVAL, "SpinSystem.decode: malformed stream (magic %d)", (int)magic);
+    if (pos != first_index) return fail(MSD_EINVAL, "SpinSystem.decode: object index %d, expected %d", pos, first_index);
+    if (!r.ch(&version)) return fail(MSD_EINVAL, "SpinSystem.decode: stream too short");
+    if (version != 0) return fail(MSD_EINVAL, "SpinSystem.decode: unknown version %d", (int)version);
+    if (!r.i32(&nx) || !r.i32(&ny) || !r.i32(&nz) || !r.f64(&alpha) || !r.f64(&dt) || !r.f64(&gamma) || !r.f64(&time))
+        return fail(MSD_EINVAL, "SpinSystem.decode: stream ends inside the header");
+    if (nx < 1 || ny < 1 || nz < 1 || (double)nx * ny * nz > (double)(1u << 28))
+        return fail(MSD_EINVAL, "SpinSystem.decode: lattice %dx%dx%d", nx, ny, nz);
+    const Dims d{nx, ny, nz, nx * ny * nz};
+    const size_t n = d.n;
+    int idx = first_index + 1;
+    std::vector<double> sp(4 * n);
+    for (int c = 0; c < 4; ++c)
+        if (int rc = unwire_darray(r, idx++, d, sp.data() + c * n)) return rc;
+    std::vector<double> site[2];
+    for (int k = 0; k < 2; ++k) {
+        int32_t has;
+        if (!r.i32(&has)) return fail(MSD_EINVAL, "SpinSystem.decode: stream ends before the site arrays");
+        if (!has) continue;
+        site[k].resize(n);
+        if (int rc = unwire_darray(r, idx++, d, site[k].data())) return rc;
+    }
+    int32_t extra;
+    if (!r.i32(&extra)) return fail(MSD_EINVAL, "SpinSystem.decode: stream ends before the extra-data count");
+    if (extra > 0) return fail(MSD_EUNSUPPORTED, "SpinSystem.decode: %d sites carry extra Lua data", extra);
+    if (extra < 0)   // every site written explicitly: accepted when each is nil
+        for (size_t i = 0; i < n; ++i) {
+            int32_t t;
+            if (!r.i32(&t)) return fail(MSD_EINVAL, "SpinSystem.decode: stream ends inside the extra data");
+            if (t != 0) return fail(MSD_EUNSUPPORTED, "SpinSystem.decode: site %zu carries extra Lua data", i);
+        }
+    msd_system* s = nullptr;
+    if (int rc = msd_system_create(nx, ny, nz, &s)) return rc;
+    std::unique_ptr<msd_system> own(s);
+    s->alpha = alpha; s->dt = dt; s->gamma = gamma; s->time = time;
+    if (int rc = msd_system_set_spins(s, sp.data(), sp.data() + n, sp.data() + 2 * n, sp.data() + 3 * n)) return rc;
+    if (!site[0].empty())
+        if (int rc = set_site_array(s, &s->site_alpha, site[0].data())) return rc;
+    if (!site[1].empty())
+        if (int rc = set_site_array(s, &s->site_gamma, site[1].data())) return rc;
+    HIPCHK(hipStreamSynchronize(G.stream));
+    if (used) *used = (int64_t)r.at;
+    if (n_objects) *n_objects = idx - first_index;
+    *out = own.release();
+    return MSD_OK;
+}
+
 extern "C" int msd_system_reset_fields(msd_system* s)
 {
     NONNULL(s);

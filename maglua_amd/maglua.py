@@ -110,6 +110,24 @@ class SpinSystem(_Handle):
     def copySpinsTo(self, other):
         call("msd_system_copy_spins_to", self._h, other.h)
 
+    # checkpoint byte stream (SpinSystem::encode/decode, cpu/core/spinsystem.cpp:669-846)
+    def encode(self, first_index=0):
+        """The bytes SpinSystem::encode appends to a checkpoint buffer holding first_index
+        objects already; returns (bytes, number of objects the system and its arrays take)."""
+        cap = 242 + 48 * self.nxyz()
+        buf = C.create_string_buffer(cap)
+        n, k = C.c_int64(), C.c_int()
+        call("msd_system_encode", self._h, int(first_index), buf, cap, C.byref(n), C.byref(k))
+        return buf.raw[:n.value], k.value
+
+    @classmethod
+    def decode(cls, data, first_index=0):
+        """Inverse of encode: returns (SpinSystem, bytes read, number of objects)."""
+        data = bytes(data)
+        p, used, k = _vp(), C.c_int64(), C.c_int()
+        call("msd_system_decode", data, len(data), int(first_index), C.byref(p), C.byref(used), C.byref(k))
+        return cls(_h=p.value), used.value, k.value
+
     # spins
     def setSpin(self, site, s, m=None):
         call("msd_system_set_spin", self._h, _index(site, self._dims), float(s[0]), float(s[1]), float(s[2]),

@@ -1,0 +1,54 @@
+"""Hand-assembled checkpoint byte streams, field by field from the reference's encoders
+(data for the checkpoint tests: each line is one encoder call of the reference).
+
+SPINSYSTEM_2x1x1: SpinSystem::encode (cpu/core/spinsystem.cpp:669-760) of a 2x1x1 system
+with alpha 0.5, dt 0.01, gamma 1, time 0.25, spins (1,0,0) and (0,0,1), ms 1, no site
+alpha/gamma, as the first object of a buffer.
+VALUES_RAW: _exportLuaVariable (common/luabaseobject/luamigrate.cpp:57-286) of the values
+1.5, "ab", true, nil, {2.0}; VALUES_PACKED: the same after the "Pack" transform
+(common/checkpoint/checkpointer.cpp:455-479)."""
+import struct
+
+
+def i(v):
+    return struct.pack("<i", v)
+
+
+def d(v):
+    return struct.pack("<d", v)
+
+
+NEW = b"\x6f"               # ENCODE_MAGIC_NEW, 111 (luabaseobject.h:64)
+SPINSYSTEM_ID = -687101837  # hash32("SpinSystem")
+ARRAY_DOUBLE_ID = -1895565407  # hash32("Array.Double")
+
+SPINSYSTEM_2x1x1 = b"".join([
+    NEW, i(0),                               # ENCODE_PREAMBLE: new object, index 0
+    b"\x00",                                 # version
+    i(2), i(1), i(1),                        # nx ny nz
+    d(0.5), d(0.01), d(1.0), d(0.25),        # alpha dt gamma time
+    # x: encodeT<dArray> (LUA_TUSERDATA, type) then Array::encode: preamble, version,
+    # dims, flag 0 and the raw values
+    i(7), i(ARRAY_DOUBLE_ID), NEW, i(1), b"\x00", i(2), i(1), i(1), i(0), d(1.0), d(0.0),
+    # y: every site equal -> flag 1 and the one value
+    i(7), i(ARRAY_DOUBLE_ID), NEW, i(2), b"\x00", i(2), i(1), i(1), i(1), d(0.0),
+    # z
+    i(7), i(ARRAY_DOUBLE_ID), NEW, i(3), b"\x00", i(2), i(1), i(1), i(0), d(0.0), d(1.0),
+    # ms
+    i(7), i(ARRAY_DOUBLE_ID), NEW, i(4), b"\x00", i(2), i(1), i(1), i(1), d(1.0),
+    i(0),                                    # site_alpha_exists
+    i(0),                                    # site_gamma_exists
+    i(0),                                    # extra data: partial form, 0 entries
+])
+
+VALUES_RAW = b"".join([
+    i(3), d(1.5),                            # LUA_TNUMBER
+    i(4), i(3), b"ab\x00",                   # LUA_TSTRING: strlen + 1, bytes, NUL
+    i(1), i(1),                              # LUA_TBOOLEAN
+    i(0),                                    # LUA_TNIL
+    i(5), NEW, i(0), i(1),                   # LUA_TTABLE: new, 0, one pair
+    i(3), d(1.0), i(3), d(2.0),              #   [1] = 2
+    i(0),                                    #   metatable: nil
+])
+
+VALUES_PACKED = b"LUA\x00" + i(0) + i(-1) + i(20 + 4 + len(VALUES_RAW)) + i(len(VALUES_RAW)) + i(5) + VALUES_RAW

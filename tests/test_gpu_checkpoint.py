@@ -1,0 +1,123 @@
+"""GPU checks of SpinSystem::encode/decode through the C ABI: the stream of a 2x1x1
+system against the hand-assembled one, random systems against the oracle's stream,
+decode of the reference's other forms, error paths, and a fused Heun run resumed from
+a checkpoint file against the same run uninterrupted."""
+import struct
+
+import numpy as np
+import pytest
+
+import maglua_amd as M
+from maglua_amd import workloads as W
+from oracle import checkpoint_oracle as O
+from tests.golden import checkpoint_streams as G
+
+pytestmark = pytest.mark.gpu
+
+
+def flat(t):
+    return np.stack([np.asarray(v).reshape(-1) for v in t])
+
+
+def test_encode_golden_2x1x1():
+    ss = M.SpinSystem(2, 1, 1)
+    ss.setSpins([1.0, 0.0], [0.0, 0.0], [0.0, 1.0])
+    ss.setAlpha(0.5)
+    ss.setTimeStep(0.01)
+    ss.setGamma(1.0)
+    ss.setTime(0.25)
+    b, k = ss.encode()
+    assert b == G.SPINSYSTEM_2x1x1 and k == 5
+
+
+def test_encode_matches_oracle_and_round_trips():
+    dims = (6, 5, 3)
+    n = 90
+    ss = M.SpinSystem(*dims)
+    ss.randomize(11)
+    ss.setAlpha(0.1)
+    ss.setGamma(1.7)
+    ss.setTimeStep(0.003)
+    ss.setTime(12.5)
+    sa = np.random.default_rng(2).uniform(0.05, 0.5, n)
+    ss.setSiteAlpha(sa)
+    x, y, z, m = flat(ss.spins())
+    sa32 = sa.astype(np.float32).astype(np.float64)     # the device holds FP32 site values
+    want, k = O.spinsystem_bytes(dims, 0.1, 0.003, 1.7, 12.5, x, y, z, m, site_alpha=sa32, first_index=3)
+    b, kk = ss.encode(first_index=3)
+    assert b == want and kk == k == 6
+    ss2, used, k2 = M.SpinSystem.decode(b, first_index=3)
+    assert used == len(b) and k2 == 6
+    assert ss2.encode(first_index=3)[0] == b
+    assert np.array_equal(flat(ss2.spins()), flat(ss.spins()))
+    assert (ss2.alpha(), ss2.gamma(), ss2.timeStep(), ss2.time()) == (0.1, 1.7, 0.003, 12.5)
+
+
+def test_decode_reference_forms():
+    """Uniform arrays (flag 1), a site gamma array and the explicit extra-data form."""
+    dims = (4, 2, 2)
+    n = 16
+    rng = np.random.default_rng(5)
+    v = rng.normal(size=(3, n))
+    v /= np.linalg.norm(v, axis=0)
+    sg = rng.integers(1, 8, n) * 0.25                    # FP32-exact values
+    b, k = O.spinsystem_bytes(dims, 0.2, 0.01, 1.0, 3.0, v[0], v[1], v[2], np.full(n, 1.0), site_gamma=sg,
+                              first_index=2, extra="all")
+    ss, used, kk = M.SpinSystem.decode(b + b"trailing", first_index=2)
+    assert used == len(b) and kk == k == 6
+    got = flat(ss.spins())
+    assert np.max(np.abs(got[:3] - v)) < 1e-12 and np.all(got[3] == 1.0)
+    want, _ = O.spinsystem_bytes(dims, 0.2, 0.01, 1.0, 3.0, *got, site_gamma=sg, first_index=2)
+    assert ss.encode(first_index=2)[0] == want
+
+
+def test_decode_errors():
+    b = G.SPINSYSTEM_2x1x1
+    with pytest.raises(M.MagLuaError, match="ends"):
+        M.SpinSystem.decode(b[:-6])
+    with pytest.raises(M.MagLuaError, match="index"):
+        M.SpinSystem.decode(b, first_index=1)
+    with pytest.raises(M.MagLuaError, match="extra Lua data"):
+        M.SpinSystem.decode(b[:-4] + struct.pack("<i", 1))
+    with pytest.raises(M.MagLuaError, match="earlier"):
+        M.SpinSystem.decode(b"\xde" + b[1:])
+    with pytest.raises(M.MagLuaError, match="version"):
+        M.SpinSystem.decode(b[:5] + b"\x01" + b[6:])
+
+
+def test_checkpointer_with_systems(tmp_path):
+    """Two systems and a repeat in one file: the second system's objects follow the
+    first's in the object list, and the repeat comes back as the same object."""
+    a, b = M.SpinSystem(3, 2, 1), M.SpinSystem(2, 2, 2)
+    a.randomize(1)
+    b.randomize(2)
+    fn = str(tmp_path / "two.dat")
+    M.checkpointSave(fn, a, 5.0, b, a)
+    ra, five, rb, ra2 = M.checkpointLoad(fn)
+    assert ra2 is ra and five == 5.0
+    assert np.array_equal(flat(ra.spins()), flat(a.spins()))
+    assert np.array_equal(flat(rb.spins()), flat(b.spins()))
+    s = M.checkpointToString(b)
+    (rb2,) = M.checkpointFromString(s)
+    assert np.array_equal(flat(rb2.spins()), flat(b.spins()))
+
+
+def test_heun_resumed_from_checkpoint(tmp_path):
+    spec = W.FilmSpec(16, 16, 2, J=1.0, K=0.1, g=0.01, pbc=(1, 1, 0), alpha=0.1, dt=0.005, T=0.0, seed=0)
+    g = W.build(M, spec)
+    ss0 = g["ss"]
+    ss0.randomize(4)
+    ops = (g["ex"], g["ani"], g["dip"], None, g["llg"])
+    sa, sb = ss0.copy(), ss0.copy()
+    M.Heun(sa, *ops).step(20)
+    M.Heun(sb, *ops).step(10)
+    fn = str(tmp_path / "resume.dat")
+    M.checkpointSave(fn, sb, "after 10 steps")
+    sc, tag = M.checkpointLoad(fn)
+    assert tag == "after 10 steps"
+    M.Heun(sc, *ops).step(10)
+    M.synchronize()
+    d = np.max(np.abs(flat(sc.spins()) - flat(sa.spins())))
+    print(f"resume: max|dm| after 10 + 10 steps vs 20 = {d:.3e}", flush=True)
+    assert d == 0.0
+    assert sc.time() == sa.time()
