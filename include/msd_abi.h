@@ -104,6 +104,18 @@ int msd_system_encode(msd_system* s, int first_index, void* buf, int64_t cap, in
  * the container).  Creates *out; *used = bytes read.  Per-site extra Lua data and arrays
  * that repeat an earlier object of the stream are MSD_EUNSUPPORTED. */
 int msd_system_decode(const void* buf, int64_t len, int first_index, msd_system** out, int64_t* used, int* n_objects);
+/* Exchange::encode/decode (cpu/exchange/spinoperationexchange.cpp:52-113, with
+ * SpinOperation::encode/decode, cpu/core/spinoperation.cpp:67-93): scale, normalisation
+ * flag, pbc and the pathways in the order they were added; one object.  Same buffer and
+ * index conventions as msd_system_encode/decode. */
+int msd_exchange_encode(msd_exchange* e, int first_index, void* buf, int64_t cap, int64_t* len, int* n_objects);
+int msd_exchange_decode(const void* buf, int64_t len, int first_index, msd_exchange** out, int64_t* used, int* n_objects);
+/* Dipole::encode/decode (cpu/dipole/spinoperationdipole.cpp:35-59 around
+ * cpu/longrange/spinoperationlongrange.cpp:55-92): scale, truncation, strength and unit
+ * cell; the tensor is rebuilt from them (a matrix from setMatrix is not in the stream,
+ * as in MagLua). */
+int msd_dipole_encode(msd_dipole* d, int first_index, void* buf, int64_t cap, int64_t* len, int* n_objects);
+int msd_dipole_decode(const void* buf, int64_t len, int first_index, msd_dipole** out, int64_t* used, int* n_objects);
 
 /* ss:resetFields() -- SpinSystem::zeroFields, spinsystem.cpp:968-980 */
 int msd_system_reset_fields(msd_system* s);

@@ -67,6 +67,10 @@ _SIGS = {
     "msd_hash32": (C.c_int, [C.c_char_p, C.POINTER(C.c_int32)]),
     "msd_system_encode": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, C.POINTER(C.c_int64), _ip]),
     "msd_system_decode": (C.c_int, [_vp, C.c_int64, C.c_int, _pp, C.POINTER(C.c_int64), _ip]),
+    "msd_exchange_encode": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, C.POINTER(C.c_int64), _ip]),
+    "msd_exchange_decode": (C.c_int, [_vp, C.c_int64, C.c_int, _pp, C.POINTER(C.c_int64), _ip]),
+    "msd_dipole_encode": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, C.POINTER(C.c_int64), _ip]),
+    "msd_dipole_decode": (C.c_int, [_vp, C.c_int64, C.c_int, _pp, C.POINTER(C.c_int64), _ip]),
     "msd_system_set_site_gamma": (C.c_int, [_vp, _dp]),
     "msd_system_reset_fields": (C.c_int, [_vp]),
     "msd_system_sum_fields": (C.c_int, [_vp]),

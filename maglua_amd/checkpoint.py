@@ -20,8 +20,8 @@ level 9 of the packed buffer) and "Base64" (the header and data as base64 text, 
 opens with QjY).
 
 Python values map to Lua values as: None - nil, bool - boolean, int/float - number,
-str - string (UTF-8), list/tuple/dict - table (lists take keys 1..n), SpinSystem -
-userdata.  Decoded tables come back as dicts, numbers as floats (integral table keys as
+str - string (UTF-8), list/tuple/dict - table (lists take keys 1..n), SpinSystem,
+Exchange and Dipole - userdata.  Decoded tables come back as dicts, numbers as floats (integral table keys as
 ints).  Table pairs are written in list order or dict insertion order; Lua writes them in
 its own table-traversal order, so streams holding tables with several keys match
 MagLua's bytes only where the two orders agree.
@@ -33,7 +33,10 @@ import struct
 import zlib
 
 from ._lib import MagLuaError, MSD_EINVAL, MSD_EUNSUPPORTED, call
-from .maglua import SpinSystem
+from .maglua import Dipole, Exchange, SpinSystem
+
+# userdata this container writes and reads, by MagLua type name
+_USERDATA = (("SpinSystem", SpinSystem), ("Exchange", Exchange), ("Dipole", Dipole))
 
 LUA_TNIL, LUA_TBOOLEAN, LUA_TNUMBER, LUA_TSTRING, LUA_TTABLE, LUA_TUSERDATA = 0, 1, 3, 4, 5, 7
 MAGIC_NEW, MAGIC_OLD = 111, 222
@@ -182,8 +185,9 @@ class Checkpointer:
                 self._export(k, out)
                 self._export(x, out)
             out += struct.pack("<i", LUA_TNIL)   # no metatable
-        elif isinstance(v, SpinSystem):
-            out += struct.pack("<ii", LUA_TUSERDATA, hash32("SpinSystem"))
+        elif isinstance(v, (SpinSystem, Exchange, Dipole)):
+            name = next(nm for nm, cls in _USERDATA if type(v) is cls)
+            out += struct.pack("<ii", LUA_TUSERDATA, hash32(name))
             for k, o in enumerate(self._objs):
                 if o is v:
                     out += struct.pack("<Bi", MAGIC_OLD, k)
@@ -225,7 +229,8 @@ class Checkpointer:
             return tab
         if t == LUA_TUSERDATA:
             type_id = r.i32()
-            if type_id != hash32("SpinSystem"):
+            cls = next((c for nm, c in _USERDATA if hash32(nm) == type_id), None)
+            if cls is None:
                 raise _err(f"Checkpointer: userdata of type id {type_id} is not supported", MSD_EUNSUPPORTED)
             if r.d[r.at] == MAGIC_OLD:
                 r.take(1)
@@ -233,10 +238,10 @@ class Checkpointer:
                 if not 0 <= pos < len(objs) or objs[pos] is None:
                     raise _err("Checkpointer: malformed data stream (object index)")
                 return objs[pos]
-            ss, used, k = SpinSystem.decode(r.d[r.at:], len(objs))
+            o, used, k = cls.decode(r.d[r.at:], len(objs))
             r.at += used
-            objs += [ss] + [None] * (k - 1)
-            return ss
+            objs += [o] + [None] * (k - 1)
+            return o
         raise _err(f"Checkpointer: Lua type {t} is not supported", MSD_EUNSUPPORTED)
 
     def add(self, v):

@@ -470,6 +470,41 @@ struct WireIn {
     bool f64(double* v) { return get(v, 8); }
 };
 
+static int wire_finish(const WireOut& w, const char* who, void* buf, int64_t cap, int64_t* len)
+{
+    *len = (int64_t)w.b.size();
+    if (!buf) return MSD_OK;
+    if (cap < *len) return fail(MSD_EINVAL, "%s: buffer of %lld bytes, %lld needed", who, (long long)cap, (long long)*len);
+    std::memcpy(buf, w.b.data(), w.b.size());
+    return MSD_OK;
+}
+
+// ENCODE_PREAMBLE of a new object, then SpinOperation::encode (cpu/core/spinoperation.cpp:67-75)
+static void wire_operator(WireOut& w, int index, const Dims& d, double scale)
+{
+    w.ch(WIRE_NEW);
+    w.i32(index);
+    w.ch(0);
+    w.i32(d.nx); w.i32(d.ny); w.i32(d.nz);
+    w.f64(scale);
+}
+
+// the preamble as decodeLuaBaseObject reads it (luabaseobject.cpp:303-350), then
+// SpinOperation::decode (spinoperation.cpp:77-93)
+static int unwire_operator(WireIn& r, const char* who, int index, int* nx, int* ny, int* nz, double* scale)
+{
+    char magic, version;
+    int32_t pos;
+    if (!r.ch(&magic) || !r.i32(&pos)) return fail(MSD_EINVAL, "%s: stream too short", who);
+    if (magic == WIRE_OLD) return fail(MSD_EINVAL, "%s: object %d was decoded earlier in the stream", who, pos);
+    if (magic != WIRE_NEW) return fail(MSD_EINVAL, "%s: malformed stream (magic %d)", who, (int)magic);
+    if (pos != index) return fail(MSD_EINVAL, "%s: object index %d, expected %d", who, pos, index);
+    if (!r.ch(&version) || !r.i32(nx) || !r.i32(ny) || !r.i32(nz) || !r.f64(scale))
+        return fail(MSD_EINVAL, "%s: stream ends inside the operator header", who);
+    if (version != 0) return fail(MSD_EINVAL, "%s: unknown operator version %d", who, (int)version);
+    return MSD_OK;
+}
+
 // encodeT<dArray> (luabaseobject.h:101-106) then Array<double>::encode
 // (cpu/array/array_core_cpu.h:200-224): one value when all sites hold the same value,
 // else the raw doubles.
@@ -883,6 +918,65 @@ extern "C" int msd_exchange_add_paths(msd_exchange* e, const int* from, const in
     for (int i = 0; i < count; ++i)
         if (J[i] != 0) e->paths.push_back({from[i], to[i], J[i]});
     e->dirty = true;
+    return MSD_OK;
+}
+
+// Exchange::encode (cpu/exchange/spinoperationexchange.cpp:52-72): version 1, the
+// normalisation flag, pbc and the pathways in the order they were added
+extern "C" int msd_exchange_encode(msd_exchange* e, int first_index, void* buf, int64_t cap, int64_t* len, int* n_objects)
+{
+    NONNULL(e); NONNULL(len);
+    if (first_index < 0) return fail(MSD_EINVAL, "Exchange.encode: object index %d < 0", first_index);
+    WireOut w;
+    w.b.reserve(64 + 16 * e->paths.size());
+    wire_operator(w, first_index, e->d, e->scale);
+    w.ch(1);
+    w.i32(e->normalize ? 1 : 0);
+    for (int k = 0; k < 3; ++k) w.i32(e->pbc[k]);
+    w.i32((int32_t)e->paths.size());
+    for (const auto& p : e->paths) { w.i32(p.from); w.i32(p.to); w.f64(p.J); }
+    if (n_objects) *n_objects = 1;
+    return wire_finish(w, "Exchange.encode", buf, cap, len);
+}
+
+// Exchange::decode (spinoperationexchange.cpp:74-113): a version-0 stream has no
+// normalisation flag and means unnormalised exchange
+extern "C" int msd_exchange_decode(const void* buf, int64_t len, int first_index, msd_exchange** out, int64_t* used, int* n_objects)
+{
+    NONNULL(buf); NONNULL(out);
+    *out = nullptr;
+    if (len < 0) return fail(MSD_EINVAL, "Exchange.decode: length %lld", (long long)len);
+    WireIn r{(const uint8_t*)buf, (size_t)len};
+    int nx, ny, nz;
+    double scale;
+    if (int rc = unwire_operator(r, "Exchange.decode", first_index, &nx, &ny, &nz, &scale)) return rc;
+    char version;
+    int32_t norm = 0, pbc[3], num;
+    if (!r.ch(&version)) return fail(MSD_EINVAL, "Exchange.decode: stream too short");
+    if (version < 0) return fail(MSD_EINVAL, "Exchange.decode: unknown version %d", (int)version);
+    if (version >= 1 && !r.i32(&norm)) return fail(MSD_EINVAL, "Exchange.decode: stream too short");
+    if (!r.i32(&pbc[0]) || !r.i32(&pbc[1]) || !r.i32(&pbc[2]) || !r.i32(&num))
+        return fail(MSD_EINVAL, "Exchange.decode: stream ends inside the header");
+    if (num < 0 || (size_t)num > (r.n - r.at) / 16) return fail(MSD_EINVAL, "Exchange.decode: %d pathways do not fit the stream", num);
+    msd_exchange* e = nullptr;
+    if (int rc = msd_exchange_create(nx, ny, nz, &e)) return rc;
+    std::unique_ptr<msd_exchange> own(e);
+    e->scale = scale;
+    e->normalize = norm != 0;
+    for (int k = 0; k < 3; ++k) e->pbc[k] = pbc[k];
+    e->paths.reserve(num);
+    for (int i = 0; i < num; ++i) {
+        int32_t f, t;
+        double J;
+        r.i32(&f); r.i32(&t); r.f64(&J);
+        if (f < 0 || f >= e->d.n || t < 0 || t >= e->d.n)
+            return fail(MSD_EINVAL, "Exchange.decode: pathway %d has a site outside 0..%d", i, e->d.n - 1);
+        e->paths.push_back({f, t, J});
+    }
+    e->dirty = true;
+    if (used) *used = (int64_t)r.at;
+    if (n_objects) *n_objects = 1;
+    *out = own.release();
     return MSD_OK;
 }
 
@@ -1434,6 +1528,58 @@ extern "C" int msd_dipole_set_unit_cell(msd_dipole* d, const double ABC[9])
     NONNULL(d); NONNULL(ABC);
     for (int i = 0; i < 9; ++i) d->ABC[i] = ABC[i];
     d->loaded = false; d->custom = false; d->spectrum = false;
+    return MSD_OK;
+}
+
+// Dipole::encode (cpu/dipole/spinoperationdipole.cpp:35-41) around LongRange::encode
+// (cpu/longrange/spinoperationlongrange.cpp:55-68): the truncation, strength and unit
+// cell from which the tensor is rebuilt; a matrix set by setMatrix is not in the stream
+extern "C" int msd_dipole_encode(msd_dipole* d, int first_index, void* buf, int64_t cap, int64_t* len, int* n_objects)
+{
+    NONNULL(d); NONNULL(len);
+    if (first_index < 0) return fail(MSD_EINVAL, "Dipole.encode: object index %d < 0", first_index);
+    WireOut w;
+    wire_operator(w, first_index, d->d, d->scale);
+    w.ch(0);
+    for (int k = 0; k < 4; ++k) w.i32(d->gmax[k]);
+    w.f64(d->g);
+    for (int k = 0; k < 9; ++k) w.f64(d->ABC[k]);
+    w.ch(0);
+    if (n_objects) *n_objects = 1;
+    return wire_finish(w, "Dipole.encode", buf, cap, len);
+}
+
+// Dipole::decode (spinoperationdipole.cpp:43-59) around LongRange::decode (:70-92)
+extern "C" int msd_dipole_decode(const void* buf, int64_t len, int first_index, msd_dipole** out, int64_t* used, int* n_objects)
+{
+    NONNULL(buf); NONNULL(out);
+    *out = nullptr;
+    if (len < 0) return fail(MSD_EINVAL, "Dipole.decode: length %lld", (long long)len);
+    WireIn r{(const uint8_t*)buf, (size_t)len};
+    int nx, ny, nz;
+    double scale, g, ABC[9];
+    if (int rc = unwire_operator(r, "Dipole.decode", first_index, &nx, &ny, &nz, &scale)) return rc;
+    char v1, v2;
+    int32_t gmax[4];
+    if (!r.ch(&v1)) return fail(MSD_EINVAL, "Dipole.decode: stream too short");
+    if (v1 != 0) return fail(MSD_EINVAL, "Dipole.decode: unknown LongRange version %d", (int)v1);
+    for (int k = 0; k < 4; ++k)
+        if (!r.i32(&gmax[k])) return fail(MSD_EINVAL, "Dipole.decode: stream ends inside the truncation");
+    if (!r.f64(&g)) return fail(MSD_EINVAL, "Dipole.decode: stream too short");
+    for (int k = 0; k < 9; ++k)
+        if (!r.f64(&ABC[k])) return fail(MSD_EINVAL, "Dipole.decode: stream ends inside the unit cell");
+    if (!r.ch(&v2)) return fail(MSD_EINVAL, "Dipole.decode: stream too short");
+    if (v2 != 0) return fail(MSD_EINVAL, "Dipole.decode: unknown version %d", (int)v2);
+    msd_dipole* d = nullptr;
+    if (int rc = msd_dipole_create(nx, ny, nz, &d)) return rc;
+    std::unique_ptr<msd_dipole> own(d);
+    if (int rc = msd_dipole_set_truncation(d, gmax)) return rc;
+    if (int rc = msd_dipole_set_unit_cell(d, ABC)) return rc;
+    d->g = g;
+    d->scale = scale;
+    if (used) *used = (int64_t)r.at;
+    if (n_objects) *n_objects = 1;
+    *out = own.release();
     return MSD_OK;
 }
 

@@ -13,6 +13,7 @@ are 1).  Failures raise MagLuaError carrying msd_last_error(), where MagLua
 raises luaL_error.  All state lives on the GPU; there is no CPU path.
 """
 import ctypes as C
+import struct
 
 import numpy as np
 
@@ -67,6 +68,26 @@ class _Handle:
             except Exception:
                 pass
             self._h = None
+
+
+def _encode_handle(fn, h, first_index):
+    """An object's checkpoint bytes through msd_*_encode: (bytes, objects taken)."""
+    n, k = C.c_int64(), C.c_int()
+    call(fn, h, int(first_index), None, 0, C.byref(n), C.byref(k))
+    buf = C.create_string_buffer(max(n.value, 1))
+    call(fn, h, int(first_index), buf, n.value, C.byref(n), C.byref(k))
+    return buf.raw[:n.value], k.value
+
+
+def _decode_handle(cls, fn, data, first_index):
+    """A new cls wrapping the handle msd_*_decode makes: (object, bytes read, objects taken)."""
+    data = bytes(data)
+    p, used, k = _vp(), C.c_int64(), C.c_int()
+    call(fn, data, len(data), int(first_index), C.byref(p), C.byref(used), C.byref(k))
+    o = cls.__new__(cls)
+    _Handle.__init__(o, p.value)
+    o._dims = struct.unpack_from("<3i", data, 6)
+    return o, used.value, k.value
 
 
 # ---------------------------------------------------------------- SpinSystem
@@ -286,6 +307,14 @@ class Exchange(_Handle):
         super().__init__(p.value)
 
     new = classmethod(lambda cls, *a: cls(*a))
+
+    # checkpoint byte stream (Exchange::encode/decode, spinoperationexchange.cpp:52-113)
+    def encode(self, first_index=0):
+        return _encode_handle("msd_exchange_encode", self._h, first_index)
+
+    @classmethod
+    def decode(cls, data, first_index=0):
+        return _decode_handle(cls, "msd_exchange_decode", data, first_index)
 
     def setPeriodicXYZ(self, px, py, pz):
         call("msd_exchange_set_pbc", self._h, int(bool(px)), int(bool(py)), int(bool(pz)))
@@ -620,6 +649,17 @@ class Dipole(_Handle):
         self._gmax = (1000, 1000, 1000, 1000)
 
     new = classmethod(lambda cls, *a: cls(*a))
+
+    # checkpoint byte stream (Dipole::encode/decode, spinoperationdipole.cpp:35-59)
+    def encode(self, first_index=0):
+        return _encode_handle("msd_dipole_encode", self._h, first_index)
+
+    @classmethod
+    def decode(cls, data, first_index=0):
+        o, used, k = _decode_handle(cls, "msd_dipole_decode", data, first_index)
+        *gmax, g = struct.unpack_from("<4id", bytes(data), 27)
+        o._gmax, o._g = tuple(gmax), g
+        return o, used, k
 
     def setStrength(self, g):
         self._g = float(g)

@@ -52,3 +52,27 @@ VALUES_RAW = b"".join([
 ])
 
 VALUES_PACKED = b"LUA\x00" + i(0) + i(-1) + i(20 + 4 + len(VALUES_RAW)) + i(len(VALUES_RAW)) + i(5) + VALUES_RAW
+
+# Exchange::encode (cpu/exchange/spinoperationexchange.cpp:52-72) of a 2x1x1 Exchange with
+# normalisation on, pbc (1,1,1) and one pathway 0 -> 1 of strength 0.5, as object 0
+EXCHANGE_2x1x1 = b"".join([
+    NEW, i(0),                               # ENCODE_PREAMBLE
+    b"\x00", i(2), i(1), i(1), d(1.0),       # SpinOperation::encode: version, dims, scale
+    b"\x01",                                 # version 1
+    i(1),                                    # normalizeMoments
+    i(1), i(1), i(1),                        # pbc
+    i(1),                                    # pathway count
+    i(0), i(1), d(0.5),                      # from, to, strength
+])
+
+# Dipole::encode (cpu/dipole/spinoperationdipole.cpp:35-41) of a 2x2x1 Dipole with
+# strength 0.25, the default truncation and the identity unit cell, as object 3
+DIPOLE_2x2x1 = b"".join([
+    NEW, i(3),
+    b"\x00", i(2), i(2), i(1), d(1.0),       # SpinOperation::encode
+    b"\x00",                                 # LongRange version
+    i(1000), i(1000), i(1000), i(1000),      # gmax
+    d(0.25),                                 # g
+    d(1.0), d(0.0), d(0.0), d(0.0), d(1.0), d(0.0), d(0.0), d(0.0), d(1.0),   # ABC
+    b"\x00",                                 # Dipole version
+])
