@@ -110,6 +110,11 @@ int msd_system_decode(const void* buf, int64_t len, int first_index, msd_system*
  * index conventions as msd_system_encode/decode. */
 int msd_exchange_encode(msd_exchange* e, int first_index, void* buf, int64_t cap, int64_t* len, int* n_objects);
 int msd_exchange_decode(const void* buf, int64_t len, int first_index, msd_exchange** out, int64_t* used, int* n_objects);
+/* Anisotropy::encode/decode (cpu/anisotropy/spinoperationanisotropy.cpp:191-240): scale
+ * and the ops (site, unit axis, K) in the order they were added; decode adds each op
+ * again, renormalising its axis as addAnisotropy does. */
+int msd_anisotropy_encode(msd_anisotropy* a, int first_index, void* buf, int64_t cap, int64_t* len, int* n_objects);
+int msd_anisotropy_decode(const void* buf, int64_t len, int first_index, msd_anisotropy** out, int64_t* used, int* n_objects);
 /* Dipole::encode/decode (cpu/dipole/spinoperationdipole.cpp:35-59 around
  * cpu/longrange/spinoperationlongrange.cpp:55-92): scale, truncation, strength and unit
  * cell; the tensor is rebuilt from them (a matrix from setMatrix is not in the stream,

@@ -69,6 +69,8 @@ _SIGS = {
     "msd_system_decode": (C.c_int, [_vp, C.c_int64, C.c_int, _pp, C.POINTER(C.c_int64), _ip]),
     "msd_exchange_encode": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, C.POINTER(C.c_int64), _ip]),
     "msd_exchange_decode": (C.c_int, [_vp, C.c_int64, C.c_int, _pp, C.POINTER(C.c_int64), _ip]),
+    "msd_anisotropy_encode": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, C.POINTER(C.c_int64), _ip]),
+    "msd_anisotropy_decode": (C.c_int, [_vp, C.c_int64, C.c_int, _pp, C.POINTER(C.c_int64), _ip]),
     "msd_dipole_encode": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, C.POINTER(C.c_int64), _ip]),
     "msd_dipole_decode": (C.c_int, [_vp, C.c_int64, C.c_int, _pp, C.POINTER(C.c_int64), _ip]),
     "msd_system_set_site_gamma": (C.c_int, [_vp, _dp]),

@@ -21,7 +21,7 @@ opens with QjY).
 
 Python values map to Lua values as: None - nil, bool - boolean, int/float - number,
 str - string (UTF-8), list/tuple/dict - table (lists take keys 1..n), SpinSystem,
-Exchange and Dipole - userdata.  Decoded tables come back as dicts, numbers as floats (integral table keys as
+Exchange, Anisotropy and Dipole - userdata.  Decoded tables come back as dicts, numbers as floats (integral table keys as
 ints).  Table pairs are written in list order or dict insertion order; Lua writes them in
 its own table-traversal order, so streams holding tables with several keys match
 MagLua's bytes only where the two orders agree.
@@ -33,10 +33,10 @@ import struct
 import zlib
 
 from ._lib import MagLuaError, MSD_EINVAL, MSD_EUNSUPPORTED, call
-from .maglua import Dipole, Exchange, SpinSystem
+from .maglua import Anisotropy, Dipole, Exchange, SpinSystem
 
 # userdata this container writes and reads, by MagLua type name
-_USERDATA = (("SpinSystem", SpinSystem), ("Exchange", Exchange), ("Dipole", Dipole))
+_USERDATA = (("SpinSystem", SpinSystem), ("Exchange", Exchange), ("Anisotropy", Anisotropy), ("Dipole", Dipole))
 
 LUA_TNIL, LUA_TBOOLEAN, LUA_TNUMBER, LUA_TSTRING, LUA_TTABLE, LUA_TUSERDATA = 0, 1, 3, 4, 5, 7
 MAGIC_NEW, MAGIC_OLD = 111, 222
@@ -185,7 +185,7 @@ class Checkpointer:
                 self._export(k, out)
                 self._export(x, out)
             out += struct.pack("<i", LUA_TNIL)   # no metatable
-        elif isinstance(v, (SpinSystem, Exchange, Dipole)):
+        elif type(v) in tuple(c for _, c in _USERDATA):
             name = next(nm for nm, cls in _USERDATA if type(v) is cls)
             out += struct.pack("<ii", LUA_TUSERDATA, hash32(name))
             for k, o in enumerate(self._objs):

@@ -1092,6 +1092,54 @@ extern "C" int msd_anisotropy_add_many(msd_anisotropy* a, const int* site, const
     return MSD_OK;
 }
 
+// Anisotropy::encode (cpu/anisotropy/spinoperationanisotropy.cpp:191-208): the ops in the
+// order they were added, each site, unit axis and strength
+extern "C" int msd_anisotropy_encode(msd_anisotropy* a, int first_index, void* buf, int64_t cap, int64_t* len, int* n_objects)
+{
+    NONNULL(a); NONNULL(len);
+    if (first_index < 0) return fail(MSD_EINVAL, "Anisotropy.encode: object index %d < 0", first_index);
+    WireOut w;
+    w.b.reserve(64 + 36 * a->ops.size());
+    wire_operator(w, first_index, a->d, a->scale);
+    w.ch(0);
+    w.i32((int32_t)a->ops.size());
+    for (const auto& o : a->ops) { w.i32(o.site); w.f64(o.ax); w.f64(o.ay); w.f64(o.az); w.f64(o.K); }
+    if (n_objects) *n_objects = 1;
+    return wire_finish(w, "Anisotropy.encode", buf, cap, len);
+}
+
+// Anisotropy::decode (spinoperationanisotropy.cpp:210-240): each op goes through add,
+// which normalises the axis again as addAnisotropy does
+extern "C" int msd_anisotropy_decode(const void* buf, int64_t len, int first_index, msd_anisotropy** out, int64_t* used, int* n_objects)
+{
+    NONNULL(buf); NONNULL(out);
+    *out = nullptr;
+    if (len < 0) return fail(MSD_EINVAL, "Anisotropy.decode: length %lld", (long long)len);
+    WireIn r{(const uint8_t*)buf, (size_t)len};
+    int nx, ny, nz;
+    double scale;
+    if (int rc = unwire_operator(r, "Anisotropy.decode", first_index, &nx, &ny, &nz, &scale)) return rc;
+    char version;
+    int32_t num;
+    if (!r.ch(&version) || !r.i32(&num)) return fail(MSD_EINVAL, "Anisotropy.decode: stream too short");
+    if (version != 0) return fail(MSD_EINVAL, "Anisotropy.decode: unknown version %d", (int)version);
+    if (num < 0 || (size_t)num > (r.n - r.at) / 36) return fail(MSD_EINVAL, "Anisotropy.decode: %d ops do not fit the stream", num);
+    msd_anisotropy* a = nullptr;
+    if (int rc = msd_anisotropy_create(nx, ny, nz, &a)) return rc;
+    std::unique_ptr<msd_anisotropy> own(a);
+    a->scale = scale;
+    for (int i = 0; i < num; ++i) {
+        int32_t site;
+        double ax[3], K;
+        r.i32(&site); r.f64(&ax[0]); r.f64(&ax[1]); r.f64(&ax[2]); r.f64(&K);
+        if (int rc = msd_anisotropy_add(a, site, ax, K, nullptr)) return rc;
+    }
+    if (used) *used = (int64_t)r.at;
+    if (n_objects) *n_objects = 1;
+    *out = own.release();
+    return MSD_OK;
+}
+
 extern "C" int msd_anisotropy_apply(msd_anisotropy* a, msd_system* s)
 {
     NONNULL(a); NONNULL(s);

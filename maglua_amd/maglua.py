@@ -368,6 +368,14 @@ class Anisotropy(_Handle):
 
     new = classmethod(lambda cls, *a: cls(*a))
 
+    # checkpoint byte stream (Anisotropy::encode/decode, spinoperationanisotropy.cpp:191-240)
+    def encode(self, first_index=0):
+        return _encode_handle("msd_anisotropy_encode", self._h, first_index)
+
+    @classmethod
+    def decode(cls, data, first_index=0):
+        return _decode_handle(cls, "msd_anisotropy_decode", data, first_index)
+
     def setScale(self, s):
         call("msd_anisotropy_set_scale", self._h, float(s))
 

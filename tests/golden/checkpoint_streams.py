@@ -76,3 +76,13 @@ DIPOLE_2x2x1 = b"".join([
     d(1.0), d(0.0), d(0.0), d(0.0), d(1.0), d(0.0), d(0.0), d(0.0), d(1.0),   # ABC
     b"\x00",                                 # Dipole version
 ])
+
+# Anisotropy::encode (cpu/anisotropy/spinoperationanisotropy.cpp:191-208) of a 2x1x1
+# Anisotropy with one op at site 1, axis z, K 0.5, as object 2
+ANISOTROPY_2x1x1 = b"".join([
+    NEW, i(2),
+    b"\x00", i(2), i(1), i(1), d(1.0),       # SpinOperation::encode
+    b"\x00",                                 # version
+    i(1),                                    # op count
+    i(1), d(0.0), d(0.0), d(1.0), d(0.5),    # site, axis, K
+])

@@ -136,6 +136,12 @@ def test_operator_streams_golden_and_round_trip():
     assert b == G.DIPOLE_2x2x1 and k == 1
     dip2, used, _ = M.Dipole.decode(b, first_index=3)
     assert used == len(b) and dip2.encode(first_index=3)[0] == b
+    ani = M.Anisotropy(2, 1, 1)
+    ani.add((2,), (0.0, 0.0, 3.0), 0.5)       # the axis is stored normalised
+    b, k = ani.encode(first_index=2)
+    assert b == G.ANISOTROPY_2x1x1 and k == 1
+    ani2, used, _ = M.Anisotropy.decode(b, first_index=2)
+    assert used == len(b) and ani2.encode(first_index=2)[0] == b
     # a version-0 Exchange stream has no normalisation flag and means unnormalised
     v0 = G.EXCHANGE_2x1x1[:26] + b"\x00" + G.EXCHANGE_2x1x1[31:]
     ex3, used, _ = M.Exchange.decode(v0)
@@ -144,15 +150,15 @@ def test_operator_streams_golden_and_round_trip():
 
 
 def test_heun_from_checkpointed_system_and_operators(tmp_path):
-    """Spins, Exchange and Dipole from one file step exactly as the originals."""
+    """Spins, Exchange, Anisotropy and Dipole from one file step exactly as the originals."""
     spec = W.FilmSpec(16, 16, 2, J=1.0, K=0.1, g=0.02, pbc=(1, 1, 0), alpha=0.1, dt=0.005, T=0.0, seed=0)
     g = W.build(M, spec)
     g["ss"].randomize(8)
     fn = str(tmp_path / "ops.dat")
-    M.checkpointSave(fn, g["ss"], g["ex"], g["dip"])
-    ss2, ex2, dip2 = M.checkpointLoad(fn)
+    M.checkpointSave(fn, g["ss"], g["ex"], g["ani"], g["dip"])
+    ss2, ex2, ani2, dip2 = M.checkpointLoad(fn)
     M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], None, g["llg"]).step(15)
-    M.Heun(ss2, ex2, g["ani"], dip2, None, g["llg"]).step(15)
+    M.Heun(ss2, ex2, ani2, dip2, None, g["llg"]).step(15)
     M.synchronize()
     d = np.max(np.abs(flat(ss2.spins()) - flat(g["ss"].spins())))
     print(f"checkpointed operators: max|dm| after 15 steps = {d:.3e}", flush=True)
