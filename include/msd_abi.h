@@ -133,6 +133,17 @@ int msd_system_slot_used(const msd_system* s, const char* slot, int* used);
 /* ss:netMoment(): sums (not means) -- cpu/core/spinsystem_luafuncs.lua:17-54.
  * out = {Sx, Sy, Sz, |S|, Sx2, Sy2, Sz2, |S2|}, FP64 accumulation on the device. */
 int msd_system_net_moment(msd_system* s, double out[8]);
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under HIP graph capture set-up, tight serving loops, harnesses that forbid
+ * hipMalloc): msd_system_net_moment over spins the caller holds in device memory,
+ * allocating nothing.  spins = device FP32 [3][n] (x, then y, then z), lo = optional
+ * device FP32 [3][n] low words added to them (NULL: none), out = host {Sx, Sy, Sz, |S|,
+ * Sx2, Sy2, Sz2, |S2|} as msd_system_net_moment.  The partial sums live in a fixed
+ * device array of 6*1024+8 doubles (49,216 bytes) per GPU, reserved for the process;
+ * n is limited to MSD_NOALLOC_MAX_SITES.  msd_init must have been called.  Calls
+ * share that array: calls that overlap on one GPU (several host threads) race. */
+#define MSD_NOALLOC_MAX_SITES (1 << 30)
+int msd_net_moment_noalloc(const float* spins, const float* lo, int n, double out[8]);
 /* sum_i h_slot(i) . s(i), FP64 accumulation (the scripts' fieldArray:dot(spinArray) sums) */
 int msd_system_field_dot(msd_system* s, const char* slot, double* out);
 /* ss:diff(other): {sum|dx|, sum|dy|, sum|dz|, |(those three)|} (SpinSystem::diff, spinsystem.cpp:301-307) */

@@ -79,6 +79,7 @@ _SIGS = {
     "msd_system_get_field": (C.c_int, [_vp, C.c_char_p, _dp, _dp, _dp]),
     "msd_system_slot_used": (C.c_int, [_vp, C.c_char_p, _ip]),
     "msd_system_net_moment": (C.c_int, [_vp, _dp]),
+    "msd_net_moment_noalloc": (C.c_int, [_vp, _vp, C.c_int, _dp]),
     "msd_system_field_dot": (C.c_int, [_vp, C.c_char_p, _dp]),
     "msd_system_diff": (C.c_int, [_vp, _vp, _dp]),
     "msd_exchange_create": (C.c_int, [C.c_int, C.c_int, C.c_int, _pp]),

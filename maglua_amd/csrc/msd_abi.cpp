@@ -57,6 +57,18 @@ static int ensure_init()
     return msd_init(0);
 }
 
+// ============================================================ allocation-free
+// Scratch memory of msd_net_moment_noalloc: the FP64 block partials of the moment
+// reduction (6 per block, at most 1024 blocks) followed by the 6 folded sums.  Each
+// GPU holds its own copy for the life of the process.  In host code `scratch` is a
+// host-side stand-in: its device address, per device, comes from hipGetSymbolAddress
+// in msd_init and is the only form passed to kernels.
+static constexpr int NOALLOC_PART_BLOCKS = 1024;
+static constexpr int NOALLOC_DOUBLES = 6 * NOALLOC_PART_BLOCKS + 8;
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+static double* g_scratch_dev[NOALLOC_MAX_DEVICES];
+
 template <class T>
 static int dalloc(T** p, size_t count)
 {
@@ -114,6 +126,10 @@ extern "C" int msd_init(int device)
     if (configure_lds(G.max_lds) != hipSuccess || configure_lds3(G.max_lds) != hipSuccess)
         G.max_lds = std::min<size_t>(G.max_lds, 65536 - 8192);
     (void)configure_fft64(65536);
+    if (device >= NOALLOC_MAX_DEVICES) return fail(MSD_EINVAL, "msd_init: device %d beyond %d", device, NOALLOC_MAX_DEVICES);
+    void* sp = nullptr;   // this device's copy of the allocation-free entry point's scratch
+    HIPCHK(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+    g_scratch_dev[device] = static_cast<double*>(sp);
     HIPCHK(hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking));
     G.device = device;
     G.ready = true;
@@ -710,6 +726,32 @@ extern "C" int msd_system_net_moment(msd_system* s, double out[8])
     if (e != hipSuccess) return fail(MSD_EDEVICE, "netMoment: %s", hipGetErrorString(e));
     double v[6];
     if (int rc = fetch_out(s, v, 6)) return rc;
+    out[0] = v[0]; out[1] = v[1]; out[2] = v[2];
+    out[3] = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    out[4] = v[3]; out[5] = v[4]; out[6] = v[5];
+    out[7] = std::sqrt(v[3] * v[3] + v[4] * v[4] + v[5] * v[5]);
+    return MSD_OK;
+}
+
+extern "C" int msd_net_moment_noalloc(const float* spins, const float* lo, int n, double out[8])
+{
+    NONNULL(spins); NONNULL(out);
+    // msd_init creates the stream and looks up the scratch: neither may happen here
+    if (!G.ready) return fail(MSD_EINVAL, "msd_net_moment_noalloc: msd_init has not been called");
+    if (n <= 0 || n > MSD_NOALLOC_MAX_SITES)
+        return fail(MSD_EINVAL, "msd_net_moment_noalloc: n = %d outside 1..%d", n, MSD_NOALLOC_MAX_SITES);
+    double* part = g_scratch_dev[G.device];
+    if (!part) return fail(MSD_EDEVICE, "msd_net_moment_noalloc: no scratch address for device %d", G.device);
+    const Dims d{n, 1, 1, n};   // the reductions see only the site count
+    const int nb = reduce_blocks(d);
+    if (nb > NOALLOC_PART_BLOCKS) return fail(MSD_EINVAL, "msd_net_moment_noalloc: %d partial blocks exceed %d", nb, NOALLOC_PART_BLOCKS);
+    double* res = part + 6 * NOALLOC_PART_BLOCKS;
+    // every partial is written by k_moment before k_fold reads it: nothing to clear
+    hipError_t e = launch_reduce_moment(d, spins, lo, part, res, G.stream);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "msd_net_moment_noalloc: %s", hipGetErrorString(e));
+    double v[6];
+    HIPCHK(hipMemcpyAsync(v, res, sizeof v, hipMemcpyDeviceToHost, G.stream));
+    HIPCHK(hipStreamSynchronize(G.stream));
     out[0] = v[0]; out[1] = v[1]; out[2] = v[2];
     out[3] = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     out[4] = v[3]; out[5] = v[4]; out[6] = v[5];
