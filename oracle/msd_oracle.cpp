@@ -253,23 +253,35 @@ extern "C" void ora_dft1d_naive(const double* in, double* out, int n, int sign)
     }
 }
 
+/* Transform the strided lines L + i*stride (i < n) of COLS neighbouring columns
+ * c0 .. c0+cnt-1: the columns are gathered together, so each cache line fetched
+ * serves every column in it; each line's arithmetic is fft_line's alone. */
+static constexpr int OCOLS = 8;
+static void fft_columns(cplx* base, size_t stride, int n, int cnt, int sign, cplx* buf, cplx* work)
+{
+    for (int i = 0; i < n; ++i)
+        for (int c = 0; c < cnt; ++c) buf[(size_t)c * n + i] = base[(size_t)i * stride + c];
+    for (int c = 0; c < cnt; ++c) fft_line(buf + (size_t)c * n, n, sign, work);
+    for (int i = 0; i < n; ++i)
+        for (int c = 0; c < cnt; ++c) base[(size_t)i * stride + c] = buf[(size_t)c * n + i];
+}
+
 /* per-layer 2D C2C (Array::fft2DTo / ifft2DTo semantics), index x + nx*(y + ny*z) */
 static void fft2d_layers(cplx* d, int nx, int ny, int nz, int sign)
 {
+    const int xb = (nx + OCOLS - 1) / OCOLS;
 #pragma omp parallel
     {
-        std::vector<cplx> line(std::max(nx, ny)), work(std::max(nx, ny));
+        std::vector<cplx> buf((size_t)OCOLS * ny), work(std::max(nx, ny));
 #pragma omp for collapse(2) schedule(static)
         for (int z = 0; z < nz; ++z)
             for (int y = 0; y < ny; ++y)
                 fft_line(d + (size_t)nx * (y + (size_t)ny * z), nx, sign, work.data());
 #pragma omp for collapse(2) schedule(static)
         for (int z = 0; z < nz; ++z)
-            for (int x = 0; x < nx; ++x) {
-                cplx* L = d + (size_t)nx * ny * z + x;
-                for (int y = 0; y < ny; ++y) line[y] = L[(size_t)nx * y];
-                fft_line(line.data(), ny, sign, work.data());
-                for (int y = 0; y < ny; ++y) L[(size_t)nx * y] = line[y];
+            for (int b = 0; b < xb; ++b) {
+                const int x0 = b * OCOLS, cnt = std::min(OCOLS, nx - x0);
+                fft_columns(d + (size_t)nx * ny * z + x0, (size_t)nx, ny, cnt, sign, buf.data(), work.data());
             }
     }
 }
@@ -876,15 +888,20 @@ int ora_lr_apply(ora_lr* l, ora_sys* s)
     std::vector<cplx> ws(n);
     for (int a = 0; a < 3; ++a) {
         std::fill(ws.begin(), ws.end(), cplx(0, 0));
-#pragma omp parallel for schedule(static)
+        /* every element sums over source layers, then components, in that order */
+        const long long IB = 4096, nib = (long long)((nxy + IB - 1) / IB);
+#pragma omp parallel for collapse(2) schedule(static)
         for (int t = 0; t < nz; ++t)
-            for (int sl = 0; sl < nz; ++sl) {
-                const size_t off = (size_t)(sl - t + nz - 1);
-                for (int b = 0; b < 3; ++b) {
-                    const cplx* T = l->Q[row[a][b]].data() + off * nxy;
-                    const cplx* M = q[b].data() + (size_t)sl * nxy;
-                    cplx* D = ws.data() + (size_t)t * nxy;
-                    for (size_t i = 0; i < nxy; ++i) D[i] += T[i] * M[i];
+            for (long long ib = 0; ib < nib; ++ib) {
+                const size_t i0 = (size_t)(ib * IB), i1 = std::min(nxy, i0 + (size_t)IB);
+                for (int sl = 0; sl < nz; ++sl) {
+                    const size_t off = (size_t)(sl - t + nz - 1);
+                    for (int b = 0; b < 3; ++b) {
+                        const cplx* T = l->Q[row[a][b]].data() + off * nxy;
+                        const cplx* M = q[b].data() + (size_t)sl * nxy;
+                        cplx* D = ws.data() + (size_t)t * nxy;
+                        for (size_t i = i0; i < i1; ++i) D[i] += T[i] * M[i];
+                    }
                 }
             }
         fft2d_layers(ws.data(), nx, ny, nz, +1);
@@ -903,14 +920,15 @@ static void fft3d(cplx* d, int nx, int ny, int nz, int sign)
 {
     fft2d_layers(d, nx, ny, nz, sign);
     const size_t nxy = (size_t)nx * ny;
+    const long long pb = (long long)((nxy + OCOLS - 1) / OCOLS);
 #pragma omp parallel
     {
-        std::vector<cplx> line(nz), work(nz);
+        std::vector<cplx> buf((size_t)OCOLS * nz), work(nz);
 #pragma omp for schedule(static)
-        for (long long p = 0; p < (long long)nxy; ++p) {
-            for (int z = 0; z < nz; ++z) line[z] = d[p + nxy * z];
-            fft_line(line.data(), nz, sign, work.data());
-            for (int z = 0; z < nz; ++z) d[p + nxy * z] = line[z];
+        for (long long b = 0; b < pb; ++b) {
+            const size_t p0 = (size_t)b * OCOLS;
+            const int cnt = (int)std::min<size_t>(OCOLS, nxy - p0);
+            fft_columns(d + p0, nxy, nz, cnt, sign, buf.data(), work.data());
         }
     }
 }
@@ -978,6 +996,43 @@ extern "C" int ora_dipole_tensor3d_open(int nx, int ny, int nz,
     return 0;
 }
 
+/* Direct real-space sum of the open-boundary dipole field at chosen sites:
+ * h_a(t) = g sum_{s != t} Gamma_ab(r_t - r_s) m_b(s) over every site s of the lattice,
+ * no periodic images, O(N) per site.  The zero-padded 3D convolution (LongRange3D on
+ * the doubled grid with the bare kernel) computes this same sum for every site at
+ * once; this form shares no FFT, tensor grid or padding with it. */
+extern "C" int ora_dipole_direct_open(int nx, int ny, int nz, const double* x, const double* y, const double* z,
+                                      int nsites, const int* sites, double g, double* out)
+{
+    if (nx < 1 || ny < 1 || nz < 1 || nsites < 0) return -1;
+    const long long n = (long long)nx * ny * nz;
+    for (int k = 0; k < nsites; ++k)
+        if (sites[k] < 0 || sites[k] >= n) return -1;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < nsites; ++k) {
+        const long long t = sites[k];
+        const int tx = (int)(t % nx), ty = (int)((t / nx) % ny), tz = (int)(t / ((long long)nx * ny));
+        double h0 = 0, h1 = 0, h2 = 0;
+        for (int sz = 0; sz < nz; ++sz)
+            for (int sy = 0; sy < ny; ++sy) {
+                const long long row = (long long)nx * (sy + (long long)ny * sz);
+                for (int sx = 0; sx < nx; ++sx) {
+                    if (sx == tx && sy == ty && sz == tz) continue;
+                    double G[6];
+                    gamma_dip(tx - sx, ty - sy, tz - sz, G);
+                    const double mx = x[row + sx], my = y[row + sx], mz = z[row + sx];
+                    h0 += G[0] * mx + G[1] * my + G[2] * mz;
+                    h1 += G[1] * mx + G[3] * my + G[4] * mz;
+                    h2 += G[2] * mx + G[4] * my + G[5] * mz;
+                }
+            }
+        out[3 * k] = g * h0;
+        out[3 * k + 1] = g * h1;
+        out[3 * k + 2] = g * h2;
+    }
+    return 0;
+}
+
 /* LongRange3D::compile/apply (cpu/longrange3d/spinoperationlongrange3d.cpp:331-432):
  * nine real tensors XX XY XZ YX YY YZ ZX ZY ZZ on the FFT grid, spectra
  * prescaled by 1/(fx fy fz); H_a = g scale Re IFFT(sum_b T^_ab M^_b).  With
@@ -1029,6 +1084,7 @@ int ora_lr3_apply(ora_lr3* l, ora_sys* s)
     const std::vector<double>* S[3] = {&s->x, &s->y, &s->z};
     for (int c = 0; c < 3; ++c) {
         q[c].assign(m, 0.0);
+#pragma omp parallel for collapse(2) schedule(static)
         for (int z = 0; z < l->nz; ++z)
             for (int y = 0; y < l->ny; ++y)
                 for (int x = 0; x < l->nx; ++x)
@@ -1039,9 +1095,11 @@ int ora_lr3_apply(ora_lr3* l, ora_sys* s)
     std::vector<double>* H[3] = {&s->hx[k], &s->hy[k], &s->hz[k]};
     std::vector<cplx> ws(m);
     for (int a = 0; a < 3; ++a) {
-        for (size_t i = 0; i < m; ++i)
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)m; ++i)
             ws[i] = l->Q[3 * a][i] * q[0][i] + l->Q[3 * a + 1][i] * q[1][i] + l->Q[3 * a + 2][i] * q[2][i];
         fft3d(ws.data(), l->fx, l->fy, l->fz, +1);
+#pragma omp parallel for collapse(2) schedule(static)
         for (int z = 0; z < l->nz; ++z)
             for (int y = 0; y < l->ny; ++y)
                 for (int x = 0; x < l->nx; ++x)
@@ -1142,6 +1200,8 @@ int ora_llg_apply(ora_llg* l, const ora_sys* from, double scale, const ora_sys* 
     } else if (dt == 0) {
         if (from != to) { to->x = from->x; to->y = from->y; to->z = from->z; }
     } else {
+        /* sites are independent (from and to alias only site by site) */
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < n; ++i) {
             const double msf = from->ms[i];
             to->ms[i] = msf;
@@ -1192,6 +1252,7 @@ int ora_llg_apply(ora_llg* l, const ora_sys* from, double scale, const ora_sys* 
         if (advance) to->time = from->time + dt;
     }
     if (!l->no_renorm) {
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < n; ++i) {
             const double mm = to->x[i] * to->x[i] + to->y[i] * to->y[i] + to->z[i] * to->z[i];
             if (mm != 0) {

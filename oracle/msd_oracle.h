@@ -103,6 +103,10 @@ int      ora_dipole_tensor3d_periodic(int nx, int ny, int nz, int R,
                                       double* XX, double* XY, double* XZ, double* YY, double* YZ, double* ZZ);
 int      ora_dipole_tensor3d_open(int nx, int ny, int nz,
                                   double* XX, double* XY, double* XZ, double* YY, double* YZ, double* ZZ);
+/* open-boundary dipole field at chosen sites by a direct real-space sum (no FFT):
+ * out[3k..3k+2] = g sum_{s != t} Gamma(r_t - r_s) m(s), t = sites[k] */
+int      ora_dipole_direct_open(int nx, int ny, int nz, const double* x, const double* y, const double* z,
+                                int nsites, const int* sites, double g, double* out);
 typedef struct ora_lr3 ora_lr3;
 ora_lr3* ora_lr3_new(int nx, int ny, int nz, int open);
 void     ora_lr3_free(ora_lr3*);

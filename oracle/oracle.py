@@ -105,6 +105,7 @@ def lib(threads=False):
         "ora_fft3d": (C.c_int, [_dp, C.c_int, C.c_int, C.c_int, C.c_int]),
         "ora_dipole_tensor3d_periodic": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
         "ora_dipole_tensor3d_open": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "ora_dipole_direct_open": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, _ip, C.c_double, _dp]),
         "ora_lr3_new": (vp, [C.c_int, C.c_int, C.c_int, C.c_int]),
         "ora_lr3_free": (None, [vp]),
         "ora_lr3_set_strength": (None, [vp, C.c_double]),
@@ -472,6 +473,17 @@ def dipole_tensor3d(nx, ny, nz, R=None, open=False, threads=True):
         assert L.ora_dipole_tensor3d_periodic(nx, ny, nz, int(R), *outs) == 0
         shp = (nz, ny, nx)
     return {k: o.reshape(shp) for k, o in zip(("XX", "XY", "XZ", "YY", "YZ", "ZZ"), outs)}
+
+
+def dipole_direct_open(dims, x, y, z, sites, g, threads=True):
+    """Open-boundary dipole field at the site indices `sites`, shaped (k, 3), by the direct
+    real-space sum (no FFT)."""
+    nx, ny, nz = dims
+    sites = np.ascontiguousarray(sites, dtype=np.int32)
+    out = np.zeros(3 * sites.size)
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, z)]
+    assert lib(threads).ora_dipole_direct_open(nx, ny, nz, *a, int(sites.size), sites, float(g), out) == 0
+    return out.reshape(-1, 3)
 
 
 class LongRange3D:

@@ -265,6 +265,37 @@ def test_thermal_amplitude_and_variance():
     assert abs(h[:, 6:].var() / sig ** 2 - 1) < 0.02
 
 
+def test_paramagnet_reaches_langevin_at_t_over_1_plus_alpha2():
+    """J = K = g = 0, B = 1 along z, T = 1, alpha = 0.5: the Heun step with the thermal
+    kick reaches <m_z> = L((1 + alpha^2) B / T) = 0.3789, the Boltzmann state at
+    T / (1 + alpha^2), not L(B / T) = 0.3130.  That is the stationary state of the
+    reference's equation: noise of variance 2 alpha T / (ms dt gamma) on the precession
+    term only (cpu/thermal/spinoperationthermal.cpp:135-191) with the LL prefactor
+    gamma / (1 + alpha^2) (cpu/llg/llgcartesian.cpp:159-187).  At dt = 0.002 the step
+    lowers <m_z> by about 2.5e-3 (measured against dt = 0.001: 0.3795 +- 0.0010)."""
+    nx = ny = 32
+    n = nx * ny
+    alpha, T, Bz = 0.5, 1.0, 1.0
+    ss = O.SpinSystem(nx, ny, 1)
+    ss.setParams(alpha=alpha, dt=0.002)
+    ss.setSpins(np.zeros(n), np.zeros(n), np.ones(n))
+    th = O.Thermal(nx, ny, 1)
+    th.setTemperature(T)
+    af = O.AppliedField(nx, ny, 1)
+    af.set(0.0, 0.0, Bz)
+    sim = O.Sim(ss, None, None, None, th, O.LLG(), seed=5)
+    sim.addAppliedField(af)
+    sim.heun(10000)
+    mz = []
+    for _ in range(40):
+        sim.heun(1000)
+        mz.append(np.mean(ss.spins()[2]))
+    mean, se = np.mean(mz), np.std(mz, ddof=1) / np.sqrt(len(mz))
+    L = lambda x: 1 / np.tanh(x) - 1 / x
+    assert abs(mean - L((1 + alpha ** 2) * Bz / T)) <= 4 * se + 4e-3
+    assert abs(mean - L(Bz / T)) > 0.04
+
+
 # --------------------------------------------------------------- LLG
 B, ALPHA, GAMMA = 1.0, 0.1, 1.0
 
@@ -544,3 +575,27 @@ def test_longrange3d_periodic_matches_image_sum():
 
     ref = _direct_field(s, images)
     assert np.allclose(h, ref, atol=1e-12)
+
+
+def test_direct_open_dipole_sum_equals_padded_convolution():
+    """The direct real-space sum (no images) and the zero-padded 3D FFT convolution with
+    the bare kernel are two forms of one field: they agree at every site of a small
+    open lattice, to FP64 rounding."""
+    dims = (7, 5, 4)
+    n = 140
+    s = np.random.default_rng(8).standard_normal((3, n))
+    ss = O.SpinSystem(*dims)
+    ss.setSpins(*s)
+    lr = O.LongRange3D(*dims, open=True)
+    lr.setDipole(O.dipole_tensor3d(*dims, open=True))
+    lr.setStrength(0.3)
+    lr.apply(ss)
+    h = np.stack(ss.field("LongRange3D")).T
+    d = O.dipole_direct_open(dims, *s, np.arange(n), 0.3)
+    assert np.abs(d - h).max() <= 1e-13 * np.abs(h).max()
+    # a single source: the field at distance r along x is g * 2 m / r^3 for m along x
+    one = np.zeros((3, n))
+    one[0, 0] = 1.0
+    d1 = O.dipole_direct_open(dims, *one, np.array([3]), 0.3)
+    assert np.allclose(d1, [[0.3 * 2 / 27, 0, 0]], rtol=1e-14, atol=1e-16)
+
