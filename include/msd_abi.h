@@ -111,8 +111,8 @@ int msd_system_decode(const void* buf, int64_t len, int first_index, msd_system*
 int msd_exchange_encode(msd_exchange* e, int first_index, void* buf, int64_t cap, int64_t* len, int* n_objects);
 int msd_exchange_decode(const void* buf, int64_t len, int first_index, msd_exchange** out, int64_t* used, int* n_objects);
 /* Anisotropy::encode/decode (cpu/anisotropy/spinoperationanisotropy.cpp:191-240): scale
- * and the ops (site, unit axis, K) in the order they were added; decode adds each op
- * again, renormalising its axis as addAnisotropy does. */
+ * and the ops (site, unit axis, K) in the order they were added; decode stores each op
+ * as read, as addAnisotropy does (:172-189), so a round trip keeps every bit. */
 int msd_anisotropy_encode(msd_anisotropy* a, int first_index, void* buf, int64_t cap, int64_t* len, int* n_objects);
 int msd_anisotropy_decode(const void* buf, int64_t len, int first_index, msd_anisotropy** out, int64_t* used, int* n_objects);
 /* Dipole::encode/decode (cpu/dipole/spinoperationdipole.cpp:35-59 around
@@ -223,9 +223,12 @@ int msd_thermal_set_scale(msd_thermal* t, double s);
 /* temp:scaleSite({site}, v) */
 int msd_thermal_scale_site(msd_thermal* t, int site, double v);
 /* Counter-based noise stream: (seed, draw, site) -> 3 normals.  apply() uses the
- * draw counter and then increments it (MagLua: Random.* RNG objects). */
+ * draw counter and then increments it (MagLua: Random.* RNG objects).  A new Thermal
+ * is keyed by a seed from /dev/urandom, as an unseeded MagLua RNG is
+ * (common/random/random.cpp:29-62); set_seed gives a reproducible stream. */
 int msd_thermal_set_seed(msd_thermal* t, uint64_t seed, uint64_t draw);
 int msd_thermal_get_draw(const msd_thermal* t, uint64_t* draw);
+int msd_thermal_get_seed(const msd_thermal* t, uint64_t* seed);
 /* temp:apply(ss): Thermal::apply, cpu/thermal/spinoperationthermal.cpp:135-191, slot "Thermal" */
 int msd_thermal_apply(msd_thermal* t, msd_system* s);
 
@@ -374,14 +377,18 @@ int msd_surrogate_field_error(msd_surrogate* s, msd_dipole* d, const msd_system*
 int msd_surrogate_field_error_longrange3d(msd_surrogate* s, msd_longrange3d* l, const msd_system* m,
                                           const msd_system* m_ref, double* rel_err, double* frozen_err);
 /* Learned dipole field in the step driver (a Dipole or LongRange3D driver): windows of
- * K steps, each started by an exact field evaluation that becomes the reference
- * and checks the surrogate's prediction for that state; a window whose check
- * exceeds tol runs exact steps.  sur = NULL returns to exact steps. */
+ * K steps on H_ref + dH(m - m_ref), H_ref the exact field of the window's starting
+ * state m_ref.  Each window is checked when it ends: the exact field of the state it
+ * reached against the surrogate's prediction, max|pred - exact| / max|exact|.  A
+ * window within tol stands (its exact field starts the next one); otherwise its
+ * starting state and thermal draw are restored and it runs again as exact steps.
+ * tol = 0 makes every window exact.  sur = NULL returns to exact steps. */
 int msd_stepper_set_surrogate(msd_stepper* st, msd_surrogate* sur, int K, double tol);
 int msd_stepper_surrogate_stats(const msd_stepper* st, long* windows, long* exact_windows, long* fallbacks,
                                 double* last_residual, double* max_residual);
 
 typedef struct msd_ensemble msd_ensemble;
+/* Members must not share a SpinSystem, a LongRange3D or a Thermal (MSD_EINVAL). */
 int msd_ensemble_create(msd_stepper* const* members, int count, msd_ensemble** out);
 int msd_ensemble_destroy(msd_ensemble* e);
 int msd_ensemble_set_graph(msd_ensemble* e, int yes);

@@ -122,6 +122,7 @@ _SIGS = {
     "msd_thermal_scale_site": (C.c_int, [_vp, C.c_int, C.c_double]),
     "msd_thermal_set_seed": (C.c_int, [_vp, C.c_uint64, C.c_uint64]),
     "msd_thermal_get_draw": (C.c_int, [_vp, _u64p]),
+    "msd_thermal_get_seed": (C.c_int, [_vp, _u64p]),
     "msd_thermal_apply": (C.c_int, [_vp, _vp]),
     "msd_dipole_create": (C.c_int, [C.c_int, C.c_int, C.c_int, _pp]),
     "msd_dipole_destroy": (C.c_int, [_vp]),

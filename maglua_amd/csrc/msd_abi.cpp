@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <map>
 #include <memory>
 #include <string>
@@ -576,7 +577,10 @@ extern "C" int msd_system_encode(msd_system* s, int first_index, void* buf, int6
 }
 
 // decodeT<dArray> (luabaseobject.cpp:303-350) then Array<double>::decode (array_core_cpu.h:226-252)
-static int unwire_darray(WireIn& r, int index, const Dims& d, double* v)
+// Reads one Array.Double of the system's size into v, which is sized only once the
+// array's header has been checked and, for an explicit array, the stream holds its
+// 8n bytes: a truncated or corrupt stream fails before any large allocation.
+static int unwire_darray(WireIn& r, int index, const Dims& d, std::vector<double>& v)
 {
     int32_t tag, type, pos, nx, ny, nz, flag;
     char magic, version;
@@ -596,11 +600,14 @@ static int unwire_darray(WireIn& r, int index, const Dims& d, double* v)
     if (flag == 1) {
         double c;
         if (!r.f64(&c)) return fail(MSD_EINVAL, "SpinSystem.decode: stream ends inside an array");
-        std::fill(v, v + d.n, c);
+        v.assign(d.n, c);
         return MSD_OK;
     }
     if (flag != 0) return fail(MSD_EINVAL, "SpinSystem.decode: array flag %d", flag);
-    if (!r.get(v, (size_t)d.n * sizeof(double))) return fail(MSD_EINVAL, "SpinSystem.decode: stream ends inside an array");
+    const size_t bytes = (size_t)d.n * sizeof(double);
+    if (bytes > r.n - r.at) return fail(MSD_EINVAL, "SpinSystem.decode: stream ends inside an array");
+    v.resize(d.n);
+    if (!r.get(v.data(), bytes)) return fail(MSD_EINVAL, "SpinSystem.decode: stream ends inside an array");
     return MSD_OK;
 }
 
@@ -629,16 +636,15 @@ extern "C" int msd_system_decode(const void* buf, int64_t len, int first_index, 
     const Dims d{nx, ny, nz, nx * ny * nz};
     const size_t n = d.n;
     int idx = first_index + 1;
-    std::vector<double> sp(4 * n);
+    std::vector<double> sp[4];   // x y z ms
     for (int c = 0; c < 4; ++c)
-        if (int rc = unwire_darray(r, idx++, d, sp.data() + c * n)) return rc;
+        if (int rc = unwire_darray(r, idx++, d, sp[c])) return rc;
     std::vector<double> site[2];
     for (int k = 0; k < 2; ++k) {
         int32_t has;
         if (!r.i32(&has)) return fail(MSD_EINVAL, "SpinSystem.decode: stream ends before the site arrays");
         if (!has) continue;
-        site[k].resize(n);
-        if (int rc = unwire_darray(r, idx++, d, site[k].data())) return rc;
+        if (int rc = unwire_darray(r, idx++, d, site[k])) return rc;
     }
     int32_t extra;
     if (!r.i32(&extra)) return fail(MSD_EINVAL, "SpinSystem.decode: stream ends before the extra-data count");
@@ -653,7 +659,7 @@ extern "C" int msd_system_decode(const void* buf, int64_t len, int first_index, 
     if (int rc = msd_system_create(nx, ny, nz, &s)) return rc;
     std::unique_ptr<msd_system> own(s);
     s->alpha = alpha; s->dt = dt; s->gamma = gamma; s->time = time;
-    if (int rc = msd_system_set_spins(s, sp.data(), sp.data() + n, sp.data() + 2 * n, sp.data() + 3 * n)) return rc;
+    if (int rc = msd_system_set_spins(s, sp[0].data(), sp[1].data(), sp[2].data(), sp[3].data())) return rc;
     if (!site[0].empty())
         if (int rc = set_site_array(s, &s->site_alpha, site[0].data())) return rc;
     if (!site[1].empty())
@@ -1174,7 +1180,10 @@ extern "C" int msd_anisotropy_decode(const void* buf, int64_t len, int first_ind
         int32_t site;
         double ax[3], K;
         r.i32(&site); r.f64(&ax[0]); r.f64(&ax[1]); r.f64(&ax[2]); r.f64(&K);
-        if (int rc = msd_anisotropy_add(a, site, ax, K, nullptr)) return rc;
+        // stored as read, as addAnisotropy does (:172-189): only the Lua add normalises
+        if (site < 0 || site >= a->d.n) return fail(MSD_EINVAL, "Anisotropy.decode: op %d names site %d", i, site);
+        a->ops.push_back({site, ax[0], ax[1], ax[2], K});
+        a->dirty = true;
     }
     if (used) *used = (int64_t)r.at;
     if (n_objects) *n_objects = 1;
@@ -1391,6 +1400,19 @@ extern "C" int msd_appliedfield_apply(msd_appliedfield* a, msd_system* s)
 }
 
 // =================================================================== Thermal
+// A seed from /dev/urandom, else from the clock: what an unseeded (or seed 0) MagLua RNG
+// takes (RNG::seed, common/random/random.cpp:29-62).  Never 0.
+static uint64_t urandom_seed()
+{
+    uint64_t v = 0;
+    if (FILE* f = fopen("/dev/urandom", "rb")) {
+        if (fread(&v, sizeof v, 1, f) != 1) v = 0;
+        fclose(f);
+    }
+    if (!v) v = (uint64_t)time(nullptr) * 0x9E3779B97F4A7C15ull;
+    return v ? v : 1;
+}
+
 struct msd_thermal {
     Dims d{};
     double T = 0, scale = 1;
@@ -1428,6 +1450,7 @@ extern "C" int msd_thermal_create(int nx, int ny, int nz, msd_thermal** out)
     msd_thermal* t = new msd_thermal;
     t->d = Dims{nx, ny, nz, nx * ny * nz};
     t->site.assign(t->d.n, 1.0);
+    t->seed = urandom_seed();
     *out = t;
     return MSD_OK;
 }
@@ -1444,6 +1467,7 @@ extern "C" int msd_thermal_scale_site(msd_thermal* t, int site, double v)
 }
 extern "C" int msd_thermal_set_seed(msd_thermal* t, uint64_t seed, uint64_t draw) { NONNULL(t); t->seed = seed; t->draw = draw; return MSD_OK; }
 extern "C" int msd_thermal_get_draw(const msd_thermal* t, uint64_t* draw) { NONNULL(t); NONNULL(draw); *draw = t->draw; return MSD_OK; }
+extern "C" int msd_thermal_get_seed(const msd_thermal* t, uint64_t* seed) { NONNULL(t); NONNULL(seed); *seed = t->seed; return MSD_OK; }
 
 struct msd_llg {
     bool thermal_first = true;
@@ -2732,10 +2756,11 @@ struct msd_stepper {
     int sur_K = 10;
     double sur_tol = 1e-3;
     float *Href = nullptr, *Mref = nullptr, *Hext = nullptr, *Hx = nullptr;   // [3][n]
+    float* Ssnap = nullptr;                   // [3][n] low words of a window's starting state
     std::vector<msd_appliedfield*> af;                                        // applied fields
     msd_uniaxial* uni = nullptr;                                              // Anisotropy.Uniaxial
     float* Happ = nullptr;                                                    // [3][n] their per-site sum
-    bool have_ref = false, sur_fallback = false;
+    bool have_ref = false;
     long sur_windows = 0, sur_exact_windows = 0, sur_fallbacks = 0;
     double sur_last_res = 0.0, sur_max_res = 0.0;
     // learned-step graphs: [0] a window's first step, [1] the others
@@ -2748,7 +2773,7 @@ struct msd_stepper {
         (void)hipStreamSynchronize(G.stream);
         drop_graph();
         dfree(P); dfree(C); dfree(Clo); dfree(draw);
-        dfree(Href); dfree(Mref); dfree(Hext); dfree(Hx); dfree(Happ);
+        dfree(Href); dfree(Mref); dfree(Hext); dfree(Hx); dfree(Ssnap); dfree(Happ);
     }
     void drop_graph()
     {
@@ -3046,47 +3071,61 @@ static int run_steps(msd_stepper* st, int nsteps, bool learned)
     return run_kind(st, nsteps - 1, 2);
 }
 
-// Windows of K steps.  At each window start the exact layered field of the
-// current state is computed; the surrogate's prediction for that state (from the
-// previous window's reference) is checked against it, and the exact field and
-// state become the new reference.  A window runs learned steps unless the check
-// exceeded the tolerance, in which case it runs exact steps (and the surrogate is
-// checked again at the next window start).
+// Windows of K steps on the surrogate field, each checked when it ends.  A window
+// starts from a state whose exact field (layered Dipole or LongRange3D) is known:
+// that field becomes H_ref and the state m_ref, and the state (hi and lo words) is
+// kept.  After the window's learned steps the exact field of the state reached is
+// compared with the surrogate's prediction for it, max|pred - exact| / max|exact|.
+// Within tol the window stands, and its exact field starts the next window.
+// Otherwise the kept state and the thermal draw counter are restored and the
+// window runs again as exact steps, so every step kept is either checked or exact.
+// tol = 0 makes every window exact: the trajectory is the exact driver's.
 static int stepper_heun_learned(msd_stepper* st, int nsteps)
 {
     if (int rc = member_begin(st)) return rc;
     if (int rc = st->sur->pack()) return rc;
     const size_t n3 = (size_t)3 * st->s->d.n;
-    int done = 0;
     const ExactField exact{st->dip, st->lr3};
+    const uint64_t draw0 = st->th ? st->th->draw : 0;
+    bool fresh = false;      // Hx holds the exact field of the current state
+    int done = 0;
     while (done < nsteps) {
-        if (int rc = exact.of(st->s->S, st->Hx)) return rc;
-        if (st->have_ref) {
-            hipError_t e = launch_cnn_input(st->s->d, st->s->S, st->Mref, (float)st->sur->s_in, st->sur->X0, G.stream);
-            if (e != hipSuccess) return fail(MSD_EDEVICE, "surrogate check: %s", hipGetErrorString(e));
-            if (int rc = st->sur->forward(st->Hext, st->Href, G.stream)) return rc;
-            double md, mb;
-            if (int rc = field_maxdiff(st->Hext, st->Hx, n3, &md, &mb)) return rc;
-            const double res = mb > 0 ? md / mb : md;
-            st->sur_last_res = res;
-            st->sur_max_res = std::max(st->sur_max_res, res);
-            st->sur_fallback = !(res <= st->sur_tol);
-            if (st->sur_fallback) st->sur_fallbacks++;
-        }
+        if (!fresh)
+            if (int rc = exact.of(st->s->S, st->Hx)) return rc;
         HIPCHK(hipMemcpyAsync(st->Href, st->Hx, n3 * sizeof(float), hipMemcpyDeviceToDevice, G.stream));
         HIPCHK(hipMemcpyAsync(st->Mref, st->s->S, n3 * sizeof(float), hipMemcpyDeviceToDevice, G.stream));
+        HIPCHK(hipMemcpyAsync(st->Ssnap, st->s->Slo, n3 * sizeof(float), hipMemcpyDeviceToDevice, G.stream));
         st->have_ref = true;
         const int L = std::min(st->sur_K, nsteps - done);
-        // The layered field pass above left the spin spectrum of S in the shared
-        // buffers, which is what an exact step's first column pass reads; the 3D
-        // pass left the field's spectrum in W, so exact 3D steps transform S again.
-        if (st->sur_fallback && st->lr3) {
-            hipError_t e = st->b3.rows(R3_FWD, false, G.stream);
-            if (e != hipSuccess) return fail(MSD_EDEVICE, "surrogate fallback: %s", hipGetErrorString(e));
-        }
-        if (int rc = run_steps(st, L, !st->sur_fallback)) return rc;
+        if (int rc = run_steps(st, L, true)) return rc;
+        // the check: the exact field of the state the window reached
+        if (int rc = exact.of(st->s->S, st->Hx)) return rc;
+        hipError_t e = launch_cnn_input(st->s->d, st->s->S, st->Mref, (float)st->sur->s_in, st->sur->X0, G.stream);
+        if (e != hipSuccess) return fail(MSD_EDEVICE, "surrogate check: %s", hipGetErrorString(e));
+        if (int rc = st->sur->forward(st->Hext, st->Href, G.stream)) return rc;
+        double md, mb;
+        if (int rc = field_maxdiff(st->Hext, st->Hx, n3, &md, &mb)) return rc;
+        const double res = mb > 0 ? md / mb : md;
+        st->sur_last_res = res;
+        st->sur_max_res = std::max(st->sur_max_res, res);
         st->sur_windows++;
-        if (st->sur_fallback) st->sur_exact_windows++;
+        fresh = st->sur_tol > 0 && res <= st->sur_tol;
+        if (!fresh) {
+            // restore the window's start (m_ref holds its hi words) and run it exactly
+            st->sur_fallbacks++;
+            st->sur_exact_windows++;
+            HIPCHK(hipMemcpyAsync(st->s->S, st->Mref, n3 * sizeof(float), hipMemcpyDeviceToDevice, G.stream));
+            HIPCHK(hipMemcpyAsync(st->s->Slo, st->Ssnap, n3 * sizeof(float), hipMemcpyDeviceToDevice, G.stream));
+            if (st->th) {
+                const unsigned long long v = (unsigned long long)(draw0 + (uint64_t)done - 1);
+                HIPCHK(hipMemcpyAsync(st->draw, &v, sizeof v, hipMemcpyHostToDevice, G.stream));
+                HIPCHK(hipStreamSynchronize(G.stream));
+            }
+            // the exact steps start from the spin spectrum of the restored state
+            e = st->lr3 ? st->b3.rows(R3_FWD, false, G.stream) : launch_rows_fwd(st->args.dp, st->s->S, G.stream);
+            if (e != hipSuccess) return fail(MSD_EDEVICE, "surrogate fallback: %s", hipGetErrorString(e));
+            if (int rc = run_steps(st, L, false)) return rc;
+        }
         done += L;
     }
     stepper_end(st, nsteps);
@@ -3212,16 +3251,15 @@ extern "C" int msd_stepper_set_surrogate(msd_stepper* st, msd_surrogate* sur, in
     if (!st->dip && !st->lr3) return fail(MSD_EUNSUPPORTED, "stepper: the surrogate replaces a Dipole or LongRange3D field");
     if (!same_dims(st->s->d, sur->d.nx, sur->d.ny, sur->d.nz)) return fail(MSD_EMISMATCH, "stepper: Surrogate size differs");
     if (K < 1) return fail(MSD_EINVAL, "stepper: refresh interval %d", K);
-    if (!(tol > 0)) return fail(MSD_EINVAL, "stepper: residual tolerance must be > 0");
+    if (!(tol >= 0)) return fail(MSD_EINVAL, "stepper: residual tolerance must be >= 0");
     const size_t n3 = (size_t)3 * st->s->d.n;
-    for (float** p : {&st->Href, &st->Mref, &st->Hext, &st->Hx})
+    for (float** p : {&st->Href, &st->Mref, &st->Hext, &st->Hx, &st->Ssnap})
         if (!*p)
             if (int rc = dalloc(p, n3)) return rc;
     st->sur = sur;
     st->sur_K = K;
     st->sur_tol = tol;
     st->have_ref = false;
-    st->sur_fallback = false;
     st->sur_windows = st->sur_exact_windows = st->sur_fallbacks = 0;
     st->sur_last_res = st->sur_max_res = 0.0;
     st->drop_learned_graphs();
@@ -3410,6 +3448,9 @@ extern "C" int msd_ensemble_create(msd_stepper* const* members, int count, msd_e
             if (members[j] == s || members[j]->s == s->s) return fail(MSD_EINVAL, "ensemble: member %d repeats a SpinSystem", i);
             if (s->lr3 && members[j]->lr3 == s->lr3)
                 return fail(MSD_EINVAL, "ensemble: members %d and %d share a LongRange3D (its spectrum buffer)", j, i);
+            // one Thermal would give both replicas the same noise and advance its draw twice a step
+            if (s->th && members[j]->th == s->th)
+                return fail(MSD_EINVAL, "ensemble: members %d and %d share a Thermal (its noise stream)", j, i);
         }
         e->m.push_back(s);
     }

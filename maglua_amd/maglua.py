@@ -13,6 +13,7 @@ are 1).  Failures raise MagLuaError carrying msd_last_error(), where MagLua
 raises luaL_error.  All state lives on the GPU; there is no CPU path.
 """
 import ctypes as C
+import os
 import struct
 
 import numpy as np
@@ -585,15 +586,25 @@ class AppliedField(_Handle):
             call("msd_appliedfield_apply", self._h, s.h)
 
 
+def _urandom_seed():
+    """A nonzero 64-bit seed from the OS entropy pool, as an unseeded MagLua RNG takes one
+    from /dev/urandom (RNG::seed, common/random/random.cpp:29-62)."""
+    while True:
+        v = int.from_bytes(os.urandom(8), "little")
+        if v:
+            return v
+
+
 class Random:
     """Seed holder standing in for Random.Isaac/MersenneTwister: the build's noise is
-    counter-based (Philox4x32-10), so an RNG object only carries the seed."""
+    counter-based (Philox4x32-10), so an RNG object only carries the seed.  No seed, or
+    seed 0, draws one from /dev/urandom as RNG::luaInit does (common/random/random.cpp:29-46)."""
 
-    def __init__(self, seed=0):
-        self.seed = int(seed)
+    def __init__(self, seed=None):
+        self.seed = int(seed) if seed else _urandom_seed()
 
     @classmethod
-    def new(cls, seed=0):
+    def new(cls, seed=None):
         return cls(seed)
 
 
@@ -602,7 +613,10 @@ class Thermal(_Handle):
 
     _destroy = "msd_thermal_destroy"
 
-    def __init__(self, *a, seed=0):
+    def __init__(self, *a, seed=None):
+        """Thermal.new(n [, rng]): the noise is keyed by the RNG's seed, else by `seed`, else
+        (no seed, or 0) by a seed from /dev/urandom, so two unseeded Thermals never share
+        a stream.  setSeed(seed, draw) gives a reproducible one."""
         rng = [v for v in a if isinstance(v, Random)]
         a = tuple(v for v in a if not isinstance(v, Random))
         self._dims = _dims_of(a)
@@ -610,7 +624,8 @@ class Thermal(_Handle):
         call("msd_thermal_create", *self._dims, C.byref(p))
         super().__init__(p.value)
         self._T = 0.0
-        call("msd_thermal_set_seed", self._h, rng[0].seed if rng else int(seed), 0)
+        key = rng[0].seed if rng else (int(seed) if seed else _urandom_seed())
+        call("msd_thermal_set_seed", self._h, key, 0)
 
     new = classmethod(lambda cls, *a, **k: cls(*a, **k))
 
@@ -635,6 +650,11 @@ class Thermal(_Handle):
     def draw(self):
         d = C.c_uint64()
         call("msd_thermal_get_draw", self._h, C.byref(d))
+        return d.value
+
+    def seed(self):
+        d = C.c_uint64()
+        call("msd_thermal_get_seed", self._h, C.byref(d))
         return d.value
 
     def apply(self, ss, rng=None):
@@ -1051,9 +1071,10 @@ class Heun(_Handle):
         call("msd_stepper_add_applied_field", self._h, af.h)
 
     def setSurrogate(self, sur, K=10, tol=1e-3):
-        """Learned dipole field: windows of K steps, each started by an exact field evaluation
-        that becomes the reference and checks the surrogate (relative max-norm residual <= tol,
-        else the window runs exact steps).  sur=None returns to exact steps."""
+        """Learned dipole field: windows of K steps from an exact reference field.  Each window
+        is checked when it ends (relative max-norm residual of the surrogate's field against
+        the exact one <= tol); a window over tol is restored and run again as exact steps.
+        tol=0 makes every window exact.  sur=None returns to exact steps."""
         self._sur = sur
         call("msd_stepper_set_surrogate", self._h, sur.h if sur is not None else None, int(K), float(tol))
 

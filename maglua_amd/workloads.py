@@ -27,7 +27,8 @@ class FilmSpec:
     gamma: float = 1.0
     dt: float = 0.002
     T: float = 0.0
-    seed: int = 0
+    seed: int = 0           # initial spins
+    noise_seed: int = 0     # key of the counter-based thermal noise stream
 
     @property
     def n(self):
@@ -100,5 +101,7 @@ def build(api, spec, with_dipole=True, with_thermal=None, threads=None):
     if with_thermal if with_thermal is not None else spec.T != 0:
         temp = api.Thermal(spec.nx, spec.ny, spec.nz, **kw)
         temp.setTemperature(spec.T)
+        if hasattr(temp, "setSeed"):        # a reproducible stream (an unseeded Thermal draws its key)
+            temp.setSeed(spec.noise_seed)
     llg = api.LLG.Cartesian() if hasattr(api.LLG, "Cartesian") else api.LLG(**kw)
     return dict(ss=ss, ex=ex, ani=ani, dip=dip, temp=temp, llg=llg)

@@ -83,6 +83,10 @@ def test_decode_errors():
         M.SpinSystem.decode(b"\xde" + b[1:])
     with pytest.raises(M.MagLuaError, match="version"):
         M.SpinSystem.decode(b[:5] + b"\x01" + b[6:])
+    # a header naming a 2^28-site lattice over arrays of another size fails before
+    # anything of the lattice's size is allocated
+    with pytest.raises(M.MagLuaError, match="array of 2x1x1"):
+        M.SpinSystem.decode(b[:6] + struct.pack("<iii", 16384, 16384, 1) + b[18:])
 
 
 def test_checkpointer_with_systems(tmp_path):
@@ -142,6 +146,11 @@ def test_operator_streams_golden_and_round_trip():
     assert b == G.ANISOTROPY_2x1x1 and k == 1
     ani2, used, _ = M.Anisotropy.decode(b, first_index=2)
     assert used == len(b) and ani2.encode(first_index=2)[0] == b
+    # decode stores an op as read (addAnisotropy), a non-unit or zero axis included
+    for az in (2.0, 0.0):
+        bz = b[:-16] + struct.pack("<d", az) + b[-8:]
+        ani3, used, _ = M.Anisotropy.decode(bz, first_index=2)
+        assert used == len(bz) and ani3.encode(first_index=2)[0] == bz
     # a version-0 Exchange stream has no normalisation flag and means unnormalised
     v0 = G.EXCHANGE_2x1x1[:26] + b"\x00" + G.EXCHANGE_2x1x1[31:]
     ex3, used, _ = M.Exchange.decode(v0)

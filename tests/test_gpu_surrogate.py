@@ -39,6 +39,17 @@ def setup(dims, seed, s_in=60.0, s_out=0.04, sigma2=2e-4, periodic=True):
     return sur, Ws, dm, T, (s_in, s_out, sigma2)
 
 
+def _forward_errors(got, want):
+    """Max and median |got - want| relative to max|want|.  The oracle rounds to BF16
+    where the kernels do, so outputs agree to FP32 accumulation order (median measured
+    5e-8 .. 1e-7); an intermediate activation that lies on a BF16 rounding boundary can
+    round the other way, one BF16 ulp (2^-8 relative) that spreads through the later
+    layers' 5x5x3 neighbourhoods (max measured up to 1.6e-4 on a 12-layer lattice)."""
+    d = np.abs(got - want).reshape(-1)
+    scale = np.abs(want).max()
+    return float(d.max() / scale), float(np.median(d) / scale)
+
+
 # film walk (every layer in one item) for 1, 2, 4 and 8 layers; the per-layer walk
 # for 3 and 5
 @pytest.mark.parametrize("dims", [(32, 4, 1), (32, 8, 4), (64, 6, 3), (64, 5, 8), (32, 7, 5), (96, 3, 2)])
@@ -46,9 +57,9 @@ def test_forward_matches_oracle(dims):
     sur, Ws, dm, T, (s_in, s_out, _) = setup(dims, 1)
     got = aos(sur.forward(), dims)
     want, _ = N.forward(Ws, dm, s_in, s_out, bf=True)
-    err = np.abs(got - want).max() / np.abs(want).max()
-    print(f"surrogate forward {dims}: rel err {err:.2e}", flush=True)
-    assert err <= 2e-3
+    err, bulk = _forward_errors(got, want)
+    print(f"surrogate forward {dims}: rel err max {err:.2e}, median {bulk:.2e}", flush=True)
+    assert bulk <= 1e-6 and err <= 1e-3
 
 
 # zero padding in x and y (open lattices) runs the per-layer walk at every depth
@@ -57,9 +68,9 @@ def test_open_forward_matches_oracle(dims):
     sur, Ws, dm, T, (s_in, s_out, _) = setup(dims, 11, periodic=False)
     got = aos(sur.forward(), dims)
     want, _ = N.forward(Ws, dm, s_in, s_out, bf=True, periodic=False)
-    err = np.abs(got - want).max() / np.abs(want).max()
-    print(f"open surrogate forward {dims}: rel err {err:.2e}", flush=True)
-    assert err <= 2e-3
+    err, bulk = _forward_errors(got, want)
+    print(f"open surrogate forward {dims}: rel err max {err:.2e}, median {bulk:.2e}", flush=True)
+    assert bulk <= 1e-6 and err <= 1e-3
     # the periodic network differs near the x and y faces: the test sees the padding
     wrap, _ = N.forward(Ws, dm, s_in, s_out, bf=True)
     assert np.abs(wrap - want).max() > 10 * np.abs(got - want).max()
@@ -202,13 +213,39 @@ def test_learned_driver_tracks_the_exact_trajectory_and_falls_back():
     dmax = float(np.abs(np.stack(a.spins()[:3]) - np.stack(b.spins()[:3])).max())
     print(f"learned vs exact after 40 steps: max|dm| {dmax:.3e}; {st}", flush=True)
     assert st["windows"] == 4
-    assert dmax <= 5e-3
+    assert dmax <= 1e-3
     assert abs(a.time() - b.time()) < 1e-12
-    # an impossible tolerance: every checked window falls back to exact steps
+    # an impossible tolerance: every window, the first included, fails its check and
+    # runs again as exact steps from its restored state and noise
     eb.setSurrogate(sur, K=10, tol=1e-12)
     eb.step(30)
     st = eb.surrogateStats()
-    assert st["windows"] == 3 and st["fallbacks"] == 2 and st["exact_windows"] == 2
+    assert st["windows"] == 3 and st["fallbacks"] == 3 and st["exact_windows"] == 3
+
+
+def test_learned_driver_with_zero_tolerance_is_the_exact_driver():
+    """tol = 0: every window is checked, restored and run exactly, so the trajectory
+    equals the exact driver's bit for bit (same state, noise and draw counter), and the
+    time and draw counter advance as one exact run's do."""
+    spec, g = film(23, T=0.3)
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"])
+    heun.step(10)
+    sur = M.Surrogate(spec.nx, spec.ny, spec.nz)
+    S.train_on_trajectory(sur, g["ss"], heun, g["dip"], steps=40, K=5, lr=2e-3, seed=4)
+    a, b = g["ss"].copy(), g["ss"].copy()
+    ta, tb = W.build(M, spec)["temp"], W.build(M, spec)["temp"]
+    for t in (ta, tb):
+        t.setSeed(77, draw=3)
+    ea = M.Heun(a, g["ex"], g["ani"], g["dip"], ta, g["llg"])
+    eb = M.Heun(b, g["ex"], g["ani"], g["dip"], tb, g["llg"])
+    eb.setSurrogate(sur, K=7, tol=0.0)
+    ea.step(30)
+    eb.step(30)
+    st = eb.surrogateStats()
+    print(f"tol=0 learned driver: {st}", flush=True)
+    assert st["windows"] == 5 and st["exact_windows"] == 5
+    assert np.array_equal(np.stack(a.spins()), np.stack(b.spins()))
+    assert a.time() == b.time() and ta.draw() == tb.draw() == 33
 
 
 def open_cube(seed, n=32, T=0.3):
@@ -250,9 +287,9 @@ def test_learned_driver_on_an_open_longrange3d_cube():
     dmax = float(np.abs(np.stack(a.spins()[:3]) - np.stack(b.spins()[:3])).max())
     print(f"open cube learned vs exact after 32 steps: max|dm| {dmax:.3e}; {st}", flush=True)
     assert st["windows"] == 4 and st["exact_windows"] == 0
-    assert dmax <= 5e-3
-    # an impossible tolerance: the first window (no reference yet) is learned, every
-    # checked one runs exact steps, which continue as an exact driver would
+    assert dmax <= 1e-3
+    # an impossible tolerance: every window fails its check and runs again exactly from
+    # its restored state, continuing exactly as an exact driver does
     eb.setSurrogate(sur, K=8, tol=1e-12)
     eb.step(8)
     c = b.copy()
@@ -262,7 +299,5 @@ def test_learned_driver_on_an_open_longrange3d_cube():
     eb.step(16)
     ec.step(16)
     st = eb.surrogateStats()
-    assert st["windows"] == 3 and st["exact_windows"] == 2
-    dfb = float(np.abs(np.stack(b.spins()[:3]) - np.stack(c.spins()[:3])).max())
-    print(f"open cube fallback vs exact: max|dm| {dfb:.3e}", flush=True)
-    assert dfb <= 1e-6
+    assert st["windows"] == 3 and st["exact_windows"] == 3
+    assert np.array_equal(np.stack(b.spins()), np.stack(c.spins()))

@@ -53,3 +53,12 @@ def test_nn_paths_counts():
     f1, _ = W.nn_paths(32, 32, 1, (1, 1, 0))
     assert len(f1) == 4 * 1024
     assert (f[:5] == 0).all()                           # site-major order
+
+
+def test_unseeded_random_draws_distinct_nonzero_seeds():
+    """Random.new() and Random.new(0) take a seed from /dev/urandom, as RNG::luaInit does
+    (common/random/random.cpp:29-62); an explicit seed is kept."""
+    from maglua_amd.maglua import Random
+    seeds = {Random().seed, Random(0).seed, Random.new().seed, Random.new(0).seed}
+    assert len(seeds) == 4 and 0 not in seeds
+    assert Random(1234).seed == 1234
