@@ -385,6 +385,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the batched-replica and 256^3 open-cube legs")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--plain", action="store_true", help="plain FP32 spin state instead of the compensated pair")
+    ap.add_argument("--film", choices=("256x256x8", "512x512x4"), default="256x256x8",
+                    help="the plain workload: the 256x256x8 film (T=0.5) or the 512x512x4 film (g=0.1, T=0.2)")
     ap.add_argument("--calibrate", action="store_true",
                     help="after timing, copy the spin system once on the device (two copy kernels of 8 MiB and "
                          "6 MiB) so a counter run can scale FETCH_SIZE / WRITE_SIZE by a copy of known size")
@@ -397,7 +399,7 @@ def main():
     import maglua_amd as M
     from maglua_amd import workloads as W
 
-    spec = W.FILM_256_T05
+    spec = W.FILM_256_T05 if args.film == "256x256x8" else W.FILM_512
     t_setup = time.perf_counter()
     g = W.build(M, spec)
     g["ss"].randomize(spec.seed)

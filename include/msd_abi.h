@@ -308,8 +308,15 @@ int msd_stepper_add_applied_field(msd_stepper* st, msd_appliedfield* a);
 /* Anisotropy.Uniaxial in the fused step's field sum (u = NULL removes it); ops added
  * later are picked up by the next msd_stepper_heun */
 int msd_stepper_set_uniaxial(msd_stepper* st, msd_uniaxial* u);
-/* Advance nsteps Heun steps; asynchronous. */
+/* Advance nsteps Heun steps; asynchronous.  The fused kernels read the step's values
+ * (temperature, dt, alpha, gamma, applied fields, ...) from a device copy that is
+ * stored on the stream when they change, so a script that calls temp:set(T) or
+ * ss:setTimeStep(dt) between steps (doc/manual/BasicMicromagneticSimulation.lua)
+ * replays the same graph; only a change of kernels or shapes captures a new one. */
 int msd_stepper_heun(msd_stepper* st, int nsteps);
+/* Number of step graphs captured and instantiated so far (no counterpart in the
+ * reference; diagnostics of the graph reuse above). */
+int msd_stepper_graph_builds(const msd_stepper* st, long* n);
 /* Kernel-level timing on the library stream, names[i]/avg_ms[i] for up to cap
  * kernels.  Layered and local forms: each launch of one Heun step is repeated
  * nsteps times back to back between two HIP events (the kernels are idempotent

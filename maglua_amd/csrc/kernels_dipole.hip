@@ -574,39 +574,46 @@ hipError_t launch_cols(const DipoleDev& dp, hipStream_t st)
     return hipGetLastError();
 }
 
+// fn (optional) receives the kernel launched: a graph node of it can later take new
+// argument values (hipGraphExecKernelNodeSetParams) without a new capture
 template <bool PRED, int NX, int TY, int TPB, bool DIP, bool COMP>
-static void launch_step_t(const StepArgs& a, int g, size_t lds, hipStream_t st)
+static void launch_step_t(const StepArgs& a, int g, size_t lds, hipStream_t st, const void** fn)
 {
-    if (PRED) k_step_pred<NX, TY, TPB, DIP, COMP><<<g, TPB, lds, st>>>(a);
-    else k_step_corr<NX, TY, TPB, DIP, COMP><<<g, TPB, lds, st>>>(a);
+    if (PRED) {
+        k_step_pred<NX, TY, TPB, DIP, COMP><<<g, TPB, lds, st>>>(a);
+        if (fn) *fn = (const void*)k_step_pred<NX, TY, TPB, DIP, COMP>;
+    } else {
+        k_step_corr<NX, TY, TPB, DIP, COMP><<<g, TPB, lds, st>>>(a);
+        if (fn) *fn = (const void*)k_step_corr<NX, TY, TPB, DIP, COMP>;
+    }
 }
 
 template <bool PRED>
-static hipError_t launch_step(const StepArgs& a, hipStream_t st)
+static hipError_t launch_step(const StepArgs& a, hipStream_t st, const void** fn)
 {
     const int g = row_tiles(a.d.ny, a.d.nz, a.ty);
     const bool comp = a.Slo != nullptr;
     if (!a.has_dip) {
-        if (comp) launch_step_t<PRED, 0, 0, ROW_TPB, false, true>(a, g, 0, st);
-        else launch_step_t<PRED, 0, 0, ROW_TPB, false, false>(a, g, 0, st);
+        if (comp) launch_step_t<PRED, 0, 0, ROW_TPB, false, true>(a, g, 0, st, fn);
+        else launch_step_t<PRED, 0, 0, ROW_TPB, false, false>(a, g, 0, st, fn);
         return hipGetLastError();
     }
     const size_t lds = a.dp.lds_rows;
 #define MSD_STEP(NXV, TYV, TPBV)                                                    \
     if (a.d.nx == NXV && a.ty == TYV && a.dp.tpb == TPBV) {                         \
-        if (comp) launch_step_t<PRED, NXV, TYV, TPBV, true, true>(a, g, lds, st);   \
-        else launch_step_t<PRED, NXV, TYV, TPBV, true, false>(a, g, lds, st);       \
+        if (comp) launch_step_t<PRED, NXV, TYV, TPBV, true, true>(a, g, lds, st, fn);  \
+        else launch_step_t<PRED, NXV, TYV, TPBV, true, false>(a, g, lds, st, fn);      \
         return hipGetLastError();                                                   \
     }
     MSD_ROW_SHAPES(MSD_STEP)
 #undef MSD_STEP
-    if (comp) launch_step_t<PRED, 0, 0, ROW_TPB, true, true>(a, g, lds, st);
-    else launch_step_t<PRED, 0, 0, ROW_TPB, true, false>(a, g, lds, st);
+    if (comp) launch_step_t<PRED, 0, 0, ROW_TPB, true, true>(a, g, lds, st, fn);
+    else launch_step_t<PRED, 0, 0, ROW_TPB, true, false>(a, g, lds, st, fn);
     return hipGetLastError();
 }
 
-hipError_t launch_step_pred(const StepArgs& a, hipStream_t st) { return launch_step<true>(a, st); }
-hipError_t launch_step_corr(const StepArgs& a, hipStream_t st) { return launch_step<false>(a, st); }
+hipError_t launch_step_pred(const StepArgs& a, hipStream_t st, const void** fn) { return launch_step<true>(a, st, fn); }
+hipError_t launch_step_corr(const StepArgs& a, hipStream_t st, const void** fn) { return launch_step<false>(a, st, fn); }
 
 const char* kernel_name(int id)
 {

@@ -2748,6 +2748,19 @@ struct msd_stepper {
     hipGraphExec_t exec = nullptr;
     StepArgs args{};
     StepArgs graph_args{};
+    // The fused step kernels of each graph ([0] exact, [1] a learned window's first
+    // step, [2] later learned steps) and which arguments they take (0 exact, 1 a
+    // learned window's first predictor, 2 the other learned passes): when only values
+    // change (temp:set(T), setTimeStep) the nodes take them in place, no new capture.
+    struct StepNode {
+        hipGraphNode_t node;
+        int variant;
+    };
+    std::vector<StepNode> gnodes[3];
+    bool gnodes_ok[3] = {false, false, false};          // every step launch of the graph found
+    std::vector<std::pair<const void*, int>> cap_log;   // step kernels launched during a capture
+    bool capturing = false;
+    long graph_builds = 0;               // graphs captured and instantiated (diagnostics)
     Batch3 b3;
     uint64_t spec_version = 0;
     uint64_t dip_epoch = 0, lr3_epoch = 0;
@@ -2780,6 +2793,8 @@ struct msd_stepper {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         exec = nullptr; graph = nullptr;
+        gnodes[0].clear();
+        gnodes_ok[0] = false;
         drop_learned_graphs();
     }
     void drop_learned_graphs()
@@ -2788,6 +2803,8 @@ struct msd_stepper {
             if (lexec[k]) (void)hipGraphExecDestroy(lexec[k]);
             if (lgraph[k]) (void)hipGraphDestroy(lgraph[k]);
             lexec[k] = nullptr; lgraph[k] = nullptr;
+            gnodes[k + 1].clear();
+            gnodes_ok[k + 1] = false;
         }
     }
     int kind() const { return lr3 ? SK_CONV3 : (dip ? SK_LAYERED : SK_LOCAL); }
@@ -2808,13 +2825,35 @@ struct msd_stepper {
     }
     int launch_step(hipStream_t st);
     int launch_learned_step(hipStream_t st, bool window_start);
+    // arguments of the learned passes: no dipole term in the fused kernels, the
+    // surrogate's field (Hext, absent at a window's first predictor) and H_ref added
+    StepArgs learned_args(bool first) const
+    {
+        StepArgs a = args;
+        a.has_dip = 0;
+        a.Hext = first ? nullptr : Hext;
+        a.Href = Href;
+        return a;
+    }
+    StepArgs variant_args(int v) const { return v == 0 ? args : learned_args(v == 1); }
+    void log_step(const void* fn, int variant)
+    {
+        if (capturing) cap_log.push_back({fn, variant});
+    }
 };
 
-static bool args_equal(const StepArgs& a, const StepArgs& b) { return std::memcmp(&a, &b, sizeof(StepArgs)) == 0; }
+// Whether two argument sets launch the same kernels with the same shapes, so that a
+// graph captured for one serves the other once its step nodes take the new values.
+static bool same_launch(const StepArgs& a, const StepArgs& b)
+{
+    return std::memcmp(&a.d, &b.d, sizeof a.d) == 0 && a.has_dip == b.has_dip && (a.Slo == nullptr) == (b.Slo == nullptr) &&
+           a.ty == b.ty && std::memcmp(&a.dp, &b.dp, sizeof a.dp) == 0;
+}
 
 int msd_stepper::refresh()
 {
-    StepArgs a{};
+    StepArgs a;
+    std::memset(&a, 0, sizeof a);     // padding too: run_kind compares whole copies
     a.d = s->d;
     a.dt = (float)s->dt;
     a.S = s->S; a.P = P; a.C = C;
@@ -2880,10 +2919,14 @@ int msd_stepper::launch_step(hipStream_t st)
         e = b3.evaluate(R3_PRED, compensated, st);
         if (e == hipSuccess) e = b3.evaluate(R3_CORR, compensated, st);
     } else {
+        const void* fp = nullptr;
+        const void* fc = nullptr;
         if (args.has_dip) e = launch_cols(args.dp, st);
-        if (e == hipSuccess) e = launch_step_pred(args, st);
+        if (e == hipSuccess) e = launch_step_pred(args, st, &fp);
         if (e == hipSuccess && args.has_dip) e = launch_cols(args.dp, st);
-        if (e == hipSuccess) e = launch_step_corr(args, st);
+        if (e == hipSuccess) e = launch_step_corr(args, st, &fc);
+        log_step(fp, 0);
+        log_step(fc, 0);
     }
     if (e != hipSuccess) return fail(MSD_EDEVICE, "step launch: %s", hipGetErrorString(e));
     return MSD_OK;
@@ -2894,28 +2937,27 @@ int msd_stepper::launch_step(hipStream_t st)
 // add both parts to exchange + anisotropy (+ thermal).
 int msd_stepper::launch_learned_step(hipStream_t st, bool window_start)
 {
-    StepArgs a = args;
-    a.has_dip = 0;
-    a.Hext = Hext;
-    a.Href = Href;
+    const StepArgs a = learned_args(false);
     const float sin = (float)sur->s_in;
     hipError_t e = hipSuccess;
+    const void* fn = nullptr;
     if (window_start) {
         // S is m_ref at a window's first step: the network's input is zero and so is
         // its output (linear layers without bias), so the predictor takes H_ref alone
-        a.Hext = nullptr;
-        e = launch_step_pred(a, st);
-        a.Hext = Hext;
+        e = launch_step_pred(learned_args(true), st, &fn);
+        log_step(fn, 1);
     } else {
         e = launch_cnn_input(s->d, s->S, Mref, sin, sur->X0, st);
         if (e != hipSuccess) return fail(MSD_EDEVICE, "learned step: %s", hipGetErrorString(e));
         if (int rc = sur->forward(Hext, nullptr, st)) return rc;
-        e = launch_step_pred(a, st);
+        e = launch_step_pred(a, st, &fn);
+        log_step(fn, 2);
     }
     if (e == hipSuccess) e = launch_cnn_input(s->d, P, Mref, sin, sur->X0, st);
     if (e != hipSuccess) return fail(MSD_EDEVICE, "learned step: %s", hipGetErrorString(e));
     if (int rc = sur->forward(Hext, nullptr, st)) return rc;
-    e = launch_step_corr(a, st);
+    e = launch_step_corr(a, st, &fn);
+    log_step(fn, 2);
     if (e != hipSuccess) return fail(MSD_EDEVICE, "learned step: %s", hipGetErrorString(e));
     return MSD_OK;
 }
@@ -3025,15 +3067,52 @@ static void stepper_end(msd_stepper* st, int nsteps)
 // later learned step
 static int capture_step(msd_stepper* st, int kind, hipGraph_t* graph, hipGraphExec_t* exec)
 {
+    st->cap_log.clear();
+    st->gnodes[kind].clear();
+    st->capturing = true;
     HIPCHK(hipStreamBeginCapture(G.stream, hipStreamCaptureModeThreadLocal));
     int rc = kind ? st->launch_learned_step(G.stream, kind == 1) : st->launch_step(G.stream);
+    st->capturing = false;
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(G.stream, &g);
     if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
     if (e != hipSuccess) return fail(MSD_EDEVICE, "graph capture: %s", hipGetErrorString(e));
     *graph = g;
     HIPCHK(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    // the fused step kernels' nodes, matched by kernel: they can take new values later
+    size_t n = 0;
+    HIPCHK(hipGraphGetNodes(g, nullptr, &n));
+    std::vector<hipGraphNode_t> nodes(n);
+    if (n) HIPCHK(hipGraphGetNodes(g, nodes.data(), &n));
+    for (hipGraphNode_t node : nodes) {
+        hipGraphNodeType t;
+        if (hipGraphNodeGetType(node, &t) != hipSuccess || t != hipGraphNodeTypeKernel) continue;
+        hipKernelNodeParams p{};
+        if (hipGraphKernelNodeGetParams(node, &p) != hipSuccess) continue;
+        for (const auto& c : st->cap_log)
+            if (c.first == p.func) { st->gnodes[kind].push_back({node, c.second}); break; }
+    }
+    // the 3D form's kernels read their arguments from device tables (Batch3), which
+    // refresh() rewrites: its graphs have no step nodes to update
+    st->gnodes_ok[kind] = st->gnodes[kind].size() == st->cap_log.size();
     return MSD_OK;
+}
+
+// New argument values into the step nodes of an instantiated graph; false when the
+// graph's step nodes were not all found (the caller captures again)
+static bool update_step_nodes(msd_stepper* st, int kind, hipGraphExec_t exec)
+{
+    if (!st->gnodes_ok[kind]) return false;
+    for (const msd_stepper::StepNode& sn : st->gnodes[kind]) {
+        StepArgs v = st->variant_args(sn.variant);
+        hipKernelNodeParams p{};
+        if (hipGraphKernelNodeGetParams(sn.node, &p) != hipSuccess) return false;
+        void* kp[1] = {&v};
+        p.kernelParams = kp;
+        p.extra = nullptr;
+        if (hipGraphExecKernelNodeSetParams(exec, sn.node, &p) != hipSuccess) return false;
+    }
+    return true;
 }
 
 // nsteps of one kind (0 exact, 1 a learned window's first step, 2 later learned
@@ -3047,11 +3126,18 @@ static int run_kind(msd_stepper* st, int nsteps, int kind)
         StepArgs* gargs = kind ? &st->lgraph_args[kind - 1] : &st->graph_args;
         const bool scales_moved =
             kind && (st->lgraph_sin[kind - 1] != st->sur->s_in || st->lgraph_sout[kind - 1] != st->sur->s_out);
-        if (!*exec || !args_equal(st->args, *gargs) || scales_moved) {
+        bool fresh = *exec && same_launch(st->args, *gargs) && !scales_moved;
+        // same kernels and shapes with other values: the step nodes take them in place
+        if (fresh && std::memcmp(&st->args, gargs, sizeof(StepArgs)) != 0) {
+            fresh = update_step_nodes(st, kind, *exec);
+            if (fresh) *gargs = st->args;
+        }
+        if (!fresh) {
             if (*exec) (void)hipGraphExecDestroy(*exec);
             if (*graph) (void)hipGraphDestroy(*graph);
             *exec = nullptr; *graph = nullptr;
             if (int rc = capture_step(st, kind, graph, exec)) return rc;
+            st->graph_builds++;
             *gargs = st->args;
             if (kind) { st->lgraph_sin[kind - 1] = st->sur->s_in; st->lgraph_sout[kind - 1] = st->sur->s_out; }
         }
@@ -3263,6 +3349,13 @@ extern "C" int msd_stepper_set_surrogate(msd_stepper* st, msd_surrogate* sur, in
     st->sur_windows = st->sur_exact_windows = st->sur_fallbacks = 0;
     st->sur_last_res = st->sur_max_res = 0.0;
     st->drop_learned_graphs();
+    return MSD_OK;
+}
+
+extern "C" int msd_stepper_graph_builds(const msd_stepper* st, long* n)
+{
+    NONNULL(st); NONNULL(n);
+    *n = st->graph_builds;
     return MSD_OK;
 }
 

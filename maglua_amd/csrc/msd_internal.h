@@ -155,8 +155,10 @@ struct StepArgs {
     float3 Bapp;
     const float* Happ;      // [3][n] or nullptr
 };
-hipError_t launch_step_pred(const StepArgs& a, hipStream_t st);
-hipError_t launch_step_corr(const StepArgs& a, hipStream_t st);
+// fn (optional) receives the kernel launched, so a graph node of it can take new
+// argument values without a new capture
+hipError_t launch_step_pred(const StepArgs& a, hipStream_t st, const void** fn = nullptr);
+hipError_t launch_step_corr(const StepArgs& a, hipStream_t st, const void** fn = nullptr);
 
 // ------------------------------------------------ 3D convolution (LongRange3D)
 // FFT grid (fx, fy, fz) equals the lattice when periodic and doubles it when

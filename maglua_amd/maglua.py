@@ -1085,6 +1085,12 @@ class Heun(_Handle):
         call("msd_stepper_autotune", self._h, int(steps), C.byref(ty), C.byref(tpb), C.byref(ms))
         return {"rows_per_tile": ty.value, "threads": tpb.value, "ms_per_step": ms.value}
 
+    def graphBuilds(self):
+        """Step graphs captured so far (values that change between steps reuse the graph)."""
+        n = C.c_long()
+        call("msd_stepper_graph_builds", self._h, C.byref(n))
+        return n.value
+
     def surrogateStats(self):
         w, e, f = C.c_long(), C.c_long(), C.c_long()
         last, mx = C.c_double(), C.c_double()

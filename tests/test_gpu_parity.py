@@ -587,3 +587,25 @@ def test_ensemble_rejects_members_sharing_a_thermal():
         M.Ensemble([h1, h2])
     h3 = M.Heun(g2["ss"], g2["ex"], g2["ani"], None, g2["temp"], g2["llg"])
     M.Ensemble([h1, h3]).step(2)
+
+
+def test_changing_temperature_and_dt_reuses_the_step_graph():
+    """temp:set(T) and setTimeStep between steps (a temperature sweep script): the graph
+    captured once replays the new values, and the steps equal direct launches bit for
+    bit (so no stale value is replayed)."""
+    spec = W.FilmSpec(32, 32, 2, T=0.4, dt=0.005)
+    runs = []
+    for graph in (True, False):
+        g = W.build(M, spec)
+        g["ss"].randomize(8)
+        heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"], graph=graph)
+        for k in range(12):
+            g["temp"].setTemperature(0.4 + 0.1 * k)
+            if k % 4 == 3:
+                g["ss"].setTimeStep(0.005 * (1 + k / 10))
+            heun.step(1)
+        M.synchronize()
+        runs.append((flat(g["ss"].spins()), g["ss"].time(), heun.graphBuilds()))
+    print(f"graph builds over 12 steps with new T every step: {runs[0][2]}", flush=True)
+    assert runs[0][2] == 1
+    assert np.array_equal(runs[0][0], runs[1][0]) and runs[0][1] == runs[1][1]
