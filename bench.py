@@ -108,14 +108,16 @@ CNN_USEFUL_FLOP = 2 * 75 * (3 * 16 + 16 * 16 + 16 * 3)      # per site per evalu
 CNN_EXEC_FLOP = 64 * 2 * 16 * 16 * 32 // 16                  # per site: 64 16x16x32 MFMAs per 16 sites (10 + 45 + 9)
 
 
-def learned_vs_plain(M, W, train_steps=300, train_K=10, Ks=(10, 8, 6, 5, 4, 3, 2), gate=1e-4, field_gate=1e-3,
-                     gate_steps=100, tol=1e-2, steps=500):
+def learned_vs_plain(M, W, train_pairs=20000, states=4, train_K=10, Ks=(10, 8, 6, 5, 4, 3, 2), gate=1e-4, tol=1e-3,
+                     gate_steps=100, steps=500):
     """The 512x512x4 film (g=0.1, T=0.2): exact layered-FFT steps against the surrogate
-    with an exact refresh every K steps, from one trained network.  K is the largest
-    of Ks that passes both gates over gate_steps steps from identical states and
-    noise: the learned trajectory within max|dm| <= gate of the exact one, and the
-    predicted dipole field at every window end within field_gate (max relative
-    error) of the exact one.  The learned rate is timed at that K."""
+    refreshed exactly every K steps, with the configured residual tolerance (1e-3
+    relative: a window whose field check at its end exceeds it is run again exactly).
+    Training: train_pairs (m - m_ref, H_dip(m) - H_dip(m_ref)) pairs from exact
+    trajectories of `states` seeded random initial states.  For each K: the learned
+    trajectory against the exact one (same state and noise) over gate_steps steps, the
+    share of windows that fell back, and the learned rate; the reported K is the
+    fastest whose trajectory stays within max|dm| <= gate."""
     import numpy as np
     from maglua_amd import surrogate as S
 
@@ -129,15 +131,61 @@ def learned_vs_plain(M, W, train_steps=300, train_K=10, Ks=(10, 8, 6, 5, 4, 3, 2
     t0 = time.perf_counter()
     heun.step(steps)
     M.synchronize()
-    plain = spec.n * steps / (time.perf_counter() - t0)
-    sur = M.Surrogate(spec.nx, spec.ny, spec.nz)
+    plain_s = (time.perf_counter() - t0) / steps
+    plain = spec.n / plain_s
+    # the local part of a step alone (exchange, anisotropy, thermal, no dipole): what a
+    # learned step costs besides its two network evaluations
+    loc = W.build(M, spec, with_dipole=False)
+    loc["ss"].randomize(2)
+    hl = M.Heun(loc["ss"], loc["ex"], loc["ani"], None, loc["temp"], loc["llg"])
+    hl.step(20)
+    M.synchronize()
     t0 = time.perf_counter()
-    losses = S.train_on_trajectory(sur, g["ss"], heun, g["dip"], steps=train_steps, K=train_K, lr=2e-3, seed=1)
+    hl.step(steps)
+    M.synchronize()
+    local_s = (time.perf_counter() - t0) / steps
+    del hl, loc
+
+    sur = M.Surrogate(spec.nx, spec.ny, spec.nz)
+    per = train_pairs // states
+    t0 = time.perf_counter()
+    losses = []
+    for k in range(states):
+        h = W.build(M, spec)
+        h["ss"].randomize(100 + k)
+        h["temp"].setSeed(100 + k)
+        hk = M.Heun(h["ss"], h["ex"], h["ani"], h["dip"], h["temp"], h["llg"])
+        hk.step(50)
+        losses += S.train_on_trajectory(sur, h["ss"], hk, h["dip"], steps=per, K=train_K, lr=2e-3, seed=1,
+                                        resume=k > 0)
+        del hk, h
     M.synchronize()
     train_s = time.perf_counter() - t0
+
+    # field error train_K steps after a refresh on a state the training never saw, beside
+    # the frozen field and the floor of a linear network with this receptive field: the
+    # exact kernel cut to |dx|, |dy| <= 6 (all four layers), setTruncation({1000, 6, 6})
     ref = g["ss"].copy()
     heun.step(train_K)
     err, frozen = sur.fieldError(g["dip"], g["ss"], ref)
+    a, b = g["ss"].spins(), ref.spins()
+    dm = [np.asarray(a[c]) - np.asarray(b[c]) for c in range(3)]
+    sd = M.SpinSystem(spec.nx, spec.ny, spec.nz)
+    sd.setSpins(*dm)
+    full = M.Dipole(spec.nx, spec.ny, spec.nz)
+    full.setStrength(spec.g)
+    cut = M.Dipole(spec.nx, spec.ny, spec.nz)
+    cut.setStrength(spec.g)
+    cut.setTruncation((1000, 6, 6))   # l_settrunc's table form: radial 1000, x and y limits 6
+    full.apply(sd)
+    hf = np.stack(sd.field("Dipole"))
+    cut.apply(sd)
+    hc = np.stack(sd.field("Dipole"))
+    g["dip"].apply(g["ss"])
+    hmax = float(np.abs(np.stack(g["ss"].field("Dipole"))).max())
+    floor = float(np.abs(hc - hf).max()) / hmax
+    del sd, full, cut, hf, hc
+
     # one network evaluation, host-timed over back-to-back calls
     sur.setInputSystems(g["ss"], ref)
     sur.forward(download=False)
@@ -147,8 +195,10 @@ def learned_vs_plain(M, W, train_steps=300, train_K=10, Ks=(10, 8, 6, 5, 4, 3, 2
         sur.forward(download=False)
     M.synchronize()
     cnn_s = (time.perf_counter() - t0) / 50
+    del ref
 
-    def trajectory_dmax(K):
+    scan, best = [], None
+    for K in Ks:
         a2, b2 = g["ss"].copy(), g["ss"].copy()
         ta2, tb2 = W.build(M, spec)["temp"], W.build(M, spec)["temp"]
         ta2.setSeed(9)
@@ -158,45 +208,45 @@ def learned_vs_plain(M, W, train_steps=300, train_K=10, Ks=(10, 8, 6, 5, 4, 3, 2
         eb2.setSurrogate(sur, K=K, tol=tol)
         ea2.step(gate_steps)
         eb2.step(gate_steps)
-        dm = float(np.abs(np.stack(a2.spins()[:3]) - np.stack(b2.spins()[:3])).max())
-        return dm, float(eb2.surrogateStats()["max_residual"])
-
-    scan = []
-    K = None
-    for k in Ks:
-        d, fe = trajectory_dmax(k)
-        scan.append({"K": k, "traj_max_dm": d, "field_err_max": fe})
-        if d <= gate and fe <= field_gate:
-            K = k
-            break
-    gate_ok = K is not None
-    if K is None:
-        K = Ks[-1]
-    # learned throughput at the chosen K
-    b = g["ss"].copy()
-    tb = W.build(M, spec)["temp"]
-    tb.setSeed(7)
-    eb = M.Heun(b, g["ex"], g["ani"], g["dip"], tb, g["llg"])
-    eb.setSurrogate(sur, K=K, tol=tol)
-    eb.step(2 * K)
-    M.synchronize()
-    t0 = time.perf_counter()
-    eb.step(steps)
-    M.synchronize()
-    learned = spec.n * steps / (time.perf_counter() - t0)
-    stats = eb.surrogateStats()
+        d = float(np.abs(np.stack(a2.spins()[:3]) - np.stack(b2.spins()[:3])).max())
+        st = eb2.surrogateStats()
+        # the learned rate at this K, fallbacks included
+        eb2.step(2 * K)
+        M.synchronize()
+        t0 = time.perf_counter()
+        eb2.step(steps)
+        M.synchronize()
+        rate = spec.n * steps / (time.perf_counter() - t0)
+        st2 = eb2.surrogateStats()
+        row = {"K": K, "traj_max_dm": d, "windows": st["windows"], "fallback_windows": st["fallbacks"],
+               "max_residual": round(st["max_residual"], 6), "learned_value": round(rate, 1),
+               "ratio": round(rate / plain, 4),
+               "timed_fallback_share": round((st2["fallbacks"] - st["fallbacks"]) /
+                                             max(1, st2["windows"] - st["windows"]), 4)}
+        scan.append(row)
+        if d <= gate and (best is None or rate > best["learned_value"]):
+            best = row
+        del a2, b2, ea2, eb2
+    gate_ok = best is not None
+    if best is None:
+        best = scan[0]
+    # Amdahl: with a free network, a learned step is the local part alone; with the
+    # network at its measured cost it adds two evaluations per step
     return {
         "workload": spec.name() + ", plain layered FFT vs surrogate refreshed every K steps",
-        "plain_value": round(plain, 1), "learned_value": round(learned, 1), "ratio": round(learned / plain, 4),
-        "unit": "spin-steps/s", "K": K, "gate": f"max|dm| <= {gate:g} over {gate_steps} steps against the exact "
-        f"trajectory (same state and noise) and dipole-field error <= {field_gate:g} at every window end",
-        "gate_met": gate_ok, "K_scan": scan, "residual_tol": tol,
-        "field_error_after_train_K": round(err, 6), "frozen_field_error_after_train_K": round(frozen, 6),
-        "windows": stats["windows"], "fallback_windows": stats["exact_windows"],
-        "max_residual": round(stats["max_residual"], 6),
-        "train_pairs": train_steps, "train_K": train_K, "train_s": round(train_s, 3),
-        "final_loss": round(float(np.mean(losses[-20:])), 6),
+        "plain_value": round(plain, 1), "learned_value": best["learned_value"], "ratio": best["ratio"],
+        "unit": "spin-steps/s", "K": best["K"], "residual_tol": tol,
+        "gate": f"max|dm| <= {gate:g} over {gate_steps} steps against the exact trajectory (same state and "
+                f"noise); every window kept has its field error at its end <= {tol:g} (others run again exactly)",
+        "gate_met": gate_ok, "K_scan": scan,
+        "plain_step_us": round(1e6 * plain_s, 2), "local_step_us": round(1e6 * local_s, 2),
         "cnn_eval_us": round(1e6 * cnn_s, 2),
+        "amdahl_bound_free_network": round(plain_s / local_s, 3),
+        "bound_at_measured_network": round(plain_s / (local_s + 2 * cnn_s), 3),
+        "field_error_after_train_K": round(err, 6), "frozen_field_error_after_train_K": round(frozen, 6),
+        "truncated_kernel_floor_after_train_K": round(floor, 6),
+        "train_pairs": per * states, "train_states": states, "train_K": train_K, "train_s": round(train_s, 2),
+        "final_loss": round(float(np.mean(losses[-200:])), 6),
         "cnn_mfma_frac_useful": round(CNN_USEFUL_FLOP * spec.n / cnn_s / BF16_DENSE_PEAK, 4),
         "cnn_mfma_frac_executed": round(CNN_EXEC_FLOP * spec.n / cnn_s / BF16_DENSE_PEAK, 4),
         "network_dtype": "bf16 operands, fp32 accumulate",
@@ -381,7 +431,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain comparison on the 512x512x4 film")
-    ap.add_argument("--train-pairs", type=int, default=600, help="surrogate training pairs for the learned leg")
+    ap.add_argument("--train-pairs", type=int, default=20000, help="surrogate training pairs for the learned leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched-replica and 256^3 open-cube legs")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--plain", action="store_true", help="plain FP32 spin state instead of the compensated pair")
@@ -440,7 +490,7 @@ def main():
     learned = None
     if not args.no_learned:
         print("# learned vs plain (512x512x4, training the surrogate first)...", file=sys.stderr, flush=True)
-        learned = learned_vs_plain(M, W, train_steps=args.train_pairs)
+        learned = learned_vs_plain(M, W, train_pairs=args.train_pairs)
 
     ensemble = cube = None
     if not args.no_extra:

@@ -38,16 +38,20 @@ def fit_scales(sur, ss, ref, dip):
     return 1.0 / rms_in, rms_out
 
 
-def train_on_trajectory(sur, ss, heun, dip, steps, K=10, lr=2e-3, seed=0, weights=None, log_every=0):
+def train_on_trajectory(sur, ss, heun, dip, steps, K=10, lr=2e-3, seed=0, weights=None, log_every=0, resume=False):
     """Advance `heun` (an exact Dipole or LongRange3D step driver on `ss`) `steps` steps, training `sur`
     on every step's (m - m_ref, H(m) - H(m_ref)) pair, m_ref the state at the start of
-    the current K-step window.  Returns the list of losses."""
-    for l, Wl in enumerate(weights if weights is not None else init_weights(seed)):
-        sur.setWeights(l, Wl)
-    sur.resetOptimizer()
+    the current K-step window.  resume=True continues from the network's current weights,
+    scales and optimizer state (another trajectory of the same training run); otherwise
+    the weights start from `weights` or init_weights(seed) and the scales are fitted
+    on the first pair.  Returns the list of losses."""
+    if not resume:
+        for l, Wl in enumerate(weights if weights is not None else init_weights(seed)):
+            sur.setWeights(l, Wl)
+        sur.resetOptimizer()
     ref = ss.copy()
     losses = []
-    scaled = False
+    scaled = resume
     for k in range(steps):
         if k % K == 0:
             ss.copySpinsTo(ref)
