@@ -307,6 +307,35 @@ def ensemble_leg(M, W, nrep=64, steps=50, R=16):
             "total_field_dot_spin_per_site": hdot}
 
 
+def films_leg(M, W, nrep=8, steps=100):
+    """nrep 256x256x8 films (the film's J, K, g, alpha, dt; temperatures 0.1..0.8, own
+    seeds) in one Ensemble: each pass of a Heun step is one launch over all films."""
+    from dataclasses import replace
+    t0 = time.perf_counter()
+    members, keep, nsites, temps = [], [], 0, []
+    for i in range(nrep):
+        T = 0.1 + (0.8 - 0.1) * i / max(1, nrep - 1)
+        spec = replace(W.FILM_256_T05, T=T, seed=i, noise_seed=i)
+        g = W.build(M, spec)
+        g["ss"].randomize(i)
+        members.append(M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"]))
+        keep.append(g)
+        nsites += spec.n
+        temps.append(round(T, 4))
+    E = M.Ensemble(members)
+    E.step(5)
+    M.synchronize()
+    setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    E.step(steps)
+    M.synchronize()
+    el = time.perf_counter() - t0
+    return {"workload": f"{nrep} films 256x256x8 (J=1, K=0.05, g=0.02, alpha=0.1, dt=0.002), T 0.1..0.8, one "
+                        "Ensemble: one launch per pass for all films",
+            "value": round(nsites * steps / el, 1), "unit": "spin-steps/s", "sites": nsites, "steps": steps,
+            "ms_per_step": round(1e3 * el / steps, 4), "setup_s": round(setup, 2), "temperatures": temps}
+
+
 def cube_learned(M, W, spec, g, ani, lr, heun, plain, train_steps=100, train_K=8, Ks=(8, 4, 2, 1), gate=1e-4,
                  field_gate=1e-3, gate_steps=40, tol=1e-2, steps=40):
     """The 256^3 open cube with the surrogate (zero-padded in x and y) standing in for
@@ -496,6 +525,7 @@ def main():
     if not args.no_extra:
         print("# batched replicas and open cubes...", file=sys.stderr, flush=True)
         ensemble = ensemble_leg(M, W)
+        ensemble["films"] = films_leg(M, W)
         cube = cube_leg(M, W)
 
     cpu = None

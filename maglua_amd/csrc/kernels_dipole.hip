@@ -200,14 +200,14 @@ __global__ void __launch_bounds__(TPB) k_rows_inv(DipoleDev dp, float* __restric
 // the per-k layer/component tensor product for component a, inverse along y of
 // its nz lines.  Three workgroups per kx (one per output component) give the
 // launch 3(nx/2+1) workgroups; the input column is read three times, from L2.
+// wg: the workgroup's place among the 3 (nx/2+1) of its film (XCD-remapped, so the
+// three workgroups of one kx run on one XCD and two of them find the column in
+// that XCD's L2)
 template <int NY, int NZ>
-__global__ void __launch_bounds__(col_tpb(NY)) k_cols(DipoleDev dp)
+__device__ __forceinline__ void cols_body(const DipoleDev& dp, int wg)
 {
     constexpr int COL_TPB = col_tpb(NY);
     extern __shared__ float2 lds[];
-    // the three workgroups of one kx run on one XCD, so two of them find the
-    // column in that XCD's L2
-    const int wg = xcd_remap(blockIdx.x, gridDim.x);
     const int kx = wg / 3;
     const int oa = wg - 3 * kx;
     const int ny = NY ? NY : dp.ny, nxh = dp.nxh;
@@ -348,6 +348,20 @@ __global__ void __launch_bounds__(col_tpb(NY)) k_cols(DipoleDev dp)
     }
 }
 
+template <int NY, int NZ>
+__global__ void __launch_bounds__(col_tpb(NY)) k_cols(DipoleDev dp)
+{
+    cols_body<NY, NZ>(dp, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+// films of one shape in one launch: `per` workgroups each, film r's at [r per, (r+1) per)
+template <int NY, int NZ>
+__global__ void __launch_bounds__(col_tpb(NY)) k_cols_batch(const DipoleDev* __restrict__ dps, int per)
+{
+    const int w = xcd_remap(blockIdx.x, gridDim.x), r = w / per;
+    cols_body<NY, NZ>(dps[r], w - r * per);
+}
+
 // ------------------------------------------------------------ fused Heun
 // Site loop of the fused kernels.  Compile-time shapes give each thread SPT
 // sites, software-pipelined: site 0's loads are issued before the inverse x
@@ -412,16 +426,16 @@ __device__ __forceinline__ void fused_sites(const StepArgs& a, const ExchangeDev
 //   P = N(S + dt dM),  C = N(S + dt/2 dM),
 // then the x-transform of P for the next column pass.  COMP carries S and C as
 // compensated FP32 pairs (hi + lo).
+// tile: the workgroup's tile among its film's nz (ny/ty), XCD-remapped layer fastest
+// (an XCD's run of tiles is a band of rows through every layer, so the z
+// neighbours of its sites are mostly in its own L2)
 template <int NX, int TY, int TPB, bool DIP, bool COMP>
-__global__ void __launch_bounds__(TPB) k_step_pred(StepArgs a)
+__device__ __forceinline__ void step_pred_body(const StepArgs& a, int tile)
 {
     extern __shared__ float2 lds[];
     const DipoleDev& dp = a.dp;
     const int nx = NX ? NX : a.d.nx, ty = TY ? TY : a.ty, lines = row_lines(ty);
     const int ny = a.d.ny;
-    // layer fastest: an XCD's run of tiles is a band of rows through every
-    // layer, so the z neighbours of its sites are mostly in its own L2
-    const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int yb = tile / a.d.nz, z = tile - yb * a.d.nz, y0 = yb * ty;
     float2* A = lds;
     float2* B = lds + (size_t)lines * nx;
@@ -435,7 +449,7 @@ __global__ void __launch_bounds__(TPB) k_step_pred(StepArgs a)
         rows_pack_spectrum<TPB, NX, TY>(dp.Hh, nx, ny, a.d.nz, ty, z, y0, A);
     }
     __syncthreads();
-    if (a.th.on && blockIdx.x == 0 && threadIdx.x == 0) *a.draw += 1ull;
+    if (a.th.on && tile == 0 && threadIdx.x == 0) *a.draw += 1ull;
     fused_sites<NX, TY, TPB, DIP, true, COMP>(a, exl, A, B, R, tw, nx, ny, ty, z, y0, 0ull);
     if constexpr (DIP) {
         __syncthreads();
@@ -447,16 +461,16 @@ __global__ void __launch_bounds__(TPB) k_step_pred(StepArgs a)
 // Corrector half (PredictorCorrector.lua:81-85) and the thermal kick (:98-103):
 //   S' = N(C + dt/2 LLG(P, H(P))),  S'' = N(S' + dt LLG(S', H_th)) when Thermal is present,
 // then the x-transform of S'' for the next step's first column pass.
+// tile: the workgroup's tile among its film's nz (ny/ty), XCD-remapped layer fastest
+// (an XCD's run of tiles is a band of rows through every layer, so the z
+// neighbours of its sites are mostly in its own L2)
 template <int NX, int TY, int TPB, bool DIP, bool COMP>
-__global__ void __launch_bounds__(TPB) k_step_corr(StepArgs a)
+__device__ __forceinline__ void step_corr_body(const StepArgs& a, int tile)
 {
     extern __shared__ float2 lds[];
     const DipoleDev& dp = a.dp;
     const int nx = NX ? NX : a.d.nx, ty = TY ? TY : a.ty, lines = row_lines(ty);
     const int ny = a.d.ny;
-    // layer fastest: an XCD's run of tiles is a band of rows through every
-    // layer, so the z neighbours of its sites are mostly in its own L2
-    const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int yb = tile / a.d.nz, z = tile - yb * a.d.nz, y0 = yb * ty;
     float2* A = lds;
     float2* B = lds + (size_t)lines * nx;
@@ -477,6 +491,34 @@ __global__ void __launch_bounds__(TPB) k_step_corr(StepArgs a)
         const float2* F = row_fft<NX, TY, TPB, -1>(R, R == A ? B : A, dp, tw);
         rows_unpack_spectrum<TPB>(F, nx, ny, a.d.nz, ty, z, y0, dp.Mh);
     }
+}
+
+template <int NX, int TY, int TPB, bool DIP, bool COMP>
+__global__ void __launch_bounds__(TPB) k_step_pred(StepArgs a)
+{
+    step_pred_body<NX, TY, TPB, DIP, COMP>(a, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+template <int NX, int TY, int TPB, bool DIP, bool COMP>
+__global__ void __launch_bounds__(TPB) k_step_corr(StepArgs a)
+{
+    step_corr_body<NX, TY, TPB, DIP, COMP>(a, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+// films of one shape in one launch: film r's arguments at as[r], its `per` tiles at
+// [r per, (r+1) per) of the remapped workgroup index
+template <int NX, int TY, int TPB, bool COMP>
+__global__ void __launch_bounds__(TPB) k_step_pred_batch(const StepArgs* __restrict__ as, int per)
+{
+    const int w = xcd_remap(blockIdx.x, gridDim.x), r = w / per;
+    step_pred_body<NX, TY, TPB, true, COMP>(as[r], w - r * per);
+}
+
+template <int NX, int TY, int TPB, bool COMP>
+__global__ void __launch_bounds__(TPB) k_step_corr_batch(const StepArgs* __restrict__ as, int per)
+{
+    const int w = xcd_remap(blockIdx.x, gridDim.x), r = w / per;
+    step_corr_body<NX, TY, TPB, true, COMP>(as[r], w - r * per);
 }
 
 // ------------------------------------------------------------- dispatch
@@ -615,6 +657,76 @@ static hipError_t launch_step(const StepArgs& a, hipStream_t st, const void** fn
 hipError_t launch_step_pred(const StepArgs& a, hipStream_t st, const void** fn) { return launch_step<true>(a, st, fn); }
 hipError_t launch_step_corr(const StepArgs& a, hipStream_t st, const void** fn) { return launch_step<false>(a, st, fn); }
 
+// nrep films shaped like a0 (dimensions, row tiles, column length, state form),
+// arguments as[] in device memory
+template <bool PRED>
+static hipError_t launch_step_batch(const StepArgs& a0, const StepArgs* as, int nrep, hipStream_t st)
+{
+    const int per = row_tiles(a0.d.ny, a0.d.nz, a0.ty), g = per * nrep;
+    const bool comp = a0.Slo != nullptr;
+    const size_t lds = a0.dp.lds_rows;
+#define MSD_STEPB(NXV, TYV, TPBV)                                                   \
+    if (a0.d.nx == NXV && a0.ty == TYV && a0.dp.tpb == TPBV) {                      \
+        if (PRED) {                                                                 \
+            if (comp) k_step_pred_batch<NXV, TYV, TPBV, true><<<g, TPBV, lds, st>>>(as, per);   \
+            else k_step_pred_batch<NXV, TYV, TPBV, false><<<g, TPBV, lds, st>>>(as, per);       \
+        } else {                                                                    \
+            if (comp) k_step_corr_batch<NXV, TYV, TPBV, true><<<g, TPBV, lds, st>>>(as, per);   \
+            else k_step_corr_batch<NXV, TYV, TPBV, false><<<g, TPBV, lds, st>>>(as, per);       \
+        }                                                                           \
+        return hipGetLastError();                                                   \
+    }
+    MSD_ROW_SHAPES(MSD_STEPB)
+#undef MSD_STEPB
+    if (PRED) {
+        if (comp) k_step_pred_batch<0, 0, ROW_TPB, true><<<g, ROW_TPB, lds, st>>>(as, per);
+        else k_step_pred_batch<0, 0, ROW_TPB, false><<<g, ROW_TPB, lds, st>>>(as, per);
+    } else {
+        if (comp) k_step_corr_batch<0, 0, ROW_TPB, true><<<g, ROW_TPB, lds, st>>>(as, per);
+        else k_step_corr_batch<0, 0, ROW_TPB, false><<<g, ROW_TPB, lds, st>>>(as, per);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_step_pred_batch(const StepArgs& a0, const StepArgs* as, int nrep, hipStream_t st)
+{
+    return launch_step_batch<true>(a0, as, nrep, st);
+}
+hipError_t launch_step_corr_batch(const StepArgs& a0, const StepArgs* as, int nrep, hipStream_t st)
+{
+    return launch_step_batch<false>(a0, as, nrep, st);
+}
+
+template <int NZ>
+static void launch_cols_batch_nz(const DipoleDev& d0, const DipoleDev* dps, int nrep, hipStream_t st)
+{
+    const int per = 3 * d0.nxh;
+#define MSD_COLB(NYV)                                                               \
+    if (d0.ny == NYV) {                                                             \
+        k_cols_batch<NYV, NZ><<<per * nrep, col_tpb(NYV), d0.lds_cols, st>>>(dps, per); \
+        return;                                                                     \
+    }
+    MSD_COL_LENGTHS(MSD_COLB)
+#undef MSD_COLB
+    k_cols_batch<0, NZ><<<per * nrep, col_tpb(0), d0.lds_cols, st>>>(dps, per);
+}
+
+hipError_t launch_cols_batch(const DipoleDev& d0, const DipoleDev* dps, int nrep, hipStream_t st)
+{
+    switch (d0.nz) {
+        case 1: launch_cols_batch_nz<1>(d0, dps, nrep, st); break;
+        case 2: launch_cols_batch_nz<2>(d0, dps, nrep, st); break;
+        case 3: launch_cols_batch_nz<3>(d0, dps, nrep, st); break;
+        case 4: launch_cols_batch_nz<4>(d0, dps, nrep, st); break;
+        case 5: launch_cols_batch_nz<5>(d0, dps, nrep, st); break;
+        case 6: launch_cols_batch_nz<6>(d0, dps, nrep, st); break;
+        case 7: launch_cols_batch_nz<7>(d0, dps, nrep, st); break;
+        case 8: launch_cols_batch_nz<8>(d0, dps, nrep, st); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 const char* kernel_name(int id)
 {
     static const char* names[] = {"k_cols", "k_step_pred", "k_step_corr", "k_rows_fwd"};
@@ -631,10 +743,11 @@ static void set_lds(const void* f, int v, hipError_t& first)
 template <int NZ>
 static void cols_attr(int v, hipError_t& first)
 {
-#define MSD_CA(NYV) set_lds((const void*)k_cols<NYV, NZ>, v, first);
+#define MSD_CA(NYV) set_lds((const void*)k_cols<NYV, NZ>, v, first); set_lds((const void*)k_cols_batch<NYV, NZ>, v, first);
     MSD_COL_LENGTHS(MSD_CA)
 #undef MSD_CA
     set_lds((const void*)k_cols<0, NZ>, v, first);
+    set_lds((const void*)k_cols_batch<0, NZ>, v, first);
 }
 
 hipError_t configure_lds(size_t max_bytes)
@@ -647,7 +760,11 @@ hipError_t configure_lds(size_t max_bytes)
     set_lds((const void*)k_step_pred<NXV, TYV, TPBV, true, true>, v, first);        \
     set_lds((const void*)k_step_pred<NXV, TYV, TPBV, true, false>, v, first);       \
     set_lds((const void*)k_step_corr<NXV, TYV, TPBV, true, true>, v, first);        \
-    set_lds((const void*)k_step_corr<NXV, TYV, TPBV, true, false>, v, first);
+    set_lds((const void*)k_step_corr<NXV, TYV, TPBV, true, false>, v, first);   \
+    set_lds((const void*)k_step_pred_batch<NXV, TYV, TPBV, true>, v, first);      \
+    set_lds((const void*)k_step_pred_batch<NXV, TYV, TPBV, false>, v, first);     \
+    set_lds((const void*)k_step_corr_batch<NXV, TYV, TPBV, true>, v, first);      \
+    set_lds((const void*)k_step_corr_batch<NXV, TYV, TPBV, false>, v, first);
     MSD_ROW_SHAPES(MSD_RA)
     MSD_RA(0, 0, ROW_TPB)
 #undef MSD_RA

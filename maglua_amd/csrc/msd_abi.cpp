@@ -2283,7 +2283,7 @@ struct Batch3 {
             if (int rc = dalloc(&reps, r.size())) return rc;
             nrep = (int)r.size();
         }
-        if (!same_shape || std::memcmp(host.data(), r.data(), sizeof(Rep3) * r.size()) != 0) {
+        if (!r.empty() && (!same_shape || std::memcmp(host.data(), r.data(), sizeof(Rep3) * r.size()) != 0)) {
             host = r;
             HIPCHK(hipMemcpyAsync(reps, host.data(), sizeof(Rep3) * host.size(), hipMemcpyHostToDevice, G.stream));
             HIPCHK(hipStreamSynchronize(G.stream));
@@ -3492,17 +3492,65 @@ extern "C" int msd_stepper_profile(msd_stepper* st, int nsteps, int cap, char na
 // Independent replicas (uneven sizes, own temperatures and seeds) advanced by
 // one set of launches per step: every member is a step driver of the local or
 // the 3D form; its descriptor is one entry of the batch.
+// Films (layered-Dipole members) of one shape -- dimensions, row tiles, column pass and
+// state form -- step together: each pass is one launch over all of them, the films'
+// arguments and dipole descriptors read from device tables.
+struct FilmGroup {
+    std::vector<msd_stepper*> mem;
+    std::vector<StepArgs> hargs;
+    std::vector<DipoleDev> hdps;
+    StepArgs* dargs = nullptr;
+    DipoleDev* ddps = nullptr;
+    static bool same_shape(const StepArgs& a, const StepArgs& b)
+    {
+        return std::memcmp(&a.d, &b.d, sizeof a.d) == 0 && a.ty == b.ty && (a.Slo == nullptr) == (b.Slo == nullptr) &&
+               a.dp.tpb == b.dp.tpb && a.dp.ty == b.dp.ty && a.dp.lds_rows == b.dp.lds_rows &&
+               a.dp.lds_cols == b.dp.lds_cols && std::memcmp(&a.dp.px, &b.dp.px, sizeof a.dp.px) == 0 &&
+               std::memcmp(&a.dp.py, &b.dp.py, sizeof a.dp.py) == 0;
+    }
+    void release() { dfree(dargs); dfree(ddps); }
+    // the tables into device memory when a value changed (not inside a capture)
+    int upload()
+    {
+        std::vector<StepArgs> a;
+        std::vector<DipoleDev> d;
+        for (msd_stepper* s : mem) { a.push_back(s->args); d.push_back(s->args.dp); }
+        if (!dargs) {
+            if (int rc = dalloc(&dargs, mem.size())) return rc;
+            if (int rc = dalloc(&ddps, mem.size())) return rc;
+        }
+        if (a.size() != hargs.size() || std::memcmp(a.data(), hargs.data(), sizeof(StepArgs) * a.size()) != 0 ||
+            std::memcmp(d.data(), hdps.data(), sizeof(DipoleDev) * d.size()) != 0) {
+            hargs = a;
+            hdps = d;
+            HIPCHK(hipMemcpyAsync(dargs, hargs.data(), sizeof(StepArgs) * hargs.size(), hipMemcpyHostToDevice, G.stream));
+            HIPCHK(hipMemcpyAsync(ddps, hdps.data(), sizeof(DipoleDev) * hdps.size(), hipMemcpyHostToDevice, G.stream));
+            HIPCHK(hipStreamSynchronize(G.stream));
+        }
+        return MSD_OK;
+    }
+    hipError_t step(bool pred, hipStream_t st) const
+    {
+        const int n = (int)mem.size();
+        hipError_t e = launch_cols_batch(hargs[0].dp, ddps, n, st);
+        if (e == hipSuccess) e = pred ? launch_step_pred_batch(hargs[0], dargs, n, st) : launch_step_corr_batch(hargs[0], dargs, n, st);
+        return e;
+    }
+};
+
 struct msd_ensemble {
     std::vector<msd_stepper*> m;
     bool compensated = true;
     bool use_graph = true;
-    Batch3 b;
+    Batch3 b;                        // local-field and LongRange3D members
+    std::vector<FilmGroup> films;    // layered-Dipole members, by shape
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     ~msd_ensemble()
     {
         (void)hipStreamSynchronize(G.stream);
         drop_graph();
+        for (FilmGroup& f : films) f.release();
     }
     void drop_graph()
     {
@@ -3515,15 +3563,42 @@ struct msd_ensemble {
         std::vector<Rep3> r;
         std::vector<int> tys;
         std::vector<size_t> lds;
+        std::vector<msd_stepper*> layered;
         for (msd_stepper* s : m) {
             s->compensated = compensated;
             if (int rc = s->refresh()) return rc;
+            if (s->kind() == SK_LAYERED) { layered.push_back(s); continue; }
             r.push_back(s->rep());
             tys.push_back(s->args.ty);
             if (s->lr3) { lds.push_back(s->lr3->lds_rows); lds.push_back(s->lr3->lds_y); lds.push_back(s->lr3->lds_z); }
             else { lds.push_back(0); lds.push_back(0); lds.push_back(0); }
         }
+        // films by shape; a shape change (compensated state, autotuned tiles) regroups
+        std::vector<std::vector<msd_stepper*>> groups;
+        for (msd_stepper* s : layered) {
+            size_t gi = 0;
+            while (gi < groups.size() && !FilmGroup::same_shape(groups[gi][0]->args, s->args)) ++gi;
+            if (gi == groups.size()) groups.emplace_back();
+            groups[gi].push_back(s);
+        }
+        bool regroup = groups.size() != films.size();
+        for (size_t gi = 0; !regroup && gi < groups.size(); ++gi) regroup = groups[gi] != films[gi].mem;
+        if (regroup) {
+            drop_graph();
+            for (FilmGroup& f : films) f.release();
+            films.assign(groups.size(), FilmGroup{});
+            for (size_t gi = 0; gi < groups.size(); ++gi) films[gi].mem = groups[gi];
+        }
+        for (FilmGroup& f : films)
+            if (int rc = f.upload()) return rc;
         return b.update(r, tys, lds);
+    }
+    hipError_t evaluate(bool pred, hipStream_t st) const
+    {
+        hipError_t e = b.evaluate(pred ? R3_PRED : R3_CORR, compensated, st);
+        for (const FilmGroup& f : films)
+            if (e == hipSuccess) e = f.step(pred, st);
+        return e;
     }
 };
 
@@ -3536,11 +3611,12 @@ extern "C" int msd_ensemble_create(msd_stepper* const* members, int count, msd_e
     for (int i = 0; i < count; ++i) {
         msd_stepper* s = members[i];
         if (!s) return fail(MSD_EINVAL, "ensemble: member %d is NULL", i);
-        if (s->kind() == SK_LAYERED) return fail(MSD_EUNSUPPORTED, "ensemble: member %d uses the layered Dipole", i);
         for (int j = 0; j < i; ++j) {
             if (members[j] == s || members[j]->s == s->s) return fail(MSD_EINVAL, "ensemble: member %d repeats a SpinSystem", i);
             if (s->lr3 && members[j]->lr3 == s->lr3)
                 return fail(MSD_EINVAL, "ensemble: members %d and %d share a LongRange3D (its spectrum buffer)", j, i);
+            if (s->dip && members[j]->dip == s->dip)
+                return fail(MSD_EINVAL, "ensemble: members %d and %d share a Dipole (its spectrum buffers)", j, i);
             // one Thermal would give both replicas the same noise and advance its draw twice a step
             if (s->th && members[j]->th == s->th)
                 return fail(MSD_EINVAL, "ensemble: members %d and %d share a Thermal (its noise stream)", j, i);
@@ -3572,14 +3648,19 @@ extern "C" int msd_ensemble_heun(msd_ensemble* e, int nsteps)
     for (msd_stepper* s : e->m) {
         if (int rc = member_begin(s)) return rc;
         stale |= s->lr3 && spectrum_stale(s);
+        // a film's first column pass reads the spin spectrum of its current state
+        if (s->kind() == SK_LAYERED && spectrum_stale(s)) {
+            hipError_t err = launch_rows_fwd(s->args.dp, s->s->S, G.stream);
+            if (err != hipSuccess) return fail(MSD_EDEVICE, "ensemble: %s", hipGetErrorString(err));
+        }
     }
     if (stale) {
         hipError_t err = e->b.rows(R3_FWD, false, G.stream);
         if (err != hipSuccess) return fail(MSD_EDEVICE, "ensemble: %s", hipGetErrorString(err));
     }
     auto one_step = [&](hipStream_t st) -> hipError_t {
-        hipError_t err = e->b.evaluate(R3_PRED, e->compensated, st);
-        if (err == hipSuccess) err = e->b.evaluate(R3_CORR, e->compensated, st);
+        hipError_t err = e->evaluate(true, st);
+        if (err == hipSuccess) err = e->evaluate(false, st);
         return err;
     };
     if (e->use_graph) {

@@ -159,6 +159,11 @@ struct StepArgs {
 // argument values without a new capture
 hipError_t launch_step_pred(const StepArgs& a, hipStream_t st, const void** fn = nullptr);
 hipError_t launch_step_corr(const StepArgs& a, hipStream_t st, const void** fn = nullptr);
+// films of one shape (a0's) in one launch per pass: arguments / dipole descriptors of
+// nrep films in device memory
+hipError_t launch_step_pred_batch(const StepArgs& a0, const StepArgs* as, int nrep, hipStream_t st);
+hipError_t launch_step_corr_batch(const StepArgs& a0, const StepArgs* as, int nrep, hipStream_t st);
+hipError_t launch_cols_batch(const DipoleDev& d0, const DipoleDev* dps, int nrep, hipStream_t st);
 
 // ------------------------------------------------ 3D convolution (LongRange3D)
 // FFT grid (fx, fy, fz) equals the lattice when periodic and doubles it when

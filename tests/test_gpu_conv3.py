@@ -232,3 +232,64 @@ def test_ensemble_against_oracle():
         sim.heun(25)
         err = float(np.abs(flat(g["ss"].spins()[:3]) - flat(o["ss"].spins()[:3])).max())
         assert err <= 1e-4
+
+
+def film(dims, T, seed, g=0.05, K=0.05):
+    spec = W.FilmSpec(*dims, J=1.0, K=K, g=g, pbc=(1, 1, 0), alpha=0.1, dt=0.004, T=T, seed=seed, noise_seed=seed)
+    gpu = W.build(M, spec)
+    gpu["ss"].setSpins(*O.random_unit_spins(seed, spec.n))
+    return spec, gpu
+
+
+def test_ensemble_with_films_matches_single_replicas():
+    """Layered-Dipole films in an Ensemble: films of one shape step in one launch per
+    pass (two shapes here), beside a LongRange3D cube and a local-field member; every
+    member equals its single-replica run bit for bit."""
+    cfg = [((64, 64, 4), 0.3, 1), ((64, 64, 4), 0.0, 2), ((32, 32, 2), 0.8, 3), ((64, 64, 4), 1.1, 4)]
+    ens, solo = [], []
+    for bucket in (ens, solo):
+        for dims, T, seed in cfg:
+            spec, g = film(dims, T, seed)
+            bucket.append((g, M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"])))
+        spec, g, o, gl, ol = cube((8, 8, 8), 0.5, 9)
+        bucket.append((g, M.Heun(g["ss"], g["ex"], g["ani"], gl, g["temp"], g["llg"])))
+        spec, g, o, gl, ol = cube((10, 6, 4), 0.2, 10, g=0.0)
+        bucket.append((g, M.Heun(g["ss"], g["ex"], g["ani"], None, g["temp"], g["llg"])))
+    E = M.Ensemble([h for (_, h) in ens])
+    E.step(7)
+    E.step(5)
+    for (_, h) in solo:
+        h.step(7)
+        h.step(5)
+    for k, ((ge, _), (gs, _)) in enumerate(zip(ens, solo)):
+        assert np.array_equal(flat(ge["ss"].spins()), flat(gs["ss"].spins())), k
+        assert ge["ss"].time() == gs["ss"].time()
+    assert ens[0][0]["temp"].draw() == 12
+
+
+def test_ensemble_films_against_oracle():
+    """Two 64x64x4 films with strong dipolar coupling (g = 0.1) in one Ensemble at
+    T = 0 and T = 0.2, 20 steps against the oracle on the same noise."""
+    members, refs = [], []
+    for T, seed in ((0.0, 5), (0.2, 6)):
+        spec, g = film((64, 64, 4), T, seed, g=0.1, K=0.02)
+        members.append((g, M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"])))
+        o = W.build(O, spec)
+        o["ss"].setSpins(*O.random_unit_spins(seed, spec.n))
+        refs.append((o, O.Sim(o["ss"], o["ex"], o["ani"], o["dip"], o["temp"], o["llg"], seed=seed)))
+    E = M.Ensemble([h for (_, h) in members])
+    E.step(20)
+    for (g, _), (o, sim) in zip(members, refs):
+        sim.heun(20)
+        err = float(np.abs(flat(g["ss"].spins()[:3]) - flat(o["ss"].spins()[:3])).max())
+        print(f"ensemble film 64x64x4: 20 steps max|dm| = {err:.3e}", flush=True)
+        assert err <= 1e-5
+
+
+def test_ensemble_rejects_films_sharing_a_dipole():
+    spec, a = film((32, 32, 2), 0.0, 1)
+    spec, b = film((32, 32, 2), 0.0, 2)
+    ha = M.Heun(a["ss"], a["ex"], a["ani"], a["dip"], None, a["llg"])
+    hb = M.Heun(b["ss"], b["ex"], b["ani"], a["dip"], None, b["llg"])
+    with pytest.raises(M.MagLuaError, match="share a Dipole"):
+        M.Ensemble([ha, hb])
