@@ -162,6 +162,7 @@ _SIGS = {
     "msd_stepper_profile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_void_p, _dp, _ip]),
     "msd_stepper_set_surrogate": (C.c_int, [_vp, _vp, C.c_int, C.c_double]),
     "msd_stepper_autotune": (C.c_int, [_vp, C.c_int, _ip, _ip, _dp]),
+    "msd_stepper_rk": (C.c_int, [_vp, C.c_char_p, C.c_int]),
     "msd_stepper_graph_builds": (C.c_int, [_vp, _lp]),
     "msd_stepper_surrogate_stats": (C.c_int, [_vp, _lp, _lp, _lp, _dp, _dp]),
     "msd_surrogate_create": (C.c_int, [C.c_int, C.c_int, C.c_int, _pp]),

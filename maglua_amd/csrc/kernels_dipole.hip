@@ -505,6 +505,110 @@ __global__ void __launch_bounds__(TPB) k_step_corr(StepArgs a)
     step_corr_body<NX, TY, TPB, DIP, COMP>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
 
+// ------------------------------------------------------ Runge-Kutta stages
+// Stage i of make_rk_step_function (common/scripts/RungeKutta.lua:185-214) on one row
+// tile: the inverse x transform of the stage's dipole spectrum, the total field at
+// the stage spins X (exchange, anisotropy, dipole, applied fields), then the row
+// out = N(...N(N(S + f0 dt F(S)) + f1 dt F(k_1))... + f_i dt F(X)), each term an
+// LLGCartesian sub-step (llgcartesian.cpp:90-278: a zero coefficient copies, every
+// sub-step renormalises to |S|), the thermal kick at the last stage, and the forward
+// x transform of the result for the next stage's column pass.
+template <int NX, int TY, int TPB, bool DIP>
+__global__ void __launch_bounds__(TPB) k_rk_stage(StepArgs a, RKStage r)
+{
+    extern __shared__ float2 lds[];
+    const DipoleDev& dp = a.dp;
+    const int nx = NX ? NX : a.d.nx, ty = TY ? TY : a.ty, lines = row_lines(ty);
+    const int ny = a.d.ny;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int yb = tile / a.d.nz, z = tile - yb * a.d.nz, y0 = yb * ty;
+    float2* A = lds;
+    float2* B = lds + (size_t)lines * nx;
+    float2* R = A;
+    const float2* tw = nullptr;
+    __shared__ int s_start[EX_LDS_CLS];
+    __shared__ ExEntry s_ent[EX_LDS_ENT];
+    const ExchangeDev exl = stage_exchange(a.ex, s_start, s_ent);
+    if constexpr (DIP) {
+        tw = row_twiddles<NX>(dp, A + (size_t)lines * nx);
+        rows_pack_spectrum<TPB, NX, TY>(dp.Hh, nx, ny, a.d.nz, ty, z, y0, A);
+    }
+    __syncthreads();
+    if constexpr (DIP) R = row_fft<NX, TY, TPB, 1>(A, B, dp, tw);
+    const uint64_t draw = (r.last && a.th.on) ? (uint64_t)*a.draw : 0ull;
+    const uint32_t n = (uint32_t)a.d.n;
+    const float* __restrict__ msa = a.S + 3 * (size_t)n;
+    const float dt = a.dt;
+    const float3 zero = make_float3(0.f, 0.f, 0.f);
+    for (int li = threadIdx.x; li < ty * nx; li += TPB) {
+        const int rr = li / nx, x = li - rr * nx;
+        const uint32_t i = (uint32_t)(x + nx * (y0 + rr + ny * z));
+        const float ms = ldg(msa, i);
+        const float3 X = make_float3(ldg(r.X, i), ldg(r.X + n, i), ldg(r.X + 2 * (size_t)n, i));
+        ExGather g;
+        exchange_gather(exl, n, i, r.X, msa, g);
+        float3 H = exchange_finish(exl, n, i, r.X, msa, ms, g);
+        const float3 ha = aniso_field(a.an, (int)i, (int)n, X, ms);
+        H.x += ha.x; H.y += ha.y; H.z += ha.z;
+        if constexpr (DIP) {
+            H.x += rowval(R, rr, x, nx); H.y += rowval(R, ty + rr, x, nx); H.z += rowval(R, 2 * ty + rr, x, nx);
+        }
+        if (a.uni.ops) { const float3 hu = uniaxial_field(a.uni, (int)i, X, ms); H.x += hu.x; H.y += hu.y; H.z += hu.z; }
+        if (a.has_app) {
+            H.x += a.Bapp.x; H.y += a.Bapp.y; H.z += a.Bapp.z;
+            if (a.Happ) { H.x += ldg(a.Happ, i); H.y += ldg(a.Happ + n, i); H.z += ldg(a.Happ + 2 * (size_t)n, i); }
+        }
+        if (r.Hout) { stg(r.Hout, i, H.x); stg(r.Hout + n, i, H.y); stg(r.Hout + 2 * (size_t)n, i, H.z); }
+        const float al = a.lp.site_alpha ? ldg(a.lp.site_alpha, i) : a.lp.alpha;
+        const float ga = a.lp.site_gamma ? ldg(a.lp.site_gamma, i) : a.lp.gamma;
+        float3 d = make_float3(ldg(a.S, i), ldg(a.S + n, i), ldg(a.S + 2 * (size_t)n, i));
+        for (int j = 0; j < r.terms; ++j) {
+            const float c = r.f[j];
+            if (j > 0 && c == 0.0f) continue;        // the script skips zero coefficients after the first
+            if (c != 0.0f && ms > 0.0f) {
+                const bool cur = j == r.terms - 1;
+                const float3 M = cur ? X : make_float3(ldg(r.K[j], i), ldg(r.K[j] + n, i), ldg(r.K[j] + 2 * (size_t)n, i));
+                const float3 Hj = cur ? H : make_float3(ldg(r.H[j], i), ldg(r.H[j] + n, i), ldg(r.H[j] + 2 * (size_t)n, i));
+                const float3 dM = a.lp.thermal_first ? llg_rhs(M, Hj, Hj, al, ga, ms, a.lp.ps, a.lp.ds)
+                                                     : llg_rhs(M, Hj, Hj, al, ga, ms, 1.0f, 1.0f);
+                const float h = c * dt;
+                d = make_float3(d.x + h * dM.x, d.y + h * dM.y, d.z + h * dM.z);
+            }
+            if (a.lp.renorm) d = renorm(d, ms);
+        }
+        if (r.last && a.th.on) {
+            const float sg = thermal_sigma(a.th, a.lp, (int)i, ms, dt);
+            if (ms > 0.0f) {
+                float3 Ht = zero;
+                if (sg != 0.0f) {
+                    const float3 gn = thermal_normals(a.th.seed, draw, (int)i);
+                    Ht = make_float3(sg * gn.x, sg * gn.y, sg * gn.z);
+                }
+                const float3 dM = a.lp.thermal_first ? llg_rhs(d, Ht, zero, al, ga, ms, a.lp.ps, a.lp.ds)
+                                                     : llg_rhs(d, Ht, Ht, al, ga, ms, 1.0f, 1.0f);
+                d = make_float3(d.x + dt * dM.x, d.y + dt * dM.y, d.z + dt * dM.z);
+            }
+            if (a.lp.renorm) d = renorm(d, ms);
+        }
+        stg(r.out, i, d.x); stg(r.out + n, i, d.y); stg(r.out + 2 * (size_t)n, i, d.z);
+        if constexpr (DIP) {
+            rowval(R, rr, x, nx) = d.x;
+            rowval(R, ty + rr, x, nx) = d.y;
+            rowval(R, 2 * ty + rr, x, nx) = d.z;
+        }
+    }
+    if constexpr (DIP) {
+        __syncthreads();
+        const float2* F = row_fft<NX, TY, TPB, -1>(R, R == A ? B : A, dp, tw);
+        rows_unpack_spectrum<TPB>(F, nx, ny, a.d.nz, ty, z, y0, dp.Mh);
+    }
+}
+
+__global__ void k_draw_advance(unsigned long long* draw)
+{
+    if (threadIdx.x == 0) *draw += 1ull;
+}
+
 // films of one shape in one launch: film r's arguments at as[r], its `per` tiles at
 // [r per, (r+1) per) of the remapped workgroup index
 template <int NX, int TY, int TPB, bool COMP>
@@ -697,6 +801,31 @@ hipError_t launch_step_corr_batch(const StepArgs& a0, const StepArgs* as, int nr
     return launch_step_batch<false>(a0, as, nrep, st);
 }
 
+hipError_t launch_rk_stage(const StepArgs& a, const RKStage& r, hipStream_t st)
+{
+    const int g = row_tiles(a.d.ny, a.d.nz, a.ty);
+    if (!a.has_dip) {
+        k_rk_stage<0, 0, ROW_TPB, false><<<g, ROW_TPB, 0, st>>>(a, r);
+        return hipGetLastError();
+    }
+    const size_t lds = a.dp.lds_rows;
+#define MSD_RK(NXV, TYV, TPBV)                                                      \
+    if (a.d.nx == NXV && a.ty == TYV && a.dp.tpb == TPBV) {                         \
+        k_rk_stage<NXV, TYV, TPBV, true><<<g, TPBV, lds, st>>>(a, r);               \
+        return hipGetLastError();                                                   \
+    }
+    MSD_ROW_SHAPES(MSD_RK)
+#undef MSD_RK
+    k_rk_stage<0, 0, ROW_TPB, true><<<g, ROW_TPB, lds, st>>>(a, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_draw_advance(unsigned long long* draw, hipStream_t st)
+{
+    k_draw_advance<<<1, 64, 0, st>>>(draw);
+    return hipGetLastError();
+}
+
 template <int NZ>
 static void launch_cols_batch_nz(const DipoleDev& d0, const DipoleDev* dps, int nrep, hipStream_t st)
 {
@@ -764,7 +893,8 @@ hipError_t configure_lds(size_t max_bytes)
     set_lds((const void*)k_step_pred_batch<NXV, TYV, TPBV, true>, v, first);      \
     set_lds((const void*)k_step_pred_batch<NXV, TYV, TPBV, false>, v, first);     \
     set_lds((const void*)k_step_corr_batch<NXV, TYV, TPBV, true>, v, first);      \
-    set_lds((const void*)k_step_corr_batch<NXV, TYV, TPBV, false>, v, first);
+    set_lds((const void*)k_step_corr_batch<NXV, TYV, TPBV, false>, v, first);     \
+    set_lds((const void*)k_rk_stage<NXV, TYV, TPBV, true>, v, first);
     MSD_ROW_SHAPES(MSD_RA)
     MSD_RA(0, 0, ROW_TPB)
 #undef MSD_RA

@@ -2761,6 +2761,13 @@ struct msd_stepper {
     std::vector<std::pair<const void*, int>> cap_log;   // step kernels launched during a capture
     bool capturing = false;
     long graph_builds = 0;               // graphs captured and instantiated (diagnostics)
+    // Runge-Kutta form (msd_stepper_rk): stage spins k_1.., stage fields, its graph
+    std::vector<float*> rk_k, rk_h;
+    float* rk_tmp = nullptr;
+    hipGraph_t rk_graph = nullptr;
+    hipGraphExec_t rk_exec = nullptr;
+    std::string rk_kind;
+    StepArgs rk_args{};
     Batch3 b3;
     uint64_t spec_version = 0;
     uint64_t dip_epoch = 0, lr3_epoch = 0;
@@ -2786,6 +2793,9 @@ struct msd_stepper {
         (void)hipStreamSynchronize(G.stream);
         drop_graph();
         dfree(P); dfree(C); dfree(Clo); dfree(draw);
+        for (float*& b : rk_k) dfree(b);
+        for (float*& b : rk_h) dfree(b);
+        dfree(rk_tmp);
         dfree(Href); dfree(Mref); dfree(Hext); dfree(Hx); dfree(Ssnap); dfree(Happ);
     }
     void drop_graph()
@@ -2796,6 +2806,9 @@ struct msd_stepper {
         gnodes[0].clear();
         gnodes_ok[0] = false;
         drop_learned_graphs();
+        if (rk_exec) (void)hipGraphExecDestroy(rk_exec);
+        if (rk_graph) (void)hipGraphDestroy(rk_graph);
+        rk_exec = nullptr; rk_graph = nullptr;
     }
     void drop_learned_graphs()
     {
@@ -3228,6 +3241,122 @@ extern "C" int msd_stepper_heun(msd_stepper* st, int nsteps)
     if (int rc = stepper_begin(st)) return rc;
     if (int rc = run_steps(st, nsteps, false)) return rc;
     stepper_end(st, nsteps);
+    return MSD_OK;
+}
+
+// Butcher tables of common/scripts/RungeKutta.lua:57-160 (published coefficients; the
+// rows, c column omitted, define the integrator a script selects by name)
+static bool butcher_rows(const std::string& k, std::vector<std::vector<double>>& t)
+{
+    if (k == "RK1") t = {{1}};
+    else if (k == "RK2") t = {{0.5}, {0, 1}};
+    else if (k == "K3") t = {{0.5}, {-1, 2}, {1.0 / 6, 2.0 / 3, 1.0 / 6}};
+    else if (k == "RK4") t = {{0.5}, {0, 0.5}, {0, 0, 1}, {1.0 / 6, 1.0 / 3, 1.0 / 3, 1.0 / 6}};
+    else if (k == "RK3") t = {{1.0 / 3}, {0, 2.0 / 3}, {0.25, 0, 0.75}};
+    else if (k == "RK4_38") t = {{1.0 / 3}, {-1.0 / 3, 1}, {1, -1, 1}, {0.125, 0.375, 0.375, 0.125}};
+    else if (k == "RK6")
+        t = {{1.0 / 3}, {0, 2.0 / 3}, {1.0 / 12, 4.0 / 12, -1.0 / 12}, {-1.0 / 16, 18.0 / 16, -3.0 / 16, -6.0 / 16},
+             {0, 9.0 / 8, -3.0 / 8, -6.0 / 8, 4.0 / 8}, {9.0 / 44, -36.0 / 44, 63.0 / 44, 72.0 / 44, 0, -64.0 / 44},
+             {11.0 / 120, 0, 81.0 / 120, 81.0 / 120, -32.0 / 120, -32.0 / 120, 11.0 / 120}};
+    else if (k == "BS3") t = {{0.5}, {0, 0.75}, {2.0 / 9, 1.0 / 3, 4.0 / 9}, {7.0 / 24, 0.25, 1.0 / 3, 0.125}};
+    else return false;
+    return true;
+}
+
+// One step of make_rk_step_function (RungeKutta.lua:185-214) with the fused stage
+// kernels: per stage, the layered column pass of the stage spins (with a Dipole)
+// and one row kernel (field, Butcher row, thermal kick at the last).
+static int rk_launch_step(msd_stepper* st, const std::vector<std::vector<double>>& bt, hipStream_t s)
+{
+    const int ns = (int)bt.size();
+    const size_t n = st->s->d.n;
+    hipError_t e = hipSuccess;
+    if (st->th) e = launch_draw_advance(st->draw, s);
+    for (int i = 0; i < ns && e == hipSuccess; ++i) {
+        if (st->args.has_dip) e = launch_cols(st->args.dp, s);
+        if (e != hipSuccess) break;
+        RKStage r{};
+        r.X = i == 0 ? st->s->S : st->rk_k[i];
+        r.Hout = st->rk_h[i];
+        // the last row writes S, but one stage that also reads S's neighbours (RK1)
+        // writes a copy first
+        r.out = i + 1 < ns ? st->rk_k[i + 1] : (ns == 1 ? st->rk_tmp : st->s->S);
+        for (int j = 0; j < i; ++j) {
+            r.K[j] = j == 0 ? st->s->S : st->rk_k[j];
+            r.H[j] = st->rk_h[j];
+        }
+        r.terms = i + 1;
+        for (int j = 0; j <= i; ++j) r.f[j] = (float)bt[i][j];
+        r.last = i + 1 == ns;
+        e = launch_rk_stage(st->args, r, s);
+    }
+    if (e == hipSuccess && ns == 1) e = hipMemcpyAsync(st->s->S, st->rk_tmp, 3 * n * sizeof(float), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "RK step: %s", hipGetErrorString(e));
+    return MSD_OK;
+}
+
+extern "C" int msd_stepper_rk(msd_stepper* st, const char* kind, int nsteps)
+{
+    NONNULL(st); NONNULL(kind);
+    std::vector<std::vector<double>> bt;
+    if (!butcher_rows(kind, bt))
+        return fail(MSD_EINVAL, "RK type must be one of \"RK1\", \"RK2\", \"K3\", \"RK3\", \"RK4\", \"RK4_38\", \"RK6\", \"BS3\"");
+    if (nsteps < 0) return fail(MSD_EINVAL, "rk: nsteps %d", nsteps);
+    if (st->lr3) return fail(MSD_EUNSUPPORTED, "rk: the fused Runge-Kutta stages cover the local and layered forms");
+    if (st->sur) return fail(MSD_EUNSUPPORTED, "rk: the learned dipole field runs with the Heun step");
+    if (nsteps == 0) return MSD_OK;
+    const int ns = (int)bt.size();
+    const size_t n3 = (size_t)3 * st->s->d.n;
+    while ((int)st->rk_k.size() < ns) {
+        float* b = nullptr;
+        if (int rc = dalloc(&b, n3)) return rc;
+        st->rk_k.push_back(b);
+    }
+    while ((int)st->rk_h.size() < ns) {
+        float* b = nullptr;
+        if (int rc = dalloc(&b, n3)) return rc;
+        st->rk_h.push_back(b);
+    }
+    if (!st->rk_tmp)
+        if (int rc = dalloc(&st->rk_tmp, n3)) return rc;
+    if (int rc = stepper_begin(st)) return rc;
+    if (st->use_graph) {
+        if (!st->rk_exec || st->rk_kind != kind || std::memcmp(&st->rk_args, &st->args, sizeof(StepArgs)) != 0) {
+            if (st->rk_exec) (void)hipGraphExecDestroy(st->rk_exec);
+            if (st->rk_graph) (void)hipGraphDestroy(st->rk_graph);
+            st->rk_exec = nullptr; st->rk_graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(G.stream, hipStreamCaptureModeThreadLocal));
+            int rc = rk_launch_step(st, bt, G.stream);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(G.stream, &g);
+            if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+            if (e != hipSuccess) return fail(MSD_EDEVICE, "RK graph capture: %s", hipGetErrorString(e));
+            st->rk_graph = g;
+            HIPCHK(hipGraphInstantiate(&st->rk_exec, g, nullptr, nullptr, 0));
+            st->graph_builds++;
+            st->rk_kind = kind;
+            st->rk_args = st->args;
+        }
+        for (int i = 0; i < nsteps; ++i) HIPCHK(hipGraphLaunch(st->rk_exec, G.stream));
+    } else {
+        for (int i = 0; i < nsteps; ++i)
+            if (int rc = rk_launch_step(st, bt, G.stream)) return rc;
+    }
+    // time as the script's sub-steps advance it: the last row's terms from S, each
+    // nonzero one adding its coefficient times dt (a zero first term copies, no time)
+    const double dt = st->s->dt;
+    for (int k = 0; k < nsteps; ++k) {
+        const std::vector<double>& f = bt.back();
+        if (f[0] * dt != 0.0) st->s->time = st->s->time + f[0] * dt;
+        for (size_t j = 1; j < f.size(); ++j)
+            if (f[j] != 0.0) st->s->time = st->s->time + f[j] * dt;
+    }
+    // the state is plain FP32 words now: the compensated low words no longer apply
+    st->s->lo_valid = false;
+    if (st->th) st->th->draw += (uint64_t)nsteps;
+    st->s->version++;
+    st->spec_version = st->s->version;
+    if (st->dip) st->dip_epoch = ++st->dip->epoch;
     return MSD_OK;
 }
 

@@ -159,6 +159,22 @@ struct StepArgs {
 // argument values without a new capture
 hipError_t launch_step_pred(const StepArgs& a, hipStream_t st, const void** fn = nullptr);
 hipError_t launch_step_corr(const StepArgs& a, hipStream_t st, const void** fn = nullptr);
+// One stage of a Runge-Kutta step (RungeKutta.lua's istep after calcField): the total
+// field at the stage spins X, then the Butcher row as chained renormalised Euler
+// sub-steps from S over the stages j <= i, then (last stage) the thermal kick.
+struct RKStage {
+    const float* X;         // stage spins k_i [3][n] (S at stage 0)
+    float* Hout;            // the stage's total field [3][n], kept for later rows, or nullptr
+    float* out;             // the row's result [3][n]: k_(i+1), or the new S at the last stage
+    const float* K[8];      // spins of k_j, j < i (K[0] = S)
+    const float* H[8];      // total fields of k_j, j < i
+    float f[8];             // the Butcher row: coefficients of j = 0 .. terms-1
+    int terms;              // i + 1
+    int last;               // 1: the last stage (thermal kick after the row)
+};
+hipError_t launch_rk_stage(const StepArgs& a, const RKStage& r, hipStream_t st);
+// *draw += 1: the thermal draw of a Runge-Kutta step, read by its last stage
+hipError_t launch_draw_advance(unsigned long long* draw, hipStream_t st);
 // films of one shape (a0's) in one launch per pass: arguments / dipole descriptors of
 // nrep films in device memory
 hipError_t launch_step_pred_batch(const StepArgs& a0, const StepArgs* as, int nrep, hipStream_t st);

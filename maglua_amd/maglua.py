@@ -1085,6 +1085,14 @@ class Heun(_Handle):
         call("msd_stepper_autotune", self._h, int(steps), C.byref(ty), C.byref(tpb), C.byref(ms))
         return {"rows_per_tile": ty.value, "threads": tpb.value, "ms_per_step": ms.value}
 
+    def rk(self, kind, n=1):
+        """n steps of make_rk_step_function(ss, kind, ...) with the fused stage kernels
+        (Butcher table `kind` of RungeKutta.lua; plain FP32 state)."""
+        if kind not in BUTCHER:
+            raise MagLuaError(-1, 'RK type must be one of "' + '", "'.join(BUTCHER) + '"')
+        call("msd_stepper_rk", self._h, kind.encode(), int(n))
+        return True
+
     def graphBuilds(self):
         """Step graphs captured so far (values that change between steps reuse the graph)."""
         n = C.c_long()
