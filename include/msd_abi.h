@@ -249,8 +249,45 @@ int msd_dipole_set_unit_cell(msd_dipole* d, const double ABC[9]);
  * oz in [-(nz-1), nz-1] (LongRange::getAB/setAB, spinoperationlongrange.cpp:153-210). */
 int msd_dipole_get_matrix(msd_dipole* d, int which, int ox, int oy, int oz, double* v);
 int msd_dipole_set_matrix(msd_dipole* d, int which, int ox, int oy, int oz, double v);
-/* dip:apply(ss): LongRange::apply, spinoperationlongrange.cpp:354-430, slot "Dipole" */
+/* dip:apply(ss): LongRange::apply, spinoperationlongrange.cpp:354-430, slot "Dipole".
+ * Also LongRange2D:apply (cpu/longrange2d/spinoperationlongrange2d.cpp:448-531) for an
+ * object made by msd_lr2d_create, into its own slot. */
 int msd_dipole_apply(msd_dipole* d, msd_system* s);
+
+/* ------------------------------------------------------------- LongRange2D */
+/* LongRange2D.new / Dipole2D.new (cpu/longrange2d/spinoperationlongrange2d.cpp:26-48,
+ * cpu/dipole2d/spinoperationdipole2d.cpp:19-40): a layered convolution with a full real
+ * tensor T[a][b][dest][src](x, y) for every ordered pair of layers (not only a function of
+ * the layer offset), per-layer source-moment and destination-field scales (default 1)
+ * and the operator scale (msd_dipole_set_scale).  The field of layer d is
+ *   H_a[d] = scale dfs[d] sum_s sms[s] sum_b T_ab[s][d] (*) m_b[s],
+ * (*) the periodic in-plane convolution: the reference's apply reads the pair stored under
+ * [s][d] for destination d and source s (spinoperationlongrange2d.cpp:463-526), and so does
+ * this one.  The object is an msd_dipole: msd_dipole_apply, msd_dipole_set_scale,
+ * msd_dipole_destroy, the step drivers and the surrogate accept it; the lattice-sum entry
+ * points (strength, truncation, unit cell, get/set_matrix, encode) refuse it.  slot names
+ * the field slot ("LongRange2D" if null or empty; Dipole2D uses "Dipole2D").  Same size
+ * limits as msd_dipole_create. */
+int msd_lr2d_create(int nx, int ny, int nz, const char* slot, msd_dipole** out);
+/* lr:setTensorArray(dest, src, AB, array) / lr:tensorArray(dest, src, AB)
+ * (spinoperationlongrange2d.cpp:168-224,585-602): dest, src 0-based layers, ab 0..8 =
+ * XX XY XZ YX YY YZ ZX ZY ZZ, a = nx*ny doubles, x fastest.  A set takes effect at the
+ * next apply or step (the reference's setCompileRequired). */
+int msd_lr2d_set_tensor(msd_dipole* d, int dest, int src, int ab, const double* a);
+int msd_lr2d_get_tensor(msd_dipole* d, int dest, int src, int ab, double* a);
+/* setSourceMomentScale / setDestinationFieldScale for all nz layers at once (:533-568);
+ * a null array leaves that set unchanged. */
+int msd_lr2d_set_layer_scales(msd_dipole* d, const double* src_moment, const double* dest_field);
+int msd_lr2d_get_layer_scales(msd_dipole* d, double* src_moment, double* dest_field);
+/* Dipole2D makeData (cpu/dipole2d/dip2d_luafuncs.lua:537-668, point tensor
+ * cpu/dipole2d/demag.cpp:3-62): fills every T[dest][src] with the truncated box sum of
+ * the dipole tensor.  ABC = [nz][3][3] cell vectors (A, B, C) per layer, offset = [nz][3],
+ * trunc = {radial, x, y, z} (the radial value is not used, as in the reference).
+ * physical = 0 reproduces the reference's tensors: it evaluates source and destination
+ * at the same in-plane cell, so only the layers' vertical coupling enters, repeated over
+ * the box.  physical = 1 places the destination at the origin (the truncated in-plane
+ * dipole sum).  The file cache of the reference (GAB_*.lua) is not kept. */
+int msd_lr2d_build_dipole2d(msd_dipole* d, const double* ABC, const double* offset, const int trunc[4], int physical);
 
 /* ------------------------------------------------------------- LongRange3D */
 /* LongRange3D.new: 3D FFT convolution, cpu/longrange3d/spinoperationlongrange3d.cpp:331-432.

@@ -133,6 +133,12 @@ _SIGS = {
     "msd_dipole_get_matrix": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
     "msd_dipole_set_matrix": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]),
     "msd_dipole_apply": (C.c_int, [_vp, _vp]),
+    "msd_lr2d_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_char_p, _pp]),
+    "msd_lr2d_set_tensor": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp]),
+    "msd_lr2d_get_tensor": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp]),
+    "msd_lr2d_set_layer_scales": (C.c_int, [_vp, _dp, _dp]),
+    "msd_lr2d_get_layer_scales": (C.c_int, [_vp, _dp, _dp]),
+    "msd_lr2d_build_dipole2d": (C.c_int, [_vp, _dp, _dp, _ip, C.c_int]),
     "msd_longrange3d_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _pp]),
     "msd_longrange3d_destroy": (C.c_int, [_vp]),
     "msd_longrange3d_set_strength": (C.c_int, [_vp, C.c_double]),

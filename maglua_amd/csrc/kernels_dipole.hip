@@ -362,6 +362,109 @@ __global__ void __launch_bounds__(col_tpb(NY)) k_cols_batch(const DipoleDev* __r
     cols_body<NY, NZ>(dps[r], w - r * per);
 }
 
+// ------------------------------------------------------------ per-layer-pair column pass
+// LongRange2D::apply (cpu/longrange2d/spinoperationlongrange2d.cpp:448-531): a full complex
+// tensor for every ordered layer pair and every component,
+//   H^_a[d] = dsc[d] sum_s ssc[s] sum_b T^_ab[s][d] M^_b[s],
+// where the reference reads the pair stored under the labels [s][d] (its tensorArray(s, d))
+// for destination layer d and source layer s; the words are stored [9][nz][nz][nxh][ny]
+// (XX XY XZ YX YY YZ ZX ZY ZZ, first label, second label) with 1/(nx ny) folded in.
+// ssc / dsc are the per-layer source-moment and destination-field scales, the operator's
+// own scale folded into dsc.  Workgroup (kx, a) transforms all 3 nz columns of kx,
+// forms output component a for every layer and transforms those nz columns back.
+template <int NY, int NZ>
+__device__ __forceinline__ void pair_cols_body(const DipoleDev& dp, int wg)
+{
+    constexpr int COL_TPB = col_tpb(NY);
+    extern __shared__ float2 lds[];
+    const int kx = wg / 3;
+    const int oa = wg - 3 * kx;
+    const int ny = NY ? NY : dp.ny, nxh = dp.nxh;
+    constexpr int lines = 3 * NZ;
+    float2* A = lds;
+    float2* B = lds + (size_t)lines * ny;
+    const float2* tw = dp.twy;
+    if constexpr (NY > 0) {
+        float2* twl = A + (size_t)lines * NY;   // in place: the twiddles follow the lines
+        for (int k = threadIdx.x; k < NY; k += COL_TPB) twl[k] = dp.twy[k];
+        batched_copy<lines * NY, COL_TPB, 16>(
+            [&](int idx) { const int l = idx / NY, y = idx - l * NY; return dp.Mh[((size_t)l * nxh + kx) * NY + y]; },
+            [&](int idx, float2 v) { A[idx] = v; });
+        tw = twl;
+    } else {
+        for (int idx = threadIdx.x; idx < lines * ny; idx += COL_TPB) {
+            const int l = idx / ny, y = idx - l * ny;
+            A[idx] = dp.Mh[((size_t)l * nxh + kx) * ny + y];
+        }
+    }
+    __syncthreads();
+    float2* R;
+    if constexpr (NY > 0) {
+        fft_ip<NY, lines, COL_TPB, -1>(A, tw);
+        R = A;
+    }
+    else R = lds_fft<-1>(A, B, lines, dp.py, tw);
+    float ssc[NZ], dsc[NZ];
+#pragma unroll
+    for (int l = 0; l < NZ; ++l) {
+        ssc[l] = dp.lsc[l];
+        dsc[l] = dp.lsc[MSD_MAX_LAYERS + l];
+    }
+    const size_t plane = (size_t)nxh * ny;
+    for (int ky = threadIdx.x; ky < ny; ky += COL_TPB) {
+        const float2* T = dp.Tp + (size_t)kx * ny + ky;
+        float2 acc[NZ];
+#pragma unroll
+        for (int d = 0; d < NZ; ++d) acc[d] = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int s = 0; s < NZ; ++s) {
+            float2 m[3];
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                const float2 v = R[(b * NZ + s) * ny + ky];
+                m[b] = make_float2(ssc[s] * v.x, ssc[s] * v.y);
+            }
+#pragma unroll
+            for (int d = 0; d < NZ; ++d)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    const float2 t = T[((size_t)((oa * 3 + b) * NZ + s) * NZ + d) * plane];
+                    acc[d].x += t.x * m[b].x - t.y * m[b].y;
+                    acc[d].y += t.x * m[b].y + t.y * m[b].x;
+                }
+        }
+#pragma unroll
+        for (int d = 0; d < NZ; ++d) R[(oa * NZ + d) * ny + ky] = make_float2(dsc[d] * acc[d].x, dsc[d] * acc[d].y);
+    }
+    __syncthreads();
+    float2* Rs = R + (size_t)oa * NZ * ny;
+    float2* Os = (R == A ? B : A) + (size_t)oa * NZ * ny;
+    float2* O;
+    if constexpr (NY > 0) {
+        fft_ip<NY, NZ, COL_TPB, 1>(Rs, tw);
+        O = Rs;
+    }
+    else O = lds_fft<1>(Rs, Os, NZ, dp.py, tw);
+    for (int idx = threadIdx.x; idx < NZ * ny; idx += COL_TPB) {
+        const int l = idx / ny, y = idx - l * ny;
+        dp.Hh[((size_t)(oa * NZ + l) * nxh + kx) * ny + y] = O[idx];
+    }
+}
+
+template <int NY, int NZ>
+__global__ void __launch_bounds__(col_tpb(NY)) k_cols_pair(DipoleDev dp)
+{
+    pair_cols_body<NY, NZ>(dp, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+// films of one shape with per-layer-pair tensors in one launch (layout as k_cols_batch)
+template <int NY, int NZ>
+__global__ void __launch_bounds__(col_tpb(NY)) k_cols_pair_batch(const DipoleDev* __restrict__ dps, int per)
+{
+    const int w = xcd_remap(blockIdx.x, gridDim.x), r = w / per;
+    pair_cols_body<NY, NZ>(dps[r], w - r * per);
+}
+
 // ------------------------------------------------------------ fused Heun
 // Site loop of the fused kernels.  Compile-time shapes give each thread SPT
 // sites, software-pipelined: site 0's loads are issued before the inverse x
@@ -704,8 +807,38 @@ static void launch_cols_nz(const DipoleDev& dp, hipStream_t st)
     k_cols<0, NZ><<<3 * dp.nxh, col_tpb(0), dp.lds_cols, st>>>(dp);
 }
 
+template <int NZ>
+static void launch_cols_pair_nz(const DipoleDev& dp, hipStream_t st)
+{
+#define MSD_COL(NYV)                                                                \
+    if (dp.ny == NYV) {                                                             \
+        k_cols_pair<NYV, NZ><<<3 * dp.nxh, col_tpb(NYV), dp.lds_cols, st>>>(dp);   \
+        return;                                                                     \
+    }
+    MSD_COL_LENGTHS(MSD_COL)
+#undef MSD_COL
+    k_cols_pair<0, NZ><<<3 * dp.nxh, col_tpb(0), dp.lds_cols, st>>>(dp);
+}
+
+static hipError_t launch_cols_pair(const DipoleDev& dp, hipStream_t st)
+{
+    switch (dp.nz) {
+        case 1: launch_cols_pair_nz<1>(dp, st); break;
+        case 2: launch_cols_pair_nz<2>(dp, st); break;
+        case 3: launch_cols_pair_nz<3>(dp, st); break;
+        case 4: launch_cols_pair_nz<4>(dp, st); break;
+        case 5: launch_cols_pair_nz<5>(dp, st); break;
+        case 6: launch_cols_pair_nz<6>(dp, st); break;
+        case 7: launch_cols_pair_nz<7>(dp, st); break;
+        case 8: launch_cols_pair_nz<8>(dp, st); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_cols(const DipoleDev& dp, hipStream_t st)
 {
+    if (dp.Tp) return launch_cols_pair(dp, st);
     switch (dp.nz) {
         case 1: launch_cols_nz<1>(dp, st); break;
         case 2: launch_cols_nz<2>(dp, st); break;
@@ -826,18 +959,21 @@ hipError_t launch_draw_advance(unsigned long long* draw, hipStream_t st)
     return hipGetLastError();
 }
 
+// every member of a batch has the form of d0 (lattice-sum words or per-pair tensors)
 template <int NZ>
 static void launch_cols_batch_nz(const DipoleDev& d0, const DipoleDev* dps, int nrep, hipStream_t st)
 {
     const int per = 3 * d0.nxh;
 #define MSD_COLB(NYV)                                                               \
     if (d0.ny == NYV) {                                                             \
-        k_cols_batch<NYV, NZ><<<per * nrep, col_tpb(NYV), d0.lds_cols, st>>>(dps, per); \
+        if (d0.Tp) k_cols_pair_batch<NYV, NZ><<<per * nrep, col_tpb(NYV), d0.lds_cols, st>>>(dps, per); \
+        else k_cols_batch<NYV, NZ><<<per * nrep, col_tpb(NYV), d0.lds_cols, st>>>(dps, per); \
         return;                                                                     \
     }
     MSD_COL_LENGTHS(MSD_COLB)
 #undef MSD_COLB
-    k_cols_batch<0, NZ><<<per * nrep, col_tpb(0), d0.lds_cols, st>>>(dps, per);
+    if (d0.Tp) k_cols_pair_batch<0, NZ><<<per * nrep, col_tpb(0), d0.lds_cols, st>>>(dps, per);
+    else k_cols_batch<0, NZ><<<per * nrep, col_tpb(0), d0.lds_cols, st>>>(dps, per);
 }
 
 hipError_t launch_cols_batch(const DipoleDev& d0, const DipoleDev* dps, int nrep, hipStream_t st)
@@ -877,6 +1013,11 @@ static void cols_attr(int v, hipError_t& first)
 #undef MSD_CA
     set_lds((const void*)k_cols<0, NZ>, v, first);
     set_lds((const void*)k_cols_batch<0, NZ>, v, first);
+#define MSD_CP(NYV) set_lds((const void*)k_cols_pair<NYV, NZ>, v, first); set_lds((const void*)k_cols_pair_batch<NYV, NZ>, v, first);
+    MSD_COL_LENGTHS(MSD_CP)
+#undef MSD_CP
+    set_lds((const void*)k_cols_pair<0, NZ>, v, first);
+    set_lds((const void*)k_cols_pair_batch<0, NZ>, v, first);
 }
 
 hipError_t configure_lds(size_t max_bytes)

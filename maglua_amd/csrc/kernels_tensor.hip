@@ -134,6 +134,63 @@ hipError_t launch_tensor_spectrum(int nx, int ny, int nz, const double* T, doubl
     return hipGetLastError();
 }
 
+// Per-layer-pair tensors (LongRange2D::compile, cpu/longrange2d/spinoperationlongrange2d.cpp:396-446):
+// every nx x ny plane of real FP64 values gets its 2D forward transform (sign -1, unnormalised),
+// prescaled by mult = 1/(nx ny).  Only the R2C half kx <= nx/2 is kept: the fields are real.
+__global__ void k_plane_dftx(int planes, int nx, int ny, const double* __restrict__ T, double2* __restrict__ U,
+                             const double2* __restrict__ tw)
+{
+    const int nxh = nx / 2 + 1;
+    const size_t total = (size_t)planes * ny * nxh;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(idx % nxh);
+        const size_t py = idx / nxh;                 // plane * ny + y
+        const double* row = T + py * nx;
+        double re = 0, im = 0;
+        int m = 0;
+        for (int x = 0; x < nx; ++x) {
+            const double2 w = tw[m];
+            re += row[x] * w.x;
+            im += row[x] * w.y;
+            m += kx; if (m >= nx) m -= nx;
+        }
+        U[idx] = make_double2(re, im);
+    }
+}
+
+__global__ void k_plane_dfty(int planes, int nx, int ny, const double2* __restrict__ U, float2* __restrict__ out,
+                             double mult, const double2* __restrict__ tw)
+{
+    const int nxh = nx / 2 + 1;
+    const size_t total = (size_t)planes * nxh * ny;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int ky = (int)(idx % ny);
+        const int kx = (int)((idx / ny) % nxh);
+        const size_t p = idx / ((size_t)ny * nxh);
+        const double2* col = U + p * ny * nxh + kx;
+        double re = 0, im = 0;
+        int m = 0;
+        for (int y = 0; y < ny; ++y) {
+            const double2 u = col[(size_t)y * nxh];
+            const double2 w = tw[m];
+            re += u.x * w.x - u.y * w.y;
+            im += u.x * w.y + u.y * w.x;
+            m += ky; if (m >= ny) m -= ny;
+        }
+        out[idx] = make_float2((float)(re * mult), (float)(im * mult));
+    }
+}
+
+hipError_t launch_plane_spectrum(int planes, int nx, int ny, const double* T, double2* U, float2* out, double mult,
+                                 const double2* twx, const double2* twy, hipStream_t st)
+{
+    const size_t tot = (size_t)planes * ny * (nx / 2 + 1);
+    const int g = (int)std::min<size_t>((tot + 255) / 256, 65536);
+    k_plane_dftx<<<g, 256, 0, st>>>(planes, nx, ny, T, U, twx);
+    k_plane_dfty<<<g, 256, 0, st>>>(planes, nx, ny, U, out, mult, twy);
+    return hipGetLastError();
+}
+
 __global__ void k_tensor_words(const double* __restrict__ W, float* __restrict__ Tw, double mult, size_t n)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)

@@ -1525,11 +1525,23 @@ struct msd_dipole {
     float* Tw = nullptr;
     float2 *Mh = nullptr, *Hh = nullptr, *twx = nullptr, *twy = nullptr;
     DipoleDev dev{};
+    // Per-layer-pair form (LongRange2D and the Dipole2D builder): the same row passes,
+    // a column pass with a full complex tensor per ordered layer pair.
+    bool pair = false;
+    std::string slot = "Dipole";
+    std::vector<double> P;    // real space [9][nz][nz][ny][nx]: XX..ZZ, labels [first][second]
+    bool pair_dirty = true;   // P changed since the words were made (LongRange2D compileRequired)
+    std::vector<double> sms, dfs;     // source-moment / destination-field scale per layer
+    float lsc_host[2 * MSD_MAX_LAYERS] = {};
+    bool lsc_valid = false;
+    float2* Tp = nullptr;     // [9][nz][nz][nxh][ny] words
+    float* lsc = nullptr;     // [2][MSD_MAX_LAYERS]
     ~msd_dipole()
     {
         (void)hipStreamSynchronize(G.stream);
         dfree(T64); dfree(U64); dfree(W64); dfree(twx64); dfree(twy64);
         dfree(Tw); dfree(Mh); dfree(Hh); dfree(twx); dfree(twy);
+        dfree(Tp); dfree(lsc);
     }
     int prepare();
 };
@@ -1564,16 +1576,18 @@ static int twiddles(int L, float2** d32, double2** d64)
     return MSD_OK;
 }
 
-extern "C" int msd_dipole_create(int nx, int ny, int nz, msd_dipole** out)
+static int layered_create(int nx, int ny, int nz, bool pair, const char* what, msd_dipole** out)
 {
     NONNULL(out);
     *out = nullptr;
-    if (nx < 1 || ny < 1 || nz < 1) return fail(MSD_EINVAL, "Dipole.new: bad size");
-    if (nx % 2) return fail(MSD_EUNSUPPORTED, "Dipole.new: nx=%d must be even (real-to-complex transform)", nx);
-    if (nz > 8) return fail(MSD_EUNSUPPORTED, "Dipole.new: nz=%d > 8 layers is not supported by the layered kernel", nz);
+    if (nx < 1 || ny < 1 || nz < 1) return fail(MSD_EINVAL, "%s.new: bad size", what);
+    if (nx % 2) return fail(MSD_EUNSUPPORTED, "%s.new: nx=%d must be even (real-to-complex transform)", what, nx);
+    if (nz > MSD_MAX_LAYERS)
+        return fail(MSD_EUNSUPPORTED, "%s.new: nz=%d > %d layers is not supported by the layered kernel", what, nz,
+                    MSD_MAX_LAYERS);
     FFTPlan px, py;
     if (!make_plan(nx, px) || !make_plan(ny, py))
-        return fail(MSD_EUNSUPPORTED, "Dipole.new: %dx%d needs prime factors in {2,3,5}", nx, ny);
+        return fail(MSD_EUNSUPPORTED, "%s.new: %dx%d needs prime factors in {2,3,5}", what, nx, ny);
     if (int rc = ensure_init()) return rc;
     std::unique_ptr<msd_dipole> p(new msd_dipole);
     p->d = Dims{nx, ny, nz, nx * ny * nz};
@@ -1611,12 +1625,24 @@ extern "C" int msd_dipole_create(int nx, int ny, int nz, msd_dipole** out)
     // compile-time column lengths also stage the workgroup's tensor words (3 x nz x ny floats)
     dv.lds_cols = pow2_len_y ? (cbuf + ny) * sizeof(float2) + (size_t)3 * nz * ny * sizeof(float) : 2 * cbuf * sizeof(float2);
     if (dv.lds_rows > G.max_lds || dv.lds_cols > G.max_lds)
-        return fail(MSD_EUNSUPPORTED, "Dipole.new: %dx%dx%d needs %zu/%zu bytes of LDS per workgroup (device limit %zu)", nx, ny,
-                    nz, dv.lds_rows, dv.lds_cols, G.max_lds);
+        return fail(MSD_EUNSUPPORTED, "%s.new: %dx%dx%d needs %zu/%zu bytes of LDS per workgroup (device limit %zu)", what,
+                    nx, ny, nz, dv.lds_rows, dv.lds_cols, G.max_lds);
     const size_t spec = (size_t)3 * nz * p->nxh * ny;
     if (int rc = dalloc(&p->Mh, spec)) return rc;
     if (int rc = dalloc(&p->Hh, spec)) return rc;
-    if (int rc = dalloc(&p->Tw, (size_t)6 * nz * p->nxh * ny)) return rc;
+    if (pair) {
+        p->pair = true;
+        p->slot = "LongRange2D";
+        p->P.assign((size_t)9 * nz * nz * nx * ny, 0.0);
+        p->sms.assign(nz, 1.0);
+        p->dfs.assign(nz, 1.0);
+        if (int rc = dalloc(&p->Tp, (size_t)9 * nz * nz * p->nxh * ny)) return rc;
+        if (int rc = dalloc(&p->lsc, (size_t)2 * MSD_MAX_LAYERS)) return rc;
+        dv.Tp = p->Tp;
+        dv.lsc = p->lsc;
+    } else {
+        if (int rc = dalloc(&p->Tw, (size_t)6 * nz * p->nxh * ny)) return rc;
+    }
     if (int rc = twiddles(nx, &p->twx, &p->twx64)) return rc;
     if (int rc = twiddles(ny, &p->twy, &p->twy64)) return rc;
     dv.Mh = p->Mh; dv.Hh = p->Hh; dv.Tw = p->Tw; dv.twx = p->twx; dv.twy = p->twy;
@@ -1624,13 +1650,31 @@ extern "C" int msd_dipole_create(int nx, int ny, int nz, msd_dipole** out)
     return MSD_OK;
 }
 
+extern "C" int msd_dipole_create(int nx, int ny, int nz, msd_dipole** out)
+{
+    return layered_create(nx, ny, nz, false, "Dipole", out);
+}
+
 extern "C" int msd_dipole_destroy(msd_dipole* d) { delete d; return MSD_OK; }
-extern "C" int msd_dipole_set_strength(msd_dipole* d, double g) { NONNULL(d); d->g = g; return MSD_OK; }
+// the lattice-sum Dipole's own entry points refuse a LongRange2D object
+#define LATTICE_ONLY(d, what)                                                                            \
+    if ((d)->pair) return fail(MSD_EUNSUPPORTED, "%s: not defined for a %s operator", what, (d)->slot.c_str())
+
+// LongRange2D's setStrength raises "deprecated" (cpu/longrange2d/longrange2d_luafuncs.lua:11-13)
+extern "C" int msd_dipole_set_strength(msd_dipole* d, double g)
+{
+    NONNULL(d);
+    if (d->pair)
+        return fail(MSD_EUNSUPPORTED, "setStrength is deprecated. Please use setSourceMomentScale and setDestinationFieldScale.");
+    d->g = g;
+    return MSD_OK;
+}
 extern "C" int msd_dipole_set_scale(msd_dipole* d, double s) { NONNULL(d); d->scale = s; return MSD_OK; }
 
 extern "C" int msd_dipole_set_truncation(msd_dipole* d, const int gmax[4])
 {
     NONNULL(d); NONNULL(gmax);
+    LATTICE_ONLY(d, "setTruncation");
     if (gmax[0] < 1) return fail(MSD_EUNSUPPORTED, "setTruncation: radial limit %d (math.huge extrapolation is not supported)", gmax[0]);
     for (int i = 0; i < 4; ++i) d->gmax[i] = gmax[i];
     d->loaded = false; d->custom = false; d->spectrum = false;
@@ -1640,6 +1684,7 @@ extern "C" int msd_dipole_set_truncation(msd_dipole* d, const int gmax[4])
 extern "C" int msd_dipole_set_unit_cell(msd_dipole* d, const double ABC[9])
 {
     NONNULL(d); NONNULL(ABC);
+    LATTICE_ONLY(d, "setUnitCell");
     for (int i = 0; i < 9; ++i) d->ABC[i] = ABC[i];
     d->loaded = false; d->custom = false; d->spectrum = false;
     return MSD_OK;
@@ -1651,6 +1696,7 @@ extern "C" int msd_dipole_set_unit_cell(msd_dipole* d, const double ABC[9])
 extern "C" int msd_dipole_encode(msd_dipole* d, int first_index, void* buf, int64_t cap, int64_t* len, int* n_objects)
 {
     NONNULL(d); NONNULL(len);
+    LATTICE_ONLY(d, "Dipole.encode");
     if (first_index < 0) return fail(MSD_EINVAL, "Dipole.encode: object index %d < 0", first_index);
     WireOut w;
     wire_operator(w, first_index, d->d, d->scale);
@@ -1713,6 +1759,7 @@ static int dipole_load(msd_dipole* d)
 // LongRange::getAB offset mapping (spinoperationlongrange.cpp:143-175)
 static int dipole_offset(msd_dipole* d, int which, int ox, int oy, int oz, size_t* idx)
 {
+    LATTICE_ONLY(d, "get/setMatrix(which, ox, oy, oz)");
     if (which < 0 || which > 5) return fail(MSD_EINVAL, "matrix index %d (0..5 = XX XY XZ YY YZ ZZ)", which);
     const int nx = d->d.nx, ny = d->d.ny, nz = d->d.nz;
     ox = ((ox % nx) + nx) % nx;
@@ -1777,8 +1824,45 @@ static int check_symmetry(msd_dipole* d)
     return MSD_OK;
 }
 
+// LongRange2D::compile (cpu/longrange2d/spinoperationlongrange2d.cpp:396-446): FP64 2D
+// transforms of every tensor plane, prescaled by 1/(nx ny), kept as FP32 words; the
+// per-layer scales (applied at apply time, :463-526) go to a small device table
+static int pair_prepare(msd_dipole* d)
+{
+    const int nx = d->d.nx, ny = d->d.ny, nz = d->d.nz;
+    if (d->pair_dirty) {
+        const int planes = 9 * nz * nz;
+        double* T = nullptr;
+        double2* U = nullptr;
+        if (int rc = dalloc(&T, (size_t)planes * nx * ny)) return rc;
+        if (int rc = dalloc(&U, (size_t)planes * ny * d->nxh)) { dfree(T); return rc; }
+        hipError_t e = hipMemcpyAsync(T, d->P.data(), sizeof(double) * d->P.size(), hipMemcpyHostToDevice, G.stream);
+        if (e == hipSuccess)
+            e = launch_plane_spectrum(planes, nx, ny, T, U, d->Tp, 1.0 / ((double)nx * (double)ny), d->twx64, d->twy64,
+                                      G.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(G.stream);
+        dfree(T);
+        dfree(U);
+        if (e != hipSuccess) return fail(MSD_EDEVICE, "%s tensor spectrum: %s", d->slot.c_str(), hipGetErrorString(e));
+        d->pair_dirty = false;
+    }
+    float want[2 * MSD_MAX_LAYERS] = {};
+    for (int l = 0; l < nz; ++l) {
+        want[l] = (float)d->sms[l];
+        want[MSD_MAX_LAYERS + l] = (float)(d->dfs[l] * d->scale);
+    }
+    if (!d->lsc_valid || std::memcmp(want, d->lsc_host, sizeof want) != 0) {
+        std::memcpy(d->lsc_host, want, sizeof want);
+        HIPCHK(hipMemcpyAsync(d->lsc, d->lsc_host, sizeof want, hipMemcpyHostToDevice, G.stream));
+        HIPCHK(hipStreamSynchronize(G.stream));
+        d->lsc_valid = true;
+    }
+    return MSD_OK;
+}
+
 int msd_dipole::prepare()
 {
+    if (pair) return pair_prepare(this);
     if (int rc = dipole_load(this)) return rc;
     if (!spectrum) {
         if (custom)
@@ -1805,15 +1889,146 @@ int msd_dipole::prepare()
 extern "C" int msd_dipole_apply(msd_dipole* d, msd_system* s)
 {
     NONNULL(d); NONNULL(s);
-    if (!same_dims(s->d, d->d.nx, d->d.ny, d->d.nz)) return fail(MSD_EMISMATCH, "Dipole:apply: size mismatch");
+    if (!same_dims(s->d, d->d.nx, d->d.ny, d->d.nz)) return fail(MSD_EMISMATCH, "%s:apply: size mismatch", d->slot.c_str());
     if (int rc = d->prepare()) return rc;
     int k;
-    if (int rc = s->mark("Dipole", &k)) return rc;
+    if (int rc = s->mark(d->slot.c_str(), &k)) return rc;
     d->epoch++;
     hipError_t e = launch_rows_fwd(d->dev, s->S, G.stream);
     if (e == hipSuccess) e = launch_cols(d->dev, G.stream);
     if (e == hipSuccess) e = launch_rows_inv(d->dev, s->slots[k].h, G.stream);
-    if (e != hipSuccess) return fail(MSD_EDEVICE, "Dipole:apply: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return fail(MSD_EDEVICE, "%s:apply: %s", d->slot.c_str(), hipGetErrorString(e));
+    return MSD_OK;
+}
+
+// ------------------------------------------------------------ LongRange2D
+// cpu/longrange2d/spinoperationlongrange2d.cpp: tensors per ordered layer pair, source-moment
+// and destination-field scales per layer, the operator scale over all of it.
+
+extern "C" int msd_lr2d_create(int nx, int ny, int nz, const char* slot, msd_dipole** out)
+{
+    if (int rc = layered_create(nx, ny, nz, true, slot && *slot ? slot : "LongRange2D", out)) return rc;
+    if (slot && *slot) (*out)->slot = slot;
+    return MSD_OK;
+}
+
+static int pair_plane(msd_dipole* d, int dest, int src, int ab, size_t* off, const char* what)
+{
+    if (!d->pair) return fail(MSD_EUNSUPPORTED, "%s: not a LongRange2D operator", what);
+    const int nz = d->d.nz;
+    if (dest < 0 || dest >= nz || src < 0 || src >= nz)
+        return fail(MSD_EINVAL, "%s: layers (%d, %d) outside 1..%d", what, dest + 1, src + 1, nz);
+    if (ab < 0 || ab > 8) return fail(MSD_EINVAL, "%s: tensor index %d (0..8 = XX XY XZ YX YY YZ ZX ZY ZZ)", what, ab);
+    *off = (((size_t)ab * nz + dest) * nz + src) * (size_t)d->d.nx * d->d.ny;
+    return MSD_OK;
+}
+
+// LongRange2D::setLAB / l_setTensorArray (spinoperationlongrange2d.cpp:198-224,594-602)
+extern "C" int msd_lr2d_set_tensor(msd_dipole* d, int dest, int src, int ab, const double* a)
+{
+    NONNULL(d); NONNULL(a);
+    size_t off;
+    if (int rc = pair_plane(d, dest, src, ab, &off, "setTensorArray")) return rc;
+    std::memcpy(d->P.data() + off, a, sizeof(double) * d->d.nx * d->d.ny);
+    d->pair_dirty = true;
+    return MSD_OK;
+}
+
+// LongRange2D::getLAB / l_getTensorArray (:168-196,585-592)
+extern "C" int msd_lr2d_get_tensor(msd_dipole* d, int dest, int src, int ab, double* a)
+{
+    NONNULL(d); NONNULL(a);
+    size_t off;
+    if (int rc = pair_plane(d, dest, src, ab, &off, "tensorArray")) return rc;
+    std::memcpy(a, d->P.data() + off, sizeof(double) * d->d.nx * d->d.ny);
+    return MSD_OK;
+}
+
+// l_ssms / l_sdfs (:533-568): one value per layer
+extern "C" int msd_lr2d_set_layer_scales(msd_dipole* d, const double* src_moment, const double* dest_field)
+{
+    NONNULL(d);
+    if (!d->pair) return fail(MSD_EUNSUPPORTED, "layer scales: not a LongRange2D operator");
+    for (int l = 0; l < d->d.nz; ++l) {
+        if (src_moment) d->sms[l] = src_moment[l];
+        if (dest_field) d->dfs[l] = dest_field[l];
+    }
+    return MSD_OK;
+}
+
+extern "C" int msd_lr2d_get_layer_scales(msd_dipole* d, double* src_moment, double* dest_field)
+{
+    NONNULL(d);
+    if (!d->pair) return fail(MSD_EUNSUPPORTED, "layer scales: not a LongRange2D operator");
+    for (int l = 0; l < d->d.nz; ++l) {
+        if (src_moment) src_moment[l] = d->sms[l];
+        if (dest_field) dest_field[l] = d->dfs[l];
+    }
+    return MSD_OK;
+}
+
+// Dipole2D point tensor (cpu/dipole2d/demag.cpp:3-62): N_ab(r) = 3 r_a r_b / r^5 - delta_ab / r^3,
+// 0 at r < 1e-80.  Component order XX XY XZ YX YY YZ ZX ZY ZZ.
+static void dipole2d_point(double rx, double ry, double rz, double N[9])
+{
+    const double r = std::sqrt(rx * rx + ry * ry + rz * rz);
+    if (r < 1e-80) {
+        for (int k = 0; k < 9; ++k) N[k] = 0;
+        return;
+    }
+    const double ir = 1.0 / r, ir3 = ir * ir * ir, ir5 = ir3 * ir * ir;
+    const double v[3] = {rx, ry, rz};
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) N[3 * a + b] = -1.0 * ((a == b ? ir3 : 0.0) - 3.0 * v[a] * v[b] * ir5);
+}
+
+// Dipole2D makeData (cpu/dipole2d/dip2d_luafuncs.lua:537-668).  For every label pair (dest,
+// src) with |src - dest| <= truncZ and every X in [-truncX, truncX], Y in [-truncY, truncY],
+// N(p_src - p_dest) is added at (X mod nx, Y mod ny) of T[dest][src], p = idx2pos(...) (:537-557):
+// X A1 + Y B1' + (C_1 + ... + C_{layer-1}) + offset[layer], with A1, B1 the first layer's
+// cell vectors.  The reference evaluates p_src and p_dest at the same (X, Y), so the
+// in-plane terms cancel and every cell of the box gets the layers' vertical coupling;
+// and its B1' is (B1y, B1y, B1z).  physical = 0 keeps both (the reference's tensors);
+// physical = 1 places the destination at the origin, p_dest = idx2pos(0, 0, dest), and
+// uses B1 as given, which makes T[dest][src](X, Y) the coupling of a source at the origin of
+// layer `dest` to a site at in-plane cell (X, Y) of layer `src` (the pair the reference's
+// apply reads as [source][destination]).  ABC: [nz][3][3] (A, B, C rows), offset: [nz][3].
+extern "C" int msd_lr2d_build_dipole2d(msd_dipole* d, const double* ABC, const double* offset, const int trunc[4],
+                                       int physical)
+{
+    NONNULL(d); NONNULL(ABC); NONNULL(offset); NONNULL(trunc);
+    if (!d->pair) return fail(MSD_EUNSUPPORTED, "Dipole2D makeData: not a LongRange2D operator");
+    const int nx = d->d.nx, ny = d->d.ny, nz = d->d.nz, tX = trunc[1], tY = trunc[2], tZ = trunc[3];
+    if (tX < 0 || tY < 0 || tZ < 0) return fail(MSD_EINVAL, "Dipole2D: negative truncation (%d, %d, %d)", tX, tY, tZ);
+    if ((double)(2 * tX + 1) * (2 * tY + 1) * nz * nz > 4e9)
+        return fail(MSD_EUNSUPPORTED, "Dipole2D: truncation box (%d, %d) too large", tX, tY);
+    auto pos = [&](int X, int Y, int layer, double p[3]) {
+        const double* A = ABC;          // first layer's A and B
+        const double* B = ABC + 3;
+        double dz[3] = {0, 0, 0};
+        for (int Z = 0; Z < layer; ++Z)
+            for (int c = 0; c < 3; ++c) dz[c] += ABC[9 * Z + 6 + c];
+        const double bj[3] = {physical ? B[0] : B[1], B[1], B[2]};
+        for (int c = 0; c < 3; ++c) p[c] = X * A[c] + Y * bj[c] + dz[c] + offset[3 * layer + c];
+    };
+    const size_t plane = (size_t)nx * ny;
+    std::fill(d->P.begin(), d->P.end(), 0.0);
+    for (int dest = 0; dest < nz; ++dest)
+        for (int src = 0; src < nz; ++src) {
+            if (std::abs(src - dest) > tZ) continue;
+            for (int X = -tX; X <= tX; ++X)
+                for (int Y = -tY; Y <= tY; ++Y) {
+                    double pd[3], ps[3], N[9];
+                    if (physical) pos(0, 0, dest, pd);
+                    else pos(X, Y, dest, pd);
+                    pos(X, Y, src, ps);
+                    dipole2d_point(ps[0] - pd[0], ps[1] - pd[1], ps[2] - pd[2], N);
+                    const int x = ((X % nx) + nx) % nx, y = ((Y % ny) + ny) % ny;
+                    for (int ab = 0; ab < 9; ++ab)
+                        d->P[(((size_t)ab * nz + dest) * nz + src) * plane + x + (size_t)nx * y] += N[ab];
+                }
+        }
+    d->pair_dirty = true;
     return MSD_OK;
 }
 
@@ -3633,7 +3848,7 @@ struct FilmGroup {
     static bool same_shape(const StepArgs& a, const StepArgs& b)
     {
         return std::memcmp(&a.d, &b.d, sizeof a.d) == 0 && a.ty == b.ty && (a.Slo == nullptr) == (b.Slo == nullptr) &&
-               a.dp.tpb == b.dp.tpb && a.dp.ty == b.dp.ty && a.dp.lds_rows == b.dp.lds_rows &&
+               (a.dp.Tp == nullptr) == (b.dp.Tp == nullptr) && a.dp.tpb == b.dp.tpb && a.dp.ty == b.dp.ty && a.dp.lds_rows == b.dp.lds_rows &&
                a.dp.lds_cols == b.dp.lds_cols && std::memcmp(&a.dp.px, &b.dp.px, sizeof a.dp.px) == 0 &&
                std::memcmp(&a.dp.py, &b.dp.py, sizeof a.dp.py) == 0;
     }

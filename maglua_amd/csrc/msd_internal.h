@@ -68,6 +68,7 @@ struct LLGDev {
 };
 
 // ------------------------------------------------------ layered dipole
+constexpr int MSD_MAX_LAYERS = 8;   // layers of the layered column kernels
 struct DipoleDev {
     int nx, ny, nz, nxh;
     int ty;                 // rows per row-tile
@@ -78,6 +79,9 @@ struct DipoleDev {
     float2* Mh;             // spin spectrum [3][nz][nxh][ny]
     float2* Hh;             // field spectrum [3][nz][nxh][ny]
     const float* Tw;        // tensor words [6][nz][nxh][ny] (XX XY XZ YY YZ ZZ; XZ,YZ imaginary)
+    // per-layer-pair form (LongRange2D and its builders); null for the lattice-sum Dipole
+    const float2* Tp;       // complex words [9][nz][nz][nxh][ny], 1/(nx ny) folded in
+    const float* lsc;       // [2][MSD_MAX_LAYERS]: source-moment scales, destination-field scales
     size_t lds_rows;        // dynamic LDS bytes of a row kernel
     size_t lds_cols;        // dynamic LDS bytes of a column kernel
 };
@@ -124,6 +128,10 @@ hipError_t launch_tensor_sum(int nx, int ny, int nz, const int gmax[4], const do
 hipError_t launch_tensor_spectrum(int nx, int ny, int nz, const double* T, double2* U, double* W,
                                   const double2* twx, const double2* twy, hipStream_t st);
 hipError_t launch_tensor_words(const double* W, float* Tw, double mult, size_t n, hipStream_t st);
+// planes of nx x ny real FP64 values T [planes][ny][nx] -> R2C spectra out [planes][nx/2+1][ny]
+// as FP32 complex, times mult; U scratch [planes][ny][nx/2+1]
+hipError_t launch_plane_spectrum(int planes, int nx, int ny, const double* T, double2* U, float2* out, double mult,
+                                 const double2* twx, const double2* twy, hipStream_t st);
 
 // fused Heun step
 // Anisotropy.Uniaxial ops as a site-sorted CSR: start [n+1], ops [nops][5] = nx ny nz K1 K2

@@ -744,6 +744,293 @@ class Dipole(_Handle):
 LongRange = Dipole
 
 
+class _TensorArray:
+    """One tensor plane of a LongRange2D operator (the reference hands out the operator's
+    own dArray): get/set/addAt take 1-based (x, y) as the reference's Array does, and
+    write into the operator's copy.  As in the reference, a change reaches the field only
+    after setCompileRequired(true) (or a makeData that sets it)."""
+
+    def __init__(self, owner, ab, dest, src):
+        self._o, self._k = owner, (ab, dest, src)
+
+    def _plane(self):
+        return self._o._T[self._k]
+
+    def nx(self):
+        return self._o._dims[0]
+
+    def ny(self):
+        return self._o._dims[1]
+
+    def get(self, x, y):
+        return float(self._plane()[int(y) - 1, int(x) - 1])
+
+    def set(self, x, y, v):
+        self._plane()[int(y) - 1, int(x) - 1] = float(v)
+        self._o._host_dirty = True
+
+    def addAt(self, x, y, v):
+        self._plane()[int(y) - 1, int(x) - 1] += float(v)
+        self._o._host_dirty = True
+
+    def zero(self):
+        self._plane()[:] = 0.0
+        self._o._host_dirty = True
+
+    def toArray(self):
+        """numpy copy [ny][nx]"""
+        return self._plane().copy()
+
+
+class LongRange2D(_Handle):
+    """LongRange2D (cpu/longrange2d/spinoperationlongrange2d.cpp): a layered in-plane FFT
+    convolution with one tensor per ordered layer pair T_ab[dest][src](x, y), per-layer
+    source-moment and destination-field scales and the operator scale, slot 'LongRange2D'.
+    The field of layer d reads the pair stored under [s][d] for every source layer s, as
+    the reference's apply does (:463-526).  Tensors are made by the function given to
+    setMakeDataFunction when new data is required (Dipole2D installs its own), or set
+    through tensorArray / setTensorArray; they are transformed when a compile is required.
+    Both flags start true, as in the reference; a plain LongRange2D with no make-data
+    function raises at apply until setNewDataRequired(false)."""
+
+    _destroy = "msd_dipole_destroy"
+    NAMES = ("XX", "XY", "XZ", "YX", "YY", "YZ", "ZX", "ZY", "ZZ")
+    _slot_default = "LongRange2D"
+
+    def __init__(self, *a):
+        self._dims = _dims_of(a)
+        nx, ny, nz = self._dims
+        p = _vp()
+        call("msd_lr2d_create", nx, ny, nz, self._slot_default.encode(), C.byref(p))
+        super().__init__(p.value)
+        self._T = np.zeros((9, nz, nz, ny, nx))
+        self._sms = np.ones(nz)
+        self._dfs = np.ones(nz)
+        self._scale = 1.0
+        self._compile = True
+        self._newdata = True
+        self._host_dirty = True
+        self._makefn = None
+        self._internal = None
+
+    new = classmethod(lambda cls, *a: cls(*a))
+
+    def nx(self):
+        return self._dims[0]
+
+    def ny(self):
+        return self._dims[1]
+
+    def nz(self):
+        return self._dims[2]
+
+    def slotName(self):
+        return self._slot_default
+
+    # ---- scales (l_ssms / l_gsms / l_sdfs / l_gdfs, :533-580)
+    def _layer(self, layer):
+        k = int(layer) - 1
+        if k < 0 or k >= self._dims[2]:
+            raise MagLuaError(-1, "Invalid Layer")
+        return k
+
+    def setSourceMomentScale(self, layer, v):
+        self._sms[self._layer(layer)] = float(v)
+        call("msd_lr2d_set_layer_scales", self._h, _d(self._sms), None)
+
+    def sourceMomentScale(self, layer):
+        return float(self._sms[self._layer(layer)])
+
+    def setDestinationFieldScale(self, layer, v):
+        self._dfs[self._layer(layer)] = float(v)
+        call("msd_lr2d_set_layer_scales", self._h, None, _d(self._dfs))
+
+    def destinationFieldScale(self, layer):
+        return float(self._dfs[self._layer(layer)])
+
+    def setScale(self, s):
+        self._scale = float(s)
+        call("msd_dipole_set_scale", self._h, self._scale)
+
+    def scale(self):
+        return self._scale
+
+    def setStrength(self, *a):
+        raise MagLuaError(-3, "setStrength is depreciated. Please use setSourceMomentScale and setDestinationFieldScale.")
+
+    def strength(self, *a):
+        raise MagLuaError(-3, "strength is depreciated. Please use sourceMomentScale and sestinationFieldScale.")
+
+    # ---- tensors (l_getTensorArray / l_setTensorArray, :585-602)
+    def _ab(self, name):
+        try:
+            return self.NAMES.index(str(name).upper()[:2])
+        except ValueError:
+            raise MagLuaError(-1, "tensor name must be one of XX XY XZ YX YY YZ ZX ZY ZZ")
+
+    def tensorArray(self, dest, src, ab):
+        return _TensorArray(self, self._ab(ab), self._layer(dest), self._layer(src))
+
+    def setTensorArray(self, dest, src, ab, a):
+        """Copies a ([ny][nx] array, nx*ny values x fastest, or a tensorArray) into the plane."""
+        v = a.toArray() if isinstance(a, _TensorArray) else np.asarray(a, dtype=np.float64)
+        self._T[self._ab(ab), self._layer(dest), self._layer(src)] = v.reshape(self._dims[1], self._dims[0])
+        self._host_dirty = True
+
+    def getMatrix(self, dest, src, x, y, ab):
+        """t.getMatrix (longrange2d_luafuncs.lua:6-9): tensorArray(dest, src, ab):get(x, y)."""
+        return self.tensorArray(dest, src, ab).get(x, y)
+
+    # ---- state flags and the make-data hook (:604-679)
+    def setCompileRequired(self, b=True):
+        self._compile = bool(b)
+
+    def setNewDataRequired(self, b=True):
+        self._newdata = bool(b)
+
+    def setInternalData(self, v):
+        self._internal = v
+
+    def internalData(self):
+        return self._internal
+
+    def setMakeDataFunction(self, f):
+        if not callable(f):
+            raise MagLuaError(-1, "function required as argument")
+        self._makefn = f
+
+    def makeDataFunction(self):
+        return self._makefn
+
+    def makeData(self):
+        """LongRange2D::makeNewData (:377-394)."""
+        if not self._newdata:
+            return
+        self._newdata = False
+        if self._makefn is None:
+            raise MagLuaError(-1, "make data function not set")
+        self._makefn(self)
+        self._compile = True
+
+    def _pull(self):
+        """Copy the library's tensors (after a builder ran there) into this object's copy."""
+        nz = self._dims[2]
+        buf = np.empty(self._dims[0] * self._dims[1])
+        for ab in range(9):
+            for dd in range(nz):
+                for ss in range(nz):
+                    call("msd_lr2d_get_tensor", self._h, dd, ss, ab, _d(buf))
+                    self._T[ab, dd, ss] = buf.reshape(self._dims[1], self._dims[0])
+        self._host_dirty = False
+
+    def _sync(self):
+        """What apply does first: new data, then the compile (the library transforms the
+        tensors it holds at the next field evaluation)."""
+        if self._newdata:
+            self.makeData()
+        if self._compile:
+            if self._host_dirty:
+                nz = self._dims[2]
+                for ab in range(9):
+                    for dd in range(nz):
+                        for ss in range(nz):
+                            call("msd_lr2d_set_tensor", self._h, dd, ss, ab,
+                                 _d(np.ascontiguousarray(self._T[ab, dd, ss]).reshape(-1)))
+                self._host_dirty = False
+            self._compile = False
+
+    def apply(self, ss):
+        self._sync()
+        for s in (ss if isinstance(ss, (list, tuple)) else (ss,)):
+            call("msd_dipole_apply", self._h, s.h)
+
+
+class Dipole2D(LongRange2D):
+    """Dipole2D (cpu/dipole2d): LongRange2D whose tensors are the truncated dipole box sums of
+    makeData (dip2d_luafuncs.lua:559-668), slot 'Dipole2D'.  Internal data: per-layer unit
+    cells (identity) and offsets (0), truncation 25 in every direction.  physical=False
+    reproduces the reference's tensors, which evaluate source and destination at the same
+    in-plane cell, so only the layers' vertical coupling enters; physical=True places the
+    destination at the origin (the truncated in-plane dipole sum).  As in the reference,
+    changing the cell, offsets or truncation after the tensors are made takes
+    setNewDataRequired() to rebuild them."""
+
+    _slot_default = "Dipole2D"
+
+    def __init__(self, *a, physical=False):
+        super().__init__(*a)
+        self._physical = bool(physical)
+        self.setMakeDataFunction(Dipole2D._make)
+        self.setInternalData(None)
+
+    def setPhysical(self, yes=True):
+        self._physical = bool(yes)
+
+    def _id(self):
+        """initializeInternalData (:6-26)"""
+        if self._internal is None:
+            nz = self._dims[2]
+            self._internal = {"ABC": [[[1, 0, 0], [0, 1, 0], [0, 0, 1]] for _ in range(nz)],
+                              "offset": [[0, 0, 0] for _ in range(nz)],
+                              "truncation": 25, "truncationX": 25, "truncationY": 25, "truncationZ": 25}
+        return self._internal
+
+    def _check_layer(self, layer):
+        if not isinstance(layer, (int, float, np.integer, np.floating)):
+            raise MagLuaError(-1, "Layer expected as first argument")
+        if layer < 1 or layer > self._dims[2]:
+            raise MagLuaError(-1, "Layer must be in the range [1:%d]" % self._dims[2])
+        return int(layer) - 1
+
+    def setUnitCell(self, layer, A, B=None, Cv=None):
+        k = self._check_layer(layer)
+        if B is None:
+            return self.setUnitCell(layer, A[0], A[1], A[2])
+        try:
+            cell = [[float(v) for v in A], [float(v) for v in B], [float(v) for v in Cv]]
+        except TypeError:
+            raise MagLuaError(-1, "Tables expected for the A, B, C components (2nd through 4th arguments)")
+        self._id()["ABC"][k] = cell
+
+    def unitCell(self, layer):
+        k = self._check_layer(layer)
+        A, B, Cv = self._id()["ABC"][k]
+        return list(A), list(B), list(Cv)
+
+    def setOffset(self, layer, v):
+        k = self._check_layer(layer)
+        try:
+            self._id()["offset"][k] = [float(x) for x in v]
+        except TypeError:
+            raise MagLuaError(-1, "Table expected for 2nd argument")
+
+    def offset(self, layer=None):
+        t = self._id()
+        if layer is None:
+            return [list(o) for o in t["offset"]]
+        return list(t["offset"][self._check_layer(layer)])
+
+    def setTruncation(self, trunc=None, truncX=None, truncY=None, truncZ=None):
+        t = self._id()
+        for k, v in (("truncation", trunc), ("truncationX", truncX), ("truncationY", truncY), ("truncationZ", truncZ)):
+            if v is not None:
+                t[k] = int(v)
+
+    def truncation(self):
+        t = self._id()
+        return t["truncation"], t["truncationX"], t["truncationY"], t["truncationZ"]
+
+    def _make(self):
+        t = self._id()
+        nz = self._dims[2]
+        ABC = np.ascontiguousarray(np.asarray(t["ABC"], dtype=np.float64).reshape(nz * 9))
+        off = np.ascontiguousarray(np.asarray(t["offset"], dtype=np.float64).reshape(nz * 3))
+        tr = (C.c_int * 4)(t["truncation"], t["truncationX"], t["truncationY"], t["truncationZ"])
+        call("msd_lr2d_build_dipole2d", self._h, _d(ABC), _d(off), tr, int(self._physical))
+        self._pull()
+        self.setCompileRequired(True)
+
+
 class LongRange3D(_Handle):
     """LongRange3D (cpu/longrange3d/spinoperationlongrange3d.cpp): 3D FFT convolution, slot
     'LongRange3D'.  open=False: periodic, dipole lattice sum to radius R (setRadius, default
@@ -1049,7 +1336,13 @@ class Heun(_Handle):
         if lr3 is not None:
             call("msd_stepper_set_longrange3d", self._h, lr3.h)
 
+    def _sync(self):
+        d = self._keep[3]
+        if isinstance(d, LongRange2D):
+            d._sync()
+
     def step(self, n=1):
+        self._sync()
         call("msd_stepper_heun", self._h, int(n))
         return True
 
@@ -1090,6 +1383,7 @@ class Heun(_Handle):
         (Butcher table `kind` of RungeKutta.lua; plain FP32 state)."""
         if kind not in BUTCHER:
             raise MagLuaError(-1, 'RK type must be one of "' + '", "'.join(BUTCHER) + '"')
+        self._sync()
         call("msd_stepper_rk", self._h, kind.encode(), int(n))
         return True
 
@@ -1223,12 +1517,15 @@ class Surrogate(_Handle):
 
 class Ensemble(_Handle):
     """Independent replicas (uneven sizes, own temperatures and seeds) advanced by one set
-    of launches per Heun pass.  members: Heun step drivers of the local or LongRange3D form."""
+    of launches per Heun pass.  members: Heun step drivers of the local, layered (Dipole,
+    LongRange2D, Dipole2D) or LongRange3D form."""
 
     _destroy = "msd_ensemble_destroy"
 
     def __init__(self, members, graph=True, compensated=True):
         self._members = list(members)
+        for m in self._members:
+            m._sync()
         arr = (C.c_void_p * len(self._members))(*[m.h for m in self._members])
         p = _vp()
         call("msd_ensemble_create", arr, len(self._members), C.byref(p))
@@ -1237,6 +1534,8 @@ class Ensemble(_Handle):
         call("msd_ensemble_set_compensated", self._h, int(bool(compensated)))
 
     def step(self, n=1):
+        for m in self._members:
+            m._sync()
         call("msd_ensemble_heun", self._h, int(n))
         return True
 

@@ -48,6 +48,15 @@
  *                          the closed form (tests).
  *   ora_sys_net_moment     SpinSystem netMoment, cpu/core/spinsystem_luafuncs.lua:17-54
  *                          (returns sums, not means).
+ *   ora_lr2d_*             LongRange2D::compile/apply,
+ *                          cpu/longrange2d/spinoperationlongrange2d.cpp:396-531 (the pair
+ *                          read as [source][destination], per-layer scales).  Pinned by a
+ *                          direct real-space circular convolution (tests).
+ *   ora_dipole2d_tensor    Dipole2D makeData and idx2pos, cpu/dipole2d/dip2d_luafuncs.lua:
+ *                          537-668, point tensor cpu/dipole2d/demag.cpp:3-62.  The reference
+ *                          form is pinned by the textbook dipole coupling of stacked sites
+ *                          and box-count closed forms; the physical form by equality with
+ *                          ora_dipole_tensor's lattice sums over the same box (tests).
  */
 #include "msd_oracle.h"
 
@@ -913,6 +922,171 @@ int ora_lr_apply(ora_lr* l, ora_sys* s)
 } /* extern "C" */
 
 /* ------------------------------------------------------------------------ */
+/* LongRange2D (a tensor per ordered layer pair) and the Dipole2D builder     */
+/* ------------------------------------------------------------------------ */
+struct ora_lr2d {
+    int nx, ny, nz;
+    double scale = 1;
+    std::vector<double> T;          /* [9][nz][nz][ny][nx], labels [dest][src] */
+    std::vector<double> sms, dfs;   /* source-moment / destination-field scale per layer */
+    std::vector<cplx> Q;            /* compiled spectra, same layout */
+    bool fresh = true;
+};
+
+extern "C" {
+ora_lr2d* ora_lr2d_new(int nx, int ny, int nz)
+{
+    if (nx < 1 || ny < 1 || nz < 1) return nullptr;
+    ora_lr2d* l = new ora_lr2d;
+    l->nx = nx; l->ny = ny; l->nz = nz;
+    l->T.assign((size_t)9 * nz * nz * nx * ny, 0.0);
+    l->sms.assign(nz, 1.0);
+    l->dfs.assign(nz, 1.0);
+    return l;
+}
+void ora_lr2d_free(ora_lr2d* l) { delete l; }
+void ora_lr2d_set_scale(ora_lr2d* l, double s) { l->scale = s; }
+void ora_lr2d_set_layer_scales(ora_lr2d* l, const double* sms, const double* dfs)
+{
+    for (int i = 0; i < l->nz; ++i) {
+        if (sms) l->sms[i] = sms[i];
+        if (dfs) l->dfs[i] = dfs[i];
+    }
+}
+
+static double* lr2d_plane(ora_lr2d* l, int dest, int src, int ab)
+{
+    if (dest < 0 || dest >= l->nz || src < 0 || src >= l->nz || ab < 0 || ab > 8) return nullptr;
+    return l->T.data() + (((size_t)ab * l->nz + dest) * l->nz + src) * (size_t)l->nx * l->ny;
+}
+
+int ora_lr2d_set_tensor(ora_lr2d* l, int dest, int src, int ab, const double* a)
+{
+    double* p = lr2d_plane(l, dest, src, ab);
+    if (!p) return -1;
+    std::memcpy(p, a, sizeof(double) * l->nx * l->ny);
+    l->fresh = true;
+    return 0;
+}
+
+int ora_lr2d_get_tensor(ora_lr2d* l, int dest, int src, int ab, double* a)
+{
+    double* p = lr2d_plane(l, dest, src, ab);
+    if (!p) return -1;
+    std::memcpy(a, p, sizeof(double) * l->nx * l->ny);
+    return 0;
+}
+
+/* LongRange2D::apply: H_a[d] = Re IFFT( sum_s sms[s] sum_b Q_ab[s][d] M^_b[s] ) dfs[d] scale,
+ * Q = FFT2D(T) / (nx ny) (compile, :396-446).  Summation order of the reference: for each
+ * component a, destination d, source s, then b = x, y, z. */
+int ora_lr2d_apply(ora_lr2d* l, ora_sys* s, const char* slot)
+{
+    const int nx = l->nx, ny = l->ny, nz = l->nz;
+    const size_t nxy = (size_t)nx * ny, n = nxy * nz;
+    if (l->fresh) {
+        const double inv = 1.0 / (double)nxy;
+        l->Q.assign(l->T.size(), cplx(0, 0));
+        for (size_t i = 0; i < l->T.size(); ++i) l->Q[i] = cplx(l->T[i], 0.0);
+        fft2d_layers(l->Q.data(), nx, ny, 9 * nz * nz, -1);
+        for (cplx& v : l->Q) v *= cplx(inv, 0);
+        l->fresh = false;
+    }
+    const int k = s->mark(slot && *slot ? slot : "LongRange2D");
+    std::vector<cplx> q[3];
+    const std::vector<double>* S[3] = {&s->x, &s->y, &s->z};
+    for (int c = 0; c < 3; ++c) {
+        q[c].resize(n);
+        for (size_t i = 0; i < n; ++i) q[c][i] = cplx((*S[c])[i], 0.0);
+        fft2d_layers(q[c].data(), nx, ny, nz, -1);
+    }
+    std::vector<double>* H[3] = {&s->hx[k], &s->hy[k], &s->hz[k]};
+    std::vector<cplx> ws(n);
+    for (int a = 0; a < 3; ++a) {
+        std::fill(ws.begin(), ws.end(), cplx(0, 0));
+#pragma omp parallel for schedule(static)
+        for (int d = 0; d < nz; ++d)
+            for (int sl = 0; sl < nz; ++sl) {
+                const cplx val(l->sms[sl], 0);
+                for (int b = 0; b < 3; ++b) {
+                    const cplx* T = l->Q.data() + (((size_t)(3 * a + b) * nz + sl) * nz + d) * nxy;
+                    const cplx* M = q[b].data() + (size_t)sl * nxy;
+                    cplx* D = ws.data() + (size_t)d * nxy;
+                    for (size_t i = 0; i < nxy; ++i) D[i] += val * T[i] * M[i];
+                }
+            }
+        fft2d_layers(ws.data(), nx, ny, nz, +1);
+        std::vector<double>& h = *H[a];
+        for (int d = 0; d < nz; ++d)
+            for (size_t i = 0; i < nxy; ++i)
+                h[(size_t)d * nxy + i] = ws[(size_t)d * nxy + i].real() * (l->dfs[d] * l->scale);
+    }
+    return 0;
+}
+
+/* demag.cpp:3-62, component order XX XY XZ YX YY YZ ZX ZY ZZ: Nxx = -(1/r^3 - 3 rx^2/r^5),
+ * Nxy = -(-3 rx ry / r^5); the others by permuting the arguments; 0 for r < 1e-80 */
+static double n2d_xx(double rx, double ry, double rz)
+{
+    const double r = std::sqrt(rx * rx + ry * ry + rz * rz);
+    if (r < 1E-80) return 0;
+    const double ir = 1.0 / r, ir3 = ir * ir * ir, ir5 = ir3 * ir * ir;
+    return -1.0 * (ir3 - 3.0 * rx * rx * ir5);
+}
+static double n2d_xy(double rx, double ry, double rz)
+{
+    const double r = std::sqrt(rx * rx + ry * ry + rz * rz);
+    if (r < 1E-80) return 0;
+    const double ir = 1.0 / r, ir3 = ir * ir * ir, ir5 = ir3 * ir * ir;
+    return -1.0 * (-3.0 * rx * ry * ir5);
+}
+
+/* Dipole2D makeData (dip2d_luafuncs.lua:559-668) with idx2pos (:537-557): ABC [nz][3][3]
+ * (rows A, B, C of each layer), off [nz][3], trunc {radial, x, y, z}; T [9][nz][nz][ny][nx].
+ * physical = 0: p_dest and p_src at the same (X, Y), B read as (B[2], B[2], B[3]) (1-based),
+ * as the reference does.  physical = 1: p_dest = idx2pos(0, 0, dest), B as given. */
+int ora_dipole2d_tensor(int nx, int ny, int nz, const double* ABC, const double* off, const int trunc[4], int physical,
+                        double* T)
+{
+    if (nx < 1 || ny < 1 || nz < 1) return -1;
+    const size_t nxy = (size_t)nx * ny;
+    std::fill(T, T + 9 * (size_t)nz * nz * nxy, 0.0);
+    auto idx2pos = [&](int i, int j, int layer, double p[3]) {   /* layer 1-based, as in Lua */
+        double dz[3] = {0, 0, 0};
+        for (int Z = 1; Z <= layer - 1; ++Z)
+            for (int c = 0; c < 3; ++c) dz[c] += ABC[9 * (Z - 1) + 6 + c];
+        const double* A1 = ABC;
+        const double* B1 = ABC + 3;
+        const double di[3] = {i * A1[0], i * A1[1], i * A1[2]};
+        const double dj[3] = {j * (physical ? B1[0] : B1[1]), j * B1[1], j * B1[2]};
+        for (int c = 0; c < 3; ++c) p[c] = di[c] + dj[c] + dz[c] + off[3 * (layer - 1) + c];
+    };
+    for (int dest = 1; dest <= nz; ++dest)
+        for (int src = 1; src <= nz; ++src) {
+            if (std::abs(src - dest) > trunc[3]) continue;
+            for (int X = -trunc[1]; X <= trunc[1]; ++X)
+                for (int Y = -trunc[2]; Y <= trunc[2]; ++Y) {
+                    double pd[3], ps[3];
+                    idx2pos(physical ? 0 : X, physical ? 0 : Y, dest, pd);
+                    idx2pos(X, Y, src, ps);
+                    const double rx = ps[0] - pd[0], ry = ps[1] - pd[1], rz = ps[2] - pd[2];
+                    long long x = X, y = Y;
+                    if (x < 0) x += 100000LL * nx;
+                    if (y < 0) y += 100000LL * ny;
+                    x %= nx;
+                    y %= ny;
+                    const double v[9] = {n2d_xx(rx, ry, rz), n2d_xy(rx, ry, rz), n2d_xy(rx, rz, ry),
+                                         n2d_xy(ry, rx, rz), n2d_xx(ry, rx, rz), n2d_xy(ry, rz, rx),
+                                         n2d_xy(rz, rx, ry), n2d_xy(rz, ry, rx), n2d_xx(rz, rx, ry)};
+                    for (int ab = 0; ab < 9; ++ab)
+                        T[(((size_t)ab * nz + (dest - 1)) * nz + (src - 1)) * nxy + (size_t)x + (size_t)nx * y] += v[ab];
+                }
+        }
+    return 0;
+}
+} /* extern "C" */
+
+/* ------------------------------------------------------------------------ */
 /* LongRange3D (3D FFT convolution) and 3D dipole tensors                    */
 /* ------------------------------------------------------------------------ */
 /* full 3D C2C transform of [nz][ny][nx] (Array::fft3DTo semantics: unnormalised) */
@@ -1272,6 +1446,8 @@ struct ora_sim {
     ora_sys* ss;
     ora_ex* ex; ora_ani* ani; ora_lr* lr; ora_th* th; ora_llg* llg;
     ora_lr3* lr3 = nullptr;
+    ora_lr2d* lr2d = nullptr;
+    std::string lr2d_slot;
     std::vector<ora_af*> af;
     std::vector<ora_uni*> uni;
     uint64_t seed = 0, step = 0;
@@ -1297,6 +1473,7 @@ void ora_sim_free(ora_sim* s)
 
 void ora_sim_set_seed(ora_sim* s, uint64_t seed) { s->seed = seed; s->step = 0; }
 void ora_sim_set_lr3(ora_sim* s, ora_lr3* l) { s->lr3 = l; }
+void ora_sim_set_lr2d(ora_sim* s, ora_lr2d* l, const char* slot) { s->lr2d = l; s->lr2d_slot = slot ? slot : ""; }
 void ora_sim_add_af(ora_sim* s, ora_af* a) { s->af.push_back(a); }
 void ora_sim_add_uni(ora_sim* s, ora_uni* u) { s->uni.push_back(u); }
 
@@ -1317,6 +1494,7 @@ int ora_sim_calc_fields(ora_sim* s, ora_sys* t)
     if (s->ani) ora_ani_apply(s->ani, t);
     if (s->lr) ora_lr_apply(s->lr, t);
     if (s->lr3) ora_lr3_apply(s->lr3, t);
+    if (s->lr2d) ora_lr2d_apply(s->lr2d, t, s->lr2d_slot.c_str());
     for (ora_uni* u : s->uni) ora_uni_apply(u, t);
     for (ora_af* a : s->af) ora_af_apply(a, t);
     ora_sys_sum_fields(t);

@@ -114,6 +114,18 @@ void     ora_lr3_set_strength(ora_lr3*, double g);
 int      ora_lr3_set_tensor(ora_lr3*, int which, const double* t);   /* 0..8 = XX XY XZ YX YY YZ ZX ZY ZZ */
 int      ora_lr3_apply(ora_lr3*, ora_sys*);
 
+/* LongRange2D: T [9][nz][nz][ny][nx] (XX..ZZ, labels [dest][src]); apply reads [src][dest] */
+typedef struct ora_lr2d ora_lr2d;
+ora_lr2d* ora_lr2d_new(int nx, int ny, int nz);
+void      ora_lr2d_free(ora_lr2d*);
+void      ora_lr2d_set_scale(ora_lr2d*, double s);
+void      ora_lr2d_set_layer_scales(ora_lr2d*, const double* src_moment, const double* dest_field);
+int       ora_lr2d_set_tensor(ora_lr2d*, int dest, int src, int ab, const double* a);
+int       ora_lr2d_get_tensor(ora_lr2d*, int dest, int src, int ab, double* a);
+int       ora_lr2d_apply(ora_lr2d*, ora_sys*, const char* slot);
+int       ora_dipole2d_tensor(int nx, int ny, int nz, const double* ABC, const double* off, const int trunc[4],
+                              int physical, double* T);
+
 ora_llg* ora_llg_new(void);
 void     ora_llg_free(ora_llg*);
 void     ora_llg_set_flags(ora_llg*, int thermal_only_first_term, int disable_renorm);
@@ -126,6 +138,7 @@ ora_sim* ora_sim_new(ora_sys* ss, ora_ex* ex, ora_ani* ani, ora_lr* lr, ora_th* 
 void     ora_sim_free(ora_sim*);
 void     ora_sim_set_seed(ora_sim*, uint64_t seed);
 void     ora_sim_set_lr3(ora_sim*, ora_lr3* lr3);
+void     ora_sim_set_lr2d(ora_sim*, ora_lr2d* lr2d, const char* slot);
 int      ora_sim_calc_fields(ora_sim*, ora_sys* target);
 int      ora_sim_heun(ora_sim*, int nsteps);                  /* one PC corrector pass per step */
 int      ora_sim_rk(ora_sim*, const char* name, int nsteps);  /* RK1, RK2, RK3, K3, RK4, RK4_38, RK6, BS3 */

@@ -112,6 +112,15 @@ def lib(threads=False):
         "ora_lr3_set_tensor": (C.c_int, [vp, C.c_int, _dp]),
         "ora_lr3_apply": (C.c_int, [vp, vp]),
         "ora_sim_set_lr3": (None, [vp, vp]),
+        "ora_lr2d_new": (vp, [C.c_int, C.c_int, C.c_int]),
+        "ora_lr2d_free": (None, [vp]),
+        "ora_lr2d_set_scale": (None, [vp, C.c_double]),
+        "ora_lr2d_set_layer_scales": (None, [vp, _dp, _dp]),
+        "ora_lr2d_set_tensor": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, _dp]),
+        "ora_lr2d_get_tensor": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, _dp]),
+        "ora_lr2d_apply": (C.c_int, [vp, vp, C.c_char_p]),
+        "ora_dipole2d_tensor": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp, C.POINTER(C.c_int), C.c_int, _dp]),
+        "ora_sim_set_lr2d": (None, [vp, vp, C.c_char_p]),
         "ora_llg_new": (vp, []),
         "ora_llg_free": (None, [vp]),
         "ora_llg_set_flags": (None, [vp, C.c_int, C.c_int]),
@@ -186,6 +195,18 @@ def dipole_tensor(nx, ny, nz, gmax=(1000, 1000, 1000, 1000), ABC=None, threads=T
     rc = L.ora_dipole_tensor(nx, ny, nz, g, A, *outs)
     assert rc == 0
     return {k: o.reshape(2 * nz - 1, ny, nx) for k, o in zip(("XX", "XY", "XZ", "YY", "YZ", "ZZ"), outs)}
+
+
+def dipole2d_tensor(nx, ny, nz, ABC=None, offset=None, trunc=(25, 25, 25, 25), physical=False, threads=True):
+    """Dipole2D makeData: array [9, nz, nz, ny, nx] (XX..ZZ, labels [dest][src]).
+    ABC: [nz][3][3] cell vectors per layer (identity), offset: [nz][3] (zeros)."""
+    L = lib(threads)
+    A = np.ascontiguousarray(np.tile(np.eye(3), (nz, 1, 1)) if ABC is None else np.asarray(ABC, dtype=np.float64)).reshape(-1)
+    off = np.ascontiguousarray(np.zeros((nz, 3)) if offset is None else np.asarray(offset, dtype=np.float64)).reshape(-1)
+    out = np.zeros(9 * nz * nz * ny * nx)
+    rc = L.ora_dipole2d_tensor(nx, ny, nz, A, off, (C.c_int * 4)(*trunc), int(physical), out)
+    assert rc == 0
+    return out.reshape(9, nz, nz, ny, nx)
 
 
 # ---------------------------------------------------------------- objects
@@ -513,6 +534,39 @@ class LongRange3D:
         self.L.ora_lr3_apply(self.h, ss.h)
 
 
+class LongRange2D:
+    """A tensor per ordered layer pair (labels [dest][src], read as [src][dest] by apply),
+    per-layer source-moment and destination-field scales, an operator scale."""
+    NAMES = ("XX", "XY", "XZ", "YX", "YY", "YZ", "ZX", "ZY", "ZZ")
+
+    def __init__(self, nx, ny=1, nz=1, slot="LongRange2D", threads=False):
+        self.L = lib(threads)
+        self.nx, self.ny, self.nz, self.slot = nx, ny, nz, slot
+        self.h = self.L.ora_lr2d_new(nx, ny, nz)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.ora_lr2d_free(self.h)
+
+    def setScale(self, s):
+        self.L.ora_lr2d_set_scale(self.h, s)
+
+    def setLayerScales(self, src_moment, dest_field):
+        self.L.ora_lr2d_set_layer_scales(self.h, np.ascontiguousarray(src_moment, dtype=np.float64),
+                                         np.ascontiguousarray(dest_field, dtype=np.float64))
+
+    def setTensors(self, T):
+        """T: [9, nz, nz, ny, nx]"""
+        T = np.asarray(T, dtype=np.float64)
+        for ab in range(9):
+            for d in range(self.nz):
+                for s_ in range(self.nz):
+                    assert self.L.ora_lr2d_set_tensor(self.h, d, s_, ab, np.ascontiguousarray(T[ab, d, s_]).reshape(-1)) == 0
+
+    def apply(self, ss):
+        self.L.ora_lr2d_apply(self.h, ss.h, self.slot.encode())
+
+
 class LLG:
     def __init__(self, threads=False, quaternion=False):
         self.L = lib(threads)
@@ -559,6 +613,10 @@ class Sim:
     def setLongRange3D(self, lr3):
         self.keep3 = lr3
         self.L.ora_sim_set_lr3(self.h, lr3.h if lr3 is not None else None)
+
+    def setLongRange2D(self, lr2d):
+        self.keep2 = lr2d
+        self.L.ora_sim_set_lr2d(self.h, lr2d.h if lr2d is not None else None, (lr2d.slot if lr2d else "").encode())
 
     def calcFields(self, ss):
         self.L.ora_sim_calc_fields(self.h, ss.h)

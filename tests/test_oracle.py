@@ -599,3 +599,94 @@ def test_direct_open_dipole_sum_equals_padded_convolution():
     d1 = O.dipole_direct_open(dims, *one, np.array([3]), 0.3)
     assert np.allclose(d1, [[0.3 * 2 / 27, 0, 0]], rtol=1e-14, atol=1e-16)
 
+
+
+# ---------------------------------------------------------- LongRange2D / Dipole2D
+def _lr2d_direct(T, sms, dfs, scale, m):
+    """Real-space circular convolution with the pair read as [src][dest]:
+    h_a[d](n) = scale dfs[d] sum_s sms[s] sum_b sum_j T_ab[s][d](n - j) m_b[s](j)."""
+    _, nz, _, ny, nx = T.shape
+    h = np.zeros((3, nz, ny, nx))
+    for a in range(3):
+        for d in range(nz):
+            for s in range(nz):
+                for b in range(3):
+                    K = T[3 * a + b, s, d]
+                    for jy in range(ny):
+                        for jx in range(nx):
+                            h[a, d] += sms[s] * np.roll(np.roll(K, jy, axis=0), jx, axis=1) * m[b, s, jy, jx]
+            h[a, d] *= scale * dfs[d]
+    return h
+
+
+def test_lr2d_apply_equals_direct_convolution():
+    nx, ny, nz = 6, 4, 3
+    rng = np.random.default_rng(11)
+    T = rng.standard_normal((9, nz, nz, ny, nx))
+    sms, dfs, scale = rng.uniform(0.5, 2, nz), rng.uniform(0.5, 2, nz), 0.7
+    lr = O.LongRange2D(nx, ny, nz)
+    lr.setTensors(T)
+    lr.setLayerScales(sms, dfs)
+    lr.setScale(scale)
+    ss = O.SpinSystem(nx, ny, nz)
+    m = rng.standard_normal((3, nz, ny, nx))
+    ss.setSpins(*(m[c].reshape(-1) for c in range(3)))
+    lr.apply(ss)
+    h = np.stack([np.asarray(v).reshape(nz, ny, nx) for v in ss.field("LongRange2D")])
+    want = _lr2d_direct(T, sms, dfs, scale, m)
+    assert np.abs(h - want).max() <= 1e-12 * np.abs(want).max()
+    # the pair is read as [src][dest]: the [dest][src] reading gives another field
+    wrong = _lr2d_direct(T.transpose(0, 2, 1, 3, 4), sms, dfs, scale, m)
+    assert np.abs(h - wrong).max() > 1e-3 * np.abs(want).max()
+
+
+def _point_dipole(r):
+    """Textbook dipole coupling (3 r r^T - |r|^2 I) / |r|^5, rows a, columns b."""
+    r = np.asarray(r, dtype=np.float64)
+    d = np.sqrt(r @ r)
+    return (3 * np.outer(r, r) - d * d * np.eye(3)) / d ** 5
+
+
+def test_dipole2d_reference_form_closed_form():
+    """The reference's makeData evaluates source and destination at the same in-plane cell:
+    within a layer every element is 0, between layers each cell of the wrapped box holds
+    (box count) x the coupling of the two stacked sites, and pairs beyond truncZ are 0."""
+    nx, ny, nz = 8, 6, 3
+    ABC = np.tile(np.eye(3), (nz, 1, 1))
+    ABC[1, 2] = (0.1, 0.0, 1.5)              # the C vector of layer 2 lifts layer 3
+    off = np.array([[0, 0, 0], [0.3, -0.2, 0.1], [0, 0, 0]], dtype=np.float64)
+    tX, tY, tZ = 5, 7, 1
+    T = O.dipole2d_tensor(nx, ny, nz, ABC, off, (25, tX, tY, tZ))
+    cx = np.array([sum(1 for X in range(-tX, tX + 1) if X % nx == x) for x in range(nx)])
+    cy = np.array([sum(1 for Y in range(-tY, tY + 1) if Y % ny == y) for y in range(ny)])
+    count = np.outer(cy, cx)                 # [y][x]
+    zsum = np.cumsum(np.vstack([np.zeros(3), ABC[:-1, 2]]), axis=0)   # C_1 + .. + C_{l-1}
+    for d in range(nz):
+        for s in range(nz):
+            blk = T[:, d, s]
+            if d == s or abs(d - s) > tZ:
+                assert np.abs(blk).max() == 0.0
+                continue
+            N = _point_dipole(zsum[s] + off[s] - zsum[d] - off[d]).reshape(-1)
+            want = N[:, None, None] * count[None]
+            assert np.allclose(blk, want, rtol=1e-13, atol=0)
+    # stacked unit spacing: Nzz = 2, Nxx = Nyy = -1 (times the count)
+    T1 = O.dipole2d_tensor(4, 4, 2, trunc=(25, 0, 0, 25))
+    assert np.allclose(T1[:, 0, 1, 0, 0], [-1, 0, 0, 0, -1, 0, 0, 0, 2])
+
+
+def test_dipole2d_physical_form_equals_lattice_sum():
+    """With the destination at the origin the box sum is the layered Dipole's lattice sum
+    over the same displacements: T[i][j] = Dipole tensor at layer offset j - i."""
+    nx, ny, nz, tX, tY = 8, 8, 3, 11, 9
+    T = O.dipole2d_tensor(nx, ny, nz, trunc=(0, tX, tY, nz), physical=True)
+    D = O.dipole_tensor(nx, ny, nz, gmax=(10 ** 6, 0, tX, tY))
+    sym = ("XX", "XY", "XZ", "XY", "YY", "YZ", "XZ", "YZ", "ZZ")
+    err = 0.0
+    for i in range(nz):
+        for j in range(nz):
+            for ab in range(9):
+                err = max(err, float(np.abs(T[ab, i, j] - D[sym[ab]][j - i + nz - 1]).max()))
+    assert err <= 1e-13
+    # and the reference form differs from it (its in-plane terms cancel)
+    assert np.abs(O.dipole2d_tensor(nx, ny, nz, trunc=(0, tX, tY, nz)) - T).max() > 0.1
