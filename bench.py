@@ -22,11 +22,6 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X published HBM3E bandwidth
 
-# HBM bytes per dispatch of each fused kernel on this workload (compensated state),
-# from two rocprofv3 counter runs: FETCH_SIZE x 1.9925 + WRITE_SIZE x 1.000, the
-# factors being what the counters read for a device copy of known size
-# (profiles/heun_film256x256x8_kernels.md has the commands and per-kernel figures).
-MEASURED_TRAFFIC = {"k_cols": 19.30e6, "k_step_pred": 47.44e6, "k_step_corr": 47.48e6}
 
 
 def kernel_bytes(spec, has_cls, comp):
@@ -101,6 +96,82 @@ def cpu_baseline(spec, budget_s=15.0, omp_budget_s=8.0):
             "cpu_model": _cpu_model(),
             "openmp": {"value": vt, "unit": "spin-steps/s", "cores": threads,
                        "sample": f"{st} Heun steps, same build with OpenMP, {et:.1f} s"}}
+
+
+def measure_traffic(film, schedule, kernel, comp, timeout_s=150):
+    """HBM bytes per dispatch of `kernel`, measured in this run: two rocprofv3 counter
+    passes (FETCH_SIZE, then WRITE_SIZE; the two do not fit one pass) over a child
+    probe of the same workload and schedule (`--traffic-probe`: a few graph-replayed
+    steps, then one device copy of the spins into a second system: 16 bytes per site,
+    and 12 more of low words with the compensated state).  Each counter's KiB per
+    dispatch is scaled by what it reads for those copies of known size.  Returns
+    (bytes, detail) or (None, reason)."""
+    from maglua_amd import workloads as W
+    spec = W.FILM_256_T05 if film == "256x256x8" else W.FILM_512
+    ncopy = 2 if comp else 1
+    copy_kib = spec.n * (16 + (12 if comp else 0)) / 1024.0
+    import csv
+    import glob
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3") or ("/opt/rocm/bin/rocprofv3" if os.path.exists("/opt/rocm/bin/rocprofv3") else None)
+    if prof is None:
+        return None, "rocprofv3 not found"
+    env = dict(os.environ, TMPDIR="/tmp",
+               MSD_ROW_TILE=f"{schedule['rows_per_tile']}:{schedule['threads']}:{schedule['split']}")
+    got = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            cmd = [prof, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", ctr, "--", sys.executable,
+                   os.path.abspath(__file__), "--traffic-probe", "--film", film] + ([] if comp else ["--plain"])
+            p = subprocess.Popen(cmd, cwd=ROOT, env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
+                                 start_new_session=True)
+            try:
+                _, err = p.communicate(timeout=timeout_s)
+            except subprocess.TimeoutExpired:
+                os.killpg(p.pid, signal.SIGKILL)
+                p.communicate()
+                return None, f"{ctr} pass timed out"
+            files = glob.glob(os.path.join(d, "**", f"{ctr}_counter_collection.csv"), recursive=True)
+            if not files:
+                return None, f"{ctr} pass gave no counters (rc {p.returncode}): " + err.decode(errors="replace")[-300:]
+            kern, copies = [], []
+            for fn in files:
+                for r in csv.DictReader(open(fn)):
+                    if r["Counter_Name"] != ctr:
+                        continue
+                    name = r["Kernel_Name"]
+                    if f"{kernel}<" in name and "batch" not in name:
+                        kern.append(float(r["Counter_Value"]))
+                    elif "copyBuffer" in name:
+                        copies.append(float(r["Counter_Value"]))
+            if not kern or len(copies) < ncopy or sum(copies[-ncopy:]) <= 0:
+                return None, f"{ctr}: {len(kern)} {kernel} dispatches, {len(copies)} copies in the counters"
+            factor = copy_kib / sum(copies[-ncopy:])
+            got[ctr] = (sum(kern) / len(kern), factor)
+    fetch = got["FETCH_SIZE"][0] * got["FETCH_SIZE"][1] * 1024
+    write = got["WRITE_SIZE"][0] * got["WRITE_SIZE"][1] * 1024
+    return fetch + write, {"fetch_kib": round(got["FETCH_SIZE"][0], 1), "fetch_factor": round(got["FETCH_SIZE"][1], 4),
+                           "write_kib": round(got["WRITE_SIZE"][0], 1), "write_factor": round(got["WRITE_SIZE"][1], 4)}
+
+
+def traffic_probe(args):
+    """The child run of measure_traffic: the workload in the schedule MSD_ROW_TILE names,
+    a few graph-replayed steps, then the calibration copies."""
+    import maglua_amd as M
+    from maglua_amd import workloads as W
+    spec = W.FILM_256_T05 if args.film == "256x256x8" else W.FILM_512
+    g = W.build(M, spec)
+    g["ss"].randomize(spec.seed)
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"], compensated=not args.plain)
+    other = M.SpinSystem(spec.nx, spec.ny, spec.nz)
+    heun.step(6)
+    M.synchronize()
+    g["ss"].copySpinsTo(other)
+    M.synchronize()
 
 
 BF16_DENSE_PEAK = 2.5e15   # MI355X published dense BF16 MFMA FLOP/s
@@ -463,13 +534,20 @@ def main():
     ap.add_argument("--train-pairs", type=int, default=20000, help="surrogate training pairs for the learned leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched-replica and 256^3 open-cube legs")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--tune-steps", type=int, default=100,
+                    help="steps the autotuner times per candidate schedule before the run (0: default schedule)")
     ap.add_argument("--plain", action="store_true", help="plain FP32 spin state instead of the compensated pair")
     ap.add_argument("--film", choices=("256x256x8", "512x512x4"), default="256x256x8",
                     help="the plain workload: the 256x256x8 film (T=0.5) or the 512x512x4 film (g=0.1, T=0.2)")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes behind roofline.traffic")
     ap.add_argument("--calibrate", action="store_true",
                     help="after timing, copy the spin system once on the device (two copy kernels of 8 MiB and "
                          "6 MiB) so a counter run can scale FETCH_SIZE / WRITE_SIZE by a copy of known size")
     args = ap.parse_args()
+    if args.traffic_probe:
+        traffic_probe(args)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "replicas would shard with no communication, see the design notes", file=sys.stderr)
@@ -484,6 +562,8 @@ def main():
     g["ss"].randomize(spec.seed)
     heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"], graph=not args.no_graph,
                   compensated=not args.plain)
+    # the autotuner picks the row-tile shape and the y-transform schedule (state restored)
+    schedule = heun.autotune(steps=args.tune_steps) if args.tune_steps > 0 else g["dip"].schedule()
     heun.step(args.warmup)
     M.synchronize()
     print(f"# setup + {args.warmup} warm-up steps: {time.perf_counter() - t_setup:.2f} s", file=sys.stderr, flush=True)
@@ -508,9 +588,14 @@ def main():
     per_step = {k: v * (2 if k == "k_cols" else 1) for k, v in prof.items()}
     dom = max(per_step, key=per_step.get)
     achieved = kb[dom] / (prof[dom] * 1e-3) / 1e9
+    traffic, tdetail = (None, "not measured (--no-traffic)")
+    if not args.no_traffic:
+        print("# roofline.traffic: two rocprofv3 counter passes over a probe run...", file=sys.stderr, flush=True)
+        traffic, tdetail = measure_traffic(args.film, schedule, dom, not args.plain)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None if args.plain else MEASURED_TRAFFIC.get(dom),
+            "traffic": None if traffic is None else round(traffic),
+            "traffic_counters": tdetail,
             "kernel": dom, "kernel_avg_ms": round(prof[dom], 5), "algorithmic_bytes": kb[dom],
             "step_model_bytes_per_spin": round(step_model_bytes(spec, not args.plain), 2),
             "step_model_GBs": round(step_model_bytes(spec, not args.plain) * value / 1e9, 1),
@@ -548,7 +633,8 @@ def main():
         "data": "synthetic (seeded sphere-uniform spins, counter-based thermal noise)",
         "config": {"workload": spec.name() + ", layered FFT dipole, Heun, plain form",
                    "lattice": [spec.nx, spec.ny, spec.nz], "spins": spec.n, "graph": not args.no_graph,
-                   "spin_state": "fp32" if args.plain else "fp32 hi+lo (compensated)"},
+                   "spin_state": "fp32" if args.plain else "fp32 hi+lo (compensated)",
+                   "schedule": {k: schedule[k] for k in ("rows_per_tile", "threads", "split")}},
         "roofline": roof,
         "cpu_baseline": cpu,
         "learned": learned,

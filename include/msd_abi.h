@@ -253,6 +253,10 @@ int msd_dipole_set_matrix(msd_dipole* d, int which, int ox, int oy, int oz, doub
  * Also LongRange2D:apply (cpu/longrange2d/spinoperationlongrange2d.cpp:448-531) for an
  * object made by msd_lr2d_create, into its own slot. */
 int msd_dipole_apply(msd_dipole* d, msd_system* s);
+/* No counterpart in the reference: the transform schedule in use -- rows per row
+ * tile, threads per row workgroup, and split = rows per tile when the row kernels
+ * take the first radix-ty stage of the y transform (0: full-length columns). */
+int msd_dipole_schedule(const msd_dipole* d, int* rows_per_tile, int* threads, int* split);
 
 /* ------------------------------------------------------------- LongRange2D */
 /* LongRange2D.new / Dipole2D.new (cpu/longrange2d/spinoperationlongrange2d.cpp:26-48,
@@ -377,9 +381,11 @@ int msd_stepper_profile(msd_stepper* st, int nsteps, int cap, char names[][32], 
  * Members must outlive the ensemble. */
 /* Autotuner (no counterpart in the reference): times `steps` steps of the layered
  * driver for every compiled row-tile shape (rows per tile, threads per workgroup)
- * and keeps the fastest; the spin state, time and thermal draw are restored.
+ * in both y-transform schedules -- split (the row kernels take the first radix-ty
+ * stage across their tile's rows, best_split = ty) and full columns (best_split = 0)
+ * -- and keeps the fastest; the spin state, time and thermal draw are restored.
  * Returns the chosen shape and its ms per step (-1 when only runtime shapes exist). */
-int msd_stepper_autotune(msd_stepper* st, int steps, int* best_ty, int* best_tpb, double* best_ms);
+int msd_stepper_autotune(msd_stepper* st, int steps, int* best_ty, int* best_tpb, int* best_split, double* best_ms);
 
 /* ---------------------------------------------------------------- surrogate
  * No counterpart in the reference: a learned predictor of the dipole field's

@@ -129,6 +129,7 @@ _SIGS = {
     "msd_dipole_set_strength": (C.c_int, [_vp, C.c_double]),
     "msd_dipole_set_scale": (C.c_int, [_vp, C.c_double]),
     "msd_dipole_set_truncation": (C.c_int, [_vp, _ip]),
+    "msd_dipole_schedule": (C.c_int, [_vp, _ip, _ip, _ip]),
     "msd_dipole_set_unit_cell": (C.c_int, [_vp, _dp]),
     "msd_dipole_get_matrix": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
     "msd_dipole_set_matrix": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]),
@@ -167,7 +168,7 @@ _SIGS = {
     "msd_stepper_heun": (C.c_int, [_vp, C.c_int]),
     "msd_stepper_profile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_void_p, _dp, _ip]),
     "msd_stepper_set_surrogate": (C.c_int, [_vp, _vp, C.c_int, C.c_double]),
-    "msd_stepper_autotune": (C.c_int, [_vp, C.c_int, _ip, _ip, _dp]),
+    "msd_stepper_autotune": (C.c_int, [_vp, C.c_int, _ip, _ip, _ip, _dp]),
     "msd_stepper_rk": (C.c_int, [_vp, C.c_char_p, C.c_int]),
     "msd_stepper_graph_builds": (C.c_int, [_vp, _lp]),
     "msd_stepper_surrogate_stats": (C.c_int, [_vp, _lp, _lp, _lp, _dp, _dp]),

@@ -28,6 +28,11 @@ static constexpr int ROW_TPB = 1024;
 // every thread takes part in the per-k product; measured +0.9% whole-step at
 // 256x256x8), 512 for longer columns (measured +1.1% at 512x512x4) and runtime lengths.
 __host__ __device__ constexpr int col_tpb(int ny) { return ny > 0 && ny <= 256 ? 256 : 512; }
+// Waves per SIMD the fused step kernels are compiled to allow: 4, a budget of 128
+// VGPRs.  Workgroups of 512 threads then fit two to a CU (without the hint the
+// corrector at 512 threads took more registers, one workgroup per CU and two rounds
+// of tiles: 28 against 20 us at 256x256x8); at 1024 threads a budget of 64 spills.
+__host__ __device__ constexpr int row_waves(int) { return 4; }
 // exchange class tables up to this size are staged in LDS by the fused kernels
 static constexpr int EX_LDS_CLS = 64, EX_LDS_ENT = 512;
 
@@ -107,6 +112,147 @@ __device__ __forceinline__ void rows_unpack_spectrum(const float2* __restrict__ 
     }
 }
 
+// Row r of a row tile lies at y = y0 + ys * r of layer z: consecutive rows
+// (y0 = yb ty, ys = 1), or with a split y transform the rows b + ly a (y0 = b = yb,
+// ys = ly).
+struct RowTile {
+    int z, yb, y0, ys;
+};
+
+__device__ __forceinline__ RowTile row_tile(const DipoleDev& dp, int tile, int nz, int ty)
+{
+    RowTile t;
+    t.yb = tile / nz;
+    t.z = tile - t.yb * nz;
+    if (dp.split) { t.y0 = t.yb; t.ys = dp.ly; }
+    else { t.y0 = t.yb * ty; t.ys = 1; }
+    return t;
+}
+
+template <int R, int SIGN> __device__ __forceinline__ void dft_r(float2* v)
+{
+    if constexpr (R == 2) dft2<SIGN>(v);
+    else if constexpr (R == 4) dft4<SIGN>(v);
+    else dft8<SIGN>(v);
+}
+
+__host__ __device__ constexpr bool split_rows(int ty) { return ty == 2 || ty == 4 || ty == 8; }
+
+// Staging area of the split row passes: the tile's 3 (nx/2+1) runs of TY words,
+// after the packed lines and the x twiddles.  Global reads and writes of the runs
+// go through it as whole float4s spread over all threads (coalesced), and one
+// thread per (c, kx) works on its run in LDS.
+template <int NX, int TY>
+__device__ __forceinline__ float4* split_stage(float2* lds)
+{
+    return reinterpret_cast<float4*>(lds + (size_t)row_lines(TY) * NX + NX);
+}
+
+// Split y transform, inverse side: for every (component c, kx <= nx/2) the TY words
+// Z_b(k1) of this tile (b = yb) become the tile's TY rows at kx,
+//   x(b + ly a) = sum_k1 exp(+2 pi i a k1 / TY) exp(+2 pi i b k1 / ny) Z_b(k1),
+// packed two rows to a complex line as rows_pack_spectrum does.
+template <int NT, int NX, int TY>
+__device__ __forceinline__ void rows_pack_split(const DipoleDev& dp, const float2* __restrict__ spec, int nz,
+                                                const RowTile& t, float2* __restrict__ lds)
+{
+    constexpr int NXH = NX / 2 + 1, HALF = NX / 2, ITEMS = 3 * NXH, CH = TY / 2;
+    const int ny = dp.ny;
+    float4* stg = split_stage<NX, TY>(lds);
+    const float4* __restrict__ src = reinterpret_cast<const float4*>(spec);
+    batched_copy<ITEMS * CH, NT, 4>(
+        [&](int idx) {
+            const int item = idx / CH, h = idx - item * CH;
+            const int c = item / NXH, kx = item - c * NXH;
+            return src[((((size_t)(c * nz + t.z) * NXH + kx) * ny + t.yb * TY) >> 1) + h];
+        },
+        [&](int idx, float4 v) { stg[idx] = v; });
+    float2 tw[TY];
+#pragma unroll
+    for (int k = 1; k < TY; ++k) tw[k] = dp.twy[t.yb * k];
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < ITEMS; idx += NT) {
+        const int c = idx / NXH, kx = idx - c * NXH;
+        float2 v[TY];
+#pragma unroll
+        for (int h = 0; h < CH; ++h) {
+            const float4 q = stg[idx * CH + h];
+            v[2 * h] = make_float2(q.x, q.y);
+            v[2 * h + 1] = make_float2(q.z, q.w);
+        }
+#pragma unroll
+        for (int k = 1; k < TY; ++k) v[k] = cmulc(v[k], tw[k]);
+        dft_r<TY, 1>(v);
+#pragma unroll
+        for (int j = 0; j < TY / 2; ++j) {
+            const float2 A = v[2 * j], B = v[2 * j + 1];
+            float2* line = lds + (size_t)(c * (TY / 2) + j) * NX;
+            line[kx] = make_float2(A.x - B.y, A.y + B.x);
+            if (kx > 0 && kx < HALF) line[NX - kx] = make_float2(A.x + B.y, B.x - A.y);
+        }
+    }
+}
+
+// Split y transform, forward side: the tile's TY transformed rows at kx become
+//   Z_b(k1) = exp(-2 pi i b k1 / ny) sum_a exp(-2 pi i a k1 / TY) X_{b + ly a}(kx),
+// stored as one run of TY words per (c, kx).  Caller synchronised before.
+template <int NT, int NX, int TY>
+__device__ __forceinline__ void rows_unpack_split(const DipoleDev& dp, const float2* __restrict__ lds, int nz,
+                                                  const RowTile& t, float2* __restrict__ spec)
+{
+    constexpr int NXH = NX / 2 + 1, ITEMS = 3 * NXH, CH = TY / 2;
+    const int ny = dp.ny;
+    float4* stg = split_stage<NX, TY>(const_cast<float2*>(lds));
+    float2 tw[TY];
+#pragma unroll
+    for (int k = 1; k < TY; ++k) tw[k] = dp.twy[t.yb * k];
+    for (int idx = threadIdx.x; idx < ITEMS; idx += NT) {
+        const int c = idx / NXH, kx = idx - c * NXH;
+        float2 v[TY];
+#pragma unroll
+        for (int j = 0; j < TY / 2; ++j) {
+            const float2* line = lds + (size_t)(c * (TY / 2) + j) * NX;
+            const float2 Zk = line[kx], Zm = line[kx ? NX - kx : 0];
+            v[2 * j] = make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y));
+            v[2 * j + 1] = make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x));
+        }
+        dft_r<TY, -1>(v);
+#pragma unroll
+        for (int k = 1; k < TY; ++k) v[k] = cmul(v[k], tw[k]);
+#pragma unroll
+        for (int h = 0; h < CH; ++h) stg[idx * CH + h] = make_float4(v[2 * h].x, v[2 * h].y, v[2 * h + 1].x, v[2 * h + 1].y);
+    }
+    __syncthreads();
+    float4* __restrict__ dst = reinterpret_cast<float4*>(spec);
+    for (int idx = threadIdx.x; idx < ITEMS * CH; idx += NT) {
+        const int item = idx / CH, h = idx - item * CH;
+        const int c = item / NXH, kx = item - c * NXH;
+        dst[((((size_t)(c * nz + t.z) * NXH + kx) * ny + t.yb * TY) >> 1) + h] = stg[idx];
+    }
+}
+
+// The field spectrum into the tile's packed LDS lines (before the inverse x transform)
+template <int NT, int NX, int TY>
+__device__ __forceinline__ void rows_in(const DipoleDev& dp, const float2* __restrict__ spec, int nx, int ny, int nz,
+                                        int ty, const RowTile& t, float2* __restrict__ lds)
+{
+    if constexpr (NX > 0 && split_rows(TY)) {
+        if (dp.split) { rows_pack_split<NT, NX, TY>(dp, spec, nz, t, lds); return; }
+    }
+    rows_pack_spectrum<NT, NX, TY>(spec, nx, ny, nz, ty, t.z, t.y0, lds);
+}
+
+// The tile's forward-transformed LDS lines into the spin spectrum
+template <int NT, int NX, int TY>
+__device__ __forceinline__ void rows_out(const DipoleDev& dp, const float2* __restrict__ lds, int nx, int ny, int nz,
+                                         int ty, const RowTile& t, float2* __restrict__ spec)
+{
+    if constexpr (NX > 0 && split_rows(TY)) {
+        if (dp.split) { rows_unpack_split<NT, NX, TY>(dp, lds, nz, t, spec); return; }
+    }
+    rows_unpack_spectrum<NT>(lds, nx, ny, nz, ty, t.z, t.y0, spec);
+}
+
 // Copy a small exchange class table into LDS so the per-site lookups cost no
 // global round trip; larger tables stay in global memory.
 __device__ __forceinline__ ExchangeDev stage_exchange(const ExchangeDev& ex, int* s_start, ExEntry* s_ent)
@@ -151,8 +297,7 @@ __global__ void __launch_bounds__(TPB) k_rows_fwd(DipoleDev dp, const float* __r
     extern __shared__ float2 lds[];
     const int nx = NX ? NX : dp.nx, ty = TY ? TY : dp.ty, lines = row_lines(ty), ny = dp.ny;
     // layer fastest: an XCD's run of tiles is a band of rows through every layer
-    const int tile = xcd_remap(blockIdx.x, gridDim.x);
-    const int yb = tile / dp.nz, z = tile - yb * dp.nz, y0 = yb * ty;
+    const RowTile t = row_tile(dp, xcd_remap(blockIdx.x, gridDim.x), dp.nz, ty);
     const int n = nx * ny * dp.nz;
     float2* A = lds;
     float2* B = lds + (size_t)lines * nx;
@@ -162,14 +307,14 @@ __global__ void __launch_bounds__(TPB) k_rows_fwd(DipoleDev dp, const float* __r
     auto ld = [&](int li) {
         const int q = li / nx, x = li - q * nx;
         const int c = q / ty, r = q - c * ty;
-        return s[(size_t)c * n + x + (size_t)nx * (y0 + r + (size_t)ny * z)];
+        return s[(size_t)c * n + x + (size_t)nx * (t.y0 + t.ys * r + (size_t)ny * t.z)];
     };
     auto st = [&](int li, float v) { const int q = li / nx; rowval(A, q, li - q * nx, nx) = v; };
     if constexpr (NX > 0) batched_copy<3 * TY * NX, TPB, 8>(ld, st);
     else for (int li = threadIdx.x; li < 3 * ty * nx; li += TPB) st(li, ld(li));
     __syncthreads();
     const float2* R = row_fft<NX, TY, TPB, -1>(A, B, dp, tw);
-    rows_unpack_spectrum<TPB>(R, nx, ny, dp.nz, ty, z, y0, dp.Mh);
+    rows_out<TPB, NX, TY>(dp, R, nx, ny, dp.nz, ty, t, dp.Mh);
 }
 
 // inverse along x: Hh -> real field rows, written to slot h
@@ -179,19 +324,18 @@ __global__ void __launch_bounds__(TPB) k_rows_inv(DipoleDev dp, float* __restric
     extern __shared__ float2 lds[];
     const int nx = NX ? NX : dp.nx, ty = TY ? TY : dp.ty, lines = row_lines(ty), ny = dp.ny;
     // layer fastest: an XCD's run of tiles is a band of rows through every layer
-    const int tile = xcd_remap(blockIdx.x, gridDim.x);
-    const int yb = tile / dp.nz, z = tile - yb * dp.nz, y0 = yb * ty;
+    const RowTile t = row_tile(dp, xcd_remap(blockIdx.x, gridDim.x), dp.nz, ty);
     const int n = nx * ny * dp.nz;
     float2* A = lds;
     float2* B = lds + (size_t)lines * nx;
     const float2* tw = row_twiddles<NX>(dp, A + (size_t)lines * nx);
-    rows_pack_spectrum<TPB, NX, TY>(dp.Hh, nx, ny, dp.nz, ty, z, y0, A);
+    rows_in<TPB, NX, TY>(dp, dp.Hh, nx, ny, dp.nz, ty, t, A);
     __syncthreads();
     float2* R = row_fft<NX, TY, TPB, 1>(A, B, dp, tw);
     for (int li = threadIdx.x; li < 3 * ty * nx; li += TPB) {
         const int q = li / nx, x = li - q * nx;
         const int c = q / ty, r = q - c * ty;
-        h[(size_t)c * n + x + (size_t)nx * (y0 + r + (size_t)ny * z)] = rowval(R, q, x, nx);
+        h[(size_t)c * n + x + (size_t)nx * (t.y0 + t.ys * r + (size_t)ny * t.z)] = rowval(R, q, x, nx);
     }
 }
 
@@ -362,6 +506,209 @@ __global__ void __launch_bounds__(col_tpb(NY)) k_cols_batch(const DipoleDev* __r
     cols_body<NY, NZ>(dps[r], w - r * per);
 }
 
+// ------------------------------------------------------------ split column pass
+// With a split y transform (dp.split = R, ny = R ly) the column work of one kx falls
+// into R independent problems, k1 = 0..R-1: the 3 nz lines b = 0..ly-1 (stride R in
+// the spectrum) transform forward to k2 (ky = k1 + R k2), take the per-k
+// layer/component tensor product for all three output components, and transform
+// back.  Workgroup (kx, k1): nxh R workgroups, no transform done twice.
+//
+// LDS holds the lines position-major, element b of line l at A[b PS + l] with an odd
+// pitch PS >= 3 nz: the transform stages give consecutive lanes consecutive lines and
+// the product gives them consecutive k2, so neither meets a bank conflict.
+//
+// Product lanes: wave w forms output component w / (ly / 64) (one wave per component
+// below ly = 64, lanes from ly on idle), so the branch on the component's odd words
+// (XZ, YZ: the third word of x and y, the first two of z) is wave-uniform.
+__host__ __device__ constexpr int split_pitch(int nz) { return (3 * nz) | 1; }
+__host__ __device__ constexpr int split_tpb(int L) { return L < 64 ? 192 : 3 * L; }
+
+// One in-place Stockham stage over LN position-major lines (pitch PS), NT threads
+// with the line index fastest across lanes.
+template <int L, int LN, int PS, int NT, int SIGN, int NS, int R>
+__device__ __forceinline__ void split_stage(float2* __restrict__ A, const float2* __restrict__ tw)
+{
+    constexpr int nb = L / R, total = LN * nb, iters = (total + NT - 1) / NT, tstep = L / (NS * R);
+    float2 v[iters][R];
+#pragma unroll
+    for (int it = 0; it < iters; ++it) {
+        const int b = (int)threadIdx.x + it * NT;
+        if (total % NT == 0 || b < total) {
+            const int l = b % LN, j = b / LN;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[it][r] = A[(j + r * nb) * PS + l];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < iters; ++it) {
+        const int b = (int)threadIdx.x + it * NT;
+        if (total % NT == 0 || b < total) {
+            const int l = b % LN, j = b / LN, k = j % NS;
+            if (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    const float2 w = tw[k * r * tstep];
+                    v[it][r] = SIGN < 0 ? cmul(v[it][r], w) : cmulc(v[it][r], w);
+                }
+            }
+            dft_r<R, SIGN>(v[it]);
+            const int o = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) A[(o + q * NS) * PS + l] = v[it][q];
+        }
+    }
+    __syncthreads();
+}
+
+template <int L, int LN, int PS, int NT, int SIGN, int NS>
+struct SplitFFT {
+    static __device__ __forceinline__ void run(float2* A, const float2* tw)
+    {
+        constexpr int R = (L / NS) % 8 == 0 ? 8 : ((L / NS) % 4 == 0 ? 4 : 2);
+        split_stage<L, LN, PS, NT, SIGN, NS, R>(A, tw);
+        SplitFFT<L, LN, PS, NT, SIGN, NS * R>::run(A, tw);
+    }
+};
+template <int L, int LN, int PS, int NT, int SIGN>
+struct SplitFFT<L, LN, PS, NT, SIGN, L> {
+    static __device__ __forceinline__ void run(float2*, const float2*) {}
+};
+
+template <int L, int NZ>
+__device__ __forceinline__ void split_cols_body(const DipoleDev& dp, int wg)
+{
+    constexpr int NT = split_tpb(L), LN = 3 * NZ, PS = split_pitch(NZ);
+    extern __shared__ float2 lds[];
+    const int R = dp.split, ny = dp.ny, nxh = dp.nxh;
+    const int kx = wg / R, k1 = wg - kx * R;
+    float2* A = lds;                                  // A[b PS + l]
+    float2* twl = A + L * PS;                         // exp(-2 pi i k / L)
+    float* wl = reinterpret_cast<float*>(twl + L);    // words [6][NZ][L]
+    const size_t lst = (size_t)nxh * ny;              // spectrum stride between lines
+    const float2* __restrict__ src = dp.Mh + (size_t)kx * ny + k1;
+    {
+        constexpr int MN = LN * L, MI = (MN + NT - 1) / NT;
+        constexpr int WN = 6 * NZ * L / 4, WI = (WN + NT - 1) / NT;
+        const float4* __restrict__ wsrc = reinterpret_cast<const float4*>(dp.Tw + (size_t)wg * (6 * NZ * L));
+        float2 mv[MI];
+        float4 wv[WI];
+        float2 tv = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < MI; ++k) {
+            const int idx = (int)threadIdx.x + k * NT;
+            if (MN % NT == 0 || idx < MN) {
+                const int l = idx / L, b = idx - l * L;
+                mv[k] = src[l * lst + (size_t)b * R];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < WI; ++k) {
+            const int idx = (int)threadIdx.x + k * NT;
+            if (WN % NT == 0 || idx < WN) wv[k] = wsrc[idx];
+        }
+        if ((int)threadIdx.x < L) tv = dp.twy[threadIdx.x * R];
+#pragma unroll
+        for (int k = 0; k < MI; ++k) {
+            const int idx = (int)threadIdx.x + k * NT;
+            if (MN % NT == 0 || idx < MN) {
+                const int l = idx / L, b = idx - l * L;
+                A[b * PS + l] = mv[k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < WI; ++k) {
+            const int idx = (int)threadIdx.x + k * NT;
+            if (WN % NT == 0 || idx < WN) reinterpret_cast<float4*>(wl)[idx] = wv[k];
+        }
+        if ((int)threadIdx.x < L) twl[threadIdx.x] = tv;
+    }
+    __syncthreads();
+    SplitFFT<L, LN, PS, NT, -1, 1>::run(A, twl);
+    {
+        // lane -> (output component oa, k2); zpart: the wave of component z
+        int oa, k2;
+        bool active;
+        if constexpr (L < 64) {
+            oa = (int)threadIdx.x >> 6;
+            k2 = (int)threadIdx.x & 63;
+            active = k2 < L;
+        } else {
+            oa = (int)threadIdx.x / L;
+            k2 = (int)threadIdx.x - oa * L;
+            active = true;
+        }
+        const bool zpart = __builtin_amdgcn_readfirstlane(oa) == 2;
+        const int c0 = oa == 0 ? 0 : (oa == 1 ? 1 : 2);
+        const int c1 = oa == 0 ? 1 : (oa == 1 ? 3 : 4);
+        const int c2 = oa == 0 ? 2 : (oa == 1 ? 4 : 5);
+        float2 acc[NZ];
+#pragma unroll
+        for (int t = 0; t < NZ; ++t) acc[t] = make_float2(0.f, 0.f);
+        if (active) {
+            float w[3][NZ];
+#pragma unroll
+            for (int o = 0; o < NZ; ++o) {
+                w[0][o] = wl[(c0 * NZ + o) * L + k2];
+                w[1][o] = wl[(c1 * NZ + o) * L + k2];
+                w[2][o] = wl[(c2 * NZ + o) * L + k2];
+            }
+            const float2* Ak = A + k2 * PS;
+            // odd words: (x, y) outputs take XZ / YZ third; z takes them first and second
+            auto mac = [&](float2& a, float wv, float2 m, bool odd, float sg) {
+                if (odd) { const float u = sg * wv; a.x -= u * m.y; a.y += u * m.x; }
+                else { a.x += wv * m.x; a.y += wv * m.y; }
+            };
+#pragma unroll
+            for (int s = 0; s < NZ; ++s) {
+                float2 m[3];
+#pragma unroll
+                for (int b = 0; b < 3; ++b) m[b] = Ak[b * NZ + s];
+#pragma unroll
+                for (int t = 0; t < NZ; ++t) {
+                    const int o = s >= t ? s - t : t - s;
+                    const float sg = s < t ? -1.0f : 1.0f;
+                    if (zpart) {
+                        mac(acc[t], w[0][o], m[0], true, sg);
+                        mac(acc[t], w[1][o], m[1], true, sg);
+                        mac(acc[t], w[2][o], m[2], false, sg);
+                    } else {
+                        mac(acc[t], w[0][o], m[0], false, sg);
+                        mac(acc[t], w[1][o], m[1], false, sg);
+                        mac(acc[t], w[2][o], m[2], true, sg);
+                    }
+                }
+            }
+        }
+        __syncthreads();    // every lane has read its sources: the products go in place
+        if (active) {
+#pragma unroll
+            for (int t = 0; t < NZ; ++t) A[k2 * PS + oa * NZ + t] = acc[t];
+        }
+    }
+    __syncthreads();
+    SplitFFT<L, LN, PS, NT, 1, 1>::run(A, twl);
+    float2* __restrict__ dst = dp.Hh + (size_t)kx * ny + k1;
+    for (int idx = threadIdx.x; idx < LN * L; idx += NT) {
+        const int l = idx / L, b = idx - l * L;
+        dst[l * lst + (size_t)b * R] = A[b * PS + l];
+    }
+}
+
+template <int L, int NZ>
+__global__ void __launch_bounds__(split_tpb(L)) k_cols_split(DipoleDev dp)
+{
+    split_cols_body<L, NZ>(dp, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+// films of one shape in one launch: `per` = nxh R workgroups each
+template <int L, int NZ>
+__global__ void __launch_bounds__(split_tpb(L)) k_cols_split_batch(const DipoleDev* __restrict__ dps, int per)
+{
+    const int w = xcd_remap(blockIdx.x, gridDim.x), r = w / per;
+    split_cols_body<L, NZ>(dps[r], w - r * per);
+}
+
 // ------------------------------------------------------------ per-layer-pair column pass
 // LongRange2D::apply (cpu/longrange2d/spinoperationlongrange2d.cpp:448-531): a full complex
 // tensor for every ordered layer pair and every component,
@@ -472,13 +819,13 @@ __global__ void __launch_bounds__(col_tpb(NY)) k_cols_pair_batch(const DipoleDev
 // finished.  Runtime shapes gather and finish each site in turn.
 template <int NX, int TY, int TPB, bool DIP, bool PRED, bool COMP>
 __device__ __forceinline__ void fused_sites(const StepArgs& a, const ExchangeDev& exl, float2* A, float2* B, float2*& R,
-                                            const float2* tw, int nx, int ny, int ty, int z, int y0, uint64_t draw)
+                                            const float2* tw, int nx, int ny, int ty, const RowTile& t, uint64_t draw)
 {
     const DipoleDev& dp = a.dp;
     auto site_of = [&](int li, int& r, int& x) {
         r = li / nx;
         x = li - r * nx;
-        return x + nx * (y0 + r + ny * z);
+        return x + nx * (t.y0 + t.ys * r + ny * t.z);
     };
     auto gather = [&](int i, SiteGather& g) {
         if constexpr (PRED) pred_gather<COMP>(a, exl, i, g);
@@ -539,7 +886,7 @@ __device__ __forceinline__ void step_pred_body(const StepArgs& a, int tile)
     const DipoleDev& dp = a.dp;
     const int nx = NX ? NX : a.d.nx, ty = TY ? TY : a.ty, lines = row_lines(ty);
     const int ny = a.d.ny;
-    const int yb = tile / a.d.nz, z = tile - yb * a.d.nz, y0 = yb * ty;
+    const RowTile t = row_tile(dp, tile, a.d.nz, ty);
     float2* A = lds;
     float2* B = lds + (size_t)lines * nx;
     float2* R = A;
@@ -549,15 +896,15 @@ __device__ __forceinline__ void step_pred_body(const StepArgs& a, int tile)
     const ExchangeDev exl = stage_exchange(a.ex, s_start, s_ent);
     if constexpr (DIP) {
         tw = row_twiddles<NX>(dp, A + (size_t)lines * nx);
-        rows_pack_spectrum<TPB, NX, TY>(dp.Hh, nx, ny, a.d.nz, ty, z, y0, A);
+        rows_in<TPB, NX, TY>(dp, dp.Hh, nx, ny, a.d.nz, ty, t, A);
     }
     __syncthreads();
     if (a.th.on && tile == 0 && threadIdx.x == 0) *a.draw += 1ull;
-    fused_sites<NX, TY, TPB, DIP, true, COMP>(a, exl, A, B, R, tw, nx, ny, ty, z, y0, 0ull);
+    fused_sites<NX, TY, TPB, DIP, true, COMP>(a, exl, A, B, R, tw, nx, ny, ty, t, 0ull);
     if constexpr (DIP) {
         __syncthreads();
         const float2* F = row_fft<NX, TY, TPB, -1>(R, R == A ? B : A, dp, tw);
-        rows_unpack_spectrum<TPB>(F, nx, ny, a.d.nz, ty, z, y0, dp.Mh);
+        rows_out<TPB, NX, TY>(dp, F, nx, ny, a.d.nz, ty, t, dp.Mh);
     }
 }
 
@@ -574,7 +921,7 @@ __device__ __forceinline__ void step_corr_body(const StepArgs& a, int tile)
     const DipoleDev& dp = a.dp;
     const int nx = NX ? NX : a.d.nx, ty = TY ? TY : a.ty, lines = row_lines(ty);
     const int ny = a.d.ny;
-    const int yb = tile / a.d.nz, z = tile - yb * a.d.nz, y0 = yb * ty;
+    const RowTile t = row_tile(dp, tile, a.d.nz, ty);
     float2* A = lds;
     float2* B = lds + (size_t)lines * nx;
     float2* R = A;
@@ -584,26 +931,26 @@ __device__ __forceinline__ void step_corr_body(const StepArgs& a, int tile)
     const ExchangeDev exl = stage_exchange(a.ex, s_start, s_ent);
     if constexpr (DIP) {
         tw = row_twiddles<NX>(dp, A + (size_t)lines * nx);
-        rows_pack_spectrum<TPB, NX, TY>(dp.Hh, nx, ny, a.d.nz, ty, z, y0, A);
+        rows_in<TPB, NX, TY>(dp, dp.Hh, nx, ny, a.d.nz, ty, t, A);
     }
     __syncthreads();
     const uint64_t draw = a.th.on ? (uint64_t)*a.draw : 0ull;
-    fused_sites<NX, TY, TPB, DIP, false, COMP>(a, exl, A, B, R, tw, nx, ny, ty, z, y0, draw);
+    fused_sites<NX, TY, TPB, DIP, false, COMP>(a, exl, A, B, R, tw, nx, ny, ty, t, draw);
     if constexpr (DIP) {
         __syncthreads();
         const float2* F = row_fft<NX, TY, TPB, -1>(R, R == A ? B : A, dp, tw);
-        rows_unpack_spectrum<TPB>(F, nx, ny, a.d.nz, ty, z, y0, dp.Mh);
+        rows_out<TPB, NX, TY>(dp, F, nx, ny, a.d.nz, ty, t, dp.Mh);
     }
 }
 
 template <int NX, int TY, int TPB, bool DIP, bool COMP>
-__global__ void __launch_bounds__(TPB) k_step_pred(StepArgs a)
+__global__ void __launch_bounds__(TPB, row_waves(TPB)) k_step_pred(StepArgs a)
 {
     step_pred_body<NX, TY, TPB, DIP, COMP>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 template <int NX, int TY, int TPB, bool DIP, bool COMP>
-__global__ void __launch_bounds__(TPB) k_step_corr(StepArgs a)
+__global__ void __launch_bounds__(TPB, row_waves(TPB)) k_step_corr(StepArgs a)
 {
     step_corr_body<NX, TY, TPB, DIP, COMP>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
@@ -623,8 +970,7 @@ __global__ void __launch_bounds__(TPB) k_rk_stage(StepArgs a, RKStage r)
     const DipoleDev& dp = a.dp;
     const int nx = NX ? NX : a.d.nx, ty = TY ? TY : a.ty, lines = row_lines(ty);
     const int ny = a.d.ny;
-    const int tile = xcd_remap(blockIdx.x, gridDim.x);
-    const int yb = tile / a.d.nz, z = tile - yb * a.d.nz, y0 = yb * ty;
+    const RowTile t = row_tile(dp, xcd_remap(blockIdx.x, gridDim.x), a.d.nz, ty);
     float2* A = lds;
     float2* B = lds + (size_t)lines * nx;
     float2* R = A;
@@ -634,7 +980,7 @@ __global__ void __launch_bounds__(TPB) k_rk_stage(StepArgs a, RKStage r)
     const ExchangeDev exl = stage_exchange(a.ex, s_start, s_ent);
     if constexpr (DIP) {
         tw = row_twiddles<NX>(dp, A + (size_t)lines * nx);
-        rows_pack_spectrum<TPB, NX, TY>(dp.Hh, nx, ny, a.d.nz, ty, z, y0, A);
+        rows_in<TPB, NX, TY>(dp, dp.Hh, nx, ny, a.d.nz, ty, t, A);
     }
     __syncthreads();
     if constexpr (DIP) R = row_fft<NX, TY, TPB, 1>(A, B, dp, tw);
@@ -645,7 +991,7 @@ __global__ void __launch_bounds__(TPB) k_rk_stage(StepArgs a, RKStage r)
     const float3 zero = make_float3(0.f, 0.f, 0.f);
     for (int li = threadIdx.x; li < ty * nx; li += TPB) {
         const int rr = li / nx, x = li - rr * nx;
-        const uint32_t i = (uint32_t)(x + nx * (y0 + rr + ny * z));
+        const uint32_t i = (uint32_t)(x + nx * (t.y0 + t.ys * rr + ny * t.z));
         const float ms = ldg(msa, i);
         const float3 X = make_float3(ldg(r.X, i), ldg(r.X + n, i), ldg(r.X + 2 * (size_t)n, i));
         ExGather g;
@@ -703,7 +1049,7 @@ __global__ void __launch_bounds__(TPB) k_rk_stage(StepArgs a, RKStage r)
     if constexpr (DIP) {
         __syncthreads();
         const float2* F = row_fft<NX, TY, TPB, -1>(R, R == A ? B : A, dp, tw);
-        rows_unpack_spectrum<TPB>(F, nx, ny, a.d.nz, ty, z, y0, dp.Mh);
+        rows_out<TPB, NX, TY>(dp, F, nx, ny, a.d.nz, ty, t, dp.Mh);
     }
 }
 
@@ -715,14 +1061,14 @@ __global__ void k_draw_advance(unsigned long long* draw)
 // films of one shape in one launch: film r's arguments at as[r], its `per` tiles at
 // [r per, (r+1) per) of the remapped workgroup index
 template <int NX, int TY, int TPB, bool COMP>
-__global__ void __launch_bounds__(TPB) k_step_pred_batch(const StepArgs* __restrict__ as, int per)
+__global__ void __launch_bounds__(TPB, row_waves(TPB)) k_step_pred_batch(const StepArgs* __restrict__ as, int per)
 {
     const int w = xcd_remap(blockIdx.x, gridDim.x), r = w / per;
     step_pred_body<NX, TY, TPB, true, COMP>(as[r], w - r * per);
 }
 
 template <int NX, int TY, int TPB, bool COMP>
-__global__ void __launch_bounds__(TPB) k_step_corr_batch(const StepArgs* __restrict__ as, int per)
+__global__ void __launch_bounds__(TPB, row_waves(TPB)) k_step_corr_batch(const StepArgs* __restrict__ as, int per)
 {
     const int w = xcd_remap(blockIdx.x, gridDim.x), r = w / per;
     step_corr_body<NX, TY, TPB, true, COMP>(as[r], w - r * per);
@@ -734,8 +1080,24 @@ __global__ void __launch_bounds__(TPB) k_step_corr_batch(const StepArgs* __restr
 // select.  Column lengths likewise.
 #define MSD_ROW_SHAPES(X)                                                           \
     X(32, 8, 256) X(64, 8, 256) X(128, 8, 512) X(256, 8, 1024) X(256, 4, 512)       \
-    X(256, 8, 512) X(256, 2, 256) X(512, 4, 1024) X(512, 2, 512) X(1024, 2, 1024)
+    X(256, 8, 512) X(256, 2, 256) X(256, 4, 1024) X(512, 4, 1024) X(512, 2, 512)    \
+    X(1024, 2, 1024)
 #define MSD_COL_LENGTHS(X) X(32) X(64) X(128) X(256) X(512) X(1024)
+// sub-column lengths ly = ny / R of the split column kernel
+#define MSD_SPLIT_LENGTHS(X) X(8) X(16) X(32) X(64) X(128) X(256)
+
+// whether the split column kernel is compiled for sub-columns of ly (R rows per tile)
+bool split_length_compiled(int ly, int R)
+{
+    (void)R;
+#define MSD_SL(L) if (ly == L) return true;
+    MSD_SPLIT_LENGTHS(MSD_SL)
+#undef MSD_SL
+    return false;
+}
+
+// lines (ly x odd pitch), twiddles and the workgroup's 6 nz ly words
+size_t split_cols_lds(int ly, int nz) { return (size_t)ly * split_pitch(nz) * 8 + (size_t)8 * ly + (size_t)24 * nz * ly; }
 
 static int row_tiles(int ny, int nz, int ty) { return nz * (ny / ty); }
 
@@ -836,8 +1198,38 @@ static hipError_t launch_cols_pair(const DipoleDev& dp, hipStream_t st)
     return hipGetLastError();
 }
 
+template <int NZ>
+static void launch_cols_split_nz(const DipoleDev& dp, hipStream_t st)
+{
+#define MSD_CS(L)                                                                   \
+    if (dp.ly == L) {                                                               \
+        k_cols_split<L, NZ><<<dp.nxh * dp.split, split_tpb(L), dp.lds_cols, st>>>(dp); \
+        return;                                                                     \
+    }
+    MSD_SPLIT_LENGTHS(MSD_CS)
+#undef MSD_CS
+}
+
+static hipError_t launch_cols_split(const DipoleDev& dp, hipStream_t st)
+{
+    if (!split_length_compiled(dp.ly, dp.split)) return hipErrorInvalidValue;
+    switch (dp.nz) {
+        case 1: launch_cols_split_nz<1>(dp, st); break;
+        case 2: launch_cols_split_nz<2>(dp, st); break;
+        case 3: launch_cols_split_nz<3>(dp, st); break;
+        case 4: launch_cols_split_nz<4>(dp, st); break;
+        case 5: launch_cols_split_nz<5>(dp, st); break;
+        case 6: launch_cols_split_nz<6>(dp, st); break;
+        case 7: launch_cols_split_nz<7>(dp, st); break;
+        case 8: launch_cols_split_nz<8>(dp, st); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_cols(const DipoleDev& dp, hipStream_t st)
 {
+    if (dp.split) return launch_cols_split(dp, st);
     if (dp.Tp) return launch_cols_pair(dp, st);
     switch (dp.nz) {
         case 1: launch_cols_nz<1>(dp, st); break;
@@ -976,8 +1368,36 @@ static void launch_cols_batch_nz(const DipoleDev& d0, const DipoleDev* dps, int 
     else k_cols_batch<0, NZ><<<per * nrep, col_tpb(0), d0.lds_cols, st>>>(dps, per);
 }
 
+template <int NZ>
+static void launch_cols_split_batch_nz(const DipoleDev& d0, const DipoleDev* dps, int nrep, hipStream_t st)
+{
+#define MSD_CSB(L)                                                                  \
+    if (d0.ly == L) {                                                               \
+        const int per = d0.nxh * d0.split;                                          \
+        k_cols_split_batch<L, NZ><<<per * nrep, split_tpb(L), d0.lds_cols, st>>>(dps, per); \
+        return;                                                                     \
+    }
+    MSD_SPLIT_LENGTHS(MSD_CSB)
+#undef MSD_CSB
+}
+
 hipError_t launch_cols_batch(const DipoleDev& d0, const DipoleDev* dps, int nrep, hipStream_t st)
 {
+    if (d0.split) {
+        if (!split_length_compiled(d0.ly, d0.split)) return hipErrorInvalidValue;
+        switch (d0.nz) {
+            case 1: launch_cols_split_batch_nz<1>(d0, dps, nrep, st); break;
+            case 2: launch_cols_split_batch_nz<2>(d0, dps, nrep, st); break;
+            case 3: launch_cols_split_batch_nz<3>(d0, dps, nrep, st); break;
+            case 4: launch_cols_split_batch_nz<4>(d0, dps, nrep, st); break;
+            case 5: launch_cols_split_batch_nz<5>(d0, dps, nrep, st); break;
+            case 6: launch_cols_split_batch_nz<6>(d0, dps, nrep, st); break;
+            case 7: launch_cols_split_batch_nz<7>(d0, dps, nrep, st); break;
+            case 8: launch_cols_split_batch_nz<8>(d0, dps, nrep, st); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (d0.nz) {
         case 1: launch_cols_batch_nz<1>(d0, dps, nrep, st); break;
         case 2: launch_cols_batch_nz<2>(d0, dps, nrep, st); break;
@@ -1018,6 +1438,9 @@ static void cols_attr(int v, hipError_t& first)
 #undef MSD_CP
     set_lds((const void*)k_cols_pair<0, NZ>, v, first);
     set_lds((const void*)k_cols_pair_batch<0, NZ>, v, first);
+#define MSD_CSA(L) set_lds((const void*)k_cols_split<L, NZ>, v, first); set_lds((const void*)k_cols_split_batch<L, NZ>, v, first);
+    MSD_SPLIT_LENGTHS(MSD_CSA)
+#undef MSD_CSA
 }
 
 hipError_t configure_lds(size_t max_bytes)

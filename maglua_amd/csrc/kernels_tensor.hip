@@ -191,16 +191,34 @@ hipError_t launch_plane_spectrum(int planes, int nx, int ny, const double* T, do
     return hipGetLastError();
 }
 
-__global__ void k_tensor_words(const double* __restrict__ W, float* __restrict__ Tw, double mult, size_t n)
+// W [6][nz][nxh][ny] -> Tw.  With a split y transform (R > 0) output word
+// (((kx R + k1) 6 + a) nz + o) ly + k2 takes W[a][o][kx][k1 + R k2]: the words of one
+// column workgroup (kx, k1) are one contiguous run.
+__global__ void k_tensor_words(const double* __restrict__ W, float* __restrict__ Tw, double mult, int nz, int nxh, int ny,
+                               int R)
 {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        Tw[i] = (float)(W[i] * mult);
+    const size_t n = (size_t)6 * nz * nxh * ny;
+    const int ly = R > 0 ? ny / R : ny;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        size_t src = i;
+        if (R > 0) {
+            const int k2 = (int)(i % ly);
+            const int o = (int)((i / ly) % nz);
+            const int a = (int)((i / ((size_t)ly * nz)) % 6);
+            const size_t w = i / ((size_t)ly * nz * 6);          // kx * R + k1
+            const int kx = (int)(w / R), k1 = (int)(w % R);
+            src = (((size_t)a * nz + o) * nxh + kx) * ny + k1 + (size_t)R * k2;
+        }
+        Tw[i] = (float)(W[src] * mult);
+    }
 }
 
-hipError_t launch_tensor_words(const double* W, float* Tw, double mult, size_t n, hipStream_t st)
+hipError_t launch_tensor_words(const double* W, float* Tw, double mult, int nz, int nxh, int ny, int split,
+                               hipStream_t st)
 {
+    const size_t n = (size_t)6 * nz * nxh * ny;
     const int g = (int)std::min<size_t>((n + 255) / 256, 65536);
-    k_tensor_words<<<g, 256, 0, st>>>(W, Tw, mult, n);
+    k_tensor_words<<<g, 256, 0, st>>>(W, Tw, mult, nz, nxh, ny, split);
     return hipGetLastError();
 }
 

@@ -1518,6 +1518,7 @@ struct msd_dipole {
     // them knows whether its cached spin spectrum is still there
     uint64_t epoch = 0;
     double words_mult = NAN;
+    int words_split = -1;     // dev.split the words were laid out for
     double* T64 = nullptr;    // [6][2nz-1][ny][nx]
     double2* U64 = nullptr;
     double* W64 = nullptr;    // [6][nz][nxh][ny]
@@ -1576,6 +1577,42 @@ static int twiddles(int L, float2** d32, double2** d64)
     return MSD_OK;
 }
 
+// Row tiles of ty rows and tpb threads, and the column form that goes with them.
+// The split y transform (DipoleDev::split) is taken when `want_split` is set, the row
+// shape is compiled, ty is 2, 4 or 8, ny / ty is a compiled split length and its
+// column workgroup fits the LDS, and the operator is the lattice-sum form; the other
+// cases transform full columns.  Compile-time shapes transform in place (one line
+// buffer plus the twiddle table), others use two ping-pong buffers.
+static int layered_shape(msd_dipole* p, int ty, int tpb, bool row_ct, bool want_split, const char* what)
+{
+    DipoleDev& dv = p->dev;
+    const int nx = p->d.nx, ny = p->d.ny, nz = p->d.nz;
+    dv.ty = ty;
+    dv.tpb = tpb;
+    const size_t rbuf = (size_t)((3 * ty + 1) / 2) * nx;
+    dv.lds_rows = (row_ct ? rbuf + nx : 2 * rbuf + nx) * sizeof(float2);
+    dv.split = 0;
+    dv.ly = ny;
+    if (want_split && row_ct && !p->pair && (ty == 2 || ty == 4 || ty == 8) && ny % ty == 0 &&
+        split_length_compiled(ny / ty, ty) && split_cols_lds(ny / ty, nz) <= G.max_lds) {
+        dv.split = ty;
+        dv.ly = ny / ty;
+        dv.lds_cols = split_cols_lds(dv.ly, nz);
+        // the row passes stage the tile's 3 (nx/2+1) runs of ty words after the lines
+        dv.lds_rows += (size_t)3 * (nx / 2 + 1) * ty * sizeof(float2);
+    } else {
+        // compile-time column lengths also stage the workgroup's tensor words (3 x nz x ny floats)
+        const bool pow2_len_y = ny >= 32 && ny <= 1024 && (ny & (ny - 1)) == 0;
+        const size_t cbuf = (size_t)3 * nz * ny;
+        dv.lds_cols = pow2_len_y ? (cbuf + ny) * sizeof(float2) + (size_t)3 * nz * ny * sizeof(float)
+                                 : 2 * cbuf * sizeof(float2);
+    }
+    if (dv.lds_rows > G.max_lds || dv.lds_cols > G.max_lds)
+        return fail(MSD_EUNSUPPORTED, "%s.new: %dx%dx%d needs %zu/%zu bytes of LDS per workgroup (device limit %zu)", what,
+                    nx, ny, nz, dv.lds_rows, dv.lds_cols, G.max_lds);
+    return MSD_OK;
+}
+
 static int layered_create(int nx, int ny, int nz, bool pair, const char* what, msd_dipole** out)
 {
     NONNULL(out);
@@ -1596,15 +1633,16 @@ static int layered_create(int nx, int ny, int nz, bool pair, const char* what, m
     dv.nx = nx; dv.ny = ny; dv.nz = nz; dv.nxh = p->nxh;
     dv.px = px; dv.py = py;
     // Row tiles: the default compile-time shape for nx (kernels_dipole.hip's
-    // MSD_ROW_SHAPES) whose rows per tile divide ny; MSD_ROW_TILE="ty:tpb" picks
-    // another compiled shape for tuning runs.  Without one, runtime-shaped
+    // MSD_ROW_SHAPES) whose rows per tile divide ny; MSD_ROW_TILE="ty:tpb[:split]" picks
+    // another compiled shape for tuning runs (split 0: full-column transforms).  Without one, runtime-shaped
     // kernels take the largest of 8,4,2,1 rows dividing ny whose two LDS
     // buffers fit 64 KiB.
-    int ty = 0, tpb = 0;
+    int ty = 0, tpb = 0, split = 0;
     row_shape_default(nx, ny, &ty, &tpb);
     if (const char* env = getenv("MSD_ROW_TILE")) {
-        int t = 0, b = 0;
-        if (sscanf(env, "%d:%d", &t, &b) == 2 && t > 0 && ny % t == 0 && row_shape_compiled(nx, t, b)) { ty = t; tpb = b; }
+        int t = 0, b = 0, sp = 1;
+        const int got = sscanf(env, "%d:%d:%d", &t, &b, &sp);
+        if (got >= 2 && t > 0 && ny % t == 0 && row_shape_compiled(nx, t, b)) { ty = t; tpb = b; split = got == 3 ? sp : 0; }
     }
     const bool row_ct = ty > 0;
     if (!row_ct) {
@@ -1615,23 +1653,12 @@ static int layered_create(int nx, int ny, int nz, bool pair, const char* what, m
             if (ny % c == 0 && need <= 65536) { ty = c; break; }
         }
     }
-    dv.ty = ty;
-    dv.tpb = tpb;
-    // Compile-time shapes (MSD_ROW_SHAPES / MSD_COL_LENGTHS) transform in place:
-    // one line buffer plus the twiddle table.  Other sizes use two ping-pong buffers.
-    const bool pow2_len_y = ny >= 32 && ny <= 1024 && (ny & (ny - 1)) == 0;
-    const size_t rbuf = (size_t)((3 * ty + 1) / 2) * nx, cbuf = (size_t)3 * nz * ny;
-    dv.lds_rows = (row_ct ? rbuf + nx : 2 * rbuf + nx) * sizeof(float2);
-    // compile-time column lengths also stage the workgroup's tensor words (3 x nz x ny floats)
-    dv.lds_cols = pow2_len_y ? (cbuf + ny) * sizeof(float2) + (size_t)3 * nz * ny * sizeof(float) : 2 * cbuf * sizeof(float2);
-    if (dv.lds_rows > G.max_lds || dv.lds_cols > G.max_lds)
-        return fail(MSD_EUNSUPPORTED, "%s.new: %dx%dx%d needs %zu/%zu bytes of LDS per workgroup (device limit %zu)", what,
-                    nx, ny, nz, dv.lds_rows, dv.lds_cols, G.max_lds);
+    if (pair) p->pair = true;
+    if (int rc = layered_shape(p.get(), ty, tpb, row_ct, split, what)) return rc;
     const size_t spec = (size_t)3 * nz * p->nxh * ny;
     if (int rc = dalloc(&p->Mh, spec)) return rc;
     if (int rc = dalloc(&p->Hh, spec)) return rc;
     if (pair) {
-        p->pair = true;
         p->slot = "LongRange2D";
         p->P.assign((size_t)9 * nz * nz * nx * ny, 0.0);
         p->sms.assign(nz, 1.0);
@@ -1878,11 +1905,21 @@ int msd_dipole::prepare()
         words_mult = NAN;
     }
     const double mult = g * scale;
-    if (!(words_mult == mult)) {
-        hipError_t e = launch_tensor_words(W64, Tw, mult, (size_t)6 * d.nz * nxh * d.ny, G.stream);
+    if (!(words_mult == mult) || words_split != dev.split) {
+        hipError_t e = launch_tensor_words(W64, Tw, mult, d.nz, nxh, d.ny, dev.split, G.stream);
         if (e != hipSuccess) return fail(MSD_EDEVICE, "dipole words: %s", hipGetErrorString(e));
         words_mult = mult;
+        words_split = dev.split;
     }
+    return MSD_OK;
+}
+
+extern "C" int msd_dipole_schedule(const msd_dipole* d, int* rows_per_tile, int* threads, int* split)
+{
+    NONNULL(d); NONNULL(rows_per_tile); NONNULL(threads); NONNULL(split);
+    *rows_per_tile = d->dev.ty;
+    *threads = d->dev.tpb;
+    *split = d->dev.split;
     return MSD_OK;
 }
 
@@ -3579,15 +3616,16 @@ extern "C" int msd_stepper_rk(msd_stepper* st, const char* kind, int nsteps)
 // compiled row-tile shape (rows per tile, threads per workgroup) of the lattice
 // and keeps the fastest.  The spin state, its low words, time and thermal draw
 // are restored after every trial, so tuning leaves the trajectory untouched.
-extern "C" int msd_stepper_autotune(msd_stepper* st, int steps, int* best_ty, int* best_tpb, double* best_ms)
+extern "C" int msd_stepper_autotune(msd_stepper* st, int steps, int* best_ty, int* best_tpb, int* best_split,
+                                    double* best_ms)
 {
-    NONNULL(st); NONNULL(best_ty); NONNULL(best_tpb); NONNULL(best_ms);
+    NONNULL(st); NONNULL(best_ty); NONNULL(best_tpb); NONNULL(best_split); NONNULL(best_ms);
     if (st->kind() != SK_LAYERED) return fail(MSD_EUNSUPPORTED, "autotune: needs a layered Dipole step driver");
     if (steps < 1) return fail(MSD_EINVAL, "autotune: steps %d", steps);
     msd_dipole* dip = st->dip;
     int tys[16], tpbs[16];
     const int nc = row_shapes_for(dip->d.nx, dip->d.ny, tys, tpbs, 16);
-    *best_ty = dip->dev.ty; *best_tpb = dip->dev.tpb; *best_ms = -1.0;
+    *best_ty = dip->dev.ty; *best_tpb = dip->dev.tpb; *best_split = dip->dev.split; *best_ms = -1.0;
     if (nc == 0) return MSD_OK;           // runtime-shaped kernels only: nothing to choose
     const size_t n = st->s->d.n;
     float *S0 = nullptr, *L0 = nullptr;
@@ -3609,19 +3647,22 @@ extern "C" int msd_stepper_autotune(msd_stepper* st, int steps, int* best_ty, in
         st->s->version++;
         return MSD_OK;
     };
-    auto use = [&](int ty, int tpb) {
-        DipoleDev& dv = dip->dev;
-        dv.ty = ty;
-        dv.tpb = tpb;
-        dv.lds_rows = (size_t)((3 * ty + 1) / 2 + 1) * dip->d.nx * sizeof(float2);
+    // every trial rebuilds the step (and the tensor words when the column form changes)
+    auto use = [&](int ty, int tpb, bool sp) -> int {
+        if (int r = layered_shape(dip, ty, tpb, true, sp, "autotune")) return r;
         st->drop_graph();
+        dip->epoch++;
+        return dip->prepare();
     };
-    const int ty_orig = dip->dev.ty, tpb_orig = dip->dev.tpb;
-    const size_t lds_orig = dip->dev.lds_rows;
-    for (int c = 0; c < nc && !rc; ++c) {
-        const size_t need = (size_t)((3 * tys[c] + 1) / 2 + 1) * dip->d.nx * sizeof(float2);
+    const DipoleDev orig = dip->dev;
+    for (int c = 0; c < 2 * nc && !rc; ++c) {
+        const int ty = tys[c / 2], tpb = tpbs[c / 2];
+        const bool sp = (c & 1) == 0;
+        const size_t need = (size_t)((3 * ty + 1) / 2 + 1) * dip->d.nx * sizeof(float2);
         if (need > G.max_lds) continue;
-        use(tys[c], tpbs[c]);
+        if (layered_shape(dip, ty, tpb, true, sp, "autotune") != MSD_OK) { dip->dev = orig; continue; }
+        if (sp && dip->dev.split == 0) continue;    // no split form for this shape: its full-column trial covers it
+        if ((rc = use(ty, tpb, sp))) break;
         if ((rc = stepper_begin(st))) break;
         if ((rc = run_steps(st, 2, false))) break;          // capture + warm
         if (hipEventRecord(e0, G.stream) != hipSuccess) { rc = fail(MSD_EDEVICE, "autotune: event"); break; }
@@ -3633,11 +3674,20 @@ extern "C" int msd_stepper_autotune(msd_stepper* st, int steps, int* best_ty, in
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, e0, e1);
         const double per = ms / steps;
-        if (*best_ms < 0 || per < *best_ms) { *best_ms = per; *best_ty = tys[c]; *best_tpb = tpbs[c]; }
+        if (*best_ms < 0 || per < *best_ms) {
+            *best_ms = per; *best_ty = ty; *best_tpb = tpb; *best_split = dip->dev.split;
+        }
         if ((rc = restore())) break;
     }
-    if (!rc && *best_ms > 0) use(*best_ty, *best_tpb);
-    else { dip->dev.ty = ty_orig; dip->dev.tpb = tpb_orig; dip->dev.lds_rows = lds_orig; st->drop_graph(); }
+    int rc2 = MSD_OK;
+    if (!rc && *best_ms > 0) rc2 = use(*best_ty, *best_tpb, *best_split != 0);
+    else {
+        dip->dev = orig;
+        st->drop_graph();
+        dip->epoch++;
+        rc2 = dip->prepare();
+    }
+    if (!rc) rc = rc2;
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     (void)hipStreamSynchronize(G.stream);
@@ -3849,8 +3899,8 @@ struct FilmGroup {
     {
         return std::memcmp(&a.d, &b.d, sizeof a.d) == 0 && a.ty == b.ty && (a.Slo == nullptr) == (b.Slo == nullptr) &&
                (a.dp.Tp == nullptr) == (b.dp.Tp == nullptr) && a.dp.tpb == b.dp.tpb && a.dp.ty == b.dp.ty && a.dp.lds_rows == b.dp.lds_rows &&
-               a.dp.lds_cols == b.dp.lds_cols && std::memcmp(&a.dp.px, &b.dp.px, sizeof a.dp.px) == 0 &&
-               std::memcmp(&a.dp.py, &b.dp.py, sizeof a.dp.py) == 0;
+               a.dp.lds_cols == b.dp.lds_cols && a.dp.split == b.dp.split && a.dp.ly == b.dp.ly &&
+               std::memcmp(&a.dp.px, &b.dp.px, sizeof a.dp.px) == 0 && std::memcmp(&a.dp.py, &b.dp.py, sizeof a.dp.py) == 0;
     }
     void release() { dfree(dargs); dfree(ddps); }
     // the tables into device memory when a value changed (not inside a capture)

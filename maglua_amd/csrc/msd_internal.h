@@ -76,9 +76,19 @@ struct DipoleDev {
     FFTPlan px, py;
     const float2* twx;      // exp(-2 pi i k / nx), k < nx
     const float2* twy;      // exp(-2 pi i k / ny), k < ny
+    // Split y transform (split = R > 0, ny = R * ly): a row tile holds the R rows
+    // b + ly*a (a < R) of one layer, b < ly.  The row kernels do the length-R DFT
+    // across the tile's rows and the twiddle exp(-+2 pi i b k1 / ny); the column
+    // kernel does length-ly transforms for each (kx, k1), ky = k1 + R*k2.  Between
+    // the passes the spectra hold, per (comp, layer, kx), [b][k1] with k1 fastest.
+    // split = 0: consecutive rows per tile, [y] per (comp, layer, kx), full-length
+    // column transforms.
+    int split, ly;
     float2* Mh;             // spin spectrum [3][nz][nxh][ny]
     float2* Hh;             // field spectrum [3][nz][nxh][ny]
-    const float* Tw;        // tensor words [6][nz][nxh][ny] (XX XY XZ YY YZ ZZ; XZ,YZ imaginary)
+    // tensor words (XX XY XZ YY YZ ZZ; XZ,YZ imaginary): [6][nz][nxh][ny], or with a
+    // split transform [nxh][R][6][nz][ly] (each column workgroup's words contiguous)
+    const float* Tw;
     // per-layer-pair form (LongRange2D and its builders); null for the lattice-sum Dipole
     const float2* Tp;       // complex words [9][nz][nz][nxh][ny], 1/(nx ny) folded in
     const float* lsc;       // [2][MSD_MAX_LAYERS]: source-moment scales, destination-field scales
@@ -127,7 +137,9 @@ hipError_t launch_tensor_sum(int nx, int ny, int nz, const int gmax[4], const do
 // twx/twy: FP64 tables (cos, -sin)(2 pi k / L)
 hipError_t launch_tensor_spectrum(int nx, int ny, int nz, const double* T, double2* U, double* W,
                                   const double2* twx, const double2* twy, hipStream_t st);
-hipError_t launch_tensor_words(const double* W, float* Tw, double mult, size_t n, hipStream_t st);
+// FP32 words Tw = mult * W; split = R > 0 lays them out [nxh][R][6][nz][ny/R] (ky = k1 + R k2)
+hipError_t launch_tensor_words(const double* W, float* Tw, double mult, int nz, int nxh, int ny, int split,
+                               hipStream_t st);
 // planes of nx x ny real FP64 values T [planes][ny][nx] -> R2C spectra out [planes][nx/2+1][ny]
 // as FP32 complex, times mult; U scratch [planes][ny][nx/2+1]
 hipError_t launch_plane_spectrum(int planes, int nx, int ny, const double* T, double2* U, float2* out, double mult,
@@ -292,5 +304,8 @@ bool row_shape_compiled(int nx, int ty, int tpb);
 // every compiled (ty, tpb) for an nx x ny layer, default first; returns the count
 int row_shapes_for(int nx, int ny, int* ty, int* tpb, int cap);
 void row_shape_default(int nx, int ny, int* ty, int* tpb);
+// split column kernel: compiled sub-column lengths and its dynamic LDS bytes
+bool split_length_compiled(int ly, int R);
+size_t split_cols_lds(int ly, int nz);
 
 }  // namespace msd

@@ -696,6 +696,13 @@ class Dipole(_Handle):
     def strength(self):
         return self._g
 
+    def schedule(self):
+        """The transform schedule in use: {"rows_per_tile", "threads", "split"} (split = rows
+        per tile when the row kernels take the first radix stage of the y transform)."""
+        ty, tpb, sp = C.c_int(), C.c_int(), C.c_int()
+        call("msd_dipole_schedule", self._h, C.byref(ty), C.byref(tpb), C.byref(sp))
+        return {"rows_per_tile": ty.value, "threads": tpb.value, "split": sp.value}
+
     def setScale(self, s):
         call("msd_dipole_set_scale", self._h, float(s))
 
@@ -1372,11 +1379,13 @@ class Heun(_Handle):
         call("msd_stepper_set_surrogate", self._h, sur.h if sur is not None else None, int(K), float(tol))
 
     def autotune(self, steps=20):
-        """Time every compiled row-tile shape for `steps` steps, keep the fastest; the state
-        is restored.  Returns {"rows_per_tile", "threads", "ms_per_step"}."""
-        ty, tpb, ms = C.c_int(), C.c_int(), C.c_double()
-        call("msd_stepper_autotune", self._h, int(steps), C.byref(ty), C.byref(tpb), C.byref(ms))
-        return {"rows_per_tile": ty.value, "threads": tpb.value, "ms_per_step": ms.value}
+        """Time every compiled row-tile shape in both y-transform schedules (split: the row
+        kernels take a radix-rows_per_tile stage; full columns) for `steps` steps and keep
+        the fastest; the state is restored.  Returns {"rows_per_tile", "threads", "split",
+        "ms_per_step"}."""
+        ty, tpb, sp, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        call("msd_stepper_autotune", self._h, int(steps), C.byref(ty), C.byref(tpb), C.byref(sp), C.byref(ms))
+        return {"rows_per_tile": ty.value, "threads": tpb.value, "split": sp.value, "ms_per_step": ms.value}
 
     def rk(self, kind, n=1):
         """n steps of make_rk_step_function(ss, kind, ...) with the fused stage kernels

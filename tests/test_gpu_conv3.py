@@ -241,10 +241,15 @@ def film(dims, T, seed, g=0.05, K=0.05):
     return spec, gpu
 
 
-def test_ensemble_with_films_matches_single_replicas():
+@pytest.mark.parametrize("schedule", [None, "8:256:8"])
+def test_ensemble_with_films_matches_single_replicas(schedule, monkeypatch):
     """Layered-Dipole films in an Ensemble: films of one shape step in one launch per
     pass (two shapes here), beside a LongRange3D cube and a local-field member; every
-    member equals its single-replica run bit for bit."""
+    member equals its single-replica run bit for bit.  Schedule "8:256:8": the 64x64
+    films take the split y transform (8 rows per tile, sub-columns of 8), the 32x32 film
+    keeps full columns (no split length 4)."""
+    if schedule:
+        monkeypatch.setenv("MSD_ROW_TILE", schedule)
     cfg = [((64, 64, 4), 0.3, 1), ((64, 64, 4), 0.0, 2), ((32, 32, 2), 0.8, 3), ((64, 64, 4), 1.1, 4)]
     ens, solo = [], []
     for bucket in (ens, solo):
@@ -255,6 +260,8 @@ def test_ensemble_with_films_matches_single_replicas():
         bucket.append((g, M.Heun(g["ss"], g["ex"], g["ani"], gl, g["temp"], g["llg"])))
         spec, g, o, gl, ol = cube((10, 6, 4), 0.2, 10, g=0.0)
         bucket.append((g, M.Heun(g["ss"], g["ex"], g["ani"], None, g["temp"], g["llg"])))
+    if schedule:
+        assert ens[0][0]["dip"].schedule()["split"] == 8 and ens[2][0]["dip"].schedule()["split"] == 0
     E = M.Ensemble([h for (_, h) in ens])
     E.step(7)
     E.step(5)
@@ -267,9 +274,13 @@ def test_ensemble_with_films_matches_single_replicas():
     assert ens[0][0]["temp"].draw() == 12
 
 
-def test_ensemble_films_against_oracle():
+@pytest.mark.parametrize("schedule", [None, "8:256:8"])
+def test_ensemble_films_against_oracle(schedule, monkeypatch):
     """Two 64x64x4 films with strong dipolar coupling (g = 0.1) in one Ensemble at
-    T = 0 and T = 0.2, 20 steps against the oracle on the same noise."""
+    T = 0 and T = 0.2, 20 steps against the oracle on the same noise, with full-column
+    and split y transforms."""
+    if schedule:
+        monkeypatch.setenv("MSD_ROW_TILE", schedule)
     members, refs = [], []
     for T, seed in ((0.0, 5), (0.2, 6)):
         spec, g = film((64, 64, 4), T, seed, g=0.1, K=0.02)

@@ -1,6 +1,8 @@
 """GPU parity: libmsd.so kernels (through the C ABI) against the FP64 oracle on
 the same seeded inputs.  Floating-point tolerances are stated per test; the
 integer Philox stream is bit-exact."""
+import os
+
 import numpy as np
 import pytest
 
@@ -413,7 +415,7 @@ def test_pc_driver_on_op_api_equals_fused_heun():
 
 
 # ------------------------------------------------------- row-tile shapes
-ROW_TILES_256 = ["8:1024", "4:512", "8:512", "2:256"]
+ROW_TILES_256 = ["8:1024", "4:512", "8:512", "2:256", "4:1024"]
 
 
 def _run_with_tile(spec, tile, steps, monkeypatch):
@@ -429,23 +431,32 @@ def _run_with_tile(spec, tile, steps, monkeypatch):
     return flat(g["ss"].spins()), g
 
 
-def test_row_tile_shapes_bit_identical(monkeypatch):
-    """Every compiled row-tile shape for nx=256 pairs the same real rows per complex
-    line and runs the same butterflies per line, so all give bit-identical steps; the
-    default is also checked against the oracle."""
+def test_row_tile_shapes_and_y_schedules(monkeypatch):
+    """Full-column transforms: every compiled row-tile shape for nx=256 pairs the same
+    real rows per complex line and runs the same butterflies per line, so all give
+    bit-identical steps.  Split y transforms: shapes with the same rows per tile run the
+    same radix-ty stage and the same sub-column transforms, so they agree bit for bit;
+    every schedule is checked against the oracle."""
     spec = W.FilmSpec(256, 64, 2, T=0.3, dt=0.004, g=0.02, K=0.05)
-    ref, _ = _run_with_tile(spec, None, 12, monkeypatch)
-    for tile in ROW_TILES_256:
-        got, _ = _run_with_tile(spec, tile, 12, monkeypatch)
-        print(f"row tile {tile}: max|diff| = {np.abs(got - ref).max():.3e}", flush=True)
-        assert np.array_equal(got, ref), tile
     o = W.build(O, spec)
     o["ss"].setSpins(*O.random_unit_spins(spec.seed, spec.n))
     sim = O.Sim(o["ss"], o["ex"], o["ani"], o["dip"], o["temp"], o["llg"], seed=0)
     sim.heun(12)
-    err = float(np.abs(ref[:3] - flat(o["ss"].spins()[:3])).max())
-    print(f"row tiles, 256x64x2 T=0.3, 12 steps vs oracle: max|dm| = {err:.3e}", flush=True)
-    assert err <= 1e-5
+    want = flat(o["ss"].spins()[:3])
+    full = [_run_with_tile(spec, t + ":0", 12, monkeypatch)[0] for t in ROW_TILES_256]
+    for t, got in zip(ROW_TILES_256, full):
+        print(f"row tile {t}, full columns: max|diff| = {np.abs(got - full[0]).max():.3e}", flush=True)
+        assert np.array_equal(got, full[0]), t
+    split = {t: _run_with_tile(spec, t + ":" + t.split(":")[0], 12, monkeypatch)[0] for t in ROW_TILES_256}
+    ref, g = _run_with_tile(spec, None, 12, monkeypatch)
+    assert g["dip"].schedule() == {"rows_per_tile": 8, "threads": 1024, "split": 0}
+    assert np.array_equal(ref, full[0])
+    assert np.array_equal(split["8:1024"], split["8:512"])
+    assert np.array_equal(split["4:1024"], split["4:512"])
+    for t, got in list(split.items()) + [("full", full[0])]:
+        err = float(np.abs(got[:3] - want).max())
+        print(f"row tile {t}, 256x64x2 T=0.3, 12 steps vs oracle: max|dm| = {err:.3e}", flush=True)
+        assert err <= 2e-6        # measured 1e-7 .. 2e-7
 
 
 def test_profile_leaves_state_one_step_ahead():
@@ -470,23 +481,31 @@ def test_profile_leaves_state_one_step_ahead():
 
 
 def test_autotune_picks_a_compiled_shape_and_leaves_the_trajectory():
+    """The tuned driver steps bit-identically to a driver created with the chosen shape
+    and schedule (MSD_ROW_TILE), and tuning leaves the state where it was."""
     spec = W.FilmSpec(256, 64, 2, T=0.3, dt=0.004, g=0.02, K=0.05)
-    runs = []
-    for tune in (False, True):
-        g = W.build(M, spec)
-        g["ss"].setSpins(*O.random_unit_spins(spec.seed, spec.n))
-        heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"])
-        heun.step(3)
-        if tune:
-            best = heun.autotune(steps=4)
-            print(f"autotune: {best}", flush=True)
-            assert (best["rows_per_tile"], best["threads"]) in {(8, 1024), (4, 512), (8, 512), (2, 256)}
-            assert best["ms_per_step"] > 0
-            assert g["temp"].draw() == 3
-        heun.step(5)
-        runs.append((flat(g["ss"].spins()), g["ss"].time()))
-    assert np.array_equal(runs[0][0], runs[1][0])
-    assert runs[0][1] == runs[1][1]
+    x, y, z = O.random_unit_spins(spec.seed, spec.n)
+    g = W.build(M, spec)
+    g["ss"].setSpins(x, y, z)
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"])
+    before = flat(g["ss"].spins())
+    best = heun.autotune(steps=4)
+    print(f"autotune: {best}", flush=True)
+    assert (best["rows_per_tile"], best["threads"]) in {(8, 1024), (4, 512), (8, 512), (2, 256), (4, 1024)}
+    assert best["split"] in (0, best["rows_per_tile"])
+    assert best["ms_per_step"] > 0
+    assert g["temp"].draw() == 0 and g["ss"].time() == 0.0
+    assert np.array_equal(flat(g["ss"].spins()), before)
+    heun.step(8)
+    os.environ["MSD_ROW_TILE"] = f'{best["rows_per_tile"]}:{best["threads"]}:{best["split"]}'
+    try:
+        h = W.build(M, spec)
+    finally:
+        del os.environ["MSD_ROW_TILE"]
+    h["ss"].setSpins(x, y, z)
+    M.Heun(h["ss"], h["ex"], h["ani"], h["dip"], h["temp"], h["llg"]).step(8)
+    assert np.array_equal(flat(g["ss"].spins()), flat(h["ss"].spins()))
+    assert g["ss"].time() == h["ss"].time()
 
 
 # ------------------------------------------------------- LLG.Quaternion
@@ -644,3 +663,55 @@ def test_fused_rk_graph_and_direct_launches_agree():
         h.rk("BS3", 4)
         runs.append(flat(g["ss"].spins()))
     assert np.array_equal(runs[0], runs[1])
+
+
+# ------------------------------------------------- split y-transform schedules
+SPLIT_CASES = [((256, 256, 8), "8:1024:8"), ((256, 256, 8), "4:512:4"), ((256, 256, 8), "2:256:2"),
+               ((512, 512, 4), "4:1024:4"), ((512, 512, 4), "2:512:2"), ((64, 64, 2), "8:256:8"),
+               ((128, 128, 3), "8:512:8"), ((256, 128, 1), "4:1024:4")]
+
+
+@pytest.mark.parametrize("dims,tile", SPLIT_CASES)
+def test_split_schedule_dipole_against_oracle(dims, tile, monkeypatch):
+    """Dipole:apply with the split y transform (the row kernels take a radix-ty stage
+    across their tile's rows, the column kernel transforms sub-columns of ny/ty) against
+    the FP64 oracle; sub-column lengths 8 .. 256, 1 to 8 layers."""
+    monkeypatch.setenv("MSD_ROW_TILE", tile)
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    s = np.random.default_rng(nx + ny + nz).standard_normal((3, n))
+    gs, os_ = M.SpinSystem(*dims), O.SpinSystem(*dims, threads=True)
+    gs.setSpins(*s)
+    os_.setSpins(*s)
+    gd, od = M.Dipole(*dims), O.Dipole(*dims, threads=True)
+    monkeypatch.delenv("MSD_ROW_TILE")
+    ty = int(tile.split(":")[0])
+    assert gd.schedule()["split"] == ty
+    for d in (gd, od):
+        d.setStrength(0.3)
+        d.setTruncation(40)
+    gd.apply(gs)
+    od.apply(os_)
+    hg, ho = flat(gs.field("Dipole")), flat(os_.field("Dipole"))
+    err = float(np.abs(hg - ho).max() / np.abs(ho).max())
+    print(f"split {tile} {dims}: Dipole:apply max rel err = {err:.3e}", flush=True)
+    assert err <= 2e-6
+
+
+@pytest.mark.parametrize("tile", ["8:256:8", "8:256:0"])
+def test_split_schedule_heun_and_rk_against_oracle(tile, monkeypatch):
+    """Fused Heun (T > 0, same noise) and fused RK4 stages on the split schedule against
+    the oracle, 30 + 10 steps; the full-column schedule of the same film beside it."""
+    spec = W.FilmSpec(64, 64, 2, T=0.3, dt=0.005, g=0.05)
+    monkeypatch.setenv("MSD_ROW_TILE", tile)
+    g, o = twin(spec)
+    monkeypatch.delenv("MSD_ROW_TILE")
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"])
+    sim = O.Sim(o["ss"], o["ex"], o["ani"], o["dip"], o["temp"], o["llg"], seed=spec.noise_seed)
+    heun.step(30)
+    sim.heun(30)
+    heun.rk("RK4", 10)
+    sim.rk("RK4", 10)
+    err = _max_dm(g, o)
+    print(f"schedule {tile}: 30 Heun + 10 RK4 steps max|dm| = {err:.3e}", flush=True)
+    assert err <= 2e-5
