@@ -324,6 +324,50 @@ def learned_vs_plain(M, W, train_pairs=20000, states=4, train_K=10, Ks=(10, 8, 6
     }
 
 
+def conv3_pass_bytes(dims, open_, comp, site_axes=False):
+    """Algorithmic HBM bytes per launch of each LongRange3D pass for one member
+    (every compulsory array once per pass; FFT intermediates inside a pass free).
+    The working spectrum W holds the nz stored planes (z >= nz of an open transform
+    are zero and never stored), fxh = fx/2+1 words per row; the tensor one FP32 word
+    per component per k with kz <= fz/2 stored (z parity of the lattice sums).
+      y forward: read the ny row-pass rows, write fy rows; inverse y the reverse;
+      z pass: read and write 3 nz fy fxh words, read 6 (fz/2+1) fy fxh tensor words;
+      row passes: the site arrays of the predictor / corrector (as the layered
+      kernels, plus a per-site axis and K when the easy axes are random) and the
+      rows of W in and out."""
+    nx, ny, nz = dims
+    fx, fy, fz = (2 * nx, 2 * ny, 2 * nz) if open_ else (nx, ny, nz)
+    fxh = fx // 2 + 1
+    n = nx * ny * nz
+    word = 3 * nz * fxh * 8                 # one y row of W, all stored planes and components
+    rows_w = 2 * word * ny                  # row pass: W rows in and out
+    lo = n * 24 if comp else 0
+    cls = n if open_ else 0                 # exchange class byte per site (faces of an open lattice)
+    axes = n * 16 if site_axes else 0
+    return {"k3_ypass_fwd": word * (ny + fy), "k3_ypass_inv": word * (ny + fy),
+            "k3_zconv": 2 * word * fy + 6 * (fz // 2 + 1) * fy * fxh * 4,
+            "k3_rows_pred": n * 16 + n * 24 + lo + cls + axes + rows_w,
+            "k3_rows_corr": n * 28 + n * 12 + lo + cls + axes + rows_w}
+
+
+def pass_roofline(prof, bytes_per_pass, step_ms, step_bytes):
+    """roofline sub-object of a leg: every pass's achieved GB/s from its algorithmic
+    bytes and event-timed duration, the dominant pass (most time per step), and the
+    whole step (all passes' bytes over the driver-timed step)."""
+    passes = {}
+    for k, ms in prof.items():
+        if k in bytes_per_pass and ms > 0:
+            gbs = bytes_per_pass[k] / (ms * 1e-3) / 1e9
+            passes[k] = {"avg_ms": round(ms, 4), "algorithmic_bytes": int(bytes_per_pass[k]),
+                         "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    dom = max(passes, key=lambda k: passes[k]["avg_ms"])
+    step_gbs = step_bytes / (step_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": dom, "achieved": passes[dom]["achieved"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": passes[dom]["frac"], "passes": passes,
+            "step": {"bytes": int(step_bytes), "ms": round(step_ms, 4), "achieved": round(step_gbs, 1),
+                     "frac": round(step_gbs / HBM_PEAK_GBS, 4)}}
+
+
 def ensemble_leg(M, W, nrep=64, steps=50, R=16):
     """64 periodic replicas with sizes drawn (seed 0) from {32^3, 64^3, 128x128x16, 96x96x24},
     temperatures 0.1..2.0, each with its own seed, LongRange3D dipole (lattice sum to R)
@@ -356,6 +400,15 @@ def ensemble_leg(M, W, nrep=64, steps=50, R=16):
     E.step(steps)
     M.synchronize()
     el = time.perf_counter() - t0
+    prof = E.profile(5)
+    M.synchronize()
+    tot = {}
+    for i in range(nrep):
+        for k, v in conv3_pass_bytes(sizes[pick[i]], False, True).items():
+            tot[k] = tot.get(k, 0) + v
+    roof = pass_roofline(prof, tot, 1e3 * el / steps,
+                         2 * (tot["k3_ypass_fwd"] + tot["k3_zconv"] + tot["k3_ypass_inv"]) + tot["k3_rows_pred"]
+                         + tot["k3_rows_corr"])
     # per-replica observables after the timed steps (outside the timing): |M| per
     # site from netMoment and the total field's dot with the spins per site, both
     # FP64 device reductions; the fields come from the operator API
@@ -374,6 +427,7 @@ def ensemble_leg(M, W, nrep=64, steps=50, R=16):
                         f"T 0.1..2.0, LongRange3D lattice sum R={R}, g=0.01, J=1, K=0.1",
             "value": round(nsites * steps / el, 1), "unit": "spin-steps/s", "sites": nsites, "steps": steps,
             "ms_per_step": round(1e3 * el / steps, 4), "setup_s": round(setup, 2),
+            "roofline": roof,
             "temperatures": [round(t, 4) for t in temps], "m_abs_per_site": m_abs,
             "total_field_dot_spin_per_site": hdot}
 
@@ -512,8 +566,13 @@ def cube_leg(M, W, n=256, steps=20, learned=True):
         drv.step(steps)
         M.synchronize()
         el = time.perf_counter() - t0
+        prof = drv.profile(3)
+        M.synchronize()
+        pb = {k: v * count for k, v in conv3_pass_bytes(dims, True, True, site_axes=True).items()}
+        step_b = 2 * (pb["k3_ypass_fwd"] + pb["k3_zconv"] + pb["k3_ypass_inv"]) + pb["k3_rows_pred"] + pb["k3_rows_corr"]
         out[label] = {"lattice": list(dims), "replicas": count, "value": round(nsites * steps / el, 1),
-                      "ms_per_step": round(1e3 * el / steps, 3), "setup_s": round(setup, 2)}
+                      "ms_per_step": round(1e3 * el / steps, 3), "setup_s": round(setup, 2),
+                      "roofline": pass_roofline(prof, pb, 1e3 * el / steps, step_b)}
         if count == 1 and learned:
             g, ani, lr = keep[0]
             out[label]["learned"] = cube_learned(M, W, spec, g, ani, lr, drv, nsites * steps / el)

@@ -241,7 +241,11 @@ int msd_dipole_destroy(msd_dipole* d);
 int msd_dipole_set_strength(msd_dipole* d, double g);
 int msd_dipole_set_scale(msd_dipole* d, double s);
 /* dip:setTruncation: gmax = {radial, x, y, z}; the x image limit reads gmax[2] and
- * the y limit gmax[3] as in getWAB_range (cpu/dipole/dipolesupport.cpp:71-127). */
+ * the y limit gmax[3] as in getWAB_range (cpu/dipole/dipolesupport.cpp:71-127).
+ * gmax[0] = -1 is setTruncation(math.huge) (the reference's marker,
+ * dipolesupport.cpp:511-519): the sum over the images |X|, |Y| <= 4999 that the
+ * reference's extrapolation loop ends with (:756-840), ~1e8 image terms per tensor
+ * element in FP64 on the device. */
 int msd_dipole_set_truncation(msd_dipole* d, const int gmax[4]);
 /* dip:setUnitCell(A,B,C): ABC row-major {A;B;C} */
 int msd_dipole_set_unit_cell(msd_dipole* d, const double ABC[9]);
@@ -451,6 +455,10 @@ int msd_ensemble_destroy(msd_ensemble* e);
 int msd_ensemble_set_graph(msd_ensemble* e, int yes);
 int msd_ensemble_set_compensated(msd_ensemble* e, int yes);
 int msd_ensemble_heun(msd_ensemble* e, int nsteps);
+/* nsteps more steps with HIP events around every pass group of a step (the
+ * LongRange3D members' y / z / inverse-y / row passes, the film groups' passes);
+ * names[i] and avg_ms[i] per pass group, *count of them (at most cap). */
+int msd_ensemble_profile(msd_ensemble* e, int nsteps, int cap, char names[][32], double* avg_ms, int* count);
 
 #ifdef __cplusplus
 }

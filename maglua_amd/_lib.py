@@ -154,6 +154,7 @@ _SIGS = {
     "msd_ensemble_set_graph": (C.c_int, [_vp, C.c_int]),
     "msd_ensemble_set_compensated": (C.c_int, [_vp, C.c_int]),
     "msd_ensemble_heun": (C.c_int, [_vp, C.c_int]),
+    "msd_ensemble_profile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_void_p, _dp, _ip]),
     "msd_llg_create": (C.c_int, [_pp]),
     "msd_llg_create_quaternion": (C.c_int, [_pp]),
     "msd_llg_destroy": (C.c_int, [_vp]),

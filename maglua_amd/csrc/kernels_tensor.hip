@@ -74,6 +74,70 @@ hipError_t launch_tensor_sum(int nx, int ny, int nz, const int gmax[4], const do
     return hipGetLastError();
 }
 
+// setTruncation(math.huge) (cpu/dipole/dipolesupport.cpp:756-840): the reference sums
+// rings of image lattices 5 wide and asks extrapolate.lua for the limit after each,
+// but reads the script's success flag from a padding nil (lua_pcall(L, 1, 2, 0) with
+// a one-value script, then lua_toboolean(L, -1)), so every element runs its 1000
+// rings and keeps the sum over the images |X|, |Y| <= 4999 with no radial or
+// in-plane cut.  One workgroup per element (in-plane offset i, j, layer offset c):
+// thread t sums the image rows X = -4999 + t, t + HUGE_TPB, ..., each over all Y, and
+// the partial sums are added in a fixed tree, so the result is reproducible.
+constexpr int HUGE_TPB = 256, HUGE_HALF = 4999;
+
+__global__ void __launch_bounds__(HUGE_TPB) k_tensor_huge(int nx, int ny, int nz, Mat9 M, size_t first,
+                                                          double* __restrict__ T)
+{
+    __shared__ double part[6][HUGE_TPB];
+    const int nl = 2 * nz - 1;
+    const size_t plane = (size_t)nx * ny * nl;
+    const size_t idx = first + blockIdx.x;
+    const int i = (int)(idx % nx);
+    const int j = (int)((idx / nx) % ny);
+    const long long zz = (long long)(idx / ((size_t)nx * ny)) - nz + 1;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int X = -HUGE_HALF + (int)threadIdx.x; X <= HUGE_HALF; X += HUGE_TPB) {
+        const double xx = (double)((long long)X * nx + i);
+        for (int Y = -HUGE_HALF; Y <= HUGE_HALF; ++Y) {
+            const double yy = (double)((long long)Y * ny + j);
+            const double rx = xx * M.a[0] + yy * M.a[3] + (double)zz * M.a[6];
+            const double ry = xx * M.a[1] + yy * M.a[4] + (double)zz * M.a[7];
+            const double rz = xx * M.a[2] + yy * M.a[5] + (double)zz * M.a[8];
+            const double r = sqrt(rx * rx + ry * ry + rz * rz);
+            if (r < 1e-40) continue;
+            const double ir = 1.0 / r, ir3 = ir * ir * ir, ir5 = ir3 * ir * ir;
+            acc[0] += -1.0 * (ir3 - 3.0 * rx * rx * ir5);
+            acc[1] += -1.0 * (-3.0 * rx * ry * ir5);
+            acc[2] += -1.0 * (-3.0 * rx * rz * ir5);
+            acc[3] += -1.0 * (ir3 - 3.0 * ry * ry * ir5);
+            acc[4] += -1.0 * (-3.0 * ry * rz * ir5);
+            acc[5] += -1.0 * (ir3 - 3.0 * rz * rz * ir5);
+        }
+    }
+    for (int a = 0; a < 6; ++a) part[a][threadIdx.x] = acc[a];
+    __syncthreads();
+    for (int w = HUGE_TPB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int a = 0; a < 6; ++a) part[a][threadIdx.x] += part[a][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) T[(size_t)threadIdx.x * plane + idx] = part[threadIdx.x][0];
+}
+
+// launched in chunks of elements (each element is ~1e8 image terms, ~50 ms of a CU)
+hipError_t launch_tensor_huge(int nx, int ny, int nz, const double ABC[9], double* T, hipStream_t st)
+{
+    Mat9 M;
+    for (int k = 0; k < 9; ++k) M.a[k] = ABC[k];
+    const size_t total = (size_t)nx * ny * (2 * nz - 1), chunk = 4096;
+    for (size_t first = 0; first < total; first += chunk) {
+        const size_t n = std::min(chunk, total - first);
+        k_tensor_huge<<<(unsigned)n, HUGE_TPB, 0, st>>>(nx, ny, nz, M, first, T);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 // x-DFT of layers z >= 0: U[a][zz][y][kx] = sum_x T[a][nz-1+zz][y][x] e^{-2 pi i kx x / nx}
 __global__ void k_tensor_dftx(int nx, int ny, int nz, const double* __restrict__ T, double2* __restrict__ U,
                               const double2* __restrict__ tw)

@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <ctime>
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
@@ -1702,7 +1703,9 @@ extern "C" int msd_dipole_set_truncation(msd_dipole* d, const int gmax[4])
 {
     NONNULL(d); NONNULL(gmax);
     LATTICE_ONLY(d, "setTruncation");
-    if (gmax[0] < 1) return fail(MSD_EUNSUPPORTED, "setTruncation: radial limit %d (math.huge extrapolation is not supported)", gmax[0]);
+    // -1: math.huge (the reference's marker, dipolesupport.cpp:511-519); its sum is the
+    // square of images |X|, |Y| <= 4999 (kernels_tensor.hip:k_tensor_huge)
+    if (gmax[0] < 1 && gmax[0] != -1) return fail(MSD_EINVAL, "setTruncation: radial limit %d", gmax[0]);
     for (int i = 0; i < 4; ++i) d->gmax[i] = gmax[i];
     d->loaded = false; d->custom = false; d->spectrum = false;
     return MSD_OK;
@@ -1776,7 +1779,9 @@ static int dipole_load(msd_dipole* d)
     const size_t m = (size_t)6 * (2 * d->d.nz - 1) * d->d.nx * d->d.ny;
     if (!d->T64)
         if (int rc = dalloc(&d->T64, m)) return rc;
-    hipError_t e = launch_tensor_sum(d->d.nx, d->d.ny, d->d.nz, d->gmax, d->ABC, d->T64, G.stream);
+    // gmax[0] = -1 marks setTruncation(math.huge) (dipolesupport.cpp:511-519, 756)
+    hipError_t e = d->gmax[0] == -1 ? launch_tensor_huge(d->d.nx, d->d.ny, d->d.nz, d->ABC, d->T64, G.stream)
+                                    : launch_tensor_sum(d->d.nx, d->d.ny, d->d.nz, d->gmax, d->ABC, d->T64, G.stream);
     if (e != hipSuccess) return fail(MSD_EDEVICE, "dipole tensor: %s", hipGetErrorString(e));
     d->loaded = true;
     d->spectrum = false;
@@ -4030,6 +4035,88 @@ extern "C" int msd_ensemble_set_compensated(msd_ensemble* e, int yes)
     e->compensated = yes != 0;
     e->drop_graph();
     return MSD_OK;
+}
+
+// Event-timed passes of an ensemble step, for nsteps steps of progress (direct
+// launches, no graph): the LongRange3D members' y, z and inverse y passes and row
+// passes, and the film groups' column and row passes, each pass group between two
+// events on the library stream.  Returns pass name -> average ms per step's launch.
+extern "C" int msd_ensemble_profile(msd_ensemble* e, int nsteps, int cap, char names[][32], double* avg_ms, int* count)
+{
+    NONNULL(e); NONNULL(names); NONNULL(avg_ms); NONNULL(count);
+    if (nsteps <= 0) return fail(MSD_EINVAL, "ensemble profile: nsteps %d", nsteps);
+    if (int rc = e->refresh()) return rc;
+    bool stale = false;
+    for (msd_stepper* s : e->m) {
+        if (int rc = member_begin(s)) return rc;
+        stale |= s->lr3 && spectrum_stale(s);
+        if (s->kind() == SK_LAYERED && spectrum_stale(s)) {
+            hipError_t err = launch_rows_fwd(s->args.dp, s->s->S, G.stream);
+            if (err != hipSuccess) return fail(MSD_EDEVICE, "ensemble profile: %s", hipGetErrorString(err));
+        }
+    }
+    if (stale) {
+        hipError_t err = e->b.rows(R3_FWD, false, G.stream);
+        if (err != hipSuccess) return fail(MSD_EDEVICE, "ensemble profile: %s", hipGetErrorString(err));
+    }
+    struct Pass {
+        const char* name;
+        std::function<hipError_t(hipStream_t)> run;
+    };
+    std::vector<Pass> seq;
+    for (int half = 0; half < 2; ++half) {
+        const bool pred = half == 0;
+        if (e->b.any_lr) {
+            seq.push_back({"k3_ypass_fwd", [e](hipStream_t st) { return e->b.ypass(-1, st); }});
+            seq.push_back({"k3_zconv", [e](hipStream_t st) { return e->b.zpass(st); }});
+            seq.push_back({"k3_ypass_inv", [e](hipStream_t st) { return e->b.ypass(1, st); }});
+        }
+        if (!e->b.rg.empty())
+            seq.push_back({pred ? "k3_rows_pred" : "k3_rows_corr",
+                           [e, pred](hipStream_t st) { return e->b.rows(pred ? R3_PRED : R3_CORR, e->compensated, st); }});
+        if (!e->films.empty())
+            seq.push_back({pred ? "films_pred" : "films_corr", [e, pred](hipStream_t st) {
+                               hipError_t r = hipSuccess;
+                               for (const FilmGroup& f : e->films)
+                                   if (r == hipSuccess) r = f.step(pred, st);
+                               return r;
+                           }});
+    }
+    const size_t per = seq.size();
+    std::vector<hipEvent_t> ev(per * (size_t)nsteps + 1);
+    for (auto& x : ev) HIPCHK(hipEventCreate(&x));
+    int rc = MSD_OK;
+    size_t q = 0;
+    HIPCHK(hipEventRecord(ev[q++], G.stream));
+    for (int i = 0; i < nsteps && !rc; ++i)
+        for (size_t k = 0; k < per && !rc; ++k) {
+            hipError_t err = seq[k].run(G.stream);
+            if (err == hipSuccess) err = hipEventRecord(ev[q++], G.stream);
+            if (err != hipSuccess) rc = fail(MSD_EDEVICE, "ensemble profile: %s", hipGetErrorString(err));
+        }
+    if (!rc) {
+        HIPCHK(hipStreamSynchronize(G.stream));
+        std::vector<std::string> nm;
+        std::vector<double> tot;
+        std::vector<int> cnt;
+        for (size_t j = 1; j < q; ++j) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, ev[j - 1], ev[j]));
+            const std::string name = seq[(j - 1) % per].name;
+            size_t u = 0;
+            while (u < nm.size() && nm[u] != name) ++u;
+            if (u == nm.size()) { nm.push_back(name); tot.push_back(0); cnt.push_back(0); }
+            tot[u] += ms; cnt[u]++;
+        }
+        *count = std::min<int>((int)nm.size(), cap);
+        for (int u = 0; u < *count; ++u) {
+            std::snprintf(names[u], 32, "%s", nm[u].c_str());
+            avg_ms[u] = tot[u] / cnt[u];
+        }
+        for (msd_stepper* s : e->m) stepper_end(s, nsteps);
+    }
+    for (auto& x : ev) (void)hipEventDestroy(x);
+    return rc;
 }
 
 extern "C" int msd_ensemble_heun(msd_ensemble* e, int nsteps)

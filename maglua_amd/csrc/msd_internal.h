@@ -133,6 +133,8 @@ hipError_t launch_rows_inv(const DipoleDev& dp, float* h, hipStream_t st);
 
 // dipole tensor precompute in FP64
 hipError_t launch_tensor_sum(int nx, int ny, int nz, const int gmax[4], const double ABC[9], double* T, hipStream_t st);
+// setTruncation(math.huge): the square image sum |X|, |Y| <= 4999 the reference ends with
+hipError_t launch_tensor_huge(int nx, int ny, int nz, const double ABC[9], double* T, hipStream_t st);
 // T: [6][2nz-1][ny][nx] real space; U scratch [6][nz][ny][nx/2+1] complex; W out [6][nz][nx/2+1][ny] words
 // twx/twy: FP64 tables (cos, -sin)(2 pi k / L)
 hipError_t launch_tensor_spectrum(int nx, int ny, int nz, const double* T, double2* U, double* W,

@@ -708,11 +708,11 @@ class Dipole(_Handle):
 
     def setTruncation(self, *a):
         """l_settrunc (spinoperationlongrange.cpp:505-573): a number sets all four
-        limits; a table {x,y,z} (or three numbers) sets x,y,z and radial = max."""
+        limits; a table {x,y,z} (or three numbers) sets x,y,z and radial = max.
+        math.huge (float("inf")) is the reference's extrapolated sum, which ends as the
+        square of images |X|, |Y| <= 4999 (dipolesupport.cpp:756-840)."""
         if len(a) == 1 and np.isscalar(a[0]):
-            if a[0] == float("inf"):
-                raise MagLuaError(-3, "setTruncation(math.huge): extrapolated lattice sums are not supported")
-            g = (int(a[0]),) * 4
+            g = (-1,) * 4 if a[0] == float("inf") else (int(a[0]),) * 4
         else:
             v = tuple(a[0]) if len(a) == 1 else tuple(a)
             lim = [max(1, int(t)) for t in (tuple(v) + (1, 1, 1))[:3]]
@@ -721,7 +721,7 @@ class Dipole(_Handle):
         call("msd_dipole_set_truncation", self._h, (C.c_int * 4)(*g))
 
     def truncation(self):
-        return self._gmax
+        return tuple(float("inf") if v == -1 else v for v in self._gmax)
 
     def setUnitCell(self, A, B, Cc):
         m = np.ascontiguousarray(np.array([A, B, Cc], dtype=np.float64).reshape(-1))
@@ -1549,6 +1549,19 @@ class Ensemble(_Handle):
         return True
 
     __call__ = step
+
+    def profile(self, n):
+        """n more steps with HIP events around every pass group of a step; returns pass
+        group -> average ms per launch (k3_ypass_fwd, k3_zconv, k3_ypass_inv,
+        k3_rows_pred, k3_rows_corr, films_pred, films_corr)."""
+        for m in self._members:
+            m._sync()
+        cap = 8
+        names = (C.c_char * 32 * cap)()
+        avg = (C.c_double * cap)()
+        cnt = C.c_int()
+        call("msd_ensemble_profile", self._h, int(n), cap, names, avg, C.byref(cnt))
+        return {bytes(names[i]).split(b"\0")[0].decode(): avg[i] for i in range(cnt.value)}
 
 
 def synchronize():

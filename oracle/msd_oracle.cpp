@@ -359,10 +359,62 @@ static void wab(const double* ABC, int nA, int nB, int ix, int iy, int iz, const
     }
 }
 
+/*
+ * setTruncation(math.huge) (gmax[0] = -1; dipolesupport.cpp:756-840): rings of image
+ * lattices 5 wide, [lmin, lmax) = [0,5), [5,10), ..., each summed by getWAB (:134-227:
+ * the full square for lmin = 0, else the 8 pieces of the annulus between the
+ * squares of half-width lmin-1 and lmax-1, in the reference's piece order) with the
+ * limits big = {1e9, ...} (no radial or in-plane cut at these sizes) and added to
+ * the running sums.  Every ring from the ninth on asks extrapolate.lua for the limit,
+ * but the call reads its success flag from the second of two results
+ * (lua_pcall(L, 1, 2, 0), lua_toboolean(L, -1)) and the script returns one value, so
+ * the flag is always the padding nil: no ring ever converges, the loop stops at
+ * maxiter = 1000 rings and keeps the last partial sum, images |X|, |Y| <= 4999.
+ * The z test reads truemax[4], one past `big`; no z cut is applied.
+ */
+static const int HUGE_RINGS = 1000, HUGE_STEP = 5;
+
+static void wab_box(const double* ABC, int nA, int nB, int ix, int iy, long long zz, long long ax, long long ay,
+                    long long bx, long long by, double acc[6])
+{
+    for (long long X = ax; X <= bx; ++X) {
+        const long long xx = X * nA + ix;
+        for (long long Y = ay; Y <= by; ++Y) {
+            const long long yy = Y * nB + iy;
+            const double rx = (double)xx * ABC[0] + (double)yy * ABC[3] + (double)zz * ABC[6];
+            const double ry = (double)xx * ABC[1] + (double)yy * ABC[4] + (double)zz * ABC[7];
+            const double rz = (double)xx * ABC[2] + (double)yy * ABC[5] + (double)zz * ABC[8];
+            double g[6];
+            gamma_dip(rx, ry, rz, g);
+            for (int i = 0; i < 6; ++i) acc[i] += g[i];
+        }
+    }
+}
+
+static void wab_huge(const double* ABC, int nA, int nB, int ix, int iy, int iz, double sum[6])
+{
+    for (int i = 0; i < 6; ++i) sum[i] = 0;
+    for (int q = 0; q < HUGE_RINGS; ++q) {
+        const long long mn = (long long)q * HUGE_STEP, mx = mn + HUGE_STEP;
+        double ring[6] = {0, 0, 0, 0, 0, 0};
+        if (mn == 0) {
+            wab_box(ABC, nA, nB, ix, iy, iz, -(mx - 1), -(mx - 1), mx - 1, mx - 1, ring);
+        } else {
+            const long long ax[8] = {-(mx - 1), -mn + 1, mn, -(mx - 1), mn, -(mx - 1), -mn + 1, mn};
+            const long long bx[8] = {-mn, mn - 1, mx - 1, -mn, mx - 1, -mn, mn - 1, mx - 1};
+            const long long ay[8] = {mn, mn, mn, -(mn - 1), -(mn - 1), -(mx - 1), -(mx - 1), -(mx - 1)};
+            const long long by[8] = {mx - 1, mx - 1, mx - 1, mn - 1, mn - 1, -mn, -mn, -mn};
+            for (int p = 0; p < 8; ++p) wab_box(ABC, nA, nB, ix, iy, iz, ax[p], ay[p], bx[p], by[p], ring);
+        }
+        for (int i = 0; i < 6; ++i) sum[i] += ring[i];
+    }
+}
+
 extern "C" int ora_dipole_tensor(int nx, int ny, int nz, const int gmax[4], const double ABC[9],
                                  double* XX, double* XY, double* XZ, double* YY, double* YZ, double* ZZ)
 {
-    if (nx < 1 || ny < 1 || nz < 1 || gmax[0] < 1) return -1;
+    if (nx < 1 || ny < 1 || nz < 1 || (gmax[0] < 1 && gmax[0] != -1)) return -1;
+    const bool huge = gmax[0] == -1;
     const int nl = 2 * nz - 1;
     double* out[6] = {XX, XY, XZ, YY, YZ, ZZ};
 #pragma omp parallel for collapse(3) schedule(dynamic, 16)
@@ -371,7 +423,8 @@ extern "C" int ora_dipole_tensor(int nx, int ny, int nz, const int gmax[4], cons
             for (int i = 0; i < nx; ++i) {
                 const int k = c - nz + 1;
                 double acc[6];
-                wab(ABC, nx, ny, i, j, k, gmax, acc);
+                if (huge) wab_huge(ABC, nx, ny, i, j, k, acc);
+                else wab(ABC, nx, ny, i, j, k, gmax, acc);
                 const size_t idx = (size_t)i + (size_t)nx * (j + (size_t)ny * c);
                 for (int a = 0; a < 6; ++a) out[a][idx] = acc[a];
             }

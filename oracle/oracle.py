@@ -453,7 +453,7 @@ class Dipole:
 
     def setTruncation(self, gmax):
         if np.isscalar(gmax):
-            gmax = (int(gmax),) * 4
+            gmax = (-1,) * 4 if gmax == float("inf") else (int(gmax),) * 4
         self.L.ora_lr_set_truncation(self.h, (C.c_int * 4)(*gmax))
 
     def setUnitCell(self, A, B, Cc):

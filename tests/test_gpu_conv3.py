@@ -304,3 +304,30 @@ def test_ensemble_rejects_films_sharing_a_dipole():
     hb = M.Heun(b["ss"], b["ex"], b["ani"], a["dip"], None, b["llg"])
     with pytest.raises(M.MagLuaError, match="share a Dipole"):
         M.Ensemble([ha, hb])
+
+
+def test_ensemble_profile_steps_like_the_ensemble():
+    """Ensemble.profile(n) times each pass group and leaves every member n steps ahead,
+    bit-identical to Ensemble.step(n)."""
+    runs = []
+    for prof in (False, True):
+        hs = []
+        for dims, T, seed in (((64, 64, 4), 0.3, 1), ((32, 32, 2), 0.0, 3)):
+            spec, g = film(dims, T, seed)
+            hs.append((g, M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"])))
+        spec, g, o, gl, ol = cube((8, 8, 8), 0.5, 9)
+        hs.append((g, M.Heun(g["ss"], g["ex"], g["ani"], gl, g["temp"], g["llg"])))
+        E = M.Ensemble([h for (_, h) in hs])
+        E.step(2)
+        if prof:
+            p = E.profile(4)
+            print(f"ensemble profile: {p}", flush=True)
+            assert set(p) == {"k3_ypass_fwd", "k3_zconv", "k3_ypass_inv", "k3_rows_pred", "k3_rows_corr",
+                              "films_pred", "films_corr"}
+            assert all(v > 0 for v in p.values())
+        else:
+            E.step(4)
+        E.step(1)
+        runs.append([(flat(g["ss"].spins()), g["ss"].time()) for (g, _) in hs])
+    for (a, ta), (b, tb) in zip(*runs):
+        assert np.array_equal(a, b) and ta == tb

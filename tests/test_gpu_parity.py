@@ -162,6 +162,38 @@ def test_dipole_op_full_size_film():
     assert np.abs(hg - ho).max() <= 2e-5 * np.abs(ho).max()
 
 
+def test_dipole_math_huge_against_oracle():
+    """setTruncation(math.huge) on a 4x2x2 film: every tensor element (~1e8 image terms
+    each, FP64 on the device) and one Dipole:apply against the oracle's ring sum."""
+    dims = (4, 2, 2)
+    gd, od = M.Dipole(*dims), O.Dipole(*dims, threads=True)
+    gd.setTruncation(float("inf"))
+    od.setTruncation(float("inf"))
+    assert gd.truncation() == (float("inf"),) * 4
+    T = O.dipole_tensor(*dims, gmax=(-1, -1, -1, -1))
+    worst, big = 0.0, max(np.abs(v).max() for v in T.values())
+    for name in M.Dipole.NAMES:
+        for c in range(3):
+            for y in range(2):
+                for x in range(4):
+                    v = gd.getMatrix(name, (x, y, c - 1))
+                    worst = max(worst, abs(v - T[name][c, y, x]))
+    print(f"math.huge tensor 4x2x2: max |diff| = {worst:.3e} (max |T| = {big:.3f})", flush=True)
+    assert worst <= 1e-12 * big
+    s = np.random.default_rng(5).standard_normal((3, 16))
+    gs, os_ = M.SpinSystem(*dims), O.SpinSystem(*dims)
+    gs.setSpins(*s)
+    os_.setSpins(*s)
+    for d in (gd, od):
+        d.setStrength(0.5)
+    gd.apply(gs)
+    od.apply(os_)
+    hg, ho = flat(gs.field("Dipole")), flat(os_.field("Dipole"))
+    err = float(np.abs(hg - ho).max() / np.abs(ho).max())
+    print(f"math.huge Dipole:apply 4x2x2: max rel err = {err:.3e}", flush=True)
+    assert err <= 2e-6
+
+
 def test_dipole_matrix_get_set():
     gd, od = M.Dipole(8, 6, 2), O.Dipole(8, 6, 2)
     for d in (gd, od):

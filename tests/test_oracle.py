@@ -129,6 +129,32 @@ def test_square_lattice_dipole_sum():
     assert abs(T["XX"].sum() - s / 2) < 1e-9
 
 
+def test_math_huge_tensor_is_the_square_image_sum():
+    """setTruncation(math.huge): the reference's 1000 rings of 5 lattices end as the sum
+    over the images |X|, |Y| <= 4999 with no cut (its extrapolation never reports
+    success).  Pinned on the 1x1 lattice by a direct numpy sum over that square, by the
+    closed form -ZZ = 4 zeta(3/2) beta(3/2) - 4 sqrt(2) / (L + 1/2) (the square's
+    1/r^3 tail to leading order), by XX = YY = -ZZ/2 (square symmetry) and the trace."""
+    L = 4999
+    T = O.dipole_tensor(1, 1, 1, gmax=(-1, -1, -1, -1), threads=False)
+    got = np.array([T[k][0, 0, 0] for k in ("XX", "XY", "XZ", "YY", "YZ", "ZZ")])
+    acc = np.zeros(6)
+    ys = np.arange(-L, L + 1, dtype=np.float64)[None, :]
+    for x0 in range(-L, L + 1, 500):
+        xs = np.arange(x0, min(x0 + 500, L + 1), dtype=np.float64)[:, None]
+        r2 = xs * xs + ys * ys
+        m = r2 > 0
+        ir = np.where(m, 1.0 / np.sqrt(np.where(m, r2, 1.0)), 0.0)
+        ir3, ir5 = ir ** 3, ir ** 5
+        acc += [(-(ir3 - 3 * xs * xs * ir5)).sum(), (3 * xs * ys * ir5).sum(), 0.0,
+                (-(ir3 - 3 * ys * ys * ir5)).sum(), 0.0, (-ir3).sum()]
+    assert np.abs(got - acc).max() <= 1e-13 * np.abs(acc).max()
+    s_inf = 4 * 2.612375348685488 * 0.8645026534612020
+    assert abs(-got[5] - (s_inf - 4 * np.sqrt(2) / (L + 0.5))) < 1e-7
+    assert abs(got[0] - got[3]) < 1e-12 and abs(got[0] + got[5] / 2) < 1e-12
+    assert abs(got[0] + got[3] + got[5]) < 1e-12
+
+
 # ---------------------------------------------------------- layered dipole
 def realspace_dipole(spins, gm, g):
     """h_t = g sum_s sum_images Gamma(r) m_s with in-plane r = r_t - r_s (+ images) and
