@@ -296,6 +296,11 @@ int msd_lr2d_get_layer_scales(msd_dipole* d, double* src_moment, double* dest_fi
  * the box.  physical = 1 places the destination at the origin (the truncated in-plane
  * dipole sum).  The file cache of the reference (GAB_*.lua) is not kept. */
 int msd_lr2d_build_dipole2d(msd_dipole* d, const double* ABC, const double* offset, const int trunc[4], int physical);
+/* Magnetostatics2D makeData (cpu/magnetostatics2d/mag2d_luafuncs.lua:598-750) into a
+ * LongRange2D operator: ABC [nz][3][3] unit cells, grain [nz][3] prism sizes, trunc =
+ * {truncation, X, Y, Z}; the prism-prism tensors of cpu/magnetostatics2d/demag.cpp, -N
+ * summed over the images of each cell, stored under the labels [source][destination]. */
+int msd_lr2d_build_magnetostatics2d(msd_dipole* d, const double* ABC, const double* grain, const int trunc[4]);
 
 /* ------------------------------------------------------------- LongRange3D */
 /* LongRange3D.new: 3D FFT convolution, cpu/longrange3d/spinoperationlongrange3d.cpp:331-432.

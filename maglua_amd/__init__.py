@@ -10,14 +10,14 @@ from .checkpoint import (  # noqa: F401
 )
 from .maglua import (  # noqa: F401
     BUTCHER, LLG, AppliedField, Anisotropy, Dipole, Dipole2D, Ensemble, Exchange, Heun, LongRange, LongRange2D,
-    LongRange3D, Random, SpinSystem,
+    LongRange3D, Magnetostatics2D, Random, SpinSystem,
     Surrogate, Thermal, make_adapt_step_function, make_pc_step_function, make_rk_step_function, philox_device,
     synchronize,
 )
 
 __all__ = [
     "MagLuaError", "SpinSystem", "Exchange", "Anisotropy", "AppliedField", "Thermal", "Dipole", "LongRange",
-    "LongRange3D", "LongRange2D", "Dipole2D",
+    "LongRange3D", "LongRange2D", "Dipole2D", "Magnetostatics2D",
     "LLG", "Random", "Heun", "Ensemble", "Surrogate", "make_pc_step_function", "make_rk_step_function",
     "make_adapt_step_function", "BUTCHER",
     "synchronize", "philox_device", "lib", "declared_symbols",

@@ -140,6 +140,7 @@ _SIGS = {
     "msd_lr2d_set_layer_scales": (C.c_int, [_vp, _dp, _dp]),
     "msd_lr2d_get_layer_scales": (C.c_int, [_vp, _dp, _dp]),
     "msd_lr2d_build_dipole2d": (C.c_int, [_vp, _dp, _dp, _ip, C.c_int]),
+    "msd_lr2d_build_magnetostatics2d": (C.c_int, [_vp, _dp, _dp, _ip]),
     "msd_longrange3d_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _pp]),
     "msd_longrange3d_destroy": (C.c_int, [_vp]),
     "msd_longrange3d_set_strength": (C.c_int, [_vp, C.c_double]),

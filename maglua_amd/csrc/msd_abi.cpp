@@ -2074,6 +2074,121 @@ extern "C" int msd_lr2d_build_dipole2d(msd_dipole* d, const double* ABC, const d
     return MSD_OK;
 }
 
+// ---------------------------------------------------------- Magnetostatics2D
+// The demagnetising tensor between two rectangular prisms (Newell, Williams and
+// Dunlop, J. Geophys. Res. 98 (1993) 9551; Donahue's form for prisms of unequal
+// size): with Newell's f (diagonal) and g (off-diagonal) potentials,
+//   N_xx(r; s, d) = D_x D_y D_z f(r) / (4 pi V_d),  D g(u) = g(u) - g(u + d) + g(u + d - s) - g(u - s)
+// along each axis, s the source prism, d the destination prism, r the vector from the
+// source's origin corner to the destination's.  N is the destination average of the
+// field of the uniformly magnetised source, per unit magnetisation, with the sign that
+// makes a cube's self term +1/3.  The other components permute the axes.
+namespace {
+double nw_asinh(double num, double den) { return den > 0.0 ? std::asinh(num / den) : 0.0; }
+double nw_atan(double num, double den) { return (num == 0.0 && den == 0.0) ? 0.0 : std::atan2(num, den); }
+
+// f(x, y, z) = y/2 (z^2 - x^2) asinh(y / sqrt(x^2 + z^2)) + z/2 (y^2 - x^2) asinh(z / sqrt(x^2 + y^2))
+//              - x y z atan(y z / (x R)) + (2 x^2 - y^2 - z^2) R / 6, even in every argument
+double newell_f(double X, double Y, double Z)
+{
+    const double x = std::fabs(X), y = std::fabs(Y), z = std::fabs(Z);
+    const double x2 = x * x, y2 = y * y, z2 = z * z, R = std::sqrt(x2 + y2 + z2);
+    return 0.5 * y * (z2 - x2) * nw_asinh(y, std::sqrt(x2 + z2)) + 0.5 * z * (y2 - x2) * nw_asinh(z, std::sqrt(x2 + y2)) -
+           x * y * z * nw_atan(y * z, x * R) + (2.0 * x2 - y2 - z2) * R / 6.0;
+}
+
+// g(x, y, z) = x y z asinh(z / sqrt(x^2 + y^2)) + y/6 (3 z^2 - y^2) asinh(x / sqrt(y^2 + z^2))
+//            + x/6 (3 z^2 - x^2) asinh(y / sqrt(x^2 + z^2)) - z^3/6 atan(x y / (z R))
+//            - z y^2/2 atan(x z / (y R)) - z x^2/2 atan(y z / (x R)) - x y R / 3,
+// odd in x and in y, even in z
+double newell_g(double X, double Y, double Z)
+{
+    const double x = std::fabs(X), y = std::fabs(Y), z = std::fabs(Z);
+    const double x2 = x * x, y2 = y * y, z2 = z * z, R = std::sqrt(x2 + y2 + z2);
+    const double v = x * y * z * nw_asinh(z, std::sqrt(x2 + y2)) + y * (3.0 * z2 - y2) / 6.0 * nw_asinh(x, std::sqrt(y2 + z2)) +
+                     x * (3.0 * z2 - x2) / 6.0 * nw_asinh(y, std::sqrt(x2 + z2)) - z2 * z / 6.0 * nw_atan(x * y, z * R) -
+                     0.5 * z * y2 * nw_atan(x * z, y * R) - 0.5 * z * x2 * nw_atan(y * z, x * R) - x * y * R / 3.0;
+    return ((X < 0) != (Y < 0)) ? -v : v;
+}
+
+// D_x D_y D_z fn over the corner offsets {0, d, d - s, -s}, signs + - + -
+template <double (*FN)(double, double, double)>
+double prism_difference(double X, double Y, double Z, const double s[3], const double d[3])
+{
+    const double off[3][4] = {{0.0, d[0], d[0] - s[0], -s[0]}, {0.0, d[1], d[1] - s[1], -s[1]}, {0.0, d[2], d[2] - s[2], -s[2]}};
+    double acc = 0.0;
+    for (int a = 0; a < 4; ++a) {
+        double ay = 0.0;
+        for (int b = 0; b < 4; ++b) {
+            double az = 0.0;
+            for (int c = 0; c < 4; ++c) az += ((c & 1) ? -1.0 : 1.0) * FN(X + off[0][a], Y + off[1][b], Z + off[2][c]);
+            ay += ((b & 1) ? -1.0 : 1.0) * az;
+        }
+        acc += ((a & 1) ? -1.0 : 1.0) * ay;
+    }
+    return acc / (4.0 * M_PI * d[0] * d[1] * d[2]);
+}
+
+// the nine components N[ab] (XX XY XZ YX YY YZ ZX ZY ZZ) at r between source s and destination d
+void prism_tensor(const double r[3], const double s[3], const double d[3], double N[9])
+{
+    auto perm = [](const double* v, int i, int j, int k, double* o) { o[0] = v[i]; o[1] = v[j]; o[2] = v[k]; };
+    double ps[3], pd[3];
+    N[0] = prism_difference<newell_f>(r[0], r[1], r[2], s, d);
+    perm(s, 1, 0, 2, ps); perm(d, 1, 0, 2, pd);
+    N[4] = prism_difference<newell_f>(r[1], r[0], r[2], ps, pd);
+    perm(s, 2, 1, 0, ps); perm(d, 2, 1, 0, pd);
+    N[8] = prism_difference<newell_f>(r[2], r[1], r[0], ps, pd);
+    N[1] = N[3] = prism_difference<newell_g>(r[0], r[1], r[2], s, d);
+    perm(s, 0, 2, 1, ps); perm(d, 0, 2, 1, pd);
+    N[2] = N[6] = prism_difference<newell_g>(r[0], r[2], r[1], ps, pd);
+    perm(s, 1, 2, 0, ps); perm(d, 1, 2, 0, pd);
+    N[5] = N[7] = prism_difference<newell_g>(r[1], r[2], r[0], ps, pd);
+}
+}  // namespace
+
+// Magnetostatics2D makeData (cpu/magnetostatics2d/mag2d_luafuncs.lua:598-750): for source
+// layer z1 and destination z2 with |z1 - z2| <= truncZ, images X, Y in [-trunc, trunc]
+// within truncX, truncY, r = X A + Y B + c(z2) - c(z1) (A, B of the first layer, c(z) the
+// sum of the C vectors of the layers below z), -N(r; grain z1, grain z2) is added at
+// (X mod nx, Y mod ny) of the pair stored under the labels [z1][z2].
+extern "C" int msd_lr2d_build_magnetostatics2d(msd_dipole* d, const double* ABC, const double* grain, const int trunc[4])
+{
+    NONNULL(d); NONNULL(ABC); NONNULL(grain); NONNULL(trunc);
+    if (!d->pair) return fail(MSD_EUNSUPPORTED, "Magnetostatics2D makeData: not a LongRange2D operator");
+    const int nx = d->d.nx, ny = d->d.ny, nz = d->d.nz, t = trunc[0], tX = trunc[1], tY = trunc[2], tZ = trunc[3];
+    if (t < 0 || tX < 0 || tY < 0 || tZ < 0)
+        return fail(MSD_EINVAL, "Magnetostatics2D: negative truncation (%d, %d, %d, %d)", t, tX, tY, tZ);
+    if ((double)(2 * t + 1) * (2 * t + 1) * nz * nz > 1e8)
+        return fail(MSD_EUNSUPPORTED, "Magnetostatics2D: truncation %d too large", t);
+    for (int k = 0; k < 3 * nz; ++k)
+        if (!(grain[k] > 0.0)) return fail(MSD_EINVAL, "Magnetostatics2D: grain sizes must be positive");
+    const double* A = ABC;
+    const double* B = ABC + 3;
+    const size_t plane = (size_t)nx * ny;
+    std::fill(d->P.begin(), d->P.end(), 0.0);
+    for (int z1 = 0; z1 < nz; ++z1)
+        for (int z2 = 0; z2 < nz; ++z2) {
+            if (std::abs(z1 - z2) > tZ) continue;
+            double c[3] = {0.0, 0.0, 0.0};
+            for (int j = std::min(z1, z2); j < std::max(z1, z2); ++j)
+                for (int k = 0; k < 3; ++k) c[k] += (z2 > z1 ? 1.0 : -1.0) * ABC[9 * j + 6 + k];
+            for (int X = -t; X <= t; ++X) {
+                if (std::abs(X) > tX) continue;
+                for (int Y = -t; Y <= t; ++Y) {
+                    if (std::abs(Y) > tY) continue;
+                    const double r[3] = {A[0] * X + B[0] * Y + c[0], A[1] * X + B[1] * Y + c[1], A[2] * X + B[2] * Y + c[2]};
+                    double N[9];
+                    prism_tensor(r, grain + 3 * z1, grain + 3 * z2, N);
+                    const size_t cell = (size_t)(((X % nx) + nx) % nx) + (size_t)nx * (((Y % ny) + ny) % ny);
+                    for (int ab = 0; ab < 9; ++ab) d->P[(((size_t)ab * nz + z1) * nz + z2) * plane + cell] -= N[ab];
+                }
+            }
+        }
+    d->pair_dirty = true;
+    return MSD_OK;
+}
+
 // dipole field of arbitrary device spins s3 ([3][n]) into h ([3][n]): the layered
 // transform of the Dipole operator without a SpinSystem
 static int dipole_field_of(msd_dipole* dip, const float* s3, float* h)

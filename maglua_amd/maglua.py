@@ -1038,6 +1038,101 @@ class Dipole2D(LongRange2D):
         self.setCompileRequired(True)
 
 
+class Magnetostatics2D(LongRange2D):
+    """Magnetostatics2D (cpu/magnetostatics2d): LongRange2D whose tensors are the
+    prism-prism demagnetising tensors of makeData (mag2d_luafuncs.lua:598-750), slot
+    'Magnetostatics2D'.  Internal data (:7-23): per-layer unit cells (identity) and prism
+    (grain) sizes (1, 1, 1), truncation 25 radially and along x, y and z.  -N(r; source
+    grain, destination grain) is summed over the images of each cell and stored under the
+    labels [source][destination].  As in the reference, the scales are the user's: 1/V of
+    the source moments and 4 pi on the destination fields in CGS, through
+    setSourceMomentScale / setDestinationFieldScale.  Changing the cells, grains or
+    truncation after the tensors are made takes setNewDataRequired() to rebuild them."""
+
+    _slot_default = "Magnetostatics2D"
+
+    def __init__(self, *a):
+        super().__init__(*a)
+        self.setMakeDataFunction(Magnetostatics2D._make)
+        self.setInternalData(None)
+
+    def _id(self):
+        """initializeInternalData (:7-23)"""
+        if self._internal is None:
+            nz = self._dims[2]
+            self._internal = {"ABC": [[[1, 0, 0], [0, 1, 0], [0, 0, 1]] for _ in range(nz)],
+                              "grainSize": [[1.0, 1.0, 1.0] for _ in range(nz)],
+                              "truncation": 25, "truncationX": 25, "truncationY": 25, "truncationZ": 25}
+        return self._internal
+
+    def _check_layer(self, layer, what="first argument"):
+        if not isinstance(layer, (int, float, np.integer, np.floating)):
+            raise MagLuaError(-1, "Layer expected as " + what)
+        if layer < 1 or layer > self._dims[2]:
+            raise MagLuaError(-1, "Layer must be in the range [1:%d]" % self._dims[2])
+        return int(layer) - 1
+
+    def setUnitCell(self, layer, A, B=None, Cv=None):
+        """setUnitCell (:64-78)"""
+        k = self._check_layer(layer)
+        try:
+            cell = [[float(v) for v in A], [float(v) for v in B], [float(v) for v in Cv]]
+        except TypeError:
+            raise MagLuaError(-1, "Tables expected for the A, B, C components (2nd through 4th arguments)")
+        self._id()["ABC"][k] = cell
+
+    def unitCell(self, layer):
+        k = self._check_layer(layer)
+        A, B, Cv = self._id()["ABC"][k]
+        return list(A), list(B), list(Cv)
+
+    def setGrainSize(self, layer, X, Y, Z):
+        """setGrainSize (:93-107)"""
+        k = self._check_layer(layer)
+        if not all(isinstance(v, (int, float, np.integer, np.floating)) for v in (X, Y, Z)):
+            raise MagLuaError(-1, "Numbers expected for the X, Y, Z components (2nd through 4th arguments)")
+        self._id()["grainSize"][k] = [float(X), float(Y), float(Z)]
+
+    def grainSize(self, layer):
+        """grainSize (:109-122): x, y, z and the prism volume"""
+        k = self._check_layer(layer, "argument")
+        x, y, z = self._id()["grainSize"][k]
+        return x, y, z, x * y * z
+
+    def setTruncation(self, trunc=None, truncX=None, truncY=None, truncZ=None):
+        """setTruncation (:124-132): a nil argument keeps that limit"""
+        t = self._id()
+        for k, v in (("truncation", trunc), ("truncationX", truncX), ("truncationY", truncY), ("truncationZ", truncZ)):
+            if v is not None:
+                t[k] = int(v)
+
+    def truncation(self):
+        t = self._id()
+        return t["truncation"], t["truncationX"], t["truncationY"], t["truncationZ"]
+
+    def getMatrix(self, dest, src, ab, x, y):
+        """getMatrix (:32-45): makes the data if needed, then tensorArray(dest, src, ab):get(x, y)
+        with 0-based offsets."""
+        self.makeData()
+        return self.tensorArray(dest, src, ab).get(x + 1, y + 1)
+
+    def setMatrix(self, dest, src, ab, x, y, v):
+        """setMatrix (:47-62)"""
+        self.makeData()
+        self.tensorArray(dest, src, ab).set(x + 1, y + 1, v)
+        self.setCompileRequired(True)
+
+    def _make(self):
+        t = self._id()
+        nz = self._dims[2]
+        ABC = np.ascontiguousarray(np.asarray(t["ABC"], dtype=np.float64).reshape(nz * 9))
+        grain = np.ascontiguousarray(np.asarray(t["grainSize"], dtype=np.float64).reshape(nz * 3))
+        tr = (C.c_int * 4)(t["truncation"], t["truncationX"], t["truncationY"], t["truncationZ"])
+        call("msd_lr2d_build_magnetostatics2d", self._h, _d(ABC), _d(grain), tr)
+        self._pull()
+        self.setCompileRequired(True)
+
+
 class LongRange3D(_Handle):
     """LongRange3D (cpu/longrange3d/spinoperationlongrange3d.cpp): 3D FFT convolution, slot
     'LongRange3D'.  open=False: periodic, dipole lattice sum to radius R (setRadius, default

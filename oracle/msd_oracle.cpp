@@ -1639,3 +1639,108 @@ int ora_sim_rk(ora_sim* s, const char* name, int nsteps)
     return 0;
 }
 } /* extern "C" */
+
+/* ------------------------------------------------------------------------ */
+/* Magnetostatics2D makeData (cpu/magnetostatics2d/mag2d_luafuncs.lua:598-750)   */
+/* ------------------------------------------------------------------------ */
+/* Newell's f and g in the form of cpu/magnetostatics2d/demag.cpp:207-260 (logarithms
+ * of squared ratios, two-argument arctangents, each term zero where its ratio is
+ * undefined), and the unequal-prism difference of :267-286: corner offsets
+ * {0, d2, d2 - d1, -d1} with signs {+, -, +, -} per axis, over 4 pi times the second
+ * (destination) prism's volume.  Permutations of :288-352. */
+static double m2_logND(double num, double den) { return (num == 0 || den == 0) ? 0.0 : std::log(num / den); }
+static double m2_atan(double num, double den) { return (num == 0 && den == 0) ? 0.0 : std::atan2(num, den); }
+
+static double m2_f(double X, double Y, double Z)
+{
+    const double x = std::fabs(X), y = std::fabs(Y), z = std::fabs(Z);
+    const double xx = x * x, yy = y * y, zz = z * z, R = std::sqrt(xx + yy + zz);
+    const double ypR2 = (y + R) * (y + R), zpR2 = (z + R) * (z + R);
+    return (1.0 / 12.0) * (2 * (2 * xx - yy - zz) * R - 12.0 * x * y * z * m2_atan(y * z, x * R) +
+                           3 * y * zz * m2_logND(ypR2, xx + zz) - 3 * y * xx * m2_logND(ypR2, xx + zz) +
+                           3 * z * yy * m2_logND(zpR2, xx + yy) - 3 * z * xx * m2_logND(zpR2, xx + yy));
+}
+
+static double m2_g(double X, double Y, double Z)
+{
+    const double x = std::fabs(X), y = std::fabs(Y), z = std::fabs(Z);
+    const double xx = x * x, yy = y * y, zz = z * z, R = std::sqrt(xx + yy + zz);
+    const double xpR2 = (x + R) * (x + R), ypR2 = (y + R) * (y + R), zpR2 = (z + R) * (z + R);
+    const double s = (X < 0 ? -1.0 : 1.0) * (Y < 0 ? -1.0 : 1.0);
+    return (s / 6.0) * (-2 * x * y * R -
+                        3 * z * (xx * m2_atan(y * z, x * R) + yy * m2_atan(x * z, y * R) + (1.0 / 3.0) * zz * m2_atan(x * y, z * R)) +
+                        3 * x * y * z * m2_logND(zpR2, xx + yy) + 0.5 * y * (3 * zz - yy) * m2_logND(xpR2, yy + zz) +
+                        0.5 * x * (3 * zz - xx) * m2_logND(ypR2, xx + zz));
+}
+
+static double m2_sum(double (*fn)(double, double, double), double X, double Y, double Z, const double d1[3], const double d2[3])
+{
+    const double sg[4] = {1.0, -1.0, 1.0, -1.0};
+    const double ox[4] = {0, d2[0], d2[0] - d1[0], -d1[0]};
+    const double oy[4] = {0, d2[1], d2[1] - d1[1], -d1[1]};
+    const double oz[4] = {0, d2[2], d2[2] - d1[2], -d1[2]};
+    double sum = 0;
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b)
+            for (int c = 0; c < 4; ++c) sum += sg[a] * sg[b] * sg[c] * fn(X + ox[a], Y + oy[b], Z + oz[c]);
+    return sum / (4.0 * PI * d2[0] * d2[1] * d2[2]);
+}
+
+/* N_ab for ab = 0..8 (XX XY XZ YX YY YZ ZX ZY ZZ), source prism d1, destination d2 */
+extern "C" double ora_mag2d_element(int ab, double X, double Y, double Z, const double d1[3], const double d2[3])
+{
+    auto P = [](const double* d, int i, int j, int k, double* o) { o[0] = d[i]; o[1] = d[j]; o[2] = d[k]; };
+    double a[3], b[3];
+    switch (ab) {
+        case 0: return m2_sum(m2_f, X, Y, Z, d1, d2);
+        case 4: P(d1, 1, 0, 2, a); P(d2, 1, 0, 2, b); return m2_sum(m2_f, Y, X, Z, a, b);
+        case 8: P(d1, 2, 1, 0, a); P(d2, 2, 1, 0, b); return m2_sum(m2_f, Z, Y, X, a, b);
+        case 1: case 3: return m2_sum(m2_g, X, Y, Z, d1, d2);
+        case 2: case 6: P(d1, 0, 2, 1, a); P(d2, 0, 2, 1, b); return m2_sum(m2_g, X, Z, Y, a, b);
+        case 5: case 7: P(d1, 1, 2, 0, a); P(d2, 1, 2, 0, b); return m2_sum(m2_g, Y, Z, X, a, b);
+        default: return 0.0;
+    }
+}
+
+/* The tensors: for source layer z1 and destination z2 (|z1 - z2| <= truncZ), images X, Y
+ * in [-trunc, trunc] with |X| <= truncX, |Y| <= truncY, r = X A + Y B + c(z2) - c(z1)
+ * (A, B of the first layer, c(z) the sum of the C vectors below layer z), add -N_ab(r;
+ * grain z1, grain z2) at (X mod nx, Y mod ny) of the pair stored under labels
+ * [z1][z2]: T [9][first = source][second = destination][ny][nx]. */
+extern "C" int ora_mag2d_tensor(int nx, int ny, int nz, const double* ABC, const double* grain, const int trunc[4], double* T)
+{
+    if (nx < 1 || ny < 1 || nz < 1) return -1;
+    const size_t nxy = (size_t)nx * ny;
+    std::fill(T, T + 9 * (size_t)nz * nz * nxy, 0.0);
+    auto cum = [&](int layer, double c[3]) {
+        c[0] = c[1] = c[2] = 0;
+        for (int j = 0; j < layer; ++j)
+            for (int k = 0; k < 3; ++k) c[k] += ABC[9 * j + 6 + k];
+    };
+    const double* A = ABC;
+    const double* B = ABC + 3;
+    const int mx = trunc[0];
+#pragma omp parallel for collapse(2) schedule(dynamic)
+    for (int z1 = 0; z1 < nz; ++z1)
+        for (int z2 = 0; z2 < nz; ++z2) {
+            if (std::abs(z1 - z2) > trunc[3]) continue;
+            double c1[3], c2[3];
+            cum(z1, c1);
+            cum(z2, c2);
+            const double* g1 = grain + 3 * z1;
+            const double* g2 = grain + 3 * z2;
+            for (int X = -mx; X <= mx; ++X) {
+                if (std::abs(X) > trunc[1]) continue;
+                for (int Y = -mx; Y <= mx; ++Y) {
+                    if (std::abs(Y) > trunc[2]) continue;
+                    const double rx = A[0] * X + B[0] * Y + (c2[0] - c1[0]);
+                    const double ry = A[1] * X + B[1] * Y + (c2[1] - c1[1]);
+                    const double rz = A[2] * X + B[2] * Y + (c2[2] - c1[2]);
+                    const size_t x = (size_t)(((X % nx) + nx) % nx), y = (size_t)(((Y % ny) + ny) % ny);
+                    for (int ab = 0; ab < 9; ++ab)
+                        T[((size_t)(ab * nz + z1) * nz + z2) * nxy + x + nx * y] -= ora_mag2d_element(ab, rx, ry, rz, g1, g2);
+                }
+            }
+        }
+    return 0;
+}

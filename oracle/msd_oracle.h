@@ -123,6 +123,10 @@ void      ora_lr2d_set_layer_scales(ora_lr2d*, const double* src_moment, const d
 int       ora_lr2d_set_tensor(ora_lr2d*, int dest, int src, int ab, const double* a);
 int       ora_lr2d_get_tensor(ora_lr2d*, int dest, int src, int ab, double* a);
 int       ora_lr2d_apply(ora_lr2d*, ora_sys*, const char* slot);
+/* Magnetostatics2D: prism-prism demag element N_ab (ab = 0..8), and makeData's tensors
+ * T [9][source layer][destination layer][ny][nx] (-N summed over images) */
+double    ora_mag2d_element(int ab, double X, double Y, double Z, const double d1[3], const double d2[3]);
+int       ora_mag2d_tensor(int nx, int ny, int nz, const double* ABC, const double* grain, const int trunc[4], double* T);
 int       ora_dipole2d_tensor(int nx, int ny, int nz, const double* ABC, const double* off, const int trunc[4],
                               int physical, double* T);
 

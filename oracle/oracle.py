@@ -120,6 +120,8 @@ def lib(threads=False):
         "ora_lr2d_get_tensor": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, _dp]),
         "ora_lr2d_apply": (C.c_int, [vp, vp, C.c_char_p]),
         "ora_dipole2d_tensor": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp, C.POINTER(C.c_int), C.c_int, _dp]),
+        "ora_mag2d_element": (C.c_double, [C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp]),
+        "ora_mag2d_tensor": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _dp, C.POINTER(C.c_int), _dp]),
         "ora_sim_set_lr2d": (None, [vp, vp, C.c_char_p]),
         "ora_llg_new": (vp, []),
         "ora_llg_free": (None, [vp]),
@@ -205,6 +207,28 @@ def dipole2d_tensor(nx, ny, nz, ABC=None, offset=None, trunc=(25, 25, 25, 25), p
     off = np.ascontiguousarray(np.zeros((nz, 3)) if offset is None else np.asarray(offset, dtype=np.float64)).reshape(-1)
     out = np.zeros(9 * nz * nz * ny * nx)
     rc = L.ora_dipole2d_tensor(nx, ny, nz, A, off, (C.c_int * 4)(*trunc), int(physical), out)
+    assert rc == 0
+    return out.reshape(9, nz, nz, ny, nx)
+
+
+MAG2D_NAMES = ("XX", "XY", "XZ", "YX", "YY", "YZ", "ZX", "ZY", "ZZ")
+
+
+def mag2d_element(ab, r, d1, d2):
+    """Magnetostatics2D.N<ab>(rx, ry, rz, source prism d1, destination prism d2)."""
+    k = MAG2D_NAMES.index(ab) if isinstance(ab, str) else int(ab)
+    return lib().ora_mag2d_element(k, float(r[0]), float(r[1]), float(r[2]),
+                                   np.ascontiguousarray(d1, dtype=np.float64), np.ascontiguousarray(d2, dtype=np.float64))
+
+
+def mag2d_tensor(nx, ny, nz, ABC=None, grain=None, trunc=(25, 25, 25, 25), threads=True):
+    """Magnetostatics2D makeData: array [9, nz, nz, ny, nx] (XX..ZZ, labels [source][dest]).
+    ABC: [nz][3][3] cell vectors per layer (identity), grain: [nz][3] prism sizes (ones)."""
+    L = lib(threads)
+    A = np.ascontiguousarray(np.tile(np.eye(3), (nz, 1, 1)) if ABC is None else np.asarray(ABC, dtype=np.float64)).reshape(-1)
+    gr = np.ascontiguousarray(np.ones((nz, 3)) if grain is None else np.asarray(grain, dtype=np.float64)).reshape(-1)
+    out = np.zeros(9 * nz * nz * ny * nx)
+    rc = L.ora_mag2d_tensor(nx, ny, nz, A, gr, (C.c_int * 4)(*trunc), out)
     assert rc == 0
     return out.reshape(9, nz, nz, ny, nx)
 

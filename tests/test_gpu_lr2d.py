@@ -193,3 +193,77 @@ def test_dipole2d_films_in_one_ensemble_match_solo_steps():
         a, b = flat(g["ss"].spins()[:3]), flat(solo_g["ss"].spins()[:3])
         print(f"ensemble member {kind}: max|ensemble - solo| = {np.abs(a - b).max():.3e}", flush=True)
         assert np.array_equal(a, b)
+
+
+def _mag2d(nx, ny, nz, trunc):
+    m = M.Magnetostatics2D(nx, ny, nz)
+    m.setUnitCell(1, [1, 0, 0], [0.2, 1.1, 0], [0.1, 0.0, 1.5])
+    m.setGrainSize(1, 1.0, 0.9, 1.2)
+    m.setGrainSize(2, 0.8, 1.0, 0.5)
+    m.setTruncation(*trunc)
+    ABC = np.tile(np.eye(3), (nz, 1, 1))
+    ABC[0] = ((1, 0, 0), (0.2, 1.1, 0), (0.1, 0.0, 1.5))
+    grain = np.ones((nz, 3))
+    grain[0], grain[1] = (1.0, 0.9, 1.2), (0.8, 1.0, 0.5)
+    sms = np.array([1.0 / m.grainSize(k + 1)[3] for k in range(nz)])
+    dfs = np.full(nz, 4 * np.pi)
+    for k in range(nz):
+        m.setSourceMomentScale(k + 1, sms[k])
+        m.setDestinationFieldScale(k + 1, dfs[k])
+    return m, O.mag2d_tensor(nx, ny, nz, ABC, grain, trunc), sms, dfs
+
+
+def test_magnetostatics2d_against_oracle():
+    """Magnetostatics2D on a 32x32x2 film with a sheared cell, a lifted second layer,
+    unequal prisms and truncation (12, 10, 8, 1), the 1/V and 4 pi scales set as a script
+    would: the tensors (the library's asinh/atan form of Newell's f and g) against the
+    oracle's makeData (the reference's log/atan2 form), and the field against the oracle's
+    apply."""
+    nx, ny, nz = 32, 32, 2
+    m, T, sms, dfs = _mag2d(nx, ny, nz, (12, 10, 8, 1))
+    ss = M.SpinSystem(nx, ny, nz)
+    x, y, z = O.random_unit_spins(11, nx * ny * nz)
+    ss.setSpins(x, y, z)
+    m.apply(ss)
+    got = np.array([[[m.tensorArray(a + 1, b + 1, n).toArray() for b in range(nz)] for a in range(nz)]
+                    for n in M.LongRange2D.NAMES])
+    terr = float(np.abs(got - T).max() / np.abs(T).max())
+    print(f"Magnetostatics2D tensors vs oracle: max rel diff = {terr:.3e}", flush=True)
+    assert terr <= 1e-10
+    assert m.getMatrix(1, 2, "ZZ", 0, 0) == got[8, 0, 1, 0, 0]
+    o = O.LongRange2D(nx, ny, nz, slot="Magnetostatics2D", threads=True)
+    o.setTensors(T)
+    o.setLayerScales(sms, dfs)
+    os_ = O.SpinSystem(nx, ny, nz, threads=True)
+    os_.setSpins(x, y, z)
+    o.apply(os_)
+    hg, ho = flat(ss.field("Magnetostatics2D")), flat(os_.field("Magnetostatics2D"))
+    err = float(np.abs(hg - ho).max() / np.abs(ho).max())
+    print(f"Magnetostatics2D 32x32x2 field: max rel err = {err:.3e}", flush=True)
+    assert err <= 5e-6
+
+
+def test_heun_with_magnetostatics2d_against_oracle():
+    """The fused Heun driver with a Magnetostatics2D operator (scaled down by 0.02 beside
+    exchange J=1) on a 32x32x2 film: 30 steps against the oracle's Heun with the same
+    tensors."""
+    spec = W.FilmSpec(32, 32, 2, J=1.0, K=0.05, g=0.0, alpha=0.1, dt=0.005, T=0.0)
+    g = W.build(M, spec, with_dipole=False)
+    o = W.build(O, spec, with_dipole=False, threads=True)
+    x, y, z = O.random_unit_spins(6, spec.n)
+    g["ss"].setSpins(x, y, z)
+    o["ss"].setSpins(x, y, z)
+    m, T, sms, dfs = _mag2d(spec.nx, spec.ny, spec.nz, (8, 8, 8, 1))
+    m.setScale(0.02)
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], m, None, g["llg"])
+    heun.step(30)
+    od = O.LongRange2D(spec.nx, spec.ny, spec.nz, slot="Magnetostatics2D", threads=True)
+    od.setTensors(T)
+    od.setLayerScales(sms, dfs)
+    od.setScale(0.02)
+    sim = O.Sim(o["ss"], o["ex"], o["ani"], None, None, o["llg"])
+    sim.setLongRange2D(od)
+    sim.heun(30)
+    err = float(np.abs(flat(g["ss"].spins()[:3]) - flat(o["ss"].spins()[:3])).max())
+    print(f"Heun + Magnetostatics2D 32x32x2, 30 steps: max|dm| = {err:.3e}", flush=True)
+    assert err <= 2e-6

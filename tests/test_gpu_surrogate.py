@@ -59,7 +59,7 @@ def test_forward_matches_oracle(dims):
     want, _ = N.forward(Ws, dm, s_in, s_out, bf=True)
     err, bulk = _forward_errors(got, want)
     print(f"surrogate forward {dims}: rel err max {err:.2e}, median {bulk:.2e}", flush=True)
-    assert bulk <= 1e-6 and err <= 1e-3
+    assert bulk <= 2e-7 and err <= 1e-3      # measured: median 1.6e-8, max 1.6e-4 (a BF16 rounding tie)
 
 
 # zero padding in x and y (open lattices) runs the per-layer walk at every depth
@@ -70,7 +70,7 @@ def test_open_forward_matches_oracle(dims):
     want, _ = N.forward(Ws, dm, s_in, s_out, bf=True, periodic=False)
     err, bulk = _forward_errors(got, want)
     print(f"open surrogate forward {dims}: rel err max {err:.2e}, median {bulk:.2e}", flush=True)
-    assert bulk <= 1e-6 and err <= 1e-3
+    assert bulk <= 2e-7 and err <= 1e-3      # measured: median 1.6e-8, max 1.6e-4 (a BF16 rounding tie)
     # the periodic network differs near the x and y faces: the test sees the padding
     wrap, _ = N.forward(Ws, dm, s_in, s_out, bf=True)
     assert np.abs(wrap - want).max() > 10 * np.abs(got - want).max()

@@ -716,3 +716,71 @@ def test_dipole2d_physical_form_equals_lattice_sum():
     assert err <= 1e-13
     # and the reference form differs from it (its in-plane terms cancel)
     assert np.abs(O.dipole2d_tensor(nx, ny, nz, trunc=(0, tX, tY, nz)) - T).max() > 0.1
+
+
+# ------------------------------------------------------------ Magnetostatics2D
+def test_mag2d_elements_closed_forms():
+    """Newell's prism-prism tensor (demag.cpp:207-352 form): a cube's self term is 1/3 on
+    the diagonal and 0 off it, any prism's self trace is 1, a wide thin plate's N_zz tends
+    to 1, and separated prisms give a traceless tensor."""
+    cube = [1.0, 1.0, 1.0]
+    N = {k: O.mag2d_element(k, (0, 0, 0), cube, cube) for k in O.MAG2D_NAMES}
+    for k in ("XX", "YY", "ZZ"):
+        assert abs(N[k] - 1 / 3) < 1e-13
+    for k in ("XY", "XZ", "YX", "YZ", "ZX", "ZY"):
+        assert abs(N[k]) < 1e-13
+    p = [1.0, 2 ** 0.5, 2 ** (1 / 3)]
+    assert abs(sum(O.mag2d_element(k, (0, 0, 0), p, p) for k in ("XX", "YY", "ZZ")) - 1) < 1e-13
+    # thin square plates: N_zz = 1 - O(t/L ln(L/t)), rising towards 1 with the aspect ratio
+    nzz = [O.mag2d_element("ZZ", (0, 0, 0), [L, L, 1.0], [L, L, 1.0]) for L in (20.0, 200.0, 2000.0)]
+    assert 0.85 < nzz[0] < nzz[1] < nzz[2] < 1 and 1 - nzz[2] < 0.1 * (1 - nzz[0])
+    q = [0.5, 2.0, 1.0]
+    for r in ((3.0, 0.5, -2.0), (-4.0, 7.0, 1.5)):
+        tr = sum(O.mag2d_element(k, r, p, q) for k in ("XX", "YY", "ZZ"))
+        assert abs(tr) < 1e-12
+
+
+def test_mag2d_far_field_is_the_point_dipole():
+    """Far apart, N_ab -> V_src / (4 pi) (delta_ab / R^3 - 3 R_a R_b / R^5) with R the
+    vector between the prisms' centres (r joins their origin corners)."""
+    s, d = np.array([1.0, 2 ** 0.5, 2 ** (1 / 3)]), np.array([0.5, 2.0, 1.0])
+    for r in ((40.3, -17.1, 13.2), (-25.0, 60.0, -5.5)):
+        R = np.array(r) + d / 2 - s / 2
+        Rn = np.linalg.norm(R)
+        want = np.prod(s) / (4 * np.pi) * (np.eye(3) / Rn ** 3 - 3 * np.outer(R, R) / Rn ** 5)
+        got = np.array([O.mag2d_element(k, r, s, d) for k in O.MAG2D_NAMES]).reshape(3, 3)
+        assert np.abs(got - want).max() <= 2e-3 * np.abs(want).max()
+
+
+def test_mag2d_source_additivity():
+    """The reference's own check (magnetostatics3d/TensorElementTest.lua): a source prism
+    split into parts gives the sum of the parts' tensors (each with its corner offset)."""
+    a, b, c = 1.0, 2 ** 0.5, 2 ** (1 / 3)
+    dest = [0.7, 1.3, 0.9]
+    for r in ((0.0, 0.0, 0.0), (2.5, -1.0, 0.3), (-3.1, 2.2, -4.0), (0.4, 0.4, 0.4)):
+        r = np.array(r)
+        for k in O.MAG2D_NAMES:
+            whole = O.mag2d_element(k, r, [2 * a, b, 3 * c], dest)
+            parts = sum(O.mag2d_element(k, r - np.array([i * a, 0.0, j * c]), [a, b, c], dest)
+                        for i in range(2) for j in range(3))
+            assert abs(whole - parts) <= 1e-12 * max(1.0, abs(whole)), (k, r)
+
+
+def test_mag2d_tensor_images_and_labels():
+    """makeData: -N summed over the images that land on each cell, stored under [source]
+    [destination]; cumulative C vectors between layers, per-layer prisms."""
+    nx, ny, nz = 4, 3, 2
+    ABC = np.array([[[1.0, 0, 0], [0.2, 1.1, 0], [0.1, 0.0, 1.5]], [[1.0, 0, 0], [0.2, 1.1, 0], [0, 0, 1.0]]])
+    grain = np.array([[1.0, 0.8, 1.5], [0.9, 1.0, 0.5]])
+    tr = (3, 2, 3, 1)
+    T = O.mag2d_tensor(nx, ny, nz, ABC, grain, tr)
+    want = np.zeros_like(T)
+    for z1 in range(nz):
+        for z2 in range(nz):
+            c = ABC[0][2] if (z1, z2) == (0, 1) else (-ABC[0][2] if (z1, z2) == (1, 0) else np.zeros(3))
+            for X in range(-2, 3):
+                for Y in range(-3, 4):
+                    r = X * ABC[0][0] + Y * ABC[0][1] + c
+                    for k in range(9):
+                        want[k, z1, z2, Y % ny, X % nx] -= O.mag2d_element(k, r, grain[z1], grain[z2])
+    assert np.abs(T - want).max() <= 1e-14 * np.abs(want).max()
