@@ -368,7 +368,7 @@ def pass_roofline(prof, bytes_per_pass, step_ms, step_bytes):
                      "frac": round(step_gbs / HBM_PEAK_GBS, 4)}}
 
 
-def ensemble_leg(M, W, nrep=64, steps=50, R=16):
+def ensemble_leg(M, W, nrep=64, steps=50, R=16, tune_steps=10):
     """64 periodic replicas with sizes drawn (seed 0) from {32^3, 64^3, 128x128x16, 96x96x24},
     temperatures 0.1..2.0, each with its own seed, LongRange3D dipole (lattice sum to R)
     g=0.01, J=1, K=0.1: one set of launches per Heun step for the whole batch."""
@@ -393,6 +393,8 @@ def ensemble_leg(M, W, nrep=64, steps=50, R=16):
         nsites += spec.n
         temps.append(T)
     E = M.Ensemble(members)
+    # LongRange3D pass widths timed in the ensemble's own launches (states restored)
+    widths = E.autotune(steps=tune_steps) if tune_steps > 0 else None
     E.step(2)
     M.synchronize()
     setup = time.perf_counter() - t0
@@ -427,7 +429,7 @@ def ensemble_leg(M, W, nrep=64, steps=50, R=16):
                         f"T 0.1..2.0, LongRange3D lattice sum R={R}, g=0.01, J=1, K=0.1",
             "value": round(nsites * steps / el, 1), "unit": "spin-steps/s", "sites": nsites, "steps": steps,
             "ms_per_step": round(1e3 * el / steps, 4), "setup_s": round(setup, 2),
-            "roofline": roof,
+            "roofline": roof, "pass_widths": widths,
             "temperatures": [round(t, 4) for t in temps], "m_abs_per_site": m_abs,
             "total_field_dot_spin_per_site": hdot}
 
@@ -534,7 +536,7 @@ def cube_learned(M, W, spec, g, ani, lr, heun, plain, train_steps=100, train_K=8
             "final_loss": round(float(np.mean(losses[-10:])), 6)}
 
 
-def cube_leg(M, W, n=256, steps=20, learned=True):
+def cube_leg(M, W, n=256, steps=20, learned=True, tune_steps=10):
     """256^3 open cube (zero-padded 512^3 transforms), J=1, K=0.05 along seeded random axes,
     g=0.05, T=0.3, dt=0.002, and a 4-replica batch of 128^3 open cubes in one launch.  On
     the single cube, the surrogate form too (cube_learned)."""
@@ -559,6 +561,7 @@ def cube_leg(M, W, n=256, steps=20, learned=True):
             keep.append((g, ani, lr))
             nsites += spec.n
         drv = members[0] if count == 1 else M.Ensemble(members)
+        widths = drv.autotune(steps=tune_steps) if tune_steps > 0 else None
         drv.step(2)
         M.synchronize()
         setup = time.perf_counter() - t0
@@ -572,7 +575,7 @@ def cube_leg(M, W, n=256, steps=20, learned=True):
         step_b = 2 * (pb["k3_ypass_fwd"] + pb["k3_zconv"] + pb["k3_ypass_inv"]) + pb["k3_rows_pred"] + pb["k3_rows_corr"]
         out[label] = {"lattice": list(dims), "replicas": count, "value": round(nsites * steps / el, 1),
                       "ms_per_step": round(1e3 * el / steps, 3), "setup_s": round(setup, 2),
-                      "roofline": pass_roofline(prof, pb, 1e3 * el / steps, step_b)}
+                      "pass_widths": widths, "roofline": pass_roofline(prof, pb, 1e3 * el / steps, step_b)}
         if count == 1 and learned:
             g, ani, lr = keep[0]
             out[label]["learned"] = cube_learned(M, W, spec, g, ani, lr, drv, nsites * steps / el)
@@ -668,9 +671,10 @@ def main():
     ensemble = cube = None
     if not args.no_extra:
         print("# batched replicas and open cubes...", file=sys.stderr, flush=True)
-        ensemble = ensemble_leg(M, W)
+        tune3 = min(args.tune_steps, 10)     # 3D pass widths: ten steps per candidate pair
+        ensemble = ensemble_leg(M, W, tune_steps=tune3)
         ensemble["films"] = films_leg(M, W)
-        cube = cube_leg(M, W)
+        cube = cube_leg(M, W, tune_steps=tune3)
 
     cpu = None
     if not args.no_cpu:

@@ -321,6 +321,17 @@ int msd_longrange3d_get_matrix(msd_longrange3d* l, int which, int ox, int oy, in
 int msd_longrange3d_set_matrix(msd_longrange3d* l, int which, int ox, int oy, int oz, double v);
 /* lr:apply(ss): slot "LongRange3D" */
 int msd_longrange3d_apply(msd_longrange3d* l, msd_system* s);
+/* Pass schedule (no counterpart in the reference: its FFTs are FFTW plans).  The
+ * compile-time y pass and z pass forms each have a few kx widths per workgroup;
+ * wy / wz = 0 keeps a width.  A width with no compiled instance at this transform
+ * length, or over the LDS per workgroup, is MSD_EUNSUPPORTED.  The field does not
+ * depend on the widths beyond FP32 rounding.  msd_longrange3d_pass_widths returns
+ * the widths in use and the ones each pass form offers, default first, in
+ * y_choices[ny] / z_choices[nz] (at most 8 each; ny or nz = 0: the runtime-length
+ * pass, whose width is fixed); the choice arrays and counts may be NULL. */
+int msd_longrange3d_set_pass_widths(msd_longrange3d* l, int wy, int wz);
+int msd_longrange3d_pass_widths(msd_longrange3d* l, int* wy, int* wz, int* y_choices, int* ny, int* z_choices,
+                                int* nz);
 
 /* --------------------------------------------------------------------- LLG */
 /* LLG.Cartesian: cpu/llg/llgcartesian.cpp:90-278 (thermalOnlyFirstTerm = true) */
@@ -395,6 +406,11 @@ int msd_stepper_profile(msd_stepper* st, int nsteps, int cap, char names[][32], 
  * -- and keeps the fastest; the spin state, time and thermal draw are restored.
  * Returns the chosen shape and its ms per step (-1 when only runtime shapes exist). */
 int msd_stepper_autotune(msd_stepper* st, int steps, int* best_ty, int* best_tpb, int* best_split, double* best_ms);
+/* Autotuner of a LongRange3D step driver (no counterpart in the reference): times
+ * `steps` steps for every pair of compiled y-pass and z-pass kx widths of its
+ * LongRange3D and keeps the fastest (msd_longrange3d_set_pass_widths); the state is
+ * restored.  Returns the widths and their ms per step (-1 when there is no choice). */
+int msd_stepper_autotune_passes(msd_stepper* st, int steps, int* best_wy, int* best_wz, double* best_ms);
 
 /* ---------------------------------------------------------------- surrogate
  * No counterpart in the reference: a learned predictor of the dipole field's
@@ -464,6 +480,14 @@ int msd_ensemble_heun(msd_ensemble* e, int nsteps);
  * LongRange3D members' y / z / inverse-y / row passes, the film groups' passes);
  * names[i] and avg_ms[i] per pass group, *count of them (at most cap). */
 int msd_ensemble_profile(msd_ensemble* e, int nsteps, int cap, char names[][32], double* avg_ms, int* count);
+/* Autotuner of an ensemble's LongRange3D pass widths (no counterpart in the
+ * reference), timed in the ensemble's own launches: members grouped by lattice (size
+ * and boundaries); one sweep over the groups tries every compiled (y, z) kx width
+ * pair on a group with the others held and keeps the fastest by `steps` steps.  The
+ * members' states are restored; the widths chosen are each LongRange3D's
+ * (msd_longrange3d_pass_widths).  Returns the ms per step reached (-1: no LongRange3D
+ * member). */
+int msd_ensemble_autotune_passes(msd_ensemble* e, int steps, double* best_ms);
 
 #ifdef __cplusplus
 }

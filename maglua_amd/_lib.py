@@ -149,6 +149,8 @@ _SIGS = {
     "msd_longrange3d_get_matrix": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
     "msd_longrange3d_set_matrix": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]),
     "msd_longrange3d_apply": (C.c_int, [_vp, _vp]),
+    "msd_longrange3d_set_pass_widths": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "msd_longrange3d_pass_widths": (C.c_int, [_vp, _ip, _ip, _ip, _ip, _ip, _ip]),
     "msd_stepper_set_longrange3d": (C.c_int, [_vp, _vp]),
     "msd_ensemble_create": (C.c_int, [_pp, C.c_int, _pp]),
     "msd_ensemble_destroy": (C.c_int, [_vp]),
@@ -171,6 +173,8 @@ _SIGS = {
     "msd_stepper_profile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_void_p, _dp, _ip]),
     "msd_stepper_set_surrogate": (C.c_int, [_vp, _vp, C.c_int, C.c_double]),
     "msd_stepper_autotune": (C.c_int, [_vp, C.c_int, _ip, _ip, _ip, _dp]),
+    "msd_stepper_autotune_passes": (C.c_int, [_vp, C.c_int, _ip, _ip, _dp]),
+    "msd_ensemble_autotune_passes": (C.c_int, [_vp, C.c_int, _dp]),
     "msd_stepper_rk": (C.c_int, [_vp, C.c_char_p, C.c_int]),
     "msd_stepper_graph_builds": (C.c_int, [_vp, _lp]),
     "msd_stepper_surrogate_stats": (C.c_int, [_vp, _lp, _lp, _lp, _dp, _dp]),

@@ -315,10 +315,11 @@ __global__ void __launch_bounds__(Y3_TPB) k3_ypass(const Rep3* __restrict__ reps
     }
 }
 
-// Compile-time y length FY (32..512 powers of two, and 96), W = 8 kx columns per
-// workgroup, transformed in place (fft_ip) in half the runtime pass's LDS.  Every y
-// row is read and written as 64-byte runs; with W's 128-byte rows and XCD-local
-// items the two blocks of a line meet in one L2 (measured faster than W = 16).
+// Compile-time y length FY (32..512 powers of two, and 96), W kx columns per
+// workgroup (8 by default, 4 and 16 for the autotuner), transformed in place
+// (fft_ip) in half the runtime pass's LDS.  At W = 8 every y row is read and written
+// as 64-byte runs; with W's 128-byte rows and XCD-local items the two blocks of a
+// line meet in one L2 (measured faster than W = 16 at 256^3 open).
 template <int SIGN, int FY, int W>
 __global__ void __launch_bounds__(Y3_TPB) k3_ypass_t(const Rep3* __restrict__ reps, const int4* __restrict__ items)
 {
@@ -352,38 +353,43 @@ __global__ void __launch_bounds__(Y3_TPB) k3_ypass_t(const Rep3* __restrict__ re
     }
 }
 
-int ypass_template_width(int fy)
-{
-    return (fy == 32 || fy == 64 || fy == 96 || fy == 128 || fy == 256 || fy == 512) ? 8 : 0;
-}
+// kx widths of the compile-time y pass, the default (measured fastest at 256^3
+// open) first; the step autotuner times the others
+static constexpr int YPASS_WIDTHS[] = {8, 16, 4};
 
-template <int SIGN>
-static const void* ypass_template(int fy)
+template <int SIGN, int FY>
+static const void* ypass_w(int w)
 {
-    switch (fy) {
-    case 32: return (const void*)k3_ypass_t<SIGN, 32, 8>;
-    case 64: return (const void*)k3_ypass_t<SIGN, 64, 8>;
-    case 96: return (const void*)k3_ypass_t<SIGN, 96, 8>;
-    case 128: return (const void*)k3_ypass_t<SIGN, 128, 8>;
-    case 256: return (const void*)k3_ypass_t<SIGN, 256, 8>;
-    case 512: return (const void*)k3_ypass_t<SIGN, 512, 8>;
+    switch (w) {
+    case 4: return (const void*)k3_ypass_t<SIGN, FY, 4>;
+    case 8: return (const void*)k3_ypass_t<SIGN, FY, 8>;
+    case 16: return (const void*)k3_ypass_t<SIGN, FY, 16>;
     default: return nullptr;
     }
 }
 
 template <int SIGN>
-static hipError_t launch_ypass_t(int yt, const Rep3* reps, const int4* items, int nitems, size_t lds, hipStream_t st)
+static const void* ypass_template(int fy, int w)
 {
-    switch (yt) {
-    case 32: k3_ypass_t<SIGN, 32, 8><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
-    case 64: k3_ypass_t<SIGN, 64, 8><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
-    case 96: k3_ypass_t<SIGN, 96, 8><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
-    case 128: k3_ypass_t<SIGN, 128, 8><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
-    case 256: k3_ypass_t<SIGN, 256, 8><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
-    case 512: k3_ypass_t<SIGN, 512, 8><<<nitems, Y3_TPB, lds, st>>>(reps, items); break;
-    default: return hipErrorInvalidValue;
+    switch (fy) {
+    case 32: return ypass_w<SIGN, 32>(w);
+    case 64: return ypass_w<SIGN, 64>(w);
+    case 96: return ypass_w<SIGN, 96>(w);
+    case 128: return ypass_w<SIGN, 128>(w);
+    case 256: return ypass_w<SIGN, 256>(w);
+    case 512: return ypass_w<SIGN, 512>(w);
+    default: return nullptr;
     }
-    return hipGetLastError();
+}
+
+int ypass_template_width(int fy) { return ypass_template<1>(fy, YPASS_WIDTHS[0]) ? YPASS_WIDTHS[0] : 0; }
+
+int ypass_widths(int fy, int* w, int cap)
+{
+    int n = 0;
+    for (int c : YPASS_WIDTHS)
+        if (ypass_template<1>(fy, c) && n < cap) w[n++] = c;
+    return n;
 }
 
 // ----------------------------------------------------------------- z pass
@@ -563,51 +569,87 @@ __global__ void __launch_bounds__(NT) k3_zconv_o(const Rep3* __restrict__ reps, 
 
 // kx columns and threads per workgroup of the open split pass of half length nz
 // (measured at nz = 128 and 256: narrow blocks, two to four workgroups per CU whose
-// memory phases overlap each other's transforms; a thread holds 6 of each line set)
+// memory phases overlap each other's transforms; a thread holds 6 of each line set).
+// The default width comes first; the step autotuner times the others.
 static constexpr int zopen_width(int nz) { return nz >= 128 ? 4 : 16; }
 static constexpr int zopen_threads(int nz) { return nz == 128 ? 256 : Z3_TPB; }
 template <int NZ>
-static const void* zopen_kernel() { return (const void*)k3_zconv_o<NZ, zopen_width(NZ), zopen_threads(NZ)>; }
-
-int zconv_template_width(int fz)
+static const void* zopen_kernel(int w)
 {
-    switch (fz) {
-    case 16: case 24: case 32: case 64: case 128: case 256: return 16;
-    case 512: return 8;
-    default: return 0;
+    constexpr int NT = zopen_threads(NZ);
+    if constexpr (NZ >= 128) {
+        switch (w) {
+        case 4: return (const void*)k3_zconv_o<NZ, 4, NT>;
+        case 2: return (const void*)k3_zconv_o<NZ, 2, NT>;
+        case 8: return (const void*)k3_zconv_o<NZ, 8, NT>;
+        default: return nullptr;
+        }
+    }
+    else {
+        switch (w) {
+        case 16: return (const void*)k3_zconv_o<NZ, 16, NT>;
+        case 8: return (const void*)k3_zconv_o<NZ, 8, NT>;
+        default: return nullptr;
+        }
     }
 }
 
-int zconv_open_width(int fz)
+// full compile-time pass of length FZ: 16 kx columns (8 at 512, whose 16 exceed the
+// LDS) by default, and the narrower widths
+template <int FZ>
+static const void* zfull_kernel(int w)
 {
-    switch (fz) {
-    case 32: return zopen_width(16);
-    case 64: return zopen_width(32);
-    case 128: return zopen_width(64);
-    case 256: return zopen_width(128);
-    case 512: return zopen_width(256);
-    default: return 0;
+    switch (w) {
+    case 16:
+        if constexpr (FZ < 512) return (const void*)k3_zconv_t<FZ, 16>;
+        else return nullptr;
+    case 8: return (const void*)k3_zconv_t<FZ, 8>;
+    case 4: return (const void*)k3_zconv_t<FZ, 4>;
+    default: return nullptr;
     }
 }
 
 // zt: fz for the full compile-time pass, fz + 1 for the open split pass
-static const void* zconv_template(int zt)
+static const void* zconv_template(int zt, int w)
 {
     switch (zt) {
-    case 16: return (const void*)k3_zconv_t<16, 16>;
-    case 24: return (const void*)k3_zconv_t<24, 16>;
-    case 32: return (const void*)k3_zconv_t<32, 16>;
-    case 64: return (const void*)k3_zconv_t<64, 16>;
-    case 128: return (const void*)k3_zconv_t<128, 16>;
-    case 256: return (const void*)k3_zconv_t<256, 16>;
-    case 512: return (const void*)k3_zconv_t<512, 8>;
-    case 33: return zopen_kernel<16>();
-    case 65: return zopen_kernel<32>();
-    case 129: return zopen_kernel<64>();
-    case 257: return zopen_kernel<128>();
-    case 513: return zopen_kernel<256>();
+    case 16: return zfull_kernel<16>(w);
+    case 24: return zfull_kernel<24>(w);
+    case 32: return zfull_kernel<32>(w);
+    case 64: return zfull_kernel<64>(w);
+    case 128: return zfull_kernel<128>(w);
+    case 256: return zfull_kernel<256>(w);
+    case 512: return zfull_kernel<512>(w);
+    case 33: return zopen_kernel<16>(w);
+    case 65: return zopen_kernel<32>(w);
+    case 129: return zopen_kernel<64>(w);
+    case 257: return zopen_kernel<128>(w);
+    case 513: return zopen_kernel<256>(w);
     default: return nullptr;
     }
+}
+
+int zconv_template_width(int fz)
+{
+    for (int w : {16, 8})
+        if (zconv_template(fz, w)) return w;
+    return 0;
+}
+
+int zconv_open_width(int fz)
+{
+    const int nz = fz / 2;
+    return (fz % 2 == 0 && zconv_template(fz + 1, zopen_width(nz))) ? zopen_width(nz) : 0;
+}
+
+int zconv_widths(int zt, int* w, int cap)
+{
+    int n = 0;
+    const int first = (zt & 1) ? zopen_width(zt / 2) : zconv_template_width(zt);
+    if (first && n < cap) w[n++] = first;
+    for (int c : {16, 8, 4, 2})
+        if (c != first && zconv_template(zt, c) && n < cap) w[n++] = c;
+    return n;
 }
 
 // --------------------------------------------------------------- launches
@@ -660,20 +702,25 @@ hipError_t launch3_rows(int mode, bool comp, const Rep3* reps, const int4* items
     return hipLaunchKernel(f, dim3(nitems), dim3(nt), args, lds, st);
 }
 
-hipError_t launch3_ypass(int sign, const Rep3* reps, const int4* items, int nitems, size_t lds, int yt, hipStream_t st)
+hipError_t launch3_ypass(int sign, const Rep3* reps, const int4* items, int nitems, size_t lds, int yt, int wy,
+                         hipStream_t st)
 {
     if (nitems <= 0) return hipSuccess;
-    if (yt) return sign < 0 ? launch_ypass_t<-1>(yt, reps, items, nitems, lds, st)
-                            : launch_ypass_t<1>(yt, reps, items, nitems, lds, st);
+    if (yt) {
+        const void* f = sign < 0 ? ypass_template<-1>(yt, wy) : ypass_template<1>(yt, wy);
+        if (!f) return hipErrorInvalidValue;
+        void* args[] = {(void*)&reps, (void*)&items};
+        return hipLaunchKernel(f, dim3(nitems), dim3(Y3_TPB), args, lds, st);
+    }
     if (sign < 0) k3_ypass<-1><<<nitems, Y3_TPB, lds, st>>>(reps, items);
     else k3_ypass<1><<<nitems, Y3_TPB, lds, st>>>(reps, items);
     return hipGetLastError();
 }
 
-hipError_t launch3_zconv(const Rep3* reps, const int4* items, int nitems, size_t lds, int zt, hipStream_t st)
+hipError_t launch3_zconv(const Rep3* reps, const int4* items, int nitems, size_t lds, int zt, int wz, hipStream_t st)
 {
     if (nitems <= 0) return hipSuccess;
-    const void* f = zt ? zconv_template(zt) : (const void*)k3_zconv;
+    const void* f = zt ? zconv_template(zt, wz) : (const void*)k3_zconv;
     if (!f) return hipErrorInvalidValue;
     const int nt = (zt & 1) ? zopen_threads((zt - 1) / 2) : Z3_TPB;   // open split: zt = 2 nz + 1
     void* args[] = {(void*)&reps, (void*)&items};
@@ -696,11 +743,13 @@ hipError_t configure_lds3(size_t max_bytes)
                 if (e != hipSuccess && first == hipSuccess) first = e;
             }
     for (int f : {16, 24, 32, 64, 96, 128, 256, 512})
-        for (const void* k : {zconv_template(f), zconv_template(f + 1), ypass_template<-1>(f), ypass_template<1>(f)}) {
-            if (!k) continue;    // no instance of that pass at this length
-            hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, v);
-            if (e != hipSuccess && first == hipSuccess) first = e;
-        }
+        for (int w : {2, 4, 8, 16})
+            for (const void* k : {zconv_template(f, w), zconv_template(f + 1, w), ypass_template<-1>(f, w),
+                                  ypass_template<1>(f, w)}) {
+                if (!k) continue;    // no instance of that pass at this length and width
+                hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+                if (e != hipSuccess && first == hipSuccess) first = e;
+            }
     (void)hipGetLastError();
     return first;
 }

@@ -68,6 +68,69 @@ extern "C" int msd_longrange3d_create(int nx, int ny, int nz, int open, msd_long
 }
 
 extern "C" int msd_longrange3d_destroy(msd_longrange3d* l) { delete l; return MSD_OK; }
+
+// LDS bytes of the compile-time y pass and z pass form at kx width w
+static size_t lds_ypass(const msd_longrange3d* l, int w) { return (size_t)w * (l->fy + CONV3_LPAD) * sizeof(float2); }
+static size_t lds_zpass(const msd_longrange3d* l, int w)
+{
+    if (l->zt & 1) return ((size_t)3 * w * (l->d.nz + CONV3_LPAD) + 2 * (size_t)l->d.nz) * sizeof(float2);
+    return (size_t)3 * w * (l->fz + CONV3_LPAD) * sizeof(float2);
+}
+
+// Whether width w is compiled for the pass form and fits the LDS
+static bool width_ok(const msd_longrange3d* l, bool ypass, int w)
+{
+    int ws[8];
+    const int n = ypass ? ypass_widths(l->yt, ws, 8) : zconv_widths(l->zt, ws, 8);
+    for (int i = 0; i < n; ++i)
+        if (ws[i] == w) return (ypass ? lds_ypass(l, w) : lds_zpass(l, w)) <= G.max_lds;
+    return false;
+}
+
+int lr3_set_widths(msd_longrange3d* l, int wy, int wz)
+{
+    if (wy == l->wy) wy = 0;    // the width in use stays (also a runtime-length pass's own)
+    if (wz == l->wz) wz = 0;
+    if (wy && (!l->yt || !width_ok(l, true, wy)))
+        return fail(MSD_EUNSUPPORTED, "LongRange3D: no y pass of width %d at y length %d", wy, l->fy);
+    if (wz && (!l->zt || !width_ok(l, false, wz)))
+        return fail(MSD_EUNSUPPORTED, "LongRange3D: no z pass of width %d at z length %d", wz, l->fz);
+    if (!wy && !wz) return MSD_OK;
+    if (wy) { l->wy = wy; l->lds_y = lds_ypass(l, wy); }
+    if (wz) { l->wz = wz; l->lds_z = lds_zpass(l, wz); }
+    // the operator API's work items follow the widths: rebuilt at the next field
+    (void)hipStreamSynchronize(G.stream);
+    dfree(l->orep); dfree(l->oitems_r); dfree(l->oitems_y); dfree(l->oitems_z);
+    l->orep = nullptr; l->oitems_r = l->oitems_y = l->oitems_z = nullptr;
+    return MSD_OK;
+}
+
+extern "C" int msd_longrange3d_set_pass_widths(msd_longrange3d* l, int wy, int wz)
+{
+    NONNULL(l);
+    if (wy < 0 || wz < 0) return fail(MSD_EINVAL, "LongRange3D: pass widths %d, %d", wy, wz);
+    return lr3_set_widths(l, wy, wz);
+}
+
+extern "C" int msd_longrange3d_pass_widths(msd_longrange3d* l, int* wy, int* wz, int* y_choices, int* ny,
+                                          int* z_choices, int* nz)
+{
+    NONNULL(l); NONNULL(wy); NONNULL(wz);
+    *wy = l->wy;
+    *wz = l->wz;
+    // the compiled widths that also fit the LDS
+    auto list = [&](bool ypass, int* out, int* n) {
+        int ws[8];
+        const int m = (ypass ? l->yt : l->zt) ? (ypass ? ypass_widths(l->yt, ws, 8) : zconv_widths(l->zt, ws, 8)) : 0;
+        int k = 0;
+        for (int i = 0; i < m; ++i)
+            if (width_ok(l, ypass, ws[i])) { if (out) out[k] = ws[i]; ++k; }
+        if (n) *n = k;
+    };
+    list(true, y_choices, ny);
+    list(false, z_choices, nz);
+    return MSD_OK;
+}
 extern "C" int msd_longrange3d_set_strength(msd_longrange3d* l, double g) { NONNULL(l); l->g = g; return MSD_OK; }
 extern "C" int msd_longrange3d_set_scale(msd_longrange3d* l, double s) { NONNULL(l); l->scale = s; return MSD_OK; }
 extern "C" int msd_longrange3d_set_radius(msd_longrange3d* l, int R)
@@ -255,9 +318,9 @@ int lr3_field_of(msd_longrange3d* l, const float* S, float* H)
     }
     HIPCHK(hipMemcpyAsync(l->orep, &r, sizeof r, hipMemcpyHostToDevice, G.stream));
     hipError_t e = launch3_rows(R3_FWD, false, l->orep, l->oitems_r, l->onr, l->lds_rows, l->rt, G.stream);
-    if (e == hipSuccess) e = launch3_ypass(-1, l->orep, l->oitems_y, l->ony, l->lds_y, l->yt, G.stream);
-    if (e == hipSuccess) e = launch3_zconv(l->orep, l->oitems_z, l->onz, l->lds_z, l->zt, G.stream);
-    if (e == hipSuccess) e = launch3_ypass(1, l->orep, l->oitems_y, l->ony, l->lds_y, l->yt, G.stream);
+    if (e == hipSuccess) e = launch3_ypass(-1, l->orep, l->oitems_y, l->ony, l->lds_y, l->yt, l->wy, G.stream);
+    if (e == hipSuccess) e = launch3_zconv(l->orep, l->oitems_z, l->onz, l->lds_z, l->zt, l->wz, G.stream);
+    if (e == hipSuccess) e = launch3_ypass(1, l->orep, l->oitems_y, l->ony, l->lds_y, l->yt, l->wy, G.stream);
     if (e == hipSuccess) e = launch3_rows(R3_INV, false, l->orep, l->oitems_r, l->onr, l->lds_rows, l->rt, G.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(G.stream);   // the descriptor lives on the host stack
     if (e != hipSuccess) return fail(MSD_EDEVICE, "LongRange3D field: %s", hipGetErrorString(e));

@@ -168,3 +168,87 @@ extern "C" int msd_ensemble_heun(msd_ensemble* e, int nsteps)
     for (msd_stepper* s : e->m) stepper_end(s, nsteps);
     return MSD_OK;
 }
+
+// Autotuner of an ensemble's LongRange3D passes, timed in the ensemble's own launches
+// (a narrow block that fills the GPU for one replica can lose to a wide one when the
+// replicas share a launch).  Members are grouped by lattice (size and boundaries);
+// one sweep over the groups tries every compiled (y, z) width pair on a group's
+// members with the other groups' widths held, keeping the fastest by `steps`
+// graph-replayed steps.  Every member's state is restored after each trial.
+extern "C" int msd_ensemble_autotune_passes(msd_ensemble* e, int steps, double* best_ms)
+{
+    NONNULL(e); NONNULL(best_ms);
+    if (steps < 1) return fail(MSD_EINVAL, "ensemble autotune: steps %d", steps);
+    *best_ms = -1.0;
+    struct Lattice {
+        Dims d;
+        bool open;
+        std::vector<msd_longrange3d*> lr;
+    };
+    std::vector<Lattice> groups;
+    for (msd_stepper* s : e->m) {
+        if (!s->lr3) continue;
+        size_t gi = 0;
+        while (gi < groups.size() && !(same_dims(groups[gi].d, s->lr3->d.nx, s->lr3->d.ny, s->lr3->d.nz) &&
+                                       groups[gi].open == s->lr3->open)) ++gi;
+        if (gi == groups.size()) groups.push_back(Lattice{s->lr3->d, s->lr3->open, {}});
+        groups[gi].lr.push_back(s->lr3);
+    }
+    if (groups.empty()) return MSD_OK;
+    std::vector<std::unique_ptr<StateKeep>> keep;
+    for (msd_stepper* s : e->m) {
+        keep.emplace_back(new StateKeep(s));
+        if (int rc = keep.back()->save()) return rc;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return fail(MSD_EDEVICE, "ensemble autotune: events");
+    }
+    auto time_once = [&](double* per) -> int {
+        if (int rc = msd_ensemble_heun(e, 2)) return rc;          // capture + warm
+        if (hipEventRecord(e0, G.stream) != hipSuccess) return fail(MSD_EDEVICE, "ensemble autotune: event");
+        if (int rc = msd_ensemble_heun(e, steps)) return rc;
+        if (hipEventRecord(e1, G.stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+            return fail(MSD_EDEVICE, "ensemble autotune: event");
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        *per = ms / steps;
+        for (auto& k : keep)
+            if (int rc = k->restore()) return rc;
+        return MSD_OK;
+    };
+    auto set_group = [](const Lattice& g, int wy, int wz) -> int {
+        for (msd_longrange3d* l : g.lr)
+            if (int rc = lr3_set_widths(l, wy, wz)) return rc;
+        return MSD_OK;
+    };
+    double best = 0.0;
+    int rc = time_once(&best);
+    for (const Lattice& g : groups) {
+        if (rc) break;
+        msd_longrange3d* l0 = g.lr[0];
+        int wys[8], wzs[8];
+        int ny = l0->yt ? ypass_widths(l0->yt, wys, 8) : 0;
+        int nz = l0->zt ? zconv_widths(l0->zt, wzs, 8) : 0;
+        if (ny == 0) { wys[0] = 0; ny = 1; }
+        if (nz == 0) { wzs[0] = 0; nz = 1; }
+        int bwy = l0->wy, bwz = l0->wz;
+        for (int c = 0; c < ny * nz && !rc; ++c) {
+            const int wy = wys[c / nz], wz = wzs[c % nz];
+            if ((wy == 0 || wy == bwy) && wz == bwz) continue;     // the current choice is timed
+            if (set_group(g, wy, wz) != MSD_OK) continue;          // over the LDS: not a candidate
+            double per = 0.0;
+            if ((rc = time_once(&per))) break;
+            if (per < best) { best = per; bwy = l0->wy; bwz = l0->wz; }
+        }
+        if (!rc) rc = set_group(g, bwy, bwz);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!rc) {
+        *best_ms = best;
+        rc = e->refresh();
+    }
+    return rc;
+}

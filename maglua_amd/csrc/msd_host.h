@@ -69,6 +69,7 @@ int dipole_field_of(msd_dipole* dip, const float* s3, float* h);
 void items3(int rep, const Dims& d, const Conv3Dev* cv, int ty, std::vector<int4>& rows, std::vector<int4>& ys, std::vector<int4>& zs);
 std::vector<int4> xcd_order(const std::vector<int4>& v);
 int lr3_field_of(msd_longrange3d* l, const float* S, float* H);
+int lr3_set_widths(msd_longrange3d* l, int wy, int wz);   // 0 keeps a width
 int field_maxdiff(const float* a, const float* b, size_t n3, double* maxdiff, double* maxb);
 int member_begin(msd_stepper* st);
 bool spectrum_stale(const msd_stepper* st);
@@ -452,14 +453,16 @@ struct Batch3 {
     // rest), so every member is transformed by the same kernels as when it runs alone
     struct ZGroup {
         int zt = 0;
+        int w = 0;           // kx width of the y / z pass form (0 for row groups)
         int4* items = nullptr;
         int n = 0;
         size_t lds = 0;
     };
     Rep3* reps = nullptr;
     int nrep = 0;
-    std::vector<ZGroup> rg, yg, zg;      // row, y-pass and z-pass groups (key: Conv3Dev::rt / yt / zt)
+    std::vector<ZGroup> rg, yg, zg;      // row, y-pass and z-pass groups (key: Conv3Dev::rt / (yt, wy) / (zt, wz))
     bool any_lr = false;
+    uint64_t gen = 0;                    // bumped whenever the tables are rebuilt (graphs holding them are stale)
     std::vector<Rep3> host;
     ~Batch3() { release(); }
     void release()
@@ -473,24 +476,35 @@ struct Batch3 {
         host.clear();
         nrep = 0;
     }
-    // rebuild the tables when the member set changed; refresh descriptors always
+    // whether member b launches the same pass forms over the same work items as a
+    static bool same_passes(const Rep3& a, const Rep3& b)
+    {
+        return a.has_lr == b.has_lr && a.a.d.nx == b.a.d.nx && a.a.d.ny == b.a.d.ny && a.a.d.nz == b.a.d.nz &&
+               a.a.ty == b.a.ty && a.cv.rt == b.cv.rt && a.cv.yt == b.cv.yt && a.cv.wy == b.cv.wy &&
+               a.cv.zt == b.cv.zt && a.cv.wz == b.cv.wz && a.cv.fxh == b.cv.fxh && a.cv.fy == b.cv.fy;
+    }
+    // rebuild the tables when the member set or a member's pass forms changed;
+    // refresh descriptors always
     int update(const std::vector<Rep3>& r, const std::vector<int>& tys, const std::vector<size_t>& lds3)
     {
-        const bool same_shape = host.size() == r.size() && reps != nullptr;
+        bool same_shape = host.size() == r.size() && reps != nullptr;
+        for (size_t i = 0; same_shape && i < r.size(); ++i) same_shape = same_passes(host[i], r[i]);
         if (!same_shape) {
             release();
+            ++gen;
             std::vector<std::vector<int4>> R, Y, Z;
             std::vector<ZGroup> rgroups, ygroups, zgroups;
             any_lr = false;
             // append a member's pass items to the group of its pass form
-            auto add = [](std::vector<ZGroup>& groups, std::vector<std::vector<int4>>& lists, int key, size_t lds,
-                          const std::vector<int4>& items) {
+            auto add = [](std::vector<ZGroup>& groups, std::vector<std::vector<int4>>& lists, int key, int w,
+                          size_t lds, const std::vector<int4>& items) {
                 if (items.empty()) return;
                 size_t gi = 0;
-                while (gi < groups.size() && groups[gi].zt != key) ++gi;
+                while (gi < groups.size() && (groups[gi].zt != key || groups[gi].w != w)) ++gi;
                 if (gi == groups.size()) {
                     groups.push_back(ZGroup{});
                     groups.back().zt = key;
+                    groups.back().w = w;
                     lists.emplace_back();
                 }
                 groups[gi].lds = std::max(groups[gi].lds, lds);
@@ -500,9 +514,10 @@ struct Batch3 {
                 std::vector<int4> ri, yi, zi;
                 items3((int)i, r[i].a.d, r[i].has_lr ? &r[i].cv : nullptr, tys[i], ri, yi, zi);
                 any_lr |= r[i].has_lr != 0;
-                add(rgroups, R, r[i].has_lr ? r[i].cv.rt : 0, lds3[3 * i], ri);
-                add(ygroups, Y, r[i].cv.yt, lds3[3 * i + 1], yi);
-                add(zgroups, Z, r[i].cv.zt, lds3[3 * i + 2], zi);
+                // the runtime-length passes read each member's width from its descriptor
+                add(rgroups, R, r[i].has_lr ? r[i].cv.rt : 0, 0, lds3[3 * i], ri);
+                add(ygroups, Y, r[i].cv.yt, r[i].cv.yt ? r[i].cv.wy : 0, lds3[3 * i + 1], yi);
+                add(zgroups, Z, r[i].cv.zt, r[i].cv.zt ? r[i].cv.wz : 0, lds3[3 * i + 2], zi);
             }
             for (size_t gi = 0; gi < rgroups.size(); ++gi) {
                 if (int rc = upload_vec(&rgroups[gi].items, R[gi])) return rc;
@@ -533,14 +548,14 @@ struct Batch3 {
     {
         hipError_t e = hipSuccess;
         for (const ZGroup& z : zg)
-            if (e == hipSuccess) e = launch3_zconv(reps, z.items, z.n, z.lds, z.zt, st);
+            if (e == hipSuccess) e = launch3_zconv(reps, z.items, z.n, z.lds, z.zt, z.w, st);
         return e;
     }
     hipError_t ypass(int sign, hipStream_t st) const
     {
         hipError_t e = hipSuccess;
         for (const ZGroup& y : yg)
-            if (e == hipSuccess) e = launch3_ypass(sign, reps, y.items, y.n, y.lds, y.zt, st);
+            if (e == hipSuccess) e = launch3_ypass(sign, reps, y.items, y.n, y.lds, y.zt, y.w, st);
         return e;
     }
     hipError_t rows(int row_mode, bool comp, hipStream_t st) const
@@ -848,6 +863,46 @@ struct FilmGroup {
     }
 };
 
+// A driver's state (spins, low words, time, thermal draw) kept across autotuner
+// trials and put back after each, so that tuning leaves the trajectory untouched.
+struct StateKeep {
+    msd_stepper* st;
+    float *S0 = nullptr, *L0 = nullptr;
+    double time0 = 0.0;
+    uint64_t draw0 = 0;
+    bool lo0 = false;
+    explicit StateKeep(msd_stepper* s) : st(s) {}
+    StateKeep(const StateKeep&) = delete;
+    StateKeep& operator=(const StateKeep&) = delete;
+    ~StateKeep()
+    {
+        (void)hipStreamSynchronize(G.stream);
+        dfree(S0); dfree(L0);
+    }
+    int save()
+    {
+        const size_t n = st->s->d.n;
+        if (int rc = dalloc(&S0, 4 * n)) return rc;
+        if (int rc = dalloc(&L0, 3 * n)) return rc;
+        time0 = st->s->time;
+        draw0 = st->th ? st->th->draw : 0;
+        lo0 = st->s->lo_valid;
+        if (int rc = copy_dev(S0, st->s->S, 4 * n)) return rc;
+        return copy_dev(L0, st->s->Slo, 3 * n);
+    }
+    int restore()
+    {
+        const size_t n = st->s->d.n;
+        if (int r = copy_dev(st->s->S, S0, 4 * n)) return r;
+        if (int r = copy_dev(st->s->Slo, L0, 3 * n)) return r;
+        st->s->time = time0;
+        st->s->lo_valid = lo0;
+        if (st->th) st->th->draw = draw0;
+        st->s->version++;
+        return MSD_OK;
+    }
+};
+
 struct msd_ensemble {
     std::vector<msd_stepper*> m;
     bool compensated = true;
@@ -901,7 +956,10 @@ struct msd_ensemble {
         }
         for (FilmGroup& f : films)
             if (int rc = f.upload()) return rc;
-        return b.update(r, tys, lds);
+        const uint64_t gen0 = b.gen;
+        if (int rc = b.update(r, tys, lds)) return rc;
+        if (b.gen != gen0) drop_graph();    // the captured launches held the old work tables
+        return MSD_OK;
     }
     hipError_t evaluate(bool pred, hipStream_t st) const
     {

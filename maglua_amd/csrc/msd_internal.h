@@ -244,15 +244,23 @@ enum Row3Mode { R3_FWD = 0, R3_INV = 1, R3_PRED = 2, R3_CORR = 3 };
 hipError_t launch3_rows(int mode, bool comp, const Rep3* reps, const int4* items, int nitems, size_t lds, int rt,
                         hipStream_t st);
 int rows_template_height(int fx);   // tile height of the compile-time row pass of length fx, 0 if none
-// yt: every replica's compile-time y length (Conv3Dev::yt), or 0 for the runtime-length pass
-hipError_t launch3_ypass(int sign, const Rep3* reps, const int4* items, int nitems, size_t lds, int yt, hipStream_t st);
-// kx width of the compile-time y pass for a y length (0 when it has no instance)
+// yt, wy: every replica's compile-time y length and kx width (Conv3Dev::yt, wy), or
+// yt = 0 for the runtime-length pass
+hipError_t launch3_ypass(int sign, const Rep3* reps, const int4* items, int nitems, size_t lds, int yt, int wy,
+                         hipStream_t st);
+// default kx width of the compile-time y pass for a y length (0 when it has no instance)
 int ypass_template_width(int fy);
-int zconv_open_width(int fz);       // kx per workgroup of the open split z pass of length fz, 0 if none
-// zt: every replica's compile-time z length (Conv3Dev::zt), or 0 for the runtime-length pass
-hipError_t launch3_zconv(const Rep3* reps, const int4* items, int nitems, size_t lds, int zt, hipStream_t st);
-// kx width of the compile-time z pass for a z length (0 when it has no instance)
+// kx widths compiled for the y pass of length fy (default first); returns how many
+int ypass_widths(int fy, int* w, int cap);
+int zconv_open_width(int fz);       // default kx per workgroup of the open split z pass of length fz, 0 if none
+// zt, wz: every replica's compile-time z pass form and kx width (Conv3Dev::zt, wz),
+// or zt = 0 for the runtime-length pass
+hipError_t launch3_zconv(const Rep3* reps, const int4* items, int nitems, size_t lds, int zt, int wz, hipStream_t st);
+// default kx width of the full compile-time z pass for a z length (0 when it has no instance)
 int zconv_template_width(int fz);
+// kx widths compiled for the z pass form zt (fz, or fz + 1 for the open split pass),
+// default first; returns how many
+int zconv_widths(int zt, int* w, int cap);
 // tensor: periodic lattice sum (R > 0) or open kernel (R == 0) on the FFT grid, FP64
 hipError_t launch3_tensor(int nx, int ny, int nz, int fx, int fy, int fz, int R, double* T, hipStream_t st);
 // real spectrum words of T: U1/U2 complex FP64 scratch [6][fz][fy][fxh]; tw* FP64 tables

@@ -67,7 +67,9 @@ int msd_stepper::refresh()
     args = a;
     if (lr3) {
         const std::vector<size_t> lds = {lr3->lds_rows, lr3->lds_y, lr3->lds_z};
+        const uint64_t gen0 = b3.gen;
         if (int rc = b3.update({rep()}, {args.ty}, lds)) return rc;
+        if (b3.gen != gen0) drop_graph();   // the captured launches held the old work tables
     }
     return MSD_OK;
 }
@@ -507,6 +509,38 @@ extern "C" int msd_stepper_rk(msd_stepper* st, const char* kind, int nsteps)
     return MSD_OK;
 }
 
+// One driver's autotuner trials: its state kept (StateKeep) and a pair of events.
+struct TrialState : StateKeep {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    explicit TrialState(msd_stepper* s) : StateKeep(s) {}
+    ~TrialState()
+    {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+    int save()
+    {
+        if (int rc = StateKeep::save()) return rc;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return fail(MSD_EDEVICE, "autotune: events");
+        return MSD_OK;
+    }
+    // ms per step of `steps` graph-replayed steps of the current configuration (after
+    // a capture and a warm step), the state restored afterwards
+    int time(int steps, double* per)
+    {
+        if (int rc = stepper_begin(st)) return rc;
+        if (int rc = run_steps(st, 2, false)) return rc;
+        if (hipEventRecord(e0, G.stream) != hipSuccess) return fail(MSD_EDEVICE, "autotune: event");
+        if (int rc = run_steps(st, steps, false)) return rc;
+        if (hipEventRecord(e1, G.stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+            return fail(MSD_EDEVICE, "autotune: event");
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        *per = ms / steps;
+        return restore();
+    }
+};
+
 // Autotuner of the layered step: times `steps` graph-replayed steps for every
 // compiled row-tile shape (rows per tile, threads per workgroup) of the lattice
 // and keeps the fastest.  The spin state, its low words, time and thermal draw
@@ -522,26 +556,8 @@ extern "C" int msd_stepper_autotune(msd_stepper* st, int steps, int* best_ty, in
     const int nc = row_shapes_for(dip->d.nx, dip->d.ny, tys, tpbs, 16);
     *best_ty = dip->dev.ty; *best_tpb = dip->dev.tpb; *best_split = dip->dev.split; *best_ms = -1.0;
     if (nc == 0) return MSD_OK;           // runtime-shaped kernels only: nothing to choose
-    const size_t n = st->s->d.n;
-    float *S0 = nullptr, *L0 = nullptr;
-    if (int rc = dalloc(&S0, 4 * n)) return rc;
-    if (int rc = dalloc(&L0, 3 * n)) { dfree(S0); return rc; }
-    const double time0 = st->s->time;
-    const uint64_t draw0 = st->th ? st->th->draw : 0;
-    const bool lo0 = st->s->lo_valid;
-    int rc = copy_dev(S0, st->s->S, 4 * n);
-    if (!rc) rc = copy_dev(L0, st->s->Slo, 3 * n);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = fail(MSD_EDEVICE, "autotune: events");
-    auto restore = [&]() -> int {
-        if (int r = copy_dev(st->s->S, S0, 4 * n)) return r;
-        if (int r = copy_dev(st->s->Slo, L0, 3 * n)) return r;
-        st->s->time = time0;
-        st->s->lo_valid = lo0;
-        if (st->th) st->th->draw = draw0;
-        st->s->version++;
-        return MSD_OK;
-    };
+    TrialState ts(st);
+    int rc = ts.save();
     // every trial rebuilds the step (and the tensor words when the column form changes)
     auto use = [&](int ty, int tpb, bool sp) -> int {
         if (int r = layered_shape(dip, ty, tpb, true, sp, "autotune")) return r;
@@ -558,21 +574,11 @@ extern "C" int msd_stepper_autotune(msd_stepper* st, int steps, int* best_ty, in
         if (layered_shape(dip, ty, tpb, true, sp, "autotune") != MSD_OK) { dip->dev = orig; continue; }
         if (sp && dip->dev.split == 0) continue;    // no split form for this shape: its full-column trial covers it
         if ((rc = use(ty, tpb, sp))) break;
-        if ((rc = stepper_begin(st))) break;
-        if ((rc = run_steps(st, 2, false))) break;          // capture + warm
-        if (hipEventRecord(e0, G.stream) != hipSuccess) { rc = fail(MSD_EDEVICE, "autotune: event"); break; }
-        if ((rc = run_steps(st, steps, false))) break;
-        if (hipEventRecord(e1, G.stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-            rc = fail(MSD_EDEVICE, "autotune: event");
-            break;
-        }
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        const double per = ms / steps;
+        double per = 0.0;
+        if ((rc = ts.time(steps, &per))) break;
         if (*best_ms < 0 || per < *best_ms) {
             *best_ms = per; *best_ty = ty; *best_tpb = tpb; *best_split = dip->dev.split;
         }
-        if ((rc = restore())) break;
     }
     int rc2 = MSD_OK;
     if (!rc && *best_ms > 0) rc2 = use(*best_ty, *best_tpb, *best_split != 0);
@@ -583,10 +589,41 @@ extern "C" int msd_stepper_autotune(msd_stepper* st, int steps, int* best_ty, in
         rc2 = dip->prepare();
     }
     if (!rc) rc = rc2;
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
     (void)hipStreamSynchronize(G.stream);
-    dfree(S0); dfree(L0);
+    if (!rc) rc = st->refresh();
+    st->spec_version = 0;
+    return rc;
+}
+
+// Autotuner of the LongRange3D step: times `steps` graph-replayed steps for every
+// pair of compiled kx widths of the y pass and of the z pass form and keeps the
+// fastest, the state restored after every trial.
+extern "C" int msd_stepper_autotune_passes(msd_stepper* st, int steps, int* best_wy, int* best_wz, double* best_ms)
+{
+    NONNULL(st); NONNULL(best_wy); NONNULL(best_wz); NONNULL(best_ms);
+    if (st->kind() != SK_CONV3) return fail(MSD_EUNSUPPORTED, "autotune: needs a LongRange3D step driver");
+    if (steps < 1) return fail(MSD_EINVAL, "autotune: steps %d", steps);
+    msd_longrange3d* l = st->lr3;
+    int wys[8], wzs[8];
+    int ny = l->yt ? ypass_widths(l->yt, wys, 8) : 0;
+    int nz = l->zt ? zconv_widths(l->zt, wzs, 8) : 0;
+    if (ny == 0) { wys[0] = 0; ny = 1; }   // runtime-length pass: its width stays
+    if (nz == 0) { wzs[0] = 0; nz = 1; }
+    const int wy0 = l->wy, wz0 = l->wz;
+    *best_wy = wy0; *best_wz = wz0; *best_ms = -1.0;
+    if (ny * nz <= 1) return MSD_OK;
+    TrialState ts(st);
+    int rc = ts.save();
+    for (int c = 0; c < ny * nz && !rc; ++c) {
+        const int wy = wys[c / nz], wz = wzs[c % nz];
+        if (lr3_set_widths(l, wy, wz) != MSD_OK) continue;    // over the LDS: not a candidate
+        double per = 0.0;
+        if ((rc = ts.time(steps, &per))) break;
+        if (*best_ms < 0 || per < *best_ms) { *best_ms = per; *best_wy = l->wy; *best_wz = l->wz; }
+    }
+    const int rc2 = (!rc && *best_ms > 0) ? lr3_set_widths(l, *best_wy, *best_wz) : lr3_set_widths(l, wy0, wz0);
+    if (!rc) rc = rc2;
+    (void)hipStreamSynchronize(G.stream);
     if (!rc) rc = st->refresh();
     st->spec_version = 0;
     return rc;

@@ -1147,6 +1147,7 @@ class LongRange3D(_Handle):
         call("msd_longrange3d_create", *self._dims, int(bool(open)), C.byref(p))
         super().__init__(p.value)
         self._g = 1.0
+        self._open = bool(open)
 
     new = classmethod(lambda cls, *a, **k: cls(*a, **k))
 
@@ -1182,6 +1183,19 @@ class LongRange3D(_Handle):
     def apply(self, ss):
         for s in (ss if isinstance(ss, (list, tuple)) else (ss,)):
             call("msd_longrange3d_apply", self._h, s.h)
+
+    def setPassWidths(self, y=0, z=0):
+        """kx columns per workgroup of the y pass and the z pass (0 keeps a width); only
+        compiled widths of the compile-time pass forms are accepted."""
+        call("msd_longrange3d_set_pass_widths", self._h, int(y), int(z))
+
+    def passWidths(self):
+        """{"y", "z"}: the widths in use; {"y_choices", "z_choices"}: the widths each pass
+        form offers, default first (empty: the runtime-length pass, width fixed)."""
+        wy, wz, ny, nz = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        yc, zc = (C.c_int * 8)(), (C.c_int * 8)()
+        call("msd_longrange3d_pass_widths", self._h, C.byref(wy), C.byref(wz), yc, C.byref(ny), zc, C.byref(nz))
+        return {"y": wy.value, "z": wz.value, "y_choices": list(yc[:ny.value]), "z_choices": list(zc[:nz.value])}
 
 
 class _LLGBase(_Handle):
@@ -1474,10 +1488,16 @@ class Heun(_Handle):
         call("msd_stepper_set_surrogate", self._h, sur.h if sur is not None else None, int(K), float(tol))
 
     def autotune(self, steps=20):
-        """Time every compiled row-tile shape in both y-transform schedules (split: the row
-        kernels take a radix-rows_per_tile stage; full columns) for `steps` steps and keep
-        the fastest; the state is restored.  Returns {"rows_per_tile", "threads", "split",
-        "ms_per_step"}."""
+        """Layered Dipole driver: time every compiled row-tile shape in both y-transform
+        schedules (split: the row kernels take a radix-rows_per_tile stage; full columns)
+        for `steps` steps and keep the fastest; returns {"rows_per_tile", "threads",
+        "split", "ms_per_step"}.  LongRange3D driver: time every pair of compiled y-pass
+        and z-pass kx widths; returns {"y_width", "z_width", "ms_per_step"}.  The state is
+        restored either way."""
+        if self._keep[6] is not None:
+            wy, wz, ms = C.c_int(), C.c_int(), C.c_double()
+            call("msd_stepper_autotune_passes", self._h, int(steps), C.byref(wy), C.byref(wz), C.byref(ms))
+            return {"y_width": wy.value, "z_width": wz.value, "ms_per_step": ms.value}
         ty, tpb, sp, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
         call("msd_stepper_autotune", self._h, int(steps), C.byref(ty), C.byref(tpb), C.byref(sp), C.byref(ms))
         return {"rows_per_tile": ty.value, "threads": tpb.value, "split": sp.value, "ms_per_step": ms.value}
@@ -1657,6 +1677,24 @@ class Ensemble(_Handle):
         cnt = C.c_int()
         call("msd_ensemble_profile", self._h, int(n), cap, names, avg, C.byref(cnt))
         return {bytes(names[i]).split(b"\0")[0].decode(): avg[i] for i in range(cnt.value)}
+
+    def autotune(self, steps=10):
+        """LongRange3D pass widths timed in the ensemble's own launches: members grouped by
+        lattice, every compiled (y, z) width pair tried on each group in turn; the members'
+        states are restored.  Returns {"ms_per_step", "widths": {"NXxNYxNZ[o]": (y, z)}}
+        (o: open boundaries)."""
+        for m in self._members:
+            m._sync()
+        ms = C.c_double()
+        call("msd_ensemble_autotune_passes", self._h, int(steps), C.byref(ms))
+        widths = {}
+        for m in self._members:
+            lr = m._keep[6]
+            if lr is not None:
+                pw = lr.passWidths()
+                key = "x".join(str(v) for v in lr._dims) + ("o" if lr._open else "")
+                widths[key] = (pw["y"], pw["z"])
+        return {"ms_per_step": ms.value, "widths": widths}
 
 
 def synchronize():

@@ -234,6 +234,134 @@ def test_ensemble_against_oracle():
         assert err <= 1e-4
 
 
+# every compiled (y width, z width) pair of the compile-time pass forms: y length 32
+# (widths 8, 16, 4), full z pass of 16, open split z passes of half length 32, 128
+# and 256; the kx blocks end partial (fxh = 4, 17, 9, 5)
+WIDTH_CASES = [((6, 32, 16), False, 8), ((16, 16, 32), True, None), ((8, 8, 128), True, None),
+               ((4, 4, 256), True, None)]
+
+
+@pytest.mark.parametrize("dims,open_,R", WIDTH_CASES)
+def test_longrange3d_pass_widths_against_oracle(dims, open_, R):
+    n = dims[0] * dims[1] * dims[2]
+    s = np.random.default_rng(n + 7).standard_normal((3, n))
+    os_ = O.SpinSystem(*dims, threads=True)
+    os_.setSpins(*s)
+    oracle_lr3(dims, open_, R, 0.3).apply(os_)
+    ho = flat(os_.field("LongRange3D"))
+    gl = M.LongRange3D(*dims, open=open_)
+    if R:
+        gl.setRadius(R)
+    gl.setStrength(0.3)
+    pw = gl.passWidths()
+    assert pw["z_choices"] and pw["z"] == pw["z_choices"][0]
+    assert pw["y"] == (pw["y_choices"][0] if pw["y_choices"] else pw["y"])
+    ys = pw["y_choices"] or [0]
+    base = None
+    for wy in ys:
+        for wz in pw["z_choices"]:
+            gl.setPassWidths(wy, wz)
+            got = gl.passWidths()
+            assert (got["y"] if wy else wy, got["z"]) == (wy, wz)
+            gs = M.SpinSystem(*dims)
+            gs.setSpins(*s)
+            gl.apply(gs)
+            hg = flat(gs.field("LongRange3D"))
+            err = np.abs(hg - ho).max() / np.abs(ho).max()
+            base = hg if base is None else base
+            print(f"lr3 {dims} open={open_} widths y={wy} z={wz}: rel err {err:.2e}, "
+                  f"vs first {np.abs(hg - base).max() / np.abs(ho).max():.2e}", flush=True)
+            assert err <= 2e-5
+    with pytest.raises(M.MagLuaError):
+        gl.setPassWidths(0, 3)
+
+
+def test_autotune_pass_widths_keeps_the_state_and_the_trajectory():
+    """The 3D autotuner restores the state it found, leaves its LongRange3D at the
+    chosen widths, and the driver then steps as one built with those widths."""
+    dims = (16, 32, 16)
+    runs = []
+    for _ in range(2):
+        spec, g, o, gl, ol = cube(dims, 0.4, seed=21)
+        runs.append((g, gl, M.Heun(g["ss"], g["ex"], g["ani"], gl, g["temp"], g["llg"])))
+    (ga, la, ha), (gb, lb, hb) = runs
+    ha.step(3)
+    hb.step(3)
+    before = flat(ga["ss"].spins())
+    t0 = ga["ss"].time()
+    tuned = ha.autotune(steps=3)
+    print("3D autotune:", tuned, flush=True)
+    assert tuned["ms_per_step"] > 0
+    assert np.array_equal(flat(ga["ss"].spins()), before) and ga["ss"].time() == t0
+    pw = la.passWidths()
+    assert (pw["y"], pw["z"]) == (tuned["y_width"], tuned["z_width"])
+    assert tuned["y_width"] in pw["y_choices"] and tuned["z_width"] in pw["z_choices"]
+    lb.setPassWidths(tuned["y_width"], tuned["z_width"])
+    ha.step(12)
+    hb.step(12)
+    assert np.array_equal(flat(ga["ss"].spins()), flat(gb["ss"].spins()))
+    assert ga["ss"].time() == gb["ss"].time()
+
+
+def test_ensemble_members_with_other_pass_widths():
+    """Members whose LongRange3D passes run at other kx widths share an ensemble (one
+    launch per pass form and width); widths changed between runs rebuild its tables and
+    graph.  Both match single replicas stepped the same way."""
+    dims = (8, 32, 16)
+    ens, solo = [], []
+    for seed in (31, 32):
+        for bucket in (ens, solo):
+            spec, g, o, gl, ol = cube(dims, 0.5, seed)
+            bucket.append((g, gl, M.Heun(g["ss"], g["ex"], g["ani"], gl, g["temp"], g["llg"])))
+    for bucket in (ens, solo):
+        bucket[1][1].setPassWidths(16, 8)
+    E = M.Ensemble([h for (_, _, h) in ens])
+    E.step(6)
+    for (_, _, h) in solo:
+        h.step(6)
+    for bucket in (ens, solo):
+        bucket[0][1].setPassWidths(4, 4)
+    E.step(6)
+    for (_, _, h) in solo:
+        h.step(6)
+    for (ge, _, _), (gs, _, _) in zip(ens, solo):
+        assert np.array_equal(flat(ge["ss"].spins()), flat(gs["ss"].spins()))
+        assert ge["ss"].time() == gs["ss"].time()
+
+
+def test_ensemble_autotune_keeps_the_states():
+    """The ensemble's pass-width autotuner restores every member, leaves each lattice's
+    members at one width pair, and the ensemble then steps as single replicas with
+    those widths."""
+    cfg = [((8, 32, 16), 0.5, 41), ((8, 32, 16), 0.2, 42), ((16, 16, 32), 0.0, 43), ((6, 8, 8), 0.3, 44)]
+    ens, solo = [], []
+    for dims, T, seed in cfg:
+        for bucket in (ens, solo):
+            spec, g, o, gl, ol = cube(dims, T, seed)
+            bucket.append((g, gl, M.Heun(g["ss"], g["ex"], g["ani"], gl, g["temp"], g["llg"])))
+    E = M.Ensemble([h for (_, _, h) in ens])
+    E.step(3)
+    for (_, _, h) in solo:
+        h.step(3)
+    before = [flat(g["ss"].spins()) for (g, _, _) in ens]
+    tuned = E.autotune(steps=2)
+    print("ensemble autotune:", tuned, flush=True)
+    assert tuned["ms_per_step"] > 0
+    for (g, _, _), b in zip(ens, before):
+        assert np.array_equal(flat(g["ss"].spins()), b)
+    w = [gl.passWidths() for (_, gl, _) in ens]
+    assert (w[0]["y"], w[0]["z"]) == (w[1]["y"], w[1]["z"])
+    for (_, gl, _), (_, sl, _) in zip(ens, solo):
+        pw = gl.passWidths()
+        sl.setPassWidths(pw["y"] if pw["y_choices"] else 0, pw["z"] if pw["z_choices"] else 0)
+    E.step(8)
+    for (_, _, h) in solo:
+        h.step(8)
+    for (ge, _, _), (gs, _, _) in zip(ens, solo):
+        assert np.array_equal(flat(ge["ss"].spins()), flat(gs["ss"].spins()))
+        assert ge["ss"].time() == gs["ss"].time()
+
+
 def film(dims, T, seed, g=0.05, K=0.05):
     spec = W.FilmSpec(*dims, J=1.0, K=K, g=g, pbc=(1, 1, 0), alpha=0.1, dt=0.004, T=T, seed=seed, noise_seed=seed)
     gpu = W.build(M, spec)
