@@ -144,7 +144,8 @@ def measure_traffic(film, schedule, kernel, comp, timeout_s=150):
                     if r["Counter_Name"] != ctr:
                         continue
                     name = r["Kernel_Name"]
-                    if f"{kernel}<" in name and "batch" not in name:
+                    # the split y transform's column kernel is k_cols_split<...>
+                    if (f"{kernel}<" in name or f"{kernel}_split<" in name) and "batch" not in name:
                         kern.append(float(r["Counter_Value"]))
                     elif "copyBuffer" in name:
                         copies.append(float(r["Counter_Value"]))
@@ -661,7 +662,9 @@ def main():
             "kernel": dom, "kernel_avg_ms": round(prof[dom], 5), "algorithmic_bytes": kb[dom],
             "step_model_bytes_per_spin": round(step_model_bytes(spec, not args.plain), 2),
             "step_model_GBs": round(step_model_bytes(spec, not args.plain) * value / 1e9, 1),
-            "kernels_avg_ms": {k: round(v, 5) for k, v in prof.items()}}
+            "kernels_avg_ms": {k: round(v, 5) for k, v in prof.items()},
+            "kernels": {k: {"avg_ms": round(v, 5), "algorithmic_bytes": kb[k], "launches_per_step": 2 if k == "k_cols" else 1,
+                            "frac": round(kb[k] / (v * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)} for k, v in prof.items() if k in kb}}
 
     learned = None
     if not args.no_learned:

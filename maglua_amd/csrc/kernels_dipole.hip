@@ -19,6 +19,8 @@
 // shaped instance (NX/NY == 0) for the other mixed-radix sizes.
 #include "msd_device.h"
 
+#include <type_traits>
+
 namespace msd {
 
 // threads per workgroup of the runtime-shaped row kernels (compile-time shapes
@@ -519,9 +521,13 @@ __global__ void __launch_bounds__(col_tpb(NY)) k_cols_batch(const DipoleDev* __r
 //
 // Product lanes: wave w forms output component w / (ly / 64) (one wave per component
 // below ly = 64, lanes from ly on idle), so the branch on the component's odd words
-// (XZ, YZ: the third word of x and y, the first two of z) is wave-uniform.
+// (XZ, YZ: the third word of x and y, the first two of z) is wave-uniform.  From
+// ly = 64 on, with an even nz, the destination layers are split in two halves over
+// twice the lanes (split_halves): twice the waves per workgroup take the latency of
+// the loads and stores, and each lane's product is half as long.
 __host__ __device__ constexpr int split_pitch(int nz) { return (3 * nz) | 1; }
-__host__ __device__ constexpr int split_tpb(int L) { return L < 64 ? 192 : 3 * L; }
+__host__ __device__ constexpr int split_halves(int L, int nz) { return (L >= 64 && nz % 2 == 0 && 3 * L <= 512) ? 2 : 1; }
+__host__ __device__ constexpr int split_tpb(int L, int nz) { return (L < 64 ? 192 : 3 * L) * split_halves(L, nz); }
 
 // One in-place Stockham stage over LN position-major lines (pitch PS), NT threads
 // with the line index fastest across lanes.
@@ -578,7 +584,8 @@ struct SplitFFT<L, LN, PS, NT, SIGN, L> {
 template <int L, int NZ>
 __device__ __forceinline__ void split_cols_body(const DipoleDev& dp, int wg)
 {
-    constexpr int NT = split_tpb(L), LN = 3 * NZ, PS = split_pitch(NZ);
+    constexpr int NT = split_tpb(L, NZ), LN = 3 * NZ, PS = split_pitch(NZ);
+    constexpr int HV = split_halves(L, NZ), TN = NZ / HV;    // destination layers per lane
     extern __shared__ float2 lds[];
     const int R = dp.split, ny = dp.ny, nxh = dp.nxh;
     const int kx = wg / R, k1 = wg - kx * R;
@@ -627,25 +634,29 @@ __device__ __forceinline__ void split_cols_body(const DipoleDev& dp, int wg)
     SplitFFT<L, LN, PS, NT, -1, 1>::run(A, twl);
     {
         // lane -> (output component oa, k2); zpart: the wave of component z
-        int oa, k2;
+        int oa, k2, t0 = 0;
         bool active;
         if constexpr (L < 64) {
             oa = (int)threadIdx.x >> 6;
             k2 = (int)threadIdx.x & 63;
             active = k2 < L;
         } else {
-            oa = (int)threadIdx.x / L;
-            k2 = (int)threadIdx.x - oa * L;
+            const int h = (int)threadIdx.x / (3 * L), rem = (int)threadIdx.x - h * 3 * L;
+            oa = rem / L;
+            k2 = rem - oa * L;
+            t0 = h * TN;                // wave-uniform: 3 L is a multiple of 64
             active = true;
         }
         const bool zpart = __builtin_amdgcn_readfirstlane(oa) == 2;
         const int c0 = oa == 0 ? 0 : (oa == 1 ? 1 : 2);
         const int c1 = oa == 0 ? 1 : (oa == 1 ? 3 : 4);
         const int c2 = oa == 0 ? 2 : (oa == 1 ? 4 : 5);
-        float2 acc[NZ];
+        float2 acc[TN];
 #pragma unroll
-        for (int t = 0; t < NZ; ++t) acc[t] = make_float2(0.f, 0.f);
-        if (active) {
+        for (int t = 0; t < TN; ++t) acc[t] = make_float2(0.f, 0.f);
+        // destination layers T0 .. T0 + TN - 1 (compile-time, so every sign folds)
+        auto product = [&](auto t0c) {
+            constexpr int T0 = decltype(t0c)::value;
             float w[3][NZ];
 #pragma unroll
             for (int o = 0; o < NZ; ++o) {
@@ -665,25 +676,31 @@ __device__ __forceinline__ void split_cols_body(const DipoleDev& dp, int wg)
 #pragma unroll
                 for (int b = 0; b < 3; ++b) m[b] = Ak[b * NZ + s];
 #pragma unroll
-                for (int t = 0; t < NZ; ++t) {
+                for (int tt = 0; tt < TN; ++tt) {
+                    const int t = T0 + tt;
                     const int o = s >= t ? s - t : t - s;
                     const float sg = s < t ? -1.0f : 1.0f;
                     if (zpart) {
-                        mac(acc[t], w[0][o], m[0], true, sg);
-                        mac(acc[t], w[1][o], m[1], true, sg);
-                        mac(acc[t], w[2][o], m[2], false, sg);
+                        mac(acc[tt], w[0][o], m[0], true, sg);
+                        mac(acc[tt], w[1][o], m[1], true, sg);
+                        mac(acc[tt], w[2][o], m[2], false, sg);
                     } else {
-                        mac(acc[t], w[0][o], m[0], false, sg);
-                        mac(acc[t], w[1][o], m[1], false, sg);
-                        mac(acc[t], w[2][o], m[2], true, sg);
+                        mac(acc[tt], w[0][o], m[0], false, sg);
+                        mac(acc[tt], w[1][o], m[1], false, sg);
+                        mac(acc[tt], w[2][o], m[2], true, sg);
                     }
                 }
             }
+        };
+        if (active) {
+            if constexpr (HV == 1) product(std::integral_constant<int, 0>{});
+            else if (t0 == 0) product(std::integral_constant<int, 0>{});
+            else product(std::integral_constant<int, TN>{});
         }
         __syncthreads();    // every lane has read its sources: the products go in place
         if (active) {
 #pragma unroll
-            for (int t = 0; t < NZ; ++t) A[k2 * PS + oa * NZ + t] = acc[t];
+            for (int tt = 0; tt < TN; ++tt) A[k2 * PS + oa * NZ + t0 + tt] = acc[tt];
         }
     }
     __syncthreads();
@@ -696,14 +713,14 @@ __device__ __forceinline__ void split_cols_body(const DipoleDev& dp, int wg)
 }
 
 template <int L, int NZ>
-__global__ void __launch_bounds__(split_tpb(L)) k_cols_split(DipoleDev dp)
+__global__ void __launch_bounds__(split_tpb(L, NZ)) k_cols_split(DipoleDev dp)
 {
     split_cols_body<L, NZ>(dp, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // films of one shape in one launch: `per` = nxh R workgroups each
 template <int L, int NZ>
-__global__ void __launch_bounds__(split_tpb(L)) k_cols_split_batch(const DipoleDev* __restrict__ dps, int per)
+__global__ void __launch_bounds__(split_tpb(L, NZ)) k_cols_split_batch(const DipoleDev* __restrict__ dps, int per)
 {
     const int w = xcd_remap(blockIdx.x, gridDim.x), r = w / per;
     split_cols_body<L, NZ>(dps[r], w - r * per);
@@ -1203,7 +1220,7 @@ static void launch_cols_split_nz(const DipoleDev& dp, hipStream_t st)
 {
 #define MSD_CS(L)                                                                   \
     if (dp.ly == L) {                                                               \
-        k_cols_split<L, NZ><<<dp.nxh * dp.split, split_tpb(L), dp.lds_cols, st>>>(dp); \
+        k_cols_split<L, NZ><<<dp.nxh * dp.split, split_tpb(L, NZ), dp.lds_cols, st>>>(dp); \
         return;                                                                     \
     }
     MSD_SPLIT_LENGTHS(MSD_CS)
@@ -1374,7 +1391,7 @@ static void launch_cols_split_batch_nz(const DipoleDev& d0, const DipoleDev* dps
 #define MSD_CSB(L)                                                                  \
     if (d0.ly == L) {                                                               \
         const int per = d0.nxh * d0.split;                                          \
-        k_cols_split_batch<L, NZ><<<per * nrep, split_tpb(L), d0.lds_cols, st>>>(dps, per); \
+        k_cols_split_batch<L, NZ><<<per * nrep, split_tpb(L, NZ), d0.lds_cols, st>>>(dps, per); \
         return;                                                                     \
     }
     MSD_SPLIT_LENGTHS(MSD_CSB)

@@ -402,6 +402,26 @@ def test_ensemble_with_films_matches_single_replicas(schedule, monkeypatch):
     assert ens[0][0]["temp"].draw() == 12
 
 
+def test_ensemble_of_wide_films_with_split_product(monkeypatch):
+    """256x256 films on the split y transform with sub-columns of 64 ("4:512:4"), whose
+    column workgroups split the layer product over two lane halves, stepped in one
+    Ensemble: each equals its single-replica run bit for bit."""
+    monkeypatch.setenv("MSD_ROW_TILE", "4:512:4")
+    cfg = [((256, 256, 2), 0.4, 21), ((256, 256, 4), 0.0, 22), ((256, 256, 2), 0.9, 23)]
+    ens, solo = [], []
+    for bucket in (ens, solo):
+        for dims, T, seed in cfg:
+            spec, g = film(dims, T, seed)
+            bucket.append((g, M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"])))
+    assert ens[0][0]["dip"].schedule()["split"] == 4
+    E = M.Ensemble([h for (_, h) in ens])
+    E.step(6)
+    for (_, h) in solo:
+        h.step(6)
+    for k, ((ge, _), (gs, _)) in enumerate(zip(ens, solo)):
+        assert np.array_equal(flat(ge["ss"].spins()), flat(gs["ss"].spins())), k
+
+
 @pytest.mark.parametrize("schedule", [None, "8:256:8"])
 def test_ensemble_films_against_oracle(schedule, monkeypatch):
     """Two 64x64x4 films with strong dipolar coupling (g = 0.1) in one Ensemble at
