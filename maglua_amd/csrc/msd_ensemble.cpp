@@ -146,19 +146,32 @@ extern "C" int msd_ensemble_heun(msd_ensemble* e, int nsteps)
         return err;
     };
     if (e->use_graph) {
-        if (!e->exec) {
+        // a graph of `reps` steps, captured once and kept until the members change
+        auto ready = [&](hipGraph_t* graph, hipGraphExec_t* exec, int reps) -> int {
+            if (*exec) return MSD_OK;
             HIPCHK(hipStreamBeginCapture(G.stream, hipStreamCaptureModeThreadLocal));
-            hipError_t err = one_step(G.stream);
+            hipError_t err = hipSuccess;
+            for (int q = 0; q < reps && err == hipSuccess; ++q) err = one_step(G.stream);
             hipGraph_t g = nullptr;
             hipError_t e2 = hipStreamEndCapture(G.stream, &g);
             if (err != hipSuccess || e2 != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
                 return fail(MSD_EDEVICE, "ensemble graph: %s", hipGetErrorString(err != hipSuccess ? err : e2));
             }
-            e->graph = g;
-            HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+            *graph = g;
+            HIPCHK(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+            return MSD_OK;
+        };
+        // runs of at least STEP_SPAN steps replay the span graph, the rest one step each
+        if (nsteps >= STEP_SPAN) {
+            if (int rc = ready(&e->mgraph, &e->mexec, STEP_SPAN)) return rc;
+            for (int i = 0; i < nsteps / STEP_SPAN; ++i) HIPCHK(hipGraphLaunch(e->mexec, G.stream));
         }
-        for (int i = 0; i < nsteps; ++i) HIPCHK(hipGraphLaunch(e->exec, G.stream));
+        const int single = nsteps >= STEP_SPAN ? nsteps % STEP_SPAN : nsteps;
+        if (single > 0) {
+            if (int rc = ready(&e->graph, &e->exec, 1)) return rc;
+            for (int i = 0; i < single; ++i) HIPCHK(hipGraphLaunch(e->exec, G.stream));
+        }
     } else {
         for (int i = 0; i < nsteps; ++i) {
             hipError_t err = one_step(G.stream);

@@ -579,6 +579,11 @@ struct Batch3 {
 };
 
 // ============================================================ step driver
+// Exact steps per span graph (msd_stepper::mgraph, msd_ensemble::mgraph): a graph
+// launch costs about 4.6 us between steps on MI355X, so 256x256x8 film steps of 64 us
+// replayed one graph per step run 7% slower than spans of 16.
+constexpr int STEP_SPAN = 16;
+
 // The step closure of PredictorCorrector.lua with one corrector pass, as fused
 // kernels.  Three forms: local fields only, layered dipole (Dipole), or the 3D
 // convolution (LongRange3D); a stepper of the 3D form is a batch of one.
@@ -714,8 +719,8 @@ struct msd_stepper {
         hipGraphNode_t node;
         int variant;
     };
-    std::vector<StepNode> gnodes[3];
-    bool gnodes_ok[3] = {false, false, false};          // every step launch of the graph found
+    std::vector<StepNode> gnodes[4];                    // [3]: the span graph's
+    bool gnodes_ok[4] = {false, false, false, false};   // every step launch of the graph found
     std::vector<std::pair<const void*, int>> cap_log;   // step kernels launched during a capture
     bool capturing = false;
     long graph_builds = 0;               // graphs captured and instantiated (diagnostics)
@@ -741,6 +746,11 @@ struct msd_stepper {
     bool have_ref = false;
     long sur_windows = 0, sur_exact_windows = 0, sur_fallbacks = 0;
     double sur_last_res = 0.0, sur_max_res = 0.0;
+    // span graph: STEP_SPAN exact steps captured in one graph, replayed for runs of
+    // at least that many steps (one graph launch's overhead per span, not per step)
+    hipGraph_t mgraph = nullptr;
+    hipGraphExec_t mexec = nullptr;
+    StepArgs mgraph_args{};
     // learned-step graphs: [0] a window's first step, [1] the others
     hipGraph_t lgraph[2] = {nullptr, nullptr};
     hipGraphExec_t lexec[2] = {nullptr, nullptr};
@@ -763,6 +773,11 @@ struct msd_stepper {
         exec = nullptr; graph = nullptr;
         gnodes[0].clear();
         gnodes_ok[0] = false;
+        if (mexec) (void)hipGraphExecDestroy(mexec);
+        if (mgraph) (void)hipGraphDestroy(mgraph);
+        mexec = nullptr; mgraph = nullptr;
+        gnodes[3].clear();
+        gnodes_ok[3] = false;
         drop_learned_graphs();
         if (rk_exec) (void)hipGraphExecDestroy(rk_exec);
         if (rk_graph) (void)hipGraphDestroy(rk_graph);
@@ -909,8 +924,10 @@ struct msd_ensemble {
     bool use_graph = true;
     Batch3 b;                        // local-field and LongRange3D members
     std::vector<FilmGroup> films;    // layered-Dipole members, by shape
-    hipGraph_t graph = nullptr;
+    hipGraph_t graph = nullptr;      // one step
     hipGraphExec_t exec = nullptr;
+    hipGraph_t mgraph = nullptr;     // STEP_SPAN steps
+    hipGraphExec_t mexec = nullptr;
     ~msd_ensemble()
     {
         (void)hipStreamSynchronize(G.stream);
@@ -922,6 +939,9 @@ struct msd_ensemble {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         exec = nullptr; graph = nullptr;
+        if (mexec) (void)hipGraphExecDestroy(mexec);
+        if (mgraph) (void)hipGraphDestroy(mgraph);
+        mexec = nullptr; mgraph = nullptr;
     }
     int refresh()
     {

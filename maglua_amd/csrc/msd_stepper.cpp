@@ -225,15 +225,18 @@ void stepper_end(msd_stepper* st, int nsteps)
     if (st->lr3) st->lr3_epoch = ++st->lr3->epoch;
 }
 
-// capture one step into a graph: kind 0 exact, 1 a learned window's first step, 2 a
-// later learned step
-static int capture_step(msd_stepper* st, int kind, hipGraph_t* graph, hipGraphExec_t* exec)
+// Capture the steps of graph slot `slot` into a graph: 0 one exact step, 1 a learned
+// window's first step, 2 a later learned step, 3 STEP_SPAN exact steps (the span graph)
+static int capture_step(msd_stepper* st, int slot, hipGraph_t* graph, hipGraphExec_t* exec)
 {
+    const int kind = slot == 3 ? 0 : slot;
     st->cap_log.clear();
-    st->gnodes[kind].clear();
+    st->gnodes[slot].clear();
     st->capturing = true;
     HIPCHK(hipStreamBeginCapture(G.stream, hipStreamCaptureModeThreadLocal));
-    int rc = kind ? st->launch_learned_step(G.stream, kind == 1) : st->launch_step(G.stream);
+    int rc = MSD_OK;
+    for (int q = 0; q < (slot == 3 ? STEP_SPAN : 1) && !rc; ++q)
+        rc = kind ? st->launch_learned_step(G.stream, kind == 1) : st->launch_step(G.stream);
     st->capturing = false;
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(G.stream, &g);
@@ -252,20 +255,20 @@ static int capture_step(msd_stepper* st, int kind, hipGraph_t* graph, hipGraphEx
         hipKernelNodeParams p{};
         if (hipGraphKernelNodeGetParams(node, &p) != hipSuccess) continue;
         for (const auto& c : st->cap_log)
-            if (c.first == p.func) { st->gnodes[kind].push_back({node, c.second}); break; }
+            if (c.first == p.func) { st->gnodes[slot].push_back({node, c.second}); break; }
     }
     // the 3D form's kernels read their arguments from device tables (Batch3), which
     // refresh() rewrites: its graphs have no step nodes to update
-    st->gnodes_ok[kind] = st->gnodes[kind].size() == st->cap_log.size();
+    st->gnodes_ok[slot] = st->gnodes[slot].size() == st->cap_log.size();
     return MSD_OK;
 }
 
 // New argument values into the step nodes of an instantiated graph; false when the
 // graph's step nodes were not all found (the caller captures again)
-static bool update_step_nodes(msd_stepper* st, int kind, hipGraphExec_t exec)
+static bool update_step_nodes(msd_stepper* st, int slot, hipGraphExec_t exec)
 {
-    if (!st->gnodes_ok[kind]) return false;
-    for (const msd_stepper::StepNode& sn : st->gnodes[kind]) {
+    if (!st->gnodes_ok[slot]) return false;
+    for (const msd_stepper::StepNode& sn : st->gnodes[slot]) {
         StepArgs v = st->variant_args(sn.variant);
         hipKernelNodeParams p{};
         if (hipGraphKernelNodeGetParams(sn.node, &p) != hipSuccess) return false;
@@ -277,33 +280,54 @@ static bool update_step_nodes(msd_stepper* st, int kind, hipGraphExec_t exec)
     return true;
 }
 
+// The instantiated graph of slot `slot` (as capture_step) for the current arguments:
+// kept, its step nodes given new values in place, or captured again.
+static int slot_graph(msd_stepper* st, int slot, hipGraphExec_t* out)
+{
+    const int kind = slot == 3 ? 0 : slot;
+    hipGraphExec_t* exec = slot == 3 ? &st->mexec : kind ? &st->lexec[kind - 1] : &st->exec;
+    hipGraph_t* graph = slot == 3 ? &st->mgraph : kind ? &st->lgraph[kind - 1] : &st->graph;
+    StepArgs* gargs = slot == 3 ? &st->mgraph_args : kind ? &st->lgraph_args[kind - 1] : &st->graph_args;
+    const bool scales_moved =
+        kind && (st->lgraph_sin[kind - 1] != st->sur->s_in || st->lgraph_sout[kind - 1] != st->sur->s_out);
+    bool fresh = *exec && same_launch(st->args, *gargs) && !scales_moved;
+    // same kernels and shapes with other values: the step nodes take them in place
+    if (fresh && std::memcmp(&st->args, gargs, sizeof(StepArgs)) != 0) {
+        fresh = update_step_nodes(st, slot, *exec);
+        if (fresh) *gargs = st->args;
+    }
+    if (!fresh) {
+        if (*exec) (void)hipGraphExecDestroy(*exec);
+        if (*graph) (void)hipGraphDestroy(*graph);
+        *exec = nullptr; *graph = nullptr;
+        if (int rc = capture_step(st, slot, graph, exec)) return rc;
+        st->graph_builds++;
+        *gargs = st->args;
+        if (kind) { st->lgraph_sin[kind - 1] = st->sur->s_in; st->lgraph_sout[kind - 1] = st->sur->s_out; }
+    }
+    *out = *exec;
+    return MSD_OK;
+}
+
 // nsteps of one kind (0 exact, 1 a learned window's first step, 2 later learned
-// steps), graph-replayed when enabled
+// steps), graph-replayed when enabled; exact runs of at least STEP_SPAN steps replay
+// the span graph and the remainder one step at a time
 static int run_kind(msd_stepper* st, int nsteps, int kind)
 {
     if (nsteps <= 0) return MSD_OK;
     if (st->use_graph) {
-        hipGraphExec_t* exec = kind ? &st->lexec[kind - 1] : &st->exec;
-        hipGraph_t* graph = kind ? &st->lgraph[kind - 1] : &st->graph;
-        StepArgs* gargs = kind ? &st->lgraph_args[kind - 1] : &st->graph_args;
-        const bool scales_moved =
-            kind && (st->lgraph_sin[kind - 1] != st->sur->s_in || st->lgraph_sout[kind - 1] != st->sur->s_out);
-        bool fresh = *exec && same_launch(st->args, *gargs) && !scales_moved;
-        // same kernels and shapes with other values: the step nodes take them in place
-        if (fresh && std::memcmp(&st->args, gargs, sizeof(StepArgs)) != 0) {
-            fresh = update_step_nodes(st, kind, *exec);
-            if (fresh) *gargs = st->args;
+        int single = nsteps;
+        if (kind == 0 && nsteps >= STEP_SPAN) {
+            hipGraphExec_t span = nullptr;
+            if (int rc = slot_graph(st, 3, &span)) return rc;
+            for (int i = 0; i < nsteps / STEP_SPAN; ++i) HIPCHK(hipGraphLaunch(span, G.stream));
+            single = nsteps % STEP_SPAN;
         }
-        if (!fresh) {
-            if (*exec) (void)hipGraphExecDestroy(*exec);
-            if (*graph) (void)hipGraphDestroy(*graph);
-            *exec = nullptr; *graph = nullptr;
-            if (int rc = capture_step(st, kind, graph, exec)) return rc;
-            st->graph_builds++;
-            *gargs = st->args;
-            if (kind) { st->lgraph_sin[kind - 1] = st->sur->s_in; st->lgraph_sout[kind - 1] = st->sur->s_out; }
+        if (single > 0) {
+            hipGraphExec_t one = nullptr;
+            if (int rc = slot_graph(st, kind, &one)) return rc;
+            for (int i = 0; i < single; ++i) HIPCHK(hipGraphLaunch(one, G.stream));
         }
-        for (int i = 0; i < nsteps; ++i) HIPCHK(hipGraphLaunch(*exec, G.stream));
     } else {
         for (int i = 0; i < nsteps; ++i)
             if (int rc = kind ? st->launch_learned_step(G.stream, kind == 1) : st->launch_step(G.stream)) return rc;
