@@ -198,6 +198,7 @@ def learned_vs_plain(M, W, train_pairs=20000, states=4, train_K=10, Ks=(10, 8, 6
     g["ss"].randomize(1)
     g["temp"].setSeed(1)
     heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"])
+    plain_schedule = heun.autotune(steps=20)      # the plain path at its best schedule
     heun.step(50)
     M.synchronize()
     t0 = time.perf_counter()
@@ -311,6 +312,7 @@ def learned_vs_plain(M, W, train_pairs=20000, states=4, train_K=10, Ks=(10, 8, 6
         "gate": f"max|dm| <= {gate:g} over {gate_steps} steps against the exact trajectory (same state and "
                 f"noise); every window kept has its field error at its end <= {tol:g} (others run again exactly)",
         "gate_met": gate_ok, "K_scan": scan,
+        "plain_schedule": plain_schedule,
         "plain_step_us": round(1e6 * plain_s, 2), "local_step_us": round(1e6 * local_s, 2),
         "cnn_eval_us": round(1e6 * cnn_s, 2),
         "amdahl_bound_free_network": round(plain_s / local_s, 3),
@@ -435,10 +437,15 @@ def ensemble_leg(M, W, nrep=64, steps=50, R=16, tune_steps=10):
             "total_field_dot_spin_per_site": hdot}
 
 
-def films_leg(M, W, nrep=8, steps=100):
+def films_leg(M, W, nrep=8, steps=100, schedule=None):
     """nrep 256x256x8 films (the film's J, K, g, alpha, dt; temperatures 0.1..0.8, own
-    seeds) in one Ensemble: each pass of a Heun step is one launch over all films."""
+    seeds) in one Ensemble: each pass of a Heun step is one launch over all films.
+    schedule: the row-tile shape and y-transform schedule the single film was tuned to
+    (the members' Dipoles are created with it)."""
     from dataclasses import replace
+    saved = os.environ.get("MSD_ROW_TILE")
+    if schedule:
+        os.environ["MSD_ROW_TILE"] = f"{schedule['rows_per_tile']}:{schedule['threads']}:{schedule['split']}"
     t0 = time.perf_counter()
     members, keep, nsites, temps = [], [], 0, []
     for i in range(nrep):
@@ -450,6 +457,10 @@ def films_leg(M, W, nrep=8, steps=100):
         keep.append(g)
         nsites += spec.n
         temps.append(round(T, 4))
+    if saved is None:
+        os.environ.pop("MSD_ROW_TILE", None)
+    else:
+        os.environ["MSD_ROW_TILE"] = saved
     E = M.Ensemble(members)
     E.step(5)
     M.synchronize()
@@ -461,7 +472,8 @@ def films_leg(M, W, nrep=8, steps=100):
     return {"workload": f"{nrep} films 256x256x8 (J=1, K=0.05, g=0.02, alpha=0.1, dt=0.002), T 0.1..0.8, one "
                         "Ensemble: one launch per pass for all films",
             "value": round(nsites * steps / el, 1), "unit": "spin-steps/s", "sites": nsites, "steps": steps,
-            "ms_per_step": round(1e3 * el / steps, 4), "setup_s": round(setup, 2), "temperatures": temps}
+            "ms_per_step": round(1e3 * el / steps, 4), "setup_s": round(setup, 2), "temperatures": temps,
+            "schedule": keep[0]["dip"].schedule()}
 
 
 def cube_learned(M, W, spec, g, ani, lr, heun, plain, train_steps=100, train_K=8, Ks=(8, 4, 2, 1), gate=1e-4,
@@ -676,7 +688,7 @@ def main():
         print("# batched replicas and open cubes...", file=sys.stderr, flush=True)
         tune3 = min(args.tune_steps, 10)     # 3D pass widths: ten steps per candidate pair
         ensemble = ensemble_leg(M, W, tune_steps=tune3)
-        ensemble["films"] = films_leg(M, W)
+        ensemble["films"] = films_leg(M, W, schedule=schedule)
         cube = cube_leg(M, W, tune_steps=tune3)
 
     cpu = None

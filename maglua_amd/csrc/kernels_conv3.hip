@@ -43,8 +43,8 @@ __device__ __forceinline__ TensorWords tensor_words(const Conv3Dev& cv, int ky, 
     const bool mir = kz >= cv.fzs;
     const size_t plane = (size_t)cv.fy * cv.fxh * cv.fzs;
     const size_t t = ((size_t)ky * cv.fxh + kx) * cv.fzs + (mir ? cv.fz - kz : kz);
-    TensorWords w{cv.Tw[t], cv.Tw[plane + t], cv.Tw[2 * plane + t], cv.Tw[3 * plane + t], cv.Tw[4 * plane + t],
-                  cv.Tw[5 * plane + t]};
+    TensorWords w{gload(cv.Tw, t), gload(cv.Tw, plane + t), gload(cv.Tw, 2 * plane + t), gload(cv.Tw, 3 * plane + t),
+                  gload(cv.Tw, 4 * plane + t), gload(cv.Tw, 5 * plane + t)};
     if (mir) { w.xz = -w.xz; w.yz = -w.yz; }
     return w;
 }
@@ -64,13 +64,14 @@ __device__ __forceinline__ void tensor_apply(const TensorWords& w, float2& px, f
 // 2l (real part) and 2l+1 (imaginary part).  Tiles read and write the spectrum
 // W[c][z][y][kx] row by row, kx fastest across the threads (contiguous runs).
 
-// Copy a small exchange class table into LDS; larger tables stay in global memory.
+// Copy the exchange class table into LDS (the fused form has one only when it fits,
+// msd_exchange::fused_dev).
 template <int MODE>
 __device__ __forceinline__ ExchangeDev rows_exchange(const ExchangeDev& ex, int* s_start, ExEntry* s_ent)
 {
     ExchangeDev e = ex;
     if (MODE == R3_PRED || MODE == R3_CORR) {
-        if (ex.mode == 1 && ex.ncls + 1 <= 64 && ex.nent <= 512) {
+        if (ex.mode == 1) {
             for (int i = threadIdx.x; i <= ex.ncls; i += blockDim.x) s_start[i] = ex.cls_start[i];
             for (int i = threadIdx.x; i < ex.nent; i += blockDim.x) s_ent[i] = ex.ent[i];
             e.cls_start = s_start;
@@ -87,11 +88,11 @@ __device__ __forceinline__ float4 rows_load_pair(const Conv3Dev& cv, int nz, int
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
     if (q0 < 3 * ty) {
         const int c0 = q0 / ty, r0 = q0 - c0 * ty;
-        const float2 a = cv.W[widx(cv, nz, c0, z, y0 + r0, kk)];
+        const float2 a = gload(cv.W, widx(cv, nz, c0, z, y0 + r0, kk));
         v.x = a.x; v.y = a.y;
         if (q1 < 3 * ty) {
             const int c1 = q1 / ty, r1 = q1 - c1 * ty;
-            const float2 b = cv.W[widx(cv, nz, c1, z, y0 + r1, kk)];
+            const float2 b = gload(cv.W, widx(cv, nz, c1, z, y0 + r1, kk));
             v.z = b.x; v.w = b.y;
         }
     }
@@ -113,10 +114,10 @@ __device__ __forceinline__ void rows_store_pair(const Conv3Dev& cv, int nz, int 
 {
     const int q0 = 2 * l, q1 = q0 + 1;
     const int c0 = q0 / ty, r0 = q0 - c0 * ty;
-    cv.W[widx(cv, nz, c0, z, y0 + r0, k)] = make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y));
+    gstore(cv.W, widx(cv, nz, c0, z, y0 + r0, k), make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y)));
     if (q1 < 3 * ty) {
         const int c1 = q1 / ty, r1 = q1 - c1 * ty;
-        cv.W[widx(cv, nz, c1, z, y0 + r1, k)] = make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x));
+        gstore(cv.W, widx(cv, nz, c1, z, y0 + r1, k), make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x)));
     }
 }
 
@@ -136,12 +137,12 @@ __device__ __forceinline__ void rows_site(const Rep3& rp, const ExchangeDev& exl
     if (x < nx) {
         const int i = x + nx * (y0 + r + a.d.ny * z);
         if (MODE == R3_FWD) {
-            v = make_float3(cv.sin[i], cv.sin[n + i], cv.sin[2 * n + i]);
+            v = make_float3(gload(cv.sin, i), gload(cv.sin, (size_t)n + i), gload(cv.sin, 2 * (size_t)n + i));
         } else {
             float3 Hd = make_float3(0.f, 0.f, 0.f);
             if (lr) Hd = make_float3(rowval3(R, r, x, fx), rowval3(R, ty + r, x, fx), rowval3(R, 2 * ty + r, x, fx));
             if (MODE == R3_INV) {
-                cv.hout[i] = Hd.x; cv.hout[n + i] = Hd.y; cv.hout[2 * n + i] = Hd.z;
+                gstore(cv.hout, i, Hd.x); gstore(cv.hout, (size_t)n + i, Hd.y); gstore(cv.hout, 2 * (size_t)n + i, Hd.z);
             } else if (MODE == R3_PRED) {
                 v = site_pred<COMP>(a, exl, i, Hd);
             } else {
@@ -160,8 +161,8 @@ template <int MODE, bool COMP>
 __global__ void __launch_bounds__(R3_TPB) k3_rows(const Rep3* __restrict__ reps, const int4* __restrict__ items)
 {
     extern __shared__ float2 lds[];
-    __shared__ int s_start[64];
-    __shared__ ExEntry s_ent[512];
+    __shared__ int s_start[EX_LDS_CLS];
+    __shared__ ExEntry s_ent[EX_LDS_ENT];
     const int4 it = items[blockIdx.x];
     const Rep3& rp = reps[it.x];
     const StepArgs& a = rp.a;
@@ -232,8 +233,8 @@ __global__ void __launch_bounds__(rows_template_threads<FX>()) k3_rows_t(const R
     constexpr int TY = rows_template_ty<FX>(), LINES = (3 * TY + 1) / 2, FXH = FX / 2 + 1, HALF = FX / 2;
     constexpr int NT = rows_template_threads<FX>();
     extern __shared__ float2 lds[];                     // [LINES][FX] lines, then [FX] twiddles
-    __shared__ int s_start[64];
-    __shared__ ExEntry s_ent[512];
+    __shared__ int s_start[EX_LDS_CLS];
+    __shared__ ExEntry s_ent[EX_LDS_ENT];
     const int4 it = items[blockIdx.x];
     const Rep3& rp = reps[it.x];
     const StepArgs& a = rp.a;
@@ -248,7 +249,7 @@ __global__ void __launch_bounds__(rows_template_threads<FX>()) k3_rows_t(const R
 
     const ExchangeDev exl = rows_exchange<MODE>(a.ex, s_start, s_ent);
     if (lr)
-        for (int i = threadIdx.x; i < FX; i += NT) s_tw[i] = cv.twx[i];
+        for (int i = threadIdx.x; i < FX; i += NT) s_tw[i] = gload(cv.twx, i);
     if (lr && MODE != R3_FWD) {
         batched_copy<LINES * FX, NT, 8>(
             [&](int idx) {
@@ -304,14 +305,14 @@ __global__ void __launch_bounds__(Y3_TPB) k3_ypass(const Rep3* __restrict__ reps
     const int yin = SIGN < 0 ? ny : fy;
     for (int idx = threadIdx.x; idx < w * fy; idx += Y3_TPB) {
         const int y = idx / w, j = idx - y * w;
-        A[(size_t)j * fy + y] = y < yin ? base[(size_t)y * fxp + j] : make_float2(0.f, 0.f);
+        A[(size_t)j * fy + y] = y < yin ? gload(base, (size_t)y * fxp + j) : make_float2(0.f, 0.f);
     }
     __syncthreads();
     const float2* R = lds_fft<SIGN>(A, B, w, cv.py, cv.twy);
     const int yout = SIGN < 0 ? fy : ny;
     for (int idx = threadIdx.x; idx < w * yout; idx += Y3_TPB) {
         const int y = idx / w, j = idx - y * w;
-        base[(size_t)y * fxp + j] = R[(size_t)j * fy + y];
+        gstore(base, (size_t)y * fxp + j, R[(size_t)j * fy + y]);
     }
 }
 
@@ -338,18 +339,18 @@ __global__ void __launch_bounds__(Y3_TPB) k3_ypass_t(const Rep3* __restrict__ re
     batched_copy<W * FY, Y3_TPB, 8>(
         [&](int idx) {
             const int y = idx / W, j = idx - y * W;
-            return (y < yin && j < w) ? base[(size_t)y * fxp + j] : make_float2(0.f, 0.f);
+            return (y < yin && j < w) ? gload(base, (size_t)y * fxp + j) : make_float2(0.f, 0.f);
         },
         [&](int idx, float2 v) {
             const int y = idx / W, j = idx - y * W;
             A[j * LS + y] = v;
         });
     __syncthreads();
-    fft_ip<FY, W, Y3_TPB, SIGN, LS>(A, cv.twy);
+    fft_ip<FY, W, Y3_TPB, SIGN, LS>(A, gview(cv.twy));
     const int yout = SIGN < 0 ? FY : ny;
     for (int idx = threadIdx.x; idx < W * yout; idx += Y3_TPB) {
         const int y = idx / W, j = idx - y * W;
-        if (j < w) base[(size_t)y * fxp + j] = A[j * LS + y];
+        if (j < w) gstore(base, (size_t)y * fxp + j, A[j * LS + y]);
     }
 }
 
@@ -409,7 +410,7 @@ __global__ void __launch_bounds__(Z3_TPB) k3_zconv(const Rep3* __restrict__ reps
     for (int idx = threadIdx.x; idx < lines * fz; idx += Z3_TPB) {
         const int zz = idx / lines, l = idx - zz * lines;
         const int c = l / w, j = l - c * w;
-        A[(size_t)l * fz + zz] = zz < nz ? cv.W[widx(cv, nz, c, zz, ky, kx0 + j)] : make_float2(0.f, 0.f);
+        A[(size_t)l * fz + zz] = zz < nz ? gload(cv.W, widx(cv, nz, c, zz, ky, kx0 + j)) : make_float2(0.f, 0.f);
     }
     __syncthreads();
     float2* R = lds_fft<-1>(A, B, lines, cv.pz, cv.twz);
@@ -424,7 +425,7 @@ __global__ void __launch_bounds__(Z3_TPB) k3_zconv(const Rep3* __restrict__ reps
     for (int idx = threadIdx.x; idx < lines * nz; idx += Z3_TPB) {
         const int zz = idx / lines, l = idx - zz * lines;
         const int c = l / w, j = l - c * w;
-        cv.W[widx(cv, nz, c, zz, ky, kx0 + j)] = O[(size_t)l * fz + zz];
+        gstore(cv.W, widx(cv, nz, c, zz, ky, kx0 + j), O[(size_t)l * fz + zz]);
     }
 }
 
@@ -449,14 +450,14 @@ __global__ void __launch_bounds__(Z3_TPB) k3_zconv_t(const Rep3* __restrict__ re
         [&](int idx) {
             const int zz = idx / lines, l = idx - zz * lines;
             const int c = l / W, j = l - c * W;
-            return (zz < nz && j < w) ? cv.W[widx(cv, nz, c, zz, ky, kx0 + j)] : make_float2(0.f, 0.f);
+            return (zz < nz && j < w) ? gload(cv.W, widx(cv, nz, c, zz, ky, kx0 + j)) : make_float2(0.f, 0.f);
         },
         [&](int idx, float2 v) {
             const int zz = idx / lines, l = idx - zz * lines;
             A[l * LS + zz] = v;
         });
     __syncthreads();
-    fft_ip<FZ, lines, Z3_TPB, -1, LS>(A, cv.twz);
+    fft_ip<FZ, lines, Z3_TPB, -1, LS>(A, gview(cv.twz));
     // kz fastest: the tensor words and the LDS lines are both read in runs
     for (int idx = threadIdx.x; idx < W * FZ; idx += Z3_TPB) {
         const int j = idx / FZ, kz = idx % FZ;
@@ -465,11 +466,11 @@ __global__ void __launch_bounds__(Z3_TPB) k3_zconv_t(const Rep3* __restrict__ re
                          A[(2 * W + j) * LS + kz]);
     }
     __syncthreads();
-    fft_ip<FZ, lines, Z3_TPB, 1, LS>(A, cv.twz);
+    fft_ip<FZ, lines, Z3_TPB, 1, LS>(A, gview(cv.twz));
     for (int idx = threadIdx.x; idx < lines * nz; idx += Z3_TPB) {
         const int zz = idx / lines, l = idx - zz * lines;
         const int c = l / W, j = l - c * W;
-        if (j < w) cv.W[widx(cv, nz, c, zz, ky, kx0 + j)] = A[l * LS + zz];
+        if (j < w) gstore(cv.W, widx(cv, nz, c, zz, ky, kx0 + j), A[l * LS + zz]);
     }
 }
 
@@ -497,8 +498,8 @@ __global__ void __launch_bounds__(NT) k3_zconv_o(const Rep3* __restrict__ reps, 
     float2* twh = lds + lines * LS;                      // exp(-2 pi i k / NZ), k < NZ
     float2* wz = twh + NZ;                               // w^z, z < NZ
     for (int i = threadIdx.x; i < NZ; i += NT) {
-        twh[i] = cv.twz[2 * i];
-        wz[i] = cv.twz[i];
+        twh[i] = gload(cv.twz, 2 * i);
+        wz[i] = gload(cv.twz, i);
     }
     // element idx = z lines + l: consecutive threads walk the kx of one plane
     float2 x[PER], y[PER];
@@ -507,7 +508,7 @@ __global__ void __launch_bounds__(NT) k3_zconv_o(const Rep3* __restrict__ reps, 
         const int idx = (int)threadIdx.x + p * NT;
         const int zz = idx / lines, l = idx - zz * lines;
         const int c = l / W, j = l - c * W;
-        x[p] = ((TOT % NT == 0 || idx < TOT) && j < w) ? cv.W[widx(cv, NZ, c, zz, ky, kx0 + j)] : make_float2(0.f, 0.f);
+        x[p] = ((TOT % NT == 0 || idx < TOT) && j < w) ? gload(cv.W, widx(cv, NZ, c, zz, ky, kx0 + j)) : make_float2(0.f, 0.f);
     }
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
@@ -562,7 +563,7 @@ __global__ void __launch_bounds__(NT) k3_zconv_o(const Rep3* __restrict__ reps, 
         const int c = l / W, j = l - c * W;
         if ((TOT % NT == 0 || idx < TOT) && j < w) {
             const float2 o = cmulc(A[l * LS + zz], wz[zz]);
-            cv.W[widx(cv, NZ, c, zz, ky, kx0 + j)] = make_float2(y[p].x + o.x, y[p].y + o.y);
+            gstore(cv.W, widx(cv, NZ, c, zz, ky, kx0 + j), make_float2(y[p].x + o.x, y[p].y + o.y));
         }
     }
 }

@@ -36,7 +36,6 @@ __host__ __device__ constexpr int col_tpb(int ny) { return ny > 0 && ny <= 256 ?
 // of tiles: 28 against 20 us at 256x256x8); at 1024 threads a budget of 64 spills.
 __host__ __device__ constexpr int row_waves(int) { return 4; }
 // exchange class tables up to this size are staged in LDS by the fused kernels
-static constexpr int EX_LDS_CLS = 64, EX_LDS_ENT = 512;
 
 // Workgroups on the same XCD (the dispatcher deals bid % 8 to XCD bid % 8) get
 // contiguous tiles: XCD r takes the r-th run of nb/8 (+1 for the first nb % 8)
@@ -70,11 +69,11 @@ __device__ __forceinline__ void rows_pack_spectrum(const float2* __restrict__ sp
         const int kk = mirror ? nx - k : k;
         const int q0 = 2 * l, q1 = q0 + 1;
         const int c0 = q0 / ty, r0 = q0 - c0 * ty;
-        float2 A = spec[((size_t)(c0 * nz + z) * nxh + kk) * ny + y0 + r0];
+        float2 A = gload(spec, ((size_t)(c0 * nz + z) * nxh + kk) * ny + y0 + r0);
         float2 B = make_float2(0.f, 0.f);
         if (q1 < 3 * ty) {
             const int c1 = q1 / ty, r1 = q1 - c1 * ty;
-            B = spec[((size_t)(c1 * nz + z) * nxh + kk) * ny + y0 + r1];
+            B = gload(spec, ((size_t)(c1 * nz + z) * nxh + kk) * ny + y0 + r1);
         }
         return make_float4(A.x, A.y, B.x, B.y);
     };
@@ -105,11 +104,11 @@ __device__ __forceinline__ void rows_unpack_spectrum(const float2* __restrict__ 
         const float2 Zm = lds[(size_t)l * nx + (k ? nx - k : 0)];
         const int q0 = 2 * l, q1 = q0 + 1;
         const int c0 = q0 / ty, r0 = q0 - c0 * ty;
-        spec[((size_t)(c0 * nz + z) * nxh + k) * ny + y0 + r0] = make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y));
+        gstore(spec, ((size_t)(c0 * nz + z) * nxh + k) * ny + y0 + r0, make_float2(0.5f * (Zk.x + Zm.x), 0.5f * (Zk.y - Zm.y)));
         if (q1 < 3 * ty) {
             const int c1 = q1 / ty, r1 = q1 - c1 * ty;
-            spec[((size_t)(c1 * nz + z) * nxh + k) * ny + y0 + r1] =
-                make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x));
+            gstore(spec, ((size_t)(c1 * nz + z) * nxh + k) * ny + y0 + r1,
+                   make_float2(0.5f * (Zk.y + Zm.y), 0.5f * (Zm.x - Zk.x)));
         }
     }
 }
@@ -166,12 +165,12 @@ __device__ __forceinline__ void rows_pack_split(const DipoleDev& dp, const float
         [&](int idx) {
             const int item = idx / CH, h = idx - item * CH;
             const int c = item / NXH, kx = item - c * NXH;
-            return src[((((size_t)(c * nz + t.z) * NXH + kx) * ny + t.yb * TY) >> 1) + h];
+            return gload(src, ((((size_t)(c * nz + t.z) * NXH + kx) * ny + t.yb * TY) >> 1) + h);
         },
         [&](int idx, float4 v) { stg[idx] = v; });
     float2 tw[TY];
 #pragma unroll
-    for (int k = 1; k < TY; ++k) tw[k] = dp.twy[t.yb * k];
+    for (int k = 1; k < TY; ++k) tw[k] = gload(dp.twy, t.yb * k);
     __syncthreads();
     for (int idx = threadIdx.x; idx < ITEMS; idx += NT) {
         const int c = idx / NXH, kx = idx - c * NXH;
@@ -207,7 +206,7 @@ __device__ __forceinline__ void rows_unpack_split(const DipoleDev& dp, const flo
     float4* stg = split_stage<NX, TY>(const_cast<float2*>(lds));
     float2 tw[TY];
 #pragma unroll
-    for (int k = 1; k < TY; ++k) tw[k] = dp.twy[t.yb * k];
+    for (int k = 1; k < TY; ++k) tw[k] = gload(dp.twy, t.yb * k);
     for (int idx = threadIdx.x; idx < ITEMS; idx += NT) {
         const int c = idx / NXH, kx = idx - c * NXH;
         float2 v[TY];
@@ -229,7 +228,7 @@ __device__ __forceinline__ void rows_unpack_split(const DipoleDev& dp, const flo
     for (int idx = threadIdx.x; idx < ITEMS * CH; idx += NT) {
         const int item = idx / CH, h = idx - item * CH;
         const int c = item / NXH, kx = item - c * NXH;
-        dst[((((size_t)(c * nz + t.z) * NXH + kx) * ny + t.yb * TY) >> 1) + h] = stg[idx];
+        gstore(dst, ((((size_t)(c * nz + t.z) * NXH + kx) * ny + t.yb * TY) >> 1) + h, stg[idx]);
     }
 }
 
@@ -260,7 +259,7 @@ __device__ __forceinline__ void rows_out(const DipoleDev& dp, const float2* __re
 __device__ __forceinline__ ExchangeDev stage_exchange(const ExchangeDev& ex, int* s_start, ExEntry* s_ent)
 {
     ExchangeDev e = ex;
-    if (ex.mode == 1 && ex.ncls + 1 <= EX_LDS_CLS && ex.nent <= EX_LDS_ENT) {
+    if (ex.mode == 1) {     // always fits (msd_exchange::fused_dev)
         for (int i = threadIdx.x; i <= ex.ncls; i += blockDim.x) s_start[i] = ex.cls_start[i];
         for (int i = threadIdx.x; i < ex.nent; i += blockDim.x) s_ent[i] = ex.ent[i];
         e.cls_start = s_start;
@@ -274,7 +273,7 @@ template <int NX>
 __device__ __forceinline__ const float2* row_twiddles(const DipoleDev& dp, float2* lds_tw)
 {
     if constexpr (NX > 0) {
-        for (int i = threadIdx.x; i < NX; i += blockDim.x) lds_tw[i] = dp.twx[i];
+        for (int i = threadIdx.x; i < NX; i += blockDim.x) lds_tw[i] = gload(dp.twx, i);
         return lds_tw;
     } else {
         return dp.twx;
@@ -376,17 +375,17 @@ __device__ __forceinline__ void cols_body(const DipoleDev& dp, int wg)
         constexpr int TWI = (NY + COL_TPB - 1) / COL_TPB;
         constexpr int MI = (lines * NY + COL_TPB - 1) / COL_TPB;
         constexpr int WI = (3 * NZ * NY + COL_TPB - 1) / COL_TPB;
-        auto ldm = [&](int idx) { const int l = idx / NY, y = idx - l * NY; return dp.Mh[((size_t)l * nxh + kx) * NY + y]; };
+        auto ldm = [&](int idx) { const int l = idx / NY, y = idx - l * NY; return gload(dp.Mh, ((size_t)l * nxh + kx) * NY + y); };
         auto ldw = [&](int idx) {
             const int l = idx / NY, y = idx - l * NY;     // l = b*NZ + o
             const int b = l / NZ, o = l - b * NZ;
             const int comp = b == 0 ? c0 : (b == 1 ? c1 : c2);
-            return dp.Tw[((size_t)(comp * NZ + o) * nxh + kx) * NY + y];
+            return gload(dp.Tw, ((size_t)(comp * NZ + o) * nxh + kx) * NY + y);
         };
         float2 twv[TWI];
 #pragma unroll
         for (int k = 0; k < TWI; ++k)
-            if (NY % COL_TPB == 0 || (int)threadIdx.x + k * COL_TPB < NY) twv[k] = dp.twy[threadIdx.x + k * COL_TPB];
+            if (NY % COL_TPB == 0 || (int)threadIdx.x + k * COL_TPB < NY) twv[k] = gload(dp.twy, threadIdx.x + k * COL_TPB);
         if constexpr (MI + WI <= 32) {
             float2 mv[MI];
             float wv[WI];
@@ -421,7 +420,7 @@ __device__ __forceinline__ void cols_body(const DipoleDev& dp, int wg)
     } else {
         for (int idx = threadIdx.x; idx < lines * ny; idx += COL_TPB) {
             const int l = idx / ny, y = idx - l * ny;
-            A[idx] = dp.Mh[((size_t)l * nxh + kx) * ny + y];
+            A[idx] = gload(dp.Mh, ((size_t)l * nxh + kx) * ny + y);
         }
     }
     __syncthreads();
@@ -441,9 +440,9 @@ __device__ __forceinline__ void cols_body(const DipoleDev& dp, int wg)
         } else {
 #pragma unroll
             for (int o = 0; o < NZ; ++o) {
-                w[0][o] = dp.Tw[((size_t)(c0 * NZ + o) * nxh + kx) * ny + ky];
-                w[1][o] = dp.Tw[((size_t)(c1 * NZ + o) * nxh + kx) * ny + ky];
-                w[2][o] = dp.Tw[((size_t)(c2 * NZ + o) * nxh + kx) * ny + ky];
+                w[0][o] = gload(dp.Tw, ((size_t)(c0 * NZ + o) * nxh + kx) * ny + ky);
+                w[1][o] = gload(dp.Tw, ((size_t)(c1 * NZ + o) * nxh + kx) * ny + ky);
+                w[2][o] = gload(dp.Tw, ((size_t)(c2 * NZ + o) * nxh + kx) * ny + ky);
             }
         }
         // source layer outer, target layer inner: one source's three components
@@ -490,7 +489,7 @@ __device__ __forceinline__ void cols_body(const DipoleDev& dp, int wg)
     else O = lds_fft<1>(Rs, Os, NZ, dp.py, tw);
     for (int idx = threadIdx.x; idx < NZ * ny; idx += COL_TPB) {
         const int l = idx / ny, y = idx - l * ny;
-        dp.Hh[((size_t)(oa * NZ + l) * nxh + kx) * ny + y] = O[idx];
+        gstore(dp.Hh, ((size_t)(oa * NZ + l) * nxh + kx) * ny + y, O[idx]);
     }
 }
 
@@ -606,15 +605,15 @@ __device__ __forceinline__ void split_cols_body(const DipoleDev& dp, int wg)
             const int idx = (int)threadIdx.x + k * NT;
             if (MN % NT == 0 || idx < MN) {
                 const int l = idx / L, b = idx - l * L;
-                mv[k] = src[l * lst + (size_t)b * R];
+                mv[k] = gload(src, l * lst + (size_t)b * R);
             }
         }
 #pragma unroll
         for (int k = 0; k < WI; ++k) {
             const int idx = (int)threadIdx.x + k * NT;
-            if (WN % NT == 0 || idx < WN) wv[k] = wsrc[idx];
+            if (WN % NT == 0 || idx < WN) wv[k] = gload(wsrc, idx);
         }
-        if ((int)threadIdx.x < L) tv = dp.twy[threadIdx.x * R];
+        if ((int)threadIdx.x < L) tv = gload(dp.twy, threadIdx.x * R);
 #pragma unroll
         for (int k = 0; k < MI; ++k) {
             const int idx = (int)threadIdx.x + k * NT;
@@ -708,7 +707,7 @@ __device__ __forceinline__ void split_cols_body(const DipoleDev& dp, int wg)
     float2* __restrict__ dst = dp.Hh + (size_t)kx * ny + k1;
     for (int idx = threadIdx.x; idx < LN * L; idx += NT) {
         const int l = idx / L, b = idx - l * L;
-        dst[l * lst + (size_t)b * R] = A[b * PS + l];
+        gstore(dst, l * lst + (size_t)b * R, A[b * PS + l]);
     }
 }
 
@@ -750,15 +749,15 @@ __device__ __forceinline__ void pair_cols_body(const DipoleDev& dp, int wg)
     const float2* tw = dp.twy;
     if constexpr (NY > 0) {
         float2* twl = A + (size_t)lines * NY;   // in place: the twiddles follow the lines
-        for (int k = threadIdx.x; k < NY; k += COL_TPB) twl[k] = dp.twy[k];
+        for (int k = threadIdx.x; k < NY; k += COL_TPB) twl[k] = gload(dp.twy, k);
         batched_copy<lines * NY, COL_TPB, 16>(
-            [&](int idx) { const int l = idx / NY, y = idx - l * NY; return dp.Mh[((size_t)l * nxh + kx) * NY + y]; },
+            [&](int idx) { const int l = idx / NY, y = idx - l * NY; return gload(dp.Mh, ((size_t)l * nxh + kx) * NY + y); },
             [&](int idx, float2 v) { A[idx] = v; });
         tw = twl;
     } else {
         for (int idx = threadIdx.x; idx < lines * ny; idx += COL_TPB) {
             const int l = idx / ny, y = idx - l * ny;
-            A[idx] = dp.Mh[((size_t)l * nxh + kx) * ny + y];
+            A[idx] = gload(dp.Mh, ((size_t)l * nxh + kx) * ny + y);
         }
     }
     __syncthreads();
@@ -811,7 +810,7 @@ __device__ __forceinline__ void pair_cols_body(const DipoleDev& dp, int wg)
     else O = lds_fft<1>(Rs, Os, NZ, dp.py, tw);
     for (int idx = threadIdx.x; idx < NZ * ny; idx += COL_TPB) {
         const int l = idx / ny, y = idx - l * ny;
-        dp.Hh[((size_t)(oa * NZ + l) * nxh + kx) * ny + y] = O[idx];
+        gstore(dp.Hh, ((size_t)(oa * NZ + l) * nxh + kx) * ny + y, O[idx]);
     }
 }
 

@@ -40,6 +40,48 @@ __device__ __forceinline__ float3 thermal_normals(uint64_t seed, uint64_t draw, 
     return make_float3(r0 * __builtin_amdgcn_cosf(a0), r0 * __builtin_amdgcn_sinf(a0), r1 * __builtin_amdgcn_cosf(a1));
 }
 
+// ------------------------------------------------------ address spaces
+// Address-space views.  A pointer that a kernel reads out of a device table (an
+// ensemble's argument records, a batch's replica descriptors, an LDS-staged class
+// table) reaches the compiler as a generic pointer, and an access through it is a
+// flat instruction: flat loads count against both the vector-memory and the LDS
+// wait counters, so every LDS wait then also waits for the global loads in flight.
+// Accesses through gview / lview are global_* and ds_* instructions.
+#define MSD_GLOBAL __attribute__((address_space(1)))
+#define MSD_LDS __attribute__((address_space(3)))
+template <typename T> __device__ __forceinline__ const MSD_GLOBAL T* gview(const T* p) { return (const MSD_GLOBAL T*)p; }
+template <typename T> __device__ __forceinline__ MSD_GLOBAL T* gview(T* p) { return (MSD_GLOBAL T*)p; }
+template <typename T> __device__ __forceinline__ const MSD_LDS T* lview(const T* p) { return (const MSD_LDS T*)p; }
+
+// Element types move through these views as builtin scalars or clang vectors (the
+// HIP vector types' members take generic references only).
+template <typename T> struct RawOf {
+    using type = T;
+    static __device__ __forceinline__ T get(type v) { return v; }
+    static __device__ __forceinline__ type put(T v) { return v; }
+};
+template <typename U, unsigned N> struct RawOf<HIP_vector_type<U, N>> {
+    using T = HIP_vector_type<U, N>;
+    using type = typename HIP_vector_base<U, N>::Native_vec_;
+    static __device__ __forceinline__ T get(type v) { T t; __builtin_memcpy(&t, &v, sizeof t); return t; }
+    static __device__ __forceinline__ type put(T v) { type r; __builtin_memcpy(&r, &v, sizeof r); return r; }
+};
+
+// element i of a global array, through the global view
+template <typename T>
+__device__ __forceinline__ T gload(const T* p, size_t i)
+{
+    using R = typename RawOf<T>::type;
+    return RawOf<T>::get(reinterpret_cast<const MSD_GLOBAL R*>(gview(p))[i]);
+}
+template <typename T>
+__device__ __forceinline__ void gstore(T* p, size_t i, T v)
+{
+    using R = typename RawOf<T>::type;
+    reinterpret_cast<MSD_GLOBAL R*>(gview(p))[i] = RawOf<T>::put(v);
+}
+
+
 // ---------------------------------------------------------------- complex
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -249,8 +291,15 @@ __device__ __forceinline__ int lds_swz(int a)
     else return a;
 }
 
-template <int L, int COUNT, int NT, int SIGN, int NS, int R, bool SWIN, bool SWOUT, int LS>
-__device__ __forceinline__ void stage_ip(float2* __restrict__ buf, const float2* __restrict__ tw)
+// twiddle i from LDS (a generic pointer into it) or from global memory (its view)
+__device__ __forceinline__ float2 twv(const float2* t, int i) { return t[i]; }
+__device__ __forceinline__ float2 twv(const MSD_GLOBAL float2* t, int i)
+{
+    return RawOf<float2>::get(reinterpret_cast<const MSD_GLOBAL RawOf<float2>::type*>(t)[i]);
+}
+
+template <int L, int COUNT, int NT, int SIGN, int NS, int R, bool SWIN, bool SWOUT, int LS, typename TWP>
+__device__ __forceinline__ void stage_ip(float2* __restrict__ buf, TWP tw)
 {
     constexpr int nb = L / R;
     constexpr int total = COUNT * nb;
@@ -279,7 +328,7 @@ __device__ __forceinline__ void stage_ip(float2* __restrict__ buf, const float2*
             if (NS > 1) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    const float2 w = tw[k * r * tstep];
+                    const float2 w = twv(tw, k * r * tstep);
                     v[it][r] = SIGN < 0 ? cmul(v[it][r], w) : cmulc(v[it][r], w);
                 }
             }
@@ -299,7 +348,8 @@ __device__ __forceinline__ void stage_ip(float2* __restrict__ buf, const float2*
 
 template <int L, int COUNT, int NT, int SIGN, int NS, int LS>
 struct FFTip {
-    static __device__ __forceinline__ void run(float2* a, const float2* tw)
+    template <typename TWP>
+    static __device__ __forceinline__ void run(float2* a, TWP tw)
     {
         constexpr int R = next_radix(L / NS);
         stage_ip<L, COUNT, NT, SIGN, NS, R, NS != 1, NS * R != L, LS>(a, tw);
@@ -308,14 +358,15 @@ struct FFTip {
 };
 template <int L, int COUNT, int NT, int SIGN, int LS>
 struct FFTip<L, COUNT, NT, SIGN, L, LS> {
-    static __device__ __forceinline__ void run(float2*, const float2*) {}
+    template <typename TWP>
+    static __device__ __forceinline__ void run(float2*, TWP) {}
 };
 
 // COUNT lines of length L transformed in place in LDS, line s at a + s*LS (a
 // stride LS > L staggers the lines' banks for transposing copies); caller
-// synchronised before
-template <int L, int COUNT, int NT, int SIGN, int LS = L>
-__device__ __forceinline__ void fft_ip(float2* a, const float2* twl)
+// synchronised before.  twl: twiddles in LDS, or the global view of a table.
+template <int L, int COUNT, int NT, int SIGN, int LS = L, typename TWP = const float2*>
+__device__ __forceinline__ void fft_ip(float2* a, TWP twl)
 {
     FFTip<L, COUNT, NT, SIGN, 1, LS>::run(a, twl);
 }
@@ -353,12 +404,16 @@ __device__ __forceinline__ void batched_copy(LD ld, ST st)
 template <typename T>
 __device__ __forceinline__ T ldg(const T* __restrict__ base, uint32_t i)
 {
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (uint64_t)(i * (uint32_t)sizeof(T)));
+    using R = typename RawOf<T>::type;
+    return RawOf<T>::get(*reinterpret_cast<const MSD_GLOBAL R*>(reinterpret_cast<const MSD_GLOBAL char*>(gview(base)) +
+                                                                (uint64_t)(i * (uint32_t)sizeof(T))));
 }
 template <typename T>
 __device__ __forceinline__ void stg(T* __restrict__ base, uint32_t i, T v)
 {
-    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (uint64_t)(i * (uint32_t)sizeof(T))) = v;
+    using R = typename RawOf<T>::type;
+    *reinterpret_cast<MSD_GLOBAL R*>(reinterpret_cast<MSD_GLOBAL char*>(gview(base)) + (uint64_t)(i * (uint32_t)sizeof(T))) =
+        RawOf<T>::put(v);
 }
 
 // Hardware reciprocal and reciprocal square root (v_rcp_f32 / v_rsq_f32, 1 ulp);
@@ -502,14 +557,17 @@ __device__ __forceinline__ void exchange_gather(const ExchangeDev& ex, uint32_t 
     g.e0 = 0;
     g.cnt = 0;
     if (ex.mode != 1) return;
+    // mode 1 is the fused kernels' LDS-staged class table
+    const MSD_LDS int* cs = lview(ex.cls_start);
+    const MSD_LDS ExEntry* en = lview(ex.ent);
     const int c = ex.cls ? (int)ldg(ex.cls, i) : 0;
-    g.e0 = ex.cls_start[c];
-    g.cnt = ex.cls_start[c + 1] - g.e0;
+    g.e0 = cs[c];
+    g.cnt = cs[c + 1] - g.e0;
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
         g.mf[u] = 0.0f;
         if (u < g.cnt) {
-            const uint32_t f = i + (uint32_t)ex.ent[g.e0 + u].dl;
+            const uint32_t f = i + (uint32_t)en[g.e0 + u].dl;
             g.mf[u] = ldg(m, f);
             g.vx[u] = ldg(s, f); g.vy[u] = ldg(s + n, f); g.vz[u] = ldg(s + 2 * (size_t)n, f);
         }
@@ -524,19 +582,21 @@ __device__ __forceinline__ float3 exchange_finish(const ExchangeDev& ex, uint32_
     const float* __restrict__ sz = s + 2 * (size_t)n;
     float3 h = make_float3(0.f, 0.f, 0.f);
     if (ex.mode == 1) {
+        const MSD_LDS ExEntry* en = lview(ex.ent);
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             if (g.mf[u] != 0.0f) {
-                const float w = ex.ent[g.e0 + u].J * rcpf(g.mf[u]);
+                const float w = en[g.e0 + u].J * rcpf(g.mf[u]);
                 h.x += g.vx[u] * w; h.y += g.vy[u] * w; h.z += g.vz[u] * w;
             }
         }
         for (int e = g.e0 + 8; e < g.e0 + g.cnt; ++e) {
-            const ExEntry E = ex.ent[e];
-            const uint32_t f = i + (uint32_t)E.dl;
+            const int dl = en[e].dl;
+            const float J = en[e].J;
+            const uint32_t f = i + (uint32_t)dl;
             const float mfe = ldg(m, f);
             if (mfe != 0.0f) {
-                const float w = E.J * rcpf(mfe);
+                const float w = J * rcpf(mfe);
                 h.x += ldg(sx, f) * w; h.y += ldg(sy, f) * w; h.z += ldg(sz, f) * w;
             }
         }

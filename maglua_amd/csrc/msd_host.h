@@ -215,7 +215,8 @@ struct msd_exchange {
     ExchangeDev fused_dev() const
     {
         ExchangeDev e = csr_dev();
-        if (lut_ok && !paths.empty()) {
+        // the fused kernels stage class tables in LDS: larger ones take the CSR form
+        if (lut_ok && !paths.empty() && ncls + 1 <= EX_LDS_CLS && nent <= EX_LDS_ENT) {
             e.mode = 1; e.cls = cls; e.cls_start = cls_start; e.ent = ent;
             e.ncls = ncls; e.nent = nent;
         }

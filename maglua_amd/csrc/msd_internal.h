@@ -27,8 +27,12 @@ struct ExEntry {
     int dl;
     float J;
 };
+// Class tables of at most this many classes (plus one) and entries are what the
+// fused step kernels hold in LDS; the fused form uses a class table only then
+// (msd_exchange::fused_dev), so in those kernels mode 1 always reads LDS.
+constexpr int EX_LDS_CLS = 64, EX_LDS_ENT = 512;
 struct ExchangeDev {
-    int mode;               // 0 none, 1 class table, 2 CSR by target
+    int mode;               // 0 none, 1 class table (fused kernels: staged in LDS), 2 CSR by target
     int normalize;
     const uint8_t* cls;     // [n] or nullptr (all class 0)
     const int* cls_start;   // [ncls+1]
