@@ -573,23 +573,24 @@ __global__ void __launch_bounds__(NT) k3_zconv_o(const Rep3* __restrict__ reps, 
 // memory phases overlap each other's transforms; a thread holds 6 of each line set).
 // The default width comes first; the step autotuner times the others.
 static constexpr int zopen_width(int nz) { return nz >= 128 ? 4 : 16; }
-static constexpr int zopen_threads(int nz) { return nz == 128 ? 256 : Z3_TPB; }
+// 8 columns at nz >= 128 take twice the threads (the same 6 elements of each line
+// set per thread): runs of 64 bytes per z plane instead of 32 at the same registers
+static constexpr int zopen_threads(int nz, int w) { return (nz == 128 ? 256 : Z3_TPB) * (nz >= 128 && w >= 8 ? 2 : 1); }
 template <int NZ>
 static const void* zopen_kernel(int w)
 {
-    constexpr int NT = zopen_threads(NZ);
     if constexpr (NZ >= 128) {
         switch (w) {
-        case 4: return (const void*)k3_zconv_o<NZ, 4, NT>;
-        case 2: return (const void*)k3_zconv_o<NZ, 2, NT>;
-        case 8: return (const void*)k3_zconv_o<NZ, 8, NT>;
+        case 4: return (const void*)k3_zconv_o<NZ, 4, zopen_threads(NZ, 4)>;
+        case 2: return (const void*)k3_zconv_o<NZ, 2, zopen_threads(NZ, 2)>;
+        case 8: return (const void*)k3_zconv_o<NZ, 8, zopen_threads(NZ, 8)>;
         default: return nullptr;
         }
     }
     else {
         switch (w) {
-        case 16: return (const void*)k3_zconv_o<NZ, 16, NT>;
-        case 8: return (const void*)k3_zconv_o<NZ, 8, NT>;
+        case 16: return (const void*)k3_zconv_o<NZ, 16, zopen_threads(NZ, 16)>;
+        case 8: return (const void*)k3_zconv_o<NZ, 8, zopen_threads(NZ, 8)>;
         default: return nullptr;
         }
     }
@@ -723,7 +724,7 @@ hipError_t launch3_zconv(const Rep3* reps, const int4* items, int nitems, size_t
     if (nitems <= 0) return hipSuccess;
     const void* f = zt ? zconv_template(zt, wz) : (const void*)k3_zconv;
     if (!f) return hipErrorInvalidValue;
-    const int nt = (zt & 1) ? zopen_threads((zt - 1) / 2) : Z3_TPB;   // open split: zt = 2 nz + 1
+    const int nt = (zt & 1) ? zopen_threads((zt - 1) / 2, wz) : Z3_TPB;   // open split: zt = 2 nz + 1
     void* args[] = {(void*)&reps, (void*)&items};
     return hipLaunchKernel(f, dim3(nitems), dim3(nt), args, lds, st);
 }
