@@ -662,6 +662,30 @@ def test_changing_temperature_and_dt_reuses_the_step_graph():
     assert np.array_equal(runs[0][0], runs[1][0]) and runs[0][1] == runs[1][1]
 
 
+def test_span_graphs_take_new_values_and_match_direct_launches():
+    """Runs of 16 steps and more replay a 16-step graph: with the temperature and the
+    time step changed between runs, and run lengths that leave remainders for the
+    one-step graph, the trajectory equals direct launches bit for bit and each graph
+    is captured once."""
+    spec = W.FilmSpec(64, 64, 2, T=0.3, dt=0.004)
+    runs = []
+    for graph in (True, False):
+        g = W.build(M, spec)
+        g["ss"].randomize(5)
+        heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"], graph=graph)
+        for k, n in enumerate((16, 37, 5, 48, 16)):
+            g["temp"].setTemperature(0.3 + 0.2 * k)
+            if k == 2:
+                g["ss"].setTimeStep(0.005)
+            heun.step(n)
+        M.synchronize()
+        runs.append((flat(g["ss"].spins()), g["ss"].time(), g["temp"].draw(), heun.graphBuilds()))
+    print(f"span graphs: builds {runs[0][3]}, draws {runs[0][2]}", flush=True)
+    assert runs[0][3] == 2                    # the 16-step graph and the one-step graph
+    assert runs[0][2] == runs[1][2] == 16 + 37 + 5 + 48 + 16
+    assert np.array_equal(runs[0][0], runs[1][0]) and runs[0][1] == runs[1][1]
+
+
 @pytest.mark.parametrize("kind,T,dipole", [("RK1", 0.0, True), ("RK2", 0.0, True), ("RK4", 0.0, True),
                                            ("BS3", 0.0, True), ("RK6", 0.0, False), ("RK4_38", 0.3, True),
                                            ("K3", 0.5, False)])
