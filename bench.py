@@ -332,9 +332,10 @@ def conv3_pass_bytes(dims, open_, comp, site_axes=False):
     (every compulsory array once per pass; FFT intermediates inside a pass free).
     The working spectrum W holds the nz stored planes (z >= nz of an open transform
     are zero and never stored), fxh = fx/2+1 words per row; the tensor one FP32 word
-    per component per k with kz <= fz/2 stored (z parity of the lattice sums).
+    per component per k with kz <= fz/2 and ky <= fy/2 stored (z and y parity of the
+    lattice sums).
       y forward: read the ny row-pass rows, write fy rows; inverse y the reverse;
-      z pass: read and write 3 nz fy fxh words, read 6 (fz/2+1) fy fxh tensor words;
+      z pass: read and write 3 nz fy fxh words, read 6 (fz/2+1) (fy/2+1) fxh tensor words;
       row passes: the site arrays of the predictor / corrector (as the layered
       kernels, plus a per-site axis and K when the easy axes are random) and the
       rows of W in and out."""
@@ -348,7 +349,7 @@ def conv3_pass_bytes(dims, open_, comp, site_axes=False):
     cls = n if open_ else 0                 # exchange class byte per site (faces of an open lattice)
     axes = n * 16 if site_axes else 0
     return {"k3_ypass_fwd": word * (ny + fy), "k3_ypass_inv": word * (ny + fy),
-            "k3_zconv": 2 * word * fy + 6 * (fz // 2 + 1) * fy * fxh * 4,
+            "k3_zconv": 2 * word * fy + 6 * (fz // 2 + 1) * (fy // 2 + 1) * fxh * 4,
             "k3_rows_pred": n * 16 + n * 24 + lo + cls + axes + rows_w,
             "k3_rows_corr": n * 28 + n * 12 + lo + cls + axes + rows_w}
 

@@ -33,19 +33,22 @@ __device__ __forceinline__ size_t widx(const Conv3Dev& cv, int nz, int c, int z,
 }
 
 // The six tensor words (xx xy xz yy yz zz) of spectrum point (kx, ky, kz), stored
-// [6][fy][fxh][fzs] with kz fastest.  With fzs < fz only kz <= fz/2 is stored and
+// [6][fys][fxh][fzs] with kz fastest.  With fzs < fz only kz <= fz/2 is stored and
 // kz beyond reads fz - kz: the tensor is even in z except xz and yz, which are odd.
+// Likewise with fys < fy only ky <= fy/2 is stored, ky beyond reads fy - ky, and xy
+// and yz, odd in y, change sign.
 struct TensorWords {
     float xx, xy, xz, yy, yz, zz;
 };
 __device__ __forceinline__ TensorWords tensor_words(const Conv3Dev& cv, int ky, int kx, int kz)
 {
-    const bool mir = kz >= cv.fzs;
-    const size_t plane = (size_t)cv.fy * cv.fxh * cv.fzs;
-    const size_t t = ((size_t)ky * cv.fxh + kx) * cv.fzs + (mir ? cv.fz - kz : kz);
+    const bool mir = kz >= cv.fzs, ymir = ky >= cv.fys;
+    const size_t plane = (size_t)cv.fys * cv.fxh * cv.fzs;
+    const size_t t = ((size_t)(ymir ? cv.fy - ky : ky) * cv.fxh + kx) * cv.fzs + (mir ? cv.fz - kz : kz);
     TensorWords w{gload(cv.Tw, t), gload(cv.Tw, plane + t), gload(cv.Tw, 2 * plane + t), gload(cv.Tw, 3 * plane + t),
                   gload(cv.Tw, 4 * plane + t), gload(cv.Tw, 5 * plane + t)};
     if (mir) { w.xz = -w.xz; w.yz = -w.yz; }
+    if (ymir) { w.xy = -w.xy; w.yz = -w.yz; }
     return w;
 }
 
@@ -955,19 +958,19 @@ __global__ void k3_fft64(Lines64 m, FFTPlan p, const double2* __restrict__ tw, c
     }
 }
 
-// FP32 tensor words [6][fy][fxh][fzs] (kz fastest, kz < fzs) from the FP64
+// FP32 tensor words [6][fys][fxh][fzs] (kz fastest, kz < fzs, ky < fys) from the FP64
 // spectrum V [6][fz][fy][fxh]
-__global__ void k3_words(const double2* __restrict__ V, float* __restrict__ Tw, double mult, int fy, int fz, int fxh,
-                         int fzs)
+__global__ void k3_words(const double2* __restrict__ V, float* __restrict__ Tw, double mult, int fy, int fys, int fz,
+                         int fxh, int fzs)
 {
-    const size_t n = (size_t)6 * fy * fxh * fzs;
+    const size_t n = (size_t)6 * fys * fxh * fzs;
     for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < n; o += (size_t)gridDim.x * blockDim.x) {
         const int kz = (int)(o % fzs);
         const size_t r = o / fzs;
         const int kx = (int)(r % fxh);
         const size_t r2 = r / fxh;
-        const int ky = (int)(r2 % fy);
-        const int a = (int)(r2 / fy);
+        const int ky = (int)(r2 % fys);
+        const int a = (int)(r2 / fys);
         Tw[o] = (float)(V[(((size_t)a * fz + kz) * fy + ky) * fxh + kx].x * mult);
     }
 }
@@ -993,7 +996,7 @@ hipError_t configure_fft64(size_t bytes)
     return hipFuncSetAttribute((const void*)k3_fft64, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-hipError_t launch3_spectrum(int fx, int fy, int fz, int fzs, const double* T, double2* U1, double2* U2, float* Tw,
+hipError_t launch3_spectrum(int fx, int fy, int fz, int fys, int fzs, const double* T, double2* U1, double2* U2, float* Tw,
                             double mult, const double2* twx, const double2* twy, const double2* twz, hipStream_t st)
 {
     const int fxh = fx / 2 + 1;
@@ -1019,7 +1022,7 @@ hipError_t launch3_spectrum(int fx, int fy, int fz, int fzs, const double* T, do
         k3_dftyz<1><<<g, 256, 0, st>>>(fy, fz, fxh, U1, U2, twy);
         k3_dftyz<2><<<g, 256, 0, st>>>(fy, fz, fxh, U2, U1, twz);
     }
-    k3_words<<<g, 256, 0, st>>>(U1, Tw, mult, fy, fz, fxh, fzs);
+    k3_words<<<g, 256, 0, st>>>(U1, Tw, mult, fy, fys, fz, fxh, fzs);
     return hipGetLastError();
 }
 

@@ -210,10 +210,12 @@ static int lr3_check_even(msd_longrange3d* l)
     return MSD_OK;
 }
 
-// Whether every component has its parity in z alone, T(x, y, -z) = +T (xx xy yy zz)
-// or -T (xz yz): then the spectrum has the same parity in kz and only kz <= fz/2
-// is stored.  The built-in tensors have it (reflection symmetry of the lattice sums).
-static int lr3_z_parity(msd_longrange3d* l, bool* has)
+// Whether every component has its parity along one axis (1 = y, 2 = z) alone:
+// T(x, -y, z) = +T (xx xz yy zz) or -T (xy yz), T(x, y, -z) = +T (xx xy yy zz) or
+// -T (xz yz).  The spectrum then has the same parity in that k and only its first
+// half is stored.  The built-in tensors have both (reflection symmetry of the
+// lattice sums).
+static int lr3_parity(msd_longrange3d* l, int axis, bool* has)
 {
     const size_t plane = (size_t)l->fx * l->fy * l->fz;
     std::vector<double> t(6 * plane);
@@ -223,12 +225,15 @@ static int lr3_z_parity(msd_longrange3d* l, bool* has)
     for (double v : t) big = std::max(big, std::fabs(v));
     *has = true;
     for (int a = 0; a < 6 && *has; ++a) {
-        const double sg = (a == 2 || a == 4) ? -1.0 : 1.0;
+        const bool odd = axis == 1 ? (a == 1 || a == 4) : (a == 2 || a == 4);
+        const double sg = odd ? -1.0 : 1.0;
         for (int z = 0; z < l->fz && *has; ++z)
-            for (size_t xy = 0; xy < (size_t)l->fx * l->fy; ++xy) {
-                const double u = t[a * plane + (size_t)z * l->fx * l->fy + xy];
-                const double v = t[a * plane + (size_t)((l->fz - z) % l->fz) * l->fx * l->fy + xy];
-                if (std::fabs(u - sg * v) > 1e-12 * big) { *has = false; break; }
+            for (int y = 0; y < l->fy && *has; ++y) {
+                const int zm = axis == 2 ? (l->fz - z) % l->fz : z, ym = axis == 1 ? (l->fy - y) % l->fy : y;
+                const double* u = &t[a * plane + ((size_t)z * l->fy + y) * l->fx];
+                const double* v = &t[a * plane + ((size_t)zm * l->fy + ym) * l->fx];
+                for (int x = 0; x < l->fx; ++x)
+                    if (std::fabs(u[x] - sg * v[x]) > 1e-12 * big) { *has = false; break; }
             }
     }
     return MSD_OK;
@@ -238,12 +243,14 @@ int msd_longrange3d::prepare()
 {
     if (int rc = lr3_load(this)) return rc;
     if (!spectrum) {
-        bool zpar = true;
+        bool zpar = true, ypar = true;
         if (custom) {
             if (int rc = lr3_check_even(this)) return rc;
-            if (int rc = lr3_z_parity(this, &zpar)) return rc;
+            if (int rc = lr3_parity(this, 2, &zpar)) return rc;
+            if (int rc = lr3_parity(this, 1, &ypar)) return rc;
         }
         fzs = zpar ? fz / 2 + 1 : fz;
+        fys = ypar ? fy / 2 + 1 : fy;
         words_mult = NAN;
         spectrum = true;
     }
@@ -253,7 +260,7 @@ int msd_longrange3d::prepare()
         double2 *U1 = nullptr, *U2 = nullptr;
         if (int rc = dalloc(&U1, wn)) return rc;
         if (int rc = dalloc(&U2, wn)) { dfree(U1); return rc; }
-        hipError_t e = launch3_spectrum(fx, fy, fz, fzs, T64, U1, U2, Tw, mult, twx64, twy64, twz64, G.stream);
+        hipError_t e = launch3_spectrum(fx, fy, fz, fys, fzs, T64, U1, U2, Tw, mult, twx64, twy64, twz64, G.stream);
         if (e == hipSuccess) e = hipStreamSynchronize(G.stream);
         dfree(U1); dfree(U2);
         if (e != hipSuccess) return fail(MSD_EDEVICE, "LongRange3D spectrum: %s", hipGetErrorString(e));
@@ -271,8 +278,16 @@ void items3(int rep, const Dims& d, const Conv3Dev* cv, int ty, std::vector<int4
     if (!cv) return;
     for (int l = 0; l < 3 * d.nz; ++l)
         for (int k = 0; k < cv->fxh; k += cv->wy) ys.push_back(make_int4(rep, l, k, 0));
-    for (int ky = 0; ky < cv->fy; ++ky)
+    // rows ky and fy - ky side by side: with the tensor stored for ky <= fy/2 they
+    // read the same words, which the second then finds in its XCD's L2
+    auto zrow = [&](int ky) {
         for (int k = 0; k < cv->fxh; k += cv->wz) zs.push_back(make_int4(rep, ky, k, 0));
+    };
+    zrow(0);
+    for (int j = 1; 2 * j <= cv->fy; ++j) {
+        zrow(j);
+        if (cv->fy - j != j) zrow(cv->fy - j);
+    }
 }
 
 // Workgroups go to the 8 XCDs round-robin by index.  Reorder a y or z pass's items

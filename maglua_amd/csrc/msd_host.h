@@ -405,12 +405,12 @@ struct msd_longrange3d {
     bool loaded = false, custom = false, spectrum = false;
     double words_mult = NAN;
     double* T64 = nullptr;   // [6][fz][fy][fx]
-    float* Tw = nullptr;     // [6][fy][fxh][fzs], fzs <= fz
+    float* Tw = nullptr;     // [6][fys][fxh][fzs], fys <= fy, fzs <= fz
     float2* W = nullptr;     // working spectrum [3][nz][fy][fxp]
     float2 *twx = nullptr, *twy = nullptr, *twz = nullptr;
     double2 *twx64 = nullptr, *twy64 = nullptr, *twz64 = nullptr;
     FFTPlan px{}, py{}, pz{};
-    int ty = 1, wy = 1, wz = 1, zt = 0, yt = 0, rt = 0, fzs = 0, fxp = 0;
+    int ty = 1, wy = 1, wz = 1, zt = 0, yt = 0, rt = 0, fzs = 0, fys = 0, fxp = 0;
     size_t lds_rows = 0, lds_y = 0, lds_z = 0;
     // single-replica descriptor and work items for the operator API
     Rep3* orep = nullptr;
@@ -428,7 +428,7 @@ struct msd_longrange3d {
     {
         Conv3Dev c{};
         c.fx = fx; c.fy = fy; c.fz = fz; c.fxh = fxh; c.fxp = fxp;
-        c.ty = ty; c.wy = wy; c.wz = wz; c.zt = zt; c.yt = yt; c.rt = rt; c.fzs = fzs;
+        c.ty = ty; c.wy = wy; c.wz = wz; c.zt = zt; c.yt = yt; c.rt = rt; c.fzs = fzs; c.fys = fys;
         c.px = px; c.py = py; c.pz = pz;
         c.twx = twx; c.twy = twy; c.twz = twz;
         c.W = W; c.Tw = Tw;

@@ -224,12 +224,13 @@ struct Conv3Dev {
     int yt;                 // y length of the compile-time y pass (0: the runtime-length pass)
     int rt;                 // x length of the compile-time row pass (0: the runtime-length pass)
     int fzs;                // kz planes of the tensor words (fz/2 + 1 when the tensor has z parity, else fz)
+    int fys;                // ky rows of the tensor words (fy/2 + 1 when the tensor has y parity, else fy)
     FFTPlan px, py, pz;
     const float2* twx;
     const float2* twy;
     const float2* twz;
     float2* W;              // working spectrum [3][nz][fy][fxp], kx < fxh used
-    const float* Tw;        // tensor words [6][fy][fxh][fzs] (real spectra), g/(fx fy fz) folded in
+    const float* Tw;        // tensor words [6][fys][fxh][fzs] (real spectra), g/(fx fy fz) folded in
     float* hout;            // op-API target slot [3][n] (row kernel mode INV)
     const float* sin;       // op-API source spins (row kernel mode FWD)
 };
@@ -268,7 +269,7 @@ int zconv_widths(int zt, int* w, int cap);
 // tensor: periodic lattice sum (R > 0) or open kernel (R == 0) on the FFT grid, FP64
 hipError_t launch3_tensor(int nx, int ny, int nz, int fx, int fy, int fz, int R, double* T, hipStream_t st);
 // real spectrum words of T: U1/U2 complex FP64 scratch [6][fz][fy][fxh]; tw* FP64 tables
-hipError_t launch3_spectrum(int fx, int fy, int fz, int fzs, const double* T, double2* U1, double2* U2, float* Tw,
+hipError_t launch3_spectrum(int fx, int fy, int fz, int fys, int fzs, const double* T, double2* U1, double2* U2, float* Tw,
                             double mult, const double2* twx, const double2* twy, const double2* twz, hipStream_t st);
 hipError_t configure_lds3(size_t max_bytes);
 hipError_t configure_fft64(size_t bytes);
