@@ -373,7 +373,9 @@ int msd_stepper_set_uniaxial(msd_stepper* st, msd_uniaxial* u);
  * (temperature, dt, alpha, gamma, applied fields, ...) from a device copy that is
  * stored on the stream when they change, so a script that calls temp:set(T) or
  * ss:setTimeStep(dt) between steps (doc/manual/BasicMicromagneticSimulation.lua)
- * replays the same graph; only a change of kernels or shapes captures a new one. */
+ * replays the same graph; only a change of kernels or shapes captures a new one.
+ * A call of nsteps >= 16 replays a graph of 16 steps nsteps/16 times and the one-step
+ * graph for the rest (one graph launch per 16 steps). */
 int msd_stepper_heun(msd_stepper* st, int nsteps);
 /* make_rk_step_function(ss, kind, calcField, llg, temp) steps (common/scripts/RungeKutta.lua:
  * 57-214) with fused stage kernels: kind names a Butcher table of the script (RK1, RK2,
