@@ -227,6 +227,12 @@ int msd_thermal_scale_site(msd_thermal* t, int site, double v);
  * is keyed by a seed from /dev/urandom, as an unseeded MagLua RNG is
  * (common/random/random.cpp:29-62); set_seed gives a reproducible stream. */
 int msd_thermal_set_seed(msd_thermal* t, uint64_t seed, uint64_t draw);
+/* Thermal::encode / Thermal::decode (cpu/thermal/spinoperationthermal.cpp:51-80): the
+ * operator preamble, version 0, the temperature and the per-site scale array (two
+ * objects: the Thermal and its Array.Double).  The noise stream is not in the
+ * reference's stream: a decoded Thermal has a seed from /dev/urandom, as a new one. */
+int msd_thermal_encode(msd_thermal* t, int first_index, void* buf, int64_t cap, int64_t* len, int* n_objects);
+int msd_thermal_decode(const void* buf, int64_t len, int first_index, msd_thermal** out, int64_t* used, int* n_objects);
 int msd_thermal_get_draw(const msd_thermal* t, uint64_t* draw);
 int msd_thermal_get_seed(const msd_thermal* t, uint64_t* seed);
 /* temp:apply(ss): Thermal::apply, cpu/thermal/spinoperationthermal.cpp:135-191, slot "Thermal" */
@@ -343,6 +349,13 @@ int msd_llg_destroy(msd_llg* l);
 int msd_llg_set_thermal_only_first_term(msd_llg* l, int yes);
 int msd_llg_set_disable_renormalization(msd_llg* l, int yes);
 int msd_llg_set_scales(msd_llg* l, double precess_scale, double damp_scale);
+/* LLG::encode / LLG::decode (cpu/llg/llg.cpp:50-88), for LLG.Cartesian and
+ * LLG.Quaternion alike (quaternion selects the decoded type): version 1 with
+ * thermalOnlyFirstTerm, disableRenormalization and the two scales; version 0 streams
+ * (no scales) decode with scales 1. */
+int msd_llg_encode(msd_llg* l, int first_index, void* buf, int64_t cap, int64_t* len, int* n_objects);
+int msd_llg_decode(const void* buf, int64_t len, int first_index, int quaternion, msd_llg** out, int64_t* used,
+                   int* n_objects);
 /* llg:apply(from [,scale] [,dmdt] [,to] [,advance]): cpu/llg/llg.cpp:100-171.
  * dmdt and to may be NULL (= from). */
 int msd_llg_apply(msd_llg* l, msd_system* from, double scale, msd_system* dmdt, msd_system* to, int advance);

@@ -121,6 +121,8 @@ _SIGS = {
     "msd_thermal_set_scale": (C.c_int, [_vp, C.c_double]),
     "msd_thermal_scale_site": (C.c_int, [_vp, C.c_int, C.c_double]),
     "msd_thermal_set_seed": (C.c_int, [_vp, C.c_uint64, C.c_uint64]),
+    "msd_thermal_encode": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, C.POINTER(C.c_int64), _ip]),
+    "msd_thermal_decode": (C.c_int, [_vp, C.c_int64, C.c_int, _pp, C.POINTER(C.c_int64), _ip]),
     "msd_thermal_get_draw": (C.c_int, [_vp, _u64p]),
     "msd_thermal_get_seed": (C.c_int, [_vp, _u64p]),
     "msd_thermal_apply": (C.c_int, [_vp, _vp]),
@@ -164,6 +166,8 @@ _SIGS = {
     "msd_llg_set_thermal_only_first_term": (C.c_int, [_vp, C.c_int]),
     "msd_llg_set_disable_renormalization": (C.c_int, [_vp, C.c_int]),
     "msd_llg_set_scales": (C.c_int, [_vp, C.c_double, C.c_double]),
+    "msd_llg_encode": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, C.POINTER(C.c_int64), _ip]),
+    "msd_llg_decode": (C.c_int, [_vp, C.c_int64, C.c_int, C.c_int, _pp, C.POINTER(C.c_int64), _ip]),
     "msd_llg_apply": (C.c_int, [_vp, _vp, C.c_double, _vp, _vp, C.c_int]),
     "msd_stepper_create": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _pp]),
     "msd_stepper_destroy": (C.c_int, [_vp]),

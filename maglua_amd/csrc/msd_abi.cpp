@@ -323,11 +323,8 @@ extern "C" int msd_system_set_site_gamma(msd_system* s, const double* g) { NONNU
 
 // ------------------------------------------------------------------ checkpoint byte stream
 // MagLua's object encoding (common/luabaseobject/luabaseobject.cpp:140-170): int32 and
-// IEEE double in host (little-endian) order, chars as one byte.  An object opens with a
-// magic char (111: new to the stream, 222: written earlier) and its index in the stream's
-// object list (luabaseobject.h:63-72).
-enum : char { WIRE_NEW = 111, WIRE_OLD = (char)222 };
-enum : int { WIRE_LUA_TUSERDATA = 7 };
+// IEEE double in host (little-endian) order, chars as one byte (WireOut / WireIn and the
+// magic chars in msd_host.h).
 
 // MurmurHash3_x86_32 (Austin Appleby's published, public-domain hash), seed 1000: MagLua's
 // type ids (common/luabaseobject/factory.cpp:123-131).  MagLua's copy rotates the tail
@@ -406,7 +403,7 @@ int unwire_operator(WireIn& r, const char* who, int index, int* nx, int* ny, int
 // encodeT<dArray> (luabaseobject.h:101-106) then Array<double>::encode
 // (cpu/array/array_core_cpu.h:200-224): one value when all sites hold the same value,
 // else the raw doubles.
-static void wire_darray(WireOut& w, int index, const Dims& d, const double* v)
+void wire_darray(WireOut& w, int index, const Dims& d, const double* v)
 {
     w.i32(WIRE_LUA_TUSERDATA);
     w.i32(type_id("Array.Double"));
@@ -461,34 +458,34 @@ extern "C" int msd_system_encode(msd_system* s, int first_index, void* buf, int6
 // Reads one Array.Double of the system's size into v, which is sized only once the
 // array's header has been checked and, for an explicit array, the stream holds its
 // 8n bytes: a truncated or corrupt stream fails before any large allocation.
-static int unwire_darray(WireIn& r, int index, const Dims& d, std::vector<double>& v)
+int unwire_darray(WireIn& r, const char* who, int index, const Dims& d, std::vector<double>& v)
 {
     int32_t tag, type, pos, nx, ny, nz, flag;
     char magic, version;
     if (!r.i32(&tag) || !r.i32(&type) || !r.ch(&magic) || !r.i32(&pos))
-        return fail(MSD_EINVAL, "SpinSystem.decode: stream ends inside an array header");
+        return fail(MSD_EINVAL, "%s: stream ends inside an array header", who);
     if (tag != WIRE_LUA_TUSERDATA || type != type_id("Array.Double"))
-        return fail(MSD_EINVAL, "SpinSystem.decode: expected an Array.Double (tag %d, type %d)", tag, type);
+        return fail(MSD_EINVAL, "%s: expected an Array.Double (tag %d, type %d)", who, tag, type);
     if (magic == WIRE_OLD)
-        return fail(MSD_EUNSUPPORTED, "SpinSystem.decode: array %d refers to an earlier object of the stream", pos);
-    if (magic != WIRE_NEW) return fail(MSD_EINVAL, "SpinSystem.decode: malformed stream (magic %d)", (int)magic);
-    if (pos != index) return fail(MSD_EINVAL, "SpinSystem.decode: object index %d, expected %d", pos, index);
+        return fail(MSD_EUNSUPPORTED, "%s: array %d refers to an earlier object of the stream", who, pos);
+    if (magic != WIRE_NEW) return fail(MSD_EINVAL, "%s: malformed stream (magic %d)", who, (int)magic);
+    if (pos != index) return fail(MSD_EINVAL, "%s: object index %d, expected %d", who, pos, index);
     if (!r.ch(&version) || !r.i32(&nx) || !r.i32(&ny) || !r.i32(&nz) || !r.i32(&flag))
-        return fail(MSD_EINVAL, "SpinSystem.decode: stream ends inside an array");
-    if (version != 0) return fail(MSD_EINVAL, "SpinSystem.decode: array version %d", (int)version);
+        return fail(MSD_EINVAL, "%s: stream ends inside an array", who);
+    if (version != 0) return fail(MSD_EINVAL, "%s: array version %d", who, (int)version);
     if (!same_dims(d, nx, ny, nz))
-        return fail(MSD_EINVAL, "SpinSystem.decode: array of %dx%dx%d in a %dx%dx%d system", nx, ny, nz, d.nx, d.ny, d.nz);
+        return fail(MSD_EINVAL, "%s: array of %dx%dx%d in a %dx%dx%d system", who, nx, ny, nz, d.nx, d.ny, d.nz);
     if (flag == 1) {
         double c;
-        if (!r.f64(&c)) return fail(MSD_EINVAL, "SpinSystem.decode: stream ends inside an array");
+        if (!r.f64(&c)) return fail(MSD_EINVAL, "%s: stream ends inside an array", who);
         v.assign(d.n, c);
         return MSD_OK;
     }
-    if (flag != 0) return fail(MSD_EINVAL, "SpinSystem.decode: array flag %d", flag);
+    if (flag != 0) return fail(MSD_EINVAL, "%s: array flag %d", who, flag);
     const size_t bytes = (size_t)d.n * sizeof(double);
-    if (bytes > r.n - r.at) return fail(MSD_EINVAL, "SpinSystem.decode: stream ends inside an array");
+    if (bytes > r.n - r.at) return fail(MSD_EINVAL, "%s: stream ends inside an array", who);
     v.resize(d.n);
-    if (!r.get(v.data(), bytes)) return fail(MSD_EINVAL, "SpinSystem.decode: stream ends inside an array");
+    if (!r.get(v.data(), bytes)) return fail(MSD_EINVAL, "%s: stream ends inside an array", who);
     return MSD_OK;
 }
 
@@ -519,13 +516,13 @@ extern "C" int msd_system_decode(const void* buf, int64_t len, int first_index, 
     int idx = first_index + 1;
     std::vector<double> sp[4];   // x y z ms
     for (int c = 0; c < 4; ++c)
-        if (int rc = unwire_darray(r, idx++, d, sp[c])) return rc;
+        if (int rc = unwire_darray(r, "SpinSystem.decode", idx++, d, sp[c])) return rc;
     std::vector<double> site[2];
     for (int k = 0; k < 2; ++k) {
         int32_t has;
         if (!r.i32(&has)) return fail(MSD_EINVAL, "SpinSystem.decode: stream ends before the site arrays");
         if (!has) continue;
-        if (int rc = unwire_darray(r, idx++, d, site[k])) return rc;
+        if (int rc = unwire_darray(r, "SpinSystem.decode", idx++, d, site[k])) return rc;
     }
     int32_t extra;
     if (!r.i32(&extra)) return fail(MSD_EINVAL, "SpinSystem.decode: stream ends before the extra-data count");

@@ -63,6 +63,9 @@ int copy_dev(float* dst, const float* src, size_t n);
 int wire_finish(const WireOut& w, const char* who, void* buf, int64_t cap, int64_t* len);
 void wire_operator(WireOut& w, int index, const Dims& d, double scale);
 int unwire_operator(WireIn& r, const char* who, int index, int* nx, int* ny, int* nz, double* scale);
+// one Array.Double object of the lattice's size (encodeT<dArray> + Array<double>::encode)
+void wire_darray(WireOut& w, int index, const Dims& d, const double* v);
+int unwire_darray(WireIn& r, const char* who, int index, const Dims& d, std::vector<double>& v);
 int twiddles(int L, float2** d32, double2** d64);
 int layered_shape(msd_dipole* p, int ty, int tpb, bool row_ct, bool want_split, const char* what);
 int dipole_field_of(msd_dipole* dip, const float* s3, float* h);
@@ -157,6 +160,10 @@ struct msd_system {
     LLGDev llgdev(const msd_llg* l) const;
 };
 
+// An object of the checkpoint stream opens with a magic char (111: new to the stream,
+// 222: written earlier) and its index in the stream's object list (luabaseobject.h:63-72).
+enum : char { WIRE_NEW = 111, WIRE_OLD = (char)222 };
+enum : int { WIRE_LUA_TUSERDATA = 7 };
 struct WireOut {
     std::vector<uint8_t> b;
     void put(const void* p, size_t n) { const uint8_t* q = (const uint8_t*)p; b.insert(b.end(), q, q + n); }

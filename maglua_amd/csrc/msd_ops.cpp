@@ -530,6 +530,109 @@ extern "C" int msd_thermal_get_draw(const msd_thermal* t, uint64_t* draw) { NONN
 extern "C" int msd_thermal_get_seed(const msd_thermal* t, uint64_t* seed) { NONNULL(t); NONNULL(seed); *seed = t->seed; return MSD_OK; }
 
 
+// Thermal::encode (cpu/thermal/spinoperationthermal.cpp:51-61): the operator preamble,
+// version 0, the temperature and the per-site scale array.  The noise stream (seed and
+// draw counter) is not part of the reference's stream: a decoded Thermal takes a seed
+// from the caller (msd_thermal_set_seed).
+extern "C" int msd_thermal_encode(msd_thermal* t, int first_index, void* buf, int64_t cap, int64_t* len, int* n_objects)
+{
+    NONNULL(t); NONNULL(len);
+    if (first_index < 0) return fail(MSD_EINVAL, "Thermal.encode: object index %d < 0", first_index);
+    WireOut w;
+    wire_operator(w, first_index, t->d, t->scale);
+    w.ch(0);
+    w.f64(t->T);
+    std::vector<double> ones;
+    const double* site = t->site.data();
+    if (t->site.size() != (size_t)t->d.n) {
+        ones.assign(t->d.n, 1.0);
+        site = ones.data();
+    }
+    wire_darray(w, first_index + 1, t->d, site);
+    if (n_objects) *n_objects = 2;
+    return wire_finish(w, "Thermal.encode", buf, cap, len);
+}
+
+// Thermal::decode (spinoperationthermal.cpp:63-80)
+extern "C" int msd_thermal_decode(const void* buf, int64_t len, int first_index, msd_thermal** out, int64_t* used,
+                                  int* n_objects)
+{
+    NONNULL(buf); NONNULL(out);
+    *out = nullptr;
+    if (len < 0) return fail(MSD_EINVAL, "Thermal.decode: length %lld", (long long)len);
+    WireIn r{(const uint8_t*)buf, (size_t)len};
+    int nx, ny, nz;
+    double scale, T;
+    char version;
+    if (int rc = unwire_operator(r, "Thermal.decode", first_index, &nx, &ny, &nz, &scale)) return rc;
+    if (!r.ch(&version)) return fail(MSD_EINVAL, "Thermal.decode: stream too short");
+    if (version != 0) return fail(MSD_EINVAL, "Thermal.decode: unknown version %d", (int)version);
+    if (!r.f64(&T)) return fail(MSD_EINVAL, "Thermal.decode: stream ends before the temperature");
+    msd_thermal* t = nullptr;
+    if (int rc = msd_thermal_create(nx, ny, nz, &t)) return rc;
+    std::unique_ptr<msd_thermal> own(t);
+    std::vector<double> site;
+    if (int rc = unwire_darray(r, "Thermal.decode", first_index + 1, t->d, site)) return rc;
+    t->T = T;
+    t->scale = scale;
+    t->site = site;
+    t->site_dirty = true;
+    if (used) *used = (int64_t)r.at;
+    if (n_objects) *n_objects = 2;
+    *out = own.release();
+    return MSD_OK;
+}
+
+// LLG::encode (cpu/llg/llg.cpp:50-60), the same for LLG.Cartesian and LLG.Quaternion:
+// the object preamble, version 1, thermalOnlyFirstTerm, disableRenormalization and the
+// precession and damping scales (no SpinOperation part).
+extern "C" int msd_llg_encode(msd_llg* l, int first_index, void* buf, int64_t cap, int64_t* len, int* n_objects)
+{
+    NONNULL(l); NONNULL(len);
+    if (first_index < 0) return fail(MSD_EINVAL, "LLG.encode: object index %d < 0", first_index);
+    WireOut w;
+    w.ch(WIRE_NEW);
+    w.i32(first_index);
+    w.ch(1);
+    w.i32(l->thermal_first ? 1 : 0);
+    w.i32(l->no_renorm ? 1 : 0);
+    w.f64(l->ps);
+    w.f64(l->ds);
+    if (n_objects) *n_objects = 1;
+    return wire_finish(w, "LLG.encode", buf, cap, len);
+}
+
+// LLG::decode (llg.cpp:62-88): version 0 has no scales (both 1)
+extern "C" int msd_llg_decode(const void* buf, int64_t len, int first_index, int quaternion, msd_llg** out, int64_t* used,
+                              int* n_objects)
+{
+    NONNULL(buf); NONNULL(out);
+    *out = nullptr;
+    if (len < 0) return fail(MSD_EINVAL, "LLG.decode: length %lld", (long long)len);
+    WireIn r{(const uint8_t*)buf, (size_t)len};
+    char magic, version;
+    int32_t pos, tf, nr;
+    if (!r.ch(&magic) || !r.i32(&pos)) return fail(MSD_EINVAL, "LLG.decode: stream too short");
+    if (magic == WIRE_OLD) return fail(MSD_EINVAL, "LLG.decode: object %d was decoded earlier in the stream", pos);
+    if (magic != WIRE_NEW) return fail(MSD_EINVAL, "LLG.decode: malformed stream (magic %d)", (int)magic);
+    if (pos != first_index) return fail(MSD_EINVAL, "LLG.decode: object index %d, expected %d", pos, first_index);
+    if (!r.ch(&version)) return fail(MSD_EINVAL, "LLG.decode: stream too short");
+    if (version != 0 && version != 1) return fail(MSD_EINVAL, "LLG.decode: unknown version %d", (int)version);
+    if (!r.i32(&tf) || !r.i32(&nr)) return fail(MSD_EINVAL, "LLG.decode: stream ends inside the flags");
+    double ps = 1.0, ds = 1.0;
+    if (version == 1 && (!r.f64(&ps) || !r.f64(&ds))) return fail(MSD_EINVAL, "LLG.decode: stream ends inside the scales");
+    msd_llg* l = nullptr;
+    if (int rc = quaternion ? msd_llg_create_quaternion(&l) : msd_llg_create(&l)) return rc;
+    l->thermal_first = tf != 0;
+    l->no_renorm = nr != 0;
+    l->ps = ps;
+    l->ds = ds;
+    if (used) *used = (int64_t)r.at;
+    if (n_objects) *n_objects = 1;
+    *out = l;
+    return MSD_OK;
+}
+
 LLGDev msd_system::llgdev(const msd_llg* l) const
 {
     LLGDev p{};

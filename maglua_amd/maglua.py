@@ -629,6 +629,17 @@ class Thermal(_Handle):
 
     new = classmethod(lambda cls, *a, **k: cls(*a, **k))
 
+    # checkpoint byte stream (Thermal::encode/decode, spinoperationthermal.cpp:51-80); the
+    # noise stream is not in it: a decoded Thermal has a fresh seed (setSeed to choose one)
+    def encode(self, first_index=0):
+        return _encode_handle("msd_thermal_encode", self._h, first_index)
+
+    @classmethod
+    def decode(cls, data, first_index=0):
+        o, used, k = _decode_handle(cls, "msd_thermal_decode", data, first_index)
+        o._T = struct.unpack_from("<d", bytes(data), 27)[0]      # after preamble, dims, scale, version
+        return o, used, k
+
     def set(self, T):
         self._T = float(T)
         call("msd_thermal_set_temperature", self._h, self._T)
@@ -1224,6 +1235,22 @@ class _LLGBase(_Handle):
 
     def setScales(self, precess=1.0, damp=1.0):
         call("msd_llg_set_scales", self._h, float(precess), float(damp))
+
+    # checkpoint byte stream (LLG::encode/decode, cpu/llg/llg.cpp:50-88)
+    def encode(self, first_index=0):
+        return _encode_handle("msd_llg_encode", self._h, first_index)
+
+    @classmethod
+    def decode(cls, data, first_index=0):
+        data = bytes(data)
+        p, used, k = _vp(), C.c_int64(), C.c_int()
+        call("msd_llg_decode", data, len(data), int(first_index), int(cls._create == "msd_llg_create_quaternion"),
+             C.byref(p), C.byref(used), C.byref(k))
+        o = cls.__new__(cls)
+        _Handle.__init__(o, p.value)
+        tf, nr = struct.unpack_from("<ii", data, 6)               # after preamble and version
+        o._tf, o._nr = bool(tf), bool(nr)
+        return o, used.value, k.value
 
     def apply(self, *args):
         """apply(from [,scale] [,dmdt] [,to] [,advance]); lists apply element-wise."""

@@ -86,3 +86,24 @@ ANISOTROPY_2x1x1 = b"".join([
     i(1),                                    # op count
     i(1), d(0.0), d(0.0), d(1.0), d(0.5),    # site, axis, K
 ])
+
+# Thermal::encode (cpu/thermal/spinoperationthermal.cpp:51-61) of a 2x1x1 Thermal with
+# scale 2, temperature 0.75 and site 2 scaled by 0.5, as object 4: the scale array is
+# encodeT<dArray> of its own object, 5
+THERMAL_2x1x1 = b"".join([
+    NEW, i(4),
+    b"\x00", i(2), i(1), i(1), d(2.0),       # SpinOperation::encode: version, dims, scale
+    b"\x00",                                 # version
+    d(0.75),                                 # temperature
+    i(7), i(ARRAY_DOUBLE_ID), NEW, i(5), b"\x00", i(2), i(1), i(1), i(0), d(1.0), d(0.5),   # scale array
+])
+
+# LLG::encode (cpu/llg/llg.cpp:50-60) of an LLG with thermalOnlyFirstTerm off,
+# renormalisation disabled, precession scale 0.5 and damping scale 2, as object 1
+LLG_FLAGS = b"".join([
+    NEW, i(1),                               # ENCODE_PREAMBLE (no SpinOperation part)
+    b"\x01",                                 # version
+    i(0),                                    # thermalOnlyFirstTerm
+    i(1),                                    # disableRenormalization
+    d(0.5), d(2.0),                          # precess_scale, damp_scale
+])

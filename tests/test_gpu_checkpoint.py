@@ -158,6 +158,55 @@ def test_operator_streams_golden_and_round_trip():
     assert ex3.encode()[0] == G.EXCHANGE_2x1x1[:27] + struct.pack("<i", 0) + G.EXCHANGE_2x1x1[31:]
 
 
+def test_thermal_and_llg_streams_golden_and_round_trip():
+    th = M.Thermal(2, 1, 1)
+    th.setScale(2.0)
+    th.set(0.75)
+    th.scaleSite((2,), 0.5)
+    b, k = th.encode(first_index=4)
+    assert b == G.THERMAL_2x1x1 and k == 2
+    th2, used, k2 = M.Thermal.decode(b, first_index=4)
+    assert used == len(b) and k2 == 2 and th2.temperature() == 0.75
+    assert th2.encode(first_index=4)[0] == b
+    assert th2.seed() != 0 and th2.seed() != th.seed()      # the noise stream is not in the stream
+    for cls in (M.LLG.Cartesian, M.LLG.Quaternion):
+        llg = cls()
+        llg.setThermalOnlyFirstTerm(False)
+        llg.setDisableRenormalization(True)
+        llg.setScales(0.5, 2.0)
+        b, k = llg.encode(first_index=1)
+        assert b == G.LLG_FLAGS and k == 1
+        l2, used, _ = cls.decode(b, first_index=1)
+        assert type(l2) is cls and used == len(b) and l2.encode(first_index=1)[0] == b
+        assert not l2.thermalOnlyFirstTerm()
+    # a version-0 LLG stream has no scales: they decode as 1
+    v0 = G.LLG_FLAGS[:5] + b"\x00" + G.LLG_FLAGS[6:14]
+    l3, used, _ = M.LLG.Cartesian.decode(v0, first_index=1)
+    assert used == len(v0)
+    assert l3.encode(first_index=1)[0] == G.LLG_FLAGS[:14] + struct.pack("<dd", 1.0, 1.0)
+
+
+def test_thermal_heun_from_checkpoint(tmp_path):
+    """A T > 0 film with its Thermal and LLG from one file, the noise stream set to the
+    original's seed and draw: the steps equal the original's bit for bit."""
+    spec = W.FilmSpec(16, 16, 2, J=1.0, K=0.1, g=0.02, pbc=(1, 1, 0), alpha=0.1, dt=0.005, T=0.6, seed=3)
+    g = W.build(M, spec)
+    g["ss"].randomize(4)
+    g["temp"].setSeed(77, 5)
+    g["llg"].setThermalOnlyFirstTerm(True)
+    fn = str(tmp_path / "thermal.dat")
+    M.checkpointSave(fn, g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"])
+    ss2, ex2, ani2, dip2, th2, llg2 = M.checkpointLoad(fn)
+    assert isinstance(th2, M.Thermal) and th2.temperature() == g["temp"].temperature()
+    th2.setSeed(g["temp"].seed(), g["temp"].draw())
+    M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"]).step(20)
+    M.Heun(ss2, ex2, ani2, dip2, th2, llg2).step(20)
+    M.synchronize()
+    d = np.max(np.abs(flat(ss2.spins()) - flat(g["ss"].spins())))
+    print(f"checkpointed Thermal and LLG: max|dm| after 20 steps at T=0.6 = {d:.3e}", flush=True)
+    assert d == 0.0
+
+
 def test_heun_from_checkpointed_system_and_operators(tmp_path):
     """Spins, Exchange, Anisotropy and Dipole from one file step exactly as the originals."""
     spec = W.FilmSpec(16, 16, 2, J=1.0, K=0.1, g=0.02, pbc=(1, 1, 0), alpha=0.1, dt=0.005, T=0.0, seed=0)
