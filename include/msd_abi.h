@@ -195,6 +195,11 @@ int msd_uniaxial_add(msd_uniaxial* u, int site, const double* axis, double K1, d
 int msd_uniaxial_num_axes(const msd_uniaxial* u, int* n);
 int msd_uniaxial_axis_at(const msd_uniaxial* u, int i, int* site, double* axis, double* K1, double* K2);
 int msd_uniaxial_apply(msd_uniaxial* u, msd_system* s);
+/* AnisotropyUniaxial::encode / decode (cpu/anisotropy/spinoperationanisotropy_uniaxial.cpp:
+ * 181-234): ops in order of addition; decode passes each axis through addAnisotropy's
+ * division by its squared length (:172-174). */
+int msd_uniaxial_encode(msd_uniaxial* u, int first_index, void* buf, int64_t cap, int64_t* len, int* n_objects);
+int msd_uniaxial_decode(const void* buf, int64_t len, int first_index, msd_uniaxial** out, int64_t* used, int* n_objects);
 
 /* ------------------------------------------------------------ AppliedField
  * AppliedField.new(n) (cpu/appliedfield/spinoperationappliedfield.cpp:18-25; x y z
@@ -213,6 +218,13 @@ int msd_appliedfield_get_site(const msd_appliedfield* a, int site, double* xyz);
  * "Zeeman" to B*scale; heterogeneous (spinoperationappliedfield_heterogeneous.cpp:88-99)
  * adds scale*h to slot "Zeeman_Heterogeneous" */
 int msd_appliedfield_apply(msd_appliedfield* a, msd_system* s);
+/* AppliedField::encode / decode (cpu/appliedfield/spinoperationappliedfield.cpp:33-58):
+ * B; AppliedField_Heterogeneous (spinoperationappliedfield_heterogeneous.cpp:39-72): three
+ * per-site arrays written by their own encode (four objects).  heterogeneous selects the
+ * decoded form. */
+int msd_appliedfield_encode(msd_appliedfield* a, int first_index, void* buf, int64_t cap, int64_t* len, int* n_objects);
+int msd_appliedfield_decode(const void* buf, int64_t len, int first_index, int heterogeneous, msd_appliedfield** out,
+                            int64_t* used, int* n_objects);
 
 /* ----------------------------------------------------------------- Thermal */
 int msd_thermal_create(int nx, int ny, int nz, msd_thermal** out);

@@ -122,6 +122,10 @@ _SIGS = {
     "msd_thermal_scale_site": (C.c_int, [_vp, C.c_int, C.c_double]),
     "msd_thermal_set_seed": (C.c_int, [_vp, C.c_uint64, C.c_uint64]),
     "msd_thermal_encode": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, C.POINTER(C.c_int64), _ip]),
+    "msd_uniaxial_encode": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, C.POINTER(C.c_int64), _ip]),
+    "msd_uniaxial_decode": (C.c_int, [_vp, C.c_int64, C.c_int, _pp, C.POINTER(C.c_int64), _ip]),
+    "msd_appliedfield_encode": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, C.POINTER(C.c_int64), _ip]),
+    "msd_appliedfield_decode": (C.c_int, [_vp, C.c_int64, C.c_int, C.c_int, _pp, C.POINTER(C.c_int64), _ip]),
     "msd_thermal_decode": (C.c_int, [_vp, C.c_int64, C.c_int, _pp, C.POINTER(C.c_int64), _ip]),
     "msd_thermal_get_draw": (C.c_int, [_vp, _u64p]),
     "msd_thermal_get_seed": (C.c_int, [_vp, _u64p]),

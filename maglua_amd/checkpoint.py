@@ -21,7 +21,8 @@ opens with QjY).
 
 Python values map to Lua values as: None - nil, bool - boolean, int/float - number,
 str - string (UTF-8), list/tuple/dict - table (lists take keys 1..n), SpinSystem,
-Exchange, Anisotropy, Dipole, Thermal, LLG.Cartesian and LLG.Quaternion - userdata.  Decoded tables come back as dicts, numbers as floats (integral table keys as
+Exchange, Anisotropy, Anisotropy.Uniaxial, Dipole, Thermal, LLG.Cartesian, LLG.Quaternion,
+AppliedField and AppliedField.Heterogeneous - userdata.  Decoded tables come back as dicts, numbers as floats (integral table keys as
 ints).  Table pairs are written in list order or dict insertion order; Lua writes them in
 its own table-traversal order, so streams holding tables with several keys match
 MagLua's bytes only where the two orders agree.
@@ -33,11 +34,13 @@ import struct
 import zlib
 
 from ._lib import MagLuaError, MSD_EINVAL, MSD_EUNSUPPORTED, call
-from .maglua import LLG, Anisotropy, Dipole, Exchange, SpinSystem, Thermal
+from .maglua import LLG, AppliedField, Anisotropy, Dipole, Exchange, SpinSystem, Thermal
 
 # userdata this container writes and reads, by MagLua type name
 _USERDATA = (("SpinSystem", SpinSystem), ("Exchange", Exchange), ("Anisotropy", Anisotropy), ("Dipole", Dipole),
-             ("Thermal", Thermal), ("LLG.Cartesian", LLG.Cartesian), ("LLG.Quaternion", LLG.Quaternion))
+             ("Thermal", Thermal), ("LLG.Cartesian", LLG.Cartesian), ("LLG.Quaternion", LLG.Quaternion),
+             ("AppliedField", AppliedField), ("AppliedField.Heterogeneous", AppliedField.Heterogeneous),
+             ("Anisotropy.Uniaxial", Anisotropy.Uniaxial))
 
 LUA_TNIL, LUA_TBOOLEAN, LUA_TNUMBER, LUA_TSTRING, LUA_TTABLE, LUA_TUSERDATA = 0, 1, 3, 4, 5, 7
 MAGIC_NEW, MAGIC_OLD = 111, 222

@@ -400,13 +400,15 @@ int unwire_operator(WireIn& r, const char* who, int index, int* nx, int* ny, int
     return MSD_OK;
 }
 
-// encodeT<dArray> (luabaseobject.h:101-106) then Array<double>::encode
-// (cpu/array/array_core_cpu.h:200-224): one value when all sites hold the same value,
-// else the raw doubles.
-void wire_darray(WireOut& w, int index, const Dims& d, const double* v)
+// encodeT<dArray> (luabaseobject.h:101-106: the Lua type and the type id) when tagged,
+// then Array<double>::encode (cpu/array/array_core_cpu.h:200-224): one value when all
+// sites hold the same value, else the raw doubles.
+void wire_darray(WireOut& w, int index, const Dims& d, const double* v, bool tagged)
 {
-    w.i32(WIRE_LUA_TUSERDATA);
-    w.i32(type_id("Array.Double"));
+    if (tagged) {
+        w.i32(WIRE_LUA_TUSERDATA);
+        w.i32(type_id("Array.Double"));
+    }
     w.ch(WIRE_NEW);
     w.i32(index);
     w.ch(0);
@@ -437,13 +439,13 @@ extern "C" int msd_system_encode(msd_system* s, int first_index, void* buf, int6
     w.ch(0);
     w.i32(s->d.nx); w.i32(s->d.ny); w.i32(s->d.nz);
     w.f64(s->alpha); w.f64(s->dt); w.f64(s->gamma); w.f64(s->time);
-    for (int c = 0; c < 4; ++c) wire_darray(w, idx++, s->d, sp.data() + c * n);
+    for (int c = 0; c < 4; ++c) wire_darray(w, idx++, s->d, sp.data() + c * n, true);
     std::vector<double> site(n);
     for (float* a : {s->site_alpha, s->site_gamma}) {
         w.i32(a ? 1 : 0);
         if (!a) continue;
         if (int rc = download_f32(site.data(), a, n)) return rc;
-        wire_darray(w, idx++, s->d, site.data());
+        wire_darray(w, idx++, s->d, site.data(), true);
     }
     w.i32(0);
     *len = (int64_t)w.b.size();
@@ -458,12 +460,12 @@ extern "C" int msd_system_encode(msd_system* s, int first_index, void* buf, int6
 // Reads one Array.Double of the system's size into v, which is sized only once the
 // array's header has been checked and, for an explicit array, the stream holds its
 // 8n bytes: a truncated or corrupt stream fails before any large allocation.
-int unwire_darray(WireIn& r, const char* who, int index, const Dims& d, std::vector<double>& v)
+int unwire_darray(WireIn& r, const char* who, int index, const Dims& d, std::vector<double>& v, bool tagged)
 {
-    int32_t tag, type, pos, nx, ny, nz, flag;
+    int32_t tag = WIRE_LUA_TUSERDATA, type = type_id("Array.Double"), pos, nx, ny, nz, flag;
     char magic, version;
-    if (!r.i32(&tag) || !r.i32(&type) || !r.ch(&magic) || !r.i32(&pos))
-        return fail(MSD_EINVAL, "%s: stream ends inside an array header", who);
+    if (tagged && (!r.i32(&tag) || !r.i32(&type))) return fail(MSD_EINVAL, "%s: stream ends inside an array header", who);
+    if (!r.ch(&magic) || !r.i32(&pos)) return fail(MSD_EINVAL, "%s: stream ends inside an array header", who);
     if (tag != WIRE_LUA_TUSERDATA || type != type_id("Array.Double"))
         return fail(MSD_EINVAL, "%s: expected an Array.Double (tag %d, type %d)", who, tag, type);
     if (magic == WIRE_OLD)
@@ -516,13 +518,13 @@ extern "C" int msd_system_decode(const void* buf, int64_t len, int first_index, 
     int idx = first_index + 1;
     std::vector<double> sp[4];   // x y z ms
     for (int c = 0; c < 4; ++c)
-        if (int rc = unwire_darray(r, "SpinSystem.decode", idx++, d, sp[c])) return rc;
+        if (int rc = unwire_darray(r, "SpinSystem.decode", idx++, d, sp[c], true)) return rc;
     std::vector<double> site[2];
     for (int k = 0; k < 2; ++k) {
         int32_t has;
         if (!r.i32(&has)) return fail(MSD_EINVAL, "SpinSystem.decode: stream ends before the site arrays");
         if (!has) continue;
-        if (int rc = unwire_darray(r, "SpinSystem.decode", idx++, d, site[k])) return rc;
+        if (int rc = unwire_darray(r, "SpinSystem.decode", idx++, d, site[k], true)) return rc;
     }
     int32_t extra;
     if (!r.i32(&extra)) return fail(MSD_EINVAL, "SpinSystem.decode: stream ends before the extra-data count");

@@ -63,9 +63,11 @@ int copy_dev(float* dst, const float* src, size_t n);
 int wire_finish(const WireOut& w, const char* who, void* buf, int64_t cap, int64_t* len);
 void wire_operator(WireOut& w, int index, const Dims& d, double scale);
 int unwire_operator(WireIn& r, const char* who, int index, int* nx, int* ny, int* nz, double* scale);
-// one Array.Double object of the lattice's size (encodeT<dArray> + Array<double>::encode)
-void wire_darray(WireOut& w, int index, const Dims& d, const double* v);
-int unwire_darray(WireIn& r, const char* who, int index, const Dims& d, std::vector<double>& v);
+// one Array.Double object of the lattice's size: Array<double>::encode, after the Lua
+// type and type id when tagged (encodeT<dArray>; an owner calling the array's encode
+// directly writes it untagged)
+void wire_darray(WireOut& w, int index, const Dims& d, const double* v, bool tagged);
+int unwire_darray(WireIn& r, const char* who, int index, const Dims& d, std::vector<double>& v, bool tagged);
 int twiddles(int L, float2** d32, double2** d64);
 int layered_shape(msd_dipole* p, int ty, int tpb, bool row_ct, bool want_split, const char* what);
 int dipole_field_of(msd_dipole* dip, const float* s3, float* h);

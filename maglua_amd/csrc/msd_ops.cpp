@@ -530,10 +530,136 @@ extern "C" int msd_thermal_get_draw(const msd_thermal* t, uint64_t* draw) { NONN
 extern "C" int msd_thermal_get_seed(const msd_thermal* t, uint64_t* seed) { NONNULL(t); NONNULL(seed); *seed = t->seed; return MSD_OK; }
 
 
+// AppliedField::encode (cpu/appliedfield/spinoperationappliedfield.cpp:33-42): the
+// operator preamble, version 0 and B.  AppliedField_Heterogeneous::encode
+// (spinoperationappliedfield_heterogeneous.cpp:39-48): the preamble, version 0 and the
+// three per-site component arrays, each written by its own encode (object preamble, no
+// Lua type tag), widened from the FP32 values the build holds.
+extern "C" int msd_appliedfield_encode(msd_appliedfield* a, int first_index, void* buf, int64_t cap, int64_t* len,
+                                       int* n_objects)
+{
+    NONNULL(a); NONNULL(len);
+    if (first_index < 0) return fail(MSD_EINVAL, "AppliedField.encode: object index %d < 0", first_index);
+    WireOut w;
+    wire_operator(w, first_index, a->d, a->scale);
+    w.ch(0);
+    if (!a->het) {
+        for (int c = 0; c < 3; ++c) w.f64(a->B[c]);
+        if (n_objects) *n_objects = 1;
+    } else {
+        std::vector<double> v(a->d.n);
+        for (int c = 0; c < 3; ++c) {
+            for (int i = 0; i < a->d.n; ++i) v[i] = a->h[(size_t)c * a->d.n + i];
+            wire_darray(w, first_index + 1 + c, a->d, v.data(), false);
+        }
+        if (n_objects) *n_objects = 4;
+    }
+    return wire_finish(w, "AppliedField.encode", buf, cap, len);
+}
+
+// AppliedField::decode (:44-58) and AppliedField_Heterogeneous::decode (:50-72); the
+// latter reads its arrays with the array's decode from the version char on, so, as with
+// Thermal, it cannot read the preambles its encode wrote: this reads what encode writes.
+extern "C" int msd_appliedfield_decode(const void* buf, int64_t len, int first_index, int heterogeneous,
+                                       msd_appliedfield** out, int64_t* used, int* n_objects)
+{
+    NONNULL(buf); NONNULL(out);
+    *out = nullptr;
+    const char* who = heterogeneous ? "AppliedField.Heterogeneous.decode" : "AppliedField.decode";
+    if (len < 0) return fail(MSD_EINVAL, "%s: length %lld", who, (long long)len);
+    WireIn r{(const uint8_t*)buf, (size_t)len};
+    int nx, ny, nz;
+    double scale;
+    char version;
+    if (int rc = unwire_operator(r, who, first_index, &nx, &ny, &nz, &scale)) return rc;
+    if (!r.ch(&version)) return fail(MSD_EINVAL, "%s: stream too short", who);
+    if (version != 0) return fail(MSD_EINVAL, "%s: unknown version %d", who, (int)version);
+    msd_appliedfield* a = nullptr;
+    if (int rc = msd_appliedfield_create(nx, ny, nz, heterogeneous, &a)) return rc;
+    std::unique_ptr<msd_appliedfield> own(a);
+    a->scale = scale;
+    if (!heterogeneous) {
+        for (int c = 0; c < 3; ++c)
+            if (!r.f64(&a->B[c])) return fail(MSD_EINVAL, "%s: stream ends inside the field", who);
+    } else {
+        std::vector<double> v;
+        for (int c = 0; c < 3; ++c) {
+            if (int rc = unwire_darray(r, who, first_index + 1 + c, a->d, v, false)) return rc;
+            for (int i = 0; i < a->d.n; ++i) a->h[(size_t)c * a->d.n + i] = (float)v[i];
+        }
+        a->dirty = true;
+    }
+    if (used) *used = (int64_t)r.at;
+    if (n_objects) *n_objects = heterogeneous ? 4 : 1;
+    *out = own.release();
+    return MSD_OK;
+}
+
+// AnisotropyUniaxial::encode (cpu/anisotropy/spinoperationanisotropy_uniaxial.cpp:181-199):
+// the operator preamble, version 0, the op count and each op (site, axis, K1, K2) in order
+// of addition.
+extern "C" int msd_uniaxial_encode(msd_uniaxial* u, int first_index, void* buf, int64_t cap, int64_t* len, int* n_objects)
+{
+    NONNULL(u); NONNULL(len);
+    if (first_index < 0) return fail(MSD_EINVAL, "Anisotropy.Uniaxial.encode: object index %d < 0", first_index);
+    WireOut w;
+    wire_operator(w, first_index, u->d, u->scale);
+    w.ch(0);
+    w.i32((int32_t)u->ops.size());
+    for (const msd_uniaxial::op& o : u->ops) {
+        w.i32(o.site);
+        for (int c = 0; c < 3; ++c) w.f64(o.n[c]);
+        w.f64(o.K1);
+        w.f64(o.K2);
+    }
+    if (n_objects) *n_objects = 1;
+    return wire_finish(w, "Anisotropy.Uniaxial.encode", buf, cap, len);
+}
+
+// AnisotropyUniaxial::decode (:201-234): each op through addAnisotropy (:158-179), which
+// divides the axis by its squared length when that is positive (a unit axis is kept)
+extern "C" int msd_uniaxial_decode(const void* buf, int64_t len, int first_index, msd_uniaxial** out, int64_t* used,
+                                   int* n_objects)
+{
+    NONNULL(buf); NONNULL(out);
+    *out = nullptr;
+    const char* who = "Anisotropy.Uniaxial.decode";
+    if (len < 0) return fail(MSD_EINVAL, "%s: length %lld", who, (long long)len);
+    WireIn r{(const uint8_t*)buf, (size_t)len};
+    int nx, ny, nz;
+    double scale;
+    char version;
+    int32_t num;
+    if (int rc = unwire_operator(r, who, first_index, &nx, &ny, &nz, &scale)) return rc;
+    if (!r.ch(&version)) return fail(MSD_EINVAL, "%s: stream too short", who);
+    if (version != 0) return fail(MSD_EINVAL, "%s: unknown version %d", who, (int)version);
+    if (!r.i32(&num)) return fail(MSD_EINVAL, "%s: stream ends before the op count", who);
+    if (num < 0 || (size_t)num > (r.n - r.at) / 44) return fail(MSD_EINVAL, "%s: %d ops do not fit the stream", who, num);
+    msd_uniaxial* u = nullptr;
+    if (int rc = msd_uniaxial_create(nx, ny, nz, &u)) return rc;
+    std::unique_ptr<msd_uniaxial> own(u);
+    u->scale = scale;
+    for (int k = 0; k < num; ++k) {
+        int32_t site;
+        double a[3], K1, K2;
+        if (!r.i32(&site) || !r.f64(&a[0]) || !r.f64(&a[1]) || !r.f64(&a[2]) || !r.f64(&K1) || !r.f64(&K2))
+            return fail(MSD_EINVAL, "%s: stream ends inside op %d", who, k);
+        if (site < 0 || site >= u->d.n) return fail(MSD_EINVAL, "%s: op %d at site %d outside the lattice", who, k, site);
+        const double l2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+        if (l2 > 0) for (double& c : a) c /= l2;
+        u->ops.push_back(msd_uniaxial::op{site, {a[0], a[1], a[2]}, K1, K2});
+    }
+    u->dirty = true;
+    if (used) *used = (int64_t)r.at;
+    if (n_objects) *n_objects = 1;
+    *out = own.release();
+    return MSD_OK;
+}
+
 // Thermal::encode (cpu/thermal/spinoperationthermal.cpp:51-61): the operator preamble,
-// version 0, the temperature and the per-site scale array.  The noise stream (seed and
-// draw counter) is not part of the reference's stream: a decoded Thermal takes a seed
-// from the caller (msd_thermal_set_seed).
+// version 0, the temperature and the per-site scale array, written by the array's own
+// encode (object preamble, no Lua type tag).  The noise stream (seed and draw counter)
+// is not part of the reference's stream: a decoded Thermal takes a fresh seed.
 extern "C" int msd_thermal_encode(msd_thermal* t, int first_index, void* buf, int64_t cap, int64_t* len, int* n_objects)
 {
     NONNULL(t); NONNULL(len);
@@ -548,12 +674,14 @@ extern "C" int msd_thermal_encode(msd_thermal* t, int first_index, void* buf, in
         ones.assign(t->d.n, 1.0);
         site = ones.data();
     }
-    wire_darray(w, first_index + 1, t->d, site);
+    wire_darray(w, first_index + 1, t->d, site, false);
     if (n_objects) *n_objects = 2;
     return wire_finish(w, "Thermal.encode", buf, cap, len);
 }
 
-// Thermal::decode (spinoperationthermal.cpp:63-80)
+// Thermal::decode (spinoperationthermal.cpp:63-80) reads the scale array with the
+// array's decode, which starts at the version char, so it cannot read the object
+// preamble its own encode wrote; this decode reads the stream the encode writes.
 extern "C" int msd_thermal_decode(const void* buf, int64_t len, int first_index, msd_thermal** out, int64_t* used,
                                   int* n_objects)
 {
@@ -572,7 +700,7 @@ extern "C" int msd_thermal_decode(const void* buf, int64_t len, int first_index,
     if (int rc = msd_thermal_create(nx, ny, nz, &t)) return rc;
     std::unique_ptr<msd_thermal> own(t);
     std::vector<double> site;
-    if (int rc = unwire_darray(r, "Thermal.decode", first_index + 1, t->d, site)) return rc;
+    if (int rc = unwire_darray(r, "Thermal.decode", first_index + 1, t->d, site, false)) return rc;
     t->T = T;
     t->scale = scale;
     t->site = site;

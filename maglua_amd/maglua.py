@@ -417,6 +417,14 @@ class _AnisotropyUniaxial(_Handle):
     def setScale(self, s):
         call("msd_uniaxial_set_scale", self._h, float(s))
 
+    # checkpoint byte stream (AnisotropyUniaxial::encode/decode, :181-234)
+    def encode(self, first_index=0):
+        return _encode_handle("msd_uniaxial_encode", self._h, first_index)
+
+    @classmethod
+    def decode(cls, data, first_index=0):
+        return _decode_handle(cls, "msd_uniaxial_decode", data, first_index)
+
     def add(self, site, axis, K1=None, K2=None):
         """add({x,y,z}, {ax,ay,az}, K1[, K2]) (l_add, :387-431).  As in the reference's
         binding, a fourth number sets K2 to K1's value (it reads K1's stack slot twice)."""
@@ -463,6 +471,17 @@ class _AnisotropyUniaxial(_Handle):
 Anisotropy.Uniaxial = _AnisotropyUniaxial
 
 
+def _decode_applied(cls, data, first_index, het):
+    data = bytes(data)
+    p, used, k = _vp(), C.c_int64(), C.c_int()
+    call("msd_appliedfield_decode", data, len(data), int(first_index), int(het), C.byref(p), C.byref(used), C.byref(k))
+    o = cls.__new__(cls)
+    _Handle.__init__(o, p.value)
+    o._dims = struct.unpack_from("<3i", data, 6)
+    o._scale = struct.unpack_from("<d", data, 18)[0]
+    return o, used.value, k.value
+
+
 class _AppliedFieldHeterogeneous(_Handle):
     """AppliedField.Heterogeneous (cpu/appliedfield/spinoperationappliedfield_heterogeneous.cpp):
     a field per site, added (times scale) to slot 'Zeeman_Heterogeneous' by apply."""
@@ -503,6 +522,14 @@ class _AppliedFieldHeterogeneous(_Handle):
     def slotName(self):
         return "Zeeman_Heterogeneous"
 
+    # checkpoint byte stream (AppliedField_Heterogeneous::encode/decode, :39-72)
+    def encode(self, first_index=0):
+        return _encode_handle("msd_appliedfield_encode", self._h, first_index)
+
+    @classmethod
+    def decode(cls, data, first_index=0):
+        return _decode_applied(cls, data, first_index, 1)
+
     def apply(self, ss):
         for s in (ss if isinstance(ss, (list, tuple)) else (ss,)):
             call("msd_appliedfield_apply", self._h, s.h)
@@ -523,6 +550,14 @@ class AppliedField(_Handle):
         self._scale = 1.0
 
     new = classmethod(lambda cls, *a: cls(*a))
+
+    # checkpoint byte stream (AppliedField::encode/decode, spinoperationappliedfield.cpp:33-58)
+    def encode(self, first_index=0):
+        return _encode_handle("msd_appliedfield_encode", self._h, first_index)
+
+    @classmethod
+    def decode(cls, data, first_index=0):
+        return _decode_applied(cls, data, first_index, 0)
 
     def _B(self):
         v = (C.c_double * 3)()

@@ -89,13 +89,13 @@ ANISOTROPY_2x1x1 = b"".join([
 
 # Thermal::encode (cpu/thermal/spinoperationthermal.cpp:51-61) of a 2x1x1 Thermal with
 # scale 2, temperature 0.75 and site 2 scaled by 0.5, as object 4: the scale array is
-# encodeT<dArray> of its own object, 5
+# written by its own encode (scale->encode: preamble as object 5, no Lua type tag)
 THERMAL_2x1x1 = b"".join([
     NEW, i(4),
     b"\x00", i(2), i(1), i(1), d(2.0),       # SpinOperation::encode: version, dims, scale
     b"\x00",                                 # version
     d(0.75),                                 # temperature
-    i(7), i(ARRAY_DOUBLE_ID), NEW, i(5), b"\x00", i(2), i(1), i(1), i(0), d(1.0), d(0.5),   # scale array
+    NEW, i(5), b"\x00", i(2), i(1), i(1), i(0), d(1.0), d(0.5),   # Array::encode of the scale
 ])
 
 # LLG::encode (cpu/llg/llg.cpp:50-60) of an LLG with thermalOnlyFirstTerm off,
@@ -106,4 +106,37 @@ LLG_FLAGS = b"".join([
     i(0),                                    # thermalOnlyFirstTerm
     i(1),                                    # disableRenormalization
     d(0.5), d(2.0),                          # precess_scale, damp_scale
+])
+
+# AppliedField::encode (cpu/appliedfield/spinoperationappliedfield.cpp:33-42) of a 2x1x1
+# field (0.1, -0.2, 0.3) with scale 1.5, as object 0
+APPLIEDFIELD_2x1x1 = b"".join([
+    NEW, i(0),
+    b"\x00", i(2), i(1), i(1), d(1.5),       # SpinOperation::encode
+    b"\x00",                                 # version
+    d(0.1), d(-0.2), d(0.3),                 # B
+])
+
+# AppliedField_Heterogeneous::encode (spinoperationappliedfield_heterogeneous.cpp:39-48)
+# of a 2x1x1 field with site 1 at (0.5, 0, 0) and site 2 at (0.5, 0.25, -1), as object 2:
+# the three component arrays written by their own encode, objects 3, 4 and 5
+APPLIEDFIELD_HET_2x1x1 = b"".join([
+    NEW, i(2),
+    b"\x00", i(2), i(1), i(1), d(1.0),       # SpinOperation::encode
+    b"\x00",                                 # version
+    NEW, i(3), b"\x00", i(2), i(1), i(1), i(1), d(0.5),              # hx: every site 0.5
+    NEW, i(4), b"\x00", i(2), i(1), i(1), i(0), d(0.0), d(0.25),     # hy
+    NEW, i(5), b"\x00", i(2), i(1), i(1), i(0), d(0.0), d(-1.0),     # hz
+])
+
+# AnisotropyUniaxial::encode (cpu/anisotropy/spinoperationanisotropy_uniaxial.cpp:181-199)
+# of a 2x1x1 operator with ops (site 1, z, K1 0.5, K2 0) and (site 0, x, 0.25, 0.125)
+# in that order, scale 2, as object 1
+UNIAXIAL_2x1x1 = b"".join([
+    NEW, i(1),
+    b"\x00", i(2), i(1), i(1), d(2.0),       # SpinOperation::encode
+    b"\x00",                                 # version
+    i(2),                                    # op count
+    i(1), d(0.0), d(0.0), d(1.0), d(0.5), d(0.0),
+    i(0), d(1.0), d(0.0), d(0.0), d(0.25), d(0.125),
 ])
