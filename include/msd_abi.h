@@ -355,7 +355,9 @@ int msd_longrange3d_pass_widths(msd_longrange3d* l, int* wy, int* wz, int* y_cho
 /* LLG.Cartesian: cpu/llg/llgcartesian.cpp:90-278 (thermalOnlyFirstTerm = true) */
 int msd_llg_create(msd_llg** out);
 /* LLG.Quaternion: cpu/llg/llgquat.cpp:127-373 (rotation of S about the damped
- * field).  Shares the flags and msd_llg_apply; the fused step driver rejects it. */
+ * field).  Shares the flags and msd_llg_apply.  The fused Heun step takes it (each
+ * turn added to the compensated state as the increment R S - S); msd_stepper_rk
+ * rejects it with MSD_EUNSUPPORTED. */
 int msd_llg_create_quaternion(msd_llg** out);
 int msd_llg_destroy(msd_llg* l);
 int msd_llg_set_thermal_only_first_term(msd_llg* l, int yes);

@@ -164,29 +164,14 @@ hipError_t launch_thermal_normals(const Dims& d, uint64_t seed, uint64_t draw, f
 }
 
 // --------------------------------------------------------------------- LLG
-// LLGQuaternion::apply at one site (cpu/llg/llgquat.cpp:127-373): S rotated about
-// A = ps H + ds a (M x h)/ms (thermalOnlyFirstTerm; h = H - H_thermal) or
-// A = H + a (S x H)/ms by g |A| dt / (1 + a^2), as the quaternion product
-// q (0,S) q* with q = (cos(th/2), sin(th/2) A/|A|).
+// LLGQuaternion::apply at one site (cpu/llg/llgquat.cpp:127-373): S turned by the
+// unit quaternion of quat_rot, as the product q (0,S) q*.
 __device__ __forceinline__ float3 quat_step(const LLGDev& lp, float3 S, float3 M, float3 H, float3 h, float al, float ga,
                                             float ms, float dt)
 {
-    float3 A;
-    if (lp.thermal_first) {
-        const float3 Mh = cross3(M, h);
-        const float f = lp.ds * al / ms;
-        A = make_float3(lp.ps * H.x + f * Mh.x, lp.ps * H.y + f * Mh.y, lp.ps * H.z + f * Mh.z);
-    } else {
-        const float3 SH = cross3(S, H);
-        const float f = al / ms;
-        A = make_float3(H.x + f * SH.x, H.y + f * SH.y, H.z + f * SH.z);
-    }
-    const float len = sqrtf(A.x * A.x + A.y * A.y + A.z * A.z);
-    if (!(len > 0.0f)) return S;
-    float sn, qw;
-    sincosf(0.5f * ga * len / (1.0f + al * al) * dt, &sn, &qw);
-    const float k = sn / len;
-    const float3 qv = make_float3(k * A.x, k * A.y, k * A.z);
+    float qw;
+    float3 qv;
+    quat_rot(lp, S, M, H, h, al, ga, ms, dt, qw, qv);
     // t = q (0,S); result = vector part of t q*
     const float tw = -(qv.x * S.x + qv.y * S.y + qv.z * S.z);
     const float3 qs = cross3(qv, S);

@@ -439,6 +439,41 @@ __device__ __forceinline__ float3 llg_rhs(float3 M, float3 H, float3 h, float al
     return make_float3(prec * MH.x - damp * MMh.x, prec * MH.y - damp * MMh.y, prec * MH.z - damp * MMh.z);
 }
 
+// LLGQuaternion::apply (cpu/llg/llgquat.cpp:127-373) at one site: S turns about
+// A = ps H + ds a (M x h)/ms (thermalOnlyFirstTerm; h = H - H_thermal) or
+// A = H + a (S x H)/ms by the angle g |A| dt / (1 + a^2).  quat_rot gives the unit
+// quaternion (qw, qv) of that turn: (1, 0) when A vanishes.
+__device__ __forceinline__ void quat_rot(const LLGDev& lp, float3 S, float3 M, float3 H, float3 h, float al, float ga,
+                                         float ms, float dt, float& qw, float3& qv)
+{
+    float3 A;
+    if (lp.thermal_first) {
+        const float3 Mh = cross3(M, h);
+        const float f = lp.ds * al / ms;
+        A = make_float3(lp.ps * H.x + f * Mh.x, lp.ps * H.y + f * Mh.y, lp.ps * H.z + f * Mh.z);
+    } else {
+        const float3 SH = cross3(S, H);
+        const float f = al / ms;
+        A = make_float3(H.x + f * SH.x, H.y + f * SH.y, H.z + f * SH.z);
+    }
+    const float len = sqrtf(A.x * A.x + A.y * A.y + A.z * A.z);
+    if (!(len > 0.0f)) { qw = 1.0f; qv = make_float3(0.f, 0.f, 0.f); return; }
+    float sn;
+    sincosf(0.5f * ga * len / (1.0f + al * al) * dt, &sn, &qw);
+    const float k = sn / len;
+    qv = make_float3(k * A.x, k * A.y, k * A.z);
+}
+
+// The turn (qw, qv) applied to S, as the increment R S - S = 2 qw (qv x S) +
+// 2 qv x (qv x S) (Rodrigues), so that a compensated state takes it as one more
+// addend, as it takes dt dM of LLG.Cartesian.
+__device__ __forceinline__ float3 quat_delta(float qw, float3 qv, float3 S)
+{
+    const float3 t = cross3(qv, S);
+    const float3 u = cross3(qv, t);
+    return make_float3(2.0f * (qw * t.x + u.x), 2.0f * (qw * t.y + u.y), 2.0f * (qw * t.z + u.z));
+}
+
 // -------------------------------------------- compensated (hi + lo) state
 // Error-free transformations (Knuth TwoSum, FMA-based product) keep the spin
 // state to ~1e-14 while every operation stays FP32.
@@ -665,15 +700,27 @@ __device__ __forceinline__ float3 pred_finish(const StepArgs& a, const ExchangeD
     if (ms > 0.0f) {
         const float al = a.lp.site_alpha ? ldg(a.lp.site_alpha, u) : a.lp.alpha;
         const float ga = a.lp.site_gamma ? ldg(a.lp.site_gamma, u) : a.lp.gamma;
-        const float3 dM = a.lp.thermal_first ? llg_rhs(s, H, H, al, ga, ms, a.lp.ps, a.lp.ds)
-                                             : llg_rhs(s, H, H, al, ga, ms, 1.0f, 1.0f);
-        p = make_float3(s.x + dt * dM.x, s.y + dt * dM.y, s.z + dt * dM.z);
-        if (COMP) {
-            df_add(c.x, clo.x, hdt * dM.x);
-            df_add(c.y, clo.y, hdt * dM.y);
-            df_add(c.z, clo.z, hdt * dM.z);
+        float3 dp, dc;      // P - S and C - S
+        if (a.lp.quaternion) {
+            float qw;
+            float3 qv;
+            quat_rot(a.lp, s, s, H, H, al, ga, ms, dt, qw, qv);
+            dp = quat_delta(qw, qv, s);
+            quat_rot(a.lp, s, s, H, H, al, ga, ms, hdt, qw, qv);
+            dc = quat_delta(qw, qv, s);
         } else {
-            c = make_float3(s.x + hdt * dM.x, s.y + hdt * dM.y, s.z + hdt * dM.z);
+            const float3 dM = a.lp.thermal_first ? llg_rhs(s, H, H, al, ga, ms, a.lp.ps, a.lp.ds)
+                                                 : llg_rhs(s, H, H, al, ga, ms, 1.0f, 1.0f);
+            dp = make_float3(dt * dM.x, dt * dM.y, dt * dM.z);
+            dc = make_float3(hdt * dM.x, hdt * dM.y, hdt * dM.z);
+        }
+        p = make_float3(s.x + dp.x, s.y + dp.y, s.z + dp.z);
+        if (COMP) {
+            df_add(c.x, clo.x, dc.x);
+            df_add(c.y, clo.y, dc.y);
+            df_add(c.z, clo.z, dc.z);
+        } else {
+            c = make_float3(s.x + dc.x, s.y + dc.y, s.z + dc.z);
         }
     }
     if (a.lp.renorm) {
@@ -734,14 +781,24 @@ __device__ __forceinline__ float3 corr_finish(const StepArgs& a, const ExchangeD
     const float al = a.lp.site_alpha ? ldg(a.lp.site_alpha, u) : a.lp.alpha;
     const float ga = a.lp.site_gamma ? ldg(a.lp.site_gamma, u) : a.lp.gamma;
     if (ms > 0.0f) {
-        const float3 dM = a.lp.thermal_first ? llg_rhs(p, H, H, al, ga, ms, a.lp.ps, a.lp.ds)
-                                             : llg_rhs(p, H, H, al, ga, ms, 1.0f, 1.0f);
-        if (COMP) {
-            df_add(v.x, vlo.x, hdt * dM.x);
-            df_add(v.y, vlo.y, hdt * dM.y);
-            df_add(v.z, vlo.z, hdt * dM.z);
+        float3 dv;          // S' - C
+        if (a.lp.quaternion) {
+            // C turned about the axis of P's field (LLG:apply(C, dt/2, P, C))
+            float qw;
+            float3 qv;
+            quat_rot(a.lp, v, p, H, H, al, ga, ms, hdt, qw, qv);
+            dv = quat_delta(qw, qv, v);
         } else {
-            v = make_float3(v.x + hdt * dM.x, v.y + hdt * dM.y, v.z + hdt * dM.z);
+            const float3 dM = a.lp.thermal_first ? llg_rhs(p, H, H, al, ga, ms, a.lp.ps, a.lp.ds)
+                                                 : llg_rhs(p, H, H, al, ga, ms, 1.0f, 1.0f);
+            dv = make_float3(hdt * dM.x, hdt * dM.y, hdt * dM.z);
+        }
+        if (COMP) {
+            df_add(v.x, vlo.x, dv.x);
+            df_add(v.y, vlo.y, dv.y);
+            df_add(v.z, vlo.z, dv.z);
+        } else {
+            v = make_float3(v.x + dv.x, v.y + dv.y, v.z + dv.z);
         }
     }
     if (a.lp.renorm) {
@@ -757,14 +814,23 @@ __device__ __forceinline__ float3 corr_finish(const StepArgs& a, const ExchangeD
                 Ht = make_float3(sg * gn.x, sg * gn.y, sg * gn.z);
             }
             const float3 zero = make_float3(0.f, 0.f, 0.f);
-            const float3 dM = a.lp.thermal_first ? llg_rhs(v, Ht, zero, al, ga, ms, a.lp.ps, a.lp.ds)
-                                                 : llg_rhs(v, Ht, Ht, al, ga, ms, 1.0f, 1.0f);
-            if (COMP) {
-                df_add(v.x, vlo.x, dt * dM.x);
-                df_add(v.y, vlo.y, dt * dM.y);
-                df_add(v.z, vlo.z, dt * dM.z);
+            float3 dv;      // the kick: only the thermal field acts (h = H - H_thermal = 0)
+            if (a.lp.quaternion) {
+                float qw;
+                float3 qv;
+                quat_rot(a.lp, v, v, Ht, a.lp.thermal_first ? zero : Ht, al, ga, ms, dt, qw, qv);
+                dv = quat_delta(qw, qv, v);
             } else {
-                v = make_float3(v.x + dt * dM.x, v.y + dt * dM.y, v.z + dt * dM.z);
+                const float3 dM = a.lp.thermal_first ? llg_rhs(v, Ht, zero, al, ga, ms, a.lp.ps, a.lp.ds)
+                                                     : llg_rhs(v, Ht, Ht, al, ga, ms, 1.0f, 1.0f);
+                dv = make_float3(dt * dM.x, dt * dM.y, dt * dM.z);
+            }
+            if (COMP) {
+                df_add(v.x, vlo.x, dv.x);
+                df_add(v.y, vlo.y, dv.y);
+                df_add(v.z, vlo.z, dv.z);
+            } else {
+                v = make_float3(v.x + dv.x, v.y + dv.y, v.z + dv.z);
             }
         }
         if (a.lp.renorm) {

@@ -133,9 +133,6 @@ extern "C" int msd_stepper_create(msd_system* s, msd_exchange* ex, msd_anisotrop
     if ((ex && !same_dims(d, ex->d.nx, ex->d.ny, ex->d.nz)) || (ani && !same_dims(d, ani->d.nx, ani->d.ny, ani->d.nz)) ||
         (dip && !same_dims(d, dip->d.nx, dip->d.ny, dip->d.nz)) || (th && !same_dims(d, th->d.nx, th->d.ny, th->d.nz)))
         return fail(MSD_EMISMATCH, "stepper: operator and SpinSystem sizes differ");
-    if (llg && llg->quaternion)
-        return fail(MSD_EUNSUPPORTED, "stepper: the fused driver integrates LLG.Cartesian; drive LLG.Quaternion "
-                                      "through the operator API (make_pc_step_function)");
     std::unique_ptr<msd_stepper> st(new msd_stepper);
     st->s = s; st->ex = ex; st->an = ani; st->dip = dip; st->th = th;
     if (llg) st->llg = *llg;
@@ -477,6 +474,9 @@ extern "C" int msd_stepper_rk(msd_stepper* st, const char* kind, int nsteps)
     if (nsteps < 0) return fail(MSD_EINVAL, "rk: nsteps %d", nsteps);
     if (st->lr3) return fail(MSD_EUNSUPPORTED, "rk: the fused Runge-Kutta stages cover the local and layered forms");
     if (st->sur) return fail(MSD_EUNSUPPORTED, "rk: the learned dipole field runs with the Heun step");
+    if (st->llg.quaternion)
+        return fail(MSD_EUNSUPPORTED, "rk: the fused Runge-Kutta stages integrate LLG.Cartesian; the Heun step "
+                                      "takes LLG.Quaternion");
     if (nsteps == 0) return MSD_OK;
     const int ns = (int)bt.size();
     const size_t n3 = (size_t)3 * st->s->d.n;

@@ -177,6 +177,21 @@ def test_heun_with_longrange3d(dims, T):
     assert err <= (1e-5 if T == 0 else 1e-4)
 
 
+def test_heun_quaternion_with_longrange3d():
+    """LLG.Quaternion through the fused 3D row kernels, against the oracle's Heun
+    pass with LLGQuaternion::apply and the thermal kick, same noise."""
+    spec, g, o, gl, ol = cube((8, 8, 8), 0.4, seed=5, dt=0.01, alpha=0.2)
+    lg, lo = M.LLG.Quaternion(), O.LLG(quaternion=True)
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], gl, g["temp"], lg)
+    sim = O.Sim(o["ss"], o["ex"], o["ani"], None, o["temp"], lo, seed=5)
+    sim.setLongRange3D(ol)
+    heun.step(30)
+    sim.heun(30)
+    err = float(np.abs(flat(g["ss"].spins()[:3]) - flat(o["ss"].spins()[:3])).max())
+    print(f"lr3 quaternion heun 8^3 T=0.4: max|dm| = {err:.3e}", flush=True)
+    assert err <= 1e-5
+
+
 def test_heun_open_longrange3d():
     dims = (12, 12, 4)
     spec = W.FilmSpec(*dims, J=1.0, K=0.05, g=0.0, pbc=(0, 0, 0), alpha=0.1, dt=0.005)
@@ -400,6 +415,26 @@ def test_ensemble_with_films_matches_single_replicas(schedule, monkeypatch):
         assert np.array_equal(flat(ge["ss"].spins()), flat(gs["ss"].spins())), k
         assert ge["ss"].time() == gs["ss"].time()
     assert ens[0][0]["temp"].draw() == 12
+
+
+def test_ensemble_with_quaternion_members_matches_single_replicas():
+    """LLG.Quaternion and LLG.Cartesian members in one Ensemble (films on the layered
+    Dipole and a LongRange3D cube): each equals its single-replica run bit for bit."""
+    cfg = [((64, 64, 4), 0.3, 31, True), ((64, 64, 4), 0.0, 32, False), ((64, 64, 4), 0.7, 33, True)]
+    ens, solo = [], []
+    for bucket in (ens, solo):
+        for dims, T, seed, quat in cfg:
+            spec, g = film(dims, T, seed)
+            llg = M.LLG.Quaternion() if quat else g["llg"]
+            bucket.append((g, M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], llg)))
+        spec, g, o, gl, ol = cube((8, 8, 8), 0.5, 34)
+        bucket.append((g, M.Heun(g["ss"], g["ex"], g["ani"], gl, g["temp"], M.LLG.Quaternion())))
+    E = M.Ensemble([h for (_, h) in ens])
+    E.step(9)
+    for (_, h) in solo:
+        h.step(9)
+    for k, ((ge, _), (gs, _)) in enumerate(zip(ens, solo)):
+        assert np.array_equal(flat(ge["ss"].spins()), flat(gs["ss"].spins())), k
 
 
 def test_ensemble_of_wide_films_with_split_product(monkeypatch):

@@ -609,11 +609,42 @@ def test_quaternion_llg_pc_driver_matches_oracle():
     assert abs(g["ss"].time() - o["ss"].time()) < 1e-12
 
 
-def test_fused_driver_rejects_quaternion():
+@pytest.mark.parametrize("T,thermal_first,comp", [(0.0, True, True), (0.4, True, True), (0.4, False, False)])
+def test_fused_heun_quaternion_matches_oracle(T, thermal_first, comp):
+    """LLG.Quaternion in the fused Heun step (each turn taken as an increment, so the
+    compensated state keeps it): 30 steps against the oracle's PredictorCorrector pass
+    with LLGQuaternion::apply and the thermal kick, same counter-based noise.  The
+    step is large enough (turns of ~0.1 rad) that LLG.Cartesian's trajectory lies far
+    outside the tolerance, so the test tells the two integrators apart."""
+    spec = W.FilmSpec(24, 16, 3, T=T, dt=0.02, alpha=0.3, K=0.1, g=0.05)
+    g, o = twin(spec)
+    c = W.build(O, spec)
+    c["ss"].setSpins(*O.random_unit_spins(spec.seed, spec.n))
+    lg, lo, lc = M.LLG.Quaternion(), O.LLG(quaternion=True), O.LLG()
+    lg.setThermalOnlyFirstTerm(thermal_first)
+    lo.setFlags(thermal_only_first_term=thermal_first)
+    lc.setFlags(thermal_only_first_term=thermal_first)
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], lg, compensated=comp)
+    sim = O.Sim(o["ss"], o["ex"], o["ani"], o["dip"], o["temp"], lo, seed=0)
+    cart = O.Sim(c["ss"], c["ex"], c["ani"], c["dip"], c["temp"], lc, seed=0)
+    heun.step(30)
+    sim.heun(30)
+    cart.heun(30)
+    err = _max_dm(g, o)
+    apart = float(np.abs(flat(o["ss"].spins()[:3]) - flat(c["ss"].spins()[:3])).max())
+    print(f"fused quaternion T={T} thermal_first={thermal_first} compensated={comp}: 30 steps "
+          f"max|dm| {err:.2e} (Cartesian trajectory {apart:.2e} away)", flush=True)
+    assert err <= 1e-5
+    assert apart >= 100 * max(err, 1e-7)
+    assert abs(g["ss"].time() - o["ss"].time()) < 1e-12
+
+
+def test_fused_rk_rejects_quaternion():
     spec = W.FilmSpec(16, 16, 1, g=0.0)
     g = W.build(M, spec)
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], None, None, M.LLG.Quaternion())
     with pytest.raises(M.MagLuaError):
-        M.Heun(g["ss"], g["ex"], g["ani"], None, None, M.LLG.Quaternion())
+        heun.rk("RK2", 1)
 
 
 def test_unseeded_thermals_get_distinct_streams():
