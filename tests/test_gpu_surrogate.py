@@ -223,10 +223,12 @@ def test_learned_driver_tracks_the_exact_trajectory_and_falls_back():
     assert st["windows"] == 3 and st["fallbacks"] == 3 and st["exact_windows"] == 3
 
 
-def test_learned_driver_with_zero_tolerance_is_the_exact_driver():
+@pytest.mark.parametrize("quat", [False, True])
+def test_learned_driver_with_zero_tolerance_is_the_exact_driver(quat):
     """tol = 0: every window is checked, restored and run exactly, so the trajectory
     equals the exact driver's bit for bit (same state, noise and draw counter), and the
-    time and draw counter advance as one exact run's do."""
+    time and draw counter advance as one exact run's do; with LLG.Cartesian and with
+    LLG.Quaternion."""
     spec, g = film(23, T=0.3)
     heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], g["llg"])
     heun.step(10)
@@ -236,8 +238,9 @@ def test_learned_driver_with_zero_tolerance_is_the_exact_driver():
     ta, tb = W.build(M, spec)["temp"], W.build(M, spec)["temp"]
     for t in (ta, tb):
         t.setSeed(77, draw=3)
-    ea = M.Heun(a, g["ex"], g["ani"], g["dip"], ta, g["llg"])
-    eb = M.Heun(b, g["ex"], g["ani"], g["dip"], tb, g["llg"])
+    llg = M.LLG.Quaternion() if quat else g["llg"]
+    ea = M.Heun(a, g["ex"], g["ani"], g["dip"], ta, llg)
+    eb = M.Heun(b, g["ex"], g["ani"], g["dip"], tb, llg)
     eb.setSurrogate(sur, K=7, tol=0.0)
     ea.step(30)
     eb.step(30)
