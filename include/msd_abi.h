@@ -355,9 +355,9 @@ int msd_longrange3d_pass_widths(msd_longrange3d* l, int* wy, int* wz, int* y_cho
 /* LLG.Cartesian: cpu/llg/llgcartesian.cpp:90-278 (thermalOnlyFirstTerm = true) */
 int msd_llg_create(msd_llg** out);
 /* LLG.Quaternion: cpu/llg/llgquat.cpp:127-373 (rotation of S about the damped
- * field).  Shares the flags and msd_llg_apply.  The fused Heun step takes it (each
- * turn added to the compensated state as the increment R S - S); msd_stepper_rk
- * rejects it with MSD_EUNSUPPORTED. */
+ * field).  Shares the flags and msd_llg_apply.  The fused Heun step and the fused
+ * Runge-Kutta stages take it (each turn added as the increment R S - S, which the
+ * compensated state keeps). */
 int msd_llg_create_quaternion(msd_llg** out);
 int msd_llg_destroy(msd_llg* l);
 int msd_llg_set_thermal_only_first_term(msd_llg* l, int yes);
@@ -407,8 +407,8 @@ int msd_stepper_heun(msd_stepper* st, int nsteps);
 /* make_rk_step_function(ss, kind, calcField, llg, temp) steps (common/scripts/RungeKutta.lua:
  * 57-214) with fused stage kernels: kind names a Butcher table of the script (RK1, RK2,
  * K3, RK3, RK4, RK4_38, RK6, BS3); each stage is the field at the stage spins (and with
- * a Dipole its column pass) and the Butcher row as chained renormalised LLG.Cartesian
- * sub-steps, then the thermal kick.  Local and layered-Dipole drivers; the state is
+ * a Dipole its column pass) and the Butcher row as chained renormalised LLG sub-steps
+ * (Cartesian Euler steps, or LLG.Quaternion turns by c dt), then the thermal kick.  Local and layered-Dipole drivers; the state is
  * carried as plain FP32 (the compensated low words are dropped). */
 int msd_stepper_rk(msd_stepper* st, const char* kind, int nsteps);
 /* Number of step graphs captured and instantiated so far (no counterpart in the

@@ -1034,10 +1034,19 @@ __global__ void __launch_bounds__(TPB) k_rk_stage(StepArgs a, RKStage r)
                 const bool cur = j == r.terms - 1;
                 const float3 M = cur ? X : make_float3(ldg(r.K[j], i), ldg(r.K[j] + n, i), ldg(r.K[j] + 2 * (size_t)n, i));
                 const float3 Hj = cur ? H : make_float3(ldg(r.H[j], i), ldg(r.H[j] + n, i), ldg(r.H[j] + 2 * (size_t)n, i));
-                const float3 dM = a.lp.thermal_first ? llg_rhs(M, Hj, Hj, al, ga, ms, a.lp.ps, a.lp.ds)
-                                                     : llg_rhs(M, Hj, Hj, al, ga, ms, 1.0f, 1.0f);
                 const float h = c * dt;
-                d = make_float3(d.x + h * dM.x, d.y + h * dM.y, d.z + h * dM.z);
+                if (a.lp.quaternion) {
+                    // LLG:apply(d, c, K_j, d): d turned about the axis of K_j's field by c dt
+                    float qw;
+                    float3 qv;
+                    quat_rot(a.lp, d, M, Hj, Hj, al, ga, ms, h, qw, qv);
+                    const float3 dd = quat_delta(qw, qv, d);
+                    d = make_float3(d.x + dd.x, d.y + dd.y, d.z + dd.z);
+                } else {
+                    const float3 dM = a.lp.thermal_first ? llg_rhs(M, Hj, Hj, al, ga, ms, a.lp.ps, a.lp.ds)
+                                                         : llg_rhs(M, Hj, Hj, al, ga, ms, 1.0f, 1.0f);
+                    d = make_float3(d.x + h * dM.x, d.y + h * dM.y, d.z + h * dM.z);
+                }
             }
             if (a.lp.renorm) d = renorm(d, ms);
         }
@@ -1049,9 +1058,17 @@ __global__ void __launch_bounds__(TPB) k_rk_stage(StepArgs a, RKStage r)
                     const float3 gn = thermal_normals(a.th.seed, draw, (int)i);
                     Ht = make_float3(sg * gn.x, sg * gn.y, sg * gn.z);
                 }
-                const float3 dM = a.lp.thermal_first ? llg_rhs(d, Ht, zero, al, ga, ms, a.lp.ps, a.lp.ds)
-                                                     : llg_rhs(d, Ht, Ht, al, ga, ms, 1.0f, 1.0f);
-                d = make_float3(d.x + dt * dM.x, d.y + dt * dM.y, d.z + dt * dM.z);
+                if (a.lp.quaternion) {
+                    float qw;
+                    float3 qv;
+                    quat_rot(a.lp, d, d, Ht, a.lp.thermal_first ? zero : Ht, al, ga, ms, dt, qw, qv);
+                    const float3 dd = quat_delta(qw, qv, d);
+                    d = make_float3(d.x + dd.x, d.y + dd.y, d.z + dd.z);
+                } else {
+                    const float3 dM = a.lp.thermal_first ? llg_rhs(d, Ht, zero, al, ga, ms, a.lp.ps, a.lp.ds)
+                                                         : llg_rhs(d, Ht, Ht, al, ga, ms, 1.0f, 1.0f);
+                    d = make_float3(d.x + dt * dM.x, d.y + dt * dM.y, d.z + dt * dM.z);
+                }
             }
             if (a.lp.renorm) d = renorm(d, ms);
         }

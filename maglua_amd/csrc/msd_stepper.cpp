@@ -474,9 +474,6 @@ extern "C" int msd_stepper_rk(msd_stepper* st, const char* kind, int nsteps)
     if (nsteps < 0) return fail(MSD_EINVAL, "rk: nsteps %d", nsteps);
     if (st->lr3) return fail(MSD_EUNSUPPORTED, "rk: the fused Runge-Kutta stages cover the local and layered forms");
     if (st->sur) return fail(MSD_EUNSUPPORTED, "rk: the learned dipole field runs with the Heun step");
-    if (st->llg.quaternion)
-        return fail(MSD_EUNSUPPORTED, "rk: the fused Runge-Kutta stages integrate LLG.Cartesian; the Heun step "
-                                      "takes LLG.Quaternion");
     if (nsteps == 0) return MSD_OK;
     const int ns = (int)bt.size();
     const size_t n3 = (size_t)3 * st->s->d.n;

@@ -639,12 +639,22 @@ def test_fused_heun_quaternion_matches_oracle(T, thermal_first, comp):
     assert abs(g["ss"].time() - o["ss"].time()) < 1e-12
 
 
-def test_fused_rk_rejects_quaternion():
-    spec = W.FilmSpec(16, 16, 1, g=0.0)
-    g = W.build(M, spec)
-    heun = M.Heun(g["ss"], g["ex"], g["ani"], None, None, M.LLG.Quaternion())
-    with pytest.raises(M.MagLuaError):
-        heun.rk("RK2", 1)
+@pytest.mark.parametrize("kind,T,dipole", [("RK2", 0.0, True), ("RK4", 0.3, True), ("K3", 0.0, False)])
+def test_fused_rk_quaternion_matches_oracle(kind, T, dipole):
+    """LLG.Quaternion in the fused Runge-Kutta stages: each chained sub-step turns the
+    running state about the stage's axis by c dt (K3's c = -1 turns it back), against
+    the oracle's RungeKutta.lua restatement with LLGQuaternion::apply, 20 steps."""
+    spec = W.FilmSpec(32, 16, 2, T=T, dt=0.01)
+    g, o = twin(spec, dipole=dipole)
+    lg, lo = M.LLG.Quaternion(), O.LLG(quaternion=True)
+    heun = M.Heun(g["ss"], g["ex"], g["ani"], g["dip"], g["temp"], lg)
+    sim = O.Sim(o["ss"], o["ex"], o["ani"], o["dip"], o["temp"], lo, seed=spec.noise_seed)
+    heun.rk(kind, 20)
+    sim.rk(kind, 20)
+    err = _max_dm(g, o)
+    print(f"fused quaternion {kind} T={T} dipole={dipole}: 20 steps max|dm| = {err:.3e}", flush=True)
+    assert err <= 1e-5
+    assert abs(g["ss"].time() - o["ss"].time()) < 1e-12
 
 
 def test_unseeded_thermals_get_distinct_streams():
