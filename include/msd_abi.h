@@ -408,8 +408,9 @@ int msd_stepper_heun(msd_stepper* st, int nsteps);
  * 57-214) with fused stage kernels: kind names a Butcher table of the script (RK1, RK2,
  * K3, RK3, RK4, RK4_38, RK6, BS3); each stage is the field at the stage spins (and with
  * a Dipole its column pass) and the Butcher row as chained renormalised LLG sub-steps
- * (Cartesian Euler steps, or LLG.Quaternion turns by c dt), then the thermal kick.  Local and layered-Dipole drivers; the state is
- * carried as plain FP32 (the compensated low words are dropped). */
+ * (Cartesian Euler steps, or LLG.Quaternion turns by c dt), then the thermal kick.
+ * Local and layered-Dipole drivers; the state is carried as plain FP32 (the
+ * compensated low words are dropped). */
 int msd_stepper_rk(msd_stepper* st, const char* kind, int nsteps);
 /* Number of step graphs captured and instantiated so far (no counterpart in the
  * reference; diagnostics of the graph reuse above). */
